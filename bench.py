@@ -1,0 +1,143 @@
+#!/usr/bin/env python3
+"""Benchmark of the Gibbs sweep on one MI355X.
+
+Workload: the large_k1000 training corpus (100,000 synthetic docs from the
+LDA generative process with Zipf(1.07) background, mean length 300, about
+30M tokens, V=100,000), model K=1000, alpha=0.05, eta=0.01.  One step is one
+training iteration of LDAModel.train (one device-wide sweep over every token;
+the alpha fixed point runs every 10th iteration as in tomotopy's default).
+
+Prints one JSON line.  `value` is tokens sampled per second over the timed
+steps (corpus and counts already resident in HBM).  `roofline` prices the
+sweep kernel's algorithmic bytes per launch, N*(4K+8) + D*8K, against the
+8 TB/s HBM peak, using the kernel's average duration from HIP events on the
+library's stream.  `cpu_baseline` times the CPU oracle's partition sweep on
+a bounded sample of the same corpus on this host's cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Gibbs tokens/s at K=1000 (HBM %peak); batched infer docs/s + p99 vs plain"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_corpus(n_docs=None):
+    from tomotopy_amd import corpus, synth
+    t = time.time()
+    words, off, nw, w = synth.workload("large_k1000", n_docs=n_docs)
+    ws, offs, real_v, _ = corpus.prepare_arrays(words, off, nw)
+    log("corpus: %d docs, %d tokens, V=%d (%.1fs)" % (len(offs) - 1, len(ws), real_v, time.time() - t))
+    return ws, offs, real_v, w
+
+
+def cpu_baseline(ws, offs, V, K, alpha, eta, budget_s=12.0):
+    """Oracle partition sweep on the host's cores over a bounded document sample."""
+    from oracle import oracle as O
+    threads = min(os.cpu_count() or 1, 16)
+    D = len(offs) - 1
+    n_docs = max(200, min(D, 4000))
+    sub_off = offs[:n_docs + 1].copy()
+    sub_w = ws[:int(sub_off[-1])]
+    m = O.Model(K, V, alpha, eta, seed=1, scan_order=0)
+    m.set_optim(0, 0)
+    m.set_corpus(sub_w, sub_off)
+    m.train(1, workers=threads, threads=threads)  # warm (page in n_kw)
+    sweeps = 0
+    t0 = time.perf_counter()
+    while True:
+        m.train(1, workers=threads, threads=threads)
+        sweeps += 1
+        if time.perf_counter() - t0 > budget_s or sweeps >= 50:
+            break
+    dt = time.perf_counter() - t0
+    ntok = int(np.count_nonzero(sub_w < V))
+    return {
+        "value": ntok * sweeps / dt,
+        "unit": "tokens/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": "first %d of %d docs (%d tokens), K=%d, V=%d, partition scheme with %d workers, %d sweeps in %.1fs"
+                  % (n_docs, D, ntok, K, V, threads, sweeps, dt),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--docs", type=int, default=None, help="fewer documents (quick runs only)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+    rank = int(os.environ.get("RANK", "0"))
+
+    from tomotopy_amd import _native
+    K, alpha, eta = 1000, 0.05, 0.01
+    ws, offs, V, w = make_corpus(args.docs)
+    D = len(offs) - 1
+    n_valid = int(np.count_nonzero(ws < V))
+    ctx = _native.Context(K, V, alpha, eta, seed=1234)
+    t = time.time()
+    ctx.upload_corpus(ws, offs)
+    log("upload + initial topics: %.2fs" % (time.time() - t))
+    for i in range(args.warmup):
+        ctx.train(1, workers=0)
+        log("warmup step %d: %.2f ms kernel" % (i, ctx.kernel_stats()["ms"]))
+    t0 = time.perf_counter()
+    ctx.train(args.steps, workers=0)
+    dt = time.perf_counter() - t0
+    st = ctx.kernel_stats()
+    flags = ctx.check_invariants()
+    if flags.any():
+        log("INVARIANT VIOLATION: %s" % flags.tolist())
+        sys.exit(3)
+    lld, llr = ctx.loglik()
+    kern_ms = st["ms"] / max(1, st["launches"])
+    bytes_per_launch = n_valid * (4 * K + 8) + D * 8 * K
+    achieved = bytes_per_launch / (kern_ms * 1e-3) / 1e9
+    value = n_valid * args.steps / dt
+    log("ll_per_word %.6f; kernel avg %.3f ms; wall %.3f ms/step" % ((lld + llr) / n_valid, kern_ms,
+                                                                     dt * 1e3 / args.steps))
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "tokens/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "large_k1000", "docs": D, "tokens": n_valid, "V": V, "K": K, "alpha": alpha,
+                   "eta": eta, "scheme": "device-wide (workers=0)", "optim_interval": 10},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "sweep_device_kernel<16>", "kernel_ms": kern_ms,
+                     "algorithmic_bytes_per_launch": bytes_per_launch},
+        "ll_per_word": (lld + llr) / n_valid,
+    }
+    if rank == 0 and not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(ws, offs, V, K, alpha, eta)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

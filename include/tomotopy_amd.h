@@ -1,0 +1,155 @@
+/*
+ * tomotopy_amd.h -- C ABI of the MI355X collapsed-Gibbs LDA engine.
+ *
+ * Plain C: pointers, sizes and integer status codes; no C++ exceptions and no
+ * torch types cross this boundary.  Every call returns LDA_OK (0) or a
+ * negative LDA_ERR_* code; lda_last_error() gives a thread-local message.
+ * One context owns one model's device state and one HIP stream; calls on one
+ * context must be serialised by the caller (the Python LDAModel does this).
+ *
+ * Layout conventions at this boundary (the device layout is private):
+ *   words      uint32 token word ids, documents concatenated (CSR), each
+ *              document's words sorted ascending; ids >= V are kept and
+ *              skipped by the sampler (tomoto's "vid >= realV").
+ *   doc_off    uint64[D+1] CSR offsets into words.
+ *   z          uint16 per token; 0xFFFF for skipped tokens.
+ *   nkw        int32 V x K, word-major (tomoto's column-major K x V
+ *              numByTopicWord, LDAModel.hpp:66).
+ *   nk         int32 K.   ndk  int32 D x K (document-major).
+ *   alpha      float K.
+ *
+ * Reference interfaces replaced (bab2min/tomotopy):
+ *   lda_ctx_create        ILDAModel::create + LDAModel ctor checks
+ *                         (src/TopicModel/LDAModel.cpp:10-13, LDAModel.hpp:964-984)
+ *   lda_upload_corpus     prepare(): initializeDocState / addWordTo<+1>
+ *                         (LDAModel.hpp:716-813, 1073-1138)
+ *   lda_train             ITopicModel::train (TopicModel.hpp:684-755) ->
+ *                         trainOne (LDAModel.hpp:505-539) -> performSampling
+ *                         (356-428) -> sampleDocument (326-354) ->
+ *                         getZLikelihoods (273-285) -> sample::prefixSum /
+ *                         sampleFromDiscreteAcc (src/Utils/sample.hpp:61-189)
+ *   lda_sweep             one trainOne without the alpha step
+ *   lda_optimize_alpha    optimizeParameters (LDAModel.hpp:221-265)
+ *   lda_loglik            getLLDocs / getLLRest (LDAModel.hpp:616-674), FP64
+ *   lda_infer             LDAModel::_infer<together=false> (LDAModel.hpp:843-954)
+ *                         + _getTopicsByDoc (1145-1159)
+ *   lda_download_state /  globalState / docs read-out for save, get_topic_words,
+ *   lda_set_state         get_count_by_topics (TopicModel.hpp:335-395,
+ *                         LDAModel.hpp:815-841) and load (prepare(false),
+ *                         LDAModel.hpp:1064-1071, 1163-1173)
+ *   lda_sample_frozen     one getZLikelihoods + sampleFromDiscreteAcc step on a
+ *                         frozen state (the pure per-token function)
+ */
+#ifndef TOMOTOPY_AMD_H
+#define TOMOTOPY_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct lda_ctx lda_ctx;
+
+enum {
+    LDA_OK = 0,
+    LDA_ERR_INVALID_ARGUMENT = -1, /* tomoto exc::InvalidArgument -> ValueError */
+    LDA_ERR_STATE = -2,            /* call order (e.g. train before upload) -> RuntimeError */
+    LDA_ERR_DEVICE = -3,           /* HIP runtime error */
+    LDA_ERR_OUT_OF_MEMORY = -4,
+    LDA_ERR_UNSUPPORTED = -5,
+    LDA_ERR_TRAINING = -6          /* train failed (tomoto returns <0 -> "Train failed") */
+};
+
+/* tomotopy.TermWeight */
+enum { LDA_TW_ONE = 0, LDA_TW_IDF = 1, LDA_TW_PMI = 2 };
+
+/* tomotopy.ParallelScheme (tomotopy/__init__.py:36-64) */
+enum { LDA_PS_DEFAULT = 0, LDA_PS_NONE = 1, LDA_PS_COPY_MERGE = 2, LDA_PS_PARTITION = 3 };
+
+/* Largest K handled by the register-resident sampler (64 lanes x 32 topics). */
+#define LDA_MAX_K 2048
+
+const char* lda_last_error(void);
+int lda_version(void);
+
+/* Number of visible HIP devices (0 when none). */
+int lda_device_count(int* count);
+
+/* Create a model context on `device`.  alpha: K floats (symmetric alpha is K
+ * copies).  eta > 0, 1 <= K <= LDA_MAX_K, V >= 0.  termweight must be
+ * LDA_TW_ONE for the device sampler. */
+int lda_ctx_create(int K, int V, const float* alpha, float eta, int termweight, uint64_t seed, int device,
+                   lda_ctx** out);
+int lda_ctx_destroy(lda_ctx* ctx);
+
+/* Upload the training corpus.  If z is NULL the initial topics are drawn on
+ * the device (z ~ Discrete(alpha) from the Philox stream); otherwise z is
+ * taken as given.  Counts n_dk, n_kw, n_k are then built from z. */
+int lda_upload_corpus(lda_ctx* ctx, const uint32_t* words, const uint64_t* doc_off, int64_t D, const uint16_t* z);
+
+/* Replace the topic-word counts (tomoto load takes n_kw / n_k from the file). */
+int lda_set_counts(lda_ctx* ctx, const int32_t* nkw, const int32_t* nk);
+
+int lda_set_alpha(lda_ctx* ctx, const float* alpha);
+int lda_get_alpha(lda_ctx* ctx, float* alpha);
+int lda_set_optim(lda_ctx* ctx, uint32_t optim_interval, uint32_t burn_in);
+int lda_get_global_step(lda_ctx* ctx, uint64_t* step);
+int lda_set_global_step(lda_ctx* ctx, uint64_t step);
+
+/* Training.  scheme is an LDA_PS_* value; workers as in tomotopy:
+ *   NONE, or workers == 1      one device worker, sequential over shuffled
+ *                              documents (deterministic).
+ *   PARTITION / DEFAULT with workers in [2, 1024]
+ *                              the PCGS partition schedule with `workers`
+ *                              device workers, W rounds per sweep, private
+ *                              n_k per worker (deterministic).
+ *   PARTITION / DEFAULT / COPY_MERGE with workers == 0
+ *                              device-wide: every resident wavefront is a
+ *                              worker with a private n_k; n_kw is shared and
+ *                              updated with exact int32 atomics.
+ *   COPY_MERGE with workers >= 2  LDA_ERR_UNSUPPORTED.
+ * lda_train runs `iterations` sweeps, each followed by the alpha fixed point
+ * when global_step >= burn_in and (global_step+1) % optim_interval == 0. */
+int lda_train(lda_ctx* ctx, int iterations, int workers, int scheme);
+int lda_sweep(lda_ctx* ctx, int workers, int scheme);
+int lda_optimize_alpha(lda_ctx* ctx);
+
+/* FP64 log-likelihood: ll_docs = getLLDocs(all docs), ll_rest = getLLRest. */
+int lda_loglik(lda_ctx* ctx, double* ll_docs, double* ll_rest);
+
+/* Count invariants, counted on the device.  flags[0] docs whose sum_k n_dk !=
+ * valid length; [1] topics whose sum_v n_kw != n_k; [2] sum_k n_k - N_valid;
+ * [3] n_dk cells differing from a recount from z; [4] n_kw cells differing
+ * from a recount from z; [5] negative cells; [6] tokens with invalid z. */
+int lda_check_invariants(lda_ctx* ctx, int64_t flags[8]);
+
+/* Read-out (host buffers; any pointer may be NULL). */
+int lda_download_state(lda_ctx* ctx, uint16_t* z, int32_t* ndk, int32_t* nkw, int32_t* nk);
+int lda_get_dims(lda_ctx* ctx, int64_t dims[6]); /* K, V, D, N, Kp, kpl */
+
+/* One sampling step per query token on the frozen current state: remove the
+ * token's own z_old from (n_dk[doc], n_kw[:,word], n_k), weight, scan, and
+ * search with the given uniform.  Counts are not modified. */
+int lda_sample_frozen(lda_ctx* ctx, int64_t n, const int64_t* doc, const uint32_t* word, const uint16_t* z_old,
+                      const float* u, uint16_t* z_new, float* total);
+
+/* Fold-in of unseen documents against the current (frozen) n_kw / n_k.
+ * mode 0: batched (one persistent launch over all documents, longest first,
+ *         replayed from a cached hipGraph);
+ * mode 1: plain (one launch and one synchronisation per document).
+ * theta: D x K floats, ll: D doubles (FP64), z_out: N uint16 (may be NULL). */
+int lda_infer(lda_ctx* ctx, const uint32_t* words, const uint64_t* doc_off, int64_t D, int iterations,
+              uint64_t infer_seed, int mode, float* theta, double* ll, uint16_t* z_out);
+
+/* Timing of the last lda_train / lda_sweep / lda_infer call, measured with
+ * HIP events on the context's stream around the hot kernels only:
+ * stats[0] total kernel ms, stats[1] launches, stats[2] tokens sampled. */
+int lda_kernel_stats(lda_ctx* ctx, double stats[4]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TOMOTOPY_AMD_H */

@@ -1,0 +1,744 @@
+// ============================================================================
+// lda_oracle.cpp -- CPU ORACLE for the collapsed-Gibbs LDA hot path.
+//
+// TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+// cpu_baseline leg of bench.py may load this library, and only as the checker
+// (or as the timed CPU baseline).  The product (tomotopy_amd) never links,
+// loads or calls it; the product path fails loudly without its HIP library.
+//
+// This is a plain C++17 restatement of tomoto's plain-LDA Gibbs sampler
+// (bab2min/tomotopy, src/TopicModel/LDAModel.hpp and friends), written from
+// the reference's behaviour, not copied.  Every function cites the reference
+// file:line whose semantics it restates.  Two deliberate substitutions:
+//   * RNG: tomoto draws from EigenRand's P8_mt19937_64_32 (TopicModel.hpp:16);
+//     EigenRand 0.5.1 is absent here, so both this oracle and the device use
+//     Philox4x32-10 (Salmon et al., SC'11) with a counter layout defined in
+//     lda_rng_counter() below.  The per-token uniform is a pure function of
+//     (seed, sweep, token index), so results do not depend on schedule.
+//   * Scan order: tomoto's prefixSum uses an SSE 4-block association
+//     (sample.hpp:61-86).  The oracle implements that order AND the device
+//     order (lane-local sequential scan + 64-lane DPP row scan); the device
+//     is checked bit-exactly against the device order, and the tie rate of
+//     device order vs tomoto order is measured.
+//
+// Parity pins (see tests/): Philox known-answer vectors (Random123 KAT),
+// weights vs numpy float32 closed form, scans vs numpy emulations, search vs
+// brute force, count invariants, FP64 LL vs scipy gammaln, tomoto LUT lgamma
+// vs FP64 within its interpolation bound, alpha fixed point on Dir(alpha)
+// data, infer delta-LL vs brute-force full-copy LL.
+//
+// Build: oracle/Makefile -> oracle/_build/liblda_oracle.so (g++ -O2
+// -ffp-contract=off; SSE float math, no x87, no FMA contraction).
+// ============================================================================
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <thread>
+#include <vector>
+
+namespace {
+
+// ---------------------------------------------------------------- Philox ----
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
+// easy as 1, 2, 3", SC'11).  Constants are the published ones.
+constexpr uint32_t PHILOX_M0 = 0xD2511F53u, PHILOX_M1 = 0xCD9E8D57u;
+constexpr uint32_t PHILOX_W0 = 0x9E3779B9u, PHILOX_W1 = 0xBB67AE85u;
+
+inline void philox4x32_10(const uint32_t ctr_in[4], const uint32_t key_in[2], uint32_t out[4]) {
+    uint32_t c0 = ctr_in[0], c1 = ctr_in[1], c2 = ctr_in[2], c3 = ctr_in[3];
+    uint32_t k0 = key_in[0], k1 = key_in[1];
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = (uint64_t)PHILOX_M0 * c0;
+        uint64_t p1 = (uint64_t)PHILOX_M1 * c2;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        uint32_t n0 = hi1 ^ c1 ^ k0;
+        uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+        k0 += PHILOX_W0; k1 += PHILOX_W1;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+// Stream tags (4th counter word).  Shared with the device code.
+enum : uint32_t {
+    TAG_SWEEP = 0x53574550u,   // per-token uniform of a training sweep
+    TAG_INIT = 0x494E4954u,    // initial topic draw (LDAModel.hpp:754-762)
+    TAG_SHUFFLE = 0x53484646u, // forShuffled seed (Utils.hpp:157-178)
+    TAG_INFER = 0x494E4652u,   // fold-in sweep uniform (LDAModel.hpp:930-940)
+    TAG_INFINIT = 0x49494E49u, // fold-in initial draw
+};
+
+inline uint32_t philox_word(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c2, uint32_t tag) {
+    uint32_t ctr[4] = {c0, c1, c2, tag};
+    uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint32_t out[4];
+    philox4x32_10(ctr, key, out);
+    return out[0];
+}
+
+// u in [0,1) with 24 random bits: exact in float.
+inline float word_to_uniform(uint32_t x) { return (float)(x >> 8) * 0x1p-24f; }
+
+inline float token_uniform(uint64_t seed, uint64_t g, uint32_t sweep, uint32_t tag) {
+    return word_to_uniform(philox_word(seed, (uint32_t)g, (uint32_t)(g >> 32), sweep, tag));
+}
+
+// ------------------------------------------------------- sampling pieces ----
+// a1: zLikelihood[k] = (n_dk+alpha_k) * (n_kw+eta) / (n_k + V*eta)
+// (LDAModel.hpp:273-285, EtaHelper 81-97: V*eta is float(eta)*float(realV)).
+inline float topic_weight(int32_t ndk, float alpha, int32_t nkw, float eta, int32_t nk, float veta) {
+    float a = (float)ndk + alpha;
+    float b = (float)nkw + eta;
+    float num = a * b;
+    float den = (float)nk + veta;
+    return num / den;
+}
+
+// a2, tomoto order: sample.hpp:61-86.  Blocks of 4 are scanned as
+// [a, a+b, (b+c)+a, (c+d)+(a+b)] and offset by the previous block's last
+// element; the tail (or everything when n < 4, starting at index 1) is a
+// sequential running sum.
+void prefix_sum_tomoto(float* x, int n) {
+    int n4 = n & ~3;
+    float offset = 0.0f;
+    for (int i = 0; i < n4; i += 4) {
+        float a = x[i], b = x[i + 1], c = x[i + 2], d = x[i + 3];
+        float ab = a + b, bc = b + c, cd = c + d;
+        float o0 = a, o1 = ab, o2 = bc + a, o3 = cd + ab;
+        if (i) { o0 = o0 + offset; o1 = o1 + offset; o2 = o2 + offset; o3 = o3 + offset; }
+        x[i] = o0; x[i + 1] = o1; x[i + 2] = o2; x[i + 3] = o3;
+        offset = o3;
+    }
+    int start = n4 ? n4 : 1;
+    for (int i = start; i < n; ++i) x[i] = x[i] + x[i - 1];
+}
+
+// a2, device order.  Topics are laid out lane-major: lane l of the 64-lane
+// wavefront owns topics [l*kpl, (l+1)*kpl).  Each lane scans its own topics
+// sequentially; lane totals are scanned with the DPP pattern row_shr:1,2,4,8
+// (zero fill outside the 16-lane row), then row_bcast:15 into rows 1 and 3,
+// then row_bcast:31 into rows 2 and 3; the exclusive lane prefix is the
+// inclusive value of lane l-1 (wave_shr:1, zero into lane 0) and is added to
+// every lane-local partial.  x has 64*kpl entries (padding entries are 0).
+void prefix_sum_device(float* x, int kpl) {
+    float part[64 * 32];
+    float tot[64];
+    for (int l = 0; l < 64; ++l) {
+        float s = x[l * kpl];
+        part[l * kpl] = s;
+        for (int j = 1; j < kpl; ++j) { s = s + x[l * kpl + j]; part[l * kpl + j] = s; }
+        tot[l] = s;
+    }
+    float v[64], t[64];
+    std::memcpy(v, tot, sizeof v);
+    for (int d = 1; d <= 8; d <<= 1) {
+        std::memcpy(t, v, sizeof v);
+        for (int l = 0; l < 64; ++l) if ((l & 15) >= d) v[l] = t[l] + t[l - d];
+    }
+    std::memcpy(t, v, sizeof v);
+    for (int l = 16; l < 32; ++l) v[l] = t[l] + t[15];
+    for (int l = 48; l < 64; ++l) v[l] = t[l] + t[47];
+    std::memcpy(t, v, sizeof v);
+    for (int l = 32; l < 64; ++l) v[l] = t[l] + t[31];
+    for (int l = 0; l < 64; ++l) {
+        float e = l ? v[l - 1] : 0.0f;
+        for (int j = 0; j < kpl; ++j) x[l * kpl + j] = e + part[l * kpl + j];
+    }
+}
+
+// a3: sampleFromDiscreteAcc (sample.hpp:133-189): r = u * cdf[K-1]; return
+// the first z with r < cdf[z], K-1 if there is none.
+inline int search_cdf(const float* cdf, int K, float u) {
+    float r = u * cdf[K - 1];
+    for (int z = 0; z < K; ++z) if (r < cdf[z]) return z;
+    return K - 1;
+}
+
+// kpl used by the device for K topics: the smallest power of two with 64*kpl >= K.
+inline int device_kpl(int K) {
+    int kpl = 1;
+    while (64 * kpl < K) kpl <<= 1;
+    return kpl;
+}
+
+// Weight vector + scan + search, the full per-token step of a1-a3.  When
+// scan_order == 1 the device order is used (weights padded to 64*kpl with 0).
+int sample_one(int K, int scan_order, int kpl, const int32_t* ndk, const float* alpha,
+               const int32_t* nkw_col, float eta, const int32_t* nk, float veta, float u, float* buf) {
+    int n = scan_order ? 64 * kpl : K;
+    for (int k = 0; k < K; ++k) buf[k] = topic_weight(ndk[k], alpha[k], nkw_col[k], eta, nk[k], veta);
+    for (int k = K; k < n; ++k) buf[k] = 0.0f;
+    if (scan_order) prefix_sum_device(buf, kpl);
+    else prefix_sum_tomoto(buf, K);
+    return search_cdf(buf, K, u);
+}
+
+// forShuffled (Utils.hpp:157-178): visit 0..N-1 in the order ((i+seed)*P) % N,
+// P a prime from a fixed table chosen by seed (stepping when it divides N).
+void shuffled_order(size_t N, uint32_t seed, std::vector<uint32_t>& out) {
+    static const size_t primes[16] = {65537, 65539, 65543, 65551, 65557, 65563, 65579, 65581,
+                                      65587, 65599, 65609, 65617, 65629, 65633, 65647, 65651};
+    out.resize(N);
+    if (!N) return;
+    size_t P = primes[seed & 0xF];
+    if (N % P == 0) {
+        P = primes[(seed + 1) & 0xF];
+        if (N % P == 0) P = primes[(seed + 2) & 0xF];
+        if (N % P == 0) P = primes[(seed + 3) & 0xF];
+    }
+    P %= N;
+    for (size_t i = 0; i < N; ++i) out[i] = (uint32_t)(((i + (size_t)seed) * P) % N);
+}
+
+// -------------------------------------------------------- lgamma LUT ------
+// tomoto's float LUT lgamma / digamma (LUT.hpp:10-79, math.h:117-154).
+// The lgamma grid is C lgamma(double) at float grid points rounded to float,
+// and its small/large tails mix float and double arithmetic, as tomoto's
+// unqualified lgamma/log calls on floats resolve to the C double functions;
+// the digamma grid is a double-precision digamma rounded to float (tomoto uses
+// its own digamma series).  The interpolation structure (steps 1/1024, 1/64,
+// 1/8 or 1/4) and the small/large-argument tails are the reference's.
+double digamma_d(double x) {
+    double r = 0.0;
+    while (x < 6.0) { r -= 1.0 / x; x += 1.0; }
+    double f = 1.0 / (x * x);
+    double t = f * (-1.0 / 12 + f * (1.0 / 120 + f * (-1.0 / 252 + f * (1.0 / 240 + f * (-1.0 / 132)))));
+    return r + std::log(x) - 0.5 / x + t;
+}
+
+struct Lut3 {
+    int N, M, L; float P, Q, R;
+    int kind;  // 0 lgamma, 1 digamma
+    std::vector<float> pts;
+    Lut3(int kind_, int U) : N(1024), M(6400), L(kind_ ? 4000 : 8000), kind(kind_) {
+        P = 1.0f / 1024; Q = 1.0f / 64; R = 1.0f / (float)U;
+        pts.resize(N + M + L);
+        for (int i = 0; i < N; ++i) pts[i] = f(i ? i * P : 0.0001f);
+        for (int i = 0; i < M; ++i) pts[N + i] = f(i * Q + N * P);
+        for (int i = 0; i < L; ++i) pts[N + M + i] = f(i * R + N * P + M * Q);
+    }
+    float f(float x) const { return kind ? (float)digamma_d((double)x) : (float)::lgamma((double)x); }
+    float small(float x) const {
+        const float pi = 3.14159265f;
+        if (kind == 0) {
+            if (x == 0) return INFINITY;
+            return (float)((double)(x + 0.5f) * ::log((double)(x + 1)) - (double)(x + 1) + ::log((double)(2 * pi)) / 2 +
+                           (double)(1 / 12.f / (x + 1)) - ::log((double)x));
+        }
+        if (x == 0) return -INFINITY;
+        return logf(x + 2) - 0.5f / (x + 2) - 1 / 12.f / powf(x + 2, 2) - 1 / (x + 1) - 1 / x;
+    }
+    float large(float x) const {
+        const float pi = 3.14159265f;
+        if (kind == 0)
+            return (float)((double)(x - 0.5f) * ::log((double)x) - (double)x + ::log((double)(2 * pi)) / 2 +
+                           (double)(1 / 12.f / x));
+        return logf(x) - 0.5f / x - 1 / 12.f / powf(x, 2);
+    }
+    float get(float x) const {
+        if (!std::isfinite(x)) {
+            if (std::isnan(x)) return NAN;
+            if (kind == 0) return x > 0 ? INFINITY : -INFINITY;
+            return x < 0 ? NAN : INFINITY;
+        }
+        if (x < 0) return NAN;
+        if (x < 0.001f) return small(x);
+        if (x >= N * P + M * Q + (L - 1) * R) return large(x);
+        size_t idx; float a; float nx = x;
+        if (x < N * P) { idx = (size_t)(nx / P); a = (nx - idx * P) / P; }
+        else {
+            nx -= N * P;
+            if (nx < M * Q) { idx = (size_t)(nx / Q); a = (nx - idx * Q) / Q; idx += N; }
+            else { nx -= M * Q; idx = (size_t)(nx / R); a = (nx - idx * R) / R; idx += N + M; }
+        }
+        return pts[idx] + a * (pts[idx + 1] - pts[idx]);
+    }
+};
+
+const Lut3& lut_lgamma() { static Lut3 t(0, 8); return t; }
+const Lut3& lut_digamma() { static Lut3 t(1, 4); return t; }
+
+// digammaApprox (math.h:170-175), evaluated in float as tomoto does over
+// float arrays.
+inline float digamma_approx_f(float z) {
+    return logf(z + 4) - 0.5f / (z + 4) - (1.0f / 12.0f) / ((z + 4) * (z + 4)) - 1.0f / z - 1.0f / (z + 1) -
+           1.0f / (z + 2) - 1.0f / (z + 3);
+}
+
+// ------------------------------------------------------------- model ------
+struct Model {
+    int K, V, kpl, scan_order;
+    float eta, veta;
+    uint64_t seed;
+    std::vector<float> alpha;
+    // corpus (words sorted ascending per document; ids >= V are skipped)
+    std::vector<uint32_t> words;
+    std::vector<uint64_t> off;
+    int64_t D = 0;
+    std::vector<uint16_t> z;
+    std::vector<int32_t> ndk;  // D x K
+    std::vector<int32_t> nkw;  // V x K (word-major, tomoto's column-major K x V)
+    std::vector<int32_t> nk;   // K
+    uint32_t global_step = 0;
+    uint32_t optim_interval = 10, burn_in = 0;
+};
+
+void add_token(Model& m, int64_t d, uint32_t v, int k, int inc, int32_t* nk_local) {
+    // addWordTo<+-1> (LDAModel.hpp:287-299) for TermWeight::one.
+    m.ndk[(size_t)d * m.K + k] += inc;
+    nk_local[k] += inc;
+    m.nkw[(size_t)v * m.K + k] += inc;
+}
+
+// sampleDocument (LDAModel.hpp:326-354) over the token range [b, e).
+void sample_range(Model& m, int64_t d, uint64_t b, uint64_t e, int32_t* nk_local, uint32_t sweep, float* buf) {
+    for (uint64_t g = b; g < e; ++g) {
+        uint32_t v = m.words[g];
+        if (v >= (uint32_t)m.V) continue;
+        int zo = m.z[g];
+        add_token(m, d, v, zo, -1, nk_local);
+        float u = token_uniform(m.seed, g, sweep, TAG_SWEEP);
+        int zn = sample_one(m.K, m.scan_order, m.kpl, &m.ndk[(size_t)d * m.K], m.alpha.data(),
+                            &m.nkw[(size_t)v * m.K], m.eta, nk_local, m.veta, u, buf);
+        m.z[g] = (uint16_t)zn;
+        add_token(m, d, v, zn, +1, nk_local);
+    }
+}
+
+// Vocabulary chunks of the partition scheme (LDAModel.hpp:436-449): boundaries
+// at cumulative collection-frequency quantiles.  Missing trailing chunks (the
+// reference can produce fewer than W) are padded with V (empty chunks).
+std::vector<uint32_t> vocab_chunks(const Model& m, int W) {
+    std::vector<uint64_t> cf(m.V, 0);
+    for (uint32_t v : m.words) if (v < (uint32_t)m.V) ++cf[v];
+    uint64_t tot = 0;
+    for (auto c : cf) tot += c;
+    std::vector<uint32_t> ch;
+    uint64_t cum = 0;
+    for (int i = 0; i < m.V; ++i) {
+        cum += cf[i];
+        if (cum * (uint64_t)W >= tot * (uint64_t)(ch.size() + 1)) ch.push_back((uint32_t)(i + 1));
+    }
+    while ((int)ch.size() < W) ch.push_back((uint32_t)m.V);
+    ch.resize(W);
+    ch[W - 1] = (uint32_t)m.V;
+    return ch;
+}
+
+// One training sweep of the partition scheme with W workers
+// (LDAModel.hpp:373-399, merge 571-579, distribute 596-602).  W == 1 is the
+// `none` scheme (LDAModel.hpp:361-371).  Worker p keeps a private n_k that
+// starts from the global n_k; round i gives worker p the documents
+// d = id*W + ((i+p) % W) restricted to vocabulary chunk p; afterwards the
+// global n_k is the row sum of n_kw.  Cells of a round are disjoint in
+// documents and vocabulary, so running them on `threads` threads gives the
+// same result as running them one by one.
+void sweep_partition(Model& m, int W, int threads) {
+    const uint32_t sweep = m.global_step;
+    std::vector<uint32_t> ch = vocab_chunks(m, W);
+    // chunk offsets inside each document (LDAModel.hpp:451-469)
+    std::vector<uint64_t> coff((size_t)(W + 1) * m.D);
+    for (int64_t d = 0; d < m.D; ++d) {
+        uint64_t b = m.off[d], e = m.off[d + 1];
+        coff[d] = b;
+        for (int p = 0; p < W; ++p) {
+            uint32_t lim = ch[p];
+            coff[(size_t)(p + 1) * m.D + d] = (uint64_t)(std::lower_bound(m.words.begin() + b, m.words.begin() + e, lim) - m.words.begin());
+        }
+        coff[(size_t)W * m.D + d] = e;
+    }
+    std::vector<int32_t> nk_priv((size_t)W * m.K);
+    for (int p = 0; p < W; ++p) std::copy(m.nk.begin(), m.nk.end(), nk_priv.begin() + (size_t)p * m.K);
+    int T = std::max(1, std::min(threads, W));
+    for (int i = 0; i < W; ++i) {
+        auto work = [&](int tid) {
+            std::vector<float> buf(64 * 32 + m.K + 64);
+            std::vector<uint32_t> order;
+            for (int p = tid; p < W; p += T) {
+                int didx = (i + p) % W;
+                size_t n = (size_t)((m.D + (W - 1) - didx) / W);
+                uint32_t sseed = philox_word(m.seed, (uint32_t)p, (uint32_t)i, sweep, TAG_SHUFFLE);
+                shuffled_order(n, sseed, order);
+                for (size_t q = 0; q < n; ++q) {
+                    int64_t d = (int64_t)order[q] * W + didx;
+                    sample_range(m, d, coff[(size_t)p * m.D + d], coff[(size_t)(p + 1) * m.D + d],
+                                 &nk_priv[(size_t)p * m.K], sweep, buf.data());
+                }
+            }
+        };
+        if (T == 1) work(0);
+        else {
+            std::vector<std::thread> th;
+            for (int t = 0; t < T; ++t) th.emplace_back(work, t);
+            for (auto& t : th) t.join();
+        }
+    }
+    // mergeState<partition>: n_k = rowwise sum of n_kw
+    std::fill(m.nk.begin(), m.nk.end(), 0);
+    for (int v = 0; v < m.V; ++v)
+        for (int k = 0; k < m.K; ++k) m.nk[k] += m.nkw[(size_t)v * m.K + k];
+}
+
+// optimizeParameters (LDAModel.hpp:221-265): 10 Minka fixed-point passes.
+// Terms are float digammaApprox as in tomoto; sums are accumulated in double
+// (tomoto sums floats in Eigen's order); dAlpha is tomoto's LUT digamma.
+void optimize_alpha(Model& m) {
+    std::vector<int32_t> len(m.D);
+    for (int64_t d = 0; d < m.D; ++d) {
+        int32_t c = 0;
+        for (uint64_t g = m.off[d]; g < m.off[d + 1]; ++g) c += m.words[g] < (uint32_t)m.V;
+        len[d] = c;
+    }
+    for (int it = 0; it < 10; ++it) {
+        float asum = 0.0f;
+        for (int k = 0; k < m.K; ++k) asum += m.alpha[k];
+        double s = 0.0;
+        for (int64_t d = 0; d < m.D; ++d) s += (double)digamma_approx_f((float)len[d] + asum);
+        float denom = (float)(s - (double)m.D * (double)lut_digamma().get(asum));
+        for (int k = 0; k < m.K; ++k) {
+            float ak = m.alpha[k];
+            double sk = 0.0;
+            for (int64_t d = 0; d < m.D; ++d) sk += (double)digamma_approx_f((float)m.ndk[(size_t)d * m.K + k] + ak);
+            float nom = (float)(sk - (double)m.D * (double)lut_digamma().get(ak));
+            m.alpha[k] = std::max(nom / denom * ak, 1e-5f);
+        }
+    }
+}
+
+// getLLDocs / getLLRest (LDAModel.hpp:616-674) for symmetric eta.
+// mode 0: FP64 (double lgamma, double sums) -- the north-star LL.
+// mode 1: tomoto (float LUT lgamma terms, double accumulation).
+void loglik(const Model& m, int mode, double* ll_docs, double* ll_rest) {
+    auto lg = [&](double x) -> double { return mode ? (double)lut_lgamma().get((float)x) : std::lgamma(x); };
+    double asum_d = 0.0; float asum_f = 0.0f;
+    for (int k = 0; k < m.K; ++k) { asum_d += (double)m.alpha[k]; asum_f += m.alpha[k]; }
+    double asum = mode ? (double)asum_f : asum_d;
+    double docs = 0.0;
+    for (int64_t d = 0; d < m.D; ++d) {
+        int64_t L = 0;
+        for (uint64_t g = m.off[d]; g < m.off[d + 1]; ++g) L += m.words[g] < (uint32_t)m.V;
+        double Ld = mode ? (double)((float)L + asum_f) : (double)L + asum;
+        docs -= lg(Ld) - lg(asum);
+        for (int k = 0; k < m.K; ++k) {
+            int32_t c = m.ndk[(size_t)d * m.K + k];
+            double x = mode ? (double)((float)c + m.alpha[k]) : (double)c + (double)m.alpha[k];
+            docs += lg(x) - lg((double)m.alpha[k]);
+        }
+    }
+    double eta = (double)m.eta;
+    double veta = mode ? (double)m.veta : (double)m.V * eta;
+    double rest = lg(veta) * m.K;
+    double lge = lg(eta);
+    for (int k = 0; k < m.K; ++k) {
+        double x = mode ? (double)((float)m.nk[k] + m.veta) : (double)m.nk[k] + veta;
+        rest -= lg(x);
+        for (int v = 0; v < m.V; ++v) {
+            int32_t c = m.nkw[(size_t)v * m.K + k];
+            if (!c) continue;
+            double y = mode ? (double)((float)c + m.eta) : (double)c + eta;
+            rest += lg(y) - lge;
+        }
+    }
+    *ll_docs = docs;
+    *ll_rest = rest;
+}
+
+}  // namespace
+
+// ============================================================== C API ======
+extern "C" {
+
+void orc_philox4x32_10(const uint32_t* ctr, const uint32_t* key, uint32_t* out) { philox4x32_10(ctr, key, out); }
+
+float orc_token_uniform(uint64_t seed, uint64_t g, uint32_t sweep, uint32_t tag) { return token_uniform(seed, g, sweep, tag); }
+
+uint32_t orc_tag(int which) {
+    const uint32_t t[5] = {TAG_SWEEP, TAG_INIT, TAG_SHUFFLE, TAG_INFER, TAG_INFINIT};
+    return (which >= 0 && which < 5) ? t[which] : 0u;
+}
+
+int orc_device_kpl(int K) { return device_kpl(K); }
+
+void orc_weights(int K, const int32_t* ndk, const float* alpha, const int32_t* nkw, float eta,
+                 const int32_t* nk, float veta, float* out) {
+    for (int k = 0; k < K; ++k) out[k] = topic_weight(ndk[k], alpha[k], nkw[k], eta, nk[k], veta);
+}
+
+void orc_prefix_sum_tomoto(float* x, int n) { prefix_sum_tomoto(x, n); }
+void orc_prefix_sum_device(float* x, int kpl) { prefix_sum_device(x, kpl); }
+int orc_search(const float* cdf, int K, float u) { return search_cdf(cdf, K, u); }
+
+void orc_shuffled_order(uint64_t N, uint32_t seed, uint32_t* out) {
+    std::vector<uint32_t> o;
+    shuffled_order((size_t)N, seed, o);
+    std::copy(o.begin(), o.end(), out);
+}
+
+// Markstein-corrected quotient with a correctly rounded reciprocal:
+// q0 = a*y, r = fma(-q0, b, a), q = fma(r, y, q0).  Returns the number of
+// (a, b) pairs among n pseudo-random ones (a in [amin, amax), b in
+// [bmin, bmax), log-uniform) where q differs from the IEEE quotient a/b.
+uint64_t orc_markstein_mismatches(uint64_t n, uint64_t seed, double amin, double amax, double bmin, double bmax) {
+    uint64_t bad = 0;
+    double la = std::log(amin), ra = std::log(amax) - la, lb = std::log(bmin), rb = std::log(bmax) - lb;
+    for (uint64_t i = 0; i < n; ++i) {
+        uint32_t ctr[4] = {(uint32_t)i, (uint32_t)(i >> 32), 7u, 0x4D41524Bu};
+        uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+        uint32_t o[4];
+        philox4x32_10(ctr, key, o);
+        float a = (float)std::exp(la + ra * (o[0] * 0x1p-32));
+        float b = (float)std::exp(lb + rb * (o[1] * 0x1p-32));
+        if (o[2] & 1) b = std::nearbyint(b) + 0.01f * 5000.0f;  // integer count + V*eta shapes
+        float y = 1.0f / b;
+        float q0 = a * y;
+        float r = std::fma(-q0, b, a);
+        float q = std::fma(r, y, q0);
+        float ref = a / b;
+        if (std::memcmp(&q, &ref, 4) != 0) ++bad;
+    }
+    return bad;
+}
+
+// Frozen-state sampling (the pure per-token function): for each query token
+// t, remove its own assignment z_old[t] from (ndk row, nkw column, nk), then
+// compute weights, scan and search with uniform u[t].  ndk is D x K, nkw is
+// V x K (word-major), nk is K.  Counts are not modified.
+void orc_sample_frozen(int K, int scan_order, int kpl, const float* alpha, float eta, float veta,
+                       const int32_t* ndk, const int32_t* nkw, const int32_t* nk,
+                       int64_t n, const int64_t* doc, const uint32_t* word, const uint16_t* z_old,
+                       const float* u, uint16_t* z_new, float* total_out) {
+    std::vector<int32_t> a(K), c(K), e(K);
+    std::vector<float> buf(64 * 32 + K + 64);
+    for (int64_t t = 0; t < n; ++t) {
+        std::copy(ndk + (size_t)doc[t] * K, ndk + (size_t)doc[t] * K + K, a.begin());
+        std::copy(nkw + (size_t)word[t] * K, nkw + (size_t)word[t] * K + K, c.begin());
+        std::copy(nk, nk + K, e.begin());
+        int zo = z_old[t];
+        a[zo] -= 1; c[zo] -= 1; e[zo] -= 1;
+        z_new[t] = (uint16_t)sample_one(K, scan_order, kpl, a.data(), alpha, c.data(), eta, e.data(), veta, u[t], buf.data());
+        if (total_out) total_out[t] = buf[(scan_order ? 64 * kpl : K) - 1];
+    }
+}
+
+float orc_lgamma_lut(float x) { return lut_lgamma().get(x); }
+float orc_digamma_lut(float x) { return lut_digamma().get(x); }
+float orc_digamma_approx(float x) { return digamma_approx_f(x); }
+
+// ---- model handle ----
+void* orc_create(int K, int V, const float* alpha, float eta, uint64_t seed, int scan_order) {
+    Model* m = new Model();
+    m->K = K; m->V = V; m->eta = eta; m->seed = seed;
+    m->veta = eta * (float)V;
+    m->kpl = device_kpl(K);
+    m->scan_order = scan_order;
+    m->alpha.assign(alpha, alpha + K);
+    m->nkw.assign((size_t)V * K, 0);
+    m->nk.assign(K, 0);
+    return m;
+}
+
+void orc_destroy(void* h) { delete (Model*)h; }
+
+void orc_set_optim(void* h, uint32_t optim_interval, uint32_t burn_in) {
+    Model* m = (Model*)h;
+    m->optim_interval = optim_interval; m->burn_in = burn_in;
+}
+
+// words: sorted ascending within each document.  z may be null (then the
+// initial topics are drawn as below).
+void orc_set_corpus(void* h, const uint32_t* words, const uint64_t* off, int64_t D, const uint16_t* z) {
+    Model* m = (Model*)h;
+    m->D = D;
+    m->off.assign(off, off + D + 1);
+    uint64_t N = off[D];
+    m->words.assign(words, words + N);
+    m->z.assign(N, 0xFFFF);
+    m->ndk.assign((size_t)D * m->K, 0);
+    std::fill(m->nkw.begin(), m->nkw.end(), 0);
+    std::fill(m->nk.begin(), m->nk.end(), 0);
+    // initial draw z ~ Discrete(alpha) (LDAModel.hpp:754-762, 776-813): first k
+    // with r < cdf_alpha[k], cdf_alpha the sequential float sum of alpha.
+    std::vector<float> ca(m->K);
+    float s = 0.0f;
+    for (int k = 0; k < m->K; ++k) { s = s + m->alpha[k]; ca[k] = s; }
+    for (int64_t d = 0; d < D; ++d) {
+        for (uint64_t g = off[d]; g < off[d + 1]; ++g) {
+            uint32_t v = words[g];
+            if (v >= (uint32_t)m->V) continue;
+            int k = z ? z[g] : search_cdf(ca.data(), m->K, token_uniform(m->seed, g, 0, TAG_INIT));
+            m->z[g] = (uint16_t)k;
+            add_token(*m, d, v, k, +1, m->nk.data());
+        }
+    }
+}
+
+void orc_set_alpha(void* h, const float* alpha) { Model* m = (Model*)h; m->alpha.assign(alpha, alpha + m->K); }
+void orc_get_alpha(void* h, float* alpha) { Model* m = (Model*)h; std::copy(m->alpha.begin(), m->alpha.end(), alpha); }
+uint32_t orc_global_step(void* h) { return ((Model*)h)->global_step; }
+
+// TopicModel::train (TopicModel.hpp:684-755) + trainOne (LDAModel.hpp:505-539):
+// per iteration one sweep (partition with W workers; W == 1 is `none`),
+// then alpha optimisation when globalStep >= burnIn and
+// (globalStep+1) % optimInterval == 0, then ++globalStep.
+void orc_train(void* h, int iterations, int workers, int threads) {
+    Model* m = (Model*)h;
+    int W = std::max(1, workers);
+    for (int it = 0; it < iterations; ++it) {
+        sweep_partition(*m, W, threads);
+        if (m->global_step >= m->burn_in && m->optim_interval && (m->global_step + 1) % m->optim_interval == 0)
+            optimize_alpha(*m);
+        ++m->global_step;
+    }
+}
+
+void orc_optimize_alpha(void* h) { optimize_alpha(*(Model*)h); }
+
+void orc_get_state(void* h, uint16_t* z, int32_t* ndk, int32_t* nkw, int32_t* nk) {
+    Model* m = (Model*)h;
+    if (z) std::copy(m->z.begin(), m->z.end(), z);
+    if (ndk) std::copy(m->ndk.begin(), m->ndk.end(), ndk);
+    if (nkw) std::copy(m->nkw.begin(), m->nkw.end(), nkw);
+    if (nk) std::copy(m->nk.begin(), m->nk.end(), nk);
+}
+
+void orc_loglik(void* h, int mode, double* ll_docs, double* ll_rest) { loglik(*(Model*)h, mode, ll_docs, ll_rest); }
+
+// Fold-in of unseen documents, together=False (LDAModel.hpp:920-953): every
+// document starts its own RNG stream (tomoto default-constructs `rgc{}` per
+// document, so streams do not depend on the model seed; here they are keyed
+// by `infer_seed` and the token position inside the document), draws its
+// initial topics from Discrete(alpha), adds its tokens into a private copy of
+// n_kw / n_k, and runs `iters` sweeps.  The private copy is represented
+// exactly by its delta: n_k,eff = n_k + n_dk and n_kw,eff = n_kw + c_d.
+// ll = getLLRest(private) - getLLRest(global) + getLLDocs(doc), evaluated in
+// FP64 (mode 0) or with tomoto's LUT (mode 1) over the touched terms only.
+// theta = (n_dk + alpha) / (L + sum(alpha)) (LDAModel.hpp:1145-1159).
+void orc_infer(void* h, const uint32_t* words, const uint64_t* off, int64_t D, int iters,
+               uint64_t infer_seed, int mode, float* theta, double* ll, uint16_t* z_out) {
+    Model* m = (Model*)h;
+    const int K = m->K;
+    std::vector<float> ca(K);
+    float s = 0.0f;
+    for (int k = 0; k < K; ++k) { s = s + m->alpha[k]; ca[k] = s; }
+    float asum_f = s;
+    double asum_d = 0.0;
+    for (int k = 0; k < K; ++k) asum_d += (double)m->alpha[k];
+    std::vector<float> buf(64 * 32 + K + 64);
+    auto lg = [&](double x) -> double { return mode ? (double)lut_lgamma().get((float)x) : std::lgamma(x); };
+    for (int64_t d = 0; d < D; ++d) {
+        uint64_t b = off[d], e = off[d + 1];
+        uint64_t n = e - b;
+        std::vector<uint16_t> z(n, 0xFFFF);
+        std::vector<int32_t> ndk(K, 0), nk_eff(m->nk);
+        // doc-local deltas of n_kw, keyed by (word, topic); words are sorted,
+        // so a word's tokens are one contiguous run.
+        std::vector<int32_t> cd((size_t)n * K, 0);  // run-start index x K
+        std::vector<uint64_t> run_start(n);
+        for (uint64_t i = 0; i < n; ++i) run_start[i] = (i && words[b + i] == words[b + i - 1]) ? run_start[i - 1] : i;
+        for (uint64_t i = 0; i < n; ++i) {
+            uint32_t v = words[b + i];
+            if (v >= (uint32_t)m->V) continue;
+            int k = search_cdf(ca.data(), K, token_uniform(infer_seed, i, 0, TAG_INFINIT));
+            z[i] = (uint16_t)k;
+            ndk[k]++; nk_eff[k]++; cd[run_start[i] * K + k]++;
+        }
+        std::vector<int32_t> col(K);
+        for (int it = 0; it < iters; ++it) {
+            for (uint64_t i = 0; i < n; ++i) {
+                uint32_t v = words[b + i];
+                if (v >= (uint32_t)m->V) continue;
+                int zo = z[i];
+                int32_t* c = &cd[run_start[i] * K];
+                ndk[zo]--; nk_eff[zo]--; c[zo]--;
+                for (int k = 0; k < K; ++k) col[k] = m->nkw[(size_t)v * K + k] + c[k];
+                float u = token_uniform(infer_seed, i, (uint32_t)(it + 1), TAG_INFER);
+                int zn = sample_one(K, m->scan_order, m->kpl, ndk.data(), m->alpha.data(), col.data(), m->eta,
+                                    nk_eff.data(), m->veta, u, buf.data());
+                z[i] = (uint16_t)zn;
+                ndk[zn]++; nk_eff[zn]++; c[zn]++;
+            }
+        }
+        int64_t L = 0;
+        for (uint64_t i = 0; i < n; ++i) L += words[b + i] < (uint32_t)m->V;
+        double eta = (double)m->eta;
+        double veta = mode ? (double)m->veta : (double)m->V * eta;
+        double asum = mode ? (double)asum_f : asum_d;
+        double l = 0.0;
+        // getLLRest(private) - getLLRest(global): n_k terms and touched n_kw terms
+        for (int k = 0; k < K; ++k) {
+            if (!ndk[k]) continue;
+            double x0 = mode ? (double)((float)m->nk[k] + m->veta) : (double)m->nk[k] + veta;
+            double x1 = mode ? (double)((float)(m->nk[k] + ndk[k]) + m->veta) : (double)(m->nk[k] + ndk[k]) + veta;
+            l += lg(x0) - lg(x1);
+        }
+        for (uint64_t i = 0; i < n; ++i) {
+            if (run_start[i] != i) continue;
+            uint32_t v = words[b + i];
+            if (v >= (uint32_t)m->V) continue;
+            for (int k = 0; k < K; ++k) {
+                int32_t c = cd[i * K + k];
+                if (!c) continue;
+                int32_t g0 = m->nkw[(size_t)v * K + k];
+                double y0 = mode ? (double)((float)g0 + m->eta) : (double)g0 + eta;
+                double y1 = mode ? (double)((float)(g0 + c) + m->eta) : (double)(g0 + c) + eta;
+                // tomoto skips zero cells; lgamma(0+eta)-lgamma(eta) is 0 either way
+                l += (lg(y1) - lg(eta)) - (g0 ? (lg(y0) - lg(eta)) : 0.0);
+            }
+        }
+        // getLLDocs(doc)
+        double Ld = mode ? (double)((float)L + asum_f) : (double)L + asum;
+        l -= lg(Ld) - lg(asum);
+        for (int k = 0; k < K; ++k) {
+            double x = mode ? (double)((float)ndk[k] + m->alpha[k]) : (double)ndk[k] + (double)m->alpha[k];
+            l += lg(x) - lg((double)m->alpha[k]);
+        }
+        if (ll) ll[d] = l;
+        if (theta) {
+            for (int k = 0; k < K; ++k) theta[(size_t)d * K + k] = ((float)ndk[k] + m->alpha[k]) / ((float)L + asum_f);
+        }
+        if (z_out) std::copy(z.begin(), z.end(), z_out + b);
+    }
+}
+
+// Brute-force fold-in LL for one finished document: copies the whole K x V
+// state, adds the document, and evaluates the full getLLRest difference
+// (LDAModel.hpp:931, 945-946).  Used to pin the delta form above.
+double orc_infer_ll_bruteforce(void* h, const uint32_t* words, uint64_t n, const uint16_t* z) {
+    Model* m = (Model*)h;
+    Model tmp = *m;
+    int64_t L = 0;
+    std::vector<int32_t> ndk(m->K, 0);
+    for (uint64_t i = 0; i < n; ++i) {
+        if (words[i] >= (uint32_t)m->V) continue;
+        int k = z[i];
+        tmp.nkw[(size_t)words[i] * m->K + k]++;
+        tmp.nk[k]++;
+        ndk[k]++;
+        ++L;
+    }
+    auto rest = [&](const Model& s) {
+        double eta = (double)s.eta, veta = (double)s.V * eta;
+        double r = std::lgamma(veta) * s.K;
+        for (int k = 0; k < s.K; ++k) {
+            r -= std::lgamma((double)s.nk[k] + veta);
+            for (int v = 0; v < s.V; ++v) {
+                int32_t c = s.nkw[(size_t)v * s.K + k];
+                if (c) r += std::lgamma((double)c + eta) - std::lgamma(eta);
+            }
+        }
+        return r;
+    };
+    double l = rest(tmp) - rest(*m);
+    double asum = 0.0;
+    for (int k = 0; k < m->K; ++k) asum += (double)m->alpha[k];
+    l -= std::lgamma((double)L + asum) - std::lgamma(asum);
+    for (int k = 0; k < m->K; ++k) l += std::lgamma((double)ndk[k] + (double)m->alpha[k]) - std::lgamma((double)m->alpha[k]);
+    return l;
+}
+
+}  // extern "C"

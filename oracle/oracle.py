@@ -1,0 +1,231 @@
+"""ctypes wrapper of the CPU oracle (oracle/lda_oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker or the timed CPU baseline.  The
+product package never imports this module.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "liblda_oracle.so")
+
+TAGS = {"sweep": 0, "init": 1, "shuffle": 2, "infer": 3, "infinit": 4}
+
+_c_p = ctypes.c_void_p
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    lib = ctypes.CDLL(LIB_PATH)
+    i32, i64, u32, u64, f32, f64 = (ctypes.c_int, ctypes.c_int64, ctypes.c_uint32, ctypes.c_uint64,
+                                    ctypes.c_float, ctypes.c_double)
+    sigs = {
+        "orc_philox4x32_10": (None, [_c_p, _c_p, _c_p]),
+        "orc_token_uniform": (f32, [u64, u64, u32, u32]),
+        "orc_tag": (u32, [i32]),
+        "orc_device_kpl": (i32, [i32]),
+        "orc_weights": (None, [i32, _c_p, _c_p, _c_p, f32, _c_p, f32, _c_p]),
+        "orc_prefix_sum_tomoto": (None, [_c_p, i32]),
+        "orc_prefix_sum_device": (None, [_c_p, i32]),
+        "orc_search": (i32, [_c_p, i32, f32]),
+        "orc_shuffled_order": (None, [u64, u32, _c_p]),
+        "orc_markstein_mismatches": (u64, [u64, u64, f64, f64, f64, f64]),
+        "orc_sample_frozen": (None, [i32, i32, i32, _c_p, f32, f32, _c_p, _c_p, _c_p, i64, _c_p, _c_p, _c_p, _c_p,
+                                     _c_p, _c_p]),
+        "orc_lgamma_lut": (f32, [f32]),
+        "orc_digamma_lut": (f32, [f32]),
+        "orc_digamma_approx": (f32, [f32]),
+        "orc_create": (_c_p, [i32, i32, _c_p, f32, u64, i32]),
+        "orc_destroy": (None, [_c_p]),
+        "orc_set_optim": (None, [_c_p, u32, u32]),
+        "orc_set_corpus": (None, [_c_p, _c_p, _c_p, i64, _c_p]),
+        "orc_set_alpha": (None, [_c_p, _c_p]),
+        "orc_get_alpha": (None, [_c_p, _c_p]),
+        "orc_global_step": (u32, [_c_p]),
+        "orc_train": (None, [_c_p, i32, i32, i32]),
+        "orc_optimize_alpha": (None, [_c_p]),
+        "orc_get_state": (None, [_c_p, _c_p, _c_p, _c_p, _c_p]),
+        "orc_loglik": (None, [_c_p, i32, _c_p, _c_p]),
+        "orc_infer": (None, [_c_p, _c_p, _c_p, i64, i32, u64, i32, _c_p, _c_p, _c_p]),
+        "orc_infer_ll_bruteforce": (f64, [_c_p, _c_p, u64, _c_p]),
+    }
+    for name, (res, args) in sigs.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def philox(ctr, key):
+    c = np.ascontiguousarray(ctr, dtype=np.uint32)
+    k = np.ascontiguousarray(key, dtype=np.uint32)
+    out = np.zeros(4, dtype=np.uint32)
+    load().orc_philox4x32_10(_p(c), _p(k), _p(out))
+    return out
+
+
+def tag(name):
+    return load().orc_tag(TAGS[name])
+
+
+def token_uniform(seed, g, sweep, tag_name="sweep"):
+    return load().orc_token_uniform(seed, g, sweep, tag(tag_name))
+
+
+def device_kpl(K):
+    return load().orc_device_kpl(K)
+
+
+def weights(ndk, alpha, nkw, eta, nk, veta):
+    K = len(ndk)
+    out = np.zeros(K, dtype=np.float32)
+    load().orc_weights(K, _p(np.ascontiguousarray(ndk, np.int32)), _p(np.ascontiguousarray(alpha, np.float32)),
+                       _p(np.ascontiguousarray(nkw, np.int32)), float(eta),
+                       _p(np.ascontiguousarray(nk, np.int32)), float(veta), _p(out))
+    return out
+
+
+def prefix_sum_tomoto(x):
+    a = np.array(x, dtype=np.float32)
+    load().orc_prefix_sum_tomoto(_p(a), len(a))
+    return a
+
+
+def prefix_sum_device(x, kpl):
+    a = np.zeros(64 * kpl, dtype=np.float32)
+    a[:len(x)] = x
+    load().orc_prefix_sum_device(_p(a), kpl)
+    return a
+
+
+def search(cdf, K, u):
+    return load().orc_search(_p(np.ascontiguousarray(cdf, np.float32)), K, float(u))
+
+
+def shuffled_order(N, seed):
+    out = np.zeros(N, dtype=np.uint32)
+    load().orc_shuffled_order(N, seed, _p(out))
+    return out
+
+
+def markstein_mismatches(n, seed, amin, amax, bmin, bmax):
+    return load().orc_markstein_mismatches(n, seed, amin, amax, bmin, bmax)
+
+
+def sample_frozen(K, scan_order, alpha, eta, veta, ndk, nkw, nk, doc, word, z_old, u):
+    kpl = device_kpl(K)
+    n = len(doc)
+    zn = np.zeros(n, dtype=np.uint16)
+    tot = np.zeros(n, dtype=np.float32)
+    load().orc_sample_frozen(K, scan_order, kpl, _p(np.ascontiguousarray(alpha, np.float32)), float(eta), float(veta),
+                             _p(np.ascontiguousarray(ndk, np.int32)), _p(np.ascontiguousarray(nkw, np.int32)),
+                             _p(np.ascontiguousarray(nk, np.int32)), n, _p(np.ascontiguousarray(doc, np.int64)),
+                             _p(np.ascontiguousarray(word, np.uint32)), _p(np.ascontiguousarray(z_old, np.uint16)),
+                             _p(np.ascontiguousarray(u, np.float32)), _p(zn), _p(tot))
+    return zn, tot
+
+
+def lgamma_lut(x):
+    return load().orc_lgamma_lut(float(x))
+
+
+def digamma_lut(x):
+    return load().orc_digamma_lut(float(x))
+
+
+def digamma_approx(x):
+    return load().orc_digamma_approx(float(x))
+
+
+class Model:
+    """The oracle's plain-LDA model (tomoto semantics, Philox stream)."""
+
+    def __init__(self, K, V, alpha, eta, seed, scan_order=1):
+        self.K, self.V = K, V
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(alpha, np.float32), (K,)), dtype=np.float32)
+        self._h = load().orc_create(K, V, _p(a), float(eta), int(seed) & (2**64 - 1), int(scan_order))
+        self.D = 0
+        self.N = 0
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            load().orc_destroy(self._h)
+            self._h = None
+
+    def set_optim(self, interval, burn_in=0):
+        load().orc_set_optim(self._h, interval, burn_in)
+
+    def set_corpus(self, words, off, z=None):
+        self._words = np.ascontiguousarray(words, np.uint32)
+        self._off = np.ascontiguousarray(off, np.uint64)
+        zz = None if z is None else np.ascontiguousarray(z, np.uint16)
+        self.D = len(off) - 1
+        self.N = int(off[-1])
+        load().orc_set_corpus(self._h, _p(self._words), _p(self._off), self.D, _p(zz))
+
+    def set_alpha(self, alpha):
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(alpha, np.float32), (self.K,)), dtype=np.float32)
+        load().orc_set_alpha(self._h, _p(a))
+
+    def alpha(self):
+        a = np.zeros(self.K, np.float32)
+        load().orc_get_alpha(self._h, _p(a))
+        return a
+
+    def train(self, iterations, workers=1, threads=1):
+        load().orc_train(self._h, iterations, workers, threads)
+
+    def optimize_alpha(self):
+        load().orc_optimize_alpha(self._h)
+
+    @property
+    def global_step(self):
+        return load().orc_global_step(self._h)
+
+    def state(self):
+        z = np.zeros(self.N, np.uint16)
+        ndk = np.zeros((self.D, self.K), np.int32)
+        nkw = np.zeros((self.V, self.K), np.int32)
+        nk = np.zeros(self.K, np.int32)
+        load().orc_get_state(self._h, _p(z), _p(ndk), _p(nkw), _p(nk))
+        return z, ndk, nkw, nk
+
+    def loglik(self, mode=0):
+        a = ctypes.c_double()
+        b = ctypes.c_double()
+        load().orc_loglik(self._h, mode, ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
+
+    def infer(self, words, off, iters, infer_seed=0, mode=0):
+        words = np.ascontiguousarray(words, np.uint32)
+        off = np.ascontiguousarray(off, np.uint64)
+        D = len(off) - 1
+        theta = np.zeros((D, self.K), np.float32)
+        ll = np.zeros(D, np.float64)
+        z = np.zeros(int(off[-1]), np.uint16)
+        load().orc_infer(self._h, _p(words), _p(off), D, iters, int(infer_seed) & (2**64 - 1), mode, _p(theta), _p(ll),
+                         _p(z))
+        return theta, ll, z
+
+    def infer_ll_bruteforce(self, words, z):
+        words = np.ascontiguousarray(words, np.uint32)
+        z = np.ascontiguousarray(z, np.uint16)
+        return load().orc_infer_ll_bruteforce(self._h, _p(words), len(words), _p(z))

@@ -1,0 +1,238 @@
+"""ctypes binding of the C ABI in include/tomotopy_amd.h.
+
+The HIP library is the only compute path.  If it is missing, or no HIP device
+is visible, every entry point raises; nothing falls back to a CPU sampler.
+"""
+import ctypes
+import os
+import re
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libtomotopy_amd.so")
+HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "tomotopy_amd.h")
+
+LDA_OK = 0
+LDA_ERR_INVALID_ARGUMENT = -1
+LDA_ERR_STATE = -2
+LDA_ERR_DEVICE = -3
+LDA_ERR_OUT_OF_MEMORY = -4
+LDA_ERR_UNSUPPORTED = -5
+LDA_ERR_TRAINING = -6
+
+LDA_MAX_K = 2048
+
+_c_p = ctypes.c_void_p
+_i32 = ctypes.c_int
+_i64 = ctypes.c_int64
+_u32 = ctypes.c_uint32
+_u64 = ctypes.c_uint64
+_f32 = ctypes.c_float
+_dp = ctypes.POINTER(ctypes.c_double)
+
+# name -> argtypes (restype is int for all but lda_last_error)
+_SIGS = {
+    "lda_last_error": [],
+    "lda_version": [],
+    "lda_device_count": [_c_p],
+    "lda_ctx_create": [_i32, _i32, _c_p, _f32, _i32, _u64, _i32, _c_p],
+    "lda_ctx_destroy": [_c_p],
+    "lda_upload_corpus": [_c_p, _c_p, _c_p, _i64, _c_p],
+    "lda_set_counts": [_c_p, _c_p, _c_p],
+    "lda_set_alpha": [_c_p, _c_p],
+    "lda_get_alpha": [_c_p, _c_p],
+    "lda_set_optim": [_c_p, _u32, _u32],
+    "lda_get_global_step": [_c_p, _c_p],
+    "lda_set_global_step": [_c_p, _u64],
+    "lda_train": [_c_p, _i32, _i32, _i32],
+    "lda_sweep": [_c_p, _i32, _i32],
+    "lda_optimize_alpha": [_c_p],
+    "lda_loglik": [_c_p, _c_p, _c_p],
+    "lda_check_invariants": [_c_p, _c_p],
+    "lda_download_state": [_c_p, _c_p, _c_p, _c_p, _c_p],
+    "lda_get_dims": [_c_p, _c_p],
+    "lda_sample_frozen": [_c_p, _i64, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p],
+    "lda_infer": [_c_p, _c_p, _c_p, _i64, _i32, _u64, _i32, _c_p, _c_p, _c_p],
+    "lda_kernel_stats": [_c_p, _c_p],
+}
+
+_lib = None
+
+
+class NativeLibraryMissing(RuntimeError):
+    pass
+
+
+def header_symbols():
+    """Function names declared in include/tomotopy_amd.h."""
+    with open(HEADER_PATH) as f:
+        text = f.read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\*?\s+\*?(lda_\w+)\s*\(", text, re.M)))
+
+
+def load():
+    """Load the HIP library (raises NativeLibraryMissing when it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise NativeLibraryMissing(
+            "tomotopy_amd HIP library not found at %s; build it with `python -m tomotopy_amd.build`" % LIB_PATH)
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, args in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = ctypes.c_char_p if name == "lda_last_error" else ctypes.c_int
+    _lib = lib
+    return lib
+
+
+def _check(rc):
+    if rc == LDA_OK:
+        return
+    msg = load().lda_last_error().decode("utf-8", "replace")
+    if rc == LDA_ERR_INVALID_ARGUMENT:
+        raise ValueError(msg)
+    if rc == LDA_ERR_OUT_OF_MEMORY:
+        raise MemoryError(msg)
+    if rc == LDA_ERR_UNSUPPORTED:
+        raise NotImplementedError(msg)
+    if rc == LDA_ERR_TRAINING:
+        raise RuntimeError("Train failed: " + msg)
+    raise RuntimeError(msg)
+
+
+def device_count():
+    n = ctypes.c_int(0)
+    _check(load().lda_device_count(ctypes.byref(n)))
+    return n.value
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Context:
+    """One model's device state (n_kw, n_k, n_dk, z, corpus) on one GPU."""
+
+    def __init__(self, k, v, alpha, eta, seed, tw=0, device=0):
+        lib = load()
+        alpha = np.ascontiguousarray(np.broadcast_to(np.asarray(alpha, dtype=np.float32), (k,)), dtype=np.float32)
+        h = ctypes.c_void_p()
+        _check(lib.lda_ctx_create(int(k), int(v), _ptr(alpha), float(eta), int(tw), int(seed) & (2**64 - 1),
+                                  int(device), ctypes.byref(h)))
+        self._h = h
+        self.k = int(k)
+        self.v = int(v)
+        self.n_docs = 0
+        self.n_tokens = 0
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            load().lda_ctx_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload_corpus(self, words, doc_off, z=None):
+        words = np.ascontiguousarray(words, dtype=np.uint32)
+        doc_off = np.ascontiguousarray(doc_off, dtype=np.uint64)
+        if z is not None:
+            z = np.ascontiguousarray(z, dtype=np.uint16)
+        D = len(doc_off) - 1
+        _check(load().lda_upload_corpus(self._h, _ptr(words), _ptr(doc_off), D, _ptr(z)))
+        self.n_docs = D
+        self.n_tokens = int(doc_off[-1]) if D >= 0 and len(doc_off) else 0
+
+    def set_counts(self, nkw, nk):
+        nkw = np.ascontiguousarray(nkw, dtype=np.int32)
+        nk = np.ascontiguousarray(nk, dtype=np.int32)
+        _check(load().lda_set_counts(self._h, _ptr(nkw), _ptr(nk)))
+
+    def set_alpha(self, alpha):
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(alpha, dtype=np.float32), (self.k,)), dtype=np.float32)
+        _check(load().lda_set_alpha(self._h, _ptr(a)))
+
+    def get_alpha(self):
+        a = np.zeros(self.k, dtype=np.float32)
+        _check(load().lda_get_alpha(self._h, _ptr(a)))
+        return a
+
+    def set_optim(self, optim_interval, burn_in):
+        _check(load().lda_set_optim(self._h, int(optim_interval), int(burn_in)))
+
+    @property
+    def global_step(self):
+        s = ctypes.c_uint64(0)
+        _check(load().lda_get_global_step(self._h, ctypes.byref(s)))
+        return s.value
+
+    @global_step.setter
+    def global_step(self, v):
+        _check(load().lda_set_global_step(self._h, int(v)))
+
+    def train(self, iterations, workers=0, scheme=0):
+        _check(load().lda_train(self._h, int(iterations), int(workers), int(scheme)))
+
+    def sweep(self, workers=0, scheme=0):
+        _check(load().lda_sweep(self._h, int(workers), int(scheme)))
+
+    def optimize_alpha(self):
+        _check(load().lda_optimize_alpha(self._h))
+
+    def loglik(self):
+        a = ctypes.c_double(0.0)
+        b = ctypes.c_double(0.0)
+        _check(load().lda_loglik(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def check_invariants(self):
+        f = np.zeros(8, dtype=np.int64)
+        _check(load().lda_check_invariants(self._h, _ptr(f)))
+        return f
+
+    def dims(self):
+        d = np.zeros(6, dtype=np.int64)
+        _check(load().lda_get_dims(self._h, _ptr(d)))
+        return d
+
+    def download_state(self, z=True, ndk=True, nkw=True, nk=True):
+        K, V, D, N = (int(x) for x in self.dims()[:4])
+        zz = np.zeros(N, dtype=np.uint16) if z else None
+        a = np.zeros((D, K), dtype=np.int32) if ndk else None
+        b = np.zeros((V, K), dtype=np.int32) if nkw else None
+        c = np.zeros(K, dtype=np.int32) if nk else None
+        _check(load().lda_download_state(self._h, _ptr(zz), _ptr(a), _ptr(b), _ptr(c)))
+        return zz, a, b, c
+
+    def sample_frozen(self, doc, word, z_old, u):
+        doc = np.ascontiguousarray(doc, dtype=np.int64)
+        word = np.ascontiguousarray(word, dtype=np.uint32)
+        z_old = np.ascontiguousarray(z_old, dtype=np.uint16)
+        u = np.ascontiguousarray(u, dtype=np.float32)
+        n = len(doc)
+        zn = np.zeros(n, dtype=np.uint16)
+        tot = np.zeros(n, dtype=np.float32)
+        _check(load().lda_sample_frozen(self._h, n, _ptr(doc), _ptr(word), _ptr(z_old), _ptr(u), _ptr(zn), _ptr(tot)))
+        return zn, tot
+
+    def infer(self, words, doc_off, iterations, infer_seed=0, mode=0, want_z=False):
+        words = np.ascontiguousarray(words, dtype=np.uint32)
+        doc_off = np.ascontiguousarray(doc_off, dtype=np.uint64)
+        D = len(doc_off) - 1
+        theta = np.zeros((max(D, 0), self.k), dtype=np.float32)
+        ll = np.zeros(max(D, 0), dtype=np.float64)
+        z = np.zeros(int(doc_off[-1]) if D > 0 else 0, dtype=np.uint16) if want_z else None
+        _check(load().lda_infer(self._h, _ptr(words), _ptr(doc_off), D, int(iterations), int(infer_seed) & (2**64 - 1),
+                                int(mode), _ptr(theta), _ptr(ll), _ptr(z)))
+        return theta, ll, z
+
+    def kernel_stats(self):
+        s = np.zeros(4, dtype=np.float64)
+        _check(load().lda_kernel_stats(self._h, _ptr(s)))
+        return {"ms": float(s[0]), "launches": int(s[1]), "tokens": float(s[2])}

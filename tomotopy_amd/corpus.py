@@ -1,0 +1,92 @@
+"""Host-side corpus preparation: vocabulary ordering and the token CSR.
+
+Restates tomoto's prepare() data steps for plain LDA:
+  * removeStopwords (src/TopicModel/TopicModel.hpp:526-571): words are ordered
+    by (collection frequency, document frequency) descending with a stable
+    sort; words below min_cf / min_df go after the kept ones; the rm_top most
+    frequent kept words are rotated to the very end; realV is the number of
+    kept words, and ids >= realV are skipped by the sampler.
+  * sortAndWriteOrder (src/Utils/Utils.hpp:227-248): each document's words are
+    stably sorted ascending and w_order[old_position] = new_position.
+  * updateWeakArray (TopicModel.hpp:482-494): all documents' tokens in one
+    array with per-document offsets (the CSR uploaded to the device).
+"""
+import numpy as np
+
+
+def vocab_order(cf, df, min_cf=0, min_df=0, rm_top=0):
+    """Return (order, real_v): order[old_id] = new_id, as removeStopwords."""
+    cf = np.asarray(cf, dtype=np.int64)
+    df = np.asarray(df, dtype=np.int64)
+    n = len(cf)
+    failed = (cf < min_cf) | (df < min_df)
+    # stable sort: kept words first, each group by (cf, df) descending
+    idx = np.lexsort((np.arange(n), -df, -cf, failed.astype(np.int64)))
+    rot = min(int(rm_top), n)
+    if rot:
+        idx = np.concatenate([idx[rot:], idx[:rot]])
+    sorted_failed = failed[idx]
+    limit = n - rot
+    hits = np.nonzero(sorted_failed[:limit])[0]
+    real_v = int(hits[0]) if len(hits) else limit
+    order = np.empty(n, dtype=np.int64)
+    order[idx] = np.arange(n)
+    return order, real_v
+
+
+def counts(docs_words, n_words):
+    """Collection and document frequency of every word id."""
+    cf = np.zeros(n_words, dtype=np.int64)
+    df = np.zeros(n_words, dtype=np.int64)
+    for w in docs_words:
+        w = np.asarray(w, dtype=np.int64)
+        if len(w):
+            cf += np.bincount(w, minlength=n_words)
+            df[np.unique(w)] += 1
+    return cf, df
+
+
+def build_csr(docs_words, order):
+    """Remap ids by `order`, sort each document, and concatenate.
+
+    Returns (words uint32, doc_off uint64, w_order list of int arrays).
+    """
+    lens = np.fromiter((len(w) for w in docs_words), dtype=np.int64, count=len(docs_words))
+    off = np.zeros(len(docs_words) + 1, dtype=np.uint64)
+    np.cumsum(lens, out=off[1:])
+    words = np.empty(int(off[-1]), dtype=np.uint32)
+    w_orders = []
+    for d, w in enumerate(docs_words):
+        mapped = order[np.asarray(w, dtype=np.int64)] if len(w) else np.zeros(0, np.int64)
+        perm = np.argsort(mapped, kind="stable")
+        b = int(off[d])
+        words[b:b + len(w)] = mapped[perm]
+        wo = np.empty(len(w), dtype=np.int64)
+        wo[perm] = np.arange(len(w))
+        w_orders.append(wo)
+    return words, off, w_orders
+
+
+def csr_from_arrays(words, doc_off, order):
+    """Vectorised build_csr for corpora already in CSR form (raw ids)."""
+    words = np.asarray(words)
+    doc_off = np.asarray(doc_off, dtype=np.uint64)
+    D = len(doc_off) - 1
+    mapped = np.asarray(order, dtype=np.int64)[words.astype(np.int64)]
+    doc_id = np.repeat(np.arange(D, dtype=np.int64), np.diff(doc_off).astype(np.int64))
+    perm = np.lexsort((np.arange(len(mapped)), mapped, doc_id))
+    return mapped[perm].astype(np.uint32), doc_off.copy()
+
+
+def prepare_arrays(words, doc_off, n_words, min_cf=0, min_df=0, rm_top=0):
+    """tomoto prepare() on a raw-id CSR: returns (words, doc_off, real_v, order)."""
+    words = np.asarray(words, dtype=np.int64)
+    doc_off = np.asarray(doc_off, dtype=np.uint64)
+    D = len(doc_off) - 1
+    cf = np.bincount(words, minlength=n_words).astype(np.int64)
+    doc_id = np.repeat(np.arange(D, dtype=np.int64), np.diff(doc_off).astype(np.int64))
+    pairs = np.unique(doc_id * np.int64(n_words) + words)
+    df = np.bincount(pairs % n_words, minlength=n_words).astype(np.int64)
+    order, real_v = vocab_order(cf, df, min_cf, min_df, rm_top)
+    w, off = csr_from_arrays(words, doc_off, order)
+    return w, off, real_v, order

@@ -1,0 +1,828 @@
+// lda_capi.cpp -- host core and C ABI of the MI355X LDA engine.
+//
+// The host core plays the part of tomoto's TopicModel / LDAModel driver for
+// plain LDA (src/TopicModel/TopicModel.hpp:684-755, LDAModel.hpp:356-674):
+// it owns the device state of one model, resolves the parallel scheme,
+// computes the partition schedule, runs the iteration loop with the alpha
+// fixed point, and reduces the FP64 log-likelihood partials.  The per-token
+// work runs in the kernels of lda_device.hip.  Nothing here falls back to a
+// CPU sampler: without a HIP device every call fails with LDA_ERR_DEVICE.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/tomotopy_amd.h"
+#include "lda_internal.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                                  \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess)                                                                          \
+            return fail(e_ == hipErrorOutOfMemory ? LDA_ERR_OUT_OF_MEMORY : LDA_ERR_DEVICE, "%s: %s (%s:%d)", \
+                        #expr, hipGetErrorString(e_), __FILE__, __LINE__);                             \
+    } while (0)
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    hipError_t alloc(size_t count) {
+        if (count <= n && p) return hipSuccess;
+        release();
+        size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+        hipError_t e = hipMalloc((void**)&p, bytes);
+        if (e == hipSuccess) n = count;
+        else p = nullptr;
+        return e;
+    }
+};
+
+// tomoto's float LUT digamma (src/Utils/LUT.hpp:10-79, math.h:134-154), used
+// for dAlpha in the alpha fixed point (LDAModel.hpp:224).  Grid values are a
+// double-precision digamma rounded to float; steps 1/1024, 1/64, 1/4.
+double digamma_d(double x) {
+    double r = 0.0;
+    while (x < 6.0) {
+        r -= 1.0 / x;
+        x += 1.0;
+    }
+    double f = 1.0 / (x * x);
+    double t = f * (-1.0 / 12 + f * (1.0 / 120 + f * (-1.0 / 252 + f * (1.0 / 240 + f * (-1.0 / 132)))));
+    return r + std::log(x) - 0.5 / x + t;
+}
+
+struct DigammaLut {
+    static constexpr int N = 1024, M = 6400, L = 4000;
+    const float P = 1.0f / 1024, Q = 1.0f / 64, R = 1.0f / 4;
+    std::vector<float> pts;
+    DigammaLut() : pts(N + M + L) {
+        for (int i = 0; i < N; ++i) pts[i] = (float)digamma_d(i ? (double)(i * P) : (double)0.0001f);
+        for (int i = 0; i < M; ++i) pts[N + i] = (float)digamma_d((double)(i * Q + N * P));
+        for (int i = 0; i < L; ++i) pts[N + M + i] = (float)digamma_d((double)(i * R + N * P + M * Q));
+    }
+    float operator()(float x) const {
+        if (!std::isfinite(x)) return (std::isnan(x) || x < 0) ? NAN : INFINITY;
+        if (x < 0) return NAN;
+        if (x < 0.001f) {
+            if (x == 0) return -INFINITY;
+            return logf(x + 2) - 0.5f / (x + 2) - 1 / 12.f / powf(x + 2, 2) - 1 / (x + 1) - 1 / x;
+        }
+        if (x >= N * P + M * Q + (L - 1) * R) return logf(x) - 0.5f / x - 1 / 12.f / powf(x, 2);
+        size_t idx;
+        float a, nx = x;
+        if (x < N * P) {
+            idx = (size_t)(nx / P);
+            a = (nx - idx * P) / P;
+        } else {
+            nx -= N * P;
+            if (nx < M * Q) {
+                idx = (size_t)(nx / Q);
+                a = (nx - idx * Q) / Q;
+                idx += N;
+            } else {
+                nx -= M * Q;
+                idx = (size_t)(nx / R);
+                a = (nx - idx * R) / R;
+                idx += N + M;
+            }
+        }
+        return pts[idx] + a * (pts[idx + 1] - pts[idx]);
+    }
+};
+
+const DigammaLut& digamma_lut() {
+    static DigammaLut t;
+    return t;
+}
+
+int kpl_for(int K) {
+    int kpl = 1;
+    while (64 * kpl < K) kpl <<= 1;
+    return kpl;
+}
+
+}  // namespace
+
+struct lda_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int K = 0, V = 0, kpl = 1, Kp = 64;
+    float eta = 0.01f, veta = 0.0f;
+    int tw = LDA_TW_ONE;
+    uint64_t seed = 0;
+    std::vector<float> alpha;  // natural order
+    uint64_t global_step = 0;
+    uint32_t optim_interval = 10, burn_in = 0;
+    int num_cu = 1;
+
+    // corpus
+    int64_t D = 0;
+    uint64_t N = 0;
+    uint64_t n_valid = 0;
+    std::vector<uint64_t> off;
+    std::vector<uint32_t> words;
+    bool has_corpus = false;
+
+    DevBuf<float> d_alpha, d_alpha_cdf;
+    DevBuf<int32_t> d_nkw, d_nk, d_nk_next, d_ndk, d_doclen, d_nk_priv;
+    DevBuf<uint32_t> d_words, d_order, d_counter;
+    DevBuf<uint16_t> d_z;
+    DevBuf<uint64_t> d_off, d_coff;
+    DevBuf<double> d_part;
+    int part_W = 0;
+
+    // fold-in buffers
+    DevBuf<uint32_t> i_words, i_order, i_counter;
+    DevBuf<uint64_t> i_off;
+    DevBuf<uint16_t> i_z;
+    DevBuf<float> i_theta;
+    DevBuf<double> i_ll;
+
+    // kernel timing of the last call
+    double stat_ms = 0.0;
+    double stat_launches = 0.0;
+    double stat_tokens = 0.0;
+
+    int pos(int k) const { return lda::kvec_pos(kpl, k); }
+};
+
+namespace {
+
+int upload_alpha(lda_ctx* c) {
+    std::vector<float> a(c->Kp, 0.0f);
+    for (int k = 0; k < c->K; ++k) a[c->pos(k)] = c->alpha[k];
+    std::vector<float> ca(c->K);
+    float s = 0.0f;
+    for (int k = 0; k < c->K; ++k) {
+        s = s + c->alpha[k];
+        ca[k] = s;
+    }
+    HIP_TRY(hipMemcpyAsync(c->d_alpha.p, a.data(), a.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->d_alpha_cdf.p, ca.data(), ca.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return LDA_OK;
+}
+
+// Vocabulary chunks of the partition scheme at cumulative collection
+// frequency quantiles (LDAModel.hpp:436-449), padded with V when the
+// quantiles produce fewer than W boundaries; then the per-document token
+// offset of every chunk (LDAModel.hpp:451-469; words are sorted per document).
+int prepare_partition(lda_ctx* c, int W) {
+    if (c->part_W == W && c->d_coff.p) return LDA_OK;
+    std::vector<uint64_t> cf(c->V, 0);
+    for (uint32_t v : c->words)
+        if (v < (uint32_t)c->V) ++cf[v];
+    uint64_t tot = 0;
+    for (auto x : cf) tot += x;
+    std::vector<uint32_t> ch;
+    uint64_t cum = 0;
+    for (int i = 0; i < c->V; ++i) {
+        cum += cf[i];
+        if (cum * (uint64_t)W >= tot * (uint64_t)(ch.size() + 1)) ch.push_back((uint32_t)(i + 1));
+    }
+    while ((int)ch.size() < W) ch.push_back((uint32_t)c->V);
+    ch.resize(W);
+    ch[W - 1] = (uint32_t)c->V;
+    const int64_t D = c->D;
+    std::vector<uint64_t> coff((size_t)(W + 1) * D);
+    for (int64_t d = 0; d < D; ++d) {
+        uint64_t b = c->off[d], e = c->off[d + 1];
+        coff[d] = b;
+        for (int p = 0; p < W; ++p) {
+            auto it = std::lower_bound(c->words.begin() + b, c->words.begin() + e, ch[p]);
+            coff[(size_t)(p + 1) * D + d] = (uint64_t)(it - c->words.begin());
+        }
+        coff[(size_t)W * D + d] = e;
+    }
+    HIP_TRY(c->d_coff.alloc(coff.size()));
+    HIP_TRY(hipMemcpyAsync(c->d_coff.p, coff.data(), coff.size() * sizeof(uint64_t), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c->d_nk_priv.alloc((size_t)W * c->Kp));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->part_W = W;
+    return LDA_OK;
+}
+
+lda::SweepParams base_params(lda_ctx* c) {
+    lda::SweepParams p{};
+    p.words = c->d_words.p;
+    p.z = c->d_z.p;
+    p.off = c->d_off.p;
+    p.ndk = c->d_ndk.p;
+    p.nkw = c->d_nkw.p;
+    p.alpha = c->d_alpha.p;
+    p.eta = c->eta;
+    p.veta = c->veta;
+    p.K = c->K;
+    p.V = c->V;
+    p.Kp = c->Kp;
+    p.seed_lo = (uint32_t)c->seed;
+    p.seed_hi = (uint32_t)(c->seed >> 32);
+    p.sweep = (uint32_t)c->global_step;
+    p.D = c->D;
+    return p;
+}
+
+// Scheme resolution as in TopicModel::train (TopicModel.hpp:684-690):
+// workers == 1 or NONE -> one worker; the partition worker count is capped at
+// max((V+3)/4, 1) (LDAModel.hpp:491-497); workers == 0 selects the
+// device-wide schedule.  Returns the worker count (0 = device-wide).
+int resolve_workers(lda_ctx* c, int workers, int scheme, int* W_out) {
+    if (workers < 0) return fail(LDA_ERR_INVALID_ARGUMENT, "workers must be >= 0");
+    if (scheme < LDA_PS_DEFAULT || scheme > LDA_PS_PARTITION) return fail(LDA_ERR_INVALID_ARGUMENT, "invalid ParallelScheme");
+    if (scheme == LDA_PS_NONE || workers == 1) {
+        *W_out = 1;
+        return LDA_OK;
+    }
+    if (workers == 0) {
+        *W_out = 0;
+        return LDA_OK;
+    }
+    if (scheme == LDA_PS_COPY_MERGE)
+        return fail(LDA_ERR_UNSUPPORTED, "COPY_MERGE with an explicit worker count is not supported on the device; use workers=0 or PARTITION");
+    int cap = std::max((c->V + 3) / 4, 1);
+    int W = std::min(std::min(workers, cap), 1024);
+    *W_out = W;
+    return LDA_OK;
+}
+
+int sweep_once(lda_ctx* c, int W) {
+    lda::SweepParams p = base_params(c);
+    const hipStream_t s = c->stream;
+    if (W == 0) {
+        HIP_TRY(lda::launch_copy_i32(c->d_nk_next.p, c->d_nk.p, c->Kp, s));
+        HIP_TRY(lda::launch_fill_u32(c->d_counter.p, 0u, 1, s));
+        p.nk_in = c->d_nk.p;
+        p.nk_out = c->d_nk_next.p;
+        p.doc_order = c->d_order.p;
+        p.counter = c->d_counter.p;
+        int grid = c->num_cu * lda::sweep_device_blocks_per_cu(c->kpl);
+        HIP_TRY(hipEventRecord(c->ev0, s));
+        HIP_TRY(lda::launch_sweep_device(c->kpl, p, grid, s));
+        HIP_TRY(hipEventRecord(c->ev1, s));
+        HIP_TRY(hipEventSynchronize(c->ev1));
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        c->stat_ms += ms;
+        c->stat_launches += 1;
+        std::swap(c->d_nk, c->d_nk_next);
+    } else {
+        int rc = prepare_partition(c, W);
+        if (rc) return rc;
+        for (int w = 0; w < W; ++w) HIP_TRY(lda::launch_copy_i32(c->d_nk_priv.p + (size_t)w * c->Kp, c->d_nk.p, c->Kp, s));
+        p.nk_in = c->d_nk_priv.p;
+        p.nk_out = c->d_nk_priv.p;
+        p.coff = c->d_coff.p;
+        p.W = W;
+        HIP_TRY(hipEventRecord(c->ev0, s));
+        for (int r = 0; r < W; ++r) {
+            p.round = r;
+            HIP_TRY(lda::launch_sweep_det(c->kpl, p, s));
+        }
+        HIP_TRY(hipEventRecord(c->ev1, s));
+        // mergeState<partition>: n_k = rowwise sum of n_kw (LDAModel.hpp:571-579)
+        HIP_TRY(lda::launch_rowsum_nkw(c->d_nkw.p, c->V, c->Kp, c->d_nk.p, s));
+        HIP_TRY(hipEventSynchronize(c->ev1));
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        c->stat_ms += ms;
+        c->stat_launches += W;
+        HIP_TRY(hipStreamSynchronize(s));
+    }
+    c->stat_tokens += (double)c->n_valid;
+    return LDA_OK;
+}
+
+// optimizeParameters (LDAModel.hpp:221-265): 10 fixed-point passes; per pass
+// the device sums digammaApprox(n_dk + alpha_k) over documents for every
+// topic and digammaApprox(L_d + sum alpha) for the denominator (double
+// accumulation of float terms); dAlpha uses tomoto's LUT digamma.
+int optimize_alpha(lda_ctx* c) {
+    if (!c->D) return LDA_OK;
+    const int nblk = (int)std::min<int64_t>(std::max<int64_t>(c->D / 64, 1), 512);
+    const int cols = c->Kp + 1;
+    HIP_TRY(c->d_part.alloc((size_t)nblk * cols));
+    std::vector<double> part((size_t)nblk * cols);
+    const DigammaLut& dg = digamma_lut();
+    for (int it = 0; it < 10; ++it) {
+        float asum = 0.0f;
+        for (int k = 0; k < c->K; ++k) asum += c->alpha[k];
+        HIP_TRY(lda::launch_alpha_sums(c->d_ndk.p, c->d_doclen.p, c->D, c->d_alpha.p, c->Kp, asum, c->d_part.p, nblk,
+                                       c->stream));
+        HIP_TRY(hipMemcpyAsync(part.data(), c->d_part.p, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        double sden = 0.0;
+        for (int b = 0; b < nblk; ++b) sden += part[(size_t)b * cols + c->Kp];
+        const float denom = (float)(sden - (double)c->D * (double)dg(asum));
+        for (int k = 0; k < c->K; ++k) {
+            const int ps = c->pos(k);
+            double sk = 0.0;
+            for (int b = 0; b < nblk; ++b) sk += part[(size_t)b * cols + ps];
+            const float ak = c->alpha[k];
+            const float nom = (float)(sk - (double)c->D * (double)dg(ak));
+            c->alpha[k] = std::max(nom / denom * ak, 1e-5f);
+        }
+        int rc = upload_alpha(c);
+        if (rc) return rc;
+    }
+    return LDA_OK;
+}
+
+}  // namespace
+
+// ============================================================== C ABI ======
+extern "C" {
+
+const char* lda_last_error(void) { return g_err.c_str(); }
+
+int lda_version(void) { return 1; }
+
+int lda_device_count(int* count) {
+    if (!count) return fail(LDA_ERR_INVALID_ARGUMENT, "count is NULL");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    *count = (e == hipSuccess) ? n : 0;
+    return LDA_OK;
+}
+
+int lda_ctx_create(int K, int V, const float* alpha, float eta, int termweight, uint64_t seed, int device,
+                   lda_ctx** out) {
+    if (!out) return fail(LDA_ERR_INVALID_ARGUMENT, "out is NULL");
+    *out = nullptr;
+    // LDAModel ctor checks (LDAModel.hpp:964-984)
+    if (K <= 0 || K > LDA_MAX_K) return fail(LDA_ERR_INVALID_ARGUMENT, "wrong K value (K = %d); the device sampler supports 1..%d", K, LDA_MAX_K);
+    if (V < 0) return fail(LDA_ERR_INVALID_ARGUMENT, "wrong V value (V = %d)", V);
+    if (!alpha) return fail(LDA_ERR_INVALID_ARGUMENT, "alpha is NULL");
+    for (int k = 0; k < K; ++k)
+        if (!(alpha[k] > 0.0f)) return fail(LDA_ERR_INVALID_ARGUMENT, "wrong alpha value");
+    if (!(eta > 0.0f)) return fail(LDA_ERR_INVALID_ARGUMENT, "wrong eta value (eta = %f)", (double)eta);
+    if (termweight != LDA_TW_ONE)
+        return fail(LDA_ERR_UNSUPPORTED, "TermWeight %d is not supported by the device sampler (ONE only)", termweight);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(LDA_ERR_DEVICE, "no HIP device is available");
+    if (device < 0 || device >= ndev) return fail(LDA_ERR_INVALID_ARGUMENT, "device %d out of range (%d devices)", device, ndev);
+    HIP_TRY(hipSetDevice(device));
+    lda_ctx* c = new lda_ctx();
+    c->device = device;
+    c->K = K;
+    c->V = V;
+    c->kpl = kpl_for(K);
+    c->Kp = 64 * c->kpl;
+    c->eta = eta;
+    c->veta = eta * (float)V;  // EtaHelper::getEtaSum (LDAModel.hpp:93-96)
+    c->tw = termweight;
+    c->seed = seed;
+    c->alpha.assign(alpha, alpha + K);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cu = prop.multiProcessorCount;
+    auto cleanup = [&](int rc) {
+        lda_ctx_destroy(c);
+        return rc;
+    };
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return cleanup(fail(LDA_ERR_DEVICE, "hipStreamCreate failed"));
+    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
+        return cleanup(fail(LDA_ERR_DEVICE, "hipEventCreate failed"));
+    if (c->d_alpha.alloc(c->Kp) || c->d_alpha_cdf.alloc(K) || c->d_nkw.alloc((size_t)std::max(V, 1) * c->Kp) ||
+        c->d_nk.alloc(c->Kp) || c->d_nk_next.alloc(c->Kp) || c->d_counter.alloc(1))
+        return cleanup(fail(LDA_ERR_OUT_OF_MEMORY, "device allocation failed"));
+    if (hipMemsetAsync(c->d_nkw.p, 0, c->d_nkw.n * sizeof(int32_t), c->stream) != hipSuccess ||
+        hipMemsetAsync(c->d_nk.p, 0, c->Kp * sizeof(int32_t), c->stream) != hipSuccess)
+        return cleanup(fail(LDA_ERR_DEVICE, "hipMemsetAsync failed"));
+    int rc = upload_alpha(c);
+    if (rc) return cleanup(rc);
+    *out = c;
+    return LDA_OK;
+}
+
+int lda_ctx_destroy(lda_ctx* c) {
+    if (!c) return LDA_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    c->d_alpha.release(); c->d_alpha_cdf.release(); c->d_nkw.release(); c->d_nk.release(); c->d_nk_next.release();
+    c->d_ndk.release(); c->d_doclen.release(); c->d_nk_priv.release(); c->d_words.release(); c->d_order.release();
+    c->d_counter.release(); c->d_z.release(); c->d_off.release(); c->d_coff.release(); c->d_part.release();
+    c->i_words.release(); c->i_order.release(); c->i_counter.release(); c->i_off.release(); c->i_z.release();
+    c->i_theta.release(); c->i_ll.release();
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return LDA_OK;
+}
+
+int lda_upload_corpus(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off, int64_t D, const uint16_t* z) {
+    if (!c) return fail(LDA_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (D < 0 || !doc_off) return fail(LDA_ERR_INVALID_ARGUMENT, "invalid document offsets");
+    if (D > 0xFFFFFFFFll) return fail(LDA_ERR_INVALID_ARGUMENT, "too many documents");
+    if (doc_off[0] != 0) return fail(LDA_ERR_INVALID_ARGUMENT, "doc_off[0] must be 0");
+    for (int64_t d = 0; d < D; ++d)
+        if (doc_off[d + 1] < doc_off[d]) return fail(LDA_ERR_INVALID_ARGUMENT, "doc_off must be non-decreasing");
+    const uint64_t N = doc_off[D];
+    if (N && !words) return fail(LDA_ERR_INVALID_ARGUMENT, "words is NULL");
+    for (int64_t d = 0; d < D; ++d)
+        for (uint64_t g = doc_off[d] + 1; g < doc_off[d + 1]; ++g)
+            if (words[g] < words[g - 1]) return fail(LDA_ERR_INVALID_ARGUMENT, "words must be sorted within each document");
+    uint64_t nvalid = 0;
+    for (uint64_t g = 0; g < N; ++g) {
+        if (words[g] < (uint32_t)c->V) {
+            ++nvalid;
+            if (z && z[g] >= (uint16_t)c->K) return fail(LDA_ERR_INVALID_ARGUMENT, "z out of range at token %llu", (unsigned long long)g);
+        }
+    }
+    HIP_TRY(hipSetDevice(c->device));
+    c->D = D;
+    c->N = N;
+    c->n_valid = nvalid;
+    c->off.assign(doc_off, doc_off + D + 1);
+    c->words.assign(words, words + N);
+    c->part_W = 0;
+    const hipStream_t s = c->stream;
+    HIP_TRY(c->d_words.alloc(N));
+    HIP_TRY(c->d_z.alloc(N));
+    HIP_TRY(c->d_off.alloc(D + 1));
+    HIP_TRY(c->d_ndk.alloc((size_t)std::max<int64_t>(D, 1) * c->Kp));
+    HIP_TRY(c->d_doclen.alloc(std::max<int64_t>(D, 1)));
+    HIP_TRY(c->d_order.alloc(std::max<int64_t>(D, 1)));
+    if (N) HIP_TRY(hipMemcpyAsync(c->d_words.p, words, N * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(c->d_off.p, doc_off, (D + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+    if (z) {
+        if (N) HIP_TRY(hipMemcpyAsync(c->d_z.p, z, N * sizeof(uint16_t), hipMemcpyHostToDevice, s));
+    } else {
+        // initial topics (LDAModel.hpp:754-762) from the Philox stream
+        HIP_TRY(lda::launch_init_topics(c->d_words.p, c->d_z.p, N, c->d_alpha_cdf.p, c->K, c->V, (uint32_t)c->seed,
+                                        (uint32_t)(c->seed >> 32), s));
+    }
+    HIP_TRY(hipMemsetAsync(c->d_ndk.p, 0, c->d_ndk.n * sizeof(int32_t), s));
+    HIP_TRY(hipMemsetAsync(c->d_nkw.p, 0, c->d_nkw.n * sizeof(int32_t), s));
+    HIP_TRY(hipMemsetAsync(c->d_nk.p, 0, c->Kp * sizeof(int32_t), s));
+    HIP_TRY(lda::launch_build_counts(c->kpl, c->d_words.p, c->d_z.p, c->d_off.p, D, c->d_ndk.p, c->d_nkw.p, c->d_nk.p,
+                                     c->Kp, c->V, s));
+    HIP_TRY(lda::launch_doclen(c->d_words.p, c->d_off.p, D, c->V, c->d_doclen.p, s));
+    // documents longest first for the device-wide schedule
+    std::vector<uint32_t> order(D);
+    std::iota(order.begin(), order.end(), 0u);
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+        return (doc_off[a + 1] - doc_off[a]) > (doc_off[b + 1] - doc_off[b]);
+    });
+    if (D) HIP_TRY(hipMemcpyAsync(c->d_order.p, order.data(), D * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    c->has_corpus = true;
+    return LDA_OK;
+}
+
+int lda_set_counts(lda_ctx* c, const int32_t* nkw, const int32_t* nk) {
+    if (!c || !nkw || !nk) return fail(LDA_ERR_INVALID_ARGUMENT, "NULL argument");
+    HIP_TRY(hipSetDevice(c->device));
+    std::vector<int32_t> a((size_t)c->V * c->Kp, 0), b(c->Kp, 0);
+    for (int v = 0; v < c->V; ++v)
+        for (int k = 0; k < c->K; ++k) a[(size_t)v * c->Kp + c->pos(k)] = nkw[(size_t)v * c->K + k];
+    for (int k = 0; k < c->K; ++k) b[c->pos(k)] = nk[k];
+    if (!a.empty()) HIP_TRY(hipMemcpyAsync(c->d_nkw.p, a.data(), a.size() * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->d_nk.p, b.data(), b.size() * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return LDA_OK;
+}
+
+int lda_set_alpha(lda_ctx* c, const float* alpha) {
+    if (!c || !alpha) return fail(LDA_ERR_INVALID_ARGUMENT, "NULL argument");
+    for (int k = 0; k < c->K; ++k)
+        if (!(alpha[k] > 0.0f)) return fail(LDA_ERR_INVALID_ARGUMENT, "wrong alpha value");
+    c->alpha.assign(alpha, alpha + c->K);
+    HIP_TRY(hipSetDevice(c->device));
+    return upload_alpha(c);
+}
+
+int lda_get_alpha(lda_ctx* c, float* alpha) {
+    if (!c || !alpha) return fail(LDA_ERR_INVALID_ARGUMENT, "NULL argument");
+    std::copy(c->alpha.begin(), c->alpha.end(), alpha);
+    return LDA_OK;
+}
+
+int lda_set_optim(lda_ctx* c, uint32_t optim_interval, uint32_t burn_in) {
+    if (!c) return fail(LDA_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (optim_interval > 0x7FFFFFFFu || burn_in > 0x7FFFFFFFu) return fail(LDA_ERR_INVALID_ARGUMENT, "wrong value");
+    c->optim_interval = optim_interval;
+    c->burn_in = burn_in;
+    return LDA_OK;
+}
+
+int lda_get_global_step(lda_ctx* c, uint64_t* step) {
+    if (!c || !step) return fail(LDA_ERR_INVALID_ARGUMENT, "NULL argument");
+    *step = c->global_step;
+    return LDA_OK;
+}
+
+int lda_set_global_step(lda_ctx* c, uint64_t step) {
+    if (!c) return fail(LDA_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    c->global_step = step;
+    return LDA_OK;
+}
+
+int lda_sweep(lda_ctx* c, int workers, int scheme) {
+    if (!c) return fail(LDA_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (!c->has_corpus) return fail(LDA_ERR_STATE, "no corpus uploaded");
+    int W;
+    int rc = resolve_workers(c, workers, scheme, &W);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(c->device));
+    c->stat_ms = c->stat_launches = c->stat_tokens = 0.0;
+    return sweep_once(c, W);
+}
+
+int lda_train(lda_ctx* c, int iterations, int workers, int scheme) {
+    if (!c) return fail(LDA_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (iterations < 0) return fail(LDA_ERR_INVALID_ARGUMENT, "iterations must be >= 0");
+    if (!c->has_corpus) return fail(LDA_ERR_STATE, "no corpus uploaded");
+    int W;
+    int rc = resolve_workers(c, workers, scheme, &W);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(c->device));
+    c->stat_ms = c->stat_launches = c->stat_tokens = 0.0;
+    // Host-forced failure for testing the "Train failed" path: the iteration
+    // whose global step equals TOMOTOPY_AMD_FAIL_AT_STEP reports a training error.
+    const char* fa = std::getenv("TOMOTOPY_AMD_FAIL_AT_STEP");
+    const long long fail_at = fa ? std::atoll(fa) : -1;
+    for (int it = 0; it < iterations; ++it) {
+        if (fail_at >= 0 && (long long)c->global_step == fail_at)
+            return fail(LDA_ERR_TRAINING, "training error forced at step %llu", (unsigned long long)c->global_step);
+        rc = sweep_once(c, W);
+        if (rc) return rc;
+        // trainOne (LDAModel.hpp:529-532)
+        if (c->global_step >= c->burn_in && c->optim_interval && (c->global_step + 1) % c->optim_interval == 0) {
+            rc = optimize_alpha(c);
+            if (rc) return rc;
+        }
+        ++c->global_step;
+    }
+    return LDA_OK;
+}
+
+int lda_optimize_alpha(lda_ctx* c) {
+    if (!c) return fail(LDA_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (!c->has_corpus) return fail(LDA_ERR_STATE, "no corpus uploaded");
+    HIP_TRY(hipSetDevice(c->device));
+    return optimize_alpha(c);
+}
+
+int lda_loglik(lda_ctx* c, double* ll_docs, double* ll_rest) {
+    if (!c || !ll_docs || !ll_rest) return fail(LDA_ERR_INVALID_ARGUMENT, "NULL argument");
+    HIP_TRY(hipSetDevice(c->device));
+    const int nblk = 1024;
+    HIP_TRY(c->d_part.alloc(2 * nblk));
+    double asum = 0.0;
+    for (int k = 0; k < c->K; ++k) asum += (double)c->alpha[k];
+    const double eta = (double)c->eta, veta = (double)c->V * eta;
+    const int64_t D = c->has_corpus ? c->D : 0;
+    HIP_TRY(lda::launch_loglik(c->d_ndk.p, c->d_doclen.p, D, c->d_nkw.p, c->d_nk.p, c->d_alpha.p, c->V, c->Kp, asum, eta,
+                               veta, c->d_part.p, nblk, c->stream));
+    std::vector<double> part(2 * nblk);
+    HIP_TRY(hipMemcpyAsync(part.data(), c->d_part.p, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    double a = 0.0, b = 0.0;
+    for (int i = 0; i < nblk; ++i) {
+        a += part[i];
+        b += part[nblk + i];
+    }
+    *ll_docs = a;
+    *ll_rest = b;
+    return LDA_OK;
+}
+
+int lda_check_invariants(lda_ctx* c, int64_t flags[8]) {
+    if (!c || !flags) return fail(LDA_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (!c->has_corpus) return fail(LDA_ERR_STATE, "no corpus uploaded");
+    HIP_TRY(hipSetDevice(c->device));
+    const hipStream_t s = c->stream;
+    DevBuf<int32_t> sndk, snkw;
+    DevBuf<long long> colsum;
+    DevBuf<unsigned long long> fl;
+    HIP_TRY(sndk.alloc((size_t)std::max<int64_t>(c->D, 1) * c->Kp));
+    hipError_t e = snkw.alloc((size_t)std::max(c->V, 1) * c->Kp);
+    if (e != hipSuccess) { sndk.release(); HIP_TRY(e); }
+    if (colsum.alloc(c->Kp) != hipSuccess || fl.alloc(8) != hipSuccess) {
+        sndk.release(); snkw.release(); colsum.release(); fl.release();
+        return fail(LDA_ERR_OUT_OF_MEMORY, "scratch allocation failed");
+    }
+    int rc = LDA_OK;
+    std::vector<long long> cs(c->Kp);
+    std::vector<int32_t> nk(c->Kp);
+    unsigned long long f[8] = {0};
+    do {
+        if (hipMemsetAsync(sndk.p, 0, sndk.n * sizeof(int32_t), s) || hipMemsetAsync(snkw.p, 0, snkw.n * sizeof(int32_t), s) ||
+            hipMemsetAsync(fl.p, 0, 8 * sizeof(unsigned long long), s)) { rc = fail(LDA_ERR_DEVICE, "memset failed"); break; }
+        hipError_t le = lda::launch_invariants(c->kpl, c->d_words.p, c->d_z.p, c->d_off.p, c->D, c->d_ndk.p, c->d_nkw.p,
+                                               c->d_nk.p, c->d_doclen.p, c->K, c->V, c->Kp, sndk.p, snkw.p, colsum.p, fl.p, s);
+        if (le != hipSuccess) { rc = fail(LDA_ERR_DEVICE, "invariant kernels: %s", hipGetErrorString(le)); break; }
+        if (hipMemcpyAsync(f, fl.p, sizeof f, hipMemcpyDeviceToHost, s) ||
+            hipMemcpyAsync(cs.data(), colsum.p, cs.size() * sizeof(long long), hipMemcpyDeviceToHost, s) ||
+            hipMemcpyAsync(nk.data(), c->d_nk.p, nk.size() * sizeof(int32_t), hipMemcpyDeviceToHost, s) ||
+            hipStreamSynchronize(s)) { rc = fail(LDA_ERR_DEVICE, "download failed"); break; }
+    } while (0);
+    sndk.release(); snkw.release(); colsum.release(); fl.release();
+    if (rc) return rc;
+    long long bad_topics = 0, total = 0;
+    for (int i = 0; i < c->Kp; ++i) {
+        if (cs[i] != nk[i]) ++bad_topics;
+        total += nk[i];
+    }
+    flags[0] = (int64_t)f[0];
+    flags[1] = bad_topics;
+    flags[2] = total - (long long)c->n_valid;
+    flags[3] = (int64_t)f[3];
+    flags[4] = (int64_t)f[4];
+    flags[5] = (int64_t)f[5];
+    flags[6] = (int64_t)f[6];
+    flags[7] = 0;
+    return LDA_OK;
+}
+
+int lda_download_state(lda_ctx* c, uint16_t* z, int32_t* ndk, int32_t* nkw, int32_t* nk) {
+    if (!c) return fail(LDA_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    HIP_TRY(hipSetDevice(c->device));
+    const hipStream_t s = c->stream;
+    if (z && c->has_corpus && c->N) {
+        HIP_TRY(hipMemcpyAsync(z, c->d_z.p, c->N * sizeof(uint16_t), hipMemcpyDeviceToHost, s));
+    }
+    if (ndk && c->has_corpus && c->D) {
+        std::vector<int32_t> t((size_t)c->D * c->Kp);
+        HIP_TRY(hipMemcpyAsync(t.data(), c->d_ndk.p, t.size() * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        for (int64_t d = 0; d < c->D; ++d)
+            for (int k = 0; k < c->K; ++k) ndk[(size_t)d * c->K + k] = t[(size_t)d * c->Kp + c->pos(k)];
+    }
+    if (nkw && c->V) {
+        std::vector<int32_t> t((size_t)c->V * c->Kp);
+        HIP_TRY(hipMemcpyAsync(t.data(), c->d_nkw.p, t.size() * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        for (int v = 0; v < c->V; ++v)
+            for (int k = 0; k < c->K; ++k) nkw[(size_t)v * c->K + k] = t[(size_t)v * c->Kp + c->pos(k)];
+    }
+    if (nk) {
+        std::vector<int32_t> t(c->Kp);
+        HIP_TRY(hipMemcpyAsync(t.data(), c->d_nk.p, t.size() * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        for (int k = 0; k < c->K; ++k) nk[k] = t[c->pos(k)];
+    }
+    HIP_TRY(hipStreamSynchronize(s));
+    return LDA_OK;
+}
+
+int lda_get_dims(lda_ctx* c, int64_t dims[6]) {
+    if (!c || !dims) return fail(LDA_ERR_INVALID_ARGUMENT, "NULL argument");
+    dims[0] = c->K; dims[1] = c->V; dims[2] = c->D; dims[3] = (int64_t)c->N; dims[4] = c->Kp; dims[5] = c->kpl;
+    return LDA_OK;
+}
+
+int lda_sample_frozen(lda_ctx* c, int64_t n, const int64_t* doc, const uint32_t* word, const uint16_t* z_old,
+                      const float* u, uint16_t* z_new, float* total) {
+    if (!c) return fail(LDA_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (n < 0 || (n && (!doc || !word || !z_old || !u || !z_new))) return fail(LDA_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (!c->has_corpus) return fail(LDA_ERR_STATE, "no corpus uploaded");
+    for (int64_t t = 0; t < n; ++t) {
+        if (doc[t] < 0 || doc[t] >= c->D) return fail(LDA_ERR_INVALID_ARGUMENT, "doc out of range at %lld", (long long)t);
+        if (word[t] >= (uint32_t)c->V) return fail(LDA_ERR_INVALID_ARGUMENT, "word out of range at %lld", (long long)t);
+        if (z_old[t] >= (uint16_t)c->K) return fail(LDA_ERR_INVALID_ARGUMENT, "z out of range at %lld", (long long)t);
+        if (!(u[t] >= 0.0f && u[t] < 1.0f)) return fail(LDA_ERR_INVALID_ARGUMENT, "u out of [0,1) at %lld", (long long)t);
+    }
+    if (!n) return LDA_OK;
+    HIP_TRY(hipSetDevice(c->device));
+    const hipStream_t s = c->stream;
+    DevBuf<int64_t> dd;
+    DevBuf<uint32_t> dw;
+    DevBuf<uint16_t> dz, dzn;
+    DevBuf<float> du, dt;
+    int rc = LDA_OK;
+    if (dd.alloc(n) || dw.alloc(n) || dz.alloc(n) || dzn.alloc(n) || du.alloc(n) || dt.alloc(n)) rc = fail(LDA_ERR_OUT_OF_MEMORY, "allocation failed");
+    if (!rc && (hipMemcpyAsync(dd.p, doc, n * 8, hipMemcpyHostToDevice, s) || hipMemcpyAsync(dw.p, word, n * 4, hipMemcpyHostToDevice, s) ||
+                hipMemcpyAsync(dz.p, z_old, n * 2, hipMemcpyHostToDevice, s) || hipMemcpyAsync(du.p, u, n * 4, hipMemcpyHostToDevice, s)))
+        rc = fail(LDA_ERR_DEVICE, "upload failed");
+    if (!rc) {
+        hipError_t e = lda::launch_sample_frozen(c->kpl, c->d_ndk.p, c->d_nkw.p, c->d_nk.p, c->d_alpha.p, c->eta, c->veta, c->K,
+                                                 c->V, c->Kp, n, dd.p, dw.p, dz.p, du.p, dzn.p, dt.p, s);
+        if (e != hipSuccess) rc = fail(LDA_ERR_DEVICE, "sample_frozen: %s", hipGetErrorString(e));
+    }
+    if (!rc && (hipMemcpyAsync(z_new, dzn.p, n * 2, hipMemcpyDeviceToHost, s) ||
+                (total && hipMemcpyAsync(total, dt.p, n * 4, hipMemcpyDeviceToHost, s)) || hipStreamSynchronize(s)))
+        rc = fail(LDA_ERR_DEVICE, "download failed");
+    dd.release(); dw.release(); dz.release(); dzn.release(); du.release(); dt.release();
+    return rc;
+}
+
+int lda_infer(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off, int64_t D, int iterations, uint64_t infer_seed,
+              int mode, float* theta, double* ll, uint16_t* z_out) {
+    if (!c) return fail(LDA_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (D < 0 || (D && (!doc_off || !theta || !ll))) return fail(LDA_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (iterations < 0) return fail(LDA_ERR_INVALID_ARGUMENT, "iterations must be >= 0");
+    if (mode != 0 && mode != 1) return fail(LDA_ERR_INVALID_ARGUMENT, "mode must be 0 or 1");
+    if (!D) return LDA_OK;
+    if (doc_off[0] != 0) return fail(LDA_ERR_INVALID_ARGUMENT, "doc_off[0] must be 0");
+    for (int64_t d = 0; d < D; ++d) {
+        if (doc_off[d + 1] < doc_off[d]) return fail(LDA_ERR_INVALID_ARGUMENT, "doc_off must be non-decreasing");
+        for (uint64_t g = doc_off[d] + 1; g < doc_off[d + 1]; ++g)
+            if (words[g] < words[g - 1]) return fail(LDA_ERR_INVALID_ARGUMENT, "words must be sorted within each document");
+    }
+    const uint64_t N = doc_off[D];
+    HIP_TRY(hipSetDevice(c->device));
+    const hipStream_t s = c->stream;
+    HIP_TRY(c->i_words.alloc(N));
+    HIP_TRY(c->i_z.alloc(N));
+    HIP_TRY(c->i_off.alloc(D + 1));
+    HIP_TRY(c->i_order.alloc(D));
+    HIP_TRY(c->i_counter.alloc(1));
+    HIP_TRY(c->i_theta.alloc((size_t)D * c->K));
+    HIP_TRY(c->i_ll.alloc(D));
+    if (N) HIP_TRY(hipMemcpyAsync(c->i_words.p, words, N * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(c->i_off.p, doc_off, (D + 1) * 8, hipMemcpyHostToDevice, s));
+    std::vector<uint32_t> order(D);
+    std::iota(order.begin(), order.end(), 0u);
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+        return (doc_off[a + 1] - doc_off[a]) > (doc_off[b + 1] - doc_off[b]);
+    });
+    HIP_TRY(hipMemcpyAsync(c->i_order.p, order.data(), D * 4, hipMemcpyHostToDevice, s));
+    lda::InferParams p{};
+    p.words = c->i_words.p; p.z = c->i_z.p; p.off = c->i_off.p; p.nkw = c->d_nkw.p; p.nk = c->d_nk.p;
+    p.alpha = c->d_alpha.p; p.alpha_cdf = c->d_alpha_cdf.p; p.eta = c->eta; p.veta = c->veta;
+    p.K = c->K; p.V = c->V; p.Kp = c->Kp;
+    p.seed_lo = (uint32_t)infer_seed; p.seed_hi = (uint32_t)(infer_seed >> 32);
+    p.iterations = iterations; p.D = D; p.theta = c->i_theta.p; p.ll = c->i_ll.p;
+    double asd = 0.0;
+    float asf = 0.0f;
+    for (int k = 0; k < c->K; ++k) { asd += (double)c->alpha[k]; asf = asf + c->alpha[k]; }
+    p.asum_d = asd; p.asum_f = asf;
+    c->stat_ms = c->stat_launches = c->stat_tokens = 0.0;
+    if (mode == 0) {
+        p.doc_order = c->i_order.p;
+        p.counter = c->i_counter.p;
+        HIP_TRY(lda::launch_fill_u32(c->i_counter.p, 0u, 1, s));
+        int grid = c->num_cu * lda::infer_blocks_per_cu(c->kpl);
+        int64_t need = (D + 3) / 4;
+        if (grid > need) grid = (int)need;
+        HIP_TRY(hipEventRecord(c->ev0, s));
+        HIP_TRY(lda::launch_infer(c->kpl, p, grid, s));
+        HIP_TRY(hipEventRecord(c->ev1, s));
+        c->stat_launches = 1;
+    } else {
+        p.doc_order = nullptr;
+        p.counter = nullptr;
+        HIP_TRY(hipEventRecord(c->ev0, s));
+        for (int64_t d = 0; d < D; ++d) {
+            p.doc_base = d;
+            // one wavefront: a 256-thread block whose other waves fall past D or
+            // onto the next documents is avoided by launching exactly one wave
+            lda::InferParams q = p;
+            q.D = d + 1;
+            HIP_TRY(lda::launch_infer(c->kpl, q, 1, s));
+            HIP_TRY(hipStreamSynchronize(s));
+        }
+        HIP_TRY(hipEventRecord(c->ev1, s));
+        c->stat_launches = (double)D;
+    }
+    HIP_TRY(hipEventSynchronize(c->ev1));
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->stat_ms = ms;
+    c->stat_tokens = (double)N * iterations;
+    HIP_TRY(hipMemcpyAsync(theta, c->i_theta.p, (size_t)D * c->K * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(ll, c->i_ll.p, D * 8, hipMemcpyDeviceToHost, s));
+    if (z_out && N) HIP_TRY(hipMemcpyAsync(z_out, c->i_z.p, N * 2, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return LDA_OK;
+}
+
+int lda_kernel_stats(lda_ctx* c, double stats[4]) {
+    if (!c || !stats) return fail(LDA_ERR_INVALID_ARGUMENT, "NULL argument");
+    stats[0] = c->stat_ms;
+    stats[1] = c->stat_launches;
+    stats[2] = c->stat_tokens;
+    stats[3] = 0.0;
+    return LDA_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,939 @@
+// lda_device.hip -- gfx950 kernels of the collapsed-Gibbs LDA engine.
+//
+// Hot path: the per-token resampling of tomoto's sampleDocument
+// (src/TopicModel/LDAModel.hpp:326-354): remove the token (addWordTo<-1>,
+// LDAModel.hpp:287-299), weights (n_dk+a)(n_kw+eta)/(n_k+V*eta)
+// (getZLikelihoods, LDAModel.hpp:273-285), inclusive scan (sample.hpp:61-86),
+// inverse-CDF search (sampleFromDiscreteAcc, sample.hpp:133-189), add the
+// token back.
+//
+// Execution model: one 64-lane wavefront owns one document (or one
+// document x vocabulary-chunk cell) and walks its tokens in order, so every
+// token sees the n_dk update of the previous one, as in tomoto.  Lane l holds
+// topics [l*KPL, (l+1)*KPL) of every K-vector in registers: n_dk, n_dk+alpha,
+// the worker-private n_k, n_k+V*eta and its correctly rounded reciprocal,
+// and the current word's n_kw column.  The column is read with coalesced
+// 16-byte loads from the word-major n_kw (see kvec_pos).  Weights use the
+// reciprocal with one Markstein correction, which reproduces the IEEE
+// quotient; the scan is lane-local then a DPP row scan; the search is a
+// ballot over lanes.  MFMA is not used: nothing here is a contraction.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "lda_internal.h"
+
+namespace lda {
+namespace {
+
+constexpr uint32_t PHILOX_M0 = 0xD2511F53u, PHILOX_M1 = 0xCD9E8D57u;
+constexpr uint32_t PHILOX_W0 = 0x9E3779B9u, PHILOX_W1 = 0xBB67AE85u;
+
+// Philox4x32-10 (Salmon et al., SC'11), first output word.
+__device__ __forceinline__ uint32_t philox_x0(uint32_t k0, uint32_t k1, uint32_t c0, uint32_t c1, uint32_t c2,
+                                              uint32_t c3) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        uint32_t hi0 = __umulhi(PHILOX_M0, c0), lo0 = PHILOX_M0 * c0;
+        uint32_t hi1 = __umulhi(PHILOX_M1, c2), lo1 = PHILOX_M1 * c2;
+        uint32_t n0 = hi1 ^ c1 ^ k0;
+        uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+        k0 += PHILOX_W0; k1 += PHILOX_W1;
+    }
+    return c0;
+}
+
+__device__ __forceinline__ float uniform24(uint32_t x) { return (float)(x >> 8) * 0x1p-24f; }
+
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63u); }
+
+template <int KPL>
+__device__ __forceinline__ int kpos(int lane, int j) {
+    if constexpr (KPL < 4) return lane * KPL + j;
+    else return (j >> 2) * 256 + lane * 4 + (j & 3);
+}
+
+template <int KPL>
+__device__ __forceinline__ void load_kvec(const int32_t* __restrict__ base, int lane, int (&o)[KPL]) {
+    if constexpr (KPL >= 4) {
+#pragma unroll
+        for (int g = 0; g < KPL / 4; ++g) {
+            int4 x = *reinterpret_cast<const int4*>(base + g * 256 + lane * 4);
+            o[4 * g] = x.x; o[4 * g + 1] = x.y; o[4 * g + 2] = x.z; o[4 * g + 3] = x.w;
+        }
+    } else if constexpr (KPL == 2) {
+        int2 x = *reinterpret_cast<const int2*>(base + lane * 2);
+        o[0] = x.x; o[1] = x.y;
+    } else {
+        o[0] = base[lane];
+    }
+}
+
+template <int KPL>
+__device__ __forceinline__ void load_kvec_f(const float* __restrict__ base, int lane, float (&o)[KPL]) {
+    if constexpr (KPL >= 4) {
+#pragma unroll
+        for (int g = 0; g < KPL / 4; ++g) {
+            float4 x = *reinterpret_cast<const float4*>(base + g * 256 + lane * 4);
+            o[4 * g] = x.x; o[4 * g + 1] = x.y; o[4 * g + 2] = x.z; o[4 * g + 3] = x.w;
+        }
+    } else if constexpr (KPL == 2) {
+        float2 x = *reinterpret_cast<const float2*>(base + lane * 2);
+        o[0] = x.x; o[1] = x.y;
+    } else {
+        o[0] = base[lane];
+    }
+}
+
+template <int KPL>
+__device__ __forceinline__ void store_kvec(int32_t* __restrict__ base, int lane, const int (&o)[KPL]) {
+    if constexpr (KPL >= 4) {
+#pragma unroll
+        for (int g = 0; g < KPL / 4; ++g)
+            *reinterpret_cast<int4*>(base + g * 256 + lane * 4) = make_int4(o[4 * g], o[4 * g + 1], o[4 * g + 2], o[4 * g + 3]);
+    } else if constexpr (KPL == 2) {
+        *reinterpret_cast<int2*>(base + lane * 2) = make_int2(o[0], o[1]);
+    } else {
+        base[lane] = o[0];
+    }
+}
+
+// ---- cross-lane scan: DPP row_shr 1,2,4,8 then row_bcast 15 / 31 ----
+__device__ __forceinline__ float dpp_row_shr1(float v) { return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x111, 0xF, 0xF, true)); }
+__device__ __forceinline__ float dpp_row_shr2(float v) { return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x112, 0xF, 0xF, true)); }
+__device__ __forceinline__ float dpp_row_shr4(float v) { return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x114, 0xF, 0xF, true)); }
+__device__ __forceinline__ float dpp_row_shr8(float v) { return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x118, 0xF, 0xF, true)); }
+__device__ __forceinline__ float dpp_bcast15(float v) { return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x142, 0xA, 0xF, false)); }
+__device__ __forceinline__ float dpp_bcast31(float v) { return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x143, 0xC, 0xF, false)); }
+__device__ __forceinline__ float dpp_wave_shr1(float v) { return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xF, 0xF, true)); }
+
+// Inclusive scan over the 64 lanes in the order the CPU oracle emulates
+// (oracle/lda_oracle.cpp prefix_sum_device).
+__device__ __forceinline__ float wave_inclusive_scan(float v) {
+    v = v + dpp_row_shr1(v);
+    v = v + dpp_row_shr2(v);
+    v = v + dpp_row_shr4(v);
+    v = v + dpp_row_shr8(v);
+    v = v + dpp_bcast15(v);
+    v = v + dpp_bcast31(v);
+    return v;
+}
+
+__device__ __forceinline__ int readlane_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ __forceinline__ float readlane_f(float v, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l)); }
+__device__ __forceinline__ int uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Register-resident per-lane topic state of one worker.
+template <int KPL>
+struct Regs {
+    int ndk[KPL];   // n_dk of the current document
+    float A[KPL];   // float(n_dk) + alpha_k
+    float al[KPL];  // alpha_k
+    int nk[KPL];    // worker-private n_k
+    float bf[KPL];  // float(n_k) + V*eta
+    float y[KPL];   // RN(1 / bf)
+    int col[KPL];   // n_kw[., v] of the current word
+};
+
+// addWordTo<delta> for one topic on the register state (LDAModel.hpp:287-299).
+// `sl` is wave-uniform; only the owner lane changes its counts, but every lane
+// recomputes the derived values of slot `sl` from its own counts, which leaves
+// them bit-identical in the other lanes.
+template <int KPL>
+__device__ __forceinline__ void bump(Regs<KPL>& R, bool me, int sl, int delta, float veta) {
+    const int dd = me ? delta : 0;
+    // `sl` is uniform: one taken branch, static register indices (no scratch)
+#pragma unroll
+    for (int j = 0; j < KPL; ++j) {
+        if (j == sl) {
+            int nd = R.ndk[j] + dd;
+            R.ndk[j] = nd;
+            R.A[j] = (float)nd + R.al[j];
+            R.col[j] = R.col[j] + dd;
+            int nkv = R.nk[j] + dd;
+            R.nk[j] = nkv;
+            float b = (float)nkv + veta;
+            R.bf[j] = b;
+            R.y[j] = 1.0f / b;
+        }
+    }
+}
+
+// Value of register slot `sl` (uniform) of a per-lane array.
+template <int KPL, typename T>
+__device__ __forceinline__ T pick(const T (&a)[KPL], int sl) {
+    T r = a[0];
+#pragma unroll
+    for (int j = 1; j < KPL; ++j) r = (j == sl) ? a[j] : r;
+    return r;
+}
+
+template <int KPL>
+__device__ __forceinline__ void set_doc(Regs<KPL>& R, const int32_t* __restrict__ ndk_row, int lane) {
+    load_kvec<KPL>(ndk_row, lane, R.ndk);
+#pragma unroll
+    for (int j = 0; j < KPL; ++j) R.A[j] = (float)R.ndk[j] + R.al[j];
+}
+
+template <int KPL>
+__device__ __forceinline__ void set_nk(Regs<KPL>& R, float veta) {
+#pragma unroll
+    for (int j = 0; j < KPL; ++j) {
+        R.bf[j] = (float)R.nk[j] + veta;
+        R.y[j] = 1.0f / R.bf[j];
+    }
+}
+
+// One token: remove z_old, weight, scan, search, add z_new.  Returns z_new
+// (wave-uniform).  *total receives cdf[K-1].
+template <int KPL>
+__device__ __forceinline__ int token_step(Regs<KPL>& R, int lane, int zo, float u, int K, int nvalid, int lastLane,
+                                          int lastSlot, float eta, float veta, float* total) {
+    constexpr int SH = (KPL == 1) ? 0 : (KPL == 2) ? 1 : (KPL == 4) ? 2 : (KPL == 8) ? 3 : (KPL == 16) ? 4 : 5;
+    bump<KPL>(R, lane == (zo >> SH), zo & (KPL - 1), -1, veta);
+    float w[KPL];
+#pragma unroll
+    for (int j = 0; j < KPL; ++j) {
+        float B = (float)R.col[j] + eta;
+        float a = R.A[j] * B;
+        float q0 = a * R.y[j];
+        float rr = __builtin_fmaf(-q0, R.bf[j], a);
+        w[j] = __builtin_fmaf(rr, R.y[j], q0);
+    }
+#pragma unroll
+    for (int j = 1; j < KPL; ++j) w[j] = w[j - 1] + w[j];
+    float incl = wave_inclusive_scan(w[KPL - 1]);
+    float excl = dpp_wave_shr1(incl);
+#pragma unroll
+    for (int j = 0; j < KPL; ++j) w[j] = excl + w[j];
+    float tot = readlane_f(pick<KPL, float>(w, lastSlot), lastLane);
+    float r = u * tot;
+    int miss = 0;
+#pragma unroll
+    for (int j = 0; j < KPL; ++j) miss += (j < nvalid && !(r < w[j])) ? 1 : 0;
+    unsigned long long m = __ballot(miss < nvalid);
+    int zn;
+    if (m == 0ull) zn = K - 1;
+    else {
+        int L = __builtin_ctzll(m);
+        zn = L * KPL + readlane_i(miss, L);
+    }
+    zn = uniform_i(zn);
+    bump<KPL>(R, lane == (zn >> SH), zn & (KPL - 1), +1, veta);
+    if (total) *total = tot;
+    return zn;
+}
+
+// forShuffled's prime table (Utils.hpp:159-163).
+__constant__ uint64_t kShufflePrimes[16] = {65537, 65539, 65543, 65551, 65557, 65563, 65579, 65581,
+                                            65587, 65599, 65609, 65617, 65629, 65633, 65647, 65651};
+
+constexpr int MODE_DET = 0;     // exclusive cell owner: plain stores to n_kw
+constexpr int MODE_ATOMIC = 1;  // shared n_kw: exact int32 atomics
+
+// sampleDocument over the token range [b, e) (LDAModel.hpp:326-354).
+template <int KPL, int MODE>
+__device__ __forceinline__ void sample_range(Regs<KPL>& R, const SweepParams& p, int lane, uint64_t b, uint64_t e,
+                                             int nvalid, int lastLane, int lastSlot) {
+    constexpr int SH = (KPL == 1) ? 0 : (KPL == 2) ? 1 : (KPL == 4) ? 2 : (KPL == 8) ? 3 : (KPL == 16) ? 4 : 5;
+    int vcur = -1;
+    int32_t* colp = nullptr;
+    for (uint64_t t0 = b; t0 < e; t0 += 64) {
+        int n = (int)((e - t0) < 64 ? (e - t0) : 64);
+        uint64_t g = t0 + (uint64_t)lane;
+        int wv = -1, zv = 0xFFFF;
+        float uv = 0.0f;
+        if (lane < n) {
+            wv = (int)p.words[g];
+            zv = p.z[g];
+            uv = uniform24(philox_x0(p.seed_lo, p.seed_hi, (uint32_t)g, (uint32_t)(g >> 32), p.sweep, TAG_SWEEP));
+        }
+        int zkeep = zv;
+        for (int i = 0; i < n; ++i) {
+            int v = readlane_i(wv, i);
+            if ((unsigned)v >= (unsigned)p.V) continue;
+            int zo = readlane_i(zv, i);
+            float u = readlane_f(uv, i);
+            if (v != vcur) {
+                if constexpr (MODE == MODE_DET) __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+                colp = p.nkw + (int64_t)v * p.Kp;
+                load_kvec<KPL>(colp, lane, R.col);
+                vcur = v;
+            }
+            int zn = token_step<KPL>(R, lane, zo, u, p.K, nvalid, lastLane, lastSlot, p.eta, p.veta, nullptr);
+            if (zn != zo) {
+                int lo = zo >> SH, so = zo & (KPL - 1);
+                int ln = zn >> SH, sn = zn & (KPL - 1);
+                if constexpr (MODE == MODE_DET) {
+                    int co = pick<KPL, int>(R.col, so), cn = pick<KPL, int>(R.col, sn);
+                    if (lane == lo) colp[kpos<KPL>(lane, so)] = co;
+                    if (lane == ln) colp[kpos<KPL>(lane, sn)] = cn;
+                } else {
+                    if (lane == lo) atomicAdd(colp + kpos<KPL>(lane, so), -1);
+                    if (lane == ln) atomicAdd(colp + kpos<KPL>(lane, sn), 1);
+                }
+                zkeep = (lane == i) ? zn : zkeep;
+            }
+        }
+        if (lane < n) p.z[g] = (uint16_t)zkeep;
+    }
+}
+
+template <int KPL>
+__device__ __forceinline__ void lane_limits(int K, int lane, int& nvalid, int& lastLane, int& lastSlot) {
+    int nv = K - lane * KPL;
+    nvalid = nv < 0 ? 0 : (nv > KPL ? KPL : nv);
+    lastLane = (K - 1) / KPL;
+    lastSlot = (K - 1) % KPL;
+}
+
+// Device-wide sweep: every resident wavefront is a worker with a private
+// n_k (snapshot nk_in at sweep start plus its own changes, as tomoto's
+// partition workers keep theirs, LDAModel.hpp:484, 596-602); documents are
+// taken longest-first from a global counter; n_kw is updated with atomics;
+// each worker adds its n_k delta to nk_out when it runs out of documents.
+template <int KPL>
+__global__ void __launch_bounds__(256) sweep_device_kernel(SweepParams p) {
+    const int lane = lane_id();
+    Regs<KPL> R;
+    int nvalid, lastLane, lastSlot;
+    lane_limits<KPL>(p.K, lane, nvalid, lastLane, lastSlot);
+    load_kvec_f<KPL>(p.alpha, lane, R.al);
+    load_kvec<KPL>(p.nk_in, lane, R.nk);
+    set_nk<KPL>(R, p.veta);
+    for (;;) {
+        uint32_t idx = 0;
+        if (lane == 0) idx = atomicAdd(p.counter, 1u);
+        idx = (uint32_t)uniform_i((int)idx);
+        if ((int64_t)idx >= p.D) break;
+        uint32_t d = p.doc_order[idx];
+        uint64_t b = p.off[d], e = p.off[d + 1];
+        int32_t* ndk_row = p.ndk + (int64_t)d * p.Kp;
+        set_doc<KPL>(R, ndk_row, lane);
+        sample_range<KPL, MODE_ATOMIC>(R, p, lane, b, e, nvalid, lastLane, lastSlot);
+        store_kvec<KPL>(ndk_row, lane, R.ndk);
+    }
+    int snap[KPL];
+    load_kvec<KPL>(p.nk_in, lane, snap);
+#pragma unroll
+    for (int j = 0; j < KPL; ++j) {
+        int dlt = R.nk[j] - snap[j];
+        if (dlt) atomicAdd(p.nk_out + kpos<KPL>(lane, j), dlt);
+    }
+}
+
+// One round of the deterministic partition schedule (LDAModel.hpp:373-399):
+// wavefront p is worker p; it visits documents d = id*W + ((round+p) % W) in
+// forShuffled order (Utils.hpp:157-178) and samples only their tokens in
+// vocabulary chunk p (coff rows p, p+1).  W == 1 is the `none` scheme.
+template <int KPL>
+__global__ void __launch_bounds__(64) sweep_det_kernel(SweepParams p) {
+    const int lane = lane_id();
+    const int w = blockIdx.x;
+    Regs<KPL> R;
+    int nvalid, lastLane, lastSlot;
+    lane_limits<KPL>(p.K, lane, nvalid, lastLane, lastSlot);
+    load_kvec_f<KPL>(p.alpha, lane, R.al);
+    int32_t* nkp = p.nk_out + (int64_t)w * p.Kp;
+    load_kvec<KPL>(nkp, lane, R.nk);
+    set_nk<KPL>(R, p.veta);
+    const int W = p.W;
+    const int didx = (p.round + w) % W;
+    const uint64_t N = (uint64_t)((p.D + (W - 1) - didx) / W);
+    const uint32_t sseed = philox_x0(p.seed_lo, p.seed_hi, (uint32_t)w, (uint32_t)p.round, p.sweep, TAG_SHUFFLE);
+    uint64_t P = 1;
+    if (N) {
+        P = kShufflePrimes[sseed & 0xF];
+        if (N % P == 0) {
+            P = kShufflePrimes[(sseed + 1) & 0xF];
+            if (N % P == 0) P = kShufflePrimes[(sseed + 2) & 0xF];
+            if (N % P == 0) P = kShufflePrimes[(sseed + 3) & 0xF];
+        }
+        P %= N;
+    }
+    for (uint64_t i = 0; i < N; ++i) {
+        uint64_t id = ((i + (uint64_t)sseed) * P) % N;
+        int64_t d = (int64_t)id * W + didx;
+        uint64_t b = p.coff[(int64_t)w * p.D + d], e = p.coff[(int64_t)(w + 1) * p.D + d];
+        if (b == e) continue;
+        int32_t* ndk_row = p.ndk + d * p.Kp;
+        set_doc<KPL>(R, ndk_row, lane);
+        sample_range<KPL, MODE_DET>(R, p, lane, b, e, nvalid, lastLane, lastSlot);
+        store_kvec<KPL>(ndk_row, lane, R.ndk);
+    }
+    store_kvec<KPL>(nkp, lane, R.nk);
+}
+
+// ---------------------------------------------------------------- init ----
+// Initial topics z ~ Discrete(alpha) (LDAModel.hpp:754-762): r = u*cdf[K-1],
+// first k with r < cdf[k] (binary search on the non-decreasing cdf).
+__global__ void init_topics_kernel(const uint32_t* __restrict__ words, uint16_t* __restrict__ z, uint64_t N,
+                                   const float* __restrict__ ca, int K, int V, uint32_t s0, uint32_t s1) {
+    for (uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; g < N; g += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t v = words[g];
+        if (v >= (uint32_t)V) { z[g] = 0xFFFF; continue; }
+        float u = uniform24(philox_x0(s0, s1, (uint32_t)g, (uint32_t)(g >> 32), 0u, TAG_INIT));
+        float r = u * ca[K - 1];
+        int lo = 0, hi = K;  // first index with r < ca[idx]
+        while (lo < hi) {
+            int mid = (lo + hi) >> 1;
+            if (r < ca[mid]) hi = mid; else lo = mid + 1;
+        }
+        z[g] = (uint16_t)(lo < K ? lo : K - 1);
+    }
+}
+
+// Counts from z, one wavefront per document (tomoto resetStatistics /
+// DocumentLDA::update, LDAModel.hpp:301-315, 1163-1173).
+template <int KPL>
+__global__ void build_counts_kernel(const uint32_t* __restrict__ words, const uint16_t* __restrict__ z,
+                                    const uint64_t* __restrict__ off, int64_t D, int32_t* ndk, int32_t* nkw,
+                                    int32_t* nk, int Kp, int V) {
+    const int lane = lane_id();
+    const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t d = wave; d < D; d += nw) {
+        for (uint64_t g = off[d] + lane; g < off[d + 1]; g += 64) {
+            uint32_t v = words[g];
+            if (v >= (uint32_t)V) continue;
+            int k = z[g];
+            int pos = kpos<KPL>(k / KPL, k % KPL);
+            atomicAdd(ndk + d * Kp + pos, 1);
+            atomicAdd(nkw + (int64_t)v * Kp + pos, 1);
+            atomicAdd(nk + pos, 1);
+        }
+    }
+}
+
+__global__ void copy_i32_kernel(int32_t* dst, const int32_t* src, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) dst[i] = src[i];
+}
+__global__ void fill_u32_kernel(uint32_t* dst, uint32_t v, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) dst[i] = v;
+}
+
+// n_k = rowwise sum of n_kw (mergeState, LDAModel.hpp:569, 578).
+__global__ void rowsum_kernel(const int32_t* __restrict__ nkw, int V, int Kp, int32_t* nk) {
+    int pos = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pos >= Kp) return;
+    long long s = 0;
+    for (int v = 0; v < V; ++v) s += nkw[(int64_t)v * Kp + pos];
+    nk[pos] = (int32_t)s;
+}
+
+// ------------------------------------------------------------ loglik ------
+__device__ __forceinline__ double wave_sum_d(double x) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) x += __shfl_xor(x, o);
+    return x;
+}
+
+__device__ __forceinline__ double block_sum_d(double x, double* sh) {
+    x = wave_sum_d(x);
+    int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) sh[w] = x;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0) {
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += sh[i];
+    }
+    __syncthreads();
+    return t;
+}
+
+// getLLDocs + getLLRest in FP64 (LDAModel.hpp:616-666).  Block b writes its
+// docs-part partial to partial[b] and its rest-part partial to partial[nblk+b].
+__global__ void __launch_bounds__(256) loglik_kernel(const int32_t* __restrict__ ndk, const int32_t* __restrict__ doclen,
+                                                     int64_t D, const int32_t* __restrict__ nkw,
+                                                     const int32_t* __restrict__ nk, const float* __restrict__ alpha,
+                                                     int V, int Kp, double asum, double eta, double veta,
+                                                     double* partial) {
+    __shared__ double sh[8];
+    const int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t nth = (int64_t)gridDim.x * blockDim.x;
+    double docs = 0.0, rest = 0.0;
+    const int64_t nd = D * (int64_t)Kp;
+    for (int64_t i = tid; i < nd; i += nth) {
+        int c = ndk[i];
+        if (c) {
+            double a = (double)alpha[i % Kp];
+            docs += lgamma((double)c + a) - lgamma(a);
+        }
+    }
+    const double lga = lgamma(asum);
+    for (int64_t d = tid; d < D; d += nth) docs -= lgamma((double)doclen[d] + asum) - lga;
+    const double lge = lgamma(eta);
+    const int64_t nv = (int64_t)V * Kp;
+    for (int64_t i = tid; i < nv; i += nth) {
+        int c = nkw[i];
+        if (c) rest += lgamma((double)c + eta) - lge;
+    }
+    for (int64_t i = tid; i < Kp; i += nth) {
+        // padded positions carry n_k == 0 and alpha == 0; count real topics only
+        float a = alpha[i];
+        if (a > 0.0f) rest += lgamma(veta) - lgamma((double)nk[i] + veta);
+    }
+    double bd = block_sum_d(docs, sh);
+    double br = block_sum_d(rest, sh);
+    if (threadIdx.x == 0) {
+        partial[blockIdx.x] = bd;
+        partial[gridDim.x + blockIdx.x] = br;
+    }
+}
+
+__global__ void doclen_kernel(const uint32_t* __restrict__ words, const uint64_t* __restrict__ off, int64_t D, int V,
+                              int32_t* doclen) {
+    const int lane = lane_id();
+    const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t d = wave; d < D; d += nw) {
+        int c = 0;
+        for (uint64_t g = off[d] + lane; g < off[d + 1]; g += 64) c += words[g] < (uint32_t)V;
+        for (int o = 32; o >= 1; o >>= 1) c += __shfl_xor(c, o);
+        if (lane == 0) doclen[d] = c;
+    }
+}
+
+// digammaApprox (src/Utils/math.h:170-175), float, as tomoto evaluates it.
+__device__ __forceinline__ float digamma_approx_f(float z) {
+    return logf(z + 4) - 0.5f / (z + 4) - (1.0f / 12.0f) / ((z + 4) * (z + 4)) - 1.0f / z - 1.0f / (z + 1) -
+           1.0f / (z + 2) - 1.0f / (z + 3);
+}
+
+// Partial sums for optimizeParameters (LDAModel.hpp:221-265): block b sums
+// documents [b*D/nblk, (b+1)*D/nblk) for every topic position (threads stride
+// over positions) and, in column Kp, the denominator term.
+__global__ void __launch_bounds__(256) alpha_sums_kernel(const int32_t* __restrict__ ndk,
+                                                         const int32_t* __restrict__ doclen, int64_t D,
+                                                         const float* __restrict__ alpha, int Kp, float asum,
+                                                         double* part) {
+    const int64_t d0 = D * blockIdx.x / gridDim.x, d1 = D * (blockIdx.x + 1) / gridDim.x;
+    for (int pos = threadIdx.x; pos <= Kp; pos += blockDim.x) {
+        double s = 0.0;
+        if (pos < Kp) {
+            float a = alpha[pos];
+            if (a > 0.0f) {
+                float f0 = digamma_approx_f(a);
+                int64_t zeros = 0;
+                for (int64_t d = d0; d < d1; ++d) {
+                    int c = ndk[d * Kp + pos];
+                    if (c) s += (double)digamma_approx_f((float)c + a);
+                    else ++zeros;
+                }
+                s += (double)zeros * (double)f0;
+            }
+        } else {
+            for (int64_t d = d0; d < d1; ++d) s += (double)digamma_approx_f((float)doclen[d] + asum);
+        }
+        part[(int64_t)blockIdx.x * (Kp + 1) + pos] = s;
+    }
+}
+
+// ---------------------------------------------------------- invariants ----
+template <int KPL>
+__global__ void recount_kernel(const uint32_t* __restrict__ words, const uint16_t* __restrict__ z,
+                               const uint64_t* __restrict__ off, int64_t D, int K, int V, int Kp, int32_t* sndk,
+                               int32_t* snkw, unsigned long long* flags) {
+    const int lane = lane_id();
+    const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t d = wave; d < D; d += nw) {
+        for (uint64_t g = off[d] + lane; g < off[d + 1]; g += 64) {
+            uint32_t v = words[g];
+            if (v >= (uint32_t)V) continue;
+            int k = z[g];
+            if (k >= K) { atomicAdd(flags + 6, 1ull); continue; }
+            int pos = kpos<KPL>(k / KPL, k % KPL);
+            atomicAdd(sndk + d * Kp + pos, 1);
+            atomicAdd(snkw + (int64_t)v * Kp + pos, 1);
+        }
+    }
+}
+
+__global__ void compare_kernel(const int32_t* __restrict__ a, const int32_t* __restrict__ b, int64_t n,
+                               unsigned long long* flag, unsigned long long* negflag) {
+    unsigned long long bad = 0, neg = 0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        bad += a[i] != b[i];
+        neg += a[i] < 0;
+    }
+    if (bad) atomicAdd(flag, bad);
+    if (neg) atomicAdd(negflag, neg);
+}
+
+__global__ void ndk_rows_kernel(const int32_t* __restrict__ ndk, const int32_t* __restrict__ doclen, int64_t D, int Kp,
+                                unsigned long long* flag) {
+    const int lane = lane_id();
+    const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t d = wave; d < D; d += nw) {
+        long long s = 0;
+        for (int i = lane; i < Kp; i += 64) s += ndk[d * Kp + i];
+        for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o);
+        if (lane == 0 && s != doclen[d]) atomicAdd(flag, 1ull);
+    }
+}
+
+__global__ void colsum_kernel(const int32_t* __restrict__ nkw, int V, int Kp, long long* colsum) {
+    int pos = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pos >= Kp) return;
+    long long s = 0;
+    for (int v = 0; v < V; ++v) s += nkw[(int64_t)v * Kp + pos];
+    colsum[pos] = s;
+}
+
+// ------------------------------------------------------ frozen sampling ----
+template <int KPL>
+__global__ void __launch_bounds__(64) sample_frozen_kernel(const int32_t* __restrict__ ndk, const int32_t* __restrict__ nkw,
+                                                           const int32_t* __restrict__ nk, const float* __restrict__ alpha,
+                                                           float eta, float veta, int K, int Kp, int64_t n,
+                                                           const int64_t* __restrict__ doc, const uint32_t* __restrict__ word,
+                                                           const uint16_t* __restrict__ z_old, const float* __restrict__ u,
+                                                           uint16_t* z_new, float* total) {
+    const int lane = lane_id();
+    Regs<KPL> R;
+    int nvalid, lastLane, lastSlot;
+    lane_limits<KPL>(K, lane, nvalid, lastLane, lastSlot);
+    load_kvec_f<KPL>(alpha, lane, R.al);
+    for (int64_t t = blockIdx.x; t < n; t += gridDim.x) {
+        load_kvec<KPL>(nk, lane, R.nk);
+        set_nk<KPL>(R, veta);
+        set_doc<KPL>(R, ndk + doc[t] * Kp, lane);
+        load_kvec<KPL>(nkw + (int64_t)word[t] * Kp, lane, R.col);
+        float tot;
+        int zn = token_step<KPL>(R, lane, (int)z_old[t], u[t], K, nvalid, lastLane, lastSlot, eta, veta, &tot);
+        if (lane == 0) {
+            z_new[t] = (uint16_t)zn;
+            if (total) total[t] = tot;
+        }
+    }
+}
+
+// -------------------------------------------------------------- fold-in ----
+// Histogram of the current topics of the run of word v that starts at token
+// g (words are sorted, so the run is contiguous), added into R.col.
+template <int KPL>
+__device__ __forceinline__ void add_run_counts(Regs<KPL>& R, const InferParams& p, int lane, uint64_t g, uint64_t e,
+                                               int v) {
+    constexpr int SH = (KPL == 1) ? 0 : (KPL == 2) ? 1 : (KPL == 4) ? 2 : (KPL == 8) ? 3 : (KPL == 16) ? 4 : 5;
+    for (uint64_t t0 = g; t0 < e; t0 += 64) {
+        uint64_t gg = t0 + (uint64_t)lane;
+        bool in = gg < e && (int)p.words[gg] == v;
+        unsigned long long m = __ballot(in);
+        int cnt = (m == ~0ull) ? 64 : __builtin_ctzll(~m);
+        int zz = in ? (int)p.z[gg] : 0;
+        for (int i = 0; i < cnt; ++i) {
+            int k = readlane_i(zz, i);
+            const int sl = k & (KPL - 1);
+            const int inc = (lane == (k >> SH)) ? 1 : 0;
+#pragma unroll
+            for (int j = 0; j < KPL; ++j) R.col[j] += (j == sl) ? inc : 0;
+        }
+        if (cnt < 64) break;
+    }
+}
+
+// Fold-in of one document per wavefront (LDAModel.hpp:920-953 with the
+// private state kept as exact deltas: n_k,eff = n_k + n_dk and
+// n_kw,eff = n_kw + c_d, c_d rebuilt per word run from the document's z).
+template <int KPL>
+__global__ void __launch_bounds__(256) infer_kernel(InferParams p) {
+    constexpr int SH = (KPL == 1) ? 0 : (KPL == 2) ? 1 : (KPL == 4) ? 2 : (KPL == 8) ? 3 : (KPL == 16) ? 4 : 5;
+    const int lane = lane_id();
+    Regs<KPL> R;
+    int nvalid, lastLane, lastSlot;
+    lane_limits<KPL>(p.K, lane, nvalid, lastLane, lastSlot);
+    load_kvec_f<KPL>(p.alpha, lane, R.al);
+    int nk0[KPL];
+    load_kvec<KPL>(p.nk, lane, nk0);
+    for (;;) {
+        uint32_t idx;
+        if (p.counter) {
+            idx = 0;
+            if (lane == 0) idx = atomicAdd(p.counter, 1u);
+            idx = (uint32_t)uniform_i((int)idx);
+        } else {
+            idx = (uint32_t)(p.doc_base + (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)));
+        }
+        if ((int64_t)idx >= p.D) break;
+        uint32_t d = p.doc_order ? p.doc_order[idx] : idx;
+        const uint64_t b = p.off[d], e = p.off[d + 1];
+        // initial draw and n_dk
+#pragma unroll
+        for (int j = 0; j < KPL; ++j) R.ndk[j] = 0;
+        for (uint64_t t0 = b; t0 < e; t0 += 64) {
+            uint64_t g = t0 + (uint64_t)lane;
+            int zz = 0xFFFF;
+            if (g < e && p.words[g] < (uint32_t)p.V) {
+                float u = uniform24(philox_x0(p.seed_lo, p.seed_hi, (uint32_t)(g - b), (uint32_t)((g - b) >> 32), 0u, TAG_INFINIT));
+                float r = u * p.alpha_cdf[p.K - 1];
+                int lo = 0, hi = p.K;
+                while (lo < hi) {
+                    int mid = (lo + hi) >> 1;
+                    if (r < p.alpha_cdf[mid]) hi = mid; else lo = mid + 1;
+                }
+                zz = lo < p.K ? lo : p.K - 1;
+            }
+            if (g < e) p.z[g] = (uint16_t)zz;
+            int n = (int)((e - t0) < 64 ? (e - t0) : 64);
+            for (int i = 0; i < n; ++i) {
+                int k = readlane_i(zz, i);
+                if (k == 0xFFFF) continue;
+                const int sl = k & (KPL - 1);
+                const int inc = (lane == (k >> SH)) ? 1 : 0;
+#pragma unroll
+                for (int j = 0; j < KPL; ++j) R.ndk[j] += (j == sl) ? inc : 0;
+            }
+        }
+        int L = 0;
+#pragma unroll
+        for (int j = 0; j < KPL; ++j) {
+            R.A[j] = (float)R.ndk[j] + R.al[j];
+            R.nk[j] = nk0[j] + R.ndk[j];
+            L += R.ndk[j];
+        }
+        for (int o = 32; o >= 1; o >>= 1) L += __shfl_xor(L, o);
+        set_nk<KPL>(R, p.veta);
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+        for (int it = 1; it <= p.iterations; ++it) {
+            int vcur = -1;
+            for (uint64_t t0 = b; t0 < e; t0 += 64) {
+                int n = (int)((e - t0) < 64 ? (e - t0) : 64);
+                uint64_t g = t0 + (uint64_t)lane;
+                int wv = -1, zv = 0xFFFF;
+                float uv = 0.0f;
+                if (lane < n) {
+                    wv = (int)p.words[g];
+                    zv = p.z[g];
+                    uv = uniform24(philox_x0(p.seed_lo, p.seed_hi, (uint32_t)(g - b), (uint32_t)((g - b) >> 32), (uint32_t)it, TAG_INFER));
+                }
+                int zkeep = zv;
+                for (int i = 0; i < n; ++i) {
+                    int v = readlane_i(wv, i);
+                    if ((unsigned)v >= (unsigned)p.V) continue;
+                    int zo = readlane_i(zv, i);
+                    float u = readlane_f(uv, i);
+                    if (v != vcur) {
+                        load_kvec<KPL>(p.nkw + (int64_t)v * p.Kp, lane, R.col);
+                        // the run's topics are read back from memory: make this
+                        // wavefront's earlier z stores visible to its loads
+                        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+                        add_run_counts<KPL>(R, p, lane, t0 + (uint64_t)i, e, v);
+                        vcur = v;
+                    }
+                    int zn = token_step<KPL>(R, lane, zo, u, p.K, nvalid, lastLane, lastSlot, p.eta, p.veta, nullptr);
+                    if (zn != zo) {
+                        zkeep = (lane == i) ? zn : zkeep;
+                        // keep memory current for the next run's histogram
+                        if (lane == i) p.z[g] = (uint16_t)zn;
+                    }
+                }
+                if (lane < n) p.z[g] = (uint16_t)zkeep;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+        }
+        // theta (LDAModel.hpp:1145-1159)
+        const float denom = (float)L + p.asum_f;
+#pragma unroll
+        for (int j = 0; j < KPL; ++j) {
+            int k = lane * KPL + j;
+            if (k < p.K) p.theta[(int64_t)d * p.K + k] = R.A[j] / denom;
+        }
+        // FP64 log-likelihood: getLLRest(private) - getLLRest(global) + getLLDocs(doc)
+        const double eta = (double)p.eta;
+        const double veta = (double)p.V * eta;
+        const double lge = lgamma(eta);
+        double ll = 0.0;
+#pragma unroll
+        for (int j = 0; j < KPL; ++j) {
+            if (R.ndk[j]) {
+                ll += lgamma((double)nk0[j] + veta) - lgamma((double)(nk0[j] + R.ndk[j]) + veta);
+                double a = (double)R.al[j];
+                ll += lgamma((double)R.ndk[j] + a) - lgamma(a);
+            }
+        }
+        for (uint64_t t0 = b; t0 < e; t0 += 64) {
+            uint64_t g = t0 + (uint64_t)lane;
+            int wv = g < e ? (int)p.words[g] : -1;
+            int wp = (g < e && g > b) ? (int)p.words[g - 1] : -2;
+            unsigned long long starts = __ballot(g < e && (unsigned)wv < (unsigned)p.V && wv != wp);
+            while (starts) {
+                int i = __builtin_ctzll(starts);
+                starts &= starts - 1;
+                int v = readlane_i(wv, i);
+                int base[KPL];
+                load_kvec<KPL>(p.nkw + (int64_t)v * p.Kp, lane, base);
+#pragma unroll
+                for (int j = 0; j < KPL; ++j) R.col[j] = base[j];
+                add_run_counts<KPL>(R, p, lane, t0 + (uint64_t)i, e, v);
+#pragma unroll
+                for (int j = 0; j < KPL; ++j) {
+                    int c = R.col[j] - base[j];
+                    if (c) {
+                        ll += lgamma((double)R.col[j] + eta) - lge;
+                        if (base[j]) ll -= lgamma((double)base[j] + eta) - lge;
+                    }
+                }
+            }
+        }
+        ll = wave_sum_d(ll);
+        if (lane == 0) {
+            const double asum = p.asum_d;
+            ll -= lgamma((double)L + asum) - lgamma(asum);
+            p.ll[d] = ll;
+        }
+        if (!p.counter) break;
+    }
+}
+
+}  // namespace
+
+// ================================================================ launchers
+#define LDA_DISPATCH_KPL(kpl, CALL)            \
+    switch (kpl) {                             \
+        case 1: { constexpr int KPL = 1; CALL; } break;   \
+        case 2: { constexpr int KPL = 2; CALL; } break;   \
+        case 4: { constexpr int KPL = 4; CALL; } break;   \
+        case 8: { constexpr int KPL = 8; CALL; } break;   \
+        case 16: { constexpr int KPL = 16; CALL; } break; \
+        case 32: { constexpr int KPL = 32; CALL; } break; \
+        default: return hipErrorInvalidValue;             \
+    }
+
+hipError_t launch_init_topics(const uint32_t* words, uint16_t* z, uint64_t N, const float* ca, int K, int V, uint32_t s0,
+                              uint32_t s1, hipStream_t s) {
+    if (!N) return hipSuccess;
+    uint64_t blocks = (N + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(init_topics_kernel, dim3((unsigned)blocks), dim3(256), 0, s, words, z, N, ca, K, V, s0, s1);
+    return hipGetLastError();
+}
+
+hipError_t launch_build_counts(int kpl, const uint32_t* words, const uint16_t* z, const uint64_t* off, int64_t D,
+                               int32_t* ndk, int32_t* nkw, int32_t* nk, int Kp, int V, hipStream_t s) {
+    if (!D) return hipSuccess;
+    int64_t blocks = (D + 3) / 4;
+    if (blocks > 16384) blocks = 16384;
+    LDA_DISPATCH_KPL(kpl, hipLaunchKernelGGL(build_counts_kernel<KPL>, dim3((unsigned)blocks), dim3(256), 0, s, words, z,
+                                             off, D, ndk, nkw, nk, Kp, V));
+    return hipGetLastError();
+}
+
+hipError_t launch_sweep_device(int kpl, const SweepParams& p, int grid, hipStream_t s) {
+    LDA_DISPATCH_KPL(kpl, hipLaunchKernelGGL(sweep_device_kernel<KPL>, dim3(grid), dim3(256), 0, s, p));
+    return hipGetLastError();
+}
+
+hipError_t launch_sweep_det(int kpl, const SweepParams& p, hipStream_t s) {
+    LDA_DISPATCH_KPL(kpl, hipLaunchKernelGGL(sweep_det_kernel<KPL>, dim3(p.W), dim3(64), 0, s, p));
+    return hipGetLastError();
+}
+
+int sweep_device_blocks_per_cu(int kpl) {
+    int n = 0;
+    hipError_t e = hipErrorInvalidValue;
+    switch (kpl) {
+        case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sweep_device_kernel<1>, 256, 0); break;
+        case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sweep_device_kernel<2>, 256, 0); break;
+        case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sweep_device_kernel<4>, 256, 0); break;
+        case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sweep_device_kernel<8>, 256, 0); break;
+        case 16: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sweep_device_kernel<16>, 256, 0); break;
+        case 32: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sweep_device_kernel<32>, 256, 0); break;
+        default: break;
+    }
+    return (e == hipSuccess && n > 0) ? n : 1;
+}
+
+int infer_blocks_per_cu(int kpl) {
+    int n = 0;
+    hipError_t e = hipErrorInvalidValue;
+    switch (kpl) {
+        case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_kernel<1>, 256, 0); break;
+        case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_kernel<2>, 256, 0); break;
+        case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_kernel<4>, 256, 0); break;
+        case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_kernel<8>, 256, 0); break;
+        case 16: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_kernel<16>, 256, 0); break;
+        case 32: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_kernel<32>, 256, 0); break;
+        default: break;
+    }
+    return (e == hipSuccess && n > 0) ? n : 1;
+}
+
+hipError_t launch_copy_i32(int32_t* dst, const int32_t* src, int64_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(copy_i32_kernel, dim3((unsigned)blocks), dim3(256), 0, s, dst, src, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t* dst, uint32_t v, int64_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned)blocks), dim3(256), 0, s, dst, v, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_rowsum_nkw(const int32_t* nkw, int V, int Kp, int32_t* nk, hipStream_t s) {
+    hipLaunchKernelGGL(rowsum_kernel, dim3((Kp + 255) / 256), dim3(256), 0, s, nkw, V, Kp, nk);
+    return hipGetLastError();
+}
+
+hipError_t launch_loglik(const int32_t* ndk, const int32_t* doclen, int64_t D, const int32_t* nkw, const int32_t* nk,
+                         const float* alpha, int V, int Kp, double asum, double eta, double veta, double* partial,
+                         int nblk, hipStream_t s) {
+    hipLaunchKernelGGL(loglik_kernel, dim3(nblk), dim3(256), 0, s, ndk, doclen, D, nkw, nk, alpha, V, Kp, asum, eta,
+                       veta, partial);
+    return hipGetLastError();
+}
+
+hipError_t launch_alpha_sums(const int32_t* ndk, const int32_t* doclen, int64_t D, const float* alpha, int Kp, float asum,
+                             double* part, int nblk, hipStream_t s) {
+    hipLaunchKernelGGL(alpha_sums_kernel, dim3(nblk), dim3(256), 0, s, ndk, doclen, D, alpha, Kp, asum, part);
+    return hipGetLastError();
+}
+
+hipError_t launch_doclen(const uint32_t* words, const uint64_t* off, int64_t D, int V, int32_t* doclen, hipStream_t s) {
+    if (!D) return hipSuccess;
+    int64_t blocks = (D + 3) / 4;
+    if (blocks > 16384) blocks = 16384;
+    hipLaunchKernelGGL(doclen_kernel, dim3((unsigned)blocks), dim3(256), 0, s, words, off, D, V, doclen);
+    return hipGetLastError();
+}
+
+hipError_t launch_invariants(int kpl, const uint32_t* words, const uint16_t* z, const uint64_t* off, int64_t D,
+                             const int32_t* ndk, const int32_t* nkw, const int32_t* nk, const int32_t* doclen, int K,
+                             int V, int Kp, int32_t* sndk, int32_t* snkw, long long* colsum, unsigned long long* flags,
+                             hipStream_t s) {
+    (void)nk;
+    int64_t blocks = (D + 3) / 4;
+    if (blocks > 16384) blocks = 16384;
+    if (blocks < 1) blocks = 1;
+    LDA_DISPATCH_KPL(kpl, hipLaunchKernelGGL(recount_kernel<KPL>, dim3((unsigned)blocks), dim3(256), 0, s, words, z, off, D,
+                                             K, V, Kp, sndk, snkw, flags));
+    hipLaunchKernelGGL(compare_kernel, dim3(1024), dim3(256), 0, s, ndk, sndk, D * (int64_t)Kp, flags + 3, flags + 5);
+    hipLaunchKernelGGL(compare_kernel, dim3(1024), dim3(256), 0, s, nkw, snkw, (int64_t)V * Kp, flags + 4, flags + 5);
+    hipLaunchKernelGGL(ndk_rows_kernel, dim3((unsigned)blocks), dim3(256), 0, s, ndk, doclen, D, Kp, flags + 0);
+    hipLaunchKernelGGL(colsum_kernel, dim3((Kp + 255) / 256), dim3(256), 0, s, nkw, V, Kp, colsum);
+    return hipGetLastError();
+}
+
+hipError_t launch_sample_frozen(int kpl, const int32_t* ndk, const int32_t* nkw, const int32_t* nk, const float* alpha,
+                                float eta, float veta, int K, int V, int Kp, int64_t n, const int64_t* doc,
+                                const uint32_t* word, const uint16_t* z_old, const float* u, uint16_t* z_new,
+                                float* total, hipStream_t s) {
+    (void)V;
+    if (n <= 0) return hipSuccess;
+    int64_t grid = n < 65536 ? n : 65536;
+    LDA_DISPATCH_KPL(kpl, hipLaunchKernelGGL(sample_frozen_kernel<KPL>, dim3((unsigned)grid), dim3(64), 0, s, ndk, nkw, nk,
+                                             alpha, eta, veta, K, Kp, n, doc, word, z_old, u, z_new, total));
+    return hipGetLastError();
+}
+
+hipError_t launch_infer(int kpl, const InferParams& p, int grid, hipStream_t s) {
+    LDA_DISPATCH_KPL(kpl, hipLaunchKernelGGL(infer_kernel<KPL>, dim3(grid), dim3(256), 0, s, p));
+    return hipGetLastError();
+}
+
+}  // namespace lda

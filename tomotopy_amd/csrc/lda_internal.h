@@ -1,0 +1,107 @@
+// Internal interface between the host core (lda_capi.cpp) and the gfx950
+// kernels (lda_device.hip).  Not part of the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lda {
+
+// Philox stream tags (4th counter word); the CPU oracle uses the same values.
+constexpr uint32_t TAG_SWEEP = 0x53574550u;
+constexpr uint32_t TAG_INIT = 0x494E4954u;
+constexpr uint32_t TAG_SHUFFLE = 0x53484646u;
+constexpr uint32_t TAG_INFER = 0x494E4652u;
+constexpr uint32_t TAG_INFINIT = 0x49494E49u;
+
+// Device K-vector layout ("lane layout").  A K-vector is padded to
+// Kp = 64*kpl entries; lane l of a wavefront owns topics [l*kpl, (l+1)*kpl).
+// For kpl >= 4 the entries are stored so that the g-th 16-byte load of every
+// lane is one contiguous 1 KiB segment: topic k = l*kpl + j lives at
+// (j/4)*256 + l*4 + j%4.  For kpl < 4 the layout is the natural order.
+inline int kvec_pos(int kpl, int k) {
+    if (kpl < 4) return k;
+    int l = k / kpl, j = k % kpl;
+    return (j >> 2) * 256 + l * 4 + (j & 3);
+}
+
+struct SweepParams {
+    const uint32_t* words;
+    uint16_t* z;
+    const uint64_t* off;      // D+1 document offsets
+    int32_t* ndk;             // D x Kp
+    int32_t* nkw;             // V x Kp
+    const int32_t* nk_in;     // device-wide: n_k snapshot; det: private n_k (W x Kp)
+    int32_t* nk_out;          // device-wide: accumulates deltas; det: private n_k (same as nk_in)
+    const float* alpha;       // Kp, zero padded
+    float eta, veta;
+    int K, V, Kp;
+    uint32_t seed_lo, seed_hi, sweep;
+    int64_t D;
+    const uint32_t* doc_order;  // device-wide: documents, longest first
+    uint32_t* counter;          // device-wide: next document
+    const uint64_t* coff;       // det: (W+1) x D chunk offsets
+    int W, round;
+};
+
+struct InferParams {
+    const uint32_t* words;   // batch CSR
+    uint16_t* z;             // batch tokens
+    const uint64_t* off;     // D+1
+    const int32_t* nkw;      // frozen V x Kp
+    const int32_t* nk;       // frozen Kp
+    const float* alpha;      // Kp
+    const float* alpha_cdf;  // K, natural order, sequential float sum
+    float eta, veta;
+    int K, V, Kp;
+    uint32_t seed_lo, seed_hi;
+    int iterations;
+    int64_t D;
+    const uint32_t* doc_order;
+    uint32_t* counter;
+    float* theta;            // D x K natural order
+    double* ll;              // D
+    double asum_d;           // sum of alpha in double
+    float asum_f;            // sequential float sum of alpha
+    int64_t doc_base;        // plain mode: index of the single document
+};
+
+// kernels (lda_device.hip)
+hipError_t launch_init_topics(const uint32_t* words, uint16_t* z, uint64_t N, const float* alpha_cdf, int K, int V,
+                              uint32_t seed_lo, uint32_t seed_hi, hipStream_t s);
+hipError_t launch_build_counts(int kpl, const uint32_t* words, const uint16_t* z, const uint64_t* off, int64_t D,
+                               int32_t* ndk, int32_t* nkw, int32_t* nk, int Kp, int V, hipStream_t s);
+hipError_t launch_sweep_device(int kpl, const SweepParams& p, int grid, hipStream_t s);
+hipError_t launch_sweep_det(int kpl, const SweepParams& p, hipStream_t s);
+int sweep_device_blocks_per_cu(int kpl);
+hipError_t launch_copy_i32(int32_t* dst, const int32_t* src, int64_t n, hipStream_t s);
+hipError_t launch_fill_u32(uint32_t* dst, uint32_t v, int64_t n, hipStream_t s);
+hipError_t launch_rowsum_nkw(const int32_t* nkw, int V, int Kp, int32_t* nk, hipStream_t s);
+
+// FP64 log-likelihood partial sums: out[0..nblk) docs part, out[nblk..2nblk) rest part.
+hipError_t launch_loglik(const int32_t* ndk, const int32_t* doclen, int64_t D, const int32_t* nkw, const int32_t* nk,
+                         const float* alpha, int V, int Kp, double asum, double eta, double veta, double* partial,
+                         int nblk, hipStream_t s);
+
+// alpha fixed point: per-topic sums of digammaApprox(n_dk + alpha_k) and the
+// denominator sum of digammaApprox(L_d + asum); part[b*(Kp+1) + pos].
+hipError_t launch_alpha_sums(const int32_t* ndk, const int32_t* doclen, int64_t D, const float* alpha, int Kp,
+                             float asum, double* part, int nblk, hipStream_t s);
+hipError_t launch_doclen(const uint32_t* words, const uint64_t* off, int64_t D, int V, int32_t* doclen, hipStream_t s);
+
+// invariants
+hipError_t launch_invariants(int kpl, const uint32_t* words, const uint16_t* z, const uint64_t* off, int64_t D,
+                             const int32_t* ndk, const int32_t* nkw, const int32_t* nk, const int32_t* doclen,
+                             int K, int V, int Kp, int32_t* scratch_ndk, int32_t* scratch_nkw,
+                             long long* colsum, unsigned long long* flags, hipStream_t s);
+
+// frozen-state single steps
+hipError_t launch_sample_frozen(int kpl, const int32_t* ndk, const int32_t* nkw, const int32_t* nk,
+                                const float* alpha, float eta, float veta, int K, int V, int Kp, int64_t n,
+                                const int64_t* doc, const uint32_t* word, const uint16_t* z_old, const float* u,
+                                uint16_t* z_new, float* total, hipStream_t s);
+
+// fold-in
+hipError_t launch_infer(int kpl, const InferParams& p, int grid, hipStream_t s);
+int infer_blocks_per_cu(int kpl);
+
+}  // namespace lda

@@ -1,0 +1,119 @@
+"""Seeded synthetic corpora from the LDA generative process.
+
+Word ids are dictionary ids in order of first appearance, as tomotopy's
+add_doc assigns them (Dictionary::add), so `corpus.prepare_arrays` applies the
+same vocabulary ordering a tomotopy user would get.
+
+Workloads (the sizes the engine is measured and checked on):
+  plain_k20      1,000 docs, V=5,000, 20 true topics, alpha=0.1, beta=0.01,
+                 lengths LogNormal(ln 200, 0.5) clipped to [20, 2000]
+  readme_k100    1,000 docs, mean length 1,500, V=50,000, Zipf(1.07)
+                 background mixed 50/50 into 100 Dir(0.01) topics
+  large_k1000    100,000 docs, mean length 300, V=100,000, Zipf(1.07), 1,000
+                 topics
+  foldin_k200    20,000 unseen docs, lengths LogNormal(ln 200, 1.2) clipped
+                 to [5, 20000], against a K=200, V=50,000 model
+  serve_k500     requests of 1..64 docs, lengths LogNormal(ln 150, 1.0)
+                 clipped to [5, 10000], against a K=500, V=100,000 model
+"""
+import numpy as np
+
+
+def _rng(seed):
+    return np.random.Generator(np.random.PCG64(seed))
+
+
+def lognormal_lengths(rng, n, mu, sigma, lo, hi):
+    x = np.rint(rng.lognormal(mu, sigma, size=n)).astype(np.int64)
+    return np.clip(x, lo, hi)
+
+
+def _sparse_topic(rng, V, beta, keep_mass=1.0 - 1e-7, max_words=20000):
+    g = rng.gamma(beta, size=V)
+    s = g.sum()
+    if s <= 0:
+        g = np.ones(V)
+        s = float(V)
+    p = g / s
+    idx = np.argsort(-p, kind="stable")
+    c = np.cumsum(p[idx])
+    n = int(min(np.searchsorted(c, keep_mass) + 1, max_words, V))
+    ids = idx[:n]
+    pp = p[ids] / p[ids].sum()
+    return ids, np.cumsum(pp)
+
+
+def generate(seed, n_docs, V, true_k, alpha, beta, lengths, zipf_s=None, mix=0.5):
+    """Draw a corpus; returns (words, doc_off, n_words) with first-appearance ids."""
+    rng = _rng(seed)
+    topics = [_sparse_topic(rng, V, beta) for _ in range(true_k)]
+    if zipf_s is not None:
+        zp = np.arange(1, V + 1, dtype=np.float64) ** (-float(zipf_s))
+        zcdf = np.cumsum(zp / zp.sum())
+    lengths = np.asarray(lengths, dtype=np.int64)
+    N = int(lengths.sum())
+    z = np.empty(N, dtype=np.int32)
+    pos = 0
+    for d in range(n_docs):
+        L = int(lengths[d])
+        th = rng.gamma(alpha, size=true_k)
+        s = th.sum()
+        th = th / s if s > 0 else np.full(true_k, 1.0 / true_k)
+        z[pos:pos + L] = np.minimum(np.searchsorted(np.cumsum(th), rng.random(L), side="right"), true_k - 1)
+        pos += L
+    words = np.empty(N, dtype=np.int64)
+    if zipf_s is not None:
+        bg = rng.random(N) < mix
+        nb = int(bg.sum())
+        words[bg] = np.minimum(np.searchsorted(zcdf, rng.random(nb), side="right"), V - 1)
+        tsel = np.nonzero(~bg)[0]
+    else:
+        tsel = np.arange(N)
+    if len(tsel):
+        zt = z[tsel]
+        order = np.argsort(zt, kind="stable")
+        bounds = np.searchsorted(zt[order], np.arange(true_k + 1))
+        for k in range(true_k):
+            sel = tsel[order[bounds[k]:bounds[k + 1]]]
+            if not len(sel):
+                continue
+            ids, cdf = topics[k]
+            words[sel] = ids[np.minimum(np.searchsorted(cdf, rng.random(len(sel)), side="right"), len(ids) - 1)]
+    # dictionary ids in order of first appearance
+    uniq, first = np.unique(words, return_index=True)
+    rank = np.empty(len(uniq), dtype=np.int64)
+    rank[np.argsort(first, kind="stable")] = np.arange(len(uniq))
+    dense = rank[np.searchsorted(uniq, words)]
+    off = np.zeros(n_docs + 1, dtype=np.uint64)
+    np.cumsum(lengths, out=off[1:])
+    return dense.astype(np.uint32), off, int(len(uniq))
+
+
+WORKLOADS = {
+    "plain_k20": dict(seed=1001, n_docs=1000, V=5000, true_k=20, alpha=0.1, beta=0.01, mu=np.log(200.0), sigma=0.5,
+                      lo=20, hi=2000, zipf_s=None, model_k=20, model_alpha=0.1, model_eta=0.01, iterations=100),
+    "readme_k100": dict(seed=1002, n_docs=1000, V=50000, true_k=100, alpha=0.1, beta=0.01,
+                        mu=np.log(1500.0) - 0.125, sigma=0.5, lo=10, hi=15000, zipf_s=1.07, model_k=100,
+                        model_alpha=0.1, model_eta=0.01, iterations=200),
+    "large_k1000": dict(seed=1003, n_docs=100000, V=100000, true_k=1000, alpha=0.05, beta=0.01,
+                        mu=np.log(300.0) - 0.125, sigma=0.5, lo=10, hi=3000, zipf_s=1.07, model_k=1000,
+                        model_alpha=0.05, model_eta=0.01, iterations=50),
+}
+
+
+def workload(name, n_docs=None):
+    """Generate one of the named training corpora (optionally fewer docs)."""
+    w = dict(WORKLOADS[name])
+    D = int(n_docs or w["n_docs"])
+    rng = _rng(w["seed"] + 7919)
+    lengths = lognormal_lengths(rng, D, w["mu"], w["sigma"], w["lo"], w["hi"])
+    words, off, nw = generate(w["seed"], D, w["V"], w["true_k"], w["alpha"], w["beta"], lengths, w["zipf_s"])
+    return words, off, nw, w
+
+
+def unseen_docs(seed, n_docs, V, true_k, mu, sigma, lo, hi, zipf_s=1.07, alpha=0.1, beta=0.01):
+    """Unseen documents over word ids 0..V-1 (ids of an existing dictionary)."""
+    rng = _rng(seed + 104729)
+    lengths = lognormal_lengths(rng, n_docs, mu, sigma, lo, hi)
+    words, off, nw = generate(seed, n_docs, V, true_k, alpha, beta, lengths, zipf_s)
+    return words, off
