@@ -1,0 +1,106 @@
+"""Host-side checks that need no GPU: the C-ABI library, corpus preparation,
+the LDAModel surface's argument handling, and the synthetic workloads."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from tomotopy_amd import _native, corpus, synth
+from tomotopy_amd.models import LDAModel, ParallelScheme, TermWeight
+
+
+def test_library_loads_and_exports_every_declared_symbol():
+    lib = _native.load()
+    names = _native.header_symbols()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(lib, n), n
+        assert n in _native._SIGS, "binding misses %s" % n
+    assert lib.lda_version() == 1
+
+
+def test_enums_match_tomotopy():
+    assert [int(x) for x in TermWeight] == [0, 1, 2]
+    assert (ParallelScheme.DEFAULT, ParallelScheme.NONE, ParallelScheme.COPY_MERGE, ParallelScheme.PARTITION) == \
+        (0, 1, 2, 3)
+
+
+def test_vocab_order_filters_and_rotation():
+    cf = np.array([5, 1, 9, 9, 3, 0])
+    df = np.array([2, 1, 3, 1, 3, 0])
+    order, real_v = corpus.vocab_order(cf, df)
+    # (9,3) > (9,1) > (5,2) > (3,3) > (1,1) > (0,0)
+    assert order.tolist() == [2, 4, 0, 1, 3, 5] and real_v == 6
+    order, real_v = corpus.vocab_order(cf, df, min_cf=2, rm_top=1)
+    # kept by frequency: 2,3,0,4 ; filtered: 1,5 ; top-1 (word 2) rotated to the end
+    new_to_old = np.argsort(order)
+    assert new_to_old.tolist() == [3, 0, 4, 1, 5, 2] and real_v == 3
+
+
+def test_build_csr_sorts_and_records_order():
+    docs = [np.array([3, 1, 2, 1]), np.array([0])]
+    words, off, w_orders = corpus.build_csr(docs, np.arange(4))
+    assert words.tolist() == [1, 1, 2, 3, 0] and off.tolist() == [0, 4, 5]
+    assert w_orders[0].tolist() == [3, 0, 2, 1]
+    # vectorised path agrees
+    w2, off2 = corpus.csr_from_arrays(np.array([3, 1, 2, 1, 0]), np.array([0, 4, 5], np.uint64), np.arange(4))
+    assert w2.tolist() == words.tolist()
+
+
+def test_prepare_arrays_matches_per_document_path():
+    words, off, nw, _ = synth.workload("plain_k20", n_docs=30)
+    ws, offs, real_v, order = corpus.prepare_arrays(words, off, nw)
+    docs = [words[int(off[d]):int(off[d + 1])] for d in range(30)]
+    cf, df = corpus.counts(docs, nw)
+    order2, real_v2 = corpus.vocab_order(cf, df)
+    w2, off2, _ = corpus.build_csr(docs, order2)
+    assert real_v == real_v2 and np.array_equal(order, order2) and np.array_equal(ws, w2)
+
+
+def test_synthetic_workload_shapes():
+    words, off, nw, w = synth.workload("plain_k20")
+    L = np.diff(off).astype(np.int64)
+    assert len(L) == 1000 and L.min() >= 20 and L.max() <= 2000
+    assert 150_000 < len(words) < 300_000 and nw <= 5000
+    again = synth.workload("plain_k20")[0]
+    assert np.array_equal(words, again)  # seeded
+
+
+def test_model_argument_checks():
+    with pytest.raises(ValueError):
+        LDAModel(k=0)
+    with pytest.raises(ValueError):
+        LDAModel(k=3, alpha=[0.1, 0.2])
+    with pytest.raises(ValueError):
+        LDAModel(k=3, alpha=-1.0)
+    with pytest.raises(ValueError):
+        LDAModel(k=3, eta=0.0)
+    with pytest.raises(ValueError):
+        LDAModel(k=3, seed="x")
+    with pytest.raises(NotImplementedError):
+        LDAModel(tw=TermWeight.PMI, k=3)
+    m = LDAModel(k=3, seed=1)
+    with pytest.raises(ValueError):
+        m.add_doc("a string")
+    with pytest.raises(ValueError):
+        m.add_doc([[0, 1]])  # tomotopy issue 187: non-str words
+    assert m.add_doc([]) is None
+    with pytest.raises(ValueError):
+        m.add_doc([], ignore_empty_words=False)
+    assert m.add_doc(["a", "b", "a"]) == 0
+    with pytest.raises(RuntimeError):
+        m.make_doc(["a"])
+    with pytest.raises(RuntimeError):
+        m.infer([])
+    assert m.ll_per_word == 0.0 and m.num_words == 0
+
+
+def test_device_calls_fail_loudly_without_gpu():
+    if _native.device_count() > 0:
+        pytest.skip("a HIP device is visible")
+    with pytest.raises(RuntimeError):
+        _native.Context(4, 10, 0.1, 0.01, 1)
+    m = LDAModel(k=2, seed=1)
+    m.add_doc(["a", "b"])
+    with pytest.raises(RuntimeError):
+        m.train(1)

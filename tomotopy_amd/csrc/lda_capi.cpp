@@ -782,8 +782,7 @@ int lda_infer(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off, int64_
         p.counter = c->i_counter.p;
         HIP_TRY(lda::launch_fill_u32(c->i_counter.p, 0u, 1, s));
         int grid = c->num_cu * lda::infer_blocks_per_cu(c->kpl);
-        int64_t need = (D + 3) / 4;
-        if (grid > need) grid = (int)need;
+        if (grid > D) grid = (int)D;
         HIP_TRY(hipEventRecord(c->ev0, s));
         HIP_TRY(lda::launch_infer(c->kpl, p, grid, s));
         HIP_TRY(hipEventRecord(c->ev1, s));
@@ -794,8 +793,6 @@ int lda_infer(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off, int64_
         HIP_TRY(hipEventRecord(c->ev0, s));
         for (int64_t d = 0; d < D; ++d) {
             p.doc_base = d;
-            // one wavefront: a 256-thread block whose other waves fall past D or
-            // onto the next documents is avoided by launching exactly one wave
             lda::InferParams q = p;
             q.D = d + 1;
             HIP_TRY(lda::launch_infer(c->kpl, q, 1, s));

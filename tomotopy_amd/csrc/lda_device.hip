@@ -53,8 +53,8 @@ __device__ __forceinline__ int kpos(int lane, int j) {
     else return (j >> 2) * 256 + lane * 4 + (j & 3);
 }
 
-template <int KPL>
-__device__ __forceinline__ void load_kvec(const int32_t* __restrict__ base, int lane, int (&o)[KPL]) {
+template <int KPL, typename V>
+__device__ __forceinline__ void load_kvec(const int32_t* __restrict__ base, int lane, V& o) {
     if constexpr (KPL >= 4) {
 #pragma unroll
         for (int g = 0; g < KPL / 4; ++g) {
@@ -69,8 +69,8 @@ __device__ __forceinline__ void load_kvec(const int32_t* __restrict__ base, int 
     }
 }
 
-template <int KPL>
-__device__ __forceinline__ void load_kvec_f(const float* __restrict__ base, int lane, float (&o)[KPL]) {
+template <int KPL, typename V>
+__device__ __forceinline__ void load_kvec_f(const float* __restrict__ base, int lane, V& o) {
     if constexpr (KPL >= 4) {
 #pragma unroll
         for (int g = 0; g < KPL / 4; ++g) {
@@ -85,8 +85,8 @@ __device__ __forceinline__ void load_kvec_f(const float* __restrict__ base, int 
     }
 }
 
-template <int KPL>
-__device__ __forceinline__ void store_kvec(int32_t* __restrict__ base, int lane, const int (&o)[KPL]) {
+template <int KPL, typename V>
+__device__ __forceinline__ void store_kvec(int32_t* __restrict__ base, int lane, const V& o) {
     if constexpr (KPL >= 4) {
 #pragma unroll
         for (int g = 0; g < KPL / 4; ++g)
@@ -123,160 +123,71 @@ __device__ __forceinline__ int readlane_i(int v, int l) { return __builtin_amdgc
 __device__ __forceinline__ float readlane_f(float v, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l)); }
 __device__ __forceinline__ int uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// Register-resident per-lane topic state of one worker.
+template <int N> using ivec = int __attribute__((ext_vector_type(N)));
+template <int N> using fvec = float __attribute__((ext_vector_type(N)));
+
 template <int KPL>
-struct Regs {
-    int ndk[KPL];   // n_dk of the current document
-    float A[KPL];   // float(n_dk) + alpha_k
-    float al[KPL];  // alpha_k
-    int nk[KPL];    // worker-private n_k
-    float bf[KPL];  // float(n_k) + V*eta
-    float y[KPL];   // RN(1 / bf)
-    int col[KPL];   // n_kw[., v] of the current word
+constexpr int kpl_shift() {
+    return (KPL == 1) ? 0 : (KPL == 2) ? 1 : (KPL == 4) ? 2 : (KPL == 8) ? 3 : (KPL == 16) ? 4 : 5;
+}
+
+// Per-wavefront LDS slice of the state that receives single-topic updates,
+// each array Kp = 64*KPL entries in the lane layout (kpos):
+//   ndk  n_dk of the current document
+//   nk   the worker-private n_k
+//   y    RN(1 / (n_k + V*eta)), kept in step with nk
+//   col  n_kw[., v] of the current word including this worker's own updates
+// A topic update is one owner-lane LDS write; the per-token reads are
+// conflict-free 16-byte ds_read per lane.
+struct WaveLds {
+    int* ndk;
+    int* nk;
+    float* y;
+    int* col;
 };
 
-// addWordTo<delta> for one topic on the register state (LDAModel.hpp:287-299).
-// `sl` is wave-uniform; only the owner lane changes its counts, but every lane
-// recomputes the derived values of slot `sl` from its own counts, which leaves
-// them bit-identical in the other lanes.
-template <int KPL>
-__device__ __forceinline__ void bump(Regs<KPL>& R, bool me, int sl, int delta, float veta) {
-    const int dd = me ? delta : 0;
-    // `sl` is uniform: one taken branch, static register indices (no scratch)
-#pragma unroll
-    for (int j = 0; j < KPL; ++j) {
-        if (j == sl) {
-            int nd = R.ndk[j] + dd;
-            R.ndk[j] = nd;
-            R.A[j] = (float)nd + R.al[j];
-            R.col[j] = R.col[j] + dd;
-            int nkv = R.nk[j] + dd;
-            R.nk[j] = nkv;
-            float b = (float)nkv + veta;
-            R.bf[j] = b;
-            R.y[j] = 1.0f / b;
-        }
-    }
-}
+extern __shared__ int lds_dyn[];
 
-// Value of register slot `sl` (uniform) of a per-lane array.
-template <int KPL, typename T>
-__device__ __forceinline__ T pick(const T (&a)[KPL], int sl) {
-    T r = a[0];
-#pragma unroll
-    for (int j = 1; j < KPL; ++j) r = (j == sl) ? a[j] : r;
-    return r;
+template <int KPL>
+__device__ __forceinline__ WaveLds wave_lds() {
+    constexpr int Kp = 64 * KPL;
+    int* base = lds_dyn + (threadIdx.x >> 6) * (4 * Kp);
+    return WaveLds{base, base + Kp, reinterpret_cast<float*>(base + 2 * Kp), base + 3 * Kp};
 }
 
 template <int KPL>
-__device__ __forceinline__ void set_doc(Regs<KPL>& R, const int32_t* __restrict__ ndk_row, int lane) {
-    load_kvec<KPL>(ndk_row, lane, R.ndk);
-#pragma unroll
-    for (int j = 0; j < KPL; ++j) R.A[j] = (float)R.ndk[j] + R.al[j];
-}
+constexpr size_t wave_lds_bytes() { return (size_t)4 * 64 * KPL * sizeof(int); }
 
+template <int KPL, typename V>
+__device__ __forceinline__ void lds_load(const int* a, int lane, V& o) { load_kvec<KPL>(a, lane, o); }
+
+template <int KPL, typename V>
+__device__ __forceinline__ void lds_store(int* a, int lane, const V& o) { store_kvec<KPL>(a, lane, o); }
+
+// addWordTo<delta> for topic z on this worker's state (LDAModel.hpp:287-299):
+// the owner lane updates n_dk, the column, the private n_k and its reciprocal.
 template <int KPL>
-__device__ __forceinline__ void set_nk(Regs<KPL>& R, float veta) {
-#pragma unroll
-    for (int j = 0; j < KPL; ++j) {
-        R.bf[j] = (float)R.nk[j] + veta;
-        R.y[j] = 1.0f / R.bf[j];
+__device__ __forceinline__ void bump(const WaveLds& S, int lane, int z, int delta, float veta) {
+    if (lane == (z >> kpl_shift<KPL>())) {
+        const int q = kpos<KPL>(lane, z & (KPL - 1));
+        S.ndk[q] += delta;
+        S.col[q] += delta;
+        const int n = S.nk[q] + delta;
+        S.nk[q] = n;
+        S.y[q] = 1.0f / ((float)n + veta);
     }
 }
 
-// One token: remove z_old, weight, scan, search, add z_new.  Returns z_new
-// (wave-uniform).  *total receives cdf[K-1].
+// Private n_k (and reciprocals) from a global K-vector.
 template <int KPL>
-__device__ __forceinline__ int token_step(Regs<KPL>& R, int lane, int zo, float u, int K, int nvalid, int lastLane,
-                                          int lastSlot, float eta, float veta, float* total) {
-    constexpr int SH = (KPL == 1) ? 0 : (KPL == 2) ? 1 : (KPL == 4) ? 2 : (KPL == 8) ? 3 : (KPL == 16) ? 4 : 5;
-    bump<KPL>(R, lane == (zo >> SH), zo & (KPL - 1), -1, veta);
-    float w[KPL];
+__device__ __forceinline__ void set_nk(const WaveLds& S, const int32_t* __restrict__ src, int lane, float veta) {
+    ivec<KPL> n;
+    load_kvec<KPL>(src, lane, n);
+    fvec<KPL> y;
 #pragma unroll
-    for (int j = 0; j < KPL; ++j) {
-        float B = (float)R.col[j] + eta;
-        float a = R.A[j] * B;
-        float q0 = a * R.y[j];
-        float rr = __builtin_fmaf(-q0, R.bf[j], a);
-        w[j] = __builtin_fmaf(rr, R.y[j], q0);
-    }
-#pragma unroll
-    for (int j = 1; j < KPL; ++j) w[j] = w[j - 1] + w[j];
-    float incl = wave_inclusive_scan(w[KPL - 1]);
-    float excl = dpp_wave_shr1(incl);
-#pragma unroll
-    for (int j = 0; j < KPL; ++j) w[j] = excl + w[j];
-    float tot = readlane_f(pick<KPL, float>(w, lastSlot), lastLane);
-    float r = u * tot;
-    int miss = 0;
-#pragma unroll
-    for (int j = 0; j < KPL; ++j) miss += (j < nvalid && !(r < w[j])) ? 1 : 0;
-    unsigned long long m = __ballot(miss < nvalid);
-    int zn;
-    if (m == 0ull) zn = K - 1;
-    else {
-        int L = __builtin_ctzll(m);
-        zn = L * KPL + readlane_i(miss, L);
-    }
-    zn = uniform_i(zn);
-    bump<KPL>(R, lane == (zn >> SH), zn & (KPL - 1), +1, veta);
-    if (total) *total = tot;
-    return zn;
-}
-
-// forShuffled's prime table (Utils.hpp:159-163).
-__constant__ uint64_t kShufflePrimes[16] = {65537, 65539, 65543, 65551, 65557, 65563, 65579, 65581,
-                                            65587, 65599, 65609, 65617, 65629, 65633, 65647, 65651};
-
-constexpr int MODE_DET = 0;     // exclusive cell owner: plain stores to n_kw
-constexpr int MODE_ATOMIC = 1;  // shared n_kw: exact int32 atomics
-
-// sampleDocument over the token range [b, e) (LDAModel.hpp:326-354).
-template <int KPL, int MODE>
-__device__ __forceinline__ void sample_range(Regs<KPL>& R, const SweepParams& p, int lane, uint64_t b, uint64_t e,
-                                             int nvalid, int lastLane, int lastSlot) {
-    constexpr int SH = (KPL == 1) ? 0 : (KPL == 2) ? 1 : (KPL == 4) ? 2 : (KPL == 8) ? 3 : (KPL == 16) ? 4 : 5;
-    int vcur = -1;
-    int32_t* colp = nullptr;
-    for (uint64_t t0 = b; t0 < e; t0 += 64) {
-        int n = (int)((e - t0) < 64 ? (e - t0) : 64);
-        uint64_t g = t0 + (uint64_t)lane;
-        int wv = -1, zv = 0xFFFF;
-        float uv = 0.0f;
-        if (lane < n) {
-            wv = (int)p.words[g];
-            zv = p.z[g];
-            uv = uniform24(philox_x0(p.seed_lo, p.seed_hi, (uint32_t)g, (uint32_t)(g >> 32), p.sweep, TAG_SWEEP));
-        }
-        int zkeep = zv;
-        for (int i = 0; i < n; ++i) {
-            int v = readlane_i(wv, i);
-            if ((unsigned)v >= (unsigned)p.V) continue;
-            int zo = readlane_i(zv, i);
-            float u = readlane_f(uv, i);
-            if (v != vcur) {
-                if constexpr (MODE == MODE_DET) __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
-                colp = p.nkw + (int64_t)v * p.Kp;
-                load_kvec<KPL>(colp, lane, R.col);
-                vcur = v;
-            }
-            int zn = token_step<KPL>(R, lane, zo, u, p.K, nvalid, lastLane, lastSlot, p.eta, p.veta, nullptr);
-            if (zn != zo) {
-                int lo = zo >> SH, so = zo & (KPL - 1);
-                int ln = zn >> SH, sn = zn & (KPL - 1);
-                if constexpr (MODE == MODE_DET) {
-                    int co = pick<KPL, int>(R.col, so), cn = pick<KPL, int>(R.col, sn);
-                    if (lane == lo) colp[kpos<KPL>(lane, so)] = co;
-                    if (lane == ln) colp[kpos<KPL>(lane, sn)] = cn;
-                } else {
-                    if (lane == lo) atomicAdd(colp + kpos<KPL>(lane, so), -1);
-                    if (lane == ln) atomicAdd(colp + kpos<KPL>(lane, sn), 1);
-                }
-                zkeep = (lane == i) ? zn : zkeep;
-            }
-        }
-        if (lane < n) p.z[g] = (uint16_t)zkeep;
-    }
+    for (int j = 0; j < KPL; ++j) y[j] = 1.0f / ((float)n[j] + veta);
+    lds_store<KPL>(S.nk, lane, n);
+    lds_store<KPL>(reinterpret_cast<int*>(S.y), lane, __builtin_bit_cast(ivec<KPL>, y));
 }
 
 template <int KPL>
@@ -287,37 +198,157 @@ __device__ __forceinline__ void lane_limits(int K, int lane, int& nvalid, int& l
     lastSlot = (K - 1) % KPL;
 }
 
+// One token: remove z_old, weight, scan, search, add z_new.  Returns z_new
+// (wave-uniform).  *total receives cdf[K-1].
+template <int KPL>
+__device__ __forceinline__ int token_step(const WaveLds& S, const fvec<KPL>& al, int lane, int zo, float u, int K,
+                                          int nvalid, int lastLane, int lastSlot, float eta, float veta, float* total) {
+    bump<KPL>(S, lane, zo, -1, veta);
+    ivec<KPL> ndk, nk, col, yb;
+    lds_load<KPL>(S.ndk, lane, ndk);
+    lds_load<KPL>(S.col, lane, col);
+    lds_load<KPL>(S.nk, lane, nk);
+    lds_load<KPL>(reinterpret_cast<const int*>(S.y), lane, yb);
+    const fvec<KPL> y = __builtin_bit_cast(fvec<KPL>, yb);
+    fvec<KPL> w;
+#pragma unroll
+    for (int j = 0; j < KPL; ++j) {
+        // ((n_dk + alpha) * (n_kw + eta)) / (n_k + V*eta); the quotient as
+        // q0 = a*y, r = fma(-q0, b, a), q = fma(r, y, q0) with y = RN(1/b)
+        float A = (float)ndk[j] + al[j];
+        float B = (float)col[j] + eta;
+        float a = A * B;
+        float bf = (float)nk[j] + veta;
+        float q0 = a * y[j];
+        float rr = __builtin_fmaf(-q0, bf, a);
+        w[j] = __builtin_fmaf(rr, y[j], q0);
+    }
+#pragma unroll
+    for (int j = 1; j < KPL; ++j) w[j] = w[j - 1] + w[j];
+    float incl = wave_inclusive_scan(w[KPL - 1]);
+    float excl = dpp_wave_shr1(incl);
+#pragma unroll
+    for (int j = 0; j < KPL; ++j) w[j] = excl + w[j];
+    float last = w[0];
+#pragma unroll
+    for (int j = 1; j < KPL; ++j) last = (j == lastSlot) ? w[j] : last;
+    const float tot = readlane_f(last, lastLane);
+    const float r = u * tot;
+    int miss = 0;
+#pragma unroll
+    for (int j = 0; j < KPL; ++j) miss += (j < nvalid && !(r < w[j])) ? 1 : 0;
+    const unsigned long long m = __ballot(miss < nvalid);
+    int zn;
+    if (m == 0ull) zn = K - 1;
+    else {
+        const int L = __builtin_ctzll(m);
+        zn = L * KPL + readlane_i(miss, L);
+    }
+    zn = uniform_i(zn);
+    bump<KPL>(S, lane, zn, +1, veta);
+    if (total) *total = tot;
+    return zn;
+}
+
+constexpr int MODE_DET = 0;     // exclusive cell owner: plain stores to n_kw
+constexpr int MODE_ATOMIC = 1;  // shared n_kw: exact int32 atomics
+
+// forShuffled's prime table (Utils.hpp:159-163).
+__constant__ uint64_t kShufflePrimes[16] = {65537, 65539, 65543, 65551, 65557, 65563, 65579, 65581,
+                                            65587, 65599, 65609, 65617, 65629, 65633, 65647, 65651};
+
+// sampleDocument over the token range [b, e) (LDAModel.hpp:326-354).  The
+// column of the next distinct word is fetched one token ahead.
+template <int KPL, int MODE>
+__device__ __forceinline__ void sample_range(const WaveLds& S, const fvec<KPL>& al, const SweepParams& p, int lane,
+                                             uint64_t b, uint64_t e, int nvalid, int lastLane, int lastSlot) {
+    constexpr int SH = kpl_shift<KPL>();
+    int vcur = -1;
+    int32_t* colp = nullptr;
+    for (uint64_t t0 = b; t0 < e; t0 += 64) {
+        const int n = (int)((e - t0) < 64 ? (e - t0) : 64);
+        const uint64_t g = t0 + (uint64_t)lane;
+        int wv = -1, zv = 0xFFFF;
+        float uv = 0.0f;
+        if (lane < n) {
+            wv = (int)p.words[g];
+            zv = p.z[g];
+            uv = uniform24(philox_x0(p.seed_lo, p.seed_hi, (uint32_t)g, (uint32_t)(g >> 32), p.sweep, TAG_SWEEP));
+        }
+        int zkeep = zv;
+        int vnext = -1;
+        ivec<KPL> ncol;
+        for (int i = 0; i < n; ++i) {
+            const int v = readlane_i(wv, i);
+            if ((unsigned)v >= (unsigned)p.V) continue;
+            if (v != vcur) {
+                colp = p.nkw + (int64_t)v * p.Kp;
+                if (v != vnext) load_kvec<KPL>(colp, lane, ncol);
+                lds_store<KPL>(S.col, lane, ncol);
+                vcur = v;
+            }
+            if (i + 1 < n) {
+                const int v2 = readlane_i(wv, i + 1);
+                if (v2 != v && (unsigned)v2 < (unsigned)p.V) {
+                    load_kvec<KPL>(p.nkw + (int64_t)v2 * p.Kp, lane, ncol);
+                    vnext = v2;
+                }
+            }
+            const int zo = readlane_i(zv, i);
+            const float u = readlane_f(uv, i);
+            const int zn = token_step<KPL>(S, al, lane, zo, u, p.K, nvalid, lastLane, lastSlot, p.eta, p.veta, nullptr);
+            if (zn != zo) {
+                const int lo = zo >> SH, ln = zn >> SH;
+                const int qo = kpos<KPL>(lane, zo & (KPL - 1)), qn = kpos<KPL>(lane, zn & (KPL - 1));
+                if constexpr (MODE == MODE_DET) {
+                    if (lane == lo) colp[qo] = S.col[qo];
+                    if (lane == ln) colp[qn] = S.col[qn];
+                } else {
+                    if (lane == lo) atomicAdd(colp + qo, -1);
+                    if (lane == ln) atomicAdd(colp + qn, 1);
+                }
+                zkeep = (lane == i) ? zn : zkeep;
+            }
+        }
+        if (lane < n) p.z[g] = (uint16_t)zkeep;
+    }
+}
+
 // Device-wide sweep: every resident wavefront is a worker with a private
 // n_k (snapshot nk_in at sweep start plus its own changes, as tomoto's
 // partition workers keep theirs, LDAModel.hpp:484, 596-602); documents are
 // taken longest-first from a global counter; n_kw is updated with atomics;
 // each worker adds its n_k delta to nk_out when it runs out of documents.
 template <int KPL>
-__global__ void __launch_bounds__(256) sweep_device_kernel(SweepParams p) {
+__global__ void __launch_bounds__(64) sweep_device_kernel(SweepParams p) {
     const int lane = lane_id();
-    Regs<KPL> R;
+    const WaveLds S = wave_lds<KPL>();
     int nvalid, lastLane, lastSlot;
     lane_limits<KPL>(p.K, lane, nvalid, lastLane, lastSlot);
-    load_kvec_f<KPL>(p.alpha, lane, R.al);
-    load_kvec<KPL>(p.nk_in, lane, R.nk);
-    set_nk<KPL>(R, p.veta);
+    fvec<KPL> al;
+    load_kvec_f<KPL>(p.alpha, lane, al);
+    set_nk<KPL>(S, p.nk_in, lane, p.veta);
     for (;;) {
         uint32_t idx = 0;
         if (lane == 0) idx = atomicAdd(p.counter, 1u);
         idx = (uint32_t)uniform_i((int)idx);
         if ((int64_t)idx >= p.D) break;
-        uint32_t d = p.doc_order[idx];
-        uint64_t b = p.off[d], e = p.off[d + 1];
+        const uint32_t d = p.doc_order[idx];
+        const uint64_t b = p.off[d], e = p.off[d + 1];
         int32_t* ndk_row = p.ndk + (int64_t)d * p.Kp;
-        set_doc<KPL>(R, ndk_row, lane);
-        sample_range<KPL, MODE_ATOMIC>(R, p, lane, b, e, nvalid, lastLane, lastSlot);
-        store_kvec<KPL>(ndk_row, lane, R.ndk);
+        ivec<KPL> nd;
+        load_kvec<KPL>(ndk_row, lane, nd);
+        lds_store<KPL>(S.ndk, lane, nd);
+        sample_range<KPL, MODE_ATOMIC>(S, al, p, lane, b, e, nvalid, lastLane, lastSlot);
+        lds_load<KPL>(S.ndk, lane, nd);
+        store_kvec<KPL>(ndk_row, lane, nd);
     }
-    int snap[KPL];
+    ivec<KPL> snap, mine;
     load_kvec<KPL>(p.nk_in, lane, snap);
+    lds_load<KPL>(S.nk, lane, mine);
 #pragma unroll
     for (int j = 0; j < KPL; ++j) {
-        int dlt = R.nk[j] - snap[j];
+        const int dlt = mine[j] - snap[j];
         if (dlt) atomicAdd(p.nk_out + kpos<KPL>(lane, j), dlt);
     }
 }
@@ -330,13 +361,13 @@ template <int KPL>
 __global__ void __launch_bounds__(64) sweep_det_kernel(SweepParams p) {
     const int lane = lane_id();
     const int w = blockIdx.x;
-    Regs<KPL> R;
+    const WaveLds S = wave_lds<KPL>();
     int nvalid, lastLane, lastSlot;
     lane_limits<KPL>(p.K, lane, nvalid, lastLane, lastSlot);
-    load_kvec_f<KPL>(p.alpha, lane, R.al);
+    fvec<KPL> al;
+    load_kvec_f<KPL>(p.alpha, lane, al);
     int32_t* nkp = p.nk_out + (int64_t)w * p.Kp;
-    load_kvec<KPL>(nkp, lane, R.nk);
-    set_nk<KPL>(R, p.veta);
+    set_nk<KPL>(S, nkp, lane, p.veta);
     const int W = p.W;
     const int didx = (p.round + w) % W;
     const uint64_t N = (uint64_t)((p.D + (W - 1) - didx) / W);
@@ -352,16 +383,23 @@ __global__ void __launch_bounds__(64) sweep_det_kernel(SweepParams p) {
         P %= N;
     }
     for (uint64_t i = 0; i < N; ++i) {
-        uint64_t id = ((i + (uint64_t)sseed) * P) % N;
-        int64_t d = (int64_t)id * W + didx;
-        uint64_t b = p.coff[(int64_t)w * p.D + d], e = p.coff[(int64_t)(w + 1) * p.D + d];
+        const uint64_t id = ((i + (uint64_t)sseed) * P) % N;
+        const int64_t d = (int64_t)id * W + didx;
+        const uint64_t b = p.coff[(int64_t)w * p.D + d], e = p.coff[(int64_t)(w + 1) * p.D + d];
         if (b == e) continue;
+        // earlier cells' n_kw stores of this worker complete before this cell reads columns
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
         int32_t* ndk_row = p.ndk + d * p.Kp;
-        set_doc<KPL>(R, ndk_row, lane);
-        sample_range<KPL, MODE_DET>(R, p, lane, b, e, nvalid, lastLane, lastSlot);
-        store_kvec<KPL>(ndk_row, lane, R.ndk);
+        ivec<KPL> nd;
+        load_kvec<KPL>(ndk_row, lane, nd);
+        lds_store<KPL>(S.ndk, lane, nd);
+        sample_range<KPL, MODE_DET>(S, al, p, lane, b, e, nvalid, lastLane, lastSlot);
+        lds_load<KPL>(S.ndk, lane, nd);
+        store_kvec<KPL>(ndk_row, lane, nd);
     }
-    store_kvec<KPL>(nkp, lane, R.nk);
+    ivec<KPL> mine;
+    lds_load<KPL>(S.nk, lane, mine);
+    store_kvec<KPL>(nkp, lane, mine);
 }
 
 // ---------------------------------------------------------------- init ----
@@ -591,17 +629,20 @@ __global__ void __launch_bounds__(64) sample_frozen_kernel(const int32_t* __rest
                                                            const uint16_t* __restrict__ z_old, const float* __restrict__ u,
                                                            uint16_t* z_new, float* total) {
     const int lane = lane_id();
-    Regs<KPL> R;
+    const WaveLds S = wave_lds<KPL>();
     int nvalid, lastLane, lastSlot;
     lane_limits<KPL>(K, lane, nvalid, lastLane, lastSlot);
-    load_kvec_f<KPL>(alpha, lane, R.al);
+    fvec<KPL> al;
+    load_kvec_f<KPL>(alpha, lane, al);
     for (int64_t t = blockIdx.x; t < n; t += gridDim.x) {
-        load_kvec<KPL>(nk, lane, R.nk);
-        set_nk<KPL>(R, veta);
-        set_doc<KPL>(R, ndk + doc[t] * Kp, lane);
-        load_kvec<KPL>(nkw + (int64_t)word[t] * Kp, lane, R.col);
+        set_nk<KPL>(S, nk, lane, veta);
+        ivec<KPL> x;
+        load_kvec<KPL>(ndk + doc[t] * Kp, lane, x);
+        lds_store<KPL>(S.ndk, lane, x);
+        load_kvec<KPL>(nkw + (int64_t)word[t] * Kp, lane, x);
+        lds_store<KPL>(S.col, lane, x);
         float tot;
-        int zn = token_step<KPL>(R, lane, (int)z_old[t], u[t], K, nvalid, lastLane, lastSlot, eta, veta, &tot);
+        const int zn = token_step<KPL>(S, al, lane, (int)z_old[t], u[t], K, nvalid, lastLane, lastSlot, eta, veta, &tot);
         if (lane == 0) {
             z_new[t] = (uint16_t)zn;
             if (total) total[t] = tot;
@@ -610,41 +651,41 @@ __global__ void __launch_bounds__(64) sample_frozen_kernel(const int32_t* __rest
 }
 
 // -------------------------------------------------------------- fold-in ----
-// Histogram of the current topics of the run of word v that starts at token
-// g (words are sorted, so the run is contiguous), added into R.col.
+// Adds the current topics of the run of word v starting at token g (words are
+// sorted, so the run is contiguous) into the LDS column: n_kw,eff = n_kw + c_d.
 template <int KPL>
-__device__ __forceinline__ void add_run_counts(Regs<KPL>& R, const InferParams& p, int lane, uint64_t g, uint64_t e,
+__device__ __forceinline__ void add_run_counts(const WaveLds& S, const InferParams& p, int lane, uint64_t g, uint64_t e,
                                                int v) {
-    constexpr int SH = (KPL == 1) ? 0 : (KPL == 2) ? 1 : (KPL == 4) ? 2 : (KPL == 8) ? 3 : (KPL == 16) ? 4 : 5;
+    constexpr int SH = kpl_shift<KPL>();
     for (uint64_t t0 = g; t0 < e; t0 += 64) {
-        uint64_t gg = t0 + (uint64_t)lane;
-        bool in = gg < e && (int)p.words[gg] == v;
-        unsigned long long m = __ballot(in);
-        int cnt = (m == ~0ull) ? 64 : __builtin_ctzll(~m);
-        int zz = in ? (int)p.z[gg] : 0;
+        const uint64_t gg = t0 + (uint64_t)lane;
+        const bool in = gg < e && (int)p.words[gg] == v;
+        const unsigned long long m = __ballot(in);
+        const int cnt = (m == ~0ull) ? 64 : __builtin_ctzll(~m);
+        const int zz = in ? (int)p.z[gg] : 0;
         for (int i = 0; i < cnt; ++i) {
-            int k = readlane_i(zz, i);
-            const int sl = k & (KPL - 1);
-            const int inc = (lane == (k >> SH)) ? 1 : 0;
-#pragma unroll
-            for (int j = 0; j < KPL; ++j) R.col[j] += (j == sl) ? inc : 0;
+            const int k = readlane_i(zz, i);
+            if (lane == (k >> SH)) S.col[kpos<KPL>(lane, k & (KPL - 1))] += 1;
         }
         if (cnt < 64) break;
     }
 }
 
-// Fold-in of one document per wavefront (LDAModel.hpp:920-953 with the
+// Fold-in of one document per wavefront (LDAModel.hpp:920-953) with the
 // private state kept as exact deltas: n_k,eff = n_k + n_dk and
-// n_kw,eff = n_kw + c_d, c_d rebuilt per word run from the document's z).
+// n_kw,eff = n_kw + c_d, c_d rebuilt per word run from the document's z.
+// Each document's stream depends only on its own tokens and `infer_seed`,
+// so its result does not depend on the batch it runs in.
 template <int KPL>
-__global__ void __launch_bounds__(256) infer_kernel(InferParams p) {
-    constexpr int SH = (KPL == 1) ? 0 : (KPL == 2) ? 1 : (KPL == 4) ? 2 : (KPL == 8) ? 3 : (KPL == 16) ? 4 : 5;
+__global__ void __launch_bounds__(64) infer_kernel(InferParams p) {
+    constexpr int SH = kpl_shift<KPL>();
     const int lane = lane_id();
-    Regs<KPL> R;
+    const WaveLds S = wave_lds<KPL>();
     int nvalid, lastLane, lastSlot;
     lane_limits<KPL>(p.K, lane, nvalid, lastLane, lastSlot);
-    load_kvec_f<KPL>(p.alpha, lane, R.al);
-    int nk0[KPL];
+    fvec<KPL> al;
+    load_kvec_f<KPL>(p.alpha, lane, al);
+    ivec<KPL> nk0;
     load_kvec<KPL>(p.nk, lane, nk0);
     for (;;) {
         uint32_t idx;
@@ -653,53 +694,55 @@ __global__ void __launch_bounds__(256) infer_kernel(InferParams p) {
             if (lane == 0) idx = atomicAdd(p.counter, 1u);
             idx = (uint32_t)uniform_i((int)idx);
         } else {
-            idx = (uint32_t)(p.doc_base + (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)));
+            idx = (uint32_t)(p.doc_base + blockIdx.x);
         }
         if ((int64_t)idx >= p.D) break;
-        uint32_t d = p.doc_order ? p.doc_order[idx] : idx;
+        const uint32_t d = p.doc_order ? p.doc_order[idx] : idx;
         const uint64_t b = p.off[d], e = p.off[d + 1];
-        // initial draw and n_dk
-#pragma unroll
-        for (int j = 0; j < KPL; ++j) R.ndk[j] = 0;
+        // initial draw z ~ Discrete(alpha) and n_dk
+        ivec<KPL> zero = 0;
+        lds_store<KPL>(S.ndk, lane, zero);
         for (uint64_t t0 = b; t0 < e; t0 += 64) {
-            uint64_t g = t0 + (uint64_t)lane;
+            const uint64_t g = t0 + (uint64_t)lane;
             int zz = 0xFFFF;
             if (g < e && p.words[g] < (uint32_t)p.V) {
-                float u = uniform24(philox_x0(p.seed_lo, p.seed_hi, (uint32_t)(g - b), (uint32_t)((g - b) >> 32), 0u, TAG_INFINIT));
-                float r = u * p.alpha_cdf[p.K - 1];
+                const float u = uniform24(philox_x0(p.seed_lo, p.seed_hi, (uint32_t)(g - b), (uint32_t)((g - b) >> 32), 0u, TAG_INFINIT));
+                const float r = u * p.alpha_cdf[p.K - 1];
                 int lo = 0, hi = p.K;
                 while (lo < hi) {
-                    int mid = (lo + hi) >> 1;
+                    const int mid = (lo + hi) >> 1;
                     if (r < p.alpha_cdf[mid]) hi = mid; else lo = mid + 1;
                 }
                 zz = lo < p.K ? lo : p.K - 1;
             }
             if (g < e) p.z[g] = (uint16_t)zz;
-            int n = (int)((e - t0) < 64 ? (e - t0) : 64);
+            const int n = (int)((e - t0) < 64 ? (e - t0) : 64);
             for (int i = 0; i < n; ++i) {
-                int k = readlane_i(zz, i);
+                const int k = readlane_i(zz, i);
                 if (k == 0xFFFF) continue;
-                const int sl = k & (KPL - 1);
-                const int inc = (lane == (k >> SH)) ? 1 : 0;
-#pragma unroll
-                for (int j = 0; j < KPL; ++j) R.ndk[j] += (j == sl) ? inc : 0;
+                if (lane == (k >> SH)) S.ndk[kpos<KPL>(lane, k & (KPL - 1))] += 1;
             }
         }
+        ivec<KPL> nd;
+        lds_load<KPL>(S.ndk, lane, nd);
+        ivec<KPL> nke = nk0 + nd;
         int L = 0;
 #pragma unroll
-        for (int j = 0; j < KPL; ++j) {
-            R.A[j] = (float)R.ndk[j] + R.al[j];
-            R.nk[j] = nk0[j] + R.ndk[j];
-            L += R.ndk[j];
-        }
+        for (int j = 0; j < KPL; ++j) L += nd[j];
         for (int o = 32; o >= 1; o >>= 1) L += __shfl_xor(L, o);
-        set_nk<KPL>(R, p.veta);
+        {
+            fvec<KPL> y;
+#pragma unroll
+            for (int j = 0; j < KPL; ++j) y[j] = 1.0f / ((float)nke[j] + p.veta);
+            lds_store<KPL>(S.nk, lane, nke);
+            lds_store<KPL>(reinterpret_cast<int*>(S.y), lane, __builtin_bit_cast(ivec<KPL>, y));
+        }
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
         for (int it = 1; it <= p.iterations; ++it) {
             int vcur = -1;
             for (uint64_t t0 = b; t0 < e; t0 += 64) {
-                int n = (int)((e - t0) < 64 ? (e - t0) : 64);
-                uint64_t g = t0 + (uint64_t)lane;
+                const int n = (int)((e - t0) < 64 ? (e - t0) : 64);
+                const uint64_t g = t0 + (uint64_t)lane;
                 int wv = -1, zv = 0xFFFF;
                 float uv = 0.0f;
                 if (lane < n) {
@@ -709,35 +752,31 @@ __global__ void __launch_bounds__(256) infer_kernel(InferParams p) {
                 }
                 int zkeep = zv;
                 for (int i = 0; i < n; ++i) {
-                    int v = readlane_i(wv, i);
+                    const int v = readlane_i(wv, i);
                     if ((unsigned)v >= (unsigned)p.V) continue;
-                    int zo = readlane_i(zv, i);
-                    float u = readlane_f(uv, i);
                     if (v != vcur) {
-                        load_kvec<KPL>(p.nkw + (int64_t)v * p.Kp, lane, R.col);
-                        // the run's topics are read back from memory: make this
-                        // wavefront's earlier z stores visible to its loads
-                        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
-                        add_run_counts<KPL>(R, p, lane, t0 + (uint64_t)i, e, v);
+                        ivec<KPL> c;
+                        load_kvec<KPL>(p.nkw + (int64_t)v * p.Kp, lane, c);
+                        lds_store<KPL>(S.col, lane, c);
+                        add_run_counts<KPL>(S, p, lane, t0 + (uint64_t)i, e, v);
                         vcur = v;
                     }
-                    int zn = token_step<KPL>(R, lane, zo, u, p.K, nvalid, lastLane, lastSlot, p.eta, p.veta, nullptr);
-                    if (zn != zo) {
-                        zkeep = (lane == i) ? zn : zkeep;
-                        // keep memory current for the next run's histogram
-                        if (lane == i) p.z[g] = (uint16_t)zn;
-                    }
+                    const int zo = readlane_i(zv, i);
+                    const float u = readlane_f(uv, i);
+                    const int zn = token_step<KPL>(S, al, lane, zo, u, p.K, nvalid, lastLane, lastSlot, p.eta, p.veta, nullptr);
+                    zkeep = (lane == i) ? zn : zkeep;
                 }
                 if (lane < n) p.z[g] = (uint16_t)zkeep;
             }
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
         }
+        lds_load<KPL>(S.ndk, lane, nd);
         // theta (LDAModel.hpp:1145-1159)
         const float denom = (float)L + p.asum_f;
 #pragma unroll
         for (int j = 0; j < KPL; ++j) {
-            int k = lane * KPL + j;
-            if (k < p.K) p.theta[(int64_t)d * p.K + k] = R.A[j] / denom;
+            const int k = lane * KPL + j;
+            if (k < p.K) p.theta[(int64_t)d * p.K + k] = ((float)nd[j] + al[j]) / denom;
         }
         // FP64 log-likelihood: getLLRest(private) - getLLRest(global) + getLLDocs(doc)
         const double eta = (double)p.eta;
@@ -746,31 +785,30 @@ __global__ void __launch_bounds__(256) infer_kernel(InferParams p) {
         double ll = 0.0;
 #pragma unroll
         for (int j = 0; j < KPL; ++j) {
-            if (R.ndk[j]) {
-                ll += lgamma((double)nk0[j] + veta) - lgamma((double)(nk0[j] + R.ndk[j]) + veta);
-                double a = (double)R.al[j];
-                ll += lgamma((double)R.ndk[j] + a) - lgamma(a);
+            if (nd[j]) {
+                ll += lgamma((double)nk0[j] + veta) - lgamma((double)(nk0[j] + nd[j]) + veta);
+                const double a = (double)al[j];
+                ll += lgamma((double)nd[j] + a) - lgamma(a);
             }
         }
         for (uint64_t t0 = b; t0 < e; t0 += 64) {
-            uint64_t g = t0 + (uint64_t)lane;
-            int wv = g < e ? (int)p.words[g] : -1;
-            int wp = (g < e && g > b) ? (int)p.words[g - 1] : -2;
+            const uint64_t g = t0 + (uint64_t)lane;
+            const int wv = g < e ? (int)p.words[g] : -1;
+            const int wp = (g < e && g > b) ? (int)p.words[g - 1] : -2;
             unsigned long long starts = __ballot(g < e && (unsigned)wv < (unsigned)p.V && wv != wp);
             while (starts) {
-                int i = __builtin_ctzll(starts);
+                const int i = __builtin_ctzll(starts);
                 starts &= starts - 1;
-                int v = readlane_i(wv, i);
-                int base[KPL];
+                const int v = readlane_i(wv, i);
+                ivec<KPL> base, c;
                 load_kvec<KPL>(p.nkw + (int64_t)v * p.Kp, lane, base);
-#pragma unroll
-                for (int j = 0; j < KPL; ++j) R.col[j] = base[j];
-                add_run_counts<KPL>(R, p, lane, t0 + (uint64_t)i, e, v);
+                lds_store<KPL>(S.col, lane, base);
+                add_run_counts<KPL>(S, p, lane, t0 + (uint64_t)i, e, v);
+                lds_load<KPL>(S.col, lane, c);
 #pragma unroll
                 for (int j = 0; j < KPL; ++j) {
-                    int c = R.col[j] - base[j];
-                    if (c) {
-                        ll += lgamma((double)R.col[j] + eta) - lge;
+                    if (c[j] != base[j]) {
+                        ll += lgamma((double)c[j] + eta) - lge;
                         if (base[j]) ll -= lgamma((double)base[j] + eta) - lge;
                     }
                 }
@@ -820,12 +858,12 @@ hipError_t launch_build_counts(int kpl, const uint32_t* words, const uint16_t* z
 }
 
 hipError_t launch_sweep_device(int kpl, const SweepParams& p, int grid, hipStream_t s) {
-    LDA_DISPATCH_KPL(kpl, hipLaunchKernelGGL(sweep_device_kernel<KPL>, dim3(grid), dim3(256), 0, s, p));
+    LDA_DISPATCH_KPL(kpl, hipLaunchKernelGGL(sweep_device_kernel<KPL>, dim3(grid), dim3(64), wave_lds_bytes<KPL>(), s, p));
     return hipGetLastError();
 }
 
 hipError_t launch_sweep_det(int kpl, const SweepParams& p, hipStream_t s) {
-    LDA_DISPATCH_KPL(kpl, hipLaunchKernelGGL(sweep_det_kernel<KPL>, dim3(p.W), dim3(64), 0, s, p));
+    LDA_DISPATCH_KPL(kpl, hipLaunchKernelGGL(sweep_det_kernel<KPL>, dim3(p.W), dim3(64), wave_lds_bytes<KPL>(), s, p));
     return hipGetLastError();
 }
 
@@ -833,12 +871,12 @@ int sweep_device_blocks_per_cu(int kpl) {
     int n = 0;
     hipError_t e = hipErrorInvalidValue;
     switch (kpl) {
-        case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sweep_device_kernel<1>, 256, 0); break;
-        case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sweep_device_kernel<2>, 256, 0); break;
-        case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sweep_device_kernel<4>, 256, 0); break;
-        case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sweep_device_kernel<8>, 256, 0); break;
-        case 16: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sweep_device_kernel<16>, 256, 0); break;
-        case 32: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sweep_device_kernel<32>, 256, 0); break;
+        case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sweep_device_kernel<1>, 64, wave_lds_bytes<1>()); break;
+        case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sweep_device_kernel<2>, 64, wave_lds_bytes<2>()); break;
+        case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sweep_device_kernel<4>, 64, wave_lds_bytes<4>()); break;
+        case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sweep_device_kernel<8>, 64, wave_lds_bytes<8>()); break;
+        case 16: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sweep_device_kernel<16>, 64, wave_lds_bytes<16>()); break;
+        case 32: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sweep_device_kernel<32>, 64, wave_lds_bytes<32>()); break;
         default: break;
     }
     return (e == hipSuccess && n > 0) ? n : 1;
@@ -848,12 +886,12 @@ int infer_blocks_per_cu(int kpl) {
     int n = 0;
     hipError_t e = hipErrorInvalidValue;
     switch (kpl) {
-        case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_kernel<1>, 256, 0); break;
-        case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_kernel<2>, 256, 0); break;
-        case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_kernel<4>, 256, 0); break;
-        case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_kernel<8>, 256, 0); break;
-        case 16: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_kernel<16>, 256, 0); break;
-        case 32: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_kernel<32>, 256, 0); break;
+        case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_kernel<1>, 64, wave_lds_bytes<1>()); break;
+        case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_kernel<2>, 64, wave_lds_bytes<2>()); break;
+        case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_kernel<4>, 64, wave_lds_bytes<4>()); break;
+        case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_kernel<8>, 64, wave_lds_bytes<8>()); break;
+        case 16: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_kernel<16>, 64, wave_lds_bytes<16>()); break;
+        case 32: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_kernel<32>, 64, wave_lds_bytes<32>()); break;
         default: break;
     }
     return (e == hipSuccess && n > 0) ? n : 1;
@@ -926,13 +964,13 @@ hipError_t launch_sample_frozen(int kpl, const int32_t* ndk, const int32_t* nkw,
     (void)V;
     if (n <= 0) return hipSuccess;
     int64_t grid = n < 65536 ? n : 65536;
-    LDA_DISPATCH_KPL(kpl, hipLaunchKernelGGL(sample_frozen_kernel<KPL>, dim3((unsigned)grid), dim3(64), 0, s, ndk, nkw, nk,
+    LDA_DISPATCH_KPL(kpl, hipLaunchKernelGGL(sample_frozen_kernel<KPL>, dim3((unsigned)grid), dim3(64), wave_lds_bytes<KPL>(), s, ndk, nkw, nk,
                                              alpha, eta, veta, K, Kp, n, doc, word, z_old, u, z_new, total));
     return hipGetLastError();
 }
 
 hipError_t launch_infer(int kpl, const InferParams& p, int grid, hipStream_t s) {
-    LDA_DISPATCH_KPL(kpl, hipLaunchKernelGGL(infer_kernel<KPL>, dim3(grid), dim3(256), 0, s, p));
+    LDA_DISPATCH_KPL(kpl, hipLaunchKernelGGL(infer_kernel<KPL>, dim3(grid), dim3(64), wave_lds_bytes<KPL>(), s, p));
     return hipGetLastError();
 }
 
