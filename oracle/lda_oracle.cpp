@@ -521,7 +521,7 @@ void orc_sample_frozen(int K, int scan_order, int kpl, const float* alpha, float
         int zo = z_old[t];
         a[zo] -= 1; c[zo] -= 1; e[zo] -= 1;
         z_new[t] = (uint16_t)sample_one(K, scan_order, kpl, a.data(), alpha, c.data(), eta, e.data(), veta, u[t], buf.data());
-        if (total_out) total_out[t] = buf[(scan_order ? 64 * kpl : K) - 1];
+        if (total_out) total_out[t] = buf[K - 1];
     }
 }
 

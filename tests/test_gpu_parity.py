@@ -140,3 +140,22 @@ def test_infer_batched_plain_and_oracle_agree():
     assert np.array_equal(th_b, th_o)
     np.testing.assert_allclose(ll_b, ll_o, rtol=1e-9)
     np.testing.assert_array_equal(ll_b, ll_p)
+
+
+def test_infer_graph_replay_and_batch_independence():
+    """The batched launch is a replayed hipGraph: replays with other batches give
+    each document the same result as when it runs alone."""
+    ws, offs, V = _corpus("plain_k20", 100)
+    K = 20
+    ctx = _native.Context(K, V, 0.1, 0.01, seed=8)
+    ctx.upload_corpus(ws, offs)
+    ctx.train(5, workers=0)
+    uw, uoff = synth.unseen_docs(91, 40, V, 20, np.log(80.0), 1.0, 2, 900, zipf_s=None)
+    uw, uoff = corpus.csr_from_arrays(uw, uoff, np.arange(V))
+    th_all, ll_all, _ = ctx.infer(uw, uoff, 15)
+    th_again, ll_again, _ = ctx.infer(uw, uoff, 15)
+    assert np.array_equal(th_all, th_again) and np.array_equal(ll_all, ll_again)
+    for d in (0, 7, 39):
+        b, e = int(uoff[d]), int(uoff[d + 1])
+        th1, ll1, _ = ctx.infer(uw[b:e], np.array([0, e - b], np.uint64), 15)
+        assert np.array_equal(th1[0], th_all[d]) and ll1[0] == ll_all[d]

@@ -160,11 +160,15 @@ struct lda_ctx {
     int part_W = 0;
 
     // fold-in buffers
-    DevBuf<uint32_t> i_words, i_order, i_counter;
-    DevBuf<uint64_t> i_off;
+    DevBuf<uint32_t> i_words, i_counter;
+    DevBuf<lda::InferJob> i_jobs;
+    DevBuf<lda::InferParams> i_params;
     DevBuf<uint16_t> i_z;
     DevBuf<float> i_theta;
     DevBuf<double> i_ll;
+    // batched fold-in graph: counter reset + persistent kernel reading i_params
+    hipGraphExec_t infer_exec = nullptr;
+    int infer_exec_grid = 0;
 
     // kernel timing of the last call
     double stat_ms = 0.0;
@@ -428,7 +432,8 @@ int lda_ctx_destroy(lda_ctx* c) {
     c->d_alpha.release(); c->d_alpha_cdf.release(); c->d_nkw.release(); c->d_nk.release(); c->d_nk_next.release();
     c->d_ndk.release(); c->d_doclen.release(); c->d_nk_priv.release(); c->d_words.release(); c->d_order.release();
     c->d_counter.release(); c->d_z.release(); c->d_off.release(); c->d_coff.release(); c->d_part.release();
-    c->i_words.release(); c->i_order.release(); c->i_counter.release(); c->i_off.release(); c->i_z.release();
+    if (c->infer_exec) (void)hipGraphExecDestroy(c->infer_exec);
+    c->i_words.release(); c->i_counter.release(); c->i_jobs.release(); c->i_params.release(); c->i_z.release();
     c->i_theta.release(); c->i_ll.release();
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -742,6 +747,7 @@ int lda_infer(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off, int64_
     if (iterations < 0) return fail(LDA_ERR_INVALID_ARGUMENT, "iterations must be >= 0");
     if (mode != 0 && mode != 1) return fail(LDA_ERR_INVALID_ARGUMENT, "mode must be 0 or 1");
     if (!D) return LDA_OK;
+    if (D > 0x7FFFFFFFll) return fail(LDA_ERR_INVALID_ARGUMENT, "too many documents");
     if (doc_off[0] != 0) return fail(LDA_ERR_INVALID_ARGUMENT, "doc_off[0] must be 0");
     for (int64_t d = 0; d < D; ++d) {
         if (doc_off[d + 1] < doc_off[d]) return fail(LDA_ERR_INVALID_ARGUMENT, "doc_off must be non-decreasing");
@@ -753,49 +759,68 @@ int lda_infer(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off, int64_
     const hipStream_t s = c->stream;
     HIP_TRY(c->i_words.alloc(N));
     HIP_TRY(c->i_z.alloc(N));
-    HIP_TRY(c->i_off.alloc(D + 1));
-    HIP_TRY(c->i_order.alloc(D));
+    HIP_TRY(c->i_jobs.alloc(D));
+    HIP_TRY(c->i_params.alloc(1));
     HIP_TRY(c->i_counter.alloc(1));
     HIP_TRY(c->i_theta.alloc((size_t)D * c->K));
     HIP_TRY(c->i_ll.alloc(D));
     if (N) HIP_TRY(hipMemcpyAsync(c->i_words.p, words, N * 4, hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(c->i_off.p, doc_off, (D + 1) * 8, hipMemcpyHostToDevice, s));
+    // jobs longest first (the persistent kernel takes them in this order)
     std::vector<uint32_t> order(D);
     std::iota(order.begin(), order.end(), 0u);
     std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
         return (doc_off[a + 1] - doc_off[a]) > (doc_off[b + 1] - doc_off[b]);
     });
-    HIP_TRY(hipMemcpyAsync(c->i_order.p, order.data(), D * 4, hipMemcpyHostToDevice, s));
+    std::vector<lda::InferJob> jobs(D);
+    for (int64_t i = 0; i < D; ++i) {
+        const uint32_t d = mode == 0 ? order[i] : (uint32_t)i;
+        jobs[i] = lda::InferJob{doc_off[d], doc_off[d + 1], 0, iterations, iterations, (int32_t)d};
+    }
+    HIP_TRY(hipMemcpyAsync(c->i_jobs.p, jobs.data(), D * sizeof(lda::InferJob), hipMemcpyHostToDevice, s));
     lda::InferParams p{};
-    p.words = c->i_words.p; p.z = c->i_z.p; p.off = c->i_off.p; p.nkw = c->d_nkw.p; p.nk = c->d_nk.p;
-    p.alpha = c->d_alpha.p; p.alpha_cdf = c->d_alpha_cdf.p; p.eta = c->eta; p.veta = c->veta;
-    p.K = c->K; p.V = c->V; p.Kp = c->Kp;
+    p.words = c->i_words.p; p.z = c->i_z.p; p.jobs = c->i_jobs.p; p.njobs = D;
+    p.nkw = c->d_nkw.p; p.nk = c->d_nk.p; p.alpha = c->d_alpha.p; p.alpha_cdf = c->d_alpha_cdf.p;
+    p.eta = c->eta; p.veta = c->veta; p.K = c->K; p.V = c->V; p.Kp = c->Kp;
     p.seed_lo = (uint32_t)infer_seed; p.seed_hi = (uint32_t)(infer_seed >> 32);
-    p.iterations = iterations; p.D = D; p.theta = c->i_theta.p; p.ll = c->i_ll.p;
+    p.theta = c->i_theta.p; p.ll = c->i_ll.p;
     double asd = 0.0;
     float asf = 0.0f;
     for (int k = 0; k < c->K; ++k) { asd += (double)c->alpha[k]; asf = asf + c->alpha[k]; }
     p.asum_d = asd; p.asum_f = asf;
     c->stat_ms = c->stat_launches = c->stat_tokens = 0.0;
     if (mode == 0) {
-        p.doc_order = c->i_order.p;
         p.counter = c->i_counter.p;
-        HIP_TRY(lda::launch_fill_u32(c->i_counter.p, 0u, 1, s));
-        int grid = c->num_cu * lda::infer_blocks_per_cu(c->kpl);
-        if (grid > D) grid = (int)D;
+        p.job_base = 0;
+        HIP_TRY(hipMemcpyAsync(c->i_params.p, &p, sizeof p, hipMemcpyHostToDevice, s));
+        const int grid = c->num_cu * lda::infer_blocks_per_cu(c->kpl);
+        if (!c->infer_exec || c->infer_exec_grid != grid) {
+            if (c->infer_exec) { (void)hipGraphExecDestroy(c->infer_exec); c->infer_exec = nullptr; }
+            hipGraph_t g = nullptr;
+            HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            hipError_t e1 = lda::launch_fill_u32(c->i_counter.p, 0u, 1, s);
+            hipError_t e2 = e1 == hipSuccess ? lda::launch_infer(c->kpl, c->i_params.p, grid, s) : e1;
+            hipError_t e3 = hipStreamEndCapture(s, &g);
+            if (e2 != hipSuccess || e3 != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                return fail(LDA_ERR_DEVICE, "fold-in graph capture failed: %s", hipGetErrorString(e2 != hipSuccess ? e2 : e3));
+            }
+            hipError_t e4 = hipGraphInstantiate(&c->infer_exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e4 != hipSuccess) return fail(LDA_ERR_DEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e4));
+            c->infer_exec_grid = grid;
+        }
         HIP_TRY(hipEventRecord(c->ev0, s));
-        HIP_TRY(lda::launch_infer(c->kpl, p, grid, s));
+        HIP_TRY(hipGraphLaunch(c->infer_exec, s));
         HIP_TRY(hipEventRecord(c->ev1, s));
         c->stat_launches = 1;
     } else {
-        p.doc_order = nullptr;
+        // plain: one launch and one synchronisation per document
         p.counter = nullptr;
         HIP_TRY(hipEventRecord(c->ev0, s));
         for (int64_t d = 0; d < D; ++d) {
-            p.doc_base = d;
-            lda::InferParams q = p;
-            q.D = d + 1;
-            HIP_TRY(lda::launch_infer(c->kpl, q, 1, s));
+            p.job_base = d;
+            HIP_TRY(hipMemcpyAsync(c->i_params.p, &p, sizeof p, hipMemcpyHostToDevice, s));
+            HIP_TRY(lda::launch_infer(c->kpl, c->i_params.p, 1, s));
             HIP_TRY(hipStreamSynchronize(s));
         }
         HIP_TRY(hipEventRecord(c->ev1, s));

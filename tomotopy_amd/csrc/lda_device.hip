@@ -674,11 +674,13 @@ __device__ __forceinline__ void add_run_counts(const WaveLds& S, const InferPara
 // Fold-in of one document per wavefront (LDAModel.hpp:920-953) with the
 // private state kept as exact deltas: n_k,eff = n_k + n_dk and
 // n_kw,eff = n_kw + c_d, c_d rebuilt per word run from the document's z.
-// Each document's stream depends only on its own tokens and `infer_seed`,
-// so its result does not depend on the batch it runs in.
+// Every draw is keyed by (infer seed, token position in the document,
+// iteration), so a document's result depends neither on the batch it runs in
+// nor on how its iterations are split across launches.
 template <int KPL>
-__global__ void __launch_bounds__(64) infer_kernel(InferParams p) {
+__global__ void __launch_bounds__(64) infer_kernel(const InferParams* __restrict__ pp) {
     constexpr int SH = kpl_shift<KPL>();
+    const InferParams p = *pp;
     const int lane = lane_id();
     const WaveLds S = wave_lds<KPL>();
     int nvalid, lastLane, lastSlot;
@@ -688,34 +690,40 @@ __global__ void __launch_bounds__(64) infer_kernel(InferParams p) {
     ivec<KPL> nk0;
     load_kvec<KPL>(p.nk, lane, nk0);
     for (;;) {
-        uint32_t idx;
+        int64_t idx;
         if (p.counter) {
-            idx = 0;
-            if (lane == 0) idx = atomicAdd(p.counter, 1u);
-            idx = (uint32_t)uniform_i((int)idx);
+            uint32_t t = 0;
+            if (lane == 0) t = atomicAdd(p.counter, 1u);
+            idx = (int64_t)(uint32_t)uniform_i((int)t);
         } else {
-            idx = (uint32_t)(p.doc_base + blockIdx.x);
+            idx = p.job_base + blockIdx.x;
         }
-        if ((int64_t)idx >= p.D) break;
-        const uint32_t d = p.doc_order ? p.doc_order[idx] : idx;
-        const uint64_t b = p.off[d], e = p.off[d + 1];
-        // initial draw z ~ Discrete(alpha) and n_dk
+        if (idx >= p.njobs) break;
+        const InferJob J = p.jobs[idx];
+        const uint64_t b = J.b, e = J.e;
+        // topics at the start of this launch: initial draw or the stored z; n_dk from them
         ivec<KPL> zero = 0;
         lds_store<KPL>(S.ndk, lane, zero);
         for (uint64_t t0 = b; t0 < e; t0 += 64) {
             const uint64_t g = t0 + (uint64_t)lane;
             int zz = 0xFFFF;
-            if (g < e && p.words[g] < (uint32_t)p.V) {
-                const float u = uniform24(philox_x0(p.seed_lo, p.seed_hi, (uint32_t)(g - b), (uint32_t)((g - b) >> 32), 0u, TAG_INFINIT));
-                const float r = u * p.alpha_cdf[p.K - 1];
-                int lo = 0, hi = p.K;
-                while (lo < hi) {
-                    const int mid = (lo + hi) >> 1;
-                    if (r < p.alpha_cdf[mid]) hi = mid; else lo = mid + 1;
+            if (g < e) {
+                if (J.it_begin == 0) {
+                    if (p.words[g] < (uint32_t)p.V) {
+                        const float u = uniform24(philox_x0(p.seed_lo, p.seed_hi, (uint32_t)(g - b), (uint32_t)((g - b) >> 32), 0u, TAG_INFINIT));
+                        const float r = u * p.alpha_cdf[p.K - 1];
+                        int lo = 0, hi = p.K;
+                        while (lo < hi) {
+                            const int mid = (lo + hi) >> 1;
+                            if (r < p.alpha_cdf[mid]) hi = mid; else lo = mid + 1;
+                        }
+                        zz = lo < p.K ? lo : p.K - 1;
+                    }
+                    p.z[g] = (uint16_t)zz;
+                } else {
+                    zz = p.z[g];
                 }
-                zz = lo < p.K ? lo : p.K - 1;
             }
-            if (g < e) p.z[g] = (uint16_t)zz;
             const int n = (int)((e - t0) < 64 ? (e - t0) : 64);
             for (int i = 0; i < n; ++i) {
                 const int k = readlane_i(zz, i);
@@ -725,12 +733,8 @@ __global__ void __launch_bounds__(64) infer_kernel(InferParams p) {
         }
         ivec<KPL> nd;
         lds_load<KPL>(S.ndk, lane, nd);
-        ivec<KPL> nke = nk0 + nd;
-        int L = 0;
-#pragma unroll
-        for (int j = 0; j < KPL; ++j) L += nd[j];
-        for (int o = 32; o >= 1; o >>= 1) L += __shfl_xor(L, o);
         {
+            const ivec<KPL> nke = nk0 + nd;
             fvec<KPL> y;
 #pragma unroll
             for (int j = 0; j < KPL; ++j) y[j] = 1.0f / ((float)nke[j] + p.veta);
@@ -738,7 +742,7 @@ __global__ void __launch_bounds__(64) infer_kernel(InferParams p) {
             lds_store<KPL>(reinterpret_cast<int*>(S.y), lane, __builtin_bit_cast(ivec<KPL>, y));
         }
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
-        for (int it = 1; it <= p.iterations; ++it) {
+        for (int it = J.it_begin + 1; it <= J.it_end; ++it) {
             int vcur = -1;
             for (uint64_t t0 = b; t0 < e; t0 += 64) {
                 const int n = (int)((e - t0) < 64 ? (e - t0) : 64);
@@ -770,13 +774,21 @@ __global__ void __launch_bounds__(64) infer_kernel(InferParams p) {
             }
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
         }
+        if (J.it_end < J.iters) {
+            if (!p.counter) break;
+            continue;
+        }
         lds_load<KPL>(S.ndk, lane, nd);
+        int L = 0;
+#pragma unroll
+        for (int j = 0; j < KPL; ++j) L += nd[j];
+        for (int o = 32; o >= 1; o >>= 1) L += __shfl_xor(L, o);
         // theta (LDAModel.hpp:1145-1159)
         const float denom = (float)L + p.asum_f;
 #pragma unroll
         for (int j = 0; j < KPL; ++j) {
             const int k = lane * KPL + j;
-            if (k < p.K) p.theta[(int64_t)d * p.K + k] = ((float)nd[j] + al[j]) / denom;
+            if (k < p.K) p.theta[(int64_t)J.slot * p.K + k] = ((float)nd[j] + al[j]) / denom;
         }
         // FP64 log-likelihood: getLLRest(private) - getLLRest(global) + getLLDocs(doc)
         const double eta = (double)p.eta;
@@ -818,7 +830,7 @@ __global__ void __launch_bounds__(64) infer_kernel(InferParams p) {
         if (lane == 0) {
             const double asum = p.asum_d;
             ll -= lgamma((double)L + asum) - lgamma(asum);
-            p.ll[d] = ll;
+            p.ll[J.slot] = ll;
         }
         if (!p.counter) break;
     }
@@ -969,8 +981,8 @@ hipError_t launch_sample_frozen(int kpl, const int32_t* ndk, const int32_t* nkw,
     return hipGetLastError();
 }
 
-hipError_t launch_infer(int kpl, const InferParams& p, int grid, hipStream_t s) {
-    LDA_DISPATCH_KPL(kpl, hipLaunchKernelGGL(infer_kernel<KPL>, dim3(grid), dim3(64), wave_lds_bytes<KPL>(), s, p));
+hipError_t launch_infer(int kpl, const InferParams* p_dev, int grid, hipStream_t s) {
+    LDA_DISPATCH_KPL(kpl, hipLaunchKernelGGL(infer_kernel<KPL>, dim3(grid), dim3(64), wave_lds_bytes<KPL>(), s, p_dev));
     return hipGetLastError();
 }
 

@@ -43,10 +43,22 @@ struct SweepParams {
     int W, round;
 };
 
+// One fold-in job: a document's tokens [b, e) in the token arena, the
+// iterations (it_begin, it_end] to run in this launch out of `iters`, and the
+// output slot of theta / ll.  it_begin == 0 draws the initial topics;
+// it_end == iters writes theta and the log-likelihood.
+struct InferJob {
+    uint64_t b, e;
+    int32_t it_begin, it_end, iters, slot;
+};
+
 struct InferParams {
-    const uint32_t* words;   // batch CSR
-    uint16_t* z;             // batch tokens
-    const uint64_t* off;     // D+1
+    const uint32_t* words;   // token arena
+    uint16_t* z;             // token arena
+    const InferJob* jobs;
+    int64_t njobs;
+    uint32_t* counter;       // next job; NULL: job = job_base + blockIdx.x, one per wavefront
+    int64_t job_base;
     const int32_t* nkw;      // frozen V x Kp
     const int32_t* nk;       // frozen Kp
     const float* alpha;      // Kp
@@ -54,15 +66,10 @@ struct InferParams {
     float eta, veta;
     int K, V, Kp;
     uint32_t seed_lo, seed_hi;
-    int iterations;
-    int64_t D;
-    const uint32_t* doc_order;
-    uint32_t* counter;
-    float* theta;            // D x K natural order
-    double* ll;              // D
+    float* theta;            // slots x K natural order
+    double* ll;              // slots
     double asum_d;           // sum of alpha in double
     float asum_f;            // sequential float sum of alpha
-    int64_t doc_base;        // plain mode: index of the single document
 };
 
 // kernels (lda_device.hip)
@@ -100,8 +107,9 @@ hipError_t launch_sample_frozen(int kpl, const int32_t* ndk, const int32_t* nkw,
                                 const int64_t* doc, const uint32_t* word, const uint16_t* z_old, const float* u,
                                 uint16_t* z_new, float* total, hipStream_t s);
 
-// fold-in
-hipError_t launch_infer(int kpl, const InferParams& p, int grid, hipStream_t s);
+// fold-in; the kernel reads its parameters from device memory so that one
+// captured graph serves every batch
+hipError_t launch_infer(int kpl, const InferParams* p_dev, int grid, hipStream_t s);
 int infer_blocks_per_cu(int kpl);
 
 }  // namespace lda
