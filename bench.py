@@ -33,12 +33,18 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_corpus(n_docs=None):
+def make_corpus(n_docs=None, cache=None):
     from tomotopy_amd import corpus, synth
     t = time.time()
+    if cache and os.path.exists(cache):
+        z = np.load(cache)
+        log("corpus loaded from %s" % cache)
+        return z["words"], z["off"], int(z["V"]), synth.WORKLOADS["large_k1000"]
     words, off, nw, w = synth.workload("large_k1000", n_docs=n_docs)
     ws, offs, real_v, _ = corpus.prepare_arrays(words, off, nw)
     log("corpus: %d docs, %d tokens, V=%d (%.1fs)" % (len(offs) - 1, len(ws), real_v, time.time() - t))
+    if cache:
+        np.savez(cache, words=ws, off=offs, V=real_v)
     return ws, offs, real_v, w
 
 
@@ -80,6 +86,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--docs", type=int, default=None, help="fewer documents (quick runs only)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--corpus-cache", default=None, help="npz file to reuse the generated corpus")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -88,7 +95,7 @@ def main():
 
     from tomotopy_amd import _native
     K, alpha, eta = 1000, 0.05, 0.01
-    ws, offs, V, w = make_corpus(args.docs)
+    ws, offs, V, w = make_corpus(args.docs, args.corpus_cache)
     D = len(offs) - 1
     n_valid = int(np.count_nonzero(ws < V))
     ctx = _native.Context(K, V, alpha, eta, seed=1234)

@@ -35,9 +35,10 @@ def _sparse_topic(rng, V, beta, keep_mass=1.0 - 1e-7, max_words=20000):
         g = np.ones(V)
         s = float(V)
     p = g / s
-    idx = np.argsort(-p, kind="stable")
+    cand = np.nonzero(p > 1e-12)[0]  # the rest carries < 1e-7 of the mass
+    idx = cand[np.argsort(-p[cand], kind="stable")]
     c = np.cumsum(p[idx])
-    n = int(min(np.searchsorted(c, keep_mass) + 1, max_words, V))
+    n = int(min(np.searchsorted(c, keep_mass) + 1, max_words, len(idx)))
     ids = idx[:n]
     pp = p[ids] / p[ids].sum()
     return ids, np.cumsum(pp)
@@ -53,14 +54,22 @@ def generate(seed, n_docs, V, true_k, alpha, beta, lengths, zipf_s=None, mix=0.5
     lengths = np.asarray(lengths, dtype=np.int64)
     N = int(lengths.sum())
     z = np.empty(N, dtype=np.int32)
-    pos = 0
-    for d in range(n_docs):
-        L = int(lengths[d])
-        th = rng.gamma(alpha, size=true_k)
-        s = th.sum()
-        th = th / s if s > 0 else np.full(true_k, 1.0 / true_k)
-        z[pos:pos + L] = np.minimum(np.searchsorted(np.cumsum(th), rng.random(L), side="right"), true_k - 1)
-        pos += L
+    off = np.zeros(n_docs + 1, dtype=np.int64)
+    np.cumsum(lengths, out=off[1:])
+    chunk = max(1, min(n_docs, 4_000_000 // max(true_k, 1)))
+    for d0 in range(0, n_docs, chunk):
+        d1 = min(n_docs, d0 + chunk)
+        th = rng.gamma(alpha, size=(d1 - d0, true_k))
+        s = th.sum(axis=1, keepdims=True)
+        th = np.where(s > 0, th / np.where(s > 0, s, 1.0), 1.0 / true_k)
+        cdf = np.cumsum(th, axis=1)
+        cdf[:, -1] = 1.0
+        # row-wise inverse CDF in one searchsorted: row r's cdf lives in [r, r+1]
+        rows = np.repeat(np.arange(d1 - d0), lengths[d0:d1])
+        u = rng.random(len(rows)) + rows
+        flat = (cdf + np.arange(d1 - d0)[:, None]).ravel()
+        idx = np.searchsorted(flat, u, side="right")
+        z[off[d0]:off[d1]] = np.minimum(idx - rows * true_k, true_k - 1)
     words = np.empty(N, dtype=np.int64)
     if zipf_s is not None:
         bg = rng.random(N) < mix
@@ -84,9 +93,7 @@ def generate(seed, n_docs, V, true_k, alpha, beta, lengths, zipf_s=None, mix=0.5
     rank = np.empty(len(uniq), dtype=np.int64)
     rank[np.argsort(first, kind="stable")] = np.arange(len(uniq))
     dense = rank[np.searchsorted(uniq, words)]
-    off = np.zeros(n_docs + 1, dtype=np.uint64)
-    np.cumsum(lengths, out=off[1:])
-    return dense.astype(np.uint32), off, int(len(uniq))
+    return dense.astype(np.uint32), off.astype(np.uint64), int(len(uniq))
 
 
 WORKLOADS = {
