@@ -1,0 +1,45 @@
+"""ll_per_word trajectories: device-wide GPU training vs the CPU oracle's
+partition schedule, 5 seeds each, on the plain_k20 workload (1,000 docs,
+K=20, alpha=0.1, eta=0.01, 100 iterations, alpha optimisation every 10).
+Acceptance: the seed-averaged ll_per_word agrees within 1% at every
+10-iteration checkpoint."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tomotopy_amd import _native, corpus, synth
+
+pytestmark = pytest.mark.gpu
+
+SEEDS = (1, 2, 3, 4, 5)
+
+
+def test_ll_per_word_trajectories_within_one_percent():
+    words, off, nw, w = synth.workload("plain_k20")
+    ws, offs, V, _ = corpus.prepare_arrays(words, off, nw)
+    K, N = 20, int(np.count_nonzero(ws < V))
+    threads = min(os.cpu_count() or 1, 16)
+    gpu, cpu = [], []
+    for seed in SEEDS:
+        ctx = _native.Context(K, V, 0.1, 0.01, seed=seed)
+        ctx.upload_corpus(ws, offs)
+        m = O.Model(K, V, 0.1, 0.01, seed, scan_order=0)
+        m.set_corpus(ws, offs)
+        tg, tc = [], []
+        for chk in range(10):
+            ctx.train(10, workers=0)
+            m.train(10, workers=8, threads=threads)
+            tg.append(sum(ctx.loglik()) / N)
+            tc.append(sum(m.loglik(0)) / N)
+        assert not ctx.check_invariants().any()
+        gpu.append(tg)
+        cpu.append(tc)
+        print("seed", seed, "gpu", np.round(tg, 4).tolist(), flush=True)
+        print("seed", seed, "cpu", np.round(tc, 4).tolist(), flush=True)
+    g = np.mean(gpu, axis=0)
+    c = np.mean(cpu, axis=0)
+    rel = np.abs(g - c) / np.abs(c)
+    print("relative difference per checkpoint:", np.round(rel, 5).tolist())
+    assert np.all(rel < 0.01), rel

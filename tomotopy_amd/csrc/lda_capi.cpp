@@ -122,6 +122,11 @@ const DigammaLut& digamma_lut() {
     return t;
 }
 
+// The sampler keeps float(n_dk) + alpha_k and recovers n_dk by rounding, which
+// is exact while n_dk + alpha_k < 2^21: documents and alpha are bounded here.
+constexpr uint64_t kMaxDocTokens = 1u << 20;
+constexpr float kMaxAlpha = (float)(1u << 20);
+
 int kpl_for(int K) {
     int kpl = 1;
     while (64 * kpl < K) kpl <<= 1;
@@ -352,6 +357,9 @@ int optimize_alpha(lda_ctx* c) {
             const float ak = c->alpha[k];
             const float nom = (float)(sk - (double)c->D * (double)dg(ak));
             c->alpha[k] = std::max(nom / denom * ak, 1e-5f);
+            if (!(c->alpha[k] < kMaxAlpha))
+                return fail(LDA_ERR_TRAINING, "alpha optimisation left the supported range (alpha_%d = %g)", k,
+                            (double)c->alpha[k]);
         }
         int rc = upload_alpha(c);
         if (rc) return rc;
@@ -384,8 +392,10 @@ int lda_ctx_create(int K, int V, const float* alpha, float eta, int termweight, 
     if (K <= 0 || K > LDA_MAX_K) return fail(LDA_ERR_INVALID_ARGUMENT, "wrong K value (K = %d); the device sampler supports 1..%d", K, LDA_MAX_K);
     if (V < 0) return fail(LDA_ERR_INVALID_ARGUMENT, "wrong V value (V = %d)", V);
     if (!alpha) return fail(LDA_ERR_INVALID_ARGUMENT, "alpha is NULL");
-    for (int k = 0; k < K; ++k)
+    for (int k = 0; k < K; ++k) {
         if (!(alpha[k] > 0.0f)) return fail(LDA_ERR_INVALID_ARGUMENT, "wrong alpha value");
+        if (!(alpha[k] < kMaxAlpha)) return fail(LDA_ERR_UNSUPPORTED, "alpha must be below 2^20 on the device");
+    }
     if (!(eta > 0.0f)) return fail(LDA_ERR_INVALID_ARGUMENT, "wrong eta value (eta = %f)", (double)eta);
     if (termweight != LDA_TW_ONE)
         return fail(LDA_ERR_UNSUPPORTED, "TermWeight %d is not supported by the device sampler (ONE only)", termweight);
@@ -447,8 +457,11 @@ int lda_upload_corpus(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off
     if (D < 0 || !doc_off) return fail(LDA_ERR_INVALID_ARGUMENT, "invalid document offsets");
     if (D > 0xFFFFFFFFll) return fail(LDA_ERR_INVALID_ARGUMENT, "too many documents");
     if (doc_off[0] != 0) return fail(LDA_ERR_INVALID_ARGUMENT, "doc_off[0] must be 0");
-    for (int64_t d = 0; d < D; ++d)
+    for (int64_t d = 0; d < D; ++d) {
         if (doc_off[d + 1] < doc_off[d]) return fail(LDA_ERR_INVALID_ARGUMENT, "doc_off must be non-decreasing");
+        if (doc_off[d + 1] - doc_off[d] > kMaxDocTokens)
+            return fail(LDA_ERR_UNSUPPORTED, "document %lld has more than 2^20 tokens", (long long)d);
+    }
     const uint64_t N = doc_off[D];
     if (N && !words) return fail(LDA_ERR_INVALID_ARGUMENT, "words is NULL");
     for (int64_t d = 0; d < D; ++d)
@@ -517,8 +530,10 @@ int lda_set_counts(lda_ctx* c, const int32_t* nkw, const int32_t* nk) {
 
 int lda_set_alpha(lda_ctx* c, const float* alpha) {
     if (!c || !alpha) return fail(LDA_ERR_INVALID_ARGUMENT, "NULL argument");
-    for (int k = 0; k < c->K; ++k)
+    for (int k = 0; k < c->K; ++k) {
         if (!(alpha[k] > 0.0f)) return fail(LDA_ERR_INVALID_ARGUMENT, "wrong alpha value");
+        if (!(alpha[k] < kMaxAlpha)) return fail(LDA_ERR_UNSUPPORTED, "alpha must be below 2^20 on the device");
+    }
     c->alpha.assign(alpha, alpha + c->K);
     HIP_TRY(hipSetDevice(c->device));
     return upload_alpha(c);
@@ -751,6 +766,8 @@ int lda_infer(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off, int64_
     if (doc_off[0] != 0) return fail(LDA_ERR_INVALID_ARGUMENT, "doc_off[0] must be 0");
     for (int64_t d = 0; d < D; ++d) {
         if (doc_off[d + 1] < doc_off[d]) return fail(LDA_ERR_INVALID_ARGUMENT, "doc_off must be non-decreasing");
+        if (doc_off[d + 1] - doc_off[d] > kMaxDocTokens)
+            return fail(LDA_ERR_UNSUPPORTED, "document %lld has more than 2^20 tokens", (long long)d);
         for (uint64_t g = doc_off[d] + 1; g < doc_off[d + 1]; ++g)
             if (words[g] < words[g - 1]) return fail(LDA_ERR_INVALID_ARGUMENT, "words must be sorted within each document");
     }

@@ -131,19 +131,21 @@ constexpr int kpl_shift() {
     return (KPL == 1) ? 0 : (KPL == 2) ? 1 : (KPL == 4) ? 2 : (KPL == 8) ? 3 : (KPL == 16) ? 4 : 5;
 }
 
-// Per-wavefront LDS slice of the state that receives single-topic updates,
-// each array Kp = 64*KPL entries in the lane layout (kpos):
-//   ndk  n_dk of the current document
-//   nk   the worker-private n_k
-//   y    RN(1 / (n_k + V*eta)), kept in step with nk
-//   col  n_kw[., v] of the current word including this worker's own updates
-// A topic update is one owner-lane LDS write; the per-token reads are
+// LDS state of the sampler, every array Kp = 64*KPL entries in the lane
+// layout (kpos).  Per wavefront (one document at a time):
+//   A    float(n_dk) + alpha_k; n_dk itself is rint(A - alpha_k), exact while
+//        n_dk + alpha_k < 2^21 (document length and alpha are checked on the host)
+//   col  n_kw[., v] of the current word, including this wavefront's updates
+// Per workgroup (the worker; its wavefronts share one private n_k):
+//   nk   the worker-private n_k (LDS atomics)
+//   y    RN(1 / (float(n_k) + V*eta)), rewritten after each n_k update
+// A topic update is one owner-lane write per array; the per-token reads are
 // conflict-free 16-byte ds_read per lane.
 struct WaveLds {
-    int* ndk;
+    float* A;
+    int* col;
     int* nk;
     float* y;
-    int* col;
 };
 
 extern __shared__ int lds_dyn[];
@@ -151,12 +153,13 @@ extern __shared__ int lds_dyn[];
 template <int KPL>
 __device__ __forceinline__ WaveLds wave_lds() {
     constexpr int Kp = 64 * KPL;
-    int* base = lds_dyn + (threadIdx.x >> 6) * (4 * Kp);
-    return WaveLds{base, base + Kp, reinterpret_cast<float*>(base + 2 * Kp), base + 3 * Kp};
+    int* w = lds_dyn + 2 * Kp + (threadIdx.x >> 6) * (2 * Kp);
+    return WaveLds{reinterpret_cast<float*>(w), w + Kp, lds_dyn, reinterpret_cast<float*>(lds_dyn + Kp)};
 }
 
+// dynamic LDS bytes of a workgroup with `waves` wavefronts
 template <int KPL>
-constexpr size_t wave_lds_bytes() { return (size_t)4 * 64 * KPL * sizeof(int); }
+constexpr size_t wave_lds_bytes(int waves = 1) { return (size_t)(2 + 2 * waves) * 64 * KPL * sizeof(int); }
 
 template <int KPL, typename V>
 __device__ __forceinline__ void lds_load(const int* a, int lane, V& o) { load_kvec<KPL>(a, lane, o); }
@@ -164,60 +167,100 @@ __device__ __forceinline__ void lds_load(const int* a, int lane, V& o) { load_kv
 template <int KPL, typename V>
 __device__ __forceinline__ void lds_store(int* a, int lane, const V& o) { store_kvec<KPL>(a, lane, o); }
 
+template <int KPL, typename V>
+__device__ __forceinline__ void lds_load_f(const float* a, int lane, V& o) { load_kvec_f<KPL>(a, lane, o); }
+
+template <int KPL, typename V>
+__device__ __forceinline__ void lds_store_f(float* a, int lane, const V& o) {
+    store_kvec<KPL>(reinterpret_cast<int*>(a), lane, __builtin_bit_cast(ivec<KPL>, o));
+}
+
 // addWordTo<delta> for topic z on this worker's state (LDAModel.hpp:287-299):
-// the owner lane updates n_dk, the column, the private n_k and its reciprocal.
+// the owner lane updates n_dk (through A), the column and the private n_k,
+// and refreshes the reciprocal.  Concurrent updates of one topic by two
+// wavefronts of a workgroup keep n_k exact; y then holds the reciprocal of a
+// value n_k had a moment earlier, which the Markstein step absorbs.
 template <int KPL>
-__device__ __forceinline__ void bump(const WaveLds& S, int lane, int z, int delta, float veta) {
+__device__ __forceinline__ void bump(const WaveLds& S, const fvec<KPL>& al, int lane, int z, int delta, float veta) {
+    const int sl = z & (KPL - 1);
+    const float a_sl = al[sl];
     if (lane == (z >> kpl_shift<KPL>())) {
-        const int q = kpos<KPL>(lane, z & (KPL - 1));
-        S.ndk[q] += delta;
+        const int q = kpos<KPL>(lane, sl);
+        const int d = (int)__builtin_rintf(S.A[q] - a_sl) + delta;
+        S.A[q] = (float)d + a_sl;
         S.col[q] += delta;
-        const int n = S.nk[q] + delta;
-        S.nk[q] = n;
+        const int n = atomicAdd(S.nk + q, delta) + delta;
         S.y[q] = 1.0f / ((float)n + veta);
     }
 }
 
-// Private n_k (and reciprocals) from a global K-vector.
+// Document counts into A = float(n_dk) + alpha.
 template <int KPL>
-__device__ __forceinline__ void set_nk(const WaveLds& S, const int32_t* __restrict__ src, int lane, float veta) {
-    ivec<KPL> n;
-    load_kvec<KPL>(src, lane, n);
+__device__ __forceinline__ void set_ndk(const WaveLds& S, const fvec<KPL>& al, const ivec<KPL>& nd, int lane) {
+    fvec<KPL> A;
+#pragma unroll
+    for (int j = 0; j < KPL; ++j) A[j] = (float)nd[j] + al[j];
+    lds_store_f<KPL>(S.A, lane, A);
+}
+
+// n_dk back from A.
+template <int KPL>
+__device__ __forceinline__ ivec<KPL> get_ndk(const WaveLds& S, const fvec<KPL>& al, int lane) {
+    fvec<KPL> A;
+    lds_load_f<KPL>(S.A, lane, A);
+    ivec<KPL> nd;
+#pragma unroll
+    for (int j = 0; j < KPL; ++j) nd[j] = (int)__builtin_rintf(A[j] - al[j]);
+    return nd;
+}
+
+// Private n_k from a K-vector: n_k and the reciprocals (one wavefront writes).
+template <int KPL>
+__device__ __forceinline__ void set_nk_vec(const WaveLds& S, const ivec<KPL>& n, int lane, float veta) {
     fvec<KPL> y;
 #pragma unroll
     for (int j = 0; j < KPL; ++j) y[j] = 1.0f / ((float)n[j] + veta);
     lds_store<KPL>(S.nk, lane, n);
-    lds_store<KPL>(reinterpret_cast<int*>(S.y), lane, __builtin_bit_cast(ivec<KPL>, y));
+    lds_store_f<KPL>(S.y, lane, y);
 }
 
 template <int KPL>
-__device__ __forceinline__ void lane_limits(int K, int lane, int& nvalid, int& lastLane, int& lastSlot) {
+__device__ __forceinline__ void set_nk(const WaveLds& S, const int32_t* __restrict__ src, int lane, float veta) {
+    ivec<KPL> n;
+    load_kvec<KPL>(src, lane, n);
+    set_nk_vec<KPL>(S, n, lane, veta);
+}
+
+template <int KPL>
+__device__ __forceinline__ void lane_limits(int K, int lane, int& nvalid, int& lastLane) {
     int nv = K - lane * KPL;
     nvalid = nv < 0 ? 0 : (nv > KPL ? KPL : nv);
     lastLane = (K - 1) / KPL;
-    lastSlot = (K - 1) % KPL;
 }
 
 // One token: remove z_old, weight, scan, search, add z_new.  Returns z_new
-// (wave-uniform).  *total receives cdf[K-1].
+// (wave-uniform).  *total receives cdf[K-1].  Padded topics (k >= K) have
+// weight exactly 0, so inside a lane the cdf of a padded slot equals that of
+// the lane's last real topic: the total can be read from slot KPL-1 of lane
+// (K-1)/KPL, and a lane's miss count over all slots stays below its number of
+// real topics exactly when one of them is hit.
 template <int KPL>
 __device__ __forceinline__ int token_step(const WaveLds& S, const fvec<KPL>& al, int lane, int zo, float u, int K,
-                                          int nvalid, int lastLane, int lastSlot, float eta, float veta, float* total) {
-    bump<KPL>(S, lane, zo, -1, veta);
-    ivec<KPL> ndk, nk, col, yb;
-    lds_load<KPL>(S.ndk, lane, ndk);
+                                          int nvalid, int lastLane, float eta, float veta, float* total) {
+    bump<KPL>(S, al, lane, zo, -1, veta);
+    fvec<KPL> A, y;
+    ivec<KPL> col, nk;
+    lds_load_f<KPL>(S.A, lane, A);
     lds_load<KPL>(S.col, lane, col);
     lds_load<KPL>(S.nk, lane, nk);
-    lds_load<KPL>(reinterpret_cast<const int*>(S.y), lane, yb);
-    const fvec<KPL> y = __builtin_bit_cast(fvec<KPL>, yb);
+    lds_load_f<KPL>(S.y, lane, y);
     fvec<KPL> w;
 #pragma unroll
     for (int j = 0; j < KPL; ++j) {
         // ((n_dk + alpha) * (n_kw + eta)) / (n_k + V*eta); the quotient as
         // q0 = a*y, r = fma(-q0, b, a), q = fma(r, y, q0) with y = RN(1/b)
-        float A = (float)ndk[j] + al[j];
         float B = (float)col[j] + eta;
-        float a = A * B;
+        float a = A[j] * B;
         float bf = (float)nk[j] + veta;
         float q0 = a * y[j];
         float rr = __builtin_fmaf(-q0, bf, a);
@@ -229,14 +272,11 @@ __device__ __forceinline__ int token_step(const WaveLds& S, const fvec<KPL>& al,
     float excl = dpp_wave_shr1(incl);
 #pragma unroll
     for (int j = 0; j < KPL; ++j) w[j] = excl + w[j];
-    float last = w[0];
-#pragma unroll
-    for (int j = 1; j < KPL; ++j) last = (j == lastSlot) ? w[j] : last;
-    const float tot = readlane_f(last, lastLane);
+    const float tot = readlane_f(w[KPL - 1], lastLane);
     const float r = u * tot;
     int miss = 0;
 #pragma unroll
-    for (int j = 0; j < KPL; ++j) miss += (j < nvalid && !(r < w[j])) ? 1 : 0;
+    for (int j = 0; j < KPL; ++j) miss += (r < w[j]) ? 0 : 1;
     const unsigned long long m = __ballot(miss < nvalid);
     int zn;
     if (m == 0ull) zn = K - 1;
@@ -245,7 +285,7 @@ __device__ __forceinline__ int token_step(const WaveLds& S, const fvec<KPL>& al,
         zn = L * KPL + readlane_i(miss, L);
     }
     zn = uniform_i(zn);
-    bump<KPL>(S, lane, zn, +1, veta);
+    bump<KPL>(S, al, lane, zn, +1, veta);
     if (total) *total = tot;
     return zn;
 }
@@ -261,7 +301,7 @@ __constant__ uint64_t kShufflePrimes[16] = {65537, 65539, 65543, 65551, 65557, 6
 // column of the next distinct word is fetched one token ahead.
 template <int KPL, int MODE>
 __device__ __forceinline__ void sample_range(const WaveLds& S, const fvec<KPL>& al, const SweepParams& p, int lane,
-                                             uint64_t b, uint64_t e, int nvalid, int lastLane, int lastSlot) {
+                                             uint64_t b, uint64_t e, int nvalid, int lastLane) {
     constexpr int SH = kpl_shift<KPL>();
     int vcur = -1;
     int32_t* colp = nullptr;
@@ -296,7 +336,7 @@ __device__ __forceinline__ void sample_range(const WaveLds& S, const fvec<KPL>& 
             }
             const int zo = readlane_i(zv, i);
             const float u = readlane_f(uv, i);
-            const int zn = token_step<KPL>(S, al, lane, zo, u, p.K, nvalid, lastLane, lastSlot, p.eta, p.veta, nullptr);
+            const int zn = token_step<KPL>(S, al, lane, zo, u, p.K, nvalid, lastLane, p.eta, p.veta, nullptr);
             if (zn != zo) {
                 const int lo = zo >> SH, ln = zn >> SH;
                 const int qo = kpos<KPL>(lane, zo & (KPL - 1)), qn = kpos<KPL>(lane, zn & (KPL - 1));
@@ -319,15 +359,18 @@ __device__ __forceinline__ void sample_range(const WaveLds& S, const fvec<KPL>& 
 // partition workers keep theirs, LDAModel.hpp:484, 596-602); documents are
 // taken longest-first from a global counter; n_kw is updated with atomics;
 // each worker adds its n_k delta to nk_out when it runs out of documents.
+constexpr int kSweepWaves = 4;  // wavefronts per workgroup of the device-wide sweep (one worker)
+
 template <int KPL>
-__global__ void __launch_bounds__(64) sweep_device_kernel(SweepParams p) {
+__global__ void __launch_bounds__(64 * kSweepWaves) sweep_device_kernel(SweepParams p) {
     const int lane = lane_id();
     const WaveLds S = wave_lds<KPL>();
-    int nvalid, lastLane, lastSlot;
-    lane_limits<KPL>(p.K, lane, nvalid, lastLane, lastSlot);
+    int nvalid, lastLane;
+    lane_limits<KPL>(p.K, lane, nvalid, lastLane);
     fvec<KPL> al;
     load_kvec_f<KPL>(p.alpha, lane, al);
-    set_nk<KPL>(S, p.nk_in, lane, p.veta);
+    if (threadIdx.x < 64) set_nk<KPL>(S, p.nk_in, lane, p.veta);
+    __syncthreads();
     for (;;) {
         uint32_t idx = 0;
         if (lane == 0) idx = atomicAdd(p.counter, 1u);
@@ -338,18 +381,21 @@ __global__ void __launch_bounds__(64) sweep_device_kernel(SweepParams p) {
         int32_t* ndk_row = p.ndk + (int64_t)d * p.Kp;
         ivec<KPL> nd;
         load_kvec<KPL>(ndk_row, lane, nd);
-        lds_store<KPL>(S.ndk, lane, nd);
-        sample_range<KPL, MODE_ATOMIC>(S, al, p, lane, b, e, nvalid, lastLane, lastSlot);
-        lds_load<KPL>(S.ndk, lane, nd);
+        set_ndk<KPL>(S, al, nd, lane);
+        sample_range<KPL, MODE_ATOMIC>(S, al, p, lane, b, e, nvalid, lastLane);
+        nd = get_ndk<KPL>(S, al, lane);
         store_kvec<KPL>(ndk_row, lane, nd);
     }
-    ivec<KPL> snap, mine;
-    load_kvec<KPL>(p.nk_in, lane, snap);
-    lds_load<KPL>(S.nk, lane, mine);
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        ivec<KPL> snap, mine;
+        load_kvec<KPL>(p.nk_in, lane, snap);
+        lds_load<KPL>(S.nk, lane, mine);
 #pragma unroll
-    for (int j = 0; j < KPL; ++j) {
-        const int dlt = mine[j] - snap[j];
-        if (dlt) atomicAdd(p.nk_out + kpos<KPL>(lane, j), dlt);
+        for (int j = 0; j < KPL; ++j) {
+            const int dlt = mine[j] - snap[j];
+            if (dlt) atomicAdd(p.nk_out + kpos<KPL>(lane, j), dlt);
+        }
     }
 }
 
@@ -362,8 +408,8 @@ __global__ void __launch_bounds__(64) sweep_det_kernel(SweepParams p) {
     const int lane = lane_id();
     const int w = blockIdx.x;
     const WaveLds S = wave_lds<KPL>();
-    int nvalid, lastLane, lastSlot;
-    lane_limits<KPL>(p.K, lane, nvalid, lastLane, lastSlot);
+    int nvalid, lastLane;
+    lane_limits<KPL>(p.K, lane, nvalid, lastLane);
     fvec<KPL> al;
     load_kvec_f<KPL>(p.alpha, lane, al);
     int32_t* nkp = p.nk_out + (int64_t)w * p.Kp;
@@ -392,9 +438,9 @@ __global__ void __launch_bounds__(64) sweep_det_kernel(SweepParams p) {
         int32_t* ndk_row = p.ndk + d * p.Kp;
         ivec<KPL> nd;
         load_kvec<KPL>(ndk_row, lane, nd);
-        lds_store<KPL>(S.ndk, lane, nd);
-        sample_range<KPL, MODE_DET>(S, al, p, lane, b, e, nvalid, lastLane, lastSlot);
-        lds_load<KPL>(S.ndk, lane, nd);
+        set_ndk<KPL>(S, al, nd, lane);
+        sample_range<KPL, MODE_DET>(S, al, p, lane, b, e, nvalid, lastLane);
+        nd = get_ndk<KPL>(S, al, lane);
         store_kvec<KPL>(ndk_row, lane, nd);
     }
     ivec<KPL> mine;
@@ -630,19 +676,19 @@ __global__ void __launch_bounds__(64) sample_frozen_kernel(const int32_t* __rest
                                                            uint16_t* z_new, float* total) {
     const int lane = lane_id();
     const WaveLds S = wave_lds<KPL>();
-    int nvalid, lastLane, lastSlot;
-    lane_limits<KPL>(K, lane, nvalid, lastLane, lastSlot);
+    int nvalid, lastLane;
+    lane_limits<KPL>(K, lane, nvalid, lastLane);
     fvec<KPL> al;
     load_kvec_f<KPL>(alpha, lane, al);
     for (int64_t t = blockIdx.x; t < n; t += gridDim.x) {
         set_nk<KPL>(S, nk, lane, veta);
         ivec<KPL> x;
         load_kvec<KPL>(ndk + doc[t] * Kp, lane, x);
-        lds_store<KPL>(S.ndk, lane, x);
+        set_ndk<KPL>(S, al, x, lane);
         load_kvec<KPL>(nkw + (int64_t)word[t] * Kp, lane, x);
         lds_store<KPL>(S.col, lane, x);
         float tot;
-        const int zn = token_step<KPL>(S, al, lane, (int)z_old[t], u[t], K, nvalid, lastLane, lastSlot, eta, veta, &tot);
+        const int zn = token_step<KPL>(S, al, lane, (int)z_old[t], u[t], K, nvalid, lastLane, eta, veta, &tot);
         if (lane == 0) {
             z_new[t] = (uint16_t)zn;
             if (total) total[t] = tot;
@@ -683,8 +729,8 @@ __global__ void __launch_bounds__(64) infer_kernel(const InferParams* __restrict
     const InferParams p = *pp;
     const int lane = lane_id();
     const WaveLds S = wave_lds<KPL>();
-    int nvalid, lastLane, lastSlot;
-    lane_limits<KPL>(p.K, lane, nvalid, lastLane, lastSlot);
+    int nvalid, lastLane;
+    lane_limits<KPL>(p.K, lane, nvalid, lastLane);
     fvec<KPL> al;
     load_kvec_f<KPL>(p.alpha, lane, al);
     ivec<KPL> nk0;
@@ -703,7 +749,7 @@ __global__ void __launch_bounds__(64) infer_kernel(const InferParams* __restrict
         const uint64_t b = J.b, e = J.e;
         // topics at the start of this launch: initial draw or the stored z; n_dk from them
         ivec<KPL> zero = 0;
-        lds_store<KPL>(S.ndk, lane, zero);
+        lds_store<KPL>(S.col, lane, zero);  // the column array holds the n_dk histogram until sampling starts
         for (uint64_t t0 = b; t0 < e; t0 += 64) {
             const uint64_t g = t0 + (uint64_t)lane;
             int zz = 0xFFFF;
@@ -728,19 +774,13 @@ __global__ void __launch_bounds__(64) infer_kernel(const InferParams* __restrict
             for (int i = 0; i < n; ++i) {
                 const int k = readlane_i(zz, i);
                 if (k == 0xFFFF) continue;
-                if (lane == (k >> SH)) S.ndk[kpos<KPL>(lane, k & (KPL - 1))] += 1;
+                if (lane == (k >> SH)) S.col[kpos<KPL>(lane, k & (KPL - 1))] += 1;
             }
         }
         ivec<KPL> nd;
-        lds_load<KPL>(S.ndk, lane, nd);
-        {
-            const ivec<KPL> nke = nk0 + nd;
-            fvec<KPL> y;
-#pragma unroll
-            for (int j = 0; j < KPL; ++j) y[j] = 1.0f / ((float)nke[j] + p.veta);
-            lds_store<KPL>(S.nk, lane, nke);
-            lds_store<KPL>(reinterpret_cast<int*>(S.y), lane, __builtin_bit_cast(ivec<KPL>, y));
-        }
+        lds_load<KPL>(S.col, lane, nd);
+        set_ndk<KPL>(S, al, nd, lane);
+        set_nk_vec<KPL>(S, nk0 + nd, lane, p.veta);
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
         for (int it = J.it_begin + 1; it <= J.it_end; ++it) {
             int vcur = -1;
@@ -767,7 +807,7 @@ __global__ void __launch_bounds__(64) infer_kernel(const InferParams* __restrict
                     }
                     const int zo = readlane_i(zv, i);
                     const float u = readlane_f(uv, i);
-                    const int zn = token_step<KPL>(S, al, lane, zo, u, p.K, nvalid, lastLane, lastSlot, p.eta, p.veta, nullptr);
+                    const int zn = token_step<KPL>(S, al, lane, zo, u, p.K, nvalid, lastLane, p.eta, p.veta, nullptr);
                     zkeep = (lane == i) ? zn : zkeep;
                 }
                 if (lane < n) p.z[g] = (uint16_t)zkeep;
@@ -778,7 +818,7 @@ __global__ void __launch_bounds__(64) infer_kernel(const InferParams* __restrict
             if (!p.counter) break;
             continue;
         }
-        lds_load<KPL>(S.ndk, lane, nd);
+        nd = get_ndk<KPL>(S, al, lane);
         int L = 0;
 #pragma unroll
         for (int j = 0; j < KPL; ++j) L += nd[j];
@@ -870,7 +910,8 @@ hipError_t launch_build_counts(int kpl, const uint32_t* words, const uint16_t* z
 }
 
 hipError_t launch_sweep_device(int kpl, const SweepParams& p, int grid, hipStream_t s) {
-    LDA_DISPATCH_KPL(kpl, hipLaunchKernelGGL(sweep_device_kernel<KPL>, dim3(grid), dim3(64), wave_lds_bytes<KPL>(), s, p));
+    LDA_DISPATCH_KPL(kpl, hipLaunchKernelGGL(sweep_device_kernel<KPL>, dim3(grid), dim3(64 * kSweepWaves),
+                                             wave_lds_bytes<KPL>(kSweepWaves), s, p));
     return hipGetLastError();
 }
 
@@ -883,12 +924,12 @@ int sweep_device_blocks_per_cu(int kpl) {
     int n = 0;
     hipError_t e = hipErrorInvalidValue;
     switch (kpl) {
-        case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sweep_device_kernel<1>, 64, wave_lds_bytes<1>()); break;
-        case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sweep_device_kernel<2>, 64, wave_lds_bytes<2>()); break;
-        case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sweep_device_kernel<4>, 64, wave_lds_bytes<4>()); break;
-        case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sweep_device_kernel<8>, 64, wave_lds_bytes<8>()); break;
-        case 16: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sweep_device_kernel<16>, 64, wave_lds_bytes<16>()); break;
-        case 32: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sweep_device_kernel<32>, 64, wave_lds_bytes<32>()); break;
+        case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sweep_device_kernel<1>, 64 * kSweepWaves, wave_lds_bytes<1>(kSweepWaves)); break;
+        case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sweep_device_kernel<2>, 64 * kSweepWaves, wave_lds_bytes<2>(kSweepWaves)); break;
+        case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sweep_device_kernel<4>, 64 * kSweepWaves, wave_lds_bytes<4>(kSweepWaves)); break;
+        case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sweep_device_kernel<8>, 64 * kSweepWaves, wave_lds_bytes<8>(kSweepWaves)); break;
+        case 16: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sweep_device_kernel<16>, 64 * kSweepWaves, wave_lds_bytes<16>(kSweepWaves)); break;
+        case 32: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sweep_device_kernel<32>, 64 * kSweepWaves, wave_lds_bytes<32>(kSweepWaves)); break;
         default: break;
     }
     return (e == hipSuccess && n > 0) ? n : 1;
