@@ -79,6 +79,50 @@ def cpu_baseline(ws, offs, V, K, alpha, eta, budget_s=12.0):
     }
 
 
+def infer_bench(plain_sample=300, iterations=100):
+    """Fold-in: batched (one persistent launch in a replayed hipGraph) vs plain
+    (one launch + synchronisation per document) against a frozen K=200 model."""
+    from tomotopy_amd import _native, corpus, synth
+    t = time.time()
+    (tw, toff, nw), (uw, uoff) = synth.foldin_workload()
+    ws, offs, V, order = corpus.prepare_arrays(tw, toff, nw)
+    uws, uoffs = corpus.csr_from_arrays(uw, uoff, order)
+    D = len(uoffs) - 1
+    log("fold-in workload: %d unseen docs, %d tokens (%.1fs)" % (D, len(uws), time.time() - t))
+    K = 200
+    ctx = _native.Context(K, V, 0.1, 0.01, seed=77)
+    ctx.upload_corpus(ws, offs)
+    ctx.train(100, workers=0)
+    # warm: graph capture and first-touch allocations
+    ctx.infer(uws[:int(uoffs[64])], uoffs[:65], 2)
+    t0 = time.perf_counter()
+    theta, ll, _ = ctx.infer(uws, uoffs, iterations, mode=0)
+    batched_s = time.perf_counter() - t0
+    kst = ctx.kernel_stats()
+    rng = np.random.default_rng(5)
+    pick = np.sort(rng.choice(D, size=min(plain_sample, D), replace=False))
+    lens = np.diff(uoffs.astype(np.int64))
+    sub_off = np.zeros(len(pick) + 1, np.uint64)
+    sub_off[1:] = np.cumsum(lens[pick])
+    sub_w = np.concatenate([uws[int(uoffs[d]):int(uoffs[d + 1])] for d in pick]).astype(np.uint32)
+    t0 = time.perf_counter()
+    th_p, ll_p, _ = ctx.infer(sub_w, sub_off, iterations, mode=1)
+    plain_s = time.perf_counter() - t0
+    same = bool(np.array_equal(th_p, theta[pick]) and np.array_equal(ll_p, ll[pick]))
+    ntok = int(np.count_nonzero(uws < V))
+    out = {
+        "workload": "foldin_k200", "docs": D, "tokens": ntok, "iterations": iterations, "K": K, "V": V,
+        "batched_docs_per_s": D / batched_s, "batched_s": batched_s, "batched_kernel_ms": kst["ms"],
+        "plain_docs_per_s": len(pick) / plain_s, "plain_sample_docs": int(len(pick)),
+        "speedup": (D / batched_s) / (len(pick) / plain_s),
+        "ll_per_word": float(ll.sum() / ntok), "plain_equals_batched": same,
+        "longest_doc": int(lens.max()),
+    }
+    log("fold-in: batched %.0f docs/s (%.3fs), plain %.1f docs/s on %d sampled docs, identical=%s"
+        % (out["batched_docs_per_s"], batched_s, out["plain_docs_per_s"], len(pick), same))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,6 +131,7 @@ def main():
     ap.add_argument("--docs", type=int, default=None, help="fewer documents (quick runs only)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--corpus-cache", default=None, help="npz file to reuse the generated corpus")
+    ap.add_argument("--no-infer", action="store_true", help="skip the fold-in measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -143,6 +188,9 @@ def main():
     }
     if rank == 0 and not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(ws, offs, V, K, alpha, eta)
+    if not args.no_infer:
+        del ctx
+        out["infer"] = infer_bench()
     print(json.dumps(out), flush=True)
 
 

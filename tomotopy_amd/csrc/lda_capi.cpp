@@ -816,6 +816,7 @@ int lda_infer(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off, int64_
             HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
             hipError_t e1 = lda::launch_fill_u32(c->i_counter.p, 0u, 1, s);
             hipError_t e2 = e1 == hipSuccess ? lda::launch_infer(c->kpl, c->i_params.p, grid, s) : e1;
+            if (e2 == hipSuccess) e2 = lda::launch_infer_finalize(c->kpl, c->i_params.p, grid, s);
             hipError_t e3 = hipStreamEndCapture(s, &g);
             if (e2 != hipSuccess || e3 != hipSuccess) {
                 if (g) (void)hipGraphDestroy(g);
@@ -838,6 +839,7 @@ int lda_infer(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off, int64_
             p.job_base = d;
             HIP_TRY(hipMemcpyAsync(c->i_params.p, &p, sizeof p, hipMemcpyHostToDevice, s));
             HIP_TRY(lda::launch_infer(c->kpl, c->i_params.p, 1, s));
+            HIP_TRY(lda::launch_infer_finalize(c->kpl, c->i_params.p, 1, s));
             HIP_TRY(hipStreamSynchronize(s));
         }
         HIP_TRY(hipEventRecord(c->ev1, s));

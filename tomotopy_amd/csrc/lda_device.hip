@@ -795,15 +795,31 @@ __global__ void __launch_bounds__(64) infer_kernel(const InferParams* __restrict
                     uv = uniform24(philox_x0(p.seed_lo, p.seed_hi, (uint32_t)(g - b), (uint32_t)((g - b) >> 32), (uint32_t)it, TAG_INFER));
                 }
                 int zkeep = zv;
+                int vnext = -1;
+                ivec<KPL> ncol;
                 for (int i = 0; i < n; ++i) {
                     const int v = readlane_i(wv, i);
                     if ((unsigned)v >= (unsigned)p.V) continue;
                     if (v != vcur) {
-                        ivec<KPL> c;
-                        load_kvec<KPL>(p.nkw + (int64_t)v * p.Kp, lane, c);
-                        lds_store<KPL>(S.col, lane, c);
-                        add_run_counts<KPL>(S, p, lane, t0 + (uint64_t)i, e, v);
+                        if (v != vnext) load_kvec<KPL>(p.nkw + (int64_t)v * p.Kp, lane, ncol);
+                        lds_store<KPL>(S.col, lane, ncol);
+                        // c_d of this word: the run's topics, from the batch
+                        // registers (not yet resampled in this iteration) and,
+                        // past the batch, from memory
+                        int j = i;
+                        for (; j < n && readlane_i(wv, j) == v; ++j) {
+                            const int k = readlane_i(zv, j);
+                            if (lane == (k >> SH)) S.col[kpos<KPL>(lane, k & (KPL - 1))] += 1;
+                        }
+                        if (j == n) add_run_counts<KPL>(S, p, lane, t0 + (uint64_t)n, e, v);
                         vcur = v;
+                    }
+                    if (i + 1 < n) {
+                        const int v2 = readlane_i(wv, i + 1);
+                        if (v2 != v && (unsigned)v2 < (unsigned)p.V) {
+                            load_kvec<KPL>(p.nkw + (int64_t)v2 * p.Kp, lane, ncol);
+                            vnext = v2;
+                        }
                     }
                     const int zo = readlane_i(zv, i);
                     const float u = readlane_f(uv, i);
@@ -814,23 +830,58 @@ __global__ void __launch_bounds__(64) infer_kernel(const InferParams* __restrict
             }
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
         }
-        if (J.it_end < J.iters) {
-            if (!p.counter) break;
-            continue;
+        if (!p.counter) break;
+    }
+}
+
+// theta and the FP64 log-likelihood of finished fold-in jobs (it_end == iters),
+// one wavefront per job: n_dk and the per-run c_d are rebuilt from z.
+// getLLRest(private) - getLLRest(global) + getLLDocs(doc), with the private
+// state's difference restricted to the terms the document touches
+// (LDAModel.hpp:945-946); theta = (n_dk + alpha) / (L + sum alpha)
+// (LDAModel.hpp:1145-1159).
+template <int KPL>
+__global__ void __launch_bounds__(64) infer_finalize_kernel(const InferParams* __restrict__ pp) {
+    constexpr int SH = kpl_shift<KPL>();
+    const InferParams p = *pp;
+    const int lane = lane_id();
+    const WaveLds S = wave_lds<KPL>();
+    fvec<KPL> al;
+    load_kvec_f<KPL>(p.alpha, lane, al);
+    ivec<KPL> nk0;
+    load_kvec<KPL>(p.nk, lane, nk0);
+    // batched: all jobs, grid-stride; plain (no counter): the launch's own job only
+    const int64_t first = p.counter ? (int64_t)blockIdx.x : p.job_base + blockIdx.x;
+    const int64_t last = p.counter ? p.njobs : (first + 1 < p.njobs ? first + 1 : p.njobs);
+    const int64_t stride = p.counter ? (int64_t)gridDim.x : 1;
+    for (int64_t idx = first; idx < last; idx += stride) {
+        const InferJob J = p.jobs[idx];
+        if (J.it_end < J.iters) continue;
+        const uint64_t b = J.b, e = J.e;
+        ivec<KPL> zero = 0;
+        lds_store<KPL>(S.col, lane, zero);
+        for (uint64_t t0 = b; t0 < e; t0 += 64) {
+            const uint64_t g = t0 + (uint64_t)lane;
+            const int zz = g < e ? (int)p.z[g] : 0xFFFF;
+            const int n = (int)((e - t0) < 64 ? (e - t0) : 64);
+            for (int i = 0; i < n; ++i) {
+                const int k = readlane_i(zz, i);
+                if (k == 0xFFFF) continue;
+                if (lane == (k >> SH)) S.col[kpos<KPL>(lane, k & (KPL - 1))] += 1;
+            }
         }
-        nd = get_ndk<KPL>(S, al, lane);
+        ivec<KPL> nd;
+        lds_load<KPL>(S.col, lane, nd);
         int L = 0;
 #pragma unroll
         for (int j = 0; j < KPL; ++j) L += nd[j];
         for (int o = 32; o >= 1; o >>= 1) L += __shfl_xor(L, o);
-        // theta (LDAModel.hpp:1145-1159)
         const float denom = (float)L + p.asum_f;
 #pragma unroll
         for (int j = 0; j < KPL; ++j) {
             const int k = lane * KPL + j;
             if (k < p.K) p.theta[(int64_t)J.slot * p.K + k] = ((float)nd[j] + al[j]) / denom;
         }
-        // FP64 log-likelihood: getLLRest(private) - getLLRest(global) + getLLDocs(doc)
         const double eta = (double)p.eta;
         const double veta = (double)p.V * eta;
         const double lge = lgamma(eta);
@@ -872,7 +923,6 @@ __global__ void __launch_bounds__(64) infer_kernel(const InferParams* __restrict
             ll -= lgamma((double)L + asum) - lgamma(asum);
             p.ll[J.slot] = ll;
         }
-        if (!p.counter) break;
     }
 }
 
@@ -1024,6 +1074,12 @@ hipError_t launch_sample_frozen(int kpl, const int32_t* ndk, const int32_t* nkw,
 
 hipError_t launch_infer(int kpl, const InferParams* p_dev, int grid, hipStream_t s) {
     LDA_DISPATCH_KPL(kpl, hipLaunchKernelGGL(infer_kernel<KPL>, dim3(grid), dim3(64), wave_lds_bytes<KPL>(), s, p_dev));
+    return hipGetLastError();
+}
+
+hipError_t launch_infer_finalize(int kpl, const InferParams* p_dev, int grid, hipStream_t s) {
+    LDA_DISPATCH_KPL(kpl, hipLaunchKernelGGL(infer_finalize_kernel<KPL>, dim3(grid), dim3(64), wave_lds_bytes<KPL>(), s,
+                                             p_dev));
     return hipGetLastError();
 }
 

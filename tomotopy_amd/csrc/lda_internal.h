@@ -110,6 +110,8 @@ hipError_t launch_sample_frozen(int kpl, const int32_t* ndk, const int32_t* nkw,
 // fold-in; the kernel reads its parameters from device memory so that one
 // captured graph serves every batch
 hipError_t launch_infer(int kpl, const InferParams* p_dev, int grid, hipStream_t s);
+// theta and FP64 log-likelihood of the jobs that finish (it_end == iters)
+hipError_t launch_infer_finalize(int kpl, const InferParams* p_dev, int grid, hipStream_t s);
 int infer_blocks_per_cu(int kpl);
 
 }  // namespace lda

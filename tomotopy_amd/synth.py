@@ -44,14 +44,19 @@ def _sparse_topic(rng, V, beta, keep_mass=1.0 - 1e-7, max_words=20000):
     return ids, np.cumsum(pp)
 
 
-def generate(seed, n_docs, V, true_k, alpha, beta, lengths, zipf_s=None, mix=0.5):
-    """Draw a corpus; returns (words, doc_off, n_words) with first-appearance ids."""
-    rng = _rng(seed)
+def _topics(rng, V, true_k, beta, zipf_s):
     topics = [_sparse_topic(rng, V, beta) for _ in range(true_k)]
+    zcdf = None
     if zipf_s is not None:
         zp = np.arange(1, V + 1, dtype=np.float64) ** (-float(zipf_s))
         zcdf = np.cumsum(zp / zp.sum())
+    return topics, zcdf
+
+
+def _draw(rng, topics, zcdf, V, true_k, alpha, lengths, mix=0.5):
+    """Raw word ids (0..V-1) of documents with the given lengths."""
     lengths = np.asarray(lengths, dtype=np.int64)
+    n_docs = len(lengths)
     N = int(lengths.sum())
     z = np.empty(N, dtype=np.int32)
     off = np.zeros(n_docs + 1, dtype=np.int64)
@@ -71,7 +76,7 @@ def generate(seed, n_docs, V, true_k, alpha, beta, lengths, zipf_s=None, mix=0.5
         idx = np.searchsorted(flat, u, side="right")
         z[off[d0]:off[d1]] = np.minimum(idx - rows * true_k, true_k - 1)
     words = np.empty(N, dtype=np.int64)
-    if zipf_s is not None:
+    if zcdf is not None:
         bg = rng.random(N) < mix
         nb = int(bg.sum())
         words[bg] = np.minimum(np.searchsorted(zcdf, rng.random(nb), side="right"), V - 1)
@@ -88,12 +93,52 @@ def generate(seed, n_docs, V, true_k, alpha, beta, lengths, zipf_s=None, mix=0.5
                 continue
             ids, cdf = topics[k]
             words[sel] = ids[np.minimum(np.searchsorted(cdf, rng.random(len(sel)), side="right"), len(ids) - 1)]
-    # dictionary ids in order of first appearance
+    return words, off.astype(np.uint64)
+
+
+def _first_appearance_ids(words, n_ids):
+    """Dictionary ids in order of first appearance; returns (dense words, raw->id map with -1 for unseen)."""
     uniq, first = np.unique(words, return_index=True)
     rank = np.empty(len(uniq), dtype=np.int64)
     rank[np.argsort(first, kind="stable")] = np.arange(len(uniq))
     dense = rank[np.searchsorted(uniq, words)]
-    return dense.astype(np.uint32), off.astype(np.uint64), int(len(uniq))
+    mapping = np.full(n_ids, -1, dtype=np.int64)
+    mapping[uniq] = rank
+    return dense, mapping
+
+
+def generate(seed, n_docs, V, true_k, alpha, beta, lengths, zipf_s=None, mix=0.5):
+    """Draw a corpus; returns (words, doc_off, n_words) with first-appearance ids."""
+    rng = _rng(seed)
+    topics, zcdf = _topics(rng, V, true_k, beta, zipf_s)
+    words, off = _draw(rng, topics, zcdf, V, true_k, alpha, lengths, mix)
+    dense, mapping = _first_appearance_ids(words, V)
+    return dense.astype(np.uint32), off, int((mapping >= 0).sum())
+
+
+def foldin_workload(train_docs=1000, unseen_docs=20000, seed=1004):
+    """Fold-in workload: a README-scale training corpus (1,000 docs, mean length
+    1,500, V=50,000, Zipf(1.07) background mixed into 100 Dir(0.01) topics) and
+    unseen documents with lengths LogNormal(ln 200, 1.2) clipped to [5, 20000]
+    drawn from the same topics.  Unseen words are given dictionary ids of the
+    training corpus; words the training corpus never saw are dropped, as
+    make_doc drops out-of-dictionary words.
+    Returns (train_words, train_off, n_words), (unseen_words, unseen_off)."""
+    V, true_k = 50000, 100
+    rng = _rng(seed)
+    topics, zcdf = _topics(rng, V, true_k, 0.01, 1.07)
+    tl = lognormal_lengths(_rng(seed + 7919), train_docs, np.log(1500.0) - 0.125, 0.5, 10, 15000)
+    tw, toff = _draw(rng, topics, zcdf, V, true_k, 0.1, tl)
+    dense, mapping = _first_appearance_ids(tw, V)
+    ul = lognormal_lengths(_rng(seed + 104729), unseen_docs, np.log(200.0), 1.2, 5, 20000)
+    uw, uoff = _draw(rng, topics, zcdf, V, true_k, 0.1, ul)
+    m = mapping[uw]
+    keep = m >= 0
+    doc_id = np.repeat(np.arange(unseen_docs), np.diff(uoff).astype(np.int64))
+    new_len = np.bincount(doc_id[keep], minlength=unseen_docs)
+    uoff2 = np.zeros(unseen_docs + 1, dtype=np.uint64)
+    np.cumsum(new_len, out=uoff2[1:])
+    return (dense.astype(np.uint32), toff, int((mapping >= 0).sum())), (m[keep].astype(np.uint32), uoff2)
 
 
 WORKLOADS = {
