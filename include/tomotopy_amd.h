@@ -105,11 +105,14 @@ int lda_set_global_step(lda_ctx* ctx, uint64_t step);
  *                              the PCGS partition schedule with `workers`
  *                              device workers, W rounds per sweep, private
  *                              n_k per worker (deterministic).
+ *   COPY_MERGE with workers = W in [2, 64]
+ *                              tomoto's copy_merge: W device workers, each
+ *                              with private copies of n_kw and n_k, merged
+ *                              by summing deltas (deterministic).
  *   PARTITION / DEFAULT / COPY_MERGE with workers == 0
- *                              device-wide: every resident wavefront is a
+ *                              device-wide: every resident workgroup is a
  *                              worker with a private n_k; n_kw is shared and
  *                              updated with exact int32 atomics.
- *   COPY_MERGE with workers >= 2  LDA_ERR_UNSUPPORTED.
  * lda_train runs `iterations` sweeps, each followed by the alpha fixed point
  * when global_step >= burn_in and (global_step+1) % optim_interval == 0. */
 int lda_train(lda_ctx* ctx, int iterations, int workers, int scheme);

@@ -384,6 +384,61 @@ void sweep_partition(Model& m, int W, int threads) {
         for (int k = 0; k < m.K; ++k) m.nk[k] += m.nkw[(size_t)v * m.K + k];
 }
 
+// One training sweep of the copy_merge scheme with W workers
+// (LDAModel.hpp:401-423, merge 555-570, distribute 586-595).  Every worker
+// starts from a private copy of n_kw and n_k; the documents are cut into
+// chStride = min(8W, D) strided tasks, task c visiting documents id*chStride + c
+// in forShuffled order.  tomoto hands tasks to whichever thread is free; here
+// task c belongs to worker c % W, which makes the schedule deterministic.
+// Merge: n_kw = old + sum_w (local_w - old), then n_k = rowwise sum of n_kw.
+void sweep_copy_merge(Model& m, int W, int threads) {
+    const uint32_t sweep = m.global_step;
+    const int64_t chStride = std::min<int64_t>((int64_t)W * 8, m.D);
+    std::vector<std::vector<int32_t>> nkw_loc(W, m.nkw);
+    std::vector<std::vector<int32_t>> nk_loc(W, m.nk);
+    auto work = [&](int w) {
+        std::vector<float> buf(64 * 32 + m.K + 64);
+        std::vector<uint32_t> order;
+        for (int64_t ch = w; ch < chStride; ch += W) {
+            size_t n = (size_t)((m.D + (chStride - 1) - ch) / chStride);
+            uint32_t sseed = philox_word(m.seed, (uint32_t)ch, 0xC0u, sweep, TAG_SHUFFLE);
+            shuffled_order(n, sseed, order);
+            for (size_t q = 0; q < n; ++q) {
+                int64_t d = (int64_t)order[q] * chStride + ch;
+                for (uint64_t g = m.off[d]; g < m.off[d + 1]; ++g) {
+                    uint32_t v = m.words[g];
+                    if (v >= (uint32_t)m.V) continue;
+                    int32_t* ndk = &m.ndk[(size_t)d * m.K];
+                    int32_t* col = &nkw_loc[w][(size_t)v * m.K];
+                    int32_t* nk = nk_loc[w].data();
+                    int zo = m.z[g];
+                    ndk[zo]--; col[zo]--; nk[zo]--;
+                    float u = token_uniform(m.seed, g, sweep, TAG_SWEEP);
+                    int zn = sample_one(m.K, m.scan_order, m.kpl, ndk, m.alpha.data(), col, m.eta, nk, m.veta, u, buf.data());
+                    m.z[g] = (uint16_t)zn;
+                    ndk[zn]++; col[zn]++; nk[zn]++;
+                }
+            }
+        }
+    };
+    int T = std::max(1, std::min(threads, W));
+    if (T == 1) for (int w = 0; w < W; ++w) work(w);
+    else {
+        std::vector<std::thread> th;
+        for (int t = 0; t < T; ++t) th.emplace_back([&, t] { for (int w = t; w < W; w += T) work(w); });
+        for (auto& t : th) t.join();
+    }
+    std::vector<int32_t> old = m.nkw;
+    for (size_t i = 0; i < m.nkw.size(); ++i) {
+        int32_t acc = old[i];
+        for (int w = 0; w < W; ++w) acc += nkw_loc[w][i] - old[i];
+        m.nkw[i] = acc;
+    }
+    std::fill(m.nk.begin(), m.nk.end(), 0);
+    for (int v = 0; v < m.V; ++v)
+        for (int k = 0; k < m.K; ++k) m.nk[k] += m.nkw[(size_t)v * m.K + k];
+}
+
 // optimizeParameters (LDAModel.hpp:221-265): 10 Minka fixed-point passes.
 // Terms are float digammaApprox as in tomoto; sums are accumulated in double
 // (tomoto sums floats in Eigen's order); dAlpha is tomoto's LUT digamma.
@@ -581,15 +636,22 @@ void orc_set_alpha(void* h, const float* alpha) { Model* m = (Model*)h; m->alpha
 void orc_get_alpha(void* h, float* alpha) { Model* m = (Model*)h; std::copy(m->alpha.begin(), m->alpha.end(), alpha); }
 uint32_t orc_global_step(void* h) { return ((Model*)h)->global_step; }
 
+void orc_train_scheme(void* h, int iterations, int workers, int threads, int scheme);
+
 // TopicModel::train (TopicModel.hpp:684-755) + trainOne (LDAModel.hpp:505-539):
 // per iteration one sweep (partition with W workers; W == 1 is `none`),
 // then alpha optimisation when globalStep >= burnIn and
 // (globalStep+1) % optimInterval == 0, then ++globalStep.
-void orc_train(void* h, int iterations, int workers, int threads) {
+void orc_train(void* h, int iterations, int workers, int threads) { orc_train_scheme(h, iterations, workers, threads, 3); }
+
+// scheme: 1 none, 2 copy_merge, 3 partition (tomotopy.ParallelScheme values)
+void orc_train_scheme(void* h, int iterations, int workers, int threads, int scheme) {
     Model* m = (Model*)h;
     int W = std::max(1, workers);
+    if (scheme == 1) W = 1;
     for (int it = 0; it < iterations; ++it) {
-        sweep_partition(*m, W, threads);
+        if (scheme == 2 && W > 1) sweep_copy_merge(*m, W, threads);
+        else sweep_partition(*m, W, threads);
         if (m->global_step >= m->burn_in && m->optim_interval && (m->global_step + 1) % m->optim_interval == 0)
             optimize_alpha(*m);
         ++m->global_step;

@@ -56,6 +56,7 @@ def load():
         "orc_get_alpha": (None, [_c_p, _c_p]),
         "orc_global_step": (u32, [_c_p]),
         "orc_train": (None, [_c_p, i32, i32, i32]),
+        "orc_train_scheme": (None, [_c_p, i32, i32, i32, i32]),
         "orc_optimize_alpha": (None, [_c_p]),
         "orc_get_state": (None, [_c_p, _c_p, _c_p, _c_p, _c_p]),
         "orc_loglik": (None, [_c_p, i32, _c_p, _c_p]),
@@ -190,8 +191,9 @@ class Model:
         load().orc_get_alpha(self._h, _p(a))
         return a
 
-    def train(self, iterations, workers=1, threads=1):
-        load().orc_train(self._h, iterations, workers, threads)
+    def train(self, iterations, workers=1, threads=1, scheme=3):
+        """scheme: 1 none, 2 copy_merge, 3 partition (tomotopy.ParallelScheme values)."""
+        load().orc_train_scheme(self._h, iterations, workers, threads, scheme)
 
     def optimize_alpha(self):
         load().orc_optimize_alpha(self._h)

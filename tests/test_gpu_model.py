@@ -1,0 +1,103 @@
+"""The LDAModel surface on the GPU, mirroring tomotopy's test/unit_test.py
+LDA cases (null_doc, train1/train4/train0, train0_without_optim,
+reproducibility, infer, uninit_doc, asymmetric alpha) on the reference's own
+sample.txt.  Iteration counts are reduced from the reference's 2000 to keep
+the suite short; the reference's cases are smoke tests plus one
+reproducibility assertion, which is kept exactly."""
+import os
+import warnings
+
+import numpy as np
+import pytest
+
+import tomotopy_amd as tp
+
+pytestmark = pytest.mark.gpu
+
+SAMPLE = os.path.join(os.path.dirname(__file__), "golden", "sample.txt")
+PARALLEL = [tp.ParallelScheme.COPY_MERGE, tp.ParallelScheme.PARTITION]
+
+
+def _lines():
+    with open(SAMPLE, encoding="utf-8") as f:
+        return [l.strip().split() for l in f]
+
+
+def _model(**kw):
+    mdl = tp.LDAModel(**kw)
+    for ch in _lines():
+        if len(ch) < 1:
+            continue
+        mdl.add_doc(ch)
+    return mdl
+
+
+@pytest.mark.parametrize("ps", PARALLEL)
+def test_null_doc(ps):
+    mdl = _model(tw=0, min_df=200, rm_top=200, k=40)
+    mdl.train(100, workers=1, parallel=ps)
+    assert mdl.num_vocabs == 0
+    assert len(mdl.docs[0].words) > 0
+    assert all(t == -1 for t in mdl.docs[0].topics)
+
+
+@pytest.mark.parametrize("ps", PARALLEL)
+@pytest.mark.parametrize("workers", [1, 4, 0])
+def test_train(ps, workers):
+    mdl = _model(tw=0, min_df=2, rm_top=2, k=40)
+    mdl.train(200, workers=workers, parallel=ps)
+    assert mdl.global_step == 200
+    assert -12 < mdl.ll_per_word < -5
+    assert sum(mdl.get_count_by_topics()) == mdl.num_words
+    mdl.summary(file=open(os.devnull, "w"))
+
+
+def test_train_without_optim_and_asymmetric_alpha():
+    mdl = _model(tw=0, min_df=2, rm_top=2, k=40, alpha=[0.1 * (i + 1) for i in range(40)])
+    mdl.optim_interval = 0
+    mdl.train(100, workers=0)
+    assert np.allclose(mdl.alpha, [0.1 * (i + 1) for i in range(40)], rtol=1e-6)
+
+
+def test_reproducibility():
+    results = []
+    for _ in range(3):
+        mdl = _model(tw=0, min_df=2, rm_top=2, seed=42, k=40)
+        mdl.train(300, workers=1)
+        results.append([mdl.get_topic_words(k) for k in range(mdl.k)])
+    assert results[0] == results[1]
+    assert results[0] == results[2]
+
+
+def test_infer_and_uninit_doc():
+    lines = _lines()
+    mdl = tp.LDAModel(tw=0, min_df=2, rm_top=2, k=40)
+    for ch in lines[5:]:
+        mdl.add_doc(ch)
+    mdl.train(50)
+    unseen = [mdl.make_doc(ch) for ch in lines[:5]]
+    with warnings.catch_warnings():
+        warnings.simplefilter("error")
+        with pytest.raises(Warning):
+            unseen[0].get_topics()
+    dists, ll = mdl.infer(unseen)
+    assert len(dists) == 5 and len(ll) == 5
+    for d in dists:
+        assert abs(sum(d) - 1.0) < 1e-4
+    with warnings.catch_warnings():
+        warnings.simplefilter("error")
+        unseen[0].get_topics()
+        unseen[0].get_topic_dist()
+    single, ll1 = mdl.infer(unseen[1])
+    assert np.allclose(single, dists[1]) and ll1 == ll[1]
+    with pytest.raises(RuntimeError):
+        mdl.add_doc(["new", "words"])
+
+
+def test_copy_train():
+    mdl = _model(tw=0, min_df=2, rm_top=2, k=10)
+    mdl.train(50)
+    new = mdl.copy()
+    del mdl
+    new.train(50)
+    assert new.global_step == 100

@@ -77,6 +77,25 @@ def test_deterministic_schedule_bit_exact(workers):
         assert not ctx.check_invariants().any()
 
 
+@pytest.mark.parametrize("workers", [2, 4])
+def test_copy_merge_schedule_bit_exact(workers):
+    """copy_merge (private n_kw copies, merged by summing deltas): z and all counts equal the oracle."""
+    ws, offs, V = _corpus("plain_k20", 70)
+    K = 20
+    ctx = _native.Context(K, V, 0.1, 0.01, seed=13)
+    ctx.upload_corpus(ws, offs)
+    ctx.set_optim(0, 0)
+    m = O.Model(K, V, 0.1, 0.01, 13, scan_order=1)
+    m.set_optim(0, 0)
+    m.set_corpus(ws, offs)
+    for it in range(3):
+        print("copy_merge sweep", it, "workers", workers, flush=True)
+        ctx.train(1, workers=workers, scheme=2)
+        m.train(1, workers=workers, scheme=2)
+        _compare_state(ctx, m)
+        assert not ctx.check_invariants().any()
+
+
 def test_training_with_alpha_optimisation_matches_oracle():
     ws, offs, V = _corpus("plain_k20", 60)
     K = 20

@@ -157,7 +157,7 @@ struct lda_ctx {
     bool has_corpus = false;
 
     DevBuf<float> d_alpha, d_alpha_cdf;
-    DevBuf<int32_t> d_nkw, d_nk, d_nk_next, d_ndk, d_doclen, d_nk_priv;
+    DevBuf<int32_t> d_nkw, d_nk, d_nk_next, d_ndk, d_doclen, d_nk_priv, d_nkw_loc;
     DevBuf<uint32_t> d_words, d_order, d_counter;
     DevBuf<uint16_t> d_z;
     DevBuf<uint64_t> d_off, d_coff;
@@ -261,9 +261,11 @@ lda::SweepParams base_params(lda_ctx* c) {
 
 // Scheme resolution as in TopicModel::train (TopicModel.hpp:684-690):
 // workers == 1 or NONE -> one worker; the partition worker count is capped at
-// max((V+3)/4, 1) (LDAModel.hpp:491-497); workers == 0 selects the
-// device-wide schedule.  Returns the worker count (0 = device-wide).
-int resolve_workers(lda_ctx* c, int workers, int scheme, int* W_out) {
+// max((V+3)/4, 1) and the copy_merge one at max((D+1)/2, 1)
+// (LDAModel.hpp:491-503); workers == 0 selects the device-wide schedule.
+// Returns the worker count (0 = device-wide) and whether copy_merge applies.
+int resolve_workers(lda_ctx* c, int workers, int scheme, int* W_out, bool* copy_merge) {
+    *copy_merge = false;
     if (workers < 0) return fail(LDA_ERR_INVALID_ARGUMENT, "workers must be >= 0");
     if (scheme < LDA_PS_DEFAULT || scheme > LDA_PS_PARTITION) return fail(LDA_ERR_INVALID_ARGUMENT, "invalid ParallelScheme");
     if (scheme == LDA_PS_NONE || workers == 1) {
@@ -274,17 +276,50 @@ int resolve_workers(lda_ctx* c, int workers, int scheme, int* W_out) {
         *W_out = 0;
         return LDA_OK;
     }
-    if (scheme == LDA_PS_COPY_MERGE)
-        return fail(LDA_ERR_UNSUPPORTED, "COPY_MERGE with an explicit worker count is not supported on the device; use workers=0 or PARTITION");
+    if (scheme == LDA_PS_COPY_MERGE) {
+        const int64_t cap_cm = std::max<int64_t>((c->D + 1) / 2, 1);
+        int W = (int)std::min<int64_t>(std::min<int64_t>(workers, cap_cm), 64);
+        if (W <= 1) { *W_out = 1; return LDA_OK; }
+        const double bytes = (double)W * c->V * c->Kp * 4.0;
+        if (bytes > 64e9) return fail(LDA_ERR_OUT_OF_MEMORY, "copy_merge with %d workers needs %.1f GB of n_kw copies", W, bytes / 1e9);
+        *W_out = W;
+        *copy_merge = true;
+        return LDA_OK;
+    }
     int cap = std::max((c->V + 3) / 4, 1);
     int W = std::min(std::min(workers, cap), 1024);
     *W_out = W;
     return LDA_OK;
 }
 
-int sweep_once(lda_ctx* c, int W) {
+int sweep_once(lda_ctx* c, int W, bool copy_merge) {
     lda::SweepParams p = base_params(c);
     const hipStream_t s = c->stream;
+    if (copy_merge) {
+        const int64_t n = (int64_t)c->V * c->Kp;
+        HIP_TRY(c->d_nkw_loc.alloc((size_t)W * std::max<int64_t>(n, 1)));
+        HIP_TRY(c->d_nk_priv.alloc((size_t)W * c->Kp));
+        c->part_W = 0;  // d_nk_priv now holds copy_merge state
+        HIP_TRY(lda::launch_replicate(c->d_nkw_loc.p, c->d_nkw.p, n, W, s));
+        HIP_TRY(lda::launch_replicate(c->d_nk_priv.p, c->d_nk.p, c->Kp, W, s));
+        p.nkw = c->d_nkw_loc.p;
+        p.nk_in = c->d_nk_priv.p;
+        p.nk_out = c->d_nk_priv.p;
+        p.W = W;
+        HIP_TRY(hipEventRecord(c->ev0, s));
+        HIP_TRY(lda::launch_sweep_copy(c->kpl, p, s));
+        HIP_TRY(hipEventRecord(c->ev1, s));
+        HIP_TRY(lda::launch_merge_copies(c->d_nkw.p, c->d_nkw_loc.p, n, W, s));
+        HIP_TRY(lda::launch_rowsum_nkw(c->d_nkw.p, c->V, c->Kp, c->d_nk.p, s));
+        HIP_TRY(hipEventSynchronize(c->ev1));
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        c->stat_ms += ms;
+        c->stat_launches += 1;
+        HIP_TRY(hipStreamSynchronize(s));
+        c->stat_tokens += (double)c->n_valid;
+        return LDA_OK;
+    }
     if (W == 0) {
         HIP_TRY(lda::launch_copy_i32(c->d_nk_next.p, c->d_nk.p, c->Kp, s));
         HIP_TRY(lda::launch_fill_u32(c->d_counter.p, 0u, 1, s));
@@ -440,7 +475,7 @@ int lda_ctx_destroy(lda_ctx* c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     c->d_alpha.release(); c->d_alpha_cdf.release(); c->d_nkw.release(); c->d_nk.release(); c->d_nk_next.release();
-    c->d_ndk.release(); c->d_doclen.release(); c->d_nk_priv.release(); c->d_words.release(); c->d_order.release();
+    c->d_ndk.release(); c->d_doclen.release(); c->d_nk_priv.release(); c->d_nkw_loc.release(); c->d_words.release(); c->d_order.release();
     c->d_counter.release(); c->d_z.release(); c->d_off.release(); c->d_coff.release(); c->d_part.release();
     if (c->infer_exec) (void)hipGraphExecDestroy(c->infer_exec);
     c->i_words.release(); c->i_counter.release(); c->i_jobs.release(); c->i_params.release(); c->i_z.release();
@@ -569,11 +604,12 @@ int lda_sweep(lda_ctx* c, int workers, int scheme) {
     if (!c) return fail(LDA_ERR_INVALID_ARGUMENT, "ctx is NULL");
     if (!c->has_corpus) return fail(LDA_ERR_STATE, "no corpus uploaded");
     int W;
-    int rc = resolve_workers(c, workers, scheme, &W);
+    bool cm;
+    int rc = resolve_workers(c, workers, scheme, &W, &cm);
     if (rc) return rc;
     HIP_TRY(hipSetDevice(c->device));
     c->stat_ms = c->stat_launches = c->stat_tokens = 0.0;
-    return sweep_once(c, W);
+    return sweep_once(c, W, cm);
 }
 
 int lda_train(lda_ctx* c, int iterations, int workers, int scheme) {
@@ -581,7 +617,8 @@ int lda_train(lda_ctx* c, int iterations, int workers, int scheme) {
     if (iterations < 0) return fail(LDA_ERR_INVALID_ARGUMENT, "iterations must be >= 0");
     if (!c->has_corpus) return fail(LDA_ERR_STATE, "no corpus uploaded");
     int W;
-    int rc = resolve_workers(c, workers, scheme, &W);
+    bool cm;
+    int rc = resolve_workers(c, workers, scheme, &W, &cm);
     if (rc) return rc;
     HIP_TRY(hipSetDevice(c->device));
     c->stat_ms = c->stat_launches = c->stat_tokens = 0.0;
@@ -592,7 +629,7 @@ int lda_train(lda_ctx* c, int iterations, int workers, int scheme) {
     for (int it = 0; it < iterations; ++it) {
         if (fail_at >= 0 && (long long)c->global_step == fail_at)
             return fail(LDA_ERR_TRAINING, "training error forced at step %llu", (unsigned long long)c->global_step);
-        rc = sweep_once(c, W);
+        rc = sweep_once(c, W, cm);
         if (rc) return rc;
         // trainOne (LDAModel.hpp:529-532)
         if (c->global_step >= c->burn_in && c->optim_interval && (c->global_step + 1) % c->optim_interval == 0) {

@@ -448,6 +448,75 @@ __global__ void __launch_bounds__(64) sweep_det_kernel(SweepParams p) {
     store_kvec<KPL>(nkp, lane, mine);
 }
 
+// copy_merge schedule (LDAModel.hpp:401-423): wavefront w is worker w with a
+// private copy of n_kw (nkw + w*V*Kp) and of n_k (nk_out + w*Kp); it runs the
+// tasks c = w, w+W, ... of chStride = min(8W, D) strided tasks, task c
+// visiting documents id*chStride + c in forShuffled order.  Workers share no
+// counts, so the result does not depend on their timing.
+template <int KPL>
+__global__ void __launch_bounds__(64) sweep_copy_kernel(SweepParams p) {
+    const int lane = lane_id();
+    const int w = blockIdx.x;
+    const WaveLds S = wave_lds<KPL>();
+    int nvalid, lastLane;
+    lane_limits<KPL>(p.K, lane, nvalid, lastLane);
+    fvec<KPL> al;
+    load_kvec_f<KPL>(p.alpha, lane, al);
+    int32_t* nkp = p.nk_out + (int64_t)w * p.Kp;
+    set_nk<KPL>(S, nkp, lane, p.veta);
+    SweepParams q = p;
+    q.nkw = p.nkw + (int64_t)w * p.V * p.Kp;
+    const int W = p.W;
+    const int64_t chStride = (int64_t)W * 8 < p.D ? (int64_t)W * 8 : p.D;
+    for (int64_t ch = w; ch < chStride; ch += W) {
+        const uint64_t N = (uint64_t)((p.D + (chStride - 1) - ch) / chStride);
+        const uint32_t sseed = philox_x0(p.seed_lo, p.seed_hi, (uint32_t)ch, 0xC0u, p.sweep, TAG_SHUFFLE);
+        uint64_t P = 1;
+        if (N) {
+            P = kShufflePrimes[sseed & 0xF];
+            if (N % P == 0) {
+                P = kShufflePrimes[(sseed + 1) & 0xF];
+                if (N % P == 0) P = kShufflePrimes[(sseed + 2) & 0xF];
+                if (N % P == 0) P = kShufflePrimes[(sseed + 3) & 0xF];
+            }
+            P %= N;
+        }
+        for (uint64_t i = 0; i < N; ++i) {
+            const int64_t d = (int64_t)(((i + (uint64_t)sseed) * P) % N) * chStride + ch;
+            const uint64_t b = p.off[d], e = p.off[d + 1];
+            if (b == e) continue;
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+            int32_t* ndk_row = p.ndk + d * p.Kp;
+            ivec<KPL> nd;
+            load_kvec<KPL>(ndk_row, lane, nd);
+            set_ndk<KPL>(S, al, nd, lane);
+            sample_range<KPL, MODE_DET>(S, al, q, lane, b, e, nvalid, lastLane);
+            nd = get_ndk<KPL>(S, al, lane);
+            store_kvec<KPL>(ndk_row, lane, nd);
+        }
+    }
+    ivec<KPL> mine;
+    lds_load<KPL>(S.nk, lane, mine);
+    store_kvec<KPL>(nkp, lane, mine);
+}
+
+// mergeState<copy_merge> (LDAModel.hpp:555-570): n_kw = old + sum_w (local_w - old).
+__global__ void merge_copies_kernel(int32_t* __restrict__ nkw, const int32_t* __restrict__ loc, int64_t n, int W) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t old = nkw[i];
+        int32_t acc = old;
+        for (int w = 0; w < W; ++w) acc += loc[(int64_t)w * n + i] - old;
+        nkw[i] = acc;
+    }
+}
+
+__global__ void replicate_kernel(int32_t* __restrict__ dst, const int32_t* __restrict__ src, int64_t n, int W) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t x = src[i];
+        for (int w = 0; w < W; ++w) dst[(int64_t)w * n + i] = x;
+    }
+}
+
 // ---------------------------------------------------------------- init ----
 // Initial topics z ~ Discrete(alpha) (LDAModel.hpp:754-762): r = u*cdf[K-1],
 // first k with r < cdf[k] (binary search on the non-decreasing cdf).
@@ -962,6 +1031,27 @@ hipError_t launch_build_counts(int kpl, const uint32_t* words, const uint16_t* z
 hipError_t launch_sweep_device(int kpl, const SweepParams& p, int grid, hipStream_t s) {
     LDA_DISPATCH_KPL(kpl, hipLaunchKernelGGL(sweep_device_kernel<KPL>, dim3(grid), dim3(64 * kSweepWaves),
                                              wave_lds_bytes<KPL>(kSweepWaves), s, p));
+    return hipGetLastError();
+}
+
+hipError_t launch_sweep_copy(int kpl, const SweepParams& p, hipStream_t s) {
+    LDA_DISPATCH_KPL(kpl, hipLaunchKernelGGL(sweep_copy_kernel<KPL>, dim3(p.W), dim3(64), wave_lds_bytes<KPL>(), s, p));
+    return hipGetLastError();
+}
+
+hipError_t launch_merge_copies(int32_t* nkw, const int32_t* loc, int64_t n, int W, hipStream_t s) {
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(merge_copies_kernel, dim3((unsigned)blocks), dim3(256), 0, s, nkw, loc, n, W);
+    return hipGetLastError();
+}
+
+hipError_t launch_replicate(int32_t* dst, const int32_t* src, int64_t n, int W, hipStream_t s) {
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(replicate_kernel, dim3((unsigned)blocks), dim3(256), 0, s, dst, src, n, W);
     return hipGetLastError();
 }
 

@@ -79,6 +79,10 @@ hipError_t launch_build_counts(int kpl, const uint32_t* words, const uint16_t* z
                                int32_t* ndk, int32_t* nkw, int32_t* nk, int Kp, int V, hipStream_t s);
 hipError_t launch_sweep_device(int kpl, const SweepParams& p, int grid, hipStream_t s);
 hipError_t launch_sweep_det(int kpl, const SweepParams& p, hipStream_t s);
+// copy_merge: W private copies of n_kw (W x V x Kp) and of n_k (W x Kp)
+hipError_t launch_sweep_copy(int kpl, const SweepParams& p, hipStream_t s);
+hipError_t launch_merge_copies(int32_t* nkw, const int32_t* loc, int64_t n, int W, hipStream_t s);
+hipError_t launch_replicate(int32_t* dst, const int32_t* src, int64_t n, int W, hipStream_t s);
 int sweep_device_blocks_per_cu(int kpl);
 hipError_t launch_copy_i32(int32_t* dst, const int32_t* src, int64_t n, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* dst, uint32_t v, int64_t n, hipStream_t s);
