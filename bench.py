@@ -123,6 +123,122 @@ def infer_bench(plain_sample=300, iterations=100):
     return out
 
 
+def _percentile(x, q):
+    return float(np.percentile(np.asarray(x, dtype=np.float64), q)) if len(x) else None
+
+
+def serve_bench(iterations=50, qps_list=(25, 50, 100, 200, 400), seconds=4.0, plain_requests=60, seed=2024):
+    """Serving loop on a device-resident K=500, V=100,000 model: Poisson request
+    arrivals at each QPS (requests of 1..64 docs, lengths LogNormal(ln 150, 1)
+    clipped to [5, 10000], 50 iterations); latency is submit to results on the
+    host, measured inside the server.  Baselines on the same requests, served
+    one at a time back to back: plain per-request GPU infer (one lda_infer
+    call per request) and plain per-document GPU infer (one launch and one
+    synchronisation per document)."""
+    import threading
+    from tomotopy_amd import _native, corpus, synth
+    t = time.time()
+    n_pool = 600
+    (tw, toff, nw), reqs_raw = synth.serve_workload(n_pool)
+    ws, offs, V, order = corpus.prepare_arrays(tw, toff, nw)
+    reqs = [corpus.csr_from_arrays(w, o, order) for w, o in reqs_raw]
+    log("serving workload: %d training docs, %d pooled requests (%d docs) (%.1fs)"
+        % (len(offs) - 1, len(reqs), sum(len(o) - 1 for _, o in reqs), time.time() - t))
+    K = 500
+    ctx = _native.Context(K, V, 0.1, 0.01, seed=55)
+    ctx.upload_corpus(ws, offs)
+    ctx.train(50, workers=0)
+    out = {"workload": "serve_k500", "K": K, "V": V, "iterations": iterations, "levels": []}
+    srv = _native.Server(ctx, hbm_budget_bytes=8 << 30, iterations=iterations, step_iters=iterations,
+                         step_tokens=20000, infer_seed=seed)
+    try:
+        for i in range(4):  # warm
+            srv.wait(srv.submit(*reqs[i]))
+        rng = np.random.default_rng(seed)
+        ll_sum, tok_sum = 0.0, 0
+        for qps in qps_list:
+            n = max(20, min(len(reqs), int(qps * seconds)))
+            gaps = rng.exponential(1.0 / qps, size=n)
+            pick = rng.integers(0, len(reqs), size=n)
+            ids = [None] * n
+            t_start = time.perf_counter()
+
+            def client():
+                due = t_start
+                for j in range(n):
+                    due += gaps[j]
+                    dt = due - time.perf_counter()
+                    if dt > 0:
+                        time.sleep(dt)
+                    ids[j] = srv.submit(*reqs[pick[j]])
+
+            th = threading.Thread(target=client)
+            th.start()
+            lat = []
+            docs = 0
+            for j in range(n):
+                while ids[j] is None:
+                    time.sleep(0.0005)
+                _, ll, ms = srv.wait(ids[j])
+                lat.append(ms)
+                docs += len(ll)
+                ll_sum += float(ll.sum())
+                tok_sum += int(np.count_nonzero(reqs[pick[j]][0] < V))
+            t_end = time.perf_counter()
+            th.join()
+            lvl = {"qps": qps, "requests": n, "docs": docs, "docs_per_s": docs / (t_end - t_start),
+                   "p50_ms": _percentile(lat, 50), "p99_ms": _percentile(lat, 99), "max_ms": max(lat)}
+            out["levels"].append(lvl)
+            log("serve qps %4d: %5d req, %7.0f docs/s, p50 %.1f ms, p99 %.1f ms"
+                % (qps, n, lvl["docs_per_s"], lvl["p50_ms"], lvl["p99_ms"]))
+        # saturation: every pooled request submitted at once
+        t0 = time.perf_counter()
+        ids = [srv.submit(*r) for r in reqs]
+        lat = [srv.wait(i)[2] for i in ids]
+        dt = time.perf_counter() - t0
+        ndocs = sum(len(o) - 1 for _, o in reqs)
+        out["saturation"] = {"requests": len(reqs), "docs": ndocs, "docs_per_s": ndocs / dt,
+                             "p50_ms": _percentile(lat, 50), "p99_ms": _percentile(lat, 99)}
+        log("serve saturation: %.0f docs/s over %d requests" % (ndocs / dt, len(reqs)))
+        st = srv.stats()
+        out["server_steps"] = st["steps"]
+        out["ll_per_word"] = ll_sum / max(tok_sum, 1)
+    finally:
+        srv.stop()
+    # baselines: the same requests served one at a time
+    sub = reqs[:plain_requests]
+    lat_req = []
+    t0 = time.perf_counter()
+    ll_sum, tok_sum = 0.0, 0
+    for w, o in sub:
+        t1 = time.perf_counter()
+        _, ll, _ = ctx.infer(w, o, iterations, infer_seed=seed, mode=0)
+        lat_req.append((time.perf_counter() - t1) * 1e3)
+        ll_sum += float(ll.sum())
+        tok_sum += int(np.count_nonzero(w < V))
+    dt = time.perf_counter() - t0
+    ndocs = sum(len(o) - 1 for _, o in sub)
+    out["plain_per_request"] = {"requests": len(sub), "docs": ndocs, "docs_per_s": ndocs / dt,
+                                "p50_ms": _percentile(lat_req, 50), "p99_ms": _percentile(lat_req, 99),
+                                "ll_per_word": ll_sum / max(tok_sum, 1)}
+    sub2 = reqs[:max(1, plain_requests // 10)]
+    t0 = time.perf_counter()
+    for w, o in sub2:
+        ctx.infer(w, o, iterations, infer_seed=seed, mode=1)
+    dt = time.perf_counter() - t0
+    nd2 = sum(len(o) - 1 for _, o in sub2)
+    out["plain_per_document"] = {"requests": len(sub2), "docs": nd2, "docs_per_s": nd2 / dt}
+    best = max(lv["docs_per_s"] for lv in out["levels"])
+    ok = [lv for lv in out["levels"] if lv["p99_ms"] is not None and lv["p99_ms"] <= 50.0]
+    out["best_docs_per_s"] = max(best, out["saturation"]["docs_per_s"])
+    out["speedup_vs_plain_per_request"] = out["best_docs_per_s"] / out["plain_per_request"]["docs_per_s"]
+    out["max_qps_with_p99_le_50ms"] = max((lv["qps"] for lv in ok), default=None)
+    log("plain per-request %.0f docs/s (p50 %.1f ms), plain per-document %.1f docs/s; serve/plain %.1fx"
+        % (out["plain_per_request"]["docs_per_s"], out["plain_per_request"]["p50_ms"],
+           out["plain_per_document"]["docs_per_s"], out["speedup_vs_plain_per_request"]))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -132,11 +248,18 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--corpus-cache", default=None, help="npz file to reuse the generated corpus")
     ap.add_argument("--no-infer", action="store_true", help="skip the fold-in measurement")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
+    ap.add_argument("--only", choices=["train", "infer", "serve"], default=None,
+                    help="run one leg only (the JSON line then carries that leg's numbers)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
     rank = int(os.environ.get("RANK", "0"))
+    if args.only in ("infer", "serve"):
+        res = infer_bench() if args.only == "infer" else serve_bench()
+        print(json.dumps({"metric": METRIC, "leg": args.only, args.only: res}), flush=True)
+        return
 
     from tomotopy_amd import _native
     K, alpha, eta = 1000, 0.05, 0.01
@@ -188,9 +311,13 @@ def main():
     }
     if rank == 0 and not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(ws, offs, V, K, alpha, eta)
-    if not args.no_infer:
+    if args.only == "train":
+        pass
+    elif not args.no_infer:
         del ctx
         out["infer"] = infer_bench()
+    if not args.no_serve and args.only is None:
+        out["serve"] = serve_bench()
     print(json.dumps(out), flush=True)
 
 

@@ -146,6 +146,37 @@ int lda_sample_frozen(lda_ctx* ctx, int64_t n, const int64_t* doc, const uint32_
 int lda_infer(lda_ctx* ctx, const uint32_t* words, const uint64_t* doc_off, int64_t D, int iterations,
               uint64_t infer_seed, int mode, float* theta, double* ll, uint16_t* z_out);
 
+/* ---- Serving: continuous-batching fold-in over the context's frozen model ----
+ * tomotopy has no server; these calls serve LDAModel.infer (py_LDA.cpp:345-411,
+ * LDAModel::_infer<together=false>, LDAModel.hpp:920-953) to a stream of
+ * requests.  A server keeps the context's n_kw / n_k on the device, admits
+ * requests in FIFO order while their tokens fit a token arena and their
+ * documents the active-document limit (both sized from hbm_budget_bytes), and
+ * runs one persistent fold-in launch per step over every admitted document.
+ * Per step a document advances by min(step_iters, max(1, step_tokens / length))
+ * iterations; requests that arrive during a step join the next one.  Results
+ * equal lda_infer(mode 0) with the same infer_seed, bit for bit.
+ * The context must not be trained, re-uploaded or destroyed while a server on
+ * it runs.  submit / wait may be called from any thread; lda_serve_stop must
+ * not overlap a wait, and drops requests that were not waited for. */
+typedef struct lda_server lda_server;
+
+int lda_serve_start(lda_ctx* ctx, uint64_t hbm_budget_bytes, int iterations, int step_iters, int64_t step_tokens,
+                    uint64_t infer_seed, lda_server** out);
+/* Copies the request (CSR as for lda_infer); returns its id at once. */
+int lda_serve_submit(lda_server* server, const uint32_t* words, const uint64_t* doc_off, int64_t D,
+                     uint64_t* request_id);
+/* Waits for a request (timeout_ms < 0: no limit; on timeout LDA_ERR_STATE and
+ * the request stays pending), copies theta (D x K) and ll (D), and the
+ * latency from submit to results-on-host in ms; then forgets the request. */
+int lda_serve_wait(lda_server* server, uint64_t request_id, int timeout_ms, float* theta, double* ll,
+                   double* latency_ms);
+/* stats: [0] steps, [1] device-step ms, [2] documents done, [3] requests done,
+ * [4] token-iterations done, [5] active documents, [6] queued requests,
+ * [7] token arena capacity. */
+int lda_serve_stats(lda_server* server, double stats[8]);
+int lda_serve_stop(lda_server* server);
+
 /* Timing of the last lda_train / lda_sweep / lda_infer call, measured with
  * HIP events on the context's stream around the hot kernels only:
  * stats[0] total kernel ms, stats[1] launches, stats[2] tokens sampled. */

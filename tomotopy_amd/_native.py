@@ -55,6 +55,11 @@ _SIGS = {
     "lda_sample_frozen": [_c_p, _i64, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p],
     "lda_infer": [_c_p, _c_p, _c_p, _i64, _i32, _u64, _i32, _c_p, _c_p, _c_p],
     "lda_kernel_stats": [_c_p, _c_p],
+    "lda_serve_start": [_c_p, _u64, _i32, _i32, _i64, _u64, _c_p],
+    "lda_serve_submit": [_c_p, _c_p, _c_p, _i64, _c_p],
+    "lda_serve_wait": [_c_p, _u64, _i32, _c_p, _c_p, _c_p],
+    "lda_serve_stats": [_c_p, _c_p],
+    "lda_serve_stop": [_c_p],
 }
 
 _lib = None
@@ -236,3 +241,69 @@ class Context:
         s = np.zeros(4, dtype=np.float64)
         _check(load().lda_kernel_stats(self._h, _ptr(s)))
         return {"ms": float(s[0]), "launches": int(s[1]), "tokens": float(s[2])}
+
+
+class Server:
+    """Continuous-batching fold-in server on a Context's frozen model.
+
+    submit() copies a request and returns its id at once; wait() blocks (the
+    GIL is released inside the C call) and returns (theta, ll, latency_ms).
+    Results equal Context.infer(mode=0) with the same infer_seed bit for bit.
+    The context must not be trained or changed while the server runs.
+    """
+
+    def __init__(self, ctx, hbm_budget_bytes=1 << 30, iterations=50, step_iters=50, step_tokens=1 << 30,
+                 infer_seed=0):
+        self._ctx = ctx  # keeps the model alive while the server runs
+        self.k = ctx.k
+        self.iterations = int(iterations)
+        h = ctypes.c_void_p()
+        _check(load().lda_serve_start(ctx._h, int(hbm_budget_bytes), int(iterations), int(step_iters),
+                                      int(step_tokens), int(infer_seed) & (2**64 - 1), ctypes.byref(h)))
+        self._h = h
+        self._ndocs = {}
+
+    def submit(self, words, doc_off):
+        words = np.ascontiguousarray(words, dtype=np.uint32)
+        doc_off = np.ascontiguousarray(doc_off, dtype=np.uint64)
+        D = len(doc_off) - 1
+        rid = ctypes.c_uint64(0)
+        _check(load().lda_serve_submit(self._h, _ptr(words), _ptr(doc_off), D, ctypes.byref(rid)))
+        self._ndocs[rid.value] = D
+        return rid.value
+
+    def wait(self, request_id, timeout_ms=-1):
+        if request_id not in self._ndocs:
+            raise ValueError("unknown request %r" % (request_id,))
+        D = self._ndocs[request_id]
+        theta = np.zeros((D, self.k), dtype=np.float32)
+        ll = np.zeros(D, dtype=np.float64)
+        lat = ctypes.c_double(0.0)
+        _check(load().lda_serve_wait(self._h, int(request_id), int(timeout_ms), _ptr(theta), _ptr(ll),
+                                     ctypes.byref(lat)))
+        del self._ndocs[request_id]
+        return theta, ll, lat.value
+
+    def stats(self):
+        s = np.zeros(8, dtype=np.float64)
+        _check(load().lda_serve_stats(self._h, _ptr(s)))
+        keys = ("steps", "busy_ms", "docs_done", "requests_done", "token_iters", "active_docs", "queued_requests",
+                "arena_tokens")
+        return dict(zip(keys, (float(x) for x in s)))
+
+    def stop(self):
+        if getattr(self, "_h", None) and self._h.value:
+            load().lda_serve_stop(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.stop()
+
+    def __del__(self):
+        try:
+            self.stop()
+        except Exception:
+            pass

@@ -38,7 +38,7 @@ def hipcc_path():
 def build(verbose=False):
     os.makedirs(OUT_DIR, exist_ok=True)
     hipcc = hipcc_path()
-    srcs = [os.path.join(CSRC, f) for f in ("lda_device.hip", "lda_capi.cpp")]
+    srcs = [os.path.join(CSRC, f) for f in ("lda_device.hip", "lda_capi.cpp", "lda_serve.cpp")]
     objs = []
     for src in srcs:
         obj = os.path.join(OUT_DIR, os.path.basename(src) + ".o")

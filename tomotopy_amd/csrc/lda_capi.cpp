@@ -19,10 +19,9 @@
 #include <string>
 #include <vector>
 
-#include "../../include/tomotopy_amd.h"
-#include "lda_internal.h"
+#include "lda_host.h"
 
-namespace {
+namespace lda_host {
 
 thread_local std::string g_err;
 
@@ -36,33 +35,14 @@ int fail(int code, const char* fmt, ...) {
     return code;
 }
 
-#define HIP_TRY(expr)                                                                                  \
-    do {                                                                                               \
-        hipError_t e_ = (expr);                                                                        \
-        if (e_ != hipSuccess)                                                                          \
-            return fail(e_ == hipErrorOutOfMemory ? LDA_ERR_OUT_OF_MEMORY : LDA_ERR_DEVICE, "%s: %s (%s:%d)", \
-                        #expr, hipGetErrorString(e_), __FILE__, __LINE__);                             \
-    } while (0)
+}  // namespace lda_host
 
-template <typename T>
-struct DevBuf {
-    T* p = nullptr;
-    size_t n = 0;
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-    hipError_t alloc(size_t count) {
-        if (count <= n && p) return hipSuccess;
-        release();
-        size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
-        hipError_t e = hipMalloc((void**)&p, bytes);
-        if (e == hipSuccess) n = count;
-        else p = nullptr;
-        return e;
-    }
-};
+using lda_host::fail;
+using lda_host::kMaxAlpha;
+using lda_host::kMaxDocTokens;
+
+namespace {
+
 
 // tomoto's float LUT digamma (src/Utils/LUT.hpp:10-79, math.h:134-154), used
 // for dAlpha in the alpha fixed point (LDAModel.hpp:224).  Grid values are a
@@ -122,11 +102,6 @@ const DigammaLut& digamma_lut() {
     return t;
 }
 
-// The sampler keeps float(n_dk) + alpha_k and recovers n_dk by rounding, which
-// is exact while n_dk + alpha_k < 2^21: documents and alpha are bounded here.
-constexpr uint64_t kMaxDocTokens = 1u << 20;
-constexpr float kMaxAlpha = (float)(1u << 20);
-
 int kpl_for(int K) {
     int kpl = 1;
     while (64 * kpl < K) kpl <<= 1;
@@ -135,53 +110,6 @@ int kpl_for(int K) {
 
 }  // namespace
 
-struct lda_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    int K = 0, V = 0, kpl = 1, Kp = 64;
-    float eta = 0.01f, veta = 0.0f;
-    int tw = LDA_TW_ONE;
-    uint64_t seed = 0;
-    std::vector<float> alpha;  // natural order
-    uint64_t global_step = 0;
-    uint32_t optim_interval = 10, burn_in = 0;
-    int num_cu = 1;
-
-    // corpus
-    int64_t D = 0;
-    uint64_t N = 0;
-    uint64_t n_valid = 0;
-    std::vector<uint64_t> off;
-    std::vector<uint32_t> words;
-    bool has_corpus = false;
-
-    DevBuf<float> d_alpha, d_alpha_cdf;
-    DevBuf<int32_t> d_nkw, d_nk, d_nk_next, d_ndk, d_doclen, d_nk_priv, d_nkw_loc;
-    DevBuf<uint32_t> d_words, d_order, d_counter;
-    DevBuf<uint16_t> d_z;
-    DevBuf<uint64_t> d_off, d_coff;
-    DevBuf<double> d_part;
-    int part_W = 0;
-
-    // fold-in buffers
-    DevBuf<uint32_t> i_words, i_counter;
-    DevBuf<lda::InferJob> i_jobs;
-    DevBuf<lda::InferParams> i_params;
-    DevBuf<uint16_t> i_z;
-    DevBuf<float> i_theta;
-    DevBuf<double> i_ll;
-    // batched fold-in graph: counter reset + persistent kernel reading i_params
-    hipGraphExec_t infer_exec = nullptr;
-    int infer_exec_grid = 0;
-
-    // kernel timing of the last call
-    double stat_ms = 0.0;
-    double stat_launches = 0.0;
-    double stat_tokens = 0.0;
-
-    int pos(int k) const { return lda::kvec_pos(kpl, k); }
-};
 
 namespace {
 
@@ -407,7 +335,7 @@ int optimize_alpha(lda_ctx* c) {
 // ============================================================== C ABI ======
 extern "C" {
 
-const char* lda_last_error(void) { return g_err.c_str(); }
+const char* lda_last_error(void) { return lda_host::g_err.c_str(); }
 
 int lda_version(void) { return 1; }
 

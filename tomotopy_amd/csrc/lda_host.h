@@ -1,0 +1,102 @@
+// Host-core internals shared by the C-ABI translation units (lda_capi.cpp,
+// lda_serve.cpp): error reporting, device buffers and the model context.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/tomotopy_amd.h"
+#include "lda_internal.h"
+
+namespace lda_host {
+
+// Sets the thread-local message returned by lda_last_error() and returns code.
+int fail(int code, const char* fmt, ...);
+
+#define HIP_TRY(expr)                                                                                  \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess)                                                                          \
+            return ::lda_host::fail(e_ == hipErrorOutOfMemory ? LDA_ERR_OUT_OF_MEMORY : LDA_ERR_DEVICE, \
+                                    "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+    } while (0)
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    hipError_t alloc(size_t count) {
+        if (count <= n && p) return hipSuccess;
+        release();
+        size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+        hipError_t e = hipMalloc((void**)&p, bytes);
+        if (e == hipSuccess) n = count;
+        else p = nullptr;
+        return e;
+    }
+};
+
+// The sampler keeps float(n_dk) + alpha_k and recovers n_dk by rounding, which
+// is exact while n_dk + alpha_k < 2^21: documents and alpha are bounded here.
+constexpr uint64_t kMaxDocTokens = 1u << 20;
+constexpr float kMaxAlpha = (float)(1u << 20);
+
+}  // namespace lda_host
+
+using lda_host::DevBuf;
+
+struct lda_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int K = 0, V = 0, kpl = 1, Kp = 64;
+    float eta = 0.01f, veta = 0.0f;
+    int tw = LDA_TW_ONE;
+    uint64_t seed = 0;
+    std::vector<float> alpha;  // natural order
+    uint64_t global_step = 0;
+    uint32_t optim_interval = 10, burn_in = 0;
+    int num_cu = 1;
+
+    // corpus
+    int64_t D = 0;
+    uint64_t N = 0;
+    uint64_t n_valid = 0;
+    std::vector<uint64_t> off;
+    std::vector<uint32_t> words;
+    bool has_corpus = false;
+
+    DevBuf<float> d_alpha, d_alpha_cdf;
+    DevBuf<int32_t> d_nkw, d_nk, d_nk_next, d_ndk, d_doclen, d_nk_priv, d_nkw_loc;
+    DevBuf<uint32_t> d_words, d_order, d_counter;
+    DevBuf<uint16_t> d_z;
+    DevBuf<uint64_t> d_off, d_coff;
+    DevBuf<double> d_part;
+    int part_W = 0;
+
+    // fold-in buffers
+    DevBuf<uint32_t> i_words, i_counter;
+    DevBuf<lda::InferJob> i_jobs;
+    DevBuf<lda::InferParams> i_params;
+    DevBuf<uint16_t> i_z;
+    DevBuf<float> i_theta;
+    DevBuf<double> i_ll;
+    // batched fold-in graph: counter reset + persistent kernel reading i_params
+    hipGraphExec_t infer_exec = nullptr;
+    int infer_exec_grid = 0;
+
+    // kernel timing of the last call
+    double stat_ms = 0.0;
+    double stat_launches = 0.0;
+    double stat_tokens = 0.0;
+
+    int pos(int k) const { return lda::kvec_pos(kpl, k); }
+};

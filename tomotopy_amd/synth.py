@@ -141,6 +141,42 @@ def foldin_workload(train_docs=1000, unseen_docs=20000, seed=1004):
     return (dense.astype(np.uint32), toff, int((mapping >= 0).sum())), (m[keep].astype(np.uint32), uoff2)
 
 
+def serve_workload(n_requests, train_docs=20000, seed=1005):
+    """Serving workload: a training corpus of 20,000 docs (lengths LogNormal
+    with mean 300, V=100,000, Zipf(1.07) background mixed into 500 Dir(0.01)
+    topics) for a K=500 model, and `n_requests` requests of 1..64 unseen
+    documents each, lengths LogNormal(ln 150, 1.0) clipped to [5, 10000],
+    drawn from the same topics.  Unseen words get the training dictionary's
+    ids; words the training corpus never saw are dropped (make_doc semantics).
+    Returns (train_words, train_off, n_words), [(words, off) per request]."""
+    V, true_k = 100000, 500
+    rng = _rng(seed)
+    topics, zcdf = _topics(rng, V, true_k, 0.01, 1.07)
+    tl = lognormal_lengths(_rng(seed + 7919), train_docs, np.log(300.0) - 0.125, 0.5, 10, 3000)
+    tw, toff = _draw(rng, topics, zcdf, V, true_k, 0.1, tl)
+    dense, mapping = _first_appearance_ids(tw, V)
+    rq = _rng(seed + 104729)
+    sizes = rq.integers(1, 65, size=n_requests)
+    lengths = lognormal_lengths(rq, int(sizes.sum()), np.log(150.0), 1.0, 5, 10000)
+    uw, uoff = _draw(rng, topics, zcdf, V, true_k, 0.1, lengths)
+    m = mapping[uw]
+    keep = m >= 0
+    nd = len(lengths)
+    doc_id = np.repeat(np.arange(nd), np.diff(uoff).astype(np.int64))
+    new_len = np.bincount(doc_id[keep], minlength=nd)
+    uoff2 = np.zeros(nd + 1, dtype=np.uint64)
+    np.cumsum(new_len, out=uoff2[1:])
+    kept = m[keep].astype(np.uint32)
+    reqs = []
+    d0 = 0
+    for n in sizes:
+        d1 = d0 + int(n)
+        b, e = int(uoff2[d0]), int(uoff2[d1])
+        reqs.append((kept[b:e], (uoff2[d0:d1 + 1] - uoff2[d0]).astype(np.uint64)))
+        d0 = d1
+    return (dense.astype(np.uint32), toff, int((mapping >= 0).sum())), reqs
+
+
 WORKLOADS = {
     "plain_k20": dict(seed=1001, n_docs=1000, V=5000, true_k=20, alpha=0.1, beta=0.01, mu=np.log(200.0), sigma=0.5,
                       lo=20, hi=2000, zipf_s=None, model_k=20, model_alpha=0.1, model_eta=0.01, iterations=100),
