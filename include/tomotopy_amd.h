@@ -146,6 +146,20 @@ int lda_sample_frozen(lda_ctx* ctx, int64_t n, const int64_t* doc, const uint32_
 int lda_infer(lda_ctx* ctx, const uint32_t* words, const uint64_t* doc_off, int64_t D, int iterations,
               uint64_t infer_seed, int mode, float* theta, double* ll, uint16_t* z_out);
 
+/* Segmented fold-in (lda_infer and the server).  tomoto samples a document's
+ * tokens one after the other (LDAModel.hpp:920-953), which bounds a long
+ * document's latency by its length.  With seg_tokens > 0 a document of n >
+ * seg_tokens tokens is split into S = min(seg_max, next power of two >=
+ * ceil(n / seg_tokens)) segments sampled by S wavefronts; segment w starts
+ * at the first word-run start at or after floor(n*w/S).  Each iteration every
+ * segment samples against n_dk as it was when the iteration started plus its
+ * own changes, and the changes are summed after the iteration.  Documents of
+ * at most seg_tokens tokens, and every document when seg_tokens == 0 (the
+ * default), are sampled exactly as tomoto does.  seg_max is 1, 2, 4 or 8 and
+ * at most *seg_supported (what the kernel fits at this K). */
+int lda_set_infer_options(lda_ctx* ctx, uint32_t seg_tokens, int seg_max);
+int lda_get_infer_options(lda_ctx* ctx, uint32_t* seg_tokens, int* seg_max, int* seg_supported);
+
 /* ---- Serving: continuous-batching fold-in over the context's frozen model ----
  * tomotopy has no server; these calls serve LDAModel.infer (py_LDA.cpp:345-411,
  * LDAModel::_infer<together=false>, LDAModel.hpp:920-953) to a stream of

@@ -680,8 +680,48 @@ void orc_loglik(void* h, int mode, double* ll_docs, double* ll_rest) { loglik(*(
 // ll = getLLRest(private) - getLLRest(global) + getLLDocs(doc), evaluated in
 // FP64 (mode 0) or with tomoto's LUT (mode 1) over the touched terms only.
 // theta = (n_dk + alpha) / (L + sum(alpha)) (LDAModel.hpp:1145-1159).
+// Segmented fold-in (the device's rule for long documents): S = 1 when
+// seg_tokens == 0 or n <= seg_tokens, else the smallest power of two >=
+// ceil(n / seg_tokens), at most seg_max; segment w starts at the first
+// word-run start at or after floor(n*w/S).  Each iteration every segment
+// samples its tokens in order against n_dk as it was when the iteration
+// started plus its own changes (n_k,eff likewise); the segments' changes are
+// then summed.  c_d needs no merging: a word's tokens lie in one segment.
+// S == 1 is the sequential sampler above.
+static int seg_count(uint64_t n, uint32_t seg_tokens, int seg_max) {
+    if (seg_tokens == 0 || n <= seg_tokens) return 1;
+    uint64_t need = (n + seg_tokens - 1) / seg_tokens;
+    int S = 1;
+    while ((uint64_t)S < need && S < seg_max) S <<= 1;
+    return S;
+}
+
+static uint64_t seg_start(const uint32_t* w, uint64_t n, int i, int S) {
+    if (i <= 0) return 0;
+    if (i >= S) return n;
+    uint64_t x = n * (uint64_t)i / (uint64_t)S;
+    if (x == 0) return 0;
+    while (x < n && w[x] == w[x - 1]) ++x;
+    return x;
+}
+
+void orc_infer_seg(void* h, const uint32_t* words, const uint64_t* off, int64_t D, int iters, uint64_t infer_seed,
+                   int mode, uint32_t seg_tokens, int seg_max, float* theta, double* ll, uint16_t* z_out);
+
+// Segment count and starts of one document (S + 1 entries in starts).
+int orc_seg_bounds(const uint32_t* words, uint64_t n, uint32_t seg_tokens, int seg_max, uint64_t* starts) {
+    const int S = seg_count(n, seg_tokens, seg_max);
+    for (int i = 0; i <= S; ++i) starts[i] = seg_start(words, n, i, S);
+    return S;
+}
+
 void orc_infer(void* h, const uint32_t* words, const uint64_t* off, int64_t D, int iters,
                uint64_t infer_seed, int mode, float* theta, double* ll, uint16_t* z_out) {
+    orc_infer_seg(h, words, off, D, iters, infer_seed, mode, 0, 1, theta, ll, z_out);
+}
+
+void orc_infer_seg(void* h, const uint32_t* words, const uint64_t* off, int64_t D, int iters, uint64_t infer_seed,
+                   int mode, uint32_t seg_tokens, int seg_max, float* theta, double* ll, uint16_t* z_out) {
     Model* m = (Model*)h;
     const int K = m->K;
     std::vector<float> ca(K);
@@ -710,20 +750,31 @@ void orc_infer(void* h, const uint32_t* words, const uint64_t* off, int64_t D, i
             ndk[k]++; nk_eff[k]++; cd[run_start[i] * K + k]++;
         }
         std::vector<int32_t> col(K);
+        const int S = seg_count(n, seg_tokens, seg_max);
+        std::vector<int32_t> snap(K), nd_s(K), nk_s(K), merged(K);
         for (int it = 0; it < iters; ++it) {
-            for (uint64_t i = 0; i < n; ++i) {
-                uint32_t v = words[b + i];
-                if (v >= (uint32_t)m->V) continue;
-                int zo = z[i];
-                int32_t* c = &cd[run_start[i] * K];
-                ndk[zo]--; nk_eff[zo]--; c[zo]--;
-                for (int k = 0; k < K; ++k) col[k] = m->nkw[(size_t)v * K + k] + c[k];
-                float u = token_uniform(infer_seed, i, (uint32_t)(it + 1), TAG_INFER);
-                int zn = sample_one(K, m->scan_order, m->kpl, ndk.data(), m->alpha.data(), col.data(), m->eta,
-                                    nk_eff.data(), m->veta, u, buf.data());
-                z[i] = (uint16_t)zn;
-                ndk[zn]++; nk_eff[zn]++; c[zn]++;
+            snap = ndk;
+            merged = ndk;
+            for (int sg = 0; sg < S; ++sg) {
+                const uint64_t s0 = seg_start(words + b, n, sg, S), s1 = seg_start(words + b, n, sg + 1, S);
+                nd_s = snap;
+                for (int k = 0; k < K; ++k) nk_s[k] = m->nk[k] + snap[k];
+                for (uint64_t i = s0; i < s1; ++i) {
+                    uint32_t v = words[b + i];
+                    if (v >= (uint32_t)m->V) continue;
+                    int zo = z[i];
+                    int32_t* c = &cd[run_start[i] * K];
+                    nd_s[zo]--; nk_s[zo]--; c[zo]--;
+                    for (int k = 0; k < K; ++k) col[k] = m->nkw[(size_t)v * K + k] + c[k];
+                    float u = token_uniform(infer_seed, i, (uint32_t)(it + 1), TAG_INFER);
+                    int zn = sample_one(K, m->scan_order, m->kpl, nd_s.data(), m->alpha.data(), col.data(), m->eta,
+                                        nk_s.data(), m->veta, u, buf.data());
+                    z[i] = (uint16_t)zn;
+                    nd_s[zn]++; nk_s[zn]++; c[zn]++;
+                }
+                for (int k = 0; k < K; ++k) merged[k] += nd_s[k] - snap[k];
             }
+            ndk = merged;
         }
         int64_t L = 0;
         for (uint64_t i = 0; i < n; ++i) L += words[b + i] < (uint32_t)m->V;

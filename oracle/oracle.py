@@ -61,6 +61,8 @@ def load():
         "orc_get_state": (None, [_c_p, _c_p, _c_p, _c_p, _c_p]),
         "orc_loglik": (None, [_c_p, i32, _c_p, _c_p]),
         "orc_infer": (None, [_c_p, _c_p, _c_p, i64, i32, u64, i32, _c_p, _c_p, _c_p]),
+        "orc_infer_seg": (None, [_c_p, _c_p, _c_p, i64, i32, u64, i32, u32, i32, _c_p, _c_p, _c_p]),
+        "orc_seg_bounds": (i32, [_c_p, u64, u32, i32, _c_p]),
         "orc_infer_ll_bruteforce": (f64, [_c_p, _c_p, u64, _c_p]),
     }
     for name, (res, args) in sigs.items():
@@ -144,6 +146,14 @@ def sample_frozen(K, scan_order, alpha, eta, veta, ndk, nkw, nk, doc, word, z_ol
     return zn, tot
 
 
+def seg_bounds(words, seg_tokens, seg_max):
+    """Segment starts (S + 1 offsets) of one document under the segmented fold-in rule."""
+    w = np.ascontiguousarray(words, np.uint32)
+    out = np.zeros(9, np.uint64)
+    S = load().orc_seg_bounds(_p(w), len(w), int(seg_tokens), int(seg_max), _p(out))
+    return out[:S + 1]
+
+
 def lgamma_lut(x):
     return load().orc_lgamma_lut(float(x))
 
@@ -216,15 +226,16 @@ class Model:
         load().orc_loglik(self._h, mode, ctypes.byref(a), ctypes.byref(b))
         return a.value, b.value
 
-    def infer(self, words, off, iters, infer_seed=0, mode=0):
+    def infer(self, words, off, iters, infer_seed=0, mode=0, seg_tokens=0, seg_max=1):
+        """Fold-in; seg_tokens / seg_max select the segmented rule (0: sequential)."""
         words = np.ascontiguousarray(words, np.uint32)
         off = np.ascontiguousarray(off, np.uint64)
         D = len(off) - 1
         theta = np.zeros((D, self.K), np.float32)
         ll = np.zeros(D, np.float64)
         z = np.zeros(int(off[-1]), np.uint16)
-        load().orc_infer(self._h, _p(words), _p(off), D, iters, int(infer_seed) & (2**64 - 1), mode, _p(theta), _p(ll),
-                         _p(z))
+        load().orc_infer_seg(self._h, _p(words), _p(off), D, iters, int(infer_seed) & (2**64 - 1), mode,
+                             int(seg_tokens), int(seg_max), _p(theta), _p(ll), _p(z))
         return theta, ll, z
 
     def infer_ll_bruteforce(self, words, z):

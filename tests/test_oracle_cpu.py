@@ -333,3 +333,133 @@ def test_infer_document_alone_equals_in_batch(small_corpus):
     b, e = int(uoff[2]), int(uoff[3])
     th1, ll1, z1 = m.infer(uw[b:e], np.array([0, e - b], np.uint64), 8)
     assert np.array_equal(th1[0], th[2]) and ll1[0] == ll[2] and np.array_equal(z1, z[b:e])
+
+
+# -------------------------------------------------------- segmented infer ---
+def test_segment_rule():
+    """S is a power of two <= seg_max, segments tile the document, every
+    segment starts at a word-run start, and no word straddles two segments."""
+    rng = np.random.default_rng(3)
+    for trial in range(300):
+        n = int(rng.integers(0, 400))
+        w = np.sort(rng.integers(0, max(1, int(rng.integers(1, 60))), n)).astype(np.uint32)
+        T = int(rng.choice([0, 1, 3, 16, 50, 200]))
+        M = int(rng.choice([1, 2, 4, 8]))
+        st = O.seg_bounds(w, T, M)
+        S = len(st) - 1
+        need = 1 if (T == 0 or n <= T) else -(-n // T)
+        expect = 1
+        while expect < need and expect < M:
+            expect *= 2
+        assert S == expect
+        assert st[0] == 0 and st[-1] == n and np.all(np.diff(st.astype(np.int64)) >= 0)
+        for x in st[1:-1]:
+            x = int(x)
+            assert x == n or x == 0 or w[x] != w[x - 1]
+        for i in range(1, S):
+            x = int(st[i])
+            nominal = (n * i) // S
+            assert x >= nominal and (x == nominal or np.all(w[nominal:x] == w[max(nominal - 1, 0)]))
+
+
+def test_segmented_infer_reduces_to_sequential(small_corpus):
+    """seg_tokens == 0, or documents no longer than seg_tokens: bit-identical to
+    the sequential fold-in."""
+    ws, offs, V = small_corpus
+    m = O.Model(20, V, 0.1, 0.01, seed=2)
+    m.set_corpus(ws, offs)
+    m.train(3, workers=1)
+    uw, uoff = synth.unseen_docs(9, 12, V, 20, np.log(60.0), 0.8, 3, 400, zipf_s=None)
+    uw, uoff = corpus.csr_from_arrays(uw, uoff, np.arange(V))
+    th0, ll0, z0 = m.infer(uw, uoff, 6)
+    th1, ll1, z1 = m.infer(uw, uoff, 6, seg_tokens=0, seg_max=8)
+    assert np.array_equal(z0, z1) and np.array_equal(th0, th1) and np.array_equal(ll0, ll1)
+    longest = int(np.diff(uoff.astype(np.int64)).max())
+    th2, ll2, z2 = m.infer(uw, uoff, 6, seg_tokens=longest, seg_max=8)
+    assert np.array_equal(z0, z2) and np.array_equal(ll0, ll2)
+
+
+def _py_segmented_infer(nkw, nk, alpha, eta, V, doc, iters, seed, seg_tokens, seg_max):
+    """Pure-Python restatement of the segmented fold-in of one document, built
+    from the separately pinned primitives (weights, device scan, search,
+    Philox uniforms)."""
+    K = len(alpha)
+    kpl = O.device_kpl(K)
+    veta = np.float32(np.float32(eta) * np.float32(V))
+    ca = np.cumsum(alpha.astype(np.float32), dtype=np.float32)
+    n = len(doc)
+    z = np.full(n, 0xFFFF, np.int64)
+    for i in range(n):
+        if doc[i] < V:
+            z[i] = O.search(ca, K, O.token_uniform(seed, i, 0, "infinit"))
+    st = O.seg_bounds(doc, seg_tokens, seg_max)
+    for it in range(1, iters + 1):
+        snap = np.bincount(z[z != 0xFFFF], minlength=K).astype(np.int64)
+        delta = np.zeros(K, np.int64)
+        for s in range(len(st) - 1):
+            nd = snap.copy()
+            for i in range(int(st[s]), int(st[s + 1])):
+                v = int(doc[i])
+                if v >= V:
+                    continue
+                zo = int(z[i])
+                nd[zo] -= 1
+                same = [j for j in range(n) if doc[j] == v and j != i]  # c_d: the document's other tokens of v
+                col = nkw[v].astype(np.int64).copy()
+                for j in same:
+                    col[z[j]] += 1
+                w = O.weights(nd, alpha, col, eta, nk + nd, veta)
+                cdf = O.prefix_sum_device(w, kpl)
+                zn = O.search(cdf, K, O.token_uniform(seed, i, it, "infer"))
+                z[i] = zn
+                nd[zn] += 1
+            delta += nd - snap
+    return z
+
+
+def test_segmented_infer_matches_python_restatement():
+    """The oracle's segmented fold-in equals a direct Python restatement of the
+    rule (per-iteration snapshot, per-segment private n_dk, summed changes)."""
+    K, V = 6, 30
+    rng = np.random.default_rng(11)
+    D = 40
+    lens = rng.integers(20, 60, D)
+    words = [np.sort(rng.integers(0, V, L)) for L in lens]
+    off = np.zeros(D + 1, np.uint64)
+    off[1:] = np.cumsum(lens)
+    alpha = np.float32(0.2)
+    m = O.Model(K, V, alpha, 0.05, seed=3)
+    m.set_corpus(np.concatenate(words).astype(np.uint32), off)
+    m.train(4, workers=1)
+    _, _, nkw, nk = m.state()
+    al = np.full(K, alpha, np.float32)
+    for seg_tokens, seg_max, n in ((10, 4, 37), (20, 2, 40), (0, 8, 25), (3, 8, 9)):
+        doc = np.sort(rng.integers(0, V, n)).astype(np.uint32)
+        o = np.array([0, n], np.uint64)
+        _, _, z = m.infer(doc, o, 3, infer_seed=7, seg_tokens=seg_tokens, seg_max=seg_max)
+        zp = _py_segmented_infer(nkw, nk.astype(np.int64), al, 0.05, V, doc, 3, 7, seg_tokens, seg_max)
+        assert np.array_equal(z.astype(np.int64), zp), (seg_tokens, seg_max)
+    # LL and theta of a segmented result stay the closed forms of its z
+    th, ll, z = m.infer(doc, o, 5, seg_tokens=2, seg_max=4)
+    nd = np.bincount(z[z != 0xFFFF], minlength=K)
+    np.testing.assert_allclose(th[0], (nd + alpha) / (len(doc) + alpha * K), rtol=1e-6)
+    bf = m.infer_ll_bruteforce(doc, z)
+    assert abs(ll[0] - bf) <= 1e-8 * max(1.0, abs(bf))
+
+
+def test_segmented_infer_same_ll_per_word_as_sequential(small_corpus):
+    """The segmented sampler targets the same posterior: over many long
+    documents its mean log-likelihood per word matches the sequential
+    sampler's within 1%."""
+    ws, offs, V = small_corpus
+    m = O.Model(20, V, 0.1, 0.01, seed=5)
+    m.set_corpus(ws, offs)
+    m.train(30, workers=1)
+    uw, uoff = synth.unseen_docs(21, 40, V, 20, np.log(600.0), 0.3, 300, 1200, zipf_s=None)
+    uw, uoff = corpus.csr_from_arrays(uw, uoff, np.arange(V))
+    n = int(np.count_nonzero(uw < V))
+    _, ll_seq, _ = m.infer(uw, uoff, 30, infer_seed=4)
+    _, ll_seg, _ = m.infer(uw, uoff, 30, infer_seed=4, seg_tokens=64, seg_max=8)
+    a, b = ll_seq.sum() / n, ll_seg.sum() / n
+    print("ll_per_word sequential %.5f segmented %.5f" % (a, b))
+    assert abs(a - b) <= 0.01 * abs(a)

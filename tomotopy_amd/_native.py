@@ -55,6 +55,8 @@ _SIGS = {
     "lda_sample_frozen": [_c_p, _i64, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p],
     "lda_infer": [_c_p, _c_p, _c_p, _i64, _i32, _u64, _i32, _c_p, _c_p, _c_p],
     "lda_kernel_stats": [_c_p, _c_p],
+    "lda_set_infer_options": [_c_p, _u32, _i32],
+    "lda_get_infer_options": [_c_p, _c_p, _c_p, _c_p],
     "lda_serve_start": [_c_p, _u64, _i32, _i32, _i64, _u64, _c_p],
     "lda_serve_submit": [_c_p, _c_p, _c_p, _i64, _c_p],
     "lda_serve_wait": [_c_p, _u64, _i32, _c_p, _c_p, _c_p],
@@ -236,6 +238,18 @@ class Context:
         _check(load().lda_infer(self._h, _ptr(words), _ptr(doc_off), D, int(iterations), int(infer_seed) & (2**64 - 1),
                                 int(mode), _ptr(theta), _ptr(ll), _ptr(z)))
         return theta, ll, z
+
+    def set_infer_options(self, seg_tokens=0, seg_max=1):
+        """Segmented fold-in: documents longer than seg_tokens tokens are sampled by
+        up to seg_max wavefronts (0: every document sequentially, as tomotopy)."""
+        _check(load().lda_set_infer_options(self._h, int(seg_tokens), int(seg_max)))
+
+    def infer_options(self):
+        t = ctypes.c_uint32(0)
+        m = ctypes.c_int(0)
+        sup = ctypes.c_int(0)
+        _check(load().lda_get_infer_options(self._h, ctypes.byref(t), ctypes.byref(m), ctypes.byref(sup)))
+        return {"seg_tokens": t.value, "seg_max": m.value, "seg_supported": sup.value}
 
     def kernel_stats(self):
         s = np.zeros(4, dtype=np.float64)

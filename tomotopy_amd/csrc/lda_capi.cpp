@@ -35,6 +35,42 @@ int fail(int code, const char* fmt, ...) {
     return code;
 }
 
+hipError_t FoldStreams::create() {
+    for (int b = 0; b < lda::kFoldBuckets - 1; ++b) {
+        hipError_t e = hipStreamCreateWithFlags(&aux[b], hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&join[b], hipEventDisableTiming);
+        if (e != hipSuccess) return e;
+    }
+    return hipEventCreateWithFlags(&fork, hipEventDisableTiming);
+}
+
+void FoldStreams::destroy() {
+    for (int b = 0; b < lda::kFoldBuckets - 1; ++b) {
+        if (aux[b]) (void)hipStreamDestroy(aux[b]);
+        if (join[b]) (void)hipEventDestroy(join[b]);
+        aux[b] = nullptr;
+        join[b] = nullptr;
+    }
+    if (fork) (void)hipEventDestroy(fork);
+    fork = nullptr;
+}
+
+hipError_t launch_fold_buckets(int kpl, const lda::InferParams* p_dev, const int grids[lda::kFoldBuckets],
+                               hipStream_t s, FoldStreams& fs) {
+    hipError_t e = hipEventRecord(fs.fork, s);
+    // the longest documents (most segments) first
+    for (int b = lda::kFoldBuckets - 1; b >= 1 && e == hipSuccess; --b) {
+        if (grids[b] <= 0) continue;
+        e = hipStreamWaitEvent(fs.aux[b - 1], fs.fork, 0);
+        if (e == hipSuccess) e = lda::launch_infer(kpl, 1 << b, p_dev, grids[b], fs.aux[b - 1]);
+        if (e == hipSuccess) e = hipEventRecord(fs.join[b - 1], fs.aux[b - 1]);
+    }
+    if (e == hipSuccess && grids[0] > 0) e = lda::launch_infer(kpl, 1, p_dev, grids[0], s);
+    for (int b = 1; b < lda::kFoldBuckets && e == hipSuccess; ++b)
+        if (grids[b] > 0) e = hipStreamWaitEvent(s, fs.join[b - 1], 0);
+    return e;
+}
+
 }  // namespace lda_host
 
 using lda_host::fail;
@@ -386,6 +422,13 @@ int lda_ctx_create(int K, int V, const float* alpha, float eta, int termweight, 
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return cleanup(fail(LDA_ERR_DEVICE, "hipStreamCreate failed"));
     if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
         return cleanup(fail(LDA_ERR_DEVICE, "hipEventCreate failed"));
+    if (c->fold_streams.create() != hipSuccess) return cleanup(fail(LDA_ERR_DEVICE, "fold-in stream creation failed"));
+    for (int b = 0; b < lda::kFoldBuckets; ++b) {
+        const int per_cu = lda::infer_blocks_per_cu(c->kpl, 1 << b);
+        c->fold_grid[b] = c->num_cu * per_cu;
+        if (per_cu > 0) c->seg_supported = 1 << b;
+        else break;
+    }
     if (c->d_alpha.alloc(c->Kp) || c->d_alpha_cdf.alloc(K) || c->d_nkw.alloc((size_t)std::max(V, 1) * c->Kp) ||
         c->d_nk.alloc(c->Kp) || c->d_nk_next.alloc(c->Kp) || c->d_counter.alloc(1))
         return cleanup(fail(LDA_ERR_OUT_OF_MEMORY, "device allocation failed"));
@@ -410,6 +453,7 @@ int lda_ctx_destroy(lda_ctx* c) {
     c->i_theta.release(); c->i_ll.release();
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    c->fold_streams.destroy();
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return LDA_OK;
@@ -743,23 +787,36 @@ int lda_infer(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off, int64_
     HIP_TRY(c->i_z.alloc(N));
     HIP_TRY(c->i_jobs.alloc(D));
     HIP_TRY(c->i_params.alloc(1));
-    HIP_TRY(c->i_counter.alloc(1));
+    HIP_TRY(c->i_counter.alloc(lda::kFoldBuckets));
     HIP_TRY(c->i_theta.alloc((size_t)D * c->K));
     HIP_TRY(c->i_ll.alloc(D));
     if (N) HIP_TRY(hipMemcpyAsync(c->i_words.p, words, N * 4, hipMemcpyHostToDevice, s));
-    // jobs longest first (the persistent kernel takes them in this order)
+    // Jobs grouped by bucket (segments per document), longest first inside a
+    // bucket (the persistent kernels take them in this order).
+    std::vector<int> segs(D);
+    for (int64_t d = 0; d < D; ++d) segs[d] = lda::fold_segments(doc_off[d + 1] - doc_off[d], c->seg_tokens, c->seg_max);
     std::vector<uint32_t> order(D);
     std::iota(order.begin(), order.end(), 0u);
     std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+        if (segs[a] != segs[b]) return segs[a] < segs[b];
         return (doc_off[a + 1] - doc_off[a]) > (doc_off[b + 1] - doc_off[b]);
     });
     std::vector<lda::InferJob> jobs(D);
+    lda::InferParams p{};
     for (int64_t i = 0; i < D; ++i) {
         const uint32_t d = mode == 0 ? order[i] : (uint32_t)i;
         jobs[i] = lda::InferJob{doc_off[d], doc_off[d + 1], 0, iterations, iterations, (int32_t)d};
     }
+    for (int bk = 0; bk < lda::kFoldBuckets; ++bk) p.bucket_lo[bk] = p.bucket_hi[bk] = 0;
+    if (mode == 0) {
+        int64_t i = 0;
+        for (int bk = 0; bk < lda::kFoldBuckets; ++bk) {
+            p.bucket_lo[bk] = i;
+            while (i < D && lda::fold_bucket(segs[order[i]]) == bk) ++i;
+            p.bucket_hi[bk] = i;
+        }
+    }
     HIP_TRY(hipMemcpyAsync(c->i_jobs.p, jobs.data(), D * sizeof(lda::InferJob), hipMemcpyHostToDevice, s));
-    lda::InferParams p{};
     p.words = c->i_words.p; p.z = c->i_z.p; p.jobs = c->i_jobs.p; p.njobs = D;
     p.nkw = c->d_nkw.p; p.nk = c->d_nk.p; p.alpha = c->d_alpha.p; p.alpha_cdf = c->d_alpha_cdf.p;
     p.eta = c->eta; p.veta = c->veta; p.K = c->K; p.V = c->V; p.Kp = c->Kp;
@@ -771,17 +828,19 @@ int lda_infer(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off, int64_
     p.asum_d = asd; p.asum_f = asf;
     c->stat_ms = c->stat_launches = c->stat_tokens = 0.0;
     if (mode == 0) {
-        p.counter = c->i_counter.p;
+        p.counters = c->i_counter.p;
         p.job_base = 0;
         HIP_TRY(hipMemcpyAsync(c->i_params.p, &p, sizeof p, hipMemcpyHostToDevice, s));
-        const int grid = c->num_cu * lda::infer_blocks_per_cu(c->kpl);
-        if (!c->infer_exec || c->infer_exec_grid != grid) {
+        // the graph launches the buckets up to seg_max, each with its full persistent grid
+        if (!c->infer_exec || c->infer_exec_segmax != c->seg_max) {
             if (c->infer_exec) { (void)hipGraphExecDestroy(c->infer_exec); c->infer_exec = nullptr; }
+            int grids[lda::kFoldBuckets];
+            for (int bk = 0; bk < lda::kFoldBuckets; ++bk) grids[bk] = (1 << bk) <= c->seg_max ? c->fold_grid[bk] : 0;
             hipGraph_t g = nullptr;
             HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            hipError_t e1 = lda::launch_fill_u32(c->i_counter.p, 0u, 1, s);
-            hipError_t e2 = e1 == hipSuccess ? lda::launch_infer(c->kpl, c->i_params.p, grid, s) : e1;
-            if (e2 == hipSuccess) e2 = lda::launch_infer_finalize(c->kpl, c->i_params.p, grid, s);
+            hipError_t e1 = lda::launch_fill_u32(c->i_counter.p, 0u, lda::kFoldBuckets, s);
+            hipError_t e2 = e1 == hipSuccess ? lda_host::launch_fold_buckets(c->kpl, c->i_params.p, grids, s, c->fold_streams) : e1;
+            if (e2 == hipSuccess) e2 = lda::launch_infer_finalize(c->kpl, c->i_params.p, c->fold_grid[0], s);
             hipError_t e3 = hipStreamEndCapture(s, &g);
             if (e2 != hipSuccess || e3 != hipSuccess) {
                 if (g) (void)hipGraphDestroy(g);
@@ -790,7 +849,7 @@ int lda_infer(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off, int64_
             hipError_t e4 = hipGraphInstantiate(&c->infer_exec, g, nullptr, nullptr, 0);
             (void)hipGraphDestroy(g);
             if (e4 != hipSuccess) return fail(LDA_ERR_DEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e4));
-            c->infer_exec_grid = grid;
+            c->infer_exec_segmax = c->seg_max;
         }
         HIP_TRY(hipEventRecord(c->ev0, s));
         HIP_TRY(hipGraphLaunch(c->infer_exec, s));
@@ -798,12 +857,12 @@ int lda_infer(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off, int64_
         c->stat_launches = 1;
     } else {
         // plain: one launch and one synchronisation per document
-        p.counter = nullptr;
+        p.counters = nullptr;
         HIP_TRY(hipEventRecord(c->ev0, s));
         for (int64_t d = 0; d < D; ++d) {
             p.job_base = d;
             HIP_TRY(hipMemcpyAsync(c->i_params.p, &p, sizeof p, hipMemcpyHostToDevice, s));
-            HIP_TRY(lda::launch_infer(c->kpl, c->i_params.p, 1, s));
+            HIP_TRY(lda::launch_infer(c->kpl, segs[d], c->i_params.p, 1, s));
             HIP_TRY(lda::launch_infer_finalize(c->kpl, c->i_params.p, 1, s));
             HIP_TRY(hipStreamSynchronize(s));
         }
@@ -819,6 +878,26 @@ int lda_infer(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off, int64_
     HIP_TRY(hipMemcpyAsync(ll, c->i_ll.p, D * 8, hipMemcpyDeviceToHost, s));
     if (z_out && N) HIP_TRY(hipMemcpyAsync(z_out, c->i_z.p, N * 2, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
+    return LDA_OK;
+}
+
+int lda_set_infer_options(lda_ctx* c, uint32_t seg_tokens, int seg_max) {
+    if (!c) return fail(LDA_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (seg_max != 1 && seg_max != 2 && seg_max != 4 && seg_max != 8)
+        return fail(LDA_ERR_INVALID_ARGUMENT, "seg_max must be 1, 2, 4 or 8");
+    if (seg_max > c->seg_supported)
+        return fail(LDA_ERR_UNSUPPORTED, "seg_max %d exceeds the %d wavefronts per document the fold-in kernel fits at K = %d",
+                    seg_max, c->seg_supported, c->K);
+    c->seg_tokens = seg_tokens;
+    c->seg_max = seg_max;
+    return LDA_OK;
+}
+
+int lda_get_infer_options(lda_ctx* c, uint32_t* seg_tokens, int* seg_max, int* seg_supported) {
+    if (!c) return fail(LDA_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (seg_tokens) *seg_tokens = c->seg_tokens;
+    if (seg_max) *seg_max = c->seg_max;
+    if (seg_supported) *seg_supported = c->seg_supported;
     return LDA_OK;
 }
 

@@ -20,6 +20,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "lda_internal.h"
 
 namespace lda {
@@ -125,6 +127,30 @@ __device__ __forceinline__ int uniform_i(int v) { return __builtin_amdgcn_readfi
 
 template <int N> using ivec = int __attribute__((ext_vector_type(N)));
 template <int N> using fvec = float __attribute__((ext_vector_type(N)));
+
+// Global-address-space views.  Pointers read from device memory (the fold-in
+// parameter block) are generic to the compiler, and generic (flat) accesses
+// make it wait for every outstanding load and LDS operation at each use;
+// these views turn them into global loads that are tracked one by one.
+template <typename T> using gbl = __attribute__((address_space(1))) T;
+template <typename T>
+__device__ __forceinline__ gbl<T>* gptr(T* p) { return (gbl<T>*)p; }
+
+template <int KPL, typename V>
+__device__ __forceinline__ void gload_kvec(const gbl<const int32_t>* base, int lane, V& o) {
+    if constexpr (KPL >= 4) {
+#pragma unroll
+        for (int g = 0; g < KPL / 4; ++g) {
+            const ivec<4> x = *reinterpret_cast<const gbl<const ivec<4>>*>(base + g * 256 + lane * 4);
+            o[4 * g] = x[0]; o[4 * g + 1] = x[1]; o[4 * g + 2] = x[2]; o[4 * g + 3] = x[3];
+        }
+    } else if constexpr (KPL == 2) {
+        const ivec<2> x = *reinterpret_cast<const gbl<const ivec<2>>*>(base + lane * 2);
+        o[0] = x[0]; o[1] = x[1];
+    } else {
+        o[0] = base[lane];
+    }
+}
 
 template <int KPL>
 constexpr int kpl_shift() {
@@ -238,6 +264,33 @@ __device__ __forceinline__ void lane_limits(int K, int lane, int& nvalid, int& l
     lastLane = (K - 1) / KPL;
 }
 
+// Inclusive scan of the lane-layout weights (lane-local, then the DPP wave
+// scan) and the inverse-CDF search of sampleFromDiscreteAcc (sample.hpp:133-189):
+// the first topic with u * total < cdf, else K-1.  Returns the topic
+// (wave-uniform); tot receives cdf[K-1].
+template <int KPL>
+__device__ __forceinline__ int scan_search(fvec<KPL>& w, float u, int K, int nvalid, int lastLane, float& tot) {
+#pragma unroll
+    for (int j = 1; j < KPL; ++j) w[j] = w[j - 1] + w[j];
+    float incl = wave_inclusive_scan(w[KPL - 1]);
+    float excl = dpp_wave_shr1(incl);
+#pragma unroll
+    for (int j = 0; j < KPL; ++j) w[j] = excl + w[j];
+    tot = readlane_f(w[KPL - 1], lastLane);
+    const float r = u * tot;
+    int miss = 0;
+#pragma unroll
+    for (int j = 0; j < KPL; ++j) miss += (r < w[j]) ? 0 : 1;
+    const unsigned long long m = __ballot(miss < nvalid);
+    int zn;
+    if (m == 0ull) zn = K - 1;
+    else {
+        const int L = __builtin_ctzll(m);
+        zn = L * KPL + readlane_i(miss, L);
+    }
+    return uniform_i(zn);
+}
+
 // One token: remove z_old, weight, scan, search, add z_new.  Returns z_new
 // (wave-uniform).  *total receives cdf[K-1].  Padded topics (k >= K) have
 // weight exactly 0, so inside a lane the cdf of a padded slot equals that of
@@ -266,25 +319,8 @@ __device__ __forceinline__ int token_step(const WaveLds& S, const fvec<KPL>& al,
         float rr = __builtin_fmaf(-q0, bf, a);
         w[j] = __builtin_fmaf(rr, y[j], q0);
     }
-#pragma unroll
-    for (int j = 1; j < KPL; ++j) w[j] = w[j - 1] + w[j];
-    float incl = wave_inclusive_scan(w[KPL - 1]);
-    float excl = dpp_wave_shr1(incl);
-#pragma unroll
-    for (int j = 0; j < KPL; ++j) w[j] = excl + w[j];
-    const float tot = readlane_f(w[KPL - 1], lastLane);
-    const float r = u * tot;
-    int miss = 0;
-#pragma unroll
-    for (int j = 0; j < KPL; ++j) miss += (r < w[j]) ? 0 : 1;
-    const unsigned long long m = __ballot(miss < nvalid);
-    int zn;
-    if (m == 0ull) zn = K - 1;
-    else {
-        const int L = __builtin_ctzll(m);
-        zn = L * KPL + readlane_i(miss, L);
-    }
-    zn = uniform_i(zn);
+    float tot;
+    const int zn = scan_search<KPL>(w, u, K, nvalid, lastLane, tot);
     bump<KPL>(S, al, lane, zn, +1, veta);
     if (total) *total = tot;
     return zn;
@@ -786,120 +822,336 @@ __device__ __forceinline__ void add_run_counts(const WaveLds& S, const InferPara
     }
 }
 
-// Fold-in of one document per wavefront (LDAModel.hpp:920-953) with the
-// private state kept as exact deltas: n_k,eff = n_k + n_dk and
-// n_kw,eff = n_kw + c_d, c_d rebuilt per word run from the document's z.
-// Every draw is keyed by (infer seed, token position in the document,
-// iteration), so a document's result depends neither on the batch it runs in
-// nor on how its iterations are split across launches.
+// Register state of a fold-in wavefront.  The wavefront is alone with its
+// document and shares no counts, so the whole sampler state stays in
+// registers; lane l holds topics [l*KPL, (l+1)*KPL) in the first KPL entries
+// of each vector.  A topic update indexes the vectors with a wave-uniform
+// slot; vectors of 16 or more entries make the compiler use indexed register
+// moves (s_set_gpr_idx) for that instead of a select per entry, so narrower
+// states are padded to 16.
 template <int KPL>
-__global__ void __launch_bounds__(64) infer_kernel(const InferParams* __restrict__ pp) {
+constexpr int fold_vw() { return KPL < 16 ? 16 : KPL; }
+
+template <int KPL>
+struct FoldVecs {
+    static constexpr int VW = fold_vw<KPL>();
+    using F = fvec<VW>;
+    using I = ivec<VW>;
+};
+
+// addWordTo<+1> of topic z on the owner lane (LDAModel.hpp:287-299).  Every
+// lane writes its slot, the non-owners their old value: a store under a lane
+// condition would make the compiler copy the whole vector.
+template <int KPL, typename F, typename I>
+__device__ __forceinline__ void fold_add(F& A, I& col, I& nk, F& bf, F& y, const F& al, int lane, int z, float veta) {
+    const int sl = z & (KPL - 1);
+    const bool own = lane == (z >> kpl_shift<KPL>());
+    const float a = al[sl], oA = A[sl], ob = bf[sl], oy = y[sl];
+    const int oc = col[sl], on = nk[sl];
+    const int d = (int)__builtin_rintf(oA - a) + 1;
+    const int n = on + 1;
+    const float b = (float)n + veta;
+    const float yn = 1.0f / b;
+    A[sl] = own ? (float)d + a : oA;
+    col[sl] = own ? oc + 1 : oc;
+    nk[sl] = own ? n : on;
+    bf[sl] = own ? b : ob;
+    y[sl] = own ? yn : oy;
+}
+
+// One fold-in token: remove z_old (addWordTo<-1>), weight, scan, search, add
+// z_new (addWordTo<+1>).
+template <int KPL, typename F, typename I>
+__device__ __forceinline__ int fold_token(F& A, I& col, I& nk, F& bf, F& y, const F& al, int lane, int zo, float u,
+                                          int K, int nvalid, int lastLane, float eta, float veta) {
+    {
+        const int sl = zo & (KPL - 1);
+        const bool own = lane == (zo >> kpl_shift<KPL>());
+        const float a = al[sl], oA = A[sl], ob = bf[sl], oy = y[sl];
+        const int oc = col[sl], on = nk[sl];
+        const int d = (int)__builtin_rintf(oA - a) - 1;
+        const int n = on - 1;
+        const float b = (float)n + veta;
+        const float yn = 1.0f / b;
+        A[sl] = own ? (float)d + a : oA;
+        col[sl] = own ? oc - 1 : oc;
+        nk[sl] = own ? n : on;
+        bf[sl] = own ? b : ob;
+        y[sl] = own ? yn : oy;
+    }
+    fvec<KPL> w;
+#pragma unroll
+    for (int j = 0; j < KPL; ++j) {
+        // ((n_dk + alpha) * (n_kw + eta)) / (n_k + V*eta), the quotient as
+        // q0 = a*y, r = fma(-q0, b, a), q = fma(r, y, q0) with y = RN(1/b)
+        const float B = (float)col[j] + eta;
+        const float a = A[j] * B;
+        const float q0 = a * y[j];
+        const float rr = __builtin_fmaf(-q0, bf[j], a);
+        w[j] = __builtin_fmaf(rr, y[j], q0);
+    }
+    float tot;
+    const int zn = scan_search<KPL>(w, u, K, nvalid, lastLane, tot);
+    fold_add<KPL>(A, col, nk, bf, y, al, lane, zn, veta);
+    return zn;
+}
+
+// Adds one count for topic k (owner lane).
+template <int KPL, typename I>
+__device__ __forceinline__ void col_add(I& col, int lane, int k) {
+    const int sl = k & (KPL - 1);
+    col[sl] = col[sl] + (lane == (k >> kpl_shift<KPL>()) ? 1 : 0);
+}
+
+// c_d of word v from the tokens [g, e) of its run that lie past the current
+// batch of 64 (words are sorted, so the run is contiguous).
+template <int KPL, typename I>
+__device__ __forceinline__ void fold_run_counts(I& col, const gbl<const uint32_t>* words,
+                                                const gbl<uint16_t>* z, int lane, uint64_t g, uint64_t e, int v) {
+    for (uint64_t t0 = g; t0 < e; t0 += 64) {
+        const uint64_t gg = t0 + (uint64_t)lane;
+        const bool in = gg < e && (int)words[gg] == v;
+        const unsigned long long m = __ballot(in);
+        const int cnt = (m == ~0ull) ? 64 : __builtin_ctzll(~m);
+        const int zz = in ? (int)z[gg] : 0;
+        for (int i = 0; i < cnt; ++i) col_add<KPL>(col, lane, readlane_i(zz, i));
+        if (cnt < 64) break;
+    }
+}
+
+// Columns of the next distinct words are fetched this many runs ahead.
+template <int KPL>
+constexpr int fold_prefetch() { return KPL <= 8 ? 3 : (KPL == 16 ? 2 : 1); }
+
+// Issues the n_kw column load of the next run start in `ahead` into slot.
+// The load is unconditional (an invalid word or an exhausted run list reads
+// the always-present n_k vector instead): a conditional load would make the
+// slot a merge of old and new values, which the compiler resolves with
+// register copies that wait for the load at once.
+template <int KPL>
+__device__ __forceinline__ void fold_issue(ivec<KPL>& slot, unsigned long long& ahead, int wv,
+                                           const gbl<const int32_t>* nkw, const gbl<const int32_t>* fallback, int V,
+                                           int Kp, int lane) {
+    int v = -1;
+    if (ahead) {
+        v = readlane_i(wv, __builtin_ctzll(ahead));
+        ahead &= ahead - 1;
+    }
+    const gbl<const int32_t>* src = (unsigned)v < (unsigned)V ? nkw + (int64_t)v * Kp : fallback;
+    gload_kvec<KPL>(src, lane, slot);
+}
+
+// Start of segment w of S of the document [b, b + n): the first word-run
+// start at or after floor(n*w/S) (the same rule as the host's and the CPU
+// oracle's), so that every word's tokens lie in one segment.
+__device__ __forceinline__ uint64_t fold_seg_start(const gbl<const uint32_t>* words, uint64_t b, uint64_t n, int w,
+                                                   int S, int lane) {
+    if (w <= 0) return b;
+    if (w >= S) return b + n;
+    uint64_t i = (n * (uint64_t)w) / (uint64_t)S;
+    if (i == 0) return b;
+    for (;; i += 64) {
+        const uint64_t g = i + (uint64_t)lane;
+        const bool stop = g >= n || words[b + g] != words[b + g - 1];
+        const unsigned long long m = __ballot(stop);
+        if (m) return b + i + (uint64_t)__builtin_ctzll(m);
+    }
+}
+
+// Fold-in (LDAModel.hpp:920-953) of one document per workgroup of S
+// wavefronts, wavefront w sampling segment w of the document (segment rule at
+// fold_segments).  The private state is kept as exact deltas: n_k,eff =
+// n_k + n_dk and n_kw,eff = n_kw + c_d, c_d rebuilt per word run from the
+// document's z.  With S > 1 the merged n_dk lives in LDS: every iteration
+// starts from it and ends by adding each segment's change.  Every draw is
+// keyed by (infer seed, token position in the document, iteration), so a
+// document's result depends neither on the batch it runs in nor on how its
+// iterations are split across launches.  Per batch of 64 tokens the run
+// starts (tokens whose word differs from the previous one) are found with
+// one ballot and their columns are loaded fold_prefetch() runs ahead into a
+// register ring that the unrolled run loop indexes statically.
+template <int KPL, int S>
+__global__ void __launch_bounds__(64 * S) infer_kernel(const InferParams* __restrict__ pp) {
     constexpr int SH = kpl_shift<KPL>();
+    constexpr int P = fold_prefetch<KPL>();
+    constexpr int Kp = 64 * KPL;
+    constexpr int bucket = S >= 8 ? 3 : S >= 4 ? 2 : S >= 2 ? 1 : 0;
+    using FV = typename FoldVecs<KPL>::F;
+    using IV = typename FoldVecs<KPL>::I;
     const InferParams p = *pp;
+    const gbl<const uint32_t>* words = gptr(p.words);
+    gbl<uint16_t>* zs = gptr(p.z);
+    const gbl<const int32_t>* nkw = gptr(p.nkw);
+    const gbl<const int32_t>* nkv = gptr(p.nk);
+    const gbl<const float>* acdf = gptr(p.alpha_cdf);
+    const gbl<const InferJob>* jobs = gptr(p.jobs);
     const int lane = lane_id();
-    const WaveLds S = wave_lds<KPL>();
+    const int wave = (int)(threadIdx.x >> 6);
+    int* nd_sh = lds_dyn;        // S > 1: merged n_dk (Kp entries, lane layout)
+    int* job_sh = lds_dyn + Kp;  // S > 1: the workgroup's job index
     int nvalid, lastLane;
     lane_limits<KPL>(p.K, lane, nvalid, lastLane);
     fvec<KPL> al;
     load_kvec_f<KPL>(p.alpha, lane, al);
     ivec<KPL> nk0;
     load_kvec<KPL>(p.nk, lane, nk0);
+    FV alw = 0.0f;
+#pragma unroll
+    for (int j = 0; j < KPL; ++j) alw[j] = al[j];
+    const int64_t lo = p.counters ? p.bucket_lo[bucket] : 0;
+    const int64_t hi = p.counters ? p.bucket_hi[bucket] : p.njobs;
     for (;;) {
         int64_t idx;
-        if (p.counter) {
-            uint32_t t = 0;
-            if (lane == 0) t = atomicAdd(p.counter, 1u);
-            idx = (int64_t)(uint32_t)uniform_i((int)t);
+        if (p.counters) {
+            if constexpr (S == 1) {
+                uint32_t t = 0;
+                if (lane == 0) t = atomicAdd(p.counters + bucket, 1u);
+                idx = lo + (int64_t)(uint32_t)uniform_i((int)t);
+            } else {
+                if (threadIdx.x == 0) job_sh[0] = (int)atomicAdd(p.counters + bucket, 1u);
+                __syncthreads();
+                idx = lo + (int64_t)(uint32_t)uniform_i(job_sh[0]);
+            }
         } else {
-            idx = p.job_base + blockIdx.x;
+            idx = p.job_base;
         }
-        if (idx >= p.njobs) break;
-        const InferJob J = p.jobs[idx];
-        const uint64_t b = J.b, e = J.e;
-        // topics at the start of this launch: initial draw or the stored z; n_dk from them
-        ivec<KPL> zero = 0;
-        lds_store<KPL>(S.col, lane, zero);  // the column array holds the n_dk histogram until sampling starts
-        for (uint64_t t0 = b; t0 < e; t0 += 64) {
+        if (idx >= hi) break;
+        const uint64_t b = jobs[idx].b, e = jobs[idx].e;
+        const int it_begin = jobs[idx].it_begin, it_end = jobs[idx].it_end;
+        const uint64_t sb = fold_seg_start(words, b, e - b, wave, S, lane);
+        const uint64_t se = fold_seg_start(words, b, e - b, wave + 1, S, lane);
+        if constexpr (S > 1) {
+            for (int i = threadIdx.x; i < Kp; i += 64 * S) nd_sh[i] = 0;
+            __syncthreads();
+        }
+        // topics at the start of this launch (initial draw or the stored z), n_dk from them
+        IV nd = 0;
+        for (uint64_t t0 = sb; t0 < se; t0 += 64) {
             const uint64_t g = t0 + (uint64_t)lane;
             int zz = 0xFFFF;
-            if (g < e) {
-                if (J.it_begin == 0) {
-                    if (p.words[g] < (uint32_t)p.V) {
+            if (g < se) {
+                if (it_begin == 0) {
+                    if (words[g] < (uint32_t)p.V) {
                         const float u = uniform24(philox_x0(p.seed_lo, p.seed_hi, (uint32_t)(g - b), (uint32_t)((g - b) >> 32), 0u, TAG_INFINIT));
-                        const float r = u * p.alpha_cdf[p.K - 1];
-                        int lo = 0, hi = p.K;
-                        while (lo < hi) {
-                            const int mid = (lo + hi) >> 1;
-                            if (r < p.alpha_cdf[mid]) hi = mid; else lo = mid + 1;
+                        const float r = u * acdf[p.K - 1];
+                        int lo2 = 0, hi2 = p.K;
+                        while (lo2 < hi2) {
+                            const int mid = (lo2 + hi2) >> 1;
+                            if (r < acdf[mid]) hi2 = mid; else lo2 = mid + 1;
                         }
-                        zz = lo < p.K ? lo : p.K - 1;
+                        zz = lo2 < p.K ? lo2 : p.K - 1;
                     }
-                    p.z[g] = (uint16_t)zz;
+                    zs[g] = (uint16_t)zz;
                 } else {
-                    zz = p.z[g];
+                    zz = zs[g];
                 }
             }
-            const int n = (int)((e - t0) < 64 ? (e - t0) : 64);
+            const int n = (int)((se - t0) < 64 ? (se - t0) : 64);
             for (int i = 0; i < n; ++i) {
                 const int k = readlane_i(zz, i);
-                if (k == 0xFFFF) continue;
-                if (lane == (k >> SH)) S.col[kpos<KPL>(lane, k & (KPL - 1))] += 1;
+                if (k != 0xFFFF) col_add<KPL>(nd, lane, k);
             }
         }
-        ivec<KPL> nd;
-        lds_load<KPL>(S.col, lane, nd);
-        set_ndk<KPL>(S, al, nd, lane);
-        set_nk_vec<KPL>(S, nk0 + nd, lane, p.veta);
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
-        for (int it = J.it_begin + 1; it <= J.it_end; ++it) {
+        ivec<KPL> snap;  // n_dk at the start of the iteration
+        if constexpr (S > 1) {
+#pragma unroll
+            for (int j = 0; j < KPL; ++j)
+                if (nd[j]) atomicAdd(nd_sh + kpos<KPL>(lane, j), nd[j]);
+        }
+        FV A = 0.0f, bf = 1.0f, y = 1.0f;
+        IV col = 0, nk = 0;
+        for (int it = it_begin + 1; it <= it_end; ++it) {
+            if constexpr (S > 1) {
+                __syncthreads();  // every segment's change is in nd_sh
+                lds_load<KPL>(nd_sh, lane, snap);
+            } else if (it == it_begin + 1) {
+#pragma unroll
+                for (int j = 0; j < KPL; ++j) snap[j] = nd[j];
+            }
+            if (S > 1 || it == it_begin + 1) {
+#pragma unroll
+                for (int j = 0; j < KPL; ++j) {
+                    A[j] = (float)snap[j] + al[j];
+                    nk[j] = nk0[j] + snap[j];
+                    bf[j] = (float)nk[j] + p.veta;
+                    y[j] = 1.0f / bf[j];
+                }
+            }
             int vcur = -1;
-            for (uint64_t t0 = b; t0 < e; t0 += 64) {
-                const int n = (int)((e - t0) < 64 ? (e - t0) : 64);
+            for (uint64_t t0 = sb; t0 < se; t0 += 64) {
+                const int n = (int)((se - t0) < 64 ? (se - t0) : 64);
                 const uint64_t g = t0 + (uint64_t)lane;
                 int wv = -1, zv = 0xFFFF;
                 float uv = 0.0f;
                 if (lane < n) {
-                    wv = (int)p.words[g];
-                    zv = p.z[g];
+                    wv = (int)words[g];
+                    zv = zs[g];
                     uv = uniform24(philox_x0(p.seed_lo, p.seed_hi, (uint32_t)(g - b), (uint32_t)((g - b) >> 32), (uint32_t)it, TAG_INFER));
                 }
+                // previous token's word (lane 0: the last word of the previous batch)
+                const int wp = __builtin_amdgcn_update_dpp(vcur, wv, 0x138, 0xF, 0xF, false);
+                const unsigned long long starts = __ballot(lane < n && wv != wp);
+                unsigned long long ahead = starts;
+                ivec<KPL> slot[P];
+#pragma unroll
+                for (int q = 0; q < P; ++q) fold_issue<KPL>(slot[q], ahead, wv, nkw, nkv, p.V, p.Kp, lane);
                 int zkeep = zv;
-                int vnext = -1;
-                ivec<KPL> ncol;
-                for (int i = 0; i < n; ++i) {
-                    const int v = readlane_i(wv, i);
-                    if ((unsigned)v >= (unsigned)p.V) continue;
-                    if (v != vcur) {
-                        if (v != vnext) load_kvec<KPL>(p.nkw + (int64_t)v * p.Kp, lane, ncol);
-                        lds_store<KPL>(S.col, lane, ncol);
-                        // c_d of this word: the run's topics, from the batch
-                        // registers (not yet resampled in this iteration) and,
-                        // past the batch, from memory
-                        int j = i;
-                        for (; j < n && readlane_i(wv, j) == v; ++j) {
-                            const int k = readlane_i(zv, j);
-                            if (lane == (k >> SH)) S.col[kpos<KPL>(lane, k & (KPL - 1))] += 1;
-                        }
-                        if (j == n) add_run_counts<KPL>(S, p, lane, t0 + (uint64_t)n, e, v);
-                        vcur = v;
+                // tokens continuing the previous batch's last run
+                const int pre = starts ? __builtin_ctzll(starts) : n;
+                if ((unsigned)vcur < (unsigned)p.V) {
+                    for (int t = 0; t < pre; ++t) {
+                        const int zn = fold_token<KPL>(A, col, nk, bf, y, alw, lane, readlane_i(zv, t), readlane_f(uv, t),
+                                                       p.K, nvalid, lastLane, p.eta, p.veta);
+                        zkeep = (lane == t) ? zn : zkeep;
                     }
-                    if (i + 1 < n) {
-                        const int v2 = readlane_i(wv, i + 1);
-                        if (v2 != v && (unsigned)v2 < (unsigned)p.V) {
-                            load_kvec<KPL>(p.nkw + (int64_t)v2 * p.Kp, lane, ncol);
-                            vnext = v2;
-                        }
-                    }
-                    const int zo = readlane_i(zv, i);
-                    const float u = readlane_f(uv, i);
-                    const int zn = token_step<KPL>(S, al, lane, zo, u, p.K, nvalid, lastLane, p.eta, p.veta, nullptr);
-                    zkeep = (lane == i) ? zn : zkeep;
                 }
-                if (lane < n) p.z[g] = (uint16_t)zkeep;
+                // Runs in token order: run r takes its column from slot r % P
+                // and refills that slot with run r + P.  The unrolled q loop
+                // keeps every slot index a constant.
+                unsigned long long todo = starts;
+                while (todo) {
+#pragma unroll
+                    for (int q = 0; q < P; ++q) {
+                        if (todo) {
+                            const int i0 = __builtin_ctzll(todo);
+                            todo &= todo - 1;
+                            const int i1 = todo ? __builtin_ctzll(todo) : n;
+                            const int v = readlane_i(wv, i0);
+                            vcur = v;
+#pragma unroll
+                            for (int j = 0; j < KPL; ++j) col[j] = slot[q][j];
+                            fold_issue<KPL>(slot[q], ahead, wv, nkw, nkv, p.V, p.Kp, lane);
+                            if ((unsigned)v < (unsigned)p.V) {
+                                // c_d of this word: the run's topics from the batch
+                                // registers (not yet resampled in this iteration)
+                                // and, past the batch, from memory
+                                for (int j = i0; j < i1; ++j) col_add<KPL>(col, lane, readlane_i(zv, j));
+                                if (i1 == n) fold_run_counts<KPL>(col, words, zs, lane, t0 + (uint64_t)n, se, v);
+                                for (int t = i0; t < i1; ++t) {
+                                    const int zn = fold_token<KPL>(A, col, nk, bf, y, alw, lane, readlane_i(zv, t),
+                                                                   readlane_f(uv, t), p.K, nvalid, lastLane, p.eta,
+                                                                   p.veta);
+                                    zkeep = (lane == t) ? zn : zkeep;
+                                }
+                            }
+                        }
+                    }
+                }
+                if (lane < n) zs[g] = (uint16_t)zkeep;
             }
-            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+            if constexpr (S > 1) {
+                if (it < it_end) {
+                    __syncthreads();  // every segment has read nd_sh for this iteration
+#pragma unroll
+                    for (int j = 0; j < KPL; ++j) {
+                        const int dlt = (int)__builtin_rintf(A[j] - al[j]) - snap[j];
+                        if (dlt) atomicAdd(nd_sh + kpos<KPL>(lane, j), dlt);
+                    }
+                }
+            }
         }
-        if (!p.counter) break;
+        if constexpr (S > 1) __syncthreads();  // nd_sh and job_sh are reused by the next job
+        if (!p.counters) break;
     }
 }
 
@@ -920,9 +1172,9 @@ __global__ void __launch_bounds__(64) infer_finalize_kernel(const InferParams* _
     ivec<KPL> nk0;
     load_kvec<KPL>(p.nk, lane, nk0);
     // batched: all jobs, grid-stride; plain (no counter): the launch's own job only
-    const int64_t first = p.counter ? (int64_t)blockIdx.x : p.job_base + blockIdx.x;
-    const int64_t last = p.counter ? p.njobs : (first + 1 < p.njobs ? first + 1 : p.njobs);
-    const int64_t stride = p.counter ? (int64_t)gridDim.x : 1;
+    const int64_t first = p.counters ? (int64_t)blockIdx.x : p.job_base + blockIdx.x;
+    const int64_t last = p.counters ? p.njobs : (first + 1 < p.njobs ? first + 1 : p.njobs);
+    const int64_t stride = p.counters ? (int64_t)gridDim.x : 1;
     for (int64_t idx = first; idx < last; idx += stride) {
         const InferJob J = p.jobs[idx];
         if (J.it_end < J.iters) continue;
@@ -1075,19 +1327,31 @@ int sweep_device_blocks_per_cu(int kpl) {
     return (e == hipSuccess && n > 0) ? n : 1;
 }
 
-int infer_blocks_per_cu(int kpl) {
+template <int KPL>
+static int infer_blocks_kpl(int segs) {
+    constexpr size_t lds = (64 * KPL + 4) * sizeof(int);
     int n = 0;
     hipError_t e = hipErrorInvalidValue;
-    switch (kpl) {
-        case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_kernel<1>, 64, wave_lds_bytes<1>()); break;
-        case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_kernel<2>, 64, wave_lds_bytes<2>()); break;
-        case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_kernel<4>, 64, wave_lds_bytes<4>()); break;
-        case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_kernel<8>, 64, wave_lds_bytes<8>()); break;
-        case 16: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_kernel<16>, 64, wave_lds_bytes<16>()); break;
-        case 32: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_kernel<32>, 64, wave_lds_bytes<32>()); break;
+    switch (segs) {
+        case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_kernel<KPL, 1>, 64, 0); break;
+        case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_kernel<KPL, 2>, 128, lds); break;
+        case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_kernel<KPL, 4>, 256, lds); break;
+        case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_kernel<KPL, 8>, 512, lds); break;
         default: break;
     }
-    return (e == hipSuccess && n > 0) ? n : 1;
+    return e == hipSuccess ? n : 0;
+}
+
+int infer_blocks_per_cu(int kpl, int segs) {
+    switch (kpl) {
+        case 1: return infer_blocks_kpl<1>(segs);
+        case 2: return infer_blocks_kpl<2>(segs);
+        case 4: return infer_blocks_kpl<4>(segs);
+        case 8: return infer_blocks_kpl<8>(segs);
+        case 16: return infer_blocks_kpl<16>(segs);
+        case 32: return infer_blocks_kpl<32>(segs);
+        default: return 0;
+    }
 }
 
 hipError_t launch_copy_i32(int32_t* dst, const int32_t* src, int64_t n, hipStream_t s) {
@@ -1162,9 +1426,22 @@ hipError_t launch_sample_frozen(int kpl, const int32_t* ndk, const int32_t* nkw,
     return hipGetLastError();
 }
 
-hipError_t launch_infer(int kpl, const InferParams* p_dev, int grid, hipStream_t s) {
-    LDA_DISPATCH_KPL(kpl, hipLaunchKernelGGL(infer_kernel<KPL>, dim3(grid), dim3(64), wave_lds_bytes<KPL>(), s, p_dev));
+template <int KPL>
+static hipError_t launch_infer_kpl(int segs, const InferParams* p_dev, int grid, hipStream_t s) {
+    constexpr size_t lds = (64 * KPL + 4) * sizeof(int);
+    switch (segs) {
+        case 1: hipLaunchKernelGGL((infer_kernel<KPL, 1>), dim3(grid), dim3(64), 0, s, p_dev); break;
+        case 2: hipLaunchKernelGGL((infer_kernel<KPL, 2>), dim3(grid), dim3(128), lds, s, p_dev); break;
+        case 4: hipLaunchKernelGGL((infer_kernel<KPL, 4>), dim3(grid), dim3(256), lds, s, p_dev); break;
+        case 8: hipLaunchKernelGGL((infer_kernel<KPL, 8>), dim3(grid), dim3(512), lds, s, p_dev); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
+}
+
+hipError_t launch_infer(int kpl, int segs, const InferParams* p_dev, int grid, hipStream_t s) {
+    LDA_DISPATCH_KPL(kpl, return launch_infer_kpl<KPL>(segs, p_dev, grid, s));
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_infer_finalize(int kpl, const InferParams* p_dev, int grid, hipStream_t s) {

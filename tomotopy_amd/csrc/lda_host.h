@@ -49,6 +49,22 @@ struct DevBuf {
 constexpr uint64_t kMaxDocTokens = 1u << 20;
 constexpr float kMaxAlpha = (float)(1u << 20);
 
+// Streams and events to run the fold-in buckets of one launch concurrently:
+// bucket 0 on the caller's stream, bucket b > 0 on aux[b-1], forked from and
+// joined back into the caller's stream (also under graph capture).
+struct FoldStreams {
+    hipStream_t aux[lda::kFoldBuckets - 1] = {};
+    hipEvent_t fork = nullptr;
+    hipEvent_t join[lda::kFoldBuckets - 1] = {};
+    hipError_t create();
+    void destroy();
+};
+
+// Launches the fold-in kernel for every bucket b with grids[b] > 0 (segments
+// 1 << b per document) on the streams of fs, joined back into s.
+hipError_t launch_fold_buckets(int kpl, const lda::InferParams* p_dev, const int grids[lda::kFoldBuckets],
+                               hipStream_t s, FoldStreams& fs);
+
 }  // namespace lda_host
 
 using lda_host::DevBuf;
@@ -82,6 +98,14 @@ struct lda_ctx {
     DevBuf<double> d_part;
     int part_W = 0;
 
+    // fold-in: segmentation (fold_segments), the largest segment count the
+    // kernels fit, persistent grid per bucket and the bucket streams
+    uint32_t seg_tokens = 0;
+    int seg_max = 1;
+    int seg_supported = 1;
+    int fold_grid[lda::kFoldBuckets] = {};
+    lda_host::FoldStreams fold_streams;
+
     // fold-in buffers
     DevBuf<uint32_t> i_words, i_counter;
     DevBuf<lda::InferJob> i_jobs;
@@ -91,7 +115,7 @@ struct lda_ctx {
     DevBuf<double> i_ll;
     // batched fold-in graph: counter reset + persistent kernel reading i_params
     hipGraphExec_t infer_exec = nullptr;
-    int infer_exec_grid = 0;
+    int infer_exec_segmax = 0;  // the buckets the cached graph launches
 
     // kernel timing of the last call
     double stat_ms = 0.0;

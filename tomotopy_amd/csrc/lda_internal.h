@@ -52,13 +52,35 @@ struct InferJob {
     int32_t it_begin, it_end, iters, slot;
 };
 
+// Segmented fold-in.  A document of n tokens is split into S segments, one
+// wavefront each, S = 1 when seg_tokens == 0 or n <= seg_tokens, else the
+// smallest power of two >= ceil(n / seg_tokens), at most seg_max.  Segment
+// w starts at the first word-run start at or after floor(n*w/S), so every
+// word's tokens lie in one segment.  Within an iteration each segment samples
+// its tokens in order against n_dk as it was when the iteration started plus
+// its own changes; the segments' changes are summed after every iteration.
+// S == 1 is the sequential sampler.  Jobs are grouped by S into buckets
+// S = 1, 2, 4, 8 (bucket log2 S), one launch each.
+constexpr int kFoldBuckets = 4;
+
+inline int fold_segments(uint64_t n, uint32_t seg_tokens, int seg_max) {
+    if (seg_tokens == 0 || n <= seg_tokens) return 1;
+    const uint64_t need = (n + seg_tokens - 1) / seg_tokens;
+    int S = 1;
+    while ((uint64_t)S < need && S < seg_max) S <<= 1;
+    return S;
+}
+
+inline int fold_bucket(int S) { return S >= 8 ? 3 : S >= 4 ? 2 : S >= 2 ? 1 : 0; }
+
 struct InferParams {
     const uint32_t* words;   // token arena
     uint16_t* z;             // token arena
-    const InferJob* jobs;
+    const InferJob* jobs;    // grouped by bucket
     int64_t njobs;
-    uint32_t* counter;       // next job; NULL: job = job_base + blockIdx.x, one per wavefront
+    uint32_t* counters;      // next job per bucket; NULL: the launch runs job job_base only
     int64_t job_base;
+    int64_t bucket_lo[kFoldBuckets], bucket_hi[kFoldBuckets];  // job index range of each bucket
     const int32_t* nkw;      // frozen V x Kp
     const int32_t* nk;       // frozen Kp
     const float* alpha;      // Kp
@@ -113,9 +135,11 @@ hipError_t launch_sample_frozen(int kpl, const int32_t* ndk, const int32_t* nkw,
 
 // fold-in; the kernel reads its parameters from device memory so that one
 // captured graph serves every batch
-hipError_t launch_infer(int kpl, const InferParams* p_dev, int grid, hipStream_t s);
+// segs: wavefronts per document (1, 2, 4 or 8), the bucket this launch serves
+hipError_t launch_infer(int kpl, int segs, const InferParams* p_dev, int grid, hipStream_t s);
 // theta and FP64 log-likelihood of the jobs that finish (it_end == iters)
 hipError_t launch_infer_finalize(int kpl, const InferParams* p_dev, int grid, hipStream_t s);
-int infer_blocks_per_cu(int kpl);
+// resident workgroups per CU of the fold-in kernel (0: this segment count does not fit)
+int infer_blocks_per_cu(int kpl, int segs);
 
 }  // namespace lda

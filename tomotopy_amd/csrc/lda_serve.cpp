@@ -57,6 +57,7 @@ struct Doc {
     int index;   // document index inside its request
     uint64_t b;  // token range [b, b + len) in the arena
     uint32_t len;
+    int segs;    // wavefronts sampling it (lda::fold_segments)
     int it_done;
 };
 
@@ -71,7 +72,10 @@ struct lda_server {
     DevBuf<uint16_t> z;
     DevBuf<lda::InferJob> jobs;
     DevBuf<lda::InferParams> params;
-    DevBuf<uint32_t> counter;
+    DevBuf<uint32_t> counter;  // one per bucket
+    lda_host::FoldStreams fs;
+    uint32_t seg_tokens = 0;
+    int seg_max = 1;
     DevBuf<float> theta;  // max_active x K: the finished documents of one step
     DevBuf<double> ll;
     // pinned staging
@@ -81,7 +85,7 @@ struct lda_server {
     double* h_ll = nullptr;
     uint64_t arena_cap = 0;
     int max_active = 0;
-    int grid = 0;
+    int grid[lda::kFoldBuckets] = {};
     std::map<uint64_t, uint64_t> free_ranges;  // arena offset -> length
     std::deque<uint64_t> queue;                // submitted, not admitted
     std::unordered_map<uint64_t, Req> reqs;
@@ -131,18 +135,21 @@ void arena_free(lda_server* s, uint64_t off, uint64_t len) {
     }
 }
 
-// One step on the device over the staged jobs (longest first); the first
-// nfin output rows (the documents this step finishes) come back to the host.
+// One step on the device over the staged jobs (grouped by bucket, longest
+// first inside a bucket; h_params holds the bucket ranges); the first nfin
+// output rows (the documents this step finishes) come back to the host.
 int run_step(lda_server* s, int64_t njobs, int nfin) {
     lda_ctx* c = s->ctx;
     const hipStream_t st = s->stream;
     s->h_params->njobs = njobs;
+    int grids[lda::kFoldBuckets];
+    for (int b = 0; b < lda::kFoldBuckets; ++b)
+        grids[b] = (int)std::min<int64_t>(s->grid[b], s->h_params->bucket_hi[b] - s->h_params->bucket_lo[b]);
     HIP_TRY(hipMemcpyAsync(s->params.p, s->h_params, sizeof(lda::InferParams), hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(s->jobs.p, s->h_jobs, njobs * sizeof(lda::InferJob), hipMemcpyHostToDevice, st));
-    HIP_TRY(lda::launch_fill_u32(s->counter.p, 0u, 1, st));
-    const int grid = (int)std::min<int64_t>(s->grid, njobs);
-    HIP_TRY(lda::launch_infer(c->kpl, s->params.p, grid, st));
-    HIP_TRY(lda::launch_infer_finalize(c->kpl, s->params.p, grid, st));
+    HIP_TRY(lda::launch_fill_u32(s->counter.p, 0u, lda::kFoldBuckets, st));
+    HIP_TRY(lda_host::launch_fold_buckets(c->kpl, s->params.p, grids, st, s->fs));
+    HIP_TRY(lda::launch_infer_finalize(c->kpl, s->params.p, (int)std::min<int64_t>(s->grid[0], njobs), st));
     if (nfin) {
         HIP_TRY(hipMemcpyAsync(s->h_theta, s->theta.p, (size_t)nfin * c->K * sizeof(float), hipMemcpyDeviceToHost, st));
         HIP_TRY(hipMemcpyAsync(s->h_ll, s->ll.p, (size_t)nfin * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -161,8 +168,10 @@ void admit(lda_server* s, std::vector<Doc>& newly, std::vector<Req*>& uploads) {
         uint64_t base = 0;
         if (n && !arena_alloc(s, n, &base)) break;
         r.arena_base = base;
-        for (int i = 0; i < r.ndocs; ++i)
-            newly.push_back(Doc{id, i, base + r.off[i], (uint32_t)(r.off[i + 1] - r.off[i]), 0});
+        for (int i = 0; i < r.ndocs; ++i) {
+            const uint32_t len = (uint32_t)(r.off[i + 1] - r.off[i]);
+            newly.push_back(Doc{id, i, base + r.off[i], len, lda::fold_segments(len, s->seg_tokens, s->seg_max), 0});
+        }
         uploads.push_back(&r);
         s->queue.pop_front();
     }
@@ -213,23 +222,37 @@ void worker_loop(lda_server* s) {
                 break;
             }
         }
-        // This step's share of each document's iterations.  Jobs run longest
-        // first; documents this step finishes take output rows 0..nfin-1.
+        // This step's share of each document's iterations: at most step_tokens
+        // token-iterations per wavefront.  Jobs are grouped by bucket and run
+        // longest first inside a bucket; documents this step finishes take
+        // output rows 0..nfin-1.
         const int64_t nb = (int64_t)batch.size();
         std::vector<int> it_end(nb);
         std::vector<int64_t> work(nb);
         int nfin = 0;
         for (int64_t i = 0; i < nb; ++i) {
             const Doc& d = batch[i];
-            int64_t chunk = s->step_tokens / std::max<int64_t>(d.len, 1);
+            const int64_t per_wave = std::max<int64_t>(((int64_t)d.len + d.segs - 1) / d.segs, 1);
+            int64_t chunk = s->step_tokens / per_wave;
             chunk = std::max<int64_t>(1, std::min<int64_t>(chunk, s->step_iters));
             it_end[i] = (int)std::min<int64_t>(d.it_done + chunk, s->iterations);
-            work[i] = (int64_t)d.len * (it_end[i] - d.it_done);
+            work[i] = per_wave * (it_end[i] - d.it_done);
             if (it_end[i] >= s->iterations) ++nfin;
         }
         std::vector<int64_t> order(nb);
         std::iota(order.begin(), order.end(), 0);
-        std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return work[a] > work[b]; });
+        std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+            if (batch[a].segs != batch[b].segs) return batch[a].segs < batch[b].segs;
+            return work[a] > work[b];
+        });
+        {
+            int64_t q = 0;
+            for (int bk = 0; bk < lda::kFoldBuckets; ++bk) {
+                s->h_params->bucket_lo[bk] = q;
+                while (q < nb && lda::fold_bucket(batch[order[q]].segs) == bk) ++q;
+                s->h_params->bucket_hi[bk] = q;
+            }
+        }
         std::vector<int> out_row(nb, -1);
         int next_row = 0;
         for (int64_t q = 0; q < nb; ++q) {
@@ -285,6 +308,7 @@ void release(lda_server* s) {
     if (s->h_jobs) (void)hipHostFree(s->h_jobs);
     if (s->h_theta) (void)hipHostFree(s->h_theta);
     if (s->h_ll) (void)hipHostFree(s->h_ll);
+    s->fs.destroy();
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
 }
@@ -317,12 +341,14 @@ int lda_serve_start(lda_ctx* ctx, uint64_t hbm_budget_bytes, int iterations, int
         release(s);
         return fail(LDA_ERR_INVALID_ARGUMENT, "HBM budget leaves no room for the token arena");
     }
-    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess || s->fs.create() != hipSuccess) {
         release(s);
-        return fail(LDA_ERR_DEVICE, "hipStreamCreate failed");
+        return fail(LDA_ERR_DEVICE, "stream creation failed");
     }
+    s->seg_tokens = ctx->seg_tokens;
+    s->seg_max = ctx->seg_max;
     if (s->words.alloc(s->arena_cap) || s->z.alloc(s->arena_cap) || s->jobs.alloc(s->max_active) || s->params.alloc(1) ||
-        s->counter.alloc(1) || s->theta.alloc((size_t)s->max_active * ctx->K) || s->ll.alloc(s->max_active) ||
+        s->counter.alloc(lda::kFoldBuckets) || s->theta.alloc((size_t)s->max_active * ctx->K) || s->ll.alloc(s->max_active) ||
         hipHostMalloc((void**)&s->h_params, sizeof(lda::InferParams), 0) ||
         hipHostMalloc((void**)&s->h_jobs, (size_t)s->max_active * sizeof(lda::InferJob), 0) ||
         hipHostMalloc((void**)&s->h_theta, (size_t)s->max_active * ctx->K * sizeof(float), 0) ||
@@ -333,7 +359,7 @@ int lda_serve_start(lda_ctx* ctx, uint64_t hbm_budget_bytes, int iterations, int
     lda::InferParams& p = *s->h_params;
     p = lda::InferParams{};
     p.words = s->words.p; p.z = s->z.p; p.jobs = s->jobs.p; p.njobs = 0;
-    p.counter = s->counter.p; p.job_base = 0;
+    p.counters = s->counter.p; p.job_base = 0;
     p.nkw = ctx->d_nkw.p; p.nk = ctx->d_nk.p; p.alpha = ctx->d_alpha.p; p.alpha_cdf = ctx->d_alpha_cdf.p;
     p.eta = ctx->eta; p.veta = ctx->veta; p.K = ctx->K; p.V = ctx->V; p.Kp = ctx->Kp;
     p.seed_lo = (uint32_t)infer_seed; p.seed_hi = (uint32_t)(infer_seed >> 32);
@@ -343,7 +369,7 @@ int lda_serve_start(lda_ctx* ctx, uint64_t hbm_budget_bytes, int iterations, int
         p.asum_d += (double)ctx->alpha[k];
         p.asum_f = p.asum_f + ctx->alpha[k];
     }
-    s->grid = ctx->num_cu * lda::infer_blocks_per_cu(ctx->kpl);
+    for (int b = 0; b < lda::kFoldBuckets; ++b) s->grid[b] = ctx->fold_grid[b];
     s->free_ranges[0] = s->arena_cap;
     s->worker = std::thread(worker_loop, s);
     *out = s;
