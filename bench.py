@@ -80,9 +80,12 @@ def cpu_baseline(ws, offs, V, K, alpha, eta, budget_s=12.0):
 
 
 def infer_bench(plain_sample=300, iterations=100):
-    """Fold-in: batched (one persistent launch in a replayed hipGraph) vs plain
-    (one launch + synchronisation per document) against a frozen K=200 model."""
-    from tomotopy_amd import _native, corpus, synth
+    """Fold-in: batched (one persistent launch per segment bucket, replayed
+    from a hipGraph) with the autotuned tokens per wavefront, against plain
+    (one launch + synchronisation per document, same segmentation) and
+    against the sequential sampler (no segmentation) for ll_per_word, on a
+    frozen K=200 model."""
+    from tomotopy_amd import _native, autotune, corpus, synth
     t = time.time()
     (tw, toff, nw), (uw, uoff) = synth.foldin_workload()
     ws, offs, V, order = corpus.prepare_arrays(tw, toff, nw)
@@ -93,7 +96,16 @@ def infer_bench(plain_sample=300, iterations=100):
     ctx = _native.Context(K, V, 0.1, 0.01, seed=77)
     ctx.upload_corpus(ws, offs)
     ctx.train(100, workers=0)
-    # warm: graph capture and first-touch allocations
+    ntok = int(np.count_nonzero(uws < V))
+    # sequential sampler (every document exactly as tomotopy samples it)
+    ctx.set_infer_options(0, 1)
+    ctx.infer(uws[:int(uoffs[64])], uoffs[:65], 2)
+    t0 = time.perf_counter()
+    _, ll_seq, _ = ctx.infer(uws, uoffs, iterations, mode=0)
+    seq_s = time.perf_counter() - t0
+    tune = autotune.tune_infer(ctx, uws, uoffs, iterations)
+    log("autotuner: seg_tokens %d seg_max %d (%s)" % (tune["seg_tokens"], tune["seg_max"], ", ".join(
+        "%d:%.0fms" % (c["seg_tokens"], c["kernel_ms"]) for c in tune["candidates"])))
     ctx.infer(uws[:int(uoffs[64])], uoffs[:65], 2)
     t0 = time.perf_counter()
     theta, ll, _ = ctx.infer(uws, uoffs, iterations, mode=0)
@@ -109,17 +121,22 @@ def infer_bench(plain_sample=300, iterations=100):
     th_p, ll_p, _ = ctx.infer(sub_w, sub_off, iterations, mode=1)
     plain_s = time.perf_counter() - t0
     same = bool(np.array_equal(th_p, theta[pick]) and np.array_equal(ll_p, ll[pick]))
-    ntok = int(np.count_nonzero(uws < V))
+    llw, llw_seq = float(ll.sum() / ntok), float(ll_seq.sum() / ntok)
     out = {
         "workload": "foldin_k200", "docs": D, "tokens": ntok, "iterations": iterations, "K": K, "V": V,
+        "seg_tokens": tune["seg_tokens"], "seg_max": tune["seg_max"], "autotune": tune,
         "batched_docs_per_s": D / batched_s, "batched_s": batched_s, "batched_kernel_ms": kst["ms"],
+        "sequential_docs_per_s": D / seq_s, "sequential_s": seq_s,
         "plain_docs_per_s": len(pick) / plain_s, "plain_sample_docs": int(len(pick)),
         "speedup": (D / batched_s) / (len(pick) / plain_s),
-        "ll_per_word": float(ll.sum() / ntok), "plain_equals_batched": same,
-        "longest_doc": int(lens.max()),
+        "ll_per_word": llw, "ll_per_word_sequential": llw_seq,
+        "ll_per_word_rel_diff": abs(llw - llw_seq) / abs(llw_seq),
+        "plain_equals_batched": same, "longest_doc": int(lens.max()),
     }
-    log("fold-in: batched %.0f docs/s (%.3fs), plain %.1f docs/s on %d sampled docs, identical=%s"
-        % (out["batched_docs_per_s"], batched_s, out["plain_docs_per_s"], len(pick), same))
+    log("fold-in: batched %.0f docs/s (%.3fs), sequential %.0f docs/s, plain %.1f docs/s on %d sampled docs, "
+        "identical=%s, ll_per_word %.5f vs sequential %.5f"
+        % (out["batched_docs_per_s"], batched_s, out["sequential_docs_per_s"], out["plain_docs_per_s"], len(pick),
+           same, llw, llw_seq))
     return out
 
 
@@ -127,7 +144,8 @@ def _percentile(x, q):
     return float(np.percentile(np.asarray(x, dtype=np.float64), q)) if len(x) else None
 
 
-def serve_bench(iterations=50, qps_list=(25, 50, 100, 200, 400), seconds=4.0, plain_requests=60, seed=2024):
+def serve_bench(iterations=50, qps_list=(25, 50, 100, 200, 400, 800), seconds=4.0, plain_requests=60, seed=2024,
+                seg_tokens=128, step_tokens=12000, lanes=1):
     """Serving loop on a device-resident K=500, V=100,000 model: Poisson request
     arrivals at each QPS (requests of 1..64 docs, lengths LogNormal(ln 150, 1)
     clipped to [5, 10000], 50 iterations); latency is submit to results on the
@@ -148,9 +166,12 @@ def serve_bench(iterations=50, qps_list=(25, 50, 100, 200, 400), seconds=4.0, pl
     ctx = _native.Context(K, V, 0.1, 0.01, seed=55)
     ctx.upload_corpus(ws, offs)
     ctx.train(50, workers=0)
-    out = {"workload": "serve_k500", "K": K, "V": V, "iterations": iterations, "levels": []}
+    seg_max = ctx.infer_options()["seg_supported"]
+    ctx.set_infer_options(seg_tokens, seg_max)
+    out = {"workload": "serve_k500", "K": K, "V": V, "iterations": iterations, "levels": [],
+           "seg_tokens": seg_tokens, "seg_max": seg_max, "step_tokens": step_tokens, "lanes": lanes}
     srv = _native.Server(ctx, hbm_budget_bytes=8 << 30, iterations=iterations, step_iters=iterations,
-                         step_tokens=20000, infer_seed=seed)
+                         step_tokens=step_tokens, infer_seed=seed, lanes=lanes)
     try:
         for i in range(4):  # warm
             srv.wait(srv.submit(*reqs[i]))
@@ -159,6 +180,7 @@ def serve_bench(iterations=50, qps_list=(25, 50, 100, 200, 400), seconds=4.0, pl
         for qps in qps_list:
             n = max(20, min(len(reqs), int(qps * seconds)))
             gaps = rng.exponential(1.0 / qps, size=n)
+            st0 = srv.stats()
             pick = rng.integers(0, len(reqs), size=n)
             ids = [None] * n
             t_start = time.perf_counter()
@@ -186,11 +208,15 @@ def serve_bench(iterations=50, qps_list=(25, 50, 100, 200, 400), seconds=4.0, pl
                 tok_sum += int(np.count_nonzero(reqs[pick[j]][0] < V))
             t_end = time.perf_counter()
             th.join()
+            st1 = srv.stats()
             lvl = {"qps": qps, "requests": n, "docs": docs, "docs_per_s": docs / (t_end - t_start),
-                   "p50_ms": _percentile(lat, 50), "p99_ms": _percentile(lat, 99), "max_ms": max(lat)}
+                   "p50_ms": _percentile(lat, 50), "p99_ms": _percentile(lat, 99), "max_ms": max(lat),
+                   "steps": st1["steps"] - st0["steps"], "step_gpu_ms": (st1["gpu_ms"] - st0["gpu_ms"]) / max(1, st1["steps"] - st0["steps"]),
+                   "step_host_ms": (st1["host_ms"] - st0["host_ms"]) / max(1, st1["steps"] - st0["steps"])}
             out["levels"].append(lvl)
-            log("serve qps %4d: %5d req, %7.0f docs/s, p50 %.1f ms, p99 %.1f ms"
-                % (qps, n, lvl["docs_per_s"], lvl["p50_ms"], lvl["p99_ms"]))
+            log("serve qps %4d: %5d req, %7.0f docs/s, p50 %.1f ms, p99 %.1f ms; %d steps, %.2f ms device + %.2f ms host each"
+                % (qps, n, lvl["docs_per_s"], lvl["p50_ms"], lvl["p99_ms"], lvl["steps"], lvl["step_gpu_ms"],
+                   lvl["step_host_ms"]))
         # saturation: every pooled request submitted at once
         t0 = time.perf_counter()
         ids = [srv.submit(*r) for r in reqs]

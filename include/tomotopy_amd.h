@@ -166,17 +166,20 @@ int lda_get_infer_options(lda_ctx* ctx, uint32_t* seg_tokens, int* seg_max, int*
  * requests.  A server keeps the context's n_kw / n_k on the device, admits
  * requests in FIFO order while their tokens fit a token arena and their
  * documents the active-document limit (both sized from hbm_budget_bytes), and
- * runs one persistent fold-in launch per step over every admitted document.
- * Per step a document advances by min(step_iters, max(1, step_tokens / length))
- * iterations; requests that arrive during a step join the next one.  Results
- * equal lda_infer(mode 0) with the same infer_seed, bit for bit.
+ * runs steps, each one persistent fold-in launch over the documents that are
+ * waiting, on `lanes` (1..4) streams: a free lane starts a step as soon as
+ * documents wait, while the other lanes' steps still run.  Per step a
+ * document advances by min(step_iters, max(1, step_tokens / tokens per
+ * wavefront)) iterations.  The server takes the context's segmentation
+ * (lda_set_infer_options) at start.  Results equal lda_infer(mode 0) with the
+ * same infer_seed, bit for bit.
  * The context must not be trained, re-uploaded or destroyed while a server on
  * it runs.  submit / wait may be called from any thread; lda_serve_stop must
  * not overlap a wait, and drops requests that were not waited for. */
 typedef struct lda_server lda_server;
 
 int lda_serve_start(lda_ctx* ctx, uint64_t hbm_budget_bytes, int iterations, int step_iters, int64_t step_tokens,
-                    uint64_t infer_seed, lda_server** out);
+                    int lanes, uint64_t infer_seed, lda_server** out);
 /* Copies the request (CSR as for lda_infer); returns its id at once. */
 int lda_serve_submit(lda_server* server, const uint32_t* words, const uint64_t* doc_off, int64_t D,
                      uint64_t* request_id);
@@ -185,10 +188,12 @@ int lda_serve_submit(lda_server* server, const uint32_t* words, const uint64_t* 
  * latency from submit to results-on-host in ms; then forgets the request. */
 int lda_serve_wait(lda_server* server, uint64_t request_id, int timeout_ms, float* theta, double* ll,
                    double* latency_ms);
-/* stats: [0] steps, [1] device-step ms, [2] documents done, [3] requests done,
- * [4] token-iterations done, [5] active documents, [6] queued requests,
- * [7] token arena capacity. */
-int lda_serve_stats(lda_server* server, double stats[8]);
+/* stats: [0] steps, [1] device-step ms (unused, 0), [2] documents done,
+ * [3] requests done, [4] token-iterations done, [5] active documents,
+ * [6] queued requests, [7] token arena capacity, [8] total step time on the
+ * device (ms, HIP events), [9] total host staging and launch time (ms),
+ * [10] longest step on the device (ms), [11] mean documents per step. */
+int lda_serve_stats(lda_server* server, double stats[12]);
 int lda_serve_stop(lda_server* server);
 
 /* Timing of the last lda_train / lda_sweep / lda_infer call, measured with

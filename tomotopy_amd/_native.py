@@ -57,7 +57,7 @@ _SIGS = {
     "lda_kernel_stats": [_c_p, _c_p],
     "lda_set_infer_options": [_c_p, _u32, _i32],
     "lda_get_infer_options": [_c_p, _c_p, _c_p, _c_p],
-    "lda_serve_start": [_c_p, _u64, _i32, _i32, _i64, _u64, _c_p],
+    "lda_serve_start": [_c_p, _u64, _i32, _i32, _i64, _i32, _u64, _c_p],
     "lda_serve_submit": [_c_p, _c_p, _c_p, _i64, _c_p],
     "lda_serve_wait": [_c_p, _u64, _i32, _c_p, _c_p, _c_p],
     "lda_serve_stats": [_c_p, _c_p],
@@ -267,13 +267,13 @@ class Server:
     """
 
     def __init__(self, ctx, hbm_budget_bytes=1 << 30, iterations=50, step_iters=50, step_tokens=1 << 30,
-                 infer_seed=0):
+                 infer_seed=0, lanes=2):
         self._ctx = ctx  # keeps the model alive while the server runs
         self.k = ctx.k
         self.iterations = int(iterations)
         h = ctypes.c_void_p()
         _check(load().lda_serve_start(ctx._h, int(hbm_budget_bytes), int(iterations), int(step_iters),
-                                      int(step_tokens), int(infer_seed) & (2**64 - 1), ctypes.byref(h)))
+                                      int(step_tokens), int(lanes), int(infer_seed) & (2**64 - 1), ctypes.byref(h)))
         self._h = h
         self._ndocs = {}
 
@@ -299,10 +299,10 @@ class Server:
         return theta, ll, lat.value
 
     def stats(self):
-        s = np.zeros(8, dtype=np.float64)
+        s = np.zeros(12, dtype=np.float64)
         _check(load().lda_serve_stats(self._h, _ptr(s)))
         keys = ("steps", "busy_ms", "docs_done", "requests_done", "token_iters", "active_docs", "queued_requests",
-                "arena_tokens")
+                "arena_tokens", "gpu_ms", "host_ms", "max_step_ms", "docs_per_step")
         return dict(zip(keys, (float(x) for x in s)))
 
     def stop(self):

@@ -824,89 +824,116 @@ __device__ __forceinline__ void add_run_counts(const WaveLds& S, const InferPara
 
 // Register state of a fold-in wavefront.  The wavefront is alone with its
 // document and shares no counts, so the whole sampler state stays in
-// registers; lane l holds topics [l*KPL, (l+1)*KPL) in the first KPL entries
-// of each vector.  A topic update indexes the vectors with a wave-uniform
-// slot; vectors of 16 or more entries make the compiler use indexed register
-// moves (s_set_gpr_idx) for that instead of a select per entry, so narrower
-// states are padded to 16.
-template <int KPL>
-constexpr int fold_vw() { return KPL < 16 ? 16 : KPL; }
+// registers: six fields of KPL entries (lane l holds topics
+// [l*KPL, (l+1)*KPL)):
+//   A    float(n_dk) + alpha_k        NK   n_k,eff = n_k + n_dk (int bits)
+//   COL  n_kw,eff = n_kw + c_d of the current word (int bits)
+//   BF   float(n_k,eff) + V*eta       Y    RN(1 / BF)      AL   alpha_k
+// A topic update indexes the state with a wave-uniform slot.  The fields are
+// packed into float vectors ("banks") of 16 or 32 entries: from 16 entries
+// on the compiler uses indexed register moves (s_set_gpr_idx, several under
+// one index setting) rather than a select per entry, and beyond 32 it falls
+// back to scratch memory.  Small KPL thus needs one bank, KPL = 32 six.
+enum FoldField { F_A = 0, F_COL = 1, F_NK = 2, F_BF = 3, F_Y = 4, F_AL = 5, F_COUNT = 6 };
 
 template <int KPL>
-struct FoldVecs {
+constexpr int fold_vw() { return F_COUNT * KPL <= 16 ? 16 : 32; }
+
+template <int KPL>
+struct FoldBanks {
     static constexpr int VW = fold_vw<KPL>();
-    using F = fvec<VW>;
-    using I = ivec<VW>;
+    static constexpr int FPV = VW / KPL;  // fields per bank
+    using V = fvec<VW>;
+    V &b0, &b1, &b2, &b3, &b4, &b5;
+
+    template <int F>
+    __device__ __forceinline__ V& bank() {
+        constexpr int i = F / FPV;
+        if constexpr (i == 0) return b0;
+        else if constexpr (i == 1) return b1;
+        else if constexpr (i == 2) return b2;
+        else if constexpr (i == 3) return b3;
+        else if constexpr (i == 4) return b4;
+        else return b5;
+    }
+    template <int F>
+    __device__ __forceinline__ float get(int sl) { return bank<F>()[(F % FPV) * KPL + sl]; }
+    template <int F>
+    __device__ __forceinline__ int geti(int sl) { return __float_as_int(get<F>(sl)); }
+    template <int F>
+    __device__ __forceinline__ void set(int sl, float x) { bank<F>()[(F % FPV) * KPL + sl] = x; }
+    template <int F>
+    __device__ __forceinline__ void seti(int sl, int x) { set<F>(sl, __int_as_float(x)); }
 };
 
-// addWordTo<+1> of topic z on the owner lane (LDAModel.hpp:287-299).  Every
+// addWordTo<delta> of topic z on the owner lane (LDAModel.hpp:287-299).  Every
 // lane writes its slot, the non-owners their old value: a store under a lane
-// condition would make the compiler copy the whole vector.
-template <int KPL, typename F, typename I>
-__device__ __forceinline__ void fold_add(F& A, I& col, I& nk, F& bf, F& y, const F& al, int lane, int z, float veta) {
+// condition would make the compiler copy the whole bank.
+template <int KPL>
+__device__ __forceinline__ void fold_bump(FoldBanks<KPL>& st, int lane, int z, int delta, float veta) {
     const int sl = z & (KPL - 1);
     const bool own = lane == (z >> kpl_shift<KPL>());
-    const float a = al[sl], oA = A[sl], ob = bf[sl], oy = y[sl];
-    const int oc = col[sl], on = nk[sl];
-    const int d = (int)__builtin_rintf(oA - a) + 1;
-    const int n = on + 1;
+    const float a = st.template get<F_AL>(sl), oA = st.template get<F_A>(sl);
+    const float ob = st.template get<F_BF>(sl), oy = st.template get<F_Y>(sl);
+    const int oc = st.template geti<F_COL>(sl), on = st.template geti<F_NK>(sl);
+    const int d = (int)__builtin_rintf(oA - a) + delta;
+    const int n = on + delta;
     const float b = (float)n + veta;
     const float yn = 1.0f / b;
-    A[sl] = own ? (float)d + a : oA;
-    col[sl] = own ? oc + 1 : oc;
-    nk[sl] = own ? n : on;
-    bf[sl] = own ? b : ob;
-    y[sl] = own ? yn : oy;
+    st.template set<F_A>(sl, own ? (float)d + a : oA);
+    st.template seti<F_COL>(sl, own ? oc + delta : oc);
+    st.template seti<F_NK>(sl, own ? n : on);
+    st.template set<F_BF>(sl, own ? b : ob);
+    st.template set<F_Y>(sl, own ? yn : oy);
+}
+
+// Weight of one topic from its cells: ((n_dk + alpha) * (n_kw + eta)) /
+// (n_k + V*eta), the quotient as q0 = a*y, r = fma(-q0, b, a),
+// q = fma(r, y, q0) with y = RN(1/b), which is the IEEE quotient.
+__device__ __forceinline__ float fold_weight(float A, int col, float bf, float y, float eta) {
+    const float B = (float)col + eta;
+    const float a = A * B;
+    const float q0 = a * y;
+    const float rr = __builtin_fmaf(-q0, bf, a);
+    return __builtin_fmaf(rr, y, q0);
 }
 
 // One fold-in token: remove z_old (addWordTo<-1>), weight, scan, search, add
 // z_new (addWordTo<+1>).
-template <int KPL, typename F, typename I>
-__device__ __forceinline__ int fold_token(F& A, I& col, I& nk, F& bf, F& y, const F& al, int lane, int zo, float u,
-                                          int K, int nvalid, int lastLane, float eta, float veta) {
-    {
-        const int sl = zo & (KPL - 1);
-        const bool own = lane == (zo >> kpl_shift<KPL>());
-        const float a = al[sl], oA = A[sl], ob = bf[sl], oy = y[sl];
-        const int oc = col[sl], on = nk[sl];
-        const int d = (int)__builtin_rintf(oA - a) - 1;
-        const int n = on - 1;
-        const float b = (float)n + veta;
-        const float yn = 1.0f / b;
-        A[sl] = own ? (float)d + a : oA;
-        col[sl] = own ? oc - 1 : oc;
-        nk[sl] = own ? n : on;
-        bf[sl] = own ? b : ob;
-        y[sl] = own ? yn : oy;
-    }
+template <int KPL>
+__device__ __forceinline__ int fold_token(FoldBanks<KPL>& st, int lane, int zo, float u, int K, int nvalid,
+                                          int lastLane, float eta, float veta) {
+    fold_bump<KPL>(st, lane, zo, -1, veta);
     fvec<KPL> w;
 #pragma unroll
-    for (int j = 0; j < KPL; ++j) {
-        // ((n_dk + alpha) * (n_kw + eta)) / (n_k + V*eta), the quotient as
-        // q0 = a*y, r = fma(-q0, b, a), q = fma(r, y, q0) with y = RN(1/b)
-        const float B = (float)col[j] + eta;
-        const float a = A[j] * B;
-        const float q0 = a * y[j];
-        const float rr = __builtin_fmaf(-q0, bf[j], a);
-        w[j] = __builtin_fmaf(rr, y[j], q0);
-    }
+    for (int j = 0; j < KPL; ++j)
+        w[j] = fold_weight(st.template get<F_A>(j), st.template geti<F_COL>(j), st.template get<F_BF>(j),
+                           st.template get<F_Y>(j), eta);
     float tot;
     const int zn = scan_search<KPL>(w, u, K, nvalid, lastLane, tot);
-    fold_add<KPL>(A, col, nk, bf, y, al, lane, zn, veta);
+    fold_bump<KPL>(st, lane, zn, +1, veta);
     return zn;
 }
 
-// Adds one count for topic k (owner lane).
-template <int KPL, typename I>
-__device__ __forceinline__ void col_add(I& col, int lane, int k) {
+// Adds one count for topic k (owner lane) to field COL.
+template <int KPL>
+__device__ __forceinline__ void col_add(FoldBanks<KPL>& st, int lane, int k) {
     const int sl = k & (KPL - 1);
-    col[sl] = col[sl] + (lane == (k >> kpl_shift<KPL>()) ? 1 : 0);
+    const int c = st.template geti<F_COL>(sl);
+    st.template seti<F_COL>(sl, c + (lane == (k >> kpl_shift<KPL>()) ? 1 : 0));
+}
+
+// Adds one count for topic k (owner lane) to a KPL-entry histogram.
+template <int KPL>
+__device__ __forceinline__ void hist_add(ivec<fold_vw<KPL>()>& h, int lane, int k) {
+    const int sl = k & (KPL - 1);
+    h[sl] = h[sl] + (lane == (k >> kpl_shift<KPL>()) ? 1 : 0);
 }
 
 // c_d of word v from the tokens [g, e) of its run that lie past the current
 // batch of 64 (words are sorted, so the run is contiguous).
-template <int KPL, typename I>
-__device__ __forceinline__ void fold_run_counts(I& col, const gbl<const uint32_t>* words,
+template <int KPL>
+__device__ __forceinline__ void fold_run_counts(FoldBanks<KPL>& st, const gbl<const uint32_t>* words,
                                                 const gbl<uint16_t>* z, int lane, uint64_t g, uint64_t e, int v) {
     for (uint64_t t0 = g; t0 < e; t0 += 64) {
         const uint64_t gg = t0 + (uint64_t)lane;
@@ -914,7 +941,7 @@ __device__ __forceinline__ void fold_run_counts(I& col, const gbl<const uint32_t
         const unsigned long long m = __ballot(in);
         const int cnt = (m == ~0ull) ? 64 : __builtin_ctzll(~m);
         const int zz = in ? (int)z[gg] : 0;
-        for (int i = 0; i < cnt; ++i) col_add<KPL>(col, lane, readlane_i(zz, i));
+        for (int i = 0; i < cnt; ++i) col_add<KPL>(st, lane, readlane_i(zz, i));
         if (cnt < 64) break;
     }
 }
@@ -976,8 +1003,7 @@ __global__ void __launch_bounds__(64 * S) infer_kernel(const InferParams* __rest
     constexpr int P = fold_prefetch<KPL>();
     constexpr int Kp = 64 * KPL;
     constexpr int bucket = S >= 8 ? 3 : S >= 4 ? 2 : S >= 2 ? 1 : 0;
-    using FV = typename FoldVecs<KPL>::F;
-    using IV = typename FoldVecs<KPL>::I;
+    constexpr int VW = fold_vw<KPL>();
     const InferParams p = *pp;
     const gbl<const uint32_t>* words = gptr(p.words);
     gbl<uint16_t>* zs = gptr(p.z);
@@ -995,9 +1021,6 @@ __global__ void __launch_bounds__(64 * S) infer_kernel(const InferParams* __rest
     load_kvec_f<KPL>(p.alpha, lane, al);
     ivec<KPL> nk0;
     load_kvec<KPL>(p.nk, lane, nk0);
-    FV alw = 0.0f;
-#pragma unroll
-    for (int j = 0; j < KPL; ++j) alw[j] = al[j];
     const int64_t lo = p.counters ? p.bucket_lo[bucket] : 0;
     const int64_t hi = p.counters ? p.bucket_hi[bucket] : p.njobs;
     for (;;) {
@@ -1025,7 +1048,7 @@ __global__ void __launch_bounds__(64 * S) infer_kernel(const InferParams* __rest
             __syncthreads();
         }
         // topics at the start of this launch (initial draw or the stored z), n_dk from them
-        IV nd = 0;
+        ivec<VW> nd = 0;
         for (uint64_t t0 = sb; t0 < se; t0 += 64) {
             const uint64_t g = t0 + (uint64_t)lane;
             int zz = 0xFFFF;
@@ -1049,7 +1072,7 @@ __global__ void __launch_bounds__(64 * S) infer_kernel(const InferParams* __rest
             const int n = (int)((se - t0) < 64 ? (se - t0) : 64);
             for (int i = 0; i < n; ++i) {
                 const int k = readlane_i(zz, i);
-                if (k != 0xFFFF) col_add<KPL>(nd, lane, k);
+                if (k != 0xFFFF) hist_add<KPL>(nd, lane, k);
             }
         }
         ivec<KPL> snap;  // n_dk at the start of the iteration
@@ -1058,8 +1081,10 @@ __global__ void __launch_bounds__(64 * S) infer_kernel(const InferParams* __rest
             for (int j = 0; j < KPL; ++j)
                 if (nd[j]) atomicAdd(nd_sh + kpos<KPL>(lane, j), nd[j]);
         }
-        FV A = 0.0f, bf = 1.0f, y = 1.0f;
-        IV col = 0, nk = 0;
+        fvec<VW> b0 = 0.0f, b1 = 0.0f, b2 = 0.0f, b3 = 0.0f, b4 = 0.0f, b5 = 0.0f;
+        FoldBanks<KPL> st{b0, b1, b2, b3, b4, b5};
+#pragma unroll
+        for (int j = 0; j < KPL; ++j) st.template set<F_AL>(j, al[j]);
         for (int it = it_begin + 1; it <= it_end; ++it) {
             if constexpr (S > 1) {
                 __syncthreads();  // every segment's change is in nd_sh
@@ -1071,10 +1096,12 @@ __global__ void __launch_bounds__(64 * S) infer_kernel(const InferParams* __rest
             if (S > 1 || it == it_begin + 1) {
 #pragma unroll
                 for (int j = 0; j < KPL; ++j) {
-                    A[j] = (float)snap[j] + al[j];
-                    nk[j] = nk0[j] + snap[j];
-                    bf[j] = (float)nk[j] + p.veta;
-                    y[j] = 1.0f / bf[j];
+                    const int n = nk0[j] + snap[j];
+                    const float bb = (float)n + p.veta;
+                    st.template set<F_A>(j, (float)snap[j] + al[j]);
+                    st.template seti<F_NK>(j, n);
+                    st.template set<F_BF>(j, bb);
+                    st.template set<F_Y>(j, 1.0f / bb);
                 }
             }
             int vcur = -1;
@@ -1100,8 +1127,8 @@ __global__ void __launch_bounds__(64 * S) infer_kernel(const InferParams* __rest
                 const int pre = starts ? __builtin_ctzll(starts) : n;
                 if ((unsigned)vcur < (unsigned)p.V) {
                     for (int t = 0; t < pre; ++t) {
-                        const int zn = fold_token<KPL>(A, col, nk, bf, y, alw, lane, readlane_i(zv, t), readlane_f(uv, t),
-                                                       p.K, nvalid, lastLane, p.eta, p.veta);
+                        const int zn = fold_token<KPL>(st, lane, readlane_i(zv, t), readlane_f(uv, t), p.K, nvalid,
+                                                       lastLane, p.eta, p.veta);
                         zkeep = (lane == t) ? zn : zkeep;
                     }
                 }
@@ -1119,18 +1146,17 @@ __global__ void __launch_bounds__(64 * S) infer_kernel(const InferParams* __rest
                             const int v = readlane_i(wv, i0);
                             vcur = v;
 #pragma unroll
-                            for (int j = 0; j < KPL; ++j) col[j] = slot[q][j];
+                            for (int j = 0; j < KPL; ++j) st.template seti<F_COL>(j, slot[q][j]);
                             fold_issue<KPL>(slot[q], ahead, wv, nkw, nkv, p.V, p.Kp, lane);
                             if ((unsigned)v < (unsigned)p.V) {
                                 // c_d of this word: the run's topics from the batch
                                 // registers (not yet resampled in this iteration)
                                 // and, past the batch, from memory
-                                for (int j = i0; j < i1; ++j) col_add<KPL>(col, lane, readlane_i(zv, j));
-                                if (i1 == n) fold_run_counts<KPL>(col, words, zs, lane, t0 + (uint64_t)n, se, v);
+                                for (int j = i0; j < i1; ++j) col_add<KPL>(st, lane, readlane_i(zv, j));
+                                if (i1 == n) fold_run_counts<KPL>(st, words, zs, lane, t0 + (uint64_t)n, se, v);
                                 for (int t = i0; t < i1; ++t) {
-                                    const int zn = fold_token<KPL>(A, col, nk, bf, y, alw, lane, readlane_i(zv, t),
-                                                                   readlane_f(uv, t), p.K, nvalid, lastLane, p.eta,
-                                                                   p.veta);
+                                    const int zn = fold_token<KPL>(st, lane, readlane_i(zv, t), readlane_f(uv, t), p.K,
+                                                                   nvalid, lastLane, p.eta, p.veta);
                                     zkeep = (lane == t) ? zn : zkeep;
                                 }
                             }
@@ -1144,7 +1170,7 @@ __global__ void __launch_bounds__(64 * S) infer_kernel(const InferParams* __rest
                     __syncthreads();  // every segment has read nd_sh for this iteration
 #pragma unroll
                     for (int j = 0; j < KPL; ++j) {
-                        const int dlt = (int)__builtin_rintf(A[j] - al[j]) - snap[j];
+                        const int dlt = (int)__builtin_rintf(st.template get<F_A>(j) - al[j]) - snap[j];
                         if (dlt) atomicAdd(nd_sh + kpos<KPL>(lane, j), dlt);
                     }
                 }

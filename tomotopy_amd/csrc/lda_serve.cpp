@@ -6,12 +6,16 @@
 //   * admission: requests are taken in FIFO order once their tokens fit the
 //     token arena (first fit over a free list) and their documents fit the
 //     active-document limit; both are sized from the HBM budget;
-//   * continuous batching: every step is one persistent fold-in launch over
-//     all admitted documents.  A document advances by up to step_iters
-//     iterations, fewer when it is long (at most step_tokens
-//     token-iterations per document per step), so one long document does not
-//     hold the others back; requests that arrive during a step join at the
-//     next one, and finished documents leave after the step that ends them;
+//   * continuous batching: a step is one persistent fold-in launch over a set
+//     of admitted documents.  Steps run on kLanes lanes (streams with their
+//     own buffers); whenever a lane is free and documents are waiting, the
+//     waiting documents (new arrivals and those whose previous step ended)
+//     start a step on it, while the other lanes' steps still run.  A
+//     document advances by up to step_iters iterations per step, fewer when
+//     it is long (at most step_tokens token-iterations per wavefront), so one
+//     long document does not hold the others back; finished documents leave
+//     after the step that ends them.  The server thread polls the lanes'
+//     completion events every 100 us while steps are in flight;
 //   * every draw is keyed by (infer seed, token position in the document,
 //     iteration), and each launch rebuilds n_dk from the stored z, so a
 //     document's result does not depend on the requests it shared the device
@@ -47,6 +51,7 @@ struct Req {
     std::vector<double> ll;
     int ndocs = 0, ndone = 0;
     uint64_t arena_base = 0;
+    bool uploaded = false;
     double t_submit = 0.0, t_done = 0.0;
     bool done = false;
     int status = LDA_OK;
@@ -59,6 +64,31 @@ struct Doc {
     uint32_t len;
     int segs;    // wavefronts sampling it (lda::fold_segments)
     int it_done;
+    int lane;    // lane of its running step, -1 while waiting
+};
+
+constexpr int kMaxLanes = 4;
+
+// One lane: a stream, its bucket streams, per-step buffers and the step in flight.
+struct Lane {
+    ::lda_server* srv = nullptr;
+    hipStream_t stream = nullptr;
+    lda_host::FoldStreams fs;
+    hipEvent_t ev = nullptr;  // recorded after the step's results are on the host
+    hipEvent_t ev0 = nullptr; // recorded before the step (timing)
+    int64_t njobs = 0;
+    DevBuf<lda::InferJob> jobs;
+    DevBuf<lda::InferParams> params;
+    DevBuf<uint32_t> counter;  // one per bucket
+    DevBuf<float> theta;       // max_active x K: the documents the step finishes
+    DevBuf<double> ll;
+    lda::InferParams* h_params = nullptr;  // pinned staging
+    lda::InferJob* h_jobs = nullptr;
+    float* h_theta = nullptr;
+    double* h_ll = nullptr;
+    bool busy = false;
+    std::vector<uint64_t> docs;  // document keys of the step
+    std::vector<int> it_end, out_row;
 };
 
 }  // namespace
@@ -67,30 +97,22 @@ struct lda_server {
     lda_ctx* ctx = nullptr;
     int iterations = 0, step_iters = 0;
     int64_t step_tokens = 0;
-    hipStream_t stream = nullptr;
+    uint64_t infer_seed = 0;
     DevBuf<uint32_t> words;  // token arena
     DevBuf<uint16_t> z;
-    DevBuf<lda::InferJob> jobs;
-    DevBuf<lda::InferParams> params;
-    DevBuf<uint32_t> counter;  // one per bucket
-    lda_host::FoldStreams fs;
     uint32_t seg_tokens = 0;
     int seg_max = 1;
-    DevBuf<float> theta;  // max_active x K: the finished documents of one step
-    DevBuf<double> ll;
-    // pinned staging
-    lda::InferParams* h_params = nullptr;
-    lda::InferJob* h_jobs = nullptr;
-    float* h_theta = nullptr;
-    double* h_ll = nullptr;
+    Lane lanes[kMaxLanes];
+    int nlanes = 2;
     uint64_t arena_cap = 0;
     int max_active = 0;
     int grid[lda::kFoldBuckets] = {};
     std::map<uint64_t, uint64_t> free_ranges;  // arena offset -> length
     std::deque<uint64_t> queue;                // submitted, not admitted
     std::unordered_map<uint64_t, Req> reqs;
-    std::vector<Doc> active;
-    uint64_t next_id = 1;
+    std::map<uint64_t, Doc> active;            // admitted documents by key
+    bool admit_blocked = false;                // the queue head did not fit; retry after a step ends
+    uint64_t next_id = 1, next_doc = 1;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::thread worker;
@@ -98,8 +120,8 @@ struct lda_server {
     int fatal = LDA_OK;
     std::string fatal_msg;
     // statistics
-    double busy_ms = 0.0;
-    int64_t steps = 0, docs_done = 0, reqs_done = 0;
+    double busy_ms = 0.0, gpu_ms = 0.0, host_ms = 0.0, max_step_ms = 0.0;
+    int64_t steps = 0, docs_done = 0, reqs_done = 0, jobs_total = 0;
     double token_iters = 0.0;
 };
 
@@ -135,46 +157,51 @@ void arena_free(lda_server* s, uint64_t off, uint64_t len) {
     }
 }
 
-// One step on the device over the staged jobs (grouped by bucket, longest
-// first inside a bucket; h_params holds the bucket ranges); the first nfin
-// output rows (the documents this step finishes) come back to the host.
-int run_step(lda_server* s, int64_t njobs, int nfin) {
+// Launches the staged step of a lane (jobs grouped by bucket, longest first
+// inside a bucket; h_params holds the bucket ranges): the first nfin output
+// rows (the documents it finishes) come back to pinned host memory, then the
+// lane's event is recorded.
+int launch_step(lda_server* s, Lane& L, int64_t njobs, int nfin) {
     lda_ctx* c = s->ctx;
-    const hipStream_t st = s->stream;
-    s->h_params->njobs = njobs;
+    const hipStream_t st = L.stream;
+    L.h_params->njobs = njobs;
     int grids[lda::kFoldBuckets];
     for (int b = 0; b < lda::kFoldBuckets; ++b)
-        grids[b] = (int)std::min<int64_t>(s->grid[b], s->h_params->bucket_hi[b] - s->h_params->bucket_lo[b]);
-    HIP_TRY(hipMemcpyAsync(s->params.p, s->h_params, sizeof(lda::InferParams), hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(s->jobs.p, s->h_jobs, njobs * sizeof(lda::InferJob), hipMemcpyHostToDevice, st));
-    HIP_TRY(lda::launch_fill_u32(s->counter.p, 0u, lda::kFoldBuckets, st));
-    HIP_TRY(lda_host::launch_fold_buckets(c->kpl, s->params.p, grids, st, s->fs));
-    HIP_TRY(lda::launch_infer_finalize(c->kpl, s->params.p, (int)std::min<int64_t>(s->grid[0], njobs), st));
+        grids[b] = (int)std::min<int64_t>(s->grid[b], L.h_params->bucket_hi[b] - L.h_params->bucket_lo[b]);
+    HIP_TRY(hipEventRecord(L.ev0, st));
+    HIP_TRY(hipMemcpyAsync(L.params.p, L.h_params, sizeof(lda::InferParams), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(L.jobs.p, L.h_jobs, njobs * sizeof(lda::InferJob), hipMemcpyHostToDevice, st));
+    HIP_TRY(lda::launch_fill_u32(L.counter.p, 0u, lda::kFoldBuckets, st));
+    HIP_TRY(lda_host::launch_fold_buckets(c->kpl, L.params.p, grids, st, L.fs));
+    HIP_TRY(lda::launch_infer_finalize(c->kpl, L.params.p, (int)std::min<int64_t>(s->grid[0], njobs), st));
     if (nfin) {
-        HIP_TRY(hipMemcpyAsync(s->h_theta, s->theta.p, (size_t)nfin * c->K * sizeof(float), hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipMemcpyAsync(s->h_ll, s->ll.p, (size_t)nfin * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(L.h_theta, L.theta.p, (size_t)nfin * c->K * sizeof(float), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(L.h_ll, L.ll.p, (size_t)nfin * sizeof(double), hipMemcpyDeviceToHost, st));
     }
-    HIP_TRY(hipStreamSynchronize(st));
+    HIP_TRY(hipEventRecord(L.ev, st));
     return LDA_OK;
 }
 
 // Admit queued requests in FIFO order while they fit (caller holds the lock).
-void admit(lda_server* s, std::vector<Doc>& newly, std::vector<Req*>& uploads) {
+void admit(lda_server* s) {
     while (!s->queue.empty()) {
         const uint64_t id = s->queue.front();
         Req& r = s->reqs[id];
-        if ((int64_t)s->active.size() + (int64_t)newly.size() + r.ndocs > s->max_active) break;
         const uint64_t n = r.off[r.ndocs];
         uint64_t base = 0;
-        if (n && !arena_alloc(s, n, &base)) break;
+        if ((int64_t)s->active.size() + r.ndocs > s->max_active || (n && !arena_alloc(s, n, &base))) {
+            s->admit_blocked = true;
+            return;
+        }
         r.arena_base = base;
         for (int i = 0; i < r.ndocs; ++i) {
             const uint32_t len = (uint32_t)(r.off[i + 1] - r.off[i]);
-            newly.push_back(Doc{id, i, base + r.off[i], len, lda::fold_segments(len, s->seg_tokens, s->seg_max), 0});
+            s->active.emplace(s->next_doc++,
+                              Doc{id, i, base + r.off[i], len, lda::fold_segments(len, s->seg_tokens, s->seg_max), 0, -1});
         }
-        uploads.push_back(&r);
         s->queue.pop_front();
     }
+    s->admit_blocked = false;
 }
 
 void fail_all(lda_server* s, int rc, double t) {
@@ -192,124 +219,187 @@ void fail_all(lda_server* s, int rc, double t) {
     s->cv_done.notify_all();
 }
 
-void worker_loop(lda_server* s) {
+// Results of a finished lane: finished documents go to their requests, the
+// others wait for their next step (caller holds the lock).
+void finish_lane(lda_server* s, Lane& L, double t) {
     const int K = s->ctx->K;
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, L.ev0, L.ev) == hipSuccess) {
+        s->gpu_ms += ms;
+        s->max_step_ms = std::max(s->max_step_ms, (double)ms);
+    }
+    bool any = false;
+    s->admit_blocked = false;  // memory and document slots may have been freed
+    for (size_t i = 0; i < L.docs.size(); ++i) {
+        auto it = s->active.find(L.docs[i]);
+        if (it == s->active.end()) continue;  // dropped after an error
+        Doc& d = it->second;
+        d.lane = -1;
+        if (L.out_row[i] < 0) {
+            d.it_done = L.it_end[i];
+            continue;
+        }
+        Req& r = s->reqs[d.req];
+        const int row = L.out_row[i];
+        std::copy(L.h_theta + (size_t)row * K, L.h_theta + (size_t)(row + 1) * K, r.theta.begin() + (size_t)d.index * K);
+        r.ll[d.index] = L.h_ll[row];
+        arena_free(s, d.b, d.len);
+        s->docs_done += 1;
+        s->token_iters += (double)d.len * s->iterations;
+        if (++r.ndone == r.ndocs) {
+            r.done = true;
+            r.t_done = t;
+            s->reqs_done += 1;
+            any = true;
+        }
+        s->active.erase(it);
+    }
+    L.docs.clear();
+    L.busy = false;
+    if (any) s->cv_done.notify_all();
+}
+
+// Stages a step of every waiting document on lane L (caller holds the lock):
+// at most step_tokens token-iterations per wavefront and step_iters
+// iterations per document; jobs grouped by bucket and longest first inside a
+// bucket; documents the step finishes take output rows 0..nfin-1.  Returns
+// the number of jobs; *uploads receives the requests whose tokens are not on
+// the device yet.
+int64_t stage_step(lda_server* s, Lane& L, int* nfin_out, std::vector<Req*>* uploads) {
+    std::vector<uint64_t> keys;
+    for (auto& kv : s->active)
+        if (kv.second.lane < 0) keys.push_back(kv.first);
+    const int64_t nb = (int64_t)keys.size();
+    std::vector<const Doc*> dd(nb);
+    std::vector<int> it_end(nb);
+    std::vector<int64_t> work(nb);
+    int nfin = 0;
+    for (int64_t i = 0; i < nb; ++i) {
+        const Doc& d = s->active[keys[i]];
+        dd[i] = &d;
+        const int64_t per_wave = std::max<int64_t>(((int64_t)d.len + d.segs - 1) / d.segs, 1);
+        int64_t chunk = s->step_tokens / per_wave;
+        chunk = std::max<int64_t>(1, std::min<int64_t>(chunk, s->step_iters));
+        it_end[i] = (int)std::min<int64_t>(d.it_done + chunk, s->iterations);
+        work[i] = per_wave * (it_end[i] - d.it_done);
+        if (it_end[i] >= s->iterations) ++nfin;
+        Req& r = s->reqs[d.req];
+        if (!r.uploaded) {
+            r.uploaded = true;
+            uploads->push_back(&r);
+        }
+    }
+    std::vector<int64_t> order(nb);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+        if (dd[a]->segs != dd[b]->segs) return dd[a]->segs < dd[b]->segs;
+        return work[a] > work[b];
+    });
+    int64_t q = 0;
+    for (int bk = 0; bk < lda::kFoldBuckets; ++bk) {
+        L.h_params->bucket_lo[bk] = q;
+        while (q < nb && lda::fold_bucket(dd[order[q]]->segs) == bk) ++q;
+        L.h_params->bucket_hi[bk] = q;
+    }
+    L.docs.resize(nb);
+    L.it_end.resize(nb);
+    L.out_row.resize(nb);
+    int next_row = 0;
+    for (int64_t j = 0; j < nb; ++j) {
+        const int64_t i = order[j];
+        const Doc& d = *dd[i];
+        const bool fin = it_end[i] >= s->iterations;
+        L.docs[j] = keys[i];
+        L.it_end[j] = it_end[i];
+        L.out_row[j] = fin ? next_row++ : -1;
+        L.h_jobs[j] = lda::InferJob{d.b, d.b + d.len, d.it_done, it_end[i], s->iterations, fin ? L.out_row[j] : 0};
+        s->active[keys[i]].lane = (int)(&L - s->lanes);
+    }
+    *nfin_out = nfin;
+    return nb;
+}
+
+void worker_loop(lda_server* s) {
     if (hipSetDevice(s->ctx->device) != hipSuccess) {
         std::lock_guard<std::mutex> lk(s->mu);
         fail_all(s, fail(LDA_ERR_DEVICE, "hipSetDevice failed in the server thread"), now_ms());
     }
-    for (;;) {
-        std::vector<Doc> newly;
-        std::vector<Req*> uploads;
-        std::vector<Doc> batch;
-        {
-            std::unique_lock<std::mutex> lk(s->mu);
-            s->cv_work.wait(lk, [&] { return s->stop || !s->queue.empty() || !s->active.empty(); });
-            if (s->stop) return;
-            admit(s, newly, uploads);
-            s->active.insert(s->active.end(), newly.begin(), newly.end());
-            batch = s->active;
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto waiting = [&] {
+        for (auto& kv : s->active)
+            if (kv.second.lane < 0) return true;
+        return false;
+    };
+    auto free_lane = [&]() -> Lane* {
+        for (int i = 0; i < s->nlanes; ++i)
+            if (!s->lanes[i].busy) return &s->lanes[i];
+        return nullptr;
+    };
+    auto in_flight = [&] {
+        for (int i = 0; i < s->nlanes; ++i)
+            if (s->lanes[i].busy) return true;
+        return false;
+    };
+    while (!s->stop) {
+        const double t = now_ms();
+        for (int i = 0; i < s->nlanes; ++i) {
+            Lane& L = s->lanes[i];
+            if (!L.busy) continue;
+            const hipError_t q = hipEventQuery(L.ev);
+            if (q == hipSuccess) finish_lane(s, L, t);
+            else if (q != hipErrorNotReady) fail_all(s, fail(LDA_ERR_DEVICE, "step failed: %s", hipGetErrorString(q)), t);
         }
-        // Token uploads: a request's documents are contiguous in the arena.
-        // A Req stays in place until its waiter erases it, which needs
-        // done == true, and only this thread sets that (below).
+        if (!s->fatal) admit(s);
+        Lane* L = s->fatal ? nullptr : free_lane();
+        if (!L || !waiting()) {
+            if (in_flight()) s->cv_work.wait_for(lk, std::chrono::microseconds(100));
+            else s->cv_work.wait(lk, [&] { return s->stop || (!s->queue.empty() && !s->admit_blocked); });
+            continue;
+        }
+        const double th0 = now_ms();
+        int nfin = 0;
+        std::vector<Req*> uploads;
+        const int64_t nb = stage_step(s, *L, &nfin, &uploads);
+        L->busy = true;
+        L->njobs = nb;
+        s->steps += 1;
+        s->jobs_total += nb;
+        // Requests stay in place until their waiter erases them, which needs
+        // done == true, set only by this thread: the uploads can run unlocked.
+        lk.unlock();
         int rc = LDA_OK;
         for (Req* r : uploads) {
             const uint64_t n = r->off[r->ndocs];
             if (n && hipMemcpyAsync(s->words.p + r->arena_base, r->words.data(), n * sizeof(uint32_t),
-                                    hipMemcpyHostToDevice, s->stream) != hipSuccess) {
+                                    hipMemcpyHostToDevice, L->stream) != hipSuccess) {
                 rc = fail(LDA_ERR_DEVICE, "token upload failed");
                 break;
             }
         }
-        // This step's share of each document's iterations: at most step_tokens
-        // token-iterations per wavefront.  Jobs are grouped by bucket and run
-        // longest first inside a bucket; documents this step finishes take
-        // output rows 0..nfin-1.
-        const int64_t nb = (int64_t)batch.size();
-        std::vector<int> it_end(nb);
-        std::vector<int64_t> work(nb);
-        int nfin = 0;
-        for (int64_t i = 0; i < nb; ++i) {
-            const Doc& d = batch[i];
-            const int64_t per_wave = std::max<int64_t>(((int64_t)d.len + d.segs - 1) / d.segs, 1);
-            int64_t chunk = s->step_tokens / per_wave;
-            chunk = std::max<int64_t>(1, std::min<int64_t>(chunk, s->step_iters));
-            it_end[i] = (int)std::min<int64_t>(d.it_done + chunk, s->iterations);
-            work[i] = per_wave * (it_end[i] - d.it_done);
-            if (it_end[i] >= s->iterations) ++nfin;
-        }
-        std::vector<int64_t> order(nb);
-        std::iota(order.begin(), order.end(), 0);
-        std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
-            if (batch[a].segs != batch[b].segs) return batch[a].segs < batch[b].segs;
-            return work[a] > work[b];
-        });
-        {
-            int64_t q = 0;
-            for (int bk = 0; bk < lda::kFoldBuckets; ++bk) {
-                s->h_params->bucket_lo[bk] = q;
-                while (q < nb && lda::fold_bucket(batch[order[q]].segs) == bk) ++q;
-                s->h_params->bucket_hi[bk] = q;
-            }
-        }
-        std::vector<int> out_row(nb, -1);
-        int next_row = 0;
-        for (int64_t q = 0; q < nb; ++q) {
-            const int64_t i = order[q];
-            const Doc& d = batch[i];
-            if (it_end[i] >= s->iterations) out_row[i] = next_row++;
-            s->h_jobs[q] = lda::InferJob{d.b, d.b + d.len, d.it_done, it_end[i], s->iterations,
-                                         out_row[i] < 0 ? 0 : out_row[i]};
-        }
-        const double t0 = now_ms();
-        if (rc == LDA_OK && nb) rc = run_step(s, nb, nfin);
-        const double t1 = now_ms();
-        std::lock_guard<std::mutex> lk(s->mu);
-        s->busy_ms += t1 - t0;
-        s->steps += 1;
+        if (rc == LDA_OK) rc = launch_step(s, *L, nb, nfin);
+        lk.lock();
+        s->host_ms += now_ms() - th0;
         if (rc != LDA_OK) {
-            fail_all(s, rc, t1);
-            continue;
+            L->busy = false;
+            fail_all(s, rc, now_ms());
         }
-        bool any = false;
-        std::vector<Doc> keep;
-        keep.reserve(nb);
-        for (int64_t i = 0; i < nb; ++i) {
-            Doc d = batch[i];
-            if (out_row[i] < 0) {
-                d.it_done = it_end[i];
-                keep.push_back(d);
-                continue;
-            }
-            Req& r = s->reqs[d.req];
-            std::copy(s->h_theta + (size_t)out_row[i] * K, s->h_theta + (size_t)(out_row[i] + 1) * K,
-                      r.theta.begin() + (size_t)d.index * K);
-            r.ll[d.index] = s->h_ll[out_row[i]];
-            arena_free(s, d.b, d.len);
-            s->docs_done += 1;
-            s->token_iters += (double)d.len * s->iterations;
-            if (++r.ndone == r.ndocs) {
-                r.done = true;
-                r.t_done = t1;
-                s->reqs_done += 1;
-                any = true;
-            }
-        }
-        s->active.swap(keep);
-        if (any) s->cv_done.notify_all();
     }
 }
 
 void release(lda_server* s) {
-    s->words.release(); s->z.release(); s->jobs.release(); s->params.release(); s->counter.release();
-    s->theta.release(); s->ll.release();
-    if (s->h_params) (void)hipHostFree(s->h_params);
-    if (s->h_jobs) (void)hipHostFree(s->h_jobs);
-    if (s->h_theta) (void)hipHostFree(s->h_theta);
-    if (s->h_ll) (void)hipHostFree(s->h_ll);
-    s->fs.destroy();
-    if (s->stream) (void)hipStreamDestroy(s->stream);
+    s->words.release();
+    s->z.release();
+    for (Lane& L : s->lanes) {
+        L.jobs.release(); L.params.release(); L.counter.release(); L.theta.release(); L.ll.release();
+        if (L.h_params) (void)hipHostFree(L.h_params);
+        if (L.h_jobs) (void)hipHostFree(L.h_jobs);
+        if (L.h_theta) (void)hipHostFree(L.h_theta);
+        if (L.h_ll) (void)hipHostFree(L.h_ll);
+        L.fs.destroy();
+        if (L.ev) (void)hipEventDestroy(L.ev);
+        if (L.ev0) (void)hipEventDestroy(L.ev0);
+        if (L.stream) (void)hipStreamDestroy(L.stream);
+    }
     delete s;
 }
 
@@ -318,11 +408,12 @@ void release(lda_server* s) {
 extern "C" {
 
 int lda_serve_start(lda_ctx* ctx, uint64_t hbm_budget_bytes, int iterations, int step_iters, int64_t step_tokens,
-                    uint64_t infer_seed, lda_server** out) {
+                    int lanes, uint64_t infer_seed, lda_server** out) {
     if (!ctx || !out) return fail(LDA_ERR_INVALID_ARGUMENT, "NULL argument");
     *out = nullptr;
     if (iterations < 0) return fail(LDA_ERR_INVALID_ARGUMENT, "iterations must be >= 0");
     if (step_iters < 1 || step_tokens < 1) return fail(LDA_ERR_INVALID_ARGUMENT, "step_iters and step_tokens must be >= 1");
+    if (lanes < 1 || lanes > kMaxLanes) return fail(LDA_ERR_INVALID_ARGUMENT, "lanes must be in 1..%d", kMaxLanes);
     if (hbm_budget_bytes < (64u << 10)) return fail(LDA_ERR_INVALID_ARGUMENT, "HBM budget below 64 KiB");
     HIP_TRY(hipSetDevice(ctx->device));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -341,33 +432,46 @@ int lda_serve_start(lda_ctx* ctx, uint64_t hbm_budget_bytes, int iterations, int
         release(s);
         return fail(LDA_ERR_INVALID_ARGUMENT, "HBM budget leaves no room for the token arena");
     }
-    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess || s->fs.create() != hipSuccess) {
-        release(s);
-        return fail(LDA_ERR_DEVICE, "stream creation failed");
-    }
     s->seg_tokens = ctx->seg_tokens;
     s->seg_max = ctx->seg_max;
-    if (s->words.alloc(s->arena_cap) || s->z.alloc(s->arena_cap) || s->jobs.alloc(s->max_active) || s->params.alloc(1) ||
-        s->counter.alloc(lda::kFoldBuckets) || s->theta.alloc((size_t)s->max_active * ctx->K) || s->ll.alloc(s->max_active) ||
-        hipHostMalloc((void**)&s->h_params, sizeof(lda::InferParams), 0) ||
-        hipHostMalloc((void**)&s->h_jobs, (size_t)s->max_active * sizeof(lda::InferJob), 0) ||
-        hipHostMalloc((void**)&s->h_theta, (size_t)s->max_active * ctx->K * sizeof(float), 0) ||
-        hipHostMalloc((void**)&s->h_ll, (size_t)s->max_active * sizeof(double), 0)) {
+    s->infer_seed = infer_seed;
+    s->nlanes = lanes;
+    if (s->words.alloc(s->arena_cap) || s->z.alloc(s->arena_cap)) {
         release(s);
         return fail(LDA_ERR_OUT_OF_MEMORY, "server allocation failed");
     }
-    lda::InferParams& p = *s->h_params;
-    p = lda::InferParams{};
-    p.words = s->words.p; p.z = s->z.p; p.jobs = s->jobs.p; p.njobs = 0;
-    p.counters = s->counter.p; p.job_base = 0;
-    p.nkw = ctx->d_nkw.p; p.nk = ctx->d_nk.p; p.alpha = ctx->d_alpha.p; p.alpha_cdf = ctx->d_alpha_cdf.p;
-    p.eta = ctx->eta; p.veta = ctx->veta; p.K = ctx->K; p.V = ctx->V; p.Kp = ctx->Kp;
-    p.seed_lo = (uint32_t)infer_seed; p.seed_hi = (uint32_t)(infer_seed >> 32);
-    p.theta = s->theta.p; p.ll = s->ll.p;
-    p.asum_d = 0.0; p.asum_f = 0.0f;
+    double asd = 0.0;
+    float asf = 0.0f;
     for (int k = 0; k < ctx->K; ++k) {
-        p.asum_d += (double)ctx->alpha[k];
-        p.asum_f = p.asum_f + ctx->alpha[k];
+        asd += (double)ctx->alpha[k];
+        asf = asf + ctx->alpha[k];
+    }
+    for (int i = 0; i < s->nlanes; ++i) {
+        Lane& L = s->lanes[i];
+        L.srv = s;
+        if (hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking) != hipSuccess || L.fs.create() != hipSuccess ||
+            hipEventCreate(&L.ev) != hipSuccess || hipEventCreate(&L.ev0) != hipSuccess) {
+            release(s);
+            return fail(LDA_ERR_DEVICE, "stream creation failed");
+        }
+        if (L.jobs.alloc(s->max_active) || L.params.alloc(1) || L.counter.alloc(lda::kFoldBuckets) ||
+            L.theta.alloc((size_t)s->max_active * ctx->K) || L.ll.alloc(s->max_active) ||
+            hipHostMalloc((void**)&L.h_params, sizeof(lda::InferParams), 0) ||
+            hipHostMalloc((void**)&L.h_jobs, (size_t)s->max_active * sizeof(lda::InferJob), 0) ||
+            hipHostMalloc((void**)&L.h_theta, (size_t)s->max_active * ctx->K * sizeof(float), 0) ||
+            hipHostMalloc((void**)&L.h_ll, (size_t)s->max_active * sizeof(double), 0)) {
+            release(s);
+            return fail(LDA_ERR_OUT_OF_MEMORY, "server allocation failed");
+        }
+        lda::InferParams& p = *L.h_params;
+        p = lda::InferParams{};
+        p.words = s->words.p; p.z = s->z.p; p.jobs = L.jobs.p; p.njobs = 0;
+        p.counters = L.counter.p; p.job_base = 0;
+        p.nkw = ctx->d_nkw.p; p.nk = ctx->d_nk.p; p.alpha = ctx->d_alpha.p; p.alpha_cdf = ctx->d_alpha_cdf.p;
+        p.eta = ctx->eta; p.veta = ctx->veta; p.K = ctx->K; p.V = ctx->V; p.Kp = ctx->Kp;
+        p.seed_lo = (uint32_t)infer_seed; p.seed_hi = (uint32_t)(infer_seed >> 32);
+        p.theta = L.theta.p; p.ll = L.ll.p;
+        p.asum_d = asd; p.asum_f = asf;
     }
     for (int b = 0; b < lda::kFoldBuckets; ++b) s->grid[b] = ctx->fold_grid[b];
     s->free_ranges[0] = s->arena_cap;
@@ -441,7 +545,7 @@ int lda_serve_wait(lda_server* s, uint64_t request_id, int timeout_ms, float* th
     return LDA_OK;
 }
 
-int lda_serve_stats(lda_server* s, double stats[8]) {
+int lda_serve_stats(lda_server* s, double stats[12]) {
     if (!s || !stats) return fail(LDA_ERR_INVALID_ARGUMENT, "NULL argument");
     std::lock_guard<std::mutex> lk(s->mu);
     stats[0] = (double)s->steps;
@@ -452,6 +556,10 @@ int lda_serve_stats(lda_server* s, double stats[8]) {
     stats[5] = (double)s->active.size();
     stats[6] = (double)s->queue.size();
     stats[7] = (double)s->arena_cap;
+    stats[8] = s->gpu_ms;
+    stats[9] = s->host_ms;
+    stats[10] = s->max_step_ms;
+    stats[11] = s->steps ? (double)s->jobs_total / (double)s->steps : 0.0;
     return LDA_OK;
 }
 
@@ -464,7 +572,8 @@ int lda_serve_stop(lda_server* s) {
     s->cv_work.notify_all();
     if (s->worker.joinable()) s->worker.join();
     (void)hipSetDevice(s->ctx->device);
-    (void)hipStreamSynchronize(s->stream);
+    for (Lane& L : s->lanes)
+        if (L.stream) (void)hipStreamSynchronize(L.stream);
     release(s);
     return LDA_OK;
 }
