@@ -19,7 +19,7 @@
  *   alpha      float K.
  *
  * Reference interfaces replaced (bab2min/tomotopy):
- *   lda_ctx_create        ILDAModel::create + LDAModel ctor checks
+ *   lda_ctx_create        ILDAModel::create + LDAModel ctor checks (TermWeight ONE/IDF/PMI)
  *                         (src/TopicModel/LDAModel.cpp:10-13, LDAModel.hpp:964-984)
  *   lda_upload_corpus     prepare(): initializeDocState / addWordTo<+1>
  *                         (LDAModel.hpp:716-813, 1073-1138)
@@ -78,16 +78,23 @@ int lda_version(void);
 int lda_device_count(int* count);
 
 /* Create a model context on `device`.  alpha: K floats (symmetric alpha is K
- * copies).  eta > 0, 1 <= K <= LDA_MAX_K, V >= 0.  termweight must be
- * LDA_TW_ONE for the device sampler. */
+ * copies).  eta > 0, 1 <= K <= LDA_MAX_K, V >= 0.  termweight: LDA_TW_*. */
 int lda_ctx_create(int K, int V, const float* alpha, float eta, int termweight, uint64_t seed, int device,
                    lda_ctx** out);
 int lda_ctx_destroy(lda_ctx* ctx);
 
 /* Upload the training corpus.  If z is NULL the initial topics are drawn on
  * the device (z ~ Discrete(alpha) from the Philox stream); otherwise z is
- * taken as given.  Counts n_dk, n_kw, n_k are then built from z. */
-int lda_upload_corpus(lda_ctx* ctx, const uint32_t* words, const uint64_t* doc_off, int64_t D, const uint16_t* z);
+ * taken as given.  Counts n_dk, n_kw, n_k are then built from z.
+ * weights: N floats, required for LDA_TW_IDF / LDA_TW_PMI models (tomoto's
+ * per-token wordWeights, LDAModel.hpp:802-809, 1104-1121; entries of words
+ * >= V are ignored), NULL for LDA_TW_ONE.  Term-weighted models keep float
+ * counts (addWordTo adds the weight; removal is max(count - weight, 0),
+ * Utils.hpp:132-155) and support training, the log-likelihood and the alpha
+ * fixed point; their fold-in, frozen sampling, copy_merge and exact
+ * invariant checks return LDA_ERR_UNSUPPORTED. */
+int lda_upload_corpus(lda_ctx* ctx, const uint32_t* words, const uint64_t* doc_off, int64_t D, const uint16_t* z,
+                      const float* weights);
 
 /* Replace the topic-word counts (tomoto load takes n_kw / n_k from the file). */
 int lda_set_counts(lda_ctx* ctx, const int32_t* nkw, const int32_t* nk);
@@ -131,6 +138,12 @@ int lda_check_invariants(lda_ctx* ctx, int64_t flags[8]);
 /* Read-out (host buffers; any pointer may be NULL). */
 int lda_download_state(lda_ctx* ctx, uint16_t* z, int32_t* ndk, int32_t* nkw, int32_t* nk);
 int lda_get_dims(lda_ctx* ctx, int64_t dims[6]); /* K, V, D, N, Kp, kpl */
+/* Read-out of a term-weighted model (float counts, natural topic order). */
+int lda_download_state_f(lda_ctx* ctx, uint16_t* z, float* ndk, float* nkw, float* nk);
+/* tomoto's weightedN (TopicModel.hpp:507-524): the token count for
+ * LDA_TW_ONE, the sum of the valid tokens' weights otherwise; ll_per_word is
+ * (ll_docs + ll_rest) / weightedN. */
+int lda_get_weighted_n(lda_ctx* ctx, double* weighted_n);
 
 /* One sampling step per query token on the frozen current state: remove the
  * token's own z_old from (n_dk[doc], n_kw[:,word], n_k), weight, scan, and

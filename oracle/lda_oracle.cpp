@@ -284,6 +284,10 @@ struct Model {
     std::vector<int32_t> ndk;  // D x K
     std::vector<int32_t> nkw;  // V x K (word-major, tomoto's column-major K x V)
     std::vector<int32_t> nk;   // K
+    // term-weighted state (TermWeight IDF / PMI): token weights, float counts,
+    // per-document weight sums
+    bool weighted = false;
+    std::vector<float> wt, ndk_f, nkw_f, nk_f, docw;
     uint32_t global_step = 0;
     uint32_t optim_interval = 10, burn_in = 0;
 };
@@ -503,6 +507,154 @@ void loglik(const Model& m, int mode, double* ll_docs, double* ll_rest) {
     *ll_rest = rest;
 }
 
+// ------------------------------------------------ term-weighted (IDF/PMI) --
+// TermWeight IDF / PMI (LDAModel.hpp:287-299, Utils.hpp:132-155): every token
+// carries a float weight w; adding a token is val += w, removing it is
+// val = max(val - w, 0) (std::max keeps val - w unless it is negative), and
+// n_dk, n_kw, n_k are float.
+inline float sub_clamp(float v, float w) {
+    float a = v - w;
+    return a < 0.0f ? 0.0f : a;
+}
+
+inline void add_token_w(Model& m, int64_t d, uint32_t v, int k, float w, bool dec, float* nk_local) {
+    float& a = m.ndk_f[(size_t)d * m.K + k];
+    float& b = nk_local[k];
+    float& c = m.nkw_f[(size_t)v * m.K + k];
+    if (dec) { a = sub_clamp(a, w); b = sub_clamp(b, w); c = sub_clamp(c, w); }
+    else { a += w; b += w; c += w; }
+}
+
+// getZLikelihoods (LDAModel.hpp:273-285) over float counts, then the same
+// prefix sum and search as the integer sampler.
+int sample_one_w(const Model& m, const float* ndk, const float* col, const float* nk, float u, float* buf) {
+    int n = m.scan_order ? 64 * m.kpl : m.K;
+    for (int k = 0; k < m.K; ++k) {
+        float a = (ndk[k] + m.alpha[k]) * (col[k] + m.eta);
+        buf[k] = a / (nk[k] + m.veta);
+    }
+    for (int k = m.K; k < n; ++k) buf[k] = 0.0f;
+    if (m.scan_order) prefix_sum_device(buf, m.kpl);
+    else prefix_sum_tomoto(buf, m.K);
+    return search_cdf(buf, m.K, u);
+}
+
+void sample_range_w(Model& m, int64_t d, uint64_t b, uint64_t e, float* nk_local, uint32_t sweep, float* buf) {
+    for (uint64_t g = b; g < e; ++g) {
+        uint32_t v = m.words[g];
+        if (v >= (uint32_t)m.V) continue;
+        const float w = m.wt[g];
+        int zo = m.z[g];
+        add_token_w(m, d, v, zo, w, true, nk_local);
+        float u = token_uniform(m.seed, g, sweep, TAG_SWEEP);
+        int zn = sample_one_w(m, &m.ndk_f[(size_t)d * m.K], &m.nkw_f[(size_t)v * m.K], nk_local, u, buf);
+        m.z[g] = (uint16_t)zn;
+        add_token_w(m, d, v, zn, w, false, nk_local);
+    }
+}
+
+// mergeState<partition> for float counts (LDAModel.hpp:572-579): n_kw =
+// max(n_kw, 0) cell by cell, then n_k = the row sums of n_kw, accumulated in
+// float over the words in ascending order.  (Eigen's rowwise().sum() groups
+// the terms by its packet width, so tomoto's n_k can differ from this one in
+// the last bits; the device follows this order.)
+void merge_w(Model& m) {
+    for (auto& x : m.nkw_f) if (x < 0.0f) x = 0.0f;
+    std::fill(m.nk_f.begin(), m.nk_f.end(), 0.0f);
+    for (int v = 0; v < m.V; ++v)
+        for (int k = 0; k < m.K; ++k) m.nk_f[k] = m.nk_f[k] + m.nkw_f[(size_t)v * m.K + k];
+}
+
+// Partition sweep (the schedule of sweep_partition) with token weights.
+void sweep_partition_w(Model& m, int W, int threads) {
+    const uint32_t sweep = m.global_step;
+    std::vector<uint32_t> ch = vocab_chunks(m, W);
+    std::vector<uint64_t> coff((size_t)(W + 1) * m.D);
+    for (int64_t d = 0; d < m.D; ++d) {
+        uint64_t b = m.off[d], e = m.off[d + 1];
+        coff[d] = b;
+        for (int p = 0; p < W; ++p)
+            coff[(size_t)(p + 1) * m.D + d] =
+                (uint64_t)(std::lower_bound(m.words.begin() + b, m.words.begin() + e, ch[p]) - m.words.begin());
+        coff[(size_t)W * m.D + d] = e;
+    }
+    std::vector<float> nk_priv((size_t)W * m.K);
+    for (int p = 0; p < W; ++p) std::copy(m.nk_f.begin(), m.nk_f.end(), nk_priv.begin() + (size_t)p * m.K);
+    int T = std::max(1, std::min(threads, W));
+    for (int i = 0; i < W; ++i) {
+        auto work = [&](int tid) {
+            std::vector<float> buf(64 * 32 + m.K + 64);
+            std::vector<uint32_t> order;
+            for (int p = tid; p < W; p += T) {
+                int didx = (i + p) % W;
+                size_t n = (size_t)((m.D + (W - 1) - didx) / W);
+                uint32_t sseed = philox_word(m.seed, (uint32_t)p, (uint32_t)i, sweep, TAG_SHUFFLE);
+                shuffled_order(n, sseed, order);
+                for (size_t q = 0; q < n; ++q) {
+                    int64_t d = (int64_t)order[q] * W + didx;
+                    sample_range_w(m, d, coff[(size_t)p * m.D + d], coff[(size_t)(p + 1) * m.D + d],
+                                   &nk_priv[(size_t)p * m.K], sweep, buf.data());
+                }
+            }
+        };
+        if (T == 1) work(0);
+        else {
+            std::vector<std::thread> th;
+            for (int t = 0; t < T; ++t) th.emplace_back(work, t);
+            for (auto& t : th) t.join();
+        }
+    }
+    merge_w(m);
+}
+
+// optimizeParameters (LDAModel.hpp:221-265) with the document length
+// replaced by the document's weight sum (getSumWordWeight).
+void optimize_alpha_w(Model& m) {
+    for (int it = 0; it < 10; ++it) {
+        float asum = 0.0f;
+        for (int k = 0; k < m.K; ++k) asum += m.alpha[k];
+        double s = 0.0;
+        for (int64_t d = 0; d < m.D; ++d) s += (double)digamma_approx_f(m.docw[d] + asum);
+        float denom = (float)(s - (double)m.D * (double)lut_digamma().get(asum));
+        for (int k = 0; k < m.K; ++k) {
+            float ak = m.alpha[k];
+            double sk = 0.0;
+            for (int64_t d = 0; d < m.D; ++d) sk += (double)digamma_approx_f(m.ndk_f[(size_t)d * m.K + k] + ak);
+            float nom = (float)(sk - (double)m.D * (double)lut_digamma().get(ak));
+            m.alpha[k] = std::max(nom / denom * ak, 1e-5f);
+        }
+    }
+}
+
+// getLLDocs + getLLRest (LDAModel.hpp:616-666) in FP64 over float counts;
+// the document term uses the weight sum.  Zero cells add lgamma(x) - lgamma(x)
+// and are skipped.
+void loglik_w(const Model& m, double* ll_docs, double* ll_rest) {
+    double asum = 0.0;
+    for (int k = 0; k < m.K; ++k) asum += (double)m.alpha[k];
+    double docs = 0.0;
+    for (int64_t d = 0; d < m.D; ++d) {
+        docs -= std::lgamma((double)m.docw[d] + asum) - std::lgamma(asum);
+        for (int k = 0; k < m.K; ++k) {
+            float c = m.ndk_f[(size_t)d * m.K + k];
+            if (c == 0.0f) continue;
+            docs += std::lgamma((double)c + (double)m.alpha[k]) - std::lgamma((double)m.alpha[k]);
+        }
+    }
+    double eta = (double)m.eta, veta = (double)m.V * eta;
+    double rest = std::lgamma(veta) * m.K, lge = std::lgamma(eta);
+    for (int k = 0; k < m.K; ++k) {
+        rest -= std::lgamma((double)m.nk_f[k] + veta);
+        for (int v = 0; v < m.V; ++v) {
+            float c = m.nkw_f[(size_t)v * m.K + k];
+            if (c == 0.0f) continue;
+            rest += std::lgamma((double)c + eta) - lge;
+        }
+    }
+    *ll_docs = docs;
+    *ll_rest = rest;
+}
+
 }  // namespace
 
 // ============================================================== C API ======
@@ -669,6 +821,70 @@ void orc_get_state(void* h, uint16_t* z, int32_t* ndk, int32_t* nkw, int32_t* nk
 }
 
 void orc_loglik(void* h, int mode, double* ll_docs, double* ll_rest) { loglik(*(Model*)h, mode, ll_docs, ll_rest); }
+
+// Term-weighted corpus (TermWeight IDF / PMI): weights[g] is token g's weight
+// (ignored for ids >= V).  Initial topics as in orc_set_corpus; the float
+// counts are accumulated token by token in corpus order (initializeDocState,
+// LDAModel.hpp:782-813), and each document's weight sum is the sequential
+// float sum of its tokens' weights (LDA.h:113-116).
+void orc_set_corpus_w(void* h, const uint32_t* words, const uint64_t* off, int64_t D, const uint16_t* z,
+                      const float* weights) {
+    Model* m = (Model*)h;
+    orc_set_corpus(h, words, off, D, z);
+    const uint64_t N = off[D];
+    m->weighted = true;
+    m->wt.assign(weights, weights + N);
+    m->ndk_f.assign((size_t)D * m->K, 0.0f);
+    m->nkw_f.assign((size_t)m->V * m->K, 0.0f);
+    m->nk_f.assign(m->K, 0.0f);
+    m->docw.assign(D, 0.0f);
+    for (int64_t d = 0; d < D; ++d) {
+        float sw = 0.0f;
+        for (uint64_t g = off[d]; g < off[d + 1]; ++g) {
+            uint32_t v = words[g];
+            if (v >= (uint32_t)m->V) continue;
+            add_token_w(*m, d, v, m->z[g], m->wt[g], false, m->nk_f.data());
+            sw += m->wt[g];
+        }
+        m->docw[d] = sw;
+    }
+}
+
+// Weighted training: scheme 1 (none) or 3 (partition, W workers), alpha
+// optimisation on the same schedule as orc_train_scheme.
+void orc_train_w(void* h, int iterations, int workers, int threads, int scheme) {
+    Model* m = (Model*)h;
+    int W = scheme == 1 ? 1 : std::max(1, workers);
+    for (int it = 0; it < iterations; ++it) {
+        sweep_partition_w(*m, W, threads);
+        if (m->global_step >= m->burn_in && m->optim_interval && (m->global_step + 1) % m->optim_interval == 0)
+            optimize_alpha_w(*m);
+        ++m->global_step;
+    }
+}
+
+void orc_optimize_alpha_w(void* h) { optimize_alpha_w(*(Model*)h); }
+void orc_merge_w(void* h) { merge_w(*(Model*)h); }
+
+void orc_get_state_w(void* h, uint16_t* z, float* ndk, float* nkw, float* nk, float* docw) {
+    Model* m = (Model*)h;
+    if (z) std::copy(m->z.begin(), m->z.end(), z);
+    if (ndk) std::copy(m->ndk_f.begin(), m->ndk_f.end(), ndk);
+    if (nkw) std::copy(m->nkw_f.begin(), m->nkw_f.end(), nkw);
+    if (nk) std::copy(m->nk_f.begin(), m->nk_f.end(), nk);
+    if (docw) std::copy(m->docw.begin(), m->docw.end(), docw);
+}
+
+// Replace the float counts (e.g. with a state downloaded from the device).
+void orc_set_state_w(void* h, const uint16_t* z, const float* ndk, const float* nkw, const float* nk) {
+    Model* m = (Model*)h;
+    if (z) std::copy(z, z + m->z.size(), m->z.begin());
+    if (ndk) std::copy(ndk, ndk + m->ndk_f.size(), m->ndk_f.begin());
+    if (nkw) std::copy(nkw, nkw + m->nkw_f.size(), m->nkw_f.begin());
+    if (nk) std::copy(nk, nk + m->nk_f.size(), m->nk_f.begin());
+}
+
+void orc_loglik_w(void* h, double* ll_docs, double* ll_rest) { loglik_w(*(Model*)h, ll_docs, ll_rest); }
 
 // Fold-in of unseen documents, together=False (LDAModel.hpp:920-953): every
 // document starts its own RNG stream (tomoto default-constructs `rgc{}` per

@@ -5,6 +5,7 @@ cpu_baseline leg of bench.py, as the checker or the timed CPU baseline.  The
 product package never imports this module.
 """
 import ctypes
+import math
 import os
 import subprocess
 
@@ -64,6 +65,13 @@ def load():
         "orc_infer_seg": (None, [_c_p, _c_p, _c_p, i64, i32, u64, i32, u32, i32, _c_p, _c_p, _c_p]),
         "orc_seg_bounds": (i32, [_c_p, u64, u32, i32, _c_p]),
         "orc_infer_ll_bruteforce": (f64, [_c_p, _c_p, u64, _c_p]),
+        "orc_set_corpus_w": (None, [_c_p, _c_p, _c_p, i64, _c_p, _c_p]),
+        "orc_train_w": (None, [_c_p, i32, i32, i32, i32]),
+        "orc_optimize_alpha_w": (None, [_c_p]),
+        "orc_merge_w": (None, [_c_p]),
+        "orc_get_state_w": (None, [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p]),
+        "orc_set_state_w": (None, [_c_p, _c_p, _c_p, _c_p, _c_p]),
+        "orc_loglik_w": (None, [_c_p, _c_p, _c_p]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -242,3 +250,96 @@ class Model:
         words = np.ascontiguousarray(words, np.uint32)
         z = np.ascontiguousarray(z, np.uint16)
         return load().orc_infer_ll_bruteforce(self._h, _p(words), len(words), _p(z))
+
+
+class WeightedModel(Model):
+    """The oracle's term-weighted model (TermWeight IDF / PMI): float counts,
+    clamped removal, weighted merge (LDAModel.hpp:287-299, 565-579)."""
+
+    def set_corpus(self, words, off, weights, z=None):
+        self._words = np.ascontiguousarray(words, np.uint32)
+        self._off = np.ascontiguousarray(off, np.uint64)
+        self._wt = np.ascontiguousarray(weights, np.float32)
+        zz = None if z is None else np.ascontiguousarray(z, np.uint16)
+        self.D = len(off) - 1
+        self.N = int(off[-1])
+        load().orc_set_corpus_w(self._h, _p(self._words), _p(self._off), self.D, _p(zz), _p(self._wt))
+
+    def train(self, iterations, workers=1, threads=1, scheme=3):
+        """scheme: 1 none, 3 partition."""
+        if scheme not in (1, 3):
+            raise ValueError("term-weighted training runs the none or partition scheme")
+        load().orc_train_w(self._h, iterations, workers, threads, scheme)
+
+    def optimize_alpha(self):
+        load().orc_optimize_alpha_w(self._h)
+
+    def merge(self):
+        load().orc_merge_w(self._h)
+
+    def state(self):
+        z = np.zeros(self.N, np.uint16)
+        ndk = np.zeros((self.D, self.K), np.float32)
+        nkw = np.zeros((self.V, self.K), np.float32)
+        nk = np.zeros(self.K, np.float32)
+        docw = np.zeros(self.D, np.float32)
+        load().orc_get_state_w(self._h, _p(z), _p(ndk), _p(nkw), _p(nk), _p(docw))
+        return z, ndk, nkw, nk, docw
+
+    def set_state(self, z=None, ndk=None, nkw=None, nk=None):
+        c = lambda a, t: None if a is None else np.ascontiguousarray(a, t)
+        z, ndk, nkw, nk = c(z, np.uint16), c(ndk, np.float32), c(nkw, np.float32), c(nk, np.float32)
+        load().orc_set_state_w(self._h, _p(z), _p(ndk), _p(nkw), _p(nk))
+
+    def loglik(self, mode=0):
+        if mode != 0:
+            raise ValueError("the term-weighted log-likelihood is evaluated in FP64 only")
+        a = ctypes.c_double()
+        b = ctypes.c_double()
+        load().orc_loglik_w(self._h, ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
+
+
+def term_weights(words, off, V, scheme):
+    """Token weights of TermWeight IDF ("idf") or PMI ("pmi"), restated in
+    plain loops from LDAModel.hpp:1096-1121 (vocabulary weights) and 800-809
+    (per-token weights).  words: per-document token ids (ids >= V are removed
+    words: weight 0, but they count in the document length and in the total
+    collection frequency).  IDF: w = float(log(D / df[v])).  PMI: vw[v] =
+    float(cf[v] / totCf), w = max(float(log(float(tf_d[v]) / vw[v] / float(len_d))), 0)
+    with the quotient in float and the logarithm taken in double."""
+    D = len(off) - 1
+    out = np.zeros(len(words), np.float32)
+    if scheme == "idf":
+        df = [0] * V
+        for d in range(D):
+            for v in set(int(x) for x in words[int(off[d]):int(off[d + 1])]):
+                if v < V:
+                    df[v] += 1
+        vw = [np.float32(math.log(D / df[v])) if df[v] else np.float32(np.inf) for v in range(V)]
+        for g, v in enumerate(words):
+            if int(v) < V:
+                out[g] = vw[int(v)]
+        return out
+    if scheme != "pmi":
+        raise ValueError(scheme)
+    cf = [0] * V
+    tot = 0
+    for v in words:
+        tot += 1
+        if int(v) < V:
+            cf[int(v)] += 1
+    vw = [np.float32(cf[v] / tot) for v in range(V)]
+    for d in range(D):
+        b, e = int(off[d]), int(off[d + 1])
+        tf = {}
+        for v in words[b:e]:
+            if int(v) < V:
+                tf[int(v)] = tf.get(int(v), 0) + 1
+        n = np.float32(e - b)
+        for g in range(b, e):
+            v = int(words[g])
+            if v < V:
+                q = np.float32(np.float32(tf[v]) / vw[v]) / n
+                out[g] = max(np.float32(math.log(float(q))), np.float32(0.0))
+    return out

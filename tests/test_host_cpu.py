@@ -77,8 +77,10 @@ def test_model_argument_checks():
         LDAModel(k=3, eta=0.0)
     with pytest.raises(ValueError):
         LDAModel(k=3, seed="x")
-    with pytest.raises(NotImplementedError):
-        LDAModel(tw=TermWeight.PMI, k=3)
+    assert LDAModel(tw=TermWeight.PMI, k=3).tw == int(TermWeight.PMI)
+    assert LDAModel(tw="idf", k=3).tw == int(TermWeight.IDF)
+    with pytest.raises(ValueError):
+        LDAModel(tw="bm25", k=3)
     m = LDAModel(k=3, seed=1)
     with pytest.raises(ValueError):
         m.add_doc("a string")

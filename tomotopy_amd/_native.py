@@ -38,7 +38,7 @@ _SIGS = {
     "lda_device_count": [_c_p],
     "lda_ctx_create": [_i32, _i32, _c_p, _f32, _i32, _u64, _i32, _c_p],
     "lda_ctx_destroy": [_c_p],
-    "lda_upload_corpus": [_c_p, _c_p, _c_p, _i64, _c_p],
+    "lda_upload_corpus": [_c_p, _c_p, _c_p, _i64, _c_p, _c_p],
     "lda_set_counts": [_c_p, _c_p, _c_p],
     "lda_set_alpha": [_c_p, _c_p],
     "lda_get_alpha": [_c_p, _c_p],
@@ -52,6 +52,8 @@ _SIGS = {
     "lda_check_invariants": [_c_p, _c_p],
     "lda_download_state": [_c_p, _c_p, _c_p, _c_p, _c_p],
     "lda_get_dims": [_c_p, _c_p],
+    "lda_download_state_f": [_c_p, _c_p, _c_p, _c_p, _c_p],
+    "lda_get_weighted_n": [_c_p, _c_p],
     "lda_sample_frozen": [_c_p, _i64, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p],
     "lda_infer": [_c_p, _c_p, _c_p, _i64, _i32, _u64, _i32, _c_p, _c_p, _c_p],
     "lda_kernel_stats": [_c_p, _c_p],
@@ -132,6 +134,7 @@ class Context:
         self._h = h
         self.k = int(k)
         self.v = int(v)
+        self.tw = int(tw)
         self.n_docs = 0
         self.n_tokens = 0
 
@@ -146,13 +149,15 @@ class Context:
         except Exception:
             pass
 
-    def upload_corpus(self, words, doc_off, z=None):
+    def upload_corpus(self, words, doc_off, z=None, weights=None):
         words = np.ascontiguousarray(words, dtype=np.uint32)
         doc_off = np.ascontiguousarray(doc_off, dtype=np.uint64)
         if z is not None:
             z = np.ascontiguousarray(z, dtype=np.uint16)
+        if weights is not None:
+            weights = np.ascontiguousarray(weights, dtype=np.float32)
         D = len(doc_off) - 1
-        _check(load().lda_upload_corpus(self._h, _ptr(words), _ptr(doc_off), D, _ptr(z)))
+        _check(load().lda_upload_corpus(self._h, _ptr(words), _ptr(doc_off), D, _ptr(z), _ptr(weights)))
         self.n_docs = D
         self.n_tokens = int(doc_off[-1]) if D >= 0 and len(doc_off) else 0
 
@@ -216,6 +221,21 @@ class Context:
         c = np.zeros(K, dtype=np.int32) if nk else None
         _check(load().lda_download_state(self._h, _ptr(zz), _ptr(a), _ptr(b), _ptr(c)))
         return zz, a, b, c
+
+    def download_state_f(self):
+        """Counts of a term-weighted model (float32)."""
+        K, V, D, N = (int(x) for x in self.dims()[:4])
+        zz = np.zeros(N, dtype=np.uint16)
+        a = np.zeros((D, K), dtype=np.float32)
+        b = np.zeros((V, K), dtype=np.float32)
+        c = np.zeros(K, dtype=np.float32)
+        _check(load().lda_download_state_f(self._h, _ptr(zz), _ptr(a), _ptr(b), _ptr(c)))
+        return zz, a, b, c
+
+    def weighted_n(self):
+        x = ctypes.c_double(0.0)
+        _check(load().lda_get_weighted_n(self._h, ctypes.byref(x)))
+        return x.value
 
     def sample_frozen(self, doc, word, z_old, u):
         doc = np.ascontiguousarray(doc, dtype=np.int64)

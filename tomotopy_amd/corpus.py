@@ -90,3 +90,53 @@ def prepare_arrays(words, doc_off, n_words, min_cf=0, min_df=0, rm_top=0):
     order, real_v = vocab_order(cf, df, min_cf, min_df, rm_top)
     w, off = csr_from_arrays(words, doc_off, order)
     return w, off, real_v, order
+
+
+def term_weights(words, doc_off, real_v, scheme):
+    """Per-token weights of TermWeight IDF or PMI, as tomoto's prepare()
+    computes them (LDAModel.hpp:1096-1121 and 800-809).
+
+    words / doc_off: the prepared CSR (sorted documents, ids >= real_v are
+    removed words).  Removed words get weight 0 but count in the document
+    length and in the total collection frequency.
+      IDF: w = float32(log(D / df[v])), df = documents containing v.
+      PMI: vw[v] = float32(cf[v] / totCf) with totCf the number of tokens,
+           w = max(float32(log(float32(float32(tf_d[v]) / vw[v]) / float32(len_d))), 0),
+           tf_d[v] = occurrences of v in document d; the quotients are
+           float32, the logarithm is taken in double.
+    """
+    words = np.asarray(words, dtype=np.int64)
+    doc_off = np.asarray(doc_off, dtype=np.uint64)
+    D = len(doc_off) - 1
+    N = len(words)
+    lens = np.diff(doc_off.astype(np.int64))
+    doc_id = np.repeat(np.arange(D, dtype=np.int64), lens)
+    valid = words < real_v
+    out = np.zeros(N, dtype=np.float32)
+    if N == 0:
+        return out
+    vw_idx = np.where(valid, words, 0)
+    if scheme == "idf":
+        pairs = np.unique(doc_id[valid] * np.int64(real_v) + words[valid])
+        df = np.bincount(pairs % max(real_v, 1), minlength=real_v).astype(np.float64)
+        with np.errstate(divide="ignore"):
+            vw = np.log(np.float64(D) / df).astype(np.float32)
+        out[valid] = vw[vw_idx[valid]]
+        return out
+    if scheme != "pmi":
+        raise ValueError("scheme must be 'idf' or 'pmi'")
+    cf = np.bincount(words[valid], minlength=real_v).astype(np.float64)
+    vw = (cf / np.float64(N)).astype(np.float32)
+    # tf: the length of the run of equal ids (documents are sorted) each token lies in
+    key = doc_id * np.int64(real_v + 1) + np.where(valid, words, real_v)
+    starts = np.ones(N, dtype=bool)
+    starts[1:] = key[1:] != key[:-1]
+    run_id = np.cumsum(starts) - 1
+    run_len = np.bincount(run_id)
+    tf = run_len[run_id].astype(np.float32)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        q = (tf / vw[vw_idx]) / lens[doc_id].astype(np.float32)
+        w = np.log(q.astype(np.float64)).astype(np.float32)
+    w = np.maximum(w, np.float32(0.0))
+    out[valid] = w[valid]
+    return out

@@ -259,6 +259,48 @@ int resolve_workers(lda_ctx* c, int workers, int scheme, int* W_out, bool* copy_
 int sweep_once(lda_ctx* c, int W, bool copy_merge) {
     lda::SweepParams p = base_params(c);
     const hipStream_t s = c->stream;
+    if (c->tw != LDA_TW_ONE) {
+        // term-weighted: device-wide or partition, then tomoto's weighted
+        // merge n_kw = max(n_kw, 0), n_k = rowsum (LDAModel.hpp:565-579)
+        if (copy_merge) return fail(LDA_ERR_UNSUPPORTED, "copy_merge is not supported for term-weighted models");
+        p.wt = c->d_wt.p;
+        float ms = 0.0f;
+        if (W == 0) {
+            HIP_TRY(lda::launch_fill_u32(c->d_counter.p, 0u, 1, s));
+            p.nk_in = c->d_nk.p;
+            p.nk_out = c->d_nk.p;
+            p.doc_order = c->d_order.p;
+            p.counter = c->d_counter.p;
+            const int grid = c->num_cu * lda::sweep_device_w_blocks_per_cu(c->kpl);
+            HIP_TRY(hipEventRecord(c->ev0, s));
+            HIP_TRY(lda::launch_sweep_device_w(c->kpl, p, grid, s));
+            HIP_TRY(hipEventRecord(c->ev1, s));
+            c->stat_launches += 1;
+        } else {
+            int rc = prepare_partition(c, W);
+            if (rc) return rc;
+            for (int w = 0; w < W; ++w) HIP_TRY(lda::launch_copy_i32(c->d_nk_priv.p + (size_t)w * c->Kp, c->d_nk.p, c->Kp, s));
+            p.nk_in = c->d_nk_priv.p;
+            p.nk_out = c->d_nk_priv.p;
+            p.coff = c->d_coff.p;
+            p.W = W;
+            HIP_TRY(hipEventRecord(c->ev0, s));
+            for (int r = 0; r < W; ++r) {
+                p.round = r;
+                HIP_TRY(lda::launch_sweep_det_w(c->kpl, p, s));
+            }
+            HIP_TRY(hipEventRecord(c->ev1, s));
+            c->stat_launches += W;
+        }
+        HIP_TRY(lda::launch_clamp_rowsum_f(reinterpret_cast<float*>(c->d_nkw.p), c->V, c->Kp,
+                                           reinterpret_cast<float*>(c->d_nk.p), s));
+        HIP_TRY(hipEventSynchronize(c->ev1));
+        HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        c->stat_ms += ms;
+        HIP_TRY(hipStreamSynchronize(s));
+        c->stat_tokens += (double)c->n_valid;
+        return LDA_OK;
+    }
     if (copy_merge) {
         const int64_t n = (int64_t)c->V * c->Kp;
         HIP_TRY(c->d_nkw_loc.alloc((size_t)W * std::max<int64_t>(n, 1)));
@@ -342,8 +384,12 @@ int optimize_alpha(lda_ctx* c) {
     for (int it = 0; it < 10; ++it) {
         float asum = 0.0f;
         for (int k = 0; k < c->K; ++k) asum += c->alpha[k];
-        HIP_TRY(lda::launch_alpha_sums(c->d_ndk.p, c->d_doclen.p, c->D, c->d_alpha.p, c->Kp, asum, c->d_part.p, nblk,
-                                       c->stream));
+        if (c->tw != LDA_TW_ONE)
+            HIP_TRY(lda::launch_alpha_sums_f(reinterpret_cast<const float*>(c->d_ndk.p), c->d_docw.p, c->D, c->d_alpha.p,
+                                             c->Kp, asum, c->d_part.p, nblk, c->stream));
+        else
+            HIP_TRY(lda::launch_alpha_sums(c->d_ndk.p, c->d_doclen.p, c->D, c->d_alpha.p, c->Kp, asum, c->d_part.p, nblk,
+                                           c->stream));
         HIP_TRY(hipMemcpyAsync(part.data(), c->d_part.p, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         HIP_TRY(hipStreamSynchronize(c->stream));
         double sden = 0.0;
@@ -396,8 +442,8 @@ int lda_ctx_create(int K, int V, const float* alpha, float eta, int termweight, 
         if (!(alpha[k] < kMaxAlpha)) return fail(LDA_ERR_UNSUPPORTED, "alpha must be below 2^20 on the device");
     }
     if (!(eta > 0.0f)) return fail(LDA_ERR_INVALID_ARGUMENT, "wrong eta value (eta = %f)", (double)eta);
-    if (termweight != LDA_TW_ONE)
-        return fail(LDA_ERR_UNSUPPORTED, "TermWeight %d is not supported by the device sampler (ONE only)", termweight);
+    if (termweight != LDA_TW_ONE && termweight != LDA_TW_IDF && termweight != LDA_TW_PMI)
+        return fail(LDA_ERR_INVALID_ARGUMENT, "invalid TermWeight %d", termweight);
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(LDA_ERR_DEVICE, "no HIP device is available");
     if (device < 0 || device >= ndev) return fail(LDA_ERR_INVALID_ARGUMENT, "device %d out of range (%d devices)", device, ndev);
@@ -459,8 +505,12 @@ int lda_ctx_destroy(lda_ctx* c) {
     return LDA_OK;
 }
 
-int lda_upload_corpus(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off, int64_t D, const uint16_t* z) {
+int lda_upload_corpus(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off, int64_t D, const uint16_t* z,
+                      const float* weights) {
     if (!c) return fail(LDA_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    const bool weighted = c->tw != LDA_TW_ONE;
+    if (weighted && D && !weights)
+        return fail(LDA_ERR_INVALID_ARGUMENT, "a term-weighted model needs token weights");
     if (D < 0 || !doc_off) return fail(LDA_ERR_INVALID_ARGUMENT, "invalid document offsets");
     if (D > 0xFFFFFFFFll) return fail(LDA_ERR_INVALID_ARGUMENT, "too many documents");
     if (doc_off[0] != 0) return fail(LDA_ERR_INVALID_ARGUMENT, "doc_off[0] must be 0");
@@ -479,6 +529,8 @@ int lda_upload_corpus(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off
         if (words[g] < (uint32_t)c->V) {
             ++nvalid;
             if (z && z[g] >= (uint16_t)c->K) return fail(LDA_ERR_INVALID_ARGUMENT, "z out of range at token %llu", (unsigned long long)g);
+            if (weighted && !(weights[g] >= 0.0f && weights[g] < 1e30f))
+                return fail(LDA_ERR_INVALID_ARGUMENT, "token weight %llu is negative or not finite", (unsigned long long)g);
         }
     }
     HIP_TRY(hipSetDevice(c->device));
@@ -504,12 +556,51 @@ int lda_upload_corpus(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off
         HIP_TRY(lda::launch_init_topics(c->d_words.p, c->d_z.p, N, c->d_alpha_cdf.p, c->K, c->V, (uint32_t)c->seed,
                                         (uint32_t)(c->seed >> 32), s));
     }
-    HIP_TRY(hipMemsetAsync(c->d_ndk.p, 0, c->d_ndk.n * sizeof(int32_t), s));
-    HIP_TRY(hipMemsetAsync(c->d_nkw.p, 0, c->d_nkw.n * sizeof(int32_t), s));
-    HIP_TRY(hipMemsetAsync(c->d_nk.p, 0, c->Kp * sizeof(int32_t), s));
-    HIP_TRY(lda::launch_build_counts(c->kpl, c->d_words.p, c->d_z.p, c->d_off.p, D, c->d_ndk.p, c->d_nkw.p, c->d_nk.p,
-                                     c->Kp, c->V, s));
-    HIP_TRY(lda::launch_doclen(c->d_words.p, c->d_off.p, D, c->V, c->d_doclen.p, s));
+    if (!weighted) {
+        HIP_TRY(hipMemsetAsync(c->d_ndk.p, 0, c->d_ndk.n * sizeof(int32_t), s));
+        HIP_TRY(hipMemsetAsync(c->d_nkw.p, 0, c->d_nkw.n * sizeof(int32_t), s));
+        HIP_TRY(hipMemsetAsync(c->d_nk.p, 0, c->Kp * sizeof(int32_t), s));
+        HIP_TRY(lda::launch_build_counts(c->kpl, c->d_words.p, c->d_z.p, c->d_off.p, D, c->d_ndk.p, c->d_nkw.p, c->d_nk.p,
+                                         c->Kp, c->V, s));
+        HIP_TRY(lda::launch_doclen(c->d_words.p, c->d_off.p, D, c->V, c->d_doclen.p, s));
+    } else {
+        // float counts: addWordTo<+w> in token order (updateStateWithDoc,
+        // LDAModel.hpp:776-813), accumulated on the host so that the sums
+        // follow tomoto's order; per-document weight sums (getSumWordWeight)
+        c->wt.assign(weights, weights + N);
+        std::vector<uint16_t> zh(N);
+        if (z) std::copy(z, z + N, zh.begin());
+        else if (N) {
+            HIP_TRY(hipMemcpyAsync(zh.data(), c->d_z.p, N * sizeof(uint16_t), hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+        }
+        std::vector<float> ndk((size_t)std::max<int64_t>(D, 1) * c->Kp, 0.0f), nkw((size_t)c->V * c->Kp, 0.0f),
+            nk(c->Kp, 0.0f), docw(std::max<int64_t>(D, 1), 0.0f);
+        c->weighted_n = 0.0;
+        for (int64_t d = 0; d < D; ++d) {
+            float sw = 0.0f;
+            for (uint64_t g = doc_off[d]; g < doc_off[d + 1]; ++g) {
+                const uint32_t v = words[g];
+                if (v >= (uint32_t)c->V) continue;
+                const float w = weights[g];
+                const int q = c->pos(zh[g]);
+                ndk[(size_t)d * c->Kp + q] += w;
+                nkw[(size_t)v * c->Kp + q] += w;
+                nk[q] += w;
+                sw += w;
+                c->weighted_n += (double)w;  // countRealN (TopicModel.hpp:507-524)
+            }
+            docw[d] = sw;
+        }
+        HIP_TRY(c->d_wt.alloc(std::max<uint64_t>(N, 1)));
+        HIP_TRY(c->d_docw.alloc(std::max<int64_t>(D, 1)));
+        if (N) HIP_TRY(hipMemcpyAsync(c->d_wt.p, weights, N * sizeof(float), hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(c->d_docw.p, docw.data(), docw.size() * sizeof(float), hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(c->d_ndk.p, ndk.data(), ndk.size() * sizeof(float), hipMemcpyHostToDevice, s));
+        if (!nkw.empty()) HIP_TRY(hipMemcpyAsync(c->d_nkw.p, nkw.data(), nkw.size() * sizeof(float), hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(c->d_nk.p, nk.data(), nk.size() * sizeof(float), hipMemcpyHostToDevice, s));
+        HIP_TRY(hipStreamSynchronize(s));
+    }
     // documents longest first for the device-wide schedule
     std::vector<uint32_t> order(D);
     std::iota(order.begin(), order.end(), 0u);
@@ -523,6 +614,7 @@ int lda_upload_corpus(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off
 }
 
 int lda_set_counts(lda_ctx* c, const int32_t* nkw, const int32_t* nk) {
+    if (c && c->tw != LDA_TW_ONE) return fail(LDA_ERR_UNSUPPORTED, "lda_set_counts takes integer counts (TermWeight ONE)");
     if (!c || !nkw || !nk) return fail(LDA_ERR_INVALID_ARGUMENT, "NULL argument");
     HIP_TRY(hipSetDevice(c->device));
     std::vector<int32_t> a((size_t)c->V * c->Kp, 0), b(c->Kp, 0);
@@ -629,8 +721,13 @@ int lda_loglik(lda_ctx* c, double* ll_docs, double* ll_rest) {
     for (int k = 0; k < c->K; ++k) asum += (double)c->alpha[k];
     const double eta = (double)c->eta, veta = (double)c->V * eta;
     const int64_t D = c->has_corpus ? c->D : 0;
-    HIP_TRY(lda::launch_loglik(c->d_ndk.p, c->d_doclen.p, D, c->d_nkw.p, c->d_nk.p, c->d_alpha.p, c->V, c->Kp, asum, eta,
-                               veta, c->d_part.p, nblk, c->stream));
+    if (c->tw != LDA_TW_ONE)
+        HIP_TRY(lda::launch_loglik_f(reinterpret_cast<const float*>(c->d_ndk.p), c->d_docw.p, D,
+                                     reinterpret_cast<const float*>(c->d_nkw.p), reinterpret_cast<const float*>(c->d_nk.p),
+                                     c->d_alpha.p, c->V, c->Kp, asum, eta, veta, c->d_part.p, nblk, c->stream));
+    else
+        HIP_TRY(lda::launch_loglik(c->d_ndk.p, c->d_doclen.p, D, c->d_nkw.p, c->d_nk.p, c->d_alpha.p, c->V, c->Kp, asum,
+                                   eta, veta, c->d_part.p, nblk, c->stream));
     std::vector<double> part(2 * nblk);
     HIP_TRY(hipMemcpyAsync(part.data(), c->d_part.p, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
@@ -647,6 +744,8 @@ int lda_loglik(lda_ctx* c, double* ll_docs, double* ll_rest) {
 int lda_check_invariants(lda_ctx* c, int64_t flags[8]) {
     if (!c || !flags) return fail(LDA_ERR_INVALID_ARGUMENT, "NULL argument");
     if (!c->has_corpus) return fail(LDA_ERR_STATE, "no corpus uploaded");
+    if (c->tw != LDA_TW_ONE)
+        return fail(LDA_ERR_UNSUPPORTED, "exact count invariants do not apply to float (term-weighted) counts");
     HIP_TRY(hipSetDevice(c->device));
     const hipStream_t s = c->stream;
     DevBuf<int32_t> sndk, snkw;
@@ -692,6 +791,43 @@ int lda_check_invariants(lda_ctx* c, int64_t flags[8]) {
     return LDA_OK;
 }
 
+int lda_download_state_f(lda_ctx* c, uint16_t* z, float* ndk, float* nkw, float* nk) {
+    if (!c) return fail(LDA_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (c->tw == LDA_TW_ONE) return fail(LDA_ERR_STATE, "integer counts: use lda_download_state");
+    HIP_TRY(hipSetDevice(c->device));
+    const hipStream_t s = c->stream;
+    if (z && c->N) HIP_TRY(hipMemcpyAsync(z, c->d_z.p, c->N * sizeof(uint16_t), hipMemcpyDeviceToHost, s));
+    std::vector<float> a, b, d;
+    if (ndk && c->D) {
+        a.resize((size_t)c->D * c->Kp);
+        HIP_TRY(hipMemcpyAsync(a.data(), c->d_ndk.p, a.size() * sizeof(float), hipMemcpyDeviceToHost, s));
+    }
+    if (nkw && c->V) {
+        b.resize((size_t)c->V * c->Kp);
+        HIP_TRY(hipMemcpyAsync(b.data(), c->d_nkw.p, b.size() * sizeof(float), hipMemcpyDeviceToHost, s));
+    }
+    if (nk) {
+        d.resize(c->Kp);
+        HIP_TRY(hipMemcpyAsync(d.data(), c->d_nk.p, d.size() * sizeof(float), hipMemcpyDeviceToHost, s));
+    }
+    HIP_TRY(hipStreamSynchronize(s));
+    for (int k = 0; k < c->K; ++k) {
+        const int q = c->pos(k);
+        if (ndk)
+            for (int64_t i2 = 0; i2 < c->D; ++i2) ndk[(size_t)i2 * c->K + k] = a[(size_t)i2 * c->Kp + q];
+        if (nkw)
+            for (int v = 0; v < c->V; ++v) nkw[(size_t)v * c->K + k] = b[(size_t)v * c->Kp + q];
+        if (nk) nk[k] = d[q];
+    }
+    return LDA_OK;
+}
+
+int lda_get_weighted_n(lda_ctx* c, double* weighted_n) {
+    if (!c || !weighted_n) return fail(LDA_ERR_INVALID_ARGUMENT, "NULL argument");
+    *weighted_n = c->tw == LDA_TW_ONE ? (double)c->n_valid : c->weighted_n;
+    return LDA_OK;
+}
+
 int lda_download_state(lda_ctx* c, uint16_t* z, int32_t* ndk, int32_t* nkw, int32_t* nk) {
     if (!c) return fail(LDA_ERR_INVALID_ARGUMENT, "ctx is NULL");
     HIP_TRY(hipSetDevice(c->device));
@@ -731,6 +867,7 @@ int lda_get_dims(lda_ctx* c, int64_t dims[6]) {
 
 int lda_sample_frozen(lda_ctx* c, int64_t n, const int64_t* doc, const uint32_t* word, const uint16_t* z_old,
                       const float* u, uint16_t* z_new, float* total) {
+    if (c && c->tw != LDA_TW_ONE) return fail(LDA_ERR_UNSUPPORTED, "lda_sample_frozen takes integer counts (TermWeight ONE)");
     if (!c) return fail(LDA_ERR_INVALID_ARGUMENT, "ctx is NULL");
     if (n < 0 || (n && (!doc || !word || !z_old || !u || !z_new))) return fail(LDA_ERR_INVALID_ARGUMENT, "NULL argument");
     if (!c->has_corpus) return fail(LDA_ERR_STATE, "no corpus uploaded");
@@ -767,6 +904,7 @@ int lda_sample_frozen(lda_ctx* c, int64_t n, const int64_t* doc, const uint32_t*
 int lda_infer(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off, int64_t D, int iterations, uint64_t infer_seed,
               int mode, float* theta, double* ll, uint16_t* z_out) {
     if (!c) return fail(LDA_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (c->tw != LDA_TW_ONE) return fail(LDA_ERR_UNSUPPORTED, "fold-in of term-weighted models is not on the device yet");
     if (D < 0 || (D && (!doc_off || !theta || !ll))) return fail(LDA_ERR_INVALID_ARGUMENT, "NULL argument");
     if (iterations < 0) return fail(LDA_ERR_INVALID_ARGUMENT, "iterations must be >= 0");
     if (mode != 0 && mode != 1) return fail(LDA_ERR_INVALID_ARGUMENT, "mode must be 0 or 1");

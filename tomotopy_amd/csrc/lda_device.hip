@@ -553,6 +553,258 @@ __global__ void replicate_kernel(int32_t* __restrict__ dst, const int32_t* __res
     }
 }
 
+// ------------------------------------------------ term-weighted training ----
+// TermWeight IDF / PMI (LDAModel.hpp:287-299, Utils.hpp:132-155): every token
+// carries a float weight w; adding it is val += w, removing it is
+// val = max(val - w, 0) (tomoto's CntUpdater<true>), and the counts are float.
+// The sampler state mirrors the integer one with float fields: per wavefront
+// n_dk (ND) and the current word's column; per workgroup n_k and its
+// reciprocal.  The weights are the same quotient ((n_dk+alpha)(n_kw+eta)) /
+// (n_k+V*eta), computed with the Markstein correction.
+struct WaveLdsW {
+    float* nd;
+    float* col;
+    float* nk;
+    float* y;
+};
+
+template <int KPL>
+__device__ __forceinline__ WaveLdsW wave_lds_w() {
+    const WaveLds S = wave_lds<KPL>();
+    return WaveLdsW{S.A, reinterpret_cast<float*>(S.col), reinterpret_cast<float*>(S.nk), S.y};
+}
+
+// tomoto's max(val - w, 0): std::max(a, 0) keeps a unless a < 0
+__device__ __forceinline__ float sub_clamp(float v, float w) {
+    const float a = v - w;
+    return a < 0.0f ? 0.0f : a;
+}
+
+// max(x - w, 0) on a shared float with a compare-and-swap loop
+__device__ __forceinline__ void atomic_sub_clamp(float* p, float w) {
+    unsigned int* u = reinterpret_cast<unsigned int*>(p);
+    unsigned int old = *u, assumed;
+    do {
+        assumed = old;
+        const float nv = sub_clamp(__uint_as_float(assumed), w);
+        old = atomicCAS(u, assumed, __float_as_uint(nv));
+    } while (old != assumed);
+}
+
+// addWordTo<+-w> of topic z on this worker's state (owner lane).  SHARED:
+// the workgroup's wavefronts share n_k (LDS atomics; y then follows a value
+// n_k had a moment earlier, which the Markstein step absorbs).
+template <int KPL, bool SHARED>
+__device__ __forceinline__ void bump_w(const WaveLdsW& S, int lane, int z, float w, bool dec, float veta) {
+    const int sl = z & (KPL - 1);
+    if (lane == (z >> kpl_shift<KPL>())) {
+        const int q = kpos<KPL>(lane, sl);
+        S.nd[q] = dec ? sub_clamp(S.nd[q], w) : S.nd[q] + w;
+        S.col[q] = dec ? sub_clamp(S.col[q], w) : S.col[q] + w;
+        float n;
+        if constexpr (SHARED) {
+            if (dec) atomic_sub_clamp(S.nk + q, w);
+            else atomicAdd(S.nk + q, w);
+            n = S.nk[q];
+        } else {
+            n = dec ? sub_clamp(S.nk[q], w) : S.nk[q] + w;
+            S.nk[q] = n;
+        }
+        S.y[q] = 1.0f / (n + veta);
+    }
+}
+
+template <int KPL, bool SHARED>
+__device__ __forceinline__ int token_step_w(const WaveLdsW& S, const fvec<KPL>& al, int lane, int zo, float wt, float u,
+                                            int K, int nvalid, int lastLane, float eta, float veta) {
+    bump_w<KPL, SHARED>(S, lane, zo, wt, true, veta);
+    fvec<KPL> nd, col, nk, y;
+    lds_load_f<KPL>(S.nd, lane, nd);
+    lds_load_f<KPL>(S.col, lane, col);
+    lds_load_f<KPL>(S.nk, lane, nk);
+    lds_load_f<KPL>(S.y, lane, y);
+    fvec<KPL> w;
+#pragma unroll
+    for (int j = 0; j < KPL; ++j) {
+        const float a = (nd[j] + al[j]) * (col[j] + eta);
+        const float bf = nk[j] + veta;
+        const float q0 = a * y[j];
+        const float rr = __builtin_fmaf(-q0, bf, a);
+        w[j] = __builtin_fmaf(rr, y[j], q0);
+    }
+    float tot;
+    const int zn = scan_search<KPL>(w, u, K, nvalid, lastLane, tot);
+    bump_w<KPL, SHARED>(S, lane, zn, wt, false, veta);
+    return zn;
+}
+
+// sampleDocument over [b, e) with token weights; MODE_DET: plain stores of
+// the column cells to n_kw, MODE_ATOMIC: float atomics (removal clamped).
+template <int KPL, int MODE>
+__device__ __forceinline__ void sample_range_w(const WaveLdsW& S, const fvec<KPL>& al, const SweepParams& p, int lane,
+                                               uint64_t b, uint64_t e, int nvalid, int lastLane) {
+    constexpr int SH = kpl_shift<KPL>();
+    constexpr bool SHARED = MODE == MODE_ATOMIC;
+    float* nkwf = reinterpret_cast<float*>(p.nkw);
+    int vcur = -1;
+    float* colp = nullptr;
+    for (uint64_t t0 = b; t0 < e; t0 += 64) {
+        const int n = (int)((e - t0) < 64 ? (e - t0) : 64);
+        const uint64_t g = t0 + (uint64_t)lane;
+        int wv = -1, zv = 0xFFFF;
+        float uv = 0.0f, tw = 0.0f;
+        if (lane < n) {
+            wv = (int)p.words[g];
+            zv = p.z[g];
+            tw = p.wt[g];
+            uv = uniform24(philox_x0(p.seed_lo, p.seed_hi, (uint32_t)g, (uint32_t)(g >> 32), p.sweep, TAG_SWEEP));
+        }
+        int zkeep = zv;
+        int vnext = -1;
+        fvec<KPL> ncol;
+        for (int i = 0; i < n; ++i) {
+            const int v = readlane_i(wv, i);
+            if ((unsigned)v >= (unsigned)p.V) continue;
+            if (v != vcur) {
+                colp = nkwf + (int64_t)v * p.Kp;
+                if (v != vnext) load_kvec_f<KPL>(colp, lane, ncol);
+                lds_store_f<KPL>(S.col, lane, ncol);
+                vcur = v;
+            }
+            if (i + 1 < n) {
+                const int v2 = readlane_i(wv, i + 1);
+                if (v2 != v && (unsigned)v2 < (unsigned)p.V) {
+                    load_kvec_f<KPL>(nkwf + (int64_t)v2 * p.Kp, lane, ncol);
+                    vnext = v2;
+                }
+            }
+            const int zo = readlane_i(zv, i);
+            const float w = readlane_f(tw, i);
+            const float u = readlane_f(uv, i);
+            const int zn = token_step_w<KPL, SHARED>(S, al, lane, zo, w, u, p.K, nvalid, lastLane, p.eta, p.veta);
+            const int lo = zo >> SH, ln = zn >> SH;
+            const int qo = kpos<KPL>(lane, zo & (KPL - 1)), qn = kpos<KPL>(lane, zn & (KPL - 1));
+            if constexpr (MODE == MODE_DET) {
+                if (lane == lo) colp[qo] = S.col[qo];
+                if (lane == ln) colp[qn] = S.col[qn];
+            } else {
+                if (lane == lo) atomic_sub_clamp(colp + qo, w);
+                if (lane == ln) atomicAdd(colp + qn, w);
+            }
+            zkeep = (lane == i) ? zn : zkeep;
+        }
+        if (lane < n) p.z[g] = (uint16_t)zkeep;
+    }
+}
+
+// Device-wide weighted sweep (see sweep_device_kernel).  n_k is rebuilt from
+// n_kw after the sweep (launch_clamp_rowsum_f), tomoto's merge rule for
+// weighted counts, so the workgroups' private n_k need no flush.
+template <int KPL>
+__global__ void __launch_bounds__(64 * kSweepWaves) sweep_device_w_kernel(SweepParams p) {
+    const int lane = lane_id();
+    const WaveLdsW S = wave_lds_w<KPL>();
+    int nvalid, lastLane;
+    lane_limits<KPL>(p.K, lane, nvalid, lastLane);
+    fvec<KPL> al;
+    load_kvec_f<KPL>(p.alpha, lane, al);
+    if (threadIdx.x < 64) {
+        fvec<KPL> n, y;
+        load_kvec_f<KPL>(reinterpret_cast<const float*>(p.nk_in), lane, n);
+#pragma unroll
+        for (int j = 0; j < KPL; ++j) y[j] = 1.0f / (n[j] + p.veta);
+        lds_store_f<KPL>(S.nk, lane, n);
+        lds_store_f<KPL>(S.y, lane, y);
+    }
+    __syncthreads();
+    for (;;) {
+        uint32_t idx = 0;
+        if (lane == 0) idx = atomicAdd(p.counter, 1u);
+        idx = (uint32_t)uniform_i((int)idx);
+        if ((int64_t)idx >= p.D) break;
+        const uint32_t d = p.doc_order[idx];
+        const uint64_t b = p.off[d], e = p.off[d + 1];
+        float* ndk_row = reinterpret_cast<float*>(p.ndk) + (int64_t)d * p.Kp;
+        fvec<KPL> nd;
+        load_kvec_f<KPL>(ndk_row, lane, nd);
+        lds_store_f<KPL>(S.nd, lane, nd);
+        sample_range_w<KPL, MODE_ATOMIC>(S, al, p, lane, b, e, nvalid, lastLane);
+        lds_load_f<KPL>(S.nd, lane, nd);
+        store_kvec<KPL>(reinterpret_cast<int32_t*>(ndk_row), lane, __builtin_bit_cast(ivec<KPL>, nd));
+    }
+}
+
+// One round of the deterministic weighted partition schedule (see
+// sweep_det_kernel): worker-private n_k, plain stores to n_kw.
+template <int KPL>
+__global__ void __launch_bounds__(64) sweep_det_w_kernel(SweepParams p) {
+    const int lane = lane_id();
+    const int w = blockIdx.x;
+    const WaveLdsW S = wave_lds_w<KPL>();
+    int nvalid, lastLane;
+    lane_limits<KPL>(p.K, lane, nvalid, lastLane);
+    fvec<KPL> al;
+    load_kvec_f<KPL>(p.alpha, lane, al);
+    float* nkp = reinterpret_cast<float*>(p.nk_out) + (int64_t)w * p.Kp;
+    {
+        fvec<KPL> n, y;
+        load_kvec_f<KPL>(nkp, lane, n);
+#pragma unroll
+        for (int j = 0; j < KPL; ++j) y[j] = 1.0f / (n[j] + p.veta);
+        lds_store_f<KPL>(S.nk, lane, n);
+        lds_store_f<KPL>(S.y, lane, y);
+    }
+    const int W = p.W;
+    const int didx = (p.round + w) % W;
+    const uint64_t N = (uint64_t)((p.D + (W - 1) - didx) / W);
+    const uint32_t sseed = philox_x0(p.seed_lo, p.seed_hi, (uint32_t)w, (uint32_t)p.round, p.sweep, TAG_SHUFFLE);
+    uint64_t P = 1;
+    if (N) {
+        P = kShufflePrimes[sseed & 0xF];
+        if (N % P == 0) {
+            P = kShufflePrimes[(sseed + 1) & 0xF];
+            if (N % P == 0) P = kShufflePrimes[(sseed + 2) & 0xF];
+            if (N % P == 0) P = kShufflePrimes[(sseed + 3) & 0xF];
+        }
+        P %= N;
+    }
+    for (uint64_t i = 0; i < N; ++i) {
+        const uint64_t id = ((i + (uint64_t)sseed) * P) % N;
+        const int64_t d = (int64_t)id * W + didx;
+        const uint64_t b = p.coff[(int64_t)w * p.D + d], e = p.coff[(int64_t)(w + 1) * p.D + d];
+        if (b == e) continue;
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+        float* ndk_row = reinterpret_cast<float*>(p.ndk) + d * p.Kp;
+        fvec<KPL> nd;
+        load_kvec_f<KPL>(ndk_row, lane, nd);
+        lds_store_f<KPL>(S.nd, lane, nd);
+        sample_range_w<KPL, MODE_DET>(S, al, p, lane, b, e, nvalid, lastLane);
+        lds_load_f<KPL>(S.nd, lane, nd);
+        store_kvec<KPL>(reinterpret_cast<int32_t*>(ndk_row), lane, __builtin_bit_cast(ivec<KPL>, nd));
+    }
+    fvec<KPL> mine;
+    lds_load_f<KPL>(S.nk, lane, mine);
+    store_kvec<KPL>(reinterpret_cast<int32_t*>(nkp), lane, __builtin_bit_cast(ivec<KPL>, mine));
+}
+
+// mergeState for weighted counts (LDAModel.hpp:565-569): n_kw = max(n_kw, 0)
+// cell by cell, then n_k = rowwise sum, accumulated in float in word order.
+__global__ void clamp_rowsum_f_kernel(float* __restrict__ nkw, int V, int Kp, float* nk) {
+    const int pos = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pos >= Kp) return;
+    float s = 0.0f;
+    for (int v = 0; v < V; ++v) {
+        float x = nkw[(int64_t)v * Kp + pos];
+        if (x < 0.0f) {
+            x = 0.0f;
+            nkw[(int64_t)v * Kp + pos] = x;
+        }
+        s = s + x;
+    }
+    nk[pos] = s;
+}
+
+
 // ---------------------------------------------------------------- init ----
 // Initial topics z ~ Discrete(alpha) (LDAModel.hpp:754-762): r = u*cdf[K-1],
 // first k with r < cdf[k] (binary search on the non-decreasing cdf).
@@ -632,9 +884,10 @@ __device__ __forceinline__ double block_sum_d(double x, double* sh) {
 
 // getLLDocs + getLLRest in FP64 (LDAModel.hpp:616-666).  Block b writes its
 // docs-part partial to partial[b] and its rest-part partial to partial[nblk+b].
-__global__ void __launch_bounds__(256) loglik_kernel(const int32_t* __restrict__ ndk, const int32_t* __restrict__ doclen,
-                                                     int64_t D, const int32_t* __restrict__ nkw,
-                                                     const int32_t* __restrict__ nk, const float* __restrict__ alpha,
+template <typename C>
+__global__ void __launch_bounds__(256) loglik_kernel(const C* __restrict__ ndk, const C* __restrict__ doclen,
+                                                     int64_t D, const C* __restrict__ nkw,
+                                                     const C* __restrict__ nk, const float* __restrict__ alpha,
                                                      int V, int Kp, double asum, double eta, double veta,
                                                      double* partial) {
     __shared__ double sh[8];
@@ -643,7 +896,7 @@ __global__ void __launch_bounds__(256) loglik_kernel(const int32_t* __restrict__
     double docs = 0.0, rest = 0.0;
     const int64_t nd = D * (int64_t)Kp;
     for (int64_t i = tid; i < nd; i += nth) {
-        int c = ndk[i];
+        const C c = ndk[i];
         if (c) {
             double a = (double)alpha[i % Kp];
             docs += lgamma((double)c + a) - lgamma(a);
@@ -654,7 +907,7 @@ __global__ void __launch_bounds__(256) loglik_kernel(const int32_t* __restrict__
     const double lge = lgamma(eta);
     const int64_t nv = (int64_t)V * Kp;
     for (int64_t i = tid; i < nv; i += nth) {
-        int c = nkw[i];
+        const C c = nkw[i];
         if (c) rest += lgamma((double)c + eta) - lge;
     }
     for (int64_t i = tid; i < Kp; i += nth) {
@@ -692,8 +945,9 @@ __device__ __forceinline__ float digamma_approx_f(float z) {
 // Partial sums for optimizeParameters (LDAModel.hpp:221-265): block b sums
 // documents [b*D/nblk, (b+1)*D/nblk) for every topic position (threads stride
 // over positions) and, in column Kp, the denominator term.
-__global__ void __launch_bounds__(256) alpha_sums_kernel(const int32_t* __restrict__ ndk,
-                                                         const int32_t* __restrict__ doclen, int64_t D,
+template <typename C>
+__global__ void __launch_bounds__(256) alpha_sums_kernel(const C* __restrict__ ndk,
+                                                         const C* __restrict__ doclen, int64_t D,
                                                          const float* __restrict__ alpha, int Kp, float asum,
                                                          double* part) {
     const int64_t d0 = D * blockIdx.x / gridDim.x, d1 = D * (blockIdx.x + 1) / gridDim.x;
@@ -705,7 +959,7 @@ __global__ void __launch_bounds__(256) alpha_sums_kernel(const int32_t* __restri
                 float f0 = digamma_approx_f(a);
                 int64_t zeros = 0;
                 for (int64_t d = d0; d < d1; ++d) {
-                    int c = ndk[d * Kp + pos];
+                    const C c = ndk[d * Kp + pos];
                     if (c) s += (double)digamma_approx_f((float)c + a);
                     else ++zeros;
                 }
@@ -1404,14 +1658,64 @@ hipError_t launch_rowsum_nkw(const int32_t* nkw, int V, int Kp, int32_t* nk, hip
 hipError_t launch_loglik(const int32_t* ndk, const int32_t* doclen, int64_t D, const int32_t* nkw, const int32_t* nk,
                          const float* alpha, int V, int Kp, double asum, double eta, double veta, double* partial,
                          int nblk, hipStream_t s) {
-    hipLaunchKernelGGL(loglik_kernel, dim3(nblk), dim3(256), 0, s, ndk, doclen, D, nkw, nk, alpha, V, Kp, asum, eta,
-                       veta, partial);
+    hipLaunchKernelGGL(loglik_kernel<int32_t>, dim3(nblk), dim3(256), 0, s, ndk, doclen, D, nkw, nk, alpha, V, Kp, asum,
+                       eta, veta, partial);
+    return hipGetLastError();
+}
+
+hipError_t launch_loglik_f(const float* ndk, const float* docw, int64_t D, const float* nkw, const float* nk,
+                           const float* alpha, int V, int Kp, double asum, double eta, double veta, double* partial,
+                           int nblk, hipStream_t s) {
+    hipLaunchKernelGGL(loglik_kernel<float>, dim3(nblk), dim3(256), 0, s, ndk, docw, D, nkw, nk, alpha, V, Kp, asum,
+                       eta, veta, partial);
     return hipGetLastError();
 }
 
 hipError_t launch_alpha_sums(const int32_t* ndk, const int32_t* doclen, int64_t D, const float* alpha, int Kp, float asum,
                              double* part, int nblk, hipStream_t s) {
-    hipLaunchKernelGGL(alpha_sums_kernel, dim3(nblk), dim3(256), 0, s, ndk, doclen, D, alpha, Kp, asum, part);
+    hipLaunchKernelGGL(alpha_sums_kernel<int32_t>, dim3(nblk), dim3(256), 0, s, ndk, doclen, D, alpha, Kp, asum, part);
+    return hipGetLastError();
+}
+
+hipError_t launch_alpha_sums_f(const float* ndk, const float* docw, int64_t D, const float* alpha, int Kp, float asum,
+                               double* part, int nblk, hipStream_t s) {
+    hipLaunchKernelGGL(alpha_sums_kernel<float>, dim3(nblk), dim3(256), 0, s, ndk, docw, D, alpha, Kp, asum, part);
+    return hipGetLastError();
+}
+
+hipError_t launch_sweep_device_w(int kpl, const SweepParams& p, int grid, hipStream_t s) {
+    LDA_DISPATCH_KPL(kpl, hipLaunchKernelGGL(sweep_device_w_kernel<KPL>, dim3(grid), dim3(64 * kSweepWaves),
+                                             wave_lds_bytes<KPL>(kSweepWaves), s, p));
+    return hipGetLastError();
+}
+
+hipError_t launch_sweep_det_w(int kpl, const SweepParams& p, hipStream_t s) {
+    LDA_DISPATCH_KPL(kpl, hipLaunchKernelGGL(sweep_det_w_kernel<KPL>, dim3(p.W), dim3(64), wave_lds_bytes<KPL>(), s, p));
+    return hipGetLastError();
+}
+
+template <int KPL>
+static int sweep_device_w_blocks_kpl() {
+    int n = 0;
+    const hipError_t e =
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sweep_device_w_kernel<KPL>, 64 * kSweepWaves, wave_lds_bytes<KPL>(kSweepWaves));
+    return (e == hipSuccess && n > 0) ? n : 1;
+}
+
+int sweep_device_w_blocks_per_cu(int kpl) {
+    switch (kpl) {
+        case 1: return sweep_device_w_blocks_kpl<1>();
+        case 2: return sweep_device_w_blocks_kpl<2>();
+        case 4: return sweep_device_w_blocks_kpl<4>();
+        case 8: return sweep_device_w_blocks_kpl<8>();
+        case 16: return sweep_device_w_blocks_kpl<16>();
+        case 32: return sweep_device_w_blocks_kpl<32>();
+        default: return 1;
+    }
+}
+
+hipError_t launch_clamp_rowsum_f(float* nkw, int V, int Kp, float* nk, hipStream_t s) {
+    hipLaunchKernelGGL(clamp_rowsum_f_kernel, dim3((Kp + 255) / 256), dim3(256), 0, s, nkw, V, Kp, nk);
     return hipGetLastError();
 }
 

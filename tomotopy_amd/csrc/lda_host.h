@@ -89,6 +89,11 @@ struct lda_ctx {
     std::vector<uint64_t> off;
     std::vector<uint32_t> words;
     bool has_corpus = false;
+    // term-weighted models (tw != LDA_TW_ONE): float counts in the count
+    // buffers, token weights, per-document weight sums and their total
+    std::vector<float> wt;
+    double weighted_n = 0.0;
+    DevBuf<float> d_wt, d_docw;
 
     DevBuf<float> d_alpha, d_alpha_cdf;
     DevBuf<int32_t> d_nkw, d_nk, d_nk_next, d_ndk, d_doclen, d_nk_priv, d_nkw_loc;

@@ -41,6 +41,7 @@ struct SweepParams {
     uint32_t* counter;          // device-wide: next document
     const uint64_t* coff;       // det: (W+1) x D chunk offsets
     int W, round;
+    const float* wt;            // term-weighted models: token weights (counts are float bits)
 };
 
 // One fold-in job: a document's tokens [b, e) in the token arena, the
@@ -93,6 +94,18 @@ struct InferParams {
     double asum_d;           // sum of alpha in double
     float asum_f;            // sequential float sum of alpha
 };
+
+// Term-weighted (IDF / PMI) training: the count buffers hold float32 values.
+hipError_t launch_sweep_device_w(int kpl, const SweepParams& p, int grid, hipStream_t s);
+hipError_t launch_sweep_det_w(int kpl, const SweepParams& p, hipStream_t s);
+int sweep_device_w_blocks_per_cu(int kpl);
+// n_kw = max(n_kw, 0), n_k = rowwise sum of n_kw, in float (LDAModel.hpp:565-569)
+hipError_t launch_clamp_rowsum_f(float* nkw, int V, int Kp, float* nk, hipStream_t s);
+hipError_t launch_loglik_f(const float* ndk, const float* docw, int64_t D, const float* nkw, const float* nk,
+                           const float* alpha, int V, int Kp, double asum, double eta, double veta, double* partial,
+                           int nblk, hipStream_t s);
+hipError_t launch_alpha_sums_f(const float* ndk, const float* docw, int64_t D, const float* alpha, int Kp, float asum,
+                               double* part, int nblk, hipStream_t s);
 
 // kernels (lda_device.hip)
 hipError_t launch_init_topics(const uint32_t* words, uint16_t* z, uint64_t N, const float* alpha_cdf, int K, int V,

@@ -70,7 +70,7 @@ class LDAModel:
         '''Parameters
 ----------
 tw : Union[int, tomotopy_amd.TermWeight]
-    term weighting scheme; the device sampler implements TermWeight.ONE
+    term weighting scheme (TermWeight.ONE, IDF or PMI)
 min_cf : int
     minimum collection frequency of words
 min_df : int
@@ -94,9 +94,6 @@ transform : Callable[dict, dict]
                                      and not k_.startswith("_")})
         self.init_params["version"] = __version__
         tw = _convert_term_weight(tw)
-        if tw != TermWeight.ONE:
-            raise NotImplementedError("TermWeight.%s is not implemented by the device sampler; use TermWeight.ONE"
-                                      % tw.name)
         # LDAModel ctor checks (LDAModel.hpp:964-984), k range (models.py:84)
         if not isinstance(k, (int, np.integer)) or not (1 <= int(k) <= _native.LDA_MAX_K):
             raise ValueError("wrong K value (K = %r); the device sampler supports 1 ~ %d" % (k, _native.LDA_MAX_K))
@@ -137,6 +134,7 @@ transform : Callable[dict, dict]
         self._prepared = False
         self._ctx = None
         self._real_v = 0
+        self._weights = None
         self._cache = {}
         if corpus is not None:
             self.add_corpus(corpus, transform)
@@ -219,8 +217,11 @@ transform : Callable[dict, dict]
         self._words, self._off, self._w_orders = words, off, w_orders
         self._real_v = int(real_v)
         self._n_valid = int(np.count_nonzero(words < real_v))
-        self._ctx = _native.Context(self._k, self._real_v, self._alphas, self._eta, self._seed)
-        self._ctx.upload_corpus(words, off)
+        self._weights = None
+        if self._tw != TermWeight.ONE:
+            self._weights = _corpus.term_weights(words, off, self._real_v, self._tw.name.lower())
+        self._ctx = _native.Context(self._k, self._real_v, self._alphas, self._eta, self._seed, tw=int(self._tw))
+        self._ctx.upload_corpus(words, off, weights=self._weights)
         self._ctx.set_optim(self._optim_interval, self._burn_in)
         self._ctx.global_step = self._global_step
         self._prepared = True
@@ -270,8 +271,11 @@ transform : Callable[dict, dict]
     # --------------------------------------------------------------- state ---
     def _state(self, what):
         if what not in self._cache:
-            z, ndk, nkw, nk = self._ctx.download_state(z=what == "z", ndk=what == "ndk", nkw=what == "nkw",
-                                                       nk=what == "nk")
+            if self._tw != TermWeight.ONE:
+                z, ndk, nkw, nk = self._ctx.download_state_f()
+            else:
+                z, ndk, nkw, nk = self._ctx.download_state(z=what == "z", ndk=what == "ndk", nkw=what == "nkw",
+                                                           nk=what == "nk")
             self._cache[what] = {"z": z, "ndk": ndk, "nkw": nkw, "nk": nk}[what]
         return self._cache[what]
 
@@ -289,7 +293,15 @@ transform : Callable[dict, dict]
         from scipy.special import gammaln  # FP64 closed form of getLLDocs for one document
         a = np.asarray(self._alphas, np.float64)
         ndk = self._doc_ndk(d).astype(np.float64)
-        return float(gammaln(a.sum()) - gammaln(ndk.sum() + a.sum()) + np.sum(gammaln(ndk + a) - gammaln(a)))
+        n = ndk.sum()
+        if self._weights is not None:  # getSumWordWeight: sequential float sum of the weights
+            b, e = int(self._off[d]), int(self._off[d + 1])
+            acc = np.float32(0.0)
+            for w, x in zip(self._words[b:e], self._weights[b:e]):
+                if w < self._real_v:
+                    acc = np.float32(acc + x)
+            n = float(acc)
+        return float(gammaln(a.sum()) - gammaln(n + a.sum()) + np.sum(gammaln(ndk + a) - gammaln(a)))
 
     # ------------------------------------------------------------ read-outs --
     def get_topic_word_dist(self, topic_id, normalize=True):
@@ -311,7 +323,12 @@ transform : Callable[dict, dict]
         return [(self._vocab[i], float(dist[i])) for i in order]
 
     def get_count_by_topics(self):
+        '''Tokens assigned to each topic (LDAModel.hpp:815-826).'''
         self._ensure_prepared()
+        if self._tw != TermWeight.ONE:
+            z = self._state("z")
+            valid = self._words < self._real_v
+            return np.bincount(z[valid].astype(np.int64), minlength=self._k).astype(np.int64).tolist()
         return self._state("nk").astype(np.int64).tolist()
 
     # ---------------------------------------------------------------- infer --
@@ -344,6 +361,9 @@ transform : Callable[dict, dict]
         return self._infer_docs(docs, iterations)
 
     def _infer_docs(self, docs, iterations, mode=0):
+        if self._tw != TermWeight.ONE:
+            raise NotImplementedError("infer() of TermWeight.%s models is not implemented on the device"
+                                      % self._tw.name)
         if not docs:
             return [], []
         lens = [len(d._word_ids) for d in docs]
@@ -421,7 +441,7 @@ transform : Callable[dict, dict]
         if not self._prepared or self._n_valid == 0:
             return 0.0
         a, b = self._ctx.loglik()
-        return (a + b) / self._n_valid
+        return (a + b) / self._ctx.weighted_n()
 
     @property
     def perplexity(self):
@@ -467,7 +487,15 @@ transform : Callable[dict, dict]
 
     @property
     def used_vocab_weighted_freq(self):
-        return self.used_vocab_freq.astype(np.float64)
+        '''Sum of the token weights of every used word (TopicModel.hpp:914-926).'''
+        if not self._prepared:
+            return np.zeros(0, np.float64)
+        if self._weights is None:
+            return self.used_vocab_freq.astype(np.float64)
+        valid = self._words < self._real_v
+        out = np.zeros(self._real_v, np.float64)
+        np.add.at(out, self._words[valid].astype(np.int64), self._weights[valid].astype(np.float64))
+        return out
 
     @property
     def removed_top_words(self):
@@ -487,8 +515,8 @@ transform : Callable[dict, dict]
         new._cache = {}
         if self._prepared:
             z = self._state("z")
-            new._ctx = _native.Context(self._k, self._real_v, self._alphas, self._eta, self._seed)
-            new._ctx.upload_corpus(self._words, self._off, z)
+            new._ctx = _native.Context(self._k, self._real_v, self._alphas, self._eta, self._seed, tw=int(self._tw))
+            new._ctx.upload_corpus(self._words, self._off, z, weights=self._weights)
             new._ctx.set_optim(self._optim_interval, self._burn_in)
             new._ctx.global_step = self._global_step
         return new
