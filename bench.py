@@ -12,7 +12,8 @@ steps (corpus and counts already resident in HBM).  `roofline` prices the
 sweep kernel's algorithmic bytes per launch, N*(4K+8) + D*8K, against the
 8 TB/s HBM peak, using the kernel's average duration from HIP events on the
 library's stream.  `cpu_baseline` times the CPU oracle's partition sweep on
-a bounded sample of the same corpus on this host's cores.
+a bounded sample of the same corpus on this host's cores.  `pmi_variant`
+repeats the training measurement with TermWeight.PMI token weights.
 """
 import argparse
 import json
@@ -77,6 +78,40 @@ def cpu_baseline(ws, offs, V, K, alpha, eta, budget_s=12.0):
         "sample": "first %d of %d docs (%d tokens), K=%d, V=%d, partition scheme with %d workers, %d sweeps in %.1fs"
                   % (n_docs, D, ntok, K, V, threads, sweeps, dt),
     }
+
+
+def pmi_bench(ws, offs, V, K, alpha, eta, steps, warmup):
+    """The same corpus and model with TermWeight.PMI token weights: float
+    counts, clamped removals and the weighted merge (n_k rebuilt from n_kw)
+    after every sweep.  Algorithmic bytes per sweep launch N*(4K+12) + D*8K
+    (the token weight adds 4 bytes per token)."""
+    from tomotopy_amd import _native, corpus
+    t = time.time()
+    wts = corpus.term_weights(ws, offs, V, "pmi")
+    D = len(offs) - 1
+    n_valid = int(np.count_nonzero(ws < V))
+    ctx = _native.Context(K, V, alpha, eta, seed=1234, tw=2)
+    ctx.upload_corpus(ws, offs, weights=wts)
+    log("PMI weights + upload: %.2fs" % (time.time() - t))
+    for i in range(warmup):
+        ctx.train(1, workers=0)
+        log("PMI warmup step %d: %.2f ms kernel" % (i, ctx.kernel_stats()["ms"]))
+    t0 = time.perf_counter()
+    ctx.train(steps, workers=0)
+    dt = time.perf_counter() - t0
+    st = ctx.kernel_stats()
+    kern_ms = st["ms"] / max(1, st["launches"])
+    bytes_per_launch = n_valid * (4 * K + 12) + D * 8 * K
+    achieved = bytes_per_launch / (kern_ms * 1e-3) / 1e9
+    lld, llr = ctx.loglik()
+    wn = ctx.weighted_n()
+    out = {"value": n_valid * steps / dt, "unit": "tokens/s", "ms_per_step": dt * 1e3 / steps, "steps": steps,
+           "kernel": "sweep_device_w_kernel<16>", "kernel_ms": kern_ms, "achieved_GBs": achieved,
+           "frac": achieved / HBM_PEAK_GBS, "algorithmic_bytes_per_launch": bytes_per_launch,
+           "ll_per_word": (lld + llr) / wn, "weighted_n": wn}
+    log("PMI: %.3g tokens/s, kernel %.3f ms (%.1f%% of HBM peak), %.3f ms/step, ll_per_word %.5f"
+        % (out["value"], kern_ms, 100 * out["frac"], out["ms_per_step"], out["ll_per_word"]))
+    return out
 
 
 def infer_bench(plain_sample=300, iterations=100):
@@ -275,6 +310,7 @@ def main():
     ap.add_argument("--corpus-cache", default=None, help="npz file to reuse the generated corpus")
     ap.add_argument("--no-infer", action="store_true", help="skip the fold-in measurement")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
+    ap.add_argument("--no-pmi", action="store_true", help="skip the TermWeight.PMI training variant")
     ap.add_argument("--only", choices=["train", "infer", "serve"], default=None,
                     help="run one leg only (the JSON line then carries that leg's numbers)")
     args = ap.parse_args()
@@ -335,12 +371,15 @@ def main():
                      "algorithmic_bytes_per_launch": bytes_per_launch},
         "ll_per_word": (lld + llr) / n_valid,
     }
+    if not args.no_pmi:
+        del ctx
+        out["pmi_variant"] = pmi_bench(ws, offs, V, K, alpha, eta, args.steps, args.warmup)
     if rank == 0 and not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(ws, offs, V, K, alpha, eta)
     if args.only == "train":
         pass
     elif not args.no_infer:
-        del ctx
+        ctx = None
         out["infer"] = infer_bench()
     if not args.no_serve and args.only is None:
         out["serve"] = serve_bench()

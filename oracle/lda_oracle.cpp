@@ -554,15 +554,22 @@ void sample_range_w(Model& m, int64_t d, uint64_t b, uint64_t e, float* nk_local
 }
 
 // mergeState<partition> for float counts (LDAModel.hpp:572-579): n_kw =
-// max(n_kw, 0) cell by cell, then n_k = the row sums of n_kw, accumulated in
-// float over the words in ascending order.  (Eigen's rowwise().sum() groups
-// the terms by its packet width, so tomoto's n_k can differ from this one in
-// the last bits; the device follows this order.)
+// max(n_kw, 0) cell by cell, then n_k = the row sums of n_kw in float, in a
+// fixed two-level order: words in blocks of 256, each block summed in
+// ascending word order, then the block sums in ascending block order.
+// (Eigen's rowwise().sum() groups the terms by its SIMD packet width, so
+// tomoto's own order depends on the build; the device follows this one.)
+constexpr int kRowsumBlock = 256;
 void merge_w(Model& m) {
     for (auto& x : m.nkw_f) if (x < 0.0f) x = 0.0f;
     std::fill(m.nk_f.begin(), m.nk_f.end(), 0.0f);
-    for (int v = 0; v < m.V; ++v)
-        for (int k = 0; k < m.K; ++k) m.nk_f[k] = m.nk_f[k] + m.nkw_f[(size_t)v * m.K + k];
+    std::vector<float> part(m.K);
+    for (int v0 = 0; v0 < m.V; v0 += kRowsumBlock) {
+        std::fill(part.begin(), part.end(), 0.0f);
+        for (int v = v0; v < std::min(m.V, v0 + kRowsumBlock); ++v)
+            for (int k = 0; k < m.K; ++k) part[k] = part[k] + m.nkw_f[(size_t)v * m.K + k];
+        for (int k = 0; k < m.K; ++k) m.nk_f[k] = m.nk_f[k] + part[k];
+    }
 }
 
 // Partition sweep (the schedule of sweep_partition) with token weights.

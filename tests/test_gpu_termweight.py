@@ -84,7 +84,9 @@ def test_weighted_device_wide_sweep(scheme, K):
     valid = ws < V
     assert z[valid].max() < K
     assert (ndk >= 0).all() and (nkw >= 0).all() and (nk >= 0).all()
-    seq = np.cumsum(nkw, axis=0, dtype=np.float32)[-1]  # sequential float sum over words
+    # float row sums: blocks of 256 words in word order, then the blocks in order
+    blocks = [np.cumsum(nkw[v0:v0 + 256], axis=0, dtype=np.float32)[-1] for v0 in range(0, V, 256)]
+    seq = np.cumsum(np.array(blocks), axis=0, dtype=np.float32)[-1]
     assert np.array_equal(nk.view(np.uint32), seq.view(np.uint32))
     _, _, _, _, docw = m.state()
     np.testing.assert_allclose(ndk.sum(axis=1), docw, rtol=2e-3, atol=1e-3)

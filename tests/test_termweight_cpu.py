@@ -138,12 +138,16 @@ def _py_weighted_sweeps(words, off, wts, K, V, alpha, eta, seed, sweeps, z0):
                 k = O.search(cdf, K, O.token_uniform(seed, g, s, "sweep"))
                 z[g] = k
                 ndk[d][k] = f(ndk[d][k] + w); nkw[v][k] = f(nkw[v][k] + w); nk[k] = f(nk[k] + w)
-        # weighted merge: n_k = sequential float row sums over words
+        # weighted merge: n_k = float row sums over words, blocks of 256 words
+        # summed in word order, then the block sums in block order
         nk = [f(0)] * K
-        for v in range(V):
-            for j in range(K):
-                nkw[v][j] = clamp(nkw[v][j])
-                nk[j] = f(nk[j] + nkw[v][j])
+        for v0 in range(0, V, 256):
+            part = [f(0)] * K
+            for v in range(v0, min(V, v0 + 256)):
+                for j in range(K):
+                    nkw[v][j] = clamp(nkw[v][j])
+                    part[j] = f(part[j] + nkw[v][j])
+            nk = [f(nk[j] + part[j]) for j in range(K)]
     return (np.array(z, np.uint16), np.array(ndk, np.float32), np.array(nkw, np.float32), np.array(nk, np.float32),
             hits[0])
 

@@ -292,7 +292,7 @@ int sweep_once(lda_ctx* c, int W, bool copy_merge) {
             HIP_TRY(hipEventRecord(c->ev1, s));
             c->stat_launches += W;
         }
-        HIP_TRY(lda::launch_clamp_rowsum_f(reinterpret_cast<float*>(c->d_nkw.p), c->V, c->Kp,
+        HIP_TRY(lda::launch_clamp_rowsum_f(reinterpret_cast<float*>(c->d_nkw.p), c->V, c->Kp, c->d_rowpart.p,
                                            reinterpret_cast<float*>(c->d_nk.p), s));
         HIP_TRY(hipEventSynchronize(c->ev1));
         HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
@@ -593,6 +593,7 @@ int lda_upload_corpus(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off
             docw[d] = sw;
         }
         HIP_TRY(c->d_wt.alloc(std::max<uint64_t>(N, 1)));
+        HIP_TRY(c->d_rowpart.alloc((size_t)std::max(1, (c->V + lda::kRowsumBlock - 1) / lda::kRowsumBlock) * c->Kp));
         HIP_TRY(c->d_docw.alloc(std::max<int64_t>(D, 1)));
         if (N) HIP_TRY(hipMemcpyAsync(c->d_wt.p, weights, N * sizeof(float), hipMemcpyHostToDevice, s));
         HIP_TRY(hipMemcpyAsync(c->d_docw.p, docw.data(), docw.size() * sizeof(float), hipMemcpyHostToDevice, s));

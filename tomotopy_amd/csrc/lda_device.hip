@@ -638,8 +638,11 @@ __device__ __forceinline__ int token_step_w(const WaveLdsW& S, const fvec<KPL>& 
     return zn;
 }
 
-// sampleDocument over [b, e) with token weights; MODE_DET: plain stores of
-// the column cells to n_kw, MODE_ATOMIC: float atomics (removal clamped).
+// sampleDocument over [b, e) with token weights.  The wavefront's own state
+// (n_dk, the column, the workgroup's n_k) follows tomoto's update rule
+// exactly.  MODE_DET: the column cells are stored to n_kw (exclusive owner),
+// MODE_ATOMIC: the token's net change goes to the shared n_kw as float
+// atomics.
 template <int KPL, int MODE>
 __device__ __forceinline__ void sample_range_w(const WaveLdsW& S, const fvec<KPL>& al, const SweepParams& p, int lane,
                                                uint64_t b, uint64_t e, int nvalid, int lastLane) {
@@ -687,9 +690,13 @@ __device__ __forceinline__ void sample_range_w(const WaveLdsW& S, const fvec<KPL
             if constexpr (MODE == MODE_DET) {
                 if (lane == lo) colp[qo] = S.col[qo];
                 if (lane == ln) colp[qn] = S.col[qn];
-            } else {
-                if (lane == lo) atomic_sub_clamp(colp + qo, w);
-                if (lane == ln) atomicAdd(colp + qn, w);
+            } else if (zn != zo) {
+                // fire-and-forget float atomics; a removal can leave a
+                // rounding-level negative cell, which the merge's clamp
+                // max(n_kw, 0) (LDAModel.hpp:574-577) removes after the sweep.
+                // An unchanged topic adds nothing to the shared column.
+                if (lane == lo) unsafeAtomicAdd(colp + qo, -w);
+                if (lane == ln) unsafeAtomicAdd(colp + qn, w);
             }
             zkeep = (lane == i) ? zn : zkeep;
         }
@@ -787,13 +794,19 @@ __global__ void __launch_bounds__(64) sweep_det_w_kernel(SweepParams p) {
     store_kvec<KPL>(reinterpret_cast<int32_t*>(nkp), lane, __builtin_bit_cast(ivec<KPL>, mine));
 }
 
-// mergeState for weighted counts (LDAModel.hpp:565-569): n_kw = max(n_kw, 0)
-// cell by cell, then n_k = rowwise sum, accumulated in float in word order.
-__global__ void clamp_rowsum_f_kernel(float* __restrict__ nkw, int V, int Kp, float* nk) {
-    const int pos = blockIdx.x * blockDim.x + threadIdx.x;
+// mergeState for weighted counts (LDAModel.hpp:565-579): n_kw = max(n_kw, 0)
+// cell by cell, then n_k = rowwise sum in a fixed two-level float order:
+// words in blocks of kRowsumBlock, each block summed in ascending word order
+// (clamp_rowsum_f_kernel, one workgroup row per block), then the block sums
+// in ascending block order (rowsum_blocks_f_kernel).
+__global__ void clamp_rowsum_f_kernel(float* __restrict__ nkw, int V, int Kp, float* __restrict__ part) {
+    const int pos = blockIdx.y * blockDim.x + threadIdx.x;
+    const int b = blockIdx.x;
     if (pos >= Kp) return;
+    const int v0 = b * kRowsumBlock;
+    const int v1 = min(V, v0 + kRowsumBlock);
     float s = 0.0f;
-    for (int v = 0; v < V; ++v) {
+    for (int v = v0; v < v1; ++v) {
         float x = nkw[(int64_t)v * Kp + pos];
         if (x < 0.0f) {
             x = 0.0f;
@@ -801,9 +814,16 @@ __global__ void clamp_rowsum_f_kernel(float* __restrict__ nkw, int V, int Kp, fl
         }
         s = s + x;
     }
-    nk[pos] = s;
+    part[(int64_t)b * Kp + pos] = s;
 }
 
+__global__ void rowsum_blocks_f_kernel(const float* __restrict__ part, int nb, int Kp, float* __restrict__ nk) {
+    const int pos = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pos >= Kp) return;
+    float s = 0.0f;
+    for (int b = 0; b < nb; ++b) s = s + part[(int64_t)b * Kp + pos];
+    nk[pos] = s;
+}
 
 // ---------------------------------------------------------------- init ----
 // Initial topics z ~ Discrete(alpha) (LDAModel.hpp:754-762): r = u*cdf[K-1],
@@ -1714,8 +1734,14 @@ int sweep_device_w_blocks_per_cu(int kpl) {
     }
 }
 
-hipError_t launch_clamp_rowsum_f(float* nkw, int V, int Kp, float* nk, hipStream_t s) {
-    hipLaunchKernelGGL(clamp_rowsum_f_kernel, dim3((Kp + 255) / 256), dim3(256), 0, s, nkw, V, Kp, nk);
+hipError_t launch_clamp_rowsum_f(float* nkw, int V, int Kp, float* part, float* nk, hipStream_t s) {
+    const int nb = (V + kRowsumBlock - 1) / kRowsumBlock;
+    if (nb > 0) {
+        hipLaunchKernelGGL(clamp_rowsum_f_kernel, dim3(nb, (Kp + 255) / 256), dim3(256), 0, s, nkw, V, Kp, part);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(rowsum_blocks_f_kernel, dim3((Kp + 255) / 256), dim3(256), 0, s, part, nb, Kp, nk);
     return hipGetLastError();
 }
 

@@ -93,7 +93,7 @@ struct lda_ctx {
     // buffers, token weights, per-document weight sums and their total
     std::vector<float> wt;
     double weighted_n = 0.0;
-    DevBuf<float> d_wt, d_docw;
+    DevBuf<float> d_wt, d_docw, d_rowpart;
 
     DevBuf<float> d_alpha, d_alpha_cdf;
     DevBuf<int32_t> d_nkw, d_nk, d_nk_next, d_ndk, d_doclen, d_nk_priv, d_nkw_loc;

@@ -100,7 +100,10 @@ hipError_t launch_sweep_device_w(int kpl, const SweepParams& p, int grid, hipStr
 hipError_t launch_sweep_det_w(int kpl, const SweepParams& p, hipStream_t s);
 int sweep_device_w_blocks_per_cu(int kpl);
 // n_kw = max(n_kw, 0), n_k = rowwise sum of n_kw, in float (LDAModel.hpp:565-569)
-hipError_t launch_clamp_rowsum_f(float* nkw, int V, int Kp, float* nk, hipStream_t s);
+// Words per block of the weighted merge's two-level row sum (n_k from n_kw).
+constexpr int kRowsumBlock = 256;
+// part: ceil(V / kRowsumBlock) * Kp floats of scratch.
+hipError_t launch_clamp_rowsum_f(float* nkw, int V, int Kp, float* part, float* nk, hipStream_t s);
 hipError_t launch_loglik_f(const float* ndk, const float* docw, int64_t D, const float* nkw, const float* nk,
                            const float* alpha, int V, int Kp, double asum, double eta, double veta, double* partial,
                            int nblk, hipStream_t s);
