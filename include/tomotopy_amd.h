@@ -96,8 +96,12 @@ int lda_ctx_destroy(lda_ctx* ctx);
 int lda_upload_corpus(lda_ctx* ctx, const uint32_t* words, const uint64_t* doc_off, int64_t D, const uint16_t* z,
                       const float* weights);
 
-/* Replace the topic-word counts (tomoto load takes n_kw / n_k from the file). */
+/* Replace the topic-word counts (tomoto load takes n_kw / n_k from the file:
+ * TopicModel.hpp:361-395, globalState read by ModelStateLDA's serializer,
+ * LDAModel.hpp:58-69).  nkw: V x K word-major, nk: K. */
 int lda_set_counts(lda_ctx* ctx, const int32_t* nkw, const int32_t* nk);
+/* The same for term-weighted models (float counts). */
+int lda_set_counts_f(lda_ctx* ctx, const float* nkw, const float* nk);
 
 int lda_set_alpha(lda_ctx* ctx, const float* alpha);
 int lda_get_alpha(lda_ctx* ctx, float* alpha);

@@ -40,6 +40,7 @@ _SIGS = {
     "lda_ctx_destroy": [_c_p],
     "lda_upload_corpus": [_c_p, _c_p, _c_p, _i64, _c_p, _c_p],
     "lda_set_counts": [_c_p, _c_p, _c_p],
+    "lda_set_counts_f": [_c_p, _c_p, _c_p],
     "lda_set_alpha": [_c_p, _c_p],
     "lda_get_alpha": [_c_p, _c_p],
     "lda_set_optim": [_c_p, _u32, _u32],
@@ -165,6 +166,11 @@ class Context:
         nkw = np.ascontiguousarray(nkw, dtype=np.int32)
         nk = np.ascontiguousarray(nk, dtype=np.int32)
         _check(load().lda_set_counts(self._h, _ptr(nkw), _ptr(nk)))
+
+    def set_counts_f(self, nkw, nk):
+        nkw = np.ascontiguousarray(nkw, dtype=np.float32)
+        nk = np.ascontiguousarray(nk, dtype=np.float32)
+        _check(load().lda_set_counts_f(self._h, _ptr(nkw), _ptr(nk)))
 
     def set_alpha(self, alpha):
         a = np.ascontiguousarray(np.broadcast_to(np.asarray(alpha, dtype=np.float32), (self.k,)), dtype=np.float32)

@@ -105,6 +105,8 @@ def term_weights(words, doc_off, real_v, scheme):
            tf_d[v] = occurrences of v in document d; the quotients are
            float32, the logarithm is taken in double.
     """
+    if scheme not in ("idf", "pmi"):
+        raise ValueError("scheme must be 'idf' or 'pmi'")
     words = np.asarray(words, dtype=np.int64)
     doc_off = np.asarray(doc_off, dtype=np.uint64)
     D = len(doc_off) - 1
@@ -113,20 +115,13 @@ def term_weights(words, doc_off, real_v, scheme):
     doc_id = np.repeat(np.arange(D, dtype=np.int64), lens)
     valid = words < real_v
     out = np.zeros(N, dtype=np.float32)
+    vw = vocab_weights(words, doc_off, real_v, scheme)
     if N == 0:
         return out
     vw_idx = np.where(valid, words, 0)
     if scheme == "idf":
-        pairs = np.unique(doc_id[valid] * np.int64(real_v) + words[valid])
-        df = np.bincount(pairs % max(real_v, 1), minlength=real_v).astype(np.float64)
-        with np.errstate(divide="ignore"):
-            vw = np.log(np.float64(D) / df).astype(np.float32)
         out[valid] = vw[vw_idx[valid]]
         return out
-    if scheme != "pmi":
-        raise ValueError("scheme must be 'idf' or 'pmi'")
-    cf = np.bincount(words[valid], minlength=real_v).astype(np.float64)
-    vw = (cf / np.float64(N)).astype(np.float32)
     # tf: the length of the run of equal ids (documents are sorted) each token lies in
     key = doc_id * np.int64(real_v + 1) + np.where(valid, words, real_v)
     starts = np.ones(N, dtype=bool)
@@ -140,3 +135,27 @@ def term_weights(words, doc_off, real_v, scheme):
     w = np.maximum(w, np.float32(0.0))
     out[valid] = w[valid]
     return out
+
+
+def vocab_weights(words, doc_off, real_v, scheme):
+    """tomoto's vocabWeights (LDAModel.hpp:1096-1121), one float32 per used
+    word: IDF float32(log(D / df[v])) (inf for a word in no document), PMI
+    float32(cf[v] / totCf) with totCf the number of tokens, removed words
+    included.  Empty for scheme 'one'."""
+    if scheme == "one":
+        return np.zeros(0, np.float32)
+    words = np.asarray(words, dtype=np.int64)
+    doc_off = np.asarray(doc_off, dtype=np.uint64)
+    D = len(doc_off) - 1
+    N = len(words)
+    valid = words < real_v
+    if scheme == "idf":
+        doc_id = np.repeat(np.arange(D, dtype=np.int64), np.diff(doc_off.astype(np.int64)))
+        pairs = np.unique(doc_id[valid] * np.int64(real_v) + words[valid])
+        df = np.bincount(pairs % max(real_v, 1), minlength=real_v).astype(np.float64)
+        with np.errstate(divide="ignore"):
+            return np.log(np.float64(D) / df).astype(np.float32)
+    if scheme != "pmi":
+        raise ValueError("scheme must be 'one', 'idf' or 'pmi'")
+    cf = np.bincount(words[valid], minlength=real_v).astype(np.float64)
+    return (cf / np.float64(max(N, 1))).astype(np.float32)

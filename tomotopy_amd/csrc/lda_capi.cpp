@@ -412,6 +412,21 @@ int optimize_alpha(lda_ctx* c) {
     return LDA_OK;
 }
 
+// n_kw (V x K, word-major) and n_k into the lane layout; T is int32_t or
+// float, the count type of the model.
+template <typename T>
+int set_counts_t(lda_ctx* c, const T* nkw, const T* nk) {
+    HIP_TRY(hipSetDevice(c->device));
+    std::vector<T> a((size_t)c->V * c->Kp, T(0)), b(c->Kp, T(0));
+    for (int v = 0; v < c->V; ++v)
+        for (int k = 0; k < c->K; ++k) a[(size_t)v * c->Kp + c->pos(k)] = nkw[(size_t)v * c->K + k];
+    for (int k = 0; k < c->K; ++k) b[c->pos(k)] = nk[k];
+    if (!a.empty()) HIP_TRY(hipMemcpyAsync(c->d_nkw.p, a.data(), a.size() * sizeof(T), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->d_nk.p, b.data(), b.size() * sizeof(T), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return LDA_OK;
+}
+
 }  // namespace
 
 // ============================================================== C ABI ======
@@ -614,18 +629,19 @@ int lda_upload_corpus(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off
     return LDA_OK;
 }
 
+
 int lda_set_counts(lda_ctx* c, const int32_t* nkw, const int32_t* nk) {
     if (c && c->tw != LDA_TW_ONE) return fail(LDA_ERR_UNSUPPORTED, "lda_set_counts takes integer counts (TermWeight ONE)");
     if (!c || !nkw || !nk) return fail(LDA_ERR_INVALID_ARGUMENT, "NULL argument");
-    HIP_TRY(hipSetDevice(c->device));
-    std::vector<int32_t> a((size_t)c->V * c->Kp, 0), b(c->Kp, 0);
-    for (int v = 0; v < c->V; ++v)
-        for (int k = 0; k < c->K; ++k) a[(size_t)v * c->Kp + c->pos(k)] = nkw[(size_t)v * c->K + k];
-    for (int k = 0; k < c->K; ++k) b[c->pos(k)] = nk[k];
-    if (!a.empty()) HIP_TRY(hipMemcpyAsync(c->d_nkw.p, a.data(), a.size() * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(hipMemcpyAsync(c->d_nk.p, b.data(), b.size() * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    return LDA_OK;
+    return set_counts_t<int32_t>(c, nkw, nk);
+}
+
+int lda_set_counts_f(lda_ctx* c, const float* nkw, const float* nk) {
+    if (c && c->tw == LDA_TW_ONE) return fail(LDA_ERR_STATE, "integer counts: use lda_set_counts");
+    if (!c || !nkw || !nk) return fail(LDA_ERR_INVALID_ARGUMENT, "NULL argument");
+    for (size_t i = 0; i < (size_t)c->V * c->K; ++i)
+        if (!(nkw[i] >= 0.0f && nkw[i] < 1e30f)) return fail(LDA_ERR_INVALID_ARGUMENT, "n_kw must be finite and >= 0");
+    return set_counts_t<float>(c, nkw, nk);
 }
 
 int lda_set_alpha(lda_ctx* c, const float* alpha) {

@@ -18,6 +18,7 @@ import numpy as np
 
 from . import _native
 from . import corpus as _corpus
+from . import serialize as _serialize
 from .utils import Corpus, Document, InferredCorpus
 
 __version__ = "0.1.0"
@@ -120,6 +121,8 @@ transform : Callable[dict, dict]
         self._tw = tw
         self._k = k
         self._alphas = alphas
+        self._alpha0 = float(alphas[0])  # tomoto's scalar `alpha` (LDAModel.hpp:965)
+        self._vocab_weights = np.zeros(0, np.float32)
         self._eta = float(np.float32(eta))
         self._min_cf, self._min_df, self._rm_top = int(min_cf), int(min_df), int(rm_top)
         self._optim_interval = 10
@@ -218,6 +221,7 @@ transform : Callable[dict, dict]
         self._real_v = int(real_v)
         self._n_valid = int(np.count_nonzero(words < real_v))
         self._weights = None
+        self._vocab_weights = _corpus.vocab_weights(words, off, self._real_v, self._tw.name.lower())
         if self._tw != TermWeight.ONE:
             self._weights = _corpus.term_weights(words, off, self._real_v, self._tw.name.lower())
         self._ctx = _native.Context(self._k, self._real_v, self._alphas, self._eta, self._seed, tw=int(self._tw))
@@ -502,6 +506,142 @@ transform : Callable[dict, dict]
         if not self._prepared or not self._rm_top:
             return []
         return list(self._vocab[len(self._vocab) - min(self._rm_top, len(self._vocab)):])
+
+    # ------------------------------------------------------------ save/load --
+    def _plain_init_params(self):
+        def plain(v):
+            if isinstance(v, IntEnum):
+                return int(v)
+            if isinstance(v, np.generic):
+                return v.item()
+            if isinstance(v, np.ndarray):
+                return v.tolist()
+            if isinstance(v, (list, tuple)):
+                return type(v)(plain(x) for x in v)
+            if isinstance(v, dict):
+                return {k: plain(x) for k, x in v.items()}
+            return v
+        return plain(getattr(self, "init_params", {}))
+
+    def _file_state(self, full):
+        self._ensure_prepared()
+        tw = self._tw.name.lower()
+        st = {
+            "tw": tw, "vocab": list(self._vocab), "vocab_cf": np.asarray(self._cf, np.uint64),
+            "vocab_df": np.asarray(self._df, np.uint64), "real_v": self._real_v,
+            "global_step": self.global_step, "extra": pickle.dumps(self._plain_init_params()),
+            "vocab_weights": self._vocab_weights, "alpha": self._alpha0,
+            "alphas": np.asarray(self._alphas, np.float32), "eta": self._eta, "K": self._k, "eta_by_word": {},
+            "burn_in": self._burn_in, "optim_interval": self._optim_interval,
+            "nk": self._state("nk"), "nkw": self._state("nkw"), "docs": None,
+        }
+        if full:
+            z = self._state("z").copy()
+            z[self._words >= self._real_v] = 0xFFFF  # tomoto's non_topic_id for removed words
+            docs = []
+            for d in range(len(self._off) - 1):
+                b, e = int(self._off[d]), int(self._off[d + 1])
+                docs.append({"weight": 1.0, "words": self._words[b:e], "w_order": self._w_orders[d],
+                             "uid": self._uids[d] or "", "zs": z[b:e],
+                             "word_weights": self._weights[b:e] if self._weights is not None else ()})
+            st["docs"] = docs
+        return st
+
+    def save(self, filename, full=True):
+        '''Save the model to `filename` in tomoto's model file layout (TopicModel.hpp:335-359).
+If `full`, the documents and their topics are saved too (the model can be
+trained further); otherwise only the topic parameters (inference only).'''
+        _serialize.write_file(filename, self._file_state(full))
+
+    def saves(self, full=True):
+        '''The bytes `save` would write.'''
+        return _serialize.dump_model(self._file_state(full))
+
+    @classmethod
+    def load(cls, filename):
+        '''Load a model file written by `save` (or by tomotopy >= 0.6.0).'''
+        with open(filename, "rb") as f:
+            data = f.read()
+        try:
+            return cls._from_file_state(_serialize.parse_model(data))
+        except _serialize.FormatError as e:
+            raise RuntimeError("'%s' is not valid model file (%s)" % (filename, e)) from None
+
+    @classmethod
+    def loads(cls, data):
+        '''Load a model from bytes produced by `saves`.'''
+        try:
+            return cls._from_file_state(_serialize.parse_model(bytes(data)))
+        except _serialize.FormatError as e:
+            raise RuntimeError("`data` is not valid model file (%s)" % e) from None
+
+    @classmethod
+    def _from_file_state(cls, st):
+        """_loadModel + prepare(false) (TopicModel.hpp:361-395, 943-947): counts
+        n_kw and n_k come from the file, n_dk is rebuilt from the topics."""
+        params = _serialize.load_extra(st["extra"])
+        seed = params.get("seed")
+        K, V = int(st["K"]), int(st["real_v"])
+        mdl = cls(tw=st["tw"], k=K, alpha=[float(a) for a in st["alphas"]] if K > 1 else float(st["alphas"][0]),
+                  eta=float(st["eta"]), seed=seed if isinstance(seed, int) else None)
+        mdl.init_params = params
+        mdl._min_cf = int(params.get("min_cf", 0) or 0)
+        mdl._min_df = int(params.get("min_df", 0) or 0)
+        mdl._rm_top = int(params.get("rm_top", 0) or 0)
+        mdl._alpha0 = float(st["alpha"])
+        mdl._vocab = list(st["vocab"])
+        mdl._dict = {w: i for i, w in enumerate(mdl._vocab)}
+        n_words = len(mdl._vocab)
+        mdl._cf = np.zeros(n_words, np.int64)
+        mdl._df = np.zeros(n_words, np.int64)
+        mdl._cf[:len(st["vocab_cf"])] = np.asarray(st["vocab_cf"][:n_words], np.int64)
+        mdl._df[:len(st["vocab_df"])] = np.asarray(st["vocab_df"][:n_words], np.int64)
+        mdl._real_v = V
+        mdl._order = np.arange(n_words)
+        mdl._vocab_weights = np.asarray(st["vocab_weights"], np.float32)
+        mdl._global_step = int(st["global_step"])
+        mdl._burn_in = int(st["burn_in"])
+        mdl._optim_interval = int(st["optim_interval"])
+        docs = st["docs"] or []
+        lens = [len(d["words"]) for d in docs]
+        off = np.zeros(len(docs) + 1, np.uint64)
+        off[1:] = np.cumsum(lens)
+        cat = lambda key, dt: (np.concatenate([np.asarray(d[key], dt) for d in docs]) if docs else np.zeros(0, dt))
+        words = cat("words", np.uint32)
+        z = cat("zs", np.uint16)
+        mdl._words, mdl._off = words, off
+        mdl._w_orders = []
+        mdl._raw_docs = []
+        mdl._uids = []
+        mdl._uid_map = {}
+        for i, d in enumerate(docs):
+            w = np.asarray(d["words"], np.int64)
+            wo = np.asarray(d["w_order"], np.int64)
+            if len(wo) != len(w):
+                wo = np.arange(len(w))
+            mdl._w_orders.append(wo)
+            mdl._raw_docs.append(w[wo])
+            mdl._uids.append(d["uid"])
+            if d["uid"]:
+                mdl._uid_map[d["uid"]] = i
+        mdl._n_valid = int(np.count_nonzero(words < V))
+        mdl._weights = None
+        if mdl._tw != TermWeight.ONE:
+            mdl._weights = cat("word_weights", np.float32) if docs else np.zeros(0, np.float32)
+            if len(mdl._weights) != len(words):
+                raise _serialize.FormatError("term-weighted documents without word weights")
+        z = np.where(words < V, z, 0).astype(np.uint16)
+        mdl._ctx = _native.Context(K, V, mdl._alphas, mdl._eta, mdl._seed, tw=int(mdl._tw))
+        mdl._ctx.upload_corpus(words, off, z, weights=mdl._weights)
+        if mdl._tw == TermWeight.ONE:
+            mdl._ctx.set_counts(st["nkw"], st["nk"])
+        else:
+            mdl._ctx.set_counts_f(st["nkw"], st["nk"])
+        mdl._ctx.set_optim(mdl._optim_interval, mdl._burn_in)
+        mdl._ctx.global_step = mdl._global_step
+        mdl._prepared = True
+        mdl._cache = {}
+        return mdl
 
     # --------------------------------------------------------------- misc ----
     def copy(self):
