@@ -30,6 +30,33 @@ METRIC = "Gibbs tokens/s at K=1000 (HBM %peak); batched infer docs/s + p99 vs pl
 HBM_PEAK_GBS = 8000.0
 
 
+def copy_probe():
+    """Calibration kernel for the byte counters: a 1 GiB device-to-device copy,
+    three times (read 1 GiB, write 1 GiB per launch)."""
+    import torch
+    n = 1 << 28
+    a = torch.ones(n, dtype=torch.int32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    return {"copy_bytes_per_launch": n * 4, "launches": 3}
+
+
+def pmc_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 byte
+    counter summary (FETCH_SIZE and WRITE_SIZE of the same bench command,
+    each scaled by what the counter reads for a copy of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic_large_k1000.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        return t["kernels"][kernel]["hbm_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -108,10 +135,49 @@ def pmi_bench(ws, offs, V, K, alpha, eta, steps, warmup):
     out = {"value": n_valid * steps / dt, "unit": "tokens/s", "ms_per_step": dt * 1e3 / steps, "steps": steps,
            "kernel": "sweep_device_w_kernel<16>", "kernel_ms": kern_ms, "achieved_GBs": achieved,
            "frac": achieved / HBM_PEAK_GBS, "algorithmic_bytes_per_launch": bytes_per_launch,
+           "traffic": pmc_traffic("sweep_device_w_kernel<16>"),
            "ll_per_word": (lld + llr) / wn, "weighted_n": wn}
     log("PMI: %.3g tokens/s, kernel %.3f ms (%.1f%% of HBM peak), %.3f ms/step, ll_per_word %.5f"
         % (out["value"], kern_ms, 100 * out["frac"], out["ms_per_step"], out["ll_per_word"]))
     return out
+
+
+def cpu_infer_baseline(ctx, ws, offs, V, K, alpha, eta, uws, uoffs, iterations, budget_s=8.0, seed=3):
+    """The oracle's fold-in (tomoto's per-document infer, together=False) on
+    the host's cores: documents drawn at random from the workload, split over
+    threads (the oracle releases the GIL), until the time budget is spent."""
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle import oracle as O
+    threads = min(os.cpu_count() or 1, 16)
+    z, _, _, _ = ctx.download_state(ndk=False, nkw=False, nk=False)
+    m = O.Model(K, V, ctx.get_alpha(), eta, seed=1, scan_order=0)
+    m.set_corpus(ws, offs, z)
+    D = len(uoffs) - 1
+    order = np.random.default_rng(seed).permutation(D)
+    lens = np.diff(uoffs.astype(np.int64))
+
+    def run(docs):
+        sub_off = np.zeros(len(docs) + 1, np.uint64)
+        sub_off[1:] = np.cumsum(lens[docs])
+        sub_w = np.concatenate([uws[int(uoffs[d]):int(uoffs[d + 1])] for d in docs]).astype(np.uint32)
+        m.infer(sub_w, sub_off, iterations, mode=1)
+        return len(docs), int(np.count_nonzero(sub_w < V))
+
+    done = toks = 0
+    pos = 0
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(threads) as ex:
+        while time.perf_counter() - t0 < budget_s and pos < D:
+            batch = [b for b in (order[pos + j:pos + j + 4] for j in range(0, threads * 16, 4)) if len(b)]
+            pos += threads * 16
+            for n, t in ex.map(run, batch):
+                done += n
+                toks += t
+    dt = time.perf_counter() - t0
+    return {"value": done / dt, "unit": "docs/s", "cores": threads, "kind": "port",
+            "sample": "%d random docs (%d tokens) of the workload, %d iterations each, oracle fold-in "
+                      "(per-document private state as a delta over the frozen model; tomoto additionally copies "
+                      "the K x V state per document), %d threads, %.1fs" % (done, toks, iterations, threads, dt)}
 
 
 def infer_bench(plain_sample=300, iterations=100):
@@ -168,6 +234,8 @@ def infer_bench(plain_sample=300, iterations=100):
         "ll_per_word_rel_diff": abs(llw - llw_seq) / abs(llw_seq),
         "plain_equals_batched": same, "longest_doc": int(lens.max()),
     }
+    out["cpu_baseline"] = cpu_infer_baseline(ctx, ws, offs, V, K, 0.1, 0.01, uws, uoffs, iterations)
+    log("fold-in CPU baseline: %.0f docs/s (%s)" % (out["cpu_baseline"]["value"], out["cpu_baseline"]["sample"]))
     log("fold-in: batched %.0f docs/s (%.3fs), sequential %.0f docs/s, plain %.1f docs/s on %d sampled docs, "
         "identical=%s, ll_per_word %.5f vs sequential %.5f"
         % (out["batched_docs_per_s"], batched_s, out["sequential_docs_per_s"], out["plain_docs_per_s"], len(pick),
@@ -179,7 +247,7 @@ def _percentile(x, q):
     return float(np.percentile(np.asarray(x, dtype=np.float64), q)) if len(x) else None
 
 
-def serve_bench(iterations=50, qps_list=(25, 50, 100, 200, 400, 800), seconds=4.0, plain_requests=60, seed=2024,
+def serve_bench(iterations=50, qps_list=(25, 50, 100, 200, 400, 800, 1200, 1600), seconds=4.0, plain_requests=60, seed=2024,
                 seg_tokens=128, step_tokens=12000, lanes=1):
     """Serving loop on a device-resident K=500, V=100,000 model: Poisson request
     arrivals at each QPS (requests of 1..64 docs, lengths LogNormal(ln 150, 1)
@@ -289,6 +357,11 @@ def serve_bench(iterations=50, qps_list=(25, 50, 100, 200, 400, 800), seconds=4.
     dt = time.perf_counter() - t0
     nd2 = sum(len(o) - 1 for _, o in sub2)
     out["plain_per_document"] = {"requests": len(sub2), "docs": nd2, "docs_per_s": nd2 / dt}
+    allw = np.concatenate([w for w, _ in reqs]).astype(np.uint32)
+    alloff = np.zeros(sum(len(o) - 1 for _, o in reqs) + 1, np.uint64)
+    alloff[1:] = np.cumsum(np.concatenate([np.diff(o.astype(np.int64)) for _, o in reqs]))
+    out["cpu_baseline"] = cpu_infer_baseline(ctx, ws, offs, V, K, 0.1, 0.01, allw, alloff, iterations)
+    log("serving CPU baseline: %.0f docs/s (%s)" % (out["cpu_baseline"]["value"], out["cpu_baseline"]["sample"]))
     best = max(lv["docs_per_s"] for lv in out["levels"])
     ok = [lv for lv in out["levels"] if lv["p99_ms"] is not None and lv["p99_ms"] <= 50.0]
     out["best_docs_per_s"] = max(best, out["saturation"]["docs_per_s"])
@@ -311,13 +384,16 @@ def main():
     ap.add_argument("--no-infer", action="store_true", help="skip the fold-in measurement")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
     ap.add_argument("--no-pmi", action="store_true", help="skip the TermWeight.PMI training variant")
-    ap.add_argument("--only", choices=["train", "infer", "serve"], default=None,
+    ap.add_argument("--only", choices=["train", "infer", "serve", "copy"], default=None,
                     help="run one leg only (the JSON line then carries that leg's numbers)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
     rank = int(os.environ.get("RANK", "0"))
+    if args.only == "copy":
+        print(json.dumps({"leg": "copy", "copy": copy_probe()}), flush=True)
+        return
     if args.only in ("infer", "serve"):
         res = infer_bench() if args.only == "infer" else serve_bench()
         print(json.dumps({"metric": METRIC, "leg": args.only, args.only: res}), flush=True)
@@ -366,7 +442,7 @@ def main():
         "config": {"workload": "large_k1000", "docs": D, "tokens": n_valid, "V": V, "K": K, "alpha": alpha,
                    "eta": eta, "scheme": "device-wide (workers=0)", "optim_interval": 10},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": pmc_traffic("sweep_device_kernel<16>"),
                      "kernel": "sweep_device_kernel<16>", "kernel_ms": kern_ms,
                      "algorithmic_bytes_per_launch": bytes_per_launch},
         "ll_per_word": (lld + llr) / n_valid,

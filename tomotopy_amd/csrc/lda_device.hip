@@ -1467,6 +1467,8 @@ __global__ void __launch_bounds__(64) infer_finalize_kernel(const InferParams* _
     const InferParams p = *pp;
     const int lane = lane_id();
     const WaveLds S = wave_lds<KPL>();
+    int* const nd_h = S.col;                          // n_dk of the document
+    int* const run_h = reinterpret_cast<int*>(S.A);   // topic counts of the open word run
     fvec<KPL> al;
     load_kvec_f<KPL>(p.alpha, lane, al);
     ivec<KPL> nk0;
@@ -1475,24 +1477,23 @@ __global__ void __launch_bounds__(64) infer_finalize_kernel(const InferParams* _
     const int64_t first = p.counters ? (int64_t)blockIdx.x : p.job_base + blockIdx.x;
     const int64_t last = p.counters ? p.njobs : (first + 1 < p.njobs ? first + 1 : p.njobs);
     const int64_t stride = p.counters ? (int64_t)gridDim.x : 1;
+    const double eta = (double)p.eta;
+    const double veta = (double)p.V * eta;
     for (int64_t idx = first; idx < last; idx += stride) {
         const InferJob J = p.jobs[idx];
         if (J.it_end < J.iters) continue;
         const uint64_t b = J.b, e = J.e;
         ivec<KPL> zero = 0;
-        lds_store<KPL>(S.col, lane, zero);
+        lds_store<KPL>(nd_h, lane, zero);
+        lds_store<KPL>(run_h, lane, zero);
+        // n_dk: integer LDS atomics, order-free
         for (uint64_t t0 = b; t0 < e; t0 += 64) {
             const uint64_t g = t0 + (uint64_t)lane;
             const int zz = g < e ? (int)p.z[g] : 0xFFFF;
-            const int n = (int)((e - t0) < 64 ? (e - t0) : 64);
-            for (int i = 0; i < n; ++i) {
-                const int k = readlane_i(zz, i);
-                if (k == 0xFFFF) continue;
-                if (lane == (k >> SH)) S.col[kpos<KPL>(lane, k & (KPL - 1))] += 1;
-            }
+            if (zz != 0xFFFF) atomicAdd(nd_h + kpos<KPL>(zz >> SH, zz & (KPL - 1)), 1);
         }
         ivec<KPL> nd;
-        lds_load<KPL>(S.col, lane, nd);
+        lds_load<KPL>(nd_h, lane, nd);
         int L = 0;
 #pragma unroll
         for (int j = 0; j < KPL; ++j) L += nd[j];
@@ -1503,9 +1504,9 @@ __global__ void __launch_bounds__(64) infer_finalize_kernel(const InferParams* _
             const int k = lane * KPL + j;
             if (k < p.K) p.theta[(int64_t)J.slot * p.K + k] = ((float)nd[j] + al[j]) / denom;
         }
-        const double eta = (double)p.eta;
-        const double veta = (double)p.V * eta;
-        const double lge = lgamma(eta);
+        // getLLRest(private) - getLLRest(global) + getLLDocs(doc) over the
+        // touched terms (LDAModel.hpp:616-666, 931-945): the n_k and n_dk
+        // terms per topic,
         double ll = 0.0;
 #pragma unroll
         for (int j = 0; j < KPL; ++j) {
@@ -1515,28 +1516,36 @@ __global__ void __launch_bounds__(64) infer_finalize_kernel(const InferParams* _
                 ll += lgamma((double)nd[j] + a) - lgamma(a);
             }
         }
+        // and the n_kw terms per token: a cell (v, k) holding c of the
+        // document's tokens adds lgamma(n_kw + c + eta) - lgamma(n_kw + eta)
+        // = sum_{r<c} log(n_kw + eta + r), so token t adds log(n_kw[v,k] +
+        // eta + r_t) with r_t its rank among the document's tokens of that
+        // cell.  Tokens are sorted by word, so equal cells lie in one word run;
+        // r_t = (equal keys on lower lanes of the chunk) + (count of the cell
+        // in earlier chunks of the same run, kept in run_h).
+        int run_v = -1;
         for (uint64_t t0 = b; t0 < e; t0 += 64) {
+            const int n = (int)((e - t0) < 64 ? (e - t0) : 64);
             const uint64_t g = t0 + (uint64_t)lane;
-            const int wv = g < e ? (int)p.words[g] : -1;
-            const int wp = (g < e && g > b) ? (int)p.words[g - 1] : -2;
-            unsigned long long starts = __ballot(g < e && (unsigned)wv < (unsigned)p.V && wv != wp);
-            while (starts) {
-                const int i = __builtin_ctzll(starts);
-                starts &= starts - 1;
-                const int v = readlane_i(wv, i);
-                ivec<KPL> base, c;
-                load_kvec<KPL>(p.nkw + (int64_t)v * p.Kp, lane, base);
-                lds_store<KPL>(S.col, lane, base);
-                add_run_counts<KPL>(S, p, lane, t0 + (uint64_t)i, e, v);
-                lds_load<KPL>(S.col, lane, c);
-#pragma unroll
-                for (int j = 0; j < KPL; ++j) {
-                    if (c[j] != base[j]) {
-                        ll += lgamma((double)c[j] + eta) - lge;
-                        if (base[j]) ll -= lgamma((double)base[j] + eta) - lge;
-                    }
-                }
+            const int v = lane < n ? (int)p.words[g] : -1;
+            const bool in = lane < n && (unsigned)v < (unsigned)p.V;
+            const int zz = in ? (int)p.z[g] : 0;
+            int r = 0;
+            for (int i = 0; i < n; ++i) {
+                const int vi = readlane_i(v, i), zi = readlane_i(zz, i);
+                r += (i < lane && vi == v && zi == zz) ? 1 : 0;
             }
+            const int q = kpos<KPL>(zz >> SH, zz & (KPL - 1));
+            if (in && v == run_v) r += run_h[q];
+            if (in) ll += log((double)p.nkw[(int64_t)v * p.Kp + q] + eta + (double)r);
+            // the run still open at the end of the chunk (ids >= V sort last)
+            const int vlast = readlane_i(v, n - 1);
+            const int open = (unsigned)vlast < (unsigned)p.V ? vlast : -1;
+            if (open != run_v) {
+                lds_store<KPL>(run_h, lane, zero);
+                run_v = open;
+            }
+            if (in && v == run_v) atomicAdd(run_h + q, 1);
         }
         ll = wave_sum_d(ll);
         if (lane == 0) {
