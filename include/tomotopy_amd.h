@@ -103,6 +103,17 @@ int lda_set_counts(lda_ctx* ctx, const int32_t* nkw, const int32_t* nk);
 /* The same for term-weighted models (float counts). */
 int lda_set_counts_f(lda_ctx* ctx, const float* nkw, const float* nk);
 
+/* Word priors (LDAModel::setWordPrior / prepareWordPriors, LDAModel.hpp:724-737,
+ * 1031-1045): word_ids[i] (ids >= V are skipped, as tomoto skips removed
+ * words) gets the topic prior priors[i*K .. i*K+K) (>= 0) in place of eta;
+ * every other word keeps eta.  The weights then use eta_kv and the per-topic
+ * sum of eta_kv (EtaHelper<_, true>, LDAModel.hpp:99-117), the initial topics
+ * are drawn from the word's eta column (LDAModel.hpp:769-773) and the
+ * log-likelihood uses eta_kv (LDAModel.hpp:638-650).  Call before
+ * lda_upload_corpus; n == 0 clears the priors.  Fold-in with priors returns
+ * LDA_ERR_UNSUPPORTED. */
+int lda_set_word_priors(lda_ctx* ctx, int64_t n, const uint32_t* word_ids, const float* priors);
+
 int lda_set_alpha(lda_ctx* ctx, const float* alpha);
 int lda_get_alpha(lda_ctx* ctx, float* alpha);
 int lda_set_optim(lda_ctx* ctx, uint32_t optim_interval, uint32_t burn_in);

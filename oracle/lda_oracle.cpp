@@ -159,6 +159,18 @@ inline int search_cdf(const float* cdf, int K, float u) {
     return K - 1;
 }
 
+// sampleFromDiscrete (sample.hpp:121-131), the initial draw with word priors
+// (LDAModel.hpp:769-773): r = u * (sequential float sum of w), then subtract
+// w[z] while r > w[z] and z < K-1.
+int sample_from_discrete(const float* w, int K, float u) {
+    float tot = 0.0f;
+    for (int k = 0; k < K; ++k) tot += w[k];
+    float r = u * tot;
+    int z = 0;
+    while (r > w[z] && z < K - 1) { r -= w[z]; ++z; }
+    return z;
+}
+
 // kpl used by the device for K topics: the smallest power of two with 64*kpl >= K.
 inline int device_kpl(int K) {
     int kpl = 1;
@@ -168,10 +180,14 @@ inline int device_kpl(int K) {
 
 // Weight vector + scan + search, the full per-token step of a1-a3.  When
 // scan_order == 1 the device order is used (weights padded to 64*kpl with 0).
+// eta_col / vetas: the word-prior variant (EtaHelper<_, true>,
+// LDAModel.hpp:99-117): eta_kv of this word and sum_v eta_kv per topic.
 int sample_one(int K, int scan_order, int kpl, const int32_t* ndk, const float* alpha,
-               const int32_t* nkw_col, float eta, const int32_t* nk, float veta, float u, float* buf) {
+               const int32_t* nkw_col, float eta, const int32_t* nk, float veta, float u, float* buf,
+               const float* eta_col = nullptr, const float* vetas = nullptr) {
     int n = scan_order ? 64 * kpl : K;
-    for (int k = 0; k < K; ++k) buf[k] = topic_weight(ndk[k], alpha[k], nkw_col[k], eta, nk[k], veta);
+    for (int k = 0; k < K; ++k)
+        buf[k] = topic_weight(ndk[k], alpha[k], nkw_col[k], eta_col ? eta_col[k] : eta, nk[k], vetas ? vetas[k] : veta);
     for (int k = K; k < n; ++k) buf[k] = 0.0f;
     if (scan_order) prefix_sum_device(buf, kpl);
     else prefix_sum_tomoto(buf, K);
@@ -284,6 +300,13 @@ struct Model {
     std::vector<int32_t> ndk;  // D x K
     std::vector<int32_t> nkw;  // V x K (word-major, tomoto's column-major K x V)
     std::vector<int32_t> nk;   // K
+    // word priors (set_word_prior): eta_kv (V x K) and its per-topic sums, in
+    // float (sequential over words, the sampler's) and double (the FP64 LL's);
+    // empty without priors
+    std::vector<float> eta_kv, etasum;
+    std::vector<double> etasum_d;
+    const float* ecol(uint32_t v) const { return eta_kv.empty() ? nullptr : &eta_kv[(size_t)v * K]; }
+    const float* esum() const { return etasum.empty() ? nullptr : etasum.data(); }
     // term-weighted state (TermWeight IDF / PMI): token weights, float counts,
     // per-document weight sums
     bool weighted = false;
@@ -308,7 +331,7 @@ void sample_range(Model& m, int64_t d, uint64_t b, uint64_t e, int32_t* nk_local
         add_token(m, d, v, zo, -1, nk_local);
         float u = token_uniform(m.seed, g, sweep, TAG_SWEEP);
         int zn = sample_one(m.K, m.scan_order, m.kpl, &m.ndk[(size_t)d * m.K], m.alpha.data(),
-                            &m.nkw[(size_t)v * m.K], m.eta, nk_local, m.veta, u, buf);
+                            &m.nkw[(size_t)v * m.K], m.eta, nk_local, m.veta, u, buf, m.ecol(v), m.esum());
         m.z[g] = (uint16_t)zn;
         add_token(m, d, v, zn, +1, nk_local);
     }
@@ -418,7 +441,8 @@ void sweep_copy_merge(Model& m, int W, int threads) {
                     int zo = m.z[g];
                     ndk[zo]--; col[zo]--; nk[zo]--;
                     float u = token_uniform(m.seed, g, sweep, TAG_SWEEP);
-                    int zn = sample_one(m.K, m.scan_order, m.kpl, ndk, m.alpha.data(), col, m.eta, nk, m.veta, u, buf.data());
+                    int zn = sample_one(m.K, m.scan_order, m.kpl, ndk, m.alpha.data(), col, m.eta, nk, m.veta, u, buf.data(),
+                                        m.ecol(v), m.esum());
                     m.z[g] = (uint16_t)zn;
                     ndk[zn]++; col[zn]++; nk[zn]++;
                 }
@@ -491,8 +515,27 @@ void loglik(const Model& m, int mode, double* ll_docs, double* ll_rest) {
     }
     double eta = (double)m.eta;
     double veta = mode ? (double)m.veta : (double)m.V * eta;
-    double rest = lg(veta) * m.K;
+    double rest = 0.0;
     double lge = lg(eta);
+    if (!m.eta_kv.empty()) {
+        // word priors (LDAModel.hpp:638-650): per-topic eta sums and eta_kv
+        for (int k = 0; k < m.K; ++k) {
+            double es = mode ? (double)m.etasum[k] : m.etasum_d[k];
+            double x = mode ? (double)((float)m.nk[k] + m.etasum[k]) : (double)m.nk[k] + es;
+            rest += lg(es) - lg(x);
+            for (int v = 0; v < m.V; ++v) {
+                int32_t c = m.nkw[(size_t)v * m.K + k];
+                if (!c) continue;
+                float e = m.eta_kv[(size_t)v * m.K + k];
+                double y = mode ? (double)((float)c + e) : (double)c + (double)e;
+                rest += lg(y) - lg((double)e);
+            }
+        }
+        *ll_docs = docs;
+        *ll_rest = rest;
+        return;
+    }
+    rest = lg(veta) * m.K;
     for (int k = 0; k < m.K; ++k) {
         double x = mode ? (double)((float)m.nk[k] + m.veta) : (double)m.nk[k] + veta;
         rest -= lg(x);
@@ -527,11 +570,14 @@ inline void add_token_w(Model& m, int64_t d, uint32_t v, int k, float w, bool de
 
 // getZLikelihoods (LDAModel.hpp:273-285) over float counts, then the same
 // prefix sum and search as the integer sampler.
-int sample_one_w(const Model& m, const float* ndk, const float* col, const float* nk, float u, float* buf) {
+int sample_one_w(const Model& m, const float* ndk, const float* col, const float* nk, float u, float* buf,
+                 const float* ecol) {
     int n = m.scan_order ? 64 * m.kpl : m.K;
+    const float* ec = ecol;
+    const float* es = m.esum();
     for (int k = 0; k < m.K; ++k) {
-        float a = (ndk[k] + m.alpha[k]) * (col[k] + m.eta);
-        buf[k] = a / (nk[k] + m.veta);
+        float a = (ndk[k] + m.alpha[k]) * (col[k] + (ec ? ec[k] : m.eta));
+        buf[k] = a / (nk[k] + (es ? es[k] : m.veta));
     }
     for (int k = m.K; k < n; ++k) buf[k] = 0.0f;
     if (m.scan_order) prefix_sum_device(buf, m.kpl);
@@ -547,7 +593,7 @@ void sample_range_w(Model& m, int64_t d, uint64_t b, uint64_t e, float* nk_local
         int zo = m.z[g];
         add_token_w(m, d, v, zo, w, true, nk_local);
         float u = token_uniform(m.seed, g, sweep, TAG_SWEEP);
-        int zn = sample_one_w(m, &m.ndk_f[(size_t)d * m.K], &m.nkw_f[(size_t)v * m.K], nk_local, u, buf);
+        int zn = sample_one_w(m, &m.ndk_f[(size_t)d * m.K], &m.nkw_f[(size_t)v * m.K], nk_local, u, buf, m.ecol(v));
         m.z[g] = (uint16_t)zn;
         add_token_w(m, d, v, zn, w, false, nk_local);
     }
@@ -649,13 +695,18 @@ void loglik_w(const Model& m, double* ll_docs, double* ll_rest) {
         }
     }
     double eta = (double)m.eta, veta = (double)m.V * eta;
-    double rest = std::lgamma(veta) * m.K, lge = std::lgamma(eta);
+    double rest = 0.0, lge = std::lgamma(eta);
     for (int k = 0; k < m.K; ++k) {
-        rest -= std::lgamma((double)m.nk_f[k] + veta);
+        double es = m.eta_kv.empty() ? veta : m.etasum_d[k];
+        rest += std::lgamma(es) - std::lgamma((double)m.nk_f[k] + es);
         for (int v = 0; v < m.V; ++v) {
             float c = m.nkw_f[(size_t)v * m.K + k];
             if (c == 0.0f) continue;
-            rest += std::lgamma((double)c + eta) - lge;
+            if (m.eta_kv.empty()) rest += std::lgamma((double)c + eta) - lge;
+            else {
+                double e = (double)m.eta_kv[(size_t)v * m.K + k];
+                rest += std::lgamma((double)c + e) - std::lgamma(e);
+            }
         }
     }
     *ll_docs = docs;
@@ -784,12 +835,31 @@ void orc_set_corpus(void* h, const uint32_t* words, const uint64_t* off, int64_t
         for (uint64_t g = off[d]; g < off[d + 1]; ++g) {
             uint32_t v = words[g];
             if (v >= (uint32_t)m->V) continue;
-            int k = z ? z[g] : search_cdf(ca.data(), m->K, token_uniform(m->seed, g, 0, TAG_INIT));
+            int k = z ? z[g]
+                : m->ecol(v) ? sample_from_discrete(m->ecol(v), m->K, token_uniform(m->seed, g, 0, TAG_INIT))
+                             : search_cdf(ca.data(), m->K, token_uniform(m->seed, g, 0, TAG_INIT));
             m->z[g] = (uint16_t)k;
             add_token(*m, d, v, k, +1, m->nk.data());
         }
     }
 }
+
+// Word priors: eta_kv (V x K, natural order); tomoto builds it as eta
+// everywhere, replaced by the given columns (prepareWordPriors,
+// LDAModel.hpp:724-737).  Call before orc_set_corpus (the initial draw uses it).
+void orc_set_priors(void* h, const float* eta_kv) {
+    Model* m = (Model*)h;
+    m->eta_kv.assign(eta_kv, eta_kv + (size_t)m->V * m->K);
+    m->etasum.assign(m->K, 0.0f);
+    m->etasum_d.assign(m->K, 0.0);
+    for (int v = 0; v < m->V; ++v)
+        for (int k = 0; k < m->K; ++k) {
+            m->etasum[k] = m->etasum[k] + eta_kv[(size_t)v * m->K + k];
+            m->etasum_d[k] += (double)eta_kv[(size_t)v * m->K + k];
+        }
+}
+
+int orc_sample_from_discrete(const float* w, int K, float u) { return sample_from_discrete(w, K, u); }
 
 void orc_set_alpha(void* h, const float* alpha) { Model* m = (Model*)h; m->alpha.assign(alpha, alpha + m->K); }
 void orc_get_alpha(void* h, float* alpha) { Model* m = (Model*)h; std::copy(m->alpha.begin(), m->alpha.end(), alpha); }

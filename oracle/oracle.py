@@ -66,6 +66,8 @@ def load():
         "orc_seg_bounds": (i32, [_c_p, u64, u32, i32, _c_p]),
         "orc_infer_ll_bruteforce": (f64, [_c_p, _c_p, u64, _c_p]),
         "orc_set_corpus_w": (None, [_c_p, _c_p, _c_p, i64, _c_p, _c_p]),
+        "orc_set_priors": (None, [_c_p, _c_p]),
+        "orc_sample_from_discrete": (i32, [_c_p, i32, f32]),
         "orc_train_w": (None, [_c_p, i32, i32, i32, i32]),
         "orc_optimize_alpha_w": (None, [_c_p]),
         "orc_merge_w": (None, [_c_p]),
@@ -129,6 +131,12 @@ def prefix_sum_device(x, kpl):
 
 def search(cdf, K, u):
     return load().orc_search(_p(np.ascontiguousarray(cdf, np.float32)), K, float(u))
+
+
+def sample_from_discrete(w, u):
+    """tomoto's sampleFromDiscrete (subtractive scan) of one uniform."""
+    w = np.ascontiguousarray(w, np.float32)
+    return load().orc_sample_from_discrete(_p(w), len(w), float(u))
 
 
 def shuffled_order(N, seed):
@@ -199,6 +207,11 @@ class Model:
         self.D = len(off) - 1
         self.N = int(off[-1])
         load().orc_set_corpus(self._h, _p(self._words), _p(self._off), self.D, _p(zz))
+
+    def set_priors(self, eta_kv):
+        """Word priors: eta_kv (V x K); call before set_corpus."""
+        self._eta_kv = np.ascontiguousarray(eta_kv, np.float32).reshape(self.V, self.K)
+        load().orc_set_priors(self._h, _p(self._eta_kv))
 
     def set_alpha(self, alpha):
         a = np.ascontiguousarray(np.broadcast_to(np.asarray(alpha, np.float32), (self.K,)), dtype=np.float32)

@@ -41,6 +41,7 @@ _SIGS = {
     "lda_upload_corpus": [_c_p, _c_p, _c_p, _i64, _c_p, _c_p],
     "lda_set_counts": [_c_p, _c_p, _c_p],
     "lda_set_counts_f": [_c_p, _c_p, _c_p],
+    "lda_set_word_priors": [_c_p, ctypes.c_int64, _c_p, _c_p],
     "lda_set_alpha": [_c_p, _c_p],
     "lda_get_alpha": [_c_p, _c_p],
     "lda_set_optim": [_c_p, _u32, _u32],
@@ -166,6 +167,12 @@ class Context:
         nkw = np.ascontiguousarray(nkw, dtype=np.int32)
         nk = np.ascontiguousarray(nk, dtype=np.int32)
         _check(load().lda_set_counts(self._h, _ptr(nkw), _ptr(nk)))
+
+    def set_word_priors(self, word_ids, priors):
+        """priors: len(word_ids) x K; call before upload_corpus."""
+        word_ids = np.ascontiguousarray(word_ids, dtype=np.uint32)
+        priors = np.ascontiguousarray(priors, dtype=np.float32).reshape(len(word_ids), self.k)
+        _check(load().lda_set_word_priors(self._h, len(word_ids), _ptr(word_ids), _ptr(priors)))
 
     def set_counts_f(self, nkw, nk):
         nkw = np.ascontiguousarray(nkw, dtype=np.float32)

@@ -220,6 +220,11 @@ lda::SweepParams base_params(lda_ctx* c) {
     p.seed_hi = (uint32_t)(c->seed >> 32);
     p.sweep = (uint32_t)c->global_step;
     p.D = c->D;
+    if (c->has_priors) {
+        p.prior_row = c->d_prior_row.p;
+        p.prior_eta = c->d_prior_eta.p;
+        p.etasum = c->d_etasum.p;
+    }
     return p;
 }
 
@@ -271,7 +276,7 @@ int sweep_once(lda_ctx* c, int W, bool copy_merge) {
             p.nk_out = c->d_nk.p;
             p.doc_order = c->d_order.p;
             p.counter = c->d_counter.p;
-            const int grid = c->num_cu * lda::sweep_device_w_blocks_per_cu(c->kpl);
+            const int grid = c->num_cu * lda::sweep_device_w_blocks_per_cu(c->kpl, c->has_priors);
             HIP_TRY(hipEventRecord(c->ev0, s));
             HIP_TRY(lda::launch_sweep_device_w(c->kpl, p, grid, s));
             HIP_TRY(hipEventRecord(c->ev1, s));
@@ -333,7 +338,7 @@ int sweep_once(lda_ctx* c, int W, bool copy_merge) {
         p.nk_out = c->d_nk_next.p;
         p.doc_order = c->d_order.p;
         p.counter = c->d_counter.p;
-        int grid = c->num_cu * lda::sweep_device_blocks_per_cu(c->kpl);
+        int grid = c->num_cu * lda::sweep_device_blocks_per_cu(c->kpl, c->has_priors);
         HIP_TRY(hipEventRecord(c->ev0, s));
         HIP_TRY(lda::launch_sweep_device(c->kpl, p, grid, s));
         HIP_TRY(hipEventRecord(c->ev1, s));
@@ -568,8 +573,10 @@ int lda_upload_corpus(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off
         if (N) HIP_TRY(hipMemcpyAsync(c->d_z.p, z, N * sizeof(uint16_t), hipMemcpyHostToDevice, s));
     } else {
         // initial topics (LDAModel.hpp:754-762) from the Philox stream
+        // (with word priors: from the word's eta column, LDAModel.hpp:769-773)
         HIP_TRY(lda::launch_init_topics(c->d_words.p, c->d_z.p, N, c->d_alpha_cdf.p, c->K, c->V, (uint32_t)c->seed,
-                                        (uint32_t)(c->seed >> 32), s));
+                                        (uint32_t)(c->seed >> 32), c->has_priors ? c->d_prior_row.p : nullptr,
+                                        c->d_prior_eta.p, c->kpl, c->Kp, c->eta, s));
     }
     if (!weighted) {
         HIP_TRY(hipMemsetAsync(c->d_ndk.p, 0, c->d_ndk.n * sizeof(int32_t), s));
@@ -642,6 +649,60 @@ int lda_set_counts_f(lda_ctx* c, const float* nkw, const float* nk) {
     for (size_t i = 0; i < (size_t)c->V * c->K; ++i)
         if (!(nkw[i] >= 0.0f && nkw[i] < 1e30f)) return fail(LDA_ERR_INVALID_ARGUMENT, "n_kw must be finite and >= 0");
     return set_counts_t<float>(c, nkw, nk);
+}
+
+int lda_set_word_priors(lda_ctx* c, int64_t n, const uint32_t* word_ids, const float* priors) {
+    if (!c || n < 0 || (n && (!word_ids || !priors))) return fail(LDA_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (c->has_corpus) return fail(LDA_ERR_STATE, "word priors must be set before the corpus is uploaded");
+    const int K = c->K, Kp = c->Kp, V = c->V;
+    std::vector<int32_t> row(std::max(V, 1), -1);
+    std::vector<float> tab;
+    int32_t rows = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        const uint32_t v = word_ids[i];
+        for (int k = 0; k < K; ++k) {
+            const float x = priors[(size_t)i * K + k];
+            if (!(x >= 0.0f && x < 1e30f)) return fail(LDA_ERR_INVALID_ARGUMENT, "priors must not be less than 0.");
+        }
+        if (v >= (uint32_t)V) continue;  // removed or unknown words are skipped (LDAModel.hpp:733)
+        if (row[v] >= 0) return fail(LDA_ERR_INVALID_ARGUMENT, "word %u has two priors", v);
+        row[v] = rows++;
+        tab.resize((size_t)rows * Kp, 0.0f);
+        for (int k = 0; k < K; ++k) tab[(size_t)(rows - 1) * Kp + c->pos(k)] = priors[(size_t)i * K + k];
+    }
+    if (n == 0) {
+        c->has_priors = false;
+        return LDA_OK;
+    }
+    // etaSumByTopic = rowwise sum of eta_kv (LDAModel.hpp:736): float, words in
+    // ascending order (the sampler's), and double (the FP64 log-likelihood's)
+    std::vector<float> es(Kp, 1.0f);
+    std::vector<double> esd(Kp, 1.0);
+    for (int k = 0; k < K; ++k) {
+        const int q = c->pos(k);
+        float sf = 0.0f;
+        double sd = 0.0;
+        for (int v = 0; v < V; ++v) {
+            const float x = row[v] >= 0 ? tab[(size_t)row[v] * Kp + q] : c->eta;
+            sf += x;
+            sd += (double)x;
+        }
+        es[q] = sf;
+        esd[q] = sd;
+    }
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(c->d_prior_row.alloc(row.size()));
+    HIP_TRY(c->d_prior_eta.alloc(std::max<size_t>(tab.size(), 1)));
+    HIP_TRY(c->d_etasum.alloc(Kp));
+    HIP_TRY(c->d_etasum_d.alloc(Kp));
+    HIP_TRY(hipMemcpyAsync(c->d_prior_row.p, row.data(), row.size() * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    if (!tab.empty())
+        HIP_TRY(hipMemcpyAsync(c->d_prior_eta.p, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->d_etasum.p, es.data(), Kp * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->d_etasum_d.p, esd.data(), Kp * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->has_priors = true;
+    return LDA_OK;
 }
 
 int lda_set_alpha(lda_ctx* c, const float* alpha) {
@@ -741,10 +802,13 @@ int lda_loglik(lda_ctx* c, double* ll_docs, double* ll_rest) {
     if (c->tw != LDA_TW_ONE)
         HIP_TRY(lda::launch_loglik_f(reinterpret_cast<const float*>(c->d_ndk.p), c->d_docw.p, D,
                                      reinterpret_cast<const float*>(c->d_nkw.p), reinterpret_cast<const float*>(c->d_nk.p),
-                                     c->d_alpha.p, c->V, c->Kp, asum, eta, veta, c->d_part.p, nblk, c->stream));
+                                     c->d_alpha.p, c->V, c->Kp, asum, eta, veta, c->d_part.p, nblk,
+                                     c->has_priors ? c->d_prior_row.p : nullptr, c->d_prior_eta.p, c->d_etasum_d.p,
+                                     c->stream));
     else
         HIP_TRY(lda::launch_loglik(c->d_ndk.p, c->d_doclen.p, D, c->d_nkw.p, c->d_nk.p, c->d_alpha.p, c->V, c->Kp, asum,
-                                   eta, veta, c->d_part.p, nblk, c->stream));
+                                   eta, veta, c->d_part.p, nblk, c->has_priors ? c->d_prior_row.p : nullptr,
+                                   c->d_prior_eta.p, c->d_etasum_d.p, c->stream));
     std::vector<double> part(2 * nblk);
     HIP_TRY(hipMemcpyAsync(part.data(), c->d_part.p, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
@@ -885,6 +949,7 @@ int lda_get_dims(lda_ctx* c, int64_t dims[6]) {
 int lda_sample_frozen(lda_ctx* c, int64_t n, const int64_t* doc, const uint32_t* word, const uint16_t* z_old,
                       const float* u, uint16_t* z_new, float* total) {
     if (c && c->tw != LDA_TW_ONE) return fail(LDA_ERR_UNSUPPORTED, "lda_sample_frozen takes integer counts (TermWeight ONE)");
+    if (c && c->has_priors) return fail(LDA_ERR_UNSUPPORTED, "lda_sample_frozen uses the symmetric eta");
     if (!c) return fail(LDA_ERR_INVALID_ARGUMENT, "ctx is NULL");
     if (n < 0 || (n && (!doc || !word || !z_old || !u || !z_new))) return fail(LDA_ERR_INVALID_ARGUMENT, "NULL argument");
     if (!c->has_corpus) return fail(LDA_ERR_STATE, "no corpus uploaded");
@@ -922,6 +987,7 @@ int lda_infer(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off, int64_
               int mode, float* theta, double* ll, uint16_t* z_out) {
     if (!c) return fail(LDA_ERR_INVALID_ARGUMENT, "ctx is NULL");
     if (c->tw != LDA_TW_ONE) return fail(LDA_ERR_UNSUPPORTED, "fold-in of term-weighted models is not on the device yet");
+    if (c->has_priors) return fail(LDA_ERR_UNSUPPORTED, "fold-in with word priors is not on the device yet");
     if (D < 0 || (D && (!doc_off || !theta || !ll))) return fail(LDA_ERR_INVALID_ARGUMENT, "NULL argument");
     if (iterations < 0) return fail(LDA_ERR_INVALID_ARGUMENT, "iterations must be >= 0");
     if (mode != 0 && mode != 1) return fail(LDA_ERR_INVALID_ARGUMENT, "mode must be 0 or 1");

@@ -206,8 +206,8 @@ __device__ __forceinline__ void lds_store_f(float* a, int lane, const V& o) {
 // and refreshes the reciprocal.  Concurrent updates of one topic by two
 // wavefronts of a workgroup keep n_k exact; y then holds the reciprocal of a
 // value n_k had a moment earlier, which the Markstein step absorbs.
-template <int KPL>
-__device__ __forceinline__ void bump(const WaveLds& S, const fvec<KPL>& al, int lane, int z, int delta, float veta) {
+template <int KPL, class E>
+__device__ __forceinline__ void bump(const WaveLds& S, const fvec<KPL>& al, int lane, int z, int delta, const E& et) {
     const int sl = z & (KPL - 1);
     const float a_sl = al[sl];
     if (lane == (z >> kpl_shift<KPL>())) {
@@ -216,7 +216,7 @@ __device__ __forceinline__ void bump(const WaveLds& S, const fvec<KPL>& al, int 
         S.A[q] = (float)d + a_sl;
         S.col[q] += delta;
         const int n = atomicAdd(S.nk + q, delta) + delta;
-        S.y[q] = 1.0f / ((float)n + veta);
+        S.y[q] = 1.0f / ((float)n + et.s_at(q));
     }
 }
 
@@ -240,21 +240,73 @@ __device__ __forceinline__ ivec<KPL> get_ndk(const WaveLds& S, const fvec<KPL>& 
     return nd;
 }
 
-// Private n_k from a K-vector: n_k and the reciprocals (one wavefront writes).
+// The eta terms of the weights (EtaHelper, LDAModel.hpp:81-117).  EtaSym:
+// eta for every cell and V*eta for every topic.  EtaPrior (word priors):
+// eta_kv of the current word (the word's prior column, or eta when it has
+// none) and sum_v eta_kv per topic, kept per lane for the lane's topics; the
+// column is reloaded when the word changes (priors are sparse: a table of
+// prior columns indexed through prior_row).  Padded topics carry eta_kv = 0
+// and a sum of 1, so their weight stays exactly 0.
+// e(j), s(j): this lane's topic slot j (static j); s_at(q): lane-layout
+// position q (the owner lane of a single-topic update).
 template <int KPL>
-__device__ __forceinline__ void set_nk_vec(const WaveLds& S, const ivec<KPL>& n, int lane, float veta) {
+struct EtaSym {
+    float eta, veta;
+    __device__ __forceinline__ float e(int) const { return eta; }
+    __device__ __forceinline__ float s(int) const { return veta; }
+    __device__ __forceinline__ float s_at(int) const { return veta; }
+    __device__ __forceinline__ void word(const SweepParams&, int, int) {}
+};
+
+template <int KPL>
+struct EtaPrior {
+    fvec<KPL> ec, vs;
+    const float* es;
+    float eta;
+    __device__ __forceinline__ float e(int j) const { return ec[j]; }
+    __device__ __forceinline__ float s(int j) const { return vs[j]; }
+    __device__ __forceinline__ float s_at(int q) const { return es[q]; }
+    __device__ __forceinline__ void word(const SweepParams& p, int v, int lane) {
+        const int r = p.prior_row[v];
+        if (r >= 0) load_kvec_f<KPL>(p.prior_eta + (int64_t)r * p.Kp, lane, ec);
+        else ec = eta;
+    }
+};
+
+template <int KPL, bool ASYM>
+struct EtaOf { using type = EtaSym<KPL>; };
+template <int KPL>
+struct EtaOf<KPL, true> { using type = EtaPrior<KPL>; };
+
+template <int KPL, bool ASYM>
+__device__ __forceinline__ typename EtaOf<KPL, ASYM>::type make_eta(const SweepParams& p, int lane) {
+    if constexpr (ASYM) {
+        EtaPrior<KPL> E;
+        E.es = p.etasum;
+        E.eta = p.eta;
+        E.ec = p.eta;
+        load_kvec_f<KPL>(p.etasum, lane, E.vs);
+        return E;
+    } else {
+        return EtaSym<KPL>{p.eta, p.veta};
+    }
+}
+
+// Private n_k from a K-vector: n_k and the reciprocals (one wavefront writes).
+template <int KPL, class E>
+__device__ __forceinline__ void set_nk_vec(const WaveLds& S, const ivec<KPL>& n, int lane, const E& et) {
     fvec<KPL> y;
 #pragma unroll
-    for (int j = 0; j < KPL; ++j) y[j] = 1.0f / ((float)n[j] + veta);
+    for (int j = 0; j < KPL; ++j) y[j] = 1.0f / ((float)n[j] + et.s(j));
     lds_store<KPL>(S.nk, lane, n);
     lds_store_f<KPL>(S.y, lane, y);
 }
 
-template <int KPL>
-__device__ __forceinline__ void set_nk(const WaveLds& S, const int32_t* __restrict__ src, int lane, float veta) {
+template <int KPL, class E>
+__device__ __forceinline__ void set_nk(const WaveLds& S, const int32_t* __restrict__ src, int lane, const E& et) {
     ivec<KPL> n;
     load_kvec<KPL>(src, lane, n);
-    set_nk_vec<KPL>(S, n, lane, veta);
+    set_nk_vec<KPL>(S, n, lane, et);
 }
 
 template <int KPL>
@@ -297,10 +349,10 @@ __device__ __forceinline__ int scan_search(fvec<KPL>& w, float u, int K, int nva
 // the lane's last real topic: the total can be read from slot KPL-1 of lane
 // (K-1)/KPL, and a lane's miss count over all slots stays below its number of
 // real topics exactly when one of them is hit.
-template <int KPL>
+template <int KPL, class E>
 __device__ __forceinline__ int token_step(const WaveLds& S, const fvec<KPL>& al, int lane, int zo, float u, int K,
-                                          int nvalid, int lastLane, float eta, float veta, float* total) {
-    bump<KPL>(S, al, lane, zo, -1, veta);
+                                          int nvalid, int lastLane, const E& et, float* total) {
+    bump<KPL>(S, al, lane, zo, -1, et);
     fvec<KPL> A, y;
     ivec<KPL> col, nk;
     lds_load_f<KPL>(S.A, lane, A);
@@ -312,16 +364,16 @@ __device__ __forceinline__ int token_step(const WaveLds& S, const fvec<KPL>& al,
     for (int j = 0; j < KPL; ++j) {
         // ((n_dk + alpha) * (n_kw + eta)) / (n_k + V*eta); the quotient as
         // q0 = a*y, r = fma(-q0, b, a), q = fma(r, y, q0) with y = RN(1/b)
-        float B = (float)col[j] + eta;
+        float B = (float)col[j] + et.e(j);
         float a = A[j] * B;
-        float bf = (float)nk[j] + veta;
+        float bf = (float)nk[j] + et.s(j);
         float q0 = a * y[j];
         float rr = __builtin_fmaf(-q0, bf, a);
         w[j] = __builtin_fmaf(rr, y[j], q0);
     }
     float tot;
     const int zn = scan_search<KPL>(w, u, K, nvalid, lastLane, tot);
-    bump<KPL>(S, al, lane, zn, +1, veta);
+    bump<KPL>(S, al, lane, zn, +1, et);
     if (total) *total = tot;
     return zn;
 }
@@ -335,9 +387,9 @@ __constant__ uint64_t kShufflePrimes[16] = {65537, 65539, 65543, 65551, 65557, 6
 
 // sampleDocument over the token range [b, e) (LDAModel.hpp:326-354).  The
 // column of the next distinct word is fetched one token ahead.
-template <int KPL, int MODE>
+template <int KPL, int MODE, class E>
 __device__ __forceinline__ void sample_range(const WaveLds& S, const fvec<KPL>& al, const SweepParams& p, int lane,
-                                             uint64_t b, uint64_t e, int nvalid, int lastLane) {
+                                             uint64_t b, uint64_t e, int nvalid, int lastLane, E& et) {
     constexpr int SH = kpl_shift<KPL>();
     int vcur = -1;
     int32_t* colp = nullptr;
@@ -361,6 +413,7 @@ __device__ __forceinline__ void sample_range(const WaveLds& S, const fvec<KPL>& 
                 colp = p.nkw + (int64_t)v * p.Kp;
                 if (v != vnext) load_kvec<KPL>(colp, lane, ncol);
                 lds_store<KPL>(S.col, lane, ncol);
+                et.word(p, v, lane);
                 vcur = v;
             }
             if (i + 1 < n) {
@@ -372,7 +425,7 @@ __device__ __forceinline__ void sample_range(const WaveLds& S, const fvec<KPL>& 
             }
             const int zo = readlane_i(zv, i);
             const float u = readlane_f(uv, i);
-            const int zn = token_step<KPL>(S, al, lane, zo, u, p.K, nvalid, lastLane, p.eta, p.veta, nullptr);
+            const int zn = token_step<KPL>(S, al, lane, zo, u, p.K, nvalid, lastLane, et, nullptr);
             if (zn != zo) {
                 const int lo = zo >> SH, ln = zn >> SH;
                 const int qo = kpos<KPL>(lane, zo & (KPL - 1)), qn = kpos<KPL>(lane, zn & (KPL - 1));
@@ -397,7 +450,7 @@ __device__ __forceinline__ void sample_range(const WaveLds& S, const fvec<KPL>& 
 // each worker adds its n_k delta to nk_out when it runs out of documents.
 constexpr int kSweepWaves = 4;  // wavefronts per workgroup of the device-wide sweep (one worker)
 
-template <int KPL>
+template <int KPL, bool ASYM>
 __global__ void __launch_bounds__(64 * kSweepWaves) sweep_device_kernel(SweepParams p) {
     const int lane = lane_id();
     const WaveLds S = wave_lds<KPL>();
@@ -405,7 +458,8 @@ __global__ void __launch_bounds__(64 * kSweepWaves) sweep_device_kernel(SweepPar
     lane_limits<KPL>(p.K, lane, nvalid, lastLane);
     fvec<KPL> al;
     load_kvec_f<KPL>(p.alpha, lane, al);
-    if (threadIdx.x < 64) set_nk<KPL>(S, p.nk_in, lane, p.veta);
+    auto et = make_eta<KPL, ASYM>(p, lane);
+    if (threadIdx.x < 64) set_nk<KPL>(S, p.nk_in, lane, et);
     __syncthreads();
     for (;;) {
         uint32_t idx = 0;
@@ -418,7 +472,7 @@ __global__ void __launch_bounds__(64 * kSweepWaves) sweep_device_kernel(SweepPar
         ivec<KPL> nd;
         load_kvec<KPL>(ndk_row, lane, nd);
         set_ndk<KPL>(S, al, nd, lane);
-        sample_range<KPL, MODE_ATOMIC>(S, al, p, lane, b, e, nvalid, lastLane);
+        sample_range<KPL, MODE_ATOMIC>(S, al, p, lane, b, e, nvalid, lastLane, et);
         nd = get_ndk<KPL>(S, al, lane);
         store_kvec<KPL>(ndk_row, lane, nd);
     }
@@ -439,7 +493,7 @@ __global__ void __launch_bounds__(64 * kSweepWaves) sweep_device_kernel(SweepPar
 // wavefront p is worker p; it visits documents d = id*W + ((round+p) % W) in
 // forShuffled order (Utils.hpp:157-178) and samples only their tokens in
 // vocabulary chunk p (coff rows p, p+1).  W == 1 is the `none` scheme.
-template <int KPL>
+template <int KPL, bool ASYM>
 __global__ void __launch_bounds__(64) sweep_det_kernel(SweepParams p) {
     const int lane = lane_id();
     const int w = blockIdx.x;
@@ -448,8 +502,9 @@ __global__ void __launch_bounds__(64) sweep_det_kernel(SweepParams p) {
     lane_limits<KPL>(p.K, lane, nvalid, lastLane);
     fvec<KPL> al;
     load_kvec_f<KPL>(p.alpha, lane, al);
+    auto et = make_eta<KPL, ASYM>(p, lane);
     int32_t* nkp = p.nk_out + (int64_t)w * p.Kp;
-    set_nk<KPL>(S, nkp, lane, p.veta);
+    set_nk<KPL>(S, nkp, lane, et);
     const int W = p.W;
     const int didx = (p.round + w) % W;
     const uint64_t N = (uint64_t)((p.D + (W - 1) - didx) / W);
@@ -475,7 +530,7 @@ __global__ void __launch_bounds__(64) sweep_det_kernel(SweepParams p) {
         ivec<KPL> nd;
         load_kvec<KPL>(ndk_row, lane, nd);
         set_ndk<KPL>(S, al, nd, lane);
-        sample_range<KPL, MODE_DET>(S, al, p, lane, b, e, nvalid, lastLane);
+        sample_range<KPL, MODE_DET>(S, al, p, lane, b, e, nvalid, lastLane, et);
         nd = get_ndk<KPL>(S, al, lane);
         store_kvec<KPL>(ndk_row, lane, nd);
     }
@@ -489,7 +544,7 @@ __global__ void __launch_bounds__(64) sweep_det_kernel(SweepParams p) {
 // tasks c = w, w+W, ... of chStride = min(8W, D) strided tasks, task c
 // visiting documents id*chStride + c in forShuffled order.  Workers share no
 // counts, so the result does not depend on their timing.
-template <int KPL>
+template <int KPL, bool ASYM>
 __global__ void __launch_bounds__(64) sweep_copy_kernel(SweepParams p) {
     const int lane = lane_id();
     const int w = blockIdx.x;
@@ -498,8 +553,9 @@ __global__ void __launch_bounds__(64) sweep_copy_kernel(SweepParams p) {
     lane_limits<KPL>(p.K, lane, nvalid, lastLane);
     fvec<KPL> al;
     load_kvec_f<KPL>(p.alpha, lane, al);
+    auto et = make_eta<KPL, ASYM>(p, lane);
     int32_t* nkp = p.nk_out + (int64_t)w * p.Kp;
-    set_nk<KPL>(S, nkp, lane, p.veta);
+    set_nk<KPL>(S, nkp, lane, et);
     SweepParams q = p;
     q.nkw = p.nkw + (int64_t)w * p.V * p.Kp;
     const int W = p.W;
@@ -526,7 +582,7 @@ __global__ void __launch_bounds__(64) sweep_copy_kernel(SweepParams p) {
             ivec<KPL> nd;
             load_kvec<KPL>(ndk_row, lane, nd);
             set_ndk<KPL>(S, al, nd, lane);
-            sample_range<KPL, MODE_DET>(S, al, q, lane, b, e, nvalid, lastLane);
+            sample_range<KPL, MODE_DET>(S, al, q, lane, b, e, nvalid, lastLane, et);
             nd = get_ndk<KPL>(S, al, lane);
             store_kvec<KPL>(ndk_row, lane, nd);
         }
@@ -594,8 +650,8 @@ __device__ __forceinline__ void atomic_sub_clamp(float* p, float w) {
 // addWordTo<+-w> of topic z on this worker's state (owner lane).  SHARED:
 // the workgroup's wavefronts share n_k (LDS atomics; y then follows a value
 // n_k had a moment earlier, which the Markstein step absorbs).
-template <int KPL, bool SHARED>
-__device__ __forceinline__ void bump_w(const WaveLdsW& S, int lane, int z, float w, bool dec, float veta) {
+template <int KPL, bool SHARED, class E>
+__device__ __forceinline__ void bump_w(const WaveLdsW& S, int lane, int z, float w, bool dec, const E& et) {
     const int sl = z & (KPL - 1);
     if (lane == (z >> kpl_shift<KPL>())) {
         const int q = kpos<KPL>(lane, sl);
@@ -610,14 +666,14 @@ __device__ __forceinline__ void bump_w(const WaveLdsW& S, int lane, int z, float
             n = dec ? sub_clamp(S.nk[q], w) : S.nk[q] + w;
             S.nk[q] = n;
         }
-        S.y[q] = 1.0f / (n + veta);
+        S.y[q] = 1.0f / (n + et.s_at(q));
     }
 }
 
-template <int KPL, bool SHARED>
+template <int KPL, bool SHARED, class E>
 __device__ __forceinline__ int token_step_w(const WaveLdsW& S, const fvec<KPL>& al, int lane, int zo, float wt, float u,
-                                            int K, int nvalid, int lastLane, float eta, float veta) {
-    bump_w<KPL, SHARED>(S, lane, zo, wt, true, veta);
+                                            int K, int nvalid, int lastLane, const E& et) {
+    bump_w<KPL, SHARED>(S, lane, zo, wt, true, et);
     fvec<KPL> nd, col, nk, y;
     lds_load_f<KPL>(S.nd, lane, nd);
     lds_load_f<KPL>(S.col, lane, col);
@@ -626,15 +682,15 @@ __device__ __forceinline__ int token_step_w(const WaveLdsW& S, const fvec<KPL>& 
     fvec<KPL> w;
 #pragma unroll
     for (int j = 0; j < KPL; ++j) {
-        const float a = (nd[j] + al[j]) * (col[j] + eta);
-        const float bf = nk[j] + veta;
+        const float a = (nd[j] + al[j]) * (col[j] + et.e(j));
+        const float bf = nk[j] + et.s(j);
         const float q0 = a * y[j];
         const float rr = __builtin_fmaf(-q0, bf, a);
         w[j] = __builtin_fmaf(rr, y[j], q0);
     }
     float tot;
     const int zn = scan_search<KPL>(w, u, K, nvalid, lastLane, tot);
-    bump_w<KPL, SHARED>(S, lane, zn, wt, false, veta);
+    bump_w<KPL, SHARED>(S, lane, zn, wt, false, et);
     return zn;
 }
 
@@ -643,9 +699,9 @@ __device__ __forceinline__ int token_step_w(const WaveLdsW& S, const fvec<KPL>& 
 // exactly.  MODE_DET: the column cells are stored to n_kw (exclusive owner),
 // MODE_ATOMIC: the token's net change goes to the shared n_kw as float
 // atomics.
-template <int KPL, int MODE>
+template <int KPL, int MODE, class E>
 __device__ __forceinline__ void sample_range_w(const WaveLdsW& S, const fvec<KPL>& al, const SweepParams& p, int lane,
-                                               uint64_t b, uint64_t e, int nvalid, int lastLane) {
+                                               uint64_t b, uint64_t e, int nvalid, int lastLane, E& et) {
     constexpr int SH = kpl_shift<KPL>();
     constexpr bool SHARED = MODE == MODE_ATOMIC;
     float* nkwf = reinterpret_cast<float*>(p.nkw);
@@ -672,6 +728,7 @@ __device__ __forceinline__ void sample_range_w(const WaveLdsW& S, const fvec<KPL
                 colp = nkwf + (int64_t)v * p.Kp;
                 if (v != vnext) load_kvec_f<KPL>(colp, lane, ncol);
                 lds_store_f<KPL>(S.col, lane, ncol);
+                et.word(p, v, lane);
                 vcur = v;
             }
             if (i + 1 < n) {
@@ -684,7 +741,7 @@ __device__ __forceinline__ void sample_range_w(const WaveLdsW& S, const fvec<KPL
             const int zo = readlane_i(zv, i);
             const float w = readlane_f(tw, i);
             const float u = readlane_f(uv, i);
-            const int zn = token_step_w<KPL, SHARED>(S, al, lane, zo, w, u, p.K, nvalid, lastLane, p.eta, p.veta);
+            const int zn = token_step_w<KPL, SHARED>(S, al, lane, zo, w, u, p.K, nvalid, lastLane, et);
             const int lo = zo >> SH, ln = zn >> SH;
             const int qo = kpos<KPL>(lane, zo & (KPL - 1)), qn = kpos<KPL>(lane, zn & (KPL - 1));
             if constexpr (MODE == MODE_DET) {
@@ -707,7 +764,7 @@ __device__ __forceinline__ void sample_range_w(const WaveLdsW& S, const fvec<KPL
 // Device-wide weighted sweep (see sweep_device_kernel).  n_k is rebuilt from
 // n_kw after the sweep (launch_clamp_rowsum_f), tomoto's merge rule for
 // weighted counts, so the workgroups' private n_k need no flush.
-template <int KPL>
+template <int KPL, bool ASYM>
 __global__ void __launch_bounds__(64 * kSweepWaves) sweep_device_w_kernel(SweepParams p) {
     const int lane = lane_id();
     const WaveLdsW S = wave_lds_w<KPL>();
@@ -715,11 +772,12 @@ __global__ void __launch_bounds__(64 * kSweepWaves) sweep_device_w_kernel(SweepP
     lane_limits<KPL>(p.K, lane, nvalid, lastLane);
     fvec<KPL> al;
     load_kvec_f<KPL>(p.alpha, lane, al);
+    auto et = make_eta<KPL, ASYM>(p, lane);
     if (threadIdx.x < 64) {
         fvec<KPL> n, y;
         load_kvec_f<KPL>(reinterpret_cast<const float*>(p.nk_in), lane, n);
 #pragma unroll
-        for (int j = 0; j < KPL; ++j) y[j] = 1.0f / (n[j] + p.veta);
+        for (int j = 0; j < KPL; ++j) y[j] = 1.0f / (n[j] + et.s(j));
         lds_store_f<KPL>(S.nk, lane, n);
         lds_store_f<KPL>(S.y, lane, y);
     }
@@ -735,7 +793,7 @@ __global__ void __launch_bounds__(64 * kSweepWaves) sweep_device_w_kernel(SweepP
         fvec<KPL> nd;
         load_kvec_f<KPL>(ndk_row, lane, nd);
         lds_store_f<KPL>(S.nd, lane, nd);
-        sample_range_w<KPL, MODE_ATOMIC>(S, al, p, lane, b, e, nvalid, lastLane);
+        sample_range_w<KPL, MODE_ATOMIC>(S, al, p, lane, b, e, nvalid, lastLane, et);
         lds_load_f<KPL>(S.nd, lane, nd);
         store_kvec<KPL>(reinterpret_cast<int32_t*>(ndk_row), lane, __builtin_bit_cast(ivec<KPL>, nd));
     }
@@ -743,7 +801,7 @@ __global__ void __launch_bounds__(64 * kSweepWaves) sweep_device_w_kernel(SweepP
 
 // One round of the deterministic weighted partition schedule (see
 // sweep_det_kernel): worker-private n_k, plain stores to n_kw.
-template <int KPL>
+template <int KPL, bool ASYM>
 __global__ void __launch_bounds__(64) sweep_det_w_kernel(SweepParams p) {
     const int lane = lane_id();
     const int w = blockIdx.x;
@@ -752,12 +810,13 @@ __global__ void __launch_bounds__(64) sweep_det_w_kernel(SweepParams p) {
     lane_limits<KPL>(p.K, lane, nvalid, lastLane);
     fvec<KPL> al;
     load_kvec_f<KPL>(p.alpha, lane, al);
+    auto et = make_eta<KPL, ASYM>(p, lane);
     float* nkp = reinterpret_cast<float*>(p.nk_out) + (int64_t)w * p.Kp;
     {
         fvec<KPL> n, y;
         load_kvec_f<KPL>(nkp, lane, n);
 #pragma unroll
-        for (int j = 0; j < KPL; ++j) y[j] = 1.0f / (n[j] + p.veta);
+        for (int j = 0; j < KPL; ++j) y[j] = 1.0f / (n[j] + et.s(j));
         lds_store_f<KPL>(S.nk, lane, n);
         lds_store_f<KPL>(S.y, lane, y);
     }
@@ -785,7 +844,7 @@ __global__ void __launch_bounds__(64) sweep_det_w_kernel(SweepParams p) {
         fvec<KPL> nd;
         load_kvec_f<KPL>(ndk_row, lane, nd);
         lds_store_f<KPL>(S.nd, lane, nd);
-        sample_range_w<KPL, MODE_DET>(S, al, p, lane, b, e, nvalid, lastLane);
+        sample_range_w<KPL, MODE_DET>(S, al, p, lane, b, e, nvalid, lastLane, et);
         lds_load_f<KPL>(S.nd, lane, nd);
         store_kvec<KPL>(reinterpret_cast<int32_t*>(ndk_row), lane, __builtin_bit_cast(ivec<KPL>, nd));
     }
@@ -828,12 +887,43 @@ __global__ void rowsum_blocks_f_kernel(const float* __restrict__ part, int nb, i
 // ---------------------------------------------------------------- init ----
 // Initial topics z ~ Discrete(alpha) (LDAModel.hpp:754-762): r = u*cdf[K-1],
 // first k with r < cdf[k] (binary search on the non-decreasing cdf).
+// Lane-layout position of topic k for a runtime KPL (kpos).
+__device__ __forceinline__ int kpos_rt(int kpl, int k) {
+    const int l = k / kpl, j = k - l * kpl;
+    return kpl < 4 ? k : (j >> 2) * 256 + l * 4 + (j & 3);
+}
+
+// With word priors the initial topic is drawn from the word's eta column by
+// sampleFromDiscrete (sample.hpp:121-131, LDAModel.hpp:769-773): r = u * (the
+// sequential float sum of the column), then subtract eta_kv while r > eta_kv
+// and k < K-1.  A word without a prior has the column eta everywhere.
+__device__ __forceinline__ int draw_from_eta_column(const float* col, int kpl, float eta, int K, float u) {
+    float tot = 0.0f;
+    for (int k = 0; k < K; ++k) tot += col ? col[kpos_rt(kpl, k)] : eta;
+    float r = u * tot;
+    int k = 0;
+    for (;;) {
+        const float w = col ? col[kpos_rt(kpl, k)] : eta;
+        if (!(r > w && k < K - 1)) break;
+        r -= w;
+        ++k;
+    }
+    return k;
+}
+
 __global__ void init_topics_kernel(const uint32_t* __restrict__ words, uint16_t* __restrict__ z, uint64_t N,
-                                   const float* __restrict__ ca, int K, int V, uint32_t s0, uint32_t s1) {
+                                   const float* __restrict__ ca, int K, int V, uint32_t s0, uint32_t s1,
+                                   const int32_t* __restrict__ prior_row, const float* __restrict__ prior_eta, int kpl,
+                                   int Kp, float eta) {
     for (uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; g < N; g += (uint64_t)gridDim.x * blockDim.x) {
         uint32_t v = words[g];
         if (v >= (uint32_t)V) { z[g] = 0xFFFF; continue; }
         float u = uniform24(philox_x0(s0, s1, (uint32_t)g, (uint32_t)(g >> 32), 0u, TAG_INIT));
+        if (prior_row) {
+            const int r = prior_row[v];
+            z[g] = (uint16_t)draw_from_eta_column(r >= 0 ? prior_eta + (int64_t)r * Kp : nullptr, kpl, eta, K, u);
+            continue;
+        }
         float r = u * ca[K - 1];
         int lo = 0, hi = K;  // first index with r < ca[idx]
         while (lo < hi) {
@@ -904,12 +994,16 @@ __device__ __forceinline__ double block_sum_d(double x, double* sh) {
 
 // getLLDocs + getLLRest in FP64 (LDAModel.hpp:616-666).  Block b writes its
 // docs-part partial to partial[b] and its rest-part partial to partial[nblk+b].
+// Word priors (LDAModel.hpp:638-650): prior_row / prior_eta give eta_kv,
+// etasum_d the per-topic sum of eta_kv in double; NULL: symmetric eta.
 template <typename C>
 __global__ void __launch_bounds__(256) loglik_kernel(const C* __restrict__ ndk, const C* __restrict__ doclen,
                                                      int64_t D, const C* __restrict__ nkw,
                                                      const C* __restrict__ nk, const float* __restrict__ alpha,
                                                      int V, int Kp, double asum, double eta, double veta,
-                                                     double* partial) {
+                                                     double* partial, const int32_t* __restrict__ prior_row,
+                                                     const float* __restrict__ prior_eta,
+                                                     const double* __restrict__ etasum_d) {
     __shared__ double sh[8];
     const int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     const int64_t nth = (int64_t)gridDim.x * blockDim.x;
@@ -928,12 +1022,20 @@ __global__ void __launch_bounds__(256) loglik_kernel(const C* __restrict__ ndk, 
     const int64_t nv = (int64_t)V * Kp;
     for (int64_t i = tid; i < nv; i += nth) {
         const C c = nkw[i];
-        if (c) rest += lgamma((double)c + eta) - lge;
+        if (!c) continue;
+        int r = -1;
+        if (prior_row) r = prior_row[i / Kp];
+        if (r < 0) rest += lgamma((double)c + eta) - lge;
+        else {
+            const double e = (double)prior_eta[(int64_t)r * Kp + i % Kp];
+            rest += lgamma((double)c + e) - lgamma(e);
+        }
     }
     for (int64_t i = tid; i < Kp; i += nth) {
         // padded positions carry n_k == 0 and alpha == 0; count real topics only
         float a = alpha[i];
-        if (a > 0.0f) rest += lgamma(veta) - lgamma((double)nk[i] + veta);
+        const double es = etasum_d ? etasum_d[i] : veta;
+        if (a > 0.0f) rest += lgamma(es) - lgamma((double)nk[i] + es);
     }
     double bd = block_sum_d(docs, sh);
     double br = block_sum_d(rest, sh);
@@ -1059,15 +1161,16 @@ __global__ void __launch_bounds__(64) sample_frozen_kernel(const int32_t* __rest
     lane_limits<KPL>(K, lane, nvalid, lastLane);
     fvec<KPL> al;
     load_kvec_f<KPL>(alpha, lane, al);
+    const EtaSym<KPL> et{eta, veta};
     for (int64_t t = blockIdx.x; t < n; t += gridDim.x) {
-        set_nk<KPL>(S, nk, lane, veta);
+        set_nk<KPL>(S, nk, lane, et);
         ivec<KPL> x;
         load_kvec<KPL>(ndk + doc[t] * Kp, lane, x);
         set_ndk<KPL>(S, al, x, lane);
         load_kvec<KPL>(nkw + (int64_t)word[t] * Kp, lane, x);
         lds_store<KPL>(S.col, lane, x);
         float tot;
-        const int zn = token_step<KPL>(S, al, lane, (int)z_old[t], u[t], K, nvalid, lastLane, eta, veta, &tot);
+        const int zn = token_step<KPL>(S, al, lane, (int)z_old[t], u[t], K, nvalid, lastLane, et, &tot);
         if (lane == 0) {
             z_new[t] = (uint16_t)zn;
             if (total) total[t] = tot;
@@ -1570,12 +1673,40 @@ __global__ void __launch_bounds__(64) infer_finalize_kernel(const InferParams* _
         default: return hipErrorInvalidValue;             \
     }
 
+// ASYM: the word-prior variant of the training kernels (SweepParams.prior_row set)
+#define LDA_DISPATCH_KPL_ETA(kpl, asym, CALL)                          \
+    if (asym) {                                                        \
+        constexpr bool ASYM = true;                                    \
+        switch (kpl) {                                                 \
+            case 1: { constexpr int KPL = 1; CALL; } break;            \
+            case 2: { constexpr int KPL = 2; CALL; } break;            \
+            case 4: { constexpr int KPL = 4; CALL; } break;            \
+            case 8: { constexpr int KPL = 8; CALL; } break;            \
+            case 16: { constexpr int KPL = 16; CALL; } break;          \
+            case 32: { constexpr int KPL = 32; CALL; } break;          \
+            default: return hipErrorInvalidValue;                      \
+        }                                                              \
+    } else {                                                           \
+        constexpr bool ASYM = false;                                   \
+        switch (kpl) {                                                 \
+            case 1: { constexpr int KPL = 1; CALL; } break;            \
+            case 2: { constexpr int KPL = 2; CALL; } break;            \
+            case 4: { constexpr int KPL = 4; CALL; } break;            \
+            case 8: { constexpr int KPL = 8; CALL; } break;            \
+            case 16: { constexpr int KPL = 16; CALL; } break;          \
+            case 32: { constexpr int KPL = 32; CALL; } break;          \
+            default: return hipErrorInvalidValue;                      \
+        }                                                              \
+    }
+
 hipError_t launch_init_topics(const uint32_t* words, uint16_t* z, uint64_t N, const float* ca, int K, int V, uint32_t s0,
-                              uint32_t s1, hipStream_t s) {
+                              uint32_t s1, const int32_t* prior_row, const float* prior_eta, int kpl, int Kp, float eta,
+                              hipStream_t s) {
     if (!N) return hipSuccess;
     uint64_t blocks = (N + 255) / 256;
     if (blocks > 65536) blocks = 65536;
-    hipLaunchKernelGGL(init_topics_kernel, dim3((unsigned)blocks), dim3(256), 0, s, words, z, N, ca, K, V, s0, s1);
+    hipLaunchKernelGGL(init_topics_kernel, dim3((unsigned)blocks), dim3(256), 0, s, words, z, N, ca, K, V, s0, s1,
+                       prior_row, prior_eta, kpl, Kp, eta);
     return hipGetLastError();
 }
 
@@ -1590,13 +1721,15 @@ hipError_t launch_build_counts(int kpl, const uint32_t* words, const uint16_t* z
 }
 
 hipError_t launch_sweep_device(int kpl, const SweepParams& p, int grid, hipStream_t s) {
-    LDA_DISPATCH_KPL(kpl, hipLaunchKernelGGL(sweep_device_kernel<KPL>, dim3(grid), dim3(64 * kSweepWaves),
-                                             wave_lds_bytes<KPL>(kSweepWaves), s, p));
+    LDA_DISPATCH_KPL_ETA(kpl, p.prior_row != nullptr,
+                         hipLaunchKernelGGL((sweep_device_kernel<KPL, ASYM>), dim3(grid), dim3(64 * kSweepWaves),
+                                            wave_lds_bytes<KPL>(kSweepWaves), s, p));
     return hipGetLastError();
 }
 
 hipError_t launch_sweep_copy(int kpl, const SweepParams& p, hipStream_t s) {
-    LDA_DISPATCH_KPL(kpl, hipLaunchKernelGGL(sweep_copy_kernel<KPL>, dim3(p.W), dim3(64), wave_lds_bytes<KPL>(), s, p));
+    LDA_DISPATCH_KPL_ETA(kpl, p.prior_row != nullptr,
+                         hipLaunchKernelGGL((sweep_copy_kernel<KPL, ASYM>), dim3(p.W), dim3(64), wave_lds_bytes<KPL>(), s, p));
     return hipGetLastError();
 }
 
@@ -1617,23 +1750,29 @@ hipError_t launch_replicate(int32_t* dst, const int32_t* src, int64_t n, int W, 
 }
 
 hipError_t launch_sweep_det(int kpl, const SweepParams& p, hipStream_t s) {
-    LDA_DISPATCH_KPL(kpl, hipLaunchKernelGGL(sweep_det_kernel<KPL>, dim3(p.W), dim3(64), wave_lds_bytes<KPL>(), s, p));
+    LDA_DISPATCH_KPL_ETA(kpl, p.prior_row != nullptr,
+                         hipLaunchKernelGGL((sweep_det_kernel<KPL, ASYM>), dim3(p.W), dim3(64), wave_lds_bytes<KPL>(), s, p));
     return hipGetLastError();
 }
 
-int sweep_device_blocks_per_cu(int kpl) {
+template <int KPL, bool ASYM>
+static int sweep_device_blocks_kpl() {
     int n = 0;
-    hipError_t e = hipErrorInvalidValue;
-    switch (kpl) {
-        case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sweep_device_kernel<1>, 64 * kSweepWaves, wave_lds_bytes<1>(kSweepWaves)); break;
-        case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sweep_device_kernel<2>, 64 * kSweepWaves, wave_lds_bytes<2>(kSweepWaves)); break;
-        case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sweep_device_kernel<4>, 64 * kSweepWaves, wave_lds_bytes<4>(kSweepWaves)); break;
-        case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sweep_device_kernel<8>, 64 * kSweepWaves, wave_lds_bytes<8>(kSweepWaves)); break;
-        case 16: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sweep_device_kernel<16>, 64 * kSweepWaves, wave_lds_bytes<16>(kSweepWaves)); break;
-        case 32: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sweep_device_kernel<32>, 64 * kSweepWaves, wave_lds_bytes<32>(kSweepWaves)); break;
-        default: break;
-    }
+    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sweep_device_kernel<KPL, ASYM>, 64 * kSweepWaves,
+                                                                      wave_lds_bytes<KPL>(kSweepWaves));
     return (e == hipSuccess && n > 0) ? n : 1;
+}
+
+int sweep_device_blocks_per_cu(int kpl, bool asym) {
+    switch (kpl) {
+        case 1: return asym ? sweep_device_blocks_kpl<1, true>() : sweep_device_blocks_kpl<1, false>();
+        case 2: return asym ? sweep_device_blocks_kpl<2, true>() : sweep_device_blocks_kpl<2, false>();
+        case 4: return asym ? sweep_device_blocks_kpl<4, true>() : sweep_device_blocks_kpl<4, false>();
+        case 8: return asym ? sweep_device_blocks_kpl<8, true>() : sweep_device_blocks_kpl<8, false>();
+        case 16: return asym ? sweep_device_blocks_kpl<16, true>() : sweep_device_blocks_kpl<16, false>();
+        case 32: return asym ? sweep_device_blocks_kpl<32, true>() : sweep_device_blocks_kpl<32, false>();
+        default: return 1;
+    }
 }
 
 template <int KPL>
@@ -1686,17 +1825,19 @@ hipError_t launch_rowsum_nkw(const int32_t* nkw, int V, int Kp, int32_t* nk, hip
 
 hipError_t launch_loglik(const int32_t* ndk, const int32_t* doclen, int64_t D, const int32_t* nkw, const int32_t* nk,
                          const float* alpha, int V, int Kp, double asum, double eta, double veta, double* partial,
-                         int nblk, hipStream_t s) {
+                         int nblk, const int32_t* prior_row, const float* prior_eta, const double* etasum_d,
+                         hipStream_t s) {
     hipLaunchKernelGGL(loglik_kernel<int32_t>, dim3(nblk), dim3(256), 0, s, ndk, doclen, D, nkw, nk, alpha, V, Kp, asum,
-                       eta, veta, partial);
+                       eta, veta, partial, prior_row, prior_eta, etasum_d);
     return hipGetLastError();
 }
 
 hipError_t launch_loglik_f(const float* ndk, const float* docw, int64_t D, const float* nkw, const float* nk,
                            const float* alpha, int V, int Kp, double asum, double eta, double veta, double* partial,
-                           int nblk, hipStream_t s) {
+                           int nblk, const int32_t* prior_row, const float* prior_eta, const double* etasum_d,
+                           hipStream_t s) {
     hipLaunchKernelGGL(loglik_kernel<float>, dim3(nblk), dim3(256), 0, s, ndk, docw, D, nkw, nk, alpha, V, Kp, asum,
-                       eta, veta, partial);
+                       eta, veta, partial, prior_row, prior_eta, etasum_d);
     return hipGetLastError();
 }
 
@@ -1713,32 +1854,34 @@ hipError_t launch_alpha_sums_f(const float* ndk, const float* docw, int64_t D, c
 }
 
 hipError_t launch_sweep_device_w(int kpl, const SweepParams& p, int grid, hipStream_t s) {
-    LDA_DISPATCH_KPL(kpl, hipLaunchKernelGGL(sweep_device_w_kernel<KPL>, dim3(grid), dim3(64 * kSweepWaves),
-                                             wave_lds_bytes<KPL>(kSweepWaves), s, p));
+    LDA_DISPATCH_KPL_ETA(kpl, p.prior_row != nullptr,
+                         hipLaunchKernelGGL((sweep_device_w_kernel<KPL, ASYM>), dim3(grid), dim3(64 * kSweepWaves),
+                                            wave_lds_bytes<KPL>(kSweepWaves), s, p));
     return hipGetLastError();
 }
 
 hipError_t launch_sweep_det_w(int kpl, const SweepParams& p, hipStream_t s) {
-    LDA_DISPATCH_KPL(kpl, hipLaunchKernelGGL(sweep_det_w_kernel<KPL>, dim3(p.W), dim3(64), wave_lds_bytes<KPL>(), s, p));
+    LDA_DISPATCH_KPL_ETA(kpl, p.prior_row != nullptr,
+                         hipLaunchKernelGGL((sweep_det_w_kernel<KPL, ASYM>), dim3(p.W), dim3(64), wave_lds_bytes<KPL>(), s, p));
     return hipGetLastError();
 }
 
-template <int KPL>
+template <int KPL, bool ASYM>
 static int sweep_device_w_blocks_kpl() {
     int n = 0;
-    const hipError_t e =
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sweep_device_w_kernel<KPL>, 64 * kSweepWaves, wave_lds_bytes<KPL>(kSweepWaves));
+    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sweep_device_w_kernel<KPL, ASYM>,
+                                                                      64 * kSweepWaves, wave_lds_bytes<KPL>(kSweepWaves));
     return (e == hipSuccess && n > 0) ? n : 1;
 }
 
-int sweep_device_w_blocks_per_cu(int kpl) {
+int sweep_device_w_blocks_per_cu(int kpl, bool asym) {
     switch (kpl) {
-        case 1: return sweep_device_w_blocks_kpl<1>();
-        case 2: return sweep_device_w_blocks_kpl<2>();
-        case 4: return sweep_device_w_blocks_kpl<4>();
-        case 8: return sweep_device_w_blocks_kpl<8>();
-        case 16: return sweep_device_w_blocks_kpl<16>();
-        case 32: return sweep_device_w_blocks_kpl<32>();
+        case 1: return asym ? sweep_device_w_blocks_kpl<1, true>() : sweep_device_w_blocks_kpl<1, false>();
+        case 2: return asym ? sweep_device_w_blocks_kpl<2, true>() : sweep_device_w_blocks_kpl<2, false>();
+        case 4: return asym ? sweep_device_w_blocks_kpl<4, true>() : sweep_device_w_blocks_kpl<4, false>();
+        case 8: return asym ? sweep_device_w_blocks_kpl<8, true>() : sweep_device_w_blocks_kpl<8, false>();
+        case 16: return asym ? sweep_device_w_blocks_kpl<16, true>() : sweep_device_w_blocks_kpl<16, false>();
+        case 32: return asym ? sweep_device_w_blocks_kpl<32, true>() : sweep_device_w_blocks_kpl<32, false>();
         default: return 1;
     }
 }

@@ -94,6 +94,13 @@ struct lda_ctx {
     std::vector<float> wt;
     double weighted_n = 0.0;
     DevBuf<float> d_wt, d_docw, d_rowpart;
+    // word priors (set_word_prior, LDAModel.hpp:724-737): prior_row[v] = row
+    // of word v's eta column in prior_eta (rows of Kp, lane layout) or -1;
+    // per-topic sums of eta_kv in float (sampler) and double (FP64 LL)
+    bool has_priors = false;
+    DevBuf<int32_t> d_prior_row;
+    DevBuf<float> d_prior_eta, d_etasum;
+    DevBuf<double> d_etasum_d;
 
     DevBuf<float> d_alpha, d_alpha_cdf;
     DevBuf<int32_t> d_nkw, d_nk, d_nk_next, d_ndk, d_doclen, d_nk_priv, d_nkw_loc;

@@ -42,6 +42,12 @@ struct SweepParams {
     const uint64_t* coff;       // det: (W+1) x D chunk offsets
     int W, round;
     const float* wt;            // term-weighted models: token weights (counts are float bits)
+    // word priors (NULL prior_row: symmetric eta): prior_row[v] is the row of
+    // word v's eta column in prior_eta (rows of Kp, lane layout) or -1;
+    // etasum = sum_v eta_kv per topic (Kp, lane layout, 1 in padded topics)
+    const int32_t* prior_row;
+    const float* prior_eta;
+    const float* etasum;
 };
 
 // One fold-in job: a document's tokens [b, e) in the token arena, the
@@ -98,7 +104,7 @@ struct InferParams {
 // Term-weighted (IDF / PMI) training: the count buffers hold float32 values.
 hipError_t launch_sweep_device_w(int kpl, const SweepParams& p, int grid, hipStream_t s);
 hipError_t launch_sweep_det_w(int kpl, const SweepParams& p, hipStream_t s);
-int sweep_device_w_blocks_per_cu(int kpl);
+int sweep_device_w_blocks_per_cu(int kpl, bool asym);
 // n_kw = max(n_kw, 0), n_k = rowwise sum of n_kw, in float (LDAModel.hpp:565-569)
 // Words per block of the weighted merge's two-level row sum (n_k from n_kw).
 constexpr int kRowsumBlock = 256;
@@ -106,13 +112,16 @@ constexpr int kRowsumBlock = 256;
 hipError_t launch_clamp_rowsum_f(float* nkw, int V, int Kp, float* part, float* nk, hipStream_t s);
 hipError_t launch_loglik_f(const float* ndk, const float* docw, int64_t D, const float* nkw, const float* nk,
                            const float* alpha, int V, int Kp, double asum, double eta, double veta, double* partial,
-                           int nblk, hipStream_t s);
+                           int nblk, const int32_t* prior_row, const float* prior_eta, const double* etasum_d,
+                           hipStream_t s);
 hipError_t launch_alpha_sums_f(const float* ndk, const float* docw, int64_t D, const float* alpha, int Kp, float asum,
                                double* part, int nblk, hipStream_t s);
 
 // kernels (lda_device.hip)
+// z ~ Discrete(alpha), or with word priors (prior_row set) z ~ the word's eta column
 hipError_t launch_init_topics(const uint32_t* words, uint16_t* z, uint64_t N, const float* alpha_cdf, int K, int V,
-                              uint32_t seed_lo, uint32_t seed_hi, hipStream_t s);
+                              uint32_t seed_lo, uint32_t seed_hi, const int32_t* prior_row, const float* prior_eta,
+                              int kpl, int Kp, float eta, hipStream_t s);
 hipError_t launch_build_counts(int kpl, const uint32_t* words, const uint16_t* z, const uint64_t* off, int64_t D,
                                int32_t* ndk, int32_t* nkw, int32_t* nk, int Kp, int V, hipStream_t s);
 hipError_t launch_sweep_device(int kpl, const SweepParams& p, int grid, hipStream_t s);
@@ -121,15 +130,17 @@ hipError_t launch_sweep_det(int kpl, const SweepParams& p, hipStream_t s);
 hipError_t launch_sweep_copy(int kpl, const SweepParams& p, hipStream_t s);
 hipError_t launch_merge_copies(int32_t* nkw, const int32_t* loc, int64_t n, int W, hipStream_t s);
 hipError_t launch_replicate(int32_t* dst, const int32_t* src, int64_t n, int W, hipStream_t s);
-int sweep_device_blocks_per_cu(int kpl);
+int sweep_device_blocks_per_cu(int kpl, bool asym);
 hipError_t launch_copy_i32(int32_t* dst, const int32_t* src, int64_t n, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* dst, uint32_t v, int64_t n, hipStream_t s);
 hipError_t launch_rowsum_nkw(const int32_t* nkw, int V, int Kp, int32_t* nk, hipStream_t s);
 
 // FP64 log-likelihood partial sums: out[0..nblk) docs part, out[nblk..2nblk) rest part.
+// prior_row / prior_eta / etasum_d: word priors (NULL: symmetric eta).
 hipError_t launch_loglik(const int32_t* ndk, const int32_t* doclen, int64_t D, const int32_t* nkw, const int32_t* nk,
                          const float* alpha, int V, int Kp, double asum, double eta, double veta, double* partial,
-                         int nblk, hipStream_t s);
+                         int nblk, const int32_t* prior_row, const float* prior_eta, const double* etasum_d,
+                         hipStream_t s);
 
 // alpha fixed point: per-topic sums of digammaApprox(n_dk + alpha_k) and the
 // denominator sum of digammaApprox(L_d + asum); part[b*(Kp+1) + pos].

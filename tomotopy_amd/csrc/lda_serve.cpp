@@ -412,6 +412,7 @@ int lda_serve_start(lda_ctx* ctx, uint64_t hbm_budget_bytes, int iterations, int
     if (!ctx || !out) return fail(LDA_ERR_INVALID_ARGUMENT, "NULL argument");
     *out = nullptr;
     if (ctx->tw != LDA_TW_ONE) return fail(LDA_ERR_UNSUPPORTED, "fold-in of term-weighted models is not on the device yet");
+    if (ctx->has_priors) return fail(LDA_ERR_UNSUPPORTED, "fold-in with word priors is not on the device yet");
     if (iterations < 0) return fail(LDA_ERR_INVALID_ARGUMENT, "iterations must be >= 0");
     if (step_iters < 1 || step_tokens < 1) return fail(LDA_ERR_INVALID_ARGUMENT, "step_iters and step_tokens must be >= 1");
     if (lanes < 1 || lanes > kMaxLanes) return fail(LDA_ERR_INVALID_ARGUMENT, "lanes must be in 1..%d", kMaxLanes);

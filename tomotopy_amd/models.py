@@ -138,6 +138,7 @@ transform : Callable[dict, dict]
         self._ctx = None
         self._real_v = 0
         self._weights = None
+        self._eta_by_word = {}  # word -> K priors (setWordPrior keeps the first one set)
         self._cache = {}
         if corpus is not None:
             self.add_corpus(corpus, transform)
@@ -225,11 +226,62 @@ transform : Callable[dict, dict]
         if self._tw != TermWeight.ONE:
             self._weights = _corpus.term_weights(words, off, self._real_v, self._tw.name.lower())
         self._ctx = _native.Context(self._k, self._real_v, self._alphas, self._eta, self._seed, tw=int(self._tw))
+        self._upload_priors(self._ctx)
         self._ctx.upload_corpus(words, off, weights=self._weights)
         self._ctx.set_optim(self._optim_interval, self._burn_in)
         self._ctx.global_step = self._global_step
         self._prepared = True
         self._cache = {}
+
+    def _upload_priors(self, ctx):
+        """prepareWordPriors (LDAModel.hpp:724-737): the prior columns of the
+        words that survived preparation (ids < realV; the others are skipped)."""
+        if not self._eta_by_word:
+            return
+        words = [w for w in self._eta_by_word if w in self._dict]
+        ids = np.array([self._dict[w] for w in words], dtype=np.uint32)
+        pri = np.array([self._eta_by_word[w] for w in words], dtype=np.float32).reshape(len(ids), self._k)
+        ctx.set_word_priors(ids, pri)
+
+    def set_word_prior(self, word, prior):
+        '''Set the topic prior of `word` (a list of K floats, or a dict {topic: prior} with eta elsewhere);
+call before `train` (py_LDA.cpp:241-269, LDAModel.hpp:1031-1045).'''
+        if self._prepared:
+            raise RuntimeError("cannot set_word_prior() after train()")
+        if isinstance(prior, dict):
+            pr = [self._eta] * self._k
+            for k, v in prior.items():
+                if not isinstance(k, (int, np.integer)) or not 0 <= int(k) < self._k:
+                    raise ValueError("`prior` must be a dict of {topic_id: float}")
+                try:
+                    pr[int(k)] = float(v)
+                except (TypeError, ValueError):
+                    raise ValueError("`prior` must be a dict of {topic_id: float}")
+        else:
+            try:
+                pr = [float(x) for x in prior]
+            except (TypeError, ValueError):
+                raise ValueError("`prior` must be a list of floats with len = k")
+        if len(pr) != self._k:
+            raise ValueError("priors.size() must be equal to K.")
+        if any(not (x >= 0) for x in pr):
+            raise ValueError("priors must not be less than 0.")
+        if not isinstance(word, str):
+            raise ValueError("`word` must be str")
+        self._word_id(word, True)
+        self._eta_by_word.setdefault(word, [float(np.float32(x)) for x in pr])
+
+    def get_word_prior(self, word):
+        '''The topic prior of `word` (LDAModel.hpp:1047-1062): after `train` the word's eta column
+(its prior, or eta everywhere) when any prior is set; an empty list otherwise.'''
+        if self._prepared and self._eta_by_word:
+            i = self._dict.get(word)
+            if i is None or i >= self._real_v:
+                return []
+            pr = self._eta_by_word.get(word)
+            return list(pr) if pr is not None else [self._eta] * self._k
+        pr = self._eta_by_word.get(word)
+        return list(pr) if pr is not None else []
 
     def _ensure_prepared(self):
         if not self._prepared:
@@ -368,6 +420,8 @@ transform : Callable[dict, dict]
         if self._tw != TermWeight.ONE:
             raise NotImplementedError("infer() of TermWeight.%s models is not implemented on the device"
                                       % self._tw.name)
+        if self._eta_by_word:
+            raise NotImplementedError("infer() with word priors is not implemented on the device")
         if not docs:
             return [], []
         lens = [len(d._word_ids) for d in docs]
@@ -531,7 +585,8 @@ transform : Callable[dict, dict]
             "vocab_df": np.asarray(self._df, np.uint64), "real_v": self._real_v,
             "global_step": self.global_step, "extra": pickle.dumps(self._plain_init_params()),
             "vocab_weights": self._vocab_weights, "alpha": self._alpha0,
-            "alphas": np.asarray(self._alphas, np.float32), "eta": self._eta, "K": self._k, "eta_by_word": {},
+            "alphas": np.asarray(self._alphas, np.float32), "eta": self._eta, "K": self._k,
+            "eta_by_word": dict(self._eta_by_word),
             "burn_in": self._burn_in, "optim_interval": self._optim_interval,
             "nk": self._state("nk"), "nkw": self._state("nkw"), "docs": None,
         }
@@ -631,7 +686,9 @@ trained further); otherwise only the topic parameters (inference only).'''
             if len(mdl._weights) != len(words):
                 raise _serialize.FormatError("term-weighted documents without word weights")
         z = np.where(words < V, z, 0).astype(np.uint16)
+        mdl._eta_by_word = {w: [float(x) for x in pr] for w, pr in st["eta_by_word"].items()}
         mdl._ctx = _native.Context(K, V, mdl._alphas, mdl._eta, mdl._seed, tw=int(mdl._tw))
+        mdl._upload_priors(mdl._ctx)
         mdl._ctx.upload_corpus(words, off, z, weights=mdl._weights)
         if mdl._tw == TermWeight.ONE:
             mdl._ctx.set_counts(st["nkw"], st["nk"])
@@ -656,6 +713,7 @@ trained further); otherwise only the topic parameters (inference only).'''
         if self._prepared:
             z = self._state("z")
             new._ctx = _native.Context(self._k, self._real_v, self._alphas, self._eta, self._seed, tw=int(self._tw))
+            new._upload_priors(new._ctx)
             new._ctx.upload_corpus(self._words, self._off, z, weights=self._weights)
             new._ctx.set_optim(self._optim_interval, self._burn_in)
             new._ctx.global_step = self._global_step
