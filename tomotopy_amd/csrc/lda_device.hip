@@ -164,7 +164,7 @@ constexpr int kpl_shift() {
 //   col  n_kw[., v] of the current word, including this wavefront's updates
 // Per workgroup (the worker; its wavefronts share one private n_k):
 //   nk   the worker-private n_k (LDS atomics)
-//   y    RN(1 / (float(n_k) + V*eta)), rewritten after each n_k update
+//   bf   float(n_k) + V*eta and y = RN(1 / bf), rewritten after each n_k update
 // A topic update is one owner-lane write per array; the per-token reads are
 // conflict-free 16-byte ds_read per lane.
 struct WaveLds {
@@ -172,6 +172,7 @@ struct WaveLds {
     int* col;
     int* nk;
     float* y;
+    float* bf;
 };
 
 extern __shared__ int lds_dyn[];
@@ -179,13 +180,14 @@ extern __shared__ int lds_dyn[];
 template <int KPL>
 __device__ __forceinline__ WaveLds wave_lds() {
     constexpr int Kp = 64 * KPL;
-    int* w = lds_dyn + 2 * Kp + (threadIdx.x >> 6) * (2 * Kp);
-    return WaveLds{reinterpret_cast<float*>(w), w + Kp, lds_dyn, reinterpret_cast<float*>(lds_dyn + Kp)};
+    int* w = lds_dyn + 3 * Kp + (threadIdx.x >> 6) * (2 * Kp);
+    return WaveLds{reinterpret_cast<float*>(w), w + Kp, lds_dyn, reinterpret_cast<float*>(lds_dyn + Kp),
+                   reinterpret_cast<float*>(lds_dyn + 2 * Kp)};
 }
 
 // dynamic LDS bytes of a workgroup with `waves` wavefronts
 template <int KPL>
-constexpr size_t wave_lds_bytes(int waves = 1) { return (size_t)(2 + 2 * waves) * 64 * KPL * sizeof(int); }
+constexpr size_t wave_lds_bytes(int waves = 1) { return (size_t)(3 + 2 * waves) * 64 * KPL * sizeof(int); }
 
 template <int KPL, typename V>
 __device__ __forceinline__ void lds_load(const int* a, int lane, V& o) { load_kvec<KPL>(a, lane, o); }
@@ -216,7 +218,9 @@ __device__ __forceinline__ void bump(const WaveLds& S, const fvec<KPL>& al, int 
         S.A[q] = (float)d + a_sl;
         S.col[q] += delta;
         const int n = atomicAdd(S.nk + q, delta) + delta;
-        S.y[q] = 1.0f / ((float)n + et.s_at(q));
+        const float b = (float)n + et.s_at(q);
+        S.bf[q] = b;
+        S.y[q] = 1.0f / b;
     }
 }
 
@@ -295,11 +299,15 @@ __device__ __forceinline__ typename EtaOf<KPL, ASYM>::type make_eta(const SweepP
 // Private n_k from a K-vector: n_k and the reciprocals (one wavefront writes).
 template <int KPL, class E>
 __device__ __forceinline__ void set_nk_vec(const WaveLds& S, const ivec<KPL>& n, int lane, const E& et) {
-    fvec<KPL> y;
+    fvec<KPL> y, b;
 #pragma unroll
-    for (int j = 0; j < KPL; ++j) y[j] = 1.0f / ((float)n[j] + et.s(j));
+    for (int j = 0; j < KPL; ++j) {
+        b[j] = (float)n[j] + et.s(j);
+        y[j] = 1.0f / b[j];
+    }
     lds_store<KPL>(S.nk, lane, n);
     lds_store_f<KPL>(S.y, lane, y);
+    lds_store_f<KPL>(S.bf, lane, b);
 }
 
 template <int KPL, class E>
@@ -353,22 +361,22 @@ template <int KPL, class E>
 __device__ __forceinline__ int token_step(const WaveLds& S, const fvec<KPL>& al, int lane, int zo, float u, int K,
                                           int nvalid, int lastLane, const E& et, float* total) {
     bump<KPL>(S, al, lane, zo, -1, et);
-    fvec<KPL> A, y;
-    ivec<KPL> col, nk;
+    fvec<KPL> A, y, bf, B;
+    ivec<KPL> col;
     lds_load_f<KPL>(S.A, lane, A);
     lds_load<KPL>(S.col, lane, col);
-    lds_load<KPL>(S.nk, lane, nk);
+#pragma unroll
+    for (int j = 0; j < KPL; ++j) B[j] = (float)col[j] + et.e(j);
+    lds_load_f<KPL>(S.bf, lane, bf);
     lds_load_f<KPL>(S.y, lane, y);
     fvec<KPL> w;
 #pragma unroll
     for (int j = 0; j < KPL; ++j) {
         // ((n_dk + alpha) * (n_kw + eta)) / (n_k + V*eta); the quotient as
         // q0 = a*y, r = fma(-q0, b, a), q = fma(r, y, q0) with y = RN(1/b)
-        float B = (float)col[j] + et.e(j);
-        float a = A[j] * B;
-        float bf = (float)nk[j] + et.s(j);
+        float a = A[j] * B[j];
         float q0 = a * y[j];
-        float rr = __builtin_fmaf(-q0, bf, a);
+        float rr = __builtin_fmaf(-q0, bf[j], a);
         w[j] = __builtin_fmaf(rr, y[j], q0);
     }
     float tot;
@@ -448,7 +456,7 @@ __device__ __forceinline__ void sample_range(const WaveLds& S, const fvec<KPL>& 
 // partition workers keep theirs, LDAModel.hpp:484, 596-602); documents are
 // taken longest-first from a global counter; n_kw is updated with atomics;
 // each worker adds its n_k delta to nk_out when it runs out of documents.
-constexpr int kSweepWaves = 4;  // wavefronts per workgroup of the device-wide sweep (one worker)
+constexpr int kSweepWaves = 8;  // wavefronts per workgroup of the device-wide sweep (one worker)
 
 template <int KPL, bool ASYM>
 __global__ void __launch_bounds__(64 * kSweepWaves) sweep_device_kernel(SweepParams p) {
