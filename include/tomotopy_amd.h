@@ -160,6 +160,12 @@ int lda_download_state_f(lda_ctx* ctx, uint16_t* z, float* ndk, float* nkw, floa
  * (ll_docs + ll_rest) / weightedN. */
 int lda_get_weighted_n(lda_ctx* ctx, double* weighted_n);
 
+/* Self-test of the device's reciprocal (no counterpart in the reference): the
+ * sampler takes RN(1/(n_k + V*eta)) as the hardware reciprocal refined by one
+ * Newton step; this counts, over all 2^23 mantissas of the binade
+ * [2^exp_shift, 2^(exp_shift+1)), where it differs from IEEE division. */
+int lda_selftest_reciprocal(int exp_shift, int device, uint64_t* mismatches);
+
 /* One sampling step per query token on the frozen current state: remove the
  * token's own z_old from (n_dk[doc], n_kw[:,word], n_k), weight, scan, and
  * search with the given uniform.  Counts are not modified. */

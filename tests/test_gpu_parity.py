@@ -178,3 +178,10 @@ def test_infer_graph_replay_and_batch_independence():
         b, e = int(uoff[d]), int(uoff[d + 1])
         th1, ll1, _ = ctx.infer(uw[b:e], np.array([0, e - b], np.uint64), 15)
         assert np.array_equal(th1[0], th_all[d]) and ll1[0] == ll_all[d]
+
+
+@pytest.mark.parametrize("exp_shift", [-20, -1, 0, 5, 10, 17, 24, 28])
+def test_device_reciprocal_is_correctly_rounded(exp_shift):
+    """The sampler's RN(1/b) (hardware reciprocal + one Newton step) equals
+    IEEE division for all 2^23 mantissas of the binade."""
+    assert _native.selftest_reciprocal(exp_shift) == 0

@@ -42,6 +42,7 @@ _SIGS = {
     "lda_set_counts": [_c_p, _c_p, _c_p],
     "lda_set_counts_f": [_c_p, _c_p, _c_p],
     "lda_set_word_priors": [_c_p, ctypes.c_int64, _c_p, _c_p],
+    "lda_selftest_reciprocal": [_i32, _i32, _c_p],
     "lda_set_alpha": [_c_p, _c_p],
     "lda_get_alpha": [_c_p, _c_p],
     "lda_set_optim": [_c_p, _u32, _u32],
@@ -354,3 +355,10 @@ class Server:
             self.stop()
         except Exception:
             pass
+
+
+def selftest_reciprocal(exp_shift, device=0):
+    """Mismatches of the device reciprocal against IEEE division over one binade."""
+    n = ctypes.c_uint64(0)
+    _check(load().lda_selftest_reciprocal(int(exp_shift), int(device), ctypes.byref(n)))
+    return int(n.value)

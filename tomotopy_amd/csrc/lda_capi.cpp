@@ -903,6 +903,23 @@ int lda_download_state_f(lda_ctx* c, uint16_t* z, float* ndk, float* nkw, float*
     return LDA_OK;
 }
 
+int lda_selftest_reciprocal(int exp_shift, int device, uint64_t* mismatches) {
+    if (!mismatches) return fail(LDA_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (exp_shift < -125 || exp_shift > 126) return fail(LDA_ERR_INVALID_ARGUMENT, "exp_shift must give a normal binade");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(LDA_ERR_DEVICE, "no HIP device is available");
+    if (device < 0 || device >= ndev) return fail(LDA_ERR_INVALID_ARGUMENT, "device %d out of range", device);
+    HIP_TRY(hipSetDevice(device));
+    DevBuf<unsigned long long> bad;
+    HIP_TRY(bad.alloc(1));
+    HIP_TRY(hipMemset(bad.p, 0, sizeof(unsigned long long)));
+    HIP_TRY(lda::launch_selftest_reciprocal(exp_shift, bad.p, nullptr));
+    unsigned long long h = 0;
+    HIP_TRY(hipMemcpy(&h, bad.p, sizeof(h), hipMemcpyDeviceToHost));
+    *mismatches = h;
+    return LDA_OK;
+}
+
 int lda_get_weighted_n(lda_ctx* c, double* weighted_n) {
     if (!c || !weighted_n) return fail(LDA_ERR_INVALID_ARGUMENT, "NULL argument");
     *weighted_n = c->tw == LDA_TW_ONE ? (double)c->n_valid : c->weighted_n;

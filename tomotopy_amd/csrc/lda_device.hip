@@ -47,6 +47,20 @@ __device__ __forceinline__ uint32_t philox_x0(uint32_t k0, uint32_t k1, uint32_t
 
 __device__ __forceinline__ float uniform24(uint32_t x) { return (float)(x >> 8) * 0x1p-24f; }
 
+// RN(1 / b) for normal b: the hardware reciprocal refined by one Newton step,
+// e = 1 - b*r0, r = r0 + e*r0 (both fused).  The refinement returns the
+// correctly rounded reciprocal for every float mantissa on gfx950 (all 2^23
+// mantissas, checked by lda_selftest_reciprocal), and the hardware
+// approximation depends on the mantissa only, so the result equals the IEEE
+// division 1.0f / b at a fraction of its instructions.  (The fold-in sampler
+// keeps the division: there the shorter sequence changes how the compiler
+// unrolls the prefetch ring, which costs more than it saves.)
+__device__ __forceinline__ float recip_rn(float b) {
+    const float r0 = __builtin_amdgcn_rcpf(b);
+    const float e = __builtin_fmaf(-b, r0, 1.0f);
+    return __builtin_fmaf(e, r0, r0);
+}
+
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63u); }
 
 template <int KPL>
@@ -220,7 +234,7 @@ __device__ __forceinline__ void bump(const WaveLds& S, const fvec<KPL>& al, int 
         const int n = atomicAdd(S.nk + q, delta) + delta;
         const float b = (float)n + et.s_at(q);
         S.bf[q] = b;
-        S.y[q] = 1.0f / b;
+        S.y[q] = recip_rn(b);
     }
 }
 
@@ -303,7 +317,7 @@ __device__ __forceinline__ void set_nk_vec(const WaveLds& S, const ivec<KPL>& n,
 #pragma unroll
     for (int j = 0; j < KPL; ++j) {
         b[j] = (float)n[j] + et.s(j);
-        y[j] = 1.0f / b[j];
+        y[j] = recip_rn(b[j]);
     }
     lds_store<KPL>(S.nk, lane, n);
     lds_store_f<KPL>(S.y, lane, y);
@@ -674,7 +688,7 @@ __device__ __forceinline__ void bump_w(const WaveLdsW& S, int lane, int z, float
             n = dec ? sub_clamp(S.nk[q], w) : S.nk[q] + w;
             S.nk[q] = n;
         }
-        S.y[q] = 1.0f / (n + et.s_at(q));
+        S.y[q] = recip_rn(n + et.s_at(q));
     }
 }
 
@@ -785,7 +799,7 @@ __global__ void __launch_bounds__(64 * kSweepWaves) sweep_device_w_kernel(SweepP
         fvec<KPL> n, y;
         load_kvec_f<KPL>(reinterpret_cast<const float*>(p.nk_in), lane, n);
 #pragma unroll
-        for (int j = 0; j < KPL; ++j) y[j] = 1.0f / (n[j] + et.s(j));
+        for (int j = 0; j < KPL; ++j) y[j] = recip_rn(n[j] + et.s(j));
         lds_store_f<KPL>(S.nk, lane, n);
         lds_store_f<KPL>(S.y, lane, y);
     }
@@ -824,7 +838,7 @@ __global__ void __launch_bounds__(64) sweep_det_w_kernel(SweepParams p) {
         fvec<KPL> n, y;
         load_kvec_f<KPL>(nkp, lane, n);
 #pragma unroll
-        for (int j = 0; j < KPL; ++j) y[j] = 1.0f / (n[j] + et.s(j));
+        for (int j = 0; j < KPL; ++j) y[j] = recip_rn(n[j] + et.s(j));
         lds_store_f<KPL>(S.nk, lane, n);
         lds_store_f<KPL>(S.y, lane, y);
     }
@@ -1892,6 +1906,18 @@ int sweep_device_w_blocks_per_cu(int kpl, bool asym) {
         case 32: return asym ? sweep_device_w_blocks_kpl<32, true>() : sweep_device_w_blocks_kpl<32, false>();
         default: return 1;
     }
+}
+
+__global__ void selftest_reciprocal_kernel(int exp_shift, unsigned long long* bad) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (1u << 23)) return;
+    const float b = __uint_as_float(((uint32_t)(127 + exp_shift) << 23) | i);
+    if (__float_as_uint(recip_rn(b)) != __float_as_uint(1.0f / b)) atomicAdd(bad, 1ull);
+}
+
+hipError_t launch_selftest_reciprocal(int exp_shift, unsigned long long* bad, hipStream_t s) {
+    hipLaunchKernelGGL(selftest_reciprocal_kernel, dim3((1u << 23) / 256), dim3(256), 0, s, exp_shift, bad);
+    return hipGetLastError();
 }
 
 hipError_t launch_clamp_rowsum_f(float* nkw, int V, int Kp, float* part, float* nk, hipStream_t s) {

@@ -119,6 +119,10 @@ hipError_t launch_alpha_sums_f(const float* ndk, const float* docw, int64_t D, c
 
 // kernels (lda_device.hip)
 // z ~ Discrete(alpha), or with word priors (prior_row set) z ~ the word's eta column
+// recip_rn against IEEE division over all 2^23 mantissas of the binade
+// [2^exp_shift, 2^(exp_shift+1)); *bad (zeroed by the caller) counts mismatches.
+hipError_t launch_selftest_reciprocal(int exp_shift, unsigned long long* bad, hipStream_t s);
+
 hipError_t launch_init_topics(const uint32_t* words, uint16_t* z, uint64_t N, const float* alpha_cdf, int K, int V,
                               uint32_t seed_lo, uint32_t seed_hi, const int32_t* prior_row, const float* prior_eta,
                               int kpl, int Kp, float eta, hipStream_t s);
