@@ -180,6 +180,16 @@ int lda_sample_frozen(lda_ctx* ctx, int64_t n, const int64_t* doc, const uint32_
 int lda_infer(lda_ctx* ctx, const uint32_t* words, const uint64_t* doc_off, int64_t D, int iterations,
               uint64_t infer_seed, int mode, float* theta, double* ll, uint16_t* z_out);
 
+/* Fold-in with together=True (LDAModel.hpp:852-892): the documents join a
+ * copy of the model's n_kw / n_k (initial topics from Discrete(alpha) keyed by
+ * infer_seed) and are sampled `iterations` times under the scheme and
+ * workers of lda_train (partition chunks from the model's word frequencies),
+ * with alpha fixed.  theta: D x K; *ll = getLLRest(copy) - getLLRest(model) +
+ * getLLDocs(documents), FP64 (one value for all documents, as tomoto);
+ * z_out: the documents' final topics (N, may be NULL). */
+int lda_infer_together(lda_ctx* ctx, const uint32_t* words, const uint64_t* doc_off, int64_t D, int iterations,
+                       uint64_t infer_seed, int workers, int scheme, float* theta, double* ll, uint16_t* z_out);
+
 /* Segmented fold-in (lda_infer and the server).  tomoto samples a document's
  * tokens one after the other (LDAModel.hpp:920-953), which bounds a long
  * document's latency by its length.  With seg_tokens > 0 a document of n >

@@ -313,6 +313,8 @@ struct Model {
     std::vector<float> wt, ndk_f, nkw_f, nk_f, docw;
     uint32_t global_step = 0;
     uint32_t optim_interval = 10, burn_in = 0;
+    // partition chunks from these word frequencies instead of the corpus's
+    std::vector<uint64_t> chunk_cf;
 };
 
 void add_token(Model& m, int64_t d, uint32_t v, int k, int inc, int32_t* nk_local) {
@@ -342,7 +344,8 @@ void sample_range(Model& m, int64_t d, uint64_t b, uint64_t e, int32_t* nk_local
 // reference can produce fewer than W) are padded with V (empty chunks).
 std::vector<uint32_t> vocab_chunks(const Model& m, int W) {
     std::vector<uint64_t> cf(m.V, 0);
-    for (uint32_t v : m.words) if (v < (uint32_t)m.V) ++cf[v];
+    if (!m.chunk_cf.empty()) cf = m.chunk_cf;  // the model's vocabCf (fold-in together)
+    else for (uint32_t v : m.words) if (v < (uint32_t)m.V) ++cf[v];
     uint64_t tot = 0;
     for (auto c : cf) tot += c;
     std::vector<uint32_t> ch;
@@ -898,6 +901,44 @@ void orc_get_state(void* h, uint16_t* z, int32_t* ndk, int32_t* nkw, int32_t* nk
 }
 
 void orc_loglik(void* h, int mode, double* ll_docs, double* ll_rest) { loglik(*(Model*)h, mode, ll_docs, ll_rest); }
+
+// Fold-in with together=True (LDAModel.hpp:852-892): the unseen documents
+// join a copy of the model's n_kw / n_k (their initial topics drawn from
+// Discrete(alpha), keyed by infer_seed), then `iters` sweeps of the scheme
+// (1 none, 2 copy_merge, 3 partition with W workers; partition chunks from the
+// model's word frequencies, as updatePartition uses vocabCf) run over them
+// with alpha fixed; ll = getLLRest(copy) - getLLRest(model) + getLLDocs(docs)
+// in FP64, theta = (n_dk + alpha) / (L + sum alpha).
+void orc_infer_together(void* h, const uint32_t* words, const uint64_t* off, int64_t D, int iters, uint64_t infer_seed,
+                        int workers, int scheme, int threads, float* theta, double* ll, uint16_t* z_out) {
+    const Model* m = (const Model*)h;
+    Model t;
+    t.K = m->K; t.V = m->V; t.kpl = m->kpl; t.scan_order = m->scan_order;
+    t.eta = m->eta; t.veta = m->veta; t.seed = infer_seed;
+    t.alpha = m->alpha;
+    t.nkw.assign((size_t)m->V * m->K, 0);
+    t.nk.assign(m->K, 0);
+    t.chunk_cf.assign(m->V, 0);
+    for (uint32_t v : m->words) if (v < (uint32_t)m->V) ++t.chunk_cf[v];
+    t.optim_interval = 0;
+    orc_set_corpus(&t, words, off, D, nullptr);
+    for (size_t i = 0; i < t.nkw.size(); ++i) t.nkw[i] += m->nkw[i];
+    for (int k = 0; k < m->K; ++k) t.nk[k] += m->nk[k];
+    orc_train_scheme(&t, iters, workers, threads, scheme);
+    double docs = 0, rest = 0, docs0 = 0, rest0 = 0;
+    loglik(t, 0, &docs, &rest);
+    loglik(*m, 0, &docs0, &rest0);
+    *ll = rest - rest0 + docs;
+    if (z_out) std::copy(t.z.begin(), t.z.end(), z_out);
+    float asum = 0.0f;
+    for (int k = 0; k < m->K; ++k) asum += m->alpha[k];
+    for (int64_t d = 0; d < D; ++d) {
+        int32_t L = 0;
+        for (int k = 0; k < m->K; ++k) L += t.ndk[(size_t)d * m->K + k];
+        for (int k = 0; k < m->K; ++k)
+            theta[(size_t)d * m->K + k] = ((float)t.ndk[(size_t)d * m->K + k] + m->alpha[k]) / ((float)L + asum);
+    }
+}
 
 // Term-weighted corpus (TermWeight IDF / PMI): weights[g] is token g's weight
 // (ignored for ids >= V).  Initial topics as in orc_set_corpus; the float

@@ -67,6 +67,7 @@ def load():
         "orc_infer_ll_bruteforce": (f64, [_c_p, _c_p, u64, _c_p]),
         "orc_set_corpus_w": (None, [_c_p, _c_p, _c_p, i64, _c_p, _c_p]),
         "orc_set_priors": (None, [_c_p, _c_p]),
+        "orc_infer_together": (None, [_c_p, _c_p, _c_p, i64, i32, u64, i32, i32, i32, _c_p, _c_p, _c_p]),
         "orc_sample_from_discrete": (i32, [_c_p, i32, f32]),
         "orc_train_w": (None, [_c_p, i32, i32, i32, i32]),
         "orc_optimize_alpha_w": (None, [_c_p]),
@@ -258,6 +259,18 @@ class Model:
         load().orc_infer_seg(self._h, _p(words), _p(off), D, iters, int(infer_seed) & (2**64 - 1), mode,
                              int(seg_tokens), int(seg_max), _p(theta), _p(ll), _p(z))
         return theta, ll, z
+
+    def infer_together(self, words, off, iters, infer_seed=0, workers=1, scheme=1, threads=1):
+        """Fold-in with together=True: returns (theta, ll, z)."""
+        words = np.ascontiguousarray(words, np.uint32)
+        off = np.ascontiguousarray(off, np.uint64)
+        D = len(off) - 1
+        theta = np.zeros((D, self.K), np.float32)
+        z = np.zeros(int(off[-1]), np.uint16)
+        ll = ctypes.c_double()
+        load().orc_infer_together(self._h, _p(words), _p(off), D, iters, int(infer_seed) & (2**64 - 1), workers,
+                                  scheme, threads, _p(theta), ctypes.byref(ll), _p(z))
+        return theta, ll.value, z
 
     def infer_ll_bruteforce(self, words, z):
         words = np.ascontiguousarray(words, np.uint32)

@@ -463,3 +463,47 @@ def test_segmented_infer_same_ll_per_word_as_sequential(small_corpus):
     a, b = ll_seq.sum() / n, ll_seg.sum() / n
     print("ll_per_word sequential %.5f segmented %.5f" % (a, b))
     assert abs(a - b) <= 0.01 * abs(a)
+
+
+def test_infer_together_closed_forms():
+    """together=True: from the returned topics, theta = (n_dk+a)/(L+sum a) and
+    ll = getLLRest(model + docs) - getLLRest(model) + getLLDocs(docs) by scipy."""
+    from scipy.special import gammaln
+    words, off, nw, _ = synth.workload("plain_k20", n_docs=60)
+    ws, offs, V, order = corpus.prepare_arrays(words, off, nw)
+    K = 9
+    alpha = (np.arange(K) * 0.01 + 0.05).astype(np.float32)
+    m = O.Model(K, V, alpha, 0.02, 3)
+    m.set_optim(0, 0)
+    m.set_corpus(ws, offs)
+    m.train(5, workers=2)
+    _, _, nkw0, nk0 = m.state()
+    uw, uoff = synth.unseen_docs(8, 12, V, 20, np.log(60.0), 1.0, 5, 300, zipf_s=None)
+    uw, uoff = corpus.csr_from_arrays(uw, uoff, np.arange(V))
+    for scheme, workers in [(1, 1), (3, 3), (2, 2)]:
+        theta, ll, z = m.infer_together(uw, uoff, 7, infer_seed=11, workers=workers, scheme=scheme)
+        D = len(uoff) - 1
+        ndk = np.zeros((D, K), np.int64)
+        nkw = nkw0.astype(np.int64).copy()
+        for d in range(D):
+            for g in range(int(uoff[d]), int(uoff[d + 1])):
+                if uw[g] < V:
+                    ndk[d, z[g]] += 1
+                    nkw[uw[g], z[g]] += 1
+        nk = nkw.sum(axis=0)
+        assert np.array_equal(nk0 + ndk.sum(axis=0), nk)
+        eta = float(np.float32(0.02))
+        a = alpha.astype(np.float64)
+
+        def rest(nkw_, nk_):
+            return K * gammaln(V * eta) - np.sum(gammaln(nk_ + V * eta)) + \
+                np.sum(np.where(nkw_ != 0, gammaln(nkw_ + eta) - gammaln(eta), 0.0))
+        L = ndk.sum(axis=1)
+        docs = np.sum(gammaln(a.sum()) - gammaln(L + a.sum())) + np.sum(gammaln(ndk + a) - gammaln(a))
+        ref = rest(nkw, nk) - rest(nkw0.astype(np.int64), nk0.astype(np.int64)) + docs
+        assert ll == pytest.approx(ref, rel=1e-9)
+        asum = np.float32(0)
+        for x in alpha:
+            asum = np.float32(asum + x)
+        th = ((ndk.astype(np.float32) + alpha) / (L.astype(np.float32)[:, None] + asum)).astype(np.float32)
+        assert np.array_equal(theta, th)

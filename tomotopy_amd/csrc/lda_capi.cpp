@@ -171,8 +171,10 @@ int upload_alpha(lda_ctx* c) {
 int prepare_partition(lda_ctx* c, int W) {
     if (c->part_W == W && c->d_coff.p) return LDA_OK;
     std::vector<uint64_t> cf(c->V, 0);
-    for (uint32_t v : c->words)
-        if (v < (uint32_t)c->V) ++cf[v];
+    if (!c->chunk_cf.empty()) cf = c->chunk_cf;
+    else
+        for (uint32_t v : c->words)
+            if (v < (uint32_t)c->V) ++cf[v];
     uint64_t tot = 0;
     for (auto x : cf) tot += x;
     std::vector<uint32_t> ch;
@@ -819,6 +821,56 @@ int lda_loglik(lda_ctx* c, double* ll_docs, double* ll_rest) {
     }
     *ll_docs = a;
     *ll_rest = b;
+    return LDA_OK;
+}
+
+int lda_infer_together(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off, int64_t D, int iterations,
+                       uint64_t infer_seed, int workers, int scheme, float* theta, double* ll, uint16_t* z_out) {
+    if (!c || !ll || D < 0 || (D && (!doc_off || !theta))) return fail(LDA_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (c->tw != LDA_TW_ONE) return fail(LDA_ERR_UNSUPPORTED, "fold-in of term-weighted models is not on the device yet");
+    if (c->has_priors) return fail(LDA_ERR_UNSUPPORTED, "fold-in with word priors is not on the device yet");
+    if (iterations < 0) return fail(LDA_ERR_INVALID_ARGUMENT, "iterations must be >= 0");
+    // the documents join a copy of the model: a context of their own whose
+    // counts are the documents' plus the model's, sampled with alpha fixed
+    lda_ctx* t = nullptr;
+    int rc = lda_ctx_create(c->K, c->V, c->alpha.data(), c->eta, LDA_TW_ONE, infer_seed, c->device, &t);
+    if (rc) return rc;
+    struct Guard {
+        lda_ctx* t;
+        ~Guard() { lda_ctx_destroy(t); }
+    } guard{t};
+    t->chunk_cf.assign(c->V, 0);
+    for (uint32_t v : c->words)
+        if (v < (uint32_t)c->V) ++t->chunk_cf[v];
+    std::vector<uint64_t> off0(1, 0);
+    rc = lda_upload_corpus(t, words, D ? doc_off : off0.data(), D, nullptr, nullptr);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(lda::launch_add_i32(t->d_nkw.p, c->d_nkw.p, (int64_t)c->V * c->Kp, t->stream));
+    HIP_TRY(lda::launch_add_i32(t->d_nk.p, c->d_nk.p, c->Kp, t->stream));
+    HIP_TRY(hipStreamSynchronize(t->stream));
+    t->optim_interval = 0;
+    rc = lda_train(t, iterations, workers, scheme);
+    if (rc) return rc;
+    double docs_t = 0, rest_t = 0, docs_c = 0, rest_c = 0;
+    rc = lda_loglik(t, &docs_t, &rest_t);
+    if (rc) return rc;
+    rc = lda_loglik(c, &docs_c, &rest_c);
+    if (rc) return rc;
+    *ll = rest_t - rest_c + docs_t;
+    // theta = (n_dk + alpha) / (L + sum alpha) (LDAModel.hpp:1145-1159)
+    std::vector<int32_t> ndk((size_t)std::max<int64_t>(D, 1) * c->K);
+    rc = lda_download_state(t, z_out, ndk.data(), nullptr, nullptr);
+    if (rc) return rc;
+    float asum = 0.0f;
+    for (int k = 0; k < c->K; ++k) asum += c->alpha[k];
+    for (int64_t d = 0; d < D; ++d) {
+        int32_t L = 0;
+        for (int k = 0; k < c->K; ++k) L += ndk[(size_t)d * c->K + k];
+        for (int k = 0; k < c->K; ++k)
+            theta[(size_t)d * c->K + k] = ((float)ndk[(size_t)d * c->K + k] + c->alpha[k]) / ((float)L + asum);
+    }
     return LDA_OK;
 }
 

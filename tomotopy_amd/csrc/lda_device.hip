@@ -981,6 +981,9 @@ __global__ void build_counts_kernel(const uint32_t* __restrict__ words, const ui
 __global__ void copy_i32_kernel(int32_t* dst, const int32_t* src, int64_t n) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) dst[i] = src[i];
 }
+__global__ void add_i32_kernel(int32_t* dst, const int32_t* src, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) dst[i] += src[i];
+}
 __global__ void fill_u32_kernel(uint32_t* dst, uint32_t v, int64_t n) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) dst[i] = v;
 }
@@ -1829,6 +1832,14 @@ hipError_t launch_copy_i32(int32_t* dst, const int32_t* src, int64_t n, hipStrea
     int64_t blocks = (n + 255) / 256;
     if (blocks > 65536) blocks = 65536;
     hipLaunchKernelGGL(copy_i32_kernel, dim3((unsigned)blocks), dim3(256), 0, s, dst, src, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_add_i32(int32_t* dst, const int32_t* src, int64_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(add_i32_kernel, dim3((unsigned)blocks), dim3(256), 0, s, dst, src, n);
     return hipGetLastError();
 }
 

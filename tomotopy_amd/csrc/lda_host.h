@@ -98,6 +98,9 @@ struct lda_ctx {
     // of word v's eta column in prior_eta (rows of Kp, lane layout) or -1;
     // per-topic sums of eta_kv in float (sampler) and double (FP64 LL)
     bool has_priors = false;
+    // partition chunks from these word frequencies instead of the corpus's
+    // (fold-in together: the model's vocabCf, LDAModel.hpp:441-449)
+    std::vector<uint64_t> chunk_cf;
     DevBuf<int32_t> d_prior_row;
     DevBuf<float> d_prior_eta, d_etasum;
     DevBuf<double> d_etasum_d;

@@ -137,6 +137,7 @@ hipError_t launch_replicate(int32_t* dst, const int32_t* src, int64_t n, int W, 
 int sweep_device_blocks_per_cu(int kpl, bool asym);
 hipError_t launch_copy_i32(int32_t* dst, const int32_t* src, int64_t n, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* dst, uint32_t v, int64_t n, hipStream_t s);
+hipError_t launch_add_i32(int32_t* dst, const int32_t* src, int64_t n, hipStream_t s);
 hipError_t launch_rowsum_nkw(const int32_t* nkw, int V, int Kp, int32_t* nk, hipStream_t s);
 
 // FP64 log-likelihood partial sums: out[0..nblk) docs part, out[nblk..2nblk) rest part.

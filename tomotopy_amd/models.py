@@ -392,18 +392,17 @@ call before `train` (py_LDA.cpp:241-269, LDAModel.hpp:1031-1045).'''
         '''Topic distributions of unseen documents (py_LDA.cpp:345-411, LDAModel.hpp:843-954).'''
         if not self._prepared:
             raise RuntimeError("cannot infer with untrained model")
-        if together:
-            raise NotImplementedError("infer(together=True) is not implemented on the device")
+        kw = {"together": bool(together), "workers": int(workers), "parallel": int(parallel)}
         if isinstance(doc, Corpus):
             docs = [self.make_doc(d) for d in doc]
-            _, ll = self._infer_docs(docs, iterations)
+            _, ll = self._infer_docs(docs, iterations, **kw)
             return InferredCorpus(docs), ll
         if isinstance(doc, Document):
             if doc._model is not self:
                 raise ValueError("`doc` was from another model, not fit to this model")
             if doc._doc_index is not None:
                 return doc.get_topic_dist(), None
-            dists, ll = self._infer_docs([doc], iterations)
+            dists, ll = self._infer_docs([doc], iterations, **kw)
             return dists[0], ll[0]
         try:
             docs = list(doc)
@@ -414,16 +413,21 @@ call before `train` (py_LDA.cpp:241-269, LDAModel.hpp:1031-1045).'''
                 raise ValueError("`doc` must be tomotopy_amd.Document type or list of tomotopy_amd.Document")
             if d._model is not self:
                 raise ValueError("`doc` was from another model, not fit to this model")
-        return self._infer_docs(docs, iterations)
+        dists, ll = self._infer_docs(docs, iterations, **kw)
+        return (dists, ll[0]) if together else (dists, ll)
 
-    def _infer_docs(self, docs, iterations, mode=0):
+    def _infer_docs(self, docs, iterations, mode=0, together=False, workers=0, parallel=0):
+        """Fold-in on the device.  together=False: every document against the
+        frozen model (one batched launch); together=True: the documents sampled
+        jointly on a copy of the model, one log-likelihood for all of them
+        (LDAModel.hpp:852-892)."""
         if self._tw != TermWeight.ONE:
             raise NotImplementedError("infer() of TermWeight.%s models is not implemented on the device"
                                       % self._tw.name)
         if self._eta_by_word:
             raise NotImplementedError("infer() with word priors is not implemented on the device")
         if not docs:
-            return [], []
+            return [], ([0.0] if together else [])
         lens = [len(d._word_ids) for d in docs]
         off = np.zeros(len(docs) + 1, dtype=np.uint64)
         off[1:] = np.cumsum(lens)
@@ -433,7 +437,12 @@ call before `train` (py_LDA.cpp:241-269, LDAModel.hpp:1031-1045).'''
             perm = np.argsort(d._word_ids, kind="stable")
             words[int(off[i]):int(off[i + 1])] = d._word_ids[perm]
             perms.append(perm)
-        theta, ll, z = self._ctx.infer(words, off, iterations, infer_seed=0, mode=mode, want_z=True)
+        if together:
+            theta, ll_all, z = self._ctx.infer_together(words, off, iterations, infer_seed=0, workers=workers,
+                                                        scheme=parallel)
+            ll = np.array([ll_all])
+        else:
+            theta, ll, z = self._ctx.infer(words, off, iterations, infer_seed=0, mode=mode, want_z=True)
         out = []
         for i, d in enumerate(docs):
             b, e = int(off[i]), int(off[i + 1])
@@ -444,7 +453,7 @@ call before `train` (py_LDA.cpp:241-269, LDAModel.hpp:1031-1045).'''
             d._z = orig
             d._ndk = np.bincount(zz[zz >= 0], minlength=self._k)
             d._theta = theta[i]
-            d._ll = ll[i]
+            d._ll = ll[0] if together else ll[i]
             out.append(theta[i].tolist())
         return out, ll.tolist()
 
