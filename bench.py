@@ -93,7 +93,7 @@ def cpu_baseline(ws, offs, V, K, alpha, eta, budget_s=12.0):
     while True:
         m.train(1, workers=threads, threads=threads)
         sweeps += 1
-        if time.perf_counter() - t0 > budget_s or sweeps >= 50:
+        if time.perf_counter() - t0 > budget_s or sweeps >= 200:
             break
     dt = time.perf_counter() - t0
     ntok = int(np.count_nonzero(sub_w < V))
