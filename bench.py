@@ -133,9 +133,9 @@ def pmi_bench(ws, offs, V, K, alpha, eta, steps, warmup):
     lld, llr = ctx.loglik()
     wn = ctx.weighted_n()
     out = {"value": n_valid * steps / dt, "unit": "tokens/s", "ms_per_step": dt * 1e3 / steps, "steps": steps,
-           "kernel": "sweep_device_w_kernel<16>", "kernel_ms": kern_ms, "achieved_GBs": achieved,
+           "kernel": "sweep_device_w_kernel<16, false>", "kernel_ms": kern_ms, "achieved_GBs": achieved,
            "frac": achieved / HBM_PEAK_GBS, "algorithmic_bytes_per_launch": bytes_per_launch,
-           "traffic": pmc_traffic("sweep_device_w_kernel<16>"),
+           "traffic": pmc_traffic("sweep_device_w_kernel<16, false>"),
            "ll_per_word": (lld + llr) / wn, "weighted_n": wn}
     log("PMI: %.3g tokens/s, kernel %.3f ms (%.1f%% of HBM peak), %.3f ms/step, ll_per_word %.5f"
         % (out["value"], kern_ms, 100 * out["frac"], out["ms_per_step"], out["ll_per_word"]))
@@ -442,8 +442,8 @@ def main():
         "config": {"workload": "large_k1000", "docs": D, "tokens": n_valid, "V": V, "K": K, "alpha": alpha,
                    "eta": eta, "scheme": "device-wide (workers=0)", "optim_interval": 10},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": pmc_traffic("sweep_device_kernel<16>"),
-                     "kernel": "sweep_device_kernel<16>", "kernel_ms": kern_ms,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": pmc_traffic("sweep_device_kernel<16, false>"),
+                     "kernel": "sweep_device_kernel<16, false>", "kernel_ms": kern_ms,
                      "algorithmic_bytes_per_launch": bytes_per_launch},
         "ll_per_word": (lld + llr) / n_valid,
     }
