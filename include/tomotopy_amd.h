@@ -90,9 +90,9 @@ int lda_ctx_destroy(lda_ctx* ctx);
  * per-token wordWeights, LDAModel.hpp:802-809, 1104-1121; entries of words
  * >= V are ignored), NULL for LDA_TW_ONE.  Term-weighted models keep float
  * counts (addWordTo adds the weight; removal is max(count - weight, 0),
- * Utils.hpp:132-155) and support training, the log-likelihood and the alpha
- * fixed point; their fold-in, frozen sampling, copy_merge and exact
- * invariant checks return LDA_ERR_UNSUPPORTED. */
+ * Utils.hpp:132-155) and support training under every scheme, the
+ * log-likelihood and the alpha fixed point; their fold-in, frozen sampling
+ * and exact invariant checks return LDA_ERR_UNSUPPORTED. */
 int lda_upload_corpus(lda_ctx* ctx, const uint32_t* words, const uint64_t* doc_off, int64_t D, const uint16_t* z,
                       const float* weights);
 

@@ -663,6 +663,58 @@ void sweep_partition_w(Model& m, int W, int threads) {
     merge_w(m);
 }
 
+// copy_merge with token weights (the schedule of sweep_copy_merge): worker
+// w samples on private copies of n_kw and n_k; the merge follows
+// mergeState<copy_merge> (LDAModel.hpp:555-569) in float: n_kw = local_0, then
+// n_kw += local_i - old for i = 1 .. W-1 in order, each difference rounded
+// first; then the weighted clamp and row sum of merge_w.
+void sweep_copy_merge_w(Model& m, int W, int threads) {
+    const uint32_t sweep = m.global_step;
+    const int64_t chStride = std::min<int64_t>((int64_t)W * 8, m.D);
+    std::vector<std::vector<float>> nkw_loc(W, m.nkw_f);
+    std::vector<std::vector<float>> nk_loc(W, m.nk_f);
+    auto work = [&](int w) {
+        std::vector<float> buf(64 * 32 + m.K + 64);
+        std::vector<uint32_t> order;
+        for (int64_t ch = w; ch < chStride; ch += W) {
+            size_t n = (size_t)((m.D + (chStride - 1) - ch) / chStride);
+            uint32_t sseed = philox_word(m.seed, (uint32_t)ch, 0xC0u, sweep, TAG_SHUFFLE);
+            shuffled_order(n, sseed, order);
+            for (size_t q = 0; q < n; ++q) {
+                int64_t d = (int64_t)order[q] * chStride + ch;
+                for (uint64_t g = m.off[d]; g < m.off[d + 1]; ++g) {
+                    uint32_t v = m.words[g];
+                    if (v >= (uint32_t)m.V) continue;
+                    const float wt = m.wt[g];
+                    float* ndk = &m.ndk_f[(size_t)d * m.K];
+                    float* col = &nkw_loc[w][(size_t)v * m.K];
+                    float* nk = nk_loc[w].data();
+                    int zo = m.z[g];
+                    ndk[zo] = sub_clamp(ndk[zo], wt); col[zo] = sub_clamp(col[zo], wt); nk[zo] = sub_clamp(nk[zo], wt);
+                    float u = token_uniform(m.seed, g, sweep, TAG_SWEEP);
+                    int zn = sample_one_w(m, ndk, col, nk, u, buf.data(), m.ecol(v));
+                    m.z[g] = (uint16_t)zn;
+                    ndk[zn] += wt; col[zn] += wt; nk[zn] += wt;
+                }
+            }
+        }
+    };
+    int T = std::max(1, std::min(threads, W));
+    if (T == 1) for (int w = 0; w < W; ++w) work(w);
+    else {
+        std::vector<std::thread> th;
+        for (int t = 0; t < T; ++t) th.emplace_back([&, t] { for (int w = t; w < W; w += T) work(w); });
+        for (auto& t : th) t.join();
+    }
+    for (size_t i = 0; i < m.nkw_f.size(); ++i) {
+        const float old = m.nkw_f[i];
+        float acc = nkw_loc[0][i];
+        for (int w = 1; w < W; ++w) acc = acc + (nkw_loc[w][i] - old);
+        m.nkw_f[i] = acc;
+    }
+    merge_w(m);
+}
+
 // optimizeParameters (LDAModel.hpp:221-265) with the document length
 // replaced by the document's weight sum (getSumWordWeight).
 void optimize_alpha_w(Model& m) {
@@ -968,13 +1020,14 @@ void orc_set_corpus_w(void* h, const uint32_t* words, const uint64_t* off, int64
     }
 }
 
-// Weighted training: scheme 1 (none) or 3 (partition, W workers), alpha
-// optimisation on the same schedule as orc_train_scheme.
+// Weighted training: scheme 1 (none), 2 (copy_merge) or 3 (partition, W
+// workers), alpha optimisation on the same schedule as orc_train_scheme.
 void orc_train_w(void* h, int iterations, int workers, int threads, int scheme) {
     Model* m = (Model*)h;
     int W = scheme == 1 ? 1 : std::max(1, workers);
     for (int it = 0; it < iterations; ++it) {
-        sweep_partition_w(*m, W, threads);
+        if (scheme == 2 && W > 1) sweep_copy_merge_w(*m, W, threads);
+        else sweep_partition_w(*m, W, threads);
         if (m->global_step >= m->burn_in && m->optim_interval && (m->global_step + 1) % m->optim_interval == 0)
             optimize_alpha_w(*m);
         ++m->global_step;

@@ -292,9 +292,7 @@ class WeightedModel(Model):
         load().orc_set_corpus_w(self._h, _p(self._words), _p(self._off), self.D, _p(zz), _p(self._wt))
 
     def train(self, iterations, workers=1, threads=1, scheme=3):
-        """scheme: 1 none, 3 partition."""
-        if scheme not in (1, 3):
-            raise ValueError("term-weighted training runs the none or partition scheme")
+        """scheme: 1 none, 2 copy_merge, 3 partition."""
         load().orc_train_w(self._h, iterations, workers, threads, scheme)
 
     def optimize_alpha(self):

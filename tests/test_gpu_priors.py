@@ -63,14 +63,15 @@ def test_prior_schedules_bit_exact(K, workers, scheme):
     assert abs(gd - od) <= 1e-9 * abs(od) and abs(gr - orr) <= 1e-9 * abs(orr)
 
 
-@pytest.mark.parametrize("K", [20, 200])
-def test_prior_weighted_partition_bit_exact(K):
+@pytest.mark.parametrize("K,scheme", [(20, 3), (200, 3), (20, 2), (200, 2)])
+def test_prior_weighted_partition_bit_exact(K, scheme):
     ctx, m, _, _, _ = _setup(K, seed=7, tw=2)
     ctx.set_optim(0, 0)
     m.set_optim(0, 0)
     for it in range(3):
-        ctx.train(1, workers=3, scheme=3)
-        m.train(1, workers=3, scheme=3)
+        print("K", K, "scheme", scheme, "sweep", it, flush=True)
+        ctx.train(1, workers=3, scheme=scheme)
+        m.train(1, workers=3, scheme=scheme)
         zg, ndkg, nkwg, nkg = ctx.download_state_f()
         zo, ndko, nkwo, nko, _ = m.state()
         assert np.array_equal(zg, zo)

@@ -1,5 +1,5 @@
 """TermWeight IDF / PMI training on the device against the CPU oracle:
-bit-exact under the deterministic schedules (none, partition), FP64
+bit-exact under the deterministic schedules (none, partition, copy_merge), FP64
 log-likelihood to 1e-9, device-wide sweep by its invariants and by the
 per-word log-likelihood of an independent chain."""
 import numpy as np
@@ -49,6 +49,19 @@ def test_weighted_schedule_bit_exact(scheme, K, workers):
         print(scheme, "K", K, "workers", workers, "sweep", it, flush=True)
         ctx.train(1, workers=workers, scheme=3 if workers > 1 else 1)
         m.train(1, workers=workers, scheme=3 if workers > 1 else 1)
+        _compare(ctx, m)
+
+
+@pytest.mark.parametrize("scheme", ["idf", "pmi"])
+@pytest.mark.parametrize("K,workers", [(20, 3), (200, 4), (1000, 2)])
+def test_weighted_copy_merge_bit_exact(scheme, K, workers):
+    ctx, m, _, _, _, _ = _pair(scheme, K, seed=K + 3 * workers, min_cf=2, rm_top=1)
+    ctx.set_optim(0, 0)
+    m.set_optim(0, 0)
+    for it in range(3):
+        print(scheme, "copy_merge K", K, "workers", workers, "sweep", it, flush=True)
+        ctx.train(1, workers=workers, scheme=2)
+        m.train(1, workers=workers, scheme=2)
         _compare(ctx, m)
 
 

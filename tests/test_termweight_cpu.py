@@ -47,11 +47,13 @@ def test_term_weights_known_answers():
     assert abs(float(pmi[5]) - math.log(7.0 / 6.0)) < 1e-6
 
 
-@pytest.mark.parametrize("workers", [1, 3])
-def test_unit_weights_reduce_to_integer_sampler(workers):
+@pytest.mark.parametrize("workers,scheme", [(1, 1), (3, 3), (3, 2), (5, 2)])
+def test_unit_weights_reduce_to_integer_sampler(workers, scheme):
     """Weights of 1.0: float counts hold integers exactly and no removal
     clamps, so the trajectory, counts, alpha and log-likelihood equal the
-    integer sampler's."""
+    integer sampler's, under each schedule (for copy_merge the float merge
+    local_0 + sum (local_w - old) is exact on integers and equals the integer
+    old + sum (local_w - old))."""
     ws, offs, V = _corpus(50)
     K = 12
     mi = O.Model(K, V, 0.1, 0.01, 7)
@@ -60,8 +62,8 @@ def test_unit_weights_reduce_to_integer_sampler(workers):
         m.set_optim(2, 0)
     mi.set_corpus(ws, offs)
     mw.set_corpus(ws, offs, np.ones(len(ws), np.float32))
-    mi.train(5, workers=workers)
-    mw.train(5, workers=workers)
+    mi.train(5, workers=workers, scheme=scheme)
+    mw.train(5, workers=workers, scheme=scheme)
     zi, ndki, nkwi, nki = mi.state()
     zw, ndkw, nkww, nkw_, docw = mw.state()
     assert np.array_equal(zi, zw)
@@ -75,7 +77,8 @@ def test_unit_weights_reduce_to_integer_sampler(workers):
                                     for d in range(len(offs) - 1)], np.float32))
 
 
-def test_weight_two_with_doubled_priors_keeps_trajectory():
+@pytest.mark.parametrize("scheme", [3, 2])
+def test_weight_two_with_doubled_priors_keeps_trajectory(scheme):
     """w = 2, alpha' = 2 alpha, eta' = 2 eta scales every topic weight by exactly
     2 (powers of two are exact), so the sampled topics are the integer
     sampler's; the counts are twice the integer ones."""
@@ -87,8 +90,8 @@ def test_weight_two_with_doubled_priors_keeps_trajectory():
         m.set_optim(0, 0)
     mi.set_corpus(ws, offs)
     mw.set_corpus(ws, offs, np.full(len(ws), 2.0, np.float32))
-    mi.train(4, workers=2)
-    mw.train(4, workers=2)
+    mi.train(4, workers=2, scheme=scheme)
+    mw.train(4, workers=2, scheme=scheme)
     zi, ndki, nkwi, nki = mi.state()
     zw, ndkw, nkww, nkw_, _ = mw.state()
     assert np.array_equal(zi, zw)
@@ -204,3 +207,26 @@ def test_weighted_loglik_closed_form():
             if ws[g] < V:
                 acc = np.float32(acc + wts[g])
         assert docw[d] == acc
+
+
+def test_weighted_copy_merge_counts_follow_z():
+    """copy_merge on IDF weights: after every merge n_kw equals the sum of the
+    weights of the word's tokens on each topic (to float rounding: the counts
+    are accumulated in float), n_k is the row sum of n_kw, and n_dk the weight
+    sum of the document's tokens; a merge that dropped a worker's changes or
+    added old twice breaks the first check by whole token weights."""
+    ws, offs, V = _corpus(60, min_cf=2)
+    wts = corpus.term_weights(ws, offs, V, "idf")
+    K = 10
+    m = O.WeightedModel(K, V, 0.1, 0.01, 11)
+    m.set_optim(0, 0)
+    m.set_corpus(ws, offs, wts)
+    for _ in range(3):
+        m.train(1, workers=4, scheme=2)
+        z, ndk, nkw, nk, _ = m.state()
+        exp = np.zeros((V, K), np.float64)
+        valid = ws < V
+        np.add.at(exp, (ws[valid].astype(np.int64), z[valid].astype(np.int64)), wts[valid].astype(np.float64))
+        np.testing.assert_allclose(nkw.reshape(V, K), exp, rtol=0, atol=2e-4 * max(1.0, exp.max()))
+        assert (nkw >= 0).all()
+        np.testing.assert_allclose(nk, nkw.reshape(V, K).sum(axis=0, dtype=np.float64), rtol=1e-5)

@@ -267,12 +267,28 @@ int sweep_once(lda_ctx* c, int W, bool copy_merge) {
     lda::SweepParams p = base_params(c);
     const hipStream_t s = c->stream;
     if (c->tw != LDA_TW_ONE) {
-        // term-weighted: device-wide or partition, then tomoto's weighted
-        // merge n_kw = max(n_kw, 0), n_k = rowsum (LDAModel.hpp:565-579)
-        if (copy_merge) return fail(LDA_ERR_UNSUPPORTED, "copy_merge is not supported for term-weighted models");
+        // term-weighted: device-wide, copy_merge or partition, then tomoto's
+        // weighted merge n_kw = max(n_kw, 0), n_k = rowsum (LDAModel.hpp:565-579)
         p.wt = c->d_wt.p;
         float ms = 0.0f;
-        if (W == 0) {
+        if (copy_merge) {
+            const int64_t n = (int64_t)c->V * c->Kp;
+            HIP_TRY(c->d_nkw_loc.alloc((size_t)W * std::max<int64_t>(n, 1)));
+            HIP_TRY(c->d_nk_priv.alloc((size_t)W * c->Kp));
+            c->part_W = 0;  // d_nk_priv now holds copy_merge state
+            HIP_TRY(lda::launch_replicate(c->d_nkw_loc.p, c->d_nkw.p, n, W, s));
+            HIP_TRY(lda::launch_replicate(c->d_nk_priv.p, c->d_nk.p, c->Kp, W, s));
+            p.nkw = c->d_nkw_loc.p;
+            p.nk_in = c->d_nk_priv.p;
+            p.nk_out = c->d_nk_priv.p;
+            p.W = W;
+            HIP_TRY(hipEventRecord(c->ev0, s));
+            HIP_TRY(lda::launch_sweep_copy_w(c->kpl, p, s));
+            HIP_TRY(hipEventRecord(c->ev1, s));
+            HIP_TRY(lda::launch_merge_copies_f(reinterpret_cast<float*>(c->d_nkw.p),
+                                               reinterpret_cast<const float*>(c->d_nkw_loc.p), n, W, s));
+            c->stat_launches += 1;
+        } else if (W == 0) {
             HIP_TRY(lda::launch_fill_u32(c->d_counter.p, 0u, 1, s));
             p.nk_in = c->d_nk.p;
             p.nk_out = c->d_nk.p;

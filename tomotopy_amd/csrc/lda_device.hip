@@ -875,6 +875,75 @@ __global__ void __launch_bounds__(64) sweep_det_w_kernel(SweepParams p) {
     store_kvec<KPL>(reinterpret_cast<int32_t*>(nkp), lane, __builtin_bit_cast(ivec<KPL>, mine));
 }
 
+// copy_merge with token weights: the schedule of sweep_copy_kernel on
+// float counts; worker w samples on its private n_kw (nkw + w*V*Kp) and n_k
+// (nk_out + w*Kp) with plain stores.
+template <int KPL, bool ASYM>
+__global__ void __launch_bounds__(64) sweep_copy_w_kernel(SweepParams p) {
+    const int lane = lane_id();
+    const int w = blockIdx.x;
+    const WaveLdsW S = wave_lds_w<KPL>();
+    int nvalid, lastLane;
+    lane_limits<KPL>(p.K, lane, nvalid, lastLane);
+    fvec<KPL> al;
+    load_kvec_f<KPL>(p.alpha, lane, al);
+    auto et = make_eta<KPL, ASYM>(p, lane);
+    float* nkp = reinterpret_cast<float*>(p.nk_out) + (int64_t)w * p.Kp;
+    {
+        fvec<KPL> n, y;
+        load_kvec_f<KPL>(nkp, lane, n);
+#pragma unroll
+        for (int j = 0; j < KPL; ++j) y[j] = recip_rn(n[j] + et.s(j));
+        lds_store_f<KPL>(S.nk, lane, n);
+        lds_store_f<KPL>(S.y, lane, y);
+    }
+    SweepParams q = p;
+    q.nkw = p.nkw + (int64_t)w * p.V * p.Kp;
+    const int W = p.W;
+    const int64_t chStride = (int64_t)W * 8 < p.D ? (int64_t)W * 8 : p.D;
+    for (int64_t ch = w; ch < chStride; ch += W) {
+        const uint64_t N = (uint64_t)((p.D + (chStride - 1) - ch) / chStride);
+        const uint32_t sseed = philox_x0(p.seed_lo, p.seed_hi, (uint32_t)ch, 0xC0u, p.sweep, TAG_SHUFFLE);
+        uint64_t P = 1;
+        if (N) {
+            P = kShufflePrimes[sseed & 0xF];
+            if (N % P == 0) {
+                P = kShufflePrimes[(sseed + 1) & 0xF];
+                if (N % P == 0) P = kShufflePrimes[(sseed + 2) & 0xF];
+                if (N % P == 0) P = kShufflePrimes[(sseed + 3) & 0xF];
+            }
+            P %= N;
+        }
+        for (uint64_t i = 0; i < N; ++i) {
+            const int64_t d = (int64_t)(((i + (uint64_t)sseed) * P) % N) * chStride + ch;
+            const uint64_t b = p.off[d], e = p.off[d + 1];
+            if (b == e) continue;
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+            float* ndk_row = reinterpret_cast<float*>(p.ndk) + d * p.Kp;
+            fvec<KPL> nd;
+            load_kvec_f<KPL>(ndk_row, lane, nd);
+            lds_store_f<KPL>(S.nd, lane, nd);
+            sample_range_w<KPL, MODE_DET>(S, al, q, lane, b, e, nvalid, lastLane, et);
+            lds_load_f<KPL>(S.nd, lane, nd);
+            store_kvec<KPL>(reinterpret_cast<int32_t*>(ndk_row), lane, __builtin_bit_cast(ivec<KPL>, nd));
+        }
+    }
+    fvec<KPL> mine;
+    lds_load_f<KPL>(S.nk, lane, mine);
+    store_kvec<KPL>(reinterpret_cast<int32_t*>(nkp), lane, __builtin_bit_cast(ivec<KPL>, mine));
+}
+
+// mergeState<copy_merge> on float counts (LDAModel.hpp:555-564):
+// n_kw = local_0, then n_kw += (local_w - old) for w = 1 .. W-1 in order.
+__global__ void merge_copies_f_kernel(float* __restrict__ nkw, const float* __restrict__ loc, int64_t n, int W) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const float old = nkw[i];
+        float acc = loc[i];
+        for (int w = 1; w < W; ++w) acc = acc + (loc[(int64_t)w * n + i] - old);
+        nkw[i] = acc;
+    }
+}
+
 // mergeState for weighted counts (LDAModel.hpp:565-579): n_kw = max(n_kw, 0)
 // cell by cell, then n_k = rowwise sum in a fixed two-level float order:
 // words in blocks of kRowsumBlock, each block summed in ascending word order
@@ -1896,6 +1965,20 @@ hipError_t launch_sweep_device_w(int kpl, const SweepParams& p, int grid, hipStr
 hipError_t launch_sweep_det_w(int kpl, const SweepParams& p, hipStream_t s) {
     LDA_DISPATCH_KPL_ETA(kpl, p.prior_row != nullptr,
                          hipLaunchKernelGGL((sweep_det_w_kernel<KPL, ASYM>), dim3(p.W), dim3(64), wave_lds_bytes<KPL>(), s, p));
+    return hipGetLastError();
+}
+
+hipError_t launch_sweep_copy_w(int kpl, const SweepParams& p, hipStream_t s) {
+    LDA_DISPATCH_KPL_ETA(kpl, p.prior_row != nullptr,
+                         hipLaunchKernelGGL((sweep_copy_w_kernel<KPL, ASYM>), dim3(p.W), dim3(64), wave_lds_bytes<KPL>(), s, p));
+    return hipGetLastError();
+}
+
+hipError_t launch_merge_copies_f(float* nkw, const float* loc, int64_t n, int W, hipStream_t s) {
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(merge_copies_f_kernel, dim3((unsigned)blocks), dim3(256), 0, s, nkw, loc, n, W);
     return hipGetLastError();
 }
 

@@ -105,6 +105,10 @@ struct InferParams {
 hipError_t launch_sweep_device_w(int kpl, const SweepParams& p, int grid, hipStream_t s);
 hipError_t launch_sweep_det_w(int kpl, const SweepParams& p, hipStream_t s);
 int sweep_device_w_blocks_per_cu(int kpl, bool asym);
+// copy_merge on float counts: W private copies as in launch_sweep_copy; the
+// merge is n_kw = local_0 + sum_{w>=1} (local_w - old) in worker order
+hipError_t launch_sweep_copy_w(int kpl, const SweepParams& p, hipStream_t s);
+hipError_t launch_merge_copies_f(float* nkw, const float* loc, int64_t n, int W, hipStream_t s);
 // n_kw = max(n_kw, 0), n_k = rowwise sum of n_kw, in float (LDAModel.hpp:565-569)
 // Words per block of the weighted merge's two-level row sum (n_k from n_kw).
 constexpr int kRowsumBlock = 256;
