@@ -176,9 +176,25 @@ int lda_sample_frozen(lda_ctx* ctx, int64_t n, const int64_t* doc, const uint32_
  * mode 0: batched (one persistent launch over all documents, longest first,
  *         replayed from a cached hipGraph);
  * mode 1: plain (one launch and one synchronisation per document).
- * theta: D x K floats, ll: D doubles (FP64), z_out: N uint16 (may be NULL). */
+ * theta: D x K floats, ll: D doubles (FP64), z_out: N uint16 (may be NULL).
+ * Models with word priors run tomoto's per-document sampler with eta_kv
+ * (initial topics from the word's eta column, LDAModel.hpp:764-775); the
+ * segmentation options do not apply to them.  Term-weighted models need the
+ * token weights: lda_infer_w. */
 int lda_infer(lda_ctx* ctx, const uint32_t* words, const uint64_t* doc_off, int64_t D, int iterations,
               uint64_t infer_seed, int mode, float* theta, double* ll, uint16_t* z_out);
+
+/* Fold-in of a term-weighted (IDF / PMI) model: as lda_infer, with the
+ * tokens' weights (tomoto computes them in initializeDocState,
+ * LDAModel.hpp:794-809, from the model's vocabWeights: IDF w = vocabWeights[v],
+ * PMI w = max(log(tf / vocabWeights[v] / len), 0)).  Each document is sampled
+ * on a private copy of the float counts (adding w, removing max(x - w, 0)),
+ * theta = (n_dk + alpha) / (sum of the weights + sum alpha) and the FP64 ll
+ * uses the float counts.  Replaces ITopicModel::infer (TopicModel.hpp:842-875,
+ * LDAModel.hpp:920-953) for these models.  weights must be NULL for
+ * TermWeight ONE models (then this is lda_infer). */
+int lda_infer_w(lda_ctx* ctx, const uint32_t* words, const uint64_t* doc_off, int64_t D, const float* weights,
+                int iterations, uint64_t infer_seed, int mode, float* theta, double* ll, uint16_t* z_out);
 
 /* Fold-in with together=True (LDAModel.hpp:852-892): the documents join a
  * copy of the model's n_kw / n_k (initial topics from Discrete(alpha) keyed by

@@ -1241,4 +1241,147 @@ double orc_infer_ll_bruteforce(void* h, const uint32_t* words, uint64_t n, const
     return l;
 }
 
+// Fold-in (together=False) of term-weighted models and of models with word
+// priors (LDAModel.hpp:920-953 with initializeDocState 776-813 and
+// updateStateWithDoc 764-775).  Per document, on a private copy of the
+// model's n_kw / n_k:
+//   init  z_i ~ Discrete(alpha), or with priors z_i ~ sampleFromDiscrete of
+//         the word's eta column (u keyed by infer_seed, the token position i
+//         inside the document, 0, TAG_INFINIT); then addWordTo<+1> with the
+//         token's weight (1 for ONE) in token order;
+//   iters sweeps over the tokens in order: remove (weighted: max(x - w, 0)),
+//         getZLikelihoods with eta_kv / sum_v eta_kv, sample (u keyed by i,
+//         it + 1, TAG_INFER), add;
+//   ll = getLLRest(private) - getLLRest(global) + getLLDocs(doc) in FP64;
+//   theta = (n_dk + alpha) / (sumWordWeight + sum alpha), the weight sum
+//         accumulated in float in token order (updateSumWordWeight, LDA.h:113-116).
+// wts: per-token weights (IDF / PMI models, whose float counts are set), or
+// NULL (int counts).  faithful = 1 keeps tomoto's full K x V private copy and
+// evaluates getLLRest over all of it; faithful = 0 keeps only the document's
+// word columns (one per run of equal word ids) and sums the LL over the cells
+// and topics the document changed (an unchanged cell contributes exactly 0).
+void orc_infer_gen(void* h, const uint32_t* words, const uint64_t* off, int64_t D, int iters, uint64_t infer_seed,
+                   const float* wts, int faithful, float* theta, double* ll, uint16_t* z_out) {
+    Model* m = (Model*)h;
+    const int K = m->K, V = m->V;
+    const bool W = wts != nullptr;
+    const bool P = !m->eta_kv.empty();
+    std::vector<float> ca(K);
+    float asf = 0.0f;
+    for (int k = 0; k < K; ++k) { asf = asf + m->alpha[k]; ca[k] = asf; }
+    double asd = 0.0;
+    for (int k = 0; k < K; ++k) asd += (double)m->alpha[k];
+    std::vector<float> buf(64 * 32 + K + 64);
+    const double eta = (double)m->eta, veta = (double)V * eta;
+    auto es_of = [&](int k) { return P ? m->etasum_d[k] : veta; };
+    auto e_of = [&](uint32_t v, int k) { return P ? (double)m->eta_kv[(size_t)v * K + k] : eta; };
+    // frozen counts as double (int or float model)
+    auto g_nk = [&](int k) { return W ? (double)m->nk_f[k] : (double)m->nk[k]; };
+    auto g_kw = [&](uint32_t v, int k) { return W ? (double)m->nkw_f[(size_t)v * K + k] : (double)m->nkw[(size_t)v * K + k]; };
+    for (int64_t d = 0; d < D; ++d) {
+        const uint64_t b = off[d], n = off[d + 1] - b;
+        std::vector<uint16_t> z(n, 0xFFFF);
+        std::vector<uint64_t> run(n);  // index of the first token of each token's run
+        for (uint64_t i = 0; i < n; ++i) run[i] = (i && words[b + i] == words[b + i - 1]) ? run[i - 1] : i;
+        // private state: n_dk, n_k, and the columns (run-start x K, or the full V x K)
+        std::vector<float> ndf(K, 0.0f), nkf;
+        std::vector<int32_t> ndi(K, 0), nki;
+        std::vector<float> cf;
+        std::vector<int32_t> ci;
+        if (W) nkf = m->nk_f; else nki = m->nk;
+        if (faithful) { if (W) cf = m->nkw_f; else ci = m->nkw; }
+        else if (W) cf.assign((size_t)n * K, 0.0f);
+        else ci.assign((size_t)n * K, 0);
+        auto cell_f = [&](uint64_t i, uint32_t v) { return faithful ? &cf[(size_t)v * K] : &cf[run[i] * K]; };
+        auto cell_i = [&](uint64_t i, uint32_t v) { return faithful ? &ci[(size_t)v * K] : &ci[run[i] * K]; };
+        if (!faithful) {
+            for (uint64_t i = 0; i < n; ++i) {
+                const uint32_t v = words[b + i];
+                if (run[i] != i || v >= (uint32_t)V) continue;
+                for (int k = 0; k < K; ++k) {
+                    if (W) cf[i * K + k] = m->nkw_f[(size_t)v * K + k];
+                    else ci[i * K + k] = m->nkw[(size_t)v * K + k];
+                }
+            }
+        }
+        float sw = 0.0f;
+        int64_t L = 0;
+        for (uint64_t i = 0; i < n; ++i) {
+            const uint32_t v = words[b + i];
+            if (v >= (uint32_t)V) continue;
+            const float u = token_uniform(infer_seed, i, 0, TAG_INFINIT);
+            const int k = P ? sample_from_discrete(m->ecol(v), K, u) : search_cdf(ca.data(), K, u);
+            z[i] = (uint16_t)k;
+            if (W) {
+                const float w = wts[b + i];
+                ndf[k] += w; nkf[k] += w; cell_f(i, v)[k] += w;
+                sw += w;
+            } else {
+                ndi[k]++; nki[k]++; cell_i(i, v)[k]++;
+            }
+            ++L;
+        }
+        for (int it = 0; it < iters; ++it) {
+            for (uint64_t i = 0; i < n; ++i) {
+                const uint32_t v = words[b + i];
+                if (v >= (uint32_t)V) continue;
+                const int zo = z[i];
+                const float u = token_uniform(infer_seed, i, (uint32_t)(it + 1), TAG_INFER);
+                int zn;
+                if (W) {
+                    const float w = wts[b + i];
+                    float* c = cell_f(i, v);
+                    ndf[zo] = sub_clamp(ndf[zo], w); nkf[zo] = sub_clamp(nkf[zo], w); c[zo] = sub_clamp(c[zo], w);
+                    zn = sample_one_w(*m, ndf.data(), c, nkf.data(), u, buf.data(), m->ecol(v));
+                    ndf[zn] += w; nkf[zn] += w; c[zn] += w;
+                } else {
+                    int32_t* c = cell_i(i, v);
+                    ndi[zo]--; nki[zo]--; c[zo]--;
+                    zn = sample_one(K, m->scan_order, m->kpl, ndi.data(), m->alpha.data(), c, m->eta, nki.data(), m->veta, u,
+                                    buf.data(), m->ecol(v), m->esum());
+                    ndi[zn]++; nki[zn]++; c[zn]++;
+                }
+                z[i] = (uint16_t)zn;
+            }
+        }
+        // getLLRest(private) - getLLRest(global)
+        double l = 0.0;
+        for (int k = 0; k < K; ++k) {
+            const double x0 = g_nk(k), x1 = W ? (double)nkf[k] : (double)nki[k];
+            if (faithful || x0 != x1) l += std::lgamma(x0 + es_of(k)) - std::lgamma(x1 + es_of(k));
+        }
+        auto cell_term = [&](double c, double e) { return c != 0.0 ? std::lgamma(c + e) - std::lgamma(e) : 0.0; };
+        if (faithful) {
+            for (uint32_t v = 0; v < (uint32_t)V; ++v)
+                for (int k = 0; k < K; ++k) {
+                    const double c1 = W ? (double)cf[(size_t)v * K + k] : (double)ci[(size_t)v * K + k];
+                    l += cell_term(c1, e_of(v, k)) - cell_term(g_kw(v, k), e_of(v, k));
+                }
+        } else {
+            for (uint64_t i = 0; i < n; ++i) {
+                const uint32_t v = words[b + i];
+                if (run[i] != i || v >= (uint32_t)V) continue;
+                for (int k = 0; k < K; ++k) {
+                    const double c0 = g_kw(v, k), c1 = W ? (double)cf[i * K + k] : (double)ci[i * K + k];
+                    if (c0 != c1) l += cell_term(c1, e_of(v, k)) - cell_term(c0, e_of(v, k));
+                }
+            }
+        }
+        // getLLDocs(doc)
+        const double Ld = W ? (double)sw : (double)L;
+        l -= std::lgamma(Ld + asd) - std::lgamma(asd);
+        for (int k = 0; k < K; ++k) {
+            const double x = W ? (double)ndf[k] : (double)ndi[k];
+            if (x != 0.0) l += std::lgamma(x + (double)m->alpha[k]) - std::lgamma((double)m->alpha[k]);
+        }
+        if (ll) ll[d] = l;
+        if (theta) {
+            const float den = (W ? sw : (float)L) + asf;
+            for (int k = 0; k < K; ++k)
+                theta[(size_t)d * K + k] = ((W ? ndf[k] : (float)ndi[k]) + m->alpha[k]) / den;
+        }
+        if (z_out) std::copy(z.begin(), z.end(), z_out + b);
+    }
+}
+
 }  // extern "C"

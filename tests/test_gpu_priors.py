@@ -119,5 +119,8 @@ def test_model_surface_with_priors(tmp_path):
     assert m2.get_word_prior(common[0]) == mdl.get_word_prior(common[0])
     assert m2.ll_per_word == pytest.approx(mdl.ll_per_word, rel=1e-12)
     m2.train(5, workers=1)
+    dist, ll = m2.infer(m2.make_doc(common * 5), iterations=50)
+    assert abs(sum(dist) - 1.0) < 1e-5 and np.isfinite(ll)
+    assert int(np.argmax(dist)) == 2  # the words' prior holds them on topic 2
     with pytest.raises(NotImplementedError):
-        m2.infer(m2.make_doc(common))
+        m2.infer([m2.make_doc(common)], together=True)

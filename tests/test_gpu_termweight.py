@@ -131,5 +131,7 @@ def test_model_surface_term_weighted(tw):
     assert wf.shape == (len(mdl.used_vocabs),) and (wf >= 0).all()
     d0 = mdl.docs[0]
     assert np.isfinite(d0.get_ll())
+    dist, ll = mdl.infer(mdl.make_doc(["w1", "w2", "w2"]))
+    assert abs(sum(dist) - 1.0) < 1e-5 and np.isfinite(ll)
     with pytest.raises(NotImplementedError):
-        mdl.infer(mdl.make_doc(["w1", "w2"]))
+        mdl.infer([mdl.make_doc(["w1", "w2"])], together=True)

@@ -92,9 +92,12 @@ def prepare_arrays(words, doc_off, n_words, min_cf=0, min_df=0, rm_top=0):
     return w, off, real_v, order
 
 
-def term_weights(words, doc_off, real_v, scheme):
+def term_weights(words, doc_off, real_v, scheme, vocab_w=None):
     """Per-token weights of TermWeight IDF or PMI, as tomoto's prepare()
-    computes them (LDAModel.hpp:1096-1121 and 800-809).
+    computes them (LDAModel.hpp:1096-1121 and 800-809).  vocab_w: the
+    vocabulary weights to use (a trained model's, for unseen documents as
+    initializeDocState does, LDAModel.hpp:794-809); None computes them from
+    this corpus.
 
     words / doc_off: the prepared CSR (sorted documents, ids >= real_v are
     removed words).  Removed words get weight 0 but count in the document
@@ -115,7 +118,7 @@ def term_weights(words, doc_off, real_v, scheme):
     doc_id = np.repeat(np.arange(D, dtype=np.int64), lens)
     valid = words < real_v
     out = np.zeros(N, dtype=np.float32)
-    vw = vocab_weights(words, doc_off, real_v, scheme)
+    vw = vocab_weights(words, doc_off, real_v, scheme) if vocab_w is None else np.asarray(vocab_w, np.float32)
     if N == 0:
         return out
     vw_idx = np.where(valid, words, 0)

@@ -535,6 +535,10 @@ int lda_ctx_destroy(lda_ctx* c) {
     if (c->infer_exec) (void)hipGraphExecDestroy(c->infer_exec);
     c->i_words.release(); c->i_counter.release(); c->i_jobs.release(); c->i_params.release(); c->i_z.release();
     c->i_theta.release(); c->i_ll.release();
+    c->d_wt.release(); c->d_docw.release(); c->d_rowpart.release();
+    c->d_prior_row.release(); c->d_prior_eta.release(); c->d_etasum.release(); c->d_etasum_d.release();
+    c->g_wt.release(); c->g_docw.release(); c->g_jobrun.release(); c->g_runword.release();
+    c->g_cells.release(); c->g_ndk.release(); c->g_nk.release();
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     c->fold_streams.destroy();
@@ -1068,17 +1072,19 @@ int lda_sample_frozen(lda_ctx* c, int64_t n, const int64_t* doc, const uint32_t*
     return rc;
 }
 
-int lda_infer(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off, int64_t D, int iterations, uint64_t infer_seed,
-              int mode, float* theta, double* ll, uint16_t* z_out) {
+// Argument checks shared by the fold-in entry points; *done = true when there
+// is nothing to do (D == 0).
+static int check_infer_args(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off, int64_t D, int iterations,
+                            int mode, float* theta, double* ll, bool* done) {
+    *done = false;
     if (!c) return fail(LDA_ERR_INVALID_ARGUMENT, "ctx is NULL");
-    if (c->tw != LDA_TW_ONE) return fail(LDA_ERR_UNSUPPORTED, "fold-in of term-weighted models is not on the device yet");
-    if (c->has_priors) return fail(LDA_ERR_UNSUPPORTED, "fold-in with word priors is not on the device yet");
     if (D < 0 || (D && (!doc_off || !theta || !ll))) return fail(LDA_ERR_INVALID_ARGUMENT, "NULL argument");
     if (iterations < 0) return fail(LDA_ERR_INVALID_ARGUMENT, "iterations must be >= 0");
     if (mode != 0 && mode != 1) return fail(LDA_ERR_INVALID_ARGUMENT, "mode must be 0 or 1");
-    if (!D) return LDA_OK;
+    if (!D) { *done = true; return LDA_OK; }
     if (D > 0x7FFFFFFFll) return fail(LDA_ERR_INVALID_ARGUMENT, "too many documents");
     if (doc_off[0] != 0) return fail(LDA_ERR_INVALID_ARGUMENT, "doc_off[0] must be 0");
+    if (doc_off[D] && !words) return fail(LDA_ERR_INVALID_ARGUMENT, "NULL argument");
     for (int64_t d = 0; d < D; ++d) {
         if (doc_off[d + 1] < doc_off[d]) return fail(LDA_ERR_INVALID_ARGUMENT, "doc_off must be non-decreasing");
         if (doc_off[d + 1] - doc_off[d] > kMaxDocTokens)
@@ -1086,6 +1092,137 @@ int lda_infer(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off, int64_
         for (uint64_t g = doc_off[d] + 1; g < doc_off[d + 1]; ++g)
             if (words[g] < words[g - 1]) return fail(LDA_ERR_INVALID_ARGUMENT, "words must be sorted within each document");
     }
+    return LDA_OK;
+}
+
+// Fold-in of term-weighted models and of models with word priors: one
+// wavefront per document (longest first in mode 0, one launch per document in
+// mode 1), each run of equal word ids with a private column, then the
+// finalize kernel.  Arguments checked by check_infer_args.
+static int infer_general(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off, int64_t D, const float* weights,
+                         int iterations, uint64_t infer_seed, int mode, float* theta, double* ll, uint16_t* z_out) {
+    const bool wt = c->tw != LDA_TW_ONE;
+    const uint64_t N = doc_off[D];
+    std::vector<uint32_t> order(D);
+    std::iota(order.begin(), order.end(), 0u);
+    if (mode == 0)
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+            return (doc_off[a + 1] - doc_off[a]) > (doc_off[b + 1] - doc_off[b]);
+        });
+    std::vector<lda::InferJob> jobs(D);
+    std::vector<int64_t> job_run(D + 1);
+    std::vector<uint32_t> run_word;
+    for (int64_t i = 0; i < D; ++i) {
+        const uint32_t d = order[i];
+        jobs[i] = lda::InferJob{doc_off[d], doc_off[d + 1], 0, iterations, iterations, (int32_t)d};
+        job_run[i] = (int64_t)run_word.size();
+        int64_t prev = -1;
+        for (uint64_t g = doc_off[d]; g < doc_off[d + 1]; ++g) {
+            const uint32_t v = words[g];
+            if (v >= (uint32_t)c->V || (int64_t)v == prev) continue;
+            run_word.push_back(v);
+            prev = v;
+        }
+    }
+    job_run[D] = (int64_t)run_word.size();
+    const size_t R = run_word.size();
+    const double cell_bytes = (double)R * c->Kp * 4.0;
+    if (cell_bytes > 64e9)
+        return fail(LDA_ERR_OUT_OF_MEMORY, "fold-in batch needs %.1f GB of private columns; split the batch", cell_bytes / 1e9);
+    HIP_TRY(hipSetDevice(c->device));
+    const hipStream_t s = c->stream;
+    HIP_TRY(c->i_words.alloc(N));
+    HIP_TRY(c->i_z.alloc(N));
+    HIP_TRY(c->i_jobs.alloc(D));
+    HIP_TRY(c->i_counter.alloc(lda::kFoldBuckets));
+    HIP_TRY(c->i_theta.alloc((size_t)D * c->K));
+    HIP_TRY(c->i_ll.alloc(D));
+    if (wt) HIP_TRY(c->g_wt.alloc(N));
+    HIP_TRY(c->g_jobrun.alloc(D + 1));
+    HIP_TRY(c->g_runword.alloc(R));
+    HIP_TRY(c->g_cells.alloc(R * (size_t)c->Kp));
+    HIP_TRY(c->g_ndk.alloc((size_t)D * c->Kp));
+    HIP_TRY(c->g_nk.alloc((size_t)D * c->Kp));
+    HIP_TRY(c->g_docw.alloc(D));
+    if (N) HIP_TRY(hipMemcpyAsync(c->i_words.p, words, N * 4, hipMemcpyHostToDevice, s));
+    if (wt && N) HIP_TRY(hipMemcpyAsync(c->g_wt.p, weights, N * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(c->i_jobs.p, jobs.data(), D * sizeof(lda::InferJob), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(c->g_jobrun.p, job_run.data(), (D + 1) * 8, hipMemcpyHostToDevice, s));
+    if (R) HIP_TRY(hipMemcpyAsync(c->g_runword.p, run_word.data(), R * 4, hipMemcpyHostToDevice, s));
+    lda::InferGenParams p{};
+    p.words = c->i_words.p; p.z = c->i_z.p; p.wt = wt ? c->g_wt.p : nullptr;
+    p.jobs = c->i_jobs.p; p.njobs = D;
+    p.job_run = c->g_jobrun.p; p.run_word = c->g_runword.p; p.cells = c->g_cells.p;
+    p.ndk_out = c->g_ndk.p; p.nk_out = c->g_nk.p; p.docw = c->g_docw.p;
+    p.nkw = c->d_nkw.p; p.nk = c->d_nk.p; p.alpha = c->d_alpha.p; p.alpha_cdf = c->d_alpha_cdf.p;
+    p.eta = c->eta; p.veta = c->veta;
+    p.eta_d = (double)c->eta; p.veta_d = (double)c->V * (double)c->eta;
+    p.K = c->K; p.V = c->V; p.Kp = c->Kp;
+    if (c->has_priors) {
+        p.prior_row = c->d_prior_row.p; p.prior_eta = c->d_prior_eta.p;
+        p.etasum = c->d_etasum.p; p.etasum_d = c->d_etasum_d.p;
+    }
+    p.seed_lo = (uint32_t)infer_seed; p.seed_hi = (uint32_t)(infer_seed >> 32);
+    p.theta = c->i_theta.p; p.ll = c->i_ll.p;
+    double asd = 0.0;
+    float asf = 0.0f;
+    for (int k = 0; k < c->K; ++k) { asd += (double)c->alpha[k]; asf = asf + c->alpha[k]; }
+    p.asum_d = asd; p.asum_f = asf;
+    const int per_cu = lda::infer_gen_blocks_per_cu(c->kpl, wt, c->has_priors);
+    if (per_cu <= 0) return fail(LDA_ERR_DEVICE, "fold-in kernel does not fit the device for K = %d", c->K);
+    const int grid = (int)std::min<int64_t>((int64_t)c->num_cu * per_cu, D);
+    c->stat_ms = c->stat_launches = c->stat_tokens = 0.0;
+    HIP_TRY(hipEventRecord(c->ev0, s));
+    if (mode == 0) {
+        HIP_TRY(lda::launch_fill_u32(c->i_counter.p, 0u, 1, s));
+        p.counter = c->i_counter.p;
+        HIP_TRY(lda::launch_infer_gen(c->kpl, p, grid, false, s));
+        HIP_TRY(lda::launch_infer_gen(c->kpl, p, grid, true, s));
+        c->stat_launches = 1;
+    } else {
+        p.counter = nullptr;
+        for (int64_t d = 0; d < D; ++d) {
+            p.job_base = d;
+            HIP_TRY(lda::launch_infer_gen(c->kpl, p, 1, false, s));
+            HIP_TRY(lda::launch_infer_gen(c->kpl, p, 1, true, s));
+            HIP_TRY(hipStreamSynchronize(s));
+        }
+        c->stat_launches = (double)D;
+    }
+    HIP_TRY(hipEventRecord(c->ev1, s));
+    HIP_TRY(hipEventSynchronize(c->ev1));
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->stat_ms = ms;
+    c->stat_tokens = (double)N * iterations;
+    HIP_TRY(hipMemcpyAsync(theta, c->i_theta.p, (size_t)D * c->K * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(ll, c->i_ll.p, D * 8, hipMemcpyDeviceToHost, s));
+    if (z_out && N) HIP_TRY(hipMemcpyAsync(z_out, c->i_z.p, N * 2, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return LDA_OK;
+}
+
+int lda_infer_w(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off, int64_t D, const float* weights,
+                int iterations, uint64_t infer_seed, int mode, float* theta, double* ll, uint16_t* z_out) {
+    bool done = false;
+    int rc = check_infer_args(c, words, doc_off, D, iterations, mode, theta, ll, &done);
+    if (rc || done) return rc;
+    if (c->tw == LDA_TW_ONE) {
+        if (weights) return fail(LDA_ERR_INVALID_ARGUMENT, "TermWeight ONE models take no token weights");
+        return lda_infer(c, words, doc_off, D, iterations, infer_seed, mode, theta, ll, z_out);
+    }
+    if (!weights && doc_off[D]) return fail(LDA_ERR_INVALID_ARGUMENT, "term-weighted models need the token weights");
+    return infer_general(c, words, doc_off, D, weights, iterations, infer_seed, mode, theta, ll, z_out);
+}
+
+int lda_infer(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off, int64_t D, int iterations, uint64_t infer_seed,
+              int mode, float* theta, double* ll, uint16_t* z_out) {
+    bool done = false;
+    int rc = check_infer_args(c, words, doc_off, D, iterations, mode, theta, ll, &done);
+    if (rc || done) return rc;
+    if (c->tw != LDA_TW_ONE)
+        return fail(LDA_ERR_INVALID_ARGUMENT, "term-weighted models need the token weights: call lda_infer_w");
+    if (c->has_priors) return infer_general(c, words, doc_off, D, nullptr, iterations, infer_seed, mode, theta, ll, z_out);
     const uint64_t N = doc_off[D];
     HIP_TRY(hipSetDevice(c->device));
     const hipStream_t s = c->stream;

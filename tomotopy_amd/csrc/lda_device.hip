@@ -1753,6 +1753,248 @@ __global__ void __launch_bounds__(64) infer_finalize_kernel(const InferParams* _
     }
 }
 
+// ------------------------------- fold-in of term-weighted / prior models ----
+// Fold-in (together=False, LDAModel.hpp:920-953) for models whose counts are
+// float (TermWeight IDF / PMI) or whose eta is per word (word priors).  One
+// wavefront per document, documents taken from a counter, longest first.
+// The document's private copy of n_kw is kept as one column per run of equal
+// word ids in `cells` (a float column's value depends on the order of its
+// updates, so it cannot be rebuilt from z); n_dk, n_k and the reciprocals
+// live in LDS with the training sampler's layout and update rule (bump /
+// bump_w), so every weight is the training sampler's.  The initial topics
+// are drawn token-parallel, then added in token order; the weight sum of the
+// document is accumulated in float in the same order.
+template <int KPL, bool ASYM>
+__device__ __forceinline__ SweepParams gen_eta_params(const InferGenParams& p) {
+    SweepParams sp{};
+    sp.eta = p.eta;
+    sp.veta = p.veta;
+    sp.K = p.K;
+    sp.V = p.V;
+    sp.Kp = p.Kp;
+    sp.prior_row = p.prior_row;
+    sp.prior_eta = p.prior_eta;
+    sp.etasum = p.etasum;
+    return sp;
+}
+
+template <int KPL, bool WT, bool ASYM>
+__global__ void __launch_bounds__(64) infer_gen_kernel(InferGenParams p) {
+    constexpr int SH = kpl_shift<KPL>();
+    const int lane = lane_id();
+    const WaveLds S = wave_lds<KPL>();
+    const WaveLdsW SW = wave_lds_w<KPL>();
+    int nvalid, lastLane;
+    lane_limits<KPL>(p.K, lane, nvalid, lastLane);
+    fvec<KPL> al;
+    load_kvec_f<KPL>(p.alpha, lane, al);
+    const SweepParams sp = gen_eta_params<KPL, ASYM>(p);
+    auto et = make_eta<KPL, ASYM>(sp, lane);
+    for (;;) {
+        int64_t idx;
+        if (p.counter) {
+            uint32_t t = 0;
+            if (lane == 0) t = atomicAdd(p.counter, 1u);
+            idx = (int64_t)(uint32_t)uniform_i((int)t);
+        } else {
+            idx = p.job_base;
+        }
+        if (idx >= p.njobs) break;
+        const InferJob J = p.jobs[idx];
+        const uint64_t b = J.b, e = J.e;
+        int64_t run = p.job_run[idx] - 1;
+        // private n_k = the model's, n_dk = 0
+        if constexpr (WT) {
+            fvec<KPL> n, y, z0 = 0.0f;
+            load_kvec_f<KPL>(reinterpret_cast<const float*>(p.nk), lane, n);
+#pragma unroll
+            for (int j = 0; j < KPL; ++j) y[j] = recip_rn(n[j] + et.s(j));
+            lds_store_f<KPL>(SW.nk, lane, n);
+            lds_store_f<KPL>(SW.y, lane, y);
+            lds_store_f<KPL>(SW.nd, lane, z0);
+        } else {
+            set_nk<KPL>(S, p.nk, lane, et);
+            ivec<KPL> z0 = 0;
+            set_ndk<KPL>(S, al, z0, lane);
+        }
+        // initial topics, added in token order; each run's column starts as the model's
+        float sw = 0.0f;
+        int vcur = -1;
+        int32_t* colp = nullptr;
+        for (uint64_t t0 = b; t0 < e; t0 += 64) {
+            const int n = (int)((e - t0) < 64 ? (e - t0) : 64);
+            const uint64_t g = t0 + (uint64_t)lane;
+            int wv = -1, zz = 0xFFFF;
+            float tw = 1.0f;
+            if (lane < n) {
+                wv = (int)p.words[g];
+                if constexpr (WT) tw = p.wt[g];
+                if ((unsigned)wv < (unsigned)p.V) {
+                    const float u = uniform24(philox_x0(p.seed_lo, p.seed_hi, (uint32_t)(g - b), (uint32_t)((g - b) >> 32), 0u, TAG_INFINIT));
+                    if constexpr (ASYM) {
+                        const int r = p.prior_row[wv];
+                        zz = draw_from_eta_column(r >= 0 ? p.prior_eta + (int64_t)r * p.Kp : nullptr, KPL, p.eta, p.K, u);
+                    } else {
+                        const float rr = u * p.alpha_cdf[p.K - 1];
+                        int lo = 0, hi = p.K;
+                        while (lo < hi) {
+                            const int mid = (lo + hi) >> 1;
+                            if (rr < p.alpha_cdf[mid]) hi = mid; else lo = mid + 1;
+                        }
+                        zz = lo < p.K ? lo : p.K - 1;
+                    }
+                }
+                p.z[g] = (uint16_t)zz;
+            }
+            for (int i = 0; i < n; ++i) {
+                const int v = readlane_i(wv, i);
+                if ((unsigned)v >= (unsigned)p.V) continue;
+                if (v != vcur) {
+                    if (colp) { ivec<KPL> c; lds_load<KPL>(S.col, lane, c); store_kvec<KPL>(colp, lane, c); }
+                    ++run;
+                    colp = p.cells + run * p.Kp;
+                    ivec<KPL> c;
+                    load_kvec<KPL>(p.nkw + (int64_t)v * p.Kp, lane, c);
+                    lds_store<KPL>(S.col, lane, c);
+                    vcur = v;
+                }
+                const int k = readlane_i(zz, i);
+                if constexpr (WT) {
+                    const float w = readlane_f(tw, i);
+                    bump_w<KPL, false>(SW, lane, k, w, false, et);
+                    sw += w;
+                } else {
+                    bump<KPL>(S, al, lane, k, +1, et);
+                    sw += 1.0f;
+                }
+            }
+        }
+        if (colp) { ivec<KPL> c; lds_load<KPL>(S.col, lane, c); store_kvec<KPL>(colp, lane, c); }
+        for (int it = 1; it <= J.iters; ++it) {
+            run = p.job_run[idx] - 1;
+            vcur = -1;
+            for (uint64_t t0 = b; t0 < e; t0 += 64) {
+                const int n = (int)((e - t0) < 64 ? (e - t0) : 64);
+                const uint64_t g = t0 + (uint64_t)lane;
+                int wv = -1, zv = 0xFFFF;
+                float uv = 0.0f, tw = 1.0f;
+                if (lane < n) {
+                    wv = (int)p.words[g];
+                    zv = p.z[g];
+                    if constexpr (WT) tw = p.wt[g];
+                    uv = uniform24(philox_x0(p.seed_lo, p.seed_hi, (uint32_t)(g - b), (uint32_t)((g - b) >> 32), (uint32_t)it, TAG_INFER));
+                }
+                int zkeep = zv;
+                for (int i = 0; i < n; ++i) {
+                    const int v = readlane_i(wv, i);
+                    if ((unsigned)v >= (unsigned)p.V) continue;
+                    if (v != vcur) {
+                        ++run;
+                        colp = p.cells + run * p.Kp;
+                        // this lane's earlier cell stores precede the column load
+                        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+                        ivec<KPL> c;
+                        load_kvec<KPL>(colp, lane, c);
+                        lds_store<KPL>(S.col, lane, c);
+                        et.word(sp, v, lane);
+                        vcur = v;
+                    }
+                    const int zo = readlane_i(zv, i);
+                    const float u = readlane_f(uv, i);
+                    int zn;
+                    if constexpr (WT) {
+                        zn = token_step_w<KPL, false>(SW, al, lane, zo, readlane_f(tw, i), u, p.K, nvalid, lastLane, et);
+                    } else {
+                        zn = token_step<KPL>(S, al, lane, zo, u, p.K, nvalid, lastLane, et, nullptr);
+                    }
+                    const int qo = kpos<KPL>(lane, zo & (KPL - 1)), qn = kpos<KPL>(lane, zn & (KPL - 1));
+                    if (lane == (zo >> SH)) colp[qo] = S.col[qo];
+                    if (lane == (zn >> SH)) colp[qn] = S.col[qn];
+                    zkeep = (lane == i) ? zn : zkeep;
+                }
+                if (lane < n) p.z[g] = (uint16_t)zkeep;
+            }
+        }
+        // the document's n_dk and n_k for the finalize kernel
+        int32_t* ndo = p.ndk_out + (int64_t)J.slot * p.Kp;
+        int32_t* nko = p.nk_out + (int64_t)J.slot * p.Kp;
+        if constexpr (WT) {
+            fvec<KPL> a, c;
+            lds_load_f<KPL>(SW.nd, lane, a);
+            lds_load_f<KPL>(SW.nk, lane, c);
+            store_kvec<KPL>(ndo, lane, __builtin_bit_cast(ivec<KPL>, a));
+            store_kvec<KPL>(nko, lane, __builtin_bit_cast(ivec<KPL>, c));
+        } else {
+            store_kvec<KPL>(ndo, lane, get_ndk<KPL>(S, al, lane));
+            ivec<KPL> c;
+            lds_load<KPL>(S.nk, lane, c);
+            store_kvec<KPL>(nko, lane, c);
+        }
+        if (lane == 0) p.docw[J.slot] = sw;
+        if (!p.counter) break;
+    }
+}
+
+// theta = (n_dk + alpha) / (weight sum + sum alpha) and the FP64
+// log-likelihood getLLRest(private) - getLLRest(model) + getLLDocs(doc)
+// (LDAModel.hpp:616-674, 945-946): the n_k terms of the topics whose n_k
+// changed, the n_kw terms of the cells of the document's runs that changed
+// (tomoto skips zero cells; lgamma(0 + eta) - lgamma(eta) is 0 either way),
+// and the document terms.  One wavefront per document.
+template <class C>
+__device__ __forceinline__ double gen_cell_term(C c, double e) {
+    return c != (C)0 ? lgamma((double)c + e) - lgamma(e) : 0.0;
+}
+
+template <int KPL, bool WT, bool ASYM>
+__global__ void __launch_bounds__(64) infer_gen_finalize_kernel(InferGenParams p) {
+    using C = typename std::conditional<WT, float, int32_t>::type;
+    const int lane = lane_id();
+    const C* nkw = reinterpret_cast<const C*>(p.nkw);
+    const C* cells = reinterpret_cast<const C*>(p.cells);
+    for (int64_t idx = p.counter ? (int64_t)blockIdx.x : p.job_base; idx < p.njobs;
+         idx += p.counter ? (int64_t)gridDim.x : p.njobs) {
+        const InferJob J = p.jobs[idx];
+        const C* nd = reinterpret_cast<const C*>(p.ndk_out) + (int64_t)J.slot * p.Kp;
+        const C* nk1 = reinterpret_cast<const C*>(p.nk_out) + (int64_t)J.slot * p.Kp;
+        const C* nk0 = reinterpret_cast<const C*>(p.nk);
+        const float den = p.docw[J.slot] + p.asum_f;
+        double ll = 0.0;
+#pragma unroll
+        for (int j = 0; j < KPL; ++j) {
+            const int k = lane * KPL + j;
+            if (k >= p.K) continue;
+            const int q = kpos<KPL>(lane, j);
+            const float a = p.alpha[q];
+            const C x = nd[q];
+            p.theta[(int64_t)J.slot * p.K + k] = ((float)x + a) / den;
+            if (x != (C)0) ll += lgamma((double)x + (double)a) - lgamma((double)a);
+            const double es = ASYM ? p.etasum_d[q] : p.veta_d;
+            if (nk0[q] != nk1[q]) ll += lgamma((double)nk0[q] + es) - lgamma((double)nk1[q] + es);
+        }
+        for (int64_t r = p.job_run[idx]; r < p.job_run[idx + 1]; ++r) {
+            const uint32_t v = p.run_word[r];
+            int pr = -1;
+            if constexpr (ASYM) pr = p.prior_row[v];
+#pragma unroll
+            for (int j = 0; j < KPL; ++j) {
+                if (lane * KPL + j >= p.K) continue;
+                const int q = kpos<KPL>(lane, j);
+                const C c0 = nkw[(int64_t)v * p.Kp + q], c1 = cells[r * p.Kp + q];
+                if (c0 == c1) continue;
+                const double e = pr >= 0 ? (double)p.prior_eta[(int64_t)pr * p.Kp + q] : p.eta_d;
+                ll += gen_cell_term(c1, e) - gen_cell_term(c0, e);
+            }
+        }
+        ll = wave_sum_d(ll);
+        if (lane == 0) {
+            const double asum = p.asum_d;
+            ll -= lgamma((double)p.docw[J.slot] + asum) - lgamma(asum);
+            p.ll[J.slot] = ll;
+        }
+    }
+}
+
 }  // namespace
 
 // ================================================================ launchers
@@ -2078,6 +2320,59 @@ static hipError_t launch_infer_kpl(int segs, const InferParams* p_dev, int grid,
 hipError_t launch_infer(int kpl, int segs, const InferParams* p_dev, int grid, hipStream_t s) {
     LDA_DISPATCH_KPL(kpl, return launch_infer_kpl<KPL>(segs, p_dev, grid, s));
     return hipErrorInvalidValue;
+}
+
+template <int KPL, bool WT, bool ASYM>
+static hipError_t launch_infer_gen_k(const InferGenParams& p, int grid, bool finalize, hipStream_t s) {
+    if (finalize)
+        hipLaunchKernelGGL((infer_gen_finalize_kernel<KPL, WT, ASYM>), dim3(grid), dim3(64), 0, s, p);
+    else
+        hipLaunchKernelGGL((infer_gen_kernel<KPL, WT, ASYM>), dim3(grid), dim3(64), wave_lds_bytes<KPL>(), s, p);
+    return hipGetLastError();
+}
+
+template <bool WT, bool ASYM>
+static hipError_t launch_infer_gen_t(int kpl, const InferGenParams& p, int grid, bool finalize, hipStream_t s) {
+    switch (kpl) {
+        case 1: return launch_infer_gen_k<1, WT, ASYM>(p, grid, finalize, s);
+        case 2: return launch_infer_gen_k<2, WT, ASYM>(p, grid, finalize, s);
+        case 4: return launch_infer_gen_k<4, WT, ASYM>(p, grid, finalize, s);
+        case 8: return launch_infer_gen_k<8, WT, ASYM>(p, grid, finalize, s);
+        case 16: return launch_infer_gen_k<16, WT, ASYM>(p, grid, finalize, s);
+        case 32: return launch_infer_gen_k<32, WT, ASYM>(p, grid, finalize, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_infer_gen(int kpl, const InferGenParams& p, int grid, bool finalize, hipStream_t s) {
+    const bool wt = p.wt != nullptr, asym = p.prior_row != nullptr;
+    if (wt) return asym ? launch_infer_gen_t<true, true>(kpl, p, grid, finalize, s)
+                        : launch_infer_gen_t<true, false>(kpl, p, grid, finalize, s);
+    return asym ? launch_infer_gen_t<false, true>(kpl, p, grid, finalize, s)
+                : launch_infer_gen_t<false, false>(kpl, p, grid, finalize, s);
+}
+
+template <int KPL>
+static int infer_gen_blocks_kpl(bool wt, bool asym) {
+    int n = 0;
+    hipError_t e;
+    if (wt) e = asym ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_gen_kernel<KPL, true, true>, 64, wave_lds_bytes<KPL>())
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_gen_kernel<KPL, true, false>, 64, wave_lds_bytes<KPL>());
+    else e = asym ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_gen_kernel<KPL, false, true>, 64, wave_lds_bytes<KPL>())
+                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_gen_kernel<KPL, false, false>, 64, wave_lds_bytes<KPL>());
+    return e == hipSuccess ? n : 0;
+}
+
+int infer_gen_blocks_per_cu(int kpl, bool wt, bool asym) {
+    switch (kpl) {
+        case 1: return infer_gen_blocks_kpl<1>(wt, asym);
+        case 2: return infer_gen_blocks_kpl<2>(wt, asym);
+        case 4: return infer_gen_blocks_kpl<4>(wt, asym);
+        case 8: return infer_gen_blocks_kpl<8>(wt, asym);
+        case 16: return infer_gen_blocks_kpl<16>(wt, asym);
+        case 32: return infer_gen_blocks_kpl<32>(wt, asym);
+        default: return 0;
+    }
 }
 
 hipError_t launch_infer_finalize(int kpl, const InferParams* p_dev, int grid, hipStream_t s) {

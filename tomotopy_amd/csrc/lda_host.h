@@ -128,6 +128,12 @@ struct lda_ctx {
     DevBuf<uint16_t> i_z;
     DevBuf<float> i_theta;
     DevBuf<double> i_ll;
+    // fold-in of term-weighted models and of models with word priors
+    // (InferGenParams): token weights, runs, private columns, outputs
+    DevBuf<float> g_wt, g_docw;
+    DevBuf<int64_t> g_jobrun;
+    DevBuf<uint32_t> g_runword;
+    DevBuf<int32_t> g_cells, g_ndk, g_nk;
     // batched fold-in graph: counter reset + persistent kernel reading i_params
     hipGraphExec_t infer_exec = nullptr;
     int infer_exec_segmax = 0;  // the buckets the cached graph launches

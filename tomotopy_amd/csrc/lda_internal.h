@@ -101,6 +101,47 @@ struct InferParams {
     float asum_f;            // sequential float sum of alpha
 };
 
+// Fold-in of term-weighted models (wt set: the count buffers hold float32
+// bits) and of models with word priors (prior_row set).  Jobs as InferJob
+// (it_begin 0, it_end == iters); the runs of equal word ids of job i are
+// [job_run[i], job_run[i+1]) in token order, each with a private column of
+// Kp cells in `cells`.
+struct InferGenParams {
+    const uint32_t* words;
+    uint16_t* z;
+    const float* wt;
+    const InferJob* jobs;
+    int64_t njobs;
+    uint32_t* counter;           // next job; NULL: the launch runs job job_base only
+    int64_t job_base;
+    const int64_t* job_run;      // njobs + 1
+    const uint32_t* run_word;
+    int32_t* cells;              // runs x Kp
+    int32_t* ndk_out;            // slots x Kp, lane layout
+    int32_t* nk_out;             // slots x Kp, lane layout
+    float* docw;                 // slots: the document's weight sum (token count for integer counts)
+    const int32_t* nkw;          // frozen V x Kp
+    const int32_t* nk;           // frozen Kp
+    const float* alpha;          // Kp, lane layout
+    const float* alpha_cdf;      // K, natural order
+    float eta, veta;
+    double eta_d, veta_d;        // double(eta), V * double(eta)
+    int K, V, Kp;
+    const int32_t* prior_row;
+    const float* prior_eta;
+    const float* etasum;
+    const double* etasum_d;
+    uint32_t seed_lo, seed_hi;
+    float* theta;                // slots x K, natural order
+    double* ll;                  // slots
+    double asum_d;
+    float asum_f;
+};
+// finalize = false: the sampler (grid persistent workgroups of one
+// wavefront); true: theta and the log-likelihood.
+hipError_t launch_infer_gen(int kpl, const InferGenParams& p, int grid, bool finalize, hipStream_t s);
+int infer_gen_blocks_per_cu(int kpl, bool wt, bool asym);
+
 // Term-weighted (IDF / PMI) training: the count buffers hold float32 values.
 hipError_t launch_sweep_device_w(int kpl, const SweepParams& p, int grid, hipStream_t s);
 hipError_t launch_sweep_det_w(int kpl, const SweepParams& p, hipStream_t s);

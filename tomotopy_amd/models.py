@@ -421,11 +421,9 @@ call before `train` (py_LDA.cpp:241-269, LDAModel.hpp:1031-1045).'''
         frozen model (one batched launch); together=True: the documents sampled
         jointly on a copy of the model, one log-likelihood for all of them
         (LDAModel.hpp:852-892)."""
-        if self._tw != TermWeight.ONE:
-            raise NotImplementedError("infer() of TermWeight.%s models is not implemented on the device"
-                                      % self._tw.name)
-        if self._eta_by_word:
-            raise NotImplementedError("infer() with word priors is not implemented on the device")
+        if together and (self._tw != TermWeight.ONE or self._eta_by_word):
+            raise NotImplementedError("infer(together=True) of term-weighted models or of models with word priors "
+                                      "is not implemented on the device")
         if not docs:
             return [], ([0.0] if together else [])
         lens = [len(d._word_ids) for d in docs]
@@ -442,7 +440,11 @@ call before `train` (py_LDA.cpp:241-269, LDAModel.hpp:1031-1045).'''
                                                         scheme=parallel)
             ll = np.array([ll_all])
         else:
-            theta, ll, z = self._ctx.infer(words, off, iterations, infer_seed=0, mode=mode, want_z=True)
+            wts = None
+            if self._tw != TermWeight.ONE:
+                wts = _corpus.term_weights(words, off, self._real_v, self._tw.name.lower(),
+                                           vocab_w=self._vocab_weights)
+            theta, ll, z = self._ctx.infer(words, off, iterations, infer_seed=0, mode=mode, want_z=True, weights=wts)
         out = []
         for i, d in enumerate(docs):
             b, e = int(off[i]), int(off[i + 1])
@@ -451,7 +453,10 @@ call before `train` (py_LDA.cpp:241-269, LDAModel.hpp:1031-1045).'''
             orig = np.empty(len(zz), dtype=np.int16)
             orig[perms[i]] = zz
             d._z = orig
-            d._ndk = np.bincount(zz[zz >= 0], minlength=self._k)
+            if self._tw != TermWeight.ONE and not together:
+                d._ndk = np.bincount(zz[zz >= 0], weights=wts[b:e][zz >= 0], minlength=self._k).astype(np.float32)
+            else:
+                d._ndk = np.bincount(zz[zz >= 0], minlength=self._k)
             d._theta = theta[i]
             d._ll = ll[0] if together else ll[i]
             out.append(theta[i].tolist())
