@@ -243,6 +243,83 @@ def infer_bench(plain_sample=300, iterations=100):
     return out
 
 
+def infer_pmi_bench(iterations=100, plain_sample=200, cpu_budget_s=8.0):
+    """Fold-in of a TermWeight.PMI model (K=200, the fold-in workload): the
+    token weights of the unseen documents from the model's vocabulary
+    weights, each document on a private copy of the float counts (one
+    wavefront per document, longest first), against plain (one launch per
+    document) and the oracle's fold-in on the host cores."""
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle import oracle as O
+    from tomotopy_amd import _native, corpus, synth
+    (tw, toff, nw), (uw, uoff) = synth.foldin_workload()
+    ws, offs, V, order = corpus.prepare_arrays(tw, toff, nw)
+    uws, uoffs = corpus.csr_from_arrays(uw, uoff, order)
+    D = len(uoffs) - 1
+    K = 200
+    wts = corpus.term_weights(ws, offs, V, "pmi")
+    vw = corpus.vocab_weights(ws, offs, V, "pmi")
+    uwt = corpus.term_weights(uws, uoffs, V, "pmi", vocab_w=vw)
+    ctx = _native.Context(K, V, 0.1, 0.01, seed=77, tw=2)
+    ctx.upload_corpus(ws, offs, weights=wts)
+    ctx.train(100, workers=0)
+    ntok = int(np.count_nonzero(uws < V))
+    e64 = int(uoffs[64])
+    ctx.infer(uws[:e64], uoffs[:65], 2, weights=uwt[:e64])
+    t0 = time.perf_counter()
+    theta, ll, _ = ctx.infer(uws, uoffs, iterations, mode=0, weights=uwt)
+    batched_s = time.perf_counter() - t0
+    kst = ctx.kernel_stats()
+    rng = np.random.default_rng(5)
+    pick = np.sort(rng.choice(D, size=min(plain_sample, D), replace=False))
+    lens = np.diff(uoffs.astype(np.int64))
+    sub_off = np.zeros(len(pick) + 1, np.uint64)
+    sub_off[1:] = np.cumsum(lens[pick])
+    sub_w = np.concatenate([uws[int(uoffs[d]):int(uoffs[d + 1])] for d in pick]).astype(np.uint32)
+    sub_t = np.concatenate([uwt[int(uoffs[d]):int(uoffs[d + 1])] for d in pick]).astype(np.float32)
+    t0 = time.perf_counter()
+    th_p, ll_p, _ = ctx.infer(sub_w, sub_off, iterations, mode=1, weights=sub_t)
+    plain_s = time.perf_counter() - t0
+    same = bool(np.array_equal(th_p, theta[pick]) and np.array_equal(ll_p, ll[pick]))
+    out = {"workload": "foldin_k200_pmi", "docs": D, "tokens": ntok, "iterations": iterations, "K": K, "V": V,
+           "batched_docs_per_s": D / batched_s, "batched_s": batched_s, "batched_kernel_ms": kst["ms"],
+           "plain_docs_per_s": len(pick) / plain_s, "plain_sample_docs": int(len(pick)),
+           "speedup": (D / batched_s) / (len(pick) / plain_s), "plain_equals_batched": same,
+           "ll_per_weight": float(ll.sum() / np.sum(uwt[uws < V].astype(np.float64)))}
+    # the oracle's fold-in on the host's cores, on the device's trained state
+    threads = min(os.cpu_count() or 1, 16)
+    z, ndk, nkw, nk = ctx.download_state_f()
+    m = O.WeightedModel(K, V, ctx.get_alpha(), 0.01, seed=1, scan_order=0)
+    m.set_corpus(ws, offs, wts, z)
+    m.set_state(z, ndk, nkw, nk)
+    order_d = rng.permutation(D)
+
+    def run(docs):
+        so = np.zeros(len(docs) + 1, np.uint64)
+        so[1:] = np.cumsum(lens[docs])
+        sw_ = np.concatenate([uws[int(uoffs[d]):int(uoffs[d + 1])] for d in docs]).astype(np.uint32)
+        st_ = np.concatenate([uwt[int(uoffs[d]):int(uoffs[d + 1])] for d in docs]).astype(np.float32)
+        m.infer_gen(sw_, so, iterations, weights=st_)
+        return len(docs)
+
+    done = pos = 0
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(threads) as ex:
+        while time.perf_counter() - t0 < cpu_budget_s and pos < D:
+            batch = [b for b in (order_d[pos + j:pos + j + 4] for j in range(0, threads * 16, 4)) if len(b)]
+            pos += threads * 16
+            done += sum(ex.map(run, batch))
+    dt = time.perf_counter() - t0
+    out["cpu_baseline"] = {"value": done / dt, "unit": "docs/s", "cores": threads, "kind": "port",
+                           "sample": "%d random docs of the workload, %d iterations each, oracle fold-in of the "
+                                     "PMI model (private columns per document), %d threads, %.1fs"
+                                     % (done, iterations, threads, dt)}
+    log("PMI fold-in: batched %.0f docs/s (%.3fs, kernels %.1f ms), plain %.1f docs/s on %d docs, identical=%s; "
+        "CPU %.0f docs/s" % (out["batched_docs_per_s"], batched_s, kst["ms"], out["plain_docs_per_s"], len(pick),
+                              same, out["cpu_baseline"]["value"]))
+    return out
+
+
 def _percentile(x, q):
     return float(np.percentile(np.asarray(x, dtype=np.float64), q)) if len(x) else None
 
@@ -384,7 +461,7 @@ def main():
     ap.add_argument("--no-infer", action="store_true", help="skip the fold-in measurement")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
     ap.add_argument("--no-pmi", action="store_true", help="skip the TermWeight.PMI training variant")
-    ap.add_argument("--only", choices=["train", "infer", "serve", "copy"], default=None,
+    ap.add_argument("--only", choices=["train", "infer", "infer_pmi", "serve", "copy"], default=None,
                     help="run one leg only (the JSON line then carries that leg's numbers)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -394,8 +471,8 @@ def main():
     if args.only == "copy":
         print(json.dumps({"leg": "copy", "copy": copy_probe()}), flush=True)
         return
-    if args.only in ("infer", "serve"):
-        res = infer_bench() if args.only == "infer" else serve_bench()
+    if args.only in ("infer", "infer_pmi", "serve"):
+        res = {"infer": infer_bench, "infer_pmi": infer_pmi_bench, "serve": serve_bench}[args.only]()
         print(json.dumps({"metric": METRIC, "leg": args.only, args.only: res}), flush=True)
         return
 
@@ -457,6 +534,7 @@ def main():
     elif not args.no_infer:
         ctx = None
         out["infer"] = infer_bench()
+        out["infer_pmi"] = infer_pmi_bench()
     if not args.no_serve and args.only is None:
         out["serve"] = serve_bench()
     print(json.dumps(out), flush=True)
