@@ -197,14 +197,25 @@ int lda_infer_w(lda_ctx* ctx, const uint32_t* words, const uint64_t* doc_off, in
                 int iterations, uint64_t infer_seed, int mode, float* theta, double* ll, uint16_t* z_out);
 
 /* Fold-in with together=True (LDAModel.hpp:852-892): the documents join a
- * copy of the model's n_kw / n_k (initial topics from Discrete(alpha) keyed by
- * infer_seed) and are sampled `iterations` times under the scheme and
+ * copy of the model's n_kw / n_k (initial topics from Discrete(alpha), or with
+ * word priors from the word's eta column, keyed by infer_seed) and are
+ * sampled `iterations` times under the scheme and
  * workers of lda_train (partition chunks from the model's word frequencies),
  * with alpha fixed.  theta: D x K; *ll = getLLRest(copy) - getLLRest(model) +
  * getLLDocs(documents), FP64 (one value for all documents, as tomoto);
  * z_out: the documents' final topics (N, may be NULL). */
 int lda_infer_together(lda_ctx* ctx, const uint32_t* words, const uint64_t* doc_off, int64_t D, int iterations,
                        uint64_t infer_seed, int workers, int scheme, float* theta, double* ll, uint16_t* z_out);
+
+/* together=True for a term-weighted model: as lda_infer_together, with the
+ * tokens' weights (see lda_infer_w).  The copy starts from the model's float
+ * counts and the documents' tokens are added onto it in document and token
+ * order (initializeDocState on tmpState, LDAModel.hpp:858-861); the sweeps
+ * use the weighted schedules and merge; theta divides by the documents'
+ * weight sums.  weights must be NULL for TermWeight ONE models. */
+int lda_infer_together_w(lda_ctx* ctx, const uint32_t* words, const uint64_t* doc_off, int64_t D, const float* weights,
+                         int iterations, uint64_t infer_seed, int workers, int scheme, float* theta, double* ll,
+                         uint16_t* z_out);
 
 /* Segmented fold-in (lda_infer and the server).  tomoto samples a document's
  * tokens one after the other (LDAModel.hpp:920-953), which bounds a long

@@ -973,6 +973,8 @@ void orc_infer_together(void* h, const uint32_t* words, const uint64_t* off, int
     t.chunk_cf.assign(m->V, 0);
     for (uint32_t v : m->words) if (v < (uint32_t)m->V) ++t.chunk_cf[v];
     t.optim_interval = 0;
+    // word priors: the copy keeps eta_kv (initial topics from the eta columns)
+    t.eta_kv = m->eta_kv; t.etasum = m->etasum; t.etasum_d = m->etasum_d;
     orc_set_corpus(&t, words, off, D, nullptr);
     for (size_t i = 0; i < t.nkw.size(); ++i) t.nkw[i] += m->nkw[i];
     for (int k = 0; k < m->K; ++k) t.nk[k] += m->nk[k];
@@ -1239,6 +1241,54 @@ double orc_infer_ll_bruteforce(void* h, const uint32_t* words, uint64_t n, const
     l -= std::lgamma((double)L + asum) - std::lgamma(asum);
     for (int k = 0; k < m->K; ++k) l += std::lgamma((double)ndk[k] + (double)m->alpha[k]) - std::lgamma((double)m->alpha[k]);
     return l;
+}
+
+// together=True for a term-weighted model (LDAModel.hpp:852-892): the copy
+// starts from the model's float n_kw / n_k and the documents' tokens are
+// added onto it in document and token order (initializeDocState on tmpState);
+// n_dk and the weight sums start at 0.  Then `iters` sweeps of the weighted
+// schedule (with its merge: clamp and row sums) with alpha fixed, and
+// ll = getLLRest(copy) - getLLRest(model) + getLLDocs(docs);
+// theta = (n_dk + alpha) / (weight sum + sum alpha).
+void orc_infer_together_w(void* h, const uint32_t* words, const uint64_t* off, int64_t D, const float* wts, int iters,
+                          uint64_t infer_seed, int workers, int scheme, int threads, float* theta, double* ll,
+                          uint16_t* z_out) {
+    const Model* m = (const Model*)h;
+    Model t;
+    t.K = m->K; t.V = m->V; t.kpl = m->kpl; t.scan_order = m->scan_order;
+    t.eta = m->eta; t.veta = m->veta; t.seed = infer_seed;
+    t.alpha = m->alpha;
+    t.nkw.assign((size_t)m->V * m->K, 0);
+    t.nk.assign(m->K, 0);
+    t.chunk_cf.assign(m->V, 0);
+    for (uint32_t v : m->words) if (v < (uint32_t)m->V) ++t.chunk_cf[v];
+    t.optim_interval = 0;
+    t.eta_kv = m->eta_kv; t.etasum = m->etasum; t.etasum_d = m->etasum_d;
+    orc_set_corpus_w(&t, words, off, D, nullptr, wts);
+    t.nkw_f = m->nkw_f;
+    t.nk_f = m->nk_f;
+    std::fill(t.ndk_f.begin(), t.ndk_f.end(), 0.0f);
+    for (int64_t d = 0; d < D; ++d) {
+        float sw = 0.0f;
+        for (uint64_t g = off[d]; g < off[d + 1]; ++g) {
+            const uint32_t v = words[g];
+            if (v >= (uint32_t)t.V) continue;
+            add_token_w(t, d, v, t.z[g], t.wt[g], false, t.nk_f.data());
+            sw += t.wt[g];
+        }
+        t.docw[d] = sw;
+    }
+    orc_train_w(&t, iters, workers, threads, scheme);
+    double docs = 0, rest = 0, docs0 = 0, rest0 = 0;
+    loglik_w(t, &docs, &rest);
+    loglik_w(*m, &docs0, &rest0);
+    *ll = rest - rest0 + docs;
+    if (z_out) std::copy(t.z.begin(), t.z.end(), z_out);
+    float asum = 0.0f;
+    for (int k = 0; k < m->K; ++k) asum += m->alpha[k];
+    for (int64_t d = 0; d < D; ++d)
+        for (int k = 0; k < m->K; ++k)
+            theta[(size_t)d * m->K + k] = (t.ndk_f[(size_t)d * m->K + k] + m->alpha[k]) / (t.docw[d] + asum);
 }
 
 // Fold-in (together=False) of term-weighted models and of models with word

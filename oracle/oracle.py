@@ -76,6 +76,7 @@ def load():
         "orc_set_state_w": (None, [_c_p, _c_p, _c_p, _c_p, _c_p]),
         "orc_loglik_w": (None, [_c_p, _c_p, _c_p]),
         "orc_infer_gen": (None, [_c_p, _c_p, _c_p, i64, i32, u64, _c_p, i32, _c_p, _c_p, _c_p]),
+        "orc_infer_together_w": (None, [_c_p, _c_p, _c_p, i64, _c_p, i32, u64, i32, i32, i32, _c_p, _c_p, _c_p]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -276,16 +277,23 @@ class Model:
                              int(bool(faithful)), _p(theta), _p(ll), _p(z))
         return theta, ll, z
 
-    def infer_together(self, words, off, iters, infer_seed=0, workers=1, scheme=1, threads=1):
-        """Fold-in with together=True: returns (theta, ll, z)."""
+    def infer_together(self, words, off, iters, infer_seed=0, workers=1, scheme=1, threads=1, weights=None):
+        """Fold-in with together=True: returns (theta, ll, z).  weights: the
+        tokens' weights of a term-weighted model."""
         words = np.ascontiguousarray(words, np.uint32)
         off = np.ascontiguousarray(off, np.uint64)
         D = len(off) - 1
         theta = np.zeros((D, self.K), np.float32)
         z = np.zeros(int(off[-1]), np.uint16)
         ll = ctypes.c_double()
-        load().orc_infer_together(self._h, _p(words), _p(off), D, iters, int(infer_seed) & (2**64 - 1), workers,
-                                  scheme, threads, _p(theta), ctypes.byref(ll), _p(z))
+        seed = int(infer_seed) & (2**64 - 1)
+        if weights is None:
+            load().orc_infer_together(self._h, _p(words), _p(off), D, iters, seed, workers,
+                                      scheme, threads, _p(theta), ctypes.byref(ll), _p(z))
+        else:
+            w = np.ascontiguousarray(weights, np.float32)
+            load().orc_infer_together_w(self._h, _p(words), _p(off), D, _p(w), iters, seed, workers,
+                                        scheme, threads, _p(theta), ctypes.byref(ll), _p(z))
         return theta, ll.value, z
 
     def infer_ll_bruteforce(self, words, z):

@@ -122,5 +122,5 @@ def test_model_surface_with_priors(tmp_path):
     dist, ll = m2.infer(m2.make_doc(common * 5), iterations=50)
     assert abs(sum(dist) - 1.0) < 1e-5 and np.isfinite(ll)
     assert int(np.argmax(dist)) == 2  # the words' prior holds them on topic 2
-    with pytest.raises(NotImplementedError):
-        m2.infer([m2.make_doc(common)], together=True)
+    dists, ll_all = m2.infer([m2.make_doc(common * 3), m2.make_doc(common)], together=True, iterations=20)
+    assert len(dists) == 2 and np.isfinite(ll_all)

@@ -133,5 +133,5 @@ def test_model_surface_term_weighted(tw):
     assert np.isfinite(d0.get_ll())
     dist, ll = mdl.infer(mdl.make_doc(["w1", "w2", "w2"]))
     assert abs(sum(dist) - 1.0) < 1e-5 and np.isfinite(ll)
-    with pytest.raises(NotImplementedError):
-        mdl.infer([mdl.make_doc(["w1", "w2"])], together=True)
+    dists, ll_all = mdl.infer([mdl.make_doc(["w1", "w2"]), mdl.make_doc(["w3", "w1"])], together=True)
+    assert len(dists) == 2 and np.isfinite(ll_all)

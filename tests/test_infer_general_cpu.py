@@ -164,3 +164,110 @@ def test_weighted_fold_in_closed_forms(kind):
         sw = float(uwt[b:e][uw[b:e] < V].astype(np.float64).sum())
         # an emptied topic keeps a rounding-level residue of the float n_dk
         np.testing.assert_allclose(th[d], (nd + ALPHA) / (sw + ALPHA * K), rtol=1e-5, atol=1e-6)
+
+
+# ------------------------------------------------------------ together=True --
+def _unseen(V, n=10, seed=8):
+    uw, uoff = synth.unseen_docs(seed, n, V, 20, np.log(60.0), 1.0, 5, 300, zipf_s=None)
+    return corpus.csr_from_arrays(uw, uoff, np.arange(V))
+
+
+@pytest.mark.parametrize("scheme,workers", [(1, 1), (3, 3), (2, 2)])
+def test_together_prior_closed_forms(scheme, workers):
+    """together=True with word priors: the initial topics come from the eta
+    columns, and from the returned topics theta = (n_dk+a)/(L+sum a) and
+    ll = getLLRest(model + docs) - getLLRest(model) + getLLDocs(docs) with
+    eta_kv and sum_v eta_kv, by scipy."""
+    K = 9
+    m, V, _, _, _, eta_kv = _setup("prior", K=K)
+    _, _, nkw0, nk0 = m.state()
+    uw, uoff = _unseen(V)
+    theta, ll, z = m.infer_together(uw, uoff, 6, infer_seed=19, workers=workers, scheme=scheme)
+    D = len(uoff) - 1
+    ndk = np.zeros((D, K), np.int64)
+    nkw = nkw0.astype(np.int64).copy()
+    for d in range(D):
+        for g in range(int(uoff[d]), int(uoff[d + 1])):
+            if uw[g] < V:
+                ndk[d, z[g]] += 1
+                nkw[uw[g], z[g]] += 1
+    nk = nkw.sum(axis=0)
+    e = eta_kv.astype(np.float64)
+    es = e.sum(axis=0)
+
+    def rest(nkw_, nk_):
+        return np.sum(gammaln(es) - gammaln(nk_ + es)) + np.sum(np.where(nkw_ != 0, gammaln(nkw_ + e) - gammaln(e), 0.0))
+    a = np.full(K, np.float64(np.float32(ALPHA)))
+    L = ndk.sum(axis=1)
+    docs = np.sum(gammaln(a.sum()) - gammaln(L + a.sum())) + np.sum(gammaln(ndk + a) - gammaln(a))
+    ref = rest(nkw, nk) - rest(nkw0.astype(np.int64), nk0.astype(np.int64)) + docs
+    assert ll == pytest.approx(ref, rel=1e-9)
+    asum = np.float32(0)
+    for _ in range(K):
+        asum = np.float32(asum + np.float32(ALPHA))
+    th = ((ndk.astype(np.float32) + np.float32(ALPHA)) / (L.astype(np.float32)[:, None] + asum)).astype(np.float32)
+    assert np.array_equal(theta, th)
+
+
+def test_together_prior_initial_draw():
+    m, V, _, _, _, eta_kv = _setup("prior", K=11)
+    uw, uoff = _unseen(V, n=6)
+    _, _, z = m.infer_together(uw, uoff, 0, infer_seed=23)
+    for g in range(len(uw)):
+        if uw[g] < V:
+            assert z[g] == O.sample_from_discrete(eta_kv[uw[g]], O.token_uniform(23, g, 0, "init"))
+
+
+@pytest.mark.parametrize("scheme,workers", [(1, 1), (3, 3), (2, 2)])
+def test_together_unit_weights_are_the_integer_together(scheme, workers):
+    words, off, nw, _ = synth.workload("plain_k20", n_docs=50)
+    ws, offs, V, _ = corpus.prepare_arrays(words, off, nw, min_cf=2)
+    K = 10
+    mi = O.Model(K, V, ALPHA, ETA, 4)
+    mw = O.WeightedModel(K, V, ALPHA, ETA, 4)
+    for mm in (mi, mw):
+        mm.set_optim(0, 0)
+    mi.set_corpus(ws, offs)
+    mw.set_corpus(ws, offs, np.ones(len(ws), np.float32))
+    mi.train(3, workers=1, scheme=1)
+    mw.train(3, workers=1, scheme=1)
+    uw, uoff = _unseen(V)
+    th0, ll0, z0 = mw.infer_together(uw, uoff, 5, infer_seed=2, workers=workers, scheme=scheme,
+                                     weights=np.ones(len(uw), np.float32))
+    th1, ll1, z1 = mi.infer_together(uw, uoff, 5, infer_seed=2, workers=workers, scheme=scheme)
+    assert np.array_equal(z0, z1)
+    assert np.array_equal(th0, th1)
+    assert ll0 == pytest.approx(ll1, rel=1e-12)
+
+
+@pytest.mark.parametrize("kind", ["idf", "pmi", "pmi_prior"])
+def test_together_weighted_closed_forms(kind):
+    """Float counts: theta and the LL match the closed forms over the weight
+    sums of the returned topics to float rounding."""
+    K = 8
+    m, V, _, _, _, eta_kv = _setup(kind, K=K)
+    _, _, nkw0, nk0, _ = m.state()
+    uw, uoff = _unseen(V)
+    uwt = corpus.term_weights(uw, uoff, V, "idf" if kind == "idf" else "pmi",
+                              vocab_w=np.linspace(0.5, 2.0, V).astype(np.float32))
+    theta, ll, z = m.infer_together(uw, uoff, 6, infer_seed=3, workers=2, scheme=3, weights=uwt)
+    D = len(uoff) - 1
+    valid = uw < V
+    doc = np.repeat(np.arange(D), np.diff(uoff.astype(np.int64)))
+    ndk = np.zeros((D, K))
+    np.add.at(ndk, (doc[valid], z[valid].astype(np.int64)), uwt[valid].astype(np.float64))
+    nkw = nkw0.astype(np.float64).copy()
+    np.add.at(nkw, (uw[valid].astype(np.int64), z[valid].astype(np.int64)), uwt[valid].astype(np.float64))
+    e = np.full((V, K), np.float64(np.float32(ETA))) if eta_kv is None else eta_kv.astype(np.float64)
+    es = np.full(K, V * np.float64(np.float32(ETA))) if eta_kv is None else e.sum(axis=0)
+
+    def rest(nkw_):
+        nk_ = nkw_.sum(axis=0)
+        return np.sum(gammaln(es) - gammaln(nk_ + es)) + np.sum(np.where(nkw_ != 0, gammaln(nkw_ + e) - gammaln(e), 0.0))
+    a = np.full(K, np.float64(np.float32(ALPHA)))
+    sw = np.zeros(D)
+    np.add.at(sw, doc[valid], uwt[valid].astype(np.float64))
+    docs = np.sum(gammaln(a.sum()) - gammaln(sw + a.sum())) + np.sum(np.where(ndk != 0, gammaln(ndk + a) - gammaln(a), 0.0))
+    ref = rest(nkw) - rest(nkw0.astype(np.float64)) + docs
+    assert abs(ll - ref) <= 1e-4 * abs(ref)
+    np.testing.assert_allclose(theta, (ndk + ALPHA) / (sw[:, None] + ALPHA * K), rtol=1e-5, atol=1e-6)

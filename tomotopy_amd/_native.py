@@ -44,6 +44,7 @@ _SIGS = {
     "lda_set_word_priors": [_c_p, ctypes.c_int64, _c_p, _c_p],
     "lda_selftest_reciprocal": [_i32, _i32, _c_p],
     "lda_infer_together": [_c_p, _c_p, _c_p, ctypes.c_int64, _i32, _u64, _i32, _i32, _c_p, _c_p, _c_p],
+    "lda_infer_together_w": [_c_p, _c_p, _c_p, _i64, _c_p, _i32, _u64, _i32, _i32, _c_p, _c_p, _c_p],
     "lda_set_alpha": [_c_p, _c_p],
     "lda_get_alpha": [_c_p, _c_p],
     "lda_set_optim": [_c_p, _u32, _u32],
@@ -285,17 +286,26 @@ class Context:
                                       int(mode), _ptr(theta), _ptr(ll), _ptr(z)))
         return theta, ll, z
 
-    def infer_together(self, words, doc_off, iterations, infer_seed=0, workers=0, scheme=0):
-        """Fold-in with together=True: (theta D x K, ll for all documents, z)."""
+    def infer_together(self, words, doc_off, iterations, infer_seed=0, workers=0, scheme=0, weights=None):
+        """Fold-in with together=True: (theta D x K, ll for all documents, z).
+        weights: the tokens' weights of a term-weighted model."""
         words = np.ascontiguousarray(words, dtype=np.uint32)
         doc_off = np.ascontiguousarray(doc_off, dtype=np.uint64)
         D = len(doc_off) - 1
         theta = np.zeros((max(D, 0), self.k), dtype=np.float32)
         z = np.zeros(int(doc_off[-1]) if len(doc_off) else 0, dtype=np.uint16)
         ll = ctypes.c_double(0.0)
-        _check(load().lda_infer_together(self._h, _ptr(words), _ptr(doc_off), D, int(iterations),
-                                         int(infer_seed) & (2**64 - 1), int(workers), int(scheme), _ptr(theta),
-                                         ctypes.byref(ll), _ptr(z)))
+        seed = int(infer_seed) & (2**64 - 1)
+        if weights is None:
+            _check(load().lda_infer_together(self._h, _ptr(words), _ptr(doc_off), D, int(iterations), seed,
+                                             int(workers), int(scheme), _ptr(theta), ctypes.byref(ll), _ptr(z)))
+        else:
+            wts = np.ascontiguousarray(weights, dtype=np.float32)
+            if len(wts) != len(words):
+                raise ValueError("weights must have one entry per token")
+            _check(load().lda_infer_together_w(self._h, _ptr(words), _ptr(doc_off), D, _ptr(wts), int(iterations),
+                                               seed, int(workers), int(scheme), _ptr(theta), ctypes.byref(ll),
+                                               _ptr(z)))
         return theta, ll.value, z
 
     def set_infer_options(self, seg_tokens=0, seg_max=1):

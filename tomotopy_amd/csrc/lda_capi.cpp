@@ -620,6 +620,12 @@ int lda_upload_corpus(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off
         }
         std::vector<float> ndk((size_t)std::max<int64_t>(D, 1) * c->Kp, 0.0f), nkw((size_t)c->V * c->Kp, 0.0f),
             nk(c->Kp, 0.0f), docw(std::max<int64_t>(D, 1), 0.0f);
+        if (c->base_nkw.size() == nkw.size() && c->base_nk.size() == nk.size()) {
+            nkw.swap(c->base_nkw);
+            nk.swap(c->base_nk);
+        }
+        c->base_nkw.clear();
+        c->base_nk.clear();
         c->weighted_n = 0.0;
         for (int64_t d = 0; d < D; ++d) {
             float sw = 0.0f;
@@ -844,32 +850,59 @@ int lda_loglik(lda_ctx* c, double* ll_docs, double* ll_rest) {
     return LDA_OK;
 }
 
-int lda_infer_together(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off, int64_t D, int iterations,
-                       uint64_t infer_seed, int workers, int scheme, float* theta, double* ll, uint16_t* z_out) {
+int lda_infer_together_w(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off, int64_t D, const float* weights,
+                         int iterations, uint64_t infer_seed, int workers, int scheme, float* theta, double* ll,
+                         uint16_t* z_out) {
     if (!c || !ll || D < 0 || (D && (!doc_off || !theta))) return fail(LDA_ERR_INVALID_ARGUMENT, "NULL argument");
-    if (c->tw != LDA_TW_ONE) return fail(LDA_ERR_UNSUPPORTED, "fold-in of term-weighted models is not on the device yet");
-    if (c->has_priors) return fail(LDA_ERR_UNSUPPORTED, "fold-in with word priors is not on the device yet");
     if (iterations < 0) return fail(LDA_ERR_INVALID_ARGUMENT, "iterations must be >= 0");
+    const bool weighted = c->tw != LDA_TW_ONE;
+    if (!weighted && weights) return fail(LDA_ERR_INVALID_ARGUMENT, "TermWeight ONE models take no token weights");
+    if (weighted && D && doc_off[D] && !weights)
+        return fail(LDA_ERR_INVALID_ARGUMENT, "term-weighted models need the token weights");
     // the documents join a copy of the model: a context of their own whose
-    // counts are the documents' plus the model's, sampled with alpha fixed
+    // counts are the model's plus the documents', sampled with alpha fixed
     lda_ctx* t = nullptr;
-    int rc = lda_ctx_create(c->K, c->V, c->alpha.data(), c->eta, LDA_TW_ONE, infer_seed, c->device, &t);
+    int rc = lda_ctx_create(c->K, c->V, c->alpha.data(), c->eta, c->tw, infer_seed, c->device, &t);
     if (rc) return rc;
     struct Guard {
         lda_ctx* t;
         ~Guard() { lda_ctx_destroy(t); }
     } guard{t};
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (c->has_priors) {
+        // the copy keeps eta_kv: initial topics from the eta columns (LDAModel.hpp:764-775)
+        HIP_TRY(t->d_prior_row.alloc(c->d_prior_row.n));
+        HIP_TRY(t->d_prior_eta.alloc(c->d_prior_eta.n));
+        HIP_TRY(t->d_etasum.alloc(c->d_etasum.n));
+        HIP_TRY(t->d_etasum_d.alloc(c->d_etasum_d.n));
+        HIP_TRY(hipMemcpyAsync(t->d_prior_row.p, c->d_prior_row.p, c->d_prior_row.n * 4, hipMemcpyDeviceToDevice, t->stream));
+        HIP_TRY(hipMemcpyAsync(t->d_prior_eta.p, c->d_prior_eta.p, c->d_prior_eta.n * 4, hipMemcpyDeviceToDevice, t->stream));
+        HIP_TRY(hipMemcpyAsync(t->d_etasum.p, c->d_etasum.p, c->d_etasum.n * 4, hipMemcpyDeviceToDevice, t->stream));
+        HIP_TRY(hipMemcpyAsync(t->d_etasum_d.p, c->d_etasum_d.p, c->d_etasum_d.n * 8, hipMemcpyDeviceToDevice, t->stream));
+        HIP_TRY(hipStreamSynchronize(t->stream));
+        t->has_priors = true;
+    }
     t->chunk_cf.assign(c->V, 0);
     for (uint32_t v : c->words)
         if (v < (uint32_t)c->V) ++t->chunk_cf[v];
+    if (weighted) {
+        // float counts: the documents' tokens are added onto the model's in
+        // document and token order (initializeDocState on tmpState)
+        t->base_nkw.resize((size_t)c->V * c->Kp);
+        t->base_nk.resize(c->Kp);
+        if (!t->base_nkw.empty())
+            HIP_TRY(hipMemcpy(t->base_nkw.data(), c->d_nkw.p, t->base_nkw.size() * 4, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(t->base_nk.data(), c->d_nk.p, c->Kp * 4, hipMemcpyDeviceToHost));
+    }
     std::vector<uint64_t> off0(1, 0);
-    rc = lda_upload_corpus(t, words, D ? doc_off : off0.data(), D, nullptr, nullptr);
+    rc = lda_upload_corpus(t, words, D ? doc_off : off0.data(), D, nullptr, weighted ? weights : nullptr);
     if (rc) return rc;
-    HIP_TRY(hipSetDevice(c->device));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    HIP_TRY(lda::launch_add_i32(t->d_nkw.p, c->d_nkw.p, (int64_t)c->V * c->Kp, t->stream));
-    HIP_TRY(lda::launch_add_i32(t->d_nk.p, c->d_nk.p, c->Kp, t->stream));
-    HIP_TRY(hipStreamSynchronize(t->stream));
+    if (!weighted) {
+        HIP_TRY(lda::launch_add_i32(t->d_nkw.p, c->d_nkw.p, (int64_t)c->V * c->Kp, t->stream));
+        HIP_TRY(lda::launch_add_i32(t->d_nk.p, c->d_nk.p, c->Kp, t->stream));
+        HIP_TRY(hipStreamSynchronize(t->stream));
+    }
     t->optim_interval = 0;
     rc = lda_train(t, iterations, workers, scheme);
     if (rc) return rc;
@@ -879,12 +912,25 @@ int lda_infer_together(lda_ctx* c, const uint32_t* words, const uint64_t* doc_of
     rc = lda_loglik(c, &docs_c, &rest_c);
     if (rc) return rc;
     *ll = rest_t - rest_c + docs_t;
-    // theta = (n_dk + alpha) / (L + sum alpha) (LDAModel.hpp:1145-1159)
+    // theta = (n_dk + alpha) / (weight sum + sum alpha) (LDAModel.hpp:1145-1159)
+    float asum = 0.0f;
+    for (int k = 0; k < c->K; ++k) asum += c->alpha[k];
+    if (weighted) {
+        std::vector<float> ndk((size_t)std::max<int64_t>(D, 1) * c->K);
+        rc = lda_download_state_f(t, z_out, ndk.data(), nullptr, nullptr);
+        if (rc) return rc;
+        for (int64_t d = 0; d < D; ++d) {
+            float sw = 0.0f;
+            for (uint64_t g = doc_off[d]; g < doc_off[d + 1]; ++g)
+                if (words[g] < (uint32_t)c->V) sw += weights[g];
+            for (int k = 0; k < c->K; ++k)
+                theta[(size_t)d * c->K + k] = (ndk[(size_t)d * c->K + k] + c->alpha[k]) / (sw + asum);
+        }
+        return LDA_OK;
+    }
     std::vector<int32_t> ndk((size_t)std::max<int64_t>(D, 1) * c->K);
     rc = lda_download_state(t, z_out, ndk.data(), nullptr, nullptr);
     if (rc) return rc;
-    float asum = 0.0f;
-    for (int k = 0; k < c->K; ++k) asum += c->alpha[k];
     for (int64_t d = 0; d < D; ++d) {
         int32_t L = 0;
         for (int k = 0; k < c->K; ++k) L += ndk[(size_t)d * c->K + k];
@@ -892,6 +938,13 @@ int lda_infer_together(lda_ctx* c, const uint32_t* words, const uint64_t* doc_of
             theta[(size_t)d * c->K + k] = ((float)ndk[(size_t)d * c->K + k] + c->alpha[k]) / ((float)L + asum);
     }
     return LDA_OK;
+}
+
+int lda_infer_together(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off, int64_t D, int iterations,
+                       uint64_t infer_seed, int workers, int scheme, float* theta, double* ll, uint16_t* z_out) {
+    if (c && c->tw != LDA_TW_ONE)
+        return fail(LDA_ERR_INVALID_ARGUMENT, "term-weighted models need the token weights: call lda_infer_together_w");
+    return lda_infer_together_w(c, words, doc_off, D, nullptr, iterations, infer_seed, workers, scheme, theta, ll, z_out);
 }
 
 int lda_check_invariants(lda_ctx* c, int64_t flags[8]) {

@@ -101,6 +101,9 @@ struct lda_ctx {
     // partition chunks from these word frequencies instead of the corpus's
     // (fold-in together: the model's vocabCf, LDAModel.hpp:441-449)
     std::vector<uint64_t> chunk_cf;
+    // term-weighted upload: the float counts start from these (lane layout,
+    // V x Kp and Kp) instead of 0 when set (fold-in together: the model's)
+    std::vector<float> base_nkw, base_nk;
     DevBuf<int32_t> d_prior_row;
     DevBuf<float> d_prior_eta, d_etasum;
     DevBuf<double> d_etasum_d;

@@ -421,9 +421,6 @@ call before `train` (py_LDA.cpp:241-269, LDAModel.hpp:1031-1045).'''
         frozen model (one batched launch); together=True: the documents sampled
         jointly on a copy of the model, one log-likelihood for all of them
         (LDAModel.hpp:852-892)."""
-        if together and (self._tw != TermWeight.ONE or self._eta_by_word):
-            raise NotImplementedError("infer(together=True) of term-weighted models or of models with word priors "
-                                      "is not implemented on the device")
         if not docs:
             return [], ([0.0] if together else [])
         lens = [len(d._word_ids) for d in docs]
@@ -435,15 +432,15 @@ call before `train` (py_LDA.cpp:241-269, LDAModel.hpp:1031-1045).'''
             perm = np.argsort(d._word_ids, kind="stable")
             words[int(off[i]):int(off[i + 1])] = d._word_ids[perm]
             perms.append(perm)
+        wts = None
+        if self._tw != TermWeight.ONE:
+            # initializeDocState (LDAModel.hpp:794-809): weights from the model's vocabulary weights
+            wts = _corpus.term_weights(words, off, self._real_v, self._tw.name.lower(), vocab_w=self._vocab_weights)
         if together:
             theta, ll_all, z = self._ctx.infer_together(words, off, iterations, infer_seed=0, workers=workers,
-                                                        scheme=parallel)
+                                                        scheme=parallel, weights=wts)
             ll = np.array([ll_all])
         else:
-            wts = None
-            if self._tw != TermWeight.ONE:
-                wts = _corpus.term_weights(words, off, self._real_v, self._tw.name.lower(),
-                                           vocab_w=self._vocab_weights)
             theta, ll, z = self._ctx.infer(words, off, iterations, infer_seed=0, mode=mode, want_z=True, weights=wts)
         out = []
         for i, d in enumerate(docs):
@@ -453,7 +450,7 @@ call before `train` (py_LDA.cpp:241-269, LDAModel.hpp:1031-1045).'''
             orig = np.empty(len(zz), dtype=np.int16)
             orig[perms[i]] = zz
             d._z = orig
-            if self._tw != TermWeight.ONE and not together:
+            if wts is not None:
                 d._ndk = np.bincount(zz[zz >= 0], weights=wts[b:e][zz >= 0], minlength=self._k).astype(np.float32)
             else:
                 d._ndk = np.bincount(zz[zz >= 0], minlength=self._k)
