@@ -96,9 +96,11 @@ def test_copy_merge_schedule_bit_exact(workers):
         assert not ctx.check_invariants().any()
 
 
-def test_training_with_alpha_optimisation_matches_oracle():
+@pytest.mark.parametrize("K", [20, 200])
+def test_training_with_alpha_optimisation_matches_oracle(K):
+    """alpha's fixed point runs on the device (count histograms, no host round
+    trip); the oracle sums the same terms document by document."""
     ws, offs, V = _corpus("plain_k20", 60)
-    K = 20
     ctx = _native.Context(K, V, 0.1, 0.01, seed=3)
     ctx.upload_corpus(ws, offs)
     ctx.set_optim(5, 0)
@@ -107,6 +109,7 @@ def test_training_with_alpha_optimisation_matches_oracle():
     m.set_corpus(ws, offs)
     ctx.train(5, workers=1, scheme=1)
     m.train(5, workers=1)
+    assert not np.array_equal(ctx.get_alpha(), np.full(K, np.float32(0.1)))  # alpha moved
     np.testing.assert_allclose(ctx.get_alpha(), m.alpha(), rtol=2e-6)
     _compare_state(ctx, m)
 

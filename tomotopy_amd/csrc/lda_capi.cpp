@@ -399,40 +399,62 @@ int sweep_once(lda_ctx* c, int W, bool copy_merge) {
 // accumulation of float terms); dAlpha uses tomoto's LUT digamma.
 int optimize_alpha(lda_ctx* c) {
     if (!c->D) return LDA_OK;
-    const int nblk = (int)std::min<int64_t>(std::max<int64_t>(c->D / 64, 1), 512);
-    const int cols = c->Kp + 1;
-    HIP_TRY(c->d_part.alloc((size_t)nblk * cols));
-    std::vector<double> part((size_t)nblk * cols);
+    const hipStream_t s = c->stream;
     const DigammaLut& dg = digamma_lut();
-    for (int it = 0; it < 10; ++it) {
-        float asum = 0.0f;
-        for (int k = 0; k < c->K; ++k) asum += c->alpha[k];
-        if (c->tw != LDA_TW_ONE)
-            HIP_TRY(lda::launch_alpha_sums_f(reinterpret_cast<const float*>(c->d_ndk.p), c->d_docw.p, c->D, c->d_alpha.p,
-                                             c->Kp, asum, c->d_part.p, nblk, c->stream));
-        else
-            HIP_TRY(lda::launch_alpha_sums(c->d_ndk.p, c->d_doclen.p, c->D, c->d_alpha.p, c->Kp, asum, c->d_part.p, nblk,
-                                           c->stream));
-        HIP_TRY(hipMemcpyAsync(part.data(), c->d_part.p, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-        HIP_TRY(hipStreamSynchronize(c->stream));
-        double sden = 0.0;
-        for (int b = 0; b < nblk; ++b) sden += part[(size_t)b * cols + c->Kp];
-        const float denom = (float)(sden - (double)c->D * (double)dg(asum));
-        for (int k = 0; k < c->K; ++k) {
-            const int ps = c->pos(k);
-            double sk = 0.0;
-            for (int b = 0; b < nblk; ++b) sk += part[(size_t)b * cols + ps];
-            const float ak = c->alpha[k];
-            const float nom = (float)(sk - (double)c->D * (double)dg(ak));
-            c->alpha[k] = std::max(nom / denom * ak, 1e-5f);
-            if (!(c->alpha[k] < kMaxAlpha))
-                return fail(LDA_ERR_TRAINING, "alpha optimisation left the supported range (alpha_%d = %g)", k,
-                            (double)c->alpha[k]);
-        }
-        int rc = upload_alpha(c);
-        if (rc) return rc;
+    if (!c->d_dglut.p) {
+        HIP_TRY(c->d_dglut.alloc(dg.pts.size()));
+        HIP_TRY(hipMemcpyAsync(c->d_dglut.p, dg.pts.data(), dg.pts.size() * sizeof(float), hipMemcpyHostToDevice, s));
     }
-    return LDA_OK;
+    HIP_TRY(c->d_aux.alloc(2));
+    HIP_TRY(c->d_bad.alloc(1));
+    HIP_TRY(lda::launch_fill_u32(c->d_bad.p, 0u, 1, s));
+    float* asum = c->d_aux.p;
+    float* den = c->d_aux.p + 1;
+    const int Lmax = (int)c->max_len;
+    // the histogram form for integer counts while its table stays small
+    const bool hist = c->tw == LDA_TW_ONE && (int64_t)c->Kp * (c->max_len + 1) <= (int64_t)1 << 26;
+    int nblk = 0;
+    if (hist) {
+        const size_t nh = (size_t)c->Kp * (Lmax + 1);
+        HIP_TRY(c->d_hist.alloc(nh));
+        HIP_TRY(c->d_hist_len.alloc(Lmax + 1));
+        HIP_TRY(lda::launch_fill_u32(reinterpret_cast<uint32_t*>(c->d_hist.p), 0u, (int64_t)nh, s));
+        HIP_TRY(lda::launch_fill_u32(reinterpret_cast<uint32_t*>(c->d_hist_len.p), 0u, Lmax + 1, s));
+        HIP_TRY(lda::launch_alpha_hist(c->d_ndk.p, c->d_doclen.p, c->D, c->Kp, Lmax, c->d_hist.p, c->d_hist_len.p, s));
+    } else {
+        nblk = (int)std::min<int64_t>(std::max<int64_t>(c->D / 32, 1), 2048);
+        HIP_TRY(c->d_part.alloc((size_t)nblk * (c->Kp + 1)));
+    }
+    for (int it = 0; it < 10; ++it) {
+        HIP_TRY(lda::launch_alpha_asum(c->d_alpha.p, c->K, c->kpl, asum, s));
+        if (hist) {
+            HIP_TRY(lda::launch_alpha_den_hist(c->d_hist_len.p, Lmax, c->D, asum, c->d_dglut.p, den, s));
+            HIP_TRY(lda::launch_alpha_update_hist(c->d_alpha.p, c->d_hist.p, Lmax, c->Kp, c->D, den, c->d_dglut.p,
+                                                  kMaxAlpha, c->d_bad.p, s));
+        } else {
+            if (c->tw != LDA_TW_ONE)
+                HIP_TRY(lda::launch_alpha_sums_f(reinterpret_cast<const float*>(c->d_ndk.p), c->d_docw.p, c->D,
+                                                 c->d_alpha.p, c->Kp, asum, c->d_part.p, nblk, s));
+            else
+                HIP_TRY(lda::launch_alpha_sums(c->d_ndk.p, c->d_doclen.p, c->D, c->d_alpha.p, c->Kp, asum, c->d_part.p,
+                                               nblk, s));
+            HIP_TRY(lda::launch_alpha_update_direct(c->d_alpha.p, c->d_part.p, nblk, c->Kp, c->D, asum, c->d_dglut.p,
+                                                    kMaxAlpha, c->d_bad.p, s));
+        }
+    }
+    std::vector<float> a(c->Kp);
+    uint32_t bad = 0;
+    HIP_TRY(hipMemcpyAsync(a.data(), c->d_alpha.p, c->Kp * sizeof(float), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(&bad, c->d_bad.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    for (int k = 0; k < c->K; ++k) c->alpha[k] = a[c->pos(k)];
+    if (bad) {
+        int k = 0;
+        while (k < c->K && c->alpha[k] < kMaxAlpha) ++k;
+        return fail(LDA_ERR_TRAINING, "alpha optimisation left the supported range (alpha_%d = %g)", k,
+                    k < c->K ? (double)c->alpha[k] : 0.0);
+    }
+    return upload_alpha(c);  // the cumulative sums for the initial draws
 }
 
 // n_kw (V x K, word-major) and n_k into the lane layout; T is int32_t or
@@ -539,6 +561,7 @@ int lda_ctx_destroy(lda_ctx* c) {
     c->d_prior_row.release(); c->d_prior_eta.release(); c->d_etasum.release(); c->d_etasum_d.release();
     c->g_wt.release(); c->g_docw.release(); c->g_jobrun.release(); c->g_runword.release();
     c->g_cells.release(); c->g_ndk.release(); c->g_nk.release();
+    c->d_dglut.release(); c->d_aux.release(); c->d_bad.release(); c->d_hist.release(); c->d_hist_len.release();
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     c->fold_streams.destroy();
@@ -567,6 +590,12 @@ int lda_upload_corpus(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off
         for (uint64_t g = doc_off[d] + 1; g < doc_off[d + 1]; ++g)
             if (words[g] < words[g - 1]) return fail(LDA_ERR_INVALID_ARGUMENT, "words must be sorted within each document");
     uint64_t nvalid = 0;
+    int64_t max_len = 0;
+    for (int64_t d = 0; d < D; ++d) {
+        int64_t L = 0;
+        for (uint64_t g = doc_off[d]; g < doc_off[d + 1]; ++g) L += words[g] < (uint32_t)c->V;
+        max_len = std::max(max_len, L);
+    }
     for (uint64_t g = 0; g < N; ++g) {
         if (words[g] < (uint32_t)c->V) {
             ++nvalid;
@@ -579,6 +608,7 @@ int lda_upload_corpus(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off
     c->D = D;
     c->N = N;
     c->n_valid = nvalid;
+    c->max_len = max_len;
     c->off.assign(doc_off, doc_off + D + 1);
     c->words.assign(words, words + N);
     c->part_W = 0;

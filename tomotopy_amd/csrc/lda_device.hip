@@ -1164,8 +1164,9 @@ __device__ __forceinline__ float digamma_approx_f(float z) {
 template <typename C>
 __global__ void __launch_bounds__(256) alpha_sums_kernel(const C* __restrict__ ndk,
                                                          const C* __restrict__ doclen, int64_t D,
-                                                         const float* __restrict__ alpha, int Kp, float asum,
-                                                         double* part) {
+                                                         const float* __restrict__ alpha, int Kp,
+                                                         const float* __restrict__ asum_p, double* part) {
+    const float asum = *asum_p;
     const int64_t d0 = D * blockIdx.x / gridDim.x, d1 = D * (blockIdx.x + 1) / gridDim.x;
     for (int pos = threadIdx.x; pos <= Kp; pos += blockDim.x) {
         double s = 0.0;
@@ -1185,6 +1186,149 @@ __global__ void __launch_bounds__(256) alpha_sums_kernel(const C* __restrict__ n
             for (int64_t d = d0; d < d1; ++d) s += (double)digamma_approx_f((float)doclen[d] + asum);
         }
         part[(int64_t)blockIdx.x * (Kp + 1) + pos] = s;
+    }
+}
+
+// ----------------------------------------- alpha fixed point on the device --
+// optimizeParameters (LDAModel.hpp:221-265), 10 passes of
+//   alpha_k <- max((S_k - D digammaT(alpha_k)) / (S - D digammaT(sum alpha)) * alpha_k, 1e-5)
+// with S_k = sum_d digammaApprox(n_dk + alpha_k), S = sum_d digammaApprox(L_d + sum alpha),
+// all on the device: the sums of a pass, the update and the next pass follow
+// each other on the stream with no host round trip.  Integer counts group the
+// terms by value (a histogram of n_dk per topic and of L_d, built once per
+// call), so a pass evaluates K x max(L_d) terms instead of D x K; the same
+// float terms are summed in double in another order.
+
+// tomoto's digammaT (math.h:150-154, the LUT of LUT.hpp): the table is built
+// on the host; the interpolation is the same float operations.
+constexpr int kDgN = 1024, kDgM = 6400, kDgL = 4000;
+__device__ float digamma_lut_dev(const float* __restrict__ pts, float x) {
+    const float P = 1.0f / 1024, Q = 1.0f / 64, R = 1.0f / 4;
+    if (!__builtin_isfinite(x)) return (__builtin_isnan(x) || x < 0) ? __builtin_nanf("") : __builtin_inff();
+    if (x < 0) return __builtin_nanf("");
+    if (x < 0.001f) {
+        if (x == 0) return -__builtin_inff();
+        return logf(x + 2) - 0.5f / (x + 2) - 1 / 12.f / powf(x + 2, 2) - 1 / (x + 1) - 1 / x;
+    }
+    if (x >= kDgN * P + kDgM * Q + (kDgL - 1) * R) return logf(x) - 0.5f / x - 1 / 12.f / powf(x, 2);
+    size_t idx;
+    float a, nx = x;
+    if (x < kDgN * P) {
+        idx = (size_t)(nx / P);
+        a = (nx - idx * P) / P;
+    } else {
+        nx -= kDgN * P;
+        if (nx < kDgM * Q) {
+            idx = (size_t)(nx / Q);
+            a = (nx - idx * Q) / Q;
+            idx += kDgN;
+        } else {
+            nx -= kDgM * Q;
+            idx = (size_t)(nx / R);
+            a = (nx - idx * R) / R;
+            idx += kDgN + kDgM;
+        }
+    }
+    return pts[idx] + a * (pts[idx + 1] - pts[idx]);
+}
+
+// sum of alpha in topic order, in float (the order of the host's sum)
+__global__ void alpha_asum_kernel(const float* __restrict__ alpha, int K, int kpl, float* out) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        float s = 0.0f;
+        for (int k = 0; k < K; ++k) s += alpha[kpos_rt(kpl, k)];
+        *out = s;
+    }
+}
+
+// one topic's update from its sum and the denominator (thread 0)
+__device__ __forceinline__ void alpha_apply(float* alpha, int pos, float a, double sk, int64_t D, float denom,
+                                            const float* dglut, float amax, unsigned* bad) {
+    const float nom = (float)(sk - (double)D * (double)digamma_lut_dev(dglut, a));
+    const float x = nom / denom * a;
+    const float an = x < 1e-5f ? 1e-5f : x;  // std::max(x, 1e-5f)
+    alpha[pos] = an;
+    if (!(an < amax)) atomicOr(bad, 1u);
+}
+
+// direct sums: part as written by alpha_sums_kernel (nblk rows of Kp + 1);
+// one workgroup per topic position
+__global__ void __launch_bounds__(256) alpha_update_direct_kernel(float* __restrict__ alpha,
+                                                                  const double* __restrict__ part, int nblk, int Kp,
+                                                                  int64_t D, const float* __restrict__ asum_p,
+                                                                  const float* __restrict__ dglut, float amax,
+                                                                  unsigned* bad) {
+    __shared__ double sh[8];
+    const int pos = blockIdx.x;
+    const float a = alpha[pos];
+    if (!(a > 0.0f)) return;  // padded position (uniform over the workgroup)
+    double sk = 0.0, den = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+        sk += part[(int64_t)b * (Kp + 1) + pos];
+        den += part[(int64_t)b * (Kp + 1) + Kp];
+    }
+    sk = block_sum_d(sk, sh);
+    den = block_sum_d(den, sh);
+    if (threadIdx.x == 0) {
+        const float denom = (float)(den - (double)D * (double)digamma_lut_dev(dglut, *asum_p));
+        alpha_apply(alpha, pos, a, sk, D, denom, dglut, amax, bad);
+    }
+}
+
+// histograms of the integer counts: hist[pos][c] = documents with n_dk = c >= 1,
+// hist_len[L] = documents of L valid tokens (every n_dk <= L <= Lmax)
+__global__ void alpha_hist_kernel(const int32_t* __restrict__ ndk, const int32_t* __restrict__ doclen, int64_t D, int Kp,
+                                  int Lmax, int32_t* hist, int32_t* hist_len) {
+    const int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t nth = (int64_t)gridDim.x * blockDim.x;
+    const int64_t n = D * (int64_t)Kp;
+    for (int64_t i = tid; i < n; i += nth) {
+        const int32_t c = ndk[i];
+        if (c > 0 && c <= Lmax) atomicAdd(hist + (i % Kp) * (int64_t)(Lmax + 1) + c, 1);
+    }
+    for (int64_t d = tid; d < D; d += nth) {
+        const int32_t L = doclen[d];
+        if (L >= 0 && L <= Lmax) atomicAdd(hist_len + L, 1);
+    }
+}
+
+// S - D digammaT(sum alpha) from the length histogram; den_out[0]
+__global__ void __launch_bounds__(256) alpha_den_hist_kernel(const int32_t* __restrict__ hist_len, int Lmax, int64_t D,
+                                                             const float* __restrict__ asum_p,
+                                                             const float* __restrict__ dglut, float* den_out) {
+    __shared__ double sh[8];
+    const float asum = *asum_p;
+    double s = 0.0;
+    for (int L = threadIdx.x; L <= Lmax; L += blockDim.x) {
+        const int32_t h = hist_len[L];
+        if (h) s += (double)h * (double)digamma_approx_f((float)L + asum);
+    }
+    s = block_sum_d(s, sh);
+    if (threadIdx.x == 0) *den_out = (float)(s - (double)D * (double)digamma_lut_dev(dglut, asum));
+}
+
+__global__ void __launch_bounds__(256) alpha_update_hist_kernel(float* __restrict__ alpha, const int32_t* __restrict__ hist,
+                                                                int Lmax, int64_t D, const float* __restrict__ den_p,
+                                                                const float* __restrict__ dglut, float amax,
+                                                                unsigned* bad) {
+    __shared__ double sh[8];
+    const int pos = blockIdx.x;
+    const float a = alpha[pos];
+    if (!(a > 0.0f)) return;  // padded position (uniform over the workgroup)
+    const int32_t* h = hist + (int64_t)pos * (Lmax + 1);
+    double sk = 0.0, nnz = 0.0;
+    for (int c = 1 + threadIdx.x; c <= Lmax; c += blockDim.x) {
+        const int32_t x = h[c];
+        if (x) {
+            sk += (double)x * (double)digamma_approx_f((float)c + a);
+            nnz += (double)x;
+        }
+    }
+    sk = block_sum_d(sk, sh);
+    nnz = block_sum_d(nnz, sh);
+    if (threadIdx.x == 0) {
+        sk += (double)(D - (int64_t)nnz) * (double)digamma_approx_f(a);  // the documents with n_dk = 0
+        alpha_apply(alpha, pos, a, sk, D, *den_p, dglut, amax, bad);
     }
 }
 
@@ -2185,15 +2329,48 @@ hipError_t launch_loglik_f(const float* ndk, const float* docw, int64_t D, const
     return hipGetLastError();
 }
 
-hipError_t launch_alpha_sums(const int32_t* ndk, const int32_t* doclen, int64_t D, const float* alpha, int Kp, float asum,
-                             double* part, int nblk, hipStream_t s) {
+hipError_t launch_alpha_sums(const int32_t* ndk, const int32_t* doclen, int64_t D, const float* alpha, int Kp,
+                             const float* asum, double* part, int nblk, hipStream_t s) {
     hipLaunchKernelGGL(alpha_sums_kernel<int32_t>, dim3(nblk), dim3(256), 0, s, ndk, doclen, D, alpha, Kp, asum, part);
     return hipGetLastError();
 }
 
-hipError_t launch_alpha_sums_f(const float* ndk, const float* docw, int64_t D, const float* alpha, int Kp, float asum,
-                               double* part, int nblk, hipStream_t s) {
+hipError_t launch_alpha_sums_f(const float* ndk, const float* docw, int64_t D, const float* alpha, int Kp,
+                               const float* asum, double* part, int nblk, hipStream_t s) {
     hipLaunchKernelGGL(alpha_sums_kernel<float>, dim3(nblk), dim3(256), 0, s, ndk, docw, D, alpha, Kp, asum, part);
+    return hipGetLastError();
+}
+
+hipError_t launch_alpha_asum(const float* alpha, int K, int kpl, float* out, hipStream_t s) {
+    hipLaunchKernelGGL(alpha_asum_kernel, dim3(1), dim3(64), 0, s, alpha, K, kpl, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_alpha_update_direct(float* alpha, const double* part, int nblk, int Kp, int64_t D, const float* asum,
+                                      const float* dglut, float amax, unsigned* bad, hipStream_t s) {
+    hipLaunchKernelGGL(alpha_update_direct_kernel, dim3(Kp), dim3(256), 0, s, alpha, part, nblk, Kp, D, asum, dglut, amax,
+                       bad);
+    return hipGetLastError();
+}
+
+hipError_t launch_alpha_hist(const int32_t* ndk, const int32_t* doclen, int64_t D, int Kp, int Lmax, int32_t* hist,
+                             int32_t* hist_len, hipStream_t s) {
+    int64_t blocks = (D * (int64_t)Kp + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(alpha_hist_kernel, dim3((unsigned)blocks), dim3(256), 0, s, ndk, doclen, D, Kp, Lmax, hist, hist_len);
+    return hipGetLastError();
+}
+
+hipError_t launch_alpha_den_hist(const int32_t* hist_len, int Lmax, int64_t D, const float* asum, const float* dglut,
+                                 float* den, hipStream_t s) {
+    hipLaunchKernelGGL(alpha_den_hist_kernel, dim3(1), dim3(256), 0, s, hist_len, Lmax, D, asum, dglut, den);
+    return hipGetLastError();
+}
+
+hipError_t launch_alpha_update_hist(float* alpha, const int32_t* hist, int Lmax, int Kp, int64_t D, const float* den,
+                                    const float* dglut, float amax, unsigned* bad, hipStream_t s) {
+    hipLaunchKernelGGL(alpha_update_hist_kernel, dim3(Kp), dim3(256), 0, s, alpha, hist, Lmax, D, den, dglut, amax, bad);
     return hipGetLastError();
 }
 

@@ -115,6 +115,13 @@ struct lda_ctx {
     DevBuf<uint64_t> d_off, d_coff;
     DevBuf<double> d_part;
     int part_W = 0;
+    // alpha fixed point on the device: tomoto's digammaT table, {sum alpha,
+    // denominator}, the out-of-range flag, the count histograms; the largest
+    // document length in valid tokens
+    DevBuf<float> d_dglut, d_aux;
+    DevBuf<uint32_t> d_bad;
+    DevBuf<int32_t> d_hist, d_hist_len;
+    int64_t max_len = 0;
 
     // fold-in: segmentation (fold_segments), the largest segment count the
     // kernels fit, persistent grid per bucket and the bucket streams

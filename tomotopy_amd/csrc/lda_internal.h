@@ -159,8 +159,8 @@ hipError_t launch_loglik_f(const float* ndk, const float* docw, int64_t D, const
                            const float* alpha, int V, int Kp, double asum, double eta, double veta, double* partial,
                            int nblk, const int32_t* prior_row, const float* prior_eta, const double* etasum_d,
                            hipStream_t s);
-hipError_t launch_alpha_sums_f(const float* ndk, const float* docw, int64_t D, const float* alpha, int Kp, float asum,
-                               double* part, int nblk, hipStream_t s);
+hipError_t launch_alpha_sums_f(const float* ndk, const float* docw, int64_t D, const float* alpha, int Kp,
+                               const float* asum, double* part, int nblk, hipStream_t s);
 
 // kernels (lda_device.hip)
 // z ~ Discrete(alpha), or with word priors (prior_row set) z ~ the word's eta column
@@ -195,7 +195,22 @@ hipError_t launch_loglik(const int32_t* ndk, const int32_t* doclen, int64_t D, c
 // alpha fixed point: per-topic sums of digammaApprox(n_dk + alpha_k) and the
 // denominator sum of digammaApprox(L_d + asum); part[b*(Kp+1) + pos].
 hipError_t launch_alpha_sums(const int32_t* ndk, const int32_t* doclen, int64_t D, const float* alpha, int Kp,
-                             float asum, double* part, int nblk, hipStream_t s);
+                             const float* asum, double* part, int nblk, hipStream_t s);
+// The alpha fixed point on the device (optimizeParameters, LDAModel.hpp:221-265).
+// asum: device float, the sum of alpha in topic order (launch_alpha_asum);
+// dglut: tomoto's digammaT table (kDigammaLutSize floats); bad: set when an
+// alpha leaves [1e-5, amax).  Direct: from alpha_sums partials.  Histogram
+// (integer counts): hist = Kp x (Lmax + 1) and hist_len = Lmax + 1 zeroed ints.
+constexpr int kDigammaLutSize = 1024 + 6400 + 4000;
+hipError_t launch_alpha_asum(const float* alpha, int K, int kpl, float* asum, hipStream_t s);
+hipError_t launch_alpha_update_direct(float* alpha, const double* part, int nblk, int Kp, int64_t D, const float* asum,
+                                      const float* dglut, float amax, unsigned* bad, hipStream_t s);
+hipError_t launch_alpha_hist(const int32_t* ndk, const int32_t* doclen, int64_t D, int Kp, int Lmax, int32_t* hist,
+                             int32_t* hist_len, hipStream_t s);
+hipError_t launch_alpha_den_hist(const int32_t* hist_len, int Lmax, int64_t D, const float* asum, const float* dglut,
+                                 float* den, hipStream_t s);
+hipError_t launch_alpha_update_hist(float* alpha, const int32_t* hist, int Lmax, int Kp, int64_t D, const float* den,
+                                    const float* dglut, float amax, unsigned* bad, hipStream_t s);
 hipError_t launch_doclen(const uint32_t* words, const uint64_t* off, int64_t D, int V, int32_t* doclen, hipStream_t s);
 
 // invariants
