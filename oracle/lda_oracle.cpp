@@ -1310,8 +1310,27 @@ void orc_infer_together_w(void* h, const uint32_t* words, const uint64_t* off, i
 // evaluates getLLRest over all of it; faithful = 0 keeps only the document's
 // word columns (one per run of equal word ids) and sums the LL over the cells
 // and topics the document changed (an unchanged cell contributes exactly 0).
+void orc_infer_gen_seg(void* h, const uint32_t* words, const uint64_t* off, int64_t D, int iters, uint64_t infer_seed,
+                       const float* wts, int faithful, uint32_t seg_tokens, int seg_max, float* theta, double* ll,
+                       uint16_t* z_out);
+
 void orc_infer_gen(void* h, const uint32_t* words, const uint64_t* off, int64_t D, int iters, uint64_t infer_seed,
                    const float* wts, int faithful, float* theta, double* ll, uint16_t* z_out) {
+    orc_infer_gen_seg(h, words, off, D, iters, infer_seed, wts, faithful, 0, 1, theta, ll, z_out);
+}
+
+// Segmented rule for these models (the device's, for long documents; the
+// integer fold-in's rule of orc_infer_seg restated on running counts): a
+// document of S > 1 segments starts every iteration with each segment on a
+// copy of n_dk and n_k as they were when the iteration started (the
+// columns are the segments' own: a word's run lies in one segment); after the
+// iteration n_dk = snap + sum_s (n_dk,s - snap) and n_k likewise, the
+// differences added in segment order (in float for term-weighted counts).
+// S == 1 samples the document sequentially as above.  The initial topics are
+// added in token order before the first iteration, as for S == 1.
+void orc_infer_gen_seg(void* h, const uint32_t* words, const uint64_t* off, int64_t D, int iters, uint64_t infer_seed,
+                       const float* wts, int faithful, uint32_t seg_tokens, int seg_max, float* theta, double* ll,
+                       uint16_t* z_out) {
     Model* m = (Model*)h;
     const int K = m->K, V = m->V;
     const bool W = wts != nullptr;
@@ -1371,8 +1390,9 @@ void orc_infer_gen(void* h, const uint32_t* words, const uint64_t* off, int64_t 
             }
             ++L;
         }
-        for (int it = 0; it < iters; ++it) {
-            for (uint64_t i = 0; i < n; ++i) {
+        // one segment's tokens [s0, s1) on the state (nd, nk)
+        auto sample_seg = [&](int it, uint64_t s0, uint64_t s1, float* ndF, float* nkF, int32_t* ndI, int32_t* nkI) {
+            for (uint64_t i = s0; i < s1; ++i) {
                 const uint32_t v = words[b + i];
                 if (v >= (uint32_t)V) continue;
                 const int zo = z[i];
@@ -1381,18 +1401,44 @@ void orc_infer_gen(void* h, const uint32_t* words, const uint64_t* off, int64_t 
                 if (W) {
                     const float w = wts[b + i];
                     float* c = cell_f(i, v);
-                    ndf[zo] = sub_clamp(ndf[zo], w); nkf[zo] = sub_clamp(nkf[zo], w); c[zo] = sub_clamp(c[zo], w);
-                    zn = sample_one_w(*m, ndf.data(), c, nkf.data(), u, buf.data(), m->ecol(v));
-                    ndf[zn] += w; nkf[zn] += w; c[zn] += w;
+                    ndF[zo] = sub_clamp(ndF[zo], w); nkF[zo] = sub_clamp(nkF[zo], w); c[zo] = sub_clamp(c[zo], w);
+                    zn = sample_one_w(*m, ndF, c, nkF, u, buf.data(), m->ecol(v));
+                    ndF[zn] += w; nkF[zn] += w; c[zn] += w;
                 } else {
                     int32_t* c = cell_i(i, v);
-                    ndi[zo]--; nki[zo]--; c[zo]--;
-                    zn = sample_one(K, m->scan_order, m->kpl, ndi.data(), m->alpha.data(), c, m->eta, nki.data(), m->veta, u,
+                    ndI[zo]--; nkI[zo]--; c[zo]--;
+                    zn = sample_one(K, m->scan_order, m->kpl, ndI, m->alpha.data(), c, m->eta, nkI, m->veta, u,
                                     buf.data(), m->ecol(v), m->esum());
-                    ndi[zn]++; nki[zn]++; c[zn]++;
+                    ndI[zn]++; nkI[zn]++; c[zn]++;
                 }
                 z[i] = (uint16_t)zn;
             }
+        };
+        const int S = faithful ? 1 : seg_count(n, seg_tokens, seg_max);
+        std::vector<float> sndf, snkf, accf, accg, segf, segg;
+        std::vector<int32_t> sndi, snki, acci, accj, segi, segj;
+        for (int it = 0; it < iters; ++it) {
+            if (S == 1) {
+                sample_seg(it, 0, n, ndf.data(), nkf.data(), ndi.data(), nki.data());
+                continue;
+            }
+            sndf = ndf; snkf = nkf; sndi = ndi; snki = nki;
+            accf = ndf; accg = nkf; acci = ndi; accj = nki;
+            for (int sg = 0; sg < S; ++sg) {
+                const uint64_t s0 = seg_start(words + b, n, sg, S), s1 = seg_start(words + b, n, sg + 1, S);
+                segf = sndf; segg = snkf; segi = sndi; segj = snki;
+                sample_seg(it, s0, s1, segf.data(), segg.data(), segi.data(), segj.data());
+                for (int k = 0; k < K; ++k) {
+                    if (W) {
+                        accf[k] = accf[k] + (segf[k] - sndf[k]);
+                        accg[k] = accg[k] + (segg[k] - snkf[k]);
+                    } else {
+                        acci[k] += segi[k] - sndi[k];
+                        accj[k] += segj[k] - snki[k];
+                    }
+                }
+            }
+            ndf = accf; nkf = accg; ndi = acci; nki = accj;
         }
         // getLLRest(private) - getLLRest(global)
         double l = 0.0;

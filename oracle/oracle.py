@@ -76,6 +76,7 @@ def load():
         "orc_set_state_w": (None, [_c_p, _c_p, _c_p, _c_p, _c_p]),
         "orc_loglik_w": (None, [_c_p, _c_p, _c_p]),
         "orc_infer_gen": (None, [_c_p, _c_p, _c_p, i64, i32, u64, _c_p, i32, _c_p, _c_p, _c_p]),
+        "orc_infer_gen_seg": (None, [_c_p, _c_p, _c_p, i64, i32, u64, _c_p, i32, u32, i32, _c_p, _c_p, _c_p]),
         "orc_infer_together_w": (None, [_c_p, _c_p, _c_p, i64, _c_p, i32, u64, i32, i32, i32, _c_p, _c_p, _c_p]),
     }
     for name, (res, args) in sigs.items():
@@ -262,10 +263,11 @@ class Model:
                              int(seg_tokens), int(seg_max), _p(theta), _p(ll), _p(z))
         return theta, ll, z
 
-    def infer_gen(self, words, off, iters, infer_seed=0, weights=None, faithful=False):
+    def infer_gen(self, words, off, iters, infer_seed=0, weights=None, faithful=False, seg_tokens=0, seg_max=1):
         """Fold-in of a term-weighted model (weights: per-token float32) or of
         a model with word priors; returns (theta, ll, z).  faithful keeps the
-        full K x V private copy per document."""
+        full K x V private copy per document; seg_tokens / seg_max select the
+        segmented rule (0: sequential)."""
         words = np.ascontiguousarray(words, np.uint32)
         off = np.ascontiguousarray(off, np.uint64)
         D = len(off) - 1
@@ -273,8 +275,8 @@ class Model:
         ll = np.zeros(D, np.float64)
         z = np.zeros(int(off[-1]), np.uint16)
         w = None if weights is None else np.ascontiguousarray(weights, np.float32)
-        load().orc_infer_gen(self._h, _p(words), _p(off), D, iters, int(infer_seed) & (2**64 - 1), _p(w),
-                             int(bool(faithful)), _p(theta), _p(ll), _p(z))
+        load().orc_infer_gen_seg(self._h, _p(words), _p(off), D, iters, int(infer_seed) & (2**64 - 1), _p(w),
+                                 int(bool(faithful)), int(seg_tokens), int(seg_max), _p(theta), _p(ll), _p(z))
         return theta, ll, z
 
     def infer_together(self, words, off, iters, infer_seed=0, workers=1, scheme=1, threads=1, weights=None):

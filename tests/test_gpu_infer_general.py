@@ -72,6 +72,15 @@ def test_term_weighted_fold_in_needs_weights():
     ctx, m, uw, uoff, uwt, V = _setup("idf", 20, seed=1)
     with pytest.raises(ValueError):
         ctx.infer(uw, uoff, 5)
+    bad = uwt.copy()
+    bad[3] = np.nan
+    with pytest.raises(ValueError):
+        ctx.infer(uw, uoff, 5, weights=bad)
+    bad[3] = -1.0
+    with pytest.raises(ValueError):
+        ctx.infer(uw, uoff, 5, weights=bad)
+    with pytest.raises(ValueError):
+        ctx.infer_together(uw, uoff, 5)
 
 
 @pytest.mark.parametrize("tw", [tp.TermWeight.IDF, tp.TermWeight.PMI])
@@ -105,3 +114,22 @@ def test_model_surface_infer_with_priors():
     doc = mdl.make_doc([vocab[int(w)] for w in words[int(off[0]):int(off[1])]])
     dist, ll = mdl.infer(doc, iterations=20)
     assert abs(sum(dist) - 1.0) < 1e-5 and np.isfinite(ll)
+
+
+@pytest.mark.parametrize("kind,K,T,M", [("prior", 20, 30, 8), ("idf", 200, 20, 8), ("pmi", 20, 12, 2),
+                                        ("pmi_prior", 500, 25, 8), ("idf", 1000, 40, 4)])
+def test_segmented_fold_in_bit_exact(kind, K, T, M):
+    """Documents longer than T tokens split into up to M wavefronts: topics,
+    theta bit-exact and LL to 1e-9 against the oracle's segmented rule,
+    batched (segment-count buckets on their own streams) and plain."""
+    ctx, m, uw, uoff, uwt, V = _setup(kind, K, seed=K + 3)
+    ctx.set_infer_options(T, M)
+    print(kind, "K", K, "seg", T, M, flush=True)
+    tho, llo, zo = m.infer_gen(uw, uoff, 9, infer_seed=31, weights=uwt, seg_tokens=T, seg_max=M)
+    _, _, zs = m.infer_gen(uw, uoff, 9, infer_seed=31, weights=uwt)
+    assert not np.array_equal(zo, zs)  # the rule does split these documents
+    for mode in (0, 1):
+        thg, llg, zg = ctx.infer(uw, uoff, 9, infer_seed=31, mode=mode, want_z=True, weights=uwt)
+        assert np.array_equal(zg, zo), "mode %d: z differs at %d tokens" % (mode, int((zg != zo).sum()))
+        assert np.array_equal(thg, tho), "mode %d theta" % mode
+        np.testing.assert_allclose(llg, llo, rtol=1e-9, atol=1e-9)

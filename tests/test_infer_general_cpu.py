@@ -271,3 +271,63 @@ def test_together_weighted_closed_forms(kind):
     ref = rest(nkw) - rest(nkw0.astype(np.float64)) + docs
     assert abs(ll - ref) <= 1e-4 * abs(ref)
     np.testing.assert_allclose(theta, (ndk + ALPHA) / (sw[:, None] + ALPHA * K), rtol=1e-5, atol=1e-6)
+
+
+# ----------------------------------------------------------- segmented ------
+@pytest.mark.parametrize("T,M", [(40, 8), (100, 2), (7, 4)])
+def test_segmented_one_is_the_integer_segmented_fold_in(T, M):
+    """Integer counts without priors: the segmented rule on running counts
+    is the integer fold-in's segmented rule (pinned by its own tests)."""
+    m, V, uw, uoff, _, _ = _setup("one")
+    th0, ll0, z0 = m.infer_gen(uw, uoff, 6, infer_seed=3, seg_tokens=T, seg_max=M)
+    th1, ll1, z1 = m.infer(uw, uoff, 6, infer_seed=3, mode=0, seg_tokens=T, seg_max=M)
+    assert np.array_equal(z0, z1)
+    assert np.array_equal(th0, th1)
+    np.testing.assert_allclose(ll0, ll1, rtol=1e-10)
+    # and it does split: the sequential sampler gives other topics
+    _, _, zs = m.infer_gen(uw, uoff, 6, infer_seed=3)
+    assert not np.array_equal(z0, zs)
+
+
+def test_segmented_unit_weights_are_the_integer_segmented_fold_in():
+    words, off, nw, _ = synth.workload("plain_k20", n_docs=50)
+    ws, offs, V, _ = corpus.prepare_arrays(words, off, nw, min_cf=2)
+    K = 10
+    mi = O.Model(K, V, ALPHA, ETA, 4)
+    mw = O.WeightedModel(K, V, ALPHA, ETA, 4)
+    for mm in (mi, mw):
+        mm.set_optim(0, 0)
+    mi.set_corpus(ws, offs)
+    mw.set_corpus(ws, offs, np.ones(len(ws), np.float32))
+    mi.train(3, workers=1, scheme=1)
+    mw.train(3, workers=1, scheme=1)
+    e = int(offs[6])
+    th0, ll0, z0 = mw.infer_gen(ws[:e], offs[:7], 5, infer_seed=2, weights=np.ones(e, np.float32), seg_tokens=30,
+                                seg_max=8)
+    th1, ll1, z1 = mi.infer_gen(ws[:e], offs[:7], 5, infer_seed=2, seg_tokens=30, seg_max=8)
+    assert np.array_equal(z0, z1)
+    assert np.array_equal(th0, th1)
+    np.testing.assert_allclose(ll0, ll1, rtol=1e-10)
+
+
+@pytest.mark.parametrize("kind", ["prior", "pmi", "pmi_prior"])
+def test_segmented_general_closed_forms(kind):
+    """Segmented fold-in of prior / PMI models: theta and the LL against the
+    closed forms from the returned topics (integer counts exactly; float
+    counts to their rounding)."""
+    K = 8
+    m, V, uw, uoff, uwt, eta_kv = _setup(kind, K=K)
+    st = m.state()
+    nkw, nk = st[2], st[3]
+    th, ll, z = m.infer_gen(uw, uoff, 8, infer_seed=4, weights=uwt, seg_tokens=25, seg_max=4)
+    e_kv = np.full((V, K), np.float32(ETA), np.float32) if eta_kv is None else eta_kv
+    es = (np.full(K, V * np.float64(np.float32(ETA))) if eta_kv is None else eta_kv.astype(np.float64).sum(axis=0))
+    alpha = np.full(K, np.float32(ALPHA))
+    for d in range(len(uoff) - 1):
+        b, e = int(uoff[d]), int(uoff[d + 1])
+        w = np.ones(e - b) if uwt is None else uwt[b:e]
+        ref, nd = _closed_form_ll(nkw, nk, e_kv, es, alpha, uw[b:e], V, z[b:e], w)
+        tol = 1e-9 if uwt is None else 1e-4
+        assert abs(ll[d] - ref) <= tol * max(1.0, abs(ref))
+        sw = float(np.asarray(w, np.float64)[uw[b:e] < V].sum())
+        np.testing.assert_allclose(th[d], (nd + ALPHA) / (sw + ALPHA * K), rtol=1e-5, atol=1e-6)

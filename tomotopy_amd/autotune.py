@@ -15,11 +15,12 @@ import numpy as np
 CANDIDATES = (0, 4096, 2048, 1024, 512, 256, 128, 64)
 
 
-def _sample(words, doc_off, max_docs, seed):
-    """All of the longest documents plus a uniform sample of the rest."""
+def _sample(words, doc_off, max_docs, seed, weights=None):
+    """All of the longest documents plus a uniform sample of the rest (and
+    their token weights, when given)."""
     D = len(doc_off) - 1
     if D <= max_docs:
-        return words, doc_off
+        return words, doc_off, weights
     lens = np.diff(doc_off.astype(np.int64))
     n_long = max_docs // 4
     longest = np.argsort(-lens, kind="stable")[:n_long]
@@ -29,29 +30,36 @@ def _sample(words, doc_off, max_docs, seed):
     sub_off = np.zeros(len(pick) + 1, np.uint64)
     sub_off[1:] = np.cumsum(lens[pick])
     sub_w = np.concatenate([words[int(doc_off[d]):int(doc_off[d + 1])] for d in pick]).astype(np.uint32)
-    return sub_w, sub_off
+    sub_t = None
+    if weights is not None:
+        sub_t = np.concatenate([weights[int(doc_off[d]):int(doc_off[d + 1])] for d in pick]).astype(np.float32)
+    return sub_w, sub_off, sub_t
 
 
 def tune_infer(ctx, words, doc_off, iterations, infer_seed=0, candidates=CANDIDATES, slack=0.10, max_docs=4000,
-               repeats=2, seed=0):
+               repeats=2, seed=0, weights=None):
     """Time the batched fold-in of a sample of (words, doc_off) for each
     candidate seg_tokens (with the largest segment count the kernel fits at
-    this K) and set the chosen one on the context.  Returns a dict with the
-    choice and the per-candidate kernel times (ms) and log-likelihoods per
-    word of the sample."""
+    this K) and set the chosen one on the context.  weights: the tokens'
+    weights of a term-weighted model.  Returns a dict with the choice and the
+    per-candidate kernel times (ms) and log-likelihoods per word (per unit of
+    weight) of the sample."""
     words = np.ascontiguousarray(words, np.uint32)
     doc_off = np.ascontiguousarray(doc_off, np.uint64)
-    sw, so = _sample(words, doc_off, max_docs, seed)
+    sw, so, st = _sample(words, doc_off, max_docs, seed, weights)
     n_valid = max(1, int(np.count_nonzero(sw < ctx.v)))
+    if st is not None:
+        n_valid = max(1e-30, float(np.sum(st[sw < ctx.v].astype(np.float64))))
     seg_max = ctx.infer_options()["seg_supported"]
     results = []
     for T in candidates:
         ctx.set_infer_options(int(T), seg_max if T else 1)
-        ctx.infer(sw[:int(so[min(8, len(so) - 1)])], so[:min(8, len(so) - 1) + 1], 1)  # graph capture
+        e8 = int(so[min(8, len(so) - 1)])
+        ctx.infer(sw[:e8], so[:min(8, len(so) - 1) + 1], 1, weights=None if st is None else st[:e8])  # graph capture
         best = None
         ll = None
         for _ in range(repeats):
-            _, ll_d, _ = ctx.infer(sw, so, iterations, infer_seed=infer_seed, mode=0)
+            _, ll_d, _ = ctx.infer(sw, so, iterations, infer_seed=infer_seed, mode=0, weights=st)
             ms = ctx.kernel_stats()["ms"]
             best = ms if best is None else min(best, ms)
             ll = float(ll_d.sum()) / n_valid

@@ -560,7 +560,7 @@ int lda_ctx_destroy(lda_ctx* c) {
     c->d_wt.release(); c->d_docw.release(); c->d_rowpart.release();
     c->d_prior_row.release(); c->d_prior_eta.release(); c->d_etasum.release(); c->d_etasum_d.release();
     c->g_wt.release(); c->g_docw.release(); c->g_jobrun.release(); c->g_runword.release();
-    c->g_cells.release(); c->g_ndk.release(); c->g_nk.release();
+    c->g_cells.release(); c->g_ndk.release(); c->g_nk.release(); c->g_segb.release(); c->g_segr.release();
     c->d_dglut.release(); c->d_aux.release(); c->d_bad.release(); c->d_hist.release(); c->d_hist_len.release();
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -1178,33 +1178,71 @@ static int check_infer_args(lda_ctx* c, const uint32_t* words, const uint64_t* d
     return LDA_OK;
 }
 
-// Fold-in of term-weighted models and of models with word priors: one
-// wavefront per document (longest first in mode 0, one launch per document in
-// mode 1), each run of equal word ids with a private column, then the
-// finalize kernel.  Arguments checked by check_infer_args.
+// Segment w's first token of an n-token document split into S segments (the
+// fold-in kernels' rule): floor(n*w/S), moved forward to a word-run start.
+static uint64_t seg_start_host(const uint32_t* w, uint64_t n, int i, int S) {
+    if (i <= 0) return 0;
+    if (i >= S) return n;
+    uint64_t x = n * (uint64_t)i / (uint64_t)S;
+    if (x == 0) return 0;
+    while (x < n && w[x] == w[x - 1]) ++x;
+    return x;
+}
+
+// Fold-in of term-weighted models and of models with word priors: each run
+// of equal word ids with a private column; documents of more than seg_tokens
+// tokens split into S segments (the integer fold-in's rule and options).
+// Mode 0: jobs grouped by S, longest first, one persistent launch per
+// segment count on the fold-in streams, then the finalize kernel; mode 1:
+// one launch per document.  Arguments checked by check_infer_args.
 static int infer_general(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off, int64_t D, const float* weights,
                          int iterations, uint64_t infer_seed, int mode, float* theta, double* ll, uint16_t* z_out) {
     const bool wt = c->tw != LDA_TW_ONE;
     const uint64_t N = doc_off[D];
+    constexpr int RW = lda::kMaxFoldSegments + 1;
+    int per_cu[lda::kFoldBuckets] = {};
+    int gen_max = 1;
+    for (int bk = 0; bk < lda::kFoldBuckets; ++bk) {
+        const int sg = 1 << bk;
+        per_cu[bk] = sg <= c->seg_max ? lda::infer_gen_blocks_per_cu(c->kpl, wt, c->has_priors, sg) : 0;
+        if (per_cu[bk] <= 0) break;
+        gen_max = sg;
+    }
+    if (per_cu[0] <= 0) return fail(LDA_ERR_DEVICE, "fold-in kernel does not fit the device for K = %d", c->K);
+    std::vector<int> segs(D);
+    for (int64_t d = 0; d < D; ++d) segs[d] = lda::fold_segments(doc_off[d + 1] - doc_off[d], c->seg_tokens, gen_max);
     std::vector<uint32_t> order(D);
     std::iota(order.begin(), order.end(), 0u);
     if (mode == 0)
         std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+            if (segs[a] != segs[b]) return segs[a] < segs[b];
             return (doc_off[a + 1] - doc_off[a]) > (doc_off[b + 1] - doc_off[b]);
         });
     std::vector<lda::InferJob> jobs(D);
-    std::vector<int64_t> job_run(D + 1);
+    std::vector<int64_t> job_run(D + 1), seg_r((size_t)D * RW, 0);
+    std::vector<uint64_t> seg_b((size_t)D * RW, 0);
     std::vector<uint32_t> run_word;
+    std::vector<uint64_t> run_tok;  // first token of each run of the current document (relative)
     for (int64_t i = 0; i < D; ++i) {
         const uint32_t d = order[i];
-        jobs[i] = lda::InferJob{doc_off[d], doc_off[d + 1], 0, iterations, iterations, (int32_t)d};
+        const uint64_t b = doc_off[d], n = doc_off[d + 1] - b;
+        jobs[i] = lda::InferJob{b, doc_off[d + 1], 0, iterations, iterations, (int32_t)d};
         job_run[i] = (int64_t)run_word.size();
+        run_tok.clear();
         int64_t prev = -1;
-        for (uint64_t g = doc_off[d]; g < doc_off[d + 1]; ++g) {
+        for (uint64_t g = b; g < b + n; ++g) {
             const uint32_t v = words[g];
             if (v >= (uint32_t)c->V || (int64_t)v == prev) continue;
             run_word.push_back(v);
+            run_tok.push_back(g - b);
             prev = v;
+        }
+        const int S = segs[d];
+        for (int w = 0; w <= S; ++w) {
+            const uint64_t x = seg_start_host(words + b, n, w, S);
+            seg_b[(size_t)i * RW + w] = b + x;
+            seg_r[(size_t)i * RW + w] =
+                job_run[i] + (int64_t)(std::lower_bound(run_tok.begin(), run_tok.end(), x) - run_tok.begin());
         }
     }
     job_run[D] = (int64_t)run_word.size();
@@ -1212,6 +1250,15 @@ static int infer_general(lda_ctx* c, const uint32_t* words, const uint64_t* doc_
     const double cell_bytes = (double)R * c->Kp * 4.0;
     if (cell_bytes > 64e9)
         return fail(LDA_ERR_OUT_OF_MEMORY, "fold-in batch needs %.1f GB of private columns; split the batch", cell_bytes / 1e9);
+    int64_t lo[lda::kFoldBuckets] = {}, hi[lda::kFoldBuckets] = {};
+    if (mode == 0) {
+        int64_t i = 0;
+        for (int bk = 0; bk < lda::kFoldBuckets; ++bk) {
+            lo[bk] = i;
+            while (i < D && lda::fold_bucket(segs[order[i]]) == bk) ++i;
+            hi[bk] = i;
+        }
+    }
     HIP_TRY(hipSetDevice(c->device));
     const hipStream_t s = c->stream;
     HIP_TRY(c->i_words.alloc(N));
@@ -1222,6 +1269,8 @@ static int infer_general(lda_ctx* c, const uint32_t* words, const uint64_t* doc_
     HIP_TRY(c->i_ll.alloc(D));
     if (wt) HIP_TRY(c->g_wt.alloc(N));
     HIP_TRY(c->g_jobrun.alloc(D + 1));
+    HIP_TRY(c->g_segr.alloc((size_t)D * RW));
+    HIP_TRY(c->g_segb.alloc((size_t)D * RW));
     HIP_TRY(c->g_runword.alloc(R));
     HIP_TRY(c->g_cells.alloc(R * (size_t)c->Kp));
     HIP_TRY(c->g_ndk.alloc((size_t)D * c->Kp));
@@ -1231,11 +1280,14 @@ static int infer_general(lda_ctx* c, const uint32_t* words, const uint64_t* doc_
     if (wt && N) HIP_TRY(hipMemcpyAsync(c->g_wt.p, weights, N * 4, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(c->i_jobs.p, jobs.data(), D * sizeof(lda::InferJob), hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(c->g_jobrun.p, job_run.data(), (D + 1) * 8, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(c->g_segr.p, seg_r.data(), seg_r.size() * 8, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(c->g_segb.p, seg_b.data(), seg_b.size() * 8, hipMemcpyHostToDevice, s));
     if (R) HIP_TRY(hipMemcpyAsync(c->g_runword.p, run_word.data(), R * 4, hipMemcpyHostToDevice, s));
     lda::InferGenParams p{};
     p.words = c->i_words.p; p.z = c->i_z.p; p.wt = wt ? c->g_wt.p : nullptr;
     p.jobs = c->i_jobs.p; p.njobs = D;
-    p.job_run = c->g_jobrun.p; p.run_word = c->g_runword.p; p.cells = c->g_cells.p;
+    p.job_run = c->g_jobrun.p; p.seg_b = c->g_segb.p; p.seg_r = c->g_segr.p;
+    p.run_word = c->g_runword.p; p.cells = c->g_cells.p;
     p.ndk_out = c->g_ndk.p; p.nk_out = c->g_nk.p; p.docw = c->g_docw.p;
     p.nkw = c->d_nkw.p; p.nk = c->d_nk.p; p.alpha = c->d_alpha.p; p.alpha_cdf = c->d_alpha_cdf.p;
     p.eta = c->eta; p.veta = c->veta;
@@ -1251,23 +1303,40 @@ static int infer_general(lda_ctx* c, const uint32_t* words, const uint64_t* doc_
     float asf = 0.0f;
     for (int k = 0; k < c->K; ++k) { asd += (double)c->alpha[k]; asf = asf + c->alpha[k]; }
     p.asum_d = asd; p.asum_f = asf;
-    const int per_cu = lda::infer_gen_blocks_per_cu(c->kpl, wt, c->has_priors);
-    if (per_cu <= 0) return fail(LDA_ERR_DEVICE, "fold-in kernel does not fit the device for K = %d", c->K);
-    const int grid = (int)std::min<int64_t>((int64_t)c->num_cu * per_cu, D);
     c->stat_ms = c->stat_launches = c->stat_tokens = 0.0;
     HIP_TRY(hipEventRecord(c->ev0, s));
     if (mode == 0) {
-        HIP_TRY(lda::launch_fill_u32(c->i_counter.p, 0u, 1, s));
+        HIP_TRY(lda::launch_fill_u32(c->i_counter.p, 0u, lda::kFoldBuckets, s));
+        lda_host::FoldStreams& fs = c->fold_streams;
+        HIP_TRY(hipEventRecord(fs.fork, s));
+        // the longest documents (most segments) first, each segment count on its own stream
+        for (int bk = lda::kFoldBuckets - 1; bk >= 0; --bk) {
+            if (hi[bk] <= lo[bk]) continue;
+            lda::InferGenParams pb = p;
+            pb.jobs += lo[bk];
+            pb.job_run += lo[bk];
+            pb.seg_b += lo[bk] * RW;
+            pb.seg_r += lo[bk] * RW;
+            pb.njobs = hi[bk] - lo[bk];
+            pb.counter = c->i_counter.p + bk;
+            const int grid = (int)std::min<int64_t>((int64_t)c->num_cu * per_cu[bk], pb.njobs);
+            const hipStream_t sb = bk == 0 ? s : fs.aux[bk - 1];
+            if (bk) HIP_TRY(hipStreamWaitEvent(sb, fs.fork, 0));
+            HIP_TRY(lda::launch_infer_gen(c->kpl, 1 << bk, pb, grid, false, sb));
+            if (bk) HIP_TRY(hipEventRecord(fs.join[bk - 1], sb));
+        }
+        for (int bk = 1; bk < lda::kFoldBuckets; ++bk)
+            if (hi[bk] > lo[bk]) HIP_TRY(hipStreamWaitEvent(s, fs.join[bk - 1], 0));
         p.counter = c->i_counter.p;
-        HIP_TRY(lda::launch_infer_gen(c->kpl, p, grid, false, s));
-        HIP_TRY(lda::launch_infer_gen(c->kpl, p, grid, true, s));
+        const int fgrid = (int)std::min<int64_t>((int64_t)c->num_cu * 16, D);
+        HIP_TRY(lda::launch_infer_gen(c->kpl, 1, p, fgrid, true, s));
         c->stat_launches = 1;
     } else {
         p.counter = nullptr;
         for (int64_t d = 0; d < D; ++d) {
             p.job_base = d;
-            HIP_TRY(lda::launch_infer_gen(c->kpl, p, 1, false, s));
-            HIP_TRY(lda::launch_infer_gen(c->kpl, p, 1, true, s));
+            HIP_TRY(lda::launch_infer_gen(c->kpl, segs[d], p, 1, false, s));
+            HIP_TRY(lda::launch_infer_gen(c->kpl, 1, p, 1, true, s));
             HIP_TRY(hipStreamSynchronize(s));
         }
         c->stat_launches = (double)D;
@@ -1295,6 +1364,9 @@ int lda_infer_w(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off, int6
         return lda_infer(c, words, doc_off, D, iterations, infer_seed, mode, theta, ll, z_out);
     }
     if (!weights && doc_off[D]) return fail(LDA_ERR_INVALID_ARGUMENT, "term-weighted models need the token weights");
+    for (uint64_t g = 0; g < doc_off[D]; ++g)
+        if (words[g] < (uint32_t)c->V && !(weights[g] >= 0.0f && weights[g] < 1e30f))
+            return fail(LDA_ERR_INVALID_ARGUMENT, "token weight %llu is negative or not finite", (unsigned long long)g);
     return infer_general(c, words, doc_off, D, weights, iterations, infer_seed, mode, theta, ll, z_out);
 }
 

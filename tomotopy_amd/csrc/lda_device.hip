@@ -1922,12 +1922,37 @@ __device__ __forceinline__ SweepParams gen_eta_params(const InferGenParams& p) {
     return sp;
 }
 
+// LDS of the fold-in kernel below: per wavefront [A | col | nk | y | bf]
+// (Kp each), then for S > 1 the iteration-start n_dk and n_k (Kp each).
+template <int KPL>
+__device__ __forceinline__ WaveLds gen_wave_lds(int wave) {
+    constexpr int Kp = 64 * KPL;
+    int* b = lds_dyn + wave * 5 * Kp;
+    return WaveLds{reinterpret_cast<float*>(b), b + Kp, b + 2 * Kp, reinterpret_cast<float*>(b + 3 * Kp),
+                   reinterpret_cast<float*>(b + 4 * Kp)};
+}
+
+template <int KPL>
+constexpr size_t gen_lds_bytes(int segs) { return (size_t)(5 * segs + (segs > 1 ? 2 : 0)) * 64 * KPL * sizeof(int); }
+
+// S = blockDim.x / 64 wavefronts per document: S == 1 samples the document
+// sequentially; S > 1 is the segmented rule (segment w = tokens
+// [seg_b[w], seg_b[w+1]) of the job, its runs numbered from seg_r[w]): the
+// initial topics are added by wavefront 0 in token order; every iteration
+// each wavefront starts from the iteration-start n_dk / n_k, samples its
+// segment, and the changes are added to the snapshot in segment order.
 template <int KPL, bool WT, bool ASYM>
-__global__ void __launch_bounds__(64) infer_gen_kernel(InferGenParams p) {
+__global__ void __launch_bounds__(512) infer_gen_kernel(InferGenParams p) {
     constexpr int SH = kpl_shift<KPL>();
+    constexpr int Kp = 64 * KPL;
     const int lane = lane_id();
-    const WaveLds S = wave_lds<KPL>();
-    const WaveLdsW SW = wave_lds_w<KPL>();
+    const int S = (int)(blockDim.x >> 6);
+    const int wave = (int)(threadIdx.x >> 6);
+    const WaveLds L = gen_wave_lds<KPL>(wave);
+    const WaveLdsW LW{L.A, reinterpret_cast<float*>(L.col), reinterpret_cast<float*>(L.nk), L.y};
+    int32_t* snap_nd = lds_dyn + S * 5 * Kp;
+    int32_t* snap_nk = snap_nd + Kp;
+    __shared__ int64_t job_sh;
     int nvalid, lastLane;
     lane_limits<KPL>(p.K, lane, nvalid, lastLane);
     fvec<KPL> al;
@@ -1936,89 +1961,132 @@ __global__ void __launch_bounds__(64) infer_gen_kernel(InferGenParams p) {
     auto et = make_eta<KPL, ASYM>(sp, lane);
     for (;;) {
         int64_t idx;
-        if (p.counter) {
+        if (!p.counter) {
+            idx = p.job_base;
+        } else if (S == 1) {
             uint32_t t = 0;
             if (lane == 0) t = atomicAdd(p.counter, 1u);
             idx = (int64_t)(uint32_t)uniform_i((int)t);
         } else {
-            idx = p.job_base;
+            if (threadIdx.x == 0) job_sh = (int64_t)atomicAdd(p.counter, 1u);
+            __syncthreads();
+            idx = job_sh;
         }
         if (idx >= p.njobs) break;
         const InferJob J = p.jobs[idx];
         const uint64_t b = J.b, e = J.e;
-        int64_t run = p.job_run[idx] - 1;
-        // private n_k = the model's, n_dk = 0
-        if constexpr (WT) {
-            fvec<KPL> n, y, z0 = 0.0f;
-            load_kvec_f<KPL>(reinterpret_cast<const float*>(p.nk), lane, n);
+        if (wave == 0) {
+            // private n_k = the model's, n_dk = 0
+            if constexpr (WT) {
+                fvec<KPL> n, y, z0 = 0.0f;
+                load_kvec_f<KPL>(reinterpret_cast<const float*>(p.nk), lane, n);
 #pragma unroll
-            for (int j = 0; j < KPL; ++j) y[j] = recip_rn(n[j] + et.s(j));
-            lds_store_f<KPL>(SW.nk, lane, n);
-            lds_store_f<KPL>(SW.y, lane, y);
-            lds_store_f<KPL>(SW.nd, lane, z0);
-        } else {
-            set_nk<KPL>(S, p.nk, lane, et);
-            ivec<KPL> z0 = 0;
-            set_ndk<KPL>(S, al, z0, lane);
-        }
-        // initial topics, added in token order; each run's column starts as the model's
-        float sw = 0.0f;
-        int vcur = -1;
-        int32_t* colp = nullptr;
-        for (uint64_t t0 = b; t0 < e; t0 += 64) {
-            const int n = (int)((e - t0) < 64 ? (e - t0) : 64);
-            const uint64_t g = t0 + (uint64_t)lane;
-            int wv = -1, zz = 0xFFFF;
-            float tw = 1.0f;
-            if (lane < n) {
-                wv = (int)p.words[g];
-                if constexpr (WT) tw = p.wt[g];
-                if ((unsigned)wv < (unsigned)p.V) {
-                    const float u = uniform24(philox_x0(p.seed_lo, p.seed_hi, (uint32_t)(g - b), (uint32_t)((g - b) >> 32), 0u, TAG_INFINIT));
-                    if constexpr (ASYM) {
-                        const int r = p.prior_row[wv];
-                        zz = draw_from_eta_column(r >= 0 ? p.prior_eta + (int64_t)r * p.Kp : nullptr, KPL, p.eta, p.K, u);
-                    } else {
-                        const float rr = u * p.alpha_cdf[p.K - 1];
-                        int lo = 0, hi = p.K;
-                        while (lo < hi) {
-                            const int mid = (lo + hi) >> 1;
-                            if (rr < p.alpha_cdf[mid]) hi = mid; else lo = mid + 1;
-                        }
-                        zz = lo < p.K ? lo : p.K - 1;
-                    }
-                }
-                p.z[g] = (uint16_t)zz;
+                for (int j = 0; j < KPL; ++j) y[j] = recip_rn(n[j] + et.s(j));
+                lds_store_f<KPL>(LW.nk, lane, n);
+                lds_store_f<KPL>(LW.y, lane, y);
+                lds_store_f<KPL>(LW.nd, lane, z0);
+            } else {
+                set_nk<KPL>(L, p.nk, lane, et);
+                ivec<KPL> z0 = 0;
+                set_ndk<KPL>(L, al, z0, lane);
             }
-            for (int i = 0; i < n; ++i) {
-                const int v = readlane_i(wv, i);
-                if ((unsigned)v >= (unsigned)p.V) continue;
-                if (v != vcur) {
-                    if (colp) { ivec<KPL> c; lds_load<KPL>(S.col, lane, c); store_kvec<KPL>(colp, lane, c); }
-                    ++run;
-                    colp = p.cells + run * p.Kp;
-                    ivec<KPL> c;
-                    load_kvec<KPL>(p.nkw + (int64_t)v * p.Kp, lane, c);
-                    lds_store<KPL>(S.col, lane, c);
-                    vcur = v;
-                }
-                const int k = readlane_i(zz, i);
-                if constexpr (WT) {
-                    const float w = readlane_f(tw, i);
-                    bump_w<KPL, false>(SW, lane, k, w, false, et);
-                    sw += w;
-                } else {
-                    bump<KPL>(S, al, lane, k, +1, et);
-                    sw += 1.0f;
-                }
-            }
-        }
-        if (colp) { ivec<KPL> c; lds_load<KPL>(S.col, lane, c); store_kvec<KPL>(colp, lane, c); }
-        for (int it = 1; it <= J.iters; ++it) {
-            run = p.job_run[idx] - 1;
-            vcur = -1;
+            // initial topics, added in token order; each run's column starts as the model's
+            int64_t run = p.job_run[idx] - 1;
+            float sw = 0.0f;
+            int vcur = -1;
+            int32_t* colp = nullptr;
             for (uint64_t t0 = b; t0 < e; t0 += 64) {
                 const int n = (int)((e - t0) < 64 ? (e - t0) : 64);
+                const uint64_t g = t0 + (uint64_t)lane;
+                int wv = -1, zz = 0xFFFF;
+                float tw = 1.0f;
+                if (lane < n) {
+                    wv = (int)p.words[g];
+                    if constexpr (WT) tw = p.wt[g];
+                    if ((unsigned)wv < (unsigned)p.V) {
+                        const float u = uniform24(philox_x0(p.seed_lo, p.seed_hi, (uint32_t)(g - b), (uint32_t)((g - b) >> 32), 0u, TAG_INFINIT));
+                        if constexpr (ASYM) {
+                            const int r = p.prior_row[wv];
+                            zz = draw_from_eta_column(r >= 0 ? p.prior_eta + (int64_t)r * p.Kp : nullptr, KPL, p.eta, p.K, u);
+                        } else {
+                            const float rr = u * p.alpha_cdf[p.K - 1];
+                            int lo = 0, hi = p.K;
+                            while (lo < hi) {
+                                const int mid = (lo + hi) >> 1;
+                                if (rr < p.alpha_cdf[mid]) hi = mid; else lo = mid + 1;
+                            }
+                            zz = lo < p.K ? lo : p.K - 1;
+                        }
+                    }
+                    p.z[g] = (uint16_t)zz;
+                }
+                for (int i = 0; i < n; ++i) {
+                    const int v = readlane_i(wv, i);
+                    if ((unsigned)v >= (unsigned)p.V) continue;
+                    if (v != vcur) {
+                        if (colp) { ivec<KPL> c; lds_load<KPL>(L.col, lane, c); store_kvec<KPL>(colp, lane, c); }
+                        ++run;
+                        colp = p.cells + run * p.Kp;
+                        ivec<KPL> c;
+                        load_kvec<KPL>(p.nkw + (int64_t)v * p.Kp, lane, c);
+                        lds_store<KPL>(L.col, lane, c);
+                        vcur = v;
+                    }
+                    const int k = readlane_i(zz, i);
+                    if constexpr (WT) {
+                        const float w = readlane_f(tw, i);
+                        bump_w<KPL, false>(LW, lane, k, w, false, et);
+                        sw += w;
+                    } else {
+                        bump<KPL>(L, al, lane, k, +1, et);
+                        sw += 1.0f;
+                    }
+                }
+            }
+            if (colp) { ivec<KPL> c; lds_load<KPL>(L.col, lane, c); store_kvec<KPL>(colp, lane, c); }
+            if (lane == 0) p.docw[J.slot] = sw;
+            if (S > 1) {
+                ivec<KPL> a, c;
+                if constexpr (WT) {
+                    fvec<KPL> x;
+                    lds_load_f<KPL>(LW.nd, lane, x);
+                    a = __builtin_bit_cast(ivec<KPL>, x);
+                } else {
+                    a = get_ndk<KPL>(L, al, lane);
+                }
+                lds_load<KPL>(L.nk, lane, c);
+                lds_store<KPL>(snap_nd, lane, a);
+                lds_store<KPL>(snap_nk, lane, c);
+            }
+        }
+        const int64_t sbase = idx * (kMaxFoldSegments + 1);
+        const uint64_t sb = S > 1 ? p.seg_b[sbase + wave] : b;
+        const uint64_t se = S > 1 ? p.seg_b[sbase + wave + 1] : e;
+        const int64_t run0 = S > 1 ? p.seg_r[sbase + wave] : p.job_run[idx];
+        for (int it = 1; it <= J.iters; ++it) {
+            if (S > 1) {
+                __syncthreads();  // the snapshot (initial state or the last merge) is complete
+                ivec<KPL> a, c;
+                lds_load<KPL>(snap_nd, lane, a);
+                lds_load<KPL>(snap_nk, lane, c);
+                if constexpr (WT) {
+                    const fvec<KPL> n = __builtin_bit_cast(fvec<KPL>, c);
+                    fvec<KPL> y;
+#pragma unroll
+                    for (int j = 0; j < KPL; ++j) y[j] = recip_rn(n[j] + et.s(j));
+                    lds_store<KPL>(reinterpret_cast<int*>(L.A), lane, a);
+                    lds_store<KPL>(L.nk, lane, c);
+                    lds_store_f<KPL>(LW.y, lane, y);
+                } else {
+                    set_ndk<KPL>(L, al, a, lane);
+                    set_nk_vec<KPL>(L, c, lane, et);
+                }
+            }
+            int64_t run = run0 - 1;
+            int vcur = -1;
+            int32_t* colp = nullptr;
+            for (uint64_t t0 = sb; t0 < se; t0 += 64) {
+                const int n = (int)((se - t0) < 64 ? (se - t0) : 64);
                 const uint64_t g = t0 + (uint64_t)lane;
                 int wv = -1, zv = 0xFFFF;
                 float uv = 0.0f, tw = 1.0f;
@@ -2039,7 +2107,7 @@ __global__ void __launch_bounds__(64) infer_gen_kernel(InferGenParams p) {
                         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
                         ivec<KPL> c;
                         load_kvec<KPL>(colp, lane, c);
-                        lds_store<KPL>(S.col, lane, c);
+                        lds_store<KPL>(L.col, lane, c);
                         et.word(sp, v, lane);
                         vcur = v;
                     }
@@ -2047,34 +2115,71 @@ __global__ void __launch_bounds__(64) infer_gen_kernel(InferGenParams p) {
                     const float u = readlane_f(uv, i);
                     int zn;
                     if constexpr (WT) {
-                        zn = token_step_w<KPL, false>(SW, al, lane, zo, readlane_f(tw, i), u, p.K, nvalid, lastLane, et);
+                        zn = token_step_w<KPL, false>(LW, al, lane, zo, readlane_f(tw, i), u, p.K, nvalid, lastLane, et);
                     } else {
-                        zn = token_step<KPL>(S, al, lane, zo, u, p.K, nvalid, lastLane, et, nullptr);
+                        zn = token_step<KPL>(L, al, lane, zo, u, p.K, nvalid, lastLane, et, nullptr);
                     }
                     const int qo = kpos<KPL>(lane, zo & (KPL - 1)), qn = kpos<KPL>(lane, zn & (KPL - 1));
-                    if (lane == (zo >> SH)) colp[qo] = S.col[qo];
-                    if (lane == (zn >> SH)) colp[qn] = S.col[qn];
+                    if (lane == (zo >> SH)) colp[qo] = L.col[qo];
+                    if (lane == (zn >> SH)) colp[qn] = L.col[qn];
                     zkeep = (lane == i) ? zn : zkeep;
                 }
                 if (lane < n) p.z[g] = (uint16_t)zkeep;
+            }
+            if (S > 1) {
+                __syncthreads();  // every segment is done
+                // snapshot + the segments' changes, in segment order
+                for (int q = (int)threadIdx.x; q < Kp; q += (int)blockDim.x) {
+                    if constexpr (WT) {
+                        const float a0 = __int_as_float(snap_nd[q]), c0 = __int_as_float(snap_nk[q]);
+                        float a = a0, c = c0;
+                        for (int w = 0; w < S; ++w) {
+                            const int* wb = lds_dyn + w * 5 * Kp;
+                            a = a + (__int_as_float(wb[q]) - a0);
+                            c = c + (__int_as_float(wb[2 * Kp + q]) - c0);
+                        }
+                        snap_nd[q] = __float_as_int(a);
+                        snap_nk[q] = __float_as_int(c);
+                    } else {
+                        const float aq = p.alpha[q];
+                        const int a0 = snap_nd[q], c0 = snap_nk[q];
+                        int a = a0, c = c0;
+                        for (int w = 0; w < S; ++w) {
+                            const int* wb = lds_dyn + w * 5 * Kp;
+                            a += (int)__builtin_rintf(__int_as_float(wb[q]) - aq) - a0;
+                            c += wb[2 * Kp + q] - c0;
+                        }
+                        snap_nd[q] = a;
+                        snap_nk[q] = c;
+                    }
+                }
             }
         }
         // the document's n_dk and n_k for the finalize kernel
         int32_t* ndo = p.ndk_out + (int64_t)J.slot * p.Kp;
         int32_t* nko = p.nk_out + (int64_t)J.slot * p.Kp;
-        if constexpr (WT) {
+        if (S > 1) {
+            __syncthreads();  // the last merge (or wavefront 0's initial state) is complete
+            if (wave == 0) {
+                ivec<KPL> a, c;
+                lds_load<KPL>(snap_nd, lane, a);
+                lds_load<KPL>(snap_nk, lane, c);
+                store_kvec<KPL>(ndo, lane, a);
+                store_kvec<KPL>(nko, lane, c);
+            }
+            __syncthreads();  // every wavefront is done with this job before job_sh changes
+        } else if constexpr (WT) {
             fvec<KPL> a, c;
-            lds_load_f<KPL>(SW.nd, lane, a);
-            lds_load_f<KPL>(SW.nk, lane, c);
+            lds_load_f<KPL>(LW.nd, lane, a);
+            lds_load_f<KPL>(LW.nk, lane, c);
             store_kvec<KPL>(ndo, lane, __builtin_bit_cast(ivec<KPL>, a));
             store_kvec<KPL>(nko, lane, __builtin_bit_cast(ivec<KPL>, c));
         } else {
-            store_kvec<KPL>(ndo, lane, get_ndk<KPL>(S, al, lane));
+            store_kvec<KPL>(ndo, lane, get_ndk<KPL>(L, al, lane));
             ivec<KPL> c;
-            lds_load<KPL>(S.nk, lane, c);
+            lds_load<KPL>(L.nk, lane, c);
             store_kvec<KPL>(nko, lane, c);
         }
-        if (lane == 0) p.docw[J.slot] = sw;
         if (!p.counter) break;
     }
 }
@@ -2500,54 +2605,57 @@ hipError_t launch_infer(int kpl, int segs, const InferParams* p_dev, int grid, h
 }
 
 template <int KPL, bool WT, bool ASYM>
-static hipError_t launch_infer_gen_k(const InferGenParams& p, int grid, bool finalize, hipStream_t s) {
+static hipError_t launch_infer_gen_k(int segs, const InferGenParams& p, int grid, bool finalize, hipStream_t s) {
     if (finalize)
         hipLaunchKernelGGL((infer_gen_finalize_kernel<KPL, WT, ASYM>), dim3(grid), dim3(64), 0, s, p);
     else
-        hipLaunchKernelGGL((infer_gen_kernel<KPL, WT, ASYM>), dim3(grid), dim3(64), wave_lds_bytes<KPL>(), s, p);
+        hipLaunchKernelGGL((infer_gen_kernel<KPL, WT, ASYM>), dim3(grid), dim3(64 * segs), gen_lds_bytes<KPL>(segs), s, p);
     return hipGetLastError();
 }
 
 template <bool WT, bool ASYM>
-static hipError_t launch_infer_gen_t(int kpl, const InferGenParams& p, int grid, bool finalize, hipStream_t s) {
+static hipError_t launch_infer_gen_t(int kpl, int segs, const InferGenParams& p, int grid, bool finalize, hipStream_t s) {
     switch (kpl) {
-        case 1: return launch_infer_gen_k<1, WT, ASYM>(p, grid, finalize, s);
-        case 2: return launch_infer_gen_k<2, WT, ASYM>(p, grid, finalize, s);
-        case 4: return launch_infer_gen_k<4, WT, ASYM>(p, grid, finalize, s);
-        case 8: return launch_infer_gen_k<8, WT, ASYM>(p, grid, finalize, s);
-        case 16: return launch_infer_gen_k<16, WT, ASYM>(p, grid, finalize, s);
-        case 32: return launch_infer_gen_k<32, WT, ASYM>(p, grid, finalize, s);
+        case 1: return launch_infer_gen_k<1, WT, ASYM>(segs, p, grid, finalize, s);
+        case 2: return launch_infer_gen_k<2, WT, ASYM>(segs, p, grid, finalize, s);
+        case 4: return launch_infer_gen_k<4, WT, ASYM>(segs, p, grid, finalize, s);
+        case 8: return launch_infer_gen_k<8, WT, ASYM>(segs, p, grid, finalize, s);
+        case 16: return launch_infer_gen_k<16, WT, ASYM>(segs, p, grid, finalize, s);
+        case 32: return launch_infer_gen_k<32, WT, ASYM>(segs, p, grid, finalize, s);
         default: return hipErrorInvalidValue;
     }
 }
 
-hipError_t launch_infer_gen(int kpl, const InferGenParams& p, int grid, bool finalize, hipStream_t s) {
+hipError_t launch_infer_gen(int kpl, int segs, const InferGenParams& p, int grid, bool finalize, hipStream_t s) {
+    if (segs != 1 && segs != 2 && segs != 4 && segs != kMaxFoldSegments) return hipErrorInvalidValue;
     const bool wt = p.wt != nullptr, asym = p.prior_row != nullptr;
-    if (wt) return asym ? launch_infer_gen_t<true, true>(kpl, p, grid, finalize, s)
-                        : launch_infer_gen_t<true, false>(kpl, p, grid, finalize, s);
-    return asym ? launch_infer_gen_t<false, true>(kpl, p, grid, finalize, s)
-                : launch_infer_gen_t<false, false>(kpl, p, grid, finalize, s);
+    if (wt) return asym ? launch_infer_gen_t<true, true>(kpl, segs, p, grid, finalize, s)
+                        : launch_infer_gen_t<true, false>(kpl, segs, p, grid, finalize, s);
+    return asym ? launch_infer_gen_t<false, true>(kpl, segs, p, grid, finalize, s)
+                : launch_infer_gen_t<false, false>(kpl, segs, p, grid, finalize, s);
 }
 
 template <int KPL>
-static int infer_gen_blocks_kpl(bool wt, bool asym) {
+static int infer_gen_blocks_kpl(bool wt, bool asym, int segs) {
     int n = 0;
+    const size_t lds = gen_lds_bytes<KPL>(segs);
+    if (lds > 160 * 1024) return 0;
     hipError_t e;
-    if (wt) e = asym ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_gen_kernel<KPL, true, true>, 64, wave_lds_bytes<KPL>())
-                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_gen_kernel<KPL, true, false>, 64, wave_lds_bytes<KPL>());
-    else e = asym ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_gen_kernel<KPL, false, true>, 64, wave_lds_bytes<KPL>())
-                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_gen_kernel<KPL, false, false>, 64, wave_lds_bytes<KPL>());
+    if (wt) e = asym ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_gen_kernel<KPL, true, true>, 64 * segs, lds)
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_gen_kernel<KPL, true, false>, 64 * segs, lds);
+    else e = asym ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_gen_kernel<KPL, false, true>, 64 * segs, lds)
+                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_gen_kernel<KPL, false, false>, 64 * segs, lds);
     return e == hipSuccess ? n : 0;
 }
 
-int infer_gen_blocks_per_cu(int kpl, bool wt, bool asym) {
+int infer_gen_blocks_per_cu(int kpl, bool wt, bool asym, int segs) {
     switch (kpl) {
-        case 1: return infer_gen_blocks_kpl<1>(wt, asym);
-        case 2: return infer_gen_blocks_kpl<2>(wt, asym);
-        case 4: return infer_gen_blocks_kpl<4>(wt, asym);
-        case 8: return infer_gen_blocks_kpl<8>(wt, asym);
-        case 16: return infer_gen_blocks_kpl<16>(wt, asym);
-        case 32: return infer_gen_blocks_kpl<32>(wt, asym);
+        case 1: return infer_gen_blocks_kpl<1>(wt, asym, segs);
+        case 2: return infer_gen_blocks_kpl<2>(wt, asym, segs);
+        case 4: return infer_gen_blocks_kpl<4>(wt, asym, segs);
+        case 8: return infer_gen_blocks_kpl<8>(wt, asym, segs);
+        case 16: return infer_gen_blocks_kpl<16>(wt, asym, segs);
+        case 32: return infer_gen_blocks_kpl<32>(wt, asym, segs);
         default: return 0;
     }
 }

@@ -69,6 +69,7 @@ struct InferJob {
 // S == 1 is the sequential sampler.  Jobs are grouped by S into buckets
 // S = 1, 2, 4, 8 (bucket log2 S), one launch each.
 constexpr int kFoldBuckets = 4;
+constexpr int kMaxFoldSegments = 8;
 
 inline int fold_segments(uint64_t n, uint32_t seg_tokens, int seg_max) {
     if (seg_tokens == 0 || n <= seg_tokens) return 1;
@@ -115,6 +116,11 @@ struct InferGenParams {
     uint32_t* counter;           // next job; NULL: the launch runs job job_base only
     int64_t job_base;
     const int64_t* job_run;      // njobs + 1
+    // segmented documents (S = workgroup size / 64 > 1): per job, rows of
+    // kMaxFoldSegments + 1 entries: segment w's first token (seg_b[w], w <= S)
+    // and the index of its first run (seg_r[w])
+    const uint64_t* seg_b;
+    const int64_t* seg_r;
     const uint32_t* run_word;
     int32_t* cells;              // runs x Kp
     int32_t* ndk_out;            // slots x Kp, lane layout
@@ -139,8 +145,9 @@ struct InferGenParams {
 };
 // finalize = false: the sampler (grid persistent workgroups of one
 // wavefront); true: theta and the log-likelihood.
-hipError_t launch_infer_gen(int kpl, const InferGenParams& p, int grid, bool finalize, hipStream_t s);
-int infer_gen_blocks_per_cu(int kpl, bool wt, bool asym);
+// segs: wavefronts per document (1, 2, 4, 8) of the sampler launch.
+hipError_t launch_infer_gen(int kpl, int segs, const InferGenParams& p, int grid, bool finalize, hipStream_t s);
+int infer_gen_blocks_per_cu(int kpl, bool wt, bool asym, int segs);
 
 // Term-weighted (IDF / PMI) training: the count buffers hold float32 values.
 hipError_t launch_sweep_device_w(int kpl, const SweepParams& p, int grid, hipStream_t s);
