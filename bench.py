@@ -246,12 +246,12 @@ def infer_bench(plain_sample=300, iterations=100):
 def infer_pmi_bench(iterations=100, plain_sample=200, cpu_budget_s=8.0):
     """Fold-in of a TermWeight.PMI model (K=200, the fold-in workload): the
     token weights of the unseen documents from the model's vocabulary
-    weights, each document on a private copy of the float counts (one
-    wavefront per document, longest first), against plain (one launch per
-    document) and the oracle's fold-in on the host cores."""
+    weights, each document on a private copy of the float counts, batched
+    with the autotuned segmentation (and sequentially), against plain (one
+    launch per document) and the oracle's fold-in on the host cores."""
     from concurrent.futures import ThreadPoolExecutor
     from oracle import oracle as O
-    from tomotopy_amd import _native, corpus, synth
+    from tomotopy_amd import _native, autotune, corpus, synth
     (tw, toff, nw), (uw, uoff) = synth.foldin_workload()
     ws, offs, V, order = corpus.prepare_arrays(tw, toff, nw)
     uws, uoffs = corpus.csr_from_arrays(uw, uoff, order)
@@ -264,8 +264,16 @@ def infer_pmi_bench(iterations=100, plain_sample=200, cpu_budget_s=8.0):
     ctx.upload_corpus(ws, offs, weights=wts)
     ctx.train(100, workers=0)
     ntok = int(np.count_nonzero(uws < V))
+    wsum = float(np.sum(uwt[uws < V].astype(np.float64)))
     e64 = int(uoffs[64])
     ctx.infer(uws[:e64], uoffs[:65], 2, weights=uwt[:e64])
+    # sequential sampler (every document exactly as tomotopy samples it)
+    t0 = time.perf_counter()
+    _, ll_seq, _ = ctx.infer(uws, uoffs, iterations, mode=0, weights=uwt)
+    seq_s = time.perf_counter() - t0
+    tune = autotune.tune_infer(ctx, uws, uoffs, iterations, weights=uwt)
+    log("PMI autotuner: seg_tokens %d seg_max %d (%s)" % (tune["seg_tokens"], tune["seg_max"], ", ".join(
+        "%d:%.0fms" % (c["seg_tokens"], c["kernel_ms"]) for c in tune["candidates"])))
     t0 = time.perf_counter()
     theta, ll, _ = ctx.infer(uws, uoffs, iterations, mode=0, weights=uwt)
     batched_s = time.perf_counter() - t0
@@ -281,11 +289,15 @@ def infer_pmi_bench(iterations=100, plain_sample=200, cpu_budget_s=8.0):
     th_p, ll_p, _ = ctx.infer(sub_w, sub_off, iterations, mode=1, weights=sub_t)
     plain_s = time.perf_counter() - t0
     same = bool(np.array_equal(th_p, theta[pick]) and np.array_equal(ll_p, ll[pick]))
+    llw, llw_seq = float(ll.sum() / wsum), float(ll_seq.sum() / wsum)
     out = {"workload": "foldin_k200_pmi", "docs": D, "tokens": ntok, "iterations": iterations, "K": K, "V": V,
+           "seg_tokens": tune["seg_tokens"], "seg_max": tune["seg_max"],
            "batched_docs_per_s": D / batched_s, "batched_s": batched_s, "batched_kernel_ms": kst["ms"],
+           "sequential_docs_per_s": D / seq_s, "sequential_s": seq_s,
            "plain_docs_per_s": len(pick) / plain_s, "plain_sample_docs": int(len(pick)),
            "speedup": (D / batched_s) / (len(pick) / plain_s), "plain_equals_batched": same,
-           "ll_per_weight": float(ll.sum() / np.sum(uwt[uws < V].astype(np.float64)))}
+           "ll_per_weight": llw, "ll_per_weight_sequential": llw_seq,
+           "ll_per_weight_rel_diff": abs(llw - llw_seq) / abs(llw_seq)}
     # the oracle's fold-in on the host's cores, on the device's trained state
     threads = min(os.cpu_count() or 1, 16)
     z, ndk, nkw, nk = ctx.download_state_f()
@@ -314,9 +326,10 @@ def infer_pmi_bench(iterations=100, plain_sample=200, cpu_budget_s=8.0):
                            "sample": "%d random docs of the workload, %d iterations each, oracle fold-in of the "
                                      "PMI model (private columns per document), %d threads, %.1fs"
                                      % (done, iterations, threads, dt)}
-    log("PMI fold-in: batched %.0f docs/s (%.3fs, kernels %.1f ms), plain %.1f docs/s on %d docs, identical=%s; "
-        "CPU %.0f docs/s" % (out["batched_docs_per_s"], batched_s, kst["ms"], out["plain_docs_per_s"], len(pick),
-                              same, out["cpu_baseline"]["value"]))
+    log("PMI fold-in: batched %.0f docs/s (%.3fs, kernels %.1f ms), sequential %.0f docs/s, plain %.1f docs/s on %d "
+        "docs, identical=%s, ll per weight %.5f vs sequential %.5f; CPU %.0f docs/s"
+        % (out["batched_docs_per_s"], batched_s, kst["ms"], out["sequential_docs_per_s"], out["plain_docs_per_s"],
+           len(pick), same, llw, llw_seq, out["cpu_baseline"]["value"]))
     return out
 
 

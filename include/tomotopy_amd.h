@@ -179,8 +179,9 @@ int lda_sample_frozen(lda_ctx* ctx, int64_t n, const int64_t* doc, const uint32_
  * theta: D x K floats, ll: D doubles (FP64), z_out: N uint16 (may be NULL).
  * Models with word priors run tomoto's per-document sampler with eta_kv
  * (initial topics from the word's eta column, LDAModel.hpp:764-775); the
- * segmentation options do not apply to them.  Term-weighted models need the
- * token weights: lda_infer_w. */
+ * segmentation options apply to them as well, with at most the segment count
+ * whose state fits the LDS at this K.  Term-weighted models need the token
+ * weights: lda_infer_w. */
 int lda_infer(lda_ctx* ctx, const uint32_t* words, const uint64_t* doc_off, int64_t D, int iterations,
               uint64_t infer_seed, int mode, float* theta, double* ll, uint16_t* z_out);
 

@@ -1935,19 +1935,21 @@ __device__ __forceinline__ WaveLds gen_wave_lds(int wave) {
 template <int KPL>
 constexpr size_t gen_lds_bytes(int segs) { return (size_t)(5 * segs + (segs > 1 ? 2 : 0)) * 64 * KPL * sizeof(int); }
 
-// S = blockDim.x / 64 wavefronts per document: S == 1 samples the document
-// sequentially; S > 1 is the segmented rule (segment w = tokens
+// S wavefronts per document: S == 1 (SEG false, its own instantiation so the
+// sequential sampler carries none of the segment bookkeeping) samples the
+// document sequentially; S = blockDim.x / 64 > 1 (SEG) is the segmented rule (segment w = tokens
 // [seg_b[w], seg_b[w+1]) of the job, its runs numbered from seg_r[w]): the
 // initial topics are added by wavefront 0 in token order; every iteration
 // each wavefront starts from the iteration-start n_dk / n_k, samples its
 // segment, and the changes are added to the snapshot in segment order.
-template <int KPL, bool WT, bool ASYM>
-__global__ void __launch_bounds__(512) infer_gen_kernel(InferGenParams p) {
+template <int KPL, bool WT, bool ASYM, bool SEG>
+__global__ void __launch_bounds__(SEG ? 64 * kMaxFoldSegments : 64) infer_gen_kernel(InferGenParams p) {
     constexpr int SH = kpl_shift<KPL>();
     constexpr int Kp = 64 * KPL;
     const int lane = lane_id();
-    const int S = (int)(blockDim.x >> 6);
-    const int wave = (int)(threadIdx.x >> 6);
+    // SEG: S = 2, 4 or 8 wavefronts (the workgroup size); otherwise one
+    const int S = SEG ? (int)(blockDim.x >> 6) : 1;
+    const int wave = SEG ? (int)(threadIdx.x >> 6) : 0;
     const WaveLds L = gen_wave_lds<KPL>(wave);
     const WaveLdsW LW{L.A, reinterpret_cast<float*>(L.col), reinterpret_cast<float*>(L.nk), L.y};
     int32_t* snap_nd = lds_dyn + S * 5 * Kp;
@@ -2608,8 +2610,11 @@ template <int KPL, bool WT, bool ASYM>
 static hipError_t launch_infer_gen_k(int segs, const InferGenParams& p, int grid, bool finalize, hipStream_t s) {
     if (finalize)
         hipLaunchKernelGGL((infer_gen_finalize_kernel<KPL, WT, ASYM>), dim3(grid), dim3(64), 0, s, p);
+    else if (segs == 1)
+        hipLaunchKernelGGL((infer_gen_kernel<KPL, WT, ASYM, false>), dim3(grid), dim3(64), gen_lds_bytes<KPL>(1), s, p);
     else
-        hipLaunchKernelGGL((infer_gen_kernel<KPL, WT, ASYM>), dim3(grid), dim3(64 * segs), gen_lds_bytes<KPL>(segs), s, p);
+        hipLaunchKernelGGL((infer_gen_kernel<KPL, WT, ASYM, true>), dim3(grid), dim3(64 * segs), gen_lds_bytes<KPL>(segs),
+                           s, p);
     return hipGetLastError();
 }
 
@@ -2635,17 +2640,22 @@ hipError_t launch_infer_gen(int kpl, int segs, const InferGenParams& p, int grid
                 : launch_infer_gen_t<false, false>(kpl, segs, p, grid, finalize, s);
 }
 
+template <int KPL, bool SEG>
+static int infer_gen_blocks_seg(bool wt, bool asym, int segs, size_t lds) {
+    int n = 0;
+    hipError_t e;
+    if (wt) e = asym ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_gen_kernel<KPL, true, true, SEG>, 64 * segs, lds)
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_gen_kernel<KPL, true, false, SEG>, 64 * segs, lds);
+    else e = asym ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_gen_kernel<KPL, false, true, SEG>, 64 * segs, lds)
+                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_gen_kernel<KPL, false, false, SEG>, 64 * segs, lds);
+    return e == hipSuccess ? n : 0;
+}
+
 template <int KPL>
 static int infer_gen_blocks_kpl(bool wt, bool asym, int segs) {
-    int n = 0;
     const size_t lds = gen_lds_bytes<KPL>(segs);
     if (lds > 160 * 1024) return 0;
-    hipError_t e;
-    if (wt) e = asym ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_gen_kernel<KPL, true, true>, 64 * segs, lds)
-                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_gen_kernel<KPL, true, false>, 64 * segs, lds);
-    else e = asym ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_gen_kernel<KPL, false, true>, 64 * segs, lds)
-                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_gen_kernel<KPL, false, false>, 64 * segs, lds);
-    return e == hipSuccess ? n : 0;
+    return segs == 1 ? infer_gen_blocks_seg<KPL, false>(wt, asym, 1, lds) : infer_gen_blocks_seg<KPL, true>(wt, asym, segs, lds);
 }
 
 int infer_gen_blocks_per_cu(int kpl, bool wt, bool asym, int segs) {
