@@ -101,3 +101,22 @@ def test_copy_train():
     del mdl
     new.train(50)
     assert new.global_step == 100
+
+
+@pytest.mark.parametrize("ps", PARALLEL)
+def test_word_prior(ps):
+    """unit_test.py word_prior: each document's first word gets the prior
+    {n % k: 1} (the first prior set for a word stays), then training and
+    summary.  Iterations reduced from 2000."""
+    k = 10
+    mdl = tp.LDAModel(tw=0, min_df=2, rm_top=2, k=k)
+    n_prior = 0
+    for n, ch in enumerate(_lines()):
+        if len(ch) < 1:
+            continue
+        mdl.add_doc(ch)
+        mdl.set_word_prior(ch[0], {n % k: 1})
+        n_prior += 1
+    mdl.train(100, workers=1, parallel=ps)
+    assert np.isfinite(mdl.ll_per_word)
+    mdl.summary(file=open(os.devnull, "w"))

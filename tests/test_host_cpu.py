@@ -106,3 +106,39 @@ def test_device_calls_fail_loudly_without_gpu():
     m.add_doc(["a", "b"])
     with pytest.raises(RuntimeError):
         m.train(1)
+
+
+def test_every_public_attribute_of_a_new_model_is_readable():
+    """unit_test.py properties: every public attribute of a model that has not
+    been trained can be read (no device is needed for that)."""
+    import tomotopy_amd as tp
+    mdl = tp.LDAModel(tw=0, min_df=2, rm_top=2, k=10)
+    for attr in [a for a in dir(mdl) if not a.startswith("_")]:
+        getattr(mdl, attr)
+
+
+def test_uid_lookup_and_corpus_documents():
+    """unit_test.py test_uid / test_empty_uid: documents keep their uid from
+    a Corpus, the model's documents are found by uid, and a Corpus item
+    carries its words and uid."""
+    import pytest
+    import tomotopy_amd as tp
+    cps = tp.utils.Corpus()
+    cps.add_doc("test text".split(), uid="001")
+    cps.add_doc("test text".split(), uid="abc")
+    cps.add_doc("test text".split(), uid="0x1f")
+    mdl = tp.LDAModel(k=2, corpus=cps)
+    assert len(cps) == len(mdl.docs)
+    assert cps[0].uid == mdl.docs[0].uid == "001"
+    assert mdl.docs["abc"].uid == "abc" and mdl.docs["0x1f"].uid == "0x1f"
+    with pytest.raises(KeyError):
+        mdl.docs["nope"]
+    assert len(mdl.docs[:2]) == 2 and len(cps[:2]) == 2
+    assert cps[1].words == ["test", "text"] and cps[1][0] == "test"
+    empty = tp.utils.Corpus()
+    for _ in range(3):
+        empty.add_doc("test text".split())
+    m2 = tp.LDAModel(corpus=empty)
+    assert len(m2.docs) == 3 and empty[0].uid == m2.docs[0].uid == ""
+    with pytest.raises(ValueError):
+        mdl.add_doc("test text".split(), uid="001")  # a uid is unique in a model

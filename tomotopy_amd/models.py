@@ -19,7 +19,7 @@ import numpy as np
 from . import _native
 from . import corpus as _corpus
 from . import serialize as _serialize
-from .utils import Corpus, Document, InferredCorpus
+from .utils import Corpus, Document, DocumentList, InferredCorpus
 
 __version__ = "0.1.0"
 
@@ -526,9 +526,11 @@ call before `train` (py_LDA.cpp:241-269, LDAModel.hpp:1031-1045).'''
 
     @property
     def docs(self):
+        """The training documents: by position, slice, or uid (docs["uid"])."""
         if not self._prepared:
-            return [Document(self, d) for d in self._raw_docs]
-        return [Document(self, self._raw_docs[i], doc_index=i, uid=self._uids[i]) for i in range(len(self._raw_docs))]
+            return DocumentList([Document(self, d, uid=u) for d, u in zip(self._raw_docs, self._uids)], self._uid_map)
+        return DocumentList([Document(self, self._raw_docs[i], doc_index=i, uid=self._uids[i])
+                             for i in range(len(self._raw_docs))], self._uid_map)
 
     @property
     def vocabs(self):

@@ -92,8 +92,47 @@ class Document:
         return [(vocab[ids[i]], float(cnt[i]) / len(self._word_ids)) for i in order]
 
 
+class RawDocument:
+    """One document of a Corpus: its words (str) and uid (tomotopy.utils.Document
+    for a corpus that is not bound to a model)."""
+
+    def __init__(self, words, uid=""):
+        self.words = list(words)
+        self.uid = uid or ""
+
+    def __len__(self):
+        return len(self.words)
+
+    def __iter__(self):
+        return iter(self.words)
+
+    def __getitem__(self, i):
+        return self.words[i]
+
+    def __repr__(self):
+        return "<tomotopy_amd.utils.RawDocument with words=\"%s\">" % " ".join(self.words[:20])
+
+
+class DocumentList(list):
+    """A model's documents: indexed by position, by slice, or by uid."""
+
+    def __init__(self, docs, uid_map=None):
+        super().__init__(docs)
+        self._uid_map = uid_map or {}
+
+    def __getitem__(self, key):
+        if isinstance(key, str):
+            if key not in self._uid_map:
+                raise KeyError("cannot find a document with uid = {}".format(repr(key)))
+            return list.__getitem__(self, self._uid_map[key])
+        if isinstance(key, slice):
+            return DocumentList(list.__getitem__(self, key))
+        return list.__getitem__(self, key)
+
+
 class Corpus:
-    """A list of raw documents (lists of words) for LDAModel(corpus=...) and infer."""
+    """A list of raw documents (words and uid) for LDAModel(corpus=...),
+    add_corpus and infer (tomotopy/utils.py Corpus for pre-tokenised words)."""
 
     def __init__(self, docs=None):
         self._docs = []
@@ -112,10 +151,15 @@ class Corpus:
         return len(self._docs)
 
     def __iter__(self):
-        return iter(self._docs)
+        return (RawDocument(w, u) for w, u in zip(self._docs, self._uids))
 
     def __getitem__(self, i):
-        return self._docs[i]
+        if isinstance(i, slice):
+            sub = Corpus()
+            sub._docs = self._docs[i]
+            sub._uids = self._uids[i]
+            return sub
+        return RawDocument(self._docs[i], self._uids[i])
 
 
 class InferredCorpus(list):
