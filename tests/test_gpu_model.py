@@ -120,3 +120,22 @@ def test_word_prior(ps):
     mdl.train(100, workers=1, parallel=ps)
     assert np.isfinite(mdl.ll_per_word)
     mdl.summary(file=open(os.devnull, "w"))
+
+
+@pytest.mark.parametrize("ps", PARALLEL)
+def test_train_multi_corpus(ps):
+    """unit_test.py train_multi_corpus with pre-tokenised words: two corpora
+    added with add_corpus, trained, and each returned document's
+    log-likelihood read after training."""
+    lines = [ch for ch in _lines() if ch]
+    corpus1, corpus2 = tp.utils.Corpus(), tp.utils.Corpus()
+    for i, ch in enumerate(lines):
+        (corpus1 if i < 10 else corpus2).add_doc(ch)
+    mdl = tp.LDAModel(min_cf=2, rm_top=2, k=10)
+    t1 = mdl.add_corpus(corpus1)
+    t2 = mdl.add_corpus(corpus2)
+    mdl.train(50, parallel=ps)
+    assert len(t1) == 10 and len(t2) == len(lines) - 10
+    for d in list(t1[:10]) + list(t2[:10]):
+        assert np.isfinite(d.get_ll())
+        assert abs(sum(d.get_topic_dist()) - 1.0) < 1e-4

@@ -142,3 +142,19 @@ def test_uid_lookup_and_corpus_documents():
     assert len(m2.docs) == 3 and empty[0].uid == m2.docs[0].uid == ""
     with pytest.raises(ValueError):
         mdl.add_doc("test text".split(), uid="001")  # a uid is unique in a model
+
+
+def test_add_corpus_returns_the_model_documents():
+    """unit_test.py test_empty_uid: add_corpus returns the model's documents,
+    which can be added to a model again."""
+    import tomotopy_amd as tp
+    cps = tp.utils.Corpus()
+    for _ in range(3):
+        cps.add_doc("test text".split())
+    mdl = tp.LDAModel()
+    ccps = mdl.add_corpus(cps)
+    assert len(ccps) == 3 and ccps[0].words == mdl.docs[0].words
+    again = mdl.add_corpus(ccps)
+    assert len(again) == 3 and len(mdl.docs) == 6
+    assert [mdl.vocabs[i] for i in mdl.docs[5].words] == ["test", "text"]
+    assert len(ccps[:2]) == 2

@@ -19,7 +19,7 @@ import numpy as np
 from . import _native
 from . import corpus as _corpus
 from . import serialize as _serialize
-from .utils import Corpus, Document, DocumentList, InferredCorpus
+from .utils import Corpus, Document, DocumentList, InferredCorpus, ModelCorpus
 
 __version__ = "0.1.0"
 
@@ -179,16 +179,28 @@ transform : Callable[dict, dict]
         return len(self._raw_docs) - 1
 
     def add_corpus(self, corpus, transform=None):
+        """Add every document of a Corpus (or of another add_corpus result);
+        returns the model's new documents as a ModelCorpus."""
         if self._prepared:
             raise RuntimeError("cannot add_corpus() after train()")
-        if not isinstance(corpus, Corpus):
+        if isinstance(corpus, ModelCorpus):
+            src = [([corpus._model._vocab[i] for i in d._word_ids], d.uid) for d in corpus]
+        elif isinstance(corpus, Corpus):
+            src = list(zip(corpus._docs, corpus._uids))
+        else:
             raise ValueError("`corpus` must be an instance of `tomotopy_amd.utils.Corpus`")
         idx = []
-        for words, uid in zip(corpus._docs, corpus._uids):
+        for words, uid in src:
             r = self.add_doc(words, uid=uid or None)
             if r is not None:
                 idx.append(r)
-        return idx
+        return ModelCorpus(self, idx)
+
+    def _document(self, i):
+        """Training document i (its topics once the model is prepared)."""
+        if self._prepared:
+            return Document(self, self._raw_docs[i], doc_index=i, uid=self._uids[i])
+        return Document(self, self._raw_docs[i], uid=self._uids[i])
 
     def make_doc(self, words):
         '''Return a Document for an unseen document (out-of-dictionary words are dropped).'''

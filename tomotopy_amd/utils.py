@@ -130,6 +130,28 @@ class DocumentList(list):
         return list.__getitem__(self, key)
 
 
+class ModelCorpus:
+    """The documents a model took from a Corpus (the result of add_corpus):
+    each item is the model's Document (topics and log-likelihood after
+    training); by position, slice or iteration.  It can be added to a model
+    again with add_corpus."""
+
+    def __init__(self, model, indices):
+        self._model = model
+        self._idx = list(indices)
+
+    def __len__(self):
+        return len(self._idx)
+
+    def __iter__(self):
+        return (self._model._document(i) for i in self._idx)
+
+    def __getitem__(self, i):
+        if isinstance(i, slice):
+            return ModelCorpus(self._model, self._idx[i])
+        return self._model._document(self._idx[i])
+
+
 class Corpus:
     """A list of raw documents (words and uid) for LDAModel(corpus=...),
     add_corpus and infer (tomotopy/utils.py Corpus for pre-tokenised words)."""
