@@ -139,3 +139,23 @@ def test_train_multi_corpus(ps):
     for d in list(t1[:10]) + list(t2[:10]):
         assert np.isfinite(d.get_ll())
         assert abs(sum(d.get_topic_dist()) - 1.0) < 1e-4
+
+
+def test_train_infer_corpus_split():
+    """unit_test.py train_infer_raw_corpus with pre-tokenised words: train on
+    four fifths of a Corpus, infer the other fifth (a Corpus slice), read
+    each result's log-likelihood and words."""
+    corpus = tp.utils.Corpus()
+    for ch in _lines():
+        if ch:
+            corpus.add_doc(ch)
+    test_size = len(corpus) // 5
+    test_set, train_set = corpus[:test_size], corpus[test_size:]
+    mdl = tp.LDAModel(min_cf=2, rm_top=2, k=10, corpus=train_set)
+    mdl.train(50)
+    mdl.summary(file=open(os.devnull, "w"))
+    result, ll = mdl.infer(test_set)
+    assert len(result) == test_size and len(ll) == test_size and np.all(np.isfinite(ll))
+    for r in result:
+        assert np.isfinite(r.get_ll())
+        assert r[0] in mdl.vocabs

@@ -35,6 +35,10 @@ class Document:
     def __len__(self):
         return len(self._word_ids)
 
+    def __getitem__(self, i):
+        """The i-th word (str) of the document."""
+        return self._model.vocabs[int(self._word_ids[i])]
+
     def __repr__(self):
         vocab = self._model.vocabs
         return "<tomotopy_amd.Document with words=\"%s\">" % " ".join(vocab[i] for i in self._word_ids[:20])
