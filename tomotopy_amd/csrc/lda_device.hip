@@ -338,6 +338,44 @@ __device__ __forceinline__ void lane_limits(int K, int lane, int& nvalid, int& l
     lastLane = (K - 1) / KPL;
 }
 
+// The number of a lane's cdf values cdf_j = excl + w[j] (j < KPL) that are
+// <= r.  w is non-decreasing (a running sum of non-negative weights) and
+// rounding is monotone, so the cdf values are too and {j : cdf_j <= r} is a
+// prefix: a binary search finds its length from log2(KPL) probes, each probe
+// chosen among the candidates by the earlier decisions (a select tree, since
+// the index differs per lane), plus the last value, which only counts when
+// every probe did.  Equal to counting every j.
+template <int KPL, int BASE, int STRIDE, int LEVELS, int B0>
+__device__ __forceinline__ float pick_probe(const fvec<KPL>& w, const bool* b) {
+    if constexpr (LEVELS == 0) {
+        return w[BASE];
+    } else {
+        const float lo = pick_probe<KPL, BASE, STRIDE, LEVELS - 1, B0 + 1>(w, b);
+        const float hi = pick_probe<KPL, BASE + STRIDE * (1 << (LEVELS - 1)), STRIDE, LEVELS - 1, B0 + 1>(w, b);
+        return b[B0] ? hi : lo;
+    }
+}
+
+template <int KPL, int STEP>
+__device__ __forceinline__ void probe_steps(const fvec<KPL>& w, float excl, float r, bool* b) {
+    if constexpr ((1 << STEP) < KPL) {
+        // probe index (decisions so far) * KPL/2^STEP + KPL/2^(STEP+1) - 1
+        const float c = pick_probe<KPL, KPL / (2 << STEP) - 1, KPL / (1 << STEP), STEP, 0>(w, b);
+        b[STEP] = !(r < excl + c);
+        probe_steps<KPL, STEP + 1>(w, excl, r, b);
+    }
+}
+
+template <int KPL>
+__device__ __forceinline__ int cdf_count_le(const fvec<KPL>& w, float excl, float r, float last) {
+    bool b[6] = {false, false, false, false, false, false};
+    probe_steps<KPL, 0>(w, excl, r, b);
+    int n = (r < last) ? 0 : 1;
+#pragma unroll
+    for (int s = 0; (1 << s) < KPL; ++s) n += b[s] ? (KPL >> (s + 1)) : 0;
+    return n;
+}
+
 // Inclusive scan of the lane-layout weights (lane-local, then the DPP wave
 // scan) and the inverse-CDF search of sampleFromDiscreteAcc (sample.hpp:133-189):
 // the first topic with u * total < cdf, else K-1.  Returns the topic
@@ -348,13 +386,12 @@ __device__ __forceinline__ int scan_search(fvec<KPL>& w, float u, int K, int nva
     for (int j = 1; j < KPL; ++j) w[j] = w[j - 1] + w[j];
     float incl = wave_inclusive_scan(w[KPL - 1]);
     float excl = dpp_wave_shr1(incl);
-#pragma unroll
-    for (int j = 0; j < KPL; ++j) w[j] = excl + w[j];
-    tot = readlane_f(w[KPL - 1], lastLane);
+    // the lane's cdf values are cdf_j = excl + w[j] (float adds, as the scan
+    // defines them); only the probed ones and the last are formed
+    const float last = excl + w[KPL - 1];
+    tot = readlane_f(last, lastLane);
     const float r = u * tot;
-    int miss = 0;
-#pragma unroll
-    for (int j = 0; j < KPL; ++j) miss += (r < w[j]) ? 0 : 1;
+    const int miss = cdf_count_le<KPL>(w, excl, r, last);
     const unsigned long long m = __ballot(miss < nvalid);
     int zn;
     if (m == 0ull) zn = K - 1;
