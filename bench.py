@@ -32,7 +32,8 @@ HBM_PEAK_GBS = 8000.0
 
 def copy_probe():
     """Calibration kernel for the byte counters: a 1 GiB device-to-device copy,
-    three times (read 1 GiB, write 1 GiB per launch)."""
+    three times (read 1 GiB, write 1 GiB per launch); then ten more, timed with
+    HIP events, for the bandwidth a plain copy reaches (2 GiB moved per launch)."""
     import torch
     n = 1 << 28
     a = torch.ones(n, dtype=torch.int32, device="cuda")
@@ -41,7 +42,15 @@ def copy_probe():
     for _ in range(3):
         b.copy_(a)
     torch.cuda.synchronize()
-    return {"copy_bytes_per_launch": n * 4, "launches": 3}
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(10):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / 10
+    return {"copy_bytes_per_launch": n * 4, "launches": 3, "timed_launches": 10, "ms_per_copy": ms,
+            "copy_GBps": 2 * n * 4 / (ms * 1e-3) / 1e9}
 
 
 def pmc_traffic(kernel):
