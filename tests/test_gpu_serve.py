@@ -131,3 +131,37 @@ def test_serve_errors_and_empty_requests():
             srv.submit(np.zeros(too_many, np.uint32), np.array([0, too_many], np.uint64))
         with pytest.raises(ValueError):
             srv.wait(123456)
+
+
+@pytest.mark.parametrize("K,step_iters,step_tokens", [(20, 4, 300), (200, 50, 1 << 30)])
+def test_serve_with_word_priors(K, step_iters, step_tokens):
+    """A model with word priors: the served results equal the stand-alone
+    fold-in bit for bit, and that one equals the oracle's fold-in with eta_kv
+    (initial topics from the eta columns)."""
+    words, off, nw, _ = synth.workload("plain_k20", n_docs=120)
+    ws, offs, V, _ = corpus.prepare_arrays(words, off, nw)
+    rng = np.random.default_rng(K)
+    eta_kv = np.full((V, K), np.float32(0.01), np.float32)
+    pw = rng.choice(V, size=30, replace=False).astype(np.uint32)
+    for v in pw:
+        eta_kv[v] = (rng.random(K) * rng.choice([0.01, 1.0, 5.0])).astype(np.float32)
+    ctx = _native.Context(K, V, 0.1, 0.01, seed=K)
+    ctx.set_word_priors(pw, eta_kv[pw])
+    ctx.upload_corpus(ws, offs)
+    ctx.train(8, workers=0)
+    z, _, _, _ = ctx.download_state()
+    m = O.Model(K, V, ctx.get_alpha(), 0.01, K, scan_order=1)
+    m.set_priors(eta_kv)
+    m.set_corpus(ws, offs, z)
+    iters, seed = 12, 5
+    reqs = _requests(V, 10, seed=K + 1)
+    print("priors K", K, flush=True)
+    with _native.Server(ctx, hbm_budget_bytes=32 << 20, iterations=iters, step_iters=step_iters,
+                        step_tokens=step_tokens, infer_seed=seed) as srv:
+        ids = [srv.submit(uw, uoff) for uw, uoff in reqs]
+        results = [srv.wait(rid, timeout_ms=60000)[:2] for rid in ids]
+    _check_same(ctx, reqs, results, iters, seed)
+    for (uw, uoff), (th, ll) in zip(reqs[:4], results[:4]):
+        th_o, ll_o, _ = m.infer_gen(uw, uoff, iters, infer_seed=seed)
+        assert np.array_equal(th, th_o)
+        np.testing.assert_allclose(ll, ll_o, rtol=1e-9)

@@ -55,17 +55,17 @@ void FoldStreams::destroy() {
     fork = nullptr;
 }
 
-hipError_t launch_fold_buckets(int kpl, const lda::InferParams* p_dev, const int grids[lda::kFoldBuckets],
+hipError_t launch_fold_buckets(int kpl, bool asym, const lda::InferParams* p_dev, const int grids[lda::kFoldBuckets],
                                hipStream_t s, FoldStreams& fs) {
     hipError_t e = hipEventRecord(fs.fork, s);
     // the longest documents (most segments) first
     for (int b = lda::kFoldBuckets - 1; b >= 1 && e == hipSuccess; --b) {
         if (grids[b] <= 0) continue;
         e = hipStreamWaitEvent(fs.aux[b - 1], fs.fork, 0);
-        if (e == hipSuccess) e = lda::launch_infer(kpl, 1 << b, p_dev, grids[b], fs.aux[b - 1]);
+        if (e == hipSuccess) e = lda::launch_infer(kpl, 1 << b, asym, p_dev, grids[b], fs.aux[b - 1]);
         if (e == hipSuccess) e = hipEventRecord(fs.join[b - 1], fs.aux[b - 1]);
     }
-    if (e == hipSuccess && grids[0] > 0) e = lda::launch_infer(kpl, 1, p_dev, grids[0], s);
+    if (e == hipSuccess && grids[0] > 0) e = lda::launch_infer(kpl, 1, asym, p_dev, grids[0], s);
     for (int b = 1; b < lda::kFoldBuckets && e == hipSuccess; ++b)
         if (grids[b] > 0) e = hipStreamWaitEvent(s, fs.join[b - 1], 0);
     return e;
@@ -148,6 +148,25 @@ int kpl_for(int K) {
 
 
 namespace {
+
+// Persistent grids of the fold-in kernel per segment bucket, and the largest
+// segment count that fits, for the context's variant (word priors or not).
+// Drops a cached fold-in graph, which launches the other variant.
+void compute_fold_grid(lda_ctx* c) {
+    c->seg_supported = 1;
+    for (int b = 0; b < lda::kFoldBuckets; ++b) c->fold_grid[b] = 0;
+    for (int b = 0; b < lda::kFoldBuckets; ++b) {
+        const int per_cu = lda::infer_blocks_per_cu(c->kpl, 1 << b, c->has_priors);
+        c->fold_grid[b] = c->num_cu * per_cu;
+        if (per_cu > 0) c->seg_supported = 1 << b;
+        else break;
+    }
+    if (c->seg_max > c->seg_supported) c->seg_max = c->seg_supported;
+    if (c->infer_exec) {
+        (void)hipGraphExecDestroy(c->infer_exec);
+        c->infer_exec = nullptr;
+    }
+}
 
 int upload_alpha(lda_ctx* c) {
     std::vector<float> a(c->Kp, 0.0f);
@@ -529,12 +548,7 @@ int lda_ctx_create(int K, int V, const float* alpha, float eta, int termweight, 
     if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
         return cleanup(fail(LDA_ERR_DEVICE, "hipEventCreate failed"));
     if (c->fold_streams.create() != hipSuccess) return cleanup(fail(LDA_ERR_DEVICE, "fold-in stream creation failed"));
-    for (int b = 0; b < lda::kFoldBuckets; ++b) {
-        const int per_cu = lda::infer_blocks_per_cu(c->kpl, 1 << b);
-        c->fold_grid[b] = c->num_cu * per_cu;
-        if (per_cu > 0) c->seg_supported = 1 << b;
-        else break;
-    }
+    compute_fold_grid(c);
     if (c->d_alpha.alloc(c->Kp) || c->d_alpha_cdf.alloc(K) || c->d_nkw.alloc((size_t)std::max(V, 1) * c->Kp) ||
         c->d_nk.alloc(c->Kp) || c->d_nk_next.alloc(c->Kp) || c->d_counter.alloc(1))
         return cleanup(fail(LDA_ERR_OUT_OF_MEMORY, "device allocation failed"));
@@ -730,6 +744,7 @@ int lda_set_word_priors(lda_ctx* c, int64_t n, const uint32_t* word_ids, const f
     }
     if (n == 0) {
         c->has_priors = false;
+        compute_fold_grid(c);
         return LDA_OK;
     }
     // etaSumByTopic = rowwise sum of eta_kv (LDAModel.hpp:736): float, words in
@@ -760,6 +775,7 @@ int lda_set_word_priors(lda_ctx* c, int64_t n, const uint32_t* word_ids, const f
     HIP_TRY(hipMemcpyAsync(c->d_etasum_d.p, esd.data(), Kp * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     c->has_priors = true;
+    compute_fold_grid(c);
     return LDA_OK;
 }
 
@@ -1377,7 +1393,6 @@ int lda_infer(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off, int64_
     if (rc || done) return rc;
     if (c->tw != LDA_TW_ONE)
         return fail(LDA_ERR_INVALID_ARGUMENT, "term-weighted models need the token weights: call lda_infer_w");
-    if (c->has_priors) return infer_general(c, words, doc_off, D, nullptr, iterations, infer_seed, mode, theta, ll, z_out);
     const uint64_t N = doc_off[D];
     HIP_TRY(hipSetDevice(c->device));
     const hipStream_t s = c->stream;
@@ -1424,6 +1439,10 @@ int lda_infer(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off, int64_
     float asf = 0.0f;
     for (int k = 0; k < c->K; ++k) { asd += (double)c->alpha[k]; asf = asf + c->alpha[k]; }
     p.asum_d = asd; p.asum_f = asf;
+    if (c->has_priors) {
+        p.prior_row = c->d_prior_row.p; p.prior_eta = c->d_prior_eta.p;
+        p.etasum = c->d_etasum.p; p.etasum_d = c->d_etasum_d.p;
+    }
     c->stat_ms = c->stat_launches = c->stat_tokens = 0.0;
     if (mode == 0) {
         p.counters = c->i_counter.p;
@@ -1437,8 +1456,10 @@ int lda_infer(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off, int64_
             hipGraph_t g = nullptr;
             HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
             hipError_t e1 = lda::launch_fill_u32(c->i_counter.p, 0u, lda::kFoldBuckets, s);
-            hipError_t e2 = e1 == hipSuccess ? lda_host::launch_fold_buckets(c->kpl, c->i_params.p, grids, s, c->fold_streams) : e1;
-            if (e2 == hipSuccess) e2 = lda::launch_infer_finalize(c->kpl, c->i_params.p, c->fold_grid[0], s);
+            hipError_t e2 = e1 == hipSuccess ? lda_host::launch_fold_buckets(c->kpl, c->has_priors, c->i_params.p, grids, s,
+                                                                             c->fold_streams)
+                                             : e1;
+            if (e2 == hipSuccess) e2 = lda::launch_infer_finalize(c->kpl, c->has_priors, c->i_params.p, c->fold_grid[0], s);
             hipError_t e3 = hipStreamEndCapture(s, &g);
             if (e2 != hipSuccess || e3 != hipSuccess) {
                 if (g) (void)hipGraphDestroy(g);
@@ -1460,8 +1481,8 @@ int lda_infer(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off, int64_
         for (int64_t d = 0; d < D; ++d) {
             p.job_base = d;
             HIP_TRY(hipMemcpyAsync(c->i_params.p, &p, sizeof p, hipMemcpyHostToDevice, s));
-            HIP_TRY(lda::launch_infer(c->kpl, segs[d], c->i_params.p, 1, s));
-            HIP_TRY(lda::launch_infer_finalize(c->kpl, c->i_params.p, 1, s));
+            HIP_TRY(lda::launch_infer(c->kpl, segs[d], c->has_priors, c->i_params.p, 1, s));
+            HIP_TRY(lda::launch_infer_finalize(c->kpl, c->has_priors, c->i_params.p, 1, s));
             HIP_TRY(hipStreamSynchronize(s));
         }
         HIP_TRY(hipEventRecord(c->ev1, s));

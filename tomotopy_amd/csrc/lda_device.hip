@@ -1481,12 +1481,15 @@ __device__ __forceinline__ void add_run_counts(const WaveLds& S, const InferPara
 //   A    float(n_dk) + alpha_k        NK   n_k,eff = n_k + n_dk (int bits)
 //   COL  n_kw,eff = n_kw + c_d of the current word (int bits)
 //   BF   float(n_k,eff) + V*eta       Y    RN(1 / BF)      AL   alpha_k
+// and with word priors (ASYM) E = eta_kv of the current word and ES =
+// sum_v eta_kv, which takes the place of V*eta in BF.
 // A topic update indexes the state with a wave-uniform slot.  The fields are
 // packed into float vectors ("banks") of 16 or 32 entries: from 16 entries
 // on the compiler uses indexed register moves (s_set_gpr_idx, several under
 // one index setting) rather than a select per entry, and beyond 32 it falls
-// back to scratch memory.  Small KPL thus needs one bank, KPL = 32 six.
-enum FoldField { F_A = 0, F_COL = 1, F_NK = 2, F_BF = 3, F_Y = 4, F_AL = 5, F_COUNT = 6 };
+// back to scratch memory.  Small KPL thus needs one bank, KPL = 32 six (eight
+// with word priors).
+enum FoldField { F_A = 0, F_COL = 1, F_NK = 2, F_BF = 3, F_Y = 4, F_AL = 5, F_E = 6, F_ES = 7, F_COUNT = 8 };
 
 template <int KPL>
 constexpr int fold_vw() { return F_COUNT * KPL <= 16 ? 16 : 32; }
@@ -1496,7 +1499,7 @@ struct FoldBanks {
     static constexpr int VW = fold_vw<KPL>();
     static constexpr int FPV = VW / KPL;  // fields per bank
     using V = fvec<VW>;
-    V &b0, &b1, &b2, &b3, &b4, &b5;
+    V &b0, &b1, &b2, &b3, &b4, &b5, &b6, &b7;
 
     template <int F>
     __device__ __forceinline__ V& bank() {
@@ -1506,7 +1509,9 @@ struct FoldBanks {
         else if constexpr (i == 2) return b2;
         else if constexpr (i == 3) return b3;
         else if constexpr (i == 4) return b4;
-        else return b5;
+        else if constexpr (i == 5) return b5;
+        else if constexpr (i == 6) return b6;
+        else return b7;
     }
     template <int F>
     __device__ __forceinline__ float get(int sl) { return bank<F>()[(F % FPV) * KPL + sl]; }
@@ -1521,7 +1526,7 @@ struct FoldBanks {
 // addWordTo<delta> of topic z on the owner lane (LDAModel.hpp:287-299).  Every
 // lane writes its slot, the non-owners their old value: a store under a lane
 // condition would make the compiler copy the whole bank.
-template <int KPL>
+template <int KPL, bool ASYM>
 __device__ __forceinline__ void fold_bump(FoldBanks<KPL>& st, int lane, int z, int delta, float veta) {
     const int sl = z & (KPL - 1);
     const bool own = lane == (z >> kpl_shift<KPL>());
@@ -1530,7 +1535,7 @@ __device__ __forceinline__ void fold_bump(FoldBanks<KPL>& st, int lane, int z, i
     const int oc = st.template geti<F_COL>(sl), on = st.template geti<F_NK>(sl);
     const int d = (int)__builtin_rintf(oA - a) + delta;
     const int n = on + delta;
-    const float b = (float)n + veta;
+    const float b = (float)n + (ASYM ? st.template get<F_ES>(sl) : veta);
     const float yn = 1.0f / b;
     st.template set<F_A>(sl, own ? (float)d + a : oA);
     st.template seti<F_COL>(sl, own ? oc + delta : oc);
@@ -1552,18 +1557,18 @@ __device__ __forceinline__ float fold_weight(float A, int col, float bf, float y
 
 // One fold-in token: remove z_old (addWordTo<-1>), weight, scan, search, add
 // z_new (addWordTo<+1>).
-template <int KPL>
+template <int KPL, bool ASYM>
 __device__ __forceinline__ int fold_token(FoldBanks<KPL>& st, int lane, int zo, float u, int K, int nvalid,
                                           int lastLane, float eta, float veta) {
-    fold_bump<KPL>(st, lane, zo, -1, veta);
+    fold_bump<KPL, ASYM>(st, lane, zo, -1, veta);
     fvec<KPL> w;
 #pragma unroll
     for (int j = 0; j < KPL; ++j)
         w[j] = fold_weight(st.template get<F_A>(j), st.template geti<F_COL>(j), st.template get<F_BF>(j),
-                           st.template get<F_Y>(j), eta);
+                           st.template get<F_Y>(j), ASYM ? st.template get<F_E>(j) : eta);
     float tot;
     const int zn = scan_search<KPL>(w, u, K, nvalid, lastLane, tot);
-    fold_bump<KPL>(st, lane, zn, +1, veta);
+    fold_bump<KPL, ASYM>(st, lane, zn, +1, veta);
     return zn;
 }
 
@@ -1649,7 +1654,7 @@ __device__ __forceinline__ uint64_t fold_seg_start(const gbl<const uint32_t>* wo
 // starts (tokens whose word differs from the previous one) are found with
 // one ballot and their columns are loaded fold_prefetch() runs ahead into a
 // register ring that the unrolled run loop indexes statically.
-template <int KPL, int S>
+template <int KPL, int S, bool ASYM>
 __global__ void __launch_bounds__(64 * S) infer_kernel(const InferParams* __restrict__ pp) {
     constexpr int SH = kpl_shift<KPL>();
     constexpr int P = fold_prefetch<KPL>();
@@ -1673,6 +1678,8 @@ __global__ void __launch_bounds__(64 * S) infer_kernel(const InferParams* __rest
     load_kvec_f<KPL>(p.alpha, lane, al);
     ivec<KPL> nk0;
     load_kvec<KPL>(p.nk, lane, nk0);
+    fvec<KPL> es = 0.0f;  // word priors: sum_v eta_kv per topic (1 in padded topics)
+    if constexpr (ASYM) load_kvec_f<KPL>(p.etasum, lane, es);
     const int64_t lo = p.counters ? p.bucket_lo[bucket] : 0;
     const int64_t hi = p.counters ? p.bucket_hi[bucket] : p.njobs;
     for (;;) {
@@ -1706,15 +1713,23 @@ __global__ void __launch_bounds__(64 * S) infer_kernel(const InferParams* __rest
             int zz = 0xFFFF;
             if (g < se) {
                 if (it_begin == 0) {
-                    if (words[g] < (uint32_t)p.V) {
+                    const uint32_t wg = words[g];
+                    if (wg < (uint32_t)p.V) {
                         const float u = uniform24(philox_x0(p.seed_lo, p.seed_hi, (uint32_t)(g - b), (uint32_t)((g - b) >> 32), 0u, TAG_INFINIT));
-                        const float r = u * acdf[p.K - 1];
-                        int lo2 = 0, hi2 = p.K;
-                        while (lo2 < hi2) {
-                            const int mid = (lo2 + hi2) >> 1;
-                            if (r < acdf[mid]) hi2 = mid; else lo2 = mid + 1;
+                        if constexpr (ASYM) {
+                            // word priors: the word's eta column (LDAModel.hpp:764-775)
+                            const int pr = p.prior_row[wg];
+                            zz = draw_from_eta_column(pr >= 0 ? p.prior_eta + (int64_t)pr * p.Kp : nullptr, KPL, p.eta,
+                                                      p.K, u);
+                        } else {
+                            const float r = u * acdf[p.K - 1];
+                            int lo2 = 0, hi2 = p.K;
+                            while (lo2 < hi2) {
+                                const int mid = (lo2 + hi2) >> 1;
+                                if (r < acdf[mid]) hi2 = mid; else lo2 = mid + 1;
+                            }
+                            zz = lo2 < p.K ? lo2 : p.K - 1;
                         }
-                        zz = lo2 < p.K ? lo2 : p.K - 1;
                     }
                     zs[g] = (uint16_t)zz;
                 } else {
@@ -1733,10 +1748,14 @@ __global__ void __launch_bounds__(64 * S) infer_kernel(const InferParams* __rest
             for (int j = 0; j < KPL; ++j)
                 if (nd[j]) atomicAdd(nd_sh + kpos<KPL>(lane, j), nd[j]);
         }
-        fvec<VW> b0 = 0.0f, b1 = 0.0f, b2 = 0.0f, b3 = 0.0f, b4 = 0.0f, b5 = 0.0f;
-        FoldBanks<KPL> st{b0, b1, b2, b3, b4, b5};
+        fvec<VW> b0 = 0.0f, b1 = 0.0f, b2 = 0.0f, b3 = 0.0f, b4 = 0.0f, b5 = 0.0f, b6 = 0.0f, b7 = 0.0f;
+        FoldBanks<KPL> st{b0, b1, b2, b3, b4, b5, b6, b7};
 #pragma unroll
         for (int j = 0; j < KPL; ++j) st.template set<F_AL>(j, al[j]);
+        if constexpr (ASYM) {
+#pragma unroll
+            for (int j = 0; j < KPL; ++j) st.template set<F_ES>(j, es[j]);
+        }
         for (int it = it_begin + 1; it <= it_end; ++it) {
             if constexpr (S > 1) {
                 __syncthreads();  // every segment's change is in nd_sh
@@ -1749,7 +1768,7 @@ __global__ void __launch_bounds__(64 * S) infer_kernel(const InferParams* __rest
 #pragma unroll
                 for (int j = 0; j < KPL; ++j) {
                     const int n = nk0[j] + snap[j];
-                    const float bb = (float)n + p.veta;
+                    const float bb = (float)n + (ASYM ? es[j] : p.veta);
                     st.template set<F_A>(j, (float)snap[j] + al[j]);
                     st.template seti<F_NK>(j, n);
                     st.template set<F_BF>(j, bb);
@@ -1779,7 +1798,7 @@ __global__ void __launch_bounds__(64 * S) infer_kernel(const InferParams* __rest
                 const int pre = starts ? __builtin_ctzll(starts) : n;
                 if ((unsigned)vcur < (unsigned)p.V) {
                     for (int t = 0; t < pre; ++t) {
-                        const int zn = fold_token<KPL>(st, lane, readlane_i(zv, t), readlane_f(uv, t), p.K, nvalid,
+                        const int zn = fold_token<KPL, ASYM>(st, lane, readlane_i(zv, t), readlane_f(uv, t), p.K, nvalid,
                                                        lastLane, p.eta, p.veta);
                         zkeep = (lane == t) ? zn : zkeep;
                     }
@@ -1799,6 +1818,16 @@ __global__ void __launch_bounds__(64 * S) infer_kernel(const InferParams* __rest
                             vcur = v;
 #pragma unroll
                             for (int j = 0; j < KPL; ++j) st.template seti<F_COL>(j, slot[q][j]);
+                            if constexpr (ASYM) {
+                                // the word's eta column: its prior row, or eta
+                                fvec<KPL> ew = p.eta;
+                                if ((unsigned)v < (unsigned)p.V) {
+                                    const int pr = p.prior_row[v];
+                                    if (pr >= 0) load_kvec_f<KPL>(p.prior_eta + (int64_t)pr * p.Kp, lane, ew);
+                                }
+#pragma unroll
+                                for (int j = 0; j < KPL; ++j) st.template set<F_E>(j, ew[j]);
+                            }
                             fold_issue<KPL>(slot[q], ahead, wv, nkw, nkv, p.V, p.Kp, lane);
                             if ((unsigned)v < (unsigned)p.V) {
                                 // c_d of this word: the run's topics from the batch
@@ -1807,7 +1836,7 @@ __global__ void __launch_bounds__(64 * S) infer_kernel(const InferParams* __rest
                                 for (int j = i0; j < i1; ++j) col_add<KPL>(st, lane, readlane_i(zv, j));
                                 if (i1 == n) fold_run_counts<KPL>(st, words, zs, lane, t0 + (uint64_t)n, se, v);
                                 for (int t = i0; t < i1; ++t) {
-                                    const int zn = fold_token<KPL>(st, lane, readlane_i(zv, t), readlane_f(uv, t), p.K,
+                                    const int zn = fold_token<KPL, ASYM>(st, lane, readlane_i(zv, t), readlane_f(uv, t), p.K,
                                                                    nvalid, lastLane, p.eta, p.veta);
                                     zkeep = (lane == t) ? zn : zkeep;
                                 }
@@ -1839,7 +1868,7 @@ __global__ void __launch_bounds__(64 * S) infer_kernel(const InferParams* __rest
 // state's difference restricted to the terms the document touches
 // (LDAModel.hpp:945-946); theta = (n_dk + alpha) / (L + sum alpha)
 // (LDAModel.hpp:1145-1159).
-template <int KPL>
+template <int KPL, bool ASYM>
 __global__ void __launch_bounds__(64) infer_finalize_kernel(const InferParams* __restrict__ pp) {
     constexpr int SH = kpl_shift<KPL>();
     const InferParams p = *pp;
@@ -1889,7 +1918,8 @@ __global__ void __launch_bounds__(64) infer_finalize_kernel(const InferParams* _
 #pragma unroll
         for (int j = 0; j < KPL; ++j) {
             if (nd[j]) {
-                ll += lgamma((double)nk0[j] + veta) - lgamma((double)(nk0[j] + nd[j]) + veta);
+                const double es = ASYM ? p.etasum_d[kpos<KPL>(lane, j)] : veta;
+                ll += lgamma((double)nk0[j] + es) - lgamma((double)(nk0[j] + nd[j]) + es);
                 const double a = (double)al[j];
                 ll += lgamma((double)nd[j] + a) - lgamma(a);
             }
@@ -1915,7 +1945,14 @@ __global__ void __launch_bounds__(64) infer_finalize_kernel(const InferParams* _
             }
             const int q = kpos<KPL>(zz >> SH, zz & (KPL - 1));
             if (in && v == run_v) r += run_h[q];
-            if (in) ll += log((double)p.nkw[(int64_t)v * p.Kp + q] + eta + (double)r);
+            if (in) {
+                double e = eta;
+                if constexpr (ASYM) {
+                    const int pr = p.prior_row[v];
+                    if (pr >= 0) e = (double)p.prior_eta[(int64_t)pr * p.Kp + q];
+                }
+                ll += log((double)p.nkw[(int64_t)v * p.Kp + q] + e + (double)r);
+            }
             // the run still open at the end of the chunk (ids >= V sort last)
             const int vlast = readlane_i(v, n - 1);
             const int open = (unsigned)vlast < (unsigned)p.V ? vlast : -1;
@@ -2399,29 +2436,29 @@ int sweep_device_blocks_per_cu(int kpl, bool asym) {
     }
 }
 
-template <int KPL>
+template <int KPL, bool ASYM>
 static int infer_blocks_kpl(int segs) {
     constexpr size_t lds = (64 * KPL + 4) * sizeof(int);
     int n = 0;
     hipError_t e = hipErrorInvalidValue;
     switch (segs) {
-        case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_kernel<KPL, 1>, 64, 0); break;
-        case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_kernel<KPL, 2>, 128, lds); break;
-        case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_kernel<KPL, 4>, 256, lds); break;
-        case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_kernel<KPL, 8>, 512, lds); break;
+        case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_kernel<KPL, 1, ASYM>, 64, 0); break;
+        case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_kernel<KPL, 2, ASYM>, 128, lds); break;
+        case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_kernel<KPL, 4, ASYM>, 256, lds); break;
+        case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_kernel<KPL, 8, ASYM>, 512, lds); break;
         default: break;
     }
     return e == hipSuccess ? n : 0;
 }
 
-int infer_blocks_per_cu(int kpl, int segs) {
+int infer_blocks_per_cu(int kpl, int segs, bool asym) {
     switch (kpl) {
-        case 1: return infer_blocks_kpl<1>(segs);
-        case 2: return infer_blocks_kpl<2>(segs);
-        case 4: return infer_blocks_kpl<4>(segs);
-        case 8: return infer_blocks_kpl<8>(segs);
-        case 16: return infer_blocks_kpl<16>(segs);
-        case 32: return infer_blocks_kpl<32>(segs);
+        case 1: return asym ? infer_blocks_kpl<1, true>(segs) : infer_blocks_kpl<1, false>(segs);
+        case 2: return asym ? infer_blocks_kpl<2, true>(segs) : infer_blocks_kpl<2, false>(segs);
+        case 4: return asym ? infer_blocks_kpl<4, true>(segs) : infer_blocks_kpl<4, false>(segs);
+        case 8: return asym ? infer_blocks_kpl<8, true>(segs) : infer_blocks_kpl<8, false>(segs);
+        case 16: return asym ? infer_blocks_kpl<16, true>(segs) : infer_blocks_kpl<16, false>(segs);
+        case 32: return asym ? infer_blocks_kpl<32, true>(segs) : infer_blocks_kpl<32, false>(segs);
         default: return 0;
     }
 }
@@ -2625,21 +2662,25 @@ hipError_t launch_sample_frozen(int kpl, const int32_t* ndk, const int32_t* nkw,
     return hipGetLastError();
 }
 
-template <int KPL>
+template <int KPL, bool ASYM>
 static hipError_t launch_infer_kpl(int segs, const InferParams* p_dev, int grid, hipStream_t s) {
     constexpr size_t lds = (64 * KPL + 4) * sizeof(int);
     switch (segs) {
-        case 1: hipLaunchKernelGGL((infer_kernel<KPL, 1>), dim3(grid), dim3(64), 0, s, p_dev); break;
-        case 2: hipLaunchKernelGGL((infer_kernel<KPL, 2>), dim3(grid), dim3(128), lds, s, p_dev); break;
-        case 4: hipLaunchKernelGGL((infer_kernel<KPL, 4>), dim3(grid), dim3(256), lds, s, p_dev); break;
-        case 8: hipLaunchKernelGGL((infer_kernel<KPL, 8>), dim3(grid), dim3(512), lds, s, p_dev); break;
+        case 1: hipLaunchKernelGGL((infer_kernel<KPL, 1, ASYM>), dim3(grid), dim3(64), 0, s, p_dev); break;
+        case 2: hipLaunchKernelGGL((infer_kernel<KPL, 2, ASYM>), dim3(grid), dim3(128), lds, s, p_dev); break;
+        case 4: hipLaunchKernelGGL((infer_kernel<KPL, 4, ASYM>), dim3(grid), dim3(256), lds, s, p_dev); break;
+        case 8: hipLaunchKernelGGL((infer_kernel<KPL, 8, ASYM>), dim3(grid), dim3(512), lds, s, p_dev); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-hipError_t launch_infer(int kpl, int segs, const InferParams* p_dev, int grid, hipStream_t s) {
-    LDA_DISPATCH_KPL(kpl, return launch_infer_kpl<KPL>(segs, p_dev, grid, s));
+hipError_t launch_infer(int kpl, int segs, bool asym, const InferParams* p_dev, int grid, hipStream_t s) {
+    if (asym) {
+        LDA_DISPATCH_KPL(kpl, return (launch_infer_kpl<KPL, true>(segs, p_dev, grid, s)));
+    } else {
+        LDA_DISPATCH_KPL(kpl, return (launch_infer_kpl<KPL, false>(segs, p_dev, grid, s)));
+    }
     return hipErrorInvalidValue;
 }
 
@@ -2707,10 +2748,19 @@ int infer_gen_blocks_per_cu(int kpl, bool wt, bool asym, int segs) {
     }
 }
 
-hipError_t launch_infer_finalize(int kpl, const InferParams* p_dev, int grid, hipStream_t s) {
-    LDA_DISPATCH_KPL(kpl, hipLaunchKernelGGL(infer_finalize_kernel<KPL>, dim3(grid), dim3(64), wave_lds_bytes<KPL>(), s,
-                                             p_dev));
+template <int KPL, bool ASYM>
+static hipError_t launch_infer_finalize_k(const InferParams* p_dev, int grid, hipStream_t s) {
+    hipLaunchKernelGGL((infer_finalize_kernel<KPL, ASYM>), dim3(grid), dim3(64), wave_lds_bytes<KPL>(), s, p_dev);
     return hipGetLastError();
+}
+
+hipError_t launch_infer_finalize(int kpl, bool asym, const InferParams* p_dev, int grid, hipStream_t s) {
+    if (asym) {
+        LDA_DISPATCH_KPL(kpl, return (launch_infer_finalize_k<KPL, true>(p_dev, grid, s)));
+    } else {
+        LDA_DISPATCH_KPL(kpl, return (launch_infer_finalize_k<KPL, false>(p_dev, grid, s)));
+    }
+    return hipErrorInvalidValue;
 }
 
 }  // namespace lda

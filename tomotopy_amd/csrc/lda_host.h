@@ -62,7 +62,7 @@ struct FoldStreams {
 
 // Launches the fold-in kernel for every bucket b with grids[b] > 0 (segments
 // 1 << b per document) on the streams of fs, joined back into s.
-hipError_t launch_fold_buckets(int kpl, const lda::InferParams* p_dev, const int grids[lda::kFoldBuckets],
+hipError_t launch_fold_buckets(int kpl, bool asym, const lda::InferParams* p_dev, const int grids[lda::kFoldBuckets],
                                hipStream_t s, FoldStreams& fs);
 
 }  // namespace lda_host

@@ -100,6 +100,12 @@ struct InferParams {
     double* ll;              // slots
     double asum_d;           // sum of alpha in double
     float asum_f;            // sequential float sum of alpha
+    // word priors (the ASYM kernels): prior_row[v] = row of word v's eta
+    // column in prior_eta (Kp, lane layout) or -1; per-topic sums of eta_kv
+    const int32_t* prior_row;
+    const float* prior_eta;
+    const float* etasum;
+    const double* etasum_d;
 };
 
 // Fold-in of term-weighted models (wt set: the count buffers hold float32
@@ -235,10 +241,11 @@ hipError_t launch_sample_frozen(int kpl, const int32_t* ndk, const int32_t* nkw,
 // fold-in; the kernel reads its parameters from device memory so that one
 // captured graph serves every batch
 // segs: wavefronts per document (1, 2, 4 or 8), the bucket this launch serves
-hipError_t launch_infer(int kpl, int segs, const InferParams* p_dev, int grid, hipStream_t s);
+// asym: the word-prior variant (InferParams.prior_row set)
+hipError_t launch_infer(int kpl, int segs, bool asym, const InferParams* p_dev, int grid, hipStream_t s);
 // theta and FP64 log-likelihood of the jobs that finish (it_end == iters)
-hipError_t launch_infer_finalize(int kpl, const InferParams* p_dev, int grid, hipStream_t s);
+hipError_t launch_infer_finalize(int kpl, bool asym, const InferParams* p_dev, int grid, hipStream_t s);
 // resident workgroups per CU of the fold-in kernel (0: this segment count does not fit)
-int infer_blocks_per_cu(int kpl, int segs);
+int infer_blocks_per_cu(int kpl, int segs, bool asym);
 
 }  // namespace lda

@@ -172,8 +172,8 @@ int launch_step(lda_server* s, Lane& L, int64_t njobs, int nfin) {
     HIP_TRY(hipMemcpyAsync(L.params.p, L.h_params, sizeof(lda::InferParams), hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(L.jobs.p, L.h_jobs, njobs * sizeof(lda::InferJob), hipMemcpyHostToDevice, st));
     HIP_TRY(lda::launch_fill_u32(L.counter.p, 0u, lda::kFoldBuckets, st));
-    HIP_TRY(lda_host::launch_fold_buckets(c->kpl, L.params.p, grids, st, L.fs));
-    HIP_TRY(lda::launch_infer_finalize(c->kpl, L.params.p, (int)std::min<int64_t>(s->grid[0], njobs), st));
+    HIP_TRY(lda_host::launch_fold_buckets(c->kpl, c->has_priors, L.params.p, grids, st, L.fs));
+    HIP_TRY(lda::launch_infer_finalize(c->kpl, c->has_priors, L.params.p, (int)std::min<int64_t>(s->grid[0], njobs), st));
     if (nfin) {
         HIP_TRY(hipMemcpyAsync(L.h_theta, L.theta.p, (size_t)nfin * c->K * sizeof(float), hipMemcpyDeviceToHost, st));
         HIP_TRY(hipMemcpyAsync(L.h_ll, L.ll.p, (size_t)nfin * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -412,7 +412,6 @@ int lda_serve_start(lda_ctx* ctx, uint64_t hbm_budget_bytes, int iterations, int
     if (!ctx || !out) return fail(LDA_ERR_INVALID_ARGUMENT, "NULL argument");
     *out = nullptr;
     if (ctx->tw != LDA_TW_ONE) return fail(LDA_ERR_UNSUPPORTED, "fold-in of term-weighted models is not on the device yet");
-    if (ctx->has_priors) return fail(LDA_ERR_UNSUPPORTED, "fold-in with word priors is not on the device yet");
     if (iterations < 0) return fail(LDA_ERR_INVALID_ARGUMENT, "iterations must be >= 0");
     if (step_iters < 1 || step_tokens < 1) return fail(LDA_ERR_INVALID_ARGUMENT, "step_iters and step_tokens must be >= 1");
     if (lanes < 1 || lanes > kMaxLanes) return fail(LDA_ERR_INVALID_ARGUMENT, "lanes must be in 1..%d", kMaxLanes);
@@ -474,6 +473,10 @@ int lda_serve_start(lda_ctx* ctx, uint64_t hbm_budget_bytes, int iterations, int
         p.seed_lo = (uint32_t)infer_seed; p.seed_hi = (uint32_t)(infer_seed >> 32);
         p.theta = L.theta.p; p.ll = L.ll.p;
         p.asum_d = asd; p.asum_f = asf;
+        if (ctx->has_priors) {
+            p.prior_row = ctx->d_prior_row.p; p.prior_eta = ctx->d_prior_eta.p;
+            p.etasum = ctx->d_etasum.p; p.etasum_d = ctx->d_etasum_d.p;
+        }
     }
     for (int b = 0; b < lda::kFoldBuckets; ++b) s->grid[b] = ctx->fold_grid[b];
     s->free_ranges[0] = s->arena_cap;
