@@ -110,8 +110,8 @@ int lda_set_counts_f(lda_ctx* ctx, const float* nkw, const float* nk);
  * sum of eta_kv (EtaHelper<_, true>, LDAModel.hpp:99-117), the initial topics
  * are drawn from the word's eta column (LDAModel.hpp:769-773) and the
  * log-likelihood uses eta_kv (LDAModel.hpp:638-650).  Call before
- * lda_upload_corpus; n == 0 clears the priors.  Fold-in with priors returns
- * LDA_ERR_UNSUPPORTED. */
+ * lda_upload_corpus; n == 0 clears the priors.  Training, fold-in (alone and
+ * together) and serving all use eta_kv. */
 int lda_set_word_priors(lda_ctx* ctx, int64_t n, const uint32_t* word_ids, const float* priors);
 
 int lda_set_alpha(lda_ctx* ctx, const float* alpha);
