@@ -244,7 +244,8 @@ int lda_get_infer_options(lda_ctx* ctx, uint32_t* seg_tokens, int* seg_max, int*
  * document advances by min(step_iters, max(1, step_tokens / tokens per
  * wavefront)) iterations.  The server takes the context's segmentation
  * (lda_set_infer_options) at start.  Results equal lda_infer(mode 0) with the
- * same infer_seed, bit for bit.
+ * same infer_seed, bit for bit (also with word priors).  TermWeight IDF / PMI
+ * models are not served (LDA_ERR_UNSUPPORTED).
  * The context must not be trained, re-uploaded or destroyed while a server on
  * it runs.  submit / wait may be called from any thread; lda_serve_stop must
  * not overlap a wait, and drops requests that were not waited for. */
