@@ -12,8 +12,12 @@ steps (corpus and counts already resident in HBM).  `roofline` prices the
 sweep kernel's algorithmic bytes per launch, N*(4K+8) + D*8K, against the
 8 TB/s HBM peak, using the kernel's average duration from HIP events on the
 library's stream.  `cpu_baseline` times the CPU oracle's partition sweep on
-a bounded sample of the same corpus on this host's cores.  `pmi_variant`
-repeats the training measurement with TermWeight.PMI token weights.
+a bounded sample of the same corpus on this host's cores (`cpu_baseline_w1`:
+one worker, tomotopy's `none` schedule).  `pmi_variant` repeats the training
+measurement with TermWeight.PMI token weights.  The fold-in legs time the
+oracle's fold-in on the host beside the device: the delta-state form
+(`cpu_baseline`) and tomoto's own form with a K x V copy and a full
+log-likelihood scan per document (`cpu_baseline_faithful`).
 """
 import argparse
 import json
@@ -85,12 +89,25 @@ def make_corpus(n_docs=None, cache=None):
     return ws, offs, real_v, w
 
 
-def cpu_baseline(ws, offs, V, K, alpha, eta, budget_s=12.0):
-    """Oracle partition sweep on the host's cores over a bounded document sample."""
+def host_cpu_model():
+    """The host CPU's model name, from /proc/cpuinfo."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(ws, offs, V, K, alpha, eta, budget_s=12.0, threads=None, max_docs=4000):
+    """Oracle partition sweep on the host's cores over a bounded document
+    sample (threads=1: tomotopy's `none` schedule on one core)."""
     from oracle import oracle as O
-    threads = min(os.cpu_count() or 1, 16)
+    threads = threads or min(os.cpu_count() or 1, 16)
     D = len(offs) - 1
-    n_docs = max(200, min(D, 4000))
+    n_docs = max(200, min(D, max_docs))
     sub_off = offs[:n_docs + 1].copy()
     sub_w = ws[:int(sub_off[-1])]
     m = O.Model(K, V, alpha, eta, seed=1, scan_order=0)
@@ -111,8 +128,9 @@ def cpu_baseline(ws, offs, V, K, alpha, eta, budget_s=12.0):
         "unit": "tokens/s",
         "cores": threads,
         "kind": "port",
-        "sample": "first %d of %d docs (%d tokens), K=%d, V=%d, partition scheme with %d workers, %d sweeps in %.1fs"
-                  % (n_docs, D, ntok, K, V, threads, sweeps, dt),
+        "sample": "first %d of %d docs (%d tokens), K=%d, V=%d, %s scheme with %d workers, %d sweeps in %.1fs"
+                  % (n_docs, D, ntok, K, V, "partition" if threads > 1 else "none", threads, sweeps, dt),
+        "cpu_model": host_cpu_model(),
     }
 
 
@@ -151,10 +169,15 @@ def pmi_bench(ws, offs, V, K, alpha, eta, steps, warmup):
     return out
 
 
-def cpu_infer_baseline(ctx, ws, offs, V, K, alpha, eta, uws, uoffs, iterations, budget_s=8.0, seed=3):
+def cpu_infer_baseline(ctx, ws, offs, V, K, alpha, eta, uws, uoffs, iterations, budget_s=8.0, seed=3,
+                       faithful=False):
     """The oracle's fold-in (tomoto's per-document infer, together=False) on
     the host's cores: documents drawn at random from the workload, split over
-    threads (the oracle releases the GIL), until the time budget is spent."""
+    threads (the oracle releases the GIL), until the time budget is spent.
+    faithful=True runs tomoto's own form (LDAModel.hpp:931, 945): a private
+    copy of the whole K x V state per document and the log-likelihood scan
+    over all of it; otherwise the private state is a delta over the frozen
+    model."""
     from concurrent.futures import ThreadPoolExecutor
     from oracle import oracle as O
     threads = min(os.cpu_count() or 1, 16)
@@ -169,7 +192,10 @@ def cpu_infer_baseline(ctx, ws, offs, V, K, alpha, eta, uws, uoffs, iterations, 
         sub_off = np.zeros(len(docs) + 1, np.uint64)
         sub_off[1:] = np.cumsum(lens[docs])
         sub_w = np.concatenate([uws[int(uoffs[d]):int(uoffs[d + 1])] for d in docs]).astype(np.uint32)
-        m.infer(sub_w, sub_off, iterations, mode=1)
+        if faithful:
+            m.infer_gen(sub_w, sub_off, iterations, faithful=2)
+        else:
+            m.infer(sub_w, sub_off, iterations, mode=1)
         return len(docs), int(np.count_nonzero(sub_w < V))
 
     done = toks = 0
@@ -177,16 +203,23 @@ def cpu_infer_baseline(ctx, ws, offs, V, K, alpha, eta, uws, uoffs, iterations, 
     t0 = time.perf_counter()
     with ThreadPoolExecutor(threads) as ex:
         while time.perf_counter() - t0 < budget_s and pos < D:
-            batch = [b for b in (order[pos + j:pos + j + 4] for j in range(0, threads * 16, 4)) if len(b)]
-            pos += threads * 16
+            # tomoto evaluates getLLRest(global) once per infer call; 16 documents per
+            # call keep that scan a small share of the faithful form's work
+            step, tasks = (16, threads) if faithful else (4, threads * 4)
+            batch = [b for b in (order[pos + j:pos + j + step] for j in range(0, tasks * step, step)) if len(b)]
+            pos += tasks * step
             for n, t in ex.map(run, batch):
                 done += n
                 toks += t
     dt = time.perf_counter() - t0
+    form = ("tomoto's form: a private copy of the K x V state per document and the log-likelihood scan over "
+            "all of it with the lgamma table, getLLRest(global) once per 16 documents" if faithful else
+            "per-document private state as a delta over the frozen model; tomoto additionally copies "
+            "the K x V state per document")
     return {"value": done / dt, "unit": "docs/s", "cores": threads, "kind": "port",
             "sample": "%d random docs (%d tokens) of the workload, %d iterations each, oracle fold-in "
-                      "(per-document private state as a delta over the frozen model; tomoto additionally copies "
-                      "the K x V state per document), %d threads, %.1fs" % (done, toks, iterations, threads, dt)}
+                      "(%s), %d threads, %.1fs" % (done, toks, iterations, form, threads, dt),
+            "cpu_model": host_cpu_model()}
 
 
 def infer_bench(plain_sample=300, iterations=100):
@@ -245,6 +278,10 @@ def infer_bench(plain_sample=300, iterations=100):
     }
     out["cpu_baseline"] = cpu_infer_baseline(ctx, ws, offs, V, K, 0.1, 0.01, uws, uoffs, iterations)
     log("fold-in CPU baseline: %.0f docs/s (%s)" % (out["cpu_baseline"]["value"], out["cpu_baseline"]["sample"]))
+    out["cpu_baseline_faithful"] = cpu_infer_baseline(ctx, ws, offs, V, K, 0.1, 0.01, uws, uoffs, iterations,
+                                                      budget_s=6.0, faithful=True)
+    log("fold-in CPU baseline, tomoto's form: %.1f docs/s (%s)"
+        % (out["cpu_baseline_faithful"]["value"], out["cpu_baseline_faithful"]["sample"]))
     log("fold-in: batched %.0f docs/s (%.3fs), sequential %.0f docs/s, plain %.1f docs/s on %d sampled docs, "
         "identical=%s, ll_per_word %.5f vs sequential %.5f"
         % (out["batched_docs_per_s"], batched_s, out["sequential_docs_per_s"], out["plain_docs_per_s"], len(pick),
@@ -561,6 +598,10 @@ def main():
         out["pmi_variant"] = pmi_bench(ws, offs, V, K, alpha, eta, args.steps, args.warmup)
     if rank == 0 and not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(ws, offs, V, K, alpha, eta)
+        out["cpu_baseline_w1"] = cpu_baseline(ws, offs, V, K, alpha, eta, budget_s=4.0, threads=1, max_docs=1000)
+        log("CPU baseline: %.3g tokens/s with %d workers, %.3g with one (%s)"
+            % (out["cpu_baseline"]["value"], out["cpu_baseline"]["cores"], out["cpu_baseline_w1"]["value"],
+               out["cpu_baseline"]["cpu_model"]))
     if args.only == "train":
         pass
     elif not args.no_infer:

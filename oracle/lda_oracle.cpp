@@ -1310,6 +1310,11 @@ void orc_infer_together_w(void* h, const uint32_t* words, const uint64_t* off, i
 // evaluates getLLRest over all of it; faithful = 0 keeps only the document's
 // word columns (one per run of equal word ids) and sums the LL over the cells
 // and topics the document changed (an unchanged cell contributes exactly 0).
+// faithful = 2 is faithful = 1 with tomoto's own evaluation of the LL
+// (LDAModel.hpp:652-664, 925-946): the table lgamma, getLLRest(global)
+// computed once per call, and the private copy scanned topic by topic over
+// every word, skipping empty cells.  It exists to time tomoto's form on the
+// CPU; its LL carries the table's interpolation error.
 void orc_infer_gen_seg(void* h, const uint32_t* words, const uint64_t* off, int64_t D, int iters, uint64_t infer_seed,
                        const float* wts, int faithful, uint32_t seg_tokens, int seg_max, float* theta, double* ll,
                        uint16_t* z_out);
@@ -1347,6 +1352,20 @@ void orc_infer_gen_seg(void* h, const uint32_t* words, const uint64_t* off, int6
     // frozen counts as double (int or float model)
     auto g_nk = [&](int k) { return W ? (double)m->nk_f[k] : (double)m->nk[k]; };
     auto g_kw = [&](uint32_t v, int k) { return W ? (double)m->nkw_f[(size_t)v * K + k] : (double)m->nkw[(size_t)v * K + k]; };
+    // getLLRest of a K x V state with the table lgamma, topic by topic over every word
+    auto rest_lut = [&](auto&& nk_of, auto&& kw_of) {
+        auto lgt = [](double x) { return (double)lut_lgamma().get((float)x); };
+        double r = 0.0;
+        for (int k = 0; k < K; ++k) {
+            r += lgt(es_of(k)) - lgt(nk_of(k) + es_of(k));
+            for (uint32_t v = 0; v < (uint32_t)V; ++v) {
+                const double c = kw_of(v, k);
+                if (c != 0.0) r += lgt(c + e_of(v, k)) - lgt(e_of(v, k));
+            }
+        }
+        return r;
+    };
+    const double grest = faithful == 2 ? rest_lut(g_nk, g_kw) : 0.0;
     for (int64_t d = 0; d < D; ++d) {
         const uint64_t b = off[d], n = off[d + 1] - b;
         std::vector<uint16_t> z(n, 0xFFFF);
@@ -1442,24 +1461,31 @@ void orc_infer_gen_seg(void* h, const uint32_t* words, const uint64_t* off, int6
         }
         // getLLRest(private) - getLLRest(global)
         double l = 0.0;
-        for (int k = 0; k < K; ++k) {
-            const double x0 = g_nk(k), x1 = W ? (double)nkf[k] : (double)nki[k];
-            if (faithful || x0 != x1) l += std::lgamma(x0 + es_of(k)) - std::lgamma(x1 + es_of(k));
-        }
-        auto cell_term = [&](double c, double e) { return c != 0.0 ? std::lgamma(c + e) - std::lgamma(e) : 0.0; };
-        if (faithful) {
-            for (uint32_t v = 0; v < (uint32_t)V; ++v)
-                for (int k = 0; k < K; ++k) {
-                    const double c1 = W ? (double)cf[(size_t)v * K + k] : (double)ci[(size_t)v * K + k];
-                    l += cell_term(c1, e_of(v, k)) - cell_term(g_kw(v, k), e_of(v, k));
-                }
+        if (faithful == 2) {
+            l = rest_lut([&](int k) { return W ? (double)nkf[k] : (double)nki[k]; },
+                         [&](uint32_t v, int k) {
+                             return W ? (double)cf[(size_t)v * K + k] : (double)ci[(size_t)v * K + k];
+                         }) - grest;
         } else {
-            for (uint64_t i = 0; i < n; ++i) {
-                const uint32_t v = words[b + i];
-                if (run[i] != i || v >= (uint32_t)V) continue;
-                for (int k = 0; k < K; ++k) {
-                    const double c0 = g_kw(v, k), c1 = W ? (double)cf[i * K + k] : (double)ci[i * K + k];
-                    if (c0 != c1) l += cell_term(c1, e_of(v, k)) - cell_term(c0, e_of(v, k));
+            for (int k = 0; k < K; ++k) {
+                const double x0 = g_nk(k), x1 = W ? (double)nkf[k] : (double)nki[k];
+                if (faithful || x0 != x1) l += std::lgamma(x0 + es_of(k)) - std::lgamma(x1 + es_of(k));
+            }
+            auto cell_term = [&](double c, double e) { return c != 0.0 ? std::lgamma(c + e) - std::lgamma(e) : 0.0; };
+            if (faithful) {
+                for (uint32_t v = 0; v < (uint32_t)V; ++v)
+                    for (int k = 0; k < K; ++k) {
+                        const double c1 = W ? (double)cf[(size_t)v * K + k] : (double)ci[(size_t)v * K + k];
+                        l += cell_term(c1, e_of(v, k)) - cell_term(g_kw(v, k), e_of(v, k));
+                    }
+            } else {
+                for (uint64_t i = 0; i < n; ++i) {
+                    const uint32_t v = words[b + i];
+                    if (run[i] != i || v >= (uint32_t)V) continue;
+                    for (int k = 0; k < K; ++k) {
+                        const double c0 = g_kw(v, k), c1 = W ? (double)cf[i * K + k] : (double)ci[i * K + k];
+                        if (c0 != c1) l += cell_term(c1, e_of(v, k)) - cell_term(c0, e_of(v, k));
+                    }
                 }
             }
         }

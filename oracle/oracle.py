@@ -266,8 +266,9 @@ class Model:
     def infer_gen(self, words, off, iters, infer_seed=0, weights=None, faithful=False, seg_tokens=0, seg_max=1):
         """Fold-in of a term-weighted model (weights: per-token float32) or of
         a model with word priors; returns (theta, ll, z).  faithful keeps the
-        full K x V private copy per document; seg_tokens / seg_max select the
-        segmented rule (0: sequential)."""
+        full K x V private copy per document (faithful=2: and evaluates the LL
+        as tomoto does, with its lgamma table, for timing tomoto's form);
+        seg_tokens / seg_max select the segmented rule (0: sequential)."""
         words = np.ascontiguousarray(words, np.uint32)
         off = np.ascontiguousarray(off, np.uint64)
         D = len(off) - 1
@@ -276,7 +277,7 @@ class Model:
         z = np.zeros(int(off[-1]), np.uint16)
         w = None if weights is None else np.ascontiguousarray(weights, np.float32)
         load().orc_infer_gen_seg(self._h, _p(words), _p(off), D, iters, int(infer_seed) & (2**64 - 1), _p(w),
-                                 int(bool(faithful)), int(seg_tokens), int(seg_max), _p(theta), _p(ll), _p(z))
+                                 int(faithful), int(seg_tokens), int(seg_max), _p(theta), _p(ll), _p(z))
         return theta, ll, z
 
     def infer_together(self, words, off, iters, infer_seed=0, workers=1, scheme=1, threads=1, weights=None):

@@ -56,6 +56,24 @@ def test_local_state_equals_full_private_copy(kind):
     np.testing.assert_allclose(ll0, ll1, rtol=1e-9)
 
 
+@pytest.mark.parametrize("kind", ["one", "prior", "pmi"])
+def test_tomoto_form_with_lgamma_table(kind):
+    """faithful=2 (tomoto's full-copy LL with its lgamma table, the form the
+    CPU baseline times): the same topics and theta; the LL within the table's
+    interpolation error of FP64; for ONE without priors, the integer fold-in's
+    table mode (changed cells only, the same table) to summation order."""
+    m, V, uw, uoff, uwt, _ = _setup(kind)
+    th0, ll0, z0 = m.infer_gen(uw, uoff, 6, infer_seed=5, weights=uwt, faithful=False)
+    th2, ll2, z2 = m.infer_gen(uw, uoff, 6, infer_seed=5, weights=uwt, faithful=2)
+    assert np.array_equal(z0, z2) and np.array_equal(th0, th2)
+    assert not np.array_equal(ll0, ll2)  # the table is in use
+    np.testing.assert_allclose(ll2, ll0, rtol=1e-4)
+    if kind == "one":
+        _, ll3, z3 = m.infer(uw, uoff, 6, infer_seed=5, mode=1)
+        assert np.array_equal(z3, z0)
+        np.testing.assert_allclose(ll2, ll3, rtol=1e-6)
+
+
 def test_one_without_priors_is_the_integer_fold_in():
     m, V, uw, uoff, _, _ = _setup("one")
     th0, ll0, z0 = m.infer_gen(uw, uoff, 7, infer_seed=9)
