@@ -383,8 +383,8 @@ def _percentile(x, q):
     return float(np.percentile(np.asarray(x, dtype=np.float64), q)) if len(x) else None
 
 
-def serve_bench(iterations=50, qps_list=(25, 50, 100, 200, 400, 800, 1200, 1600), seconds=4.0, plain_requests=60, seed=2024,
-                seg_tokens=128, step_tokens=12000, lanes=1):
+def serve_bench(iterations=50, qps_list=(1, 2, 5, 10, 20, 50, 100, 200, 400, 800, 1200, 1600), seconds=4.0,
+                plain_requests=60, seed=2024, seg_tokens=128, step_tokens=12000, lanes=1):
     """Serving loop on a device-resident K=500, V=100,000 model: Poisson request
     arrivals at each QPS (requests of 1..64 docs, lengths LogNormal(ln 150, 1)
     clipped to [5, 10000], 50 iterations); latency is submit to results on the
@@ -417,7 +417,7 @@ def serve_bench(iterations=50, qps_list=(25, 50, 100, 200, 400, 800, 1200, 1600)
         rng = np.random.default_rng(seed)
         ll_sum, tok_sum = 0.0, 0
         for qps in qps_list:
-            n = max(20, min(len(reqs), int(qps * seconds)))
+            n = max(8, min(len(reqs), int(qps * seconds)))  # at least 8 requests: 8 s at 1 request/s
             gaps = rng.exponential(1.0 / qps, size=n)
             st0 = srv.stats()
             pick = rng.integers(0, len(reqs), size=n)
