@@ -52,9 +52,7 @@ __device__ __forceinline__ float uniform24(uint32_t x) { return (float)(x >> 8) 
 // correctly rounded reciprocal for every float mantissa on gfx950 (all 2^23
 // mantissas, checked by lda_selftest_reciprocal), and the hardware
 // approximation depends on the mantissa only, so the result equals the IEEE
-// division 1.0f / b at a fraction of its instructions.  (The fold-in sampler
-// keeps the division: there the shorter sequence changes how the compiler
-// unrolls the prefetch ring, which costs more than it saves.)
+// division 1.0f / b at a fraction of its instructions.
 __device__ __forceinline__ float recip_rn(float b) {
     const float r0 = __builtin_amdgcn_rcpf(b);
     const float e = __builtin_fmaf(-b, r0, 1.0f);
@@ -1548,7 +1546,7 @@ __device__ __forceinline__ void fold_bump(FoldBanks<KPL>& st, int lane, int z, i
     st.template set<F_BF>(sl, own ? b : ob);
     if constexpr (!fold_inline_y<KPL>()) {
         const float oy = st.template get<F_Y>(sl);
-        const float yn = 1.0f / b;
+        const float yn = recip_rn(b);
         st.template set<F_Y>(sl, own ? yn : oy);
     }
 }
