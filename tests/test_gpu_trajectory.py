@@ -1,8 +1,12 @@
 """ll_per_word trajectories: device-wide GPU training vs the CPU oracle's
-partition schedule, 5 seeds each, on the plain_k20 workload (1,000 docs,
+partition schedule, 20 seeds each, on the plain_k20 workload (1,000 docs,
 K=20, alpha=0.1, eta=0.01, 100 iterations, alpha optimisation every 10).
 Acceptance: the seed-averaged ll_per_word agrees within 1% at every
-10-iteration checkpoint."""
+10-iteration checkpoint.  The device-wide schedule's atomics make every run
+a different chain, and at K=20 single seeds settle in optima about 2% apart;
+with 5 seeds per side the standard error of the difference of the means is
+near 0.6%, which crosses 1% in a few runs out of a hundred, so 20 seeds per
+side (near 0.3%) are averaged."""
 import os
 
 import numpy as np
@@ -13,7 +17,7 @@ from tomotopy_amd import _native, corpus, synth
 
 pytestmark = pytest.mark.gpu
 
-SEEDS = (1, 2, 3, 4, 5)
+SEEDS = tuple(range(1, 21))
 
 
 def test_ll_per_word_trajectories_within_one_percent():
