@@ -83,6 +83,20 @@ int lda_ctx_create(int K, int V, const float* alpha, float eta, int termweight, 
                    lda_ctx** out);
 int lda_ctx_destroy(lda_ctx* ctx);
 
+/* A deep copy of the context on the same device: configuration, corpus,
+ * topics, every count buffer (integer or float), word priors, alpha, the
+ * global step and the fold-in options, copied device to device, so that the
+ * copy trains and folds in exactly as the original would.  Replaces
+ * TopicModel::copy (TopicModel.hpp:623-628: the derived model's copy
+ * constructor; py LDAModel.copy, tomotopy/models.py:295-299). */
+int lda_ctx_copy(lda_ctx* ctx, lda_ctx** out);
+
+/* Test hook (no counterpart in the reference; for tests of the error path
+ * that tomotopy maps to RuntimeError("Train failed"), py_LDA.cpp:306-309):
+ * the lda_train iteration that starts at global step `step` fails with
+ * LDA_ERR_TRAINING.  step < 0 clears it.  Applies to this context only. */
+int lda_debug_fail_at_step(lda_ctx* ctx, int64_t step);
+
 /* Upload the training corpus.  If z is NULL the initial topics are drawn on
  * the device (z ~ Discrete(alpha) from the Philox stream); otherwise z is
  * taken as given.  Counts n_dk, n_kw, n_k are then built from z.
@@ -125,19 +139,48 @@ int lda_set_global_step(lda_ctx* ctx, uint64_t step);
  *                              documents (deterministic).
  *   PARTITION / DEFAULT with workers in [2, 1024]
  *                              the PCGS partition schedule with `workers`
- *                              device workers, W rounds per sweep, private
- *                              n_k per worker (deterministic).
+ *                              device workers (capped at max((V+3)/4, 1) as
+ *                              tomoto caps it), W rounds per sweep replayed
+ *                              from a graph, private n_k per worker
+ *                              (deterministic).
+ *   PARTITION with workers == 0
+ *                              the same schedule with the device's worker
+ *                              count: the largest W that leaves each round
+ *                              about 48 or more tokens per worker, within
+ *                              tomoto's cap, the document count and the
+ *                              resident capacity (lda_resolve_schedule
+ *                              reports it).
  *   COPY_MERGE with workers = W in [2, 64]
  *                              tomoto's copy_merge: W device workers, each
  *                              with private copies of n_kw and n_k, merged
- *                              by summing deltas (deterministic).
- *   PARTITION / DEFAULT / COPY_MERGE with workers == 0
- *                              device-wide: every resident workgroup is a
- *                              worker with a private n_k; n_kw is shared and
- *                              updated with exact int32 atomics.
+ *                              by summing deltas (deterministic); workers ==
+ *                              0 picks W = min(64, (D+1)/2, copies in 16 GB).
+ *   DEFAULT with workers == 0  device-wide (no counterpart in tomotopy,
+ *                              whose DEFAULT is PARTITION): every workgroup
+ *                              is a worker with a private n_k; n_kw is
+ *                              shared and updated with exact int32 atomics;
+ *                              the column of the current word is read when
+ *                              the word's run starts; at most ceil(D / 8)
+ *                              documents in flight unless
+ *                              lda_set_device_schedule says otherwise.
  * lda_train runs `iterations` sweeps, each followed by the alpha fixed point
  * when global_step >= burn_in and (global_step+1) % optim_interval == 0. */
 int lda_train(lda_ctx* ctx, int iterations, int workers, int scheme);
+/* The schedule lda_train(workers, scheme) runs on this context: *scheme_out
+ * LDA_PS_NONE, LDA_PS_PARTITION or LDA_PS_COPY_MERGE with *workers_out
+ * workers, or LDA_PS_DEFAULT for the device-wide schedule with *workers_out
+ * workgroups.  Mirrors TopicModel::train's resolution (TopicModel.hpp:684-690,
+ * getRealScheme 664-682, estimateMaxThreads LDAModel.hpp:491-503). */
+int lda_resolve_schedule(lda_ctx* ctx, int workers, int scheme, int* scheme_out, int* workers_out);
+/* Concurrency of the device-wide schedule (no counterpart in the reference):
+ * at most max_workers workgroups (0: as many as are resident), each of
+ * waves_per_worker (1..8) wavefronts that share one private n_k.
+ * max_workers < 0 restores the default, which keeps at most ceil(D / 8)
+ * documents in flight (documents sampled at the same time miss each other's
+ * count changes; an eighth of the corpus keeps ll_per_word within about
+ * 0.15% of tomoto's partition chain) and runs the full grid of 8-wavefront
+ * workgroups once D is at least 8x the resident wavefronts. */
+int lda_set_device_schedule(lda_ctx* ctx, int max_workers, int waves_per_worker);
 int lda_sweep(lda_ctx* ctx, int workers, int scheme);
 int lda_optimize_alpha(lda_ctx* ctx);
 

@@ -38,6 +38,8 @@ _SIGS = {
     "lda_device_count": [_c_p],
     "lda_ctx_create": [_i32, _i32, _c_p, _f32, _i32, _u64, _i32, _c_p],
     "lda_ctx_destroy": [_c_p],
+    "lda_ctx_copy": [_c_p, _c_p],
+    "lda_debug_fail_at_step": [_c_p, _i64],
     "lda_upload_corpus": [_c_p, _c_p, _c_p, _i64, _c_p, _c_p],
     "lda_set_counts": [_c_p, _c_p, _c_p],
     "lda_set_counts_f": [_c_p, _c_p, _c_p],
@@ -52,6 +54,8 @@ _SIGS = {
     "lda_set_global_step": [_c_p, _u64],
     "lda_train": [_c_p, _i32, _i32, _i32],
     "lda_sweep": [_c_p, _i32, _i32],
+    "lda_resolve_schedule": [_c_p, _i32, _i32, _c_p, _c_p],
+    "lda_set_device_schedule": [_c_p, _i32, _i32],
     "lda_optimize_alpha": [_c_p],
     "lda_loglik": [_c_p, _c_p, _c_p],
     "lda_check_invariants": [_c_p, _c_p],
@@ -144,6 +148,20 @@ class Context:
         self.n_docs = 0
         self.n_tokens = 0
 
+    def copy(self):
+        """A deep copy of this context's device state (lda_ctx_copy)."""
+        h = ctypes.c_void_p()
+        _check(load().lda_ctx_copy(self._h, ctypes.byref(h)))
+        new = Context.__new__(Context)
+        new._h = h
+        new.k, new.v, new.tw = self.k, self.v, self.tw
+        new.n_docs, new.n_tokens = self.n_docs, self.n_tokens
+        return new
+
+    def fail_at_step(self, step):
+        """Test hook: the training iteration starting at global step `step` fails (-1 clears)."""
+        _check(load().lda_debug_fail_at_step(self._h, int(step)))
+
     def close(self):
         if getattr(self, "_h", None) and self._h.value:
             load().lda_ctx_destroy(self._h)
@@ -207,6 +225,19 @@ class Context:
 
     def train(self, iterations, workers=0, scheme=0):
         _check(load().lda_train(self._h, int(iterations), int(workers), int(scheme)))
+
+    def resolve_schedule(self, workers=0, scheme=0):
+        """(scheme, workers) that train(workers, scheme) runs: scheme 1 none, 2 copy_merge,
+        3 partition, 0 the device-wide schedule (workers = its workgroups)."""
+        so, wo = ctypes.c_int(0), ctypes.c_int(0)
+        _check(load().lda_resolve_schedule(self._h, int(workers), int(scheme), ctypes.byref(so), ctypes.byref(wo)))
+        return so.value, wo.value
+
+    def set_device_schedule(self, max_workers=-1, waves_per_worker=8):
+        """Concurrency of the device-wide schedule: at most max_workers workgroups (0: all
+        resident), each of waves_per_worker wavefronts sharing one private n_k; max_workers < 0:
+        the default (at most ceil(D / 8) documents in flight)."""
+        _check(load().lda_set_device_schedule(self._h, int(max_workers), int(waves_per_worker)))
 
     def sweep(self, workers=0, scheme=0):
         _check(load().lda_sweep(self._h, int(workers), int(scheme)))

@@ -249,10 +249,81 @@ lda::SweepParams base_params(lda_ctx* c) {
     return p;
 }
 
+// Worker counts the device picks when the caller passes workers == 0 with an
+// explicit PARTITION or COPY_MERGE (tomotopy passes 0 for "as many as the
+// hardware has", TopicModel.hpp:686).
+//   partition: each round gives every worker about n_valid / W^2 tokens; W is
+//   the largest count that keeps that at kPartitionRoundTokens or more (a
+//   round shorter than that is mostly launch and n_dk staging), within
+//   tomoto's cap max((V+3)/4, 1) (LDAModel.hpp:491-503), the documents and
+//   the resident capacity of the round kernel.
+//   copy_merge: tomoto's cap max((D+1)/2, 1), at most 64 workers and at most
+//   16 GB of private n_kw copies.
+constexpr double kPartitionRoundTokens = 48.0;
+
+int device_partition_workers(lda_ctx* c) {
+    const int cap = std::max((c->V + 3) / 4, 1);
+    const int resident = c->num_cu * lda::sweep_det_blocks_per_cu(c->kpl, c->has_priors, c->tw != LDA_TW_ONE);
+    int W = (int)std::floor(std::sqrt((double)c->n_valid / kPartitionRoundTokens));
+    W = std::min(W, cap);
+    W = std::min<int64_t>(W, std::max<int64_t>(c->D, 1));
+    W = std::min(W, std::min(resident, 1024));
+    return std::max(W, 1);
+}
+
+int device_copy_workers(lda_ctx* c) {
+    const int64_t cap_cm = std::max<int64_t>((c->D + 1) / 2, 1);
+    const double per_copy = std::max(1.0, (double)c->V * c->Kp * 4.0);
+    const int64_t by_mem = std::max<int64_t>((int64_t)(16e9 / per_copy), 1);
+    return (int)std::min<int64_t>(std::min<int64_t>(cap_cm, by_mem), 64);
+}
+
+// Concurrency of the device-wide schedule.  Every document in flight samples
+// against columns and n_k that miss the other in-flight documents' changes
+// (AD-LDA staleness); the larger the share of the corpus in flight, the more
+// the chain lags tomoto's partition chain while it is still descending (on
+// the K=1000 corpus cut to 10,000 documents, 41% in flight lagged by up to
+// 0.66% of ll_per_word, an eighth by at most 0.13%).  So by default at most
+// ceil(D / kInFlightDiv) documents are in flight: as 1-wavefront workgroups
+// (one per CU where possible) when that is below the resident capacity of
+// those, else as 8-wavefront workgroups up to the resident capacity.  Large
+// corpora (D >= 8 x resident wavefronts) run the full grid.
+constexpr int64_t kInFlightDiv = 8;
+
+int device_wide_grid(lda_ctx* c, int* waves_out) {
+    const bool wtd = c->tw != LDA_TW_ONE;
+    auto per_cu = [&](int waves) {
+        return wtd ? lda::sweep_device_w_blocks_per_cu(c->kpl, c->has_priors, waves)
+                   : lda::sweep_device_blocks_per_cu(c->kpl, c->has_priors, waves);
+    };
+    if (!c->dev_auto) {
+        int grid = c->num_cu * per_cu(c->dev_waves);
+        if (c->dev_workers > 0) grid = std::min(grid, c->dev_workers);
+        *waves_out = c->dev_waves;
+        return grid;
+    }
+    const int W8 = lda::kMaxSweepWaves;
+    const int64_t inflight = std::max<int64_t>((c->D + kInFlightDiv - 1) / kInFlightDiv, 1);
+    const int full8 = c->num_cu * per_cu(W8);
+    if (inflight >= (int64_t)full8 * W8) {
+        *waves_out = W8;
+        return full8;
+    }
+    const int full1 = c->num_cu * per_cu(1);
+    if (inflight <= full1) {
+        *waves_out = 1;
+        return (int)inflight;
+    }
+    *waves_out = W8;
+    return (int)std::min<int64_t>((inflight + W8 - 1) / W8, full8);
+}
+
 // Scheme resolution as in TopicModel::train (TopicModel.hpp:684-690):
 // workers == 1 or NONE -> one worker; the partition worker count is capped at
 // max((V+3)/4, 1) and the copy_merge one at max((D+1)/2, 1)
-// (LDAModel.hpp:491-503); workers == 0 selects the device-wide schedule.
+// (LDAModel.hpp:491-503).  workers == 0 with PARTITION or COPY_MERGE runs
+// that scheme with the device's worker count; workers == 0 with DEFAULT
+// selects the device-wide schedule (every resident workgroup a worker).
 // Returns the worker count (0 = device-wide) and whether copy_merge applies.
 int resolve_workers(lda_ctx* c, int workers, int scheme, int* W_out, bool* copy_merge) {
     *copy_merge = false;
@@ -262,13 +333,13 @@ int resolve_workers(lda_ctx* c, int workers, int scheme, int* W_out, bool* copy_
         *W_out = 1;
         return LDA_OK;
     }
-    if (workers == 0) {
+    if (workers == 0 && scheme == LDA_PS_DEFAULT) {
         *W_out = 0;
         return LDA_OK;
     }
     if (scheme == LDA_PS_COPY_MERGE) {
         const int64_t cap_cm = std::max<int64_t>((c->D + 1) / 2, 1);
-        int W = (int)std::min<int64_t>(std::min<int64_t>(workers, cap_cm), 64);
+        int W = workers == 0 ? device_copy_workers(c) : (int)std::min<int64_t>(std::min<int64_t>(workers, cap_cm), 64);
         if (W <= 1) { *W_out = 1; return LDA_OK; }
         const double bytes = (double)W * c->V * c->Kp * 4.0;
         if (bytes > 64e9) return fail(LDA_ERR_OUT_OF_MEMORY, "copy_merge with %d workers needs %.1f GB of n_kw copies", W, bytes / 1e9);
@@ -276,9 +347,56 @@ int resolve_workers(lda_ctx* c, int workers, int scheme, int* W_out, bool* copy_
         *copy_merge = true;
         return LDA_OK;
     }
-    int cap = std::max((c->V + 3) / 4, 1);
-    int W = std::min(std::min(workers, cap), 1024);
+    int W;
+    if (workers == 0) W = device_partition_workers(c);
+    else W = std::min(std::min(workers, std::max((c->V + 3) / 4, 1)), 1024);
     *W_out = W;
+    return LDA_OK;
+}
+
+// The W rounds of one partition sweep (LDAModel.hpp:373-399), captured once
+// as a graph of W launches and replayed; the sweep number, which keys the
+// rounds' random streams, is written to d_sweep by a fill kernel first.  The
+// graph is rebuilt when the buffers or sizes it was captured with change.
+bool same_round_params(const lda::SweepParams& a, const lda::SweepParams& b) {
+    return a.words == b.words && a.z == b.z && a.off == b.off && a.ndk == b.ndk && a.nkw == b.nkw &&
+           a.nk_in == b.nk_in && a.nk_out == b.nk_out && a.alpha == b.alpha && a.eta == b.eta && a.veta == b.veta &&
+           a.K == b.K && a.V == b.V && a.Kp == b.Kp && a.seed_lo == b.seed_lo && a.seed_hi == b.seed_hi &&
+           a.D == b.D && a.coff == b.coff && a.W == b.W && a.wt == b.wt && a.prior_row == b.prior_row &&
+           a.prior_eta == b.prior_eta && a.etasum == b.etasum && a.sweep_dev == b.sweep_dev;
+}
+
+int run_partition_rounds(lda_ctx* c, lda::SweepParams p, int W, bool weighted) {
+    const hipStream_t s = c->stream;
+    HIP_TRY(c->d_sweep.alloc(1));
+    p.sweep_dev = c->d_sweep.p;
+    const lda::SweepParams key = p;
+    const bool same = c->part_exec && c->part_key_w == weighted && same_round_params(key, c->part_key);
+    if (!same) {
+        if (c->part_exec) {
+            (void)hipGraphExecDestroy(c->part_exec);
+            c->part_exec = nullptr;
+        }
+        hipGraph_t g = nullptr;
+        HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        hipError_t e = hipSuccess;
+        for (int r = 0; r < W && e == hipSuccess; ++r) {
+            p.round = r;
+            e = weighted ? lda::launch_sweep_det_w(c->kpl, p, s) : lda::launch_sweep_det(c->kpl, p, s);
+        }
+        const hipError_t e2 = hipStreamEndCapture(s, &g);
+        if (e == hipSuccess) e = e2;
+        if (e == hipSuccess) e = hipGraphInstantiate(&c->part_exec, g, nullptr, nullptr, 0);
+        if (g) (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            c->part_exec = nullptr;
+            return fail(LDA_ERR_DEVICE, "partition graph: %s", hipGetErrorString(e));
+        }
+        c->part_key = key;
+        c->part_key_w = weighted;
+    }
+    HIP_TRY(lda::launch_fill_u32(c->d_sweep.p, p.sweep, 1, s));
+    HIP_TRY(hipGraphLaunch(c->part_exec, s));
     return LDA_OK;
 }
 
@@ -313,24 +431,23 @@ int sweep_once(lda_ctx* c, int W, bool copy_merge) {
             p.nk_out = c->d_nk.p;
             p.doc_order = c->d_order.p;
             p.counter = c->d_counter.p;
-            const int grid = c->num_cu * lda::sweep_device_w_blocks_per_cu(c->kpl, c->has_priors);
+            int waves = 1;
+            const int grid = device_wide_grid(c, &waves);
             HIP_TRY(hipEventRecord(c->ev0, s));
-            HIP_TRY(lda::launch_sweep_device_w(c->kpl, p, grid, s));
+            HIP_TRY(lda::launch_sweep_device_w(c->kpl, p, grid, waves, s));
             HIP_TRY(hipEventRecord(c->ev1, s));
             c->stat_launches += 1;
         } else {
             int rc = prepare_partition(c, W);
             if (rc) return rc;
-            for (int w = 0; w < W; ++w) HIP_TRY(lda::launch_copy_i32(c->d_nk_priv.p + (size_t)w * c->Kp, c->d_nk.p, c->Kp, s));
+            HIP_TRY(lda::launch_replicate(c->d_nk_priv.p, c->d_nk.p, c->Kp, W, s));
             p.nk_in = c->d_nk_priv.p;
             p.nk_out = c->d_nk_priv.p;
             p.coff = c->d_coff.p;
             p.W = W;
             HIP_TRY(hipEventRecord(c->ev0, s));
-            for (int r = 0; r < W; ++r) {
-                p.round = r;
-                HIP_TRY(lda::launch_sweep_det_w(c->kpl, p, s));
-            }
+            rc = run_partition_rounds(c, p, W, true);
+            if (rc) return rc;
             HIP_TRY(hipEventRecord(c->ev1, s));
             c->stat_launches += W;
         }
@@ -375,9 +492,10 @@ int sweep_once(lda_ctx* c, int W, bool copy_merge) {
         p.nk_out = c->d_nk_next.p;
         p.doc_order = c->d_order.p;
         p.counter = c->d_counter.p;
-        int grid = c->num_cu * lda::sweep_device_blocks_per_cu(c->kpl, c->has_priors);
+        int waves = 1;
+        const int grid = device_wide_grid(c, &waves);
         HIP_TRY(hipEventRecord(c->ev0, s));
-        HIP_TRY(lda::launch_sweep_device(c->kpl, p, grid, s));
+        HIP_TRY(lda::launch_sweep_device(c->kpl, p, grid, waves, s));
         HIP_TRY(hipEventRecord(c->ev1, s));
         HIP_TRY(hipEventSynchronize(c->ev1));
         float ms = 0.0f;
@@ -388,16 +506,14 @@ int sweep_once(lda_ctx* c, int W, bool copy_merge) {
     } else {
         int rc = prepare_partition(c, W);
         if (rc) return rc;
-        for (int w = 0; w < W; ++w) HIP_TRY(lda::launch_copy_i32(c->d_nk_priv.p + (size_t)w * c->Kp, c->d_nk.p, c->Kp, s));
+        HIP_TRY(lda::launch_replicate(c->d_nk_priv.p, c->d_nk.p, c->Kp, W, s));
         p.nk_in = c->d_nk_priv.p;
         p.nk_out = c->d_nk_priv.p;
         p.coff = c->d_coff.p;
         p.W = W;
         HIP_TRY(hipEventRecord(c->ev0, s));
-        for (int r = 0; r < W; ++r) {
-            p.round = r;
-            HIP_TRY(lda::launch_sweep_det(c->kpl, p, s));
-        }
+        rc = run_partition_rounds(c, p, W, false);
+        if (rc) return rc;
         HIP_TRY(hipEventRecord(c->ev1, s));
         // mergeState<partition>: n_k = rowwise sum of n_kw (LDAModel.hpp:571-579)
         HIP_TRY(lda::launch_rowsum_nkw(c->d_nkw.p, c->V, c->Kp, c->d_nk.p, s));
@@ -493,6 +609,17 @@ int set_counts_t(lda_ctx* c, const T* nkw, const T* nk) {
 
 }  // namespace
 
+namespace {
+// Device-to-device copy of a whole buffer (its allocated capacity).
+template <typename T>
+hipError_t dup_buf(DevBuf<T>& dst, const DevBuf<T>& src, hipStream_t s) {
+    if (!src.p) return hipSuccess;
+    hipError_t e = dst.alloc(src.n);
+    if (e == hipSuccess) e = hipMemcpyAsync(dst.p, src.p, src.n * sizeof(T), hipMemcpyDeviceToDevice, s);
+    return e;
+}
+}  // namespace
+
 // ============================================================== C ABI ======
 extern "C" {
 
@@ -569,6 +696,8 @@ int lda_ctx_destroy(lda_ctx* c) {
     c->d_ndk.release(); c->d_doclen.release(); c->d_nk_priv.release(); c->d_nkw_loc.release(); c->d_words.release(); c->d_order.release();
     c->d_counter.release(); c->d_z.release(); c->d_off.release(); c->d_coff.release(); c->d_part.release();
     if (c->infer_exec) (void)hipGraphExecDestroy(c->infer_exec);
+    if (c->part_exec) (void)hipGraphExecDestroy(c->part_exec);
+    c->d_sweep.release();
     c->i_words.release(); c->i_counter.release(); c->i_jobs.release(); c->i_params.release(); c->i_z.release();
     c->i_theta.release(); c->i_ll.release();
     c->d_wt.release(); c->d_docw.release(); c->d_rowpart.release();
@@ -581,6 +710,69 @@ int lda_ctx_destroy(lda_ctx* c) {
     c->fold_streams.destroy();
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
+    return LDA_OK;
+}
+
+int lda_ctx_copy(lda_ctx* c, lda_ctx** out) {
+    if (!c || !out) return fail(LDA_ERR_INVALID_ARGUMENT, "NULL argument");
+    *out = nullptr;
+    lda_ctx* t = nullptr;
+    int rc = lda_ctx_create(c->K, c->V, c->alpha.data(), c->eta, c->tw, c->seed, c->device, &t);
+    if (rc) return rc;
+    auto cleanup = [&](int code) {
+        lda_ctx_destroy(t);
+        return code;
+    };
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return cleanup(fail(LDA_ERR_DEVICE, "hipStreamSynchronize failed"));
+    t->global_step = c->global_step;
+    t->optim_interval = c->optim_interval;
+    t->burn_in = c->burn_in;
+    t->D = c->D;
+    t->N = c->N;
+    t->n_valid = c->n_valid;
+    t->off = c->off;
+    t->words = c->words;
+    t->has_corpus = c->has_corpus;
+    t->wt = c->wt;
+    t->weighted_n = c->weighted_n;
+    t->has_priors = c->has_priors;
+    t->chunk_cf = c->chunk_cf;
+    t->max_len = c->max_len;
+    t->seg_tokens = c->seg_tokens;
+    t->seg_max = c->seg_max;
+    t->dev_auto = c->dev_auto;
+    t->dev_workers = c->dev_workers;
+    t->dev_waves = c->dev_waves;
+    const hipStream_t s = t->stream;
+    hipError_t e = hipSuccess;
+    // the count buffers first (lda_ctx_create sized them; copy exactly)
+    if (e == hipSuccess) e = hipMemcpyAsync(t->d_nkw.p, c->d_nkw.p, c->d_nkw.n * sizeof(int32_t), hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(t->d_nk.p, c->d_nk.p, c->Kp * sizeof(int32_t), hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess) e = dup_buf(t->d_words, c->d_words, s);
+    if (e == hipSuccess) e = dup_buf(t->d_z, c->d_z, s);
+    if (e == hipSuccess) e = dup_buf(t->d_off, c->d_off, s);
+    if (e == hipSuccess) e = dup_buf(t->d_ndk, c->d_ndk, s);
+    if (e == hipSuccess) e = dup_buf(t->d_doclen, c->d_doclen, s);
+    if (e == hipSuccess) e = dup_buf(t->d_order, c->d_order, s);
+    if (e == hipSuccess) e = dup_buf(t->d_wt, c->d_wt, s);
+    if (e == hipSuccess) e = dup_buf(t->d_docw, c->d_docw, s);
+    if (e == hipSuccess && c->d_rowpart.p) e = t->d_rowpart.alloc(c->d_rowpart.n);
+    if (e == hipSuccess) e = dup_buf(t->d_prior_row, c->d_prior_row, s);
+    if (e == hipSuccess) e = dup_buf(t->d_prior_eta, c->d_prior_eta, s);
+    if (e == hipSuccess) e = dup_buf(t->d_etasum, c->d_etasum, s);
+    if (e == hipSuccess) e = dup_buf(t->d_etasum_d, c->d_etasum_d, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess)
+        return cleanup(fail(e == hipErrorOutOfMemory ? LDA_ERR_OUT_OF_MEMORY : LDA_ERR_DEVICE, "model copy: %s",
+                            hipGetErrorString(e)));
+    compute_fold_grid(t);
+    *out = t;
+    return LDA_OK;
+}
+
+int lda_debug_fail_at_step(lda_ctx* c, int64_t step) {
+    if (!c) return fail(LDA_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    c->fail_at_step = step < 0 ? -1 : step;
     return LDA_OK;
 }
 
@@ -816,6 +1008,39 @@ int lda_set_global_step(lda_ctx* c, uint64_t step) {
     return LDA_OK;
 }
 
+int lda_set_device_schedule(lda_ctx* c, int max_workers, int waves_per_worker) {
+    if (!c) return fail(LDA_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (max_workers < 0) {
+        c->dev_auto = true;
+        return LDA_OK;
+    }
+    if (waves_per_worker < 1 || waves_per_worker > lda::kMaxSweepWaves)
+        return fail(LDA_ERR_INVALID_ARGUMENT, "waves_per_worker must be in [1, %d]", lda::kMaxSweepWaves);
+    c->dev_auto = false;
+    c->dev_workers = max_workers;
+    c->dev_waves = waves_per_worker;
+    return LDA_OK;
+}
+
+int lda_resolve_schedule(lda_ctx* c, int workers, int scheme, int* scheme_out, int* workers_out) {
+    if (!c || !scheme_out || !workers_out) return fail(LDA_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (!c->has_corpus) return fail(LDA_ERR_STATE, "no corpus uploaded");
+    int W;
+    bool cm;
+    int rc = resolve_workers(c, workers, scheme, &W, &cm);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(c->device));
+    if (W == 0) {
+        int waves = 1;
+        *scheme_out = LDA_PS_DEFAULT;
+        *workers_out = device_wide_grid(c, &waves);
+    } else {
+        *scheme_out = W == 1 ? LDA_PS_NONE : cm ? LDA_PS_COPY_MERGE : LDA_PS_PARTITION;
+        *workers_out = W;
+    }
+    return LDA_OK;
+}
+
 int lda_sweep(lda_ctx* c, int workers, int scheme) {
     if (!c) return fail(LDA_ERR_INVALID_ARGUMENT, "ctx is NULL");
     if (!c->has_corpus) return fail(LDA_ERR_STATE, "no corpus uploaded");
@@ -838,12 +1063,8 @@ int lda_train(lda_ctx* c, int iterations, int workers, int scheme) {
     if (rc) return rc;
     HIP_TRY(hipSetDevice(c->device));
     c->stat_ms = c->stat_launches = c->stat_tokens = 0.0;
-    // Host-forced failure for testing the "Train failed" path: the iteration
-    // whose global step equals TOMOTOPY_AMD_FAIL_AT_STEP reports a training error.
-    const char* fa = std::getenv("TOMOTOPY_AMD_FAIL_AT_STEP");
-    const long long fail_at = fa ? std::atoll(fa) : -1;
     for (int it = 0; it < iterations; ++it) {
-        if (fail_at >= 0 && (long long)c->global_step == fail_at)
+        if (c->fail_at_step >= 0 && (uint64_t)c->fail_at_step == c->global_step)
             return fail(LDA_ERR_TRAINING, "training error forced at step %llu", (unsigned long long)c->global_step);
         rc = sweep_once(c, W, cm);
         if (rc) return rc;

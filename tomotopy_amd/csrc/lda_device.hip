@@ -445,8 +445,8 @@ __constant__ uint64_t kShufflePrimes[16] = {65537, 65539, 65543, 65551, 65557, 6
 // sampleDocument over the token range [b, e) (LDAModel.hpp:326-354).  The
 // column of the next distinct word is fetched one token ahead.
 template <int KPL, int MODE, class E>
-__device__ __forceinline__ void sample_range(const WaveLds& S, const fvec<KPL>& al, const SweepParams& p, int lane,
-                                             uint64_t b, uint64_t e, int nvalid, int lastLane, E& et) {
+__device__ __forceinline__ void sample_range(const WaveLds& S, const fvec<KPL>& al, const SweepParams& p, uint32_t sweep,
+                                             int lane, uint64_t b, uint64_t e, int nvalid, int lastLane, E& et) {
     constexpr int SH = kpl_shift<KPL>();
     int vcur = -1;
     int32_t* colp = nullptr;
@@ -458,7 +458,7 @@ __device__ __forceinline__ void sample_range(const WaveLds& S, const fvec<KPL>& 
         if (lane < n) {
             wv = (int)p.words[g];
             zv = p.z[g];
-            uv = uniform24(philox_x0(p.seed_lo, p.seed_hi, (uint32_t)g, (uint32_t)(g >> 32), p.sweep, TAG_SWEEP));
+            uv = uniform24(philox_x0(p.seed_lo, p.seed_hi, (uint32_t)g, (uint32_t)(g >> 32), sweep, TAG_SWEEP));
         }
         int zkeep = zv;
         int vnext = -1;
@@ -505,7 +505,7 @@ __device__ __forceinline__ void sample_range(const WaveLds& S, const fvec<KPL>& 
 // partition workers keep theirs, LDAModel.hpp:484, 596-602); documents are
 // taken longest-first from a global counter; n_kw is updated with atomics;
 // each worker adds its n_k delta to nk_out when it runs out of documents.
-constexpr int kSweepWaves = 8;  // wavefronts per workgroup of the device-wide sweep (one worker)
+constexpr int kSweepWaves = kMaxSweepWaves;  // most wavefronts per workgroup of the device-wide sweep (one worker)
 
 template <int KPL, bool ASYM>
 __global__ void __launch_bounds__(64 * kSweepWaves) sweep_device_kernel(SweepParams p) {
@@ -529,7 +529,7 @@ __global__ void __launch_bounds__(64 * kSweepWaves) sweep_device_kernel(SweepPar
         ivec<KPL> nd;
         load_kvec<KPL>(ndk_row, lane, nd);
         set_ndk<KPL>(S, al, nd, lane);
-        sample_range<KPL, MODE_ATOMIC>(S, al, p, lane, b, e, nvalid, lastLane, et);
+        sample_range<KPL, MODE_ATOMIC>(S, al, p, p.sweep, lane, b, e, nvalid, lastLane, et);
         nd = get_ndk<KPL>(S, al, lane);
         store_kvec<KPL>(ndk_row, lane, nd);
     }
@@ -563,9 +563,10 @@ __global__ void __launch_bounds__(64) sweep_det_kernel(SweepParams p) {
     int32_t* nkp = p.nk_out + (int64_t)w * p.Kp;
     set_nk<KPL>(S, nkp, lane, et);
     const int W = p.W;
+    const uint32_t sweep = p.sweep_dev ? (uint32_t)uniform_i((int)*p.sweep_dev) : p.sweep;
     const int didx = (p.round + w) % W;
     const uint64_t N = (uint64_t)((p.D + (W - 1) - didx) / W);
-    const uint32_t sseed = philox_x0(p.seed_lo, p.seed_hi, (uint32_t)w, (uint32_t)p.round, p.sweep, TAG_SHUFFLE);
+    const uint32_t sseed = philox_x0(p.seed_lo, p.seed_hi, (uint32_t)w, (uint32_t)p.round, sweep, TAG_SHUFFLE);
     uint64_t P = 1;
     if (N) {
         P = kShufflePrimes[sseed & 0xF];
@@ -587,7 +588,7 @@ __global__ void __launch_bounds__(64) sweep_det_kernel(SweepParams p) {
         ivec<KPL> nd;
         load_kvec<KPL>(ndk_row, lane, nd);
         set_ndk<KPL>(S, al, nd, lane);
-        sample_range<KPL, MODE_DET>(S, al, p, lane, b, e, nvalid, lastLane, et);
+        sample_range<KPL, MODE_DET>(S, al, p, sweep, lane, b, e, nvalid, lastLane, et);
         nd = get_ndk<KPL>(S, al, lane);
         store_kvec<KPL>(ndk_row, lane, nd);
     }
@@ -639,7 +640,7 @@ __global__ void __launch_bounds__(64) sweep_copy_kernel(SweepParams p) {
             ivec<KPL> nd;
             load_kvec<KPL>(ndk_row, lane, nd);
             set_ndk<KPL>(S, al, nd, lane);
-            sample_range<KPL, MODE_DET>(S, al, q, lane, b, e, nvalid, lastLane, et);
+            sample_range<KPL, MODE_DET>(S, al, q, q.sweep, lane, b, e, nvalid, lastLane, et);
             nd = get_ndk<KPL>(S, al, lane);
             store_kvec<KPL>(ndk_row, lane, nd);
         }
@@ -757,8 +758,9 @@ __device__ __forceinline__ int token_step_w(const WaveLdsW& S, const fvec<KPL>& 
 // MODE_ATOMIC: the token's net change goes to the shared n_kw as float
 // atomics.
 template <int KPL, int MODE, class E>
-__device__ __forceinline__ void sample_range_w(const WaveLdsW& S, const fvec<KPL>& al, const SweepParams& p, int lane,
-                                               uint64_t b, uint64_t e, int nvalid, int lastLane, E& et) {
+__device__ __forceinline__ void sample_range_w(const WaveLdsW& S, const fvec<KPL>& al, const SweepParams& p,
+                                               uint32_t sweep, int lane, uint64_t b, uint64_t e, int nvalid, int lastLane,
+                                               E& et) {
     constexpr int SH = kpl_shift<KPL>();
     constexpr bool SHARED = MODE == MODE_ATOMIC;
     float* nkwf = reinterpret_cast<float*>(p.nkw);
@@ -773,7 +775,7 @@ __device__ __forceinline__ void sample_range_w(const WaveLdsW& S, const fvec<KPL
             wv = (int)p.words[g];
             zv = p.z[g];
             tw = p.wt[g];
-            uv = uniform24(philox_x0(p.seed_lo, p.seed_hi, (uint32_t)g, (uint32_t)(g >> 32), p.sweep, TAG_SWEEP));
+            uv = uniform24(philox_x0(p.seed_lo, p.seed_hi, (uint32_t)g, (uint32_t)(g >> 32), sweep, TAG_SWEEP));
         }
         int zkeep = zv;
         int vnext = -1;
@@ -850,7 +852,7 @@ __global__ void __launch_bounds__(64 * kSweepWaves) sweep_device_w_kernel(SweepP
         fvec<KPL> nd;
         load_kvec_f<KPL>(ndk_row, lane, nd);
         lds_store_f<KPL>(S.nd, lane, nd);
-        sample_range_w<KPL, MODE_ATOMIC>(S, al, p, lane, b, e, nvalid, lastLane, et);
+        sample_range_w<KPL, MODE_ATOMIC>(S, al, p, p.sweep, lane, b, e, nvalid, lastLane, et);
         lds_load_f<KPL>(S.nd, lane, nd);
         store_kvec<KPL>(reinterpret_cast<int32_t*>(ndk_row), lane, __builtin_bit_cast(ivec<KPL>, nd));
     }
@@ -878,9 +880,10 @@ __global__ void __launch_bounds__(64) sweep_det_w_kernel(SweepParams p) {
         lds_store_f<KPL>(S.y, lane, y);
     }
     const int W = p.W;
+    const uint32_t sweep = p.sweep_dev ? (uint32_t)uniform_i((int)*p.sweep_dev) : p.sweep;
     const int didx = (p.round + w) % W;
     const uint64_t N = (uint64_t)((p.D + (W - 1) - didx) / W);
-    const uint32_t sseed = philox_x0(p.seed_lo, p.seed_hi, (uint32_t)w, (uint32_t)p.round, p.sweep, TAG_SHUFFLE);
+    const uint32_t sseed = philox_x0(p.seed_lo, p.seed_hi, (uint32_t)w, (uint32_t)p.round, sweep, TAG_SHUFFLE);
     uint64_t P = 1;
     if (N) {
         P = kShufflePrimes[sseed & 0xF];
@@ -901,7 +904,7 @@ __global__ void __launch_bounds__(64) sweep_det_w_kernel(SweepParams p) {
         fvec<KPL> nd;
         load_kvec_f<KPL>(ndk_row, lane, nd);
         lds_store_f<KPL>(S.nd, lane, nd);
-        sample_range_w<KPL, MODE_DET>(S, al, p, lane, b, e, nvalid, lastLane, et);
+        sample_range_w<KPL, MODE_DET>(S, al, p, sweep, lane, b, e, nvalid, lastLane, et);
         lds_load_f<KPL>(S.nd, lane, nd);
         store_kvec<KPL>(reinterpret_cast<int32_t*>(ndk_row), lane, __builtin_bit_cast(ivec<KPL>, nd));
     }
@@ -958,7 +961,7 @@ __global__ void __launch_bounds__(64) sweep_copy_w_kernel(SweepParams p) {
             fvec<KPL> nd;
             load_kvec_f<KPL>(ndk_row, lane, nd);
             lds_store_f<KPL>(S.nd, lane, nd);
-            sample_range_w<KPL, MODE_DET>(S, al, q, lane, b, e, nvalid, lastLane, et);
+            sample_range_w<KPL, MODE_DET>(S, al, q, q.sweep, lane, b, e, nvalid, lastLane, et);
             lds_load_f<KPL>(S.nd, lane, nd);
             store_kvec<KPL>(reinterpret_cast<int32_t*>(ndk_row), lane, __builtin_bit_cast(ivec<KPL>, nd));
         }
@@ -2389,10 +2392,11 @@ hipError_t launch_build_counts(int kpl, const uint32_t* words, const uint16_t* z
     return hipGetLastError();
 }
 
-hipError_t launch_sweep_device(int kpl, const SweepParams& p, int grid, hipStream_t s) {
+hipError_t launch_sweep_device(int kpl, const SweepParams& p, int grid, int waves, hipStream_t s) {
+    if (waves < 1 || waves > kSweepWaves) return hipErrorInvalidValue;
     LDA_DISPATCH_KPL_ETA(kpl, p.prior_row != nullptr,
-                         hipLaunchKernelGGL((sweep_device_kernel<KPL, ASYM>), dim3(grid), dim3(64 * kSweepWaves),
-                                            wave_lds_bytes<KPL>(kSweepWaves), s, p));
+                         hipLaunchKernelGGL((sweep_device_kernel<KPL, ASYM>), dim3(grid), dim3(64 * waves),
+                                            wave_lds_bytes<KPL>(waves), s, p));
     return hipGetLastError();
 }
 
@@ -2425,21 +2429,21 @@ hipError_t launch_sweep_det(int kpl, const SweepParams& p, hipStream_t s) {
 }
 
 template <int KPL, bool ASYM>
-static int sweep_device_blocks_kpl() {
+static int sweep_device_blocks_kpl(int waves) {
     int n = 0;
-    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sweep_device_kernel<KPL, ASYM>, 64 * kSweepWaves,
-                                                                      wave_lds_bytes<KPL>(kSweepWaves));
+    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sweep_device_kernel<KPL, ASYM>, 64 * waves,
+                                                                      wave_lds_bytes<KPL>(waves));
     return (e == hipSuccess && n > 0) ? n : 1;
 }
 
-int sweep_device_blocks_per_cu(int kpl, bool asym) {
+int sweep_device_blocks_per_cu(int kpl, bool asym, int waves) {
     switch (kpl) {
-        case 1: return asym ? sweep_device_blocks_kpl<1, true>() : sweep_device_blocks_kpl<1, false>();
-        case 2: return asym ? sweep_device_blocks_kpl<2, true>() : sweep_device_blocks_kpl<2, false>();
-        case 4: return asym ? sweep_device_blocks_kpl<4, true>() : sweep_device_blocks_kpl<4, false>();
-        case 8: return asym ? sweep_device_blocks_kpl<8, true>() : sweep_device_blocks_kpl<8, false>();
-        case 16: return asym ? sweep_device_blocks_kpl<16, true>() : sweep_device_blocks_kpl<16, false>();
-        case 32: return asym ? sweep_device_blocks_kpl<32, true>() : sweep_device_blocks_kpl<32, false>();
+        case 1: return asym ? sweep_device_blocks_kpl<1, true>(waves) : sweep_device_blocks_kpl<1, false>(waves);
+        case 2: return asym ? sweep_device_blocks_kpl<2, true>(waves) : sweep_device_blocks_kpl<2, false>(waves);
+        case 4: return asym ? sweep_device_blocks_kpl<4, true>(waves) : sweep_device_blocks_kpl<4, false>(waves);
+        case 8: return asym ? sweep_device_blocks_kpl<8, true>(waves) : sweep_device_blocks_kpl<8, false>(waves);
+        case 16: return asym ? sweep_device_blocks_kpl<16, true>(waves) : sweep_device_blocks_kpl<16, false>(waves);
+        case 32: return asym ? sweep_device_blocks_kpl<32, true>(waves) : sweep_device_blocks_kpl<32, false>(waves);
         default: return 1;
     }
 }
@@ -2563,10 +2567,11 @@ hipError_t launch_alpha_update_hist(float* alpha, const int32_t* hist, int Lmax,
     return hipGetLastError();
 }
 
-hipError_t launch_sweep_device_w(int kpl, const SweepParams& p, int grid, hipStream_t s) {
+hipError_t launch_sweep_device_w(int kpl, const SweepParams& p, int grid, int waves, hipStream_t s) {
+    if (waves < 1 || waves > kSweepWaves) return hipErrorInvalidValue;
     LDA_DISPATCH_KPL_ETA(kpl, p.prior_row != nullptr,
-                         hipLaunchKernelGGL((sweep_device_w_kernel<KPL, ASYM>), dim3(grid), dim3(64 * kSweepWaves),
-                                            wave_lds_bytes<KPL>(kSweepWaves), s, p));
+                         hipLaunchKernelGGL((sweep_device_w_kernel<KPL, ASYM>), dim3(grid), dim3(64 * waves),
+                                            wave_lds_bytes<KPL>(waves), s, p));
     return hipGetLastError();
 }
 
@@ -2591,21 +2596,42 @@ hipError_t launch_merge_copies_f(float* nkw, const float* loc, int64_t n, int W,
 }
 
 template <int KPL, bool ASYM>
-static int sweep_device_w_blocks_kpl() {
+static int sweep_det_blocks_kpl(bool weighted) {
     int n = 0;
-    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sweep_device_w_kernel<KPL, ASYM>,
-                                                                      64 * kSweepWaves, wave_lds_bytes<KPL>(kSweepWaves));
+    const hipError_t e =
+        weighted ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sweep_det_w_kernel<KPL, ASYM>, 64, wave_lds_bytes<KPL>())
+                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sweep_det_kernel<KPL, ASYM>, 64, wave_lds_bytes<KPL>());
     return (e == hipSuccess && n > 0) ? n : 1;
 }
 
-int sweep_device_w_blocks_per_cu(int kpl, bool asym) {
+int sweep_det_blocks_per_cu(int kpl, bool asym, bool weighted) {
     switch (kpl) {
-        case 1: return asym ? sweep_device_w_blocks_kpl<1, true>() : sweep_device_w_blocks_kpl<1, false>();
-        case 2: return asym ? sweep_device_w_blocks_kpl<2, true>() : sweep_device_w_blocks_kpl<2, false>();
-        case 4: return asym ? sweep_device_w_blocks_kpl<4, true>() : sweep_device_w_blocks_kpl<4, false>();
-        case 8: return asym ? sweep_device_w_blocks_kpl<8, true>() : sweep_device_w_blocks_kpl<8, false>();
-        case 16: return asym ? sweep_device_w_blocks_kpl<16, true>() : sweep_device_w_blocks_kpl<16, false>();
-        case 32: return asym ? sweep_device_w_blocks_kpl<32, true>() : sweep_device_w_blocks_kpl<32, false>();
+        case 1: return asym ? sweep_det_blocks_kpl<1, true>(weighted) : sweep_det_blocks_kpl<1, false>(weighted);
+        case 2: return asym ? sweep_det_blocks_kpl<2, true>(weighted) : sweep_det_blocks_kpl<2, false>(weighted);
+        case 4: return asym ? sweep_det_blocks_kpl<4, true>(weighted) : sweep_det_blocks_kpl<4, false>(weighted);
+        case 8: return asym ? sweep_det_blocks_kpl<8, true>(weighted) : sweep_det_blocks_kpl<8, false>(weighted);
+        case 16: return asym ? sweep_det_blocks_kpl<16, true>(weighted) : sweep_det_blocks_kpl<16, false>(weighted);
+        case 32: return asym ? sweep_det_blocks_kpl<32, true>(weighted) : sweep_det_blocks_kpl<32, false>(weighted);
+        default: return 1;
+    }
+}
+
+template <int KPL, bool ASYM>
+static int sweep_device_w_blocks_kpl(int waves) {
+    int n = 0;
+    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sweep_device_w_kernel<KPL, ASYM>,
+                                                                      64 * waves, wave_lds_bytes<KPL>(waves));
+    return (e == hipSuccess && n > 0) ? n : 1;
+}
+
+int sweep_device_w_blocks_per_cu(int kpl, bool asym, int waves) {
+    switch (kpl) {
+        case 1: return asym ? sweep_device_w_blocks_kpl<1, true>(waves) : sweep_device_w_blocks_kpl<1, false>(waves);
+        case 2: return asym ? sweep_device_w_blocks_kpl<2, true>(waves) : sweep_device_w_blocks_kpl<2, false>(waves);
+        case 4: return asym ? sweep_device_w_blocks_kpl<4, true>(waves) : sweep_device_w_blocks_kpl<4, false>(waves);
+        case 8: return asym ? sweep_device_w_blocks_kpl<8, true>(waves) : sweep_device_w_blocks_kpl<8, false>(waves);
+        case 16: return asym ? sweep_device_w_blocks_kpl<16, true>(waves) : sweep_device_w_blocks_kpl<16, false>(waves);
+        case 32: return asym ? sweep_device_w_blocks_kpl<32, true>(waves) : sweep_device_w_blocks_kpl<32, false>(waves);
         default: return 1;
     }
 }

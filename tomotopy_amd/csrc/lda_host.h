@@ -149,6 +149,25 @@ struct lda_ctx {
     hipGraphExec_t infer_exec = nullptr;
     int infer_exec_segmax = 0;  // the buckets the cached graph launches
 
+    // device-wide schedule: dev_auto picks the concurrency from the corpus
+    // (at most ceil(D / kInFlightDiv) documents in flight); otherwise at most
+    // dev_workers workgroups (0: as many as are resident), each of dev_waves
+    // wavefronts sharing one private n_k
+    bool dev_auto = true;
+    int dev_workers = 0;
+    int dev_waves = lda::kMaxSweepWaves;
+    // partition rounds of one sweep captured as a graph (rebuilt when the
+    // parameters it was captured with change); the sweep number is read
+    // from d_sweep so that one graph serves every sweep
+    hipGraphExec_t part_exec = nullptr;
+    lda::SweepParams part_key{};
+    bool part_key_w = false;
+    DevBuf<uint32_t> d_sweep;
+
+    // lda_debug_fail_at_step: the training iteration at this global step
+    // reports LDA_ERR_TRAINING (-1: never); private copies never inherit it
+    int64_t fail_at_step = -1;
+
     // kernel timing of the last call
     double stat_ms = 0.0;
     double stat_launches = 0.0;

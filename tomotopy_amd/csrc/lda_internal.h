@@ -48,6 +48,9 @@ struct SweepParams {
     const int32_t* prior_row;
     const float* prior_eta;
     const float* etasum;
+    // partition rounds replayed from a graph: the sweep number is read here
+    // (NULL: use `sweep`)
+    const uint32_t* sweep_dev;
 };
 
 // One fold-in job: a document's tokens [b, e) in the token arena, the
@@ -156,9 +159,9 @@ hipError_t launch_infer_gen(int kpl, int segs, const InferGenParams& p, int grid
 int infer_gen_blocks_per_cu(int kpl, bool wt, bool asym, int segs);
 
 // Term-weighted (IDF / PMI) training: the count buffers hold float32 values.
-hipError_t launch_sweep_device_w(int kpl, const SweepParams& p, int grid, hipStream_t s);
+hipError_t launch_sweep_device_w(int kpl, const SweepParams& p, int grid, int waves, hipStream_t s);
 hipError_t launch_sweep_det_w(int kpl, const SweepParams& p, hipStream_t s);
-int sweep_device_w_blocks_per_cu(int kpl, bool asym);
+int sweep_device_w_blocks_per_cu(int kpl, bool asym, int waves);
 // copy_merge on float counts: W private copies as in launch_sweep_copy; the
 // merge is n_kw = local_0 + sum_{w>=1} (local_w - old) in worker order
 hipError_t launch_sweep_copy_w(int kpl, const SweepParams& p, hipStream_t s);
@@ -186,13 +189,19 @@ hipError_t launch_init_topics(const uint32_t* words, uint16_t* z, uint64_t N, co
                               int kpl, int Kp, float eta, hipStream_t s);
 hipError_t launch_build_counts(int kpl, const uint32_t* words, const uint16_t* z, const uint64_t* off, int64_t D,
                                int32_t* ndk, int32_t* nkw, int32_t* nk, int Kp, int V, hipStream_t s);
-hipError_t launch_sweep_device(int kpl, const SweepParams& p, int grid, hipStream_t s);
+// Device-wide sweep: grid workgroups (workers) of `waves` wavefronts (1..8)
+// sharing one private n_k.
+constexpr int kMaxSweepWaves = 8;
+hipError_t launch_sweep_device(int kpl, const SweepParams& p, int grid, int waves, hipStream_t s);
 hipError_t launch_sweep_det(int kpl, const SweepParams& p, hipStream_t s);
 // copy_merge: W private copies of n_kw (W x V x Kp) and of n_k (W x Kp)
 hipError_t launch_sweep_copy(int kpl, const SweepParams& p, hipStream_t s);
 hipError_t launch_merge_copies(int32_t* nkw, const int32_t* loc, int64_t n, int W, hipStream_t s);
 hipError_t launch_replicate(int32_t* dst, const int32_t* src, int64_t n, int W, hipStream_t s);
-int sweep_device_blocks_per_cu(int kpl, bool asym);
+int sweep_device_blocks_per_cu(int kpl, bool asym, int waves);
+// Resident one-wavefront workgroups per CU of the partition round kernel
+// (integer or float counts).
+int sweep_det_blocks_per_cu(int kpl, bool asym, bool weighted);
 hipError_t launch_copy_i32(int32_t* dst, const int32_t* src, int64_t n, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* dst, uint32_t v, int64_t n, hipStream_t s);
 hipError_t launch_add_i32(int32_t* dst, const int32_t* src, int64_t n, hipStream_t s);

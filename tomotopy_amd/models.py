@@ -325,8 +325,13 @@ call before `train` (py_LDA.cpp:241-269, LDAModel.hpp:1031-1045).'''
             if callback is not None:
                 callback(self, it, iterations)
             n = min(callback_interval, iterations - it)
-            self._ctx.train(n, workers=int(workers), scheme=parallel)
-            self._cache = {}
+            try:
+                self._ctx.train(n, workers=int(workers), scheme=parallel)
+            finally:
+                # sweeps before a failure changed the state
+                self._cache = {}
+                self._global_step = self._ctx.global_step
+                self._alphas = self._ctx.get_alpha()
             it += n
         if callback is not None:
             callback(self, iterations, iterations)
@@ -736,12 +741,10 @@ trained further); otherwise only the topic parameters (inference only).'''
         new._ctx = None
         new._cache = {}
         if self._prepared:
-            z = self._state("z")
-            new._ctx = _native.Context(self._k, self._real_v, self._alphas, self._eta, self._seed, tw=int(self._tw))
-            new._upload_priors(new._ctx)
-            new._ctx.upload_corpus(self._words, self._off, z, weights=self._weights)
-            new._ctx.set_optim(self._optim_interval, self._burn_in)
-            new._ctx.global_step = self._global_step
+            # the device state is copied as is (topics, n_dk, n_kw, n_k, the
+            # float counts of IDF / PMI models, priors, alpha, global step),
+            # as tomoto's copy constructor copies globalState and the documents
+            new._ctx = self._ctx.copy()
         return new
 
     def summary(self, initial_hp=True, params=True, topic_word_top_n=5, file=None, flush=False):
