@@ -228,6 +228,21 @@ int lda_sample_frozen(lda_ctx* ctx, int64_t n, const int64_t* doc, const uint32_
 int lda_infer(lda_ctx* ctx, const uint32_t* words, const uint64_t* doc_off, int64_t D, int iterations,
               uint64_t infer_seed, int mode, float* theta, double* ll, uint16_t* z_out);
 
+/* Batched fold-in that allocates no device memory (no counterpart in the
+ * reference; for callers that cannot allocate: code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same results as
+ * lda_infer(mode 0) bit for bit; the token arena, jobs, parameters, bucket
+ * counters and outputs of the call are staged in a 64 MiB device array that
+ * the library holds once per GPU for the life of the process.  A call whose
+ * arena (documents x 32 B + tokens x 6 B + documents x (4K + 8) B, each
+ * region rounded up to 256 B, plus 512 B) exceeds 64 MiB fails with
+ * LDA_ERR_INVALID_ARGUMENT.  Every call on one GPU shares that array, which
+ * keeps the last call's contents: calls that overlap on one GPU (from
+ * several threads or contexts) race and must be serialised by the caller.
+ * TermWeight ONE models (with or without word priors). */
+int lda_infer_noalloc(lda_ctx* ctx, const uint32_t* words, const uint64_t* doc_off, int64_t D, int iterations,
+                      uint64_t infer_seed, float* theta, double* ll, uint16_t* z_out);
+
 /* Fold-in of a term-weighted (IDF / PMI) model: as lda_infer, with the
  * tokens' weights (tomoto computes them in initializeDocState,
  * LDAModel.hpp:794-809, from the model's vocabWeights: IDF w = vocabWeights[v],

@@ -65,6 +65,7 @@ _SIGS = {
     "lda_get_weighted_n": [_c_p, _c_p],
     "lda_sample_frozen": [_c_p, _i64, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p],
     "lda_infer": [_c_p, _c_p, _c_p, _i64, _i32, _u64, _i32, _c_p, _c_p, _c_p],
+    "lda_infer_noalloc": [_c_p, _c_p, _c_p, _i64, _i32, _u64, _c_p, _c_p, _c_p],
     "lda_infer_w": [_c_p, _c_p, _c_p, _i64, _c_p, _i32, _u64, _i32, _c_p, _c_p, _c_p],
     "lda_kernel_stats": [_c_p, _c_p],
     "lda_set_infer_options": [_c_p, _u32, _i32],
@@ -315,6 +316,19 @@ class Context:
                 raise ValueError("weights must have one entry per token")
             _check(load().lda_infer_w(self._h, _ptr(words), _ptr(doc_off), D, _ptr(wts), int(iterations), seed,
                                       int(mode), _ptr(theta), _ptr(ll), _ptr(z)))
+        return theta, ll, z
+
+    def infer_noalloc(self, words, doc_off, iterations, infer_seed=0, want_z=False):
+        """Batched fold-in staged in the library's fixed device scratch (lda_infer_noalloc):
+        no device allocation; same results as infer(mode=0)."""
+        words = np.ascontiguousarray(words, dtype=np.uint32)
+        doc_off = np.ascontiguousarray(doc_off, dtype=np.uint64)
+        D = len(doc_off) - 1
+        theta = np.zeros((max(D, 0), self.k), dtype=np.float32)
+        ll = np.zeros(max(D, 0), dtype=np.float64)
+        z = np.zeros(int(doc_off[-1]) if D > 0 else 0, dtype=np.uint16) if want_z else None
+        _check(load().lda_infer_noalloc(self._h, _ptr(words), _ptr(doc_off), D, int(iterations),
+                                        int(infer_seed) & (2**64 - 1), _ptr(theta), _ptr(ll), _ptr(z)))
         return theta, ll, z
 
     def infer_together(self, words, doc_off, iterations, infer_seed=0, workers=0, scheme=0, weights=None):

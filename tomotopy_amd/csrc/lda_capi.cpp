@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -610,6 +611,116 @@ int set_counts_t(lda_ctx* c, const T* nkw, const T* nk) {
 }  // namespace
 
 namespace {
+
+// Device buffers of one batched / plain fold-in call: the token arena
+// (words, z), the jobs, the parameter block, the bucket counters and the
+// outputs.
+struct FoldArena {
+    uint32_t* words;
+    uint16_t* z;
+    lda::InferJob* jobs;
+    lda::InferParams* params;
+    uint32_t* counters;
+    float* theta;
+    double* ll;
+};
+
+// Jobs of a fold-in call grouped by bucket (segments per document), longest
+// first inside a bucket (the persistent kernels take them in this order), and
+// the parameter block pointing into the arena.  mode 1 keeps document order.
+void build_fold_jobs(lda_ctx* c, const uint64_t* doc_off, int64_t D, int iterations, uint64_t infer_seed, int mode,
+                     const FoldArena& a, std::vector<int>& segs, std::vector<lda::InferJob>& jobs,
+                     lda::InferParams& p) {
+    segs.resize(D);
+    for (int64_t d = 0; d < D; ++d) segs[d] = lda::fold_segments(doc_off[d + 1] - doc_off[d], c->seg_tokens, c->seg_max);
+    std::vector<uint32_t> order(D);
+    std::iota(order.begin(), order.end(), 0u);
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+        if (segs[x] != segs[y]) return segs[x] < segs[y];
+        return (doc_off[x + 1] - doc_off[x]) > (doc_off[y + 1] - doc_off[y]);
+    });
+    jobs.resize(D);
+    p = lda::InferParams{};
+    for (int64_t i = 0; i < D; ++i) {
+        const uint32_t d = mode == 0 ? order[i] : (uint32_t)i;
+        jobs[i] = lda::InferJob{doc_off[d], doc_off[d + 1], 0, iterations, iterations, (int32_t)d};
+    }
+    for (int bk = 0; bk < lda::kFoldBuckets; ++bk) p.bucket_lo[bk] = p.bucket_hi[bk] = 0;
+    if (mode == 0) {
+        int64_t i = 0;
+        for (int bk = 0; bk < lda::kFoldBuckets; ++bk) {
+            p.bucket_lo[bk] = i;
+            while (i < D && lda::fold_bucket(segs[order[i]]) == bk) ++i;
+            p.bucket_hi[bk] = i;
+        }
+    }
+    p.words = a.words; p.z = a.z; p.jobs = a.jobs; p.njobs = D;
+    p.nkw = c->d_nkw.p; p.nk = c->d_nk.p; p.alpha = c->d_alpha.p; p.alpha_cdf = c->d_alpha_cdf.p;
+    p.eta = c->eta; p.veta = c->veta; p.K = c->K; p.V = c->V; p.Kp = c->Kp;
+    p.seed_lo = (uint32_t)infer_seed; p.seed_hi = (uint32_t)(infer_seed >> 32);
+    p.theta = a.theta; p.ll = a.ll;
+    double asd = 0.0;
+    float asf = 0.0f;
+    for (int k = 0; k < c->K; ++k) { asd += (double)c->alpha[k]; asf = asf + c->alpha[k]; }
+    p.asum_d = asd; p.asum_f = asf;
+    if (c->has_priors) {
+        p.prior_row = c->d_prior_row.p; p.prior_eta = c->d_prior_eta.p;
+        p.etasum = c->d_etasum.p; p.etasum_d = c->d_etasum_d.p;
+    }
+    p.counters = mode == 0 ? a.counters : nullptr;
+    p.job_base = 0;
+}
+
+// Outputs of a finished fold-in call back to the host.
+int fold_download(lda_ctx* c, const FoldArena& a, int64_t D, uint64_t N, int iterations, float* theta, double* ll,
+                  uint16_t* z_out) {
+    const hipStream_t s = c->stream;
+    HIP_TRY(hipEventSynchronize(c->ev1));
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->stat_ms = ms;
+    c->stat_tokens = (double)N * iterations;
+    HIP_TRY(hipMemcpyAsync(theta, a.theta, (size_t)D * c->K * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(ll, a.ll, D * 8, hipMemcpyDeviceToHost, s));
+    if (z_out && N) HIP_TRY(hipMemcpyAsync(z_out, a.z, N * 2, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return LDA_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Fold-in without allocation (lda_infer_noalloc).  Every device buffer of the
+// call lives in `scratch`, a file-scope device array of kNoAllocScratchBytes
+// that each GPU holds once for the life of the process.  Host code never uses
+// the array's name as an address: its device address is taken once per
+// device with hipGetSymbolAddress (at context creation, with that device
+// current) and kept in g_scratch_dev.
+namespace {
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxDevices] = {};
+
+int noalloc_scratch_init(int device) {
+    if (device < 0 || device >= kMaxDevices) return fail(LDA_ERR_DEVICE, "device %d beyond the scratch table", device);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_dev[device]) return LDA_OK;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return fail(LDA_ERR_DEVICE, "hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+    g_scratch_dev[device] = ptr;
+    return LDA_OK;
+}
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+}  // namespace
+
+namespace {
 // Device-to-device copy of a whole buffer (its allocated capacity).
 template <typename T>
 hipError_t dup_buf(DevBuf<T>& dst, const DevBuf<T>& src, hipStream_t s) {
@@ -676,6 +787,7 @@ int lda_ctx_create(int K, int V, const float* alpha, float eta, int termweight, 
         return cleanup(fail(LDA_ERR_DEVICE, "hipEventCreate failed"));
     if (c->fold_streams.create() != hipSuccess) return cleanup(fail(LDA_ERR_DEVICE, "fold-in stream creation failed"));
     compute_fold_grid(c);
+    if (int src = noalloc_scratch_init(device)) return cleanup(src);
     if (c->d_alpha.alloc(c->Kp) || c->d_alpha_cdf.alloc(K) || c->d_nkw.alloc((size_t)std::max(V, 1) * c->Kp) ||
         c->d_nk.alloc(c->Kp) || c->d_nk_next.alloc(c->Kp) || c->d_counter.alloc(1))
         return cleanup(fail(LDA_ERR_OUT_OF_MEMORY, "device allocation failed"));
@@ -1624,51 +1736,16 @@ int lda_infer(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off, int64_
     HIP_TRY(c->i_counter.alloc(lda::kFoldBuckets));
     HIP_TRY(c->i_theta.alloc((size_t)D * c->K));
     HIP_TRY(c->i_ll.alloc(D));
-    if (N) HIP_TRY(hipMemcpyAsync(c->i_words.p, words, N * 4, hipMemcpyHostToDevice, s));
-    // Jobs grouped by bucket (segments per document), longest first inside a
-    // bucket (the persistent kernels take them in this order).
-    std::vector<int> segs(D);
-    for (int64_t d = 0; d < D; ++d) segs[d] = lda::fold_segments(doc_off[d + 1] - doc_off[d], c->seg_tokens, c->seg_max);
-    std::vector<uint32_t> order(D);
-    std::iota(order.begin(), order.end(), 0u);
-    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
-        if (segs[a] != segs[b]) return segs[a] < segs[b];
-        return (doc_off[a + 1] - doc_off[a]) > (doc_off[b + 1] - doc_off[b]);
-    });
-    std::vector<lda::InferJob> jobs(D);
-    lda::InferParams p{};
-    for (int64_t i = 0; i < D; ++i) {
-        const uint32_t d = mode == 0 ? order[i] : (uint32_t)i;
-        jobs[i] = lda::InferJob{doc_off[d], doc_off[d + 1], 0, iterations, iterations, (int32_t)d};
-    }
-    for (int bk = 0; bk < lda::kFoldBuckets; ++bk) p.bucket_lo[bk] = p.bucket_hi[bk] = 0;
-    if (mode == 0) {
-        int64_t i = 0;
-        for (int bk = 0; bk < lda::kFoldBuckets; ++bk) {
-            p.bucket_lo[bk] = i;
-            while (i < D && lda::fold_bucket(segs[order[i]]) == bk) ++i;
-            p.bucket_hi[bk] = i;
-        }
-    }
-    HIP_TRY(hipMemcpyAsync(c->i_jobs.p, jobs.data(), D * sizeof(lda::InferJob), hipMemcpyHostToDevice, s));
-    p.words = c->i_words.p; p.z = c->i_z.p; p.jobs = c->i_jobs.p; p.njobs = D;
-    p.nkw = c->d_nkw.p; p.nk = c->d_nk.p; p.alpha = c->d_alpha.p; p.alpha_cdf = c->d_alpha_cdf.p;
-    p.eta = c->eta; p.veta = c->veta; p.K = c->K; p.V = c->V; p.Kp = c->Kp;
-    p.seed_lo = (uint32_t)infer_seed; p.seed_hi = (uint32_t)(infer_seed >> 32);
-    p.theta = c->i_theta.p; p.ll = c->i_ll.p;
-    double asd = 0.0;
-    float asf = 0.0f;
-    for (int k = 0; k < c->K; ++k) { asd += (double)c->alpha[k]; asf = asf + c->alpha[k]; }
-    p.asum_d = asd; p.asum_f = asf;
-    if (c->has_priors) {
-        p.prior_row = c->d_prior_row.p; p.prior_eta = c->d_prior_eta.p;
-        p.etasum = c->d_etasum.p; p.etasum_d = c->d_etasum_d.p;
-    }
+    const FoldArena a{c->i_words.p, c->i_z.p, c->i_jobs.p, c->i_params.p, c->i_counter.p, c->i_theta.p, c->i_ll.p};
+    if (N) HIP_TRY(hipMemcpyAsync(a.words, words, N * 4, hipMemcpyHostToDevice, s));
+    std::vector<int> segs;
+    std::vector<lda::InferJob> jobs;
+    lda::InferParams p;
+    build_fold_jobs(c, doc_off, D, iterations, infer_seed, mode, a, segs, jobs, p);
+    HIP_TRY(hipMemcpyAsync(a.jobs, jobs.data(), D * sizeof(lda::InferJob), hipMemcpyHostToDevice, s));
     c->stat_ms = c->stat_launches = c->stat_tokens = 0.0;
     if (mode == 0) {
-        p.counters = c->i_counter.p;
-        p.job_base = 0;
-        HIP_TRY(hipMemcpyAsync(c->i_params.p, &p, sizeof p, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(a.params, &p, sizeof p, hipMemcpyHostToDevice, s));
         // the graph launches the buckets up to seg_max, each with its full persistent grid
         if (!c->infer_exec || c->infer_exec_segmax != c->seg_max) {
             if (c->infer_exec) { (void)hipGraphExecDestroy(c->infer_exec); c->infer_exec = nullptr; }
@@ -1676,11 +1753,11 @@ int lda_infer(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off, int64_
             for (int bk = 0; bk < lda::kFoldBuckets; ++bk) grids[bk] = (1 << bk) <= c->seg_max ? c->fold_grid[bk] : 0;
             hipGraph_t g = nullptr;
             HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            hipError_t e1 = lda::launch_fill_u32(c->i_counter.p, 0u, lda::kFoldBuckets, s);
-            hipError_t e2 = e1 == hipSuccess ? lda_host::launch_fold_buckets(c->kpl, c->has_priors, c->i_params.p, grids, s,
+            hipError_t e1 = lda::launch_fill_u32(a.counters, 0u, lda::kFoldBuckets, s);
+            hipError_t e2 = e1 == hipSuccess ? lda_host::launch_fold_buckets(c->kpl, c->has_priors, a.params, grids, s,
                                                                              c->fold_streams)
                                              : e1;
-            if (e2 == hipSuccess) e2 = lda::launch_infer_finalize(c->kpl, c->has_priors, c->i_params.p, c->fold_grid[0], s);
+            if (e2 == hipSuccess) e2 = lda::launch_infer_finalize(c->kpl, c->has_priors, a.params, c->fold_grid[0], s);
             hipError_t e3 = hipStreamEndCapture(s, &g);
             if (e2 != hipSuccess || e3 != hipSuccess) {
                 if (g) (void)hipGraphDestroy(g);
@@ -1697,28 +1774,73 @@ int lda_infer(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off, int64_
         c->stat_launches = 1;
     } else {
         // plain: one launch and one synchronisation per document
-        p.counters = nullptr;
         HIP_TRY(hipEventRecord(c->ev0, s));
         for (int64_t d = 0; d < D; ++d) {
             p.job_base = d;
-            HIP_TRY(hipMemcpyAsync(c->i_params.p, &p, sizeof p, hipMemcpyHostToDevice, s));
-            HIP_TRY(lda::launch_infer(c->kpl, segs[d], c->has_priors, c->i_params.p, 1, s));
-            HIP_TRY(lda::launch_infer_finalize(c->kpl, c->has_priors, c->i_params.p, 1, s));
+            HIP_TRY(hipMemcpyAsync(a.params, &p, sizeof p, hipMemcpyHostToDevice, s));
+            HIP_TRY(lda::launch_infer(c->kpl, segs[d], c->has_priors, a.params, 1, s));
+            HIP_TRY(lda::launch_infer_finalize(c->kpl, c->has_priors, a.params, 1, s));
             HIP_TRY(hipStreamSynchronize(s));
         }
         HIP_TRY(hipEventRecord(c->ev1, s));
         c->stat_launches = (double)D;
     }
-    HIP_TRY(hipEventSynchronize(c->ev1));
-    float ms = 0.0f;
-    HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    c->stat_ms = ms;
-    c->stat_tokens = (double)N * iterations;
-    HIP_TRY(hipMemcpyAsync(theta, c->i_theta.p, (size_t)D * c->K * 4, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipMemcpyAsync(ll, c->i_ll.p, D * 8, hipMemcpyDeviceToHost, s));
-    if (z_out && N) HIP_TRY(hipMemcpyAsync(z_out, c->i_z.p, N * 2, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    return LDA_OK;
+    return fold_download(c, a, D, N, iterations, theta, ll, z_out);
+}
+
+int lda_infer_noalloc(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off, int64_t D, int iterations,
+                      uint64_t infer_seed, float* theta, double* ll, uint16_t* z_out) {
+    bool done = false;
+    int rc = check_infer_args(c, words, doc_off, D, iterations, 0, theta, ll, &done);
+    if (rc || done) return rc;
+    if (c->tw != LDA_TW_ONE)
+        return fail(LDA_ERR_INVALID_ARGUMENT, "term-weighted models need the token weights: call lda_infer_w");
+    const uint64_t N = doc_off[D];
+    // the arena inside the scratch array, each region 256-byte aligned
+    const size_t o_params = 0;
+    const size_t o_counters = o_params + align256(sizeof(lda::InferParams));
+    const size_t o_jobs = o_counters + align256(lda::kFoldBuckets * sizeof(uint32_t));
+    const size_t o_words = o_jobs + align256((size_t)D * sizeof(lda::InferJob));
+    const size_t o_z = o_words + align256((size_t)N * sizeof(uint32_t));
+    const size_t o_theta = o_z + align256((size_t)N * sizeof(uint16_t));
+    const size_t o_ll = o_theta + align256((size_t)D * c->K * sizeof(float));
+    const size_t need = o_ll + align256((size_t)D * sizeof(double));
+    if (need > kNoAllocScratchBytes)
+        return fail(LDA_ERR_INVALID_ARGUMENT,
+                    "lda_infer_noalloc: %lld documents of %llu tokens at K = %d need %.1f MiB of scratch; the limit is "
+                    "%zu MiB (use lda_infer)",
+                    (long long)D, (unsigned long long)N, c->K, need / 1048576.0, kNoAllocScratchBytes >> 20);
+    unsigned char* base;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        base = static_cast<unsigned char*>(g_scratch_dev[c->device]);
+    }
+    if (!base) return fail(LDA_ERR_STATE, "the fold-in scratch of device %d is not set up", c->device);
+    HIP_TRY(hipSetDevice(c->device));
+    const hipStream_t s = c->stream;
+    const FoldArena a{reinterpret_cast<uint32_t*>(base + o_words), reinterpret_cast<uint16_t*>(base + o_z),
+                      reinterpret_cast<lda::InferJob*>(base + o_jobs), reinterpret_cast<lda::InferParams*>(base + o_params),
+                      reinterpret_cast<uint32_t*>(base + o_counters), reinterpret_cast<float*>(base + o_theta),
+                      reinterpret_cast<double*>(base + o_ll)};
+    std::vector<int> segs;
+    std::vector<lda::InferJob> jobs;
+    lda::InferParams p;
+    build_fold_jobs(c, doc_off, D, iterations, infer_seed, 0, a, segs, jobs, p);
+    if (N) HIP_TRY(hipMemcpyAsync(a.words, words, N * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(a.jobs, jobs.data(), D * sizeof(lda::InferJob), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(a.params, &p, sizeof p, hipMemcpyHostToDevice, s));
+    c->stat_ms = c->stat_launches = c->stat_tokens = 0.0;
+    int grids[lda::kFoldBuckets];
+    for (int bk = 0; bk < lda::kFoldBuckets; ++bk) grids[bk] = (1 << bk) <= c->seg_max ? c->fold_grid[bk] : 0;
+    HIP_TRY(hipEventRecord(c->ev0, s));
+    // the bucket counters start at zero (a fill kernel: the previous call's
+    // values are still there)
+    HIP_TRY(lda::launch_fill_u32(a.counters, 0u, lda::kFoldBuckets, s));
+    HIP_TRY(lda_host::launch_fold_buckets(c->kpl, c->has_priors, a.params, grids, s, c->fold_streams));
+    HIP_TRY(lda::launch_infer_finalize(c->kpl, c->has_priors, a.params, c->fold_grid[0], s));
+    HIP_TRY(hipEventRecord(c->ev1, s));
+    c->stat_launches = 1;
+    return fold_download(c, a, D, N, iterations, theta, ll, z_out);
 }
 
 int lda_set_infer_options(lda_ctx* c, uint32_t seg_tokens, int seg_max) {
