@@ -329,8 +329,21 @@ int lda_serve_stop(lda_server* server);
 
 /* Timing of the last lda_train / lda_sweep / lda_infer call, measured with
  * HIP events on the context's stream around the hot kernels only:
- * stats[0] total kernel ms, stats[1] launches, stats[2] tokens sampled. */
+ * stats[0] total kernel ms, stats[1] launches, stats[2] tokens sampled,
+ * stats[3] 1 when the call's device-wide sweeps ran on narrow columns. */
 int lda_kernel_stats(lda_ctx* ctx, double stats[4]);
+
+/* Narrow columns in the device-wide sweep of TermWeight ONE models (no
+ * counterpart in the reference; on by default): for the duration of an
+ * lda_train / lda_sweep call, the n_kw columns of words with fewer than 2^15
+ * tokens are kept as u16 (after the last word with more, in word-id order:
+ * prepared vocabularies are sorted by frequency, so that is every word past
+ * the most frequent few), halving the bytes the sweep reads per token of
+ * those words; they are written back to the int32 counts at the end of the
+ * call.  Counts outside [0, 2^15) in those columns (set from outside and not
+ * matching the corpus) make the call run on int32 columns.  enable = 0 turns
+ * the narrow columns off. */
+int lda_set_narrow_columns(lda_ctx* ctx, int enable);
 
 #ifdef __cplusplus
 }

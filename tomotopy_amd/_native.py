@@ -68,6 +68,7 @@ _SIGS = {
     "lda_infer_noalloc": [_c_p, _c_p, _c_p, _i64, _i32, _u64, _c_p, _c_p, _c_p],
     "lda_infer_w": [_c_p, _c_p, _c_p, _i64, _c_p, _i32, _u64, _i32, _c_p, _c_p, _c_p],
     "lda_kernel_stats": [_c_p, _c_p],
+    "lda_set_narrow_columns": [_c_p, _i32],
     "lda_set_infer_options": [_c_p, _u32, _i32],
     "lda_get_infer_options": [_c_p, _c_p, _c_p, _c_p],
     "lda_serve_start": [_c_p, _u64, _i32, _i32, _i64, _i32, _u64, _c_p],
@@ -368,7 +369,11 @@ class Context:
     def kernel_stats(self):
         s = np.zeros(4, dtype=np.float64)
         _check(load().lda_kernel_stats(self._h, _ptr(s)))
-        return {"ms": float(s[0]), "launches": int(s[1]), "tokens": float(s[2])}
+        return {"ms": float(s[0]), "launches": int(s[1]), "tokens": float(s[2]), "narrow": bool(s[3])}
+
+    def set_narrow_columns(self, enable=True):
+        """u16 n_kw columns of the less frequent words in the device-wide sweep (on by default)."""
+        _check(load().lda_set_narrow_columns(self._h, int(bool(enable))))
 
 
 class Server:

@@ -247,7 +247,42 @@ lda::SweepParams base_params(lda_ctx* c) {
         p.prior_eta = c->d_prior_eta.p;
         p.etasum = c->d_etasum.p;
     }
+    if (c->narrow_active) {
+        p.nkw16 = c->d_nkw16.p;
+        p.nkw_hot = c->nkw_hot;
+    }
     return p;
+}
+
+// Narrow columns for the device-wide sweeps of one lda_train / lda_sweep
+// call (integer counts): pack rows [nkw_hot, V) of n_kw into u16 once at the
+// start of the call and write them back at its end, so that every other
+// kernel keeps reading int32 columns.  When a cell is outside [0, 2^15) (counts
+// set from outside that do not match the corpus) the call runs on int32
+// columns instead.
+int narrow_begin(lda_ctx* c, int W, bool copy_merge) {
+    c->narrow_active = false;
+    c->narrow_used = false;
+    if (!c->narrow_enabled || W != 0 || copy_merge || c->tw != LDA_TW_ONE || c->nkw_hot >= c->V) return LDA_OK;
+    const hipStream_t s = c->stream;
+    HIP_TRY(c->d_nkw16.alloc((size_t)(c->V - c->nkw_hot) * c->Kp));
+    HIP_TRY(c->d_bad.alloc(1));
+    HIP_TRY(lda::launch_fill_u32(c->d_bad.p, 0u, 1, s));
+    HIP_TRY(lda::launch_pack_narrow(c->d_nkw.p, c->d_nkw16.p, c->nkw_hot, c->V, c->Kp, c->d_bad.p, s));
+    uint32_t bad = 0;
+    HIP_TRY(hipMemcpyAsync(&bad, c->d_bad.p, sizeof bad, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    c->narrow_active = bad == 0;
+    c->narrow_used = c->narrow_active;
+    return LDA_OK;
+}
+
+int narrow_end(lda_ctx* c) {
+    if (!c->narrow_active) return LDA_OK;
+    c->narrow_active = false;
+    HIP_TRY(lda::launch_unpack_narrow(c->d_nkw.p, c->d_nkw16.p, c->nkw_hot, c->V, c->Kp, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return LDA_OK;
 }
 
 // Worker counts the device picks when the caller passes workers == 0 with an
@@ -852,6 +887,8 @@ int lda_ctx_copy(lda_ctx* c, lda_ctx** out) {
     t->max_len = c->max_len;
     t->seg_tokens = c->seg_tokens;
     t->seg_max = c->seg_max;
+    t->narrow_enabled = c->narrow_enabled;
+    t->nkw_hot = c->nkw_hot;
     t->dev_auto = c->dev_auto;
     t->dev_workers = c->dev_workers;
     t->dev_waves = c->dev_waves;
@@ -999,6 +1036,17 @@ int lda_upload_corpus(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off
         if (!nkw.empty()) HIP_TRY(hipMemcpyAsync(c->d_nkw.p, nkw.data(), nkw.size() * sizeof(float), hipMemcpyHostToDevice, s));
         HIP_TRY(hipMemcpyAsync(c->d_nk.p, nk.data(), nk.size() * sizeof(float), hipMemcpyHostToDevice, s));
         HIP_TRY(hipStreamSynchronize(s));
+    }
+    // the most frequent words keep int32 columns in the narrow-column sweep:
+    // every word up to the last one with kNarrowCf or more tokens
+    {
+        std::vector<uint64_t> cf(c->V, 0);
+        for (uint64_t g = 0; g < N; ++g)
+            if (words[g] < (uint32_t)c->V) ++cf[words[g]];
+        int hot = 0;
+        for (int v = 0; v < c->V; ++v)
+            if (cf[v] >= lda::kNarrowCf) hot = v + 1;
+        c->nkw_hot = hot;
     }
     // documents longest first for the device-wide schedule
     std::vector<uint32_t> order(D);
@@ -1162,7 +1210,11 @@ int lda_sweep(lda_ctx* c, int workers, int scheme) {
     if (rc) return rc;
     HIP_TRY(hipSetDevice(c->device));
     c->stat_ms = c->stat_launches = c->stat_tokens = 0.0;
-    return sweep_once(c, W, cm);
+    rc = narrow_begin(c, W, cm);
+    if (rc) return rc;
+    rc = sweep_once(c, W, cm);
+    const int rc2 = narrow_end(c);
+    return rc ? rc : rc2;
 }
 
 int lda_train(lda_ctx* c, int iterations, int workers, int scheme) {
@@ -1175,19 +1227,24 @@ int lda_train(lda_ctx* c, int iterations, int workers, int scheme) {
     if (rc) return rc;
     HIP_TRY(hipSetDevice(c->device));
     c->stat_ms = c->stat_launches = c->stat_tokens = 0.0;
-    for (int it = 0; it < iterations; ++it) {
-        if (c->fail_at_step >= 0 && (uint64_t)c->fail_at_step == c->global_step)
-            return fail(LDA_ERR_TRAINING, "training error forced at step %llu", (unsigned long long)c->global_step);
+    rc = narrow_begin(c, W, cm);
+    if (rc) return rc;
+    for (int it = 0; it < iterations && rc == LDA_OK; ++it) {
+        if (c->fail_at_step >= 0 && (uint64_t)c->fail_at_step == c->global_step) {
+            rc = fail(LDA_ERR_TRAINING, "training error forced at step %llu", (unsigned long long)c->global_step);
+            break;
+        }
         rc = sweep_once(c, W, cm);
-        if (rc) return rc;
+        if (rc) break;
         // trainOne (LDAModel.hpp:529-532)
         if (c->global_step >= c->burn_in && c->optim_interval && (c->global_step + 1) % c->optim_interval == 0) {
             rc = optimize_alpha(c);
-            if (rc) return rc;
+            if (rc) break;
         }
         ++c->global_step;
     }
-    return LDA_OK;
+    const int rc2 = narrow_end(c);
+    return rc ? rc : rc2;
 }
 
 int lda_optimize_alpha(lda_ctx* c) {
@@ -1868,7 +1925,13 @@ int lda_kernel_stats(lda_ctx* c, double stats[4]) {
     stats[0] = c->stat_ms;
     stats[1] = c->stat_launches;
     stats[2] = c->stat_tokens;
-    stats[3] = 0.0;
+    stats[3] = c->narrow_used ? 1.0 : 0.0;
+    return LDA_OK;
+}
+
+int lda_set_narrow_columns(lda_ctx* c, int enable) {
+    if (!c) return fail(LDA_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    c->narrow_enabled = enable != 0;
     return LDA_OK;
 }
 

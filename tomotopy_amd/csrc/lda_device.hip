@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
 #include <type_traits>
 
 #include "lda_internal.h"
@@ -109,6 +110,26 @@ __device__ __forceinline__ void store_kvec(int32_t* __restrict__ base, int lane,
         *reinterpret_cast<int2*>(base + lane * 2) = make_int2(o[0], o[1]);
     } else {
         base[lane] = o[0];
+    }
+}
+
+// A K-vector of u16 cells in the lane layout (same positions as load_kvec):
+// for KPL >= 4 each lane reads 8 bytes per group of 4 topics, 512 contiguous
+// bytes per wavefront load.
+template <int KPL, typename V>
+__device__ __forceinline__ void load_kvec_u16(const uint16_t* __restrict__ base, int lane, V& o) {
+    if constexpr (KPL >= 4) {
+#pragma unroll
+        for (int g = 0; g < KPL / 4; ++g) {
+            const uint2 x = *reinterpret_cast<const uint2*>(base + g * 256 + lane * 4);
+            o[4 * g] = (int)(x.x & 0xFFFFu); o[4 * g + 1] = (int)(x.x >> 16);
+            o[4 * g + 2] = (int)(x.y & 0xFFFFu); o[4 * g + 3] = (int)(x.y >> 16);
+        }
+    } else if constexpr (KPL == 2) {
+        const uint32_t x = *reinterpret_cast<const uint32_t*>(base + lane * 2);
+        o[0] = (int)(x & 0xFFFFu); o[1] = (int)(x >> 16);
+    } else {
+        o[0] = (int)base[lane];
     }
 }
 
@@ -442,11 +463,35 @@ constexpr int MODE_ATOMIC = 1;  // shared n_kw: exact int32 atomics
 __constant__ uint64_t kShufflePrimes[16] = {65537, 65539, 65543, 65551, 65557, 65563, 65579, 65581,
                                             65587, 65599, 65609, 65617, 65629, 65633, 65647, 65651};
 
+// Column of word v in the shared n_kw: int32 row of nkw, or with narrow
+// columns (NARROW, words v >= nkw_hot) the u16 row of nkw16.
+template <int KPL, bool NARROW, typename V>
+__device__ __forceinline__ void load_col(const SweepParams& p, int v, int lane, V& o) {
+    if (NARROW && v >= p.nkw_hot) load_kvec_u16<KPL>(p.nkw16 + (int64_t)(v - p.nkw_hot) * p.Kp, lane, o);
+    else load_kvec<KPL>(p.nkw + (int64_t)v * p.Kp, lane, o);
+}
+
+// n_kw[q, v] += delta with an exact atomic.  A u16 cell is updated by a 32-bit
+// atomic add on the word that holds it, the delta shifted into its half:
+// cells stay in [0, 2^16) (a removal takes away a token the cell counts, and
+// a cell of a narrow word never exceeds the word's token count < 2^15), so
+// no carry or borrow crosses into the other half.
+template <bool NARROW>
+__device__ __forceinline__ void col_atomic(const SweepParams& p, int v, int q, int delta) {
+    if (NARROW && v >= p.nkw_hot) {
+        uint32_t* w = reinterpret_cast<uint32_t*>(p.nkw16 + (int64_t)(v - p.nkw_hot) * p.Kp) + (q >> 1);
+        atomicAdd(w, (uint32_t)delta << ((q & 1) * 16));
+    } else {
+        atomicAdd(p.nkw + (int64_t)v * p.Kp + q, delta);
+    }
+}
+
 // sampleDocument over the token range [b, e) (LDAModel.hpp:326-354).  The
 // column of the next distinct word is fetched one token ahead.
-template <int KPL, int MODE, class E>
+template <int KPL, int MODE, class E, bool NARROW = false>
 __device__ __forceinline__ void sample_range(const WaveLds& S, const fvec<KPL>& al, const SweepParams& p, uint32_t sweep,
                                              int lane, uint64_t b, uint64_t e, int nvalid, int lastLane, E& et) {
+    static_assert(!NARROW || MODE == MODE_ATOMIC, "narrow columns are for the shared device-wide n_kw");
     constexpr int SH = kpl_shift<KPL>();
     int vcur = -1;
     int32_t* colp = nullptr;
@@ -468,7 +513,7 @@ __device__ __forceinline__ void sample_range(const WaveLds& S, const fvec<KPL>& 
             if ((unsigned)v >= (unsigned)p.V) continue;
             if (v != vcur) {
                 colp = p.nkw + (int64_t)v * p.Kp;
-                if (v != vnext) load_kvec<KPL>(colp, lane, ncol);
+                if (v != vnext) load_col<KPL, NARROW>(p, v, lane, ncol);
                 lds_store<KPL>(S.col, lane, ncol);
                 et.word(p, v, lane);
                 vcur = v;
@@ -476,7 +521,7 @@ __device__ __forceinline__ void sample_range(const WaveLds& S, const fvec<KPL>& 
             if (i + 1 < n) {
                 const int v2 = readlane_i(wv, i + 1);
                 if (v2 != v && (unsigned)v2 < (unsigned)p.V) {
-                    load_kvec<KPL>(p.nkw + (int64_t)v2 * p.Kp, lane, ncol);
+                    load_col<KPL, NARROW>(p, v2, lane, ncol);
                     vnext = v2;
                 }
             }
@@ -490,8 +535,8 @@ __device__ __forceinline__ void sample_range(const WaveLds& S, const fvec<KPL>& 
                     if (lane == lo) colp[qo] = S.col[qo];
                     if (lane == ln) colp[qn] = S.col[qn];
                 } else {
-                    if (lane == lo) atomicAdd(colp + qo, -1);
-                    if (lane == ln) atomicAdd(colp + qn, 1);
+                    if (lane == lo) col_atomic<NARROW>(p, vcur, qo, -1);
+                    if (lane == ln) col_atomic<NARROW>(p, vcur, qn, 1);
                 }
                 zkeep = (lane == i) ? zn : zkeep;
             }
@@ -507,7 +552,7 @@ __device__ __forceinline__ void sample_range(const WaveLds& S, const fvec<KPL>& 
 // each worker adds its n_k delta to nk_out when it runs out of documents.
 constexpr int kSweepWaves = kMaxSweepWaves;  // most wavefronts per workgroup of the device-wide sweep (one worker)
 
-template <int KPL, bool ASYM>
+template <int KPL, bool ASYM, bool NARROW>
 __global__ void __launch_bounds__(64 * kSweepWaves) sweep_device_kernel(SweepParams p) {
     const int lane = lane_id();
     const WaveLds S = wave_lds<KPL>();
@@ -529,7 +574,7 @@ __global__ void __launch_bounds__(64 * kSweepWaves) sweep_device_kernel(SweepPar
         ivec<KPL> nd;
         load_kvec<KPL>(ndk_row, lane, nd);
         set_ndk<KPL>(S, al, nd, lane);
-        sample_range<KPL, MODE_ATOMIC>(S, al, p, p.sweep, lane, b, e, nvalid, lastLane, et);
+        sample_range<KPL, MODE_ATOMIC, decltype(et), NARROW>(S, al, p, p.sweep, lane, b, e, nvalid, lastLane, et);
         nd = get_ndk<KPL>(S, al, lane);
         store_kvec<KPL>(ndk_row, lane, nd);
     }
@@ -1083,6 +1128,23 @@ __global__ void build_counts_kernel(const uint32_t* __restrict__ words, const ui
             atomicAdd(nk + pos, 1);
         }
     }
+}
+
+__global__ void pack_narrow_kernel(const int32_t* __restrict__ nkw, uint16_t* __restrict__ nkw16, int64_t off,
+                                   int64_t n, uint32_t* bad) {
+    bool out = false;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t x = nkw[off + i];
+        out |= x < 0 || x >= (int32_t)kNarrowCf;
+        nkw16[i] = (uint16_t)x;
+    }
+    if (out) atomicOr(bad, 1u);
+}
+
+__global__ void unpack_narrow_kernel(int32_t* __restrict__ nkw, const uint16_t* __restrict__ nkw16, int64_t off,
+                                     int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        nkw[off + i] = (int32_t)nkw16[i];
 }
 
 __global__ void copy_i32_kernel(int32_t* dst, const int32_t* src, int64_t n) {
@@ -2394,9 +2456,15 @@ hipError_t launch_build_counts(int kpl, const uint32_t* words, const uint16_t* z
 
 hipError_t launch_sweep_device(int kpl, const SweepParams& p, int grid, int waves, hipStream_t s) {
     if (waves < 1 || waves > kSweepWaves) return hipErrorInvalidValue;
-    LDA_DISPATCH_KPL_ETA(kpl, p.prior_row != nullptr,
-                         hipLaunchKernelGGL((sweep_device_kernel<KPL, ASYM>), dim3(grid), dim3(64 * waves),
-                                            wave_lds_bytes<KPL>(waves), s, p));
+    if (p.nkw16) {
+        LDA_DISPATCH_KPL_ETA(kpl, p.prior_row != nullptr,
+                             hipLaunchKernelGGL((sweep_device_kernel<KPL, ASYM, true>), dim3(grid), dim3(64 * waves),
+                                                wave_lds_bytes<KPL>(waves), s, p));
+    } else {
+        LDA_DISPATCH_KPL_ETA(kpl, p.prior_row != nullptr,
+                             hipLaunchKernelGGL((sweep_device_kernel<KPL, ASYM, false>), dim3(grid), dim3(64 * waves),
+                                                wave_lds_bytes<KPL>(waves), s, p));
+    }
     return hipGetLastError();
 }
 
@@ -2431,8 +2499,8 @@ hipError_t launch_sweep_det(int kpl, const SweepParams& p, hipStream_t s) {
 template <int KPL, bool ASYM>
 static int sweep_device_blocks_kpl(int waves) {
     int n = 0;
-    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sweep_device_kernel<KPL, ASYM>, 64 * waves,
-                                                                      wave_lds_bytes<KPL>(waves));
+    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sweep_device_kernel<KPL, ASYM, false>,
+                                                                      64 * waves, wave_lds_bytes<KPL>(waves));
     return (e == hipSuccess && n > 0) ? n : 1;
 }
 
@@ -2473,6 +2541,22 @@ int infer_blocks_per_cu(int kpl, int segs, bool asym) {
         case 32: return asym ? infer_blocks_kpl<32, true>(segs) : infer_blocks_kpl<32, false>(segs);
         default: return 0;
     }
+}
+
+hipError_t launch_pack_narrow(const int32_t* nkw, uint16_t* nkw16, int hot, int V, int Kp, uint32_t* bad, hipStream_t s) {
+    const int64_t n = (int64_t)(V - hot) * Kp;
+    if (n <= 0) return hipSuccess;
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 16384);
+    hipLaunchKernelGGL(pack_narrow_kernel, dim3((unsigned)blocks), dim3(256), 0, s, nkw, nkw16, (int64_t)hot * Kp, n, bad);
+    return hipGetLastError();
+}
+
+hipError_t launch_unpack_narrow(int32_t* nkw, const uint16_t* nkw16, int hot, int V, int Kp, hipStream_t s) {
+    const int64_t n = (int64_t)(V - hot) * Kp;
+    if (n <= 0) return hipSuccess;
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 16384);
+    hipLaunchKernelGGL(unpack_narrow_kernel, dim3((unsigned)blocks), dim3(256), 0, s, nkw, nkw16, (int64_t)hot * Kp, n);
+    return hipGetLastError();
 }
 
 hipError_t launch_copy_i32(int32_t* dst, const int32_t* src, int64_t n, hipStream_t s) {

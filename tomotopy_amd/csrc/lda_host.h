@@ -164,6 +164,16 @@ struct lda_ctx {
     bool part_key_w = false;
     DevBuf<uint32_t> d_sweep;
 
+    // narrow columns of the device-wide sweep (TermWeight ONE): words
+    // [nkw_hot, V) have fewer than kNarrowCf tokens; during an lda_train /
+    // lda_sweep call their counts live as u16 in d_nkw16 (narrow_active) and
+    // are written back to d_nkw at the end of the call
+    bool narrow_enabled = true;
+    bool narrow_active = false;
+    bool narrow_used = false;  // the last call ran on narrow columns
+    int nkw_hot = 0;
+    DevBuf<uint16_t> d_nkw16;
+
     // lda_debug_fail_at_step: the training iteration at this global step
     // reports LDA_ERR_TRAINING (-1: never); private copies never inherit it
     int64_t fail_at_step = -1;

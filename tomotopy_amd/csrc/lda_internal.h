@@ -51,7 +51,17 @@ struct SweepParams {
     // partition rounds replayed from a graph: the sweep number is read here
     // (NULL: use `sweep`)
     const uint32_t* sweep_dev;
+    // narrow columns of the device-wide sweep (NULL: every column in nkw):
+    // words v >= nkw_hot keep their counts as u16 in nkw16, row v - nkw_hot
+    // (same lane layout); their int32 columns in nkw are stale meanwhile
+    uint16_t* nkw16;
+    int nkw_hot;
 };
+
+// Words with fewer than kNarrowCf tokens have every n_kw cell below 2^15
+// while the counts are consistent, and a sweep can raise a cell by at most
+// the word's token count, so u16 cells cannot overflow.
+constexpr uint64_t kNarrowCf = 1u << 15;
 
 // One fold-in job: a document's tokens [b, e) in the token arena, the
 // iterations (it_begin, it_end] to run in this launch out of `iters`, and the
@@ -193,6 +203,10 @@ hipError_t launch_build_counts(int kpl, const uint32_t* words, const uint16_t* z
 // sharing one private n_k.
 constexpr int kMaxSweepWaves = 8;
 hipError_t launch_sweep_device(int kpl, const SweepParams& p, int grid, int waves, hipStream_t s);
+// n_kw rows [hot, V) between int32 (nkw) and u16 (nkw16, row v - hot); pack
+// sets *bad when a cell is outside [0, kNarrowCf)
+hipError_t launch_pack_narrow(const int32_t* nkw, uint16_t* nkw16, int hot, int V, int Kp, uint32_t* bad, hipStream_t s);
+hipError_t launch_unpack_narrow(int32_t* nkw, const uint16_t* nkw16, int hot, int V, int Kp, hipStream_t s);
 hipError_t launch_sweep_det(int kpl, const SweepParams& p, hipStream_t s);
 // copy_merge: W private copies of n_kw (W x V x Kp) and of n_k (W x Kp)
 hipError_t launch_sweep_copy(int kpl, const SweepParams& p, hipStream_t s);
