@@ -34,40 +34,41 @@ METRIC = "Gibbs tokens/s at K=1000 (HBM %peak); batched infer docs/s + p99 vs pl
 HBM_PEAK_GBS = 8000.0
 
 
-def copy_probe():
-    """Calibration kernel for the byte counters: a 1 GiB device-to-device copy,
-    three times (read 1 GiB, write 1 GiB per launch); then ten more, timed with
-    HIP events, for the bandwidth a plain copy reaches (2 GiB moved per launch)."""
-    import torch
-    n = 1 << 28
-    a = torch.ones(n, dtype=torch.int32, device="cuda")
-    b = torch.empty_like(a)
-    torch.cuda.synchronize()
-    for _ in range(3):
-        b.copy_(a)
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(10):
-        b.copy_(a)
-    e1.record()
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / 10
-    return {"copy_bytes_per_launch": n * 4, "launches": 3, "timed_launches": 10, "ms_per_copy": ms,
-            "copy_GBps": 2 * n * 4 / (ms * 1e-3) / 1e9}
+def bandwidth_probe(nbytes=1 << 30, reps=20):
+    """The device's streaming ceilings from the library's hand-written probe
+    kernels (lda_probe_bandwidth): a streaming read of 1 GiB in 16-byte
+    vectors, and a copy (bytes read + bytes written), best of four grids."""
+    from tomotopy_amd import _native
+    rd, cp = _native.probe_bandwidth(nbytes, reps)
+    return {"bytes": nbytes, "reps": reps, "read_GBps": rd, "copy_GBps": cp,
+            "kernels": ["probe_read_kernel", "probe_copy_kernel"]}
+
+
+def host_threads():
+    """Worker threads for the CPU baselines: the host cores this process may
+    use (OMP_NUM_THREADS where the host sets the share of its cores that one
+    GPU's jobs get, else every core)."""
+    n = os.cpu_count() or 1
+    try:
+        share = int(os.environ.get("OMP_NUM_THREADS", "0"))
+    except ValueError:
+        share = 0
+    return max(1, min(n, share) if share > 0 else n)
 
 
 def pmc_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 byte
-    counter summary (FETCH_SIZE and WRITE_SIZE of the same bench command,
-    each scaled by what the counter reads for a copy of known size), or None."""
+    counter summary (FETCH_SIZE and WRITE_SIZE runs of the bench's training
+    leg, each scaled by what the counter reads for the probe copy of known
+    size), with the commit the counters were taken on; (None, None) if the
+    summary has no such kernel."""
     path = os.path.join(ROOT, "profiles", "traffic_large_k1000.json")
     try:
         with open(path) as f:
             t = json.load(f)
-        return t["kernels"][kernel]["hbm_bytes_per_launch"]
+        return t["kernels"][kernel]["hbm_bytes_per_launch"], t.get("commit")
     except (OSError, KeyError, ValueError):
-        return None
+        return None, None
 
 
 def log(*a):
@@ -101,15 +102,19 @@ def host_cpu_model():
     return "unknown"
 
 
-def cpu_baseline(ws, offs, V, K, alpha, eta, budget_s=12.0, threads=None, max_docs=4000):
-    """Oracle partition sweep on the host's cores over a bounded document
-    sample (threads=1: tomotopy's `none` schedule on one core)."""
+def cpu_baseline(ws, offs, V, K, alpha, eta, budget_s=12.0, threads=None, max_docs=4000, seed=11):
+    """Oracle partition sweep with one worker per host thread (tomoto's
+    default, W = the cores) over a random sample of max_docs documents
+    (threads=1: tomotopy's `none` schedule on one core)."""
     from oracle import oracle as O
-    threads = threads or min(os.cpu_count() or 1, 16)
+    threads = threads or host_threads()
     D = len(offs) - 1
-    n_docs = max(200, min(D, max_docs))
-    sub_off = offs[:n_docs + 1].copy()
-    sub_w = ws[:int(sub_off[-1])]
+    n_docs = max(1, min(D, max_docs))
+    pick = np.sort(np.random.default_rng(seed).choice(D, size=n_docs, replace=False))
+    lens = np.diff(offs.astype(np.int64))
+    sub_off = np.zeros(n_docs + 1, np.uint64)
+    sub_off[1:] = np.cumsum(lens[pick])
+    sub_w = np.concatenate([ws[int(offs[d]):int(offs[d + 1])] for d in pick]).astype(np.uint32)
     m = O.Model(K, V, alpha, eta, seed=1, scan_order=0)
     m.set_optim(0, 0)
     m.set_corpus(sub_w, sub_off)
@@ -128,10 +133,112 @@ def cpu_baseline(ws, offs, V, K, alpha, eta, budget_s=12.0, threads=None, max_do
         "unit": "tokens/s",
         "cores": threads,
         "kind": "port",
-        "sample": "first %d of %d docs (%d tokens), K=%d, V=%d, %s scheme with %d workers, %d sweeps in %.1fs"
-                  % (n_docs, D, ntok, K, V, "partition" if threads > 1 else "none", threads, sweeps, dt),
-        "cpu_model": host_cpu_model(),
+        "sample": "%d random docs of %d (%d tokens, seed %d), K=%d, V=%d, %s scheme with %d workers, %d sweeps in "
+                  "%.1fs" % (n_docs, D, ntok, seed, K, V, "partition" if threads > 1 else "none", threads, sweeps, dt),
+        "cpu_model": host_cpu_model(), "host_cpus": os.cpu_count(),
     }
+
+
+def narrow_hot(ws, V):
+    """Words that keep int32 columns in the narrow-column sweep: every word id
+    up to the last one with 2^15 or more tokens (the library's rule)."""
+    cf = np.bincount(ws[ws < V].astype(np.int64), minlength=V)
+    big = np.nonzero(cf >= 1 << 15)[0]
+    return int(big[-1]) + 1 if len(big) else 0
+
+
+def sweep_bytes(ws, offs, V, K, hot):
+    """Algorithmic bytes of one device-wide sweep launch.  Per token the
+    column read, 4K bytes for a word with an int32 column (ids < hot) and 2K
+    for a word with a narrow u16 column, plus 8 (word id, topic read and
+    written); per document 8K (n_dk staged in and written back).  `runs` is
+    the same with one column read per run of equal words in a document (the
+    sampler keeps the column while the word repeats)."""
+    valid = ws < V
+    v = ws.astype(np.int64)
+    col = np.where(v < hot, 4 * K, 2 * K)
+    D = len(offs) - 1
+    N = int(np.count_nonzero(valid))
+    per_doc = D * 8 * K
+    tokens = int(col[valid].sum()) + 8 * N + per_doc
+    start = np.ones(len(ws), dtype=bool)
+    start[1:] = ws[1:] != ws[:-1]
+    start[offs[:-1].astype(np.int64)[np.diff(offs.astype(np.int64)) > 0]] = True
+    runs_mask = start & valid
+    runs = int(col[runs_mask].sum()) + 8 * N + per_doc
+    dense = N * (4 * K + 8) + per_doc
+    return {"per_token": tokens, "per_run": runs, "dense_int32": dense, "runs": int(runs_mask.sum()), "tokens": N,
+            "hot_words": hot}
+
+
+def partition_chunks(ws, V, W):
+    """Token counts of the W vocabulary chunks of tomoto's partition scheme
+    (boundaries at cumulative collection-frequency quantiles,
+    LDAModel.hpp:436-449, as the library computes them)."""
+    cf = np.bincount(ws[ws < V].astype(np.int64), minlength=V)
+    tot, cum, ch = int(cf.sum()), 0, []
+    for i in range(V):
+        cum += int(cf[i])
+        if cum * W >= tot * (len(ch) + 1):
+            ch.append(i + 1)
+    while len(ch) < W:
+        ch.append(V)
+    ch = ch[:W]
+    ch[-1] = V
+    bounds = [0] + ch
+    return [int(cf[bounds[j]:bounds[j + 1]].sum()) for j in range(W)]
+
+
+def partition_bench(name, ws, offs, V, K, alpha, sweeps, cpu_sweeps=0):
+    """tomoto's partition schedule on the device with its own worker count
+    (train(workers=0, parallel=PARTITION)) beside the device-wide schedule on
+    the same corpus.  One worker owns each vocabulary chunk for the whole
+    sweep, so the chunk holding the most frequent words is sampled by one
+    wavefront in sequence whatever W is: that chain bounds the sweep."""
+    from tomotopy_amd import _native
+    eta = 0.01
+    N = int(np.count_nonzero(ws < V))
+    D = len(offs) - 1
+    ctx = _native.Context(K, V, alpha, eta, seed=1234)
+    ctx.upload_corpus(ws, offs)
+    ctx.set_optim(0, 0)
+    _, W = ctx.resolve_schedule(0, 3)
+    ctx.train(1, workers=0, scheme=3)  # graph capture
+    t0 = time.perf_counter()
+    ctx.train(sweeps, workers=0, scheme=3)
+    dt = (time.perf_counter() - t0) / sweeps
+    kms = ctx.kernel_stats()["ms"] / sweeps
+    chunks = partition_chunks(ws, V, W)
+    dw_sched = ctx.resolve_schedule(0, 0)
+    ctx.train(1, workers=0, scheme=0)
+    t0 = time.perf_counter()
+    ctx.train(sweeps, workers=0, scheme=0)
+    dt_dw = (time.perf_counter() - t0) / sweeps
+    dense = N * (4 * K + 8) + D * 8 * K
+    out = {"workload": name, "K": K, "V": V, "docs": D, "tokens": N, "workers": W, "rounds_per_sweep": W,
+           "value": N / dt, "unit": "tokens/s", "ms_per_sweep": dt * 1e3, "kernel_ms_per_sweep": kms,
+           "frac_dense_formula": dense / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+           "largest_chunk_tokens": max(chunks), "mean_chunk_tokens": N / W,
+           "us_per_token_on_largest_chunk": kms * 1e3 / max(chunks),
+           "device_wide": {"value": N / dt_dw, "ms_per_sweep": dt_dw * 1e3, "workgroups": dw_sched[1]}}
+    ctx.close()
+    if cpu_sweeps:
+        from oracle import oracle as O
+        thr = host_threads()
+        m = O.Model(K, V, alpha, eta, seed=1, scan_order=0)
+        m.set_optim(0, 0)
+        m.set_corpus(ws, offs)
+        m.train(1, workers=thr, threads=thr)
+        t0 = time.perf_counter()
+        m.train(cpu_sweeps, workers=thr, threads=thr)
+        dtc = (time.perf_counter() - t0) / cpu_sweeps
+        out["cpu_baseline"] = {"value": N / dtc, "unit": "tokens/s", "cores": thr, "kind": "port",
+                               "sample": "the whole corpus, partition scheme with %d workers, %d sweeps" % (thr, cpu_sweeps),
+                               "cpu_model": host_cpu_model()}
+    log("partition %s: W=%d, %.3g tokens/s (%.1f ms/sweep; largest chunk %d tokens = %.2f us/token), device-wide "
+        "%.3g tokens/s" % (name, W, out["value"], dt * 1e3, max(chunks), out["us_per_token_on_largest_chunk"],
+                            out["device_wide"]["value"]))
+    return out
 
 
 def pmi_bench(ws, offs, V, K, alpha, eta, steps, warmup):
@@ -162,7 +269,7 @@ def pmi_bench(ws, offs, V, K, alpha, eta, steps, warmup):
     out = {"value": n_valid * steps / dt, "unit": "tokens/s", "ms_per_step": dt * 1e3 / steps, "steps": steps,
            "kernel": "sweep_device_w_kernel<16, false>", "kernel_ms": kern_ms, "achieved_GBs": achieved,
            "frac": achieved / HBM_PEAK_GBS, "algorithmic_bytes_per_launch": bytes_per_launch,
-           "traffic": pmc_traffic("sweep_device_w_kernel<16, false>"),
+           "traffic": pmc_traffic("sweep_device_w_kernel<16, false>")[0],
            "ll_per_word": (lld + llr) / wn, "weighted_n": wn}
     log("PMI: %.3g tokens/s, kernel %.3f ms (%.1f%% of HBM peak), %.3f ms/step, ll_per_word %.5f"
         % (out["value"], kern_ms, 100 * out["frac"], out["ms_per_step"], out["ll_per_word"]))
@@ -180,7 +287,7 @@ def cpu_infer_baseline(ctx, ws, offs, V, K, alpha, eta, uws, uoffs, iterations, 
     model."""
     from concurrent.futures import ThreadPoolExecutor
     from oracle import oracle as O
-    threads = min(os.cpu_count() or 1, 16)
+    threads = host_threads()
     z, _, _, _ = ctx.download_state(ndk=False, nkw=False, nk=False)
     m = O.Model(K, V, ctx.get_alpha(), eta, seed=1, scan_order=0)
     m.set_corpus(ws, offs, z)
@@ -345,7 +452,7 @@ def infer_pmi_bench(iterations=100, plain_sample=200, cpu_budget_s=8.0):
            "ll_per_weight": llw, "ll_per_weight_sequential": llw_seq,
            "ll_per_weight_rel_diff": abs(llw - llw_seq) / abs(llw_seq)}
     # the oracle's fold-in on the host's cores, on the device's trained state
-    threads = min(os.cpu_count() or 1, 16)
+    threads = host_threads()
     z, ndk, nkw, nk = ctx.download_state_f()
     m = O.WeightedModel(K, V, ctx.get_alpha(), 0.01, seed=1, scan_order=0)
     m.set_corpus(ws, offs, wts, z)
@@ -384,16 +491,21 @@ def _percentile(x, q):
 
 
 def serve_bench(iterations=50, qps_list=(1, 2, 5, 10, 20, 50, 100, 200, 400, 800, 1200, 1600), seconds=4.0,
-                plain_requests=60, seed=2024, seg_tokens=128, step_tokens=12000, lanes=1):
+                plain_requests=60, seed=2024, seg_tokens=None, step_tokens=12000, lanes=1, lanes_sweep=(2, 4)):
     """Serving loop on a device-resident K=500, V=100,000 model: Poisson request
     arrivals at each QPS (requests of 1..64 docs, lengths LogNormal(ln 150, 1)
     clipped to [5, 10000], 50 iterations); latency is submit to results on the
-    host, measured inside the server.  Baselines on the same requests, served
-    one at a time back to back: plain per-request GPU infer (one lda_infer
-    call per request) and plain per-document GPU infer (one launch and one
-    synchronisation per document)."""
+    host, measured inside the server.  The segmentation is the autotuner's
+    choice on the pooled requests' documents among the candidates whose ll
+    per word stays within 0.1% of exact sampling; the served ll per word is
+    set against exact sampling (no segmentation) of the same requests.
+    Baselines on the same requests, served one at a time back to back: plain
+    per-request GPU infer (one lda_infer call per request, with the served
+    segmentation and exact), and plain per-document GPU infer (one launch and
+    one synchronisation per document).  The top rate is repeated with 2 and 4
+    server lanes."""
     import threading
-    from tomotopy_amd import _native, corpus, synth
+    from tomotopy_amd import _native, autotune, corpus, synth
     t = time.time()
     n_pool = 600
     (tw, toff, nw), reqs_raw = synth.serve_workload(n_pool)
@@ -405,57 +517,77 @@ def serve_bench(iterations=50, qps_list=(1, 2, 5, 10, 20, 50, 100, 200, 400, 800
     ctx = _native.Context(K, V, 0.1, 0.01, seed=55)
     ctx.upload_corpus(ws, offs)
     ctx.train(50, workers=0)
-    seg_max = ctx.infer_options()["seg_supported"]
+    allw = np.concatenate([w for w, _ in reqs]).astype(np.uint32)
+    alloff = np.zeros(sum(len(o) - 1 for _, o in reqs) + 1, np.uint64)
+    alloff[1:] = np.cumsum(np.concatenate([np.diff(o.astype(np.int64)) for _, o in reqs]))
+    tune = None
+    if seg_tokens is None:
+        tune = autotune.tune_infer(ctx, allw, alloff, iterations, infer_seed=seed)
+        seg_tokens, seg_max = tune["seg_tokens"], tune["seg_max"]
+        log("serve autotuner: seg_tokens %d seg_max %d (%s)" % (seg_tokens, seg_max, ", ".join(
+            "%d:%.0fms/%.2e" % (c["seg_tokens"], c["kernel_ms"], c["ll_rel_diff"]) for c in tune["candidates"])))
+    else:
+        seg_max = ctx.infer_options()["seg_supported"] if seg_tokens else 1
     ctx.set_infer_options(seg_tokens, seg_max)
     out = {"workload": "serve_k500", "K": K, "V": V, "iterations": iterations, "levels": [],
-           "seg_tokens": seg_tokens, "seg_max": seg_max, "step_tokens": step_tokens, "lanes": lanes}
+           "seg_tokens": seg_tokens, "seg_max": seg_max, "step_tokens": step_tokens, "lanes": lanes,
+           "autotune": tune}
     srv = _native.Server(ctx, hbm_budget_bytes=8 << 30, iterations=iterations, step_iters=iterations,
                          step_tokens=step_tokens, infer_seed=seed, lanes=lanes)
+
+    def level(srv, qps, rng):
+        n = max(8, min(len(reqs), int(qps * seconds)))  # at least 8 requests: 8 s at 1 request/s
+        gaps = rng.exponential(1.0 / qps, size=n)
+        st0 = srv.stats()
+        pick = rng.integers(0, len(reqs), size=n)
+        ids = [None] * n
+        t_start = time.perf_counter()
+
+        def client():
+            due = t_start
+            for j in range(n):
+                due += gaps[j]
+                dt = due - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+                ids[j] = srv.submit(*reqs[pick[j]])
+
+        th = threading.Thread(target=client)
+        th.start()
+        lat = []
+        docs = 0
+        lls, toks = 0.0, 0
+        for j in range(n):
+            while ids[j] is None:
+                time.sleep(0.0005)
+            _, ll, ms = srv.wait(ids[j])
+            lat.append(ms)
+            docs += len(ll)
+            lls += float(ll.sum())
+            toks += int(np.count_nonzero(reqs[pick[j]][0] < V))
+        t_end = time.perf_counter()
+        th.join()
+        st1 = srv.stats()
+        nst = max(1, st1["steps"] - st0["steps"])
+        lvl = {"qps": qps, "requests": n, "docs": docs, "docs_per_s": docs / (t_end - t_start),
+               "p50_ms": _percentile(lat, 50), "p99_ms": _percentile(lat, 99), "max_ms": max(lat),
+               "steps": st1["steps"] - st0["steps"], "step_gpu_ms": (st1["gpu_ms"] - st0["gpu_ms"]) / nst,
+               "step_host_ms": (st1["host_ms"] - st0["host_ms"]) / nst}
+        log("serve qps %4d: %5d req, %7.0f docs/s, p50 %.1f ms, p99 %.1f ms; %d steps, %.2f ms device + %.2f ms host each"
+            % (qps, n, lvl["docs_per_s"], lvl["p50_ms"], lvl["p99_ms"], lvl["steps"], lvl["step_gpu_ms"],
+               lvl["step_host_ms"]))
+        return lvl, lls, toks
+
     try:
         for i in range(4):  # warm
             srv.wait(srv.submit(*reqs[i]))
         rng = np.random.default_rng(seed)
         ll_sum, tok_sum = 0.0, 0
         for qps in qps_list:
-            n = max(8, min(len(reqs), int(qps * seconds)))  # at least 8 requests: 8 s at 1 request/s
-            gaps = rng.exponential(1.0 / qps, size=n)
-            st0 = srv.stats()
-            pick = rng.integers(0, len(reqs), size=n)
-            ids = [None] * n
-            t_start = time.perf_counter()
-
-            def client():
-                due = t_start
-                for j in range(n):
-                    due += gaps[j]
-                    dt = due - time.perf_counter()
-                    if dt > 0:
-                        time.sleep(dt)
-                    ids[j] = srv.submit(*reqs[pick[j]])
-
-            th = threading.Thread(target=client)
-            th.start()
-            lat = []
-            docs = 0
-            for j in range(n):
-                while ids[j] is None:
-                    time.sleep(0.0005)
-                _, ll, ms = srv.wait(ids[j])
-                lat.append(ms)
-                docs += len(ll)
-                ll_sum += float(ll.sum())
-                tok_sum += int(np.count_nonzero(reqs[pick[j]][0] < V))
-            t_end = time.perf_counter()
-            th.join()
-            st1 = srv.stats()
-            lvl = {"qps": qps, "requests": n, "docs": docs, "docs_per_s": docs / (t_end - t_start),
-                   "p50_ms": _percentile(lat, 50), "p99_ms": _percentile(lat, 99), "max_ms": max(lat),
-                   "steps": st1["steps"] - st0["steps"], "step_gpu_ms": (st1["gpu_ms"] - st0["gpu_ms"]) / max(1, st1["steps"] - st0["steps"]),
-                   "step_host_ms": (st1["host_ms"] - st0["host_ms"]) / max(1, st1["steps"] - st0["steps"])}
+            lvl, a, b = level(srv, qps, rng)
             out["levels"].append(lvl)
-            log("serve qps %4d: %5d req, %7.0f docs/s, p50 %.1f ms, p99 %.1f ms; %d steps, %.2f ms device + %.2f ms host each"
-                % (qps, n, lvl["docs_per_s"], lvl["p50_ms"], lvl["p99_ms"], lvl["steps"], lvl["step_gpu_ms"],
-                   lvl["step_host_ms"]))
+            ll_sum += a
+            tok_sum += b
         # saturation: every pooled request submitted at once
         t0 = time.perf_counter()
         ids = [srv.submit(*r) for r in reqs]
@@ -472,6 +604,32 @@ def serve_bench(iterations=50, qps_list=(1, 2, 5, 10, 20, 50, 100, 200, 400, 800
         out["ll_per_word"] = ll_sum / max(tok_sum, 1)
     finally:
         srv.stop()
+    out["lanes_sweep"] = []
+    for ln in lanes_sweep:
+        srv2 = _native.Server(ctx, hbm_budget_bytes=8 << 30, iterations=iterations, step_iters=iterations,
+                              step_tokens=step_tokens, infer_seed=seed, lanes=ln)
+        try:
+            for i in range(4):
+                srv2.wait(srv2.submit(*reqs[i]))
+            lvl, _, _ = level(srv2, qps_list[-1], np.random.default_rng(seed + ln))
+        finally:
+            srv2.stop()
+        lvl["lanes"] = ln
+        out["lanes_sweep"].append(lvl)
+    # served ll against exact sampling (no segmentation) of the same requests
+    served_tok = int(np.count_nonzero(allw < V))
+    out["served_ll_per_word"] = float(sum(float(np.sum(x)) for x in served_ll)) / max(served_tok, 1)
+    if seg_tokens:
+        ctx.set_infer_options(0, 1)
+        _, ll_exact, _ = ctx.infer(allw, alloff, iterations, infer_seed=seed, mode=0)
+        ctx.set_infer_options(seg_tokens, seg_max)
+        out["exact_ll_per_word"] = float(ll_exact.sum()) / max(served_tok, 1)
+    else:
+        out["exact_ll_per_word"] = out["served_ll_per_word"]
+    out["served_vs_exact_ll_rel_diff"] = abs(out["served_ll_per_word"] - out["exact_ll_per_word"]) / abs(
+        out["exact_ll_per_word"])
+    log("served ll per word %.5f vs exact sampling %.5f (rel %.2e)" % (
+        out["served_ll_per_word"], out["exact_ll_per_word"], out["served_vs_exact_ll_rel_diff"]))
     # baselines: the same requests served one at a time
     sub = reqs[:plain_requests]
     lat_req = []
@@ -489,8 +647,21 @@ def serve_bench(iterations=50, qps_list=(1, 2, 5, 10, 20, 50, 100, 200, 400, 800
     ndocs = sum(len(o) - 1 for _, o in sub)
     out["plain_per_request"] = {"requests": len(sub), "docs": ndocs, "docs_per_s": ndocs / dt,
                                 "p50_ms": _percentile(lat_req, 50), "p99_ms": _percentile(lat_req, 99),
-                                "ll_per_word": ll_sum / max(tok_sum, 1),
+                                "ll_per_word": ll_sum / max(tok_sum, 1), "seg_tokens": seg_tokens,
                                 "served_ll_identical": same_ll}
+    if seg_tokens:
+        ctx.set_infer_options(0, 1)
+        lat_x = []
+        t0 = time.perf_counter()
+        for w, o in sub:
+            t1 = time.perf_counter()
+            ctx.infer(w, o, iterations, infer_seed=seed, mode=0)
+            lat_x.append((time.perf_counter() - t1) * 1e3)
+        dtx = time.perf_counter() - t0
+        ctx.set_infer_options(seg_tokens, seg_max)
+        out["plain_per_request_exact"] = {"requests": len(sub), "docs": ndocs, "docs_per_s": ndocs / dtx,
+                                          "p50_ms": _percentile(lat_x, 50), "p99_ms": _percentile(lat_x, 99),
+                                          "seg_tokens": 0}
     sub2 = reqs[:max(1, plain_requests // 10)]
     t0 = time.perf_counter()
     for w, o in sub2:
@@ -498,9 +669,6 @@ def serve_bench(iterations=50, qps_list=(1, 2, 5, 10, 20, 50, 100, 200, 400, 800
     dt = time.perf_counter() - t0
     nd2 = sum(len(o) - 1 for _, o in sub2)
     out["plain_per_document"] = {"requests": len(sub2), "docs": nd2, "docs_per_s": nd2 / dt}
-    allw = np.concatenate([w for w, _ in reqs]).astype(np.uint32)
-    alloff = np.zeros(sum(len(o) - 1 for _, o in reqs) + 1, np.uint64)
-    alloff[1:] = np.cumsum(np.concatenate([np.diff(o.astype(np.int64)) for _, o in reqs]))
     out["cpu_baseline"] = cpu_infer_baseline(ctx, ws, offs, V, K, 0.1, 0.01, allw, alloff, iterations)
     log("serving CPU baseline: %.0f docs/s (%s)" % (out["cpu_baseline"]["value"], out["cpu_baseline"]["sample"]))
     best = max(lv["docs_per_s"] for lv in out["levels"])
@@ -530,15 +698,16 @@ def main():
     ap.add_argument("--no-infer", action="store_true", help="skip the fold-in measurement")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
     ap.add_argument("--no-pmi", action="store_true", help="skip the TermWeight.PMI training variant")
-    ap.add_argument("--only", choices=["train", "infer", "infer_pmi", "serve", "copy"], default=None,
+    ap.add_argument("--no-partition", action="store_true", help="skip the partition-schedule legs")
+    ap.add_argument("--only", choices=["train", "infer", "infer_pmi", "serve", "probe"], default=None,
                     help="run one leg only (the JSON line then carries that leg's numbers)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
     rank = int(os.environ.get("RANK", "0"))
-    if args.only == "copy":
-        print(json.dumps({"leg": "copy", "copy": copy_probe()}), flush=True)
+    if args.only == "probe":
+        print(json.dumps({"leg": "probe", "probe": bandwidth_probe()}), flush=True)
         return
     if args.only in ("infer", "infer_pmi", "serve"):
         res = {"infer": infer_bench, "infer_pmi": infer_pmi_bench, "serve": serve_bench}[args.only]()
@@ -567,11 +736,16 @@ def main():
         sys.exit(3)
     lld, llr = ctx.loglik()
     kern_ms = st["ms"] / max(1, st["launches"])
-    bytes_per_launch = n_valid * (4 * K + 8) + D * 8 * K
+    narrow = bool(st["narrow"])
+    kname = "sweep_device_kernel<16, false, %s>" % ("true" if narrow else "false")
+    sb = sweep_bytes(ws, offs, V, K, narrow_hot(ws, V) if narrow else V)
+    bytes_per_launch = sb["per_token"]
     achieved = bytes_per_launch / (kern_ms * 1e-3) / 1e9
     value = n_valid * args.steps / dt
-    log("ll_per_word %.6f; kernel avg %.3f ms; wall %.3f ms/step" % ((lld + llr) / n_valid, kern_ms,
-                                                                     dt * 1e3 / args.steps))
+    traffic, traffic_commit = pmc_traffic(kname)
+    probe = bandwidth_probe()
+    log("ll_per_word %.6f; kernel avg %.3f ms; wall %.3f ms/step; probe read %.0f GB/s, copy %.0f GB/s"
+        % ((lld + llr) / n_valid, kern_ms, dt * 1e3 / args.steps, probe["read_GBps"], probe["copy_GBps"]))
     out = {
         "metric": METRIC,
         "value": value,
@@ -586,16 +760,33 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "large_k1000", "docs": D, "tokens": n_valid, "V": V, "K": K, "alpha": alpha,
-                   "eta": eta, "scheme": "device-wide (workers=0)", "optim_interval": 10},
+                   "eta": eta, "scheme": "device-wide (workers=0)", "optim_interval": 10,
+                   "narrow_columns": narrow, "int32_column_words": sb["hot_words"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": pmc_traffic("sweep_device_kernel<16, false>"),
-                     "kernel": "sweep_device_kernel<16, false>", "kernel_ms": kern_ms,
-                     "algorithmic_bytes_per_launch": bytes_per_launch},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_commit": traffic_commit,
+                     "kernel": kname, "kernel_ms": kern_ms, "algorithmic_bytes_per_launch": bytes_per_launch,
+                     "bytes_model": "per token 4K (int32 column) or 2K (u16 column) + 8, per document 8K",
+                     "runs": sb["runs"], "algorithmic_bytes_per_launch_runs": sb["per_run"],
+                     "frac_runs": sb["per_run"] / (kern_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "dense_int32_bytes_per_launch": sb["dense_int32"],
+                     "dense_int32_GBps_equivalent": sb["dense_int32"] / (kern_ms * 1e-3) / 1e9,
+                     "read_ceiling_GBps": probe["read_GBps"], "copy_ceiling_GBps": probe["copy_GBps"]},
         "ll_per_word": (lld + llr) / n_valid,
     }
+    if traffic:
+        out["roofline"]["traffic_GBps"] = traffic / (kern_ms * 1e-3) / 1e9
+        out["roofline"]["traffic_frac_of_read_ceiling"] = out["roofline"]["traffic_GBps"] / probe["read_GBps"]
     if not args.no_pmi:
-        del ctx
+        ctx = None
         out["pmi_variant"] = pmi_bench(ws, offs, V, K, alpha, eta, args.steps, args.warmup)
+    if not args.no_partition and args.only in (None, "train"):
+        ctx = None
+        out["partition_c3"] = partition_bench("large_k1000", ws, offs, V, K, alpha, sweeps=2)
+        from tomotopy_amd import corpus as _corpus, synth as _synth
+        rw, ro, rnw, _ = _synth.workload("readme_k100")
+        rws, roffs, rV, _ = _corpus.prepare_arrays(rw, ro, rnw)
+        out["partition_c2"] = partition_bench("readme_k100", rws, roffs, rV, 100, 0.1, sweeps=5,
+                                              cpu_sweeps=0 if args.no_cpu_baseline else 5)
     if rank == 0 and not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(ws, offs, V, K, alpha, eta)
         out["cpu_baseline_w1"] = cpu_baseline(ws, offs, V, K, alpha, eta, budget_s=4.0, threads=1, max_docs=1000)

@@ -209,6 +209,13 @@ int lda_get_weighted_n(lda_ctx* ctx, double* weighted_n);
  * [2^exp_shift, 2^(exp_shift+1)), where it differs from IEEE division. */
 int lda_selftest_reciprocal(int exp_shift, int device, uint64_t* mismatches);
 
+/* Streaming bandwidth of `device` (no counterpart in the reference; the
+ * ceiling the sweep's bytes are set against): out[0] GB/s of a hand-written
+ * streaming read of `bytes` (>= 1 MiB, multiple of 16) in 16-byte vectors,
+ * out[1] GB/s of a copy (bytes read + bytes written), each over `reps`
+ * launches after two warm-up launches, the best of four grid sizes.  Allocates 2 x bytes for the call. */
+int lda_probe_bandwidth(int device, uint64_t bytes, int reps, double out[2]);
+
 /* One sampling step per query token on the frozen current state: remove the
  * token's own z_old from (n_dk[doc], n_kw[:,word], n_k), weight, scan, and
  * search with the given uniform.  Counts are not modified. */

@@ -45,6 +45,7 @@ _SIGS = {
     "lda_set_counts_f": [_c_p, _c_p, _c_p],
     "lda_set_word_priors": [_c_p, ctypes.c_int64, _c_p, _c_p],
     "lda_selftest_reciprocal": [_i32, _i32, _c_p],
+    "lda_probe_bandwidth": [_i32, _u64, _i32, _c_p],
     "lda_infer_together": [_c_p, _c_p, _c_p, ctypes.c_int64, _i32, _u64, _i32, _i32, _c_p, _c_p, _c_p],
     "lda_infer_together_w": [_c_p, _c_p, _c_p, _i64, _c_p, _i32, _u64, _i32, _i32, _c_p, _c_p, _c_p],
     "lda_set_alpha": [_c_p, _c_p],
@@ -447,3 +448,10 @@ def selftest_reciprocal(exp_shift, device=0):
     n = ctypes.c_uint64(0)
     _check(load().lda_selftest_reciprocal(int(exp_shift), int(device), ctypes.byref(n)))
     return int(n.value)
+
+
+def probe_bandwidth(bytes_=1 << 30, reps=20, device=0):
+    """(streaming read GB/s, copy GB/s) of the device (lda_probe_bandwidth)."""
+    out = np.zeros(2, dtype=np.float64)
+    _check(load().lda_probe_bandwidth(int(device), int(bytes_), int(reps), _ptr(out)))
+    return float(out[0]), float(out[1])

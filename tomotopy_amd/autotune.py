@@ -7,8 +7,10 @@ kernel).  seg_tokens trades fidelity for speed: documents of at most
 seg_tokens tokens are sampled sequentially, exactly as tomotopy does, while
 longer ones are split into segments that see each other's n_dk changes one
 iteration late.  The tuner times the batched launch for each candidate on a
-sample of the workload and keeps the largest seg_tokens (the most documents
-sampled exactly) whose time is within `slack` of the fastest candidate.
+sample of the workload, keeps the candidates whose log-likelihood per word
+of the sample is within `ll_tol` (relative) of the exact sampler's
+(seg_tokens = 0), and among those the largest seg_tokens (the most documents
+sampled exactly) whose time is within `slack` of the fastest of them.
 """
 import numpy as np
 
@@ -37,13 +39,15 @@ def _sample(words, doc_off, max_docs, seed, weights=None):
 
 
 def tune_infer(ctx, words, doc_off, iterations, infer_seed=0, candidates=CANDIDATES, slack=0.10, max_docs=4000,
-               repeats=2, seed=0, weights=None):
+               repeats=2, seed=0, weights=None, ll_tol=1e-3):
     """Time the batched fold-in of a sample of (words, doc_off) for each
     candidate seg_tokens (with the largest segment count the kernel fits at
     this K) and set the chosen one on the context.  weights: the tokens'
-    weights of a term-weighted model.  Returns a dict with the choice and the
-    per-candidate kernel times (ms) and log-likelihoods per word (per unit of
-    weight) of the sample."""
+    weights of a term-weighted model.  Candidates whose sample ll per word
+    differs from seg_tokens = 0's by more than ll_tol (relative) are not
+    chosen (0 must be among the candidates for the guard to apply).  Returns
+    a dict with the choice and the per-candidate kernel times (ms) and
+    log-likelihoods per word (per unit of weight) of the sample."""
     words = np.ascontiguousarray(words, np.uint32)
     doc_off = np.ascontiguousarray(doc_off, np.uint64)
     sw, so, st = _sample(words, doc_off, max_docs, seed, weights)
@@ -64,10 +68,14 @@ def tune_infer(ctx, words, doc_off, iterations, infer_seed=0, candidates=CANDIDA
             best = ms if best is None else min(best, ms)
             ll = float(ll_d.sum()) / n_valid
         results.append({"seg_tokens": int(T), "seg_max": seg_max if T else 1, "kernel_ms": best, "ll_per_word": ll})
-    fastest = min(r["kernel_ms"] for r in results)
-    ok = [r for r in results if r["kernel_ms"] <= (1.0 + slack) * fastest]
+    exact = [r["ll_per_word"] for r in results if r["seg_tokens"] == 0]
+    for r in results:
+        r["ll_rel_diff"] = abs(r["ll_per_word"] - exact[0]) / abs(exact[0]) if exact else 0.0
+    faithful = [r for r in results if r["ll_rel_diff"] <= ll_tol] or results
+    fastest = min(r["kernel_ms"] for r in faithful)
+    ok = [r for r in faithful if r["kernel_ms"] <= (1.0 + slack) * fastest]
     # largest seg_tokens among the acceptable ones; 0 (never segment) counts as the largest
     choice = max(ok, key=lambda r: (r["seg_tokens"] == 0, r["seg_tokens"]))
     ctx.set_infer_options(choice["seg_tokens"], choice["seg_max"])
-    return {"seg_tokens": choice["seg_tokens"], "seg_max": choice["seg_max"], "slack": slack,
+    return {"seg_tokens": choice["seg_tokens"], "seg_max": choice["seg_max"], "slack": slack, "ll_tol": ll_tol,
             "sample_docs": int(len(so) - 1), "candidates": results}

@@ -1481,6 +1481,61 @@ int lda_selftest_reciprocal(int exp_shift, int device, uint64_t* mismatches) {
     return LDA_OK;
 }
 
+int lda_probe_bandwidth(int device, uint64_t bytes, int reps, double out[2]) {
+    if (!out || reps <= 0 || bytes < (1u << 20) || bytes % 16) return fail(LDA_ERR_INVALID_ARGUMENT, "invalid probe size");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(LDA_ERR_DEVICE, "no HIP device is available");
+    if (device < 0 || device >= ndev) return fail(LDA_ERR_INVALID_ARGUMENT, "device %d out of range", device);
+    HIP_TRY(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device));
+    DevBuf<int32_t> a, b, sink;
+    auto done = [&](int rc) {
+        a.release(); b.release(); sink.release();
+        return rc;
+    };
+    if (a.alloc(bytes / 4) || b.alloc(bytes / 4) || sink.alloc(1)) return done(fail(LDA_ERR_OUT_OF_MEMORY, "probe buffers"));
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess)
+        return done(fail(LDA_ERR_DEVICE, "probe stream / events"));
+    auto run1 = [&](bool copy, int grid, double* gbps) -> hipError_t {
+        hipError_t e = lda::launch_fill_u32(reinterpret_cast<uint32_t*>(a.p), 1u, (int64_t)(bytes / 4), s);
+        for (int w = 0; w < 2 && e == hipSuccess; ++w)
+            e = copy ? lda::launch_probe_copy(a.p, b.p, (int64_t)bytes, grid, s)
+                     : lda::launch_probe_read(a.p, (int64_t)bytes, sink.p, grid, s);
+        if (e == hipSuccess) e = hipEventRecord(e0, s);
+        for (int r = 0; r < reps && e == hipSuccess; ++r)
+            e = copy ? lda::launch_probe_copy(a.p, b.p, (int64_t)bytes, grid, s)
+                     : lda::launch_probe_read(a.p, (int64_t)bytes, sink.p, grid, s);
+        if (e == hipSuccess) e = hipEventRecord(e1, s);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        *gbps = (copy ? 2.0 : 1.0) * (double)bytes * reps / (ms * 1e-3) / 1e9;
+        return e;
+    };
+    // the best of four grid sizes (4 to 32 workgroups of 256 threads per CU)
+    auto run = [&](bool copy, double* gbps) -> hipError_t {
+        *gbps = 0.0;
+        for (int m = 4; m <= 32; m *= 2) {
+            double g = 0.0;
+            const hipError_t e = run1(copy, prop.multiProcessorCount * m, &g);
+            if (e != hipSuccess) return e;
+            *gbps = std::max(*gbps, g);
+        }
+        return hipSuccess;
+    };
+    hipError_t e = run(false, &out[0]);
+    if (e == hipSuccess) e = run(true, &out[1]);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(s);
+    if (e != hipSuccess) return done(fail(LDA_ERR_DEVICE, "probe: %s", hipGetErrorString(e)));
+    return done(LDA_OK);
+}
+
 int lda_get_weighted_n(lda_ctx* c, double* weighted_n) {
     if (!c || !weighted_n) return fail(LDA_ERR_INVALID_ARGUMENT, "NULL argument");
     *weighted_n = c->tw == LDA_TW_ONE ? (double)c->n_valid : c->weighted_n;

@@ -1130,6 +1130,25 @@ __global__ void build_counts_kernel(const uint32_t* __restrict__ words, const ui
     }
 }
 
+// Streaming-bandwidth probes: every thread reads (and for the copy writes)
+// 16-byte vectors in a grid-stride loop.  The read probe folds what it reads
+// into a value stored only when it equals a key the data cannot produce (the
+// buffer holds small integers), so the loads stay and the kernel writes
+// nothing.
+__global__ void __launch_bounds__(256) probe_read_kernel(const int4* __restrict__ src, int64_t n, int* __restrict__ sink) {
+    int acc = 0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int4 x = src[i];
+        acc ^= x.x ^ x.y ^ x.z ^ x.w;
+    }
+    if (acc == 0x7EADBEEF) sink[0] = acc;
+}
+
+__global__ void __launch_bounds__(256) probe_copy_kernel(const int4* __restrict__ src, int4* __restrict__ dst, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 __global__ void pack_narrow_kernel(const int32_t* __restrict__ nkw, uint16_t* __restrict__ nkw16, int64_t off,
                                    int64_t n, uint32_t* bad) {
     bool out = false;
@@ -2541,6 +2560,17 @@ int infer_blocks_per_cu(int kpl, int segs, bool asym) {
         case 32: return asym ? infer_blocks_kpl<32, true>(segs) : infer_blocks_kpl<32, false>(segs);
         default: return 0;
     }
+}
+
+hipError_t launch_probe_read(const void* src, int64_t bytes, int* sink, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(probe_read_kernel, dim3(grid), dim3(256), 0, s, reinterpret_cast<const int4*>(src), bytes / 16, sink);
+    return hipGetLastError();
+}
+
+hipError_t launch_probe_copy(const void* src, void* dst, int64_t bytes, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(probe_copy_kernel, dim3(grid), dim3(256), 0, s, reinterpret_cast<const int4*>(src),
+                       reinterpret_cast<int4*>(dst), bytes / 16);
+    return hipGetLastError();
 }
 
 hipError_t launch_pack_narrow(const int32_t* nkw, uint16_t* nkw16, int hot, int V, int Kp, uint32_t* bad, hipStream_t s) {

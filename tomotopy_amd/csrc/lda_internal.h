@@ -206,6 +206,9 @@ hipError_t launch_sweep_device(int kpl, const SweepParams& p, int grid, int wave
 // n_kw rows [hot, V) between int32 (nkw) and u16 (nkw16, row v - hot); pack
 // sets *bad when a cell is outside [0, kNarrowCf)
 hipError_t launch_pack_narrow(const int32_t* nkw, uint16_t* nkw16, int hot, int V, int Kp, uint32_t* bad, hipStream_t s);
+// Bandwidth probes: a streaming read of `bytes` (multiple of 16) and a copy.
+hipError_t launch_probe_read(const void* src, int64_t bytes, int* sink, int grid, hipStream_t s);
+hipError_t launch_probe_copy(const void* src, void* dst, int64_t bytes, int grid, hipStream_t s);
 hipError_t launch_unpack_narrow(int32_t* nkw, const uint16_t* nkw16, int hot, int V, int Kp, hipStream_t s);
 hipError_t launch_sweep_det(int kpl, const SweepParams& p, hipStream_t s);
 // copy_merge: W private copies of n_kw (W x V x Kp) and of n_k (W x Kp)
