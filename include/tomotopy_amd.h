@@ -209,6 +209,24 @@ int lda_get_weighted_n(lda_ctx* ctx, double* weighted_n);
  * [2^exp_shift, 2^(exp_shift+1)), where it differs from IEEE division. */
 int lda_selftest_reciprocal(int exp_shift, int device, uint64_t* mismatches);
 
+/* Read-outs on the device (the whole n_kw / n_dk / z stay in HBM):
+ *   lda_topic_word_dist  topic's (n_kw + eta) / (n_k + V * eta) for every word
+ *                        (out: V floats; normalize = 0: n_kw + eta), in
+ *                        tomoto's float operations (_getWidsByTopic,
+ *                        LDAModel.hpp:828-840; TopicModel::getWidsByTopic,
+ *                        TopicModel.hpp:777-780).
+ *   lda_count_by_topic   tokens per topic over the words < V (out: K; tomoto's
+ *                        _getTopicsCount, LDAModel.hpp:815-826).
+ *   lda_doc_topic_dist   training document doc's (n_dk + alpha) / (weight sum
+ *                        + sum alpha) (out: K; normalize = 0: n_dk + alpha;
+ *                        _getTopicsByDoc, LDAModel.hpp:1145-1159); the weight
+ *                        sum is the document's valid token count, or for IDF /
+ *                        PMI models the float sum of its token weights in
+ *                        token order; sum alpha is a float sum in topic order. */
+int lda_topic_word_dist(lda_ctx* ctx, int topic, int normalize, float* out);
+int lda_count_by_topic(lda_ctx* ctx, uint64_t* out);
+int lda_doc_topic_dist(lda_ctx* ctx, int64_t doc, int normalize, float* out);
+
 /* Streaming bandwidth of `device` (no counterpart in the reference; the
  * ceiling the sweep's bytes are set against): out[0] GB/s of a hand-written
  * streaming read of `bytes` (>= 1 MiB, multiple of 16) in 16-byte vectors,

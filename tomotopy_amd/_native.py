@@ -46,6 +46,9 @@ _SIGS = {
     "lda_set_word_priors": [_c_p, ctypes.c_int64, _c_p, _c_p],
     "lda_selftest_reciprocal": [_i32, _i32, _c_p],
     "lda_probe_bandwidth": [_i32, _u64, _i32, _c_p],
+    "lda_topic_word_dist": [_c_p, _i32, _i32, _c_p],
+    "lda_count_by_topic": [_c_p, _c_p],
+    "lda_doc_topic_dist": [_c_p, _i64, _i32, _c_p],
     "lda_infer_together": [_c_p, _c_p, _c_p, ctypes.c_int64, _i32, _u64, _i32, _i32, _c_p, _c_p, _c_p],
     "lda_infer_together_w": [_c_p, _c_p, _c_p, _i64, _c_p, _i32, _u64, _i32, _i32, _c_p, _c_p, _c_p],
     "lda_set_alpha": [_c_p, _c_p],
@@ -160,6 +163,24 @@ class Context:
         new.k, new.v, new.tw = self.k, self.v, self.tw
         new.n_docs, new.n_tokens = self.n_docs, self.n_tokens
         return new
+
+    def topic_word_dist(self, topic, normalize=True):
+        """(n_kw + eta) / (n_k + V*eta) of one topic over the words, read out on the device."""
+        out = np.zeros(self.v, dtype=np.float32)
+        _check(load().lda_topic_word_dist(self._h, int(topic), int(bool(normalize)), _ptr(out)))
+        return out
+
+    def count_by_topic(self):
+        """Tokens assigned to each topic (valid tokens only), counted on the device."""
+        out = np.zeros(self.k, dtype=np.uint64)
+        _check(load().lda_count_by_topic(self._h, _ptr(out)))
+        return out
+
+    def doc_topic_dist(self, doc, normalize=True):
+        """(n_dk + alpha) / (weight sum + sum alpha) of training document `doc`."""
+        out = np.zeros(self.k, dtype=np.float32)
+        _check(load().lda_doc_topic_dist(self._h, int(doc), int(bool(normalize)), _ptr(out)))
+        return out
 
     def fail_at_step(self, step):
         """Test hook: the training iteration starting at global step `step` fails (-1 clears)."""

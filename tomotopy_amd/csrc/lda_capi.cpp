@@ -852,6 +852,7 @@ int lda_ctx_destroy(lda_ctx* c) {
     c->g_wt.release(); c->g_docw.release(); c->g_jobrun.release(); c->g_runword.release();
     c->g_cells.release(); c->g_ndk.release(); c->g_nk.release(); c->g_segb.release(); c->g_segr.release();
     c->d_dglut.release(); c->d_aux.release(); c->d_bad.release(); c->d_hist.release(); c->d_hist_len.release();
+    c->d_nkw16.release(); c->r_out.release(); c->r_cnt.release();
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     c->fold_streams.destroy();
@@ -1478,6 +1479,82 @@ int lda_selftest_reciprocal(int exp_shift, int device, uint64_t* mismatches) {
     unsigned long long h = 0;
     HIP_TRY(hipMemcpy(&h, bad.p, sizeof(h), hipMemcpyDeviceToHost));
     *mismatches = h;
+    return LDA_OK;
+}
+
+int lda_topic_word_dist(lda_ctx* c, int topic, int normalize, float* out) {
+    if (!c || !out) return fail(LDA_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (topic < 0 || topic >= c->K) return fail(LDA_ERR_INVALID_ARGUMENT, "topic_id must < K");
+    if (!c->V) return LDA_OK;
+    HIP_TRY(hipSetDevice(c->device));
+    const hipStream_t s = c->stream;
+    const int q = c->pos(topic);
+    // sum = n_k + V * eta in float: V * eta as float(V) * eta (LDAModel.hpp:833)
+    float sum = 1.0f;
+    const float veta = (float)c->V * c->eta;
+    if (c->tw != LDA_TW_ONE) {
+        float nk = 0.0f;
+        HIP_TRY(hipMemcpyAsync(&nk, reinterpret_cast<const float*>(c->d_nk.p) + q, 4, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        sum = nk + veta;
+    } else {
+        int32_t nk = 0;
+        HIP_TRY(hipMemcpyAsync(&nk, c->d_nk.p + q, 4, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        sum = (float)nk + veta;
+    }
+    HIP_TRY(c->r_out.alloc(c->V));
+    HIP_TRY(lda::launch_topic_word_dist(c->d_nkw.p, c->tw != LDA_TW_ONE, c->V, c->Kp, q, c->eta, sum, normalize,
+                                        c->r_out.p, s));
+    HIP_TRY(hipMemcpyAsync(out, c->r_out.p, (size_t)c->V * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return LDA_OK;
+}
+
+int lda_count_by_topic(lda_ctx* c, uint64_t* out) {
+    if (!c || !out) return fail(LDA_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (!c->has_corpus) {
+        std::fill(out, out + c->K, (uint64_t)0);
+        return LDA_OK;
+    }
+    HIP_TRY(hipSetDevice(c->device));
+    const hipStream_t s = c->stream;
+    HIP_TRY(c->r_cnt.alloc(c->K));
+    HIP_TRY(lda::launch_fill_u32(reinterpret_cast<uint32_t*>(c->r_cnt.p), 0u, 2 * (int64_t)c->K, s));
+    if (c->N)
+        HIP_TRY(lda::launch_count_topics(c->d_words.p, c->d_z.p, c->N, c->V, c->K,
+                                         reinterpret_cast<unsigned long long*>(c->r_cnt.p), s));
+    HIP_TRY(hipMemcpyAsync(out, c->r_cnt.p, (size_t)c->K * 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return LDA_OK;
+}
+
+int lda_doc_topic_dist(lda_ctx* c, int64_t doc, int normalize, float* out) {
+    if (!c || !out) return fail(LDA_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (!c->has_corpus || doc < 0 || doc >= c->D) return fail(LDA_ERR_INVALID_ARGUMENT, "document index out of range");
+    HIP_TRY(hipSetDevice(c->device));
+    const hipStream_t s = c->stream;
+    std::vector<int32_t> row(c->Kp);
+    float docw = 0.0f;
+    int32_t L = 0;
+    HIP_TRY(hipMemcpyAsync(row.data(), c->d_ndk.p + (size_t)doc * c->Kp, (size_t)c->Kp * 4, hipMemcpyDeviceToHost, s));
+    if (c->tw != LDA_TW_ONE) HIP_TRY(hipMemcpyAsync(&docw, c->d_docw.p + doc, 4, hipMemcpyDeviceToHost, s));
+    else HIP_TRY(hipMemcpyAsync(&L, c->d_doclen.p + doc, 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    // _getTopicsByDoc (LDAModel.hpp:1145-1159): (n_dk + alpha) / (weight sum +
+    // sum alpha) in float; the weight sum is the valid token count (int32) for
+    // TermWeight ONE and the float sum of the token weights otherwise
+    float asum = 0.0f;
+    for (int k = 0; k < c->K; ++k) asum += c->alpha[k];
+    const float den = (c->tw != LDA_TW_ONE ? docw : (float)L) + asum;
+    for (int k = 0; k < c->K; ++k) {
+        const int32_t bits = row[c->pos(k)];
+        float n;
+        if (c->tw != LDA_TW_ONE) std::memcpy(&n, &bits, 4);
+        else n = (float)bits;
+        const float x = n + c->alpha[k];
+        out[k] = normalize ? x / den : x;
+    }
     return LDA_OK;
 }
 

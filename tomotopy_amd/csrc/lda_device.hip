@@ -1130,6 +1130,34 @@ __global__ void build_counts_kernel(const uint32_t* __restrict__ words, const ui
     }
 }
 
+// Read-outs.  One topic's row of n_kw over all words (a gather of one cell
+// per word column) as tomoto's _getWidsByTopic computes it
+// (LDAModel.hpp:828-840): (n_kw + eta) / (n_k + V * eta) in float, the sum
+// formed by the caller; C is int32_t (TermWeight ONE) or float.
+template <typename C>
+__global__ void topic_word_dist_kernel(const C* __restrict__ nkw, int V, int Kp, int pos, float eta, float sum,
+                                       int normalize, float* __restrict__ out) {
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x) {
+        const float x = (float)nkw[(int64_t)v * Kp + pos] + eta;
+        out[v] = normalize ? x / sum : x;
+    }
+}
+
+// Tokens per topic over the valid tokens (tomoto's _getTopicsCount,
+// LDAModel.hpp:815-826): a workgroup histogram in LDS, added to the totals.
+__global__ void __launch_bounds__(256) count_topics_kernel(const uint32_t* __restrict__ words,
+                                                           const uint16_t* __restrict__ z, uint64_t N, int V, int K,
+                                                           unsigned long long* __restrict__ cnt) {
+    extern __shared__ int lds_hist[];
+    for (int k = threadIdx.x; k < K; k += blockDim.x) lds_hist[k] = 0;
+    __syncthreads();
+    for (uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; g < N; g += (uint64_t)gridDim.x * blockDim.x)
+        if (words[g] < (uint32_t)V) atomicAdd(&lds_hist[z[g]], 1);
+    __syncthreads();
+    for (int k = threadIdx.x; k < K; k += blockDim.x)
+        if (lds_hist[k]) atomicAdd(&cnt[k], (unsigned long long)lds_hist[k]);
+}
+
 // Streaming-bandwidth probes: every thread reads (and for the copy writes)
 // 16-byte vectors in a grid-stride loop.  The read probe folds what it reads
 // into a value stored only when it equals a key the data cannot produce (the
@@ -2560,6 +2588,25 @@ int infer_blocks_per_cu(int kpl, int segs, bool asym) {
         case 32: return asym ? infer_blocks_kpl<32, true>(segs) : infer_blocks_kpl<32, false>(segs);
         default: return 0;
     }
+}
+
+hipError_t launch_topic_word_dist(const void* nkw, bool fl, int V, int Kp, int pos, float eta, float sum, int normalize,
+                                  float* out, hipStream_t s) {
+    const int blocks = std::max(1, std::min((V + 255) / 256, 4096));
+    if (fl)
+        hipLaunchKernelGGL(topic_word_dist_kernel<float>, dim3(blocks), dim3(256), 0, s,
+                           reinterpret_cast<const float*>(nkw), V, Kp, pos, eta, sum, normalize, out);
+    else
+        hipLaunchKernelGGL(topic_word_dist_kernel<int32_t>, dim3(blocks), dim3(256), 0, s,
+                           reinterpret_cast<const int32_t*>(nkw), V, Kp, pos, eta, sum, normalize, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_count_topics(const uint32_t* words, const uint16_t* z, uint64_t N, int V, int K,
+                               unsigned long long* cnt, hipStream_t s) {
+    const int blocks = (int)std::max<uint64_t>(1, std::min<uint64_t>((N + 255) / 256, 1024));
+    hipLaunchKernelGGL(count_topics_kernel, dim3(blocks), dim3(256), K * sizeof(int), s, words, z, N, V, K, cnt);
+    return hipGetLastError();
 }
 
 hipError_t launch_probe_read(const void* src, int64_t bytes, int* sink, int grid, hipStream_t s) {

@@ -174,6 +174,10 @@ struct lda_ctx {
     int nkw_hot = 0;
     DevBuf<uint16_t> d_nkw16;
 
+    // read-out staging (lda_topic_word_dist, lda_count_by_topic)
+    DevBuf<float> r_out;
+    DevBuf<uint64_t> r_cnt;
+
     // lda_debug_fail_at_step: the training iteration at this global step
     // reports LDA_ERR_TRAINING (-1: never); private copies never inherit it
     int64_t fail_at_step = -1;

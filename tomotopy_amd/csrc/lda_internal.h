@@ -206,6 +206,12 @@ hipError_t launch_sweep_device(int kpl, const SweepParams& p, int grid, int wave
 // n_kw rows [hot, V) between int32 (nkw) and u16 (nkw16, row v - hot); pack
 // sets *bad when a cell is outside [0, kNarrowCf)
 hipError_t launch_pack_narrow(const int32_t* nkw, uint16_t* nkw16, int hot, int V, int Kp, uint32_t* bad, hipStream_t s);
+// Read-outs: one topic's (n_kw + eta) / sum over the words (fl: float
+// counts); tokens per topic over the valid tokens (cnt zeroed by the caller).
+hipError_t launch_topic_word_dist(const void* nkw, bool fl, int V, int Kp, int pos, float eta, float sum, int normalize,
+                                  float* out, hipStream_t s);
+hipError_t launch_count_topics(const uint32_t* words, const uint16_t* z, uint64_t N, int V, int K,
+                               unsigned long long* cnt, hipStream_t s);
 // Bandwidth probes: a streaming read of `bytes` (multiple of 16) and a copy.
 hipError_t launch_probe_read(const void* src, int64_t bytes, int* sink, int grid, hipStream_t s);
 hipError_t launch_probe_copy(const void* src, void* dst, int64_t bytes, int grid, hipStream_t s);

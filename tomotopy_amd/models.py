@@ -36,8 +36,11 @@ class ParallelScheme(IntEnum):
 
     On the device: NONE (or workers=1) runs one worker; PARTITION/DEFAULT with
     workers=W >= 2 runs tomoto's partition schedule with W device workers;
-    workers=0 runs the device-wide schedule (every resident wavefront a
-    worker, exact atomics on n_kw).
+    COPY_MERGE with workers=W >= 2 tomoto's copy_merge with W workers;
+    PARTITION or COPY_MERGE with workers=0 that scheme with the device's own
+    worker count; DEFAULT with workers=0 the device-wide schedule (every
+    workgroup a worker with a private n_k, exact atomics on n_kw, at most an
+    eighth of the documents in flight), which tomotopy does not have.
     """
     DEFAULT = 0
     NONE = 1
@@ -62,9 +65,14 @@ def _format_numpy(arr, prefix=""):
 class LDAModel:
     """Latent Dirichlet Allocation by collapsed Gibbs sampling on an MI355X.
 
+The first train() with workers=0 and parallel=DEFAULT warns once per process
+that it runs the device-wide schedule rather than tomotopy's partition.
+
 > * Blei, D.M., Ng, A.Y., & Jordan, M.I. (2003). Latent dirichlet allocation. JMLR 3, 993-1022.
 > * Newman, D., Asuncion, A., Smyth, P., & Welling, M. (2009). Distributed algorithms for topic models. JMLR 10, 1801-1828.
 """
+
+    _dw_warned = False
 
     def __init__(self, tw="one", min_cf=0, min_df=0, rm_top=0, k=1, alpha=0.1, eta=0.01, seed=None, corpus=None,
                  transform=None):
@@ -302,7 +310,15 @@ call before `train` (py_LDA.cpp:241-269, LDAModel.hpp:1031-1045).'''
     # ---------------------------------------------------------------- train --
     def train(self, iterations=10, workers=0, parallel=0, freeze_topics=False, callback_interval=10, callback=None,
               show_progress=False):
-        '''Train the model with `iterations` Gibbs sweeps on the GPU (py_LDA.cpp:277-320).'''
+        '''Train the model with `iterations` Gibbs sweeps on the GPU (py_LDA.cpp:277-320).
+With workers=0 and parallel=DEFAULT the sweeps run the device-wide schedule,
+which tomotopy does not have (its DEFAULT is PARTITION): pass
+parallel=PARTITION for tomoto's schedule with the device's worker count.'''
+        if workers == 0 and int(parallel) == int(ParallelScheme.DEFAULT) and not LDAModel._dw_warned:
+            LDAModel._dw_warned = True
+            warnings.warn("workers=0 with ParallelScheme.DEFAULT runs the device-wide schedule (shared n_kw with exact "
+                          "atomics, private n_k per workgroup), not tomotopy's partition schedule; pass "
+                          "parallel=ParallelScheme.PARTITION for that", RuntimeWarning, stacklevel=2)
         if self._seed_given and workers != 1:
             warnings.warn("The training result may differ even with fixed seed if `workers` != 1.", RuntimeWarning,
                           stacklevel=2)
@@ -378,17 +394,15 @@ call before `train` (py_LDA.cpp:241-269, LDAModel.hpp:1031-1045).'''
 
     # ------------------------------------------------------------ read-outs --
     def get_topic_word_dist(self, topic_id, normalize=True):
-        '''(n_kw + eta) / (n_k + V*eta) for every used word (LDAModel.hpp:815-841).'''
+        '''(n_kw + eta) / (n_k + V*eta) for every used word (LDAModel.hpp:828-840), read out on the device.'''
         self._ensure_prepared()
         if not 0 <= topic_id < self._k:
             raise ValueError("topic_id must < K")
-        col = self._state("nkw")[:, topic_id].astype(np.float32) + np.float32(self._eta)
-        if not normalize:
-            return col.tolist()
-        den = np.float32(np.float32(self._state("nk")[topic_id]) + np.float32(self._real_v) * np.float32(self._eta))
-        return (col / den).astype(np.float32).tolist()
+        return self._ctx.topic_word_dist(int(topic_id), normalize).tolist()
 
     def get_topic_words(self, topic_id, top_n=10, return_id=False):
+        '''The top_n words of a topic by get_topic_word_dist (TopicModel.hpp:285-288, 777-815); words
+of equal probability are listed by ascending id (tomoto's std::sort leaves their order unspecified).'''
         dist = np.asarray(self.get_topic_word_dist(topic_id))
         order = np.argsort(-dist, kind="stable")[:top_n]
         if return_id:
@@ -396,13 +410,9 @@ call before `train` (py_LDA.cpp:241-269, LDAModel.hpp:1031-1045).'''
         return [(self._vocab[i], float(dist[i])) for i in order]
 
     def get_count_by_topics(self):
-        '''Tokens assigned to each topic (LDAModel.hpp:815-826).'''
+        '''Tokens assigned to each topic (LDAModel.hpp:815-826), counted on the device.'''
         self._ensure_prepared()
-        if self._tw != TermWeight.ONE:
-            z = self._state("z")
-            valid = self._words < self._real_v
-            return np.bincount(z[valid].astype(np.int64), minlength=self._k).astype(np.int64).tolist()
-        return self._state("nk").astype(np.int64).tolist()
+        return self._ctx.count_by_topic().astype(np.int64).tolist()
 
     # ---------------------------------------------------------------- infer --
     def infer(self, doc, iterations=100, tolerance=-1, workers=0, parallel=0, together=False, transform=None):

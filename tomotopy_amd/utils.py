@@ -61,7 +61,7 @@ class Document:
     def get_topic_dist(self, normalize=True):
         """(n_dk + alpha) / (L_d + sum(alpha)) (LDAModel.hpp:1145-1159)."""
         if self._doc_index is not None:
-            ndk = self._model._doc_ndk(self._doc_index).astype(np.float32)
+            return self._model._ctx.doc_topic_dist(self._doc_index, normalize).tolist()
         elif self._ndk is None:
             warnings.warn(_NO_TOPIC_MSG, RuntimeWarning, stacklevel=2)
             return [0.0] * self._model.k
