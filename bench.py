@@ -490,48 +490,58 @@ def _percentile(x, q):
     return float(np.percentile(np.asarray(x, dtype=np.float64), q)) if len(x) else None
 
 
-def serve_bench(iterations=50, qps_list=(1, 2, 5, 10, 20, 50, 100, 200, 400, 800, 1200, 1600), seconds=4.0,
-                plain_requests=60, seed=2024, seg_tokens=None, step_tokens=12000, lanes=1, lanes_sweep=(2, 4)):
-    """Serving loop on a device-resident K=500, V=100,000 model: Poisson request
-    arrivals at each QPS (requests of 1..64 docs, lengths LogNormal(ln 150, 1)
-    clipped to [5, 10000], 50 iterations); latency is submit to results on the
-    host, measured inside the server.  The segmentation is the autotuner's
-    choice on the pooled requests' documents among the candidates whose ll
-    per word stays within 0.1% of exact sampling; the served ll per word is
-    set against exact sampling (no segmentation) of the same requests.
-    Baselines on the same requests, served one at a time back to back: plain
-    per-request GPU infer (one lda_infer call per request, with the served
-    segmentation and exact), and plain per-document GPU infer (one launch and
-    one synchronisation per document).  The top rate is repeated with 2 and 4
-    server lanes."""
-    import threading
-    from tomotopy_amd import _native, autotune, corpus, synth
+def serve_setup(n_pool=600):
+    """The serving workload (600 pooled requests) and its K=500, V=100,000
+    model trained 50 iterations on the device."""
+    from tomotopy_amd import _native, corpus, synth
     t = time.time()
-    n_pool = 600
     (tw, toff, nw), reqs_raw = synth.serve_workload(n_pool)
     ws, offs, V, order = corpus.prepare_arrays(tw, toff, nw)
     reqs = [corpus.csr_from_arrays(w, o, order) for w, o in reqs_raw]
     log("serving workload: %d training docs, %d pooled requests (%d docs) (%.1fs)"
         % (len(offs) - 1, len(reqs), sum(len(o) - 1 for _, o in reqs), time.time() - t))
-    K = 500
-    ctx = _native.Context(K, V, 0.1, 0.01, seed=55)
+    ctx = _native.Context(500, V, 0.1, 0.01, seed=55)
     ctx.upload_corpus(ws, offs)
     ctx.train(50, workers=0)
     allw = np.concatenate([w for w, _ in reqs]).astype(np.uint32)
     alloff = np.zeros(sum(len(o) - 1 for _, o in reqs) + 1, np.uint64)
     alloff[1:] = np.cumsum(np.concatenate([np.diff(o.astype(np.int64)) for _, o in reqs]))
+    return ctx, ws, offs, V, reqs, allw, alloff
+
+
+def serve_bench(setup=None, iterations=50, qps_list=(1, 2, 5, 10, 20, 50, 100, 200, 400, 800, 1200, 1600),
+                seconds=4.0, plain_requests=60, seed=2024, seg_tokens=None, step_tokens=12000, lanes=1,
+                lanes_sweep=(2, 4), ll_tol=1e-3, cpu=True):
+    """Serving loop on a device-resident K=500, V=100,000 model: Poisson request
+    arrivals at each QPS (requests of 1..64 docs, lengths LogNormal(ln 150, 1)
+    clipped to [5, 10000], 50 iterations); latency is submit to results on the
+    host, measured inside the server.  The segmentation is the autotuner's
+    choice on the pooled requests' documents among the candidates whose ll
+    per word stays within ll_tol of exact sampling (None: the fastest, no ll
+    guard); the served ll per word is set against exact sampling (no
+    segmentation) of the same requests.  Baselines on the same requests,
+    served one at a time back to back: plain per-request GPU infer (one
+    lda_infer call per request, with the served segmentation and exact), and
+    (cpu=True) plain per-document GPU infer (one launch and one
+    synchronisation per document) and the CPU oracle.  The top rate is
+    repeated with each lane count of lanes_sweep."""
+    import threading
+    from tomotopy_amd import _native, autotune
+    ctx, ws, offs, V, reqs, allw, alloff = setup or serve_setup()
+    K = ctx.k
     tune = None
     if seg_tokens is None:
-        tune = autotune.tune_infer(ctx, allw, alloff, iterations, infer_seed=seed)
+        tune = autotune.tune_infer(ctx, allw, alloff, iterations, infer_seed=seed,
+                                   ll_tol=float("inf") if ll_tol is None else ll_tol)
         seg_tokens, seg_max = tune["seg_tokens"], tune["seg_max"]
-        log("serve autotuner: seg_tokens %d seg_max %d (%s)" % (seg_tokens, seg_max, ", ".join(
+        log("serve autotuner (ll guard %s): seg_tokens %d seg_max %d (%s)" % (ll_tol, seg_tokens, seg_max, ", ".join(
             "%d:%.0fms/%.2e" % (c["seg_tokens"], c["kernel_ms"], c["ll_rel_diff"]) for c in tune["candidates"])))
     else:
         seg_max = ctx.infer_options()["seg_supported"] if seg_tokens else 1
     ctx.set_infer_options(seg_tokens, seg_max)
     out = {"workload": "serve_k500", "K": K, "V": V, "iterations": iterations, "levels": [],
            "seg_tokens": seg_tokens, "seg_max": seg_max, "step_tokens": step_tokens, "lanes": lanes,
-           "autotune": tune}
+           "ll_guard": ll_tol, "autotune": tune}
     srv = _native.Server(ctx, hbm_budget_bytes=8 << 30, iterations=iterations, step_iters=iterations,
                          step_tokens=step_tokens, infer_seed=seed, lanes=lanes)
 
@@ -662,15 +672,16 @@ def serve_bench(iterations=50, qps_list=(1, 2, 5, 10, 20, 50, 100, 200, 400, 800
         out["plain_per_request_exact"] = {"requests": len(sub), "docs": ndocs, "docs_per_s": ndocs / dtx,
                                           "p50_ms": _percentile(lat_x, 50), "p99_ms": _percentile(lat_x, 99),
                                           "seg_tokens": 0}
-    sub2 = reqs[:max(1, plain_requests // 10)]
-    t0 = time.perf_counter()
-    for w, o in sub2:
-        ctx.infer(w, o, iterations, infer_seed=seed, mode=1)
-    dt = time.perf_counter() - t0
-    nd2 = sum(len(o) - 1 for _, o in sub2)
-    out["plain_per_document"] = {"requests": len(sub2), "docs": nd2, "docs_per_s": nd2 / dt}
-    out["cpu_baseline"] = cpu_infer_baseline(ctx, ws, offs, V, K, 0.1, 0.01, allw, alloff, iterations)
-    log("serving CPU baseline: %.0f docs/s (%s)" % (out["cpu_baseline"]["value"], out["cpu_baseline"]["sample"]))
+    if cpu:
+        sub2 = reqs[:max(1, plain_requests // 10)]
+        t0 = time.perf_counter()
+        for w, o in sub2:
+            ctx.infer(w, o, iterations, infer_seed=seed, mode=1)
+        dt = time.perf_counter() - t0
+        nd2 = sum(len(o) - 1 for _, o in sub2)
+        out["plain_per_document"] = {"requests": len(sub2), "docs": nd2, "docs_per_s": nd2 / dt}
+        out["cpu_baseline"] = cpu_infer_baseline(ctx, ws, offs, V, K, 0.1, 0.01, allw, alloff, iterations)
+        log("serving CPU baseline: %.0f docs/s (%s)" % (out["cpu_baseline"]["value"], out["cpu_baseline"]["sample"]))
     best = max(lv["docs_per_s"] for lv in out["levels"])
     ok = [lv for lv in out["levels"] if lv["p99_ms"] is not None and lv["p99_ms"] <= 50.0]
     out["best_docs_per_s"] = max(best, out["saturation"]["docs_per_s"])
@@ -679,11 +690,23 @@ def serve_bench(iterations=50, qps_list=(1, 2, 5, 10, 20, 50, 100, 200, 400, 800
     best_ok = max((lv["docs_per_s"] for lv in ok), default=0.0)
     out["docs_per_s_with_p99_le_50ms"] = best_ok
     out["speedup_vs_plain_per_request_at_p99_le_50ms"] = best_ok / out["plain_per_request"]["docs_per_s"]
-    log("plain per-request %.0f docs/s (p50 %.1f ms), plain per-document %.1f docs/s; serve/plain %.1fx at p99 <= 50 ms "
-        "(%.1fx at saturation); served ll identical to plain on the same requests: %s"
-        % (out["plain_per_request"]["docs_per_s"], out["plain_per_request"]["p50_ms"],
-           out["plain_per_document"]["docs_per_s"], out["speedup_vs_plain_per_request_at_p99_le_50ms"],
-           out["speedup_vs_plain_per_request"], same_ll))
+    log("seg_tokens %d: plain per-request %.0f docs/s (p50 %.1f ms); serve/plain %.1fx at p99 <= 50 ms (%.1fx at "
+        "saturation); served ll identical to plain on the same requests: %s; served ll vs exact sampling %.2e"
+        % (seg_tokens, out["plain_per_request"]["docs_per_s"], out["plain_per_request"]["p50_ms"],
+           out["speedup_vs_plain_per_request_at_p99_le_50ms"], out["speedup_vs_plain_per_request"], same_ll,
+           out["served_vs_exact_ll_rel_diff"]))
+    return out
+
+
+def serve_both():
+    """Two operating points of the server on one model: the fidelity point
+    (the autotuner's segmentation within 0.1% ll of exact sampling; lanes
+    sweep and every baseline) and the latency point (seg_tokens = 128: every
+    document over 128 tokens split, which shortens the per-wavefront chains
+    that set the latency; its ll against exact sampling reported)."""
+    setup = serve_setup()
+    out = serve_bench(setup, ll_tol=1e-3)
+    out["latency_point"] = serve_bench(setup, seg_tokens=128, lanes_sweep=(), cpu=False)
     return out
 
 
@@ -710,7 +733,7 @@ def main():
         print(json.dumps({"leg": "probe", "probe": bandwidth_probe()}), flush=True)
         return
     if args.only in ("infer", "infer_pmi", "serve"):
-        res = {"infer": infer_bench, "infer_pmi": infer_pmi_bench, "serve": serve_bench}[args.only]()
+        res = {"infer": infer_bench, "infer_pmi": infer_pmi_bench, "serve": serve_both}[args.only]()
         print(json.dumps({"metric": METRIC, "leg": args.only, args.only: res}), flush=True)
         return
 
@@ -800,7 +823,7 @@ def main():
         out["infer"] = infer_bench()
         out["infer_pmi"] = infer_pmi_bench()
     if not args.no_serve and args.only is None:
-        out["serve"] = serve_bench()
+        out["serve"] = serve_both()
     print(json.dumps(out), flush=True)
 
 

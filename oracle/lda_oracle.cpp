@@ -1075,8 +1075,12 @@ void orc_loglik_w(void* h, double* ll_docs, double* ll_rest) { loglik_w(*(Model*
 // word-run start at or after floor(n*w/S).  Each iteration every segment
 // samples its tokens in order against n_dk as it was when the iteration
 // started plus its own changes (n_k,eff likewise); the segments' changes are
-// then summed.  c_d needs no merging: a word's tokens lie in one segment.
-// S == 1 is the sequential sampler above.
+// then summed.  With sync_tokens > 0 the iteration is cut into steps: in step
+// j every segment samples its tokens [start + j*sync_tokens, start +
+// (j+1)*sync_tokens) against n_dk at the start of the step plus its own
+// changes, and the changes are summed after every step (the integer device
+// kernel merges after every 64-token batch).  c_d needs no merging: a word's
+// tokens lie in one segment.  S == 1 is the sequential sampler above.
 static int seg_count(uint64_t n, uint32_t seg_tokens, int seg_max) {
     if (seg_tokens == 0 || n <= seg_tokens) return 1;
     uint64_t need = (n + seg_tokens - 1) / seg_tokens;
@@ -1095,7 +1099,8 @@ static uint64_t seg_start(const uint32_t* w, uint64_t n, int i, int S) {
 }
 
 void orc_infer_seg(void* h, const uint32_t* words, const uint64_t* off, int64_t D, int iters, uint64_t infer_seed,
-                   int mode, uint32_t seg_tokens, int seg_max, float* theta, double* ll, uint16_t* z_out);
+                   int mode, uint32_t seg_tokens, int seg_max, uint32_t sync_tokens, float* theta, double* ll,
+                   uint16_t* z_out);
 
 // Segment count and starts of one document (S + 1 entries in starts).
 int orc_seg_bounds(const uint32_t* words, uint64_t n, uint32_t seg_tokens, int seg_max, uint64_t* starts) {
@@ -1106,11 +1111,12 @@ int orc_seg_bounds(const uint32_t* words, uint64_t n, uint32_t seg_tokens, int s
 
 void orc_infer(void* h, const uint32_t* words, const uint64_t* off, int64_t D, int iters,
                uint64_t infer_seed, int mode, float* theta, double* ll, uint16_t* z_out) {
-    orc_infer_seg(h, words, off, D, iters, infer_seed, mode, 0, 1, theta, ll, z_out);
+    orc_infer_seg(h, words, off, D, iters, infer_seed, mode, 0, 1, 0, theta, ll, z_out);
 }
 
 void orc_infer_seg(void* h, const uint32_t* words, const uint64_t* off, int64_t D, int iters, uint64_t infer_seed,
-                   int mode, uint32_t seg_tokens, int seg_max, float* theta, double* ll, uint16_t* z_out) {
+                   int mode, uint32_t seg_tokens, int seg_max, uint32_t sync_tokens, float* theta, double* ll,
+                   uint16_t* z_out) {
     Model* m = (Model*)h;
     const int K = m->K;
     std::vector<float> ca(K);
@@ -1141,11 +1147,18 @@ void orc_infer_seg(void* h, const uint32_t* words, const uint64_t* off, int64_t 
         std::vector<int32_t> col(K);
         const int S = seg_count(n, seg_tokens, seg_max);
         std::vector<int32_t> snap(K), nd_s(K), nk_s(K), merged(K);
-        for (int it = 0; it < iters; ++it) {
+        // steps per iteration: one, or enough for the longest segment
+        uint64_t longest = 0;
+        for (int sg = 0; sg < S; ++sg)
+            longest = std::max(longest, seg_start(words + b, n, sg + 1, S) - seg_start(words + b, n, sg, S));
+        const uint64_t step = (S > 1 && sync_tokens) ? sync_tokens : std::max<uint64_t>(longest, 1);
+        const uint64_t nsteps = std::max<uint64_t>((longest + step - 1) / step, 1);
+        for (int it = 0; it < iters; ++it) for (uint64_t j = 0; j < nsteps; ++j) {
             snap = ndk;
             merged = ndk;
             for (int sg = 0; sg < S; ++sg) {
-                const uint64_t s0 = seg_start(words + b, n, sg, S), s1 = seg_start(words + b, n, sg + 1, S);
+                const uint64_t a0 = seg_start(words + b, n, sg, S), a1 = seg_start(words + b, n, sg + 1, S);
+                const uint64_t s0 = std::min(a1, a0 + j * step), s1 = std::min(a1, a0 + (j + 1) * step);
                 nd_s = snap;
                 for (int k = 0; k < K; ++k) nk_s[k] = m->nk[k] + snap[k];
                 for (uint64_t i = s0; i < s1; ++i) {

@@ -62,7 +62,7 @@ def load():
         "orc_get_state": (None, [_c_p, _c_p, _c_p, _c_p, _c_p]),
         "orc_loglik": (None, [_c_p, i32, _c_p, _c_p]),
         "orc_infer": (None, [_c_p, _c_p, _c_p, i64, i32, u64, i32, _c_p, _c_p, _c_p]),
-        "orc_infer_seg": (None, [_c_p, _c_p, _c_p, i64, i32, u64, i32, u32, i32, _c_p, _c_p, _c_p]),
+        "orc_infer_seg": (None, [_c_p, _c_p, _c_p, i64, i32, u64, i32, u32, i32, u32, _c_p, _c_p, _c_p]),
         "orc_seg_bounds": (i32, [_c_p, u64, u32, i32, _c_p]),
         "orc_infer_ll_bruteforce": (f64, [_c_p, _c_p, u64, _c_p]),
         "orc_set_corpus_w": (None, [_c_p, _c_p, _c_p, i64, _c_p, _c_p]),
@@ -251,8 +251,11 @@ class Model:
         load().orc_loglik(self._h, mode, ctypes.byref(a), ctypes.byref(b))
         return a.value, b.value
 
-    def infer(self, words, off, iters, infer_seed=0, mode=0, seg_tokens=0, seg_max=1):
-        """Fold-in; seg_tokens / seg_max select the segmented rule (0: sequential)."""
+    def infer(self, words, off, iters, infer_seed=0, mode=0, seg_tokens=0, seg_max=1, seg_sync=64):
+        """Fold-in; seg_tokens / seg_max select the segmented rule (0: sequential);
+        seg_sync: the segments merge their n_dk changes after every seg_sync tokens
+        of each segment (64: the integer device kernel's rule), or once per
+        iteration (0)."""
         words = np.ascontiguousarray(words, np.uint32)
         off = np.ascontiguousarray(off, np.uint64)
         D = len(off) - 1
@@ -260,7 +263,7 @@ class Model:
         ll = np.zeros(D, np.float64)
         z = np.zeros(int(off[-1]), np.uint16)
         load().orc_infer_seg(self._h, _p(words), _p(off), D, iters, int(infer_seed) & (2**64 - 1), mode,
-                             int(seg_tokens), int(seg_max), _p(theta), _p(ll), _p(z))
+                             int(seg_tokens), int(seg_max), int(seg_sync), _p(theta), _p(ll), _p(z))
         return theta, ll, z
 
     def infer_gen(self, words, off, iters, infer_seed=0, weights=None, faithful=False, seg_tokens=0, seg_max=1):

@@ -295,10 +295,11 @@ def test_together_weighted_closed_forms(kind):
 @pytest.mark.parametrize("T,M", [(40, 8), (100, 2), (7, 4)])
 def test_segmented_one_is_the_integer_segmented_fold_in(T, M):
     """Integer counts without priors: the segmented rule on running counts
-    is the integer fold-in's segmented rule (pinned by its own tests)."""
+    (merged once per iteration) is the integer fold-in's segmented rule with
+    per-iteration merges (pinned by its own tests)."""
     m, V, uw, uoff, _, _ = _setup("one")
     th0, ll0, z0 = m.infer_gen(uw, uoff, 6, infer_seed=3, seg_tokens=T, seg_max=M)
-    th1, ll1, z1 = m.infer(uw, uoff, 6, infer_seed=3, mode=0, seg_tokens=T, seg_max=M)
+    th1, ll1, z1 = m.infer(uw, uoff, 6, infer_seed=3, mode=0, seg_tokens=T, seg_max=M, seg_sync=0)
     assert np.array_equal(z0, z1)
     assert np.array_equal(th0, th1)
     np.testing.assert_allclose(ll0, ll1, rtol=1e-10)

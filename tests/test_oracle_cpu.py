@@ -379,10 +379,12 @@ def test_segmented_infer_reduces_to_sequential(small_corpus):
     assert np.array_equal(z0, z2) and np.array_equal(ll0, ll2)
 
 
-def _py_segmented_infer(nkw, nk, alpha, eta, V, doc, iters, seed, seg_tokens, seg_max):
+def _py_segmented_infer(nkw, nk, alpha, eta, V, doc, iters, seed, seg_tokens, seg_max, sync=0):
     """Pure-Python restatement of the segmented fold-in of one document, built
     from the separately pinned primitives (weights, device scan, search,
-    Philox uniforms)."""
+    Philox uniforms): per step (the whole iteration when sync == 0, else
+    `sync` tokens of every segment) each segment samples against the step's
+    starting n_dk plus its own changes, and the changes are summed."""
     K = len(alpha)
     kpl = O.device_kpl(K)
     veta = np.float32(np.float32(eta) * np.float32(V))
@@ -393,12 +395,17 @@ def _py_segmented_infer(nkw, nk, alpha, eta, V, doc, iters, seed, seg_tokens, se
         if doc[i] < V:
             z[i] = O.search(ca, K, O.token_uniform(seed, i, 0, "infinit"))
     st = O.seg_bounds(doc, seg_tokens, seg_max)
-    for it in range(1, iters + 1):
+    S = len(st) - 1
+    longest = max(int(st[s + 1]) - int(st[s]) for s in range(S))
+    step = sync if (S > 1 and sync) else max(longest, 1)
+    steps = [(it, q) for it in range(1, iters + 1) for q in range(max(1, -(-longest // step)))]
+    for it, q in steps:
         snap = np.bincount(z[z != 0xFFFF], minlength=K).astype(np.int64)
         delta = np.zeros(K, np.int64)
-        for s in range(len(st) - 1):
+        for s in range(S):
             nd = snap.copy()
-            for i in range(int(st[s]), int(st[s + 1])):
+            a0, a1 = int(st[s]), int(st[s + 1])
+            for i in range(min(a1, a0 + q * step), min(a1, a0 + (q + 1) * step)):
                 v = int(doc[i])
                 if v >= V:
                     continue
@@ -433,12 +440,19 @@ def test_segmented_infer_matches_python_restatement():
     m.train(4, workers=1)
     _, _, nkw, nk = m.state()
     al = np.full(K, alpha, np.float32)
-    for seg_tokens, seg_max, n in ((10, 4, 37), (20, 2, 40), (0, 8, 25), (3, 8, 9)):
+    for seg_tokens, seg_max, n, sync in ((10, 4, 37, 0), (20, 2, 40, 0), (0, 8, 25, 0), (3, 8, 9, 0),
+                                         (10, 4, 37, 3), (20, 2, 40, 7), (3, 8, 9, 1), (10, 4, 37, 64)):
         doc = np.sort(rng.integers(0, V, n)).astype(np.uint32)
         o = np.array([0, n], np.uint64)
-        _, _, z = m.infer(doc, o, 3, infer_seed=7, seg_tokens=seg_tokens, seg_max=seg_max)
-        zp = _py_segmented_infer(nkw, nk.astype(np.int64), al, 0.05, V, doc, 3, 7, seg_tokens, seg_max)
-        assert np.array_equal(z.astype(np.int64), zp), (seg_tokens, seg_max)
+        _, _, z = m.infer(doc, o, 3, infer_seed=7, seg_tokens=seg_tokens, seg_max=seg_max, seg_sync=sync)
+        zp = _py_segmented_infer(nkw, nk.astype(np.int64), al, 0.05, V, doc, 3, 7, seg_tokens, seg_max, sync)
+        assert np.array_equal(z.astype(np.int64), zp), (seg_tokens, seg_max, sync)
+    # the steps matter: merging every 3 tokens samples other topics than once per iteration
+    doc = np.sort(rng.integers(0, V, 60)).astype(np.uint32)
+    o = np.array([0, 60], np.uint64)
+    _, _, za = m.infer(doc, o, 3, infer_seed=7, seg_tokens=10, seg_max=4, seg_sync=0)
+    _, _, zb = m.infer(doc, o, 3, infer_seed=7, seg_tokens=10, seg_max=4, seg_sync=3)
+    assert not np.array_equal(za, zb)
     # LL and theta of a segmented result stay the closed forms of its z
     th, ll, z = m.infer(doc, o, 5, seg_tokens=2, seg_max=4)
     nd = np.bincount(z[z != 0xFFFF], minlength=K)

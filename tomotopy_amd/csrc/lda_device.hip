@@ -1791,7 +1791,7 @@ __global__ void __launch_bounds__(64 * S) infer_kernel(const InferParams* __rest
     const int lane = lane_id();
     const int wave = (int)(threadIdx.x >> 6);
     int* nd_sh = lds_dyn;        // S > 1: merged n_dk (Kp entries, lane layout)
-    int* job_sh = lds_dyn + Kp;  // S > 1: the workgroup's job index
+    int* job_sh = lds_dyn + Kp;  // S > 1: the workgroup's job index, then the S segment lengths
     int nvalid, lastLane;
     lane_limits<KPL>(p.K, lane, nvalid, lastLane);
     fvec<KPL> al;
@@ -1824,7 +1824,17 @@ __global__ void __launch_bounds__(64 * S) infer_kernel(const InferParams* __rest
         const uint64_t se = fold_seg_start(words, b, e - b, wave + 1, S, lane);
         if constexpr (S > 1) {
             for (int i = threadIdx.x; i < Kp; i += 64 * S) nd_sh[i] = 0;
+            if (lane == 0) job_sh[1 + wave] = (int)(se - sb);
             __syncthreads();
+        }
+        // batches of 64 tokens per iteration: every segment runs as many as
+        // the longest one has (the merges after each batch are workgroup
+        // barriers); one segment runs its own
+        int nbatch = (int)((se - sb + 63) >> 6);
+        if constexpr (S > 1) {
+            nbatch = 0;
+#pragma unroll
+            for (int w = 0; w < S; ++w) nbatch = max(nbatch, (job_sh[1 + w] + 63) >> 6);
         }
         // topics at the start of this launch (initial draw or the stored z), n_dk from them
         ivec<VW> nd = 0;
@@ -1877,102 +1887,105 @@ __global__ void __launch_bounds__(64 * S) infer_kernel(const InferParams* __rest
             for (int j = 0; j < KPL; ++j) st.template set<F_ES>(j, es[j]);
         }
         for (int it = it_begin + 1; it <= it_end; ++it) {
-            if constexpr (S > 1) {
-                __syncthreads();  // every segment's change is in nd_sh
-                lds_load<KPL>(nd_sh, lane, snap);
-            } else if (it == it_begin + 1) {
-#pragma unroll
-                for (int j = 0; j < KPL; ++j) snap[j] = nd[j];
-            }
-            if (S > 1 || it == it_begin + 1) {
-#pragma unroll
-                for (int j = 0; j < KPL; ++j) {
-                    const int n = nk0[j] + snap[j];
-                    const float bb = (float)n + (ASYM ? es[j] : p.veta);
-                    st.template set<F_A>(j, (float)snap[j] + al[j]);
-                    st.template seti<F_NK>(j, n);
-                    st.template set<F_BF>(j, bb);
-                    if constexpr (!fold_inline_y<KPL>()) st.template set<F_Y>(j, 1.0f / bb);
-                }
-            }
             int vcur = -1;
-            for (uint64_t t0 = sb; t0 < se; t0 += 64) {
-                const int n = (int)((se - t0) < 64 ? (se - t0) : 64);
-                const uint64_t g = t0 + (uint64_t)lane;
-                int wv = -1, zv = 0xFFFF;
-                float uv = 0.0f;
-                if (lane < n) {
-                    wv = (int)words[g];
-                    zv = zs[g];
-                    uv = uniform24(philox_x0(p.seed_lo, p.seed_hi, (uint32_t)(g - b), (uint32_t)((g - b) >> 32), (uint32_t)it, TAG_INFER));
-                }
-                // previous token's word (lane 0: the last word of the previous batch)
-                const int wp = __builtin_amdgcn_update_dpp(vcur, wv, 0x138, 0xF, 0xF, false);
-                const unsigned long long starts = __ballot(lane < n && wv != wp);
-                unsigned long long ahead = starts;
-                ivec<KPL> slot[P];
+            for (int bt = 0; bt < nbatch; ++bt) {
+                if constexpr (S > 1) {
+                    __syncthreads();  // every segment's change is in nd_sh
+                    lds_load<KPL>(nd_sh, lane, snap);
+                } else if (it == it_begin + 1 && bt == 0) {
 #pragma unroll
-                for (int q = 0; q < P; ++q) fold_issue<KPL>(slot[q], ahead, wv, nkw, nkv, p.V, p.Kp, lane);
-                int zkeep = zv;
-                // tokens continuing the previous batch's last run
-                const int pre = starts ? __builtin_ctzll(starts) : n;
-                if ((unsigned)vcur < (unsigned)p.V) {
-                    for (int t = 0; t < pre; ++t) {
-                        const int zn = fold_token<KPL, ASYM>(st, lane, readlane_i(zv, t), readlane_f(uv, t), p.K, nvalid,
-                                                       lastLane, p.eta, p.veta);
-                        zkeep = (lane == t) ? zn : zkeep;
+                    for (int j = 0; j < KPL; ++j) snap[j] = nd[j];
+                }
+                if (S > 1 || (it == it_begin + 1 && bt == 0)) {
+#pragma unroll
+                    for (int j = 0; j < KPL; ++j) {
+                        const int n = nk0[j] + snap[j];
+                        const float bb = (float)n + (ASYM ? es[j] : p.veta);
+                        st.template set<F_A>(j, (float)snap[j] + al[j]);
+                        st.template seti<F_NK>(j, n);
+                        st.template set<F_BF>(j, bb);
+                        if constexpr (!fold_inline_y<KPL>()) st.template set<F_Y>(j, 1.0f / bb);
                     }
                 }
-                // Runs in token order: run r takes its column from slot r % P
-                // and refills that slot with run r + P.  The unrolled q loop
-                // keeps every slot index a constant.
-                unsigned long long todo = starts;
-                while (todo) {
+                const uint64_t t0 = sb + 64 * (uint64_t)bt;
+                if (t0 < se) {
+                    const int n = (int)((se - t0) < 64 ? (se - t0) : 64);
+                    const uint64_t g = t0 + (uint64_t)lane;
+                    int wv = -1, zv = 0xFFFF;
+                    float uv = 0.0f;
+                    if (lane < n) {
+                        wv = (int)words[g];
+                        zv = zs[g];
+                        uv = uniform24(philox_x0(p.seed_lo, p.seed_hi, (uint32_t)(g - b), (uint32_t)((g - b) >> 32), (uint32_t)it, TAG_INFER));
+                    }
+                    // previous token's word (lane 0: the last word of the previous batch)
+                    const int wp = __builtin_amdgcn_update_dpp(vcur, wv, 0x138, 0xF, 0xF, false);
+                    const unsigned long long starts = __ballot(lane < n && wv != wp);
+                    unsigned long long ahead = starts;
+                    ivec<KPL> slot[P];
 #pragma unroll
-                    for (int q = 0; q < P; ++q) {
-                        if (todo) {
-                            const int i0 = __builtin_ctzll(todo);
-                            todo &= todo - 1;
-                            const int i1 = todo ? __builtin_ctzll(todo) : n;
-                            const int v = readlane_i(wv, i0);
-                            vcur = v;
+                    for (int q = 0; q < P; ++q) fold_issue<KPL>(slot[q], ahead, wv, nkw, nkv, p.V, p.Kp, lane);
+                    int zkeep = zv;
+                    // tokens continuing the previous batch's last run
+                    const int pre = starts ? __builtin_ctzll(starts) : n;
+                    if ((unsigned)vcur < (unsigned)p.V) {
+                        for (int t = 0; t < pre; ++t) {
+                            const int zn = fold_token<KPL, ASYM>(st, lane, readlane_i(zv, t), readlane_f(uv, t), p.K, nvalid,
+                                                           lastLane, p.eta, p.veta);
+                            zkeep = (lane == t) ? zn : zkeep;
+                        }
+                    }
+                    // Runs in token order: run r takes its column from slot r % P
+                    // and refills that slot with run r + P.  The unrolled q loop
+                    // keeps every slot index a constant.
+                    unsigned long long todo = starts;
+                    while (todo) {
 #pragma unroll
-                            for (int j = 0; j < KPL; ++j) st.template seti<F_COL>(j, slot[q][j]);
-                            if constexpr (ASYM) {
-                                // the word's eta column: its prior row, or eta
-                                fvec<KPL> ew = p.eta;
-                                if ((unsigned)v < (unsigned)p.V) {
-                                    const int pr = p.prior_row[v];
-                                    if (pr >= 0) load_kvec_f<KPL>(p.prior_eta + (int64_t)pr * p.Kp, lane, ew);
+                        for (int q = 0; q < P; ++q) {
+                            if (todo) {
+                                const int i0 = __builtin_ctzll(todo);
+                                todo &= todo - 1;
+                                const int i1 = todo ? __builtin_ctzll(todo) : n;
+                                const int v = readlane_i(wv, i0);
+                                vcur = v;
+#pragma unroll
+                                for (int j = 0; j < KPL; ++j) st.template seti<F_COL>(j, slot[q][j]);
+                                if constexpr (ASYM) {
+                                    // the word's eta column: its prior row, or eta
+                                    fvec<KPL> ew = p.eta;
+                                    if ((unsigned)v < (unsigned)p.V) {
+                                        const int pr = p.prior_row[v];
+                                        if (pr >= 0) load_kvec_f<KPL>(p.prior_eta + (int64_t)pr * p.Kp, lane, ew);
+                                    }
+#pragma unroll
+                                    for (int j = 0; j < KPL; ++j) st.template set<F_E>(j, ew[j]);
                                 }
-#pragma unroll
-                                for (int j = 0; j < KPL; ++j) st.template set<F_E>(j, ew[j]);
-                            }
-                            fold_issue<KPL>(slot[q], ahead, wv, nkw, nkv, p.V, p.Kp, lane);
-                            if ((unsigned)v < (unsigned)p.V) {
-                                // c_d of this word: the run's topics from the batch
-                                // registers (not yet resampled in this iteration)
-                                // and, past the batch, from memory
-                                for (int j = i0; j < i1; ++j) col_add<KPL>(st, lane, readlane_i(zv, j));
-                                if (i1 == n) fold_run_counts<KPL>(st, words, zs, lane, t0 + (uint64_t)n, se, v);
-                                for (int t = i0; t < i1; ++t) {
-                                    const int zn = fold_token<KPL, ASYM>(st, lane, readlane_i(zv, t), readlane_f(uv, t), p.K,
-                                                                   nvalid, lastLane, p.eta, p.veta);
-                                    zkeep = (lane == t) ? zn : zkeep;
+                                fold_issue<KPL>(slot[q], ahead, wv, nkw, nkv, p.V, p.Kp, lane);
+                                if ((unsigned)v < (unsigned)p.V) {
+                                    // c_d of this word: the run's topics from the batch
+                                    // registers (not yet resampled in this iteration)
+                                    // and, past the batch, from memory
+                                    for (int j = i0; j < i1; ++j) col_add<KPL>(st, lane, readlane_i(zv, j));
+                                    if (i1 == n) fold_run_counts<KPL>(st, words, zs, lane, t0 + (uint64_t)n, se, v);
+                                    for (int t = i0; t < i1; ++t) {
+                                        const int zn = fold_token<KPL, ASYM>(st, lane, readlane_i(zv, t), readlane_f(uv, t), p.K,
+                                                                       nvalid, lastLane, p.eta, p.veta);
+                                        zkeep = (lane == t) ? zn : zkeep;
+                                    }
                                 }
                             }
                         }
                     }
+                    if (lane < n) zs[g] = (uint16_t)zkeep;
                 }
-                if (lane < n) zs[g] = (uint16_t)zkeep;
-            }
-            if constexpr (S > 1) {
-                if (it < it_end) {
-                    __syncthreads();  // every segment has read nd_sh for this iteration
+                if constexpr (S > 1) {
+                    if (it < it_end || bt + 1 < nbatch) {
+                        __syncthreads();  // every segment has read nd_sh for this batch
 #pragma unroll
-                    for (int j = 0; j < KPL; ++j) {
-                        const int dlt = (int)__builtin_rintf(st.template get<F_A>(j) - al[j]) - snap[j];
-                        if (dlt) atomicAdd(nd_sh + kpos<KPL>(lane, j), dlt);
+                        for (int j = 0; j < KPL; ++j) {
+                            const int dlt = (int)__builtin_rintf(st.template get<F_A>(j) - al[j]) - snap[j];
+                            if (dlt) atomicAdd(nd_sh + kpos<KPL>(lane, j), dlt);
+                        }
                     }
                 }
             }
@@ -2565,7 +2578,7 @@ int sweep_device_blocks_per_cu(int kpl, bool asym, int waves) {
 
 template <int KPL, bool ASYM>
 static int infer_blocks_kpl(int segs) {
-    constexpr size_t lds = (64 * KPL + 4) * sizeof(int);
+    constexpr size_t lds = (64 * KPL + 12) * sizeof(int);
     int n = 0;
     hipError_t e = hipErrorInvalidValue;
     switch (segs) {
@@ -2859,7 +2872,7 @@ hipError_t launch_sample_frozen(int kpl, const int32_t* ndk, const int32_t* nkw,
 
 template <int KPL, bool ASYM>
 static hipError_t launch_infer_kpl(int segs, const InferParams* p_dev, int grid, hipStream_t s) {
-    constexpr size_t lds = (64 * KPL + 4) * sizeof(int);
+    constexpr size_t lds = (64 * KPL + 12) * sizeof(int);
     switch (segs) {
         case 1: hipLaunchKernelGGL((infer_kernel<KPL, 1, ASYM>), dim3(grid), dim3(64), 0, s, p_dev); break;
         case 2: hipLaunchKernelGGL((infer_kernel<KPL, 2, ASYM>), dim3(grid), dim3(128), lds, s, p_dev); break;
