@@ -2393,6 +2393,292 @@ __global__ void __launch_bounds__(SEG ? 64 * kMaxFoldSegments : 64) infer_gen_ke
     }
 }
 
+// ------------------------- register-resident fold-in of IDF / PMI models ----
+// The state of infer_gen_kernel<KPL, true, false> kept in registers, as the
+// integer fold-in keeps its own (FoldBanks, indexed by a wave-uniform slot):
+//   F_A   n_dk (float)           F_COL  the current run's private column
+//   F_NK  n_k of the private copy (float)
+//   F_Y   RN(1 / (F_NK + V*eta)) (formed in the weight loop for KPL <= 4)
+//   F_AL  alpha_k
+// The count update is bump_w's (tomoto's CntUpdater<true>: adding w,
+// removing max(x - w, 0); LDAModel.hpp:287-299, Utils.hpp:132-155) and the
+// weights are token_step_w's, so every draw equals the LDS kernel's.
+template <int KPL>
+__device__ __forceinline__ void fold_bump_w(FoldBanks<KPL>& st, int lane, int z, float w, bool dec, float veta) {
+    const int sl = z & (KPL - 1);
+    const bool own = lane == (z >> kpl_shift<KPL>());
+    const float od = st.template get<F_A>(sl), oc = st.template get<F_COL>(sl), on = st.template get<F_NK>(sl);
+    const float nd = dec ? sub_clamp(od, w) : od + w;
+    const float nc = dec ? sub_clamp(oc, w) : oc + w;
+    const float nn = dec ? sub_clamp(on, w) : on + w;
+    st.template set<F_A>(sl, own ? nd : od);
+    st.template set<F_COL>(sl, own ? nc : oc);
+    st.template set<F_NK>(sl, own ? nn : on);
+    if constexpr (!fold_inline_y<KPL>()) {
+        const float oy = st.template get<F_Y>(sl);
+        const float yn = recip_rn(nn + veta);
+        st.template set<F_Y>(sl, own ? yn : oy);
+    }
+}
+
+template <int KPL>
+__device__ __forceinline__ int fold_token_w(FoldBanks<KPL>& st, int lane, int zo, float wt, float u, int K, int nvalid,
+                                            int lastLane, float eta, float veta) {
+    fold_bump_w<KPL>(st, lane, zo, wt, true, veta);
+    fvec<KPL> w;
+#pragma unroll
+    for (int j = 0; j < KPL; ++j) {
+        const float a = (st.template get<F_A>(j) + st.template get<F_AL>(j)) * (st.template get<F_COL>(j) + eta);
+        const float bf = st.template get<F_NK>(j) + veta;
+        const float y = fold_inline_y<KPL>() ? recip_rn(bf) : st.template get<F_Y>(j);
+        const float q0 = a * y;
+        const float rr = __builtin_fmaf(-q0, bf, a);
+        w[j] = __builtin_fmaf(rr, y, q0);
+    }
+    float tot;
+    const int zn = scan_search<KPL>(w, u, K, nvalid, lastLane, tot);
+    fold_bump_w<KPL>(st, lane, zn, wt, false, veta);
+    return zn;
+}
+
+template <int KPL>
+__device__ __forceinline__ void fold_set_state_w(FoldBanks<KPL>& st, const fvec<KPL>& nd, const fvec<KPL>& nk,
+                                                 float veta) {
+#pragma unroll
+    for (int j = 0; j < KPL; ++j) {
+        st.template set<F_A>(j, nd[j]);
+        st.template set<F_NK>(j, nk[j]);
+        if constexpr (!fold_inline_y<KPL>()) st.template set<F_Y>(j, recip_rn(nk[j] + veta));
+    }
+}
+
+template <int KPL>
+__device__ __forceinline__ void fold_load_col_w(FoldBanks<KPL>& st, const fvec<KPL>& c) {
+#pragma unroll
+    for (int j = 0; j < KPL; ++j) st.template set<F_COL>(j, c[j]);
+}
+
+template <int KPL>
+__device__ __forceinline__ void fold_store_col_w(FoldBanks<KPL>& st, float* colp, int lane) {
+    fvec<KPL> c;
+#pragma unroll
+    for (int j = 0; j < KPL; ++j) c[j] = st.template get<F_COL>(j);
+    lds_store_f<KPL>(colp, lane, c);  // store_kvec of the float bits (global memory here)
+}
+
+// The register-resident weighted fold-in holds up to 8 topics per lane (K <=
+// 512) without spilling; larger K keeps the LDS-state kernel.
+constexpr int kFoldWMaxKpl = 8;
+
+// LDS of infer_w_kernel with S > 1 wavefronts: per wavefront its n_dk and n_k
+// at the end of an iteration (Kp each), then the merged n_dk and n_k.
+template <int KPL>
+constexpr size_t gen_w_lds_bytes(int segs) { return (size_t)(segs > 1 ? 2 * segs + 2 : 0) * 64 * KPL * sizeof(float); }
+
+// Fold-in of an IDF / PMI model without word priors: infer_gen_kernel's
+// schedule (wavefront 0 adds the initial topics in token order; the
+// segments start every iteration from the merged n_dk / n_k, whose changes
+// are added to it in segment order) and private run columns in HBM, with the
+// per-token state in registers.  A run's column is loaded when the run
+// starts (the next run's a token ahead) and stored when it ends; every
+// column is read once per iteration and was last stored in the previous
+// one, so one wait per iteration orders the loads after those stores.
+template <int KPL, bool SEG>
+__global__ void __launch_bounds__(SEG ? 64 * kMaxFoldSegments : 64) infer_w_kernel(InferGenParams p) {
+    constexpr int Kp = 64 * KPL;
+    constexpr int VW = fold_vw<KPL>();
+    const int lane = lane_id();
+    const int S = SEG ? (int)(blockDim.x >> 6) : 1;
+    const int wave = SEG ? (int)(threadIdx.x >> 6) : 0;
+    float* lds_f = reinterpret_cast<float*>(lds_dyn);
+    float* my_nd = lds_f + wave * 2 * Kp;
+    float* my_nk = my_nd + Kp;
+    float* snap_nd = lds_f + S * 2 * Kp;
+    float* snap_nk = snap_nd + Kp;
+    __shared__ int64_t job_sh;
+    int nvalid, lastLane;
+    lane_limits<KPL>(p.K, lane, nvalid, lastLane);
+    fvec<KPL> al, nk0;
+    load_kvec_f<KPL>(p.alpha, lane, al);
+    load_kvec_f<KPL>(reinterpret_cast<const float*>(p.nk), lane, nk0);
+    const float eta = p.eta, veta = p.veta;
+    float* cells = reinterpret_cast<float*>(p.cells);
+    const float* nkwf = reinterpret_cast<const float*>(p.nkw);
+    for (;;) {
+        int64_t idx;
+        if (!p.counter) {
+            idx = p.job_base;
+        } else if (S == 1) {
+            uint32_t t = 0;
+            if (lane == 0) t = atomicAdd(p.counter, 1u);
+            idx = (int64_t)(uint32_t)uniform_i((int)t);
+        } else {
+            if (threadIdx.x == 0) job_sh = (int64_t)atomicAdd(p.counter, 1u);
+            __syncthreads();
+            idx = job_sh;
+        }
+        if (idx >= p.njobs) break;
+        const InferJob J = p.jobs[idx];
+        const uint64_t b = J.b, e = J.e;
+        fvec<VW> b0 = 0.0f, b1 = 0.0f, b2 = 0.0f, b3 = 0.0f, b4 = 0.0f, b5 = 0.0f, b6 = 0.0f, b7 = 0.0f;
+        FoldBanks<KPL> st{b0, b1, b2, b3, b4, b5, b6, b7};
+#pragma unroll
+        for (int j = 0; j < KPL; ++j) st.template set<F_AL>(j, al[j]);
+        if (wave == 0) {
+            // private n_k = the model's, n_dk = 0; initial topics added in token
+            // order, each run's column starting as the model's
+            fold_set_state_w<KPL>(st, fvec<KPL>(0.0f), nk0, veta);
+            int64_t run = p.job_run[idx] - 1;
+            float sw = 0.0f;
+            int vcur = -1;
+            float* colp = nullptr;
+            for (uint64_t t0 = b; t0 < e; t0 += 64) {
+                const int n = (int)((e - t0) < 64 ? (e - t0) : 64);
+                const uint64_t g = t0 + (uint64_t)lane;
+                int wv = -1, zz = 0xFFFF;
+                float tw = 1.0f;
+                if (lane < n) {
+                    wv = (int)p.words[g];
+                    tw = p.wt[g];
+                    if ((unsigned)wv < (unsigned)p.V) {
+                        const float u = uniform24(philox_x0(p.seed_lo, p.seed_hi, (uint32_t)(g - b), (uint32_t)((g - b) >> 32), 0u, TAG_INFINIT));
+                        const float rr = u * p.alpha_cdf[p.K - 1];
+                        int lo = 0, hi = p.K;
+                        while (lo < hi) {
+                            const int mid = (lo + hi) >> 1;
+                            if (rr < p.alpha_cdf[mid]) hi = mid; else lo = mid + 1;
+                        }
+                        zz = lo < p.K ? lo : p.K - 1;
+                    }
+                    p.z[g] = (uint16_t)zz;
+                }
+                for (int i = 0; i < n; ++i) {
+                    const int v = readlane_i(wv, i);
+                    if ((unsigned)v >= (unsigned)p.V) continue;
+                    if (v != vcur) {
+                        if (colp) fold_store_col_w<KPL>(st, colp, lane);
+                        ++run;
+                        colp = cells + run * p.Kp;
+                        fvec<KPL> c;
+                        load_kvec_f<KPL>(nkwf + (int64_t)v * p.Kp, lane, c);
+                        fold_load_col_w<KPL>(st, c);
+                        vcur = v;
+                    }
+                    const float w = readlane_f(tw, i);
+                    fold_bump_w<KPL>(st, lane, readlane_i(zz, i), w, false, veta);
+                    sw += w;
+                }
+            }
+            if (colp) fold_store_col_w<KPL>(st, colp, lane);
+            if (lane == 0) p.docw[J.slot] = sw;
+            if (S > 1) {
+                fvec<KPL> a, c;
+#pragma unroll
+                for (int j = 0; j < KPL; ++j) { a[j] = st.template get<F_A>(j); c[j] = st.template get<F_NK>(j); }
+                lds_store_f<KPL>(snap_nd, lane, a);
+                lds_store_f<KPL>(snap_nk, lane, c);
+            }
+        }
+        const int64_t sbase = idx * (kMaxFoldSegments + 1);
+        const uint64_t sb = S > 1 ? p.seg_b[sbase + wave] : b;
+        const uint64_t se = S > 1 ? p.seg_b[sbase + wave + 1] : e;
+        const int64_t run0 = S > 1 ? p.seg_r[sbase + wave] : p.job_run[idx];
+        for (int it = 1; it <= J.iters; ++it) {
+            if (S > 1) {
+                __syncthreads();  // the merged state (initial or the last merge) is complete
+                fvec<KPL> a, c;
+                lds_load_f<KPL>(snap_nd, lane, a);
+                lds_load_f<KPL>(snap_nk, lane, c);
+                fold_set_state_w<KPL>(st, a, c, veta);
+            } else {
+                // the previous iteration's column stores precede this iteration's loads
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+            }
+            int64_t run = run0 - 1;
+            int vcur = -1, vnext = -1;
+            float* colp = nullptr;
+            fvec<KPL> ncol;
+            for (uint64_t t0 = sb; t0 < se; t0 += 64) {
+                const int n = (int)((se - t0) < 64 ? (se - t0) : 64);
+                const uint64_t g = t0 + (uint64_t)lane;
+                int wv = -1, zv = 0xFFFF;
+                float uv = 0.0f, tw = 1.0f;
+                if (lane < n) {
+                    wv = (int)p.words[g];
+                    zv = p.z[g];
+                    tw = p.wt[g];
+                    uv = uniform24(philox_x0(p.seed_lo, p.seed_hi, (uint32_t)(g - b), (uint32_t)((g - b) >> 32), (uint32_t)it, TAG_INFER));
+                }
+                int zkeep = zv;
+                for (int i = 0; i < n; ++i) {
+                    const int v = readlane_i(wv, i);
+                    if ((unsigned)v >= (unsigned)p.V) continue;
+                    if (v != vcur) {
+                        if (colp) fold_store_col_w<KPL>(st, colp, lane);
+                        ++run;
+                        colp = cells + run * p.Kp;
+                        if (v != vnext) load_kvec_f<KPL>(colp, lane, ncol);
+                        fold_load_col_w<KPL>(st, ncol);
+                        vcur = v;
+                    }
+                    if (i + 1 < n) {
+                        const int v2 = readlane_i(wv, i + 1);
+                        if (v2 != v && (unsigned)v2 < (unsigned)p.V) {
+                            load_kvec_f<KPL>(colp + p.Kp, lane, ncol);
+                            vnext = v2;
+                        }
+                    }
+                    const int zn = fold_token_w<KPL>(st, lane, readlane_i(zv, i), readlane_f(tw, i), readlane_f(uv, i),
+                                                     p.K, nvalid, lastLane, eta, veta);
+                    zkeep = (lane == i) ? zn : zkeep;
+                }
+                if (lane < n) p.z[g] = (uint16_t)zkeep;
+            }
+            if (colp) fold_store_col_w<KPL>(st, colp, lane);
+            if (S > 1) {
+                fvec<KPL> a, c;
+#pragma unroll
+                for (int j = 0; j < KPL; ++j) { a[j] = st.template get<F_A>(j); c[j] = st.template get<F_NK>(j); }
+                lds_store_f<KPL>(my_nd, lane, a);
+                lds_store_f<KPL>(my_nk, lane, c);
+                __syncthreads();  // every segment is done
+                // the merged state + the segments' changes, in segment order
+                for (int q = (int)threadIdx.x; q < Kp; q += (int)blockDim.x) {
+                    const float a0 = snap_nd[q], c0 = snap_nk[q];
+                    float aa = a0, cc = c0;
+                    for (int w = 0; w < S; ++w) {
+                        aa = aa + (lds_f[w * 2 * Kp + q] - a0);
+                        cc = cc + (lds_f[w * 2 * Kp + Kp + q] - c0);
+                    }
+                    snap_nd[q] = aa;
+                    snap_nk[q] = cc;
+                }
+            }
+        }
+        // the document's n_dk and n_k for the finalize kernel
+        float* ndo = reinterpret_cast<float*>(p.ndk_out) + (int64_t)J.slot * p.Kp;
+        float* nko = reinterpret_cast<float*>(p.nk_out) + (int64_t)J.slot * p.Kp;
+        if (S > 1) {
+            __syncthreads();  // the last merge (or wavefront 0's initial state) is complete
+            if (wave == 0) {
+                fvec<KPL> a, c;
+                lds_load_f<KPL>(snap_nd, lane, a);
+                lds_load_f<KPL>(snap_nk, lane, c);
+                lds_store_f<KPL>(ndo, lane, a);
+                lds_store_f<KPL>(nko, lane, c);
+            }
+            __syncthreads();  // every wavefront is done with this job before job_sh and the LDS change
+        } else {
+            fvec<KPL> a, c;
+#pragma unroll
+            for (int j = 0; j < KPL; ++j) { a[j] = st.template get<F_A>(j); c[j] = st.template get<F_NK>(j); }
+            lds_store_f<KPL>(ndo, lane, a);
+            lds_store_f<KPL>(nko, lane, c);
+        }
+        if (!p.counter) break;
+    }
+}
+
 // theta = (n_dk + alpha) / (weight sum + sum alpha) and the FP64
 // log-likelihood getLLRest(private) - getLLRest(model) + getLLDocs(doc)
 // (LDAModel.hpp:616-674, 945-946): the n_k terms of the topics whose n_k
@@ -2896,6 +3182,10 @@ template <int KPL, bool WT, bool ASYM>
 static hipError_t launch_infer_gen_k(int segs, const InferGenParams& p, int grid, bool finalize, hipStream_t s) {
     if (finalize)
         hipLaunchKernelGGL((infer_gen_finalize_kernel<KPL, WT, ASYM>), dim3(grid), dim3(64), 0, s, p);
+    else if (WT && !ASYM && KPL <= kFoldWMaxKpl && segs == 1)
+        hipLaunchKernelGGL((infer_w_kernel<KPL, false>), dim3(grid), dim3(64), 0, s, p);
+    else if (WT && !ASYM && KPL <= kFoldWMaxKpl)
+        hipLaunchKernelGGL((infer_w_kernel<KPL, true>), dim3(grid), dim3(64 * segs), gen_w_lds_bytes<KPL>(segs), s, p);
     else if (segs == 1)
         hipLaunchKernelGGL((infer_gen_kernel<KPL, WT, ASYM, false>), dim3(grid), dim3(64), gen_lds_bytes<KPL>(1), s, p);
     else
@@ -2939,6 +3229,16 @@ static int infer_gen_blocks_seg(bool wt, bool asym, int segs, size_t lds) {
 
 template <int KPL>
 static int infer_gen_blocks_kpl(bool wt, bool asym, int segs) {
+    if (wt && !asym && KPL <= kFoldWMaxKpl) {
+        // the register-resident kernel (infer_w_kernel)
+        const size_t lds = gen_w_lds_bytes<KPL>(segs);
+        if (lds > 160 * 1024) return 0;
+        int n = 0;
+        const hipError_t e = segs == 1
+            ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_w_kernel<KPL, false>, 64, lds)
+            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, infer_w_kernel<KPL, true>, 64 * segs, lds);
+        return e == hipSuccess ? n : 0;
+    }
     const size_t lds = gen_lds_bytes<KPL>(segs);
     if (lds > 160 * 1024) return 0;
     return segs == 1 ? infer_gen_blocks_seg<KPL, false>(wt, asym, 1, lds) : infer_gen_blocks_seg<KPL, true>(wt, asym, segs, lds);
