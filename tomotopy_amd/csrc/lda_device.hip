@@ -2458,12 +2458,20 @@ __device__ __forceinline__ void fold_load_col_w(FoldBanks<KPL>& st, const fvec<K
     for (int j = 0; j < KPL; ++j) st.template set<F_COL>(j, c[j]);
 }
 
+// Column loads and stores of the lanes that hold a topic below K (lanes
+// from ceil(K / KPL) on hold padding only; their column entries read 0).
 template <int KPL>
-__device__ __forceinline__ void fold_store_col_w(FoldBanks<KPL>& st, float* colp, int lane) {
+__device__ __forceinline__ void fold_fetch_col_w(const float* colp, int lane, bool live, fvec<KPL>& c) {
+    if (live) load_kvec_f<KPL>(colp, lane, c);
+    else c = fvec<KPL>(0.0f);
+}
+
+template <int KPL>
+__device__ __forceinline__ void fold_store_col_w(FoldBanks<KPL>& st, float* colp, int lane, bool live) {
     fvec<KPL> c;
 #pragma unroll
     for (int j = 0; j < KPL; ++j) c[j] = st.template get<F_COL>(j);
-    lds_store_f<KPL>(colp, lane, c);  // store_kvec of the float bits (global memory here)
+    if (live) lds_store_f<KPL>(colp, lane, c);  // store_kvec of the float bits (global memory here)
 }
 
 // The register-resident weighted fold-in holds up to 8 topics per lane (K <=
@@ -2479,10 +2487,11 @@ constexpr size_t gen_w_lds_bytes(int segs) { return (size_t)(segs > 1 ? 2 * segs
 // schedule (wavefront 0 adds the initial topics in token order; the
 // segments start every iteration from the merged n_dk / n_k, whose changes
 // are added to it in segment order) and private run columns in HBM, with the
-// per-token state in registers.  A run's column is loaded when the run
-// starts (the next run's a token ahead) and stored when it ends; every
-// column is read once per iteration and was last stored in the previous
-// one, so one wait per iteration orders the loads after those stores.
+// per-token state in registers.  The columns of a segment's runs are
+// consecutive cells, so each run's column is loaded two runs ahead of its
+// use (by run index) and stored when the run ends; every column is read
+// once per iteration and was last stored in the previous one, so one wait
+// per iteration orders the loads after those stores.
 template <int KPL, bool SEG>
 __global__ void __launch_bounds__(SEG ? 64 * kMaxFoldSegments : 64) infer_w_kernel(InferGenParams p) {
     constexpr int Kp = 64 * KPL;
@@ -2498,6 +2507,7 @@ __global__ void __launch_bounds__(SEG ? 64 * kMaxFoldSegments : 64) infer_w_kern
     __shared__ int64_t job_sh;
     int nvalid, lastLane;
     lane_limits<KPL>(p.K, lane, nvalid, lastLane);
+    const bool live = lane * KPL < p.K;
     fvec<KPL> al, nk0;
     load_kvec_f<KPL>(p.alpha, lane, al);
     load_kvec_f<KPL>(reinterpret_cast<const float*>(p.nk), lane, nk0);
@@ -2556,11 +2566,11 @@ __global__ void __launch_bounds__(SEG ? 64 * kMaxFoldSegments : 64) infer_w_kern
                     const int v = readlane_i(wv, i);
                     if ((unsigned)v >= (unsigned)p.V) continue;
                     if (v != vcur) {
-                        if (colp) fold_store_col_w<KPL>(st, colp, lane);
+                        if (colp) fold_store_col_w<KPL>(st, colp, lane, live);
                         ++run;
                         colp = cells + run * p.Kp;
                         fvec<KPL> c;
-                        load_kvec_f<KPL>(nkwf + (int64_t)v * p.Kp, lane, c);
+                        fold_fetch_col_w<KPL>(nkwf + (int64_t)v * p.Kp, lane, live, c);
                         fold_load_col_w<KPL>(st, c);
                         vcur = v;
                     }
@@ -2569,7 +2579,7 @@ __global__ void __launch_bounds__(SEG ? 64 * kMaxFoldSegments : 64) infer_w_kern
                     sw += w;
                 }
             }
-            if (colp) fold_store_col_w<KPL>(st, colp, lane);
+            if (colp) fold_store_col_w<KPL>(st, colp, lane, live);
             if (lane == 0) p.docw[J.slot] = sw;
             if (S > 1) {
                 fvec<KPL> a, c;
@@ -2583,6 +2593,7 @@ __global__ void __launch_bounds__(SEG ? 64 * kMaxFoldSegments : 64) infer_w_kern
         const uint64_t sb = S > 1 ? p.seg_b[sbase + wave] : b;
         const uint64_t se = S > 1 ? p.seg_b[sbase + wave + 1] : e;
         const int64_t run0 = S > 1 ? p.seg_r[sbase + wave] : p.job_run[idx];
+        const int64_t rend = S > 1 ? p.seg_r[sbase + wave + 1] : p.job_run[idx + 1];
         for (int it = 1; it <= J.iters; ++it) {
             if (S > 1) {
                 __syncthreads();  // the merged state (initial or the last merge) is complete
@@ -2595,9 +2606,11 @@ __global__ void __launch_bounds__(SEG ? 64 * kMaxFoldSegments : 64) infer_w_kern
                 __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
             }
             int64_t run = run0 - 1;
-            int vcur = -1, vnext = -1;
+            int vcur = -1;
             float* colp = nullptr;
-            fvec<KPL> ncol;
+            fvec<KPL> c1 = 0.0f, c2 = 0.0f;  // the columns of runs run + 1 and run + 2
+            if (run0 < rend) fold_fetch_col_w<KPL>(cells + run0 * p.Kp, lane, live, c1);
+            if (run0 + 1 < rend) fold_fetch_col_w<KPL>(cells + (run0 + 1) * p.Kp, lane, live, c2);
             for (uint64_t t0 = sb; t0 < se; t0 += 64) {
                 const int n = (int)((se - t0) < 64 ? (se - t0) : 64);
                 const uint64_t g = t0 + (uint64_t)lane;
@@ -2614,19 +2627,13 @@ __global__ void __launch_bounds__(SEG ? 64 * kMaxFoldSegments : 64) infer_w_kern
                     const int v = readlane_i(wv, i);
                     if ((unsigned)v >= (unsigned)p.V) continue;
                     if (v != vcur) {
-                        if (colp) fold_store_col_w<KPL>(st, colp, lane);
+                        if (colp) fold_store_col_w<KPL>(st, colp, lane, live);
                         ++run;
                         colp = cells + run * p.Kp;
-                        if (v != vnext) load_kvec_f<KPL>(colp, lane, ncol);
-                        fold_load_col_w<KPL>(st, ncol);
+                        fold_load_col_w<KPL>(st, c1);
+                        c1 = c2;
+                        if (run + 2 < rend) fold_fetch_col_w<KPL>(colp + 2 * p.Kp, lane, live, c2);
                         vcur = v;
-                    }
-                    if (i + 1 < n) {
-                        const int v2 = readlane_i(wv, i + 1);
-                        if (v2 != v && (unsigned)v2 < (unsigned)p.V) {
-                            load_kvec_f<KPL>(colp + p.Kp, lane, ncol);
-                            vnext = v2;
-                        }
                     }
                     const int zn = fold_token_w<KPL>(st, lane, readlane_i(zv, i), readlane_f(tw, i), readlane_f(uv, i),
                                                      p.K, nvalid, lastLane, eta, veta);
@@ -2634,7 +2641,7 @@ __global__ void __launch_bounds__(SEG ? 64 * kMaxFoldSegments : 64) infer_w_kern
                 }
                 if (lane < n) p.z[g] = (uint16_t)zkeep;
             }
-            if (colp) fold_store_col_w<KPL>(st, colp, lane);
+            if (colp) fold_store_col_w<KPL>(st, colp, lane, live);
             if (S > 1) {
                 fvec<KPL> a, c;
 #pragma unroll
