@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -18,6 +19,7 @@
 #include <mutex>
 #include <numeric>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "lda_host.h"
@@ -77,6 +79,29 @@ hipError_t launch_fold_buckets(int kpl, bool asym, const lda::InferParams* p_dev
 using lda_host::fail;
 using lda_host::kMaxAlpha;
 using lda_host::kMaxDocTokens;
+
+// f(lo, hi) over [0, n) in contiguous ranges on up to 16 host threads (one
+// range per grain items at least), the calling thread taking the first.
+template <class F>
+static void host_parallel(int64_t n, int64_t grain, const F& f) {
+    const int64_t hc = std::max<int64_t>(1, (int64_t)std::thread::hardware_concurrency());
+    const int64_t nt = std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(hc, 16), (n + grain - 1) / grain));
+    if (nt == 1) {
+        f((int64_t)0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    th.reserve((size_t)nt - 1);
+    for (int64_t t = 1; t < nt; ++t) th.emplace_back([&f, n, nt, t] { f(n * t / nt, n * (t + 1) / nt); });
+    f((int64_t)0, n / nt);
+    for (auto& x : th) x.join();
+}
+
+// Lowers `at` to v if v is smaller (the first failing index over threads).
+static void atomic_min_i64(std::atomic<int64_t>& at, int64_t v) {
+    int64_t cur = at.load();
+    while (v < cur && !at.compare_exchange_weak(cur, v)) {}
+}
 
 namespace {
 
@@ -851,6 +876,7 @@ int lda_ctx_destroy(lda_ctx* c) {
     c->d_prior_row.release(); c->d_prior_eta.release(); c->d_etasum.release(); c->d_etasum_d.release();
     c->g_wt.release(); c->g_docw.release(); c->g_jobrun.release(); c->g_runword.release();
     c->g_cells.release(); c->g_ndk.release(); c->g_nk.release(); c->g_segb.release(); c->g_segr.release();
+    c->g_chunkbase.release(); c->g_chunkrb.release(); c->g_chunkll.release();
     c->d_dglut.release(); c->d_aux.release(); c->d_bad.release(); c->d_hist.release(); c->d_hist_len.release();
     c->d_nkw16.release(); c->r_out.release(); c->r_cnt.release();
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -1710,9 +1736,16 @@ static int check_infer_args(lda_ctx* c, const uint32_t* words, const uint64_t* d
         if (doc_off[d + 1] < doc_off[d]) return fail(LDA_ERR_INVALID_ARGUMENT, "doc_off must be non-decreasing");
         if (doc_off[d + 1] - doc_off[d] > kMaxDocTokens)
             return fail(LDA_ERR_UNSUPPORTED, "document %lld has more than 2^20 tokens", (long long)d);
-        for (uint64_t g = doc_off[d] + 1; g < doc_off[d + 1]; ++g)
-            if (words[g] < words[g - 1]) return fail(LDA_ERR_INVALID_ARGUMENT, "words must be sorted within each document");
     }
+    std::atomic<int64_t> unsorted(INT64_MAX);
+    host_parallel(D, 1024, [&](int64_t lo, int64_t hi) {
+        for (int64_t d = lo; d < hi; ++d)
+            for (uint64_t g = doc_off[d] + 1; g < doc_off[d + 1]; ++g)
+                if (words[g] < words[g - 1]) { atomic_min_i64(unsorted, d); return; }
+    });
+    if (unsorted.load() != INT64_MAX)
+        return fail(LDA_ERR_INVALID_ARGUMENT, "words must be sorted within each document (document %lld)",
+                    (long long)unsorted.load());
     return LDA_OK;
 }
 
@@ -1731,8 +1764,8 @@ static uint64_t seg_start_host(const uint32_t* w, uint64_t n, int i, int S) {
 // of equal word ids with a private column; documents of more than seg_tokens
 // tokens split into S segments (the integer fold-in's rule and options).
 // Mode 0: jobs grouped by S, longest first, one persistent launch per
-// segment count on the fold-in streams, then the finalize kernel; mode 1:
-// one launch per document.  Arguments checked by check_infer_args.
+// segment count on the fold-in streams, each followed on its stream by the
+// finalize kernel of its documents; mode 1: one launch per document.  Arguments checked by check_infer_args.
 static int infer_general(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off, int64_t D, const float* weights,
                          int iterations, uint64_t infer_seed, int mode, float* theta, double* ll, uint16_t* z_out) {
     const bool wt = c->tw != LDA_TW_ONE;
@@ -1759,32 +1792,59 @@ static int infer_general(lda_ctx* c, const uint32_t* words, const uint64_t* doc_
     std::vector<lda::InferJob> jobs(D);
     std::vector<int64_t> job_run(D + 1), seg_r((size_t)D * RW, 0);
     std::vector<uint64_t> seg_b((size_t)D * RW, 0);
-    std::vector<uint32_t> run_word;
-    std::vector<uint64_t> run_tok;  // first token of each run of the current document (relative)
+    // runs of equal valid word ids: counted per job, then written from each
+    // job's first run index (both passes over the jobs on the host threads)
+    const uint32_t Vu = (uint32_t)c->V;
+    host_parallel(D, 256, [&](int64_t lo, int64_t hi) {
+        for (int64_t i = lo; i < hi; ++i) {
+            const uint32_t d = order[i];
+            jobs[i] = lda::InferJob{doc_off[d], doc_off[d + 1], 0, iterations, iterations, (int32_t)d};
+            int64_t nr = 0, prev = -1;
+            for (uint64_t g = doc_off[d]; g < doc_off[d + 1]; ++g) {
+                const uint32_t v = words[g];
+                if (v >= Vu || (int64_t)v == prev) continue;
+                ++nr;
+                prev = v;
+            }
+            job_run[i + 1] = nr;
+        }
+    });
+    job_run[0] = 0;
+    for (int64_t i = 0; i < D; ++i) job_run[i + 1] += job_run[i];
+    const size_t R = (size_t)job_run[D];
+    std::vector<uint32_t> run_word(R);
+    host_parallel(D, 256, [&](int64_t lo, int64_t hi) {
+        std::vector<uint64_t> run_tok;  // first token of each run of the document (relative)
+        for (int64_t i = lo; i < hi; ++i) {
+            const uint32_t d = order[i];
+            const uint64_t b = doc_off[d], n = doc_off[d + 1] - b;
+            run_tok.clear();
+            int64_t r = job_run[i], prev = -1;
+            for (uint64_t g = b; g < b + n; ++g) {
+                const uint32_t v = words[g];
+                if (v >= Vu || (int64_t)v == prev) continue;
+                run_word[(size_t)r++] = v;
+                run_tok.push_back(g - b);
+                prev = v;
+            }
+            const int S = segs[d];
+            for (int w = 0; w <= S; ++w) {
+                const uint64_t x = seg_start_host(words + b, n, w, S);
+                seg_b[(size_t)i * RW + w] = b + x;
+                seg_r[(size_t)i * RW + w] =
+                    job_run[i] + (int64_t)(std::lower_bound(run_tok.begin(), run_tok.end(), x) - run_tok.begin());
+            }
+        }
+    });
+    // the finalize's chunks: at most kGenChunkRuns runs of one document each
+    std::vector<int64_t> chunk_base(D + 1), chunk_rb;
     for (int64_t i = 0; i < D; ++i) {
-        const uint32_t d = order[i];
-        const uint64_t b = doc_off[d], n = doc_off[d + 1] - b;
-        jobs[i] = lda::InferJob{b, doc_off[d + 1], 0, iterations, iterations, (int32_t)d};
-        job_run[i] = (int64_t)run_word.size();
-        run_tok.clear();
-        int64_t prev = -1;
-        for (uint64_t g = b; g < b + n; ++g) {
-            const uint32_t v = words[g];
-            if (v >= (uint32_t)c->V || (int64_t)v == prev) continue;
-            run_word.push_back(v);
-            run_tok.push_back(g - b);
-            prev = v;
-        }
-        const int S = segs[d];
-        for (int w = 0; w <= S; ++w) {
-            const uint64_t x = seg_start_host(words + b, n, w, S);
-            seg_b[(size_t)i * RW + w] = b + x;
-            seg_r[(size_t)i * RW + w] =
-                job_run[i] + (int64_t)(std::lower_bound(run_tok.begin(), run_tok.end(), x) - run_tok.begin());
-        }
+        chunk_base[i] = (int64_t)chunk_rb.size();
+        for (int64_t r = job_run[i]; r < job_run[i + 1]; r += lda::kGenChunkRuns) chunk_rb.push_back(r);
     }
-    job_run[D] = (int64_t)run_word.size();
-    const size_t R = run_word.size();
+    chunk_base[D] = (int64_t)chunk_rb.size();
+    const size_t NC = chunk_rb.size();
+    chunk_rb.push_back(job_run[D]);
     const double cell_bytes = (double)R * c->Kp * 4.0;
     if (cell_bytes > 64e9)
         return fail(LDA_ERR_OUT_OF_MEMORY, "fold-in batch needs %.1f GB of private columns; split the batch", cell_bytes / 1e9);
@@ -1814,17 +1874,23 @@ static int infer_general(lda_ctx* c, const uint32_t* words, const uint64_t* doc_
     HIP_TRY(c->g_ndk.alloc((size_t)D * c->Kp));
     HIP_TRY(c->g_nk.alloc((size_t)D * c->Kp));
     HIP_TRY(c->g_docw.alloc(D));
+    HIP_TRY(c->g_chunkbase.alloc(D + 1));
+    HIP_TRY(c->g_chunkrb.alloc(NC + 1));
+    HIP_TRY(c->g_chunkll.alloc(NC ? NC : 1));
     if (N) HIP_TRY(hipMemcpyAsync(c->i_words.p, words, N * 4, hipMemcpyHostToDevice, s));
     if (wt && N) HIP_TRY(hipMemcpyAsync(c->g_wt.p, weights, N * 4, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(c->i_jobs.p, jobs.data(), D * sizeof(lda::InferJob), hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(c->g_jobrun.p, job_run.data(), (D + 1) * 8, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(c->g_segr.p, seg_r.data(), seg_r.size() * 8, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(c->g_chunkbase.p, chunk_base.data(), (D + 1) * 8, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(c->g_chunkrb.p, chunk_rb.data(), (NC + 1) * 8, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(c->g_segb.p, seg_b.data(), seg_b.size() * 8, hipMemcpyHostToDevice, s));
     if (R) HIP_TRY(hipMemcpyAsync(c->g_runword.p, run_word.data(), R * 4, hipMemcpyHostToDevice, s));
     lda::InferGenParams p{};
     p.words = c->i_words.p; p.z = c->i_z.p; p.wt = wt ? c->g_wt.p : nullptr;
     p.jobs = c->i_jobs.p; p.njobs = D;
     p.job_run = c->g_jobrun.p; p.seg_b = c->g_segb.p; p.seg_r = c->g_segr.p;
+    p.chunk_base = c->g_chunkbase.p; p.chunk_rb = c->g_chunkrb.p; p.chunk_ll = c->g_chunkll.p;
     p.run_word = c->g_runword.p; p.cells = c->g_cells.p;
     p.ndk_out = c->g_ndk.p; p.nk_out = c->g_nk.p; p.docw = c->g_docw.p;
     p.nkw = c->d_nkw.p; p.nk = c->d_nk.p; p.alpha = c->d_alpha.p; p.alpha_cdf = c->d_alpha_cdf.p;
@@ -1855,19 +1921,21 @@ static int infer_general(lda_ctx* c, const uint32_t* words, const uint64_t* doc_
             pb.job_run += lo[bk];
             pb.seg_b += lo[bk] * RW;
             pb.seg_r += lo[bk] * RW;
+            pb.chunk_base += lo[bk];
             pb.njobs = hi[bk] - lo[bk];
             pb.counter = c->i_counter.p + bk;
             const int grid = (int)std::min<int64_t>((int64_t)c->num_cu * per_cu[bk], pb.njobs);
             const hipStream_t sb = bk == 0 ? s : fs.aux[bk - 1];
             if (bk) HIP_TRY(hipStreamWaitEvent(sb, fs.fork, 0));
             HIP_TRY(lda::launch_infer_gen(c->kpl, 1 << bk, pb, grid, false, sb));
+            // the bucket's theta and ll as soon as its documents are done, beside the other buckets' sampling
+            const int64_t nch = chunk_base[hi[bk]] - chunk_base[lo[bk]];
+            const int fgrid = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)c->num_cu * 16, std::max(nch, pb.njobs)));
+            HIP_TRY(lda::launch_infer_gen(c->kpl, 1, pb, fgrid, true, sb));
             if (bk) HIP_TRY(hipEventRecord(fs.join[bk - 1], sb));
         }
         for (int bk = 1; bk < lda::kFoldBuckets; ++bk)
             if (hi[bk] > lo[bk]) HIP_TRY(hipStreamWaitEvent(s, fs.join[bk - 1], 0));
-        p.counter = c->i_counter.p;
-        const int fgrid = (int)std::min<int64_t>((int64_t)c->num_cu * 16, D);
-        HIP_TRY(lda::launch_infer_gen(c->kpl, 1, p, fgrid, true, s));
         c->stat_launches = 1;
     } else {
         p.counter = nullptr;
@@ -1902,9 +1970,16 @@ int lda_infer_w(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off, int6
         return lda_infer(c, words, doc_off, D, iterations, infer_seed, mode, theta, ll, z_out);
     }
     if (!weights && doc_off[D]) return fail(LDA_ERR_INVALID_ARGUMENT, "term-weighted models need the token weights");
-    for (uint64_t g = 0; g < doc_off[D]; ++g)
-        if (words[g] < (uint32_t)c->V && !(weights[g] >= 0.0f && weights[g] < 1e30f))
-            return fail(LDA_ERR_INVALID_ARGUMENT, "token weight %llu is negative or not finite", (unsigned long long)g);
+    std::atomic<int64_t> bad(INT64_MAX);
+    host_parallel((int64_t)doc_off[D], 1 << 16, [&](int64_t lo, int64_t hi) {
+        for (int64_t g = lo; g < hi; ++g)
+            if (words[g] < (uint32_t)c->V && !(weights[g] >= 0.0f && weights[g] < 1e30f)) {
+                atomic_min_i64(bad, g);
+                return;
+            }
+    });
+    if (bad.load() != INT64_MAX)
+        return fail(LDA_ERR_INVALID_ARGUMENT, "token weight %llu is negative or not finite", (unsigned long long)bad.load());
     return infer_general(c, words, doc_off, D, weights, iterations, infer_seed, mode, theta, ll, z_out);
 }
 

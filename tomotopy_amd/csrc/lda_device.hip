@@ -2688,21 +2688,86 @@ __global__ void __launch_bounds__(SEG ? 64 * kMaxFoldSegments : 64) infer_w_kern
 
 // theta = (n_dk + alpha) / (weight sum + sum alpha) and the FP64
 // log-likelihood getLLRest(private) - getLLRest(model) + getLLDocs(doc)
-// (LDAModel.hpp:616-674, 945-946): the n_k terms of the topics whose n_k
-// changed, the n_kw terms of the cells of the document's runs that changed
-// (tomoto skips zero cells; lgamma(0 + eta) - lgamma(eta) is 0 either way),
-// and the document terms.  One wavefront per document.
-template <class C>
-__device__ __forceinline__ double gen_cell_term(C c, double e) {
-    return c != (C)0 ? lgamma((double)c + e) - lgamma(e) : 0.0;
-}
-
+// (LDAModel.hpp:616-674, 945-946) in two launches.  The run pass sums the
+// n_kw terms of the cells that changed, gen_cell_term(private) -
+// gen_cell_term(model) = lgamma(a) - lgamma(b) with a = private + e (e when
+// the private cell is 0: tomoto skips zero cells, and lgamma(0 + e) -
+// lgamma(e) is 0 either way) and b likewise from the model's cell, over
+// chunks of at most kGenChunkRuns runs of one document, one wavefront per
+// chunk: the (a, b) pairs of the changed cells queue in LDS and are taken 64
+// at a time, so every lane evaluates lgamma however few of its own cells
+// changed.  The document pass adds the n_k terms of the topics whose n_k
+// changed, the document terms and the document's chunk sums in chunk
+// order; one wavefront per document.  Both sums are fixed by the document
+// alone, so a document's ll does not depend on the batch around it.
 template <int KPL, bool WT, bool ASYM>
-__global__ void __launch_bounds__(64) infer_gen_finalize_kernel(InferGenParams p) {
+__global__ void __launch_bounds__(64) infer_gen_runs_kernel(InferGenParams p) {
     using C = typename std::conditional<WT, float, int32_t>::type;
+    using CV = typename std::conditional<WT, fvec<KPL>, ivec<KPL>>::type;
+    __shared__ double qa[128], qb[128];
     const int lane = lane_id();
     const C* nkw = reinterpret_cast<const C*>(p.nkw);
     const C* cells = reinterpret_cast<const C*>(p.cells);
+    const int64_t cb = p.counter ? p.chunk_base[0] : p.chunk_base[p.job_base];
+    const int64_t ce = p.counter ? p.chunk_base[p.njobs] : p.chunk_base[p.job_base + 1];
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (int64_t ch = cb + (int64_t)blockIdx.x; ch < ce; ch += (int64_t)gridDim.x) {
+        const int64_t rb = p.chunk_rb[ch], re = p.chunk_rb[ch + 1];
+        double acc = 0.0;
+        int qn = 0;
+        CV m0, m1, n0, n1;  // this run's model and private columns, and the next run's
+        auto fetch = [&](int64_t r, CV& x0, CV& x1) {
+            const uint32_t v = p.run_word[r];
+            if constexpr (WT) {
+                load_kvec_f<KPL>(nkw + (int64_t)v * p.Kp, lane, x0);
+                load_kvec_f<KPL>(cells + r * p.Kp, lane, x1);
+            } else {
+                load_kvec<KPL>(nkw + (int64_t)v * p.Kp, lane, x0);
+                load_kvec<KPL>(cells + r * p.Kp, lane, x1);
+            }
+        };
+        if (rb < re) fetch(rb, n0, n1);
+        for (int64_t r = rb; r < re; ++r) {
+            m0 = n0;
+            m1 = n1;
+            if (r + 1 < re) fetch(r + 1, n0, n1);
+            int pr = -1;
+            if constexpr (ASYM) pr = p.prior_row[p.run_word[r]];
+#pragma unroll
+            for (int j = 0; j < KPL; ++j) {
+                const C c0 = m0[j], c1 = m1[j];
+                const bool d = lane * KPL + j < p.K && c0 != c1;
+                const unsigned long long m = __ballot(d);
+                if (!m) continue;
+                if (d) {
+                    const int q = kpos<KPL>(lane, j);
+                    const double e = ASYM && pr >= 0 ? (double)p.prior_eta[(int64_t)pr * p.Kp + q] : p.eta_d;
+                    const int at = qn + __popcll(m & below);
+                    qa[at] = c1 != (C)0 ? (double)c1 + e : e;
+                    qb[at] = c0 != (C)0 ? (double)c0 + e : e;
+                }
+                qn += __popcll(m);
+                if (qn >= 64) {
+                    __syncthreads();
+                    const int at = qn - 64 + lane;
+                    acc += lgamma(qa[at]) - lgamma(qb[at]);
+                    qn -= 64;
+                    __syncthreads();
+                }
+            }
+        }
+        __syncthreads();
+        if (lane < qn) acc += lgamma(qa[lane]) - lgamma(qb[lane]);
+        acc = wave_sum_d(acc);
+        if (lane == 0) p.chunk_ll[ch] = acc;
+        __syncthreads();  // the queue is read before the next chunk writes it
+    }
+}
+
+template <int KPL, bool WT, bool ASYM>
+__global__ void __launch_bounds__(64) infer_gen_docs_kernel(InferGenParams p) {
+    using C = typename std::conditional<WT, float, int32_t>::type;
+    const int lane = lane_id();
     for (int64_t idx = p.counter ? (int64_t)blockIdx.x : p.job_base; idx < p.njobs;
          idx += p.counter ? (int64_t)gridDim.x : p.njobs) {
         const InferJob J = p.jobs[idx];
@@ -2723,22 +2788,9 @@ __global__ void __launch_bounds__(64) infer_gen_finalize_kernel(InferGenParams p
             const double es = ASYM ? p.etasum_d[q] : p.veta_d;
             if (nk0[q] != nk1[q]) ll += lgamma((double)nk0[q] + es) - lgamma((double)nk1[q] + es);
         }
-        for (int64_t r = p.job_run[idx]; r < p.job_run[idx + 1]; ++r) {
-            const uint32_t v = p.run_word[r];
-            int pr = -1;
-            if constexpr (ASYM) pr = p.prior_row[v];
-#pragma unroll
-            for (int j = 0; j < KPL; ++j) {
-                if (lane * KPL + j >= p.K) continue;
-                const int q = kpos<KPL>(lane, j);
-                const C c0 = nkw[(int64_t)v * p.Kp + q], c1 = cells[r * p.Kp + q];
-                if (c0 == c1) continue;
-                const double e = pr >= 0 ? (double)p.prior_eta[(int64_t)pr * p.Kp + q] : p.eta_d;
-                ll += gen_cell_term(c1, e) - gen_cell_term(c0, e);
-            }
-        }
         ll = wave_sum_d(ll);
         if (lane == 0) {
+            for (int64_t ch = p.chunk_base[idx]; ch < p.chunk_base[idx + 1]; ++ch) ll += p.chunk_ll[ch];
             const double asum = p.asum_d;
             ll -= lgamma((double)p.docw[J.slot] + asum) - lgamma(asum);
             p.ll[J.slot] = ll;
@@ -3187,9 +3239,12 @@ hipError_t launch_infer(int kpl, int segs, bool asym, const InferParams* p_dev, 
 
 template <int KPL, bool WT, bool ASYM>
 static hipError_t launch_infer_gen_k(int segs, const InferGenParams& p, int grid, bool finalize, hipStream_t s) {
-    if (finalize)
-        hipLaunchKernelGGL((infer_gen_finalize_kernel<KPL, WT, ASYM>), dim3(grid), dim3(64), 0, s, p);
-    else if (WT && !ASYM && KPL <= kFoldWMaxKpl && segs == 1)
+    if (finalize) {
+        hipLaunchKernelGGL((infer_gen_runs_kernel<KPL, WT, ASYM>), dim3(grid), dim3(64), 0, s, p);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL((infer_gen_docs_kernel<KPL, WT, ASYM>), dim3(grid), dim3(64), 0, s, p);
+    } else if (WT && !ASYM && KPL <= kFoldWMaxKpl && segs == 1)
         hipLaunchKernelGGL((infer_w_kernel<KPL, false>), dim3(grid), dim3(64), 0, s, p);
     else if (WT && !ASYM && KPL <= kFoldWMaxKpl)
         hipLaunchKernelGGL((infer_w_kernel<KPL, true>), dim3(grid), dim3(64 * segs), gen_w_lds_bytes<KPL>(segs), s, p);

@@ -141,7 +141,8 @@ struct lda_ctx {
     // fold-in of term-weighted models and of models with word priors
     // (InferGenParams): token weights, runs, private columns, outputs
     DevBuf<float> g_wt, g_docw;
-    DevBuf<int64_t> g_jobrun, g_segr;
+    DevBuf<int64_t> g_jobrun, g_segr, g_chunkbase, g_chunkrb;
+    DevBuf<double> g_chunkll;
     DevBuf<uint64_t> g_segb;
     DevBuf<uint32_t> g_runword;
     DevBuf<int32_t> g_cells, g_ndk, g_nk;

@@ -161,9 +161,17 @@ struct InferGenParams {
     double* ll;                  // slots
     double asum_d;
     float asum_f;
+    // the finalize's run pass: chunks of at most kGenChunkRuns runs of one
+    // document; chunk_base (njobs + 1) is each job's first chunk, chunk_rb
+    // (chunks + 1) each chunk's first run, chunk_ll (chunks) its n_kw terms
+    const int64_t* chunk_base;
+    const int64_t* chunk_rb;
+    double* chunk_ll;
 };
+constexpr int64_t kGenChunkRuns = 128;
 // finalize = false: the sampler (grid persistent workgroups of one
-// wavefront); true: theta and the log-likelihood.
+// wavefront); true: theta and the log-likelihood (the run pass over the
+// jobs' chunks, then the document pass, each with grid wavefronts).
 // segs: wavefronts per document (1, 2, 4, 8) of the sampler launch.
 hipError_t launch_infer_gen(int kpl, int segs, const InferGenParams& p, int grid, bool finalize, hipStream_t s);
 int infer_gen_blocks_per_cu(int kpl, bool wt, bool asym, int segs);
