@@ -69,7 +69,7 @@ enum { LDA_TW_ONE = 0, LDA_TW_IDF = 1, LDA_TW_PMI = 2 };
 enum { LDA_PS_DEFAULT = 0, LDA_PS_NONE = 1, LDA_PS_COPY_MERGE = 2, LDA_PS_PARTITION = 3 };
 
 /* Largest K handled by the register-resident sampler (64 lanes x 32 topics). */
-#define LDA_MAX_K 2048
+#define LDA_MAX_K 32767
 
 const char* lda_last_error(void);
 int lda_version(void);
@@ -264,7 +264,7 @@ int lda_infer(lda_ctx* ctx, const uint32_t* words, const uint64_t* doc_off, int6
  * LDA_ERR_INVALID_ARGUMENT.  Every call on one GPU shares that array, which
  * keeps the last call's contents: calls that overlap on one GPU (from
  * several threads or contexts) race and must be serialised by the caller.
- * TermWeight ONE models (with or without word priors). */
+ * TermWeight ONE models (with or without word priors), K <= 2048. */
 int lda_infer_noalloc(lda_ctx* ctx, const uint32_t* words, const uint64_t* doc_off, int64_t D, int iterations,
                       uint64_t infer_seed, float* theta, double* ll, uint16_t* z_out);
 
@@ -328,7 +328,7 @@ int lda_get_infer_options(lda_ctx* ctx, uint32_t* seg_tokens, int* seg_max, int*
  * wavefront)) iterations.  The server takes the context's segmentation
  * (lda_set_infer_options) at start.  Results equal lda_infer(mode 0) with the
  * same infer_seed, bit for bit (also with word priors).  TermWeight IDF / PMI
- * models are not served (LDA_ERR_UNSUPPORTED).
+ * models and K above 2048 are not served (LDA_ERR_UNSUPPORTED).
  * The context must not be trained, re-uploaded or destroyed while a server on
  * it runs.  submit / wait may be called from any thread; lda_serve_stop must
  * not overlap a wait, and drops requests that were not waited for. */

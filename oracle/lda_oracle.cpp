@@ -126,7 +126,9 @@ void prefix_sum_tomoto(float* x, int n) {
 // inclusive value of lane l-1 (wave_shr:1, zero into lane 0) and is added to
 // every lane-local partial.  x has 64*kpl entries (padding entries are 0).
 void prefix_sum_device(float* x, int kpl) {
-    float part[64 * 32];
+    thread_local std::vector<float> part_buf;
+    part_buf.resize((size_t)64 * kpl);
+    float* part = part_buf.data();
     float tot[64];
     for (int l = 0; l < 64; ++l) {
         float s = x[l * kpl];
@@ -387,7 +389,7 @@ void sweep_partition(Model& m, int W, int threads) {
     int T = std::max(1, std::min(threads, W));
     for (int i = 0; i < W; ++i) {
         auto work = [&](int tid) {
-            std::vector<float> buf(64 * 32 + m.K + 64);
+            std::vector<float> buf(64 * device_kpl(m.K) + m.K + 64);
             std::vector<uint32_t> order;
             for (int p = tid; p < W; p += T) {
                 int didx = (i + p) % W;
@@ -427,7 +429,7 @@ void sweep_copy_merge(Model& m, int W, int threads) {
     std::vector<std::vector<int32_t>> nkw_loc(W, m.nkw);
     std::vector<std::vector<int32_t>> nk_loc(W, m.nk);
     auto work = [&](int w) {
-        std::vector<float> buf(64 * 32 + m.K + 64);
+        std::vector<float> buf(64 * device_kpl(m.K) + m.K + 64);
         std::vector<uint32_t> order;
         for (int64_t ch = w; ch < chStride; ch += W) {
             size_t n = (size_t)((m.D + (chStride - 1) - ch) / chStride);
@@ -639,7 +641,7 @@ void sweep_partition_w(Model& m, int W, int threads) {
     int T = std::max(1, std::min(threads, W));
     for (int i = 0; i < W; ++i) {
         auto work = [&](int tid) {
-            std::vector<float> buf(64 * 32 + m.K + 64);
+            std::vector<float> buf(64 * device_kpl(m.K) + m.K + 64);
             std::vector<uint32_t> order;
             for (int p = tid; p < W; p += T) {
                 int didx = (i + p) % W;
@@ -674,7 +676,7 @@ void sweep_copy_merge_w(Model& m, int W, int threads) {
     std::vector<std::vector<float>> nkw_loc(W, m.nkw_f);
     std::vector<std::vector<float>> nk_loc(W, m.nk_f);
     auto work = [&](int w) {
-        std::vector<float> buf(64 * 32 + m.K + 64);
+        std::vector<float> buf(64 * device_kpl(m.K) + m.K + 64);
         std::vector<uint32_t> order;
         for (int64_t ch = w; ch < chStride; ch += W) {
             size_t n = (size_t)((m.D + (chStride - 1) - ch) / chStride);
@@ -833,7 +835,7 @@ void orc_sample_frozen(int K, int scan_order, int kpl, const float* alpha, float
                        int64_t n, const int64_t* doc, const uint32_t* word, const uint16_t* z_old,
                        const float* u, uint16_t* z_new, float* total_out) {
     std::vector<int32_t> a(K), c(K), e(K);
-    std::vector<float> buf(64 * 32 + K + 64);
+    std::vector<float> buf(64 * device_kpl(K) + K + 64);
     for (int64_t t = 0; t < n; ++t) {
         std::copy(ndk + (size_t)doc[t] * K, ndk + (size_t)doc[t] * K + K, a.begin());
         std::copy(nkw + (size_t)word[t] * K, nkw + (size_t)word[t] * K + K, c.begin());
@@ -1125,7 +1127,7 @@ void orc_infer_seg(void* h, const uint32_t* words, const uint64_t* off, int64_t 
     float asum_f = s;
     double asum_d = 0.0;
     for (int k = 0; k < K; ++k) asum_d += (double)m->alpha[k];
-    std::vector<float> buf(64 * 32 + K + 64);
+    std::vector<float> buf(64 * device_kpl(K) + K + 64);
     auto lg = [&](double x) -> double { return mode ? (double)lut_lgamma().get((float)x) : std::lgamma(x); };
     for (int64_t d = 0; d < D; ++d) {
         uint64_t b = off[d], e = off[d + 1];
@@ -1358,7 +1360,7 @@ void orc_infer_gen_seg(void* h, const uint32_t* words, const uint64_t* off, int6
     for (int k = 0; k < K; ++k) { asf = asf + m->alpha[k]; ca[k] = asf; }
     double asd = 0.0;
     for (int k = 0; k < K; ++k) asd += (double)m->alpha[k];
-    std::vector<float> buf(64 * 32 + K + 64);
+    std::vector<float> buf(64 * device_kpl(K) + K + 64);
     const double eta = (double)m->eta, veta = (double)V * eta;
     auto es_of = [&](int k) { return P ? m->etasum_d[k] : veta; };
     auto e_of = [&](uint32_t v, int k) { return P ? (double)m->eta_kv[(size_t)v * K + k] : eta; };

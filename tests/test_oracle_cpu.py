@@ -100,7 +100,7 @@ def test_prefix_sum_tomoto_matches_reference(ref, n):
         assert a.tobytes() == b.tobytes()
 
 
-@pytest.mark.parametrize("kpl", [1, 2, 4, 8, 16, 32])
+@pytest.mark.parametrize("kpl", [1, 2, 4, 8, 16, 32, 64, 128, 512])
 def test_prefix_sum_device_is_a_prefix_sum(kpl):
     rng = np.random.default_rng(kpl)
     x = (rng.random(64 * kpl) * 10.0 ** rng.integers(-3, 3, 64 * kpl)).astype(np.float32)

@@ -21,7 +21,7 @@ LDA_ERR_OUT_OF_MEMORY = -4
 LDA_ERR_UNSUPPORTED = -5
 LDA_ERR_TRAINING = -6
 
-LDA_MAX_K = 2048
+LDA_MAX_K = 32767
 
 _c_p = ctypes.c_void_p
 _i32 = ctypes.c_int

@@ -38,17 +38,21 @@ def hipcc_path():
 def build(verbose=False):
     os.makedirs(OUT_DIR, exist_ok=True)
     hipcc = hipcc_path()
-    srcs = [os.path.join(CSRC, f) for f in ("lda_device.hip", "lda_capi.cpp", "lda_serve.cpp")]
-    objs = []
-    for src in srcs:
-        obj = os.path.join(OUT_DIR, os.path.basename(src) + ".o")
-        cmd = [hipcc] + HIP_FLAGS + ["-c", src, "-o", obj]
-        if verbose:
+    srcs = [os.path.join(CSRC, f) for f in ("lda_device.hip", "lda_bigk.hip", "lda_capi.cpp", "lda_serve.cpp")]
+    objs = [os.path.join(OUT_DIR, os.path.basename(src) + ".o") for src in srcs]
+    cmds = [[hipcc] + HIP_FLAGS + ["-c", src, "-o", obj] for src, obj in zip(srcs, objs)]
+    if verbose:
+        for cmd in cmds:
             print(" ".join(cmd), flush=True)
-        r = subprocess.run(cmd, capture_output=True, text=True)
-        if r.returncode != 0:
-            raise RuntimeError("hipcc failed for %s:\n%s" % (src, r.stderr))
-        objs.append(obj)
+    # the translation units compile side by side
+    procs = [subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True) for cmd in cmds]
+    errs = []
+    for src, pr in zip(srcs, procs):
+        _, err = pr.communicate()
+        if pr.returncode != 0:
+            errs.append("hipcc failed for %s:\n%s" % (src, err))
+    if errs:
+        raise RuntimeError("\n".join(errs))
     out = os.path.join(OUT_DIR, LIB_NAME)
     tmp = out + ".tmp"
     cmd = [hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs

@@ -1,0 +1,757 @@
+// lda_bigk.hip -- gfx950 sampler kernels for K above 2048 topics (up to
+// tomotopy's 32767, LDAModel.hpp:964-984 with Tid = uint16).
+//
+// The K <= 2048 kernels keep each lane's KPL <= 32 topics of every K-vector in
+// registers or LDS.  Beyond that the per-document state no longer fits, so
+// these kernels keep it in device memory and walk each lane's topics in
+// chunks of 16:
+//   * n_dk is the document's row of ndk itself (one wavefront owns a document
+//     while it samples it, in every schedule); the private n_k of a worker is
+//     its row of per-worker counts; the current word's column is the n_kw row
+//     itself where the schedule gives this worker the cells exclusively
+//     (none / partition / copy_merge, fold-in's private run columns) and a
+//     per-wavefront copy where n_kw is shared (the device-wide sweep, whose
+//     cell updates go to n_kw as atomics).
+//   * Every position of a K-vector is read and written by one lane only (the
+//     lane-layout owner of that topic), so a lane's own program order orders
+//     its count updates before its next reads; no cross-lane traffic goes
+//     through memory.
+//   * The lane layout is the K <= 2048 one (topic k = l*KPL + j at
+//     (j/4)*256 + 4l + j%4), so chunk c of every lane is the contiguous block
+//     of positions [1024c, 1024(c+1)) and the host's layout conversions,
+//     read-outs, LL and alpha kernels apply unchanged.
+// The arithmetic is the K <= 2048 sampler's: weights
+// ((n_dk+alpha)(n_kw+eta))/(n_k+V*eta) with the Markstein quotient, each
+// lane's running sum over its topics in order (chunk after chunk), the DPP
+// wave scan of the lane totals, and the count of a lane's cdf values <= u*total.
+// The per-chunk last values of the running sum sit in LDS; the search finds
+// the first chunk whose last cdf value exceeds r and recomputes that chunk's
+// 16 weights (the same operations on the same counts, so the same values) to
+// count inside it.  The result is the K <= 2048 sampler's draw for the same
+// state: the oracle's device-order scan covers every topics-per-lane count.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <type_traits>
+
+#include "lda_internal.h"
+#include "lda_kernels_common.h"
+
+namespace lda {
+namespace {
+
+constexpr int BK_CH = 16;               // topics per lane per chunk
+constexpr int BK_CHUNK = 64 * BK_CH;    // positions per chunk
+constexpr int BK_MAX_CHUNKS = kBigMaxKpl / BK_CH;
+
+template <bool WT>
+using bk_t = typename std::conditional<WT, float, int32_t>::type;
+
+__device__ __forceinline__ void bk_lane_limits(int K, int lane, int kpl, int& nvalid, int& lastLane) {
+    const int nv = K - lane * kpl;
+    nvalid = nv < 0 ? 0 : (nv > kpl ? kpl : nv);
+    lastLane = (K - 1) / kpl;
+}
+
+// eta of the current word and the per-topic eta sums, one chunk at a time
+// (EtaHelper, LDAModel.hpp:81-117).  Sym: eta and V*eta.  Prior: the word's
+// eta column (a row of the sparse prior table, or eta) and sum_v eta_kv.
+struct BkEtaSym {
+    float eta, veta;
+    __device__ __forceinline__ void word(int) {}
+    __device__ __forceinline__ void chunk(int, int, fvec<BK_CH>& e, fvec<BK_CH>& s) const {
+        e = eta;
+        s = veta;
+    }
+};
+
+struct BkEtaPrior {
+    const int32_t* prior_row;
+    const float* prior_eta;
+    const float* etasum;
+    const float* ecol;  // current word's eta column (NULL: eta everywhere)
+    float eta;
+    int Kp;
+    __device__ __forceinline__ void word(int v) {
+        const int r = prior_row[v];
+        ecol = r >= 0 ? prior_eta + (int64_t)r * Kp : nullptr;
+    }
+    __device__ __forceinline__ void chunk(int c, int lane, fvec<BK_CH>& e, fvec<BK_CH>& s) const {
+        if (ecol) load_kvec_f<BK_CH>(ecol + c * BK_CHUNK, lane, e);
+        else e = eta;
+        load_kvec_f<BK_CH>(etasum + c * BK_CHUNK, lane, s);
+    }
+};
+
+template <bool ASYM>
+struct BkEtaOf { using type = BkEtaSym; };
+template <>
+struct BkEtaOf<true> { using type = BkEtaPrior; };
+
+template <bool ASYM>
+__device__ __forceinline__ typename BkEtaOf<ASYM>::type bk_make_eta(const int32_t* prior_row, const float* prior_eta,
+                                                                  const float* etasum, float eta, float veta, int Kp) {
+    if constexpr (ASYM) return BkEtaPrior{prior_row, prior_eta, etasum, nullptr, eta, Kp};
+    else return BkEtaSym{eta, veta};
+}
+
+// The 16 weights of chunk c of this lane (getZLikelihoods, LDAModel.hpp:273-285),
+// in the K <= 2048 sampler's operations: integer counts form A = float(n_dk) +
+// alpha, B = float(n_kw) + eta, b = float(n_k) + V*eta; float counts
+// (n_dk + alpha)(n_kw + eta) and n_k + V*eta; then q = a/b as a*y with one
+// Markstein correction, y = RN(1/b).
+template <bool WT, class E>
+__device__ __forceinline__ void bk_weights(const bk_t<WT>* nd, const bk_t<WT>* col, const bk_t<WT>* nk,
+                                           const float* alpha, int c, int lane, const E& et, fvec<BK_CH>& w) {
+    const int o = c * BK_CHUNK;
+    fvec<BK_CH> al, e, s;
+    load_kvec_f<BK_CH>(alpha + o, lane, al);
+    et.chunk(c, lane, e, s);
+    if constexpr (WT) {
+        fvec<BK_CH> a0, c0, n0;
+        load_kvec_f<BK_CH>(nd + o, lane, a0);
+        load_kvec_f<BK_CH>(col + o, lane, c0);
+        load_kvec_f<BK_CH>(nk + o, lane, n0);
+#pragma unroll
+        for (int j = 0; j < BK_CH; ++j) {
+            const float a = (a0[j] + al[j]) * (c0[j] + e[j]);
+            const float bf = n0[j] + s[j];
+            const float y = recip_rn(bf);
+            const float q0 = a * y;
+            const float rr = __builtin_fmaf(-q0, bf, a);
+            w[j] = __builtin_fmaf(rr, y, q0);
+        }
+    } else {
+        ivec<BK_CH> a0, c0, n0;
+        load_kvec<BK_CH>(nd + o, lane, a0);
+        load_kvec<BK_CH>(col + o, lane, c0);
+        load_kvec<BK_CH>(nk + o, lane, n0);
+#pragma unroll
+        for (int j = 0; j < BK_CH; ++j) {
+            const float A = (float)a0[j] + al[j];
+            const float B = (float)c0[j] + e[j];
+            const float bf = (float)n0[j] + s[j];
+            const float y = recip_rn(bf);
+            const float a = A * B;
+            const float q0 = a * y;
+            const float rr = __builtin_fmaf(-q0, bf, a);
+            w[j] = __builtin_fmaf(rr, y, q0);
+        }
+    }
+}
+
+// addWordTo for topic z (LDAModel.hpp:287-299) by the owner lane: +-1 for
+// integer counts; float counts add w or take max(x - w, 0) (updateCnt,
+// Utils.hpp:132-155).
+template <bool WT>
+__device__ __forceinline__ void bk_bump(bk_t<WT>* nd, bk_t<WT>* col, bk_t<WT>* nk, int lane, int kpl, int ksh, int z,
+                                        float w, bool dec) {
+    if (lane == (z >> ksh)) {
+        const int sl = z & (kpl - 1);
+        const int q = (sl >> 2) * 256 + lane * 4 + (sl & 3);
+        if constexpr (WT) {
+            nd[q] = dec ? sub_clamp(nd[q], w) : nd[q] + w;
+            col[q] = dec ? sub_clamp(col[q], w) : col[q] + w;
+            nk[q] = dec ? sub_clamp(nk[q], w) : nk[q] + w;
+        } else {
+            const int d = dec ? -1 : 1;
+            nd[q] += d;
+            col[q] += d;
+            nk[q] += d;
+        }
+    }
+}
+
+// Running sum of the chunk's weights from `carry` (the lane's earlier chunks).
+__device__ __forceinline__ void bk_prefix(fvec<BK_CH>& w, float carry) {
+    w[0] = carry + w[0];
+#pragma unroll
+    for (int j = 1; j < BK_CH; ++j) w[j] = w[j - 1] + w[j];
+}
+
+// One token: remove zo, weights, scan, search, add the new topic (returned,
+// wave-uniform).  lastc: this wavefront's LDS row of chunk last values
+// ([chunk][lane]).  *total receives cdf[K-1].
+template <bool WT, class E>
+__device__ __forceinline__ int bk_token(bk_t<WT>* nd, bk_t<WT>* col, bk_t<WT>* nk, const float* alpha, int lane,
+                                        int kpl, int ksh, int zo, float tw, float u, int K, int nvalid, int lastLane,
+                                        const E& et, float* lastc, float* total) {
+    bk_bump<WT>(nd, col, nk, lane, kpl, ksh, zo, tw, true);
+    const int C = kpl / BK_CH;
+    float carry = 0.0f;
+    for (int c = 0; c < C; ++c) {
+        fvec<BK_CH> w;
+        bk_weights<WT>(nd, col, nk, alpha, c, lane, et, w);
+        bk_prefix(w, carry);
+        carry = w[BK_CH - 1];
+        lastc[c * 64 + lane] = carry;
+    }
+    const float incl = wave_inclusive_scan(carry);
+    const float excl = dpp_wave_shr1(incl);
+    const float last = excl + carry;
+    const float tot = readlane_f(last, lastLane);
+    const float r = u * tot;
+    // the lane's cdf values are non-decreasing, so the chunks whose last cdf
+    // value is <= r form a prefix; count inside the first chunk beyond it
+    int cs = 0;
+    for (int c = 0; c < C; ++c) cs += (r < excl + lastc[c * 64 + lane]) ? 0 : 1;
+    int miss = kpl;
+    if (cs < C) {
+        fvec<BK_CH> w;
+        bk_weights<WT>(nd, col, nk, alpha, cs, lane, et, w);
+        bk_prefix(w, cs ? lastc[(cs - 1) * 64 + lane] : 0.0f);
+        miss = cs * BK_CH + cdf_count_le<BK_CH>(w, excl, r, excl + w[BK_CH - 1]);
+    }
+    const unsigned long long m = __ballot(miss < nvalid);
+    int zn;
+    if (m == 0ull) zn = K - 1;
+    else {
+        const int L = __builtin_ctzll(m);
+        zn = L * kpl + readlane_i(miss, L);
+    }
+    zn = uniform_i(zn);
+    bk_bump<WT>(nd, col, nk, lane, kpl, ksh, zn, tw, false);
+    if (total) *total = tot;
+    return zn;
+}
+
+// This lane's positions of a K-vector: dst = src.
+template <typename T>
+__device__ __forceinline__ void bk_copy(T* dst, const T* src, int lane, int kpl) {
+    for (int c = 0; c < kpl / BK_CH; ++c) {
+        ivec<BK_CH> x;
+        load_kvec<BK_CH>(reinterpret_cast<const int32_t*>(src) + c * BK_CHUNK, lane, x);
+        store_kvec<BK_CH>(reinterpret_cast<int32_t*>(dst) + c * BK_CHUNK, lane, x);
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void bk_fill(T* dst, int lane, int kpl) {
+    ivec<BK_CH> z = 0;
+    for (int c = 0; c < kpl / BK_CH; ++c) store_kvec<BK_CH>(reinterpret_cast<int32_t*>(dst) + c * BK_CHUNK, lane, z);
+}
+
+// sampleDocument over the token range [b, e) (LDAModel.hpp:326-354) on the
+// state (nd, nk).  MODE_DET: the column is the n_kw row itself (this worker
+// owns its cells); MODE_ATOMIC: colbuf holds this wavefront's copy of the
+// column (read when the word's run starts, plus its own updates) and the
+// token's net change goes to the shared n_kw as atomics.
+template <int MODE, bool WT, class E>
+__device__ __forceinline__ void bk_sample_range(bk_t<WT>* nd, bk_t<WT>* nk, bk_t<WT>* colbuf, const SweepParams& p,
+                                                uint32_t sweep, int lane, uint64_t b, uint64_t e, int kpl, int ksh,
+                                                int nvalid, int lastLane, E& et, float* lastc) {
+    using T = bk_t<WT>;
+    T* nkwT = reinterpret_cast<T*>(p.nkw);
+    int vcur = -1;
+    T* col = nullptr;
+    for (uint64_t t0 = b; t0 < e; t0 += 64) {
+        const int n = (int)((e - t0) < 64 ? (e - t0) : 64);
+        const uint64_t g = t0 + (uint64_t)lane;
+        int wv = -1, zv = 0xFFFF;
+        float uv = 0.0f, tw = 1.0f;
+        if (lane < n) {
+            wv = (int)p.words[g];
+            zv = p.z[g];
+            if constexpr (WT) tw = p.wt[g];
+            uv = uniform24(philox_x0(p.seed_lo, p.seed_hi, (uint32_t)g, (uint32_t)(g >> 32), sweep, TAG_SWEEP));
+        }
+        int zkeep = zv;
+        for (int i = 0; i < n; ++i) {
+            const int v = readlane_i(wv, i);
+            if ((unsigned)v >= (unsigned)p.V) continue;
+            if (v != vcur) {
+                if constexpr (MODE == MODE_DET) {
+                    col = nkwT + (int64_t)v * p.Kp;
+                } else {
+                    bk_copy(colbuf, nkwT + (int64_t)v * p.Kp, lane, kpl);
+                    col = colbuf;
+                }
+                et.word(v);
+                vcur = v;
+            }
+            const int zo = readlane_i(zv, i);
+            const float w = readlane_f(tw, i);
+            const float u = readlane_f(uv, i);
+            const int zn = bk_token<WT>(nd, col, nk, p.alpha, lane, kpl, ksh, zo, w, u, p.K, nvalid, lastLane, et, lastc,
+                                        nullptr);
+            if (zn != zo) {
+                if constexpr (MODE == MODE_ATOMIC) {
+                    T* cp = nkwT + (int64_t)v * p.Kp;
+                    const int qo = kpos_rt(kpl, zo), qn = kpos_rt(kpl, zn);
+                    if constexpr (WT) {
+                        // the merge's clamp max(n_kw, 0) removes rounding-level negatives
+                        if (lane == (zo >> ksh)) unsafeAtomicAdd(cp + qo, -w);
+                        if (lane == (zn >> ksh)) unsafeAtomicAdd(cp + qn, w);
+                    } else {
+                        if (lane == (zo >> ksh)) atomicAdd(cp + qo, -1);
+                        if (lane == (zn >> ksh)) atomicAdd(cp + qn, 1);
+                    }
+                }
+                zkeep = (lane == i) ? zn : zkeep;
+            }
+        }
+        if (lane < n) p.z[g] = (uint16_t)zkeep;
+    }
+}
+
+__device__ __forceinline__ uint64_t bk_prime(uint64_t N, uint32_t sseed) {
+    uint64_t P = 1;
+    if (N) {
+        P = kShufflePrimes[sseed & 0xF];
+        if (N % P == 0) {
+            P = kShufflePrimes[(sseed + 1) & 0xF];
+            if (N % P == 0) P = kShufflePrimes[(sseed + 2) & 0xF];
+            if (N % P == 0) P = kShufflePrimes[(sseed + 3) & 0xF];
+        }
+        P %= N;
+    }
+    return P;
+}
+
+// One round of the partition schedule (LDAModel.hpp:373-399), `none` for
+// W == 1: worker w = blockIdx.x visits documents id*W + (round+w) % W in
+// forShuffled order (Utils.hpp:157-178), their tokens in vocabulary chunk w;
+// private n_k in nk_out row w.
+template <bool WT, bool ASYM>
+__global__ void __launch_bounds__(64) bk_sweep_det_kernel(SweepParams p, int kpl, int ksh) {
+    using T = bk_t<WT>;
+    __shared__ float lastc[BK_MAX_CHUNKS * 64];
+    const int lane = lane_id();
+    const int w = blockIdx.x;
+    int nvalid, lastLane;
+    bk_lane_limits(p.K, lane, kpl, nvalid, lastLane);
+    auto et = bk_make_eta<ASYM>(p.prior_row, p.prior_eta, p.etasum, p.eta, p.veta, p.Kp);
+    T* nkp = reinterpret_cast<T*>(p.nk_out) + (int64_t)w * p.Kp;
+    const int W = p.W;
+    const uint32_t sweep = p.sweep_dev ? (uint32_t)uniform_i((int)*p.sweep_dev) : p.sweep;
+    const int didx = (p.round + w) % W;
+    const uint64_t N = (uint64_t)((p.D + (W - 1) - didx) / W);
+    const uint32_t sseed = philox_x0(p.seed_lo, p.seed_hi, (uint32_t)w, (uint32_t)p.round, sweep, TAG_SHUFFLE);
+    const uint64_t P = bk_prime(N, sseed);
+    for (uint64_t i = 0; i < N; ++i) {
+        const uint64_t id = ((i + (uint64_t)sseed) * P) % N;
+        const int64_t d = (int64_t)id * W + didx;
+        const uint64_t b = p.coff[(int64_t)w * p.D + d], e = p.coff[(int64_t)(w + 1) * p.D + d];
+        if (b == e) continue;
+        T* nd = reinterpret_cast<T*>(p.ndk) + d * p.Kp;
+        bk_sample_range<MODE_DET, WT>(nd, nkp, nullptr, p, sweep, lane, b, e, kpl, ksh, nvalid, lastLane, et, lastc);
+    }
+}
+
+// copy_merge (LDAModel.hpp:401-423): worker w samples on its private n_kw
+// (nkw + w*V*Kp) and n_k (nk_out row w) the tasks c = w, w+W, ... of
+// min(8W, D) strided tasks, each in forShuffled order.
+template <bool WT, bool ASYM>
+__global__ void __launch_bounds__(64) bk_sweep_copy_kernel(SweepParams p, int kpl, int ksh) {
+    using T = bk_t<WT>;
+    __shared__ float lastc[BK_MAX_CHUNKS * 64];
+    const int lane = lane_id();
+    const int w = blockIdx.x;
+    int nvalid, lastLane;
+    bk_lane_limits(p.K, lane, kpl, nvalid, lastLane);
+    auto et = bk_make_eta<ASYM>(p.prior_row, p.prior_eta, p.etasum, p.eta, p.veta, p.Kp);
+    T* nkp = reinterpret_cast<T*>(p.nk_out) + (int64_t)w * p.Kp;
+    SweepParams q = p;
+    q.nkw = p.nkw + (int64_t)w * p.V * p.Kp;
+    const int W = p.W;
+    const int64_t chStride = (int64_t)W * 8 < p.D ? (int64_t)W * 8 : p.D;
+    for (int64_t ch = w; ch < chStride; ch += W) {
+        const uint64_t N = (uint64_t)((p.D + (chStride - 1) - ch) / chStride);
+        const uint32_t sseed = philox_x0(p.seed_lo, p.seed_hi, (uint32_t)ch, 0xC0u, p.sweep, TAG_SHUFFLE);
+        const uint64_t P = bk_prime(N, sseed);
+        for (uint64_t i = 0; i < N; ++i) {
+            const int64_t d = (int64_t)(((i + (uint64_t)sseed) * P) % N) * chStride + ch;
+            const uint64_t b = p.off[d], e = p.off[d + 1];
+            if (b == e) continue;
+            T* nd = reinterpret_cast<T*>(p.ndk) + d * p.Kp;
+            bk_sample_range<MODE_DET, WT>(nd, nkp, nullptr, q, q.sweep, lane, b, e, kpl, ksh, nvalid, lastLane, et,
+                                          lastc);
+        }
+    }
+}
+
+// Device-wide sweep: every wavefront is a worker with a private n_k (the
+// sweep-start snapshot plus its own changes) and a private copy of the
+// current column, in bk_scratch rows 2*blockIdx.x and 2*blockIdx.x + 1;
+// documents come longest first from a global counter; n_kw takes the
+// changes as atomics.  Integer counts add each worker's n_k change to nk_out
+// at the end; float counts get n_k from the merge's row sums.
+template <bool WT, bool ASYM>
+__global__ void __launch_bounds__(64) bk_sweep_device_kernel(SweepParams p, int kpl, int ksh) {
+    using T = bk_t<WT>;
+    __shared__ float lastc[BK_MAX_CHUNKS * 64];
+    const int lane = lane_id();
+    int nvalid, lastLane;
+    bk_lane_limits(p.K, lane, kpl, nvalid, lastLane);
+    auto et = bk_make_eta<ASYM>(p.prior_row, p.prior_eta, p.etasum, p.eta, p.veta, p.Kp);
+    T* nkp = reinterpret_cast<T*>(p.bk_scratch) + (int64_t)blockIdx.x * 2 * p.Kp;
+    T* colbuf = nkp + p.Kp;
+    const T* snap = reinterpret_cast<const T*>(p.nk_in);
+    bk_copy(nkp, snap, lane, kpl);
+    for (;;) {
+        uint32_t idx = 0;
+        if (lane == 0) idx = atomicAdd(p.counter, 1u);
+        idx = (uint32_t)uniform_i((int)idx);
+        if ((int64_t)idx >= p.D) break;
+        const uint32_t d = p.doc_order[idx];
+        T* nd = reinterpret_cast<T*>(p.ndk) + (int64_t)d * p.Kp;
+        bk_sample_range<MODE_ATOMIC, WT>(nd, nkp, colbuf, p, p.sweep, lane, p.off[d], p.off[d + 1], kpl, ksh, nvalid,
+                                         lastLane, et, lastc);
+    }
+    if constexpr (!WT) {
+        for (int c = 0; c < kpl / BK_CH; ++c) {
+            ivec<BK_CH> a, s;
+            load_kvec<BK_CH>(nkp + c * BK_CHUNK, lane, a);
+            load_kvec<BK_CH>(snap + c * BK_CHUNK, lane, s);
+#pragma unroll
+            for (int j = 0; j < BK_CH; ++j) {
+                const int dlt = a[j] - s[j];
+                if (dlt) atomicAdd(p.nk_out + c * BK_CHUNK + (j >> 2) * 256 + lane * 4 + (j & 3), dlt);
+            }
+        }
+    }
+}
+
+// Frozen single steps: for token t the state (n_dk of doc[t], the column of
+// word[t], n_k) is copied into this wavefront's scratch rows, the token is
+// removed, sampled with u[t] and added back.
+__global__ void __launch_bounds__(64) bk_sample_frozen_kernel(const int32_t* __restrict__ ndk,
+                                                              const int32_t* __restrict__ nkw,
+                                                              const int32_t* __restrict__ nk, const float* alpha,
+                                                              float eta, float veta, int K, int Kp, int64_t n,
+                                                              const int64_t* doc, const uint32_t* word,
+                                                              const uint16_t* z_old, const float* u, uint16_t* z_new,
+                                                              float* total, int kpl, int ksh, int32_t* scratch) {
+    __shared__ float lastc[BK_MAX_CHUNKS * 64];
+    const int lane = lane_id();
+    int nvalid, lastLane;
+    bk_lane_limits(K, lane, kpl, nvalid, lastLane);
+    const BkEtaSym et{eta, veta};
+    int32_t* nd = scratch + (int64_t)blockIdx.x * 3 * Kp;
+    int32_t* col = nd + Kp;
+    int32_t* nkc = col + Kp;
+    for (int64_t t = blockIdx.x; t < n; t += gridDim.x) {
+        bk_copy(nd, ndk + doc[t] * Kp, lane, kpl);
+        bk_copy(col, nkw + (int64_t)word[t] * Kp, lane, kpl);
+        bk_copy(nkc, nk, lane, kpl);
+        float tot;
+        const int zn = bk_token<false>(nd, col, nkc, alpha, lane, kpl, ksh, (int)z_old[t], 1.0f, u[t], K, nvalid,
+                                       lastLane, et, lastc, &tot);
+        if (lane == 0) {
+            z_new[t] = (uint16_t)zn;
+            if (total) total[t] = tot;
+        }
+    }
+}
+
+// Fold-in (together=False, LDAModel.hpp:920-953) of one document per
+// wavefront, documents from a counter (longest first) or job job_base alone.
+// The document's private state: n_dk in ndk_out row slot, n_k of the private
+// copy in nk_out row slot, and for each run of equal word ids its private
+// column in `cells` (initialised from the model's column).  Initial topics
+// from Discrete(alpha) (or the word's eta column with priors), added in token
+// order; then `iters` sequential sweeps.  Integer counts reproduce the
+// integer fold-in (n_kw + c_d per run); float counts follow tomoto's clamped
+// updates in token order.
+template <bool WT, bool ASYM>
+__global__ void __launch_bounds__(64) bk_infer_gen_kernel(InferGenParams p, int kpl, int ksh) {
+    using T = bk_t<WT>;
+    __shared__ float lastc[BK_MAX_CHUNKS * 64];
+    const int lane = lane_id();
+    int nvalid, lastLane;
+    bk_lane_limits(p.K, lane, kpl, nvalid, lastLane);
+    auto et = bk_make_eta<ASYM>(p.prior_row, p.prior_eta, p.etasum, p.eta, p.veta, p.Kp);
+    const T* nkwT = reinterpret_cast<const T*>(p.nkw);
+    T* cells = reinterpret_cast<T*>(p.cells);
+    for (;;) {
+        int64_t idx;
+        if (!p.counter) {
+            idx = p.job_base;
+        } else {
+            uint32_t t = 0;
+            if (lane == 0) t = atomicAdd(p.counter, 1u);
+            idx = (int64_t)(uint32_t)uniform_i((int)t);
+        }
+        if (idx >= p.njobs) break;
+        const InferJob J = p.jobs[idx];
+        const uint64_t b = J.b, e = J.e;
+        T* nd = reinterpret_cast<T*>(p.ndk_out) + (int64_t)J.slot * p.Kp;
+        T* nk = reinterpret_cast<T*>(p.nk_out) + (int64_t)J.slot * p.Kp;
+        bk_fill(nd, lane, kpl);
+        bk_copy(nk, reinterpret_cast<const T*>(p.nk), lane, kpl);
+        // initial topics, added in token order; each run's column starts as the model's
+        {
+            int64_t run = p.job_run[idx] - 1;
+            float sw = 0.0f;
+            int vcur = -1;
+            T* col = nullptr;
+            for (uint64_t t0 = b; t0 < e; t0 += 64) {
+                const int n = (int)((e - t0) < 64 ? (e - t0) : 64);
+                const uint64_t g = t0 + (uint64_t)lane;
+                int wv = -1, zz = 0xFFFF;
+                float tw = 1.0f;
+                if (lane < n) {
+                    wv = (int)p.words[g];
+                    if constexpr (WT) tw = p.wt[g];
+                    if ((unsigned)wv < (unsigned)p.V) {
+                        const float u = uniform24(philox_x0(p.seed_lo, p.seed_hi, (uint32_t)(g - b),
+                                                            (uint32_t)((g - b) >> 32), 0u, TAG_INFINIT));
+                        if constexpr (ASYM) {
+                            const int r = p.prior_row[wv];
+                            zz = draw_from_eta_column(r >= 0 ? p.prior_eta + (int64_t)r * p.Kp : nullptr, kpl, p.eta,
+                                                      p.K, u);
+                        } else {
+                            const float rr = u * p.alpha_cdf[p.K - 1];
+                            int lo = 0, hi = p.K;
+                            while (lo < hi) {
+                                const int mid = (lo + hi) >> 1;
+                                if (rr < p.alpha_cdf[mid]) hi = mid;
+                                else lo = mid + 1;
+                            }
+                            zz = lo < p.K ? lo : p.K - 1;
+                        }
+                    }
+                    p.z[g] = (uint16_t)zz;
+                }
+                for (int i = 0; i < n; ++i) {
+                    const int v = readlane_i(wv, i);
+                    if ((unsigned)v >= (unsigned)p.V) continue;
+                    if (v != vcur) {
+                        ++run;
+                        col = cells + run * p.Kp;
+                        bk_copy(col, nkwT + (int64_t)v * p.Kp, lane, kpl);
+                        vcur = v;
+                    }
+                    const int k = readlane_i(zz, i);
+                    const float w = readlane_f(tw, i);
+                    bk_bump<WT>(nd, col, nk, lane, kpl, ksh, k, w, false);
+                    sw += WT ? w : 1.0f;
+                }
+            }
+            if (lane == 0) p.docw[J.slot] = sw;
+        }
+        for (int it = 1; it <= J.iters; ++it) {
+            int64_t run = p.job_run[idx] - 1;
+            int vcur = -1;
+            T* col = nullptr;
+            for (uint64_t t0 = b; t0 < e; t0 += 64) {
+                const int n = (int)((e - t0) < 64 ? (e - t0) : 64);
+                const uint64_t g = t0 + (uint64_t)lane;
+                int wv = -1, zv = 0xFFFF;
+                float uv = 0.0f, tw = 1.0f;
+                if (lane < n) {
+                    wv = (int)p.words[g];
+                    zv = p.z[g];
+                    if constexpr (WT) tw = p.wt[g];
+                    uv = uniform24(philox_x0(p.seed_lo, p.seed_hi, (uint32_t)(g - b), (uint32_t)((g - b) >> 32),
+                                             (uint32_t)it, TAG_INFER));
+                }
+                int zkeep = zv;
+                for (int i = 0; i < n; ++i) {
+                    const int v = readlane_i(wv, i);
+                    if ((unsigned)v >= (unsigned)p.V) continue;
+                    if (v != vcur) {
+                        ++run;
+                        col = cells + run * p.Kp;
+                        et.word(v);
+                        vcur = v;
+                    }
+                    const int zo = readlane_i(zv, i);
+                    const int zn = bk_token<WT>(nd, col, nk, p.alpha, lane, kpl, ksh, zo, readlane_f(tw, i),
+                                                readlane_f(uv, i), p.K, nvalid, lastLane, et, lastc, nullptr);
+                    zkeep = (lane == i) ? zn : zkeep;
+                }
+                if (lane < n) p.z[g] = (uint16_t)zkeep;
+            }
+        }
+        if (!p.counter) break;
+    }
+}
+
+// The fold-in's log-likelihood and theta (LDAModel.hpp:931-945), in two
+// passes as for K <= 2048: chunks of at most kGenChunkRuns runs add
+// lgamma(cell + eta) - lgamma(model cell + eta) over the cells that changed
+// (queued 64 at a time in LDS so every lane evaluates one pair), then one
+// wavefront per document adds the n_dk and n_k terms and its chunk sums in
+// chunk order.
+template <bool WT, bool ASYM>
+__global__ void __launch_bounds__(64) bk_infer_gen_runs_kernel(InferGenParams p, int kpl) {
+    using T = bk_t<WT>;
+    using CV = typename std::conditional<WT, fvec<BK_CH>, ivec<BK_CH>>::type;
+    __shared__ double qa[128], qb[128];
+    const int lane = lane_id();
+    const T* nkw = reinterpret_cast<const T*>(p.nkw);
+    const T* cells = reinterpret_cast<const T*>(p.cells);
+    const int64_t cb = p.counter ? p.chunk_base[0] : p.chunk_base[p.job_base];
+    const int64_t ce = p.counter ? p.chunk_base[p.njobs] : p.chunk_base[p.job_base + 1];
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (int64_t ch = cb + (int64_t)blockIdx.x; ch < ce; ch += (int64_t)gridDim.x) {
+        const int64_t rb = p.chunk_rb[ch], re = p.chunk_rb[ch + 1];
+        double acc = 0.0;
+        int qn = 0;
+        for (int64_t r = rb; r < re; ++r) {
+            const uint32_t v = p.run_word[r];
+            int pr = -1;
+            if constexpr (ASYM) pr = p.prior_row[v];
+            for (int c = 0; c < kpl / BK_CH; ++c) {
+                CV m0, m1;
+                if constexpr (WT) {
+                    load_kvec_f<BK_CH>(nkw + (int64_t)v * p.Kp + c * BK_CHUNK, lane, m0);
+                    load_kvec_f<BK_CH>(cells + r * p.Kp + c * BK_CHUNK, lane, m1);
+                } else {
+                    load_kvec<BK_CH>(nkw + (int64_t)v * p.Kp + c * BK_CHUNK, lane, m0);
+                    load_kvec<BK_CH>(cells + r * p.Kp + c * BK_CHUNK, lane, m1);
+                }
+#pragma unroll
+                for (int j = 0; j < BK_CH; ++j) {
+                    const T c0 = m0[j], c1 = m1[j];
+                    const bool d = lane * kpl + c * BK_CH + j < p.K && c0 != c1;
+                    const unsigned long long m = __ballot(d);
+                    if (!m) continue;
+                    if (d) {
+                        const int q = c * BK_CHUNK + (j >> 2) * 256 + lane * 4 + (j & 3);
+                        const double e = ASYM && pr >= 0 ? (double)p.prior_eta[(int64_t)pr * p.Kp + q] : p.eta_d;
+                        const int at = qn + __popcll(m & below);
+                        qa[at] = c1 != (T)0 ? (double)c1 + e : e;
+                        qb[at] = c0 != (T)0 ? (double)c0 + e : e;
+                    }
+                    qn += __popcll(m);
+                    if (qn >= 64) {
+                        __syncthreads();
+                        const int at = qn - 64 + lane;
+                        acc += lgamma(qa[at]) - lgamma(qb[at]);
+                        qn -= 64;
+                        __syncthreads();
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        if (lane < qn) acc += lgamma(qa[lane]) - lgamma(qb[lane]);
+        acc = wave_sum_d(acc);
+        if (lane == 0) p.chunk_ll[ch] = acc;
+        __syncthreads();
+    }
+}
+
+template <bool WT, bool ASYM>
+__global__ void __launch_bounds__(64) bk_infer_gen_docs_kernel(InferGenParams p, int kpl) {
+    using T = bk_t<WT>;
+    const int lane = lane_id();
+    for (int64_t idx = p.counter ? (int64_t)blockIdx.x : p.job_base; idx < p.njobs;
+         idx += p.counter ? (int64_t)gridDim.x : p.njobs) {
+        const InferJob J = p.jobs[idx];
+        const T* nd = reinterpret_cast<const T*>(p.ndk_out) + (int64_t)J.slot * p.Kp;
+        const T* nk1 = reinterpret_cast<const T*>(p.nk_out) + (int64_t)J.slot * p.Kp;
+        const T* nk0 = reinterpret_cast<const T*>(p.nk);
+        const float den = p.docw[J.slot] + p.asum_f;
+        double ll = 0.0;
+        for (int j = 0; j < kpl; ++j) {
+            const int k = lane * kpl + j;
+            if (k >= p.K) break;
+            const int q = (j >> 2) * 256 + lane * 4 + (j & 3);
+            const float a = p.alpha[q];
+            const T x = nd[q];
+            p.theta[(int64_t)J.slot * p.K + k] = ((float)x + a) / den;
+            if (x != (T)0) ll += lgamma((double)x + (double)a) - lgamma((double)a);
+            const double es = ASYM ? p.etasum_d[q] : p.veta_d;
+            if (nk0[q] != nk1[q]) ll += lgamma((double)nk0[q] + es) - lgamma((double)nk1[q] + es);
+        }
+        ll = wave_sum_d(ll);
+        if (lane == 0) {
+            for (int64_t ch = p.chunk_base[idx]; ch < p.chunk_base[idx + 1]; ++ch) ll += p.chunk_ll[ch];
+            const double asum = p.asum_d;
+            ll -= lgamma((double)p.docw[J.slot] + asum) - lgamma(asum);
+            p.ll[J.slot] = ll;
+        }
+    }
+}
+
+int bk_shift(int kpl) {
+    int s = 0;
+    while ((1 << s) < kpl) ++s;
+    return s;
+}
+
+bool bk_kpl_ok(int kpl) { return kpl > kSmallMaxKpl && kpl <= kBigMaxKpl && (kpl & (kpl - 1)) == 0; }
+
+#define LDA_BK_VARIANTS(WT, ASYM, CALL)                                   \
+    if (WT) {                                                             \
+        if (ASYM) { constexpr bool W_ = true, A_ = true; CALL; }          \
+        else { constexpr bool W_ = true, A_ = false; CALL; }              \
+    } else {                                                              \
+        if (ASYM) { constexpr bool W_ = false, A_ = true; CALL; }         \
+        else { constexpr bool W_ = false, A_ = false; CALL; }             \
+    }
+
+}  // namespace
+
+// ================================================================ launchers
+hipError_t launch_bk_sweep_det(int kpl, const SweepParams& p, bool weighted, hipStream_t s) {
+    if (!bk_kpl_ok(kpl)) return hipErrorInvalidValue;
+    const int sh = bk_shift(kpl);
+    LDA_BK_VARIANTS(weighted, p.prior_row != nullptr,
+                    hipLaunchKernelGGL((bk_sweep_det_kernel<W_, A_>), dim3(p.W), dim3(64), 0, s, p, kpl, sh));
+    return hipGetLastError();
+}
+
+hipError_t launch_bk_sweep_copy(int kpl, const SweepParams& p, bool weighted, hipStream_t s) {
+    if (!bk_kpl_ok(kpl)) return hipErrorInvalidValue;
+    const int sh = bk_shift(kpl);
+    LDA_BK_VARIANTS(weighted, p.prior_row != nullptr,
+                    hipLaunchKernelGGL((bk_sweep_copy_kernel<W_, A_>), dim3(p.W), dim3(64), 0, s, p, kpl, sh));
+    return hipGetLastError();
+}
+
+hipError_t launch_bk_sweep_device(int kpl, const SweepParams& p, bool weighted, int grid, hipStream_t s) {
+    if (!bk_kpl_ok(kpl) || !p.bk_scratch || grid <= 0) return hipErrorInvalidValue;
+    const int sh = bk_shift(kpl);
+    LDA_BK_VARIANTS(weighted, p.prior_row != nullptr,
+                    hipLaunchKernelGGL((bk_sweep_device_kernel<W_, A_>), dim3(grid), dim3(64), 0, s, p, kpl, sh));
+    return hipGetLastError();
+}
+
+int bk_sweep_blocks_per_cu(bool weighted, bool asym, bool device) {
+    int n = 0;
+    hipError_t e = hipSuccess;
+    LDA_BK_VARIANTS(weighted, asym,
+                    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                        &n, device ? (const void*)bk_sweep_device_kernel<W_, A_> : (const void*)bk_sweep_det_kernel<W_, A_>,
+                        64, 0));
+    return e == hipSuccess ? n : 0;
+}
+
+hipError_t launch_bk_sample_frozen(int kpl, const int32_t* ndk, const int32_t* nkw, const int32_t* nk,
+                                   const float* alpha, float eta, float veta, int K, int Kp, int64_t n, int grid,
+                                   const int64_t* doc, const uint32_t* word, const uint16_t* z_old, const float* u,
+                                   uint16_t* z_new, float* total, int32_t* scratch, hipStream_t s) {
+    if (!bk_kpl_ok(kpl) || !scratch || grid <= 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(bk_sample_frozen_kernel, dim3(grid), dim3(64), 0, s, ndk, nkw, nk, alpha, eta, veta, K, Kp, n,
+                       doc, word, z_old, u, z_new, total, kpl, bk_shift(kpl), scratch);
+    return hipGetLastError();
+}
+
+hipError_t launch_bk_infer_gen(int kpl, const InferGenParams& p, int grid, bool finalize, hipStream_t s) {
+    if (!bk_kpl_ok(kpl) || grid <= 0) return hipErrorInvalidValue;
+    const bool wt = p.wt != nullptr, asym = p.prior_row != nullptr;
+    if (finalize) {
+        LDA_BK_VARIANTS(wt, asym, {
+            hipLaunchKernelGGL((bk_infer_gen_runs_kernel<W_, A_>), dim3(grid), dim3(64), 0, s, p, kpl);
+            hipLaunchKernelGGL((bk_infer_gen_docs_kernel<W_, A_>), dim3(grid), dim3(64), 0, s, p, kpl);
+        });
+    } else {
+        const int sh = bk_shift(kpl);
+        LDA_BK_VARIANTS(wt, asym,
+                        hipLaunchKernelGGL((bk_infer_gen_kernel<W_, A_>), dim3(grid), dim3(64), 0, s, p, kpl, sh));
+    }
+    return hipGetLastError();
+}
+
+int bk_infer_gen_blocks_per_cu(bool wt, bool asym) {
+    int n = 0;
+    hipError_t e = hipSuccess;
+    LDA_BK_VARIANTS(wt, asym, e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, bk_infer_gen_kernel<W_, A_>, 64, 0));
+    return e == hipSuccess ? n : 0;
+}
+
+}  // namespace lda

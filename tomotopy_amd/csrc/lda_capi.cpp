@@ -288,7 +288,9 @@ lda::SweepParams base_params(lda_ctx* c) {
 int narrow_begin(lda_ctx* c, int W, bool copy_merge) {
     c->narrow_active = false;
     c->narrow_used = false;
-    if (!c->narrow_enabled || W != 0 || copy_merge || c->tw != LDA_TW_ONE || c->nkw_hot >= c->V) return LDA_OK;
+    if (!c->narrow_enabled || W != 0 || copy_merge || c->tw != LDA_TW_ONE || c->nkw_hot >= c->V ||
+        c->kpl > lda::kSmallMaxKpl)
+        return LDA_OK;
     const hipStream_t s = c->stream;
     HIP_TRY(c->d_nkw16.alloc((size_t)(c->V - c->nkw_hot) * c->Kp));
     HIP_TRY(c->d_bad.alloc(1));
@@ -358,13 +360,19 @@ int device_wide_grid(lda_ctx* c, int* waves_out) {
                    : lda::sweep_device_blocks_per_cu(c->kpl, c->has_priors, waves);
     };
     if (!c->dev_auto) {
-        int grid = c->num_cu * per_cu(c->dev_waves);
+        const int waves = c->kpl > lda::kSmallMaxKpl ? 1 : c->dev_waves;
+        int grid = c->num_cu * per_cu(waves);
         if (c->dev_workers > 0) grid = std::min(grid, c->dev_workers);
-        *waves_out = c->dev_waves;
+        *waves_out = waves;
         return grid;
     }
-    const int W8 = lda::kMaxSweepWaves;
     const int64_t inflight = std::max<int64_t>((c->D + kInFlightDiv - 1) / kInFlightDiv, 1);
+    if (c->kpl > lda::kSmallMaxKpl) {
+        // K above 2048: one-wavefront workers only
+        *waves_out = 1;
+        return (int)std::min<int64_t>(inflight, (int64_t)c->num_cu * per_cu(1));
+    }
+    const int W8 = lda::kMaxSweepWaves;
     const int full8 = c->num_cu * per_cu(W8);
     if (inflight >= (int64_t)full8 * W8) {
         *waves_out = W8;
@@ -494,6 +502,10 @@ int sweep_once(lda_ctx* c, int W, bool copy_merge) {
             p.counter = c->d_counter.p;
             int waves = 1;
             const int grid = device_wide_grid(c, &waves);
+            if (c->kpl > lda::kSmallMaxKpl) {
+                HIP_TRY(c->d_bk_scratch.alloc((size_t)grid * 2 * c->Kp));
+                p.bk_scratch = c->d_bk_scratch.p;
+            }
             HIP_TRY(hipEventRecord(c->ev0, s));
             HIP_TRY(lda::launch_sweep_device_w(c->kpl, p, grid, waves, s));
             HIP_TRY(hipEventRecord(c->ev1, s));
@@ -555,6 +567,10 @@ int sweep_once(lda_ctx* c, int W, bool copy_merge) {
         p.counter = c->d_counter.p;
         int waves = 1;
         const int grid = device_wide_grid(c, &waves);
+        if (c->kpl > lda::kSmallMaxKpl) {
+            HIP_TRY(c->d_bk_scratch.alloc((size_t)grid * 2 * c->Kp));
+            p.bk_scratch = c->d_bk_scratch.p;
+        }
         HIP_TRY(hipEventRecord(c->ev0, s));
         HIP_TRY(lda::launch_sweep_device(c->kpl, p, grid, waves, s));
         HIP_TRY(hipEventRecord(c->ev1, s));
@@ -866,6 +882,7 @@ int lda_ctx_destroy(lda_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     c->d_alpha.release(); c->d_alpha_cdf.release(); c->d_nkw.release(); c->d_nk.release(); c->d_nk_next.release();
     c->d_ndk.release(); c->d_doclen.release(); c->d_nk_priv.release(); c->d_nkw_loc.release(); c->d_words.release(); c->d_order.release();
+    c->d_bk_scratch.release();
     c->d_counter.release(); c->d_z.release(); c->d_off.release(); c->d_coff.release(); c->d_part.release();
     if (c->infer_exec) (void)hipGraphExecDestroy(c->infer_exec);
     if (c->part_exec) (void)hipGraphExecDestroy(c->part_exec);
@@ -1702,20 +1719,25 @@ int lda_sample_frozen(lda_ctx* c, int64_t n, const int64_t* doc, const uint32_t*
     DevBuf<uint32_t> dw;
     DevBuf<uint16_t> dz, dzn;
     DevBuf<float> du, dt;
+    DevBuf<int32_t> dsc;
     int rc = LDA_OK;
-    if (dd.alloc(n) || dw.alloc(n) || dz.alloc(n) || dzn.alloc(n) || du.alloc(n) || dt.alloc(n)) rc = fail(LDA_ERR_OUT_OF_MEMORY, "allocation failed");
+    // K above 2048: three rows of Kp counts per workgroup
+    const int bk_grid = (int)std::min<int64_t>(n, 1024);
+    if (dd.alloc(n) || dw.alloc(n) || dz.alloc(n) || dzn.alloc(n) || du.alloc(n) || dt.alloc(n) ||
+        (c->kpl > lda::kSmallMaxKpl && dsc.alloc((size_t)bk_grid * 3 * c->Kp)))
+        rc = fail(LDA_ERR_OUT_OF_MEMORY, "allocation failed");
     if (!rc && (hipMemcpyAsync(dd.p, doc, n * 8, hipMemcpyHostToDevice, s) || hipMemcpyAsync(dw.p, word, n * 4, hipMemcpyHostToDevice, s) ||
                 hipMemcpyAsync(dz.p, z_old, n * 2, hipMemcpyHostToDevice, s) || hipMemcpyAsync(du.p, u, n * 4, hipMemcpyHostToDevice, s)))
         rc = fail(LDA_ERR_DEVICE, "upload failed");
     if (!rc) {
         hipError_t e = lda::launch_sample_frozen(c->kpl, c->d_ndk.p, c->d_nkw.p, c->d_nk.p, c->d_alpha.p, c->eta, c->veta, c->K,
-                                                 c->V, c->Kp, n, dd.p, dw.p, dz.p, du.p, dzn.p, dt.p, s);
+                                                 c->V, c->Kp, n, dd.p, dw.p, dz.p, du.p, dzn.p, dt.p, dsc.p, bk_grid, s);
         if (e != hipSuccess) rc = fail(LDA_ERR_DEVICE, "sample_frozen: %s", hipGetErrorString(e));
     }
     if (!rc && (hipMemcpyAsync(z_new, dzn.p, n * 2, hipMemcpyDeviceToHost, s) ||
                 (total && hipMemcpyAsync(total, dt.p, n * 4, hipMemcpyDeviceToHost, s)) || hipStreamSynchronize(s)))
         rc = fail(LDA_ERR_DEVICE, "download failed");
-    dd.release(); dw.release(); dz.release(); dzn.release(); du.release(); dt.release();
+    dd.release(); dw.release(); dz.release(); dzn.release(); du.release(); dt.release(); dsc.release();
     return rc;
 }
 
@@ -1990,6 +2012,10 @@ int lda_infer(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off, int64_
     if (rc || done) return rc;
     if (c->tw != LDA_TW_ONE)
         return fail(LDA_ERR_INVALID_ARGUMENT, "term-weighted models need the token weights: call lda_infer_w");
+    // K above 2048: the run-column fold-in (one wavefront per document), the
+    // same draws as the register-resident sampler for integer counts
+    if (c->kpl > lda::kSmallMaxKpl)
+        return infer_general(c, words, doc_off, D, nullptr, iterations, infer_seed, mode, theta, ll, z_out);
     const uint64_t N = doc_off[D];
     HIP_TRY(hipSetDevice(c->device));
     const hipStream_t s = c->stream;
@@ -2059,6 +2085,8 @@ int lda_infer_noalloc(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off
     if (rc || done) return rc;
     if (c->tw != LDA_TW_ONE)
         return fail(LDA_ERR_INVALID_ARGUMENT, "term-weighted models need the token weights: call lda_infer_w");
+    if (c->kpl > lda::kSmallMaxKpl)
+        return fail(LDA_ERR_UNSUPPORTED, "lda_infer_noalloc takes K <= 2048 (K = %d: use lda_infer)", c->K);
     const uint64_t N = doc_off[D];
     // the arena inside the scratch array, each region 256-byte aligned
     const size_t o_params = 0;

@@ -110,6 +110,8 @@ struct lda_ctx {
 
     DevBuf<float> d_alpha, d_alpha_cdf;
     DevBuf<int32_t> d_nkw, d_nk, d_nk_next, d_ndk, d_doclen, d_nk_priv, d_nkw_loc;
+    // K above 2048: the device-wide sweep's per-worker rows (private n_k and column copy)
+    DevBuf<int32_t> d_bk_scratch;
     DevBuf<uint32_t> d_words, d_order, d_counter;
     DevBuf<uint16_t> d_z;
     DevBuf<uint64_t> d_off, d_coff;

@@ -56,7 +56,16 @@ struct SweepParams {
     // (same lane layout); their int32 columns in nkw are stale meanwhile
     uint16_t* nkw16;
     int nkw_hot;
+    // K above 2048 (kpl > kSmallMaxKpl), device-wide sweep: per worker two
+    // rows of Kp counts (its private n_k and its copy of the current column)
+    void* bk_scratch;
 };
+
+// Topics per lane: the register / LDS kernels take kpl <= kSmallMaxKpl
+// (K <= 2048); lda_bigk.hip takes kSmallMaxKpl < kpl <= kBigMaxKpl
+// (K <= 32768, tomotopy's limit being 32767).
+constexpr int kSmallMaxKpl = 32;
+constexpr int kBigMaxKpl = 512;
 
 // Words with fewer than kNarrowCf tokens have every n_kw cell below 2^15
 // while the counts are consistent, and a sweep can raise a cell by at most
@@ -273,10 +282,11 @@ hipError_t launch_invariants(int kpl, const uint32_t* words, const uint16_t* z, 
                              long long* colsum, unsigned long long* flags, hipStream_t s);
 
 // frozen-state single steps
+// bk_scratch: 3 * Kp counts per workgroup of bk_grid (kpl > kSmallMaxKpl only)
 hipError_t launch_sample_frozen(int kpl, const int32_t* ndk, const int32_t* nkw, const int32_t* nk,
                                 const float* alpha, float eta, float veta, int K, int V, int Kp, int64_t n,
                                 const int64_t* doc, const uint32_t* word, const uint16_t* z_old, const float* u,
-                                uint16_t* z_new, float* total, hipStream_t s);
+                                uint16_t* z_new, float* total, int32_t* bk_scratch, int bk_grid, hipStream_t s);
 
 // fold-in; the kernel reads its parameters from device memory so that one
 // captured graph serves every batch
@@ -287,5 +297,20 @@ hipError_t launch_infer(int kpl, int segs, bool asym, const InferParams* p_dev, 
 hipError_t launch_infer_finalize(int kpl, bool asym, const InferParams* p_dev, int grid, hipStream_t s);
 // resident workgroups per CU of the fold-in kernel (0: this segment count does not fit)
 int infer_blocks_per_cu(int kpl, int segs, bool asym);
+
+// K above 2048 (lda_bigk.hip): one wavefront per worker or document, state in
+// device memory.  Device-wide: grid workers, p.bk_scratch holds 2 * Kp counts
+// per worker.  Frozen steps: scratch holds 3 * Kp counts per workgroup of grid.
+// Fold-in: one wavefront per document, no segmentation.
+hipError_t launch_bk_sweep_det(int kpl, const SweepParams& p, bool weighted, hipStream_t s);
+hipError_t launch_bk_sweep_copy(int kpl, const SweepParams& p, bool weighted, hipStream_t s);
+hipError_t launch_bk_sweep_device(int kpl, const SweepParams& p, bool weighted, int grid, hipStream_t s);
+int bk_sweep_blocks_per_cu(bool weighted, bool asym, bool device);
+hipError_t launch_bk_sample_frozen(int kpl, const int32_t* ndk, const int32_t* nkw, const int32_t* nk,
+                                   const float* alpha, float eta, float veta, int K, int Kp, int64_t n, int grid,
+                                   const int64_t* doc, const uint32_t* word, const uint16_t* z_old, const float* u,
+                                   uint16_t* z_new, float* total, int32_t* scratch, hipStream_t s);
+hipError_t launch_bk_infer_gen(int kpl, const InferGenParams& p, int grid, bool finalize, hipStream_t s);
+int bk_infer_gen_blocks_per_cu(bool wt, bool asym);
 
 }  // namespace lda

@@ -411,7 +411,10 @@ int lda_serve_start(lda_ctx* ctx, uint64_t hbm_budget_bytes, int iterations, int
                     int lanes, uint64_t infer_seed, lda_server** out) {
     if (!ctx || !out) return fail(LDA_ERR_INVALID_ARGUMENT, "NULL argument");
     *out = nullptr;
-    if (ctx->tw != LDA_TW_ONE) return fail(LDA_ERR_UNSUPPORTED, "fold-in of term-weighted models is not on the device yet");
+    if (ctx->tw != LDA_TW_ONE)
+        return fail(LDA_ERR_UNSUPPORTED, "the server takes TermWeight ONE models; fold IDF / PMI models in with lda_infer_w");
+    if (ctx->kpl > lda::kSmallMaxKpl)
+        return fail(LDA_ERR_UNSUPPORTED, "the server takes K <= 2048 (K = %d: fold in with lda_infer)", ctx->K);
     if (iterations < 0) return fail(LDA_ERR_INVALID_ARGUMENT, "iterations must be >= 0");
     if (step_iters < 1 || step_tokens < 1) return fail(LDA_ERR_INVALID_ARGUMENT, "step_iters and step_tokens must be >= 1");
     if (lanes < 1 || lanes > kMaxLanes) return fail(LDA_ERR_INVALID_ARGUMENT, "lanes must be in 1..%d", kMaxLanes);

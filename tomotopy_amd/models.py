@@ -87,7 +87,7 @@ min_df : int
 rm_top : int
     the number of top words to be removed
 k : int
-    the number of topics between 1 ~ 2048
+    the number of topics between 1 ~ 32767
 alpha : Union[float, Iterable[float]]
     hyperparameter of Dirichlet distribution for document-topic
 eta : float
