@@ -328,7 +328,12 @@ int lda_get_infer_options(lda_ctx* ctx, uint32_t* seg_tokens, int* seg_max, int*
  * wavefront)) iterations.  The server takes the context's segmentation
  * (lda_set_infer_options) at start.  Results equal lda_infer(mode 0) with the
  * same infer_seed, bit for bit (also with word priors).  TermWeight IDF / PMI
- * models and K above 2048 are not served (LDA_ERR_UNSUPPORTED).
+ * models and models with K above 2048 are served by the run-column fold-in:
+ * each lane is a thread that folds the queued requests in as one batch, every
+ * document running all its iterations in that step (step_iters / step_tokens
+ * do not apply); their requests go through lda_serve_submit_w (IDF / PMI:
+ * token weights as for lda_infer_w) and results equal lda_infer_w / lda_infer
+ * (mode 0) with the same infer_seed, bit for bit.
  * The context must not be trained, re-uploaded or destroyed while a server on
  * it runs.  submit / wait may be called from any thread; lda_serve_stop must
  * not overlap a wait, and drops requests that were not waited for. */
@@ -339,6 +344,13 @@ int lda_serve_start(lda_ctx* ctx, uint64_t hbm_budget_bytes, int iterations, int
 /* Copies the request (CSR as for lda_infer); returns its id at once. */
 int lda_serve_submit(lda_server* server, const uint32_t* words, const uint64_t* doc_off, int64_t D,
                      uint64_t* request_id);
+/* lda_serve_submit with token weights: requests to the server of a
+ * TermWeight IDF / PMI model (weights[g] >= 0 and finite for valid words, as
+ * LDAModel::infer takes them from the vocabulary weights,
+ * LDAModel.hpp:794-809).  Replaces the same binding as lda_serve_submit. */
+int lda_serve_submit_w(lda_server* s, const uint32_t* words, const uint64_t* doc_off, int64_t D, const float* weights,
+                       uint64_t* request_id);
+
 /* Waits for a request (timeout_ms < 0: no limit; on timeout LDA_ERR_STATE and
  * the request stays pending), copies theta (D x K) and ll (D), and the
  * latency from submit to results-on-host in ms; then forgets the request. */

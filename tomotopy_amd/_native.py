@@ -77,6 +77,7 @@ _SIGS = {
     "lda_get_infer_options": [_c_p, _c_p, _c_p, _c_p],
     "lda_serve_start": [_c_p, _u64, _i32, _i32, _i64, _i32, _u64, _c_p],
     "lda_serve_submit": [_c_p, _c_p, _c_p, _i64, _c_p],
+    "lda_serve_submit_w": [_c_p, _c_p, _c_p, _i64, _c_p, _c_p],
     "lda_serve_wait": [_c_p, _u64, _i32, _c_p, _c_p, _c_p],
     "lda_serve_stats": [_c_p, _c_p],
     "lda_serve_stop": [_c_p],
@@ -403,7 +404,8 @@ class Server:
 
     submit() copies a request and returns its id at once; wait() blocks (the
     GIL is released inside the C call) and returns (theta, ll, latency_ms).
-    Results equal Context.infer(mode=0) with the same infer_seed bit for bit.
+    Results equal Context.infer(mode=0) with the same infer_seed bit for bit
+    (for a term-weighted model, submit the token weights as for infer).
     The context must not be trained or changed while the server runs.
     """
 
@@ -418,12 +420,19 @@ class Server:
         self._h = h
         self._ndocs = {}
 
-    def submit(self, words, doc_off):
+    def submit(self, words, doc_off, weights=None):
+        """weights: the tokens' weights, for the server of a term-weighted model."""
         words = np.ascontiguousarray(words, dtype=np.uint32)
         doc_off = np.ascontiguousarray(doc_off, dtype=np.uint64)
         D = len(doc_off) - 1
         rid = ctypes.c_uint64(0)
-        _check(load().lda_serve_submit(self._h, _ptr(words), _ptr(doc_off), D, ctypes.byref(rid)))
+        if weights is None:
+            _check(load().lda_serve_submit(self._h, _ptr(words), _ptr(doc_off), D, ctypes.byref(rid)))
+        else:
+            wts = np.ascontiguousarray(weights, dtype=np.float32)
+            if len(wts) != len(words):
+                raise ValueError("weights must have one entry per token")
+            _check(load().lda_serve_submit_w(self._h, _ptr(words), _ptr(doc_off), D, _ptr(wts), ctypes.byref(rid)))
         self._ndocs[rid.value] = D
         return rid.value
 

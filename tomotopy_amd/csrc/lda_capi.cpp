@@ -891,9 +891,7 @@ int lda_ctx_destroy(lda_ctx* c) {
     c->i_theta.release(); c->i_ll.release();
     c->d_wt.release(); c->d_docw.release(); c->d_rowpart.release();
     c->d_prior_row.release(); c->d_prior_eta.release(); c->d_etasum.release(); c->d_etasum_d.release();
-    c->g_wt.release(); c->g_docw.release(); c->g_jobrun.release(); c->g_runword.release();
-    c->g_cells.release(); c->g_ndk.release(); c->g_nk.release(); c->g_segb.release(); c->g_segr.release();
-    c->g_chunkbase.release(); c->g_chunkrb.release(); c->g_chunkll.release();
+    c->gen.release();
     c->d_dglut.release(); c->d_aux.release(); c->d_bad.release(); c->d_hist.release(); c->d_hist_len.release();
     c->d_nkw16.release(); c->r_out.release(); c->r_cnt.release();
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -1788,8 +1786,12 @@ static uint64_t seg_start_host(const uint32_t* w, uint64_t n, int i, int S) {
 // Mode 0: jobs grouped by S, longest first, one persistent launch per
 // segment count on the fold-in streams, each followed on its stream by the
 // finalize kernel of its documents; mode 1: one launch per document.  Arguments checked by check_infer_args.
-static int infer_general(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off, int64_t D, const float* weights,
-                         int iterations, uint64_t infer_seed, int mode, float* theta, double* ll, uint16_t* z_out) {
+}  // extern "C"
+
+int lda_host::infer_general_on(lda_ctx* c, lda_host::GenFold& B, hipStream_t s, lda_host::FoldStreams& fs,
+                               hipEvent_t e0, hipEvent_t e1, const uint32_t* words, const uint64_t* doc_off, int64_t D,
+                               const float* weights, int iterations, uint64_t infer_seed, int mode, float* theta,
+                               double* ll, uint16_t* z_out, float* ms_out) {
     const bool wt = c->tw != LDA_TW_ONE;
     const uint64_t N = doc_off[D];
     constexpr int RW = lda::kMaxFoldSegments + 1;
@@ -1880,41 +1882,40 @@ static int infer_general(lda_ctx* c, const uint32_t* words, const uint64_t* doc_
         }
     }
     HIP_TRY(hipSetDevice(c->device));
-    const hipStream_t s = c->stream;
-    HIP_TRY(c->i_words.alloc(N));
-    HIP_TRY(c->i_z.alloc(N));
-    HIP_TRY(c->i_jobs.alloc(D));
-    HIP_TRY(c->i_counter.alloc(lda::kFoldBuckets));
-    HIP_TRY(c->i_theta.alloc((size_t)D * c->K));
-    HIP_TRY(c->i_ll.alloc(D));
-    if (wt) HIP_TRY(c->g_wt.alloc(N));
-    HIP_TRY(c->g_jobrun.alloc(D + 1));
-    HIP_TRY(c->g_segr.alloc((size_t)D * RW));
-    HIP_TRY(c->g_segb.alloc((size_t)D * RW));
-    HIP_TRY(c->g_runword.alloc(R));
-    HIP_TRY(c->g_cells.alloc(R * (size_t)c->Kp));
-    HIP_TRY(c->g_ndk.alloc((size_t)D * c->Kp));
-    HIP_TRY(c->g_nk.alloc((size_t)D * c->Kp));
-    HIP_TRY(c->g_docw.alloc(D));
-    HIP_TRY(c->g_chunkbase.alloc(D + 1));
-    HIP_TRY(c->g_chunkrb.alloc(NC + 1));
-    HIP_TRY(c->g_chunkll.alloc(NC ? NC : 1));
-    if (N) HIP_TRY(hipMemcpyAsync(c->i_words.p, words, N * 4, hipMemcpyHostToDevice, s));
-    if (wt && N) HIP_TRY(hipMemcpyAsync(c->g_wt.p, weights, N * 4, hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(c->i_jobs.p, jobs.data(), D * sizeof(lda::InferJob), hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(c->g_jobrun.p, job_run.data(), (D + 1) * 8, hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(c->g_segr.p, seg_r.data(), seg_r.size() * 8, hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(c->g_chunkbase.p, chunk_base.data(), (D + 1) * 8, hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(c->g_chunkrb.p, chunk_rb.data(), (NC + 1) * 8, hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(c->g_segb.p, seg_b.data(), seg_b.size() * 8, hipMemcpyHostToDevice, s));
-    if (R) HIP_TRY(hipMemcpyAsync(c->g_runword.p, run_word.data(), R * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(B.words.alloc(N));
+    HIP_TRY(B.z.alloc(N));
+    HIP_TRY(B.jobs.alloc(D));
+    HIP_TRY(B.counter.alloc(lda::kFoldBuckets));
+    HIP_TRY(B.theta.alloc((size_t)D * c->K));
+    HIP_TRY(B.ll.alloc(D));
+    if (wt) HIP_TRY(B.wt.alloc(N));
+    HIP_TRY(B.jobrun.alloc(D + 1));
+    HIP_TRY(B.segr.alloc((size_t)D * RW));
+    HIP_TRY(B.segb.alloc((size_t)D * RW));
+    HIP_TRY(B.runword.alloc(R));
+    HIP_TRY(B.cells.alloc(R * (size_t)c->Kp));
+    HIP_TRY(B.ndk.alloc((size_t)D * c->Kp));
+    HIP_TRY(B.nk.alloc((size_t)D * c->Kp));
+    HIP_TRY(B.docw.alloc(D));
+    HIP_TRY(B.chunkbase.alloc(D + 1));
+    HIP_TRY(B.chunkrb.alloc(NC + 1));
+    HIP_TRY(B.chunkll.alloc(NC ? NC : 1));
+    if (N) HIP_TRY(hipMemcpyAsync(B.words.p, words, N * 4, hipMemcpyHostToDevice, s));
+    if (wt && N) HIP_TRY(hipMemcpyAsync(B.wt.p, weights, N * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(B.jobs.p, jobs.data(), D * sizeof(lda::InferJob), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(B.jobrun.p, job_run.data(), (D + 1) * 8, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(B.segr.p, seg_r.data(), seg_r.size() * 8, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(B.chunkbase.p, chunk_base.data(), (D + 1) * 8, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(B.chunkrb.p, chunk_rb.data(), (NC + 1) * 8, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(B.segb.p, seg_b.data(), seg_b.size() * 8, hipMemcpyHostToDevice, s));
+    if (R) HIP_TRY(hipMemcpyAsync(B.runword.p, run_word.data(), R * 4, hipMemcpyHostToDevice, s));
     lda::InferGenParams p{};
-    p.words = c->i_words.p; p.z = c->i_z.p; p.wt = wt ? c->g_wt.p : nullptr;
-    p.jobs = c->i_jobs.p; p.njobs = D;
-    p.job_run = c->g_jobrun.p; p.seg_b = c->g_segb.p; p.seg_r = c->g_segr.p;
-    p.chunk_base = c->g_chunkbase.p; p.chunk_rb = c->g_chunkrb.p; p.chunk_ll = c->g_chunkll.p;
-    p.run_word = c->g_runword.p; p.cells = c->g_cells.p;
-    p.ndk_out = c->g_ndk.p; p.nk_out = c->g_nk.p; p.docw = c->g_docw.p;
+    p.words = B.words.p; p.z = B.z.p; p.wt = wt ? B.wt.p : nullptr;
+    p.jobs = B.jobs.p; p.njobs = D;
+    p.job_run = B.jobrun.p; p.seg_b = B.segb.p; p.seg_r = B.segr.p;
+    p.chunk_base = B.chunkbase.p; p.chunk_rb = B.chunkrb.p; p.chunk_ll = B.chunkll.p;
+    p.run_word = B.runword.p; p.cells = B.cells.p;
+    p.ndk_out = B.ndk.p; p.nk_out = B.nk.p; p.docw = B.docw.p;
     p.nkw = c->d_nkw.p; p.nk = c->d_nk.p; p.alpha = c->d_alpha.p; p.alpha_cdf = c->d_alpha_cdf.p;
     p.eta = c->eta; p.veta = c->veta;
     p.eta_d = (double)c->eta; p.veta_d = (double)c->V * (double)c->eta;
@@ -1924,16 +1925,14 @@ static int infer_general(lda_ctx* c, const uint32_t* words, const uint64_t* doc_
         p.etasum = c->d_etasum.p; p.etasum_d = c->d_etasum_d.p;
     }
     p.seed_lo = (uint32_t)infer_seed; p.seed_hi = (uint32_t)(infer_seed >> 32);
-    p.theta = c->i_theta.p; p.ll = c->i_ll.p;
+    p.theta = B.theta.p; p.ll = B.ll.p;
     double asd = 0.0;
     float asf = 0.0f;
     for (int k = 0; k < c->K; ++k) { asd += (double)c->alpha[k]; asf = asf + c->alpha[k]; }
     p.asum_d = asd; p.asum_f = asf;
-    c->stat_ms = c->stat_launches = c->stat_tokens = 0.0;
-    HIP_TRY(hipEventRecord(c->ev0, s));
+    HIP_TRY(hipEventRecord(e0, s));
     if (mode == 0) {
-        HIP_TRY(lda::launch_fill_u32(c->i_counter.p, 0u, lda::kFoldBuckets, s));
-        lda_host::FoldStreams& fs = c->fold_streams;
+        HIP_TRY(lda::launch_fill_u32(B.counter.p, 0u, lda::kFoldBuckets, s));
         HIP_TRY(hipEventRecord(fs.fork, s));
         // the longest documents (most segments) first, each segment count on its own stream
         for (int bk = lda::kFoldBuckets - 1; bk >= 0; --bk) {
@@ -1945,7 +1944,7 @@ static int infer_general(lda_ctx* c, const uint32_t* words, const uint64_t* doc_
             pb.seg_r += lo[bk] * RW;
             pb.chunk_base += lo[bk];
             pb.njobs = hi[bk] - lo[bk];
-            pb.counter = c->i_counter.p + bk;
+            pb.counter = B.counter.p + bk;
             const int grid = (int)std::min<int64_t>((int64_t)c->num_cu * per_cu[bk], pb.njobs);
             const hipStream_t sb = bk == 0 ? s : fs.aux[bk - 1];
             if (bk) HIP_TRY(hipStreamWaitEvent(sb, fs.fork, 0));
@@ -1958,7 +1957,6 @@ static int infer_general(lda_ctx* c, const uint32_t* words, const uint64_t* doc_
         }
         for (int bk = 1; bk < lda::kFoldBuckets; ++bk)
             if (hi[bk] > lo[bk]) HIP_TRY(hipStreamWaitEvent(s, fs.join[bk - 1], 0));
-        c->stat_launches = 1;
     } else {
         p.counter = nullptr;
         for (int64_t d = 0; d < D; ++d) {
@@ -1967,18 +1965,31 @@ static int infer_general(lda_ctx* c, const uint32_t* words, const uint64_t* doc_
             HIP_TRY(lda::launch_infer_gen(c->kpl, 1, p, 1, true, s));
             HIP_TRY(hipStreamSynchronize(s));
         }
-        c->stat_launches = (double)D;
     }
-    HIP_TRY(hipEventRecord(c->ev1, s));
-    HIP_TRY(hipEventSynchronize(c->ev1));
+    HIP_TRY(hipEventRecord(e1, s));
+    HIP_TRY(hipEventSynchronize(e1));
     float ms = 0.0f;
-    HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    c->stat_ms = ms;
-    c->stat_tokens = (double)N * iterations;
-    HIP_TRY(hipMemcpyAsync(theta, c->i_theta.p, (size_t)D * c->K * 4, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipMemcpyAsync(ll, c->i_ll.p, D * 8, hipMemcpyDeviceToHost, s));
-    if (z_out && N) HIP_TRY(hipMemcpyAsync(z_out, c->i_z.p, N * 2, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    if (ms_out) *ms_out = ms;
+    HIP_TRY(hipMemcpyAsync(theta, B.theta.p, (size_t)D * c->K * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(ll, B.ll.p, D * 8, hipMemcpyDeviceToHost, s));
+    if (z_out && N) HIP_TRY(hipMemcpyAsync(z_out, B.z.p, N * 2, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
+    return LDA_OK;
+}
+
+extern "C" {
+
+static int infer_general(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off, int64_t D, const float* weights,
+                         int iterations, uint64_t infer_seed, int mode, float* theta, double* ll, uint16_t* z_out) {
+    c->stat_ms = c->stat_launches = c->stat_tokens = 0.0;
+    float ms = 0.0f;
+    const int rc = lda_host::infer_general_on(c, c->gen, c->stream, c->fold_streams, c->ev0, c->ev1, words, doc_off, D,
+                                              weights, iterations, infer_seed, mode, theta, ll, z_out, &ms);
+    if (rc) return rc;
+    c->stat_ms = ms;
+    c->stat_launches = mode == 0 ? 1.0 : (double)D;
+    c->stat_tokens = (double)doc_off[D] * iterations;
     return LDA_OK;
 }
 

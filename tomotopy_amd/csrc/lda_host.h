@@ -65,6 +65,35 @@ struct FoldStreams {
 hipError_t launch_fold_buckets(int kpl, bool asym, const lda::InferParams* p_dev, const int grids[lda::kFoldBuckets],
                                hipStream_t s, FoldStreams& fs);
 
+// Device buffers of one run-column fold-in call (InferGenParams): the token
+// arena, token weights, jobs, runs, private columns, per-document state and
+// outputs.  A context holds one set; each server lane of a term-weighted
+// model holds its own.
+struct GenFold {
+    DevBuf<uint32_t> words, counter, runword;
+    DevBuf<uint16_t> z;
+    DevBuf<lda::InferJob> jobs;
+    DevBuf<float> theta, wt, docw;
+    DevBuf<double> ll, chunkll;
+    DevBuf<int64_t> jobrun, segr, chunkbase, chunkrb;
+    DevBuf<uint64_t> segb;
+    DevBuf<int32_t> cells, ndk, nk;
+    void release() {
+        words.release(); counter.release(); runword.release(); z.release(); jobs.release(); theta.release();
+        wt.release(); docw.release(); ll.release(); chunkll.release(); jobrun.release(); segr.release();
+        chunkbase.release(); chunkrb.release(); segb.release(); cells.release(); ndk.release(); nk.release();
+    }
+};
+
+// The run-column fold-in of D documents (host arrays; arguments checked by
+// the caller) on buffers B and stream s with bucket streams fs: mode 0 all
+// documents batched, mode 1 one launch per document.  e0 / e1 time the
+// kernels (*ms_out).  Used by lda_infer_w, lda_infer (K above 2048) and the
+// server of term-weighted models.
+int infer_general_on(lda_ctx* c, GenFold& B, hipStream_t s, FoldStreams& fs, hipEvent_t e0, hipEvent_t e1,
+                     const uint32_t* words, const uint64_t* doc_off, int64_t D, const float* weights, int iterations,
+                     uint64_t infer_seed, int mode, float* theta, double* ll, uint16_t* z_out, float* ms_out);
+
 }  // namespace lda_host
 
 using lda_host::DevBuf;
@@ -140,14 +169,8 @@ struct lda_ctx {
     DevBuf<uint16_t> i_z;
     DevBuf<float> i_theta;
     DevBuf<double> i_ll;
-    // fold-in of term-weighted models and of models with word priors
-    // (InferGenParams): token weights, runs, private columns, outputs
-    DevBuf<float> g_wt, g_docw;
-    DevBuf<int64_t> g_jobrun, g_segr, g_chunkbase, g_chunkrb;
-    DevBuf<double> g_chunkll;
-    DevBuf<uint64_t> g_segb;
-    DevBuf<uint32_t> g_runword;
-    DevBuf<int32_t> g_cells, g_ndk, g_nk;
+    // the run-column fold-in (term-weighted models, K above 2048)
+    lda_host::GenFold gen;
     // batched fold-in graph: counter reset + persistent kernel reading i_params
     hipGraphExec_t infer_exec = nullptr;
     int infer_exec_segmax = 0;  // the buckets the cached graph launches

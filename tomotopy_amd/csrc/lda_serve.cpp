@@ -21,6 +21,14 @@
 //     document's result does not depend on the requests it shared the device
 //     with nor on how its iterations were split into steps: it equals
 //     lda_infer on the same document bit for bit.
+// Term-weighted (IDF / PMI) models, and K above 2048, are served by the
+// run-column fold-in instead: float counts depend on the order of their
+// updates and cannot be rebuilt from z, so every document runs all its
+// iterations within the step that starts it.  Each lane is a thread that
+// takes the queued requests in FIFO order (up to the step's document and
+// token limits), folds them in as one batch (lda_infer_w mode 0 on the
+// lane's own buffers and streams) and hands the results back; results equal
+// lda_infer_w (or lda_infer) mode 0 with the same infer_seed, bit for bit.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -46,6 +54,7 @@ double now_ms() {
 
 struct Req {
     std::vector<uint32_t> words;
+    std::vector<float> wt;      // token weights (term-weighted models)
     std::vector<uint64_t> off;  // ndocs + 1
     std::vector<float> theta;   // ndocs x K
     std::vector<double> ll;
@@ -89,6 +98,7 @@ struct Lane {
     bool busy = false;
     std::vector<uint64_t> docs;  // document keys of the step
     std::vector<int> it_end, out_row;
+    lda_host::GenFold gen;       // run-column steps: the lane's fold-in buffers
 };
 
 }  // namespace
@@ -116,6 +126,10 @@ struct lda_server {
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::thread worker;
+    // run-column steps (term-weighted models, K above 2048): one thread per lane
+    bool generic = false, weighted = false;
+    std::vector<std::thread> gworkers;
+    int64_t inflight_docs = 0;
     bool stop = false;
     int fatal = LDA_OK;
     std::string fatal_msg;
@@ -386,11 +400,95 @@ void worker_loop(lda_server* s) {
     }
 }
 
+// A lane of the run-column server: takes queued requests (FIFO, at most
+// max_active documents and arena_cap tokens per step, at least one request),
+// folds them in as one batch and returns the results.
+void worker_loop_gen(lda_server* s, int lane) {
+    Lane& L = s->lanes[lane];
+    lda_ctx* c = s->ctx;
+    if (hipSetDevice(c->device) != hipSuccess) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        fail_all(s, fail(LDA_ERR_DEVICE, "hipSetDevice failed in the server thread"), now_ms());
+        return;
+    }
+    std::vector<uint32_t> words;
+    std::vector<float> wts;
+    std::vector<uint64_t> off;
+    std::vector<float> theta;
+    std::vector<double> ll;
+    std::unique_lock<std::mutex> lk(s->mu);
+    for (;;) {
+        s->cv_work.wait(lk, [&] { return s->stop || (!s->queue.empty() && !s->fatal); });
+        if (s->stop) break;
+        const double th0 = now_ms();
+        std::vector<Req*> batch;
+        int64_t nd = 0;
+        uint64_t nt = 0;
+        while (!s->queue.empty()) {
+            Req& r = s->reqs[s->queue.front()];
+            const uint64_t n = r.off[r.ndocs];
+            if (!batch.empty() && (nd + r.ndocs > s->max_active || nt + n > s->arena_cap)) break;
+            batch.push_back(&r);
+            nd += r.ndocs;
+            nt += n;
+            s->queue.pop_front();
+        }
+        s->inflight_docs += nd;
+        lk.unlock();
+        // Requests stay in place until their waiter erases them, which needs
+        // done == true, set only under the lock below.
+        words.resize(nt);
+        wts.resize(s->weighted ? nt : 0);
+        off.assign(1, 0);
+        for (Req* r : batch) {
+            const uint64_t base = off.back();
+            const uint64_t n = r->off[r->ndocs];
+            std::copy(r->words.begin(), r->words.end(), words.begin() + base);
+            if (s->weighted) std::copy(r->wt.begin(), r->wt.end(), wts.begin() + base);
+            for (int i = 1; i <= r->ndocs; ++i) off.push_back(base + r->off[i]);
+            (void)n;
+        }
+        theta.resize((size_t)nd * c->K);
+        ll.resize(nd);
+        float ms = 0.0f;
+        const int rc = lda_host::infer_general_on(c, L.gen, L.stream, L.fs, L.ev0, L.ev, words.data(), off.data(), nd,
+                                                  s->weighted ? wts.data() : nullptr, s->iterations, s->infer_seed, 0,
+                                                  theta.data(), ll.data(), nullptr, &ms);
+        const double t = now_ms();
+        lk.lock();
+        s->inflight_docs -= nd;
+        if (rc != LDA_OK) {
+            fail_all(s, rc, t);
+            continue;
+        }
+        int64_t row = 0;
+        for (Req* r : batch) {
+            std::copy(theta.begin() + row * c->K, theta.begin() + (row + r->ndocs) * c->K, r->theta.begin());
+            std::copy(ll.begin() + row, ll.begin() + row + r->ndocs, r->ll.begin());
+            row += r->ndocs;
+            r->ndone = r->ndocs;
+            r->done = true;
+            r->t_done = t;
+        }
+        s->steps += 1;
+        s->jobs_total += nd;
+        s->docs_done += nd;
+        s->reqs_done += (int64_t)batch.size();
+        s->token_iters += (double)nt * s->iterations;
+        s->gpu_ms += ms;
+        s->max_step_ms = std::max(s->max_step_ms, (double)ms);
+        s->busy_ms += t - th0;
+        s->host_ms += std::max(0.0, t - th0 - (double)ms);
+        s->cv_done.notify_all();
+    }
+}
+
 void release(lda_server* s) {
     s->words.release();
     s->z.release();
     for (Lane& L : s->lanes) {
         L.jobs.release(); L.params.release(); L.counter.release(); L.theta.release(); L.ll.release();
+        L.gen.release();
         if (L.h_params) (void)hipHostFree(L.h_params);
         if (L.h_jobs) (void)hipHostFree(L.h_jobs);
         if (L.h_theta) (void)hipHostFree(L.h_theta);
@@ -411,16 +509,43 @@ int lda_serve_start(lda_ctx* ctx, uint64_t hbm_budget_bytes, int iterations, int
                     int lanes, uint64_t infer_seed, lda_server** out) {
     if (!ctx || !out) return fail(LDA_ERR_INVALID_ARGUMENT, "NULL argument");
     *out = nullptr;
-    if (ctx->tw != LDA_TW_ONE)
-        return fail(LDA_ERR_UNSUPPORTED, "the server takes TermWeight ONE models; fold IDF / PMI models in with lda_infer_w");
-    if (ctx->kpl > lda::kSmallMaxKpl)
-        return fail(LDA_ERR_UNSUPPORTED, "the server takes K <= 2048 (K = %d: fold in with lda_infer)", ctx->K);
     if (iterations < 0) return fail(LDA_ERR_INVALID_ARGUMENT, "iterations must be >= 0");
     if (step_iters < 1 || step_tokens < 1) return fail(LDA_ERR_INVALID_ARGUMENT, "step_iters and step_tokens must be >= 1");
     if (lanes < 1 || lanes > kMaxLanes) return fail(LDA_ERR_INVALID_ARGUMENT, "lanes must be in 1..%d", kMaxLanes);
     if (hbm_budget_bytes < (64u << 10)) return fail(LDA_ERR_INVALID_ARGUMENT, "HBM budget below 64 KiB");
     HIP_TRY(hipSetDevice(ctx->device));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (ctx->tw != LDA_TW_ONE || ctx->kpl > lda::kSmallMaxKpl) {
+        // run-column steps: per lane and step at most max_active documents
+        // and arena_cap tokens, sized so that a step's private columns (at
+        // most one Kp column per token), state rows and arrays fit the budget
+        lda_server* s = new lda_server();
+        s->ctx = ctx;
+        s->generic = true;
+        s->weighted = ctx->tw != LDA_TW_ONE;
+        s->iterations = iterations;
+        s->step_iters = step_iters;
+        s->step_tokens = step_tokens;
+        s->infer_seed = infer_seed;
+        s->nlanes = lanes;
+        const uint64_t lane_budget = hbm_budget_bytes / (uint64_t)lanes;
+        const uint64_t per_doc = (uint64_t)ctx->Kp * 8 + (uint64_t)ctx->K * 4 + 256;
+        const uint64_t per_token = (uint64_t)ctx->Kp * 4 + 24;
+        s->max_active = (int)std::min<uint64_t>(1u << 16, std::max<uint64_t>(1, lane_budget / 4 / per_doc));
+        s->arena_cap = std::max<uint64_t>(1, (lane_budget - std::min(lane_budget, (uint64_t)s->max_active * per_doc)) / per_token);
+        for (int i = 0; i < s->nlanes; ++i) {
+            Lane& L = s->lanes[i];
+            L.srv = s;
+            if (hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking) != hipSuccess || L.fs.create() != hipSuccess ||
+                hipEventCreate(&L.ev) != hipSuccess || hipEventCreate(&L.ev0) != hipSuccess) {
+                release(s);
+                return fail(LDA_ERR_DEVICE, "stream creation failed");
+            }
+        }
+        for (int i = 0; i < s->nlanes; ++i) s->gworkers.emplace_back(worker_loop_gen, s, i);
+        *out = s;
+        return LDA_OK;
+    }
     // budget: 1/8 for per-document outputs and jobs, the rest for the token arena
     const size_t per_doc = ctx->K * sizeof(float) + sizeof(double) + sizeof(lda::InferJob);
     const size_t per_token = sizeof(uint32_t) + sizeof(uint16_t);
@@ -488,8 +613,16 @@ int lda_serve_start(lda_ctx* ctx, uint64_t hbm_budget_bytes, int iterations, int
     return LDA_OK;
 }
 
-int lda_serve_submit(lda_server* s, const uint32_t* words, const uint64_t* doc_off, int64_t D, uint64_t* request_id) {
+}  // extern "C"
+
+namespace {
+
+int submit_impl(lda_server* s, const uint32_t* words, const uint64_t* doc_off, int64_t D, const float* weights,
+                uint64_t* request_id) {
     if (!s || !request_id || D < 0 || (D && !doc_off)) return fail(LDA_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (s->weighted && D && doc_off[D] && !weights)
+        return fail(LDA_ERR_INVALID_ARGUMENT, "a term-weighted model's server needs the token weights (lda_serve_submit_w)");
+    if (!s->weighted && weights) return fail(LDA_ERR_INVALID_ARGUMENT, "TermWeight ONE models take no token weights");
     if (D > s->max_active)
         return fail(LDA_ERR_OUT_OF_MEMORY, "request has %lld documents; the server admits at most %d at once", (long long)D,
                     s->max_active);
@@ -507,8 +640,13 @@ int lda_serve_submit(lda_server* s, const uint32_t* words, const uint64_t* doc_o
     if (N > s->arena_cap)
         return fail(LDA_ERR_OUT_OF_MEMORY, "request has %llu tokens; the server's token arena holds %llu",
                     (unsigned long long)N, (unsigned long long)s->arena_cap);
+    if (weights)
+        for (uint64_t g = 0; g < N; ++g)
+            if (words[g] < (uint32_t)s->ctx->V && !(weights[g] >= 0.0f && weights[g] < 1e30f))
+                return fail(LDA_ERR_INVALID_ARGUMENT, "token weight %llu is negative or not finite", (unsigned long long)g);
     Req r;
     if (N) r.words.assign(words, words + N);
+    if (N && weights) r.wt.assign(weights, weights + N);
     if (D) r.off.assign(doc_off, doc_off + D + 1);
     else r.off.assign(1, 0);
     r.ndocs = (int)D;
@@ -530,6 +668,19 @@ int lda_serve_submit(lda_server* s, const uint32_t* words, const uint64_t* doc_o
     }
     s->cv_work.notify_one();
     return LDA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lda_serve_submit(lda_server* s, const uint32_t* words, const uint64_t* doc_off, int64_t D, uint64_t* request_id) {
+    return submit_impl(s, words, doc_off, D, nullptr, request_id);
+}
+
+int lda_serve_submit_w(lda_server* s, const uint32_t* words, const uint64_t* doc_off, int64_t D, const float* weights,
+                       uint64_t* request_id) {
+    return submit_impl(s, words, doc_off, D, weights, request_id);
 }
 
 int lda_serve_wait(lda_server* s, uint64_t request_id, int timeout_ms, float* theta, double* ll, double* latency_ms) {
@@ -561,7 +712,7 @@ int lda_serve_stats(lda_server* s, double stats[12]) {
     stats[2] = (double)s->docs_done;
     stats[3] = (double)s->reqs_done;
     stats[4] = s->token_iters;
-    stats[5] = (double)s->active.size();
+    stats[5] = s->generic ? (double)s->inflight_docs : (double)s->active.size();
     stats[6] = (double)s->queue.size();
     stats[7] = (double)s->arena_cap;
     stats[8] = s->gpu_ms;
@@ -579,6 +730,8 @@ int lda_serve_stop(lda_server* s) {
     }
     s->cv_work.notify_all();
     if (s->worker.joinable()) s->worker.join();
+    for (auto& t : s->gworkers)
+        if (t.joinable()) t.join();
     (void)hipSetDevice(s->ctx->device);
     for (Lane& L : s->lanes)
         if (L.stream) (void)hipStreamSynchronize(L.stream);
