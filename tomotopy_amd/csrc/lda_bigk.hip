@@ -40,9 +40,9 @@
 namespace lda {
 namespace {
 
-constexpr int BK_CH = 16;               // topics per lane per chunk
-constexpr int BK_CHUNK = 64 * BK_CH;    // positions per chunk
-constexpr int BK_MAX_CHUNKS = kBigMaxKpl / BK_CH;
+// CH topics per lane per chunk (16, or KPL itself when it is smaller); chunk
+// c of every lane is the position block [64*CH*c, 64*CH*(c+1))
+constexpr int BK_MAX_CHUNKS = kBigMaxKpl / 16;
 
 template <bool WT>
 using bk_t = typename std::conditional<WT, float, int32_t>::type;
@@ -59,7 +59,8 @@ __device__ __forceinline__ void bk_lane_limits(int K, int lane, int kpl, int& nv
 struct BkEtaSym {
     float eta, veta;
     __device__ __forceinline__ void word(int) {}
-    __device__ __forceinline__ void chunk(int, int, fvec<BK_CH>& e, fvec<BK_CH>& s) const {
+    template <int CH>
+    __device__ __forceinline__ void chunk(int, int, fvec<CH>& e, fvec<CH>& s) const {
         e = eta;
         s = veta;
     }
@@ -76,10 +77,11 @@ struct BkEtaPrior {
         const int r = prior_row[v];
         ecol = r >= 0 ? prior_eta + (int64_t)r * Kp : nullptr;
     }
-    __device__ __forceinline__ void chunk(int c, int lane, fvec<BK_CH>& e, fvec<BK_CH>& s) const {
-        if (ecol) load_kvec_f<BK_CH>(ecol + c * BK_CHUNK, lane, e);
+    template <int CH>
+    __device__ __forceinline__ void chunk(int c, int lane, fvec<CH>& e, fvec<CH>& s) const {
+        if (ecol) load_kvec_f<CH>(ecol + c * 64 * CH, lane, e);
         else e = eta;
-        load_kvec_f<BK_CH>(etasum + c * BK_CHUNK, lane, s);
+        load_kvec_f<CH>(etasum + c * 64 * CH, lane, s);
     }
 };
 
@@ -100,20 +102,20 @@ __device__ __forceinline__ typename BkEtaOf<ASYM>::type bk_make_eta(const int32_
 // alpha, B = float(n_kw) + eta, b = float(n_k) + V*eta; float counts
 // (n_dk + alpha)(n_kw + eta) and n_k + V*eta; then q = a/b as a*y with one
 // Markstein correction, y = RN(1/b).
-template <bool WT, class E>
+template <int CH, bool WT, class E>
 __device__ __forceinline__ void bk_weights(const bk_t<WT>* nd, const bk_t<WT>* col, const bk_t<WT>* nk,
-                                           const float* alpha, int c, int lane, const E& et, fvec<BK_CH>& w) {
-    const int o = c * BK_CHUNK;
-    fvec<BK_CH> al, e, s;
-    load_kvec_f<BK_CH>(alpha + o, lane, al);
-    et.chunk(c, lane, e, s);
+                                           const float* alpha, int c, int lane, const E& et, fvec<CH>& w) {
+    const int o = c * (64 * CH);
+    fvec<CH> al, e, s;
+    load_kvec_f<CH>(alpha + o, lane, al);
+    et.template chunk<CH>(c, lane, e, s);
     if constexpr (WT) {
-        fvec<BK_CH> a0, c0, n0;
-        load_kvec_f<BK_CH>(nd + o, lane, a0);
-        load_kvec_f<BK_CH>(col + o, lane, c0);
-        load_kvec_f<BK_CH>(nk + o, lane, n0);
+        fvec<CH> a0, c0, n0;
+        load_kvec_f<CH>(nd + o, lane, a0);
+        load_kvec_f<CH>(col + o, lane, c0);
+        load_kvec_f<CH>(nk + o, lane, n0);
 #pragma unroll
-        for (int j = 0; j < BK_CH; ++j) {
+        for (int j = 0; j < CH; ++j) {
             const float a = (a0[j] + al[j]) * (c0[j] + e[j]);
             const float bf = n0[j] + s[j];
             const float y = recip_rn(bf);
@@ -122,12 +124,12 @@ __device__ __forceinline__ void bk_weights(const bk_t<WT>* nd, const bk_t<WT>* c
             w[j] = __builtin_fmaf(rr, y, q0);
         }
     } else {
-        ivec<BK_CH> a0, c0, n0;
-        load_kvec<BK_CH>(nd + o, lane, a0);
-        load_kvec<BK_CH>(col + o, lane, c0);
-        load_kvec<BK_CH>(nk + o, lane, n0);
+        ivec<CH> a0, c0, n0;
+        load_kvec<CH>(nd + o, lane, a0);
+        load_kvec<CH>(col + o, lane, c0);
+        load_kvec<CH>(nk + o, lane, n0);
 #pragma unroll
-        for (int j = 0; j < BK_CH; ++j) {
+        for (int j = 0; j < CH; ++j) {
             const float A = (float)a0[j] + al[j];
             const float B = (float)c0[j] + e[j];
             const float bf = (float)n0[j] + s[j];
@@ -147,8 +149,7 @@ template <bool WT>
 __device__ __forceinline__ void bk_bump(bk_t<WT>* nd, bk_t<WT>* col, bk_t<WT>* nk, int lane, int kpl, int ksh, int z,
                                         float w, bool dec) {
     if (lane == (z >> ksh)) {
-        const int sl = z & (kpl - 1);
-        const int q = (sl >> 2) * 256 + lane * 4 + (sl & 3);
+        const int q = kpos_rt(kpl, z);
         if constexpr (WT) {
             nd[q] = dec ? sub_clamp(nd[q], w) : nd[q] + w;
             col[q] = dec ? sub_clamp(col[q], w) : col[q] + w;
@@ -163,27 +164,28 @@ __device__ __forceinline__ void bk_bump(bk_t<WT>* nd, bk_t<WT>* col, bk_t<WT>* n
 }
 
 // Running sum of the chunk's weights from `carry` (the lane's earlier chunks).
-__device__ __forceinline__ void bk_prefix(fvec<BK_CH>& w, float carry) {
+template <int CH>
+__device__ __forceinline__ void bk_prefix(fvec<CH>& w, float carry) {
     w[0] = carry + w[0];
 #pragma unroll
-    for (int j = 1; j < BK_CH; ++j) w[j] = w[j - 1] + w[j];
+    for (int j = 1; j < CH; ++j) w[j] = w[j - 1] + w[j];
 }
 
 // One token: remove zo, weights, scan, search, add the new topic (returned,
 // wave-uniform).  lastc: this wavefront's LDS row of chunk last values
 // ([chunk][lane]).  *total receives cdf[K-1].
-template <bool WT, class E>
+template <int CH, bool WT, class E>
 __device__ __forceinline__ int bk_token(bk_t<WT>* nd, bk_t<WT>* col, bk_t<WT>* nk, const float* alpha, int lane,
                                         int kpl, int ksh, int zo, float tw, float u, int K, int nvalid, int lastLane,
                                         const E& et, float* lastc, float* total) {
     bk_bump<WT>(nd, col, nk, lane, kpl, ksh, zo, tw, true);
-    const int C = kpl / BK_CH;
+    const int C = kpl / CH;
     float carry = 0.0f;
     for (int c = 0; c < C; ++c) {
-        fvec<BK_CH> w;
-        bk_weights<WT>(nd, col, nk, alpha, c, lane, et, w);
-        bk_prefix(w, carry);
-        carry = w[BK_CH - 1];
+        fvec<CH> w;
+        bk_weights<CH, WT>(nd, col, nk, alpha, c, lane, et, w);
+        bk_prefix<CH>(w, carry);
+        carry = w[CH - 1];
         lastc[c * 64 + lane] = carry;
     }
     const float incl = wave_inclusive_scan(carry);
@@ -197,10 +199,10 @@ __device__ __forceinline__ int bk_token(bk_t<WT>* nd, bk_t<WT>* col, bk_t<WT>* n
     for (int c = 0; c < C; ++c) cs += (r < excl + lastc[c * 64 + lane]) ? 0 : 1;
     int miss = kpl;
     if (cs < C) {
-        fvec<BK_CH> w;
-        bk_weights<WT>(nd, col, nk, alpha, cs, lane, et, w);
-        bk_prefix(w, cs ? lastc[(cs - 1) * 64 + lane] : 0.0f);
-        miss = cs * BK_CH + cdf_count_le<BK_CH>(w, excl, r, excl + w[BK_CH - 1]);
+        fvec<CH> w;
+        bk_weights<CH, WT>(nd, col, nk, alpha, cs, lane, et, w);
+        bk_prefix<CH>(w, cs ? lastc[(cs - 1) * 64 + lane] : 0.0f);
+        miss = cs * CH + cdf_count_le<CH>(w, excl, r, excl + w[CH - 1]);
     }
     const unsigned long long m = __ballot(miss < nvalid);
     int zn;
@@ -216,19 +218,19 @@ __device__ __forceinline__ int bk_token(bk_t<WT>* nd, bk_t<WT>* col, bk_t<WT>* n
 }
 
 // This lane's positions of a K-vector: dst = src.
-template <typename T>
+template <int CH, typename T>
 __device__ __forceinline__ void bk_copy(T* dst, const T* src, int lane, int kpl) {
-    for (int c = 0; c < kpl / BK_CH; ++c) {
-        ivec<BK_CH> x;
-        load_kvec<BK_CH>(reinterpret_cast<const int32_t*>(src) + c * BK_CHUNK, lane, x);
-        store_kvec<BK_CH>(reinterpret_cast<int32_t*>(dst) + c * BK_CHUNK, lane, x);
+    for (int c = 0; c < kpl / CH; ++c) {
+        ivec<CH> x;
+        load_kvec<CH>(reinterpret_cast<const int32_t*>(src) + c * (64 * CH), lane, x);
+        store_kvec<CH>(reinterpret_cast<int32_t*>(dst) + c * (64 * CH), lane, x);
     }
 }
 
-template <typename T>
+template <int CH, typename T>
 __device__ __forceinline__ void bk_fill(T* dst, int lane, int kpl) {
-    ivec<BK_CH> z = 0;
-    for (int c = 0; c < kpl / BK_CH; ++c) store_kvec<BK_CH>(reinterpret_cast<int32_t*>(dst) + c * BK_CHUNK, lane, z);
+    ivec<CH> z = 0;
+    for (int c = 0; c < kpl / CH; ++c) store_kvec<CH>(reinterpret_cast<int32_t*>(dst) + c * (64 * CH), lane, z);
 }
 
 // sampleDocument over the token range [b, e) (LDAModel.hpp:326-354) on the
@@ -236,7 +238,7 @@ __device__ __forceinline__ void bk_fill(T* dst, int lane, int kpl) {
 // owns its cells); MODE_ATOMIC: colbuf holds this wavefront's copy of the
 // column (read when the word's run starts, plus its own updates) and the
 // token's net change goes to the shared n_kw as atomics.
-template <int MODE, bool WT, class E>
+template <int CH, int MODE, bool WT, class E>
 __device__ __forceinline__ void bk_sample_range(bk_t<WT>* nd, bk_t<WT>* nk, bk_t<WT>* colbuf, const SweepParams& p,
                                                 uint32_t sweep, int lane, uint64_t b, uint64_t e, int kpl, int ksh,
                                                 int nvalid, int lastLane, E& et, float* lastc) {
@@ -263,7 +265,7 @@ __device__ __forceinline__ void bk_sample_range(bk_t<WT>* nd, bk_t<WT>* nk, bk_t
                 if constexpr (MODE == MODE_DET) {
                     col = nkwT + (int64_t)v * p.Kp;
                 } else {
-                    bk_copy(colbuf, nkwT + (int64_t)v * p.Kp, lane, kpl);
+                    bk_copy<CH>(colbuf, nkwT + (int64_t)v * p.Kp, lane, kpl);
                     col = colbuf;
                 }
                 et.word(v);
@@ -272,7 +274,7 @@ __device__ __forceinline__ void bk_sample_range(bk_t<WT>* nd, bk_t<WT>* nk, bk_t
             const int zo = readlane_i(zv, i);
             const float w = readlane_f(tw, i);
             const float u = readlane_f(uv, i);
-            const int zn = bk_token<WT>(nd, col, nk, p.alpha, lane, kpl, ksh, zo, w, u, p.K, nvalid, lastLane, et, lastc,
+            const int zn = bk_token<CH, WT>(nd, col, nk, p.alpha, lane, kpl, ksh, zo, w, u, p.K, nvalid, lastLane, et, lastc,
                                         nullptr);
             if (zn != zo) {
                 if constexpr (MODE == MODE_ATOMIC) {
@@ -312,7 +314,7 @@ __device__ __forceinline__ uint64_t bk_prime(uint64_t N, uint32_t sseed) {
 // W == 1: worker w = blockIdx.x visits documents id*W + (round+w) % W in
 // forShuffled order (Utils.hpp:157-178), their tokens in vocabulary chunk w;
 // private n_k in nk_out row w.
-template <bool WT, bool ASYM>
+template <int CH, bool WT, bool ASYM>
 __global__ void __launch_bounds__(64) bk_sweep_det_kernel(SweepParams p, int kpl, int ksh) {
     using T = bk_t<WT>;
     __shared__ float lastc[BK_MAX_CHUNKS * 64];
@@ -334,14 +336,14 @@ __global__ void __launch_bounds__(64) bk_sweep_det_kernel(SweepParams p, int kpl
         const uint64_t b = p.coff[(int64_t)w * p.D + d], e = p.coff[(int64_t)(w + 1) * p.D + d];
         if (b == e) continue;
         T* nd = reinterpret_cast<T*>(p.ndk) + d * p.Kp;
-        bk_sample_range<MODE_DET, WT>(nd, nkp, nullptr, p, sweep, lane, b, e, kpl, ksh, nvalid, lastLane, et, lastc);
+        bk_sample_range<CH, MODE_DET, WT>(nd, nkp, nullptr, p, sweep, lane, b, e, kpl, ksh, nvalid, lastLane, et, lastc);
     }
 }
 
 // copy_merge (LDAModel.hpp:401-423): worker w samples on its private n_kw
 // (nkw + w*V*Kp) and n_k (nk_out row w) the tasks c = w, w+W, ... of
 // min(8W, D) strided tasks, each in forShuffled order.
-template <bool WT, bool ASYM>
+template <int CH, bool WT, bool ASYM>
 __global__ void __launch_bounds__(64) bk_sweep_copy_kernel(SweepParams p, int kpl, int ksh) {
     using T = bk_t<WT>;
     __shared__ float lastc[BK_MAX_CHUNKS * 64];
@@ -364,7 +366,7 @@ __global__ void __launch_bounds__(64) bk_sweep_copy_kernel(SweepParams p, int kp
             const uint64_t b = p.off[d], e = p.off[d + 1];
             if (b == e) continue;
             T* nd = reinterpret_cast<T*>(p.ndk) + d * p.Kp;
-            bk_sample_range<MODE_DET, WT>(nd, nkp, nullptr, q, q.sweep, lane, b, e, kpl, ksh, nvalid, lastLane, et,
+            bk_sample_range<CH, MODE_DET, WT>(nd, nkp, nullptr, q, q.sweep, lane, b, e, kpl, ksh, nvalid, lastLane, et,
                                           lastc);
         }
     }
@@ -376,7 +378,7 @@ __global__ void __launch_bounds__(64) bk_sweep_copy_kernel(SweepParams p, int kp
 // documents come longest first from a global counter; n_kw takes the
 // changes as atomics.  Integer counts add each worker's n_k change to nk_out
 // at the end; float counts get n_k from the merge's row sums.
-template <bool WT, bool ASYM>
+template <int CH, bool WT, bool ASYM>
 __global__ void __launch_bounds__(64) bk_sweep_device_kernel(SweepParams p, int kpl, int ksh) {
     using T = bk_t<WT>;
     __shared__ float lastc[BK_MAX_CHUNKS * 64];
@@ -387,7 +389,7 @@ __global__ void __launch_bounds__(64) bk_sweep_device_kernel(SweepParams p, int 
     T* nkp = reinterpret_cast<T*>(p.bk_scratch) + (int64_t)blockIdx.x * 2 * p.Kp;
     T* colbuf = nkp + p.Kp;
     const T* snap = reinterpret_cast<const T*>(p.nk_in);
-    bk_copy(nkp, snap, lane, kpl);
+    bk_copy<CH>(nkp, snap, lane, kpl);
     for (;;) {
         uint32_t idx = 0;
         if (lane == 0) idx = atomicAdd(p.counter, 1u);
@@ -395,18 +397,18 @@ __global__ void __launch_bounds__(64) bk_sweep_device_kernel(SweepParams p, int 
         if ((int64_t)idx >= p.D) break;
         const uint32_t d = p.doc_order[idx];
         T* nd = reinterpret_cast<T*>(p.ndk) + (int64_t)d * p.Kp;
-        bk_sample_range<MODE_ATOMIC, WT>(nd, nkp, colbuf, p, p.sweep, lane, p.off[d], p.off[d + 1], kpl, ksh, nvalid,
+        bk_sample_range<CH, MODE_ATOMIC, WT>(nd, nkp, colbuf, p, p.sweep, lane, p.off[d], p.off[d + 1], kpl, ksh, nvalid,
                                          lastLane, et, lastc);
     }
     if constexpr (!WT) {
-        for (int c = 0; c < kpl / BK_CH; ++c) {
-            ivec<BK_CH> a, s;
-            load_kvec<BK_CH>(nkp + c * BK_CHUNK, lane, a);
-            load_kvec<BK_CH>(snap + c * BK_CHUNK, lane, s);
+        for (int c = 0; c < kpl / CH; ++c) {
+            ivec<CH> a, s;
+            load_kvec<CH>(nkp + c * (64 * CH), lane, a);
+            load_kvec<CH>(snap + c * (64 * CH), lane, s);
 #pragma unroll
-            for (int j = 0; j < BK_CH; ++j) {
+            for (int j = 0; j < CH; ++j) {
                 const int dlt = a[j] - s[j];
-                if (dlt) atomicAdd(p.nk_out + c * BK_CHUNK + (j >> 2) * 256 + lane * 4 + (j & 3), dlt);
+                if (dlt) atomicAdd(p.nk_out + kpos_rt(kpl, lane * kpl + c * CH + j), dlt);
             }
         }
     }
@@ -415,6 +417,7 @@ __global__ void __launch_bounds__(64) bk_sweep_device_kernel(SweepParams p, int 
 // Frozen single steps: for token t the state (n_dk of doc[t], the column of
 // word[t], n_k) is copied into this wavefront's scratch rows, the token is
 // removed, sampled with u[t] and added back.
+template <int CH>
 __global__ void __launch_bounds__(64) bk_sample_frozen_kernel(const int32_t* __restrict__ ndk,
                                                               const int32_t* __restrict__ nkw,
                                                               const int32_t* __restrict__ nk, const float* alpha,
@@ -431,11 +434,11 @@ __global__ void __launch_bounds__(64) bk_sample_frozen_kernel(const int32_t* __r
     int32_t* col = nd + Kp;
     int32_t* nkc = col + Kp;
     for (int64_t t = blockIdx.x; t < n; t += gridDim.x) {
-        bk_copy(nd, ndk + doc[t] * Kp, lane, kpl);
-        bk_copy(col, nkw + (int64_t)word[t] * Kp, lane, kpl);
-        bk_copy(nkc, nk, lane, kpl);
+        bk_copy<CH>(nd, ndk + doc[t] * Kp, lane, kpl);
+        bk_copy<CH>(col, nkw + (int64_t)word[t] * Kp, lane, kpl);
+        bk_copy<CH>(nkc, nk, lane, kpl);
         float tot;
-        const int zn = bk_token<false>(nd, col, nkc, alpha, lane, kpl, ksh, (int)z_old[t], 1.0f, u[t], K, nvalid,
+        const int zn = bk_token<CH, false>(nd, col, nkc, alpha, lane, kpl, ksh, (int)z_old[t], 1.0f, u[t], K, nvalid,
                                        lastLane, et, lastc, &tot);
         if (lane == 0) {
             z_new[t] = (uint16_t)zn;
@@ -453,7 +456,7 @@ __global__ void __launch_bounds__(64) bk_sample_frozen_kernel(const int32_t* __r
 // order; then `iters` sequential sweeps.  Integer counts reproduce the
 // integer fold-in (n_kw + c_d per run); float counts follow tomoto's clamped
 // updates in token order.
-template <bool WT, bool ASYM>
+template <int CH, bool WT, bool ASYM>
 __global__ void __launch_bounds__(64) bk_infer_gen_kernel(InferGenParams p, int kpl, int ksh) {
     using T = bk_t<WT>;
     __shared__ float lastc[BK_MAX_CHUNKS * 64];
@@ -477,8 +480,8 @@ __global__ void __launch_bounds__(64) bk_infer_gen_kernel(InferGenParams p, int 
         const uint64_t b = J.b, e = J.e;
         T* nd = reinterpret_cast<T*>(p.ndk_out) + (int64_t)J.slot * p.Kp;
         T* nk = reinterpret_cast<T*>(p.nk_out) + (int64_t)J.slot * p.Kp;
-        bk_fill(nd, lane, kpl);
-        bk_copy(nk, reinterpret_cast<const T*>(p.nk), lane, kpl);
+        bk_fill<CH>(nd, lane, kpl);
+        bk_copy<CH>(nk, reinterpret_cast<const T*>(p.nk), lane, kpl);
         // initial topics, added in token order; each run's column starts as the model's
         {
             int64_t run = p.job_run[idx] - 1;
@@ -519,7 +522,7 @@ __global__ void __launch_bounds__(64) bk_infer_gen_kernel(InferGenParams p, int 
                     if (v != vcur) {
                         ++run;
                         col = cells + run * p.Kp;
-                        bk_copy(col, nkwT + (int64_t)v * p.Kp, lane, kpl);
+                        bk_copy<CH>(col, nkwT + (int64_t)v * p.Kp, lane, kpl);
                         vcur = v;
                     }
                     const int k = readlane_i(zz, i);
@@ -557,7 +560,7 @@ __global__ void __launch_bounds__(64) bk_infer_gen_kernel(InferGenParams p, int 
                         vcur = v;
                     }
                     const int zo = readlane_i(zv, i);
-                    const int zn = bk_token<WT>(nd, col, nk, p.alpha, lane, kpl, ksh, zo, readlane_f(tw, i),
+                    const int zn = bk_token<CH, WT>(nd, col, nk, p.alpha, lane, kpl, ksh, zo, readlane_f(tw, i),
                                                 readlane_f(uv, i), p.K, nvalid, lastLane, et, lastc, nullptr);
                     zkeep = (lane == i) ? zn : zkeep;
                 }
@@ -574,10 +577,10 @@ __global__ void __launch_bounds__(64) bk_infer_gen_kernel(InferGenParams p, int 
 // (queued 64 at a time in LDS so every lane evaluates one pair), then one
 // wavefront per document adds the n_dk and n_k terms and its chunk sums in
 // chunk order.
-template <bool WT, bool ASYM>
+template <int CH, bool WT, bool ASYM>
 __global__ void __launch_bounds__(64) bk_infer_gen_runs_kernel(InferGenParams p, int kpl) {
     using T = bk_t<WT>;
-    using CV = typename std::conditional<WT, fvec<BK_CH>, ivec<BK_CH>>::type;
+    using CV = typename std::conditional<WT, fvec<CH>, ivec<CH>>::type;
     __shared__ double qa[128], qb[128];
     const int lane = lane_id();
     const T* nkw = reinterpret_cast<const T*>(p.nkw);
@@ -593,23 +596,23 @@ __global__ void __launch_bounds__(64) bk_infer_gen_runs_kernel(InferGenParams p,
             const uint32_t v = p.run_word[r];
             int pr = -1;
             if constexpr (ASYM) pr = p.prior_row[v];
-            for (int c = 0; c < kpl / BK_CH; ++c) {
+            for (int c = 0; c < kpl / CH; ++c) {
                 CV m0, m1;
                 if constexpr (WT) {
-                    load_kvec_f<BK_CH>(nkw + (int64_t)v * p.Kp + c * BK_CHUNK, lane, m0);
-                    load_kvec_f<BK_CH>(cells + r * p.Kp + c * BK_CHUNK, lane, m1);
+                    load_kvec_f<CH>(nkw + (int64_t)v * p.Kp + c * (64 * CH), lane, m0);
+                    load_kvec_f<CH>(cells + r * p.Kp + c * (64 * CH), lane, m1);
                 } else {
-                    load_kvec<BK_CH>(nkw + (int64_t)v * p.Kp + c * BK_CHUNK, lane, m0);
-                    load_kvec<BK_CH>(cells + r * p.Kp + c * BK_CHUNK, lane, m1);
+                    load_kvec<CH>(nkw + (int64_t)v * p.Kp + c * (64 * CH), lane, m0);
+                    load_kvec<CH>(cells + r * p.Kp + c * (64 * CH), lane, m1);
                 }
 #pragma unroll
-                for (int j = 0; j < BK_CH; ++j) {
+                for (int j = 0; j < CH; ++j) {
                     const T c0 = m0[j], c1 = m1[j];
-                    const bool d = lane * kpl + c * BK_CH + j < p.K && c0 != c1;
+                    const bool d = lane * kpl + c * CH + j < p.K && c0 != c1;
                     const unsigned long long m = __ballot(d);
                     if (!m) continue;
                     if (d) {
-                        const int q = c * BK_CHUNK + (j >> 2) * 256 + lane * 4 + (j & 3);
+                        const int q = kpos_rt(kpl, lane * kpl + c * CH + j);
                         const double e = ASYM && pr >= 0 ? (double)p.prior_eta[(int64_t)pr * p.Kp + q] : p.eta_d;
                         const int at = qn + __popcll(m & below);
                         qa[at] = c1 != (T)0 ? (double)c1 + e : e;
@@ -649,7 +652,7 @@ __global__ void __launch_bounds__(64) bk_infer_gen_docs_kernel(InferGenParams p,
         for (int j = 0; j < kpl; ++j) {
             const int k = lane * kpl + j;
             if (k >= p.K) break;
-            const int q = (j >> 2) * 256 + lane * 4 + (j & 3);
+            const int q = kpos_rt(kpl, k);
             const float a = p.alpha[q];
             const T x = nd[q];
             p.theta[(int64_t)J.slot * p.K + k] = ((float)x + a) / den;
@@ -673,15 +676,31 @@ int bk_shift(int kpl) {
     return s;
 }
 
-bool bk_kpl_ok(int kpl) { return kpl > kSmallMaxKpl && kpl <= kBigMaxKpl && (kpl & (kpl - 1)) == 0; }
+bool bk_kpl_ok(int kpl) { return kpl >= 1 && kpl <= kBigMaxKpl && (kpl & (kpl - 1)) == 0; }
 
-#define LDA_BK_VARIANTS(WT, ASYM, CALL)                                   \
-    if (WT) {                                                             \
-        if (ASYM) { constexpr bool W_ = true, A_ = true; CALL; }          \
-        else { constexpr bool W_ = true, A_ = false; CALL; }              \
-    } else {                                                              \
-        if (ASYM) { constexpr bool W_ = false, A_ = true; CALL; }         \
-        else { constexpr bool W_ = false, A_ = false; CALL; }             \
+// Instantiations: CH = 16 for every count type (K above 256, which includes
+// every K above 2048); CH = KPL < 16 for integer counts only (the exact-count
+// path of K <= 2048, taken for documents over 2^20 tokens or alpha >= 2^20;
+// float counts need no such path).
+#define LDA_BK_WA(WT, ASYM, ...)                                          \
+    if (WT) {                                                              \
+        if (ASYM) { constexpr bool W_ = true, A_ = true; __VA_ARGS__; }           \
+        else { constexpr bool W_ = true, A_ = false; __VA_ARGS__; }               \
+    } else {                                                               \
+        if (ASYM) { constexpr bool W_ = false, A_ = true; __VA_ARGS__; }          \
+        else { constexpr bool W_ = false, A_ = false; __VA_ARGS__; }              \
+    }
+#define LDA_BK_A(ASYM, ...)                                               \
+    if (ASYM) { constexpr bool W_ = false, A_ = true; __VA_ARGS__; }              \
+    else { constexpr bool W_ = false, A_ = false; __VA_ARGS__; }
+#define LDA_BK_DISPATCH(KPL_, WT, ASYM, ...)                              \
+    switch ((KPL_) >= 16 ? 16 : (KPL_)) {                                  \
+        case 16: { constexpr int CH_ = 16; LDA_BK_WA(WT, ASYM, __VA_ARGS__) } break; \
+        case 8: if (WT) return hipErrorInvalidValue; { constexpr int CH_ = 8; LDA_BK_A(ASYM, __VA_ARGS__) } break; \
+        case 4: if (WT) return hipErrorInvalidValue; { constexpr int CH_ = 4; LDA_BK_A(ASYM, __VA_ARGS__) } break; \
+        case 2: if (WT) return hipErrorInvalidValue; { constexpr int CH_ = 2; LDA_BK_A(ASYM, __VA_ARGS__) } break; \
+        case 1: if (WT) return hipErrorInvalidValue; { constexpr int CH_ = 1; LDA_BK_A(ASYM, __VA_ARGS__) } break; \
+        default: return hipErrorInvalidValue;                              \
     }
 
 }  // namespace
@@ -690,34 +709,39 @@ bool bk_kpl_ok(int kpl) { return kpl > kSmallMaxKpl && kpl <= kBigMaxKpl && (kpl
 hipError_t launch_bk_sweep_det(int kpl, const SweepParams& p, bool weighted, hipStream_t s) {
     if (!bk_kpl_ok(kpl)) return hipErrorInvalidValue;
     const int sh = bk_shift(kpl);
-    LDA_BK_VARIANTS(weighted, p.prior_row != nullptr,
-                    hipLaunchKernelGGL((bk_sweep_det_kernel<W_, A_>), dim3(p.W), dim3(64), 0, s, p, kpl, sh));
+    LDA_BK_DISPATCH(kpl, weighted, p.prior_row != nullptr,
+                    hipLaunchKernelGGL((bk_sweep_det_kernel<CH_, W_, A_>), dim3(p.W), dim3(64), 0, s, p, kpl, sh));
     return hipGetLastError();
 }
 
 hipError_t launch_bk_sweep_copy(int kpl, const SweepParams& p, bool weighted, hipStream_t s) {
     if (!bk_kpl_ok(kpl)) return hipErrorInvalidValue;
     const int sh = bk_shift(kpl);
-    LDA_BK_VARIANTS(weighted, p.prior_row != nullptr,
-                    hipLaunchKernelGGL((bk_sweep_copy_kernel<W_, A_>), dim3(p.W), dim3(64), 0, s, p, kpl, sh));
+    LDA_BK_DISPATCH(kpl, weighted, p.prior_row != nullptr,
+                    hipLaunchKernelGGL((bk_sweep_copy_kernel<CH_, W_, A_>), dim3(p.W), dim3(64), 0, s, p, kpl, sh));
     return hipGetLastError();
 }
 
 hipError_t launch_bk_sweep_device(int kpl, const SweepParams& p, bool weighted, int grid, hipStream_t s) {
     if (!bk_kpl_ok(kpl) || !p.bk_scratch || grid <= 0) return hipErrorInvalidValue;
     const int sh = bk_shift(kpl);
-    LDA_BK_VARIANTS(weighted, p.prior_row != nullptr,
-                    hipLaunchKernelGGL((bk_sweep_device_kernel<W_, A_>), dim3(grid), dim3(64), 0, s, p, kpl, sh));
+    LDA_BK_DISPATCH(kpl, weighted, p.prior_row != nullptr,
+                    hipLaunchKernelGGL((bk_sweep_device_kernel<CH_, W_, A_>), dim3(grid), dim3(64), 0, s, p, kpl, sh));
     return hipGetLastError();
 }
 
+// Resident one-wavefront workgroups per CU of the CH = 16 kernels (the other
+// instantiations use fewer registers): the persistent sweeps take work from
+// counters and the partition rounds do not wait on each other, so a count
+// above the real one only queues workgroups.
 int bk_sweep_blocks_per_cu(bool weighted, bool asym, bool device) {
     int n = 0;
     hipError_t e = hipSuccess;
-    LDA_BK_VARIANTS(weighted, asym,
-                    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                        &n, device ? (const void*)bk_sweep_device_kernel<W_, A_> : (const void*)bk_sweep_det_kernel<W_, A_>,
-                        64, 0));
+    constexpr int CH_ = 16;
+    LDA_BK_WA(weighted, asym,
+              e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                  &n, device ? (const void*)bk_sweep_device_kernel<CH_, W_, A_> : (const void*)bk_sweep_det_kernel<CH_, W_, A_>,
+                  64, 0));
     return e == hipSuccess ? n : 0;
 }
 
@@ -726,8 +750,10 @@ hipError_t launch_bk_sample_frozen(int kpl, const int32_t* ndk, const int32_t* n
                                    const int64_t* doc, const uint32_t* word, const uint16_t* z_old, const float* u,
                                    uint16_t* z_new, float* total, int32_t* scratch, hipStream_t s) {
     if (!bk_kpl_ok(kpl) || !scratch || grid <= 0) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(bk_sample_frozen_kernel, dim3(grid), dim3(64), 0, s, ndk, nkw, nk, alpha, eta, veta, K, Kp, n,
-                       doc, word, z_old, u, z_new, total, kpl, bk_shift(kpl), scratch);
+    const int sh = bk_shift(kpl);
+    LDA_BK_DISPATCH(kpl, false, false,
+                    hipLaunchKernelGGL(bk_sample_frozen_kernel<CH_>, dim3(grid), dim3(64), 0, s, ndk, nkw, nk, alpha, eta,
+                                       veta, K, Kp, n, doc, word, z_old, u, z_new, total, kpl, sh, scratch));
     return hipGetLastError();
 }
 
@@ -735,14 +761,14 @@ hipError_t launch_bk_infer_gen(int kpl, const InferGenParams& p, int grid, bool 
     if (!bk_kpl_ok(kpl) || grid <= 0) return hipErrorInvalidValue;
     const bool wt = p.wt != nullptr, asym = p.prior_row != nullptr;
     if (finalize) {
-        LDA_BK_VARIANTS(wt, asym, {
-            hipLaunchKernelGGL((bk_infer_gen_runs_kernel<W_, A_>), dim3(grid), dim3(64), 0, s, p, kpl);
+        LDA_BK_DISPATCH(kpl, wt, asym, {
+            hipLaunchKernelGGL((bk_infer_gen_runs_kernel<CH_, W_, A_>), dim3(grid), dim3(64), 0, s, p, kpl);
             hipLaunchKernelGGL((bk_infer_gen_docs_kernel<W_, A_>), dim3(grid), dim3(64), 0, s, p, kpl);
         });
     } else {
         const int sh = bk_shift(kpl);
-        LDA_BK_VARIANTS(wt, asym,
-                        hipLaunchKernelGGL((bk_infer_gen_kernel<W_, A_>), dim3(grid), dim3(64), 0, s, p, kpl, sh));
+        LDA_BK_DISPATCH(kpl, wt, asym,
+                        hipLaunchKernelGGL((bk_infer_gen_kernel<CH_, W_, A_>), dim3(grid), dim3(64), 0, s, p, kpl, sh));
     }
     return hipGetLastError();
 }
@@ -750,7 +776,8 @@ hipError_t launch_bk_infer_gen(int kpl, const InferGenParams& p, int grid, bool 
 int bk_infer_gen_blocks_per_cu(bool wt, bool asym) {
     int n = 0;
     hipError_t e = hipSuccess;
-    LDA_BK_VARIANTS(wt, asym, e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, bk_infer_gen_kernel<W_, A_>, 64, 0));
+    constexpr int CH_ = 16;
+    LDA_BK_WA(wt, asym, e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, bk_infer_gen_kernel<CH_, W_, A_>, 64, 0));
     return e == hipSuccess ? n : 0;
 }
 

@@ -276,6 +276,7 @@ lda::SweepParams base_params(lda_ctx* c) {
         p.nkw16 = c->d_nkw16.p;
         p.nkw_hot = c->nkw_hot;
     }
+    p.exact = c->exact_counts() ? 1 : 0;
     return p;
 }
 
@@ -289,7 +290,7 @@ int narrow_begin(lda_ctx* c, int W, bool copy_merge) {
     c->narrow_active = false;
     c->narrow_used = false;
     if (!c->narrow_enabled || W != 0 || copy_merge || c->tw != LDA_TW_ONE || c->nkw_hot >= c->V ||
-        c->kpl > lda::kSmallMaxKpl)
+        c->kpl > lda::kSmallMaxKpl || c->exact_counts())
         return LDA_OK;
     const hipStream_t s = c->stream;
     HIP_TRY(c->d_nkw16.alloc((size_t)(c->V - c->nkw_hot) * c->Kp));
@@ -360,17 +361,19 @@ int device_wide_grid(lda_ctx* c, int* waves_out) {
                    : lda::sweep_device_blocks_per_cu(c->kpl, c->has_priors, waves);
     };
     if (!c->dev_auto) {
-        const int waves = c->kpl > lda::kSmallMaxKpl ? 1 : c->dev_waves;
+        const bool one = c->kpl > lda::kSmallMaxKpl || c->exact_counts();
+        const int waves = one ? 1 : c->dev_waves;
         int grid = c->num_cu * per_cu(waves);
         if (c->dev_workers > 0) grid = std::min(grid, c->dev_workers);
         *waves_out = waves;
         return grid;
     }
     const int64_t inflight = std::max<int64_t>((c->D + kInFlightDiv - 1) / kInFlightDiv, 1);
-    if (c->kpl > lda::kSmallMaxKpl) {
-        // K above 2048: one-wavefront workers only
+    if (c->kpl > lda::kSmallMaxKpl || c->exact_counts()) {
+        // K above 2048 or exact counts: one-wavefront workers only
         *waves_out = 1;
-        return (int)std::min<int64_t>(inflight, (int64_t)c->num_cu * per_cu(1));
+        const int per = std::max(lda::bk_sweep_blocks_per_cu(wtd, c->has_priors, true), 1);
+        return (int)std::min<int64_t>(inflight, (int64_t)c->num_cu * per);
     }
     const int W8 = lda::kMaxSweepWaves;
     const int full8 = c->num_cu * per_cu(W8);
@@ -502,7 +505,7 @@ int sweep_once(lda_ctx* c, int W, bool copy_merge) {
             p.counter = c->d_counter.p;
             int waves = 1;
             const int grid = device_wide_grid(c, &waves);
-            if (c->kpl > lda::kSmallMaxKpl) {
+            if (c->kpl > lda::kSmallMaxKpl || p.exact) {
                 HIP_TRY(c->d_bk_scratch.alloc((size_t)grid * 2 * c->Kp));
                 p.bk_scratch = c->d_bk_scratch.p;
             }
@@ -567,7 +570,7 @@ int sweep_once(lda_ctx* c, int W, bool copy_merge) {
         p.counter = c->d_counter.p;
         int waves = 1;
         const int grid = device_wide_grid(c, &waves);
-        if (c->kpl > lda::kSmallMaxKpl) {
+        if (c->kpl > lda::kSmallMaxKpl || p.exact) {
             HIP_TRY(c->d_bk_scratch.alloc((size_t)grid * 2 * c->Kp));
             p.bk_scratch = c->d_bk_scratch.p;
         }
@@ -660,12 +663,8 @@ int optimize_alpha(lda_ctx* c) {
     HIP_TRY(hipMemcpyAsync(&bad, c->d_bad.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     for (int k = 0; k < c->K; ++k) c->alpha[k] = a[c->pos(k)];
-    if (bad) {
-        int k = 0;
-        while (k < c->K && c->alpha[k] < kMaxAlpha) ++k;
-        return fail(LDA_ERR_TRAINING, "alpha optimisation left the supported range (alpha_%d = %g)", k,
-                    k < c->K ? (double)c->alpha[k] : 0.0);
-    }
+    // an alpha reached 2^20: integer counts sample on the exact-count kernels from now on
+    if (bad) c->exact_alpha = true;
     return upload_alpha(c);  // the cumulative sums for the initial draws
 }
 
@@ -832,7 +831,7 @@ int lda_ctx_create(int K, int V, const float* alpha, float eta, int termweight, 
     if (!alpha) return fail(LDA_ERR_INVALID_ARGUMENT, "alpha is NULL");
     for (int k = 0; k < K; ++k) {
         if (!(alpha[k] > 0.0f)) return fail(LDA_ERR_INVALID_ARGUMENT, "wrong alpha value");
-        if (!(alpha[k] < kMaxAlpha)) return fail(LDA_ERR_UNSUPPORTED, "alpha must be below 2^20 on the device");
+        if (!(alpha[k] < 3.0e38f)) return fail(LDA_ERR_INVALID_ARGUMENT, "wrong alpha value");
     }
     if (!(eta > 0.0f)) return fail(LDA_ERR_INVALID_ARGUMENT, "wrong eta value (eta = %f)", (double)eta);
     if (termweight != LDA_TW_ONE && termweight != LDA_TW_IDF && termweight != LDA_TW_PMI)
@@ -852,6 +851,7 @@ int lda_ctx_create(int K, int V, const float* alpha, float eta, int termweight, 
     c->tw = termweight;
     c->seed = seed;
     c->alpha.assign(alpha, alpha + K);
+    for (int k = 0; k < K; ++k) c->exact_alpha = c->exact_alpha || !(alpha[k] < kMaxAlpha);
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cu = prop.multiProcessorCount;
     auto cleanup = [&](int rc) {
@@ -978,8 +978,6 @@ int lda_upload_corpus(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off
     if (doc_off[0] != 0) return fail(LDA_ERR_INVALID_ARGUMENT, "doc_off[0] must be 0");
     for (int64_t d = 0; d < D; ++d) {
         if (doc_off[d + 1] < doc_off[d]) return fail(LDA_ERR_INVALID_ARGUMENT, "doc_off must be non-decreasing");
-        if (doc_off[d + 1] - doc_off[d] > kMaxDocTokens)
-            return fail(LDA_ERR_UNSUPPORTED, "document %lld has more than 2^20 tokens", (long long)d);
     }
     const uint64_t N = doc_off[D];
     if (N && !words) return fail(LDA_ERR_INVALID_ARGUMENT, "words is NULL");
@@ -1177,9 +1175,11 @@ int lda_set_alpha(lda_ctx* c, const float* alpha) {
     if (!c || !alpha) return fail(LDA_ERR_INVALID_ARGUMENT, "NULL argument");
     for (int k = 0; k < c->K; ++k) {
         if (!(alpha[k] > 0.0f)) return fail(LDA_ERR_INVALID_ARGUMENT, "wrong alpha value");
-        if (!(alpha[k] < kMaxAlpha)) return fail(LDA_ERR_UNSUPPORTED, "alpha must be below 2^20 on the device");
+        if (!(alpha[k] < 3.0e38f)) return fail(LDA_ERR_INVALID_ARGUMENT, "wrong alpha value");
     }
     c->alpha.assign(alpha, alpha + c->K);
+    c->exact_alpha = false;
+    for (int k = 0; k < c->K; ++k) c->exact_alpha = c->exact_alpha || !(alpha[k] < kMaxAlpha);
     HIP_TRY(hipSetDevice(c->device));
     return upload_alpha(c);
 }
@@ -1722,7 +1722,7 @@ int lda_sample_frozen(lda_ctx* c, int64_t n, const int64_t* doc, const uint32_t*
     // K above 2048: three rows of Kp counts per workgroup
     const int bk_grid = (int)std::min<int64_t>(n, 1024);
     if (dd.alloc(n) || dw.alloc(n) || dz.alloc(n) || dzn.alloc(n) || du.alloc(n) || dt.alloc(n) ||
-        (c->kpl > lda::kSmallMaxKpl && dsc.alloc((size_t)bk_grid * 3 * c->Kp)))
+        ((c->kpl > lda::kSmallMaxKpl || c->exact_counts()) && dsc.alloc((size_t)bk_grid * 3 * c->Kp)))
         rc = fail(LDA_ERR_OUT_OF_MEMORY, "allocation failed");
     if (!rc && (hipMemcpyAsync(dd.p, doc, n * 8, hipMemcpyHostToDevice, s) || hipMemcpyAsync(dw.p, word, n * 4, hipMemcpyHostToDevice, s) ||
                 hipMemcpyAsync(dz.p, z_old, n * 2, hipMemcpyHostToDevice, s) || hipMemcpyAsync(du.p, u, n * 4, hipMemcpyHostToDevice, s)))
@@ -1754,8 +1754,6 @@ static int check_infer_args(lda_ctx* c, const uint32_t* words, const uint64_t* d
     if (doc_off[D] && !words) return fail(LDA_ERR_INVALID_ARGUMENT, "NULL argument");
     for (int64_t d = 0; d < D; ++d) {
         if (doc_off[d + 1] < doc_off[d]) return fail(LDA_ERR_INVALID_ARGUMENT, "doc_off must be non-decreasing");
-        if (doc_off[d + 1] - doc_off[d] > kMaxDocTokens)
-            return fail(LDA_ERR_UNSUPPORTED, "document %lld has more than 2^20 tokens", (long long)d);
     }
     std::atomic<int64_t> unsorted(INT64_MAX);
     host_parallel(D, 1024, [&](int64_t lo, int64_t hi) {
@@ -1795,11 +1793,19 @@ int lda_host::infer_general_on(lda_ctx* c, lda_host::GenFold& B, hipStream_t s, 
     const bool wt = c->tw != LDA_TW_ONE;
     const uint64_t N = doc_off[D];
     constexpr int RW = lda::kMaxFoldSegments + 1;
+    // integer counts with a document over 2^20 tokens or an alpha >= 2^20:
+    // the exact-count kernels (one wavefront per document)
+    bool exact = false;
+    if (!wt) {
+        exact = c->exact_alpha;
+        for (int64_t d = 0; d < D && !exact; ++d) exact = doc_off[d + 1] - doc_off[d] > lda_host::kMaxDocTokens;
+    }
     int per_cu[lda::kFoldBuckets] = {};
     int gen_max = 1;
     for (int bk = 0; bk < lda::kFoldBuckets; ++bk) {
         const int sg = 1 << bk;
-        per_cu[bk] = sg <= c->seg_max ? lda::infer_gen_blocks_per_cu(c->kpl, wt, c->has_priors, sg) : 0;
+        if (exact) per_cu[bk] = sg == 1 ? std::max(lda::bk_infer_gen_blocks_per_cu(false, c->has_priors), 1) : 0;
+        else per_cu[bk] = sg <= c->seg_max ? lda::infer_gen_blocks_per_cu(c->kpl, wt, c->has_priors, sg) : 0;
         if (per_cu[bk] <= 0) break;
         gen_max = sg;
     }
@@ -1910,6 +1916,7 @@ int lda_host::infer_general_on(lda_ctx* c, lda_host::GenFold& B, hipStream_t s, 
     HIP_TRY(hipMemcpyAsync(B.segb.p, seg_b.data(), seg_b.size() * 8, hipMemcpyHostToDevice, s));
     if (R) HIP_TRY(hipMemcpyAsync(B.runword.p, run_word.data(), R * 4, hipMemcpyHostToDevice, s));
     lda::InferGenParams p{};
+    p.exact = exact ? 1 : 0;
     p.words = B.words.p; p.z = B.z.p; p.wt = wt ? B.wt.p : nullptr;
     p.jobs = B.jobs.p; p.njobs = D;
     p.job_run = B.jobrun.p; p.seg_b = B.segb.p; p.seg_r = B.segr.p;
@@ -2025,7 +2032,9 @@ int lda_infer(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off, int64_
         return fail(LDA_ERR_INVALID_ARGUMENT, "term-weighted models need the token weights: call lda_infer_w");
     // K above 2048: the run-column fold-in (one wavefront per document), the
     // same draws as the register-resident sampler for integer counts
-    if (c->kpl > lda::kSmallMaxKpl)
+    bool run_column = c->kpl > lda::kSmallMaxKpl || c->exact_alpha;
+    for (int64_t d = 0; d < D && !run_column; ++d) run_column = doc_off[d + 1] - doc_off[d] > kMaxDocTokens;
+    if (run_column)
         return infer_general(c, words, doc_off, D, nullptr, iterations, infer_seed, mode, theta, ll, z_out);
     const uint64_t N = doc_off[D];
     HIP_TRY(hipSetDevice(c->device));
@@ -2098,6 +2107,11 @@ int lda_infer_noalloc(lda_ctx* c, const uint32_t* words, const uint64_t* doc_off
         return fail(LDA_ERR_INVALID_ARGUMENT, "term-weighted models need the token weights: call lda_infer_w");
     if (c->kpl > lda::kSmallMaxKpl)
         return fail(LDA_ERR_UNSUPPORTED, "lda_infer_noalloc takes K <= 2048 (K = %d: use lda_infer)", c->K);
+    if (c->exact_alpha)
+        return fail(LDA_ERR_UNSUPPORTED, "lda_infer_noalloc takes alpha below 2^20 (use lda_infer)");
+    for (int64_t d = 0; d < D; ++d)
+        if (doc_off[d + 1] - doc_off[d] > kMaxDocTokens)
+            return fail(LDA_ERR_UNSUPPORTED, "lda_infer_noalloc takes documents of at most 2^20 tokens (use lda_infer)");
     const uint64_t N = doc_off[D];
     // the arena inside the scratch array, each region 256-byte aligned
     const size_t o_params = 0;

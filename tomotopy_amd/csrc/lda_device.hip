@@ -32,7 +32,8 @@ namespace {
 // LDS state of the sampler, every array Kp = 64*KPL entries in the lane
 // layout (kpos).  Per wavefront (one document at a time):
 //   A    float(n_dk) + alpha_k; n_dk itself is rint(A - alpha_k), exact while
-//        n_dk + alpha_k < 2^21 (document length and alpha are checked on the host)
+//        n_dk + alpha_k < 2^21 (the host sends models with a document over 2^20
+//        tokens or an alpha >= 2^20 to the exact-count kernels of lda_bigk.hip)
 //   col  n_kw[., v] of the current word, including this wavefront's updates
 // Per workgroup (the worker; its wavefronts share one private n_k):
 //   nk   the worker-private n_k (LDS atomics)
@@ -2578,7 +2579,8 @@ hipError_t launch_build_counts(int kpl, const uint32_t* words, const uint16_t* z
 
 hipError_t launch_sweep_device(int kpl, const SweepParams& p, int grid, int waves, hipStream_t s) {
     if (waves < 1 || waves > kSweepWaves) return hipErrorInvalidValue;
-    if (kpl > kSmallMaxKpl) return waves == 1 && !p.nkw16 ? launch_bk_sweep_device(kpl, p, false, grid, s) : hipErrorInvalidValue;
+    if (kpl > kSmallMaxKpl || p.exact)
+        return waves == 1 && !p.nkw16 ? launch_bk_sweep_device(kpl, p, false, grid, s) : hipErrorInvalidValue;
     if (p.nkw16) {
         LDA_DISPATCH_KPL_ETA(kpl, p.prior_row != nullptr,
                              hipLaunchKernelGGL((sweep_device_kernel<KPL, ASYM, true>), dim3(grid), dim3(64 * waves),
@@ -2592,7 +2594,7 @@ hipError_t launch_sweep_device(int kpl, const SweepParams& p, int grid, int wave
 }
 
 hipError_t launch_sweep_copy(int kpl, const SweepParams& p, hipStream_t s) {
-    if (kpl > kSmallMaxKpl) return launch_bk_sweep_copy(kpl, p, false, s);
+    if (kpl > kSmallMaxKpl || p.exact) return launch_bk_sweep_copy(kpl, p, false, s);
     LDA_DISPATCH_KPL_ETA(kpl, p.prior_row != nullptr,
                          hipLaunchKernelGGL((sweep_copy_kernel<KPL, ASYM>), dim3(p.W), dim3(64), wave_lds_bytes<KPL>(), s, p));
     return hipGetLastError();
@@ -2615,7 +2617,7 @@ hipError_t launch_replicate(int32_t* dst, const int32_t* src, int64_t n, int W, 
 }
 
 hipError_t launch_sweep_det(int kpl, const SweepParams& p, hipStream_t s) {
-    if (kpl > kSmallMaxKpl) return launch_bk_sweep_det(kpl, p, false, s);
+    if (kpl > kSmallMaxKpl || p.exact) return launch_bk_sweep_det(kpl, p, false, s);
     LDA_DISPATCH_KPL_ETA(kpl, p.prior_row != nullptr,
                          hipLaunchKernelGGL((sweep_det_kernel<KPL, ASYM>), dim3(p.W), dim3(64), wave_lds_bytes<KPL>(), s, p));
     return hipGetLastError();
@@ -2935,7 +2937,7 @@ hipError_t launch_sample_frozen(int kpl, const int32_t* ndk, const int32_t* nkw,
                                 float* total, int32_t* bk_scratch, int bk_grid, hipStream_t s) {
     (void)V;
     if (n <= 0) return hipSuccess;
-    if (kpl > kSmallMaxKpl)
+    if (kpl > kSmallMaxKpl || bk_scratch)
         return launch_bk_sample_frozen(kpl, ndk, nkw, nk, alpha, eta, veta, K, Kp, n, bk_grid, doc, word, z_old, u, z_new,
                                        total, bk_scratch, s);
     int64_t grid = n < 65536 ? n : 65536;
@@ -3000,7 +3002,8 @@ static hipError_t launch_infer_gen_t(int kpl, int segs, const InferGenParams& p,
 
 hipError_t launch_infer_gen(int kpl, int segs, const InferGenParams& p, int grid, bool finalize, hipStream_t s) {
     if (segs != 1 && segs != 2 && segs != 4 && segs != kMaxFoldSegments) return hipErrorInvalidValue;
-    if (kpl > kSmallMaxKpl) return segs == 1 ? launch_bk_infer_gen(kpl, p, grid, finalize, s) : hipErrorInvalidValue;
+    if (kpl > kSmallMaxKpl || p.exact)
+        return segs == 1 ? launch_bk_infer_gen(kpl, p, grid, finalize, s) : hipErrorInvalidValue;
     const bool wt = p.wt != nullptr, asym = p.prior_row != nullptr;
     if (wt) return asym ? launch_infer_gen_t<true, true>(kpl, segs, p, grid, finalize, s)
                         : launch_infer_gen_t<true, false>(kpl, segs, p, grid, finalize, s);

@@ -44,8 +44,10 @@ struct DevBuf {
     }
 };
 
-// The sampler keeps float(n_dk) + alpha_k and recovers n_dk by rounding, which
-// is exact while n_dk + alpha_k < 2^21: documents and alpha are bounded here.
+// The K <= 2048 samplers keep float(n_dk) + alpha_k and recover n_dk by
+// rounding, which is exact while n_dk + alpha_k < 2^21.  Integer counts with a
+// document over kMaxDocTokens tokens or an alpha >= kMaxAlpha sample on the
+// exact-count kernels instead (SweepParams::exact).
 constexpr uint64_t kMaxDocTokens = 1u << 20;
 constexpr float kMaxAlpha = (float)(1u << 20);
 
@@ -110,6 +112,11 @@ struct lda_ctx {
     uint64_t global_step = 0;
     uint32_t optim_interval = 10, burn_in = 0;
     int num_cu = 1;
+
+    // an alpha reached 2^20 (set or optimised): integer counts then sample on
+    // the exact-count kernels (SweepParams::exact)
+    bool exact_alpha = false;
+    bool exact_counts() const { return tw == LDA_TW_ONE && (exact_alpha || (uint64_t)max_len > lda_host::kMaxDocTokens); }
 
     // corpus
     int64_t D = 0;

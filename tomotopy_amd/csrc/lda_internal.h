@@ -59,6 +59,11 @@ struct SweepParams {
     // K above 2048 (kpl > kSmallMaxKpl), device-wide sweep: per worker two
     // rows of Kp counts (its private n_k and its copy of the current column)
     void* bk_scratch;
+    // integer counts with a document over 2^20 tokens or an alpha >= 2^20:
+    // the K <= 2048 kernels keep float(n_dk) + alpha and recover n_dk from it,
+    // exact only below 2^21, so these run on the lda_bigk.hip kernels (n_dk
+    // kept as integers) at any K
+    int exact;
 };
 
 // Topics per lane: the register / LDS kernels take kpl <= kSmallMaxKpl
@@ -176,6 +181,9 @@ struct InferGenParams {
     const int64_t* chunk_base;
     const int64_t* chunk_rb;
     double* chunk_ll;
+    // integer counts with a document over 2^20 tokens or an alpha >= 2^20
+    // (see SweepParams::exact): the lda_bigk.hip fold-in at any K
+    int exact;
 };
 constexpr int64_t kGenChunkRuns = 128;
 // finalize = false: the sampler (grid persistent workgroups of one

@@ -21,8 +21,8 @@
 //     document's result does not depend on the requests it shared the device
 //     with nor on how its iterations were split into steps: it equals
 //     lda_infer on the same document bit for bit.
-// Term-weighted (IDF / PMI) models, and K above 2048, are served by the
-// run-column fold-in instead: float counts depend on the order of their
+// Term-weighted (IDF / PMI) models, K above 2048 and alpha >= 2^20 are
+// served by the run-column fold-in instead: float counts depend on the order of their
 // updates and cannot be rebuilt from z, so every document runs all its
 // iterations within the step that starts it.  Each lane is a thread that
 // takes the queued requests in FIFO order (up to the step's document and
@@ -515,7 +515,7 @@ int lda_serve_start(lda_ctx* ctx, uint64_t hbm_budget_bytes, int iterations, int
     if (hbm_budget_bytes < (64u << 10)) return fail(LDA_ERR_INVALID_ARGUMENT, "HBM budget below 64 KiB");
     HIP_TRY(hipSetDevice(ctx->device));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
-    if (ctx->tw != LDA_TW_ONE || ctx->kpl > lda::kSmallMaxKpl) {
+    if (ctx->tw != LDA_TW_ONE || ctx->kpl > lda::kSmallMaxKpl || ctx->exact_alpha) {
         // run-column steps: per lane and step at most max_active documents
         // and arena_cap tokens, sized so that a step's private columns (at
         // most one Kp column per token), state rows and arrays fit the budget
@@ -629,8 +629,10 @@ int submit_impl(lda_server* s, const uint32_t* words, const uint64_t* doc_off, i
     if (D && doc_off[0] != 0) return fail(LDA_ERR_INVALID_ARGUMENT, "doc_off[0] must be 0");
     for (int64_t d = 0; d < D; ++d) {
         if (doc_off[d + 1] < doc_off[d]) return fail(LDA_ERR_INVALID_ARGUMENT, "doc_off must be non-decreasing");
-        if (doc_off[d + 1] - doc_off[d] > lda_host::kMaxDocTokens)
-            return fail(LDA_ERR_UNSUPPORTED, "document %lld has more than 2^20 tokens", (long long)d);
+        if (!s->generic && doc_off[d + 1] - doc_off[d] > lda_host::kMaxDocTokens)
+            return fail(LDA_ERR_UNSUPPORTED,
+                        "document %lld has more than 2^20 tokens: this server keeps float(n_dk) + alpha per topic; "
+                        "fold it in with lda_infer", (long long)d);
     }
     const uint64_t N = D ? doc_off[D] : 0;
     if (N && !words) return fail(LDA_ERR_INVALID_ARGUMENT, "NULL argument");
