@@ -255,3 +255,33 @@ def test_model_surface_big_k():
     assert np.isfinite(ll)
     with pytest.raises(ValueError):
         tp.LDAModel(k=32768)
+
+
+def test_fold_in_batch_split_into_groups():
+    """At K = 32767 a batch whose private columns exceed the launch budget
+    (8 GB: here 400 documents of about 250 runs x 128 KiB) is folded in as
+    consecutive groups; every document's result equals its fold-in alone and
+    the oracle's."""
+    K = 32767
+    words, off, nw, _ = synth.workload("large_k1000", n_docs=600)
+    ws, offs, V, _ = corpus.prepare_arrays(words, off, nw)
+    ctx = _native.Context(K, V, 0.05, 0.01, seed=3)
+    ctx.upload_corpus(ws[: int(offs[200])], offs[:201].astype(np.uint64))
+    ctx.set_optim(0, 0)
+    ctx.train(1, workers=0)
+    z, _, _, _ = ctx.download_state()
+    uw, uoff = ws[int(offs[200]):int(offs[600])], (offs[200:601] - offs[200]).astype(np.uint64)
+    runs = sum(len(np.unique(uw[int(uoff[d]):int(uoff[d + 1])])) for d in range(400))
+    print("runs", runs, "GB", runs * 32768 * 4 / 1e9, flush=True)
+    assert runs * 32768 * 4 > 8e9
+    th, ll, zg = ctx.infer(uw, uoff, 3, infer_seed=9, mode=0, want_z=True)
+    for d in (0, 137, 399):
+        b, e = int(uoff[d]), int(uoff[d + 1])
+        th1, ll1, z1 = ctx.infer(uw[b:e], np.array([0, e - b], np.uint64), 3, infer_seed=9, mode=0, want_z=True)
+        assert np.array_equal(th1[0], th[d]) and ll1[0] == ll[d] and np.array_equal(z1, zg[b:e])
+    m = O.Model(K, V, 0.05, 0.01, 3, scan_order=1)
+    m.set_corpus(ws[: int(offs[200])], offs[:201].astype(np.uint64), z)
+    sel = np.array([0, uoff[1], uoff[2]], np.uint64)
+    tho, llo, zo = m.infer(uw[: int(uoff[2])], sel, 3, infer_seed=9)
+    assert np.array_equal(zo, zg[: int(uoff[2])]) and np.array_equal(tho, th[:2])
+    np.testing.assert_allclose(llo, ll[:2], rtol=1e-9)

@@ -1786,7 +1786,59 @@ static uint64_t seg_start_host(const uint32_t* w, uint64_t n, int i, int S) {
 // finalize kernel of its documents; mode 1: one launch per document.  Arguments checked by check_infer_args.
 }  // extern "C"
 
+static int infer_general_batch(lda_ctx* c, lda_host::GenFold& B, hipStream_t s, lda_host::FoldStreams& fs,
+                               hipEvent_t e0, hipEvent_t e1, const uint32_t* words, const uint64_t* doc_off, int64_t D,
+                               const float* weights, int iterations, uint64_t infer_seed, int mode, float* theta,
+                               double* ll, uint16_t* z_out, float* ms_out);
+
+// Private columns of one fold-in launch: batches whose runs need more are
+// folded in as consecutive groups of documents (each document's result does
+// not depend on the others in its batch).
+constexpr double kGenCellBudget = 8e9;
+
 int lda_host::infer_general_on(lda_ctx* c, lda_host::GenFold& B, hipStream_t s, lda_host::FoldStreams& fs,
+                               hipEvent_t e0, hipEvent_t e1, const uint32_t* words, const uint64_t* doc_off, int64_t D,
+                               const float* weights, int iterations, uint64_t infer_seed, int mode, float* theta,
+                               double* ll, uint16_t* z_out, float* ms_out) {
+    const uint32_t Vu = (uint32_t)c->V;
+    std::vector<int64_t> runs(D, 0);
+    host_parallel(D, 256, [&](int64_t lo, int64_t hi) {
+        for (int64_t d = lo; d < hi; ++d) {
+            int64_t nr = 0, prev = -1;
+            for (uint64_t g = doc_off[d]; g < doc_off[d + 1]; ++g) {
+                const uint32_t v = words[g];
+                if (v >= Vu || (int64_t)v == prev) continue;
+                ++nr;
+                prev = v;
+            }
+            runs[d] = nr;
+        }
+    });
+    const double per_run = (double)c->Kp * 4.0;
+    float total_ms = 0.0f;
+    std::vector<uint64_t> off;
+    for (int64_t g0 = 0; g0 < D;) {
+        int64_t g1 = g0 + 1;
+        double bytes = (double)runs[g0] * per_run;
+        while (g1 < D && bytes + (double)runs[g1] * per_run <= kGenCellBudget) bytes += (double)runs[g1++] * per_run;
+        if (g0 == 0 && g1 == D) return infer_general_batch(c, B, s, fs, e0, e1, words, doc_off, D, weights, iterations,
+                                                          infer_seed, mode, theta, ll, z_out, ms_out);
+        const uint64_t base = doc_off[g0];
+        off.resize((size_t)(g1 - g0) + 1);
+        for (int64_t d = g0; d <= g1; ++d) off[(size_t)(d - g0)] = doc_off[d] - base;
+        float ms = 0.0f;
+        const int rc = infer_general_batch(c, B, s, fs, e0, e1, words + base, off.data(), g1 - g0,
+                                           weights ? weights + base : nullptr, iterations, infer_seed, mode,
+                                           theta + (size_t)g0 * c->K, ll + g0, z_out ? z_out + base : nullptr, &ms);
+        if (rc) return rc;
+        total_ms += ms;
+        g0 = g1;
+    }
+    if (ms_out) *ms_out = total_ms;
+    return LDA_OK;
+}
+
+static int infer_general_batch(lda_ctx* c, lda_host::GenFold& B, hipStream_t s, lda_host::FoldStreams& fs,
                                hipEvent_t e0, hipEvent_t e1, const uint32_t* words, const uint64_t* doc_off, int64_t D,
                                const float* weights, int iterations, uint64_t infer_seed, int mode, float* theta,
                                double* ll, uint16_t* z_out, float* ms_out) {
@@ -1877,7 +1929,7 @@ int lda_host::infer_general_on(lda_ctx* c, lda_host::GenFold& B, hipStream_t s, 
     chunk_rb.push_back(job_run[D]);
     const double cell_bytes = (double)R * c->Kp * 4.0;
     if (cell_bytes > 64e9)
-        return fail(LDA_ERR_OUT_OF_MEMORY, "fold-in batch needs %.1f GB of private columns; split the batch", cell_bytes / 1e9);
+        return fail(LDA_ERR_OUT_OF_MEMORY, "a document needs %.1f GB of private columns", cell_bytes / 1e9);
     int64_t lo[lda::kFoldBuckets] = {}, hi[lda::kFoldBuckets] = {};
     if (mode == 0) {
         int64_t i = 0;
