@@ -710,6 +710,161 @@ def serve_both():
     return out
 
 
+def big_k_bench(K=4096, n_docs=10000, sweeps=5, fold_docs=1000, fold_iters=20, cpu_budget_s=8.0):
+    """K above 2048 (the device-memory-state samplers): the device-wide sweep
+    on the large_k1000 generator cut to n_docs documents (tokens/s), and the
+    fold-in of fold_docs of its documents (docs/s); the CPU oracle's
+    partition sweep and fold-in on bounded samples of the same data."""
+    from oracle import oracle as O
+    from tomotopy_amd import _native, corpus, synth
+    words, off, nw, _ = synth.workload("large_k1000", n_docs=n_docs + fold_docs)
+    ws, offs, V, _ = corpus.prepare_arrays(words, off, nw)
+    tw, toff = ws[: int(offs[n_docs])], offs[: n_docs + 1].astype(np.uint64)
+    uw, uoff = ws[int(offs[n_docs]):], (offs[n_docs:] - offs[n_docs]).astype(np.uint64)
+    n_valid = int(np.count_nonzero(tw < V))
+    ctx = _native.Context(K, V, 0.05, 0.01, seed=77)
+    ctx.upload_corpus(tw, toff)
+    ctx.set_optim(0, 0)
+    _, W = ctx.resolve_schedule(0, 0)
+    ctx.train(2, workers=0)
+    t = time.perf_counter()
+    ctx.train(sweeps, workers=0)
+    dt = (time.perf_counter() - t) / sweeps
+    ms = ctx.kernel_stats()["ms"] / max(1, ctx.kernel_stats()["launches"])
+    ctx.infer(uw[: int(uoff[50])], uoff[:51], 2)
+    t = time.perf_counter()
+    ctx.infer(uw, uoff, fold_iters)
+    dtf = time.perf_counter() - t
+    out = {"workload": "large_k1000 cut to %d docs" % n_docs, "K": K, "tokens": n_valid, "workers": W,
+           "sweep_ms": dt * 1e3, "kernel_ms": ms, "tokens_per_s": n_valid / dt,
+           "fold_in": {"docs": fold_docs, "iterations": fold_iters, "docs_per_s": fold_docs / dtf}}
+    # CPU oracle: partition sweeps on a sample, fold-in of a document sample
+    z, _, _, _ = ctx.download_state()
+    thr = host_threads()
+    sd = max(1, min(n_docs, 400))
+    m = O.Model(K, V, 0.05, 0.01, 77, scan_order=1)
+    m.set_optim(0, 0)
+    m.set_corpus(tw[: int(toff[sd])], toff[: sd + 1])
+    m.train(1, workers=thr, threads=thr)
+    t, n = time.perf_counter(), 0
+    while time.perf_counter() - t < cpu_budget_s / 2:
+        m.train(1, workers=thr, threads=thr)
+        n += 1
+    stoks = int(np.count_nonzero(tw[: int(toff[sd])] < V))
+    out["cpu_baseline"] = {"value": stoks * n / (time.perf_counter() - t), "unit": "tokens/s", "cores": thr,
+                           "kind": "port", "sample": "first %d docs (%d tokens), partition with %d workers, %d sweeps"
+                                                     % (sd, stoks, thr, n)}
+    m2 = O.Model(K, V, 0.05, 0.01, 77, scan_order=1)
+    m2.set_corpus(tw, toff, z)
+    fd = 4 * thr
+    from concurrent.futures import ThreadPoolExecutor
+
+    def fold(d):  # the oracle releases the GIL
+        b, e = int(uoff[d]), int(uoff[d + 1])
+        m2.infer(uw[b:e], np.array([0, e - b], np.uint64), fold_iters, infer_seed=1)
+
+    t = time.perf_counter()
+    with ThreadPoolExecutor(thr) as ex:
+        list(ex.map(fold, range(fd)))
+    out["fold_in"]["cpu_baseline"] = {"value": fd / (time.perf_counter() - t), "unit": "docs/s", "cores": thr,
+                                      "sample": "first %d documents, %d iterations, one per thread task"
+                                                % (fd, fold_iters)}
+    log("K=%d: device-wide %.1f ms/sweep (%d workers) = %.3g tokens/s, CPU oracle %.3g; fold-in %.0f docs/s, CPU %.0f"
+        % (K, dt * 1e3, W, n_valid / dt, out["cpu_baseline"]["value"], fold_docs / dtf,
+           out["fold_in"]["cpu_baseline"]["value"]))
+    ctx.close()
+    return out
+
+
+def serve_pmi_bench(iterations=50, plain_requests=60, seed=7, rates=(50, 400)):
+    """The server of a TermWeight.PMI model (run-column steps): the serve_k500
+    workload with a K=500 PMI model; saturation throughput on 1 and 2 lanes,
+    p50 / p99 at Poisson arrival rates, plain per-request lda_infer_w back to
+    back, and served results against lda_infer_w on the same requests."""
+    import threading
+    from tomotopy_amd import _native, corpus, synth
+    (tw, toff, nw), reqs_raw = synth.serve_workload(600)
+    ws, offs, V, order = corpus.prepare_arrays(tw, toff, nw)
+    reqs = [corpus.csr_from_arrays(w, o, order) for w, o in reqs_raw]
+    vw = corpus.vocab_weights(ws, offs, V, "pmi")
+    wts = corpus.term_weights(ws, offs, V, "pmi")
+    ctx = _native.Context(500, V, 0.1, 0.01, seed=56, tw=2)
+    ctx.upload_corpus(ws, offs, weights=wts)
+    ctx.train(50, workers=0)
+    rq = [(w, o, vw[w].astype(np.float32)) for w, o in reqs]
+    ndocs = sum(len(o) - 1 for _, o, _ in rq)
+    from tomotopy_amd import autotune
+    allw = np.concatenate([w for w, _, _ in rq]).astype(np.uint32)
+    allx = np.concatenate([x for _, _, x in rq]).astype(np.float32)
+    alloff = np.zeros(ndocs + 1, np.uint64)
+    alloff[1:] = np.cumsum(np.concatenate([np.diff(o.astype(np.int64)) for _, o, _ in rq]))
+    tune = autotune.tune_infer(ctx, allw, alloff, iterations, infer_seed=seed, weights=allx)
+    ctx.set_infer_options(tune["seg_tokens"], tune["seg_max"])
+    log("serve PMI autotuner: seg_tokens %d seg_max %d" % (tune["seg_tokens"], tune["seg_max"]))
+    out = {"workload": "serve_k500, TermWeight.PMI", "K": 500, "iterations": iterations, "saturation": {},
+           "levels": [], "seg_tokens": tune["seg_tokens"], "seg_max": tune["seg_max"]}
+    for lanes in (1, 2):
+        with _native.Server(ctx, hbm_budget_bytes=8 << 30, iterations=iterations, infer_seed=seed,
+                            lanes=lanes) as srv:
+            srv.wait(srv.submit(*rq[0][:2], weights=rq[0][2]))
+            t0 = time.perf_counter()
+            ids = [srv.submit(w, o, weights=x) for w, o, x in rq]
+            res = [srv.wait(i) for i in ids]
+            dt = time.perf_counter() - t0
+            lat = [r[2] for r in res]
+            out["saturation"]["lanes_%d" % lanes] = {"docs_per_s": ndocs / dt, "p50_ms": _percentile(lat, 50),
+                                                     "p99_ms": _percentile(lat, 99)}
+            log("serve PMI saturation, %d lane(s): %.0f docs/s" % (lanes, ndocs / dt))
+            if lanes == 1:
+                same = True
+                for j in (0, 17, 333):
+                    th0, ll0 = ctx.infer(rq[j][0], rq[j][1], iterations, infer_seed=seed, weights=rq[j][2])[:2]
+                    same = same and np.array_equal(res[j][0], th0) and np.array_equal(res[j][1], ll0)
+                out["served_equals_infer_w"] = bool(same)
+                rng = np.random.default_rng(seed)
+                for qps in rates:
+                    n = max(8, int(qps * 3.0))
+                    gaps = rng.exponential(1.0 / qps, size=n)
+                    pick = rng.integers(0, len(rq), size=n)
+                    ids = [None] * n
+                    t_start = time.perf_counter()
+
+                    def client():
+                        due = t_start
+                        for j in range(n):
+                            due += gaps[j]
+                            d = due - time.perf_counter()
+                            if d > 0:
+                                time.sleep(d)
+                            w, o, x = rq[pick[j]]
+                            ids[j] = srv.submit(w, o, weights=x)
+
+                    th = threading.Thread(target=client)
+                    th.start()
+                    lat, docs = [], 0
+                    for j in range(n):
+                        while ids[j] is None:
+                            time.sleep(0.0005)
+                        _, ll, ms_ = srv.wait(ids[j])
+                        lat.append(ms_)
+                        docs += len(ll)
+                    th.join()
+                    el = time.perf_counter() - t_start
+                    out["levels"].append({"qps": qps, "requests": n, "docs_per_s": docs / el,
+                                          "p50_ms": _percentile(lat, 50), "p99_ms": _percentile(lat, 99)})
+                    log("serve PMI qps %d: %.0f docs/s, p50 %.1f ms, p99 %.1f ms"
+                        % (qps, docs / el, _percentile(lat, 50), _percentile(lat, 99)))
+    t0 = time.perf_counter()
+    pd = 0
+    for w, o, x in rq[:plain_requests]:
+        ctx.infer(w, o, iterations, infer_seed=seed, weights=x)
+        pd += len(o) - 1
+    out["plain_per_request"] = {"requests": plain_requests, "docs_per_s": pd / (time.perf_counter() - t0)}
+    log("serve PMI plain per-request lda_infer_w: %.0f docs/s" % out["plain_per_request"]["docs_per_s"])
+    ctx.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -722,7 +877,9 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
     ap.add_argument("--no-pmi", action="store_true", help="skip the TermWeight.PMI training variant")
     ap.add_argument("--no-partition", action="store_true", help="skip the partition-schedule legs")
-    ap.add_argument("--only", choices=["train", "infer", "infer_pmi", "serve", "probe"], default=None,
+    ap.add_argument("--no-big-k", action="store_true", help="skip the K=4096 and PMI-server legs")
+    ap.add_argument("--only", choices=["train", "infer", "infer_pmi", "serve", "probe", "big_k", "serve_pmi"],
+                    default=None,
                     help="run one leg only (the JSON line then carries that leg's numbers)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -732,8 +889,9 @@ def main():
     if args.only == "probe":
         print(json.dumps({"leg": "probe", "probe": bandwidth_probe()}), flush=True)
         return
-    if args.only in ("infer", "infer_pmi", "serve"):
-        res = {"infer": infer_bench, "infer_pmi": infer_pmi_bench, "serve": serve_both}[args.only]()
+    if args.only in ("infer", "infer_pmi", "serve", "big_k", "serve_pmi"):
+        res = {"infer": infer_bench, "infer_pmi": infer_pmi_bench, "serve": serve_both, "big_k": big_k_bench,
+               "serve_pmi": serve_pmi_bench}[args.only]()
         print(json.dumps({"metric": METRIC, "leg": args.only, args.only: res}), flush=True)
         return
 
@@ -824,6 +982,9 @@ def main():
         out["infer_pmi"] = infer_pmi_bench()
     if not args.no_serve and args.only is None:
         out["serve"] = serve_both()
+    if not args.no_big_k and args.only is None:
+        out["big_k"] = big_k_bench()
+        out["serve_pmi"] = serve_pmi_bench()
     print(json.dumps(out), flush=True)
 
 
