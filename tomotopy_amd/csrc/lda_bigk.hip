@@ -145,7 +145,10 @@ __device__ __forceinline__ void bk_weights(const bk_t<WT>* nd, const bk_t<WT>* c
 // addWordTo for topic z (LDAModel.hpp:287-299) by the owner lane: +-1 for
 // integer counts; float counts add w or take max(x - w, 0) (updateCnt,
 // Utils.hpp:132-155).
-template <bool WT>
+// SHARED: n_k is the workgroup's, in LDS, shared by its wavefronts (LDS
+// atomics; a wavefront reading it may see another's update of a moment ago,
+// the K <= 2048 device-wide sweep's rule).
+template <bool WT, bool SHARED = false>
 __device__ __forceinline__ void bk_bump(bk_t<WT>* nd, bk_t<WT>* col, bk_t<WT>* nk, int lane, int kpl, int ksh, int z,
                                         float w, bool dec) {
     if (lane == (z >> ksh)) {
@@ -153,12 +156,18 @@ __device__ __forceinline__ void bk_bump(bk_t<WT>* nd, bk_t<WT>* col, bk_t<WT>* n
         if constexpr (WT) {
             nd[q] = dec ? sub_clamp(nd[q], w) : nd[q] + w;
             col[q] = dec ? sub_clamp(col[q], w) : col[q] + w;
-            nk[q] = dec ? sub_clamp(nk[q], w) : nk[q] + w;
+            if constexpr (SHARED) {
+                if (dec) atomic_sub_clamp(nk + q, w);
+                else atomicAdd(nk + q, w);
+            } else {
+                nk[q] = dec ? sub_clamp(nk[q], w) : nk[q] + w;
+            }
         } else {
             const int d = dec ? -1 : 1;
             nd[q] += d;
             col[q] += d;
-            nk[q] += d;
+            if constexpr (SHARED) atomicAdd(nk + q, d);
+            else nk[q] += d;
         }
     }
 }
@@ -174,11 +183,11 @@ __device__ __forceinline__ void bk_prefix(fvec<CH>& w, float carry) {
 // One token: remove zo, weights, scan, search, add the new topic (returned,
 // wave-uniform).  lastc: this wavefront's LDS row of chunk last values
 // ([chunk][lane]).  *total receives cdf[K-1].
-template <int CH, bool WT, class E>
+template <int CH, bool WT, class E, bool SHARED = false>
 __device__ __forceinline__ int bk_token(bk_t<WT>* nd, bk_t<WT>* col, bk_t<WT>* nk, const float* alpha, int lane,
                                         int kpl, int ksh, int zo, float tw, float u, int K, int nvalid, int lastLane,
                                         const E& et, float* lastc, float* total) {
-    bk_bump<WT>(nd, col, nk, lane, kpl, ksh, zo, tw, true);
+    bk_bump<WT, SHARED>(nd, col, nk, lane, kpl, ksh, zo, tw, true);
     const int C = kpl / CH;
     float carry = 0.0f;
     for (int c = 0; c < C; ++c) {
@@ -212,7 +221,7 @@ __device__ __forceinline__ int bk_token(bk_t<WT>* nd, bk_t<WT>* col, bk_t<WT>* n
         zn = L * kpl + readlane_i(miss, L);
     }
     zn = uniform_i(zn);
-    bk_bump<WT>(nd, col, nk, lane, kpl, ksh, zn, tw, false);
+    bk_bump<WT, SHARED>(nd, col, nk, lane, kpl, ksh, zn, tw, false);
     if (total) *total = tot;
     return zn;
 }
@@ -238,10 +247,10 @@ __device__ __forceinline__ void bk_fill(T* dst, int lane, int kpl) {
 // owns its cells); MODE_ATOMIC: colbuf holds this wavefront's copy of the
 // column (read when the word's run starts, plus its own updates) and the
 // token's net change goes to the shared n_kw as atomics.
-template <int CH, int MODE, bool WT, class E>
-__device__ __forceinline__ void bk_sample_range(bk_t<WT>* nd, bk_t<WT>* nk, bk_t<WT>* colbuf, const SweepParams& p,
-                                                uint32_t sweep, int lane, uint64_t b, uint64_t e, int kpl, int ksh,
-                                                int nvalid, int lastLane, E& et, float* lastc) {
+template <int CH, int MODE, bool WT, class E, bool SHARED = false>
+__device__ __forceinline__ void bk_sample_range(bk_t<WT>* nd, bk_t<WT>* nk, bk_t<WT>* colbuf, const float* alpha,
+                                                const SweepParams& p, uint32_t sweep, int lane, uint64_t b, uint64_t e,
+                                                int kpl, int ksh, int nvalid, int lastLane, E& et, float* lastc) {
     using T = bk_t<WT>;
     T* nkwT = reinterpret_cast<T*>(p.nkw);
     int vcur = -1;
@@ -274,7 +283,7 @@ __device__ __forceinline__ void bk_sample_range(bk_t<WT>* nd, bk_t<WT>* nk, bk_t
             const int zo = readlane_i(zv, i);
             const float w = readlane_f(tw, i);
             const float u = readlane_f(uv, i);
-            const int zn = bk_token<CH, WT>(nd, col, nk, p.alpha, lane, kpl, ksh, zo, w, u, p.K, nvalid, lastLane, et, lastc,
+            const int zn = bk_token<CH, WT, E, SHARED>(nd, col, nk, alpha, lane, kpl, ksh, zo, w, u, p.K, nvalid, lastLane, et, lastc,
                                         nullptr);
             if (zn != zo) {
                 if constexpr (MODE == MODE_ATOMIC) {
@@ -336,7 +345,8 @@ __global__ void __launch_bounds__(64) bk_sweep_det_kernel(SweepParams p, int kpl
         const uint64_t b = p.coff[(int64_t)w * p.D + d], e = p.coff[(int64_t)(w + 1) * p.D + d];
         if (b == e) continue;
         T* nd = reinterpret_cast<T*>(p.ndk) + d * p.Kp;
-        bk_sample_range<CH, MODE_DET, WT>(nd, nkp, nullptr, p, sweep, lane, b, e, kpl, ksh, nvalid, lastLane, et, lastc);
+        bk_sample_range<CH, MODE_DET, WT>(nd, nkp, nullptr, p.alpha, p, sweep, lane, b, e, kpl, ksh, nvalid, lastLane, et,
+                                          lastc);
     }
 }
 
@@ -366,8 +376,8 @@ __global__ void __launch_bounds__(64) bk_sweep_copy_kernel(SweepParams p, int kp
             const uint64_t b = p.off[d], e = p.off[d + 1];
             if (b == e) continue;
             T* nd = reinterpret_cast<T*>(p.ndk) + d * p.Kp;
-            bk_sample_range<CH, MODE_DET, WT>(nd, nkp, nullptr, q, q.sweep, lane, b, e, kpl, ksh, nvalid, lastLane, et,
-                                          lastc);
+            bk_sample_range<CH, MODE_DET, WT>(nd, nkp, nullptr, p.alpha, q, q.sweep, lane, b, e, kpl, ksh, nvalid,
+                                              lastLane, et, lastc);
         }
     }
 }
@@ -397,8 +407,8 @@ __global__ void __launch_bounds__(64) bk_sweep_device_kernel(SweepParams p, int 
         if ((int64_t)idx >= p.D) break;
         const uint32_t d = p.doc_order[idx];
         T* nd = reinterpret_cast<T*>(p.ndk) + (int64_t)d * p.Kp;
-        bk_sample_range<CH, MODE_ATOMIC, WT>(nd, nkp, colbuf, p, p.sweep, lane, p.off[d], p.off[d + 1], kpl, ksh, nvalid,
-                                         lastLane, et, lastc);
+        bk_sample_range<CH, MODE_ATOMIC, WT>(nd, nkp, colbuf, p.alpha, p, p.sweep, lane, p.off[d], p.off[d + 1], kpl,
+                                             ksh, nvalid, lastLane, et, lastc);
     }
     if constexpr (!WT) {
         for (int c = 0; c < kpl / CH; ++c) {
@@ -410,6 +420,51 @@ __global__ void __launch_bounds__(64) bk_sweep_device_kernel(SweepParams p, int 
                 const int dlt = a[j] - s[j];
                 if (dlt) atomicAdd(p.nk_out + kpos_rt(kpl, lane * kpl + c * CH + j), dlt);
             }
+        }
+    }
+}
+
+// The same schedule with workgroups of several wavefronts sharing, in LDS,
+// alpha and one private n_k (the snapshot plus the workgroup's changes, LDS
+// atomics), so that per token only the n_dk row and the wavefront's column
+// copy (bk_scratch row blockIdx.x * waves + wave) come from device memory.
+// Dynamic LDS: alpha (Kp) | n_k (Kp) | per wavefront the chunk last values.
+extern __shared__ int bk_lds[];
+
+template <int CH, bool WT, bool ASYM>
+__global__ void __launch_bounds__(64 * kMaxSweepWaves) bk_sweep_device_lds_kernel(SweepParams p, int kpl, int ksh) {
+    using T = bk_t<WT>;
+    const int lane = lane_id();
+    const int wave = (int)(threadIdx.x >> 6), waves = (int)(blockDim.x >> 6);
+    const int Kp = p.Kp;
+    float* al = reinterpret_cast<float*>(bk_lds);
+    T* nkl = reinterpret_cast<T*>(bk_lds + Kp);
+    float* lastc = reinterpret_cast<float*>(bk_lds + 2 * Kp) + wave * (kpl / CH) * 64;
+    int nvalid, lastLane;
+    bk_lane_limits(p.K, lane, kpl, nvalid, lastLane);
+    auto et = bk_make_eta<ASYM>(p.prior_row, p.prior_eta, p.etasum, p.eta, p.veta, Kp);
+    const T* snap = reinterpret_cast<const T*>(p.nk_in);
+    for (int q = (int)threadIdx.x; q < Kp; q += (int)blockDim.x) {
+        al[q] = p.alpha[q];
+        nkl[q] = snap[q];
+    }
+    __syncthreads();
+    T* colbuf = reinterpret_cast<T*>(p.bk_scratch) + ((int64_t)blockIdx.x * waves + wave) * Kp;
+    for (;;) {
+        uint32_t idx = 0;
+        if (lane == 0) idx = atomicAdd(p.counter, 1u);
+        idx = (uint32_t)uniform_i((int)idx);
+        if ((int64_t)idx >= p.D) break;
+        const uint32_t d = p.doc_order[idx];
+        T* nd = reinterpret_cast<T*>(p.ndk) + (int64_t)d * Kp;
+        bk_sample_range<CH, MODE_ATOMIC, WT, decltype(et), true>(nd, nkl, colbuf, al, p, p.sweep, lane, p.off[d],
+                                                                 p.off[d + 1], kpl, ksh, nvalid, lastLane, et, lastc);
+    }
+    if constexpr (!WT) {
+        __syncthreads();
+        for (int q = (int)threadIdx.x; q < Kp; q += (int)blockDim.x) {
+            const int dlt = nkl[q] - snap[q];
+            if (dlt) atomicAdd(p.nk_out + q, dlt);
         }
     }
 }
@@ -722,12 +777,37 @@ hipError_t launch_bk_sweep_copy(int kpl, const SweepParams& p, bool weighted, hi
     return hipGetLastError();
 }
 
-hipError_t launch_bk_sweep_device(int kpl, const SweepParams& p, bool weighted, int grid, hipStream_t s) {
-    if (!bk_kpl_ok(kpl) || !p.bk_scratch || grid <= 0) return hipErrorInvalidValue;
+size_t bk_sweep_device_lds_bytes(int kpl, int Kp, int waves) {
+    const int ch = kpl >= 16 ? 16 : kpl;
+    return ((size_t)2 * Kp + (size_t)waves * (kpl / ch) * 64) * sizeof(int);
+}
+
+hipError_t launch_bk_sweep_device(int kpl, const SweepParams& p, bool weighted, int grid, int waves, hipStream_t s) {
+    if (!bk_kpl_ok(kpl) || !p.bk_scratch || grid <= 0 || waves < 1 || waves > kMaxSweepWaves) return hipErrorInvalidValue;
     const int sh = bk_shift(kpl);
-    LDA_BK_DISPATCH(kpl, weighted, p.prior_row != nullptr,
-                    hipLaunchKernelGGL((bk_sweep_device_kernel<CH_, W_, A_>), dim3(grid), dim3(64), 0, s, p, kpl, sh));
+    if (waves == 1) {
+        LDA_BK_DISPATCH(kpl, weighted, p.prior_row != nullptr,
+                        hipLaunchKernelGGL((bk_sweep_device_kernel<CH_, W_, A_>), dim3(grid), dim3(64), 0, s, p, kpl, sh));
+    } else {
+        const size_t lds = bk_sweep_device_lds_bytes(kpl, p.Kp, waves);
+        LDA_BK_DISPATCH(kpl, weighted, p.prior_row != nullptr,
+                        hipLaunchKernelGGL((bk_sweep_device_lds_kernel<CH_, W_, A_>), dim3(grid), dim3(64 * waves), lds,
+                                           s, p, kpl, sh));
+    }
     return hipGetLastError();
+}
+
+// Resident workgroups per CU of the LDS variant (0: it does not fit).
+int bk_sweep_device_lds_blocks_per_cu(bool weighted, bool asym, int kpl, int Kp, int waves) {
+    const size_t lds = bk_sweep_device_lds_bytes(kpl, Kp, waves);
+    if (lds > 160 * 1024) return 0;
+    int n = 0;
+    hipError_t e = hipSuccess;
+    constexpr int CH_ = 16;
+    LDA_BK_WA(weighted, asym,
+              e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, bk_sweep_device_lds_kernel<CH_, W_, A_>, 64 * waves,
+                                                               lds));
+    return e == hipSuccess ? n : 0;
 }
 
 // Resident one-wavefront workgroups per CU of the CH = 16 kernels (the other

@@ -353,6 +353,7 @@ int device_copy_workers(lda_ctx* c) {
 // those, else as 8-wavefront workgroups up to the resident capacity.  Large
 // corpora (D >= 8 x resident wavefronts) run the full grid.
 constexpr int64_t kInFlightDiv = 8;
+constexpr int kBigSweepWaves = 4;
 
 int device_wide_grid(lda_ctx* c, int* waves_out) {
     const bool wtd = c->tw != LDA_TW_ONE;
@@ -370,7 +371,17 @@ int device_wide_grid(lda_ctx* c, int* waves_out) {
     }
     const int64_t inflight = std::max<int64_t>((c->D + kInFlightDiv - 1) / kInFlightDiv, 1);
     if (c->kpl > lda::kSmallMaxKpl || c->exact_counts()) {
-        // K above 2048 or exact counts: one-wavefront workers only
+        // K above 2048 or exact counts: workgroups of kBigSweepWaves wavefronts
+        // sharing alpha and n_k in LDS while four such workgroups fit a CU
+        // (K <= 4096), else one-wavefront workers with device-memory n_k
+        if (lda::bk_sweep_device_lds_bytes(c->kpl, c->Kp, kBigSweepWaves) <= 40 * 1024) {
+            const int per = lda::bk_sweep_device_lds_blocks_per_cu(wtd, c->has_priors, c->kpl, c->Kp, kBigSweepWaves);
+            if (per > 0) {
+                *waves_out = kBigSweepWaves;
+                const int64_t wg = (inflight + kBigSweepWaves - 1) / kBigSweepWaves;
+                return (int)std::max<int64_t>(1, std::min<int64_t>(wg, (int64_t)c->num_cu * per));
+            }
+        }
         *waves_out = 1;
         const int per = std::max(lda::bk_sweep_blocks_per_cu(wtd, c->has_priors, true), 1);
         return (int)std::min<int64_t>(inflight, (int64_t)c->num_cu * per);
@@ -506,7 +517,7 @@ int sweep_once(lda_ctx* c, int W, bool copy_merge) {
             int waves = 1;
             const int grid = device_wide_grid(c, &waves);
             if (c->kpl > lda::kSmallMaxKpl || p.exact) {
-                HIP_TRY(c->d_bk_scratch.alloc((size_t)grid * 2 * c->Kp));
+                HIP_TRY(c->d_bk_scratch.alloc((size_t)grid * std::max(2, waves) * c->Kp));
                 p.bk_scratch = c->d_bk_scratch.p;
             }
             HIP_TRY(hipEventRecord(c->ev0, s));
@@ -571,7 +582,7 @@ int sweep_once(lda_ctx* c, int W, bool copy_merge) {
         int waves = 1;
         const int grid = device_wide_grid(c, &waves);
         if (c->kpl > lda::kSmallMaxKpl || p.exact) {
-            HIP_TRY(c->d_bk_scratch.alloc((size_t)grid * 2 * c->Kp));
+            HIP_TRY(c->d_bk_scratch.alloc((size_t)grid * std::max(2, waves) * c->Kp));
             p.bk_scratch = c->d_bk_scratch.p;
         }
         HIP_TRY(hipEventRecord(c->ev0, s));

@@ -2580,7 +2580,7 @@ hipError_t launch_build_counts(int kpl, const uint32_t* words, const uint16_t* z
 hipError_t launch_sweep_device(int kpl, const SweepParams& p, int grid, int waves, hipStream_t s) {
     if (waves < 1 || waves > kSweepWaves) return hipErrorInvalidValue;
     if (kpl > kSmallMaxKpl || p.exact)
-        return waves == 1 && !p.nkw16 ? launch_bk_sweep_device(kpl, p, false, grid, s) : hipErrorInvalidValue;
+        return !p.nkw16 ? launch_bk_sweep_device(kpl, p, false, grid, waves, s) : hipErrorInvalidValue;
     if (p.nkw16) {
         LDA_DISPATCH_KPL_ETA(kpl, p.prior_row != nullptr,
                              hipLaunchKernelGGL((sweep_device_kernel<KPL, ASYM, true>), dim3(grid), dim3(64 * waves),
@@ -2811,7 +2811,7 @@ hipError_t launch_alpha_update_hist(float* alpha, const int32_t* hist, int Lmax,
 
 hipError_t launch_sweep_device_w(int kpl, const SweepParams& p, int grid, int waves, hipStream_t s) {
     if (waves < 1 || waves > kSweepWaves) return hipErrorInvalidValue;
-    if (kpl > kSmallMaxKpl) return waves == 1 ? launch_bk_sweep_device(kpl, p, true, grid, s) : hipErrorInvalidValue;
+    if (kpl > kSmallMaxKpl) return launch_bk_sweep_device(kpl, p, true, grid, waves, s);
     LDA_DISPATCH_KPL_ETA(kpl, p.prior_row != nullptr,
                          hipLaunchKernelGGL((sweep_device_w_kernel<KPL, ASYM>), dim3(grid), dim3(64 * waves),
                                             wave_lds_bytes<KPL>(waves), s, p));

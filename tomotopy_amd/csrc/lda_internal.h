@@ -312,7 +312,12 @@ int infer_blocks_per_cu(int kpl, int segs, bool asym);
 // Fold-in: one wavefront per document, no segmentation.
 hipError_t launch_bk_sweep_det(int kpl, const SweepParams& p, bool weighted, hipStream_t s);
 hipError_t launch_bk_sweep_copy(int kpl, const SweepParams& p, bool weighted, hipStream_t s);
-hipError_t launch_bk_sweep_device(int kpl, const SweepParams& p, bool weighted, int grid, hipStream_t s);
+// waves == 1: one-wavefront workers, n_k and alpha read from device memory;
+// waves > 1: workgroups of `waves` wavefronts sharing alpha and n_k in LDS
+// (bk_scratch then holds one column row per wavefront)
+hipError_t launch_bk_sweep_device(int kpl, const SweepParams& p, bool weighted, int grid, int waves, hipStream_t s);
+size_t bk_sweep_device_lds_bytes(int kpl, int Kp, int waves);
+int bk_sweep_device_lds_blocks_per_cu(bool weighted, bool asym, int kpl, int Kp, int waves);
 int bk_sweep_blocks_per_cu(bool weighted, bool asym, bool device);
 hipError_t launch_bk_sample_frozen(int kpl, const int32_t* ndk, const int32_t* nkw, const int32_t* nk,
                                    const float* alpha, float eta, float veta, int K, int Kp, int64_t n, int grid,
