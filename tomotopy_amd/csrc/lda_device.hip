@@ -2326,6 +2326,10 @@ __global__ void __launch_bounds__(SEG ? 64 * kMaxFoldSegments : 64) infer_w_kern
             int64_t run = run0 - 1;
             int vcur = -1;
             float* colp = nullptr;
+            // this lane's cells of the open run changed in this pass (it owned a
+            // removed or added topic): only those lanes store the column back;
+            // the others' cells in HBM are still current
+            bool touched = false;
             fvec<KPL> c1 = 0.0f, c2 = 0.0f;  // the columns of runs run + 1 and run + 2
             if (run0 < rend) fold_fetch_col_w<KPL>(cells + run0 * p.Kp, lane, live, c1);
             if (run0 + 1 < rend) fold_fetch_col_w<KPL>(cells + (run0 + 1) * p.Kp, lane, live, c2);
@@ -2345,7 +2349,8 @@ __global__ void __launch_bounds__(SEG ? 64 * kMaxFoldSegments : 64) infer_w_kern
                     const int v = readlane_i(wv, i);
                     if ((unsigned)v >= (unsigned)p.V) continue;
                     if (v != vcur) {
-                        if (colp) fold_store_col_w<KPL>(st, colp, lane, live);
+                        if (colp) fold_store_col_w<KPL>(st, colp, lane, live && touched);
+                        touched = false;
                         ++run;
                         colp = cells + run * p.Kp;
                         fold_load_col_w<KPL>(st, c1);
@@ -2353,13 +2358,15 @@ __global__ void __launch_bounds__(SEG ? 64 * kMaxFoldSegments : 64) infer_w_kern
                         if (run + 2 < rend) fold_fetch_col_w<KPL>(colp + 2 * p.Kp, lane, live, c2);
                         vcur = v;
                     }
-                    const int zn = fold_token_w<KPL>(st, lane, readlane_i(zv, i), readlane_f(tw, i), readlane_f(uv, i),
-                                                     p.K, nvalid, lastLane, eta, veta);
+                    const int zo = readlane_i(zv, i);
+                    const int zn = fold_token_w<KPL>(st, lane, zo, readlane_f(tw, i), readlane_f(uv, i), p.K, nvalid,
+                                                     lastLane, eta, veta);
+                    touched = touched || lane == (zo >> kpl_shift<KPL>()) || lane == (zn >> kpl_shift<KPL>());
                     zkeep = (lane == i) ? zn : zkeep;
                 }
                 if (lane < n) p.z[g] = (uint16_t)zkeep;
             }
-            if (colp) fold_store_col_w<KPL>(st, colp, lane, live);
+            if (colp) fold_store_col_w<KPL>(st, colp, lane, live && touched);
             if (S > 1) {
                 fvec<KPL> a, c;
 #pragma unroll
