@@ -97,50 +97,57 @@ __device__ __forceinline__ typename BkEtaOf<ASYM>::type bk_make_eta(const int32_
     else return BkEtaSym{eta, veta};
 }
 
-// The 16 weights of chunk c of this lane (getZLikelihoods, LDAModel.hpp:273-285),
-// in the K <= 2048 sampler's operations: integer counts form A = float(n_dk) +
-// alpha, B = float(n_kw) + eta, b = float(n_k) + V*eta; float counts
-// (n_dk + alpha)(n_kw + eta) and n_k + V*eta; then q = a/b as a*y with one
-// Markstein correction, y = RN(1/b).
-template <int CH, bool WT, class E>
-__device__ __forceinline__ void bk_weights(const bk_t<WT>* nd, const bk_t<WT>* col, const bk_t<WT>* nk,
-                                           const float* alpha, int c, int lane, const E& et, fvec<CH>& w) {
-    const int o = c * (64 * CH);
+// One chunk of this lane's state (CH topics of n_dk, the column, n_k, alpha
+// and the eta terms) in registers, loaded apart from its use so that the
+// next chunk's loads are in flight while this one is weighed.
+template <int CH, bool WT>
+struct BkChunk {
+    using V = typename std::conditional<WT, fvec<CH>, ivec<CH>>::type;
+    V nd, col, nk;
     fvec<CH> al, e, s;
-    load_kvec_f<CH>(alpha + o, lane, al);
-    et.template chunk<CH>(c, lane, e, s);
-    if constexpr (WT) {
-        fvec<CH> a0, c0, n0;
-        load_kvec_f<CH>(nd + o, lane, a0);
-        load_kvec_f<CH>(col + o, lane, c0);
-        load_kvec_f<CH>(nk + o, lane, n0);
-#pragma unroll
-        for (int j = 0; j < CH; ++j) {
-            const float a = (a0[j] + al[j]) * (c0[j] + e[j]);
-            const float bf = n0[j] + s[j];
-            const float y = recip_rn(bf);
-            const float q0 = a * y;
-            const float rr = __builtin_fmaf(-q0, bf, a);
-            w[j] = __builtin_fmaf(rr, y, q0);
+
+    template <class E>
+    __device__ __forceinline__ void load(const bk_t<WT>* ndp, const bk_t<WT>* colp, const bk_t<WT>* nkp,
+                                         const float* alpha, int c, int lane, const E& et) {
+        const int o = c * (64 * CH);
+        load_kvec_f<CH>(alpha + o, lane, al);
+        et.template chunk<CH>(c, lane, e, s);
+        if constexpr (WT) {
+            load_kvec_f<CH>(ndp + o, lane, nd);
+            load_kvec_f<CH>(colp + o, lane, col);
+            load_kvec_f<CH>(nkp + o, lane, nk);
+        } else {
+            load_kvec<CH>(ndp + o, lane, nd);
+            load_kvec<CH>(colp + o, lane, col);
+            load_kvec<CH>(nkp + o, lane, nk);
         }
-    } else {
-        ivec<CH> a0, c0, n0;
-        load_kvec<CH>(nd + o, lane, a0);
-        load_kvec<CH>(col + o, lane, c0);
-        load_kvec<CH>(nk + o, lane, n0);
+    }
+
+    // The weights (getZLikelihoods, LDAModel.hpp:273-285) in the K <= 2048
+    // sampler's operations: integer counts form A = float(n_dk) + alpha,
+    // B = float(n_kw) + eta, b = float(n_k) + V*eta; float counts
+    // (n_dk + alpha)(n_kw + eta) and n_k + V*eta; then q = a/b as a*y with
+    // one Markstein correction, y = RN(1/b).
+    __device__ __forceinline__ void weights(fvec<CH>& w) const {
 #pragma unroll
         for (int j = 0; j < CH; ++j) {
-            const float A = (float)a0[j] + al[j];
-            const float B = (float)c0[j] + e[j];
-            const float bf = (float)n0[j] + s[j];
+            float a, bf;
+            if constexpr (WT) {
+                a = (nd[j] + al[j]) * (col[j] + e[j]);
+                bf = nk[j] + s[j];
+            } else {
+                const float A = (float)nd[j] + al[j];
+                const float B = (float)col[j] + e[j];
+                bf = (float)nk[j] + s[j];
+                a = A * B;
+            }
             const float y = recip_rn(bf);
-            const float a = A * B;
             const float q0 = a * y;
             const float rr = __builtin_fmaf(-q0, bf, a);
             w[j] = __builtin_fmaf(rr, y, q0);
         }
     }
-}
+};
 
 // addWordTo for topic z (LDAModel.hpp:287-299) by the owner lane: +-1 for
 // integer counts; float counts add w or take max(x - w, 0) (updateCnt,
@@ -152,22 +159,35 @@ template <bool WT, bool SHARED = false>
 __device__ __forceinline__ void bk_bump(bk_t<WT>* nd, bk_t<WT>* col, bk_t<WT>* nk, int lane, int kpl, int ksh, int z,
                                         float w, bool dec) {
     if (lane == (z >> ksh)) {
+        // the three cells are read before any is written: the compiler cannot
+        // tell the arrays apart, and interleaved read-modify-writes would
+        // cost three memory round trips instead of one
         const int q = kpos_rt(kpl, z);
+        const bk_t<WT> a = nd[q], c = col[q];
         if constexpr (WT) {
-            nd[q] = dec ? sub_clamp(nd[q], w) : nd[q] + w;
-            col[q] = dec ? sub_clamp(col[q], w) : col[q] + w;
             if constexpr (SHARED) {
+                nd[q] = dec ? sub_clamp(a, w) : a + w;
+                col[q] = dec ? sub_clamp(c, w) : c + w;
                 if (dec) atomic_sub_clamp(nk + q, w);
                 else atomicAdd(nk + q, w);
             } else {
-                nk[q] = dec ? sub_clamp(nk[q], w) : nk[q] + w;
+                const float n = nk[q];
+                nd[q] = dec ? sub_clamp(a, w) : a + w;
+                col[q] = dec ? sub_clamp(c, w) : c + w;
+                nk[q] = dec ? sub_clamp(n, w) : n + w;
             }
         } else {
             const int d = dec ? -1 : 1;
-            nd[q] += d;
-            col[q] += d;
-            if constexpr (SHARED) atomicAdd(nk + q, d);
-            else nk[q] += d;
+            if constexpr (SHARED) {
+                nd[q] = a + d;
+                col[q] = c + d;
+                atomicAdd(nk + q, d);
+            } else {
+                const int n = nk[q];
+                nd[q] = a + d;
+                col[q] = c + d;
+                nk[q] = n + d;
+            }
         }
     }
 }
@@ -182,20 +202,39 @@ __device__ __forceinline__ void bk_prefix(fvec<CH>& w, float carry) {
 
 // One token: remove zo, weights, scan, search, add the new topic (returned,
 // wave-uniform).  lastc: this wavefront's LDS row of chunk last values
-// ([chunk][lane]).  *total receives cdf[K-1].
+// ([chunk][lane]).  cdf: NULL, or this wavefront's LDS copy of every running
+// sum (Kp floats, lane layout), from which the search reads the chunk it
+// needs instead of weighing it again.  *total receives cdf[K-1].
 template <int CH, bool WT, class E, bool SHARED = false>
 __device__ __forceinline__ int bk_token(bk_t<WT>* nd, bk_t<WT>* col, bk_t<WT>* nk, const float* alpha, int lane,
                                         int kpl, int ksh, int zo, float tw, float u, int K, int nvalid, int lastLane,
-                                        const E& et, float* lastc, float* total) {
+                                        const E& et, float* lastc, float* cdf, float* total) {
     bk_bump<WT, SHARED>(nd, col, nk, lane, kpl, ksh, zo, tw, true);
     const int C = kpl / CH;
     float carry = 0.0f;
-    for (int c = 0; c < C; ++c) {
-        fvec<CH> w;
-        bk_weights<CH, WT>(nd, col, nk, alpha, c, lane, et, w);
-        bk_prefix<CH>(w, carry);
-        carry = w[CH - 1];
-        lastc[c * 64 + lane] = carry;
+    // chunks in pairs: the next chunk's loads are issued before this one is weighed
+    BkChunk<CH, WT> A, B;
+    A.load(nd, col, nk, alpha, 0, lane, et);
+    for (int c = 0; c < C; c += 2) {
+        if (c + 1 < C) B.load(nd, col, nk, alpha, c + 1, lane, et);
+        {
+            fvec<CH> w;
+            A.weights(w);
+            bk_prefix<CH>(w, carry);
+            carry = w[CH - 1];
+            lastc[c * 64 + lane] = carry;
+            if (cdf) store_kvec<CH>(reinterpret_cast<int*>(cdf + c * (64 * CH)), lane, __builtin_bit_cast(ivec<CH>, w));
+        }
+        if (c + 2 < C) A.load(nd, col, nk, alpha, c + 2, lane, et);
+        if (c + 1 < C) {
+            fvec<CH> w;
+            B.weights(w);
+            bk_prefix<CH>(w, carry);
+            carry = w[CH - 1];
+            lastc[(c + 1) * 64 + lane] = carry;
+            if (cdf)
+                store_kvec<CH>(reinterpret_cast<int*>(cdf + (c + 1) * (64 * CH)), lane, __builtin_bit_cast(ivec<CH>, w));
+        }
     }
     const float incl = wave_inclusive_scan(carry);
     const float excl = dpp_wave_shr1(incl);
@@ -209,8 +248,13 @@ __device__ __forceinline__ int bk_token(bk_t<WT>* nd, bk_t<WT>* col, bk_t<WT>* n
     int miss = kpl;
     if (cs < C) {
         fvec<CH> w;
-        bk_weights<CH, WT>(nd, col, nk, alpha, cs, lane, et, w);
-        bk_prefix<CH>(w, cs ? lastc[(cs - 1) * 64 + lane] : 0.0f);
+        if (cdf) {
+            load_kvec_f<CH>(cdf + cs * (64 * CH), lane, w);
+        } else {
+            A.load(nd, col, nk, alpha, cs, lane, et);
+            A.weights(w);
+            bk_prefix<CH>(w, cs ? lastc[(cs - 1) * 64 + lane] : 0.0f);
+        }
         miss = cs * CH + cdf_count_le<CH>(w, excl, r, excl + w[CH - 1]);
     }
     const unsigned long long m = __ballot(miss < nvalid);
@@ -250,7 +294,8 @@ __device__ __forceinline__ void bk_fill(T* dst, int lane, int kpl) {
 template <int CH, int MODE, bool WT, class E, bool SHARED = false>
 __device__ __forceinline__ void bk_sample_range(bk_t<WT>* nd, bk_t<WT>* nk, bk_t<WT>* colbuf, const float* alpha,
                                                 const SweepParams& p, uint32_t sweep, int lane, uint64_t b, uint64_t e,
-                                                int kpl, int ksh, int nvalid, int lastLane, E& et, float* lastc) {
+                                                int kpl, int ksh, int nvalid, int lastLane, E& et, float* lastc,
+                                                float* cdf = nullptr) {
     using T = bk_t<WT>;
     T* nkwT = reinterpret_cast<T*>(p.nkw);
     int vcur = -1;
@@ -283,8 +328,8 @@ __device__ __forceinline__ void bk_sample_range(bk_t<WT>* nd, bk_t<WT>* nk, bk_t
             const int zo = readlane_i(zv, i);
             const float w = readlane_f(tw, i);
             const float u = readlane_f(uv, i);
-            const int zn = bk_token<CH, WT, E, SHARED>(nd, col, nk, alpha, lane, kpl, ksh, zo, w, u, p.K, nvalid, lastLane, et, lastc,
-                                        nullptr);
+            const int zn = bk_token<CH, WT, E, SHARED>(nd, col, nk, alpha, lane, kpl, ksh, zo, w, u, p.K, nvalid, lastLane,
+                                                       et, lastc, cdf, nullptr);
             if (zn != zo) {
                 if constexpr (MODE == MODE_ATOMIC) {
                     T* cp = nkwT + (int64_t)v * p.Kp;
@@ -304,6 +349,14 @@ __device__ __forceinline__ void bk_sample_range(bk_t<WT>* nd, bk_t<WT>* nk, bk_t
         if (lane < n) p.z[g] = (uint16_t)zkeep;
     }
 }
+
+// Dynamic LDS of the kernels below.  One-wavefront kernels: the running sums
+// (Kp floats, when kept) then the chunk last values (64 per chunk).  The
+// sweeps keep the sums at every K (a few wavefronts run at once); the kernels
+// that fill the chip with one document per wavefront keep them up to 64 topics
+// per lane, where they still leave room for several wavefronts per CU.
+extern __shared__ int bk_lds[];
+constexpr int kBkCdfManyMaxKpl = 64;
 
 __device__ __forceinline__ uint64_t bk_prime(uint64_t N, uint32_t sseed) {
     uint64_t P = 1;
@@ -326,7 +379,8 @@ __device__ __forceinline__ uint64_t bk_prime(uint64_t N, uint32_t sseed) {
 template <int CH, bool WT, bool ASYM>
 __global__ void __launch_bounds__(64) bk_sweep_det_kernel(SweepParams p, int kpl, int ksh) {
     using T = bk_t<WT>;
-    __shared__ float lastc[BK_MAX_CHUNKS * 64];
+    float* cdf = reinterpret_cast<float*>(bk_lds);
+    float* lastc = cdf + p.Kp;
     const int lane = lane_id();
     const int w = blockIdx.x;
     int nvalid, lastLane;
@@ -346,7 +400,7 @@ __global__ void __launch_bounds__(64) bk_sweep_det_kernel(SweepParams p, int kpl
         if (b == e) continue;
         T* nd = reinterpret_cast<T*>(p.ndk) + d * p.Kp;
         bk_sample_range<CH, MODE_DET, WT>(nd, nkp, nullptr, p.alpha, p, sweep, lane, b, e, kpl, ksh, nvalid, lastLane, et,
-                                          lastc);
+                                          lastc, cdf);
     }
 }
 
@@ -356,7 +410,8 @@ __global__ void __launch_bounds__(64) bk_sweep_det_kernel(SweepParams p, int kpl
 template <int CH, bool WT, bool ASYM>
 __global__ void __launch_bounds__(64) bk_sweep_copy_kernel(SweepParams p, int kpl, int ksh) {
     using T = bk_t<WT>;
-    __shared__ float lastc[BK_MAX_CHUNKS * 64];
+    float* cdf = reinterpret_cast<float*>(bk_lds);
+    float* lastc = cdf + p.Kp;
     const int lane = lane_id();
     const int w = blockIdx.x;
     int nvalid, lastLane;
@@ -377,7 +432,7 @@ __global__ void __launch_bounds__(64) bk_sweep_copy_kernel(SweepParams p, int kp
             if (b == e) continue;
             T* nd = reinterpret_cast<T*>(p.ndk) + d * p.Kp;
             bk_sample_range<CH, MODE_DET, WT>(nd, nkp, nullptr, p.alpha, q, q.sweep, lane, b, e, kpl, ksh, nvalid,
-                                              lastLane, et, lastc);
+                                              lastLane, et, lastc, cdf);
         }
     }
 }
@@ -429,7 +484,6 @@ __global__ void __launch_bounds__(64) bk_sweep_device_kernel(SweepParams p, int 
 // atomics), so that per token only the n_dk row and the wavefront's column
 // copy (bk_scratch row blockIdx.x * waves + wave) come from device memory.
 // Dynamic LDS: alpha (Kp) | n_k (Kp) | per wavefront the chunk last values.
-extern __shared__ int bk_lds[];
 
 template <int CH, bool WT, bool ASYM>
 __global__ void __launch_bounds__(64 * kMaxSweepWaves) bk_sweep_device_lds_kernel(SweepParams p, int kpl, int ksh) {
@@ -480,7 +534,8 @@ __global__ void __launch_bounds__(64) bk_sample_frozen_kernel(const int32_t* __r
                                                               const int64_t* doc, const uint32_t* word,
                                                               const uint16_t* z_old, const float* u, uint16_t* z_new,
                                                               float* total, int kpl, int ksh, int32_t* scratch) {
-    __shared__ float lastc[BK_MAX_CHUNKS * 64];
+    float* cdf = kpl <= kBkCdfManyMaxKpl ? reinterpret_cast<float*>(bk_lds) : nullptr;
+    float* lastc = reinterpret_cast<float*>(bk_lds) + (cdf ? Kp : 0);
     const int lane = lane_id();
     int nvalid, lastLane;
     bk_lane_limits(K, lane, kpl, nvalid, lastLane);
@@ -494,7 +549,7 @@ __global__ void __launch_bounds__(64) bk_sample_frozen_kernel(const int32_t* __r
         bk_copy<CH>(nkc, nk, lane, kpl);
         float tot;
         const int zn = bk_token<CH, false>(nd, col, nkc, alpha, lane, kpl, ksh, (int)z_old[t], 1.0f, u[t], K, nvalid,
-                                       lastLane, et, lastc, &tot);
+                                       lastLane, et, lastc, cdf, &tot);
         if (lane == 0) {
             z_new[t] = (uint16_t)zn;
             if (total) total[t] = tot;
@@ -514,7 +569,8 @@ __global__ void __launch_bounds__(64) bk_sample_frozen_kernel(const int32_t* __r
 template <int CH, bool WT, bool ASYM>
 __global__ void __launch_bounds__(64) bk_infer_gen_kernel(InferGenParams p, int kpl, int ksh) {
     using T = bk_t<WT>;
-    __shared__ float lastc[BK_MAX_CHUNKS * 64];
+    float* cdf = kpl <= kBkCdfManyMaxKpl ? reinterpret_cast<float*>(bk_lds) : nullptr;
+    float* lastc = reinterpret_cast<float*>(bk_lds) + (cdf ? p.Kp : 0);
     const int lane = lane_id();
     int nvalid, lastLane;
     bk_lane_limits(p.K, lane, kpl, nvalid, lastLane);
@@ -616,7 +672,8 @@ __global__ void __launch_bounds__(64) bk_infer_gen_kernel(InferGenParams p, int 
                     }
                     const int zo = readlane_i(zv, i);
                     const int zn = bk_token<CH, WT>(nd, col, nk, p.alpha, lane, kpl, ksh, zo, readlane_f(tw, i),
-                                                readlane_f(uv, i), p.K, nvalid, lastLane, et, lastc, nullptr);
+                                                readlane_f(uv, i), p.K, nvalid, lastLane, et, lastc, cdf,
+                                                nullptr);
                     zkeep = (lane == i) ? zn : zkeep;
                 }
                 if (lane < n) p.z[g] = (uint16_t)zkeep;
@@ -733,6 +790,14 @@ int bk_shift(int kpl) {
 
 bool bk_kpl_ok(int kpl) { return kpl >= 1 && kpl <= kBigMaxKpl && (kpl & (kpl - 1)) == 0; }
 
+// Dynamic LDS of the one-wavefront kernels that keep their running sums:
+// Kp floats of sums and 64 chunk last values per chunk.
+size_t bk_cdf_lds_bytes(int kpl, bool many = false) {
+    const int ch = kpl >= 16 ? 16 : kpl;
+    const size_t sums = (many && kpl > kBkCdfManyMaxKpl) ? 0 : (size_t)64 * kpl;
+    return (sums + (size_t)64 * (kpl / ch)) * sizeof(float);
+}
+
 // Instantiations: CH = 16 for every count type (K above 256, which includes
 // every K above 2048); CH = KPL < 16 for integer counts only (the exact-count
 // path of K <= 2048, taken for documents over 2^20 tokens or alpha >= 2^20;
@@ -765,7 +830,8 @@ hipError_t launch_bk_sweep_det(int kpl, const SweepParams& p, bool weighted, hip
     if (!bk_kpl_ok(kpl)) return hipErrorInvalidValue;
     const int sh = bk_shift(kpl);
     LDA_BK_DISPATCH(kpl, weighted, p.prior_row != nullptr,
-                    hipLaunchKernelGGL((bk_sweep_det_kernel<CH_, W_, A_>), dim3(p.W), dim3(64), 0, s, p, kpl, sh));
+                    hipLaunchKernelGGL((bk_sweep_det_kernel<CH_, W_, A_>), dim3(p.W), dim3(64), bk_cdf_lds_bytes(kpl), s,
+                                       p, kpl, sh));
     return hipGetLastError();
 }
 
@@ -773,7 +839,8 @@ hipError_t launch_bk_sweep_copy(int kpl, const SweepParams& p, bool weighted, hi
     if (!bk_kpl_ok(kpl)) return hipErrorInvalidValue;
     const int sh = bk_shift(kpl);
     LDA_BK_DISPATCH(kpl, weighted, p.prior_row != nullptr,
-                    hipLaunchKernelGGL((bk_sweep_copy_kernel<CH_, W_, A_>), dim3(p.W), dim3(64), 0, s, p, kpl, sh));
+                    hipLaunchKernelGGL((bk_sweep_copy_kernel<CH_, W_, A_>), dim3(p.W), dim3(64), bk_cdf_lds_bytes(kpl), s,
+                                       p, kpl, sh));
     return hipGetLastError();
 }
 
@@ -814,14 +881,14 @@ int bk_sweep_device_lds_blocks_per_cu(bool weighted, bool asym, int kpl, int Kp,
 // instantiations use fewer registers): the persistent sweeps take work from
 // counters and the partition rounds do not wait on each other, so a count
 // above the real one only queues workgroups.
-int bk_sweep_blocks_per_cu(bool weighted, bool asym, bool device) {
+int bk_sweep_blocks_per_cu(bool weighted, bool asym, bool device, int kpl) {
     int n = 0;
     hipError_t e = hipSuccess;
     constexpr int CH_ = 16;
     LDA_BK_WA(weighted, asym,
               e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
                   &n, device ? (const void*)bk_sweep_device_kernel<CH_, W_, A_> : (const void*)bk_sweep_det_kernel<CH_, W_, A_>,
-                  64, 0));
+                  64, device ? 0 : bk_cdf_lds_bytes(kpl)));
     return e == hipSuccess ? n : 0;
 }
 
@@ -832,7 +899,8 @@ hipError_t launch_bk_sample_frozen(int kpl, const int32_t* ndk, const int32_t* n
     if (!bk_kpl_ok(kpl) || !scratch || grid <= 0) return hipErrorInvalidValue;
     const int sh = bk_shift(kpl);
     LDA_BK_DISPATCH(kpl, false, false,
-                    hipLaunchKernelGGL(bk_sample_frozen_kernel<CH_>, dim3(grid), dim3(64), 0, s, ndk, nkw, nk, alpha, eta,
+                    hipLaunchKernelGGL(bk_sample_frozen_kernel<CH_>, dim3(grid), dim3(64), bk_cdf_lds_bytes(kpl, true), s, ndk,
+                                       nkw, nk, alpha, eta,
                                        veta, K, Kp, n, doc, word, z_old, u, z_new, total, kpl, sh, scratch));
     return hipGetLastError();
 }
@@ -848,16 +916,19 @@ hipError_t launch_bk_infer_gen(int kpl, const InferGenParams& p, int grid, bool 
     } else {
         const int sh = bk_shift(kpl);
         LDA_BK_DISPATCH(kpl, wt, asym,
-                        hipLaunchKernelGGL((bk_infer_gen_kernel<CH_, W_, A_>), dim3(grid), dim3(64), 0, s, p, kpl, sh));
+                        hipLaunchKernelGGL((bk_infer_gen_kernel<CH_, W_, A_>), dim3(grid), dim3(64), bk_cdf_lds_bytes(kpl, true), s,
+                                           p, kpl, sh));
     }
     return hipGetLastError();
 }
 
-int bk_infer_gen_blocks_per_cu(bool wt, bool asym) {
+int bk_infer_gen_blocks_per_cu(bool wt, bool asym, int kpl) {
     int n = 0;
     hipError_t e = hipSuccess;
     constexpr int CH_ = 16;
-    LDA_BK_WA(wt, asym, e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, bk_infer_gen_kernel<CH_, W_, A_>, 64, 0));
+    LDA_BK_WA(wt, asym,
+              e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, bk_infer_gen_kernel<CH_, W_, A_>, 64,
+                                                               bk_cdf_lds_bytes(kpl, true)));
     return e == hipSuccess ? n : 0;
 }
 

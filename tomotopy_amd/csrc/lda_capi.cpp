@@ -383,7 +383,7 @@ int device_wide_grid(lda_ctx* c, int* waves_out) {
             }
         }
         *waves_out = 1;
-        const int per = std::max(lda::bk_sweep_blocks_per_cu(wtd, c->has_priors, true), 1);
+        const int per = std::max(lda::bk_sweep_blocks_per_cu(wtd, c->has_priors, true, c->kpl), 1);
         return (int)std::min<int64_t>(inflight, (int64_t)c->num_cu * per);
     }
     const int W8 = lda::kMaxSweepWaves;
@@ -1867,7 +1867,7 @@ static int infer_general_batch(lda_ctx* c, lda_host::GenFold& B, hipStream_t s, 
     int gen_max = 1;
     for (int bk = 0; bk < lda::kFoldBuckets; ++bk) {
         const int sg = 1 << bk;
-        if (exact) per_cu[bk] = sg == 1 ? std::max(lda::bk_infer_gen_blocks_per_cu(false, c->has_priors), 1) : 0;
+        if (exact) per_cu[bk] = sg == 1 ? std::max(lda::bk_infer_gen_blocks_per_cu(false, c->has_priors, c->kpl), 1) : 0;
         else per_cu[bk] = sg <= c->seg_max ? lda::infer_gen_blocks_per_cu(c->kpl, wt, c->has_priors, sg) : 0;
         if (per_cu[bk] <= 0) break;
         gen_max = sg;

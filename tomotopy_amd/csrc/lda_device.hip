@@ -2639,7 +2639,7 @@ static int sweep_device_blocks_kpl(int waves) {
 }
 
 int sweep_device_blocks_per_cu(int kpl, bool asym, int waves) {
-    if (kpl > kSmallMaxKpl) return waves == 1 ? std::max(bk_sweep_blocks_per_cu(false, asym, true), 1) : 0;
+    if (kpl > kSmallMaxKpl) return waves == 1 ? std::max(bk_sweep_blocks_per_cu(false, asym, true, kpl), 1) : 0;
     switch (kpl) {
         case 1: return asym ? sweep_device_blocks_kpl<1, true>(waves) : sweep_device_blocks_kpl<1, false>(waves);
         case 2: return asym ? sweep_device_blocks_kpl<2, true>(waves) : sweep_device_blocks_kpl<2, false>(waves);
@@ -2857,7 +2857,7 @@ static int sweep_det_blocks_kpl(bool weighted) {
 }
 
 int sweep_det_blocks_per_cu(int kpl, bool asym, bool weighted) {
-    if (kpl > kSmallMaxKpl) return std::max(bk_sweep_blocks_per_cu(weighted, asym, false), 1);
+    if (kpl > kSmallMaxKpl) return std::max(bk_sweep_blocks_per_cu(weighted, asym, false, kpl), 1);
     switch (kpl) {
         case 1: return asym ? sweep_det_blocks_kpl<1, true>(weighted) : sweep_det_blocks_kpl<1, false>(weighted);
         case 2: return asym ? sweep_det_blocks_kpl<2, true>(weighted) : sweep_det_blocks_kpl<2, false>(weighted);
@@ -2878,7 +2878,7 @@ static int sweep_device_w_blocks_kpl(int waves) {
 }
 
 int sweep_device_w_blocks_per_cu(int kpl, bool asym, int waves) {
-    if (kpl > kSmallMaxKpl) return waves == 1 ? std::max(bk_sweep_blocks_per_cu(true, asym, true), 1) : 0;
+    if (kpl > kSmallMaxKpl) return waves == 1 ? std::max(bk_sweep_blocks_per_cu(true, asym, true, kpl), 1) : 0;
     switch (kpl) {
         case 1: return asym ? sweep_device_w_blocks_kpl<1, true>(waves) : sweep_device_w_blocks_kpl<1, false>(waves);
         case 2: return asym ? sweep_device_w_blocks_kpl<2, true>(waves) : sweep_device_w_blocks_kpl<2, false>(waves);
@@ -3047,7 +3047,7 @@ static int infer_gen_blocks_kpl(bool wt, bool asym, int segs) {
 }
 
 int infer_gen_blocks_per_cu(int kpl, bool wt, bool asym, int segs) {
-    if (kpl > kSmallMaxKpl) return segs == 1 ? bk_infer_gen_blocks_per_cu(wt, asym) : 0;
+    if (kpl > kSmallMaxKpl) return segs == 1 ? bk_infer_gen_blocks_per_cu(wt, asym, kpl) : 0;
     switch (kpl) {
         case 1: return infer_gen_blocks_kpl<1>(wt, asym, segs);
         case 2: return infer_gen_blocks_kpl<2>(wt, asym, segs);

@@ -318,12 +318,12 @@ hipError_t launch_bk_sweep_copy(int kpl, const SweepParams& p, bool weighted, hi
 hipError_t launch_bk_sweep_device(int kpl, const SweepParams& p, bool weighted, int grid, int waves, hipStream_t s);
 size_t bk_sweep_device_lds_bytes(int kpl, int Kp, int waves);
 int bk_sweep_device_lds_blocks_per_cu(bool weighted, bool asym, int kpl, int Kp, int waves);
-int bk_sweep_blocks_per_cu(bool weighted, bool asym, bool device);
+int bk_sweep_blocks_per_cu(bool weighted, bool asym, bool device, int kpl);
 hipError_t launch_bk_sample_frozen(int kpl, const int32_t* ndk, const int32_t* nkw, const int32_t* nk,
                                    const float* alpha, float eta, float veta, int K, int Kp, int64_t n, int grid,
                                    const int64_t* doc, const uint32_t* word, const uint16_t* z_old, const float* u,
                                    uint16_t* z_new, float* total, int32_t* scratch, hipStream_t s);
 hipError_t launch_bk_infer_gen(int kpl, const InferGenParams& p, int grid, bool finalize, hipStream_t s);
-int bk_infer_gen_blocks_per_cu(bool wt, bool asym);
+int bk_infer_gen_blocks_per_cu(bool wt, bool asym, int kpl);
 
 }  // namespace lda
