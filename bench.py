@@ -244,8 +244,11 @@ def partition_bench(name, ws, offs, V, K, alpha, sweeps, cpu_sweeps=0):
 def pmi_bench(ws, offs, V, K, alpha, eta, steps, warmup):
     """The same corpus and model with TermWeight.PMI token weights: float
     counts, clamped removals and the weighted merge (n_k rebuilt from n_kw)
-    after every sweep.  Algorithmic bytes per sweep launch N*(4K+12) + D*8K
-    (the token weight adds 4 bytes per token)."""
+    after every sweep.  Algorithmic bytes per sweep launch: one float column
+    read (4K) per run of equal words in a document, 12 per token (word, topic
+    read and written, weight), 8K per document; `frac_per_token` counts a
+    column read for every token instead (N*(4K+12) + D*8K), which exceeds what
+    reaches HBM since a run's tokens share one read."""
     from tomotopy_amd import _native, corpus
     t = time.time()
     wts = corpus.term_weights(ws, offs, V, "pmi")
@@ -262,13 +265,17 @@ def pmi_bench(ws, offs, V, K, alpha, eta, steps, warmup):
     dt = time.perf_counter() - t0
     st = ctx.kernel_stats()
     kern_ms = st["ms"] / max(1, st["launches"])
-    bytes_per_launch = n_valid * (4 * K + 12) + D * 8 * K
+    per_token_bytes = n_valid * (4 * K + 12) + D * 8 * K
+    sb = sweep_bytes(ws, offs, V, K, V)  # every column float: 4K per run
+    bytes_per_launch = sb["per_run"] + 4 * n_valid
     achieved = bytes_per_launch / (kern_ms * 1e-3) / 1e9
     lld, llr = ctx.loglik()
     wn = ctx.weighted_n()
     out = {"value": n_valid * steps / dt, "unit": "tokens/s", "ms_per_step": dt * 1e3 / steps, "steps": steps,
            "kernel": "sweep_device_w_kernel<16, false>", "kernel_ms": kern_ms, "achieved_GBs": achieved,
-           "frac": achieved / HBM_PEAK_GBS, "algorithmic_bytes_per_launch": bytes_per_launch,
+           "frac": achieved / HBM_PEAK_GBS, "algorithmic_bytes_per_launch": bytes_per_launch, "runs": sb["runs"],
+           "algorithmic_bytes_per_launch_per_token": per_token_bytes,
+           "frac_per_token": per_token_bytes / (kern_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
            "traffic": pmc_traffic("sweep_device_w_kernel<16, false>")[0],
            "ll_per_word": (lld + llr) / wn, "weighted_n": wn}
     log("PMI: %.3g tokens/s, kernel %.3f ms (%.1f%% of HBM peak), %.3f ms/step, ll_per_word %.5f"

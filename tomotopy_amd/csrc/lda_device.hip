@@ -87,10 +87,16 @@ __device__ __forceinline__ void bump(const WaveLds& S, const fvec<KPL>& al, int 
     const float a_sl = al[sl];
     if (lane == (z >> kpl_shift<KPL>())) {
         const int q = kpos<KPL>(lane, sl);
-        const int d = (int)__builtin_rintf(S.A[q] - a_sl) + delta;
-        S.A[q] = (float)d + a_sl;
-        S.col[q] += delta;
+        // the n_k atomic first, then the A and column reads: all three are in
+        // flight together and one wait covers them
         const int n = atomicAdd(S.nk + q, delta) + delta;
+        const float a = S.A[q];
+        const int c = S.col[q];
+        // n_dk +/- 1 as an exact float integer; A = that + alpha is the value
+        // (float)n_dk + alpha that set_ndk writes
+        const float d = __builtin_rintf(a - a_sl) + (float)delta;
+        S.A[q] = d + a_sl;
+        S.col[q] = c + delta;
         const float b = (float)n + et.s_at(q);
         S.bf[q] = b;
         S.y[q] = recip_rn(b);
@@ -227,14 +233,6 @@ __device__ __forceinline__ int token_step(const WaveLds& S, const fvec<KPL>& al,
 }
 
 
-// Column of word v in the shared n_kw: int32 row of nkw, or with narrow
-// columns (NARROW, words v >= nkw_hot) the u16 row of nkw16.
-template <int KPL, bool NARROW, typename V>
-__device__ __forceinline__ void load_col(const SweepParams& p, int v, int lane, V& o) {
-    if (NARROW && v >= p.nkw_hot) load_kvec_u16<KPL>(p.nkw16 + (int64_t)(v - p.nkw_hot) * p.Kp, lane, o);
-    else load_kvec<KPL>(p.nkw + (int64_t)v * p.Kp, lane, o);
-}
-
 // n_kw[q, v] += delta with an exact atomic.  A u16 cell is updated by a 32-bit
 // atomic add on the word that holds it, the delta shifted into its half:
 // cells stay in [0, 2^16) (a removal takes away a token the cell counts, and
@@ -250,42 +248,120 @@ __device__ __forceinline__ void col_atomic(const SweepParams& p, int v, int q, i
     }
 }
 
-// sampleDocument over the token range [b, e) (LDAModel.hpp:326-354).  The
-// column of the next distinct word is fetched one token ahead.
+// The next word's column as loaded: int32 cells, or for a narrow word
+// (NARROW, v >= nkw_hot) its u16 cells packed in pairs, unpacked only when the
+// column is stored to LDS, so that the load stays in flight until the word's
+// first token.
+template <int KPL, bool NARROW>
+__device__ __forceinline__ void fetch_col(const SweepParams& p, int v, int lane, ivec<KPL>& raw) {
+    if (NARROW && v >= p.nkw_hot) {
+        const uint16_t* base = p.nkw16 + (int64_t)(v - p.nkw_hot) * p.Kp;
+        if constexpr (KPL >= 4) {
+#pragma unroll
+            for (int g = 0; g < KPL / 4; ++g) {
+                const uint2 x = *reinterpret_cast<const uint2*>(base + g * 256 + lane * 4);
+                raw[2 * g] = (int)x.x;
+                raw[2 * g + 1] = (int)x.y;
+            }
+        } else if constexpr (KPL == 2) {
+            raw[0] = (int)*reinterpret_cast<const uint32_t*>(base + lane * 2);
+        } else {
+            raw[0] = (int)base[lane];
+        }
+    } else {
+        load_kvec<KPL>(p.nkw + (int64_t)v * p.Kp, lane, raw);
+    }
+}
+
+template <int KPL, bool NARROW>
+__device__ __forceinline__ void store_col(const WaveLds& S, const SweepParams& p, int v, int lane,
+                                          const ivec<KPL>& raw) {
+    if (NARROW && v >= p.nkw_hot) {
+        ivec<KPL> o;
+        if constexpr (KPL >= 4) {
+#pragma unroll
+            for (int g = 0; g < KPL / 4; ++g) {
+                const uint32_t x = (uint32_t)raw[2 * g], y = (uint32_t)raw[2 * g + 1];
+                o[4 * g] = (int)(x & 0xFFFFu);
+                o[4 * g + 1] = (int)(x >> 16);
+                o[4 * g + 2] = (int)(y & 0xFFFFu);
+                o[4 * g + 3] = (int)(y >> 16);
+            }
+        } else if constexpr (KPL == 2) {
+            o[0] = (int)((uint32_t)raw[0] & 0xFFFFu);
+            o[1] = (int)((uint32_t)raw[0] >> 16);
+        } else {
+            o[0] = raw[0];
+        }
+        lds_store<KPL>(S.col, lane, o);
+    } else {
+        lds_store<KPL>(S.col, lane, raw);
+    }
+}
+
+// Tokens per group of shared n_kw atomics in the device-wide sweeps.  At
+// K = 1000 over 30M tokens, groups of 16 / 32 / 64 take 11.5 / 11.1 / 10.9 ms
+// per sweep (14.0 ms with per-token atomics) and leave ll_per_word after 12
+// sweeps 0.02% / 0.03% / 0.06% below the per-token schedule's.
+constexpr int kAtomicFlush = 32;
+
+// sampleDocument over the token range [b, e) (LDAModel.hpp:326-354).  Tokens
+// come in blocks of 64, token t of a block on lane t (its word, z and u); the
+// column of the next distinct word is fetched one token ahead and, for a
+// narrow word, unpacked only when it is stored to LDS.  MODE_ATOMIC sends the
+// n_kw changes of each group of kAtomicFlush tokens to the shared n_kw at the
+// group's end as whole-wavefront atomics, lane t those of token t (its word,
+// old and new topic): issued per token from single lanes they would hold every
+// later load wait behind an atomic's round trip (the vector memory counter
+// retires in order).  Other workers see them up to one group later, as they
+// already sample from column copies; this wavefront never rereads a column
+// with changes pending (a document's words come in one run each, and its last
+// group is sent before the next document).
 template <int KPL, int MODE, class E, bool NARROW = false>
 __device__ __forceinline__ void sample_range(const WaveLds& S, const fvec<KPL>& al, const SweepParams& p, uint32_t sweep,
                                              int lane, uint64_t b, uint64_t e, int nvalid, int lastLane, E& et) {
     static_assert(!NARROW || MODE == MODE_ATOMIC, "narrow columns are for the shared device-wide n_kw");
     constexpr int SH = kpl_shift<KPL>();
-    int vcur = -1;
+    constexpr int TF = MODE == MODE_ATOMIC ? kAtomicFlush : 64;
+    static_assert(TF >= 1 && TF <= 64 && (64 % TF) == 0, "flush groups divide the 64-token block");
+    int vcur = -1, vnext = -1;
     int32_t* colp = nullptr;
-    for (uint64_t t0 = b; t0 < e; t0 += 64) {
-        const int n = (int)((e - t0) < 64 ? (e - t0) : 64);
-        const uint64_t g = t0 + (uint64_t)lane;
-        int wv = -1, zv = 0xFFFF;
-        float uv = 0.0f;
-        if (lane < n) {
-            wv = (int)p.words[g];
-            zv = p.z[g];
-            uv = uniform24(philox_x0(p.seed_lo, p.seed_hi, (uint32_t)g, (uint32_t)(g >> 32), sweep, TAG_SWEEP));
+    ivec<KPL> ncol;
+    // the current block of up to 64 tokens, token t of it on lane t
+    int nb = 0, wv = -1, zv = 0xFFFF, zkeep = 0xFFFF;
+    float uv = 0.0f;
+    uint64_t g = 0;
+    for (uint64_t t0 = b; t0 < e; t0 += TF) {
+        const int off = (int)((t0 - b) & 63);
+        if (off == 0) {
+            nb = (int)((e - t0) < 64 ? (e - t0) : 64);
+            g = t0 + (uint64_t)lane;
+            wv = -1;
+            zv = 0xFFFF;
+            uv = 0.0f;
+            if (lane < nb) {
+                wv = (int)p.words[g];
+                zv = p.z[g];
+                uv = uniform24(philox_x0(p.seed_lo, p.seed_hi, (uint32_t)g, (uint32_t)(g >> 32), sweep, TAG_SWEEP));
+            }
+            zkeep = zv;
+            vnext = -1;
         }
-        int zkeep = zv;
-        int vnext = -1;
-        ivec<KPL> ncol;
-        for (int i = 0; i < n; ++i) {
+        const int i1 = off + TF < nb ? off + TF : nb;
+        for (int i = off; i < i1; ++i) {
             const int v = readlane_i(wv, i);
             if ((unsigned)v >= (unsigned)p.V) continue;
             if (v != vcur) {
                 colp = p.nkw + (int64_t)v * p.Kp;
-                if (v != vnext) load_col<KPL, NARROW>(p, v, lane, ncol);
-                lds_store<KPL>(S.col, lane, ncol);
+                if (v != vnext) fetch_col<KPL, NARROW>(p, v, lane, ncol);
+                store_col<KPL, NARROW>(S, p, v, lane, ncol);
                 et.word(p, v, lane);
                 vcur = v;
             }
-            if (i + 1 < n) {
+            if (i + 1 < nb) {
                 const int v2 = readlane_i(wv, i + 1);
                 if (v2 != v && (unsigned)v2 < (unsigned)p.V) {
-                    load_col<KPL, NARROW>(p, v2, lane, ncol);
+                    fetch_col<KPL, NARROW>(p, v2, lane, ncol);
                     vnext = v2;
                 }
             }
@@ -293,19 +369,22 @@ __device__ __forceinline__ void sample_range(const WaveLds& S, const fvec<KPL>& 
             const float u = readlane_f(uv, i);
             const int zn = token_step<KPL>(S, al, lane, zo, u, p.K, nvalid, lastLane, et, nullptr);
             if (zn != zo) {
-                const int lo = zo >> SH, ln = zn >> SH;
-                const int qo = kpos<KPL>(lane, zo & (KPL - 1)), qn = kpos<KPL>(lane, zn & (KPL - 1));
                 if constexpr (MODE == MODE_DET) {
+                    const int lo = zo >> SH, ln = zn >> SH;
+                    const int qo = kpos<KPL>(lane, zo & (KPL - 1)), qn = kpos<KPL>(lane, zn & (KPL - 1));
                     if (lane == lo) colp[qo] = S.col[qo];
                     if (lane == ln) colp[qn] = S.col[qn];
-                } else {
-                    if (lane == lo) col_atomic<NARROW>(p, vcur, qo, -1);
-                    if (lane == ln) col_atomic<NARROW>(p, vcur, qn, 1);
                 }
                 zkeep = (lane == i) ? zn : zkeep;
             }
         }
-        if (lane < n) p.z[g] = (uint16_t)zkeep;
+        if constexpr (MODE == MODE_ATOMIC) {
+            if (lane >= off && lane < i1 && zkeep != zv && (unsigned)wv < (unsigned)p.V) {
+                col_atomic<NARROW>(p, wv, kpos<KPL>(zv >> SH, zv & (KPL - 1)), -1);
+                col_atomic<NARROW>(p, wv, kpos<KPL>(zkeep >> SH, zkeep & (KPL - 1)), 1);
+            }
+        }
+        if (i1 == nb && lane < nb) p.z[g] = (uint16_t)zkeep;
     }
 }
 
@@ -556,24 +635,35 @@ __device__ __forceinline__ void sample_range_w(const WaveLdsW& S, const fvec<KPL
                                                E& et) {
     constexpr int SH = kpl_shift<KPL>();
     constexpr bool SHARED = MODE == MODE_ATOMIC;
+    constexpr int TF = MODE == MODE_ATOMIC ? kAtomicFlush : 64;
     float* nkwf = reinterpret_cast<float*>(p.nkw);
-    int vcur = -1;
+    int vcur = -1, vnext = -1;
     float* colp = nullptr;
-    for (uint64_t t0 = b; t0 < e; t0 += 64) {
-        const int n = (int)((e - t0) < 64 ? (e - t0) : 64);
-        const uint64_t g = t0 + (uint64_t)lane;
-        int wv = -1, zv = 0xFFFF;
-        float uv = 0.0f, tw = 0.0f;
-        if (lane < n) {
-            wv = (int)p.words[g];
-            zv = p.z[g];
-            tw = p.wt[g];
-            uv = uniform24(philox_x0(p.seed_lo, p.seed_hi, (uint32_t)g, (uint32_t)(g >> 32), sweep, TAG_SWEEP));
+    fvec<KPL> ncol;
+    // the current block of up to 64 tokens, token t of it on lane t
+    int nb = 0, wv = -1, zv = 0xFFFF, zkeep = 0xFFFF;
+    float uv = 0.0f, tw = 0.0f;
+    uint64_t g = 0;
+    for (uint64_t t0 = b; t0 < e; t0 += TF) {
+        const int off = (int)((t0 - b) & 63);
+        if (off == 0) {
+            nb = (int)((e - t0) < 64 ? (e - t0) : 64);
+            g = t0 + (uint64_t)lane;
+            wv = -1;
+            zv = 0xFFFF;
+            uv = 0.0f;
+            tw = 0.0f;
+            if (lane < nb) {
+                wv = (int)p.words[g];
+                zv = p.z[g];
+                tw = p.wt[g];
+                uv = uniform24(philox_x0(p.seed_lo, p.seed_hi, (uint32_t)g, (uint32_t)(g >> 32), sweep, TAG_SWEEP));
+            }
+            zkeep = zv;
+            vnext = -1;
         }
-        int zkeep = zv;
-        int vnext = -1;
-        fvec<KPL> ncol;
-        for (int i = 0; i < n; ++i) {
+        const int i1 = off + TF < nb ? off + TF : nb;
+        for (int i = off; i < i1; ++i) {
             const int v = readlane_i(wv, i);
             if ((unsigned)v >= (unsigned)p.V) continue;
             if (v != vcur) {
@@ -583,7 +673,7 @@ __device__ __forceinline__ void sample_range_w(const WaveLdsW& S, const fvec<KPL
                 et.word(p, v, lane);
                 vcur = v;
             }
-            if (i + 1 < n) {
+            if (i + 1 < nb) {
                 const int v2 = readlane_i(wv, i + 1);
                 if (v2 != v && (unsigned)v2 < (unsigned)p.V) {
                     load_kvec_f<KPL>(nkwf + (int64_t)v2 * p.Kp, lane, ncol);
@@ -594,22 +684,26 @@ __device__ __forceinline__ void sample_range_w(const WaveLdsW& S, const fvec<KPL
             const float w = readlane_f(tw, i);
             const float u = readlane_f(uv, i);
             const int zn = token_step_w<KPL, SHARED>(S, al, lane, zo, w, u, p.K, nvalid, lastLane, et);
-            const int lo = zo >> SH, ln = zn >> SH;
-            const int qo = kpos<KPL>(lane, zo & (KPL - 1)), qn = kpos<KPL>(lane, zn & (KPL - 1));
             if constexpr (MODE == MODE_DET) {
+                const int lo = zo >> SH, ln = zn >> SH;
+                const int qo = kpos<KPL>(lane, zo & (KPL - 1)), qn = kpos<KPL>(lane, zn & (KPL - 1));
                 if (lane == lo) colp[qo] = S.col[qo];
                 if (lane == ln) colp[qn] = S.col[qn];
-            } else if (zn != zo) {
-                // fire-and-forget float atomics; a removal can leave a
-                // rounding-level negative cell, which the merge's clamp
-                // max(n_kw, 0) (LDAModel.hpp:574-577) removes after the sweep.
-                // An unchanged topic adds nothing to the shared column.
-                if (lane == lo) unsafeAtomicAdd(colp + qo, -w);
-                if (lane == ln) unsafeAtomicAdd(colp + qn, w);
             }
             zkeep = (lane == i) ? zn : zkeep;
         }
-        if (lane < n) p.z[g] = (uint16_t)zkeep;
+        if constexpr (MODE == MODE_ATOMIC) {
+            // the group's changes as whole-wavefront float atomics (see
+            // sample_range); a removal can leave a rounding-level negative
+            // cell, which the merge's clamp max(n_kw, 0) (LDAModel.hpp:574-577)
+            // removes after the sweep.  An unchanged topic adds nothing.
+            if (lane >= off && lane < i1 && zkeep != zv && (unsigned)wv < (unsigned)p.V) {
+                float* c = nkwf + (int64_t)wv * p.Kp;
+                unsafeAtomicAdd(c + kpos<KPL>(zv >> SH, zv & (KPL - 1)), -tw);
+                unsafeAtomicAdd(c + kpos<KPL>(zkeep >> SH, zkeep & (KPL - 1)), tw);
+            }
+        }
+        if (i1 == nb && lane < nb) p.z[g] = (uint16_t)zkeep;
     }
 }
 
