@@ -355,6 +355,25 @@ int device_copy_workers(lda_ctx* c) {
 constexpr int64_t kInFlightDiv = 8;
 constexpr int kBigSweepWaves = 4;
 
+// Tokens per group of shared n_kw atomics in the device-wide sweeps (K <=
+// 2048).  A wavefront sends its tokens' n_kw changes once per group, as whole-
+// wavefront atomics; per-token single-lane atomics held every later load wait
+// behind an atomic's round trip (K = 1000, 30M tokens, 4,096 wavefronts: 14.0
+// ms per sweep; groups of 4 / 8 / 16 / 32: 12.1 / 11.8 / 11.5 / 11.1 ms).  The
+// changes held back meanwhile are one more source of staleness: on the 3M-token
+// cut of that corpus (1,250 wavefronts) the chain's largest lag behind the
+// oracle's partition schedule while descending grows from 0.15% of
+// ll_per_word (groups of one token) to 0.17 / 0.21 / 0.25 / 0.34% for those
+// groups.  The group is the largest power of two up to 32 that keeps the
+// changes held back by all wavefronts at most 1/128 of the tokens.
+constexpr int64_t kHeldBackDiv = 128;
+
+int device_col_group(const lda_ctx* c, int64_t wavefronts) {
+    int g = 32;
+    while (g > 1 && (int64_t)g * wavefronts * kHeldBackDiv > (int64_t)c->n_valid) g >>= 1;
+    return g;
+}
+
 int device_wide_grid(lda_ctx* c, int* waves_out) {
     const bool wtd = c->tw != LDA_TW_ONE;
     auto per_cu = [&](int waves) {
@@ -520,6 +539,7 @@ int sweep_once(lda_ctx* c, int W, bool copy_merge) {
                 HIP_TRY(c->d_bk_scratch.alloc((size_t)grid * std::max(2, waves) * c->Kp));
                 p.bk_scratch = c->d_bk_scratch.p;
             }
+            p.col_group = device_col_group(c, (int64_t)grid * waves);
             HIP_TRY(hipEventRecord(c->ev0, s));
             HIP_TRY(lda::launch_sweep_device_w(c->kpl, p, grid, waves, s));
             HIP_TRY(hipEventRecord(c->ev1, s));
@@ -585,6 +605,7 @@ int sweep_once(lda_ctx* c, int W, bool copy_merge) {
             HIP_TRY(c->d_bk_scratch.alloc((size_t)grid * std::max(2, waves) * c->Kp));
             p.bk_scratch = c->d_bk_scratch.p;
         }
+        p.col_group = device_col_group(c, (int64_t)grid * waves);
         HIP_TRY(hipEventRecord(c->ev0, s));
         HIP_TRY(lda::launch_sweep_device(c->kpl, p, grid, waves, s));
         HIP_TRY(hipEventRecord(c->ev1, s));

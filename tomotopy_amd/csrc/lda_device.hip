@@ -299,17 +299,18 @@ __device__ __forceinline__ void store_col(const WaveLds& S, const SweepParams& p
     }
 }
 
-// Tokens per group of shared n_kw atomics in the device-wide sweeps.  At
-// K = 1000 over 30M tokens, groups of 16 / 32 / 64 take 11.5 / 11.1 / 10.9 ms
-// per sweep (14.0 ms with per-token atomics) and leave ll_per_word after 12
-// sweeps 0.02% / 0.03% / 0.06% below the per-token schedule's.
-constexpr int kAtomicFlush = 32;
+// Tokens per group of shared n_kw atomics in a device-wide sweep (the host
+// sets SweepParams::col_group, a power of two up to 64; 64 otherwise).
+__device__ __forceinline__ int col_group_of(const SweepParams& p) {
+    const int g = p.col_group;
+    return (g >= 1 && g <= 64 && (g & (g - 1)) == 0) ? g : 64;
+}
 
 // sampleDocument over the token range [b, e) (LDAModel.hpp:326-354).  Tokens
 // come in blocks of 64, token t of a block on lane t (its word, z and u); the
 // column of the next distinct word is fetched one token ahead and, for a
 // narrow word, unpacked only when it is stored to LDS.  MODE_ATOMIC sends the
-// n_kw changes of each group of kAtomicFlush tokens to the shared n_kw at the
+// n_kw changes of each group of col_group tokens to the shared n_kw at the
 // group's end as whole-wavefront atomics, lane t those of token t (its word,
 // old and new topic): issued per token from single lanes they would hold every
 // later load wait behind an atomic's round trip (the vector memory counter
@@ -322,8 +323,7 @@ __device__ __forceinline__ void sample_range(const WaveLds& S, const fvec<KPL>& 
                                              int lane, uint64_t b, uint64_t e, int nvalid, int lastLane, E& et) {
     static_assert(!NARROW || MODE == MODE_ATOMIC, "narrow columns are for the shared device-wide n_kw");
     constexpr int SH = kpl_shift<KPL>();
-    constexpr int TF = MODE == MODE_ATOMIC ? kAtomicFlush : 64;
-    static_assert(TF >= 1 && TF <= 64 && (64 % TF) == 0, "flush groups divide the 64-token block");
+    const int TF = MODE == MODE_ATOMIC ? col_group_of(p) : 64;  // divides the 64-token block
     int vcur = -1, vnext = -1;
     int32_t* colp = nullptr;
     ivec<KPL> ncol;
@@ -635,7 +635,7 @@ __device__ __forceinline__ void sample_range_w(const WaveLdsW& S, const fvec<KPL
                                                E& et) {
     constexpr int SH = kpl_shift<KPL>();
     constexpr bool SHARED = MODE == MODE_ATOMIC;
-    constexpr int TF = MODE == MODE_ATOMIC ? kAtomicFlush : 64;
+    const int TF = MODE == MODE_ATOMIC ? col_group_of(p) : 64;
     float* nkwf = reinterpret_cast<float*>(p.nkw);
     int vcur = -1, vnext = -1;
     float* colp = nullptr;

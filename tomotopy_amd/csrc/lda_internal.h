@@ -64,6 +64,10 @@ struct SweepParams {
     // exact only below 2^21, so these run on the lda_bigk.hip kernels (n_dk
     // kept as integers) at any K
     int exact;
+    // device-wide sweeps (K <= 2048): tokens per group of shared n_kw atomics
+    // (a power of two up to 64; a wavefront holds back at most that many
+    // tokens' n_kw changes)
+    int col_group;
 };
 
 // Topics per lane: the register / LDS kernels take kpl <= kSmallMaxKpl
