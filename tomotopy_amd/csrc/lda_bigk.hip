@@ -298,21 +298,40 @@ __device__ __forceinline__ void bk_sample_range(bk_t<WT>* nd, bk_t<WT>* nk, bk_t
                                                 float* cdf = nullptr) {
     using T = bk_t<WT>;
     T* nkwT = reinterpret_cast<T*>(p.nkw);
+    // MODE_ATOMIC: the n_kw changes of each group of col_group tokens go out at
+    // the group's end as whole-wavefront atomics, lane t those of token t (the
+    // K <= 2048 sweep's rule: per-token single-lane atomics would hold the
+    // next column copy's loads behind their round trips)
+    int TF = 64;
+    if constexpr (MODE == MODE_ATOMIC) {
+        const int cg = p.col_group;
+        TF = (cg >= 1 && cg <= 64 && (cg & (cg - 1)) == 0) ? cg : 64;
+    }
     int vcur = -1;
     T* col = nullptr;
-    for (uint64_t t0 = b; t0 < e; t0 += 64) {
-        const int n = (int)((e - t0) < 64 ? (e - t0) : 64);
-        const uint64_t g = t0 + (uint64_t)lane;
-        int wv = -1, zv = 0xFFFF;
-        float uv = 0.0f, tw = 1.0f;
-        if (lane < n) {
-            wv = (int)p.words[g];
-            zv = p.z[g];
-            if constexpr (WT) tw = p.wt[g];
-            uv = uniform24(philox_x0(p.seed_lo, p.seed_hi, (uint32_t)g, (uint32_t)(g >> 32), sweep, TAG_SWEEP));
+    // the current block of up to 64 tokens, token t of it on lane t
+    int nb = 0, wv = -1, zv = 0xFFFF, zkeep = 0xFFFF;
+    float uv = 0.0f, tw = 1.0f;
+    uint64_t g = 0;
+    for (uint64_t t0 = b; t0 < e; t0 += TF) {
+        const int off = (int)((t0 - b) & 63);
+        if (off == 0) {
+            nb = (int)((e - t0) < 64 ? (e - t0) : 64);
+            g = t0 + (uint64_t)lane;
+            wv = -1;
+            zv = 0xFFFF;
+            uv = 0.0f;
+            tw = 1.0f;
+            if (lane < nb) {
+                wv = (int)p.words[g];
+                zv = p.z[g];
+                if constexpr (WT) tw = p.wt[g];
+                uv = uniform24(philox_x0(p.seed_lo, p.seed_hi, (uint32_t)g, (uint32_t)(g >> 32), sweep, TAG_SWEEP));
+            }
+            zkeep = zv;
         }
-        int zkeep = zv;
-        for (int i = 0; i < n; ++i) {
+        const int i1 = off + TF < nb ? off + TF : nb;
+        for (int i = off; i < i1; ++i) {
             const int v = readlane_i(wv, i);
             if ((unsigned)v >= (unsigned)p.V) continue;
             if (v != vcur) {
@@ -330,23 +349,23 @@ __device__ __forceinline__ void bk_sample_range(bk_t<WT>* nd, bk_t<WT>* nk, bk_t
             const float u = readlane_f(uv, i);
             const int zn = bk_token<CH, WT, E, SHARED>(nd, col, nk, alpha, lane, kpl, ksh, zo, w, u, p.K, nvalid, lastLane,
                                                        et, lastc, cdf, nullptr);
-            if (zn != zo) {
-                if constexpr (MODE == MODE_ATOMIC) {
-                    T* cp = nkwT + (int64_t)v * p.Kp;
-                    const int qo = kpos_rt(kpl, zo), qn = kpos_rt(kpl, zn);
-                    if constexpr (WT) {
-                        // the merge's clamp max(n_kw, 0) removes rounding-level negatives
-                        if (lane == (zo >> ksh)) unsafeAtomicAdd(cp + qo, -w);
-                        if (lane == (zn >> ksh)) unsafeAtomicAdd(cp + qn, w);
-                    } else {
-                        if (lane == (zo >> ksh)) atomicAdd(cp + qo, -1);
-                        if (lane == (zn >> ksh)) atomicAdd(cp + qn, 1);
-                    }
+            zkeep = (lane == i) ? zn : zkeep;
+        }
+        if constexpr (MODE == MODE_ATOMIC) {
+            if (lane >= off && lane < i1 && zkeep != zv && (unsigned)wv < (unsigned)p.V) {
+                T* cp = nkwT + (int64_t)wv * p.Kp;
+                const int qo = kpos_rt(kpl, zv), qn = kpos_rt(kpl, zkeep);
+                if constexpr (WT) {
+                    // the merge's clamp max(n_kw, 0) removes rounding-level negatives
+                    unsafeAtomicAdd(cp + qo, -tw);
+                    unsafeAtomicAdd(cp + qn, tw);
+                } else {
+                    atomicAdd(cp + qo, -1);
+                    atomicAdd(cp + qn, 1);
                 }
-                zkeep = (lane == i) ? zn : zkeep;
             }
         }
-        if (lane < n) p.z[g] = (uint16_t)zkeep;
+        if (i1 == nb && lane < nb) p.z[g] = (uint16_t)zkeep;
     }
 }
 
