@@ -158,7 +158,9 @@ int lda_set_global_step(lda_ctx* ctx, uint64_t step);
  *   DEFAULT with workers == 0  device-wide (no counterpart in tomotopy,
  *                              whose DEFAULT is PARTITION): every workgroup
  *                              is a worker with a private n_k; n_kw is
- *                              shared and updated with exact int32 atomics;
+ *                              shared and updated with exact int32 atomics,
+ *                              each wavefront sending those of a group of
+ *                              tokens at once (lda_set_atomic_group);
  *                              the column of the current word is read when
  *                              the word's run starts; at most ceil(D / 8)
  *                              documents in flight unless
@@ -381,6 +383,18 @@ int lda_kernel_stats(lda_ctx* ctx, double stats[4]);
  * matching the corpus) make the call run on int32 columns.  enable = 0 turns
  * the narrow columns off. */
 int lda_set_narrow_columns(lda_ctx* ctx, int enable);
+
+/* Tokens per group of shared n_kw atomics in the device-wide sweeps with K <=
+ * 2048 and K above it (no counterpart in the reference): each wavefront sends
+ * the n_kw changes of a group of its tokens at the group's end, as one atomic
+ * per token from the lane that holds it, instead of one single-lane atomic per
+ * token, which held the wavefront's next loads behind the atomic's round trip.
+ * A wavefront never rereads a column with changes pending, so one wavefront
+ * alone samples the same chain for every group; other workers see the changes
+ * up to one group later.  tokens = 0 (the default) picks the largest power of
+ * two up to 32 that keeps the changes held back by all wavefronts at most
+ * 1/128 of the corpus's tokens; otherwise a power of two up to 64. */
+int lda_set_atomic_group(lda_ctx* ctx, int tokens);
 
 #ifdef __cplusplus
 }

@@ -73,6 +73,7 @@ _SIGS = {
     "lda_infer_w": [_c_p, _c_p, _c_p, _i64, _c_p, _i32, _u64, _i32, _c_p, _c_p, _c_p],
     "lda_kernel_stats": [_c_p, _c_p],
     "lda_set_narrow_columns": [_c_p, _i32],
+    "lda_set_atomic_group": [_c_p, _i32],
     "lda_set_infer_options": [_c_p, _u32, _i32],
     "lda_get_infer_options": [_c_p, _c_p, _c_p, _c_p],
     "lda_serve_start": [_c_p, _u64, _i32, _i32, _i64, _i32, _u64, _c_p],
@@ -397,6 +398,10 @@ class Context:
     def set_narrow_columns(self, enable=True):
         """u16 n_kw columns of the less frequent words in the device-wide sweep (on by default)."""
         _check(load().lda_set_narrow_columns(self._h, int(bool(enable))))
+
+    def set_atomic_group(self, tokens=0):
+        """Tokens per group of shared n_kw atomics in the device-wide sweeps (0: chosen per launch)."""
+        _check(load().lda_set_atomic_group(self._h, int(tokens)))
 
 
 class Server:

@@ -369,6 +369,7 @@ constexpr int kBigSweepWaves = 4;
 constexpr int64_t kHeldBackDiv = 128;
 
 int device_col_group(const lda_ctx* c, int64_t wavefronts) {
+    if (c->dev_col_group > 0) return c->dev_col_group;
     int g = 32;
     while (g > 1 && (int64_t)g * wavefronts * kHeldBackDiv > (int64_t)c->n_valid) g >>= 1;
     return g;
@@ -966,6 +967,7 @@ int lda_ctx_copy(lda_ctx* c, lda_ctx** out) {
     t->dev_auto = c->dev_auto;
     t->dev_workers = c->dev_workers;
     t->dev_waves = c->dev_waves;
+    t->dev_col_group = c->dev_col_group;
     const hipStream_t s = t->stream;
     hipError_t e = hipSuccess;
     // the count buffers first (lda_ctx_create sized them; copy exactly)
@@ -2276,6 +2278,14 @@ int lda_kernel_stats(lda_ctx* c, double stats[4]) {
 int lda_set_narrow_columns(lda_ctx* c, int enable) {
     if (!c) return fail(LDA_ERR_INVALID_ARGUMENT, "ctx is NULL");
     c->narrow_enabled = enable != 0;
+    return LDA_OK;
+}
+
+int lda_set_atomic_group(lda_ctx* c, int tokens) {
+    if (!c) return fail(LDA_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (tokens < 0 || tokens > 64 || (tokens & (tokens - 1)) != 0)
+        return fail(LDA_ERR_INVALID_ARGUMENT, "tokens must be 0 or a power of two up to 64 (got %d)", tokens);
+    c->dev_col_group = tokens;
     return LDA_OK;
 }
 

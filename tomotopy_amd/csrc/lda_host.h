@@ -189,6 +189,9 @@ struct lda_ctx {
     bool dev_auto = true;
     int dev_workers = 0;
     int dev_waves = lda::kMaxSweepWaves;
+    // tokens per group of shared n_kw atomics in the device-wide sweeps
+    // (0: chosen per launch from the corpus and the wavefront count)
+    int dev_col_group = 0;
     // partition rounds of one sweep captured as a graph (rebuilt when the
     // parameters it was captured with change); the sweep number is read
     // from d_sweep so that one graph serves every sweep
