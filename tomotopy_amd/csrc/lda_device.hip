@@ -81,8 +81,11 @@ __device__ __forceinline__ void lds_store_f(float* a, int lane, const V& o) {
 // and refreshes the reciprocal.  Concurrent updates of one topic by two
 // wavefronts of a workgroup keep n_k exact; y then holds the reciprocal of a
 // value n_k had a moment earlier, which the Markstein step absorbs.
+// colg: the word's n_kw row when this worker owns its cells (the
+// deterministic schedules), which then takes the new cell value directly.
 template <int KPL, class E>
-__device__ __forceinline__ void bump(const WaveLds& S, const fvec<KPL>& al, int lane, int z, int delta, const E& et) {
+__device__ __forceinline__ void bump(const WaveLds& S, const fvec<KPL>& al, int lane, int z, int delta, const E& et,
+                                     int32_t* colg = nullptr) {
     const int sl = z & (KPL - 1);
     const float a_sl = al[sl];
     if (lane == (z >> kpl_shift<KPL>())) {
@@ -97,6 +100,7 @@ __device__ __forceinline__ void bump(const WaveLds& S, const fvec<KPL>& al, int 
         const float d = __builtin_rintf(a - a_sl) + (float)delta;
         S.A[q] = d + a_sl;
         S.col[q] = c + delta;
+        if (colg) colg[q] = c + delta;
         const float b = (float)n + et.s_at(q);
         S.bf[q] = b;
         S.y[q] = recip_rn(b);
@@ -205,8 +209,9 @@ __device__ __forceinline__ void set_nk(const WaveLds& S, const int32_t* __restri
 // real topics exactly when one of them is hit.
 template <int KPL, class E>
 __device__ __forceinline__ int token_step(const WaveLds& S, const fvec<KPL>& al, int lane, int zo, float u, int K,
-                                          int nvalid, int lastLane, const E& et, float* total) {
-    bump<KPL>(S, al, lane, zo, -1, et);
+                                          int nvalid, int lastLane, const E& et, float* total,
+                                          int32_t* colg = nullptr) {
+    bump<KPL>(S, al, lane, zo, -1, et, colg);
     fvec<KPL> A, y, bf, B;
     ivec<KPL> col;
     lds_load_f<KPL>(S.A, lane, A);
@@ -227,7 +232,7 @@ __device__ __forceinline__ int token_step(const WaveLds& S, const fvec<KPL>& al,
     }
     float tot;
     const int zn = scan_search<KPL>(w, u, K, nvalid, lastLane, tot);
-    bump<KPL>(S, al, lane, zn, +1, et);
+    bump<KPL>(S, al, lane, zn, +1, et, colg);
     if (total) *total = tot;
     return zn;
 }
@@ -367,16 +372,11 @@ __device__ __forceinline__ void sample_range(const WaveLds& S, const fvec<KPL>& 
             }
             const int zo = readlane_i(zv, i);
             const float u = readlane_f(uv, i);
-            const int zn = token_step<KPL>(S, al, lane, zo, u, p.K, nvalid, lastLane, et, nullptr);
-            if (zn != zo) {
-                if constexpr (MODE == MODE_DET) {
-                    const int lo = zo >> SH, ln = zn >> SH;
-                    const int qo = kpos<KPL>(lane, zo & (KPL - 1)), qn = kpos<KPL>(lane, zn & (KPL - 1));
-                    if (lane == lo) colp[qo] = S.col[qo];
-                    if (lane == ln) colp[qn] = S.col[qn];
-                }
-                zkeep = (lane == i) ? zn : zkeep;
-            }
+            // MODE_DET: the worker owns the word's cells, so the count
+            // updates write them to n_kw as they go
+            const int zn = token_step<KPL>(S, al, lane, zo, u, p.K, nvalid, lastLane, et, nullptr,
+                                           MODE == MODE_DET ? colp : nullptr);
+            zkeep = (lane == i) ? zn : zkeep;
         }
         if constexpr (MODE == MODE_ATOMIC) {
             if (lane >= off && lane < i1 && zkeep != zv && (unsigned)wv < (unsigned)p.V) {
@@ -439,7 +439,7 @@ __global__ void __launch_bounds__(64 * kSweepWaves) sweep_device_kernel(SweepPar
 // forShuffled order (Utils.hpp:157-178) and samples only their tokens in
 // vocabulary chunk p (coff rows p, p+1).  W == 1 is the `none` scheme.
 template <int KPL, bool ASYM>
-__global__ void __launch_bounds__(64) sweep_det_kernel(SweepParams p) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) sweep_det_kernel(SweepParams p) {
     const int lane = lane_id();
     const int w = blockIdx.x;
     const WaveLds S = wave_lds<KPL>();
@@ -491,7 +491,7 @@ __global__ void __launch_bounds__(64) sweep_det_kernel(SweepParams p) {
 // visiting documents id*chStride + c in forShuffled order.  Workers share no
 // counts, so the result does not depend on their timing.
 template <int KPL, bool ASYM>
-__global__ void __launch_bounds__(64) sweep_copy_kernel(SweepParams p) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) sweep_copy_kernel(SweepParams p) {
     const int lane = lane_id();
     const int w = blockIdx.x;
     const WaveLds S = wave_lds<KPL>();
@@ -581,29 +581,35 @@ __device__ __forceinline__ WaveLdsW wave_lds_w() {
 // the workgroup's wavefronts share n_k (LDS atomics; y then follows a value
 // n_k had a moment earlier, which the Markstein step absorbs).
 template <int KPL, bool SHARED, class E>
-__device__ __forceinline__ void bump_w(const WaveLdsW& S, int lane, int z, float w, bool dec, const E& et) {
+__device__ __forceinline__ void bump_w(const WaveLdsW& S, int lane, int z, float w, bool dec, const E& et,
+                                       float* colg = nullptr) {
     const int sl = z & (KPL - 1);
     if (lane == (z >> kpl_shift<KPL>())) {
         const int q = kpos<KPL>(lane, sl);
-        S.nd[q] = dec ? sub_clamp(S.nd[q], w) : S.nd[q] + w;
-        S.col[q] = dec ? sub_clamp(S.col[q], w) : S.col[q] + w;
+        // the cells' reads first, so that they are in flight together
+        const float d0 = S.nd[q], c0 = S.col[q];
         float n;
         if constexpr (SHARED) {
             if (dec) atomic_sub_clamp(S.nk + q, w);
             else atomicAdd(S.nk + q, w);
             n = S.nk[q];
         } else {
-            n = dec ? sub_clamp(S.nk[q], w) : S.nk[q] + w;
+            const float n0 = S.nk[q];
+            n = dec ? sub_clamp(n0, w) : n0 + w;
             S.nk[q] = n;
         }
+        const float c1 = dec ? sub_clamp(c0, w) : c0 + w;
+        S.nd[q] = dec ? sub_clamp(d0, w) : d0 + w;
+        S.col[q] = c1;
+        if (colg) colg[q] = c1;
         S.y[q] = recip_rn(n + et.s_at(q));
     }
 }
 
 template <int KPL, bool SHARED, class E>
 __device__ __forceinline__ int token_step_w(const WaveLdsW& S, const fvec<KPL>& al, int lane, int zo, float wt, float u,
-                                            int K, int nvalid, int lastLane, const E& et) {
-    bump_w<KPL, SHARED>(S, lane, zo, wt, true, et);
+                                            int K, int nvalid, int lastLane, const E& et, float* colg = nullptr) {
+    bump_w<KPL, SHARED>(S, lane, zo, wt, true, et, colg);
     fvec<KPL> nd, col, nk, y;
     lds_load_f<KPL>(S.nd, lane, nd);
     lds_load_f<KPL>(S.col, lane, col);
@@ -620,7 +626,7 @@ __device__ __forceinline__ int token_step_w(const WaveLdsW& S, const fvec<KPL>& 
     }
     float tot;
     const int zn = scan_search<KPL>(w, u, K, nvalid, lastLane, tot);
-    bump_w<KPL, SHARED>(S, lane, zn, wt, false, et);
+    bump_w<KPL, SHARED>(S, lane, zn, wt, false, et, colg);
     return zn;
 }
 
@@ -683,13 +689,10 @@ __device__ __forceinline__ void sample_range_w(const WaveLdsW& S, const fvec<KPL
             const int zo = readlane_i(zv, i);
             const float w = readlane_f(tw, i);
             const float u = readlane_f(uv, i);
-            const int zn = token_step_w<KPL, SHARED>(S, al, lane, zo, w, u, p.K, nvalid, lastLane, et);
-            if constexpr (MODE == MODE_DET) {
-                const int lo = zo >> SH, ln = zn >> SH;
-                const int qo = kpos<KPL>(lane, zo & (KPL - 1)), qn = kpos<KPL>(lane, zn & (KPL - 1));
-                if (lane == lo) colp[qo] = S.col[qo];
-                if (lane == ln) colp[qn] = S.col[qn];
-            }
+            // MODE_DET: the worker owns the word's cells, so the count
+            // updates write them to n_kw as they go
+            const int zn = token_step_w<KPL, SHARED>(S, al, lane, zo, w, u, p.K, nvalid, lastLane, et,
+                                                     MODE == MODE_DET ? colp : nullptr);
             zkeep = (lane == i) ? zn : zkeep;
         }
         if constexpr (MODE == MODE_ATOMIC) {
@@ -748,7 +751,7 @@ __global__ void __launch_bounds__(64 * kSweepWaves) sweep_device_w_kernel(SweepP
 // One round of the deterministic weighted partition schedule (see
 // sweep_det_kernel): worker-private n_k, plain stores to n_kw.
 template <int KPL, bool ASYM>
-__global__ void __launch_bounds__(64) sweep_det_w_kernel(SweepParams p) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) sweep_det_w_kernel(SweepParams p) {
     const int lane = lane_id();
     const int w = blockIdx.x;
     const WaveLdsW S = wave_lds_w<KPL>();
@@ -804,7 +807,7 @@ __global__ void __launch_bounds__(64) sweep_det_w_kernel(SweepParams p) {
 // float counts; worker w samples on its private n_kw (nkw + w*V*Kp) and n_k
 // (nk_out + w*Kp) with plain stores.
 template <int KPL, bool ASYM>
-__global__ void __launch_bounds__(64) sweep_copy_w_kernel(SweepParams p) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) sweep_copy_w_kernel(SweepParams p) {
     const int lane = lane_id();
     const int w = blockIdx.x;
     const WaveLdsW S = wave_lds_w<KPL>();
