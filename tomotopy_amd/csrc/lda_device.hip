@@ -391,7 +391,8 @@ __device__ __forceinline__ void sample_range(const WaveLds& S, const fvec<KPL>& 
 // Device-wide sweep: every resident wavefront is a worker with a private
 // n_k (snapshot nk_in at sweep start plus its own changes, as tomoto's
 // partition workers keep theirs, LDAModel.hpp:484, 596-602); documents are
-// taken longest-first from a global counter; n_kw is updated with atomics;
+// taken longest-first from a global counter; n_kw is updated with atomics
+// (per group of tokens, see sample_range);
 // each worker adds its n_k delta to nk_out when it runs out of documents.
 constexpr int kSweepWaves = kMaxSweepWaves;  // most wavefronts per workgroup of the device-wide sweep (one worker)
 
