@@ -502,17 +502,25 @@ __global__ void __launch_bounds__(64) bk_sweep_device_kernel(SweepParams p, int 
 // alpha and one private n_k (the snapshot plus the workgroup's changes, LDS
 // atomics), so that per token only the n_dk row and the wavefront's column
 // copy (bk_scratch row blockIdx.x * waves + wave) come from device memory.
-// Dynamic LDS: alpha (Kp) | n_k (Kp) | per wavefront the chunk last values.
+// ROWS: those two rows are in LDS too (the n_dk row staged in at the
+// document's start and written back at its end; each lane copies and uses
+// only its own positions, so no barrier is needed).
+// Dynamic LDS: alpha (Kp) | n_k (Kp) | per wavefront the chunk last values
+// (and with ROWS its n_dk row and column copy, Kp each).
 
-template <int CH, bool WT, bool ASYM>
+template <int CH, bool WT, bool ASYM, int ROWS>
 __global__ void __launch_bounds__(64 * kMaxSweepWaves) bk_sweep_device_lds_kernel(SweepParams p, int kpl, int ksh) {
     using T = bk_t<WT>;
     const int lane = lane_id();
     const int wave = (int)(threadIdx.x >> 6), waves = (int)(blockDim.x >> 6);
     const int Kp = p.Kp;
+    const int lc = (kpl / CH) * 64;
     float* al = reinterpret_cast<float*>(bk_lds);
     T* nkl = reinterpret_cast<T*>(bk_lds + Kp);
-    float* lastc = reinterpret_cast<float*>(bk_lds + 2 * Kp) + wave * (kpl / CH) * 64;
+    constexpr bool ND_L = (ROWS & 1) != 0, COL_L = (ROWS & 2) != 0;
+    float* lastc = reinterpret_cast<float*>(bk_lds + 2 * Kp) + wave * (lc + (ND_L ? Kp : 0) + (COL_L ? Kp : 0));
+    T* ndl = reinterpret_cast<T*>(lastc + lc);
+    T* coll = ndl + (ND_L ? Kp : 0);
     int nvalid, lastLane;
     bk_lane_limits(p.K, lane, kpl, nvalid, lastLane);
     auto et = bk_make_eta<ASYM>(p.prior_row, p.prior_eta, p.etasum, p.eta, p.veta, Kp);
@@ -522,7 +530,7 @@ __global__ void __launch_bounds__(64 * kMaxSweepWaves) bk_sweep_device_lds_kerne
         nkl[q] = snap[q];
     }
     __syncthreads();
-    T* colbuf = reinterpret_cast<T*>(p.bk_scratch) + ((int64_t)blockIdx.x * waves + wave) * Kp;
+    T* colbuf = COL_L ? coll : reinterpret_cast<T*>(p.bk_scratch) + ((int64_t)blockIdx.x * waves + wave) * Kp;
     for (;;) {
         uint32_t idx = 0;
         if (lane == 0) idx = atomicAdd(p.counter, 1u);
@@ -530,8 +538,17 @@ __global__ void __launch_bounds__(64 * kMaxSweepWaves) bk_sweep_device_lds_kerne
         if ((int64_t)idx >= p.D) break;
         const uint32_t d = p.doc_order[idx];
         T* nd = reinterpret_cast<T*>(p.ndk) + (int64_t)d * Kp;
-        bk_sample_range<CH, MODE_ATOMIC, WT, decltype(et), true>(nd, nkl, colbuf, al, p, p.sweep, lane, p.off[d],
-                                                                 p.off[d + 1], kpl, ksh, nvalid, lastLane, et, lastc);
+        if constexpr (ND_L) {
+            bk_copy<CH>(ndl, nd, lane, kpl);
+            bk_sample_range<CH, MODE_ATOMIC, WT, decltype(et), true>(ndl, nkl, colbuf, al, p, p.sweep, lane, p.off[d],
+                                                                     p.off[d + 1], kpl, ksh, nvalid, lastLane, et,
+                                                                     lastc);
+            bk_copy<CH>(nd, ndl, lane, kpl);
+        } else {
+            bk_sample_range<CH, MODE_ATOMIC, WT, decltype(et), true>(nd, nkl, colbuf, al, p, p.sweep, lane, p.off[d],
+                                                                     p.off[d + 1], kpl, ksh, nvalid, lastLane, et,
+                                                                     lastc);
+        }
     }
     if constexpr (!WT) {
         __syncthreads();
@@ -863,36 +880,61 @@ hipError_t launch_bk_sweep_copy(int kpl, const SweepParams& p, bool weighted, hi
     return hipGetLastError();
 }
 
-size_t bk_sweep_device_lds_bytes(int kpl, int Kp, int waves) {
+size_t bk_sweep_device_lds_bytes(int kpl, int Kp, int waves, int rows) {
     const int ch = kpl >= 16 ? 16 : kpl;
-    return ((size_t)2 * Kp + (size_t)waves * (kpl / ch) * 64) * sizeof(int);
+    const size_t per = (size_t)(kpl / ch) * 64 + ((rows & 1) ? (size_t)Kp : 0) + ((rows & 2) ? (size_t)Kp : 0);
+    return ((size_t)2 * Kp + (size_t)waves * per) * sizeof(int);
 }
 
 hipError_t launch_bk_sweep_device(int kpl, const SweepParams& p, bool weighted, int grid, int waves, hipStream_t s) {
     if (!bk_kpl_ok(kpl) || !p.bk_scratch || grid <= 0 || waves < 1 || waves > kMaxSweepWaves) return hipErrorInvalidValue;
+    const int rows = p.bk_rows & 3;
     const int sh = bk_shift(kpl);
-    if (waves == 1) {
+    if (waves == 1 && !rows) {
         LDA_BK_DISPATCH(kpl, weighted, p.prior_row != nullptr,
                         hipLaunchKernelGGL((bk_sweep_device_kernel<CH_, W_, A_>), dim3(grid), dim3(64), 0, s, p, kpl, sh));
     } else {
-        const size_t lds = bk_sweep_device_lds_bytes(kpl, p.Kp, waves);
-        LDA_BK_DISPATCH(kpl, weighted, p.prior_row != nullptr,
-                        hipLaunchKernelGGL((bk_sweep_device_lds_kernel<CH_, W_, A_>), dim3(grid), dim3(64 * waves), lds,
-                                           s, p, kpl, sh));
+        const size_t lds = bk_sweep_device_lds_bytes(kpl, p.Kp, waves, rows);
+        if (lds > 160 * 1024) return hipErrorInvalidValue;
+        if (rows == 3) {
+            LDA_BK_DISPATCH(kpl, weighted, p.prior_row != nullptr,
+                            hipLaunchKernelGGL((bk_sweep_device_lds_kernel<CH_, W_, A_, 3>), dim3(grid),
+                                               dim3(64 * waves), lds, s, p, kpl, sh));
+        } else if (rows == 1) {
+            LDA_BK_DISPATCH(kpl, weighted, p.prior_row != nullptr,
+                            hipLaunchKernelGGL((bk_sweep_device_lds_kernel<CH_, W_, A_, 1>), dim3(grid),
+                                               dim3(64 * waves), lds, s, p, kpl, sh));
+        } else if (rows == 0) {
+            LDA_BK_DISPATCH(kpl, weighted, p.prior_row != nullptr,
+                            hipLaunchKernelGGL((bk_sweep_device_lds_kernel<CH_, W_, A_, 0>), dim3(grid),
+                                               dim3(64 * waves), lds, s, p, kpl, sh));
+        } else {
+            return hipErrorInvalidValue;
+        }
     }
     return hipGetLastError();
 }
 
 // Resident workgroups per CU of the LDS variant (0: it does not fit).
-int bk_sweep_device_lds_blocks_per_cu(bool weighted, bool asym, int kpl, int Kp, int waves) {
-    const size_t lds = bk_sweep_device_lds_bytes(kpl, Kp, waves);
+int bk_sweep_device_lds_blocks_per_cu(bool weighted, bool asym, int kpl, int Kp, int waves, int rows) {
+    const size_t lds = bk_sweep_device_lds_bytes(kpl, Kp, waves, rows);
     if (lds > 160 * 1024) return 0;
     int n = 0;
     hipError_t e = hipSuccess;
     constexpr int CH_ = 16;
-    LDA_BK_WA(weighted, asym,
-              e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, bk_sweep_device_lds_kernel<CH_, W_, A_>, 64 * waves,
-                                                               lds));
+    if (rows == 3) {
+        LDA_BK_WA(weighted, asym,
+                  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, bk_sweep_device_lds_kernel<CH_, W_, A_, 3>,
+                                                                   64 * waves, lds));
+    } else if (rows == 1) {
+        LDA_BK_WA(weighted, asym,
+                  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, bk_sweep_device_lds_kernel<CH_, W_, A_, 1>,
+                                                                   64 * waves, lds));
+    } else if (rows == 0) {
+        LDA_BK_WA(weighted, asym,
+                  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, bk_sweep_device_lds_kernel<CH_, W_, A_, 0>,
+                                                                   64 * waves, lds));
+    }
     return e == hipSuccess ? n : 0;
 }
 

@@ -355,6 +355,7 @@ int device_copy_workers(lda_ctx* c) {
 constexpr int64_t kInFlightDiv = 8;
 constexpr int kBigSweepWaves = 4;
 
+
 // Tokens per group of shared n_kw atomics in the device-wide sweeps (K <=
 // 2048).  A wavefront sends its tokens' n_kw changes once per group, as whole-
 // wavefront atomics; per-token single-lane atomics held every later load wait
@@ -375,7 +376,8 @@ int device_col_group(const lda_ctx* c, int64_t wavefronts) {
     return g;
 }
 
-int device_wide_grid(lda_ctx* c, int* waves_out) {
+int device_wide_grid(lda_ctx* c, int* waves_out, int* rows_out = nullptr) {
+    if (rows_out) *rows_out = 0;
     const bool wtd = c->tw != LDA_TW_ONE;
     auto per_cu = [&](int waves) {
         return wtd ? lda::sweep_device_w_blocks_per_cu(c->kpl, c->has_priors, waves)
@@ -391,11 +393,30 @@ int device_wide_grid(lda_ctx* c, int* waves_out) {
     }
     const int64_t inflight = std::max<int64_t>((c->D + kInFlightDiv - 1) / kInFlightDiv, 1);
     if (c->kpl > lda::kSmallMaxKpl || c->exact_counts()) {
-        // K above 2048 or exact counts: workgroups of kBigSweepWaves wavefronts
-        // sharing alpha and n_k in LDS while four such workgroups fit a CU
-        // (K <= 4096), else one-wavefront workers with device-memory n_k
-        if (lda::bk_sweep_device_lds_bytes(c->kpl, c->Kp, kBigSweepWaves) <= 40 * 1024) {
-            const int per = lda::bk_sweep_device_lds_blocks_per_cu(wtd, c->has_priors, c->kpl, c->Kp, kBigSweepWaves);
+        // K above 2048 or exact counts.  While at least two wavefronts fit a
+        // workgroup with alpha, the shared n_k and each wavefront's n_dk row
+        // and column copy in LDS (K <= 4096: three), that workgroup, so that a
+        // token reads no device memory (K = 4096, 3M tokens: 18.7 ms per sweep
+        // against 22.1 with the rows in device memory; with only n_dk in LDS
+        // 19.3 ms at six wavefronts, 21.0 at four; at K = 8192 one such
+        // wavefront per CU takes 95 ms, two with only n_dk 64 ms, against
+        // 56.5 ms for the one-wavefront workers below).  Otherwise workgroups
+        // of kBigSweepWaves wavefronts sharing alpha and n_k in LDS while four
+        // such workgroups fit a CU, else one-wavefront workers with
+        // device-memory n_k.
+        if (c->kpl > lda::kSmallMaxKpl) {
+            for (int w = kBigSweepWaves; w >= 2; --w) {
+                const int per = lda::bk_sweep_device_lds_blocks_per_cu(wtd, c->has_priors, c->kpl, c->Kp, w, 3);
+                if (per <= 0) continue;
+                *waves_out = w;
+                if (rows_out) *rows_out = 3;
+                const int64_t wg = (inflight + w - 1) / w;
+                return (int)std::max<int64_t>(1, std::min<int64_t>(wg, (int64_t)c->num_cu * per));
+            }
+        }
+        if (lda::bk_sweep_device_lds_bytes(c->kpl, c->Kp, kBigSweepWaves, 0) <= 40 * 1024) {
+            const int per =
+                lda::bk_sweep_device_lds_blocks_per_cu(wtd, c->has_priors, c->kpl, c->Kp, kBigSweepWaves, 0);
             if (per > 0) {
                 *waves_out = kBigSweepWaves;
                 const int64_t wg = (inflight + kBigSweepWaves - 1) / kBigSweepWaves;
@@ -535,11 +556,13 @@ int sweep_once(lda_ctx* c, int W, bool copy_merge) {
             p.doc_order = c->d_order.p;
             p.counter = c->d_counter.p;
             int waves = 1;
-            const int grid = device_wide_grid(c, &waves);
+            int rows = 0;
+            const int grid = device_wide_grid(c, &waves, &rows);
             if (c->kpl > lda::kSmallMaxKpl || p.exact) {
                 HIP_TRY(c->d_bk_scratch.alloc((size_t)grid * std::max(2, waves) * c->Kp));
                 p.bk_scratch = c->d_bk_scratch.p;
             }
+            p.bk_rows = rows;
             p.col_group = device_col_group(c, (int64_t)grid * waves);
             HIP_TRY(hipEventRecord(c->ev0, s));
             HIP_TRY(lda::launch_sweep_device_w(c->kpl, p, grid, waves, s));
@@ -601,11 +624,13 @@ int sweep_once(lda_ctx* c, int W, bool copy_merge) {
         p.doc_order = c->d_order.p;
         p.counter = c->d_counter.p;
         int waves = 1;
-        const int grid = device_wide_grid(c, &waves);
+        int rows = 0;
+        const int grid = device_wide_grid(c, &waves, &rows);
         if (c->kpl > lda::kSmallMaxKpl || p.exact) {
             HIP_TRY(c->d_bk_scratch.alloc((size_t)grid * std::max(2, waves) * c->Kp));
             p.bk_scratch = c->d_bk_scratch.p;
         }
+        p.bk_rows = rows;
         p.col_group = device_col_group(c, (int64_t)grid * waves);
         HIP_TRY(hipEventRecord(c->ev0, s));
         HIP_TRY(lda::launch_sweep_device(c->kpl, p, grid, waves, s));

@@ -68,6 +68,10 @@ struct SweepParams {
     // (a power of two up to 64; a wavefront holds back at most that many
     // tokens' n_kw changes)
     int col_group;
+    // device-wide sweeps with K above 2048 in workgroups of several
+    // wavefronts: each wavefront's n_dk row (bit 0) and column copy (bit 1)
+    // in LDS as well
+    int bk_rows;
 };
 
 // Topics per lane: the register / LDS kernels take kpl <= kSmallMaxKpl
@@ -320,8 +324,8 @@ hipError_t launch_bk_sweep_copy(int kpl, const SweepParams& p, bool weighted, hi
 // waves > 1: workgroups of `waves` wavefronts sharing alpha and n_k in LDS
 // (bk_scratch then holds one column row per wavefront)
 hipError_t launch_bk_sweep_device(int kpl, const SweepParams& p, bool weighted, int grid, int waves, hipStream_t s);
-size_t bk_sweep_device_lds_bytes(int kpl, int Kp, int waves);
-int bk_sweep_device_lds_blocks_per_cu(bool weighted, bool asym, int kpl, int Kp, int waves);
+size_t bk_sweep_device_lds_bytes(int kpl, int Kp, int waves, int rows);
+int bk_sweep_device_lds_blocks_per_cu(bool weighted, bool asym, int kpl, int Kp, int waves, int rows);
 int bk_sweep_blocks_per_cu(bool weighted, bool asym, bool device, int kpl);
 hipError_t launch_bk_sample_frozen(int kpl, const int32_t* ndk, const int32_t* nkw, const int32_t* nk,
                                    const float* alpha, float eta, float veta, int K, int Kp, int64_t n, int grid,
