@@ -171,6 +171,35 @@ def test_device_wide_sweep_big_k(tw):
     assert ll1 > ll0
 
 
+@pytest.mark.parametrize("K", [3000, 4096])
+def test_device_wide_sweep_big_k_priors_and_rows(K):
+    """The device-wide schedule with word priors at K = 3000 and 4096, where
+    workgroups keep each wavefront's n_dk row and column copy in LDS: exact
+    count invariants after every sweep, each document's row written back, and
+    the LL rises."""
+    ws, offs, V = _corpus(300)
+    rng = np.random.default_rng(K)
+    pw = rng.choice(V, size=min(30, V), replace=False).astype(np.uint32)
+    pri = (rng.random((len(pw), K)) * 2.0).astype(np.float32)
+    ctx = _native.Context(K, V, 0.1, 0.01, seed=K)
+    ctx.set_word_priors(pw, pri)
+    ctx.upload_corpus(ws, offs)
+    ctx.set_optim(0, 0)
+    ll0 = sum(ctx.loglik())
+    assert ctx.resolve_schedule(0, 0)[0] == 0
+    for it in range(3):
+        ctx.train(1, workers=0, scheme=0)
+        assert not ctx.check_invariants().any()
+    _, ndk, nkw, nk = ctx.download_state()
+    assert np.array_equal(nk, nkw.sum(axis=0))
+    cs = np.concatenate([[0], np.cumsum(ws < V)])
+    o = offs.astype(np.int64)
+    assert np.array_equal(ndk.sum(axis=1), cs[o[1:]] - cs[o[:-1]])
+    ll1 = sum(ctx.loglik())
+    print("K", K, "ll", ll0, "->", ll1, flush=True)
+    assert ll1 > ll0
+
+
 @pytest.mark.parametrize("kind", ["one", "pmi", "prior"])
 def test_fold_in_bit_exact_big_k(kind):
     """Fold-in at K = 4096 (one wavefront per document, no segmentation):
