@@ -325,12 +325,16 @@ __device__ __forceinline__ int col_group_of(const SweepParams& p) {
 // group is sent before the next document).
 template <int KPL, int MODE, class E, bool NARROW = false>
 __device__ __forceinline__ void sample_range(const WaveLds& S, const fvec<KPL>& al, const SweepParams& p, uint32_t sweep,
-                                             int lane, uint64_t b, uint64_t e, int nvalid, int lastLane, E& et) {
+                                             int lane, uint64_t b, uint64_t e, int nvalid, int lastLane, E& et,
+                                             int* vcur_io = nullptr) {
     static_assert(!NARROW || MODE == MODE_ATOMIC, "narrow columns are for the shared device-wide n_kw");
     constexpr int SH = kpl_shift<KPL>();
     const int TF = MODE == MODE_ATOMIC ? col_group_of(p) : 64;  // divides the 64-token block
-    int vcur = -1, vnext = -1;
-    int32_t* colp = nullptr;
+    // vcur_io (MODE_DET): the word whose column S.col holds from the worker's
+    // previous range; its cells are this worker's alone for the round and
+    // S.col carries every update, so a range starting with it reads no column
+    int vcur = vcur_io ? *vcur_io : -1, vnext = -1;
+    int32_t* colp = vcur >= 0 ? p.nkw + (int64_t)vcur * p.Kp : nullptr;
     ivec<KPL> ncol;
     // the current block of up to 64 tokens, token t of it on lane t
     int nb = 0, wv = -1, zv = 0xFFFF, zkeep = 0xFFFF;
@@ -386,6 +390,7 @@ __device__ __forceinline__ void sample_range(const WaveLds& S, const fvec<KPL>& 
         }
         if (i1 == nb && lane < nb) p.z[g] = (uint16_t)zkeep;
     }
+    if (vcur_io) *vcur_io = vcur;
 }
 
 // Device-wide sweep: every resident wavefront is a worker with a private
@@ -466,19 +471,38 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2)))
         }
         P %= N;
     }
+    // Each cell's bounds and n_dk row are loaded while the previous cell is
+    // sampled (a worker's documents in a round are distinct and no other
+    // worker touches them), and the current word's column stays in LDS from
+    // one cell to the next (the chunk's cells are this worker's alone).
+    auto doc_of = [&](uint64_t i) { return (int64_t)(((i + (uint64_t)sseed) * P) % N) * W + didx; };
+    int vcur = -1;
+    int64_t dn = 0;
+    uint64_t bn = 0, en = 0;
+    ivec<KPL> ndn;
+    if (N) {
+        dn = doc_of(0);
+        bn = p.coff[(int64_t)w * p.D + dn];
+        en = p.coff[(int64_t)(w + 1) * p.D + dn];
+        load_kvec<KPL>(p.ndk + dn * p.Kp, lane, ndn);
+    }
     for (uint64_t i = 0; i < N; ++i) {
-        const uint64_t id = ((i + (uint64_t)sseed) * P) % N;
-        const int64_t d = (int64_t)id * W + didx;
-        const uint64_t b = p.coff[(int64_t)w * p.D + d], e = p.coff[(int64_t)(w + 1) * p.D + d];
-        if (b == e) continue;
+        const int64_t d = dn;
+        const uint64_t b = bn, e = en;
+        const ivec<KPL> nd0 = ndn;
         // earlier cells' n_kw stores of this worker complete before this cell reads columns
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+        if (i + 1 < N) {
+            dn = doc_of(i + 1);
+            bn = p.coff[(int64_t)w * p.D + dn];
+            en = p.coff[(int64_t)(w + 1) * p.D + dn];
+            load_kvec<KPL>(p.ndk + dn * p.Kp, lane, ndn);  // (unused when the cell is empty)
+        }
+        if (b == e) continue;
         int32_t* ndk_row = p.ndk + d * p.Kp;
-        ivec<KPL> nd;
-        load_kvec<KPL>(ndk_row, lane, nd);
-        set_ndk<KPL>(S, al, nd, lane);
-        sample_range<KPL, MODE_DET>(S, al, p, sweep, lane, b, e, nvalid, lastLane, et);
-        nd = get_ndk<KPL>(S, al, lane);
+        set_ndk<KPL>(S, al, nd0, lane);
+        sample_range<KPL, MODE_DET>(S, al, p, sweep, lane, b, e, nvalid, lastLane, et, &vcur);
+        const ivec<KPL> nd = get_ndk<KPL>(S, al, lane);
         store_kvec<KPL>(ndk_row, lane, nd);
     }
     ivec<KPL> mine;
@@ -507,6 +531,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2)))
     q.nkw = p.nkw + (int64_t)w * p.V * p.Kp;
     const int W = p.W;
     const int64_t chStride = (int64_t)W * 8 < p.D ? (int64_t)W * 8 : p.D;
+    int vcur = -1;
     for (int64_t ch = w; ch < chStride; ch += W) {
         const uint64_t N = (uint64_t)((p.D + (chStride - 1) - ch) / chStride);
         const uint32_t sseed = philox_x0(p.seed_lo, p.seed_hi, (uint32_t)ch, 0xC0u, p.sweep, TAG_SHUFFLE);
@@ -529,7 +554,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2)))
             ivec<KPL> nd;
             load_kvec<KPL>(ndk_row, lane, nd);
             set_ndk<KPL>(S, al, nd, lane);
-            sample_range<KPL, MODE_DET>(S, al, q, q.sweep, lane, b, e, nvalid, lastLane, et);
+            // the worker's private n_kw copy: its column in LDS stays current
+            sample_range<KPL, MODE_DET>(S, al, q, q.sweep, lane, b, e, nvalid, lastLane, et, &vcur);
             nd = get_ndk<KPL>(S, al, lane);
             store_kvec<KPL>(ndk_row, lane, nd);
         }
