@@ -1052,13 +1052,21 @@ __global__ void fill_u32_kernel(uint32_t* dst, uint32_t v, int64_t n) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) dst[i] = v;
 }
 
-// n_k = rowwise sum of n_kw (mergeState, LDAModel.hpp:569, 578).
-__global__ void rowsum_kernel(const int32_t* __restrict__ nkw, int V, int Kp, int32_t* nk) {
-    int pos = blockIdx.x * blockDim.x + threadIdx.x;
+// n_k = the rowwise sum of n_kw (mergeState, LDAModel.hpp:569, 578).  Integer
+// sums are exact in any order, so
+// blocks of `words` words (kRowsumWords, more for a vocabulary beyond 65535
+// such chunks) x 256 positions add their partial sums into nk (cleared first)
+// with atomics: the whole V x Kp array is read in parallel.
+constexpr int kRowsumWords = 256;
+
+__global__ void rowsum_chunks_kernel(const int32_t* __restrict__ nkw, int V, int Kp, int words, int32_t* nk) {
+    const int pos = blockIdx.x * blockDim.x + threadIdx.x;
     if (pos >= Kp) return;
-    long long s = 0;
-    for (int v = 0; v < V; ++v) s += nkw[(int64_t)v * Kp + pos];
-    nk[pos] = (int32_t)s;
+    const int64_t v0 = (int64_t)blockIdx.y * words;
+    const int64_t v1 = v0 + words < V ? v0 + words : V;
+    int s = 0;
+    for (int64_t v = v0; v < v1; ++v) s += nkw[v * Kp + pos];
+    if (s) atomicAdd(nk + pos, s);
 }
 
 // ------------------------------------------------------------ loglik ------
@@ -1366,12 +1374,17 @@ __global__ void ndk_rows_kernel(const int32_t* __restrict__ ndk, const int32_t* 
     }
 }
 
-__global__ void colsum_kernel(const int32_t* __restrict__ nkw, int V, int Kp, long long* colsum) {
-    int pos = blockIdx.x * blockDim.x + threadIdx.x;
+// Per topic position the sum of n_kw over the words, in 64 bits: chunks of
+// `words` words add their partial sums with atomics into colsum (cleared
+// first; two's complement, so negative cells of a broken state sum exactly).
+__global__ void colsum_chunks_kernel(const int32_t* __restrict__ nkw, int V, int Kp, int words, long long* colsum) {
+    const int pos = blockIdx.x * blockDim.x + threadIdx.x;
     if (pos >= Kp) return;
+    const int64_t v0 = (int64_t)blockIdx.y * words;
+    const int64_t v1 = v0 + words < V ? v0 + words : V;
     long long s = 0;
-    for (int v = 0; v < V; ++v) s += nkw[(int64_t)v * Kp + pos];
-    colsum[pos] = s;
+    for (int64_t v = v0; v < v1; ++v) s += nkw[v * Kp + pos];
+    if (s) atomicAdd(reinterpret_cast<unsigned long long*>(colsum + pos), (unsigned long long)s);
 }
 
 // ------------------------------------------------------ frozen sampling ----
@@ -2873,7 +2886,11 @@ hipError_t launch_fill_u32(uint32_t* dst, uint32_t v, int64_t n, hipStream_t s) 
 }
 
 hipError_t launch_rowsum_nkw(const int32_t* nkw, int V, int Kp, int32_t* nk, hipStream_t s) {
-    hipLaunchKernelGGL(rowsum_kernel, dim3((Kp + 255) / 256), dim3(256), 0, s, nkw, V, Kp, nk);
+    hipError_t e = launch_fill_u32(reinterpret_cast<uint32_t*>(nk), 0u, Kp, s);
+    if (e != hipSuccess || V <= 0) return e;
+    const int words = std::max(kRowsumWords, (V + 65534) / 65535);
+    const unsigned chunks = (unsigned)((V + words - 1) / words);
+    hipLaunchKernelGGL(rowsum_chunks_kernel, dim3((Kp + 255) / 256, chunks), dim3(256), 0, s, nkw, V, Kp, words, nk);
     return hipGetLastError();
 }
 
@@ -3058,7 +3075,14 @@ hipError_t launch_invariants(int kpl, const uint32_t* words, const uint16_t* z, 
     hipLaunchKernelGGL(compare_kernel, dim3(1024), dim3(256), 0, s, ndk, sndk, D * (int64_t)Kp, flags + 3, flags + 5);
     hipLaunchKernelGGL(compare_kernel, dim3(1024), dim3(256), 0, s, nkw, snkw, (int64_t)V * Kp, flags + 4, flags + 5);
     hipLaunchKernelGGL(ndk_rows_kernel, dim3((unsigned)blocks), dim3(256), 0, s, ndk, doclen, D, Kp, flags + 0);
-    hipLaunchKernelGGL(colsum_kernel, dim3((Kp + 255) / 256), dim3(256), 0, s, nkw, V, Kp, colsum);
+    hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned)((2 * (int64_t)Kp + 255) / 256)), dim3(256), 0, s,
+                       reinterpret_cast<uint32_t*>(colsum), 0u, 2 * (int64_t)Kp);
+    if (V > 0) {
+        const int words = std::max(kRowsumWords, (V + 65534) / 65535);
+        const unsigned chunks = (unsigned)((V + words - 1) / words);
+        hipLaunchKernelGGL(colsum_chunks_kernel, dim3((Kp + 255) / 256, chunks), dim3(256), 0, s, nkw, V, Kp, words,
+                           colsum);
+    }
     return hipGetLastError();
 }
 
