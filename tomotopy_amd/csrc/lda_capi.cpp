@@ -524,6 +524,18 @@ int run_partition_rounds(lda_ctx* c, lda::SweepParams p, int W, bool weighted) {
     return LDA_OK;
 }
 
+// The time of the device-wide sweep whose events are pending (see lda_ctx).
+int dw_read(lda_ctx* c) {
+    if (c->dw_pending < 0) return LDA_OK;
+    const int k = c->dw_pending;
+    c->dw_pending = -1;
+    HIP_TRY(hipEventSynchronize(c->dw_ev[k][1]));
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, c->dw_ev[k][0], c->dw_ev[k][1]));
+    c->stat_ms += ms;
+    return LDA_OK;
+}
+
 int sweep_once(lda_ctx* c, int W, bool copy_merge) {
     lda::SweepParams p = base_params(c);
     const hipStream_t s = c->stream;
@@ -532,6 +544,7 @@ int sweep_once(lda_ctx* c, int W, bool copy_merge) {
         // weighted merge n_kw = max(n_kw, 0), n_k = rowsum (LDAModel.hpp:565-579)
         p.wt = c->d_wt.p;
         float ms = 0.0f;
+        int dw = -1;  // the device-wide sweep's event pair
         if (copy_merge) {
             const int64_t n = (int64_t)c->V * c->Kp;
             HIP_TRY(c->d_nkw_loc.alloc((size_t)W * std::max<int64_t>(n, 1)));
@@ -564,9 +577,11 @@ int sweep_once(lda_ctx* c, int W, bool copy_merge) {
             }
             p.bk_rows = rows;
             p.col_group = device_col_group(c, (int64_t)grid * waves);
-            HIP_TRY(hipEventRecord(c->ev0, s));
+            dw = c->dw_next;
+            c->dw_next ^= 1;
+            HIP_TRY(hipEventRecord(c->dw_ev[dw][0], s));
             HIP_TRY(lda::launch_sweep_device_w(c->kpl, p, grid, waves, s));
-            HIP_TRY(hipEventRecord(c->ev1, s));
+            HIP_TRY(hipEventRecord(c->dw_ev[dw][1], s));
             c->stat_launches += 1;
         } else {
             int rc = prepare_partition(c, W);
@@ -584,10 +599,16 @@ int sweep_once(lda_ctx* c, int W, bool copy_merge) {
         }
         HIP_TRY(lda::launch_clamp_rowsum_f(reinterpret_cast<float*>(c->d_nkw.p), c->V, c->Kp, c->d_rowpart.p,
                                            reinterpret_cast<float*>(c->d_nk.p), s));
-        HIP_TRY(hipEventSynchronize(c->ev1));
-        HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-        c->stat_ms += ms;
-        HIP_TRY(hipStreamSynchronize(s));
+        if (dw >= 0) {
+            // the previous sweep's time; this one's is read after the next is queued
+            if (int rc = dw_read(c)) return rc;
+            c->dw_pending = dw;
+        } else {
+            HIP_TRY(hipEventSynchronize(c->ev1));
+            HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+            c->stat_ms += ms;
+            HIP_TRY(hipStreamSynchronize(s));
+        }
         c->stat_tokens += (double)c->n_valid;
         return LDA_OK;
     }
@@ -632,13 +653,14 @@ int sweep_once(lda_ctx* c, int W, bool copy_merge) {
         }
         p.bk_rows = rows;
         p.col_group = device_col_group(c, (int64_t)grid * waves);
-        HIP_TRY(hipEventRecord(c->ev0, s));
+        const int dw = c->dw_next;
+        c->dw_next ^= 1;
+        HIP_TRY(hipEventRecord(c->dw_ev[dw][0], s));
         HIP_TRY(lda::launch_sweep_device(c->kpl, p, grid, waves, s));
-        HIP_TRY(hipEventRecord(c->ev1, s));
-        HIP_TRY(hipEventSynchronize(c->ev1));
-        float ms = 0.0f;
-        HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-        c->stat_ms += ms;
+        HIP_TRY(hipEventRecord(c->dw_ev[dw][1], s));
+        // the previous sweep's time; this one's is read after the next is queued
+        if (int rc = dw_read(c)) return rc;
+        c->dw_pending = dw;
         c->stat_launches += 1;
         std::swap(c->d_nk, c->d_nk_next);
     } else {
@@ -919,6 +941,9 @@ int lda_ctx_create(int K, int V, const float* alpha, float eta, int termweight, 
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return cleanup(fail(LDA_ERR_DEVICE, "hipStreamCreate failed"));
     if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
         return cleanup(fail(LDA_ERR_DEVICE, "hipEventCreate failed"));
+    for (auto& pair : c->dw_ev)
+        for (auto& ev : pair)
+            if (hipEventCreate(&ev) != hipSuccess) return cleanup(fail(LDA_ERR_DEVICE, "hipEventCreate failed"));
     if (c->fold_streams.create() != hipSuccess) return cleanup(fail(LDA_ERR_DEVICE, "fold-in stream creation failed"));
     compute_fold_grid(c);
     if (int src = noalloc_scratch_init(device)) return cleanup(src);
@@ -954,6 +979,9 @@ int lda_ctx_destroy(lda_ctx* c) {
     c->d_nkw16.release(); c->r_out.release(); c->r_cnt.release();
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    for (auto& pair : c->dw_ev)
+        for (auto& ev : pair)
+            if (ev) (void)hipEventDestroy(ev);
     c->fold_streams.destroy();
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1314,8 +1342,9 @@ int lda_sweep(lda_ctx* c, int workers, int scheme) {
     rc = narrow_begin(c, W, cm);
     if (rc) return rc;
     rc = sweep_once(c, W, cm);
+    const int rc1 = dw_read(c);
     const int rc2 = narrow_end(c);
-    return rc ? rc : rc2;
+    return rc ? rc : (rc1 ? rc1 : rc2);
 }
 
 int lda_train(lda_ctx* c, int iterations, int workers, int scheme) {
@@ -1344,8 +1373,9 @@ int lda_train(lda_ctx* c, int iterations, int workers, int scheme) {
         }
         ++c->global_step;
     }
+    const int rc1 = dw_read(c);
     const int rc2 = narrow_end(c);
-    return rc ? rc : rc2;
+    return rc ? rc : (rc1 ? rc1 : rc2);
 }
 
 int lda_optimize_alpha(lda_ctx* c) {

@@ -104,6 +104,11 @@ struct lda_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // device-wide sweeps: event pairs of the last two sweeps; a sweep's time
+    // is read once the next one is queued, so the host does not leave the
+    // device idle between sweeps (dw_pending: the pair not yet read, or -1)
+    hipEvent_t dw_ev[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    int dw_next = 0, dw_pending = -1;
     int K = 0, V = 0, kpl = 1, Kp = 64;
     float eta = 0.01f, veta = 0.0f;
     int tw = LDA_TW_ONE;
