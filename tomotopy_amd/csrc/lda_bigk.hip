@@ -369,6 +369,11 @@ __device__ __forceinline__ void bk_sample_range(bk_t<WT>* nd, bk_t<WT>* nk, bk_t
     }
 }
 
+// The one-wavefront deterministic sweeps keep the document's n_dk row and the
+// worker's n_k in LDS as well up to 128 topics per lane (98 KB with the
+// running sums at 128).
+__host__ __device__ constexpr bool bk_det_rows(int kpl) { return kpl <= 128; }
+
 // Dynamic LDS of the kernels below.  One-wavefront kernels: the running sums
 // (Kp floats, when kept) then the chunk last values (64 per chunk).  The
 // sweeps keep the sums at every K (a few wavefronts run at once); the kernels
@@ -400,12 +405,19 @@ __global__ void __launch_bounds__(64) bk_sweep_det_kernel(SweepParams p, int kpl
     using T = bk_t<WT>;
     float* cdf = reinterpret_cast<float*>(bk_lds);
     float* lastc = cdf + p.Kp;
+    // with rows (bk_det_rows): the document's n_dk row and the worker's n_k
+    // in LDS after the chunk last values, staged in and written back
+    T* ndl = reinterpret_cast<T*>(lastc + (kpl / CH) * 64);
+    T* nkl = ndl + p.Kp;
+    const bool rows = bk_det_rows(kpl);
     const int lane = lane_id();
     const int w = blockIdx.x;
     int nvalid, lastLane;
     bk_lane_limits(p.K, lane, kpl, nvalid, lastLane);
     auto et = bk_make_eta<ASYM>(p.prior_row, p.prior_eta, p.etasum, p.eta, p.veta, p.Kp);
-    T* nkp = reinterpret_cast<T*>(p.nk_out) + (int64_t)w * p.Kp;
+    T* nkg = reinterpret_cast<T*>(p.nk_out) + (int64_t)w * p.Kp;
+    if (rows) bk_copy<CH>(nkl, nkg, lane, kpl);
+    T* nkp = rows ? nkl : nkg;
     const int W = p.W;
     const uint32_t sweep = p.sweep_dev ? (uint32_t)uniform_i((int)*p.sweep_dev) : p.sweep;
     const int didx = (p.round + w) % W;
@@ -418,9 +430,12 @@ __global__ void __launch_bounds__(64) bk_sweep_det_kernel(SweepParams p, int kpl
         const uint64_t b = p.coff[(int64_t)w * p.D + d], e = p.coff[(int64_t)(w + 1) * p.D + d];
         if (b == e) continue;
         T* nd = reinterpret_cast<T*>(p.ndk) + d * p.Kp;
-        bk_sample_range<CH, MODE_DET, WT>(nd, nkp, nullptr, p.alpha, p, sweep, lane, b, e, kpl, ksh, nvalid, lastLane, et,
-                                          lastc, cdf);
+        if (rows) bk_copy<CH>(ndl, nd, lane, kpl);
+        bk_sample_range<CH, MODE_DET, WT>(rows ? ndl : nd, nkp, nullptr, p.alpha, p, sweep, lane, b, e, kpl, ksh, nvalid,
+                                          lastLane, et, lastc, cdf);
+        if (rows) bk_copy<CH>(nd, ndl, lane, kpl);
     }
+    if (rows) bk_copy<CH>(nkg, nkl, lane, kpl);
 }
 
 // copy_merge (LDAModel.hpp:401-423): worker w samples on its private n_kw
@@ -431,12 +446,17 @@ __global__ void __launch_bounds__(64) bk_sweep_copy_kernel(SweepParams p, int kp
     using T = bk_t<WT>;
     float* cdf = reinterpret_cast<float*>(bk_lds);
     float* lastc = cdf + p.Kp;
+    T* ndl = reinterpret_cast<T*>(lastc + (kpl / CH) * 64);  // rows in LDS, as in bk_sweep_det_kernel
+    T* nkl = ndl + p.Kp;
+    const bool rows = bk_det_rows(kpl);
     const int lane = lane_id();
     const int w = blockIdx.x;
     int nvalid, lastLane;
     bk_lane_limits(p.K, lane, kpl, nvalid, lastLane);
     auto et = bk_make_eta<ASYM>(p.prior_row, p.prior_eta, p.etasum, p.eta, p.veta, p.Kp);
-    T* nkp = reinterpret_cast<T*>(p.nk_out) + (int64_t)w * p.Kp;
+    T* nkg = reinterpret_cast<T*>(p.nk_out) + (int64_t)w * p.Kp;
+    if (rows) bk_copy<CH>(nkl, nkg, lane, kpl);
+    T* nkp = rows ? nkl : nkg;
     SweepParams q = p;
     q.nkw = p.nkw + (int64_t)w * p.V * p.Kp;
     const int W = p.W;
@@ -450,10 +470,13 @@ __global__ void __launch_bounds__(64) bk_sweep_copy_kernel(SweepParams p, int kp
             const uint64_t b = p.off[d], e = p.off[d + 1];
             if (b == e) continue;
             T* nd = reinterpret_cast<T*>(p.ndk) + d * p.Kp;
-            bk_sample_range<CH, MODE_DET, WT>(nd, nkp, nullptr, p.alpha, q, q.sweep, lane, b, e, kpl, ksh, nvalid,
-                                              lastLane, et, lastc, cdf);
+            if (rows) bk_copy<CH>(ndl, nd, lane, kpl);
+            bk_sample_range<CH, MODE_DET, WT>(rows ? ndl : nd, nkp, nullptr, p.alpha, q, q.sweep, lane, b, e, kpl, ksh,
+                                              nvalid, lastLane, et, lastc, cdf);
+            if (rows) bk_copy<CH>(nd, ndl, lane, kpl);
         }
     }
+    if (rows) bk_copy<CH>(nkg, nkl, lane, kpl);
 }
 
 // Device-wide sweep: every wavefront is a worker with a private n_k (the
@@ -845,6 +868,10 @@ size_t bk_cdf_lds_bytes(int kpl, bool many = false) {
     return (sums + (size_t)64 * (kpl / ch)) * sizeof(float);
 }
 
+size_t bk_det_lds_bytes(int kpl) {
+    return bk_cdf_lds_bytes(kpl) + (bk_det_rows(kpl) ? (size_t)2 * 64 * kpl * sizeof(int) : 0);
+}
+
 // Instantiations: CH = 16 for every count type (K above 256, which includes
 // every K above 2048); CH = KPL < 16 for integer counts only (the exact-count
 // path of K <= 2048, taken for documents over 2^20 tokens or alpha >= 2^20;
@@ -877,7 +904,7 @@ hipError_t launch_bk_sweep_det(int kpl, const SweepParams& p, bool weighted, hip
     if (!bk_kpl_ok(kpl)) return hipErrorInvalidValue;
     const int sh = bk_shift(kpl);
     LDA_BK_DISPATCH(kpl, weighted, p.prior_row != nullptr,
-                    hipLaunchKernelGGL((bk_sweep_det_kernel<CH_, W_, A_>), dim3(p.W), dim3(64), bk_cdf_lds_bytes(kpl), s,
+                    hipLaunchKernelGGL((bk_sweep_det_kernel<CH_, W_, A_>), dim3(p.W), dim3(64), bk_det_lds_bytes(kpl), s,
                                        p, kpl, sh));
     return hipGetLastError();
 }
@@ -886,7 +913,7 @@ hipError_t launch_bk_sweep_copy(int kpl, const SweepParams& p, bool weighted, hi
     if (!bk_kpl_ok(kpl)) return hipErrorInvalidValue;
     const int sh = bk_shift(kpl);
     LDA_BK_DISPATCH(kpl, weighted, p.prior_row != nullptr,
-                    hipLaunchKernelGGL((bk_sweep_copy_kernel<CH_, W_, A_>), dim3(p.W), dim3(64), bk_cdf_lds_bytes(kpl), s,
+                    hipLaunchKernelGGL((bk_sweep_copy_kernel<CH_, W_, A_>), dim3(p.W), dim3(64), bk_det_lds_bytes(kpl), s,
                                        p, kpl, sh));
     return hipGetLastError();
 }
@@ -960,7 +987,7 @@ int bk_sweep_blocks_per_cu(bool weighted, bool asym, bool device, int kpl) {
     LDA_BK_WA(weighted, asym,
               e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
                   &n, device ? (const void*)bk_sweep_device_kernel<CH_, W_, A_> : (const void*)bk_sweep_det_kernel<CH_, W_, A_>,
-                  64, device ? 0 : bk_cdf_lds_bytes(kpl)));
+                  64, device ? 0 : bk_det_lds_bytes(kpl)));
     return e == hipSuccess ? n : 0;
 }
 
