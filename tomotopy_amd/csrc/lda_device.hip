@@ -81,11 +81,8 @@ __device__ __forceinline__ void lds_store_f(float* a, int lane, const V& o) {
 // and refreshes the reciprocal.  Concurrent updates of one topic by two
 // wavefronts of a workgroup keep n_k exact; y then holds the reciprocal of a
 // value n_k had a moment earlier, which the Markstein step absorbs.
-// colg: the word's n_kw row when this worker owns its cells (the
-// deterministic schedules), which then takes the new cell value directly.
 template <int KPL, class E>
-__device__ __forceinline__ void bump(const WaveLds& S, const fvec<KPL>& al, int lane, int z, int delta, const E& et,
-                                     int32_t* colg = nullptr) {
+__device__ __forceinline__ void bump(const WaveLds& S, const fvec<KPL>& al, int lane, int z, int delta, const E& et) {
     const int sl = z & (KPL - 1);
     const float a_sl = al[sl];
     if (lane == (z >> kpl_shift<KPL>())) {
@@ -100,7 +97,6 @@ __device__ __forceinline__ void bump(const WaveLds& S, const fvec<KPL>& al, int 
         const float d = __builtin_rintf(a - a_sl) + (float)delta;
         S.A[q] = d + a_sl;
         S.col[q] = c + delta;
-        if (colg) colg[q] = c + delta;
         const float b = (float)n + et.s_at(q);
         S.bf[q] = b;
         S.y[q] = recip_rn(b);
@@ -209,9 +205,8 @@ __device__ __forceinline__ void set_nk(const WaveLds& S, const int32_t* __restri
 // real topics exactly when one of them is hit.
 template <int KPL, class E>
 __device__ __forceinline__ int token_step(const WaveLds& S, const fvec<KPL>& al, int lane, int zo, float u, int K,
-                                          int nvalid, int lastLane, const E& et, float* total,
-                                          int32_t* colg = nullptr) {
-    bump<KPL>(S, al, lane, zo, -1, et, colg);
+                                          int nvalid, int lastLane, const E& et, float* total) {
+    bump<KPL>(S, al, lane, zo, -1, et);
     fvec<KPL> A, y, bf, B;
     ivec<KPL> col;
     lds_load_f<KPL>(S.A, lane, A);
@@ -232,8 +227,72 @@ __device__ __forceinline__ int token_step(const WaveLds& S, const fvec<KPL>& al,
     }
     float tot;
     const int zn = scan_search<KPL>(w, u, K, nvalid, lastLane, tot);
-    bump<KPL>(S, al, lane, zn, +1, et, colg);
+    bump<KPL>(S, al, lane, zn, +1, et);
     if (total) *total = tot;
+    return zn;
+}
+
+// addWordTo<delta> on a one-wavefront worker's state held in registers (the
+// arrays as just loaded from LDS): the owner lane derives the new cell values
+// from its registers, writes them through to LDS and to the owned n_kw cell
+// colg, and keeps its registers current.  The values are bump's.
+template <int KPL, class E>
+__device__ __forceinline__ void bump_regs(const WaveLds& S, const fvec<KPL>& al, int lane, int z, int delta,
+                                          const E& et, int32_t* colg, fvec<KPL>& A, ivec<KPL>& col, ivec<KPL>& n,
+                                          fvec<KPL>& bf, fvec<KPL>& y) {
+    const int sl = z & (KPL - 1);
+    if (lane == (z >> kpl_shift<KPL>())) {
+        const int q = kpos<KPL>(lane, sl);
+        const float a_sl = al[sl];
+        const float a = (__builtin_rintf(A[sl] - a_sl) + (float)delta) + a_sl;
+        const int c = col[sl] + delta;
+        const int m = n[sl] + delta;
+        const float b = (float)m + et.s_at(q);
+        const float r = recip_rn(b);
+        S.A[q] = a;
+        S.col[q] = c;
+        S.nk[q] = m;
+        S.bf[q] = b;
+        S.y[q] = r;
+        if (colg) colg[q] = c;
+        A[sl] = a;
+        col[sl] = c;
+        n[sl] = m;
+        bf[sl] = b;
+        y[sl] = r;
+    }
+}
+
+// token_step for a worker that is one wavefront alone (its n_k private to it,
+// as in the deterministic schedules): all five arrays are read once, the
+// removal and the addition are applied to them in registers by their owner
+// lanes and written through, so a token waits for one LDS round trip instead
+// of three (bump waits for its cell reads before it writes).  Same operations
+// on the same counts as token_step, hence the same draw.
+template <int KPL, class E>
+__device__ __forceinline__ int token_step_solo(const WaveLds& S, const fvec<KPL>& al, int lane, int zo, float u,
+                                               int K, int nvalid, int lastLane, const E& et, int32_t* colg) {
+    fvec<KPL> A, y, bf, B;
+    ivec<KPL> col, n;
+    lds_load_f<KPL>(S.A, lane, A);
+    lds_load<KPL>(S.col, lane, col);
+    lds_load<KPL>(S.nk, lane, n);
+    lds_load_f<KPL>(S.bf, lane, bf);
+    lds_load_f<KPL>(S.y, lane, y);
+    bump_regs<KPL>(S, al, lane, zo, -1, et, colg, A, col, n, bf, y);
+#pragma unroll
+    for (int j = 0; j < KPL; ++j) B[j] = (float)col[j] + et.e(j);
+    fvec<KPL> w;
+#pragma unroll
+    for (int j = 0; j < KPL; ++j) {
+        float a = A[j] * B[j];
+        float q0 = a * y[j];
+        float rr = __builtin_fmaf(-q0, bf[j], a);
+        w[j] = __builtin_fmaf(rr, y[j], q0);
+    }
+    float tot;
+    const int zn = scan_search<KPL>(w, u, K, nvalid, lastLane, tot);
+    bump_regs<KPL>(S, al, lane, zn, +1, et, colg, A, col, n, bf, y);
     return zn;
 }
 
@@ -376,10 +435,13 @@ __device__ __forceinline__ void sample_range(const WaveLds& S, const fvec<KPL>& 
             }
             const int zo = readlane_i(zv, i);
             const float u = readlane_f(uv, i);
-            // MODE_DET: the worker owns the word's cells, so the count
-            // updates write them to n_kw as they go
-            const int zn = token_step<KPL>(S, al, lane, zo, u, p.K, nvalid, lastLane, et, nullptr,
-                                           MODE == MODE_DET ? colp : nullptr);
+            // MODE_DET (one-wavefront workers): the worker owns the word's
+            // cells, so the count updates write them to n_kw as they go
+            int zn;
+            if constexpr (MODE == MODE_DET)
+                zn = token_step_solo<KPL>(S, al, lane, zo, u, p.K, nvalid, lastLane, et, colp);
+            else
+                zn = token_step<KPL>(S, al, lane, zo, u, p.K, nvalid, lastLane, et, nullptr);
             zkeep = (lane == i) ? zn : zkeep;
         }
         if constexpr (MODE == MODE_ATOMIC) {
