@@ -232,70 +232,90 @@ __device__ __forceinline__ int token_step(const WaveLds& S, const fvec<KPL>& al,
     return zn;
 }
 
-// addWordTo<delta> on a one-wavefront worker's state held in registers (the
-// arrays as just loaded from LDS): the owner lane derives the new cell values
-// from its registers, writes them through to LDS and to the owned n_kw cell
-// colg, and keeps its registers current.  The values are bump's.
+// State of a worker that is one wavefront alone (the deterministic
+// schedules), held in registers from token to token: A = float(n_dk) + alpha
+// of the current document, the current word's column, the worker's private
+// n_k, bf = float(n_k) + V*eta (sum_v eta_kv with word priors) and
+// y = RN(1 / bf), each for this lane's KPL topics.  No other wavefront reads
+// them, so a token touches no LDS.  The members refer to five separate
+// vectors of the kernel: one vector each, indexed by a wave-uniform slot,
+// stays in registers (indexed moves), where an aggregate of them would be
+// placed in memory.
+template <int KPL>
+struct SoloState {
+    fvec<KPL>& A;
+    fvec<KPL>& bf;
+    fvec<KPL>& y;
+    ivec<KPL>& col;
+    ivec<KPL>& n;
+};
+
 template <int KPL, class E>
-__device__ __forceinline__ void bump_regs(const WaveLds& S, const fvec<KPL>& al, int lane, int z, int delta,
-                                          const E& et, int32_t* colg, fvec<KPL>& A, ivec<KPL>& col, ivec<KPL>& n,
-                                          fvec<KPL>& bf, fvec<KPL>& y) {
-    const int sl = z & (KPL - 1);
-    if (lane == (z >> kpl_shift<KPL>())) {
-        const int q = kpos<KPL>(lane, sl);
-        const float a_sl = al[sl];
-        const float a = (__builtin_rintf(A[sl] - a_sl) + (float)delta) + a_sl;
-        const int c = col[sl] + delta;
-        const int m = n[sl] + delta;
-        const float b = (float)m + et.s_at(q);
-        const float r = recip_rn(b);
-        S.A[q] = a;
-        S.col[q] = c;
-        S.nk[q] = m;
-        S.bf[q] = b;
-        S.y[q] = r;
-        if (colg) colg[q] = c;
-        A[sl] = a;
-        col[sl] = c;
-        n[sl] = m;
-        bf[sl] = b;
-        y[sl] = r;
+__device__ __forceinline__ void solo_set_nk(SoloState<KPL>& st, const int32_t* __restrict__ src, int lane,
+                                            const E& et) {
+    load_kvec<KPL>(src, lane, st.n);
+#pragma unroll
+    for (int j = 0; j < KPL; ++j) {
+        st.bf[j] = (float)st.n[j] + et.s(j);
+        st.y[j] = recip_rn(st.bf[j]);
     }
 }
 
-// token_step for a worker that is one wavefront alone (its n_k private to it,
-// as in the deterministic schedules): all five arrays are read once, the
-// removal and the addition are applied to them in registers by their owner
-// lanes and written through, so a token waits for one LDS round trip instead
-// of three (bump waits for its cell reads before it writes).  Same operations
-// on the same counts as token_step, hence the same draw.
-template <int KPL, class E>
-__device__ __forceinline__ int token_step_solo(const WaveLds& S, const fvec<KPL>& al, int lane, int zo, float u,
-                                               int K, int nvalid, int lastLane, const E& et, int32_t* colg) {
-    fvec<KPL> A, y, bf, B;
-    ivec<KPL> col, n;
-    lds_load_f<KPL>(S.A, lane, A);
-    lds_load<KPL>(S.col, lane, col);
-    lds_load<KPL>(S.nk, lane, n);
-    lds_load_f<KPL>(S.bf, lane, bf);
-    lds_load_f<KPL>(S.y, lane, y);
-    bump_regs<KPL>(S, al, lane, zo, -1, et, colg, A, col, n, bf, y);
+// A document's counts in and out of a SoloState (set_ndk, get_ndk).
+template <int KPL>
+__device__ __forceinline__ void solo_set_ndk(SoloState<KPL>& st, const fvec<KPL>& al, const ivec<KPL>& nd) {
 #pragma unroll
-    for (int j = 0; j < KPL; ++j) B[j] = (float)col[j] + et.e(j);
+    for (int j = 0; j < KPL; ++j) st.A[j] = (float)nd[j] + al[j];
+}
+
+template <int KPL>
+__device__ __forceinline__ ivec<KPL> solo_get_ndk(const SoloState<KPL>& st, const fvec<KPL>& al) {
+    ivec<KPL> nd;
+#pragma unroll
+    for (int j = 0; j < KPL; ++j) nd[j] = (int)__builtin_rintf(st.A[j] - al[j]);
+    return nd;
+}
+
+// addWordTo<delta> (LDAModel.hpp:287-299) on a SoloState: the owner lane
+// updates its slot (bump's values) and stores the new cell to the word's
+// n_kw row colg, whose cells the worker owns for the round.
+template <int KPL, class E>
+__device__ __forceinline__ void bump_solo(SoloState<KPL>& st, const fvec<KPL>& al, int lane, int z, int delta,
+                                          const E& et, int32_t* colg) {
+    const int sl = z & (KPL - 1);
+    if (lane == (z >> kpl_shift<KPL>())) {
+        const float a_sl = al[sl];
+        st.A[sl] = (__builtin_rintf(st.A[sl] - a_sl) + (float)delta) + a_sl;
+        const int c = st.col[sl] + delta;
+        st.col[sl] = c;
+        const int m = st.n[sl] + delta;
+        st.n[sl] = m;
+        const float b = (float)m + et.s(sl);
+        st.bf[sl] = b;
+        st.y[sl] = recip_rn(b);
+        colg[kpos<KPL>(lane, sl)] = c;
+    }
+}
+
+// token_step on a SoloState: the same operations on the same counts, hence
+// the same draw.
+template <int KPL, class E>
+__device__ __forceinline__ int token_step_solo(SoloState<KPL>& st, const fvec<KPL>& al, int lane, int zo, float u,
+                                               int K, int nvalid, int lastLane, const E& et, int32_t* colg) {
+    bump_solo<KPL>(st, al, lane, zo, -1, et, colg);
     fvec<KPL> w;
 #pragma unroll
     for (int j = 0; j < KPL; ++j) {
-        float a = A[j] * B[j];
-        float q0 = a * y[j];
-        float rr = __builtin_fmaf(-q0, bf[j], a);
-        w[j] = __builtin_fmaf(rr, y[j], q0);
+        float a = st.A[j] * ((float)st.col[j] + et.e(j));
+        float q0 = a * st.y[j];
+        float rr = __builtin_fmaf(-q0, st.bf[j], a);
+        w[j] = __builtin_fmaf(rr, st.y[j], q0);
     }
     float tot;
     const int zn = scan_search<KPL>(w, u, K, nvalid, lastLane, tot);
-    bump_regs<KPL>(S, al, lane, zn, +1, et, colg, A, col, n, bf, y);
+    bump_solo<KPL>(st, al, lane, zn, +1, et, colg);
     return zn;
 }
-
 
 // n_kw[q, v] += delta with an exact atomic.  A u16 cell is updated by a 32-bit
 // atomic add on the word that holds it, the delta shifted into its half:
@@ -370,30 +390,24 @@ __device__ __forceinline__ int col_group_of(const SweepParams& p) {
     return (g >= 1 && g <= 64 && (g & (g - 1)) == 0) ? g : 64;
 }
 
-// sampleDocument over the token range [b, e) (LDAModel.hpp:326-354).  Tokens
-// come in blocks of 64, token t of a block on lane t (its word, z and u); the
-// column of the next distinct word is fetched one token ahead and, for a
-// narrow word, unpacked only when it is stored to LDS.  MODE_ATOMIC sends the
-// n_kw changes of each group of col_group tokens to the shared n_kw at the
-// group's end as whole-wavefront atomics, lane t those of token t (its word,
-// old and new topic): issued per token from single lanes they would hold every
-// later load wait behind an atomic's round trip (the vector memory counter
-// retires in order).  Other workers see them up to one group later, as they
-// already sample from column copies; this wavefront never rereads a column
-// with changes pending (a document's words come in one run each, and its last
-// group is sent before the next document).
-template <int KPL, int MODE, class E, bool NARROW = false>
+// sampleDocument over the token range [b, e) (LDAModel.hpp:326-354) for a
+// device-wide worker.  Tokens come in blocks of 64, token t of a block on
+// lane t (its word, z and u); the column of the next distinct word is fetched
+// one token ahead and, for a narrow word, unpacked only when it is stored to
+// LDS.  The n_kw changes of each group of col_group tokens go to the shared
+// n_kw at the group's end as whole-wavefront atomics, lane t those of token t
+// (its word, old and new topic): issued per token from single lanes they
+// would hold every later load wait behind an atomic's round trip (the vector
+// memory counter retires in order).  Other workers see them up to one group
+// later, as they already sample from column copies; this wavefront never
+// rereads a column with changes pending (a document's words come in one run
+// each, and its last group is sent before the next document).
+template <int KPL, class E, bool NARROW = false>
 __device__ __forceinline__ void sample_range(const WaveLds& S, const fvec<KPL>& al, const SweepParams& p, uint32_t sweep,
-                                             int lane, uint64_t b, uint64_t e, int nvalid, int lastLane, E& et,
-                                             int* vcur_io = nullptr) {
-    static_assert(!NARROW || MODE == MODE_ATOMIC, "narrow columns are for the shared device-wide n_kw");
+                                             int lane, uint64_t b, uint64_t e, int nvalid, int lastLane, E& et) {
     constexpr int SH = kpl_shift<KPL>();
-    const int TF = MODE == MODE_ATOMIC ? col_group_of(p) : 64;  // divides the 64-token block
-    // vcur_io (MODE_DET): the word whose column S.col holds from the worker's
-    // previous range; its cells are this worker's alone for the round and
-    // S.col carries every update, so a range starting with it reads no column
-    int vcur = vcur_io ? *vcur_io : -1, vnext = -1;
-    int32_t* colp = vcur >= 0 ? p.nkw + (int64_t)vcur * p.Kp : nullptr;
+    const int TF = col_group_of(p);  // divides the 64-token block
+    int vcur = -1, vnext = -1;
     ivec<KPL> ncol;
     // the current block of up to 64 tokens, token t of it on lane t
     int nb = 0, wv = -1, zv = 0xFFFF, zkeep = 0xFFFF;
@@ -420,7 +434,6 @@ __device__ __forceinline__ void sample_range(const WaveLds& S, const fvec<KPL>& 
             const int v = readlane_i(wv, i);
             if ((unsigned)v >= (unsigned)p.V) continue;
             if (v != vcur) {
-                colp = p.nkw + (int64_t)v * p.Kp;
                 if (v != vnext) fetch_col<KPL, NARROW>(p, v, lane, ncol);
                 store_col<KPL, NARROW>(S, p, v, lane, ncol);
                 et.word(p, v, lane);
@@ -435,24 +448,68 @@ __device__ __forceinline__ void sample_range(const WaveLds& S, const fvec<KPL>& 
             }
             const int zo = readlane_i(zv, i);
             const float u = readlane_f(uv, i);
-            // MODE_DET (one-wavefront workers): the worker owns the word's
-            // cells, so the count updates write them to n_kw as they go
-            int zn;
-            if constexpr (MODE == MODE_DET)
-                zn = token_step_solo<KPL>(S, al, lane, zo, u, p.K, nvalid, lastLane, et, colp);
-            else
-                zn = token_step<KPL>(S, al, lane, zo, u, p.K, nvalid, lastLane, et, nullptr);
+            const int zn = token_step<KPL>(S, al, lane, zo, u, p.K, nvalid, lastLane, et, nullptr);
             zkeep = (lane == i) ? zn : zkeep;
         }
-        if constexpr (MODE == MODE_ATOMIC) {
-            if (lane >= off && lane < i1 && zkeep != zv && (unsigned)wv < (unsigned)p.V) {
-                col_atomic<NARROW>(p, wv, kpos<KPL>(zv >> SH, zv & (KPL - 1)), -1);
-                col_atomic<NARROW>(p, wv, kpos<KPL>(zkeep >> SH, zkeep & (KPL - 1)), 1);
-            }
+        if (lane >= off && lane < i1 && zkeep != zv && (unsigned)wv < (unsigned)p.V) {
+            col_atomic<NARROW>(p, wv, kpos<KPL>(zv >> SH, zv & (KPL - 1)), -1);
+            col_atomic<NARROW>(p, wv, kpos<KPL>(zkeep >> SH, zkeep & (KPL - 1)), 1);
         }
         if (i1 == nb && lane < nb) p.z[g] = (uint16_t)zkeep;
     }
-    if (vcur_io) *vcur_io = vcur;
+}
+
+// sampleDocument over the token range [b, e) for a one-wavefront worker of
+// the deterministic schedules, its state in registers (SoloState).  The
+// worker owns the cells of the words it samples for the round, so the count
+// updates store them to n_kw as they go, and the current word's column
+// carries every update: vcur is the word whose column st.col holds from the
+// worker's previous range, and a range starting with it reads no column.
+// Other columns are fetched one token ahead; the caller orders the worker's
+// earlier n_kw stores before a range's reads with a fence.
+template <int KPL, class E>
+__device__ __forceinline__ void sample_range_solo(SoloState<KPL>& st, const fvec<KPL>& al, const SweepParams& p,
+                                                  uint32_t sweep, int lane, uint64_t b, uint64_t e, int nvalid,
+                                                  int lastLane, E& et, int& vcur) {
+    int vnext = -1;
+    int32_t* colp = vcur >= 0 ? p.nkw + (int64_t)vcur * p.Kp : nullptr;
+    ivec<KPL> ncol;
+    for (uint64_t t0 = b; t0 < e; t0 += 64) {
+        const int nb = (int)((e - t0) < 64 ? (e - t0) : 64);
+        const uint64_t g = t0 + (uint64_t)lane;
+        int wv = -1, zv = 0xFFFF;
+        float uv = 0.0f;
+        if (lane < nb) {
+            wv = (int)p.words[g];
+            zv = p.z[g];
+            uv = uniform24(philox_x0(p.seed_lo, p.seed_hi, (uint32_t)g, (uint32_t)(g >> 32), sweep, TAG_SWEEP));
+        }
+        int zkeep = zv;
+        vnext = -1;
+        for (int i = 0; i < nb; ++i) {
+            const int v = readlane_i(wv, i);
+            if ((unsigned)v >= (unsigned)p.V) continue;
+            if (v != vcur) {
+                colp = p.nkw + (int64_t)v * p.Kp;
+                if (v != vnext) load_kvec<KPL>(colp, lane, ncol);
+                st.col = ncol;
+                et.word(p, v, lane);
+                vcur = v;
+            }
+            if (i + 1 < nb) {
+                const int v2 = readlane_i(wv, i + 1);
+                if (v2 != v && (unsigned)v2 < (unsigned)p.V) {
+                    load_kvec<KPL>(p.nkw + (int64_t)v2 * p.Kp, lane, ncol);
+                    vnext = v2;
+                }
+            }
+            const int zo = readlane_i(zv, i);
+            const float u = readlane_f(uv, i);
+            const int zn = token_step_solo<KPL>(st, al, lane, zo, u, p.K, nvalid, lastLane, et, colp);
+            zkeep = (lane == i) ? zn : zkeep;
+        }
+        if (lane < nb) p.z[g] = (uint16_t)zkeep;
+    }
 }
 
 // Device-wide sweep: every resident wavefront is a worker with a private
@@ -485,7 +542,7 @@ __global__ void __launch_bounds__(64 * kSweepWaves) sweep_device_kernel(SweepPar
         ivec<KPL> nd;
         load_kvec<KPL>(ndk_row, lane, nd);
         set_ndk<KPL>(S, al, nd, lane);
-        sample_range<KPL, MODE_ATOMIC, decltype(et), NARROW>(S, al, p, p.sweep, lane, b, e, nvalid, lastLane, et);
+        sample_range<KPL, decltype(et), NARROW>(S, al, p, p.sweep, lane, b, e, nvalid, lastLane, et);
         nd = get_ndk<KPL>(S, al, lane);
         store_kvec<KPL>(ndk_row, lane, nd);
     }
@@ -510,14 +567,16 @@ template <int KPL, bool ASYM>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) sweep_det_kernel(SweepParams p) {
     const int lane = lane_id();
     const int w = blockIdx.x;
-    const WaveLds S = wave_lds<KPL>();
     int nvalid, lastLane;
     lane_limits<KPL>(p.K, lane, nvalid, lastLane);
     fvec<KPL> al;
     load_kvec_f<KPL>(p.alpha, lane, al);
     auto et = make_eta<KPL, ASYM>(p, lane);
     int32_t* nkp = p.nk_out + (int64_t)w * p.Kp;
-    set_nk<KPL>(S, nkp, lane, et);
+    fvec<KPL> sA, sbf, sy;
+    ivec<KPL> scol, sn;
+    SoloState<KPL> st{sA, sbf, sy, scol, sn};
+    solo_set_nk<KPL>(st, nkp, lane, et);
     const int W = p.W;
     const uint32_t sweep = p.sweep_dev ? (uint32_t)uniform_i((int)*p.sweep_dev) : p.sweep;
     const int didx = (p.round + w) % W;
@@ -535,8 +594,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2)))
     }
     // Each cell's bounds and n_dk row are loaded while the previous cell is
     // sampled (a worker's documents in a round are distinct and no other
-    // worker touches them), and the current word's column stays in LDS from
-    // one cell to the next (the chunk's cells are this worker's alone).
+    // worker touches them), and the current word's column stays in registers
+    // from one cell to the next (the chunk's cells are this worker's alone).
     auto doc_of = [&](uint64_t i) { return (int64_t)(((i + (uint64_t)sseed) * P) % N) * W + didx; };
     int vcur = -1;
     int64_t dn = 0;
@@ -561,15 +620,11 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2)))
             load_kvec<KPL>(p.ndk + dn * p.Kp, lane, ndn);  // (unused when the cell is empty)
         }
         if (b == e) continue;
-        int32_t* ndk_row = p.ndk + d * p.Kp;
-        set_ndk<KPL>(S, al, nd0, lane);
-        sample_range<KPL, MODE_DET>(S, al, p, sweep, lane, b, e, nvalid, lastLane, et, &vcur);
-        const ivec<KPL> nd = get_ndk<KPL>(S, al, lane);
-        store_kvec<KPL>(ndk_row, lane, nd);
+        solo_set_ndk<KPL>(st, al, nd0);
+        sample_range_solo<KPL>(st, al, p, sweep, lane, b, e, nvalid, lastLane, et, vcur);
+        store_kvec<KPL>(p.ndk + d * p.Kp, lane, solo_get_ndk<KPL>(st, al));
     }
-    ivec<KPL> mine;
-    lds_load<KPL>(S.nk, lane, mine);
-    store_kvec<KPL>(nkp, lane, mine);
+    store_kvec<KPL>(nkp, lane, st.n);
 }
 
 // copy_merge schedule (LDAModel.hpp:401-423): wavefront w is worker w with a
@@ -581,14 +636,16 @@ template <int KPL, bool ASYM>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) sweep_copy_kernel(SweepParams p) {
     const int lane = lane_id();
     const int w = blockIdx.x;
-    const WaveLds S = wave_lds<KPL>();
     int nvalid, lastLane;
     lane_limits<KPL>(p.K, lane, nvalid, lastLane);
     fvec<KPL> al;
     load_kvec_f<KPL>(p.alpha, lane, al);
     auto et = make_eta<KPL, ASYM>(p, lane);
     int32_t* nkp = p.nk_out + (int64_t)w * p.Kp;
-    set_nk<KPL>(S, nkp, lane, et);
+    fvec<KPL> sA, sbf, sy;
+    ivec<KPL> scol, sn;
+    SoloState<KPL> st{sA, sbf, sy, scol, sn};
+    solo_set_nk<KPL>(st, nkp, lane, et);
     SweepParams q = p;
     q.nkw = p.nkw + (int64_t)w * p.V * p.Kp;
     const int W = p.W;
@@ -615,16 +672,13 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2)))
             int32_t* ndk_row = p.ndk + d * p.Kp;
             ivec<KPL> nd;
             load_kvec<KPL>(ndk_row, lane, nd);
-            set_ndk<KPL>(S, al, nd, lane);
-            // the worker's private n_kw copy: its column in LDS stays current
-            sample_range<KPL, MODE_DET>(S, al, q, q.sweep, lane, b, e, nvalid, lastLane, et, &vcur);
-            nd = get_ndk<KPL>(S, al, lane);
-            store_kvec<KPL>(ndk_row, lane, nd);
+            solo_set_ndk<KPL>(st, al, nd);
+            // the worker's private n_kw copy: its column in registers stays current
+            sample_range_solo<KPL>(st, al, q, q.sweep, lane, b, e, nvalid, lastLane, et, vcur);
+            store_kvec<KPL>(ndk_row, lane, solo_get_ndk<KPL>(st, al));
         }
     }
-    ivec<KPL> mine;
-    lds_load<KPL>(S.nk, lane, mine);
-    store_kvec<KPL>(nkp, lane, mine);
+    store_kvec<KPL>(nkp, lane, st.n);
 }
 
 // mergeState<copy_merge> (LDAModel.hpp:555-570): n_kw = old + sum_w (local_w - old).
