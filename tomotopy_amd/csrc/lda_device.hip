@@ -474,6 +474,9 @@ __device__ __forceinline__ void sample_range_solo(SoloState<KPL>& st, const fvec
     int vnext = -1;
     int32_t* colp = vcur >= 0 ? p.nkw + (int64_t)vcur * p.Kp : nullptr;
     ivec<KPL> ncol;
+    // the range bounds are wave-uniform: keep the block loop on scalars
+    b = ((uint64_t)(uint32_t)uniform_i((int)(b >> 32)) << 32) | (uint32_t)uniform_i((int)b);
+    e = ((uint64_t)(uint32_t)uniform_i((int)(e >> 32)) << 32) | (uint32_t)uniform_i((int)e);
     for (uint64_t t0 = b; t0 < e; t0 += 64) {
         const int nb = (int)((e - t0) < 64 ? (e - t0) : 64);
         const uint64_t g = t0 + (uint64_t)lane;
@@ -484,11 +487,16 @@ __device__ __forceinline__ void sample_range_solo(SoloState<KPL>& st, const fvec
             zv = p.z[g];
             uv = uniform24(philox_x0(p.seed_lo, p.seed_hi, (uint32_t)g, (uint32_t)(g >> 32), sweep, TAG_SWEEP));
         }
+        // per block, as bit masks over its tokens: which are sampled (word
+        // in range) and after which one the next token's new word is fetched
+        const int wn = __builtin_amdgcn_update_dpp(0, wv, 0x130, 0xF, 0xF, true);  // token t+1's word on lane t
+        const uint64_t live = __ballot(lane < nb && (unsigned)wv < (unsigned)p.V);
+        const uint64_t pref = __ballot(lane + 1 < nb && wn != wv && (unsigned)wn < (unsigned)p.V);
         int zkeep = zv;
         vnext = -1;
         for (int i = 0; i < nb; ++i) {
+            if (!((live >> i) & 1)) continue;
             const int v = readlane_i(wv, i);
-            if ((unsigned)v >= (unsigned)p.V) continue;
             if (v != vcur) {
                 colp = p.nkw + (int64_t)v * p.Kp;
                 if (v != vnext) load_kvec<KPL>(colp, lane, ncol);
@@ -496,12 +504,10 @@ __device__ __forceinline__ void sample_range_solo(SoloState<KPL>& st, const fvec
                 et.word(p, v, lane);
                 vcur = v;
             }
-            if (i + 1 < nb) {
+            if ((pref >> i) & 1) {
                 const int v2 = readlane_i(wv, i + 1);
-                if (v2 != v && (unsigned)v2 < (unsigned)p.V) {
-                    load_kvec<KPL>(p.nkw + (int64_t)v2 * p.Kp, lane, ncol);
-                    vnext = v2;
-                }
+                load_kvec<KPL>(p.nkw + (int64_t)v2 * p.Kp, lane, ncol);
+                vnext = v2;
             }
             const int zo = readlane_i(zv, i);
             const float u = readlane_f(uv, i);
