@@ -390,6 +390,22 @@ __device__ __forceinline__ int col_group_of(const SweepParams& p) {
     return (g >= 1 && g <= 64 && (g & (g - 1)) == 0) ? g : 64;
 }
 
+// A wave-uniform 64-bit value held in scalar registers (range bounds), so
+// that loops over it run on the scalar unit.
+__device__ __forceinline__ uint64_t uniform_u64(uint64_t x) {
+    return ((uint64_t)(uint32_t)uniform_i((int)(x >> 32)) << 32) | (uint32_t)uniform_i((int)x);
+}
+
+// Bit masks over a block of nb tokens (token t's word wv on lane t): `live`,
+// the tokens that are sampled (word in range), and `pref`, the tokens after
+// which the next token's word differs and is in range (its column is fetched
+// one token ahead).
+__device__ __forceinline__ void block_masks(int wv, int nb, int V, int lane, uint64_t& live, uint64_t& pref) {
+    const int wn = __builtin_amdgcn_update_dpp(0, wv, 0x130, 0xF, 0xF, true);  // wave_shl:1, token t+1's word
+    live = __ballot(lane < nb && (unsigned)wv < (unsigned)V);
+    pref = __ballot(lane + 1 < nb && wn != wv && (unsigned)wn < (unsigned)V);
+}
+
 // sampleDocument over the token range [b, e) (LDAModel.hpp:326-354) for a
 // device-wide worker.  Tokens come in blocks of 64, token t of a block on
 // lane t (its word, z and u); the column of the next distinct word is fetched
@@ -409,10 +425,12 @@ __device__ __forceinline__ void sample_range(const WaveLds& S, const fvec<KPL>& 
     const int TF = col_group_of(p);  // divides the 64-token block
     int vcur = -1, vnext = -1;
     ivec<KPL> ncol;
-    // the current block of up to 64 tokens, token t of it on lane t
+    b = uniform_u64(b);
+    e = uniform_u64(e);
+    // the current block of up to 64 tokens, token t of it on lane t, and its masks
     int nb = 0, wv = -1, zv = 0xFFFF, zkeep = 0xFFFF;
     float uv = 0.0f;
-    uint64_t g = 0;
+    uint64_t g = 0, live = 0, pref = 0;
     for (uint64_t t0 = b; t0 < e; t0 += TF) {
         const int off = (int)((t0 - b) & 63);
         if (off == 0) {
@@ -426,25 +444,24 @@ __device__ __forceinline__ void sample_range(const WaveLds& S, const fvec<KPL>& 
                 zv = p.z[g];
                 uv = uniform24(philox_x0(p.seed_lo, p.seed_hi, (uint32_t)g, (uint32_t)(g >> 32), sweep, TAG_SWEEP));
             }
+            block_masks(wv, nb, p.V, lane, live, pref);
             zkeep = zv;
             vnext = -1;
         }
         const int i1 = off + TF < nb ? off + TF : nb;
         for (int i = off; i < i1; ++i) {
+            if (!((live >> i) & 1)) continue;
             const int v = readlane_i(wv, i);
-            if ((unsigned)v >= (unsigned)p.V) continue;
             if (v != vcur) {
                 if (v != vnext) fetch_col<KPL, NARROW>(p, v, lane, ncol);
                 store_col<KPL, NARROW>(S, p, v, lane, ncol);
                 et.word(p, v, lane);
                 vcur = v;
             }
-            if (i + 1 < nb) {
+            if ((pref >> i) & 1) {
                 const int v2 = readlane_i(wv, i + 1);
-                if (v2 != v && (unsigned)v2 < (unsigned)p.V) {
-                    fetch_col<KPL, NARROW>(p, v2, lane, ncol);
-                    vnext = v2;
-                }
+                fetch_col<KPL, NARROW>(p, v2, lane, ncol);
+                vnext = v2;
             }
             const int zo = readlane_i(zv, i);
             const float u = readlane_f(uv, i);
@@ -474,9 +491,8 @@ __device__ __forceinline__ void sample_range_solo(SoloState<KPL>& st, const fvec
     int vnext = -1;
     int32_t* colp = vcur >= 0 ? p.nkw + (int64_t)vcur * p.Kp : nullptr;
     ivec<KPL> ncol;
-    // the range bounds are wave-uniform: keep the block loop on scalars
-    b = ((uint64_t)(uint32_t)uniform_i((int)(b >> 32)) << 32) | (uint32_t)uniform_i((int)b);
-    e = ((uint64_t)(uint32_t)uniform_i((int)(e >> 32)) << 32) | (uint32_t)uniform_i((int)e);
+    b = uniform_u64(b);
+    e = uniform_u64(e);
     for (uint64_t t0 = b; t0 < e; t0 += 64) {
         const int nb = (int)((e - t0) < 64 ? (e - t0) : 64);
         const uint64_t g = t0 + (uint64_t)lane;
@@ -487,11 +503,8 @@ __device__ __forceinline__ void sample_range_solo(SoloState<KPL>& st, const fvec
             zv = p.z[g];
             uv = uniform24(philox_x0(p.seed_lo, p.seed_hi, (uint32_t)g, (uint32_t)(g >> 32), sweep, TAG_SWEEP));
         }
-        // per block, as bit masks over its tokens: which are sampled (word
-        // in range) and after which one the next token's new word is fetched
-        const int wn = __builtin_amdgcn_update_dpp(0, wv, 0x130, 0xF, 0xF, true);  // token t+1's word on lane t
-        const uint64_t live = __ballot(lane < nb && (unsigned)wv < (unsigned)p.V);
-        const uint64_t pref = __ballot(lane + 1 < nb && wn != wv && (unsigned)wn < (unsigned)p.V);
+        uint64_t live, pref;
+        block_masks(wv, nb, p.V, lane, live, pref);
         int zkeep = zv;
         vnext = -1;
         for (int i = 0; i < nb; ++i) {
@@ -795,10 +808,12 @@ __device__ __forceinline__ void sample_range_w(const WaveLdsW& S, const fvec<KPL
     int vcur = -1, vnext = -1;
     float* colp = nullptr;
     fvec<KPL> ncol;
-    // the current block of up to 64 tokens, token t of it on lane t
+    b = uniform_u64(b);
+    e = uniform_u64(e);
+    // the current block of up to 64 tokens, token t of it on lane t, and its masks
     int nb = 0, wv = -1, zv = 0xFFFF, zkeep = 0xFFFF;
     float uv = 0.0f, tw = 0.0f;
-    uint64_t g = 0;
+    uint64_t g = 0, live = 0, pref = 0;
     for (uint64_t t0 = b; t0 < e; t0 += TF) {
         const int off = (int)((t0 - b) & 63);
         if (off == 0) {
@@ -814,13 +829,14 @@ __device__ __forceinline__ void sample_range_w(const WaveLdsW& S, const fvec<KPL
                 tw = p.wt[g];
                 uv = uniform24(philox_x0(p.seed_lo, p.seed_hi, (uint32_t)g, (uint32_t)(g >> 32), sweep, TAG_SWEEP));
             }
+            block_masks(wv, nb, p.V, lane, live, pref);
             zkeep = zv;
             vnext = -1;
         }
         const int i1 = off + TF < nb ? off + TF : nb;
         for (int i = off; i < i1; ++i) {
+            if (!((live >> i) & 1)) continue;
             const int v = readlane_i(wv, i);
-            if ((unsigned)v >= (unsigned)p.V) continue;
             if (v != vcur) {
                 colp = nkwf + (int64_t)v * p.Kp;
                 if (v != vnext) load_kvec_f<KPL>(colp, lane, ncol);
@@ -828,12 +844,10 @@ __device__ __forceinline__ void sample_range_w(const WaveLdsW& S, const fvec<KPL
                 et.word(p, v, lane);
                 vcur = v;
             }
-            if (i + 1 < nb) {
+            if ((pref >> i) & 1) {
                 const int v2 = readlane_i(wv, i + 1);
-                if (v2 != v && (unsigned)v2 < (unsigned)p.V) {
-                    load_kvec_f<KPL>(nkwf + (int64_t)v2 * p.Kp, lane, ncol);
-                    vnext = v2;
-                }
+                load_kvec_f<KPL>(nkwf + (int64_t)v2 * p.Kp, lane, ncol);
+                vnext = v2;
             }
             const int zo = readlane_i(zv, i);
             const float w = readlane_f(tw, i);
