@@ -743,8 +743,7 @@ __device__ __forceinline__ WaveLdsW wave_lds_w() {
 // the workgroup's wavefronts share n_k (LDS atomics; y then follows a value
 // n_k had a moment earlier, which the Markstein step absorbs).
 template <int KPL, bool SHARED, class E>
-__device__ __forceinline__ void bump_w(const WaveLdsW& S, int lane, int z, float w, bool dec, const E& et,
-                                       float* colg = nullptr) {
+__device__ __forceinline__ void bump_w(const WaveLdsW& S, int lane, int z, float w, bool dec, const E& et) {
     const int sl = z & (KPL - 1);
     if (lane == (z >> kpl_shift<KPL>())) {
         const int q = kpos<KPL>(lane, sl);
@@ -763,15 +762,14 @@ __device__ __forceinline__ void bump_w(const WaveLdsW& S, int lane, int z, float
         const float c1 = dec ? sub_clamp(c0, w) : c0 + w;
         S.nd[q] = dec ? sub_clamp(d0, w) : d0 + w;
         S.col[q] = c1;
-        if (colg) colg[q] = c1;
         S.y[q] = recip_rn(n + et.s_at(q));
     }
 }
 
 template <int KPL, bool SHARED, class E>
 __device__ __forceinline__ int token_step_w(const WaveLdsW& S, const fvec<KPL>& al, int lane, int zo, float wt, float u,
-                                            int K, int nvalid, int lastLane, const E& et, float* colg = nullptr) {
-    bump_w<KPL, SHARED>(S, lane, zo, wt, true, et, colg);
+                                            int K, int nvalid, int lastLane, const E& et) {
+    bump_w<KPL, SHARED>(S, lane, zo, wt, true, et);
     fvec<KPL> nd, col, nk, y;
     lds_load_f<KPL>(S.nd, lane, nd);
     lds_load_f<KPL>(S.col, lane, col);
@@ -788,25 +786,125 @@ __device__ __forceinline__ int token_step_w(const WaveLdsW& S, const fvec<KPL>& 
     }
     float tot;
     const int zn = scan_search<KPL>(w, u, K, nvalid, lastLane, tot);
-    bump_w<KPL, SHARED>(S, lane, zn, wt, false, et, colg);
+    bump_w<KPL, SHARED>(S, lane, zn, wt, false, et);
     return zn;
 }
 
-// sampleDocument over [b, e) with token weights.  The wavefront's own state
-// (n_dk, the column, the workgroup's n_k) follows tomoto's update rule
-// exactly.  MODE_DET: the column cells are stored to n_kw (exclusive owner),
-// MODE_ATOMIC: the token's net change goes to the shared n_kw as float
-// atomics.
-template <int KPL, int MODE, class E>
+// A one-wavefront worker's weighted state in registers (see SoloState):
+// n_dk, the current word's column and the worker's n_k as floats, and
+// y = RN(1 / (n_k + V*eta)), each for this lane's KPL topics.
+template <int KPL>
+struct SoloStateW {
+    fvec<KPL>& nd;
+    fvec<KPL>& col;
+    fvec<KPL>& nk;
+    fvec<KPL>& y;
+};
+
+// addWordTo<+-w> (bump_w's values) on a SoloStateW; the owner lane stores
+// the new cell to the word's n_kw row colg, whose cells the worker owns.
+template <int KPL, class E>
+__device__ __forceinline__ void bump_solo_w(const SoloStateW<KPL>& st, int lane, int z, float w, bool dec,
+                                            const E& et, float* colg) {
+    const int sl = z & (KPL - 1);
+    if (lane == (z >> kpl_shift<KPL>())) {
+        const float n0 = st.nk[sl], c0 = st.col[sl], d0 = st.nd[sl];
+        const float n = dec ? sub_clamp(n0, w) : n0 + w;
+        const float c1 = dec ? sub_clamp(c0, w) : c0 + w;
+        st.nk[sl] = n;
+        st.col[sl] = c1;
+        st.nd[sl] = dec ? sub_clamp(d0, w) : d0 + w;
+        st.y[sl] = recip_rn(n + et.s(sl));
+        colg[kpos<KPL>(lane, sl)] = c1;
+    }
+}
+
+// token_step_w on a SoloStateW: the same operations, hence the same draw.
+template <int KPL, class E>
+__device__ __forceinline__ int token_step_solo_w(const SoloStateW<KPL>& st, const fvec<KPL>& al, int lane, int zo,
+                                                 float wt, float u, int K, int nvalid, int lastLane, const E& et,
+                                                 float* colg) {
+    bump_solo_w<KPL>(st, lane, zo, wt, true, et, colg);
+    fvec<KPL> w;
+#pragma unroll
+    for (int j = 0; j < KPL; ++j) {
+        const float a = (st.nd[j] + al[j]) * (st.col[j] + et.e(j));
+        const float bf = st.nk[j] + et.s(j);
+        const float q0 = a * st.y[j];
+        const float rr = __builtin_fmaf(-q0, bf, a);
+        w[j] = __builtin_fmaf(rr, st.y[j], q0);
+    }
+    float tot;
+    const int zn = scan_search<KPL>(w, u, K, nvalid, lastLane, tot);
+    bump_solo_w<KPL>(st, lane, zn, wt, false, et, colg);
+    return zn;
+}
+
+// sample_range_solo with token weights: sampleDocument over [b, e) for a
+// one-wavefront worker of the deterministic weighted schedules (the column
+// of vcur carries over from the worker's previous range).
+template <int KPL, class E>
+__device__ __forceinline__ void sample_range_solo_w(const SoloStateW<KPL>& st, const fvec<KPL>& al,
+                                                    const SweepParams& p, uint32_t sweep, int lane, uint64_t b,
+                                                    uint64_t e, int nvalid, int lastLane, E& et, int& vcur) {
+    float* nkwf = reinterpret_cast<float*>(p.nkw);
+    int vnext = -1;
+    float* colp = vcur >= 0 ? nkwf + (int64_t)vcur * p.Kp : nullptr;
+    fvec<KPL> ncol;
+    b = uniform_u64(b);
+    e = uniform_u64(e);
+    for (uint64_t t0 = b; t0 < e; t0 += 64) {
+        const int nb = (int)((e - t0) < 64 ? (e - t0) : 64);
+        const uint64_t g = t0 + (uint64_t)lane;
+        int wv = -1, zv = 0xFFFF;
+        float uv = 0.0f, tw = 0.0f;
+        if (lane < nb) {
+            wv = (int)p.words[g];
+            zv = p.z[g];
+            tw = p.wt[g];
+            uv = uniform24(philox_x0(p.seed_lo, p.seed_hi, (uint32_t)g, (uint32_t)(g >> 32), sweep, TAG_SWEEP));
+        }
+        uint64_t live, pref;
+        block_masks(wv, nb, p.V, lane, live, pref);
+        int zkeep = zv;
+        vnext = -1;
+        for (int i = 0; i < nb; ++i) {
+            if (!((live >> i) & 1)) continue;
+            const int v = readlane_i(wv, i);
+            if (v != vcur) {
+                colp = nkwf + (int64_t)v * p.Kp;
+                if (v != vnext) load_kvec_f<KPL>(colp, lane, ncol);
+                st.col = ncol;
+                et.word(p, v, lane);
+                vcur = v;
+            }
+            if ((pref >> i) & 1) {
+                const int v2 = readlane_i(wv, i + 1);
+                load_kvec_f<KPL>(nkwf + (int64_t)v2 * p.Kp, lane, ncol);
+                vnext = v2;
+            }
+            const int zo = readlane_i(zv, i);
+            const float w = readlane_f(tw, i);
+            const float u = readlane_f(uv, i);
+            const int zn = token_step_solo_w<KPL>(st, al, lane, zo, w, u, p.K, nvalid, lastLane, et, colp);
+            zkeep = (lane == i) ? zn : zkeep;
+        }
+        if (lane < nb) p.z[g] = (uint16_t)zkeep;
+    }
+}
+
+// sampleDocument over [b, e) with token weights for a device-wide worker.
+// The wavefront's own state (n_dk, the column, the workgroup's n_k) follows
+// tomoto's update rule exactly; the tokens' net changes go to the shared
+// n_kw as float atomics, per group of tokens as in sample_range.
+template <int KPL, class E>
 __device__ __forceinline__ void sample_range_w(const WaveLdsW& S, const fvec<KPL>& al, const SweepParams& p,
                                                uint32_t sweep, int lane, uint64_t b, uint64_t e, int nvalid, int lastLane,
                                                E& et) {
     constexpr int SH = kpl_shift<KPL>();
-    constexpr bool SHARED = MODE == MODE_ATOMIC;
-    const int TF = MODE == MODE_ATOMIC ? col_group_of(p) : 64;
+    const int TF = col_group_of(p);
     float* nkwf = reinterpret_cast<float*>(p.nkw);
     int vcur = -1, vnext = -1;
-    float* colp = nullptr;
     fvec<KPL> ncol;
     b = uniform_u64(b);
     e = uniform_u64(e);
@@ -838,8 +936,7 @@ __device__ __forceinline__ void sample_range_w(const WaveLdsW& S, const fvec<KPL
             if (!((live >> i) & 1)) continue;
             const int v = readlane_i(wv, i);
             if (v != vcur) {
-                colp = nkwf + (int64_t)v * p.Kp;
-                if (v != vnext) load_kvec_f<KPL>(colp, lane, ncol);
+                if (v != vnext) load_kvec_f<KPL>(nkwf + (int64_t)v * p.Kp, lane, ncol);
                 lds_store_f<KPL>(S.col, lane, ncol);
                 et.word(p, v, lane);
                 vcur = v;
@@ -852,22 +949,17 @@ __device__ __forceinline__ void sample_range_w(const WaveLdsW& S, const fvec<KPL
             const int zo = readlane_i(zv, i);
             const float w = readlane_f(tw, i);
             const float u = readlane_f(uv, i);
-            // MODE_DET: the worker owns the word's cells, so the count
-            // updates write them to n_kw as they go
-            const int zn = token_step_w<KPL, SHARED>(S, al, lane, zo, w, u, p.K, nvalid, lastLane, et,
-                                                     MODE == MODE_DET ? colp : nullptr);
+            const int zn = token_step_w<KPL, true>(S, al, lane, zo, w, u, p.K, nvalid, lastLane, et);
             zkeep = (lane == i) ? zn : zkeep;
         }
-        if constexpr (MODE == MODE_ATOMIC) {
-            // the group's changes as whole-wavefront float atomics (see
-            // sample_range); a removal can leave a rounding-level negative
-            // cell, which the merge's clamp max(n_kw, 0) (LDAModel.hpp:574-577)
-            // removes after the sweep.  An unchanged topic adds nothing.
-            if (lane >= off && lane < i1 && zkeep != zv && (unsigned)wv < (unsigned)p.V) {
-                float* c = nkwf + (int64_t)wv * p.Kp;
-                unsafeAtomicAdd(c + kpos<KPL>(zv >> SH, zv & (KPL - 1)), -tw);
-                unsafeAtomicAdd(c + kpos<KPL>(zkeep >> SH, zkeep & (KPL - 1)), tw);
-            }
+        // the group's changes as whole-wavefront float atomics (see
+        // sample_range); a removal can leave a rounding-level negative cell,
+        // which the merge's clamp max(n_kw, 0) (LDAModel.hpp:574-577) removes
+        // after the sweep.  An unchanged topic adds nothing.
+        if (lane >= off && lane < i1 && zkeep != zv && (unsigned)wv < (unsigned)p.V) {
+            float* c = nkwf + (int64_t)wv * p.Kp;
+            unsafeAtomicAdd(c + kpos<KPL>(zv >> SH, zv & (KPL - 1)), -tw);
+            unsafeAtomicAdd(c + kpos<KPL>(zkeep >> SH, zkeep & (KPL - 1)), tw);
         }
         if (i1 == nb && lane < nb) p.z[g] = (uint16_t)zkeep;
     }
@@ -905,7 +997,7 @@ __global__ void __launch_bounds__(64 * kSweepWaves) sweep_device_w_kernel(SweepP
         fvec<KPL> nd;
         load_kvec_f<KPL>(ndk_row, lane, nd);
         lds_store_f<KPL>(S.nd, lane, nd);
-        sample_range_w<KPL, MODE_ATOMIC>(S, al, p, p.sweep, lane, b, e, nvalid, lastLane, et);
+        sample_range_w<KPL>(S, al, p, p.sweep, lane, b, e, nvalid, lastLane, et);
         lds_load_f<KPL>(S.nd, lane, nd);
         store_kvec<KPL>(reinterpret_cast<int32_t*>(ndk_row), lane, __builtin_bit_cast(ivec<KPL>, nd));
     }
@@ -917,21 +1009,18 @@ template <int KPL, bool ASYM>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) sweep_det_w_kernel(SweepParams p) {
     const int lane = lane_id();
     const int w = blockIdx.x;
-    const WaveLdsW S = wave_lds_w<KPL>();
     int nvalid, lastLane;
     lane_limits<KPL>(p.K, lane, nvalid, lastLane);
     fvec<KPL> al;
     load_kvec_f<KPL>(p.alpha, lane, al);
     auto et = make_eta<KPL, ASYM>(p, lane);
     float* nkp = reinterpret_cast<float*>(p.nk_out) + (int64_t)w * p.Kp;
-    {
-        fvec<KPL> n, y;
-        load_kvec_f<KPL>(nkp, lane, n);
+    fvec<KPL> snd, scol, snk, sy;
+    const SoloStateW<KPL> st{snd, scol, snk, sy};
+    load_kvec_f<KPL>(nkp, lane, snk);
 #pragma unroll
-        for (int j = 0; j < KPL; ++j) y[j] = recip_rn(n[j] + et.s(j));
-        lds_store_f<KPL>(S.nk, lane, n);
-        lds_store_f<KPL>(S.y, lane, y);
-    }
+    for (int j = 0; j < KPL; ++j) sy[j] = recip_rn(snk[j] + et.s(j));
+    int vcur = -1;
     const int W = p.W;
     const uint32_t sweep = p.sweep_dev ? (uint32_t)uniform_i((int)*p.sweep_dev) : p.sweep;
     const int didx = (p.round + w) % W;
@@ -954,16 +1043,11 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2)))
         if (b == e) continue;
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
         float* ndk_row = reinterpret_cast<float*>(p.ndk) + d * p.Kp;
-        fvec<KPL> nd;
-        load_kvec_f<KPL>(ndk_row, lane, nd);
-        lds_store_f<KPL>(S.nd, lane, nd);
-        sample_range_w<KPL, MODE_DET>(S, al, p, sweep, lane, b, e, nvalid, lastLane, et);
-        lds_load_f<KPL>(S.nd, lane, nd);
-        store_kvec<KPL>(reinterpret_cast<int32_t*>(ndk_row), lane, __builtin_bit_cast(ivec<KPL>, nd));
+        load_kvec_f<KPL>(ndk_row, lane, snd);
+        sample_range_solo_w<KPL>(st, al, p, sweep, lane, b, e, nvalid, lastLane, et, vcur);
+        store_kvec<KPL>(reinterpret_cast<int32_t*>(ndk_row), lane, __builtin_bit_cast(ivec<KPL>, snd));
     }
-    fvec<KPL> mine;
-    lds_load_f<KPL>(S.nk, lane, mine);
-    store_kvec<KPL>(reinterpret_cast<int32_t*>(nkp), lane, __builtin_bit_cast(ivec<KPL>, mine));
+    store_kvec<KPL>(reinterpret_cast<int32_t*>(nkp), lane, __builtin_bit_cast(ivec<KPL>, snk));
 }
 
 // copy_merge with token weights: the schedule of sweep_copy_kernel on
@@ -973,21 +1057,18 @@ template <int KPL, bool ASYM>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) sweep_copy_w_kernel(SweepParams p) {
     const int lane = lane_id();
     const int w = blockIdx.x;
-    const WaveLdsW S = wave_lds_w<KPL>();
     int nvalid, lastLane;
     lane_limits<KPL>(p.K, lane, nvalid, lastLane);
     fvec<KPL> al;
     load_kvec_f<KPL>(p.alpha, lane, al);
     auto et = make_eta<KPL, ASYM>(p, lane);
     float* nkp = reinterpret_cast<float*>(p.nk_out) + (int64_t)w * p.Kp;
-    {
-        fvec<KPL> n, y;
-        load_kvec_f<KPL>(nkp, lane, n);
+    fvec<KPL> snd, scol, snk, sy;
+    const SoloStateW<KPL> st{snd, scol, snk, sy};
+    load_kvec_f<KPL>(nkp, lane, snk);
 #pragma unroll
-        for (int j = 0; j < KPL; ++j) y[j] = recip_rn(n[j] + et.s(j));
-        lds_store_f<KPL>(S.nk, lane, n);
-        lds_store_f<KPL>(S.y, lane, y);
-    }
+    for (int j = 0; j < KPL; ++j) sy[j] = recip_rn(snk[j] + et.s(j));
+    int vcur = -1;
     SweepParams q = p;
     q.nkw = p.nkw + (int64_t)w * p.V * p.Kp;
     const int W = p.W;
@@ -1011,17 +1092,12 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2)))
             if (b == e) continue;
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
             float* ndk_row = reinterpret_cast<float*>(p.ndk) + d * p.Kp;
-            fvec<KPL> nd;
-            load_kvec_f<KPL>(ndk_row, lane, nd);
-            lds_store_f<KPL>(S.nd, lane, nd);
-            sample_range_w<KPL, MODE_DET>(S, al, q, q.sweep, lane, b, e, nvalid, lastLane, et);
-            lds_load_f<KPL>(S.nd, lane, nd);
-            store_kvec<KPL>(reinterpret_cast<int32_t*>(ndk_row), lane, __builtin_bit_cast(ivec<KPL>, nd));
+            load_kvec_f<KPL>(ndk_row, lane, snd);
+            sample_range_solo_w<KPL>(st, al, q, q.sweep, lane, b, e, nvalid, lastLane, et, vcur);
+            store_kvec<KPL>(reinterpret_cast<int32_t*>(ndk_row), lane, __builtin_bit_cast(ivec<KPL>, snd));
         }
     }
-    fvec<KPL> mine;
-    lds_load_f<KPL>(S.nk, lane, mine);
-    store_kvec<KPL>(reinterpret_cast<int32_t*>(nkp), lane, __builtin_bit_cast(ivec<KPL>, mine));
+    store_kvec<KPL>(reinterpret_cast<int32_t*>(nkp), lane, __builtin_bit_cast(ivec<KPL>, snk));
 }
 
 // mergeState<copy_merge> on float counts (LDAModel.hpp:555-564):
