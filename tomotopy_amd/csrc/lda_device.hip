@@ -2952,7 +2952,7 @@ hipError_t launch_sweep_device(int kpl, const SweepParams& p, int grid, int wave
 hipError_t launch_sweep_copy(int kpl, const SweepParams& p, hipStream_t s) {
     if (kpl > kSmallMaxKpl || p.exact) return launch_bk_sweep_copy(kpl, p, false, s);
     LDA_DISPATCH_KPL_ETA(kpl, p.prior_row != nullptr,
-                         hipLaunchKernelGGL((sweep_copy_kernel<KPL, ASYM>), dim3(p.W), dim3(64), wave_lds_bytes<KPL>(), s, p));
+                         hipLaunchKernelGGL((sweep_copy_kernel<KPL, ASYM>), dim3(p.W), dim3(64), 0, s, p));
     return hipGetLastError();
 }
 
@@ -2975,7 +2975,7 @@ hipError_t launch_replicate(int32_t* dst, const int32_t* src, int64_t n, int W, 
 hipError_t launch_sweep_det(int kpl, const SweepParams& p, hipStream_t s) {
     if (kpl > kSmallMaxKpl || p.exact) return launch_bk_sweep_det(kpl, p, false, s);
     LDA_DISPATCH_KPL_ETA(kpl, p.prior_row != nullptr,
-                         hipLaunchKernelGGL((sweep_det_kernel<KPL, ASYM>), dim3(p.W), dim3(64), wave_lds_bytes<KPL>(), s, p));
+                         hipLaunchKernelGGL((sweep_det_kernel<KPL, ASYM>), dim3(p.W), dim3(64), 0, s, p));
     return hipGetLastError();
 }
 
@@ -3181,14 +3181,14 @@ hipError_t launch_sweep_device_w(int kpl, const SweepParams& p, int grid, int wa
 hipError_t launch_sweep_det_w(int kpl, const SweepParams& p, hipStream_t s) {
     if (kpl > kSmallMaxKpl) return launch_bk_sweep_det(kpl, p, true, s);
     LDA_DISPATCH_KPL_ETA(kpl, p.prior_row != nullptr,
-                         hipLaunchKernelGGL((sweep_det_w_kernel<KPL, ASYM>), dim3(p.W), dim3(64), wave_lds_bytes<KPL>(), s, p));
+                         hipLaunchKernelGGL((sweep_det_w_kernel<KPL, ASYM>), dim3(p.W), dim3(64), 0, s, p));
     return hipGetLastError();
 }
 
 hipError_t launch_sweep_copy_w(int kpl, const SweepParams& p, hipStream_t s) {
     if (kpl > kSmallMaxKpl) return launch_bk_sweep_copy(kpl, p, true, s);
     LDA_DISPATCH_KPL_ETA(kpl, p.prior_row != nullptr,
-                         hipLaunchKernelGGL((sweep_copy_w_kernel<KPL, ASYM>), dim3(p.W), dim3(64), wave_lds_bytes<KPL>(), s, p));
+                         hipLaunchKernelGGL((sweep_copy_w_kernel<KPL, ASYM>), dim3(p.W), dim3(64), 0, s, p));
     return hipGetLastError();
 }
 
@@ -3204,8 +3204,8 @@ template <int KPL, bool ASYM>
 static int sweep_det_blocks_kpl(bool weighted) {
     int n = 0;
     const hipError_t e =
-        weighted ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sweep_det_w_kernel<KPL, ASYM>, 64, wave_lds_bytes<KPL>())
-                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sweep_det_kernel<KPL, ASYM>, 64, wave_lds_bytes<KPL>());
+        weighted ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sweep_det_w_kernel<KPL, ASYM>, 64, 0)
+                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sweep_det_kernel<KPL, ASYM>, 64, 0);
     return (e == hipSuccess && n > 0) ? n : 1;
 }
 
