@@ -154,7 +154,7 @@ int lda_set_global_step(lda_ctx* ctx, uint64_t step);
  *                              tomoto's copy_merge: W device workers, each
  *                              with private copies of n_kw and n_k, merged
  *                              by summing deltas (deterministic); workers ==
- *                              0 picks W = min(64, (D+1)/2, copies in 16 GB).
+ *                              0 picks W = min(64, (D+1)/2, copies in 32 GB).
  *   DEFAULT with workers == 0  device-wide (no counterpart in tomotopy,
  *                              whose DEFAULT is PARTITION): every workgroup
  *                              is a worker with a private n_k; n_kw is

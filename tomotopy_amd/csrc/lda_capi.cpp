@@ -322,7 +322,7 @@ int narrow_end(lda_ctx* c) {
 //   tomoto's cap max((V+3)/4, 1) (LDAModel.hpp:491-503), the documents and
 //   the resident capacity of the round kernel.
 //   copy_merge: tomoto's cap max((D+1)/2, 1), at most 64 workers and at most
-//   16 GB of private n_kw copies.
+//   32 GB of private n_kw copies (a ninth of the HBM).
 constexpr double kPartitionRoundTokens = 48.0;
 
 int device_partition_workers(lda_ctx* c) {
@@ -338,7 +338,7 @@ int device_partition_workers(lda_ctx* c) {
 int device_copy_workers(lda_ctx* c) {
     const int64_t cap_cm = std::max<int64_t>((c->D + 1) / 2, 1);
     const double per_copy = std::max(1.0, (double)c->V * c->Kp * 4.0);
-    const int64_t by_mem = std::max<int64_t>((int64_t)(16e9 / per_copy), 1);
+    const int64_t by_mem = std::max<int64_t>((int64_t)(32e9 / per_copy), 1);
     return (int)std::min<int64_t>(std::min<int64_t>(cap_cm, by_mem), 64);
 }
 
